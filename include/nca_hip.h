@@ -1,0 +1,202 @@
+/*
+ * nca_hip.h — C-ABI of the MI355X (gfx950) hot path of NestedCrossApproximation.jl:
+ * the batched bottom-up NCA compressor and the H^2 matrix-vector product.
+ *
+ * Every entry point replaces one seam of the reference (file:line cited at each
+ * declaration) and is what the Julia package extension binds with `ccall`.
+ *
+ * Conventions
+ *  - Status: every function returns int32_t, NCA_OK (0) on success; the message of
+ *    the last failure of the calling thread is nca_last_error(). No C++ exception
+ *    crosses the ABI.
+ *  - Complex numbers are interleaved (re, im) doubles (Julia ComplexF64 layout);
+ *    matrices are column-major.
+ *  - Arrays that mirror Julia fields (CSR pointers, node ids, point ids, pivots)
+ *    are 1-based, exactly as the Julia structs hold them; packed-block offsets are
+ *    0-based element offsets (the binding creates them when it flattens a
+ *    Vector{Matrix}).
+ *  - Points are 3 x n column-major doubles (reinterpret of Vector{SVector{3,Float64}}).
+ *  - Handles own device memory; host arrays stay owned by the caller (Julia pins
+ *    them with GC.@preserve for the duration of the call).
+ *  - A context and its stream serve one host task at a time.
+ */
+#ifndef NCA_HIP_H
+#define NCA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nca_ctx nca_ctx;
+typedef struct nca_points nca_points;
+typedef struct nca_stack nca_stack;
+typedef struct nca_coupling nca_coupling;
+typedef struct nca_h2 nca_h2;
+
+enum {
+    NCA_OK = 0,
+    NCA_ERR_ARGUMENT = 1,  /* ArgumentError */
+    NCA_ERR_DIMENSION = 2, /* DimensionMismatch */
+    NCA_ERR_HIP = 3,       /* HIP runtime failure */
+    NCA_ERR_SINGULAR = 4,  /* SingularCrossMatrixError */
+    NCA_ERR_LIMIT = 5      /* a block exceeds a documented kernel limit */
+};
+
+enum { NCA_OP_N = 0, NCA_OP_T = 1, NCA_OP_C = 2 };
+enum { NCA_PREC_FP64 = 0, NCA_PREC_MIXED = 1 };
+
+/* Helmholtz point kernel G(x, y) = cis(sign * k * |x - y|) / (4 pi |x - y|),
+ * zero at |x - y| == 0. sign = +1: README.md:104-108 (cis(k r)/(4 pi r));
+ * sign = -1: test/nca/test_petrovgalerkinnca.jl:14-17 (exp(-i k r)). */
+typedef struct {
+    double wavenumber;
+    int32_t sign;
+    int32_t reserved;
+} nca_helmholtz;
+
+/* ---- runtime ------------------------------------------------------------ */
+const char* nca_last_error(void);
+int32_t nca_abi_version(void);
+int32_t nca_ctx_create(int32_t device, nca_ctx** out);
+int32_t nca_ctx_destroy(nca_ctx* ctx);
+int32_t nca_ctx_synchronize(nca_ctx* ctx);
+/* The hipStream_t all work of the context runs on (for device-pointer callers). */
+int32_t nca_ctx_stream(nca_ctx* ctx, void** stream);
+
+/* Upload n points (3 x n, column-major). */
+int32_t nca_points_create(nca_ctx* ctx, const double* xyz, int64_t n, nca_points** out);
+int32_t nca_points_destroy(nca_points* pts);
+
+/* ---- seam 1: Phase 2 level assembly ------------------------------------ */
+/* Replaces assemble_level(farmatrix, rowidcs, colidcs; maxrank, ntasks,
+ * transposed) (src/compressors/batchedbottomup.jl:101-116) for a HIP kernel
+ * matrix; the device layout is that of the reference's GPU shard
+ * (ext/NestedCrossApproximationCUDAExt/distributedblockstack.jl:7-27,80-86):
+ * one (sum m_b) x maxrank column-major buffer, block b in rows
+ * rowoffsets[b]+1 : rowoffsets[b+1], columns 1:n_b, zero padding elsewhere.
+ * rowptr/colptr: 1-based CSR over the flattened rowidcs/colidcs (length nb+1).
+ * transposed == 0: entry (a, c) = G(test[rowidx], trial[colidx]);
+ * transposed != 0: entry (a, c) = G(test[colidx], trial[rowidx]), i.e.
+ * transpose(A[colidcs[b], rowidcs[b]]) without conjugation (:98-99).
+ * Errors: NCA_ERR_ARGUMENT when a block has more than maxrank columns
+ * (distributedblockstack.jl:59-62). */
+int32_t nca_level_assemble(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test,
+                           const nca_points* trial, int64_t nblocks, const int64_t* rowptr,
+                           const int64_t* rowidx, const int64_t* colptr, const int64_t* colidx,
+                           int64_t maxrank, int32_t transposed, nca_stack** out);
+/* A stack from host data (same layout; rowoffsets 0-based cumulative, nb+1). */
+int32_t nca_stack_upload(nca_ctx* ctx, const double* buffer, int64_t nblocks, const int64_t* rowoffsets,
+                         const int64_t* ncols, int64_t maxrank, nca_stack** out);
+int32_t nca_stack_shape(const nca_stack* st, int64_t* nblocks, int64_t* nrows, int64_t* maxrank);
+int32_t nca_stack_download(nca_ctx* ctx, const nca_stack* st, double* buffer);
+int32_t nca_stack_destroy(nca_stack* st);
+
+/* ---- seam 2: Phase 3 compression --------------------------------------- */
+/* Batched fully pivoted ACA with the sampled stop, on an internal working copy
+ * (the stack stays pristine, as compresslevel.jl:31-38 copies it). Replaces the
+ * ACA callable cfg(buffer, rowoffsets, ncols, maxrank; sampled=true)
+ * (compresslevel.jl:34-36; CPU: _compressblock, batchedbottomup.jl:118-125).
+ * Outputs (host): ranks[nb]; rowpivots/colpivots maxrank x nb column-major,
+ * 1-based block-local, zero past the rank. */
+int32_t nca_aca_fullpiv(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, int32_t* ranks,
+                        int64_t* rowpivots, int64_t* colpivots);
+
+/* Batched skeleton LU + solve: X_b = L_b * P_b^{-1}, L_b = A_b[:, colpivots],
+ * P_b = L_b[rowpivots, :], m_b x r_b column-major at xoffsets[b] (0-based
+ * complex offsets, nb+1, computed by the caller from ranks and block rows).
+ * Replaces _gather_solve_tensors! + getrf/getrs_strided_batched!
+ * (compresslevel.jl:52-137; Z = X^T) and, on the CPU, nestedtestbasis /
+ * testtransfer with _crossfactorization (nestedbasis.jl:43-57,
+ * transfermatrices.jl:183-211, utils.jl:57-80). independent[b] = -1 when the
+ * cross matrix is regular, else the number of independent pivots of the
+ * singularity rule min|U_ii|^2 <= (10 eps)^2 max|U_ii|^2 (utils.jl:72-79);
+ * the call then returns NCA_ERR_SINGULAR. */
+int32_t nca_skeleton_solve(nca_ctx* ctx, const nca_stack* st, const int32_t* ranks, const int64_t* rowpivots,
+                           const int64_t* colpivots, const int64_t* xoffsets, double* X, int64_t* independent);
+
+/* Both steps in one call, the pivots staying on the device in between
+ * (compresslevel.jl:31-45). xoffsets is an output here (nb+1); X must hold
+ * xcapacity complex numbers (sum_b m_b * min(m_b, n_b, maxrank) always suffices). */
+int32_t nca_compress_level(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, int32_t* ranks,
+                           int64_t* rowpivots, int64_t* colpivots, int64_t xcapacity, int64_t* xoffsets,
+                           double* X, int64_t* independent);
+
+/* ---- seam 3: coupling matrices ----------------------------------------- */
+/* Replaces assemble_couplingstore(farmatrix, testpivots, trialpivots,
+ * testfardata, trialfardata) (src/couplingmatrices.jl:11-53): block i of the
+ * test CSR row of t is A[testpivots[t], trialpivots[farnodes[i]]] (r_t x r_s,
+ * column-major), packed in CSR order. Kept on the device for nca_h2_create.
+ * farptr/farnodes: CouplingTraversalPlan (1-based); pivptr/piv: 1-based CSR of
+ * the pivots of every node (empty for clusters without pivots). */
+int32_t nca_coupling_assemble(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test,
+                              const nca_points* trial, int64_t ntest, const int64_t* farptr,
+                              const int64_t* farnodes, int64_t ntrial, const int64_t* testpivptr,
+                              const int64_t* testpiv, const int64_t* trialpivptr, const int64_t* trialpiv,
+                              nca_coupling** out);
+int32_t nca_coupling_size(const nca_coupling* cp, int64_t* nblocks, int64_t* nentries);
+/* offsets: nblocks+1 (0-based complex offsets); blocks: nentries complex. */
+int32_t nca_coupling_download(nca_ctx* ctx, const nca_coupling* cp, int64_t* offsets, double* blocks);
+int32_t nca_coupling_destroy(nca_coupling* cp);
+
+/* ---- seam 4: the H^2 operator and its product --------------------------- */
+/* Everything PetrovGalerkinNCACore holds for the far field
+ * (src/nca/petrovgalerkinnca.jl:43-101, plan types src/nca/abstractnca.jl:1-108)
+ * plus the near-field plan; the near blocks themselves are evaluated on the fly.
+ * Tree layout: perm = values in tree order (1-based point ids), lo/hi = 1-based
+ * inclusive positions of every node's values (hi = lo - 1 when empty). */
+typedef struct {
+    int64_t M, N, nn_test, nn_trial;
+    const int64_t *test_perm, *test_lo, *test_hi;
+    const int64_t *trial_perm, *trial_lo, *trial_hi;
+    /* CoefficientPlan.globalidxtocoeffptr of both sides (nn+1, 1-based) */
+    const int64_t *test_cptr, *trial_cptr;
+    /* BasisStore: plan.nodes + blocks (test |t| x r_t, trial r_s x |s|) */
+    int64_t nbasis_test, nbasis_trial;
+    const int64_t *basis_test_nodes, *basis_test_off, *basis_trial_nodes, *basis_trial_off;
+    const double *basis_test, *basis_trial;
+    /* TransferStore: TransferTraversalPlan (leveltoidcsptr, levelidcs,
+     * idxtochdsptr, chdsidcs) + blocks (test r_c x r_p, trial r_p x r_c) */
+    int64_t nlev_test, nlev_trial;
+    const int64_t *tt_levptr, *tt_levidcs, *tt_chdptr, *tt_chds, *tt_off;
+    const int64_t *st_levptr, *st_levidcs, *st_chdptr, *st_chds, *st_off;
+    const double *tt_blocks, *st_blocks;
+    /* CouplingTraversalPlan (nodetofarnodesptr, farnodes); blocks from nca_coupling */
+    const int64_t *cp_ptr, *cp_nodes;
+    /* near interactions: test node -> near trial nodes (1-based CSR) */
+    const int64_t *nearptr, *nearnodes;
+} nca_h2_desc;
+
+/* Build the device-resident operator. The coupling store is borrowed: it must
+ * outlive the operator. precision: NCA_PREC_FP64 or NCA_PREC_MIXED. */
+int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test,
+                      const nca_points* trial, const nca_h2_desc* desc, const nca_coupling* coupling,
+                      int32_t precision, nca_h2** out);
+int32_t nca_h2_destroy(nca_h2* h2);
+int32_t nca_h2_set_precision(nca_h2* h2, int32_t precision);
+
+/* Y = alpha * op(A) * X + beta * Y with op N/T/C, nrhs columns. Replaces
+ * LinearMaps._unsafe_mul!(y, A::Union{PetrovGalerkinNCA, TransposeMap,
+ * AdjointMap}, x) (src/nca/petrovgalerkinnca.jl:415-425, stages :431-589) and
+ * the 5-argument mul!. Host pointers; the call returns when Y is written. */
+int32_t nca_h2_matvec(nca_h2* h2, int32_t op, int64_t nrhs, const double* x, int64_t ldx, double* y,
+                      int64_t ldy, const double* alpha, const double* beta);
+/* Same with device pointers, enqueued on the context stream (no host sync). */
+int32_t nca_h2_matvec_device(nca_h2* h2, int32_t op, int64_t nrhs, const void* x, int64_t ldx, void* y,
+                             int64_t ldy, const double* alpha, const double* beta);
+
+typedef struct {
+    int64_t coupling_blocks, coupling_entries;
+    int64_t basis_entries_test, basis_entries_trial;
+    int64_t transfer_entries_test, transfer_entries_trial;
+    int64_t near_pairs, near_entries;
+    int64_t coeffs_test, coeffs_trial;
+    int64_t plan_bytes;
+} nca_h2_stats;
+int32_t nca_h2_get_stats(const nca_h2* h2, nca_h2_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

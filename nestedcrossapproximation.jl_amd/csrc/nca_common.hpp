@@ -1,0 +1,120 @@
+// Internal runtime pieces shared by the translation units of libnca_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/nca_hip.h"
+
+namespace nca {
+
+struct Error : std::runtime_error {
+    int32_t code;
+    Error(int32_t c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define NCA_HIP(call)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw ::nca::Error(NCA_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define NCA_REQUIRE(cond, code, msg)                         \
+    do {                                                     \
+        if (!(cond)) throw ::nca::Error((code), (msg));      \
+    } while (0)
+
+template <class F>
+int32_t guarded(F&& f) {
+    try {
+        f();
+        return NCA_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return NCA_ERR_HIP;
+    } catch (...) {
+        set_last_error("unknown error");
+        return NCA_ERR_HIP;
+    }
+}
+
+// Owning device allocation.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) NCA_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        if (count > n) alloc(count);
+        if (count) NCA_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+}  // namespace nca
+
+struct nca_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct nca_points {
+    nca_ctx* ctx = nullptr;
+    int64_t n = 0;
+    nca::DevBuf<double> xyz;  // 3 x n, point-major (x, y, z per point)
+    std::vector<double> host; // kept for host-side bookkeeping
+};
+
+struct nca_stack {
+    nca_ctx* ctx = nullptr;
+    int64_t nb = 0, nrows = 0, maxrank = 0;
+    std::vector<int64_t> rowoffsets;  // nb+1, 0-based cumulative
+    std::vector<int64_t> ncols;       // nb
+    nca::DevBuf<double2> buffer;      // nrows x maxrank, column-major
+};
+
+struct nca_coupling {
+    nca_ctx* ctx = nullptr;
+    int64_t ntest = 0, ntrial = 0, nblocks = 0, nentries = 0;
+    std::vector<int64_t> ptr, nodes;  // 0-based CSR (test node -> trial nodes)
+    std::vector<int64_t> offsets;     // nblocks+1, 0-based complex offsets
+    std::vector<int32_t> testrank, trialrank;
+    nca::DevBuf<double2> blocks;
+};
+
+namespace nca {
+inline void set_device(const nca_ctx* ctx) { NCA_HIP(hipSetDevice(ctx->device)); }
+}  // namespace nca

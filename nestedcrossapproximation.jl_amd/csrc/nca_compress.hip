@@ -1,0 +1,735 @@
+// Phase 2 / Phase 3 of the batched bottom-up compressor and the coupling
+// matrices, for gfx950: level assembly, batched fully pivoted ACA, batched
+// skeleton LU + solve, coupling evaluation.
+//
+// Compiled with -ffp-contract=off: pivots and ranks must not depend on which
+// products the compiler fuses. The ACA norms are summed in one fixed order
+// (256 per-thread partial sums over the column-major entries e = t, t+256, ...,
+// then a halving tree), so the stopping decisions are reproducible; argmax
+// ties go to the lowest column-major index, which makes the reduction order
+// irrelevant for the pivots themselves.
+
+#include <algorithm>
+#include <numeric>
+
+#include "nca_common.hpp"
+#include "nca_math.hpp"
+
+namespace nca {
+
+constexpr int kWG = 256;
+// ACA with the residual in LDS: m*n <= kLdsCap, m <= kLdsMaxM, n <= kLdsMaxN.
+constexpr int kLdsCap = 3072;
+constexpr int kLdsMaxM = 512;
+constexpr int kLdsMaxN = 64;
+// ACA with the residual in a global working copy.
+constexpr int kGlobMaxM = 2048;
+constexpr int kGlobMaxN = 256;
+// Skeleton solve: the r x r cross matrix lives in LDS.
+constexpr int kSolveMaxRank = 96;
+// Coupling: test pivots of one cluster in LDS.
+constexpr int kCouplingMaxRank = 256;
+
+// ---------------------------------------------------------------------------
+// Level assembly: one thread per buffer entry (row fastest).
+__global__ void __launch_bounds__(kWG)
+    k_assemble_level(const double* __restrict__ rowpts, const double* __restrict__ colpts,
+                     const int32_t* __restrict__ rowblock, const int32_t* __restrict__ rowidx,
+                     const int64_t* __restrict__ colptr, const int32_t* __restrict__ colidx, int64_t ld,
+                     int64_t maxrank, double k, int sign, int transposed, double2* __restrict__ buf) {
+    const int64_t idx = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    if (idx >= ld * maxrank) return;
+    const int64_t row = idx % ld, col = idx / ld;
+    const int b = rowblock[row];
+    const int64_t c0 = colptr[b];
+    double2 g = make_double2(0.0, 0.0);
+    if (col < colptr[b + 1] - c0) {
+        const double* xr = rowpts + 3 * (int64_t)rowidx[row];
+        const double* xc = colpts + 3 * (int64_t)colidx[c0 + col];
+        g = transposed ? helmholtz(k, sign, xc[0], xc[1], xc[2], xr[0], xr[1], xr[2])
+                       : helmholtz(k, sign, xr[0], xr[1], xr[2], xc[0], xc[1], xc[2]);
+    }
+    buf[idx] = g;
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup reductions. Sum: halving tree over 256 slots (slot t += slot t+s).
+// Argmax: larger value, ties to the smaller index.
+struct ArgRed {
+    double sum[kWG];
+    double val[kWG];
+    int idx[kWG];
+};
+
+__device__ __forceinline__ void reduce_sum_argmax(ArgRed& red, double p, double bv, int bi, double& sum,
+                                                  double& vmax, int& imax) {
+    const int t = threadIdx.x;
+    red.sum[t] = p;
+    red.val[t] = bv;
+    red.idx[t] = bi;
+    __syncthreads();
+    for (int s = kWG / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            red.sum[t] = red.sum[t] + red.sum[t + s];
+            const double v2 = red.val[t + s];
+            const int i2 = red.idx[t + s];
+            if (v2 > red.val[t] || (v2 == red.val[t] && i2 < red.idx[t])) {
+                red.val[t] = v2;
+                red.idx[t] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    sum = red.sum[0];
+    vmax = red.val[0];
+    imax = red.idx[0];
+}
+
+// Batched fully pivoted ACA with the sampled stop; one workgroup per block.
+//   r crosses accepted, residual R_r, s = block columns;
+//   stop when r == min(maxrank, m, n) or
+//        ||R_r||_F^2 * (sampled ? s / (s - r) : 1) <= tol^2 ||A||_F^2;
+//   pivot = first argmax |R_r|^2 (column-major), R -= R(:,j) (R(i,:) / R(i,j)),
+//   row i and column j set to zero.
+template <bool kLds>
+__global__ void __launch_bounds__(kWG)
+    k_aca(const double2* __restrict__ buf, double2* __restrict__ work, int64_t ld,
+          const int64_t* __restrict__ rowoff, const int32_t* __restrict__ ncols,
+          const int32_t* __restrict__ blocklist, int64_t maxrank, double tol, int sampled,
+          int32_t* __restrict__ ranks, int64_t* __restrict__ rowpiv, int64_t* __restrict__ colpiv) {
+    extern __shared__ double2 dyn[];
+    __shared__ ArgRed red;
+    const int t = threadIdx.x;
+    const int b = blocklist[blockIdx.x];
+    const int64_t r0 = rowoff[b];
+    const int m = (int)(rowoff[b + 1] - r0);
+    const int n = ncols[b];
+    for (int64_t i = t; i < maxrank; i += kWG) {
+        rowpiv[i + b * maxrank] = 0;
+        colpiv[i + b * maxrank] = 0;
+    }
+    int kmax = (int)maxrank;
+    kmax = m < kmax ? m : kmax;
+    kmax = n < kmax ? n : kmax;
+    if (kmax <= 0) {
+        if (t == 0) ranks[b] = 0;
+        return;
+    }
+    double2* R;
+    int64_t ldr;
+    double2 *u, *v;
+    if (kLds) {
+        R = dyn;
+        ldr = m;
+        u = dyn + kLdsCap;
+        v = u + kLdsMaxM;
+    } else {
+        R = work + r0;
+        ldr = ld;
+        u = dyn;
+        v = dyn + kGlobMaxM;
+    }
+    const int mn = m * n;
+    const int qstep = kWG / m, rstep = kWG % m;
+    double p = 0.0, bv = -1.0;
+    int bi = 0;
+    {
+        int a = t % m, c = t / m;
+        for (int e = t; e < mn; e += kWG) {
+            const double2 val = buf[r0 + a + (int64_t)c * ld];
+            R[a + c * ldr] = val;
+            const double w = cabs2(val);
+            p += w;
+            if (w > bv) { bv = w; bi = e; }
+            a += rstep;
+            c += qstep;
+            if (a >= m) { a -= m; ++c; }
+        }
+    }
+    double nrm2, bmax;
+    int bidx;
+    reduce_sum_argmax(red, p, bv, bi, nrm2, bmax, bidx);
+    const double nrmA2 = nrm2;
+    const double tol2 = tol * tol;
+    int r = 0;
+    while (r < kmax) {
+        const double scale = sampled ? (double)n / (double)(n - r) : 1.0;
+        if (nrm2 * scale <= tol2 * nrmA2) break;
+        if (!(bmax > 0.0)) break;
+        const int i = bidx % m, j = bidx / m;
+        const double2 piv = R[i + j * ldr];
+        const double d = cabs2(piv);
+        const double2 inv = make_double2(piv.x / d, -piv.y / d);
+        for (int a = t; a < m; a += kWG) u[a] = R[a + j * ldr];
+        for (int c = t; c < n; c += kWG) v[c] = cmul(R[i + c * ldr], inv);
+        if (t == 0) {
+            rowpiv[r + b * maxrank] = i + 1;
+            colpiv[r + b * maxrank] = j + 1;
+        }
+        ++r;
+        __syncthreads();
+        p = 0.0;
+        bv = -1.0;
+        bi = 0;
+        int a = t % m, c = t / m;
+        for (int e = t; e < mn; e += kWG) {
+            double2 val;
+            if (a == i || c == j) {
+                val = make_double2(0.0, 0.0);
+            } else {
+                val = csub(R[a + c * ldr], cmul(u[a], v[c]));
+            }
+            R[a + c * ldr] = val;
+            const double w = cabs2(val);
+            p += w;
+            if (w > bv) { bv = w; bi = e; }
+            a += rstep;
+            c += qstep;
+            if (a >= m) { a -= m; ++c; }
+        }
+        reduce_sum_argmax(red, p, bv, bi, nrm2, bmax, bidx);
+    }
+    if (t == 0) ranks[b] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Skeleton solve, one workgroup per block: LU with partial pivoting (pivot =
+// first max |re|+|im|, LAPACK getrf form) of P = A[rowpiv, colpiv] in LDS, then
+// per row a of L = A[:, colpiv] (one thread per row): w U = l, y L = w,
+// interchanges undone last to first; X = L P^{-1} written m x r column-major.
+__global__ void __launch_bounds__(kWG)
+    k_skeleton(const double2* __restrict__ buf, int64_t ld, const int64_t* __restrict__ rowoff,
+               const int32_t* __restrict__ ranks, const int64_t* __restrict__ rowpiv,
+               const int64_t* __restrict__ colpiv, int64_t maxrank, const int64_t* __restrict__ xoff,
+               double2* __restrict__ X, int64_t* __restrict__ indep) {
+    extern __shared__ double2 A[];
+    __shared__ int ipiv[kSolveMaxRank];
+    __shared__ int cpiv[kSolveMaxRank];
+    __shared__ ArgRed red;
+    const int t = threadIdx.x;
+    const int b = blockIdx.x;
+    const int r = ranks[b];
+    if (r == 0) {
+        if (t == 0) indep[b] = -1;
+        return;
+    }
+    const int64_t r0 = rowoff[b];
+    const int m = (int)(rowoff[b + 1] - r0);
+    for (int c = t; c < r; c += kWG) cpiv[c] = (int)colpiv[c + b * maxrank] - 1;
+    __syncthreads();
+    for (int e = t; e < r * r; e += kWG) {
+        const int kk = e % r, c = e / r;
+        A[e] = buf[r0 + (rowpiv[kk + b * maxrank] - 1) + (int64_t)cpiv[c] * ld];
+    }
+    __syncthreads();
+    int info = 0;
+    for (int k = 0; k < r; ++k) {
+        double v = -1.0;
+        int ii = r;
+        if (k + t < r) {
+            v = cabs1(A[(k + t) + k * r]);
+            ii = k + t;
+        }
+        double dummy, best;
+        int p;
+        reduce_sum_argmax(red, 0.0, v, ii, dummy, best, p);
+        if (t == 0) ipiv[k] = p;
+        if (best != 0.0) {
+            if (p != k)
+                for (int c = t; c < r; c += kWG) {
+                    const double2 tmp = A[k + c * r];
+                    A[k + c * r] = A[p + c * r];
+                    A[p + c * r] = tmp;
+                }
+            __syncthreads();
+            for (int i = k + 1 + t; i < r; i += kWG) A[i + k * r] = cdiv(A[i + k * r], A[k + k * r]);
+        } else if (info == 0) {
+            info = k + 1;
+        }
+        __syncthreads();
+        const int nt = r - k - 1;
+        for (int e = t; e < nt * nt; e += kWG) {
+            const int i = k + 1 + e % nt, c = k + 1 + e / nt;
+            A[i + c * r] = csub(A[i + c * r], cmul(A[i + k * r], A[k + c * r]));
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        double d2max = 0.0, d2min = INFINITY;
+        for (int i = 0; i < r; ++i) {
+            const double d2 = cabs2(A[i + i * r]);
+            if (d2 > d2max) d2max = d2;
+            if (d2 < d2min) d2min = d2;
+        }
+        const double eps = 2.220446049250313e-16;
+        const double tol2 = (10.0 * eps) * (10.0 * eps) * d2max;
+        int64_t result = -1;
+        if (!(info == 0 && d2min > tol2)) {
+            int64_t rank = 0;
+            for (int i = 0; i < r; ++i) rank += cabs2(A[i + i * r]) > tol2;
+            result = rank;
+        }
+        indep[b] = result;
+    }
+    double2* Xb = X + xoff[b];
+    for (int a = t; a < m; a += kWG) {
+        double2* xr = Xb + a;
+        for (int c = 0; c < r; ++c) {
+            double2 acc = buf[r0 + a + (int64_t)cpiv[c] * ld];
+            for (int kk = 0; kk < c; ++kk) acc = csub(acc, cmul(xr[(int64_t)kk * m], A[kk + c * r]));
+            xr[(int64_t)c * m] = cdiv(acc, A[c + c * r]);
+        }
+        for (int c = r - 1; c >= 0; --c) {
+            double2 acc = xr[(int64_t)c * m];
+            for (int kk = c + 1; kk < r; ++kk) acc = csub(acc, cmul(xr[(int64_t)kk * m], A[kk + c * r]));
+            xr[(int64_t)c * m] = acc;
+        }
+        for (int kk = r - 1; kk >= 0; --kk) {
+            const int p = ipiv[kk];
+            if (p != kk) {
+                const double2 tmp = xr[(int64_t)kk * m];
+                xr[(int64_t)kk * m] = xr[(int64_t)p * m];
+                xr[(int64_t)p * m] = tmp;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Coupling blocks of one test cluster t: the blocks of its CSR row are
+// contiguous and share r_t rows, so together they form one r_t x C column-major
+// matrix; one thread per entry, the block of a column found by a forward scan.
+__global__ void __launch_bounds__(kWG)
+    k_coupling(const double* __restrict__ testpts, const double* __restrict__ trialpts,
+               const int64_t* __restrict__ farptr, const int32_t* __restrict__ farnodes,
+               const int64_t* __restrict__ offsets, const int64_t* __restrict__ testpivptr,
+               const int32_t* __restrict__ testpiv, const int64_t* __restrict__ trialpivptr,
+               const int32_t* __restrict__ trialpiv, double k, int sign, double2* __restrict__ blocks) {
+    __shared__ double tp[3 * kCouplingMaxRank];
+    const int t = blockIdx.x;
+    const int64_t q0 = farptr[t], q1 = farptr[t + 1];
+    const int rt = (int)(testpivptr[t + 1] - testpivptr[t]);
+    if (rt == 0 || q0 == q1) return;
+    for (int i = threadIdx.x; i < rt; i += kWG) {
+        const double* x = testpts + 3 * (int64_t)testpiv[testpivptr[t] + i];
+        tp[3 * i] = x[0];
+        tp[3 * i + 1] = x[1];
+        tp[3 * i + 2] = x[2];
+    }
+    __syncthreads();
+    const int64_t base = offsets[q0];
+    const int64_t total = offsets[q1] - base;
+    int64_t q = q0;
+    int64_t colbeg = 0, colend = (offsets[q0 + 1] - base) / rt;
+    for (int64_t e = threadIdx.x; e < total; e += kWG) {
+        const int64_t col = e / rt;
+        const int row = (int)(e - col * rt);
+        while (col >= colend) {
+            ++q;
+            colbeg = colend;
+            colend = (offsets[q + 1] - base) / rt;
+        }
+        const int s = farnodes[q];
+        const double* y = trialpts + 3 * (int64_t)trialpiv[trialpivptr[s] + (col - colbeg)];
+        blocks[base + e] = helmholtz(k, sign, tp[3 * row], tp[3 * row + 1], tp[3 * row + 2], y[0], y[1], y[2]);
+    }
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + kWG - 1) / kWG); }
+
+}  // namespace nca
+
+using namespace nca;
+
+extern "C" {
+
+int32_t nca_level_assemble(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test,
+                           const nca_points* trial, int64_t nblocks, const int64_t* rowptr,
+                           const int64_t* rowidx, const int64_t* colptr, const int64_t* colidx,
+                           int64_t maxrank, int32_t transposed, nca_stack** out) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && kernel && test && trial && out && nblocks >= 0 && maxrank >= 0, NCA_ERR_ARGUMENT,
+                    "nca_level_assemble: invalid argument");
+        NCA_REQUIRE(nblocks == 0 || (rowptr && colptr), NCA_ERR_ARGUMENT, "nca_level_assemble: missing CSR");
+        set_device(ctx);
+        const nca_points* rp = transposed ? trial : test;
+        const nca_points* cpnts = transposed ? test : trial;
+        auto st = std::make_unique<nca_stack>();
+        st->ctx = ctx;
+        st->nb = nblocks;
+        st->maxrank = maxrank;
+        st->rowoffsets.assign(nblocks + 1, 0);
+        st->ncols.assign(nblocks, 0);
+        for (int64_t b = 0; b < nblocks; ++b) {
+            const int64_t m = rowptr[b + 1] - rowptr[b], n = colptr[b + 1] - colptr[b];
+            NCA_REQUIRE(m >= 0 && n >= 0, NCA_ERR_ARGUMENT, "nca_level_assemble: decreasing CSR pointer");
+            NCA_REQUIRE(n <= maxrank, NCA_ERR_ARGUMENT,
+                        "a block has more columns than maxrank=" + std::to_string(maxrank));
+            st->rowoffsets[b + 1] = st->rowoffsets[b] + m;
+            st->ncols[b] = n;
+        }
+        const int64_t ld = st->rowoffsets[nblocks];
+        st->nrows = ld;
+        std::vector<int32_t> rowblock(ld), ridx(ld);
+        for (int64_t b = 0; b < nblocks; ++b)
+            for (int64_t a = 0; a < st->rowoffsets[b + 1] - st->rowoffsets[b]; ++a) {
+                const int64_t pnt = rowidx[rowptr[b] - rowptr[0] + a] - 1;
+                NCA_REQUIRE(pnt >= 0 && pnt < rp->n, NCA_ERR_ARGUMENT, "nca_level_assemble: row index out of range");
+                rowblock[st->rowoffsets[b] + a] = (int32_t)b;
+                ridx[st->rowoffsets[b] + a] = (int32_t)pnt;
+            }
+        const int64_t ncoltot = nblocks ? colptr[nblocks] - colptr[0] : 0;
+        std::vector<int64_t> cptr0(nblocks + 1, 0);
+        std::vector<int32_t> cidx(ncoltot);
+        for (int64_t b = 0; b <= nblocks; ++b) cptr0[b] = nblocks ? colptr[b] - colptr[0] : 0;
+        for (int64_t i = 0; i < ncoltot; ++i) {
+            const int64_t pnt = colidx[i] - 1;
+            NCA_REQUIRE(pnt >= 0 && pnt < cpnts->n, NCA_ERR_ARGUMENT, "nca_level_assemble: column index out of range");
+            cidx[i] = (int32_t)pnt;
+        }
+        st->buffer.alloc((size_t)(ld * maxrank));
+        if (ld * maxrank > 0) {
+            DevBuf<int32_t> d_rowblock, d_ridx, d_cidx;
+            DevBuf<int64_t> d_cptr;
+            d_rowblock.upload(rowblock, ctx->stream);
+            d_ridx.upload(ridx, ctx->stream);
+            d_cidx.upload(cidx, ctx->stream);
+            d_cptr.upload(cptr0, ctx->stream);
+            hipLaunchKernelGGL(k_assemble_level, dim3(grid_for(ld * maxrank)), dim3(kWG), 0, ctx->stream,
+                               rp->xyz.p, cpnts->xyz.p, d_rowblock.p, d_ridx.p, d_cptr.p, d_cidx.p, ld, maxrank,
+                               kernel->wavenumber, (int)kernel->sign, (int)(transposed != 0), st->buffer.p);
+            NCA_HIP(hipGetLastError());
+            NCA_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        *out = st.release();
+    });
+}
+
+int32_t nca_stack_upload(nca_ctx* ctx, const double* buffer, int64_t nblocks, const int64_t* rowoffsets,
+                         const int64_t* ncols, int64_t maxrank, nca_stack** out) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && out && nblocks >= 0 && maxrank >= 0, NCA_ERR_ARGUMENT, "nca_stack_upload: invalid argument");
+        set_device(ctx);
+        auto st = std::make_unique<nca_stack>();
+        st->ctx = ctx;
+        st->nb = nblocks;
+        st->maxrank = maxrank;
+        st->rowoffsets.assign(rowoffsets, rowoffsets + nblocks + 1);
+        st->ncols.assign(ncols, ncols + nblocks);
+        NCA_REQUIRE(st->rowoffsets[0] == 0, NCA_ERR_ARGUMENT, "nca_stack_upload: rowoffsets[0] must be 0");
+        for (int64_t b = 0; b < nblocks; ++b) {
+            NCA_REQUIRE(st->rowoffsets[b + 1] >= st->rowoffsets[b], NCA_ERR_ARGUMENT,
+                        "nca_stack_upload: decreasing rowoffsets");
+            NCA_REQUIRE(st->ncols[b] >= 0 && st->ncols[b] <= maxrank, NCA_ERR_ARGUMENT,
+                        "a block has more columns than maxrank=" + std::to_string(maxrank));
+        }
+        st->nrows = st->rowoffsets[nblocks];
+        st->buffer.upload(reinterpret_cast<const double2*>(buffer), (size_t)(st->nrows * maxrank), ctx->stream);
+        NCA_HIP(hipStreamSynchronize(ctx->stream));
+        *out = st.release();
+    });
+}
+
+int32_t nca_stack_shape(const nca_stack* st, int64_t* nblocks, int64_t* nrows, int64_t* maxrank) {
+    return guarded([&] {
+        NCA_REQUIRE(st, NCA_ERR_ARGUMENT, "nca_stack_shape: null stack");
+        if (nblocks) *nblocks = st->nb;
+        if (nrows) *nrows = st->nrows;
+        if (maxrank) *maxrank = st->maxrank;
+    });
+}
+
+int32_t nca_stack_download(nca_ctx* ctx, const nca_stack* st, double* buffer) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && st && buffer, NCA_ERR_ARGUMENT, "nca_stack_download: invalid argument");
+        set_device(ctx);
+        const size_t bytes = (size_t)(st->nrows * st->maxrank) * sizeof(double2);
+        if (bytes) NCA_HIP(hipMemcpyAsync(buffer, st->buffer.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        NCA_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int32_t nca_stack_destroy(nca_stack* st) {
+    return guarded([&] {
+        if (st) {
+            if (st->ctx) set_device(st->ctx);
+            delete st;
+        }
+    });
+}
+
+}  // extern "C"
+
+namespace nca {
+
+// Device-side results of one ACA launch set.
+struct AcaResult {
+    DevBuf<int32_t> ranks;
+    DevBuf<int64_t> rowpiv, colpiv;
+};
+
+static void launch_aca(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, AcaResult& res) {
+    const int64_t nb = st->nb;
+    res.ranks.alloc(std::max<int64_t>(nb, 1));
+    res.rowpiv.alloc(std::max<int64_t>(nb * st->maxrank, 1));
+    res.colpiv.alloc(std::max<int64_t>(nb * st->maxrank, 1));
+    if (nb == 0) return;
+    std::vector<int32_t> small, big, nc32(nb);
+    for (int64_t b = 0; b < nb; ++b) {
+        const int64_t m = st->rowoffsets[b + 1] - st->rowoffsets[b];
+        const int64_t n = st->ncols[b];
+        nc32[b] = (int32_t)n;
+        if (m * n <= kLdsCap && m <= kLdsMaxM && n <= kLdsMaxN) {
+            small.push_back((int32_t)b);
+        } else {
+            NCA_REQUIRE(m <= kGlobMaxM && n <= kGlobMaxN, NCA_ERR_LIMIT,
+                        "batched ACA: block " + std::to_string(b + 1) + " of " + std::to_string(m) + " x " +
+                            std::to_string(n) + " exceeds " + std::to_string(kGlobMaxM) + " x " +
+                            std::to_string(kGlobMaxN));
+            big.push_back((int32_t)b);
+        }
+    }
+    DevBuf<int64_t> d_rowoff;
+    DevBuf<int32_t> d_ncols, d_small, d_big;
+    d_rowoff.upload(st->rowoffsets, ctx->stream);
+    d_ncols.upload(nc32, ctx->stream);
+    DevBuf<double2> work;
+    if (!small.empty()) {
+        d_small.upload(small, ctx->stream);
+        const size_t lds = sizeof(double2) * (kLdsCap + kLdsMaxM + kLdsMaxN);
+        hipLaunchKernelGGL(k_aca<true>, dim3((unsigned)small.size()), dim3(kWG), lds, ctx->stream, st->buffer.p,
+                           nullptr, st->nrows, d_rowoff.p, d_ncols.p, d_small.p, st->maxrank, tol, (int)sampled,
+                           res.ranks.p, res.rowpiv.p, res.colpiv.p);
+        NCA_HIP(hipGetLastError());
+    }
+    if (!big.empty()) {
+        d_big.upload(big, ctx->stream);
+        work.alloc((size_t)(st->nrows * st->maxrank));
+        const size_t lds = sizeof(double2) * (kGlobMaxM + kGlobMaxN);
+        hipLaunchKernelGGL(k_aca<false>, dim3((unsigned)big.size()), dim3(kWG), lds, ctx->stream, st->buffer.p,
+                           work.p, st->nrows, d_rowoff.p, d_ncols.p, d_big.p, st->maxrank, tol, (int)sampled,
+                           res.ranks.p, res.rowpiv.p, res.colpiv.p);
+        NCA_HIP(hipGetLastError());
+    }
+    NCA_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+static void launch_skeleton(nca_ctx* ctx, const nca_stack* st, const int32_t* ranks_h, const DevBuf<int32_t>& ranks,
+                            const DevBuf<int64_t>& rowpiv, const DevBuf<int64_t>& colpiv,
+                            const std::vector<int64_t>& xoff, DevBuf<double2>& X, DevBuf<int64_t>& indep) {
+    const int64_t nb = st->nb;
+    indep.alloc(std::max<int64_t>(nb, 1));
+    X.alloc(std::max<int64_t>(xoff[nb], 1));
+    if (nb == 0) return;
+    int rmax = 0;
+    for (int64_t b = 0; b < nb; ++b) rmax = std::max(rmax, (int)ranks_h[b]);
+    NCA_REQUIRE(rmax <= kSolveMaxRank, NCA_ERR_LIMIT,
+                "skeleton solve: rank " + std::to_string(rmax) + " exceeds " + std::to_string(kSolveMaxRank));
+    DevBuf<int64_t> d_rowoff, d_xoff;
+    d_rowoff.upload(st->rowoffsets, ctx->stream);
+    d_xoff.upload(xoff, ctx->stream);
+    const size_t lds = sizeof(double2) * (size_t)std::max(1, rmax * rmax);
+    if (lds > 32768)
+        NCA_HIP(hipFuncSetAttribute((const void*)k_skeleton, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_skeleton, dim3((unsigned)nb), dim3(kWG), lds, ctx->stream, st->buffer.p, st->nrows,
+                       d_rowoff.p, ranks.p, rowpiv.p, colpiv.p, st->maxrank, d_xoff.p, X.p, indep.p);
+    NCA_HIP(hipGetLastError());
+    NCA_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+static void singular_message(const nca_stack* st, const int32_t* ranks, const std::vector<int64_t>& indep) {
+    for (int64_t b = 0; b < st->nb; ++b)
+        if (indep[b] >= 0)
+            throw Error(NCA_ERR_SINGULAR, "SingularCrossMatrixError: block " + std::to_string(b + 1) + ": " +
+                                              std::to_string(ranks[b]) + " pivots were selected but only " +
+                                              std::to_string(indep[b]) + " of them are linearly independent");
+}
+
+}  // namespace nca
+
+extern "C" {
+
+int32_t nca_aca_fullpiv(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, int32_t* ranks,
+                        int64_t* rowpivots, int64_t* colpivots) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && st && ranks && rowpivots && colpivots, NCA_ERR_ARGUMENT, "nca_aca_fullpiv: invalid argument");
+        set_device(ctx);
+        AcaResult res;
+        launch_aca(ctx, st, tol, sampled, res);
+        if (st->nb == 0) return;
+        NCA_HIP(hipMemcpy(ranks, res.ranks.p, sizeof(int32_t) * st->nb, hipMemcpyDeviceToHost));
+        NCA_HIP(hipMemcpy(rowpivots, res.rowpiv.p, sizeof(int64_t) * st->nb * st->maxrank, hipMemcpyDeviceToHost));
+        NCA_HIP(hipMemcpy(colpivots, res.colpiv.p, sizeof(int64_t) * st->nb * st->maxrank, hipMemcpyDeviceToHost));
+    });
+}
+
+int32_t nca_skeleton_solve(nca_ctx* ctx, const nca_stack* st, const int32_t* ranks, const int64_t* rowpivots,
+                           const int64_t* colpivots, const int64_t* xoffsets, double* X, int64_t* independent) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && st && ranks && rowpivots && colpivots && xoffsets && independent, NCA_ERR_ARGUMENT,
+                    "nca_skeleton_solve: invalid argument");
+        set_device(ctx);
+        const int64_t nb = st->nb;
+        for (int64_t b = 0; b < nb; ++b) {
+            const int64_t m = st->rowoffsets[b + 1] - st->rowoffsets[b];
+            NCA_REQUIRE(ranks[b] >= 0 && ranks[b] <= std::min<int64_t>(m, st->ncols[b]), NCA_ERR_ARGUMENT,
+                        "nca_skeleton_solve: rank out of range");
+            for (int64_t i = 0; i < ranks[b]; ++i) {
+                const int64_t rp = rowpivots[i + b * st->maxrank], cp = colpivots[i + b * st->maxrank];
+                NCA_REQUIRE(rp >= 1 && rp <= m && cp >= 1 && cp <= st->ncols[b], NCA_ERR_ARGUMENT,
+                            "nca_skeleton_solve: pivot out of range");
+            }
+            NCA_REQUIRE(xoffsets[b + 1] - xoffsets[b] >= m * ranks[b], NCA_ERR_DIMENSION,
+                        "nca_skeleton_solve: xoffsets leave too little room");
+        }
+        std::vector<int64_t> xoff(xoffsets, xoffsets + nb + 1);
+        DevBuf<int32_t> d_ranks;
+        DevBuf<int64_t> d_rp, d_cp, d_indep;
+        DevBuf<double2> d_X;
+        d_ranks.upload(ranks, nb, ctx->stream);
+        d_rp.upload(rowpivots, nb * st->maxrank, ctx->stream);
+        d_cp.upload(colpivots, nb * st->maxrank, ctx->stream);
+        launch_skeleton(ctx, st, ranks, d_ranks, d_rp, d_cp, xoff, d_X, d_indep);
+        if (nb == 0) return;
+        std::vector<int64_t> indep(nb);
+        NCA_HIP(hipMemcpy(indep.data(), d_indep.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost));
+        if (xoff[nb] > 0) NCA_HIP(hipMemcpy(X, d_X.p, sizeof(double2) * xoff[nb], hipMemcpyDeviceToHost));
+        std::copy(indep.begin(), indep.end(), independent);
+        singular_message(st, ranks, indep);
+    });
+}
+
+int32_t nca_compress_level(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, int32_t* ranks,
+                           int64_t* rowpivots, int64_t* colpivots, int64_t xcapacity, int64_t* xoffsets,
+                           double* X, int64_t* independent) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && st && ranks && rowpivots && colpivots && xoffsets && independent, NCA_ERR_ARGUMENT,
+                    "nca_compress_level: invalid argument");
+        set_device(ctx);
+        const int64_t nb = st->nb;
+        AcaResult res;
+        launch_aca(ctx, st, tol, sampled, res);
+        xoffsets[0] = 0;
+        if (nb == 0) return;
+        NCA_HIP(hipMemcpy(ranks, res.ranks.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
+        std::vector<int64_t> xoff(nb + 1, 0);
+        for (int64_t b = 0; b < nb; ++b)
+            xoff[b + 1] = xoff[b] + (st->rowoffsets[b + 1] - st->rowoffsets[b]) * (int64_t)ranks[b];
+        NCA_REQUIRE(xoff[nb] <= xcapacity, NCA_ERR_DIMENSION,
+                    "nca_compress_level: X holds " + std::to_string(xcapacity) + " entries, " +
+                        std::to_string(xoff[nb]) + " needed");
+        DevBuf<double2> d_X;
+        DevBuf<int64_t> d_indep;
+        launch_skeleton(ctx, st, ranks, res.ranks, res.rowpiv, res.colpiv, xoff, d_X, d_indep);
+        NCA_HIP(hipMemcpy(rowpivots, res.rowpiv.p, sizeof(int64_t) * nb * st->maxrank, hipMemcpyDeviceToHost));
+        NCA_HIP(hipMemcpy(colpivots, res.colpiv.p, sizeof(int64_t) * nb * st->maxrank, hipMemcpyDeviceToHost));
+        std::vector<int64_t> indep(nb);
+        NCA_HIP(hipMemcpy(indep.data(), d_indep.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost));
+        if (xoff[nb] > 0) NCA_HIP(hipMemcpy(X, d_X.p, sizeof(double2) * xoff[nb], hipMemcpyDeviceToHost));
+        std::copy(xoff.begin(), xoff.end(), xoffsets);
+        std::copy(indep.begin(), indep.end(), independent);
+        singular_message(st, ranks, indep);
+    });
+}
+
+int32_t nca_coupling_assemble(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test,
+                              const nca_points* trial, int64_t ntest, const int64_t* farptr,
+                              const int64_t* farnodes, int64_t ntrial, const int64_t* testpivptr,
+                              const int64_t* testpiv, const int64_t* trialpivptr, const int64_t* trialpiv,
+                              nca_coupling** out) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && kernel && test && trial && out && farptr && testpivptr && trialpivptr && ntest >= 0 &&
+                        ntrial >= 0,
+                    NCA_ERR_ARGUMENT, "nca_coupling_assemble: invalid argument");
+        set_device(ctx);
+        auto cp = std::make_unique<nca_coupling>();
+        cp->ctx = ctx;
+        cp->ntest = ntest;
+        cp->ntrial = ntrial;
+        cp->ptr.resize(ntest + 1);
+        for (int64_t t = 0; t <= ntest; ++t) cp->ptr[t] = farptr[t] - farptr[0];
+        cp->nblocks = cp->ptr[ntest];
+        cp->nodes.resize(cp->nblocks);
+        cp->testrank.resize(ntest);
+        cp->trialrank.resize(ntrial);
+        int rmax = 0;
+        for (int64_t t = 0; t < ntest; ++t) {
+            cp->testrank[t] = (int32_t)(testpivptr[t + 1] - testpivptr[t]);
+            rmax = std::max(rmax, cp->testrank[t]);
+        }
+        for (int64_t s = 0; s < ntrial; ++s) cp->trialrank[s] = (int32_t)(trialpivptr[s + 1] - trialpivptr[s]);
+        NCA_REQUIRE(rmax <= kCouplingMaxRank, NCA_ERR_LIMIT, "coupling: test rank exceeds 256");
+        cp->offsets.assign(cp->nblocks + 1, 0);
+        std::vector<int32_t> nodes32(cp->nblocks);
+        for (int64_t t = 0; t < ntest; ++t)
+            for (int64_t i = cp->ptr[t]; i < cp->ptr[t + 1]; ++i) {
+                const int64_t s = farnodes[i] - 1;
+                NCA_REQUIRE(s >= 0 && s < ntrial, NCA_ERR_ARGUMENT, "nca_coupling_assemble: far node out of range");
+                cp->nodes[i] = s;
+                nodes32[i] = (int32_t)s;
+                cp->offsets[i + 1] = cp->offsets[i] + (int64_t)cp->testrank[t] * cp->trialrank[s];
+            }
+        cp->nentries = cp->offsets[cp->nblocks];
+        const int64_t ntp = testpivptr[ntest] - testpivptr[0], nsp = trialpivptr[ntrial] - trialpivptr[0];
+        std::vector<int64_t> tptr(ntest + 1), sptr(ntrial + 1);
+        for (int64_t t = 0; t <= ntest; ++t) tptr[t] = testpivptr[t] - testpivptr[0];
+        for (int64_t s = 0; s <= ntrial; ++s) sptr[s] = trialpivptr[s] - trialpivptr[0];
+        std::vector<int32_t> tpiv(ntp), spiv(nsp);
+        for (int64_t i = 0; i < ntp; ++i) {
+            NCA_REQUIRE(testpiv[i] >= 1 && testpiv[i] <= test->n, NCA_ERR_ARGUMENT, "coupling: test pivot out of range");
+            tpiv[i] = (int32_t)(testpiv[i] - 1);
+        }
+        for (int64_t i = 0; i < nsp; ++i) {
+            NCA_REQUIRE(trialpiv[i] >= 1 && trialpiv[i] <= trial->n, NCA_ERR_ARGUMENT,
+                        "coupling: trial pivot out of range");
+            spiv[i] = (int32_t)(trialpiv[i] - 1);
+        }
+        cp->blocks.alloc((size_t)std::max<int64_t>(cp->nentries, 1));
+        if (cp->nentries > 0) {
+            DevBuf<int64_t> d_ptr, d_off, d_tptr, d_sptr;
+            DevBuf<int32_t> d_nodes, d_tpiv, d_spiv;
+            d_ptr.upload(cp->ptr, ctx->stream);
+            d_off.upload(cp->offsets, ctx->stream);
+            d_tptr.upload(tptr, ctx->stream);
+            d_sptr.upload(sptr, ctx->stream);
+            d_nodes.upload(nodes32, ctx->stream);
+            d_tpiv.upload(tpiv, ctx->stream);
+            d_spiv.upload(spiv, ctx->stream);
+            hipLaunchKernelGGL(k_coupling, dim3((unsigned)ntest), dim3(kWG), 0, ctx->stream, test->xyz.p,
+                               trial->xyz.p, d_ptr.p, d_nodes.p, d_off.p, d_tptr.p, d_tpiv.p, d_sptr.p, d_spiv.p,
+                               kernel->wavenumber, (int)kernel->sign, cp->blocks.p);
+            NCA_HIP(hipGetLastError());
+            NCA_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        *out = cp.release();
+    });
+}
+
+int32_t nca_coupling_size(const nca_coupling* cp, int64_t* nblocks, int64_t* nentries) {
+    return guarded([&] {
+        NCA_REQUIRE(cp, NCA_ERR_ARGUMENT, "nca_coupling_size: null handle");
+        if (nblocks) *nblocks = cp->nblocks;
+        if (nentries) *nentries = cp->nentries;
+    });
+}
+
+int32_t nca_coupling_download(nca_ctx* ctx, const nca_coupling* cp, int64_t* offsets, double* blocks) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && cp, NCA_ERR_ARGUMENT, "nca_coupling_download: invalid argument");
+        set_device(ctx);
+        if (offsets) std::copy(cp->offsets.begin(), cp->offsets.end(), offsets);
+        if (blocks && cp->nentries > 0)
+            NCA_HIP(hipMemcpy(blocks, cp->blocks.p, sizeof(double2) * cp->nentries, hipMemcpyDeviceToHost));
+    });
+}
+
+int32_t nca_coupling_destroy(nca_coupling* cp) {
+    return guarded([&] {
+        if (cp) {
+            if (cp->ctx) set_device(cp->ctx);
+            delete cp;
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,752 @@
+// The H^2 matrix-vector product on gfx950 (src/nca/petrovgalerkinnca.jl:431-589):
+//   gather   xs = x[perm_src]                         (source points in tree order)
+//   stage 1  xh_s = B_s xs_s          per source leaf  (aggregate.jl:1-19)
+//   stage 2  xh_p += E_c xh_c         per level, deepest first (aggregate.jl:44-67)
+//   stage 3  yh_t += S_ts xh_s        coupling (couple.jl:1-24; reverse :119-142)
+//   stage 4  yh_c += E_c yh_p         per level, root first (disaggregate.jl:1-23)
+//   stage 5  y[perm_dst] = alpha (B_t yh_t + near) + beta y   per destination leaf
+//            (disaggregate.jl:62-80 fused with the near field, which is evaluated
+//            on the fly from the point coordinates instead of being stored).
+// op T/C swap the two sides and use every operator transposed (conjugated).
+// Coefficient vectors hold the R right-hand sides interleaved: xh[k * R + q].
+
+#include <algorithm>
+#include <memory>
+
+#include "nca_common.hpp"
+#include "nca_math.hpp"
+
+namespace nca {
+
+constexpr int kWave = 64;
+
+struct SideDev {
+    int64_t npts = 0, nn = 0, K = 0;
+    DevBuf<int32_t> perm, lo, hi, cptr;
+    DevBuf<double> pts;  // coordinates in tree order, 3 per position
+    // leaf bases
+    int64_t nbasis = 0;
+    DevBuf<int32_t> bnodes;
+    DevBuf<int64_t> boff;
+    DevBuf<double2> bblocks;
+    // transfers, grouped by level: parents [levptr[l], levptr[l+1]) of parents array
+    int64_t nlev = 0;
+    std::vector<int64_t> levptr;
+    DevBuf<int32_t> parents, chds;
+    DevBuf<int64_t> chdptr, toff;
+    DevBuf<double2> tblocks;
+    // all leaves of the tree (destinations of stage 5) and their basis slot (-1: none)
+    int64_t nleaves = 0;
+    DevBuf<int32_t> leaves, leafbasis;
+    // near lists by node of this side (as destination): ptr (nn+1), nodes of the other side
+    DevBuf<int64_t> nearptr;
+    DevBuf<int32_t> nearnodes;
+    // host copies used during creation
+    std::vector<int32_t> h_lo, h_hi, h_cptr, h_perm;
+    int64_t basis_entries = 0, transfer_entries = 0;
+};
+
+}  // namespace nca
+
+struct nca_h2 {
+    nca_ctx* ctx = nullptr;
+    double k = 0.0;
+    int sign = 1;
+    int precision = NCA_PREC_FP64;
+    int64_t M = 0, N = 0;
+    nca::SideDev test, trial;
+    const nca_coupling* coupling = nullptr;
+    // forward coupling: per test node entry base and column base, per column the source coefficient
+    nca::DevBuf<int64_t> f_base, f_colbase;
+    nca::DevBuf<int32_t> f_ncols, f_colcoef;
+    // reverse coupling: per trial node list of (test node, block offset)
+    nca::DevBuf<int64_t> r_ptr, r_off;
+    nca::DevBuf<int32_t> r_test;
+    // workspace
+    int64_t ws_rhs = 0;
+    nca::DevBuf<double2> ws_xs, ws_xh, ws_yh, ws_xin, ws_yio;
+    nca_h2_stats stats{};
+};
+
+namespace nca {
+
+// ---------------------------------------------------------------------------
+__global__ void k_fill(double2* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_double2(0.0, 0.0);
+}
+
+// xs[k*R + q] = x[perm[k] + q*ldx]
+__global__ void k_gather(const double2* __restrict__ x, int64_t ldx, const int32_t* __restrict__ perm, int64_t npts,
+                         int64_t R, double2* __restrict__ xs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts * R) return;
+    const int64_t kk = i / R, q = i - kk * R;
+    xs[i] = x[(int64_t)perm[kk] + q * ldx];
+}
+
+template <bool CJ>
+__device__ __forceinline__ double2 ldc(const double2* p, int64_t i) {
+    double2 v = p[i];
+    if (CJ) v.y = -v.y;
+    return v;
+}
+
+// Stage 1: one wave per source leaf. TR: the block is the destination-side
+// basis used transposed (U_t, |t| x r), else V_s (r x |s|).
+template <int NR, bool TR, bool CJ>
+__global__ void __launch_bounds__(kWave)
+    k_project(const int32_t* __restrict__ bnodes, const int64_t* __restrict__ boff, const double2* __restrict__ blocks,
+              const int32_t* __restrict__ lo, const int32_t* __restrict__ hi, const int32_t* __restrict__ cptr,
+              const double2* __restrict__ xs, int64_t R, int64_t q0, double2* __restrict__ xh) {
+    const int i = blockIdx.x;
+    const int s = bnodes[i];
+    const int r = cptr[s + 1] - cptr[s];
+    const int l0 = lo[s], ns = hi[s] - l0;
+    const double2* B = blocks + boff[i];
+    for (int a = threadIdx.x; a < r; a += kWave) {
+        double2 acc[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] = make_double2(0.0, 0.0);
+        for (int j = 0; j < ns; ++j) {
+            const double2 b = ldc<CJ>(B, TR ? (int64_t)j + (int64_t)a * ns : (int64_t)a + (int64_t)j * r);
+            const double2* xv = xs + (int64_t)(l0 + j) * R + q0;
+#pragma unroll
+            for (int q = 0; q < NR; ++q) acc[q] = cfma(b, xv[q], acc[q]);
+        }
+        double2* out = xh + (int64_t)(cptr[s] + a) * R + q0;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) out[q] = cadd(out[q], acc[q]);
+    }
+}
+
+// Stage 2 (TR = false: E (r_p x r_c), forward trial transfers; TR = true: E
+// (r_c x r_p) used transposed) and stage 4 (DOWN: children accumulate from the
+// parent; TR = false: E (r_c x r_p), TR = true: E (r_p x r_c) transposed).
+// One wave per parent of the level.
+template <int NR, bool DOWN, bool TR, bool CJ>
+__global__ void __launch_bounds__(kWave)
+    k_transfer(const int32_t* __restrict__ parents, const int64_t* __restrict__ chdptr,
+               const int32_t* __restrict__ chds, const int64_t* __restrict__ toff, const double2* __restrict__ blocks,
+               const int32_t* __restrict__ cptr, int64_t pbase, int64_t R, int64_t q0, double2* __restrict__ h) {
+    const int64_t pi = pbase + blockIdx.x;
+    const int p = parents[pi];
+    const int rp = cptr[p + 1] - cptr[p];
+    for (int64_t e = chdptr[pi]; e < chdptr[pi + 1]; ++e) {
+        const int c = chds[e];
+        const int rc = cptr[c + 1] - cptr[c];
+        const double2* E = blocks + toff[e];
+        const int nout = DOWN ? rc : rp, nin = DOWN ? rp : rc;
+        const int outbase = DOWN ? cptr[c] : cptr[p], inbase = DOWN ? cptr[p] : cptr[c];
+        for (int a = threadIdx.x; a < nout; a += kWave) {
+            double2 acc[NR];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) acc[q] = make_double2(0.0, 0.0);
+            for (int j = 0; j < nin; ++j) {
+                const double2 b = ldc<CJ>(E, TR ? (int64_t)j + (int64_t)a * nin : (int64_t)a + (int64_t)j * nout);
+                const double2* hv = h + (int64_t)(inbase + j) * R + q0;
+#pragma unroll
+                for (int q = 0; q < NR; ++q) acc[q] = cfma(b, hv[q], acc[q]);
+            }
+            double2* out = h + (int64_t)(outbase + a) * R + q0;
+#pragma unroll
+            for (int q = 0; q < NR; ++q) out[q] = cadd(out[q], acc[q]);
+        }
+        // the parent's coefficients are complete before the next edge only
+        // matters for the upward pass, where different edges add into the same
+        // parent from the same lanes: no cross-lane hazard.
+    }
+}
+
+// Stage 3 forward: one wave per test cluster t. The coupling blocks of its row
+// form one r_t x C column-major matrix; lane = (row, column slot).
+template <int NR, bool CJ>
+__global__ void __launch_bounds__(kWave)
+    k_couple_fwd(const double2* __restrict__ S, const int64_t* __restrict__ base, const int64_t* __restrict__ colbase,
+                 const int32_t* __restrict__ ncols, const int32_t* __restrict__ colcoef,
+                 const int32_t* __restrict__ cptr_dst, const double2* __restrict__ xh, int64_t R, int64_t q0,
+                 double2* __restrict__ yh) {
+    __shared__ double2 red[kWave];
+    const int t = blockIdx.x;
+    const int rt = cptr_dst[t + 1] - cptr_dst[t];
+    const int C = ncols[t];
+    if (rt == 0 || C == 0) return;
+    const double2* St = S + base[t];
+    const int32_t* cc = colcoef + colbase[t];
+    const int lane = threadIdx.x;
+    for (int r0 = 0; r0 < rt; r0 += kWave) {
+        const int rows = min(rt - r0, kWave);
+        const int ncs = kWave / rows;
+        const int row = r0 + lane % rows, slot = lane / rows;
+        double2 acc[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] = make_double2(0.0, 0.0);
+        if (slot < ncs) {
+            for (int col = slot; col < C; col += ncs) {
+                const double2 sv = ldc<CJ>(St, (int64_t)row + (int64_t)col * rt);
+                const double2* xv = xh + (int64_t)cc[col] * R + q0;
+#pragma unroll
+                for (int q = 0; q < NR; ++q) acc[q] = cfma(sv, xv[q], acc[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            __syncthreads();
+            red[lane] = (slot < ncs) ? acc[q] : make_double2(0.0, 0.0);
+            __syncthreads();
+            if (slot == 0) {
+                double2 sum = red[lane];
+                for (int c = 1; c < ncs; ++c) sum = cadd(sum, red[lane + c * rows]);
+                double2* out = yh + (int64_t)(cptr_dst[t] + row) * R + q0 + q;
+                *out = cadd(*out, sum);
+            }
+        }
+    }
+}
+
+// Stage 3 reverse (T/C): one wave per trial cluster s: yh_s += sum_t S_ts^T xh_t.
+template <int NR, bool CJ>
+__global__ void __launch_bounds__(kWave)
+    k_couple_rev(const double2* __restrict__ S, const int64_t* __restrict__ rptr, const int32_t* __restrict__ rtest,
+                 const int64_t* __restrict__ roff, const int32_t* __restrict__ cptr_src,
+                 const int32_t* __restrict__ cptr_dst, const double2* __restrict__ xh, int64_t R, int64_t q0,
+                 double2* __restrict__ yh) {
+    const int s = blockIdx.x;
+    const int rs = cptr_dst[s + 1] - cptr_dst[s];
+    if (rs == 0) return;
+    for (int a = threadIdx.x; a < rs; a += kWave) {
+        double2 acc[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] = make_double2(0.0, 0.0);
+        for (int64_t e = rptr[s]; e < rptr[s + 1]; ++e) {
+            const int t = rtest[e];
+            const int rt = cptr_src[t + 1] - cptr_src[t];
+            const double2* Sb = S + roff[e] + (int64_t)a * rt;
+            const double2* xv0 = xh + (int64_t)cptr_src[t] * R + q0;
+            for (int j = 0; j < rt; ++j) {
+                const double2 sv = ldc<CJ>(Sb, j);
+#pragma unroll
+                for (int q = 0; q < NR; ++q) acc[q] = cfma(sv, xv0[(int64_t)j * R + q], acc[q]);
+            }
+        }
+        double2* out = yh + (int64_t)(cptr_dst[s] + a) * R + q0;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) out[q] = cadd(out[q], acc[q]);
+    }
+}
+
+// Stage 5 + near field, one wave per destination leaf; lane = (row, slot).
+// FWD: destination = test side, G(x_dst, y_src); else G(x_src, y_dst).
+template <int NR, bool FWD, bool CJ>
+__global__ void __launch_bounds__(kWave)
+    k_leaf_out(const int32_t* __restrict__ leaves, const int32_t* __restrict__ leafbasis,
+               const int64_t* __restrict__ boff, const double2* __restrict__ bblocks, const int32_t* __restrict__ dlo,
+               const int32_t* __restrict__ dhi, const int32_t* __restrict__ dperm, const int32_t* __restrict__ dcptr,
+               const double* __restrict__ dpts, const int64_t* __restrict__ nearptr,
+               const int32_t* __restrict__ nearnodes, const int32_t* __restrict__ slo, const int32_t* __restrict__ shi,
+               const double* __restrict__ spts, const double2* __restrict__ yh, const double2* __restrict__ xs,
+               int64_t R, int64_t q0, double kw, int sign, int nearon, double2 alpha, double2 beta,
+               double2* __restrict__ y, int64_t ldy) {
+    __shared__ double2 red[kWave];
+    const int t = leaves[blockIdx.x];
+    const int bi = leafbasis[blockIdx.x];
+    const int l0 = dlo[t], nt = dhi[t] - l0;
+    const int rt = dcptr[t + 1] - dcptr[t];
+    const int lane = threadIdx.x;
+    for (int r0 = 0; r0 < nt; r0 += kWave) {
+        const int rows = min(nt - r0, kWave);
+        const int ncs = kWave / rows;
+        const int row = r0 + lane % rows, slot = lane / rows;
+        double2 acc[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] = make_double2(0.0, 0.0);
+        if (slot < ncs) {
+            if (bi >= 0) {
+                const double2* B = bblocks + boff[bi];
+                for (int j = slot; j < rt; j += ncs) {
+                    // FWD: U (nt x r); else V (r x nt) transposed
+                    const double2 b = ldc<CJ>(B, FWD ? (int64_t)row + (int64_t)j * nt : (int64_t)j + (int64_t)row * rt);
+                    const double2* hv = yh + (int64_t)(dcptr[t] + j) * R + q0;
+#pragma unroll
+                    for (int q = 0; q < NR; ++q) acc[q] = cfma(b, hv[q], acc[q]);
+                }
+            }
+            if (nearon) {
+                const double x0 = dpts[3 * (l0 + row)], x1 = dpts[3 * (l0 + row) + 1], x2 = dpts[3 * (l0 + row) + 2];
+                for (int64_t e = nearptr[t]; e < nearptr[t + 1]; ++e) {
+                    const int s = nearnodes[e];
+                    for (int b = slo[s] + slot; b < shi[s]; b += ncs) {
+                        const double y0 = spts[3 * b], y1 = spts[3 * b + 1], y2 = spts[3 * b + 2];
+                        double2 g = FWD ? helmholtz(kw, sign, x0, x1, x2, y0, y1, y2)
+                                        : helmholtz(kw, sign, y0, y1, y2, x0, x1, x2);
+                        if (CJ) g.y = -g.y;
+                        const double2* xv = xs + (int64_t)b * R + q0;
+#pragma unroll
+                        for (int q = 0; q < NR; ++q) acc[q] = cfma(g, xv[q], acc[q]);
+                    }
+                }
+            }
+        }
+        const int64_t dst = dperm[l0 + row];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            __syncthreads();
+            red[lane] = (slot < ncs) ? acc[q] : make_double2(0.0, 0.0);
+            __syncthreads();
+            if (slot == 0) {
+                double2 sum = red[lane];
+                for (int c = 1; c < ncs; ++c) sum = cadd(sum, red[lane + c * rows]);
+                double2* out = y + dst + (q0 + q) * ldy;
+                double2 v = cmul(alpha, sum);
+                if (beta.x != 0.0 || beta.y != 0.0) v = cadd(v, cmul(beta, *out));
+                *out = v;
+            }
+        }
+    }
+}
+
+inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+template <int NR, int OP>
+void run_chunk(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, int64_t R, int64_t q0,
+               double2 alpha, double2 beta) {
+    constexpr bool FWD = OP == NCA_OP_N;
+    constexpr bool CJ = OP == NCA_OP_C;
+    hipStream_t st = h->ctx->stream;
+    SideDev& src = FWD ? h->trial : h->test;
+    SideDev& dst = FWD ? h->test : h->trial;
+    double2* xs = h->ws_xs.p;
+    double2* xh = h->ws_xh.p;
+    double2* yh = h->ws_yh.p;
+    if (src.nbasis > 0)
+        hipLaunchKernelGGL((k_project<NR, !FWD, CJ>), dim3((unsigned)src.nbasis), dim3(kWave), 0, st, src.bnodes.p,
+                           src.boff.p, src.bblocks.p, src.lo.p, src.hi.p, src.cptr.p, xs, R, q0, xh);
+    // upward: deepest level first; source transfers (trial E forward, test E transposed)
+    for (int64_t l = src.nlev - 1; l >= 0; --l) {
+        const int64_t np = src.levptr[l + 1] - src.levptr[l];
+        if (np == 0) continue;
+        hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ>), dim3((unsigned)np), dim3(kWave), 0, st, src.parents.p,
+                           src.chdptr.p, src.chds.p, src.toff.p, src.tblocks.p, src.cptr.p, src.levptr[l], R, q0, xh);
+    }
+    const double2* S = h->coupling->blocks.p;
+    if (FWD) {
+        if (h->coupling->nentries > 0)
+            hipLaunchKernelGGL((k_couple_fwd<NR, CJ>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->f_base.p,
+                               h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
+    } else {
+        if (h->coupling->nentries > 0)
+            hipLaunchKernelGGL((k_couple_rev<NR, CJ>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->r_ptr.p,
+                               h->r_test.p, h->r_off.p, src.cptr.p, dst.cptr.p, xh, R, q0, yh);
+    }
+    // downward: root level first; destination transfers (test E forward, trial E transposed)
+    for (int64_t l = 0; l < dst.nlev; ++l) {
+        const int64_t np = dst.levptr[l + 1] - dst.levptr[l];
+        if (np == 0) continue;
+        hipLaunchKernelGGL((k_transfer<NR, true, !FWD, CJ>), dim3((unsigned)np), dim3(kWave), 0, st, dst.parents.p,
+                           dst.chdptr.p, dst.chds.p, dst.toff.p, dst.tblocks.p, dst.cptr.p, dst.levptr[l], R, q0, yh);
+    }
+    if (dst.nleaves > 0)
+        hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st, dst.leaves.p,
+                           dst.leafbasis.p, dst.boff.p, dst.bblocks.p, dst.lo.p, dst.hi.p, dst.perm.p, dst.cptr.p,
+                           dst.pts.p, dst.nearptr.p, dst.nearnodes.p, src.lo.p, src.hi.p, src.pts.p, yh, xs, R, q0,
+                           h->k, h->sign, 1, alpha, beta, y, ldy);
+    NCA_HIP(hipGetLastError());
+}
+
+template <int OP>
+void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, int64_t R, double2 alpha,
+            double2 beta) {
+    constexpr bool FWD = OP == NCA_OP_N;
+    hipStream_t st = h->ctx->stream;
+    SideDev& src = FWD ? h->trial : h->test;
+    SideDev& dst = FWD ? h->test : h->trial;
+    if (src.npts * R > 0)
+        hipLaunchKernelGGL(k_gather, dim3(blocks_for(src.npts * R, 256)), dim3(256), 0, st, x, ldx, src.perm.p,
+                           src.npts, R, h->ws_xs.p);
+    if (src.K * R > 0)
+        hipLaunchKernelGGL(k_fill, dim3(blocks_for(src.K * R, 256)), dim3(256), 0, st, h->ws_xh.p, src.K * R);
+    if (dst.K * R > 0)
+        hipLaunchKernelGGL(k_fill, dim3(blocks_for(dst.K * R, 256)), dim3(256), 0, st, h->ws_yh.p, dst.K * R);
+    int64_t q0 = 0;
+    while (q0 < R) {
+        const int64_t left = R - q0;
+        if (left >= 16) { run_chunk<16, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta); q0 += 16; }
+        else if (left >= 8) { run_chunk<8, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta); q0 += 8; }
+        else if (left >= 4) { run_chunk<4, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta); q0 += 4; }
+        else if (left >= 2) { run_chunk<2, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta); q0 += 2; }
+        else { run_chunk<1, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta); q0 += 1; }
+    }
+}
+
+void ensure_workspace(nca_h2* h, int64_t R) {
+    if (R <= h->ws_rhs) return;
+    const int64_t np = std::max(h->M, h->N), K = std::max(h->test.K, h->trial.K);
+    h->ws_xs.alloc((size_t)std::max<int64_t>(np * R, 1));
+    h->ws_xh.alloc((size_t)std::max<int64_t>(K * R, 1));
+    h->ws_yh.alloc((size_t)std::max<int64_t>(K * R, 1));
+    h->ws_rhs = R;
+}
+
+void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t ldx, double2* y, int64_t ldy,
+                   const double* alpha, const double* beta) {
+    NCA_REQUIRE(op >= NCA_OP_N && op <= NCA_OP_C, NCA_ERR_ARGUMENT, "nca_h2_matvec: op must be 0 (N), 1 (T) or 2 (C)");
+    NCA_REQUIRE(R >= 0, NCA_ERR_ARGUMENT, "nca_h2_matvec: negative nrhs");
+    const int64_t nx = op == NCA_OP_N ? h->N : h->M, ny = op == NCA_OP_N ? h->M : h->N;
+    NCA_REQUIRE(ldx >= nx && ldy >= ny, NCA_ERR_DIMENSION, "DimensionMismatch: leading dimension too small");
+    if (R == 0 || ny == 0) return;
+    ensure_workspace(h, R);
+    const double2 al = make_double2(alpha ? alpha[0] : 1.0, alpha ? alpha[1] : 0.0);
+    const double2 be = make_double2(beta ? beta[0] : 0.0, beta ? beta[1] : 0.0);
+    if (op == NCA_OP_N) run_op<NCA_OP_N>(h, x, ldx, y, ldy, R, al, be);
+    else if (op == NCA_OP_T) run_op<NCA_OP_T>(h, x, ldx, y, ldy, R, al, be);
+    else run_op<NCA_OP_C>(h, x, ldx, y, ldy, R, al, be);
+}
+
+// ---------------------------------------------------------------------------
+// Creation: 1-based Julia-shaped plans -> device arrays.
+void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, const int64_t* perm, const int64_t* lo,
+                const int64_t* hi, const int64_t* cptr, const nca_points* pts, int64_t nbasis, const int64_t* bnodes,
+                const int64_t* boff, const double* bblocks, bool testside, int64_t nlev, const int64_t* levptr,
+                const int64_t* levidcs, const int64_t* chdptr, const int64_t* chds, const int64_t* toff,
+                const double* tblocks) {
+    sd.npts = npts;
+    sd.nn = nn;
+    NCA_REQUIRE(pts->n == npts, NCA_ERR_DIMENSION, "nca_h2_create: point count differs from the matrix size");
+    sd.h_perm.resize(npts);
+    std::vector<char> seen(npts, 0);
+    for (int64_t i = 0; i < npts; ++i) {
+        const int64_t p = perm[i] - 1;
+        NCA_REQUIRE(p >= 0 && p < npts && !seen[p], NCA_ERR_ARGUMENT, "nca_h2_create: perm is not a permutation");
+        seen[p] = 1;
+        sd.h_perm[i] = (int32_t)p;
+    }
+    sd.h_lo.resize(nn);
+    sd.h_hi.resize(nn);
+    sd.h_cptr.resize(nn + 1);
+    for (int64_t t = 0; t < nn; ++t) {
+        sd.h_lo[t] = (int32_t)(lo[t] - 1);
+        sd.h_hi[t] = (int32_t)hi[t];
+        NCA_REQUIRE(sd.h_lo[t] >= 0 && sd.h_hi[t] >= sd.h_lo[t] && sd.h_hi[t] <= npts, NCA_ERR_ARGUMENT,
+                    "nca_h2_create: node range out of bounds");
+    }
+    for (int64_t t = 0; t <= nn; ++t) sd.h_cptr[t] = (int32_t)(cptr[t] - 1);
+    for (int64_t t = 0; t < nn; ++t)
+        NCA_REQUIRE(sd.h_cptr[t + 1] >= sd.h_cptr[t], NCA_ERR_ARGUMENT, "nca_h2_create: decreasing coefficient plan");
+    sd.K = sd.h_cptr[nn];
+    std::vector<double> pp(3 * npts);
+    for (int64_t i = 0; i < npts; ++i)
+        for (int d = 0; d < 3; ++d) pp[3 * i + d] = pts->host[3 * sd.h_perm[i] + d];
+    sd.perm.upload(sd.h_perm, st);
+    sd.lo.upload(sd.h_lo, st);
+    sd.hi.upload(sd.h_hi, st);
+    sd.cptr.upload(sd.h_cptr, st);
+    sd.pts.upload(pp, st);
+    // bases
+    sd.nbasis = nbasis;
+    std::vector<int32_t> bn(nbasis);
+    std::vector<int64_t> bo(nbasis + 1);
+    for (int64_t i = 0; i < nbasis; ++i) {
+        const int64_t t = bnodes[i] - 1;
+        NCA_REQUIRE(t >= 0 && t < nn, NCA_ERR_ARGUMENT, "nca_h2_create: basis node out of range");
+        bn[i] = (int32_t)t;
+        const int64_t expect = (int64_t)(sd.h_hi[t] - sd.h_lo[t]) * (sd.h_cptr[t + 1] - sd.h_cptr[t]);
+        NCA_REQUIRE(boff[i + 1] - boff[i] == expect, NCA_ERR_DIMENSION,
+                    std::string("DimensionMismatch: ") + (testside ? "test" : "trial") + " basis of node " +
+                        std::to_string(t + 1) + " has " + std::to_string(boff[i + 1] - boff[i]) + " entries, " +
+                        std::to_string(expect) + " expected");
+    }
+    for (int64_t i = 0; i <= nbasis; ++i) bo[i] = nbasis ? boff[i] - boff[0] : 0;
+    sd.basis_entries = nbasis ? bo[nbasis] : 0;
+    sd.bnodes.upload(bn, st);
+    sd.boff.upload(bo, st);
+    if (nbasis) sd.bblocks.upload(reinterpret_cast<const double2*>(bblocks) + boff[0], (size_t)bo[nbasis], st);
+    // transfers
+    sd.nlev = nlev;
+    sd.levptr.assign(nlev + 1, 0);
+    const int64_t npar = nlev ? levptr[nlev] - levptr[0] : 0;
+    for (int64_t l = 0; l <= nlev; ++l) sd.levptr[l] = nlev ? levptr[l] - levptr[0] : 0;
+    std::vector<int32_t> par(npar), ch;
+    std::vector<int64_t> cptr0(npar + 1, 0), to;
+    const int64_t nedge = npar ? chdptr[npar] - chdptr[0] : 0;
+    ch.resize(nedge);
+    to.resize(nedge + 1);
+    std::vector<char> ischild(nn, 0);
+    for (int64_t i = 0; i < npar; ++i) {
+        const int64_t p = levidcs[i] - 1;
+        NCA_REQUIRE(p >= 0 && p < nn, NCA_ERR_ARGUMENT, "nca_h2_create: transfer parent out of range");
+        par[i] = (int32_t)p;
+        cptr0[i + 1] = chdptr[i + 1] - chdptr[0];
+        for (int64_t e = chdptr[i] - chdptr[0]; e < chdptr[i + 1] - chdptr[0]; ++e) {
+            const int64_t c = chds[e] - 1;
+            NCA_REQUIRE(c >= 0 && c < nn && !ischild[c], NCA_ERR_ARGUMENT,
+                        "nca_h2_create: transfer child out of range or with two parents");
+            ischild[c] = 1;
+            ch[e] = (int32_t)c;
+            const int64_t rp = sd.h_cptr[p + 1] - sd.h_cptr[p], rc = sd.h_cptr[c + 1] - sd.h_cptr[c];
+            NCA_REQUIRE(toff[e + 1] - toff[e] == rp * rc, NCA_ERR_DIMENSION,
+                        "DimensionMismatch: transfer block of child " + std::to_string(c + 1) + " has " +
+                            std::to_string(toff[e + 1] - toff[e]) + " entries, " + std::to_string(rp * rc) +
+                            " expected");
+        }
+    }
+    for (int64_t e = 0; e <= nedge; ++e) to[e] = nedge ? toff[e] - toff[0] : 0;
+    sd.transfer_entries = nedge ? to[nedge] : 0;
+    sd.parents.upload(par, st);
+    sd.chdptr.upload(cptr0, st);
+    sd.chds.upload(ch, st);
+    sd.toff.upload(to, st);
+    if (nedge && to[nedge]) sd.tblocks.upload(reinterpret_cast<const double2*>(tblocks) + toff[0], (size_t)to[nedge], st);
+}
+
+// Leaves of the side = nodes whose range is not split by any other node: a
+// node is a leaf iff it is nobody's transfer parent and no node's range is a
+// strict subset starting inside it. With ranges only, a leaf is a node that no
+// other node lies strictly inside; computed by sorting ranges.
+void build_leaves(SideDev& sd, hipStream_t st) {
+    const int64_t nn = sd.nn;
+    std::vector<int64_t> order(nn);
+    for (int64_t i = 0; i < nn; ++i) order[i] = i;
+    // sort by (lo asc, hi desc): a node is a leaf iff the next node in this
+    // order does not start inside its range (nested ranges follow their parent)
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        if (sd.h_lo[a] != sd.h_lo[b]) return sd.h_lo[a] < sd.h_lo[b];
+        if (sd.h_hi[a] != sd.h_hi[b]) return sd.h_hi[a] > sd.h_hi[b];
+        return a < b;
+    });
+    std::vector<int32_t> leaves;
+    std::vector<char> isleaf(nn, 0);
+    for (int64_t k = 0; k < nn; ++k) {
+        const int64_t a = order[k];
+        if (sd.h_hi[a] == sd.h_lo[a]) continue;
+        bool leaf = true;
+        if (k + 1 < nn) {
+            const int64_t b = order[k + 1];
+            if (sd.h_lo[b] < sd.h_hi[a]) leaf = false;
+        }
+        if (leaf) { isleaf[a] = 1; }
+    }
+    std::vector<int64_t> covered(sd.npts + 1, 0);
+    for (int64_t a = 0; a < nn; ++a)
+        if (isleaf[a]) {
+            leaves.push_back((int32_t)a);
+            covered[sd.h_lo[a]] += 1;
+            covered[sd.h_hi[a]] -= 1;
+        }
+    int64_t run = 0;
+    for (int64_t i = 0; i < sd.npts; ++i) {
+        run += covered[i];
+        NCA_REQUIRE(run == 1, NCA_ERR_ARGUMENT, "nca_h2_create: the leaves do not partition the points");
+    }
+    sd.nleaves = (int64_t)leaves.size();
+    sd.leaves.upload(leaves, st);
+}
+
+}  // namespace nca
+
+using namespace nca;
+
+extern "C" {
+
+int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test, const nca_points* trial,
+                      const nca_h2_desc* d, const nca_coupling* coupling, int32_t precision, nca_h2** out) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && kernel && test && trial && d && coupling && out, NCA_ERR_ARGUMENT,
+                    "nca_h2_create: invalid argument");
+        NCA_REQUIRE(precision == NCA_PREC_FP64 || precision == NCA_PREC_MIXED, NCA_ERR_ARGUMENT,
+                    "nca_h2_create: unknown precision");
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        auto h = std::make_unique<nca_h2>();
+        h->ctx = ctx;
+        h->k = kernel->wavenumber;
+        h->sign = kernel->sign;
+        h->precision = precision;
+        h->M = d->M;
+        h->N = d->N;
+        h->coupling = coupling;
+        build_side(h->test, st, d->M, d->nn_test, d->test_perm, d->test_lo, d->test_hi, d->test_cptr, test,
+                   d->nbasis_test, d->basis_test_nodes, d->basis_test_off, d->basis_test, true, d->nlev_test,
+                   d->tt_levptr, d->tt_levidcs, d->tt_chdptr, d->tt_chds, d->tt_off, d->tt_blocks);
+        build_side(h->trial, st, d->N, d->nn_trial, d->trial_perm, d->trial_lo, d->trial_hi, d->trial_cptr, trial,
+                   d->nbasis_trial, d->basis_trial_nodes, d->basis_trial_off, d->basis_trial, false, d->nlev_trial,
+                   d->st_levptr, d->st_levidcs, d->st_chdptr, d->st_chds, d->st_off, d->st_blocks);
+        build_leaves(h->test, st);
+        build_leaves(h->trial, st);
+        // basis slot of every leaf
+        for (SideDev* sd : {&h->test, &h->trial}) {
+            std::vector<int32_t> slot(sd->nn, -1), lb;
+            std::vector<int32_t> bn(sd->nbasis);
+            if (sd->nbasis) NCA_HIP(hipMemcpy(bn.data(), sd->bnodes.p, sizeof(int32_t) * sd->nbasis, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < sd->nbasis; ++i) slot[bn[i]] = (int32_t)i;
+            std::vector<int32_t> lv(sd->nleaves);
+            if (sd->nleaves) NCA_HIP(hipMemcpy(lv.data(), sd->leaves.p, sizeof(int32_t) * sd->nleaves, hipMemcpyDeviceToHost));
+            std::vector<char> isleaf(sd->nn, 0);
+            for (int32_t l : lv) {
+                lb.push_back(slot[l]);
+                isleaf[l] = 1;
+            }
+            for (int64_t i = 0; i < sd->nbasis; ++i)
+                NCA_REQUIRE(isleaf[bn[i]], NCA_ERR_ARGUMENT, "nca_h2_create: a basis belongs to a node that is not a leaf");
+            sd->leafbasis.upload(lb, st);
+        }
+        // coupling plan must match the store
+        const int64_t ntest = d->nn_test;
+        NCA_REQUIRE(coupling->ntest == ntest && coupling->ntrial == d->nn_trial, NCA_ERR_DIMENSION,
+                    "nca_h2_create: coupling store of another tree pair");
+        for (int64_t t = 0; t <= ntest; ++t)
+            NCA_REQUIRE(d->cp_ptr[t] - d->cp_ptr[0] == coupling->ptr[t], NCA_ERR_ARGUMENT,
+                        "nca_h2_create: coupling plan differs from the coupling store");
+        for (int64_t i = 0; i < coupling->nblocks; ++i)
+            NCA_REQUIRE(d->cp_nodes[i] - 1 == coupling->nodes[i], NCA_ERR_ARGUMENT,
+                        "nca_h2_create: coupling plan differs from the coupling store");
+        for (int64_t t = 0; t < ntest; ++t)
+            NCA_REQUIRE(coupling->testrank[t] == h->test.h_cptr[t + 1] - h->test.h_cptr[t], NCA_ERR_DIMENSION,
+                        "nca_h2_create: coupling ranks differ from the coefficient plan");
+        for (int64_t s = 0; s < d->nn_trial; ++s)
+            NCA_REQUIRE(coupling->trialrank[s] == h->trial.h_cptr[s + 1] - h->trial.h_cptr[s], NCA_ERR_DIMENSION,
+                        "nca_h2_create: coupling ranks differ from the coefficient plan");
+        // forward coupling: per test node base/colbase/ncols; per column its source coefficient
+        {
+            std::vector<int64_t> base(ntest), colbase(ntest);
+            std::vector<int32_t> ncols(ntest), colcoef;
+            for (int64_t t = 0; t < ntest; ++t) {
+                base[t] = coupling->offsets[coupling->ptr[t]];
+                colbase[t] = (int64_t)colcoef.size();
+                const int rt = coupling->testrank[t];
+                int64_t C = 0;
+                if (rt > 0)
+                    for (int64_t i = coupling->ptr[t]; i < coupling->ptr[t + 1]; ++i) {
+                        const int64_t s = coupling->nodes[i];
+                        for (int j = 0; j < coupling->trialrank[s]; ++j) colcoef.push_back(h->trial.h_cptr[s] + j);
+                        C += coupling->trialrank[s];
+                    }
+                ncols[t] = (int32_t)C;
+            }
+            h->f_base.upload(base, st);
+            h->f_colbase.upload(colbase, st);
+            h->f_ncols.upload(ncols, st);
+            h->f_colcoef.upload(colcoef, st);
+            h->stats.plan_bytes += (int64_t)colcoef.size() * 4 + ntest * 20;
+        }
+        // reverse coupling: blocks by trial node
+        {
+            const int64_t ntrial = d->nn_trial;
+            std::vector<int64_t> rptr(ntrial + 1, 0), roff(coupling->nblocks);
+            std::vector<int32_t> rtest(coupling->nblocks);
+            for (int64_t i = 0; i < coupling->nblocks; ++i) rptr[coupling->nodes[i] + 1]++;
+            for (int64_t s = 0; s < ntrial; ++s) rptr[s + 1] += rptr[s];
+            std::vector<int64_t> cur(rptr.begin(), rptr.end() - 1);
+            for (int64_t t = 0; t < ntest; ++t)
+                for (int64_t i = coupling->ptr[t]; i < coupling->ptr[t + 1]; ++i) {
+                    const int64_t s = coupling->nodes[i];
+                    rtest[cur[s]] = (int32_t)t;
+                    roff[cur[s]] = coupling->offsets[i];
+                    cur[s]++;
+                }
+            h->r_ptr.upload(rptr, st);
+            h->r_off.upload(roff, st);
+            h->r_test.upload(rtest, st);
+        }
+        // near lists, both orientations
+        {
+            const int64_t ntrial = d->nn_trial;
+            const int64_t nnz = d->nearptr[ntest] - d->nearptr[0];
+            std::vector<int64_t> np(ntest + 1), tp(ntrial + 1, 0);
+            std::vector<int32_t> nn32(nnz), tn(nnz);
+            int64_t near_entries = 0;
+            for (int64_t t = 0; t <= ntest; ++t) np[t] = d->nearptr[t] - d->nearptr[0];
+            for (int64_t t = 0; t < ntest; ++t)
+                for (int64_t i = np[t]; i < np[t + 1]; ++i) {
+                    const int64_t s = d->nearnodes[i] - 1;
+                    NCA_REQUIRE(s >= 0 && s < ntrial, NCA_ERR_ARGUMENT, "nca_h2_create: near node out of range");
+                    nn32[i] = (int32_t)s;
+                    tp[s + 1]++;
+                    near_entries += (int64_t)(h->test.h_hi[t] - h->test.h_lo[t]) * (h->trial.h_hi[s] - h->trial.h_lo[s]);
+                }
+            for (int64_t s = 0; s < ntrial; ++s) tp[s + 1] += tp[s];
+            std::vector<int64_t> cur(tp.begin(), tp.end() - 1);
+            for (int64_t t = 0; t < ntest; ++t)
+                for (int64_t i = np[t]; i < np[t + 1]; ++i) tn[cur[nn32[i]]++] = (int32_t)t;
+            h->test.nearptr.upload(np, st);
+            h->test.nearnodes.upload(nn32, st);
+            h->trial.nearptr.upload(tp, st);
+            h->trial.nearnodes.upload(tn, st);
+            h->stats.near_pairs = nnz;
+            h->stats.near_entries = near_entries;
+            h->stats.plan_bytes += nnz * 12 + (ntest + ntrial) * 8;
+        }
+        h->stats.coupling_blocks = coupling->nblocks;
+        h->stats.coupling_entries = coupling->nentries;
+        h->stats.basis_entries_test = h->test.basis_entries;
+        h->stats.basis_entries_trial = h->trial.basis_entries;
+        h->stats.transfer_entries_test = h->test.transfer_entries;
+        h->stats.transfer_entries_trial = h->trial.transfer_entries;
+        h->stats.coeffs_test = h->test.K;
+        h->stats.coeffs_trial = h->trial.K;
+        NCA_HIP(hipStreamSynchronize(st));
+        *out = h.release();
+    });
+}
+
+int32_t nca_h2_destroy(nca_h2* h2) {
+    return guarded([&] {
+        if (!h2) return;
+        set_device(h2->ctx);
+        (void)hipStreamSynchronize(h2->ctx->stream);
+        delete h2;
+    });
+}
+
+int32_t nca_h2_set_precision(nca_h2* h2, int32_t precision) {
+    return guarded([&] {
+        NCA_REQUIRE(h2, NCA_ERR_ARGUMENT, "nca_h2_set_precision: null operator");
+        NCA_REQUIRE(precision == NCA_PREC_FP64 || precision == NCA_PREC_MIXED, NCA_ERR_ARGUMENT,
+                    "nca_h2_set_precision: unknown precision");
+        h2->precision = precision;
+    });
+}
+
+int32_t nca_h2_matvec_device(nca_h2* h2, int32_t op, int64_t nrhs, const void* x, int64_t ldx, void* y,
+                             int64_t ldy, const double* alpha, const double* beta) {
+    return guarded([&] {
+        NCA_REQUIRE(h2 && (nrhs == 0 || (x && y)), NCA_ERR_ARGUMENT, "nca_h2_matvec_device: invalid argument");
+        set_device(h2->ctx);
+        matvec_device(h2, op, nrhs, static_cast<const double2*>(x), ldx, static_cast<double2*>(y), ldy, alpha, beta);
+    });
+}
+
+int32_t nca_h2_matvec(nca_h2* h2, int32_t op, int64_t nrhs, const double* x, int64_t ldx, double* y, int64_t ldy,
+                      const double* alpha, const double* beta) {
+    return guarded([&] {
+        NCA_REQUIRE(h2 && (nrhs == 0 || (x && y)), NCA_ERR_ARGUMENT, "nca_h2_matvec: invalid argument");
+        NCA_REQUIRE(op >= NCA_OP_N && op <= NCA_OP_C, NCA_ERR_ARGUMENT, "nca_h2_matvec: op must be 0 (N), 1 (T) or 2 (C)");
+        set_device(h2->ctx);
+        const int64_t nx = op == NCA_OP_N ? h2->N : h2->M, ny = op == NCA_OP_N ? h2->M : h2->N;
+        NCA_REQUIRE(ldx >= nx && ldy >= ny, NCA_ERR_DIMENSION, "DimensionMismatch: leading dimension too small");
+        if (nrhs == 0 || ny == 0) return;
+        hipStream_t st = h2->ctx->stream;
+        h2->ws_xin.alloc((size_t)std::max<int64_t>(nx * nrhs, 1));
+        h2->ws_yio.alloc((size_t)(ny * nrhs));
+        if (nx > 0)
+            NCA_HIP(hipMemcpy2DAsync(h2->ws_xin.p, nx * sizeof(double2), x, ldx * sizeof(double2), nx * sizeof(double2),
+                                     nrhs, hipMemcpyHostToDevice, st));
+        const bool usey = beta && (beta[0] != 0.0 || beta[1] != 0.0);
+        if (usey)
+            NCA_HIP(hipMemcpy2DAsync(h2->ws_yio.p, ny * sizeof(double2), y, ldy * sizeof(double2), ny * sizeof(double2),
+                                     nrhs, hipMemcpyHostToDevice, st));
+        matvec_device(h2, op, nrhs, h2->ws_xin.p, nx, h2->ws_yio.p, ny, alpha, beta);
+        NCA_HIP(hipMemcpy2DAsync(y, ldy * sizeof(double2), h2->ws_yio.p, ny * sizeof(double2), ny * sizeof(double2),
+                                 nrhs, hipMemcpyDeviceToHost, st));
+        NCA_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int32_t nca_h2_get_stats(const nca_h2* h2, nca_h2_stats* out) {
+    return guarded([&] {
+        NCA_REQUIRE(h2 && out, NCA_ERR_ARGUMENT, "nca_h2_get_stats: invalid argument");
+        *out = h2->stats;
+    });
+}
+
+}  // extern "C"

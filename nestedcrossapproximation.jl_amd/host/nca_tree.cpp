@@ -1,0 +1,498 @@
+// Host-side cluster trees, admissibility partition and geometric far-pivot
+// selection (Phase 1 of the batched bottom-up compressor).
+//
+// These are the inputs of the GPU hot path. In the reference they come from
+// third-party Julia packages that are not vendored (H2Trees.jl for the trees,
+// AdaptiveCrossApproximation.jl for `interactions`/`isnear` and
+// `TreeMimicryPivoting`); the functions here are deterministic stand-ins with
+// the same contracts:
+//   * a node's values are a contiguous range of a point permutation, children
+//     partition their parent, node ids are assigned level by level;
+//   * far/near CSR by test node (reference FarData, farinteractions.jl:16-19);
+//   * far field of a node = own far nodes followed by those of every ancestor
+//     (farinteractions.jl:38-60);
+//   * the number of far pivots follows _selectfarpivots/_clustersize
+//     (batchedbottomup.jl:54-72, pivots.jl:15-25).
+// All indices in this file are 0-based.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include <omp.h>
+
+namespace {
+
+struct Tree {
+    int64_t n = 0;
+    std::vector<int64_t> parent, level, lo, hi, childptr, children, perm;
+    std::vector<double> center, radius;
+};
+
+struct Partition {
+    std::vector<int64_t> nearptr, nearnodes, farptr, farnodes;
+};
+
+struct Csr {
+    std::vector<int64_t> ptr, idx;
+};
+
+inline double dist2(const double* a, const double* b) {
+    double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+    return dx * dx + dy * dy + dz * dz;
+}
+
+void finish_children(Tree& t) {
+    const int64_t nn = (int64_t)t.parent.size();
+    std::vector<int64_t> count(nn + 1, 0);
+    for (int64_t i = 1; i < nn; ++i) count[t.parent[i] + 1]++;
+    t.childptr.assign(nn + 1, 0);
+    for (int64_t i = 0; i < nn; ++i) t.childptr[i + 1] = t.childptr[i] + count[i + 1];
+    t.children.assign(t.childptr[nn], 0);
+    std::vector<int64_t> cur(t.childptr.begin(), t.childptr.end() - 1);
+    for (int64_t i = 1; i < nn; ++i) t.children[cur[t.parent[i]]++] = i;
+}
+
+// Regular 2^3 tree: the bounding cube is halved while a box holds more than
+// `minvalues` points and its half size exceeds `minhalfsize`; empty octants are
+// dropped. Radius of a box = half size * sqrt(3) (the ball around the cube).
+Tree build_twon(const double* pts, int64_t n, int64_t minvalues, double minhalfsize) {
+    Tree t;
+    t.n = n;
+    t.perm.resize(n);
+    for (int64_t i = 0; i < n; ++i) t.perm[i] = i;
+    double mn[3] = {std::numeric_limits<double>::max(), std::numeric_limits<double>::max(),
+                    std::numeric_limits<double>::max()};
+    double mx[3] = {-mn[0], -mn[1], -mn[2]};
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = std::min(mn[d], pts[3 * i + d]);
+            mx[d] = std::max(mx[d], pts[3 * i + d]);
+        }
+    if (n == 0) {
+        for (int d = 0; d < 3; ++d) mn[d] = mx[d] = 0.0;
+    }
+    double h = 0.0;
+    for (int d = 0; d < 3; ++d) h = std::max(h, 0.5 * (mx[d] - mn[d]));
+    if (h <= 0.0) h = 1e-300;
+    std::vector<double> half;
+    t.parent.push_back(-1);
+    t.level.push_back(0);
+    t.lo.push_back(0);
+    t.hi.push_back(n);
+    for (int d = 0; d < 3; ++d) t.center.push_back(0.5 * (mn[d] + mx[d]));
+    half.push_back(h);
+    std::vector<int64_t> tmp(n);
+    for (int64_t node = 0; node < (int64_t)t.parent.size(); ++node) {
+        const int64_t lo = t.lo[node], hi = t.hi[node];
+        const double hs = half[node];
+        if (hi - lo <= minvalues || hs <= minhalfsize) continue;
+        const double c[3] = {t.center[3 * node], t.center[3 * node + 1], t.center[3 * node + 2]};
+        int64_t cnt[8] = {0};
+        auto octant = [&](int64_t p) {
+            const double* x = pts + 3 * p;
+            return (x[0] >= c[0] ? 1 : 0) | (x[1] >= c[1] ? 2 : 0) | (x[2] >= c[2] ? 4 : 0);
+        };
+        for (int64_t k = lo; k < hi; ++k) cnt[octant(t.perm[k])]++;
+        int nonempty = 0;
+        for (int o = 0; o < 8; ++o) nonempty += cnt[o] > 0;
+        if (nonempty <= 1 && hs <= 1e-12) continue;
+        int64_t start[8];
+        int64_t s = lo;
+        for (int o = 0; o < 8; ++o) {
+            start[o] = s;
+            s += cnt[o];
+        }
+        int64_t pos[8];
+        std::memcpy(pos, start, sizeof(pos));
+        for (int64_t k = lo; k < hi; ++k) tmp[pos[octant(t.perm[k])]++] = t.perm[k];
+        std::memcpy(&t.perm[lo], &tmp[lo], sizeof(int64_t) * (hi - lo));
+        for (int o = 0; o < 8; ++o) {
+            if (cnt[o] == 0) continue;
+            t.parent.push_back(node);
+            t.level.push_back(t.level[node] + 1);
+            t.lo.push_back(start[o]);
+            t.hi.push_back(start[o] + cnt[o]);
+            const double q = 0.5 * hs;
+            t.center.push_back(c[0] + ((o & 1) ? q : -q));
+            t.center.push_back(c[1] + ((o & 2) ? q : -q));
+            t.center.push_back(c[2] + ((o & 4) ? q : -q));
+            half.push_back(q);
+        }
+    }
+    const int64_t nn = (int64_t)t.parent.size();
+    t.radius.resize(nn);
+    for (int64_t i = 0; i < nn; ++i) t.radius[i] = half[i] * std::sqrt(3.0);
+    finish_children(t);
+    return t;
+}
+
+// Deterministic k-means splitting: seeds by farthest-point traversal from the
+// point farthest from the centroid, Lloyd iterations with ties to the lower
+// cluster. Center = centroid, radius = largest distance to it (bounding sphere).
+void centroid_radius(const double* pts, const int64_t* ids, int64_t m, double* c, double* r) {
+    double s[3] = {0, 0, 0};
+    for (int64_t k = 0; k < m; ++k)
+        for (int d = 0; d < 3; ++d) s[d] += pts[3 * ids[k] + d];
+    for (int d = 0; d < 3; ++d) c[d] = m > 0 ? s[d] / (double)m : 0.0;
+    double r2 = 0.0;
+    for (int64_t k = 0; k < m; ++k) r2 = std::max(r2, dist2(pts + 3 * ids[k], c));
+    *r = std::sqrt(r2);
+}
+
+Tree build_kmeans(const double* pts, int64_t n, int64_t nclusters, int64_t minvalues) {
+    Tree t;
+    t.n = n;
+    t.perm.resize(n);
+    for (int64_t i = 0; i < n; ++i) t.perm[i] = i;
+    t.parent.push_back(-1);
+    t.level.push_back(0);
+    t.lo.push_back(0);
+    t.hi.push_back(n);
+    double c[3], r;
+    centroid_radius(pts, t.perm.data(), n, c, &r);
+    for (int d = 0; d < 3; ++d) t.center.push_back(c[d]);
+    t.radius.push_back(r);
+    const int k = (int)std::max<int64_t>(2, nclusters);
+    std::vector<int64_t> tmp(n);
+    std::vector<int> assign;
+    for (int64_t node = 0; node < (int64_t)t.parent.size(); ++node) {
+        const int64_t lo = t.lo[node], hi = t.hi[node], m = hi - lo;
+        if (m <= minvalues) continue;
+        const int64_t* ids = &t.perm[lo];
+        std::vector<double> cent(3 * k);
+        const double* c0 = &t.center[3 * node];
+        int64_t first = 0;
+        double best = -1.0;
+        for (int64_t q = 0; q < m; ++q) {
+            double d2 = dist2(pts + 3 * ids[q], c0);
+            if (d2 > best) { best = d2; first = q; }
+        }
+        for (int d = 0; d < 3; ++d) cent[d] = pts[3 * ids[first] + d];
+        std::vector<double> mind(m, std::numeric_limits<double>::max());
+        int kk = 1;
+        for (; kk < k; ++kk) {
+            int64_t arg = -1;
+            double bd = 0.0;
+            for (int64_t q = 0; q < m; ++q) {
+                mind[q] = std::min(mind[q], dist2(pts + 3 * ids[q], &cent[3 * (kk - 1)]));
+                if (mind[q] > bd) { bd = mind[q]; arg = q; }
+            }
+            if (arg < 0) break;
+            for (int d = 0; d < 3; ++d) cent[3 * kk + d] = pts[3 * ids[arg] + d];
+        }
+        const int kc = kk;
+        if (kc < 2) continue;  // all points coincide
+        assign.assign(m, -1);
+        for (int it = 0; it < 100; ++it) {
+            bool changed = false;
+            for (int64_t q = 0; q < m; ++q) {
+                int a = 0;
+                double bd = dist2(pts + 3 * ids[q], &cent[0]);
+                for (int j = 1; j < kc; ++j) {
+                    double d2 = dist2(pts + 3 * ids[q], &cent[3 * j]);
+                    if (d2 < bd) { bd = d2; a = j; }
+                }
+                if (a != assign[q]) { assign[q] = a; changed = true; }
+            }
+            if (!changed) break;
+            std::vector<double> s(3 * kc, 0.0);
+            std::vector<int64_t> cnt(kc, 0);
+            for (int64_t q = 0; q < m; ++q) {
+                cnt[assign[q]]++;
+                for (int d = 0; d < 3; ++d) s[3 * assign[q] + d] += pts[3 * ids[q] + d];
+            }
+            for (int j = 0; j < kc; ++j)
+                if (cnt[j] > 0)
+                    for (int d = 0; d < 3; ++d) cent[3 * j + d] = s[3 * j + d] / (double)cnt[j];
+        }
+        std::vector<int64_t> cnt(kc, 0);
+        for (int64_t q = 0; q < m; ++q) cnt[assign[q]]++;
+        int nonempty = 0;
+        for (int j = 0; j < kc; ++j) nonempty += cnt[j] > 0;
+        if (nonempty < 2) continue;
+        std::vector<int64_t> start(kc), pos(kc);
+        int64_t s = lo;
+        for (int j = 0; j < kc; ++j) { start[j] = s; pos[j] = s; s += cnt[j]; }
+        for (int64_t q = 0; q < m; ++q) tmp[pos[assign[q]]++] = ids[q];
+        std::memcpy(&t.perm[lo], &tmp[lo], sizeof(int64_t) * m);
+        for (int j = 0; j < kc; ++j) {
+            if (cnt[j] == 0) continue;
+            t.parent.push_back(node);
+            t.level.push_back(t.level[node] + 1);
+            t.lo.push_back(start[j]);
+            t.hi.push_back(start[j] + cnt[j]);
+            centroid_radius(pts, &t.perm[start[j]], cnt[j], c, &r);
+            for (int d = 0; d < 3; ++d) t.center.push_back(c[d]);
+            t.radius.push_back(r);
+        }
+    }
+    finish_children(t);
+    return t;
+}
+
+struct TreeView {
+    int64_t nn;
+    const int64_t* childptr;
+    const int64_t* children;
+    const double* center;
+    const double* radius;
+    bool leaf(int64_t i) const { return childptr[i + 1] == childptr[i]; }
+};
+
+// Standard admissibility with bounding balls: a pair is near iff
+// max(diam_t, diam_s) > eta * dist(t, s), dist = |c_t - c_s| - (r_t + r_s).
+inline bool isnear_pair(const TreeView& a, const TreeView& b, int64_t t, int64_t s, double eta) {
+    double d = std::sqrt(dist2(a.center + 3 * t, b.center + 3 * s)) - (a.radius[t] + b.radius[s]);
+    double diam = 2.0 * std::max(a.radius[t], b.radius[s]);
+    return diam > eta * d;
+}
+
+void traverse(const TreeView& a, const TreeView& b, int64_t t, int64_t s, double eta,
+              std::vector<std::pair<int64_t, int64_t>>& nearp,
+              std::vector<std::pair<int64_t, int64_t>>& farp) {
+    if (!isnear_pair(a, b, t, s, eta)) {
+        farp.emplace_back(t, s);
+        return;
+    }
+    const bool lt = a.leaf(t), ls = b.leaf(s);
+    if (lt && ls) {
+        nearp.emplace_back(t, s);
+    } else if (lt) {
+        for (int64_t k = b.childptr[s]; k < b.childptr[s + 1]; ++k)
+            traverse(a, b, t, b.children[k], eta, nearp, farp);
+    } else if (ls) {
+        for (int64_t k = a.childptr[t]; k < a.childptr[t + 1]; ++k)
+            traverse(a, b, a.children[k], s, eta, nearp, farp);
+    } else {
+        for (int64_t i = a.childptr[t]; i < a.childptr[t + 1]; ++i)
+            for (int64_t k = b.childptr[s]; k < b.childptr[s + 1]; ++k)
+                traverse(a, b, a.children[i], b.children[k], eta, nearp, farp);
+    }
+}
+
+void to_csr(int64_t nrows, std::vector<std::pair<int64_t, int64_t>>& pairs, std::vector<int64_t>& ptr,
+            std::vector<int64_t>& idx) {
+    ptr.assign(nrows + 1, 0);
+    for (auto& p : pairs) ptr[p.first + 1]++;
+    for (int64_t i = 0; i < nrows; ++i) ptr[i + 1] += ptr[i];
+    idx.assign(pairs.size(), 0);
+    std::vector<int64_t> cur(ptr.begin(), ptr.end() - 1);
+    for (auto& p : pairs) idx[cur[p.first]++] = p.second;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < nrows; ++i) std::sort(idx.begin() + ptr[i], idx.begin() + ptr[i + 1]);
+}
+
+Partition interactions(const TreeView& a, const TreeView& b, double eta) {
+    // Expand the pair traversal breadth-first to a frontier, then finish every
+    // frontier pair depth-first in parallel; the CSR is sorted afterwards, so
+    // the result does not depend on the schedule.
+    std::vector<std::pair<int64_t, int64_t>> nearp, farp, frontier{{0, 0}};
+    while (!frontier.empty() && frontier.size() < 8192) {
+        std::vector<std::pair<int64_t, int64_t>> next;
+        bool expanded = false;
+        for (auto& pr : frontier) {
+            int64_t t = pr.first, s = pr.second;
+            if (!isnear_pair(a, b, t, s, eta)) { farp.push_back(pr); continue; }
+            const bool lt = a.leaf(t), ls = b.leaf(s);
+            if (lt && ls) { nearp.push_back(pr); continue; }
+            expanded = true;
+            if (lt) {
+                for (int64_t k = b.childptr[s]; k < b.childptr[s + 1]; ++k) next.emplace_back(t, b.children[k]);
+            } else if (ls) {
+                for (int64_t k = a.childptr[t]; k < a.childptr[t + 1]; ++k) next.emplace_back(a.children[k], s);
+            } else {
+                for (int64_t i = a.childptr[t]; i < a.childptr[t + 1]; ++i)
+                    for (int64_t k = b.childptr[s]; k < b.childptr[s + 1]; ++k)
+                        next.emplace_back(a.children[i], b.children[k]);
+            }
+        }
+        frontier.swap(next);
+        if (!expanded) break;
+    }
+    const int64_t nf = (int64_t)frontier.size();
+    int nth = omp_get_max_threads();
+    std::vector<std::vector<std::pair<int64_t, int64_t>>> ln(nth), lf(nth);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < nf; ++i) {
+        int tid = omp_get_thread_num();
+        traverse(a, b, frontier[i].first, frontier[i].second, eta, ln[tid], lf[tid]);
+    }
+    for (int i = 0; i < nth; ++i) {
+        nearp.insert(nearp.end(), ln[i].begin(), ln[i].end());
+        farp.insert(farp.end(), lf[i].begin(), lf[i].end());
+    }
+    Partition p;
+    to_csr(a.nn, nearp, p.nearptr, p.nearnodes);
+    to_csr(a.nn, farp, p.farptr, p.farnodes);
+    return p;
+}
+
+// Phase 1 stand-in for geometric (tree-mimicry) pivoting. For node t:
+//   F(t) = own far nodes, then the far nodes of every ancestor;
+//   n = min(maxrank, |t|) capped by the points F(t) holds (_clustersize);
+//   candidates = an even subsample of every far node's points;
+//   first pivot = candidate nearest to the center of t, every next pivot
+//   maximizes (distance to the chosen pivots / distance to the center of t)^2.
+// Ties go to the first candidate in far-field order.
+Csr farpivots(int64_t nn_self, const int64_t* parent, const int64_t* lo, const int64_t* hi,
+              const double* center, const int64_t* other_lo, const int64_t* other_hi,
+              const int64_t* other_perm, const double* other_pts, const int64_t* farptr,
+              const int64_t* farnodes, int64_t maxrank) {
+    std::vector<std::vector<int64_t>> result(nn_self);
+#pragma omp parallel
+    {
+        std::vector<int64_t> F, pool;
+        std::vector<double> d2c, mind;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t t = 0; t < nn_self; ++t) {
+            F.clear();
+            for (int64_t node = t; node >= 0; node = parent[node])
+                for (int64_t k = farptr[node]; k < farptr[node + 1]; ++k) F.push_back(farnodes[k]);
+            if (F.empty()) continue;
+            const int64_t limit = std::min<int64_t>(maxrank, hi[t] - lo[t]);
+            int64_t n = 0, total = 0;
+            for (int64_t f : F) {
+                const int64_t sz = other_hi[f] - other_lo[f];
+                total += sz;
+                if (n < limit) n += std::min(sz, limit - n);
+            }
+            if (n == 0) continue;
+            int64_t per = std::max<int64_t>(1, (std::max<int64_t>(8 * n, 512) + (int64_t)F.size() - 1) /
+                                                   (int64_t)F.size());
+            for (;;) {
+                pool.clear();
+                for (int64_t f : F) {
+                    const int64_t sz = other_hi[f] - other_lo[f];
+                    const int64_t k = std::min(sz, per);
+                    for (int64_t j = 0; j < k; ++j) pool.push_back(other_perm[other_lo[f] + (j * sz) / k]);
+                }
+                if ((int64_t)pool.size() >= n || (int64_t)pool.size() >= total) break;
+                per *= 2;
+            }
+            const int64_t np = (int64_t)pool.size();
+            const double* ct = center + 3 * t;
+            d2c.resize(np);
+            mind.assign(np, std::numeric_limits<double>::infinity());
+            int64_t first = 0;
+            for (int64_t q = 0; q < np; ++q) {
+                d2c[q] = std::max(dist2(other_pts + 3 * pool[q], ct), 1e-300);
+                if (d2c[q] < d2c[first]) first = q;
+            }
+            std::vector<int64_t>& out = result[t];
+            out.reserve(n);
+            int64_t cur = first;
+            for (int64_t sel = 0; sel < n; ++sel) {
+                out.push_back(pool[cur]);
+                mind[cur] = -1.0;  // chosen
+                const double* pc = other_pts + 3 * pool[cur];
+                int64_t arg = -1;
+                double best = -1.0;
+                for (int64_t q = 0; q < np; ++q) {
+                    if (mind[q] < 0.0) continue;
+                    double d2 = dist2(other_pts + 3 * pool[q], pc);
+                    if (d2 < mind[q]) mind[q] = d2;
+                    double score = mind[q] / d2c[q];
+                    if (score > best) { best = score; arg = q; }
+                }
+                if (arg < 0) break;
+                cur = arg;
+            }
+        }
+    }
+    Csr c;
+    c.ptr.assign(nn_self + 1, 0);
+    for (int64_t t = 0; t < nn_self; ++t) c.ptr[t + 1] = c.ptr[t] + (int64_t)result[t].size();
+    c.idx.resize(c.ptr[nn_self]);
+    for (int64_t t = 0; t < nn_self; ++t)
+        std::copy(result[t].begin(), result[t].end(), c.idx.begin() + c.ptr[t]);
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+typedef struct ncat_tree ncat_tree;
+typedef struct ncat_partition ncat_partition;
+typedef struct ncat_csr ncat_csr;
+
+ncat_tree* ncat_tree_twon(const double* pts, int64_t n, int64_t minvalues, double minhalfsize) {
+    return reinterpret_cast<ncat_tree*>(new Tree(build_twon(pts, n, minvalues, minhalfsize)));
+}
+
+ncat_tree* ncat_tree_kmeans(const double* pts, int64_t n, int64_t nclusters, int64_t minvalues) {
+    return reinterpret_cast<ncat_tree*>(new Tree(build_kmeans(pts, n, nclusters, minvalues)));
+}
+
+int64_t ncat_tree_nnodes(const ncat_tree* h) {
+    return (int64_t)reinterpret_cast<const Tree*>(h)->parent.size();
+}
+
+// parent/level/lo/hi/radius: nnodes; childptr: nnodes+1; children: nnodes-1;
+// perm: npoints; center: 3*nnodes.
+void ncat_tree_export(const ncat_tree* h, int64_t* parent, int64_t* level, int64_t* lo, int64_t* hi,
+                      int64_t* childptr, int64_t* children, int64_t* perm, double* center, double* radius) {
+    const Tree& t = *reinterpret_cast<const Tree*>(h);
+    const size_t nn = t.parent.size();
+    std::copy(t.parent.begin(), t.parent.end(), parent);
+    std::copy(t.level.begin(), t.level.end(), level);
+    std::copy(t.lo.begin(), t.lo.end(), lo);
+    std::copy(t.hi.begin(), t.hi.end(), hi);
+    std::copy(t.childptr.begin(), t.childptr.end(), childptr);
+    std::copy(t.children.begin(), t.children.end(), children);
+    std::copy(t.perm.begin(), t.perm.end(), perm);
+    std::copy(t.center.begin(), t.center.begin() + 3 * nn, center);
+    std::copy(t.radius.begin(), t.radius.end(), radius);
+}
+
+void ncat_tree_free(ncat_tree* h) { delete reinterpret_cast<Tree*>(h); }
+
+ncat_partition* ncat_interactions(int64_t nn_a, const int64_t* childptr_a, const int64_t* children_a,
+                                  const double* center_a, const double* radius_a, int64_t nn_b,
+                                  const int64_t* childptr_b, const int64_t* children_b,
+                                  const double* center_b, const double* radius_b, double eta) {
+    TreeView a{nn_a, childptr_a, children_a, center_a, radius_a};
+    TreeView b{nn_b, childptr_b, children_b, center_b, radius_b};
+    return reinterpret_cast<ncat_partition*>(new Partition(interactions(a, b, eta)));
+}
+
+void ncat_partition_sizes(const ncat_partition* h, int64_t* nnear, int64_t* nfar) {
+    const Partition& p = *reinterpret_cast<const Partition*>(h);
+    *nnear = (int64_t)p.nearnodes.size();
+    *nfar = (int64_t)p.farnodes.size();
+}
+
+void ncat_partition_export(const ncat_partition* h, int64_t* nearptr, int64_t* nearnodes, int64_t* farptr,
+                           int64_t* farnodes) {
+    const Partition& p = *reinterpret_cast<const Partition*>(h);
+    std::copy(p.nearptr.begin(), p.nearptr.end(), nearptr);
+    std::copy(p.nearnodes.begin(), p.nearnodes.end(), nearnodes);
+    std::copy(p.farptr.begin(), p.farptr.end(), farptr);
+    std::copy(p.farnodes.begin(), p.farnodes.end(), farnodes);
+}
+
+void ncat_partition_free(ncat_partition* h) { delete reinterpret_cast<Partition*>(h); }
+
+ncat_csr* ncat_farpivots(int64_t nn_self, const int64_t* parent, const int64_t* lo, const int64_t* hi,
+                         const double* center, const int64_t* other_lo, const int64_t* other_hi,
+                         const int64_t* other_perm, const double* other_pts, const int64_t* farptr,
+                         const int64_t* farnodes, int64_t maxrank) {
+    return reinterpret_cast<ncat_csr*>(new Csr(farpivots(nn_self, parent, lo, hi, center, other_lo, other_hi,
+                                                         other_perm, other_pts, farptr, farnodes, maxrank)));
+}
+
+int64_t ncat_csr_nnz(const ncat_csr* h) { return (int64_t)reinterpret_cast<const Csr*>(h)->idx.size(); }
+
+void ncat_csr_export(const ncat_csr* h, int64_t* ptr, int64_t* idx) {
+    const Csr& c = *reinterpret_cast<const Csr*>(h);
+    std::copy(c.ptr.begin(), c.ptr.end(), ptr);
+    std::copy(c.idx.begin(), c.idx.end(), idx);
+}
+
+void ncat_csr_free(ncat_csr* h) { delete reinterpret_cast<Csr*>(h); }
+
+}  // extern "C"

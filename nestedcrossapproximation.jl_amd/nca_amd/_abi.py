@@ -1,0 +1,111 @@
+"""ctypes declarations of include/nca_hip.h."""
+import ctypes
+
+P = ctypes.c_void_p
+PP = ctypes.POINTER(ctypes.c_void_p)
+I32 = ctypes.c_int32
+I64 = ctypes.c_int64
+D = ctypes.c_double
+
+NCA_OK = 0
+NCA_ERR_ARGUMENT = 1
+NCA_ERR_DIMENSION = 2
+NCA_ERR_HIP = 3
+NCA_ERR_SINGULAR = 4
+NCA_ERR_LIMIT = 5
+
+NCA_OP_N, NCA_OP_T, NCA_OP_C = 0, 1, 2
+NCA_PREC_FP64, NCA_PREC_MIXED = 0, 1
+
+
+class Helmholtz(ctypes.Structure):
+    _fields_ = [("wavenumber", D), ("sign", I32), ("reserved", I32)]
+
+
+class H2Desc(ctypes.Structure):
+    _fields_ = [
+        ("M", I64), ("N", I64), ("nn_test", I64), ("nn_trial", I64),
+        ("test_perm", P), ("test_lo", P), ("test_hi", P),
+        ("trial_perm", P), ("trial_lo", P), ("trial_hi", P),
+        ("test_cptr", P), ("trial_cptr", P),
+        ("nbasis_test", I64), ("nbasis_trial", I64),
+        ("basis_test_nodes", P), ("basis_test_off", P), ("basis_trial_nodes", P), ("basis_trial_off", P),
+        ("basis_test", P), ("basis_trial", P),
+        ("nlev_test", I64), ("nlev_trial", I64),
+        ("tt_levptr", P), ("tt_levidcs", P), ("tt_chdptr", P), ("tt_chds", P), ("tt_off", P),
+        ("st_levptr", P), ("st_levidcs", P), ("st_chdptr", P), ("st_chds", P), ("st_off", P),
+        ("tt_blocks", P), ("st_blocks", P),
+        ("cp_ptr", P), ("cp_nodes", P),
+        ("nearptr", P), ("nearnodes", P),
+    ]
+
+
+class H2Stats(ctypes.Structure):
+    _fields_ = [(n, I64) for n in (
+        "coupling_blocks", "coupling_entries", "basis_entries_test", "basis_entries_trial",
+        "transfer_entries_test", "transfer_entries_trial", "near_pairs", "near_entries",
+        "coeffs_test", "coeffs_trial", "plan_bytes")]
+
+
+SIGNATURES = {
+    "nca_last_error": (ctypes.c_char_p, []),
+    "nca_abi_version": (I32, []),
+    "nca_ctx_create": (I32, [I32, PP]),
+    "nca_ctx_destroy": (I32, [P]),
+    "nca_ctx_synchronize": (I32, [P]),
+    "nca_ctx_stream": (I32, [P, PP]),
+    "nca_points_create": (I32, [P, P, I64, PP]),
+    "nca_points_destroy": (I32, [P]),
+    "nca_level_assemble": (I32, [P, P, P, P, I64, P, P, P, P, I64, I32, PP]),
+    "nca_stack_upload": (I32, [P, P, I64, P, P, I64, PP]),
+    "nca_stack_shape": (I32, [P, P, P, P]),
+    "nca_stack_download": (I32, [P, P, P]),
+    "nca_stack_destroy": (I32, [P]),
+    "nca_aca_fullpiv": (I32, [P, P, D, I32, P, P, P]),
+    "nca_skeleton_solve": (I32, [P, P, P, P, P, P, P, P]),
+    "nca_compress_level": (I32, [P, P, D, I32, P, P, P, I64, P, P, P]),
+    "nca_coupling_assemble": (I32, [P, P, P, P, I64, P, P, I64, P, P, P, P, PP]),
+    "nca_coupling_size": (I32, [P, P, P]),
+    "nca_coupling_download": (I32, [P, P, P, P]),
+    "nca_coupling_destroy": (I32, [P]),
+    "nca_h2_create": (I32, [P, P, P, P, P, P, I32, PP]),
+    "nca_h2_destroy": (I32, [P]),
+    "nca_h2_set_precision": (I32, [P, I32]),
+    "nca_h2_matvec": (I32, [P, I32, I64, P, I64, P, I64, P, P]),
+    "nca_h2_matvec_device": (I32, [P, I32, I64, P, I64, P, I64, P, P]),
+    "nca_h2_get_stats": (I32, [P, P]),
+}
+
+
+def declare(lib):
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+class NCAError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class SingularCrossMatrixError(NCAError):
+    pass
+
+
+class DimensionMismatch(NCAError):
+    pass
+
+
+class ArgumentError(NCAError):
+    pass
+
+
+def check(lib, status):
+    if status == NCA_OK:
+        return
+    msg = lib.nca_last_error().decode(errors="replace")
+    cls = {NCA_ERR_SINGULAR: SingularCrossMatrixError, NCA_ERR_DIMENSION: DimensionMismatch,
+           NCA_ERR_ARGUMENT: ArgumentError}.get(status, NCAError)
+    raise cls(status, msg)

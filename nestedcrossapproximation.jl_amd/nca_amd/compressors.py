@@ -1,0 +1,369 @@
+"""Batched bottom-up compressor (src/compressors/batchedbottomup.jl) on the GPU.
+
+The level driver keeps the structure of testbases/trialbases(::BatchedBottomUp)
+(batchedbottomup.jl:246-447); Phase 2 and Phase 3 go through the two seams
+assemble_level (:101-116) and compresslevel_test!/trial! (:145-244), which here
+call the HIP C-ABI. Index sets are numpy CSR arrays instead of
+Vector{Vector{Int}}; node ids are 0-based in Python.
+"""
+import ctypes
+import os
+import time
+
+import numpy as np
+
+from . import _abi
+from ._native import hip_lib
+from .runtime import call, _p
+from . import trees as _trees
+
+
+def concat_ranges(starts, lengths):
+    """Indices of the concatenated ranges [s, s + l)."""
+    lengths = np.asarray(lengths, np.int64)
+    starts = np.asarray(starts, np.int64)
+    total = int(lengths.sum())
+    if total == 0:
+        return np.zeros(0, np.int64)
+    ends = np.cumsum(lengths)
+    out = np.ones(total, np.int64)
+    nz = lengths > 0
+    s_nz, l_nz, e_nz = starts[nz], lengths[nz], ends[nz]
+    out[0] = s_nz[0]
+    if len(s_nz) > 1:
+        out[e_nz[:-1]] = s_nz[1:] - (s_nz[:-1] + l_nz[:-1] - 1)
+    return np.cumsum(out)
+
+
+class ACA:
+    """Fully pivoted ACA with the Frobenius-norm stop, the dense compressor of
+    BatchedBottomUp: ACA(FullPivoting(), FullPivoting(), FNormEstimator(tol))."""
+
+    def __init__(self, tol=1e-4, rowpivoting="full", columnpivoting="full"):
+        if rowpivoting != "full" or columnpivoting != "full":
+            raise ValueError("the batched compressor takes fully pivoted ACA")
+        self.tol = float(tol)
+
+
+class GeometricPivoting:
+    """Phase-1 far-pivot selector (stand-in for TreeMimicryPivoting; see nca_tree.cpp)."""
+
+    def select(self, selftree, othertree, fd, maxrank):
+        return _trees.farpivots(selftree, othertree, fd, maxrank)
+
+
+class BatchedBottomUp:
+    """BatchedBottomUp(; pivotselector, compressor) (batchedbottomup.jl:30-51)."""
+
+    def __init__(self, pivotselector=None, compressor=None):
+        if pivotselector is None or compressor is None:
+            raise TypeError("BatchedBottomUp needs the keywords pivotselector and compressor")
+        self.pivotselector = pivotselector
+        self.compressor = compressor
+
+
+def _timing():
+    return os.environ.get("NCA_BATCHED_TIMING", "0") == "1"
+
+
+# --------------------------------------------------------------------------
+# Seam 1: Phase 2.
+class LevelStack:
+    """Device-resident level buffer returned by assemble_level for a HIP kernel
+    matrix; its type routes compresslevel_*! to the HIP methods."""
+
+    def __init__(self, farmatrix, handle, nb, rowoffsets, ncols, maxrank):
+        self.farmatrix = farmatrix
+        self.handle = handle
+        self.nb = nb
+        self.rowoffsets = rowoffsets
+        self.ncols = ncols
+        self.maxrank = maxrank
+
+    def download(self):
+        ld = int(self.rowoffsets[-1])
+        buf = np.zeros((ld, self.maxrank), np.complex128, order="F")
+        call(hip_lib().nca_stack_download(self.farmatrix.ctx.handle, self.handle, _p(buf)))
+        return buf
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                hip_lib().nca_stack_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def assemble_level(farmatrix, rowptr, rowidx, colptr, colidx, maxrank, transposed=False):
+    """Phase 2 for one level: block b = A[rowidx[rowptr[b]:rowptr[b+1]], colidx[...]]
+    (0-based CSR of 0-based point ids), or its transposed-assembly form."""
+    nb = len(rowptr) - 1
+    rp1 = np.ascontiguousarray(rowptr + 1, np.int64)
+    cp1 = np.ascontiguousarray(colptr + 1, np.int64)
+    ri1 = np.ascontiguousarray(rowidx + 1, np.int64)
+    ci1 = np.ascontiguousarray(colidx + 1, np.int64)
+    h = ctypes.c_void_p()
+    kern = farmatrix.kernel.abi()
+    call(hip_lib().nca_level_assemble(farmatrix.ctx.handle, ctypes.byref(kern), farmatrix.test.handle,
+                                      farmatrix.trial.handle, nb, _p(rp1), _p(ri1) if len(ri1) else None, _p(cp1),
+                                      _p(ci1) if len(ci1) else None, int(maxrank), int(bool(transposed)),
+                                      ctypes.byref(h)))
+    return LevelStack(farmatrix, h, nb, np.asarray(rowptr, np.int64).copy(), np.diff(colptr), maxrank)
+
+
+def stack_from_host(farmatrix, buffer, rowoffsets, ncols, maxrank):
+    """A LevelStack from host data in the level-buffer layout (for tests)."""
+    buf = np.asfortranarray(np.asarray(buffer, dtype=np.complex128))
+    ro = np.ascontiguousarray(rowoffsets, np.int64)
+    nc = np.ascontiguousarray(ncols, np.int64)
+    h = ctypes.c_void_p()
+    call(hip_lib().nca_stack_upload(farmatrix.ctx.handle, _p(buf), len(nc), _p(ro), _p(nc), int(maxrank),
+                                    ctypes.byref(h)))
+    return LevelStack(farmatrix, h, len(nc), ro, nc, maxrank)
+
+
+def aca_fullpiv(stack, tol, sampled=True):
+    """Batched ACA on a stack: (ranks, rowpivots, colpivots) with maxrank x nb
+    1-based pivots as the device method returns them."""
+    nb, mr = stack.nb, stack.maxrank
+    ranks = np.zeros(max(nb, 1), np.int32)
+    rp = np.zeros((mr, max(nb, 1)), np.int64, order="F")
+    cp = np.zeros((mr, max(nb, 1)), np.int64, order="F")
+    call(hip_lib().nca_aca_fullpiv(stack.farmatrix.ctx.handle, stack.handle, float(tol), int(bool(sampled)),
+                                   _p(ranks), _p(rp), _p(cp)))
+    return ranks[:nb], rp[:, :nb], cp[:, :nb]
+
+
+def skeleton_solve(stack, ranks, rowpivots, colpivots):
+    """X_b = L_b / P_b per block; returns (xoffsets, X, independent)."""
+    nb = stack.nb
+    m = np.diff(stack.rowoffsets)
+    xoff = np.zeros(nb + 1, np.int64)
+    np.cumsum(m * np.asarray(ranks, np.int64), out=xoff[1:])
+    X = np.zeros(max(int(xoff[-1]), 1), np.complex128)
+    indep = np.zeros(max(nb, 1), np.int64)
+    rk = np.ascontiguousarray(ranks, np.int32)
+    rp = np.asfortranarray(rowpivots, np.int64)
+    cp = np.asfortranarray(colpivots, np.int64)
+    status = hip_lib().nca_skeleton_solve(stack.farmatrix.ctx.handle, stack.handle, _p(rk), _p(rp), _p(cp),
+                                          _p(xoff), _p(X), _p(indep))
+    return xoff, X[:xoff[-1]], indep[:nb], status
+
+
+def compress_level(stack, tol, sampled=True):
+    """ACA + skeleton solve in one call (pivots stay on the device in between)."""
+    nb, mr = stack.nb, stack.maxrank
+    m = np.diff(stack.rowoffsets)
+    cap = int(np.sum(m * np.minimum(np.minimum(m, stack.ncols), mr))) if nb else 0
+    ranks = np.zeros(max(nb, 1), np.int32)
+    rp = np.zeros((mr, max(nb, 1)), np.int64, order="F")
+    cp = np.zeros((mr, max(nb, 1)), np.int64, order="F")
+    xoff = np.zeros(nb + 1, np.int64)
+    X = np.zeros(max(cap, 1), np.complex128)
+    indep = np.zeros(max(nb, 1), np.int64)
+    call(hip_lib().nca_compress_level(stack.farmatrix.ctx.handle, stack.handle, float(tol), int(bool(sampled)),
+                                      _p(ranks), _p(rp), _p(cp), cap, _p(xoff), _p(X), _p(indep)))
+    return ranks[:nb], rp[:, :nb], cp[:, :nb], xoff, X[:xoff[-1]]
+
+
+# --------------------------------------------------------------------------
+class SideFactors:
+    """Result of one side: pivots per node (0-based CSR of point ids; nodes
+    without far field are unassigned), and per active node its block X (m x r,
+    column-major) together with the child ranges of its rows."""
+
+    def __init__(self, nn):
+        self.nn = nn
+        self.assigned = np.zeros(nn, bool)
+        self.rank = np.zeros(nn, np.int64)
+        self.pivots = [None] * nn
+        self.X = [None] * nn
+        self.m = np.zeros(nn, np.int64)
+        self.level_transfer_nodes = {}
+
+    def pivot_csr(self):
+        ptr = np.zeros(self.nn + 1, np.int64)
+        np.cumsum(self.rank, out=ptr[1:])
+        idx = np.concatenate([p for p in self.pivots if p is not None and len(p)] or [np.zeros(0, np.int64)])
+        return ptr, idx.astype(np.int64)
+
+
+def _compresslevel(cfg, storage, subtree, activeclusters, childranges, tvalues_ptr, tvalues_idx, out):
+    """compresslevel_test!/trial! for a HIP LevelStack (batchedbottomup.jl:145-244,
+    compresslevel.jl:242-316): batched ACA + skeleton solve, then the pivots
+    tpivots[t] = tvalues[t][rowpivots] and the factor X of every block."""
+    ranks, rp, cp, xoff, X = compress_level(storage, cfg.tol, sampled=True)
+    m = np.diff(storage.rowoffsets)
+    for i, t in enumerate(activeclusters):
+        r = int(ranks[i])
+        vals = tvalues_idx[tvalues_ptr[i]:tvalues_ptr[i + 1]]
+        out.assigned[t] = True
+        out.rank[t] = r
+        out.pivots[t] = vals[rp[:r, i] - 1].astype(np.int64)
+        out.m[t] = m[i]
+        out.X[t] = X[xoff[i]:xoff[i + 1]].reshape((m[i], r), order="F") if r else np.zeros((m[i], 0), np.complex128)
+
+
+def batched_side(compressor, farmatrix, tree, fd, side, maxrank=50, farpivots=None):
+    """testbases (side 0) / trialbases (side 1) of BatchedBottomUp, returning
+    SideFactors. farpivots: optional Phase-1 result (0-based CSR)."""
+    selftree = tree.testtree if side == 0 else tree.trialtree
+    othertree = tree.trialtree if side == 0 else tree.testtree
+    if farpivots is None:
+        farpivots = compressor.pivotselector.select(selftree, othertree, fd, maxrank)
+    fptr, fidx = farpivots
+    nfar = np.diff(fptr)
+    out = SideFactors(selftree.nnodes)
+    out.farpivots = farpivots
+    leaf = selftree.leafmask()
+    for lev in sorted(selftree.levels().tolist(), reverse=True):
+        nodes = selftree.levelnodes(lev)
+        active = nodes[nfar[nodes] > 0]
+        if len(active) == 0:
+            continue
+        t0 = time.perf_counter()
+        # rows: leaf -> its points, inner -> the concatenated pivots of its children
+        starts, lengths = [], []
+        counts = np.zeros(len(active), np.int64)
+        rows_list = []
+        for i, t in enumerate(active):
+            if leaf[t]:
+                rows_list.append(selftree.perm[selftree.lo[t]:selftree.hi[t]])
+            else:
+                ch = selftree.childrenof(t)
+                rows_list.append(np.concatenate([out.pivots[c] if out.pivots[c] is not None else np.zeros(0, np.int64)
+                                                 for c in ch]) if len(ch) else np.zeros(0, np.int64))
+            counts[i] = len(rows_list[-1])
+        rowptr = np.zeros(len(active) + 1, np.int64)
+        np.cumsum(counts, out=rowptr[1:])
+        rowidx = np.concatenate(rows_list).astype(np.int64) if rowptr[-1] else np.zeros(0, np.int64)
+        colptr = np.zeros(len(active) + 1, np.int64)
+        np.cumsum(nfar[active], out=colptr[1:])
+        colidx = concat_ranges(fptr[active], nfar[active])
+        colidx = fidx[colidx]
+        storage = assemble_level(farmatrix, rowptr, rowidx, colptr, colidx, maxrank, transposed=(side == 1))
+        t1 = time.perf_counter()
+        _compresslevel(compressor.compressor, storage, selftree, active, None, rowptr, rowidx, out)
+        t2 = time.perf_counter()
+        if _timing():
+            print(f"[batchedbottomup {'test' if side == 0 else 'trial'}] level {lev + 1}: {len(active)} blocks | "
+                  f"assemble={t1 - t0:.4f}s | compress(aca+basis/transfer)={t2 - t1:.4f}s", flush=True)
+        out.level_transfer_nodes[lev] = [int(t) for t in nodes if not leaf[t] and out.assigned[t]]
+    return out
+
+
+class BasisStore:
+    def __init__(self, nodes, offsets, blocks):
+        self.nodes = nodes        # 0-based node ids
+        self.offsets = offsets    # nblocks+1 complex offsets
+        self.blocks = blocks      # packed column-major blocks
+
+
+class TransferStore:
+    def __init__(self, leveltoidcsptr, levelidcs, idxtochdsptr, chdsidcs, offsets, blocks):
+        self.leveltoidcsptr = leveltoidcsptr  # 0-based
+        self.levelidcs = levelidcs
+        self.idxtochdsptr = idxtochdsptr
+        self.chdsidcs = chdsidcs
+        self.offsets = offsets
+        self.blocks = blocks
+
+
+def stores_from_factors(f, tree, side):
+    """_build_basis_storage / _build_transfer_storage (nestedbasis.jl:1-4,
+    transfermatrices.jl:58-95) from the block factors X: test basis X (m x r),
+    trial basis X^T; test transfer X[childrows, :], trial transfer its transpose."""
+    leaf = tree.leafmask()
+    bnodes = [t for t in range(f.nn) if f.assigned[t] and leaf[t]]
+    bl, boff = [], [0]
+    for t in bnodes:
+        X = f.X[t]
+        B = X if side == 0 else X.T
+        bl.append(np.asfortranarray(B).ravel(order="F"))
+        boff.append(boff[-1] + X.size)
+    basis = BasisStore(np.asarray(bnodes, np.int64), np.asarray(boff, np.int64),
+                       np.concatenate(bl) if bl else np.zeros(0, np.complex128))
+    levels = sorted(tree.levels().tolist())
+    levptr, levidcs, chdptr, chds, tl, toff = [0], [], [0], [], [], [0]
+    for lev in levels:
+        for p in f.level_transfer_nodes.get(lev, []):
+            levidcs.append(p)
+            X = f.X[p]
+            o = 0
+            for c in tree.childrenof(p):
+                rc = int(f.rank[c]) if f.assigned[c] else 0
+                E = X[o:o + rc, :]
+                o += rc
+                B = E if side == 0 else E.T
+                tl.append(np.asfortranarray(B).ravel(order="F"))
+                toff.append(toff[-1] + E.size)
+                chds.append(int(c))
+            chdptr.append(len(chds))
+        levptr.append(len(levidcs))
+    transfer = TransferStore(np.asarray(levptr, np.int64), np.asarray(levidcs, np.int64),
+                             np.asarray(chdptr, np.int64), np.asarray(chds, np.int64), np.asarray(toff, np.int64),
+                             np.concatenate(tl) if tl else np.zeros(0, np.complex128))
+    return basis, transfer
+
+
+def testbases(compressor, farmatrix, tree, fardata, maxrank=50, farpivots=None):
+    f = batched_side(compressor, farmatrix, tree, fardata, 0, maxrank, farpivots)
+    b, t = stores_from_factors(f, tree.testtree, 0)
+    return b, t, f
+
+
+def trialbases(compressor, farmatrix, tree, fardata, maxrank=50, farpivots=None):
+    f = batched_side(compressor, farmatrix, tree, fardata, 1, maxrank, farpivots)
+    b, t = stores_from_factors(f, tree.trialtree, 1)
+    return b, t, f
+
+
+# --------------------------------------------------------------------------
+class CouplingStore:
+    """Device-resident coupling matrices in test-CSR order (couplingmatrices.jl:11-53)."""
+
+    def __init__(self, farmatrix, handle, ptr, nodes):
+        self.farmatrix = farmatrix
+        self.handle = handle
+        self.nodetofarnodesptr = ptr
+        self.farnodes = nodes
+
+    def size(self):
+        nb = ctypes.c_int64()
+        ne = ctypes.c_int64()
+        call(hip_lib().nca_coupling_size(self.handle, ctypes.byref(nb), ctypes.byref(ne)))
+        return nb.value, ne.value
+
+    def download(self):
+        nb, ne = self.size()
+        off = np.zeros(nb + 1, np.int64)
+        blocks = np.zeros(max(ne, 1), np.complex128)
+        call(hip_lib().nca_coupling_download(self.farmatrix.ctx.handle, self.handle, _p(off), _p(blocks)))
+        return off, blocks[:ne]
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                hip_lib().nca_coupling_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def assemble_couplingstore(farmatrix, testpivots, trialpivots, testfardata):
+    """testpivots/trialpivots: 0-based CSR (ptr, point ids) of every node."""
+    tptr, tidx = testpivots
+    sptr, sidx = trialpivots
+    fp = np.ascontiguousarray(testfardata.nodetofarnodesptr + 1, np.int64)
+    fn = np.ascontiguousarray(testfardata.farnodes + 1, np.int64)
+    a = [np.ascontiguousarray(x, np.int64) for x in (tptr + 1, tidx + 1, sptr + 1, sidx + 1)]
+    h = ctypes.c_void_p()
+    kern = farmatrix.kernel.abi()
+    ntest = len(tptr) - 1
+    ntrial = len(sptr) - 1
+    call(hip_lib().nca_coupling_assemble(farmatrix.ctx.handle, ctypes.byref(kern), farmatrix.test.handle,
+                                         farmatrix.trial.handle, ntest, _p(fp), _p(fn) if len(fn) else None, ntrial,
+                                         _p(a[0]), _p(a[1]) if len(a[1]) else None, _p(a[2]),
+                                         _p(a[3]) if len(a[3]) else None, ctypes.byref(h)))
+    return CouplingStore(farmatrix, h, testfardata.nodetofarnodesptr, testfardata.farnodes)

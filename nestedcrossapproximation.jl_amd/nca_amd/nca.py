@@ -1,0 +1,209 @@
+"""PetrovGalerkinNCA on the GPU: assembly through the batched bottom-up
+compressor and the H^2 product as a linear map (src/nca/petrovgalerkinnca.jl)."""
+import ctypes
+import threading
+import time
+
+import numpy as np
+
+from . import _abi
+from ._native import hip_lib
+from .runtime import call, _p, HIPKernelMatrix, default_context
+from . import trees as _trees
+from . import compressors as _cmp
+
+
+def coefficient_ptr(f):
+    """plan_from_pivots (abstractnca.jl:110-119), 0-based."""
+    ptr = np.zeros(f.nn + 1, np.int64)
+    np.cumsum(np.where(f.assigned, f.rank, 0), out=ptr[1:])
+    return ptr
+
+
+def h2desc(tree, partition, testfardata, testf, trialf, testbasis, testtransfer, trialbasis, trialtransfer):
+    """The nca_h2_desc arrays, 1-based as the Julia structs hold them."""
+    tt, st = tree.testtree, tree.trialtree
+    i64 = lambda a: np.ascontiguousarray(a, np.int64)
+    c128 = lambda a: np.ascontiguousarray(a, np.complex128) if len(a) else np.zeros(1, np.complex128)
+    d = dict(
+        M=tt.npoints, N=st.npoints, nn_test=tt.nnodes, nn_trial=st.nnodes,
+        test_perm=i64(tt.perm + 1), test_lo=i64(tt.lo + 1), test_hi=i64(tt.hi),
+        trial_perm=i64(st.perm + 1), trial_lo=i64(st.lo + 1), trial_hi=i64(st.hi),
+        test_cptr=i64(coefficient_ptr(testf) + 1), trial_cptr=i64(coefficient_ptr(trialf) + 1),
+        nbasis_test=len(testbasis.nodes), nbasis_trial=len(trialbasis.nodes),
+        basis_test_nodes=i64(testbasis.nodes + 1), basis_test_off=i64(testbasis.offsets),
+        basis_trial_nodes=i64(trialbasis.nodes + 1), basis_trial_off=i64(trialbasis.offsets),
+        basis_test=c128(testbasis.blocks), basis_trial=c128(trialbasis.blocks),
+        nlev_test=len(testtransfer.leveltoidcsptr) - 1, nlev_trial=len(trialtransfer.leveltoidcsptr) - 1,
+        tt_levptr=i64(testtransfer.leveltoidcsptr + 1), tt_levidcs=i64(testtransfer.levelidcs + 1),
+        tt_chdptr=i64(testtransfer.idxtochdsptr + 1), tt_chds=i64(testtransfer.chdsidcs + 1),
+        tt_off=i64(testtransfer.offsets),
+        st_levptr=i64(trialtransfer.leveltoidcsptr + 1), st_levidcs=i64(trialtransfer.levelidcs + 1),
+        st_chdptr=i64(trialtransfer.idxtochdsptr + 1), st_chds=i64(trialtransfer.chdsidcs + 1),
+        st_off=i64(trialtransfer.offsets),
+        tt_blocks=c128(testtransfer.blocks), st_blocks=c128(trialtransfer.blocks),
+        cp_ptr=i64(testfardata.nodetofarnodesptr + 1), cp_nodes=i64(testfardata.farnodes + 1),
+        nearptr=i64(partition.nearptr + 1), nearnodes=i64(partition.nearnodes + 1),
+    )
+    for key in list(d):
+        if isinstance(d[key], np.ndarray) and d[key].size == 0:
+            d[key] = np.zeros(1, d[key].dtype)
+    return d
+
+
+def _abi_desc(d):
+    desc = _abi.H2Desc()
+    for name, _ in _abi.H2Desc._fields_:
+        v = d[name]
+        if isinstance(v, np.ndarray):
+            setattr(desc, name, _p(v))
+        else:
+            setattr(desc, name, int(v))
+    return desc
+
+
+class ConcurrentWorkspaceError(RuntimeError):
+    def __init__(self):
+        super().__init__("ConcurrentWorkspaceError: the workspace is in use by another matrix-vector product. "
+                         "Create one H2Workspace per task; they can share the core.")
+
+
+class PetrovGalerkinNCACore:
+    """Read-only part: the device operator plus the host-side plans and factors."""
+
+    def __init__(self, kernel, testpoints, trialpoints, tree, testcompressor, trialcompressor, eta=1.0,
+                 maxrank=50, precision=_abi.NCA_PREC_FP64, ctx=None, verbose=False):
+        self.ctx = ctx or default_context()
+        self.kernel = kernel
+        self.tree = tree
+        self.maxrank = maxrank
+        timings = {}
+        t0 = time.perf_counter()
+        self.partition = _trees.interactions(tree, eta)
+        self.testfardata, self.trialfardata = _trees.fardata(tree, self.partition)
+        timings["partition"] = time.perf_counter() - t0
+        self.farmatrix = HIPKernelMatrix(kernel, testpoints, trialpoints, self.ctx)
+        t0 = time.perf_counter()
+        fp_test = testcompressor.pivotselector.select(tree.testtree, tree.trialtree, self.testfardata, maxrank)
+        fp_trial = trialcompressor.pivotselector.select(tree.trialtree, tree.testtree, self.trialfardata, maxrank)
+        timings["phase1"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        self.testbasis, self.testtransfer, self.testfactors = _cmp.testbases(
+            testcompressor, self.farmatrix, tree, self.testfardata, maxrank, fp_test)
+        self.trialbasis, self.trialtransfer, self.trialfactors = _cmp.trialbases(
+            trialcompressor, self.farmatrix, tree, self.trialfardata, maxrank, fp_trial)
+        self.testpivots = self.testfactors.pivot_csr()
+        self.trialpivots = self.trialfactors.pivot_csr()
+        self.couplingstore = _cmp.assemble_couplingstore(self.farmatrix, self.testpivots, self.trialpivots,
+                                                         self.testfardata)
+        timings["assembly"] = time.perf_counter() - t0
+        self.desc = h2desc(tree, self.partition, self.testfardata, self.testfactors, self.trialfactors,
+                           self.testbasis, self.testtransfer, self.trialbasis, self.trialtransfer)
+        t0 = time.perf_counter()
+        h = ctypes.c_void_p()
+        kern = kernel.abi()
+        abidesc = _abi_desc(self.desc)
+        call(hip_lib().nca_h2_create(self.ctx.handle, ctypes.byref(kern), self.farmatrix.test.handle,
+                                     self.farmatrix.trial.handle, ctypes.byref(abidesc), self.couplingstore.handle,
+                                     int(precision), ctypes.byref(h)))
+        self.handle = h
+        timings["h2_create"] = time.perf_counter() - t0
+        self.timings = timings
+        self.dim = (tree.testtree.npoints, tree.trialtree.npoints)
+        if verbose:
+            print("assembly timings:", {k: round(v, 4) for k, v in timings.items()}, flush=True)
+
+    def stats(self):
+        s = _abi.H2Stats()
+        call(hip_lib().nca_h2_get_stats(self.handle, ctypes.byref(s)))
+        return {n: getattr(s, n) for n, _ in _abi.H2Stats._fields_}
+
+    def set_precision(self, precision):
+        call(hip_lib().nca_h2_set_precision(self.handle, int(precision)))
+
+    def matvec_device(self, op, nrhs, xptr, ldx, yptr, ldy, alpha=1.0, beta=0.0):
+        al = np.array([complex(alpha).real, complex(alpha).imag])
+        be = np.array([complex(beta).real, complex(beta).imag])
+        call(hip_lib().nca_h2_matvec_device(self.handle, int(op), int(nrhs), ctypes.c_void_p(xptr), int(ldx),
+                                            ctypes.c_void_p(yptr), int(ldy), _p(al), _p(be)))
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                hip_lib().nca_h2_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class H2Workspace:
+    """The H^2 matrix as a linear map over a shared core; one product at a time
+    (petrovgalerkinnca.jl:253-300)."""
+
+    def __init__(self, core, op=_abi.NCA_OP_N):
+        self.core = core
+        self.op = op
+        self._busy = threading.Lock()
+
+    @property
+    def shape(self):
+        M, N = self.core.dim
+        return (M, N) if self.op == _abi.NCA_OP_N else (N, M)
+
+    @property
+    def T(self):
+        return transpose(self)
+
+    def _mul(self, y, x, alpha, beta):
+        if not self._busy.acquire(blocking=False):
+            raise ConcurrentWorkspaceError()
+        try:
+            M, N = self.shape
+            x = np.asarray(x)
+            vec = x.ndim == 1
+            X = np.asfortranarray(x.reshape(x.shape[0], -1).astype(np.complex128))
+            if X.shape[0] != N or y.shape[0] != M or (not vec and y.reshape(M, -1).shape[1] != X.shape[1]):
+                raise ValueError(f"DimensionMismatch: A is {M} x {N}, x has {X.shape[0]} rows, y has {y.shape[0]}")
+            Y = np.asfortranarray(y.reshape(M, -1).astype(np.complex128))
+            al = np.array([complex(alpha).real, complex(alpha).imag])
+            be = np.array([complex(beta).real, complex(beta).imag])
+            call(hip_lib().nca_h2_matvec(self.core.handle, int(self.op), X.shape[1], _p(X), X.shape[0], _p(Y),
+                                         Y.shape[0], _p(al), _p(be)))
+            y[...] = Y.reshape(y.shape)
+            return y
+        finally:
+            self._busy.release()
+
+    def __matmul__(self, x):
+        x = np.asarray(x)
+        y = np.zeros((self.shape[0],) + x.shape[1:], np.complex128)
+        return self._mul(y, x, 1.0, 0.0)
+
+
+def mul(y, A, x, alpha=1.0, beta=0.0):
+    """mul!(y, A, x, alpha, beta)."""
+    return A._mul(y, x, alpha, beta)
+
+
+def transpose(A):
+    ops = {_abi.NCA_OP_N: _abi.NCA_OP_T, _abi.NCA_OP_T: _abi.NCA_OP_N}
+    if A.op not in ops:
+        raise ValueError("transpose of an adjoint map")
+    w = H2Workspace(A.core, ops[A.op])
+    w._busy = A._busy
+    return w
+
+
+def adjoint(A):
+    ops = {_abi.NCA_OP_N: _abi.NCA_OP_C, _abi.NCA_OP_C: _abi.NCA_OP_N}
+    if A.op not in ops:
+        raise ValueError("adjoint of a transposed map")
+    w = H2Workspace(A.core, ops[A.op])
+    w._busy = A._busy
+    return w
+
+
+def PetrovGalerkinNCA(kernel, testpoints, trialpoints, tree, testcompressor, trialcompressor, **kw):
+    """Assemble the H^2 matrix and return it as an H2Workspace (petrovgalerkinnca.jl:327-331)."""
+    return H2Workspace(PetrovGalerkinNCACore(kernel, testpoints, trialpoints, tree, testcompressor,
+                                             trialcompressor, **kw))

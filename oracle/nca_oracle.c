@@ -1,0 +1,752 @@
+/*
+ * ORACLE — test infrastructure only (see nca_oracle.h).
+ *
+ * FP64 CPU restatement of the batched bottom-up NCA compressor and the H^2
+ * matvec. Every function cites the reference file it restates. Compiled with
+ * -ffp-contract=off so that no multiply-add is fused behind our back: the
+ * pivot-deciding arithmetic (entries, residual updates, norms) is written out
+ * operation by operation, in a fixed order.
+ */
+#include "nca_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+    double re, im;
+} cplx;
+
+static inline cplx cmul(cplx a, cplx b) {
+    cplx r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    return r;
+}
+static inline cplx csub(cplx a, cplx b) {
+    cplx r = {a.re - b.re, a.im - b.im};
+    return r;
+}
+static inline cplx cadd(cplx a, cplx b) {
+    cplx r = {a.re + b.re, a.im + b.im};
+    return r;
+}
+static inline double cabs2(cplx a) { return a.re * a.re + a.im * a.im; }
+static inline double cabs1(cplx a) { return fabs(a.re) + fabs(a.im); }
+static inline cplx cdiv(cplx a, cplx b) {
+    double d = b.re * b.re + b.im * b.im;
+    cplx r = {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+    return r;
+}
+static inline cplx cconj(cplx a) {
+    cplx r = {a.re, -a.im};
+    return r;
+}
+
+/* ------------------------------------------------------------------------- */
+/* sin/cos: Cody-Waite reduction by pi/2 in three parts (the first step exact
+ * under fma), then Taylor polynomials through x^19 (sin) and x^20 (cos) on
+ * |y| <= pi/4. Coefficients are (-1)^k/(2k+1)! and (-1)^k/(2k)! rounded to
+ * double. Same method as the device code, written independently here. */
+#define ORC_TWO_OVER_PI 0x1.45f306dc9c883p-1
+#define ORC_PIO2_HI 0x1.921fb54442d18p+0
+#define ORC_PIO2_MID 0x1.1a62633145c07p-54
+#define ORC_PIO2_LO -0x1.f1976b7ed8fbcp-110
+#define ORC_FOURPI 0x1.921fb54442d18p+3
+
+void orc_sincos(double x, double* s_out, double* c_out) {
+    double q = rint(x * ORC_TWO_OVER_PI);
+    double y = fma(-q, ORC_PIO2_HI, x);
+    y = fma(-q, ORC_PIO2_MID, y);
+    y = fma(-q, ORC_PIO2_LO, y);
+    double z = y * y;
+    double ps = -0x1.2f49b46814157p-57;
+    ps = fma(ps, z, 0x1.952c77030ad4ap-49);
+    ps = fma(ps, z, -0x1.ae7f3e733b81fp-41);
+    ps = fma(ps, z, 0x1.6124613a86d09p-33);
+    ps = fma(ps, z, -0x1.ae64567f544e4p-26);
+    ps = fma(ps, z, 0x1.71de3a556c734p-19);
+    ps = fma(ps, z, -0x1.a01a01a01a01ap-13);
+    ps = fma(ps, z, 0x1.1111111111111p-7);
+    ps = fma(ps, z, -0x1.5555555555555p-3);
+    double sn = fma(y * z, ps, y);
+    double pc = 0x1.e542ba4020225p-62;
+    pc = fma(pc, z, -0x1.6827863b97d97p-53);
+    pc = fma(pc, z, 0x1.ae7f3e733b81fp-45);
+    pc = fma(pc, z, -0x1.93974a8c07c9dp-37);
+    pc = fma(pc, z, 0x1.1eed8eff8d898p-29);
+    pc = fma(pc, z, -0x1.27e4fb7789f5cp-22);
+    pc = fma(pc, z, 0x1.a01a01a01a01ap-16);
+    pc = fma(pc, z, -0x1.6c16c16c16c17p-10);
+    pc = fma(pc, z, 0x1.5555555555555p-5);
+    double cs = fma(z * z, pc, fma(-0.5, z, 1.0));
+    long long n = (long long)q;
+    switch (n & 3) {
+        case 0: *s_out = sn; *c_out = cs; break;
+        case 1: *s_out = cs; *c_out = -sn; break;
+        case 2: *s_out = -sn; *c_out = -cs; break;
+        default: *s_out = -cs; *c_out = sn; break;
+    }
+}
+
+/* Helmholtz point kernel cis(sign*k*r)/(4*pi*r), zero at r == 0
+ * (README.md:104-108; test/nca/test_petrovgalerkinnca.jl:14-17). */
+static inline cplx entry(double k, int32_t sign, const double* x, const double* y) {
+    double dx = x[0] - y[0], dy = x[1] - y[1], dz = x[2] - y[2];
+    double r2 = dx * dx + dy * dy;
+    r2 = r2 + dz * dz;
+    cplx g = {0.0, 0.0};
+    if (r2 == 0.0) return g;
+    double r = sqrt(r2);
+    double s, c;
+    orc_sincos(k * r, &s, &c);
+    double d = ORC_FOURPI * r;
+    g.re = c / d;
+    g.im = s / d;
+    if (sign < 0) g.im = -g.im;
+    return g;
+}
+
+void orc_entry(double k, int32_t sign, const double* x, const double* y, double* g) {
+    cplx v = entry(k, sign, x, y);
+    g[0] = v.re;
+    g[1] = v.im;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Phase 2: assemble_level (batchedbottomup.jl:101-116) into the column-padded
+ * level buffer of the device path (distributedblockstack.jl:7-13,80-86). With
+ * transposed != 0 block b is transpose(A[colidx_b, rowidx_b]) without
+ * conjugation (batchedbottomup.jl:98-99,110-111). */
+int32_t orc_assemble_level(double k, int32_t sign, const double* testpts, const double* trialpts,
+                           int64_t nb, const int64_t* rowptr, const int64_t* rowidx,
+                           const int64_t* colptr, const int64_t* colidx, int64_t maxrank,
+                           int32_t transposed, double* buffer) {
+    int64_t ld = rowptr[nb] - rowptr[0];
+    cplx* buf = (cplx*)buffer;
+    memset(buf, 0, sizeof(cplx) * (size_t)(ld * maxrank));
+    for (int64_t b = 0; b < nb; ++b)
+        if (colptr[b + 1] - colptr[b] > maxrank) return 1;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t b = 0; b < nb; ++b) {
+        int64_t r0 = rowptr[b] - rowptr[0];
+        int64_t m = rowptr[b + 1] - rowptr[b];
+        int64_t n = colptr[b + 1] - colptr[b];
+        for (int64_t j = 0; j < n; ++j) {
+            int64_t cj = colidx[colptr[b] - 1 + j] - 1;
+            for (int64_t a = 0; a < m; ++a) {
+                int64_t ra = rowidx[rowptr[b] - 1 + a] - 1;
+                buf[r0 + a + j * ld] = transposed ? entry(k, sign, testpts + 3 * cj, trialpts + 3 * ra)
+                                                  : entry(k, sign, testpts + 3 * ra, trialpts + 3 * cj);
+            }
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Fully pivoted ACA with the sampled stop (batchedbottomup.jl:118-125,136-138;
+ * call site compresslevel.jl:34-36; the arithmetic lives in the absent
+ * AdaptiveCrossApproximation.jl 0.4 and is restated here):
+ *   r accepted crosses, residual R_r, s = number of block columns;
+ *   stop when r == min(maxrank, m, n), or when
+ *       ||R_r||_F^2 * (sampled ? s/(s-r) : 1) <= tol^2 * ||A||_F^2,
+ *   else pivot (i, j) = first argmax |R_r(i,j)|^2 in column-major order,
+ *   R_{r+1} = R_r - R_r(:,j) * (R_r(i,:) / R_r(i,j)), row i and column j set
+ *   to exactly zero.
+ * Squared norms are summed in a fixed order: 256 partial sums, entry e of the
+ * column-major block into slot e mod 256 in increasing e, then a halving tree
+ * (slot t += slot t+s for s = 128, 64, ..., 1). */
+#define ORC_SLOTS 256
+
+static double sum_slots(double* p) {
+    for (int s = ORC_SLOTS / 2; s > 0; s >>= 1)
+        for (int t = 0; t < s; ++t) p[t] = p[t] + p[t + s];
+    return p[0];
+}
+
+int64_t orc_aca_fullpiv(double* block, int64_t m, int64_t n, int64_t ld, int64_t maxrank, double tol,
+                        int32_t sampled, int64_t* rowpiv, int64_t* colpiv) {
+    cplx* B = (cplx*)block;
+    int64_t kmax = maxrank;
+    if (m < kmax) kmax = m;
+    if (n < kmax) kmax = n;
+    if (kmax <= 0) return 0;
+    double p[ORC_SLOTS];
+    memset(p, 0, sizeof(p));
+    double best = -1.0;
+    int64_t bi = 0;
+    for (int64_t b = 0; b < n; ++b)
+        for (int64_t a = 0; a < m; ++a) {
+            int64_t e = a + b * m;
+            double v = cabs2(B[a + b * ld]);
+            p[e & (ORC_SLOTS - 1)] += v;
+            if (v > best) {
+                best = v;
+                bi = e;
+            }
+        }
+    const double nrmA2 = sum_slots(p);
+    double nrm2 = nrmA2;
+    const double tol2 = tol * tol;
+    cplx* u = (cplx*)malloc(sizeof(cplx) * (size_t)m);
+    cplx* v = (cplx*)malloc(sizeof(cplx) * (size_t)n);
+    int64_t r = 0;
+    while (r < kmax) {
+        double scale = sampled ? (double)n / (double)(n - r) : 1.0;
+        if (nrm2 * scale <= tol2 * nrmA2) break;
+        if (!(best > 0.0)) break;
+        int64_t i = bi % m, j = bi / m;
+        cplx piv = B[i + j * ld];
+        double d = cabs2(piv);
+        cplx inv = {piv.re / d, -piv.im / d};
+        for (int64_t a = 0; a < m; ++a) u[a] = B[a + j * ld];
+        for (int64_t c = 0; c < n; ++c) v[c] = cmul(B[i + c * ld], inv);
+        rowpiv[r] = i + 1;
+        colpiv[r] = j + 1;
+        ++r;
+        memset(p, 0, sizeof(p));
+        best = -1.0;
+        bi = 0;
+        for (int64_t c = 0; c < n; ++c)
+            for (int64_t a = 0; a < m; ++a) {
+                int64_t e = a + c * m;
+                cplx val;
+                if (a == i || c == j) {
+                    val.re = 0.0;
+                    val.im = 0.0;
+                } else {
+                    val = csub(B[a + c * ld], cmul(u[a], v[c]));
+                }
+                B[a + c * ld] = val;
+                double w = cabs2(val);
+                p[e & (ORC_SLOTS - 1)] += w;
+                if (w > best) {
+                    best = w;
+                    bi = e;
+                }
+            }
+        nrm2 = sum_slots(p);
+    }
+    free(u);
+    free(v);
+    return r;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Skeleton solve X = L * P^{-1}, P = L[rowpiv, :] (r x r), as nestedtestbasis /
+ * testtransfer do with rdiv! (nestedbasis.jl:43-49, transfermatrices.jl:183-187)
+ * and as the device path does for both sides (compresslevel.jl:14-25).
+ * LU with partial pivoting in LAPACK getrf form (pivot = first max |re|+|im|),
+ * then per row of L: w U = l (forward), y L = w (backward, unit diagonal),
+ * x = y with the interchanges undone from the last to the first.
+ * Singularity rule of _crossfactorization (utils.jl:57-80). */
+int64_t orc_skeleton_solve(const double* Lin, int64_t m, int64_t r, const int64_t* rowpiv, double* Xout) {
+    if (r == 0) return -1;
+    const cplx* L = (const cplx*)Lin;
+    cplx* X = (cplx*)Xout;
+    cplx* A = (cplx*)malloc(sizeof(cplx) * (size_t)(r * r));
+    int64_t* ipiv = (int64_t*)malloc(sizeof(int64_t) * (size_t)r);
+    cplx* w = (cplx*)malloc(sizeof(cplx) * (size_t)r);
+    for (int64_t c = 0; c < r; ++c)
+        for (int64_t a = 0; a < r; ++a) A[a + c * r] = L[(rowpiv[a] - 1) + c * m];
+    int64_t info = 0;
+    for (int64_t k = 0; k < r; ++k) {
+        int64_t p = k;
+        double bestv = cabs1(A[k + k * r]);
+        for (int64_t i = k + 1; i < r; ++i) {
+            double v = cabs1(A[i + k * r]);
+            if (v > bestv) {
+                bestv = v;
+                p = i;
+            }
+        }
+        ipiv[k] = p;
+        if (bestv != 0.0) {
+            if (p != k)
+                for (int64_t c = 0; c < r; ++c) {
+                    cplx t = A[k + c * r];
+                    A[k + c * r] = A[p + c * r];
+                    A[p + c * r] = t;
+                }
+            for (int64_t i = k + 1; i < r; ++i) A[i + k * r] = cdiv(A[i + k * r], A[k + k * r]);
+        } else if (info == 0) {
+            info = k + 1;
+        }
+        for (int64_t c = k + 1; c < r; ++c)
+            for (int64_t i = k + 1; i < r; ++i) A[i + c * r] = csub(A[i + c * r], cmul(A[i + k * r], A[k + c * r]));
+    }
+    double d2max = 0.0, d2min = INFINITY;
+    for (int64_t i = 0; i < r; ++i) {
+        double d2 = cabs2(A[i + i * r]);
+        if (d2 > d2max) d2max = d2;
+        if (d2 < d2min) d2min = d2;
+    }
+    double tol2 = (10.0 * DBL_EPSILON) * (10.0 * DBL_EPSILON) * d2max;
+    int64_t result = -1;
+    if (!(info == 0 && d2min > tol2)) {
+        int64_t rank = 0;
+        for (int64_t i = 0; i < r; ++i) rank += cabs2(A[i + i * r]) > tol2;
+        result = rank;
+    }
+    for (int64_t a = 0; a < m; ++a) {
+        for (int64_t c = 0; c < r; ++c) {
+            cplx acc = L[a + c * m];
+            for (int64_t k = 0; k < c; ++k) acc = csub(acc, cmul(w[k], A[k + c * r]));
+            w[c] = cdiv(acc, A[c + c * r]);
+        }
+        for (int64_t c = r - 1; c >= 0; --c) {
+            cplx acc = w[c];
+            for (int64_t k = c + 1; k < r; ++k) acc = csub(acc, cmul(w[k], A[k + c * r]));
+            w[c] = acc;
+        }
+        for (int64_t k = r - 1; k >= 0; --k) {
+            int64_t p = ipiv[k];
+            if (p != k) {
+                cplx t = w[k];
+                w[k] = w[p];
+                w[p] = t;
+            }
+        }
+        for (int64_t c = 0; c < r; ++c) X[a + c * m] = w[c];
+    }
+    free(A);
+    free(ipiv);
+    free(w);
+    return result;
+}
+
+/* ------------------------------------------------------------------------- */
+/* testbases / trialbases (::BatchedBottomUp) (batchedbottomup.jl:246-447):
+ * levels from the deepest to the root; active clusters have far pivots; rows
+ * of a leaf block are its points, of an inner block the concatenated pivots of
+ * its children (:282-303); Phase 3 per block (compresslevel_test!/trial!
+ * :145-244). side 0 = test, 1 = trial (transposed assembly, :408-410). */
+typedef struct {
+    int64_t rank, m;
+    int64_t* piv; /* global 1-based point ids, rank entries */
+    cplx* X;      /* m x rank */
+} node_result;
+
+int64_t orc_bases(int32_t side, double k, int32_t sign, const double* testpts, const double* trialpts,
+                  int64_t nn, const int64_t* perm, const int64_t* nodelo, const int64_t* nodehi,
+                  const int64_t* level, const int64_t* childptr, const int64_t* children,
+                  const int64_t* farpivptr, const int64_t* farpiv, int64_t maxrank, double tol,
+                  int32_t sampled, int64_t* pivptr, int64_t* pivots, int64_t* xoffsets, double* Xout,
+                  int64_t xcap) {
+    node_result* res = (node_result*)calloc((size_t)nn, sizeof(node_result));
+    int64_t maxlev = 0, minlev = INT64_MAX;
+    for (int64_t t = 0; t < nn; ++t) {
+        res[t].rank = -1;
+        if (level[t] > maxlev) maxlev = level[t];
+        if (level[t] < minlev) minlev = level[t];
+    }
+    int64_t singular = 0;
+    int64_t* active = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nn > 0 ? nn : 1));
+    for (int64_t lev = maxlev; lev >= minlev && !singular; --lev) {
+        int64_t na = 0;
+        for (int64_t t = 0; t < nn; ++t)
+            if (level[t] == lev && farpivptr[t + 1] > farpivptr[t]) active[na++] = t;
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int64_t q = 0; q < na; ++q) {
+            int64_t t = active[q];
+            int leaf = childptr[t + 1] == childptr[t];
+            int64_t m = 0;
+            if (leaf) {
+                m = nodehi[t] - nodelo[t] + 1;
+            } else {
+                for (int64_t c = childptr[t] - 1; c < childptr[t + 1] - 1; ++c) {
+                    int64_t ch = children[c] - 1;
+                    if (res[ch].rank > 0) m += res[ch].rank;
+                }
+            }
+            int64_t* rows = (int64_t*)malloc(sizeof(int64_t) * (size_t)(m > 0 ? m : 1));
+            if (leaf) {
+                for (int64_t a = 0; a < m; ++a) rows[a] = perm[nodelo[t] - 1 + a];
+            } else {
+                int64_t o = 0;
+                for (int64_t c = childptr[t] - 1; c < childptr[t + 1] - 1; ++c) {
+                    int64_t ch = children[c] - 1;
+                    for (int64_t a = 0; a < res[ch].rank; ++a) rows[o++] = res[ch].piv[a];
+                }
+            }
+            int64_t n = farpivptr[t + 1] - farpivptr[t];
+            const int64_t* cols = farpiv + farpivptr[t] - 1;
+            cplx* blk = (cplx*)malloc(sizeof(cplx) * (size_t)(m * n > 0 ? m * n : 1));
+            cplx* work = (cplx*)malloc(sizeof(cplx) * (size_t)(m * n > 0 ? m * n : 1));
+            for (int64_t j = 0; j < n; ++j)
+                for (int64_t a = 0; a < m; ++a) {
+                    const double* rp = (side == 0 ? testpts : trialpts) + 3 * (rows[a] - 1);
+                    const double* cp = (side == 0 ? trialpts : testpts) + 3 * (cols[j] - 1);
+                    blk[a + j * m] = side == 0 ? entry(k, sign, rp, cp) : entry(k, sign, cp, rp);
+                }
+            memcpy(work, blk, sizeof(cplx) * (size_t)(m * n));
+            int64_t* rpv = (int64_t*)malloc(sizeof(int64_t) * (size_t)(maxrank > 0 ? maxrank : 1));
+            int64_t* cpv = (int64_t*)malloc(sizeof(int64_t) * (size_t)(maxrank > 0 ? maxrank : 1));
+            int64_t r = orc_aca_fullpiv((double*)work, m, n, m, maxrank, tol, sampled, rpv, cpv);
+            cplx* Lm = (cplx*)malloc(sizeof(cplx) * (size_t)(m * r > 0 ? m * r : 1));
+            for (int64_t c = 0; c < r; ++c)
+                for (int64_t a = 0; a < m; ++a) Lm[a + c * m] = blk[a + (cpv[c] - 1) * m];
+            cplx* X = (cplx*)malloc(sizeof(cplx) * (size_t)(m * r > 0 ? m * r : 1));
+            int64_t st = orc_skeleton_solve((const double*)Lm, m, r, rpv, (double*)X);
+            if (st >= 0) {
+#pragma omp critical
+                { if (!singular) singular = t + 1; }
+            }
+            res[t].rank = r;
+            res[t].m = m;
+            res[t].piv = (int64_t*)malloc(sizeof(int64_t) * (size_t)(r > 0 ? r : 1));
+            for (int64_t a = 0; a < r; ++a) res[t].piv[a] = rows[rpv[a] - 1];
+            res[t].X = X;
+            free(rows);
+            free(blk);
+            free(work);
+            free(rpv);
+            free(cpv);
+            free(Lm);
+        }
+    }
+    free(active);
+    int64_t ret = singular ? -singular : 0;
+    pivptr[0] = 1;
+    xoffsets[0] = 0;
+    for (int64_t t = 0; t < nn; ++t) {
+        int64_t r = res[t].rank > 0 ? res[t].rank : 0;
+        pivptr[t + 1] = pivptr[t] + r;
+        xoffsets[t + 1] = xoffsets[t] + (res[t].rank > 0 ? res[t].m * r : 0);
+    }
+    if (xoffsets[nn] > xcap && ret == 0) ret = 1;
+    for (int64_t t = 0; t < nn; ++t) {
+        int64_t r = res[t].rank > 0 ? res[t].rank : 0;
+        for (int64_t a = 0; a < r; ++a) pivots[pivptr[t] - 1 + a] = res[t].piv[a];
+        if (ret == 0 && r > 0)
+            memcpy((cplx*)Xout + xoffsets[t], res[t].X, sizeof(cplx) * (size_t)(res[t].m * r));
+        free(res[t].piv);
+        free(res[t].X);
+    }
+    /* nodes with rank 0 but far pivots still count as assigned; the caller
+     * reads the ranks from pivptr */
+    free(res);
+    return ret;
+}
+
+/* ------------------------------------------------------------------------- */
+/* assemble_couplingstore (couplingmatrices.jl:11-53): block i of the test CSR
+ * row of t is A[testpivots[t], trialpivots[farnodes[i]]]; empty sides give
+ * empty blocks. */
+void orc_coupling(double k, int32_t sign, const double* testpts, const double* trialpts, int64_t ntest,
+                  const int64_t* farptr, const int64_t* farnodes, const int64_t* testpivptr,
+                  const int64_t* testpiv, const int64_t* trialpivptr, const int64_t* trialpiv,
+                  const int64_t* offsets, double* blocks) {
+    cplx* out = (cplx*)blocks;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t t = 0; t < ntest; ++t) {
+        int64_t rt = testpivptr[t + 1] - testpivptr[t];
+        for (int64_t i = farptr[t] - 1; i < farptr[t + 1] - 1; ++i) {
+            int64_t s = farnodes[i] - 1;
+            int64_t rs = trialpivptr[s + 1] - trialpivptr[s];
+            cplx* S = out + offsets[i];
+            for (int64_t j = 0; j < rs; ++j)
+                for (int64_t a = 0; a < rt; ++a)
+                    S[a + j * rt] = entry(k, sign, testpts + 3 * (testpiv[testpivptr[t] - 1 + a] - 1),
+                                          trialpts + 3 * (trialpiv[trialpivptr[s] - 1 + j] - 1));
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* H^2 matvec (petrovgalerkinnca.jl:431-589): leaf projection, upward transfer,
+ * coupling (forward: per test cluster; reverse: accumulated serially as
+ * couple.jl:119-142 does), downward transfer, leaf expansion, near field
+ * (evaluated here instead of stored). op 1/2 swap the sides and transpose
+ * (and conjugate) every operator. */
+static inline cplx ld_c(const double* base, int64_t idx, int conj) {
+    cplx v = {base[2 * idx], base[2 * idx + 1]};
+    if (conj) v.im = -v.im;
+    return v;
+}
+
+void orc_h2_matvec(const orc_h2* h, int32_t op, int64_t nrhs, const double* xin, int64_t ldx, double* yio,
+                   int64_t ldy, const double* alpha, const double* beta, int32_t nearon, int64_t nrows,
+                   const int64_t* rows) {
+    const int fwd = op == 0;
+    const int cj = op == 2;
+    /* source side (x) and destination side (y) */
+    const int64_t nsrc_pts = fwd ? h->N : h->M, ndst_pts = fwd ? h->M : h->N;
+    const int64_t* src_perm = fwd ? h->trial_perm : h->test_perm;
+    const int64_t* src_lo = fwd ? h->trial_lo : h->test_lo;
+    const int64_t* src_hi = fwd ? h->trial_hi : h->test_hi;
+    const int64_t* dst_perm = fwd ? h->test_perm : h->trial_perm;
+    const int64_t* dst_lo = fwd ? h->test_lo : h->trial_lo;
+    const int64_t* dst_hi = fwd ? h->test_hi : h->trial_hi;
+    const int64_t* src_cptr = fwd ? h->trial_cptr : h->test_cptr;
+    const int64_t* dst_cptr = fwd ? h->test_cptr : h->trial_cptr;
+    const int64_t nn_src = fwd ? h->nn_trial : h->nn_test;
+    const int64_t nn_dst = fwd ? h->nn_test : h->nn_trial;
+    const int64_t Ks = src_cptr[nn_src] - 1, Kd = dst_cptr[nn_dst] - 1;
+    (void)nsrc_pts;
+    const cplx* X = (const cplx*)xin;
+    cplx* Y = (cplx*)yio;
+    cplx* xh = (cplx*)calloc((size_t)(Ks * nrhs + 1), sizeof(cplx));
+    cplx* yh = (cplx*)calloc((size_t)(Kd * nrhs + 1), sizeof(cplx));
+    cplx* acc = (cplx*)calloc((size_t)(ndst_pts * nrhs + 1), sizeof(cplx));
+    char* want = NULL;
+    if (nrows >= 0) {
+        want = (char*)calloc((size_t)(ndst_pts + 1), 1);
+        for (int64_t i = 0; i < nrows; ++i) want[rows[i]] = 1;
+    }
+
+    /* Stage 1: leaf projection (aggregate.jl:1-19). */
+    {
+        int64_t nb = fwd ? h->nbasis_trial : h->nbasis_test;
+        const int64_t* nodes = fwd ? h->basis_trial_nodes : h->basis_test_nodes;
+        const int64_t* off = fwd ? h->basis_trial_off : h->basis_test_off;
+        const double* blocks = fwd ? h->basis_trial : h->basis_test;
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int64_t i = 0; i < nb; ++i) {
+            int64_t s = nodes[i] - 1;
+            int64_t r = src_cptr[s + 1] - src_cptr[s];
+            int64_t ns = src_hi[s] - src_lo[s] + 1;
+            for (int64_t q = 0; q < nrhs; ++q)
+                for (int64_t a = 0; a < r; ++a) {
+                    cplx sum = {0.0, 0.0};
+                    for (int64_t j = 0; j < ns; ++j) {
+                        /* forward: V (r x ns) as is; transposed: U (ns x r) transposed */
+                        cplx b = ld_c(blocks, off[i] + (fwd ? a + j * r : j + a * ns), cj);
+                        cplx xv = X[(src_perm[src_lo[s] - 1 + j] - 1) + q * ldx];
+                        sum = cadd(sum, cmul(b, xv));
+                    }
+                    xh[(src_cptr[s] - 1 + a) + q * Ks] = sum;
+                }
+        }
+    }
+    /* Stage 2: upward transfer, deepest level first (aggregate.jl:44-67). */
+    {
+        int64_t nlev = fwd ? h->nlev_trial : h->nlev_test;
+        const int64_t* levptr = fwd ? h->st_levptr : h->tt_levptr;
+        const int64_t* levidcs = fwd ? h->st_levidcs : h->tt_levidcs;
+        const int64_t* chdptr = fwd ? h->st_chdptr : h->tt_chdptr;
+        const int64_t* chds = fwd ? h->st_chds : h->tt_chds;
+        const int64_t* off = fwd ? h->st_off : h->tt_off;
+        const double* blocks = fwd ? h->st_blocks : h->tt_blocks;
+        for (int64_t lev = nlev - 1; lev >= 0; --lev) {
+#pragma omp parallel for schedule(dynamic, 16)
+            for (int64_t idx = levptr[lev] - 1; idx < levptr[lev + 1] - 1; ++idx) {
+                int64_t p = levidcs[idx] - 1;
+                int64_t rp = src_cptr[p + 1] - src_cptr[p];
+                for (int64_t e = chdptr[idx] - 1; e < chdptr[idx + 1] - 1; ++e) {
+                    int64_t c = chds[e] - 1;
+                    int64_t rc = src_cptr[c + 1] - src_cptr[c];
+                    for (int64_t q = 0; q < nrhs; ++q)
+                        for (int64_t a = 0; a < rp; ++a) {
+                            cplx sum = xh[(src_cptr[p] - 1 + a) + q * Ks];
+                            for (int64_t j = 0; j < rc; ++j) {
+                                /* forward: trial E (rp x rc); transposed: test E (rc x rp) transposed */
+                                cplx b = ld_c(blocks, off[e] + (fwd ? a + j * rp : j + a * rc), cj);
+                                sum = cadd(sum, cmul(b, xh[(src_cptr[c] - 1 + j) + q * Ks]));
+                            }
+                            xh[(src_cptr[p] - 1 + a) + q * Ks] = sum;
+                        }
+                }
+            }
+        }
+    }
+    /* Stage 3: coupling (couple.jl:1-24 forward, :119-142 reverse). */
+    if (fwd) {
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int64_t t = 0; t < h->nn_test; ++t) {
+            int64_t rt = h->test_cptr[t + 1] - h->test_cptr[t];
+            for (int64_t i = h->cp_ptr[t] - 1; i < h->cp_ptr[t + 1] - 1; ++i) {
+                int64_t s = h->cp_nodes[i] - 1;
+                int64_t rs = h->trial_cptr[s + 1] - h->trial_cptr[s];
+                for (int64_t q = 0; q < nrhs; ++q)
+                    for (int64_t a = 0; a < rt; ++a) {
+                        cplx sum = yh[(h->test_cptr[t] - 1 + a) + q * Kd];
+                        for (int64_t j = 0; j < rs; ++j)
+                            sum = cadd(sum, cmul(ld_c(h->cp_blocks, h->cp_off[i] + a + j * rt, cj),
+                                                 xh[(h->trial_cptr[s] - 1 + j) + q * Ks]));
+                        yh[(h->test_cptr[t] - 1 + a) + q * Kd] = sum;
+                    }
+            }
+        }
+    } else {
+        for (int64_t t = 0; t < h->nn_test; ++t) {
+            int64_t rt = h->test_cptr[t + 1] - h->test_cptr[t];
+            for (int64_t i = h->cp_ptr[t] - 1; i < h->cp_ptr[t + 1] - 1; ++i) {
+                int64_t s = h->cp_nodes[i] - 1;
+                int64_t rs = h->trial_cptr[s + 1] - h->trial_cptr[s];
+                for (int64_t q = 0; q < nrhs; ++q)
+                    for (int64_t a = 0; a < rs; ++a) {
+                        cplx sum = yh[(h->trial_cptr[s] - 1 + a) + q * Kd];
+                        for (int64_t j = 0; j < rt; ++j)
+                            sum = cadd(sum, cmul(ld_c(h->cp_blocks, h->cp_off[i] + j + a * rt, cj),
+                                                 xh[(h->test_cptr[t] - 1 + j) + q * Ks]));
+                        yh[(h->trial_cptr[s] - 1 + a) + q * Kd] = sum;
+                    }
+            }
+        }
+    }
+    /* Stage 4: downward transfer, root level first (disaggregate.jl:1-23). */
+    {
+        int64_t nlev = fwd ? h->nlev_test : h->nlev_trial;
+        const int64_t* levptr = fwd ? h->tt_levptr : h->st_levptr;
+        const int64_t* levidcs = fwd ? h->tt_levidcs : h->st_levidcs;
+        const int64_t* chdptr = fwd ? h->tt_chdptr : h->st_chdptr;
+        const int64_t* chds = fwd ? h->tt_chds : h->st_chds;
+        const int64_t* off = fwd ? h->tt_off : h->st_off;
+        const double* blocks = fwd ? h->tt_blocks : h->st_blocks;
+        for (int64_t lev = 0; lev < nlev; ++lev) {
+#pragma omp parallel for schedule(dynamic, 16)
+            for (int64_t idx = levptr[lev] - 1; idx < levptr[lev + 1] - 1; ++idx) {
+                int64_t p = levidcs[idx] - 1;
+                int64_t rp = dst_cptr[p + 1] - dst_cptr[p];
+                for (int64_t e = chdptr[idx] - 1; e < chdptr[idx + 1] - 1; ++e) {
+                    int64_t c = chds[e] - 1;
+                    int64_t rc = dst_cptr[c + 1] - dst_cptr[c];
+                    for (int64_t q = 0; q < nrhs; ++q)
+                        for (int64_t a = 0; a < rc; ++a) {
+                            cplx sum = yh[(dst_cptr[c] - 1 + a) + q * Kd];
+                            for (int64_t j = 0; j < rp; ++j) {
+                                /* forward: test E (rc x rp); transposed: trial E (rp x rc) transposed */
+                                cplx b = ld_c(blocks, off[e] + (fwd ? a + j * rc : j + a * rp), cj);
+                                sum = cadd(sum, cmul(b, yh[(dst_cptr[p] - 1 + j) + q * Kd]));
+                            }
+                            yh[(dst_cptr[c] - 1 + a) + q * Kd] = sum;
+                        }
+                }
+            }
+        }
+    }
+    /* Stage 5: leaf expansion (disaggregate.jl:62-80). */
+    {
+        int64_t nb = fwd ? h->nbasis_test : h->nbasis_trial;
+        const int64_t* nodes = fwd ? h->basis_test_nodes : h->basis_trial_nodes;
+        const int64_t* off = fwd ? h->basis_test_off : h->basis_trial_off;
+        const double* blocks = fwd ? h->basis_test : h->basis_trial;
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int64_t i = 0; i < nb; ++i) {
+            int64_t t = nodes[i] - 1;
+            int64_t r = dst_cptr[t + 1] - dst_cptr[t];
+            int64_t nt = dst_hi[t] - dst_lo[t] + 1;
+            for (int64_t q = 0; q < nrhs; ++q)
+                for (int64_t a = 0; a < nt; ++a) {
+                    int64_t row = dst_perm[dst_lo[t] - 1 + a] - 1;
+                    if (want && !want[row]) continue;
+                    cplx sum = acc[row + q * ndst_pts];
+                    for (int64_t j = 0; j < r; ++j) {
+                        /* forward: U (nt x r) as is; transposed: V (r x nt) transposed */
+                        cplx b = ld_c(blocks, off[i] + (fwd ? a + j * nt : j + a * r), cj);
+                        sum = cadd(sum, cmul(b, yh[(dst_cptr[t] - 1 + j) + q * Kd]));
+                    }
+                    acc[row + q * ndst_pts] = sum;
+                }
+        }
+    }
+    /* Near field (petrovgalerkinnca.jl:474,531,587; blocks of abstractnca.jl:134-178). */
+    if (nearon) {
+        if (fwd) {
+#pragma omp parallel for schedule(dynamic, 4)
+            for (int64_t t = 0; t < h->nn_test; ++t) {
+                for (int64_t a = h->test_lo[t] - 1; a < h->test_hi[t] && h->nearptr[t + 1] > h->nearptr[t]; ++a) {
+                    int64_t row = h->test_perm[a] - 1;
+                    if (want && !want[row]) continue;
+                    for (int64_t q = 0; q < nrhs; ++q) {
+                        cplx sum = acc[row + q * ndst_pts];
+                        for (int64_t i = h->nearptr[t] - 1; i < h->nearptr[t + 1] - 1; ++i) {
+                            int64_t s = h->nearnodes[i] - 1;
+                            for (int64_t b = h->trial_lo[s] - 1; b < h->trial_hi[s]; ++b) {
+                                int64_t col = h->trial_perm[b] - 1;
+                                cplx g = entry(h->k, h->sign, h->testpts + 3 * row, h->trialpts + 3 * col);
+                                sum = cadd(sum, cmul(g, X[col + q * ldx]));
+                            }
+                        }
+                        acc[row + q * ndst_pts] = sum;
+                    }
+                }
+            }
+        } else {
+            /* transpose the near CSR: trial node -> test nodes */
+            int64_t nnz = h->nearptr[h->nn_test] - 1;
+            int64_t* tptr = (int64_t*)calloc((size_t)(h->nn_trial + 1), sizeof(int64_t));
+            int64_t* tidx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nnz > 0 ? nnz : 1));
+            for (int64_t i = 0; i < nnz; ++i) tptr[h->nearnodes[i]]++;
+            for (int64_t s = 0; s < h->nn_trial; ++s) tptr[s + 1] += tptr[s];
+            int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * (size_t)(h->nn_trial + 1));
+            memcpy(cur, tptr, sizeof(int64_t) * (size_t)(h->nn_trial + 1));
+            for (int64_t t = 0; t < h->nn_test; ++t)
+                for (int64_t i = h->nearptr[t] - 1; i < h->nearptr[t + 1] - 1; ++i)
+                    tidx[cur[h->nearnodes[i] - 1]++] = t;
+#pragma omp parallel for schedule(dynamic, 4)
+            for (int64_t s = 0; s < h->nn_trial; ++s) {
+                for (int64_t b = h->trial_lo[s] - 1; b < h->trial_hi[s] && tptr[s + 1] > tptr[s]; ++b) {
+                    int64_t row = h->trial_perm[b] - 1;
+                    if (want && !want[row]) continue;
+                    for (int64_t q = 0; q < nrhs; ++q) {
+                        cplx sum = acc[row + q * ndst_pts];
+                        for (int64_t i = tptr[s]; i < tptr[s + 1]; ++i) {
+                            int64_t t = tidx[i];
+                            for (int64_t a = h->test_lo[t] - 1; a < h->test_hi[t]; ++a) {
+                                int64_t col = h->test_perm[a] - 1;
+                                cplx g = entry(h->k, h->sign, h->testpts + 3 * col, h->trialpts + 3 * row);
+                                if (cj) g = cconj(g);
+                                sum = cadd(sum, cmul(g, X[col + q * ldx]));
+                            }
+                        }
+                        acc[row + q * ndst_pts] = sum;
+                    }
+                }
+            }
+            free(tptr);
+            free(tidx);
+            free(cur);
+        }
+    }
+    cplx al = {alpha[0], alpha[1]}, be = {beta[0], beta[1]};
+    for (int64_t q = 0; q < nrhs; ++q)
+        for (int64_t i = 0; i < ndst_pts; ++i) {
+            if (want && !want[i]) continue;
+            cplx yo = Y[i + q * ldy];
+            cplx v = cmul(al, acc[i + q * ndst_pts]);
+            if (be.re != 0.0 || be.im != 0.0) v = cadd(v, cmul(be, yo));
+            Y[i + q * ldy] = v;
+        }
+    free(xh);
+    free(yh);
+    free(acc);
+    free(want);
+}
+
+void orc_dense_rows(double k, int32_t sign, const double* testpts, const double* trialpts, int64_t N,
+                    int64_t nrows, const int64_t* rows, int64_t nrhs, const double* xin, int64_t ldx,
+                    double* yout) {
+    const cplx* X = (const cplx*)xin;
+    cplx* Y = (cplx*)yout;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t i = 0; i < nrows; ++i) {
+        const double* x = testpts + 3 * rows[i];
+        for (int64_t q = 0; q < nrhs; ++q) {
+            cplx sum = {0.0, 0.0};
+            for (int64_t j = 0; j < N; ++j) sum = cadd(sum, cmul(entry(k, sign, x, trialpts + 3 * j), X[j + q * ldx]));
+            Y[i + q * nrows] = sum;
+        }
+    }
+}
+
+int32_t orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_set_num_threads(int32_t n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

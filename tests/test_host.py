@@ -1,0 +1,129 @@
+"""Host-side logic (trees, partition, far data, Phase 1, plans) and the C-ABI
+library's exports. No GPU compute."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import nca_amd
+from nca_amd import compressors as C
+from nca_amd import geometry
+from nca_amd import nca as NC
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "nca_hip.h")).read()
+    names = set(re.findall(r"\b(nca_[a-z0-9_]+)\s*\(", header))
+    assert len(names) >= 25
+    lib = ctypes.CDLL(nca_amd._native.HIP_LIB)
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    assert set(nca_amd._abi.SIGNATURES) == names
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    from nca_amd import _native
+    monkeypatch.setattr(_native, "_hip", None)
+    monkeypatch.setattr(_native, "HIP_LIB", "/nonexistent/libnca_hip.so")
+    with pytest.raises(_native.NativeLibraryMissing):
+        _native.hip_lib()
+
+
+@pytest.mark.parametrize("kind", ["twon", "kmeans"])
+def test_tree_structure(kind):
+    pts = geometry.fibonacci_sphere(3000)
+    t = nca_amd.TwoNTree(pts, minvalues=40) if kind == "twon" else nca_amd.KMeansTree(pts, 2, 40)
+    assert sorted(t.perm.tolist()) == list(range(3000))
+    assert t.lo[0] == 0 and t.hi[0] == 3000 and t.parent[0] == -1
+    cover = np.zeros(3000, np.int64)
+    for n in range(t.nnodes):
+        ch = t.childrenof(n)
+        if len(ch):
+            # children partition the parent, in order
+            assert t.lo[ch[0]] == t.lo[n] and t.hi[ch[-1]] == t.hi[n]
+            assert np.all(t.lo[ch[1:]] == t.hi[ch[:-1]])
+            assert np.all(t.level[ch] == t.level[n] + 1)
+        else:
+            assert t.nvalues(n) <= 40 or kind == "twon"
+            cover[t.lo[n]:t.hi[n]] += 1
+    assert np.all(cover == 1)
+    # level order: ids increase with the level
+    assert np.all(np.diff(t.level) >= 0)
+
+
+def test_partition_and_fardata():
+    tp = geometry.grid_square(32)
+    sp = tp + np.array([1.1, 0.0, 0.2])
+    tree = nca_amd.BlockTree(nca_amd.KMeansTree(tp, 2, 32), nca_amd.KMeansTree(sp, 2, 32))
+    part = nca_amd.interactions(tree)
+    tt, st = tree.testtree, tree.trialtree
+    cover = np.zeros((1024, 1024), np.int64)
+    for t in range(tt.nnodes):
+        for s in part.farnodes[part.farptr[t]:part.farptr[t + 1]]:
+            cover[np.ix_(tt.values(t), st.values(s))] += 1
+        for s in part.nearnodes[part.nearptr[t]:part.nearptr[t + 1]]:
+            assert tt.isleaf(t) and st.isleaf(s)
+            cover[np.ix_(tt.values(t), st.values(s))] += 1
+    assert np.all(cover == 1)
+    tf, sf = nca_amd.fardata(tree, part)
+    pairs = {(t, s) for t in range(tt.nnodes) for s in tf.farnodesof(t)}
+    tpairs = {(t, s) for s in range(st.nnodes) for t in sf.farnodesof(s)}
+    assert pairs == tpairs
+    for s in range(st.nnodes):
+        assert list(sf.farnodesof(s)) == sorted(sf.farnodesof(s))
+
+
+def test_farfield_and_phase1_pivots():
+    pts = geometry.fibonacci_sphere(4096)
+    tr = nca_amd.TwoNTree(pts, minvalues=64)
+    tree = nca_amd.BlockTree(tr, tr)
+    part = nca_amd.interactions(tree)
+    tf, _ = nca_amd.fardata(tree, part)
+    ptr, idx = nca_amd.farpivots(tr, tr, tf, 50)
+    assert np.array_equal(nca_amd.farpivots(tr, tr, tf, 50)[1], idx)  # deterministic
+    for t in range(tr.nnodes):
+        F = nca_amd.farfield(tr, tf, t)
+        own = tf.farnodesof(t)
+        assert list(F[:len(own)]) == list(own)
+        piv = idx[ptr[t]:ptr[t + 1]]
+        if len(F) == 0:
+            assert len(piv) == 0
+            continue
+        farpts = np.concatenate([tr.values(f) for f in F])
+        # _selectfarpivots: min(maxrank, |t|, far points), distinct, from the far field
+        assert len(piv) == min(50, tr.nvalues(t), len(farpts))
+        assert len(set(piv.tolist())) == len(piv)
+        assert set(piv.tolist()) <= set(farpts.tolist())
+
+
+def test_plan_from_pivots():
+    """test/nca/test_abstractnca.jl:9-24."""
+    f = C.SideFactors(4)
+    f.assigned[:] = True
+    f.rank[:] = [2, 3, 0, 1]
+    assert list(NC.coefficient_ptr(f) + 1) == [1, 3, 6, 6, 7]
+    g = C.SideFactors(4)
+    g.assigned[1] = g.assigned[3] = True
+    g.rank[1], g.rank[3] = 2, 0
+    assert list(NC.coefficient_ptr(g) + 1) == [1, 1, 3, 3, 3]
+    assert list(NC.coefficient_ptr(C.SideFactors(0)) + 1) == [1]
+
+
+def test_concat_ranges():
+    """ptrfromcounts-style CSR helpers (test/utils/test_utils.jl:6-20)."""
+    assert list(C.concat_ranges([5, 0, 10, 3], [2, 0, 3, 1])) == [5, 6, 10, 11, 12, 3]
+    assert len(C.concat_ranges([], [])) == 0
+    assert list(C.concat_ranges([7], [0])) == []
+
+
+def test_batched_bottom_up_constructor():
+    with pytest.raises(TypeError):
+        C.BatchedBottomUp(pivotselector=C.GeometricPivoting())
+    with pytest.raises(TypeError):
+        C.BatchedBottomUp(compressor=C.ACA(1e-3))
+    b = C.BatchedBottomUp(C.GeometricPivoting(), C.ACA(1e-3))
+    assert b.compressor.tol == 1e-3

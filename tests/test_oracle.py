@@ -1,0 +1,326 @@
+"""Pins the oracle (oracle/nca_oracle.c) against the reference's known answers,
+closed forms and brute force, before it is trusted as the parity checker."""
+import mpmath
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from helpers import grid_problem, sphere_problem, oracle_assembly, oracle_operator, relerr
+from nca_amd import geometry
+
+
+def G(k, x, y, sign=1):
+    r = np.linalg.norm(np.asarray(x) - np.asarray(y))
+    return 0j if r == 0 else np.exp(1j * sign * k * r) / (4 * np.pi * r)
+
+
+# ---------------------------------------------------------------- sin / cos
+def test_sincos_against_mpmath():
+    rng = np.random.default_rng(11)
+    xs = np.concatenate([rng.uniform(-120, 120, 3000), rng.uniform(-1, 1, 500),
+                         np.arange(-80, 81) * np.pi / 2 + rng.uniform(-1e-9, 1e-9, 161), [0.0, 1e-300, 5e-8]])
+    worst = 0.0
+    for x in xs:
+        s, c = O.sincos(x)
+        ms, mc = mpmath.sin(mpmath.mpf(x)), mpmath.cos(mpmath.mpf(x))
+        # absolute error in units of 2^-53 (the values lie in [-1, 1])
+        err = max(abs(float(mpmath.mpf(s) - ms)), abs(float(mpmath.mpf(c) - mc))) / 2.0**-53
+        worst = max(worst, err)
+    assert worst <= 2.0, worst
+
+
+def test_sincos_quadrants_exact_at_zero():
+    assert O.sincos(0.0) == (0.0, 1.0)
+    s, c = O.sincos(np.pi / 2)
+    assert abs(s - 1.0) < 2e-16 and abs(c) < 2e-16
+
+
+# ---------------------------------------------------------------- entries
+def G_exact(k, x, y, sign):
+    mpmath.mp.prec = 200
+    r = mpmath.sqrt(sum((mpmath.mpf(float(a)) - mpmath.mpf(float(b))) ** 2 for a, b in zip(x, y)))
+    v = mpmath.expjpi(sign * mpmath.mpf(k) * r / mpmath.pi) / (4 * mpmath.pi * r)
+    return complex(v)
+
+
+@pytest.mark.parametrize("sign", [1, -1])
+def test_entry_closed_form(sign):
+    """cis(sign k r)/(4 pi r) against a 200-bit evaluation at the same double
+    inputs; the bound is the rounding of r, k r (phase error ~ k r ulp) and the
+    final division."""
+    rng = np.random.default_rng(3)
+    for _ in range(400):
+        x, y = rng.standard_normal(3), rng.standard_normal(3)
+        k = rng.uniform(0.1, 60)
+        g = O.entry(k, sign, x, y)
+        ref = G_exact(k, x, y, sign)
+        kr = k * np.linalg.norm(x - y)
+        assert abs(g - ref) <= abs(ref) * 2.0**-52 * (3.0 + 2.0 * kr), (g, ref, kr)
+        assert abs(g - G(k, x, y, sign)) <= abs(ref) * 2.0**-52 * (6.0 + 4.0 * kr)
+    assert O.entry(2.0, 1, [0.5, 0.5, 0.5], [0.5, 0.5, 0.5]) == 0j
+
+
+def test_assemble_level_transposed_layout():
+    """blocks[i] == transpose(dense[fartest[i], owntrial[i]]) (test/compressors/test_trialassembly.jl:23-30)."""
+    rng = np.random.default_rng(17)
+    for nt, ns in ((160, 160), (96, 160), (160, 96)):
+        tp = rng.random((nt, 3))
+        sp = rng.random((ns, 3)) + np.array([10.0, 0.0, 0.0])
+        owntrial = [[ns - 4, ns - 3, ns - 2, ns - 1, ns], [2, 4, 6, 8]]
+        fartest = [[1, 3, 5], [nt - 1, nt]]
+        buf, ro = O.assemble_level(0.7, 1, tp, sp, owntrial, fartest, 10, transposed=True)
+        for i in range(2):
+            blk = buf[ro[i]:ro[i + 1], :]
+            ref = np.array([[G(0.7, tp[c - 1], sp[r - 1]) for c in fartest[i]] for r in owntrial[i]])
+            np.testing.assert_allclose(blk[:, :len(fartest[i])], ref, rtol=1e-14)
+            assert np.all(blk[:, len(fartest[i]):] == 0)
+    with pytest.raises(ValueError):
+        O.assemble_level(0.7, 1, tp, sp, [[1]], [list(range(1, 12))], 10)
+
+
+# ---------------------------------------------------------------- ACA
+def test_aca_sampled_stop_known_answer():
+    """test/compressors/test_batchedbottomup.jl:43-52."""
+    A = np.zeros((5, 3))
+    A[0, 0], A[1, 1], A[2, 2] = 1, 0.009, 0.001
+    assert O.aca_fullpiv(A, 3, 1e-2, sampled=False)[0] == 1
+    assert O.aca_fullpiv(A, 3, 1e-2, sampled=True)[0] == 2
+
+
+def test_aca_exact_rank_blocks():
+    """Exact-rank blocks give their rank and CUR relerr < 1e-9
+    (examples/batchedbottomup_compression_validation.jl:40-73)."""
+    rng = np.random.default_rng(1)
+    cg = lambda *s: rng.standard_normal(s) + 1j * rng.standard_normal(s)
+    for m, n, r in ((120, 25, 8), (64, 30, 17), (200, 12, 5), (90, 20, 20)):
+        A = cg(m, r) @ cg(r, n)
+        rank, rp, cp = O.aca_fullpiv(A, 30, 1e-12, sampled=True)
+        assert rank == r
+        rpz, cpz = rp - 1, cp - 1
+        recon = A[:, cpz] @ np.linalg.solve(A[np.ix_(rpz, cpz)], A[rpz, :])
+        assert relerr(recon, A) < 1e-9
+        assert len(set(rp)) == rank and len(set(cp)) == rank
+
+
+def test_aca_zero_block():
+    assert O.aca_fullpiv(np.zeros((4, 3)), 30, 1e-3)[0] == 0
+    assert O.aca_fullpiv(np.zeros((0, 3)), 30, 1e-3)[0] == 0
+
+
+def test_aca_residual_invariant():
+    """||X A[rp,:] - A|| / ||A|| * sqrt(n / (n - r)) <= 1.01 tol and r < min(m, n)
+    on decaying blocks (test/gpu/test_compresslevel.jl:127-189)."""
+    rng = np.random.default_rng(41)
+    cg = lambda *s: rng.standard_normal(s) + 1j * rng.standard_normal(s)
+    tol = 1e-3
+    for m, n in ((90, 30), (65, 18)):
+        U = np.linalg.qr(cg(m, n))[0][:, :n]
+        V = np.linalg.qr(cg(n, n))[0]
+        A = U @ np.diag(10.0 ** np.linspace(0, -7, n)) @ V.conj().T
+        rank, rp, cp = O.aca_fullpiv(A, 30, tol, sampled=True)
+        assert 0 < rank < min(m, n)
+        X, st = O.skeleton_solve(A[:, cp - 1], rp)
+        assert st == -1
+        recon = X @ A[rp - 1, :]
+        assert relerr(recon, A) * np.sqrt(n / (n - rank)) <= 1.01 * tol
+
+
+def brute_force_aca(A, maxrank, tol, sampled):
+    """Greedy full pivoting written with numpy (argmax over |R|^2 in column-major
+    order, exact residual Frobenius norms)."""
+    R = np.array(A, dtype=np.complex128)
+    m, n = R.shape
+    nA2 = np.sum(np.abs(R) ** 2)
+    rows, cols = [], []
+    while len(rows) < min(maxrank, m, n):
+        r = len(rows)
+        n2 = np.sum(np.abs(R) ** 2)
+        if n2 * (n / (n - r) if sampled else 1.0) <= tol * tol * nA2:
+            break
+        e = int(np.argmax((np.abs(R) ** 2).ravel(order="F")))
+        i, j = e % m, e // m
+        if R[i, j] == 0:
+            break
+        R = R - np.outer(R[:, j], R[i, :] / R[i, j])
+        R[i, :] = 0
+        R[:, j] = 0
+        rows.append(i + 1)
+        cols.append(j + 1)
+    return len(rows), rows, cols
+
+
+def test_aca_matches_brute_force_on_kernel_blocks():
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        m, n = rng.integers(2, 40), rng.integers(2, 30)
+        x = rng.random((m, 3))
+        y = rng.random((n, 3)) + np.array([2.0 + rng.random(), 0, 0])
+        A = np.array([[G(3.0, a, b) for b in y] for a in x])
+        for tol in (1e-2, 1e-4, 1e-8):
+            got = O.aca_fullpiv(A, 50, tol, sampled=True)
+            ref = brute_force_aca(A, 50, tol, True)
+            assert got[0] == ref[0] and list(got[1]) == ref[1] and list(got[2]) == ref[2]
+
+
+# ---------------------------------------------------------------- skeleton solve
+def test_skeleton_solve_closed_form():
+    """Z = transpose(L / L[rows, :]) at rtol 1e-11, ranks up to 80
+    (test/gpu/test_compresslevel.jl:65-125)."""
+    rng = np.random.default_rng(17)
+    for m, r in ((90, 8), (100, 35), (160, 80)):
+        rows = rng.permutation(m)[:r]
+        L = rng.standard_normal((m, r)) + 1j * rng.standard_normal((m, r))
+        L[rows, :] += r * np.eye(r)
+        X, st = O.skeleton_solve(L, rows + 1)
+        assert st == -1
+        ref = np.linalg.solve(L[rows, :].T, L.T).T
+        np.testing.assert_allclose(X, ref, rtol=1e-11, atol=1e-11)
+        np.testing.assert_allclose(X[rows, :], np.eye(r), atol=1e-12)
+
+
+def test_skeleton_solve_singular():
+    """[1 2; 1 2]: 2 pivots, rank 1 (test/utils/test_utils.jl:54-71); duplicated
+    rows are dependent (test/test_nestedbasis.jl:55-65)."""
+    L = np.array([[1, 2], [1, 2]], dtype=np.complex128)
+    _, st = O.skeleton_solve(L, [1, 2])
+    assert st == 1
+    rng = np.random.default_rng(3)
+    B = rng.random((8, 2)) + 0j
+    B[6, :] = B[2, :]
+    _, st = O.skeleton_solve(B, [3, 7])
+    assert st >= 0
+    _, st = O.skeleton_solve(np.array([[2, 1], [1, 3]], dtype=np.complex128), [1, 2])
+    assert st == -1
+
+
+# ---------------------------------------------------------------- coupling
+def test_coupling_known_layout():
+    """Blocks in test-CSR order, empty blocks kept (test/test_couplingmatrices.jl:12-39)."""
+    rng = np.random.default_rng(1)
+    tp, sp = rng.random((9, 3)), rng.random((7, 3)) + np.array([3.0, 0, 0])
+    farptr = np.array([1, 3, 3, 5]) - 1
+    farnodes = np.array([2, 4, 1, 4]) - 1
+    testpiv = [[1, 4], [2], [5, 9, 3]]
+    trialpiv = [[7, 1], [3], [2, 6], []]
+    csr = lambda L: (np.concatenate([[0], np.cumsum([len(v) for v in L])]),
+                     np.array([i - 1 for v in L for i in v], np.int64))
+    tpp, tpi = csr(testpiv)
+    spp, spi = csr(trialpiv)
+    off, blocks = O.coupling(1.3, 1, tp, sp, farptr, farnodes, tpp, tpi, spp, spi)
+    expect = [[[1, 4], [3]], [[1, 4], []], [[5, 9, 3], [7, 1]], [[5, 9, 3], []]]
+    assert list(np.diff(off)) == [2, 0, 6, 0]
+    for i, (rows, cols) in enumerate(expect):
+        blk = blocks[off[i]:off[i + 1]].reshape((len(rows), len(cols)), order="F")
+        ref = np.array([[G(1.3, tp[a - 1], sp[b - 1]) for b in cols] for a in rows]).reshape(len(rows), len(cols))
+        np.testing.assert_allclose(blk, ref, rtol=1e-14)
+
+
+# ---------------------------------------------------------------- matvec
+def dense_h2(p, ft, fs, H):
+    """The compressed operator as a dense matrix, assembled block by block from
+    the factors: near blocks exactly, every far pair as U_t S_ts V_s with U_t and
+    V_s expanded through the transfers."""
+    tt, st = p.tree.testtree, p.tree.trialtree
+
+    def expand(f, tree, side):
+        # basis of every active node in its own point order: leaf X (or X^T);
+        # inner: stack child bases times transfer blocks
+        B = {}
+        for lev in sorted(tree.levels().tolist(), reverse=True):
+            for t in tree.levelnodes(lev):
+                if not f.assigned[t]:
+                    continue
+                if tree.isleaf(t):
+                    B[t] = f.X[t]
+                else:
+                    parts, o = [], 0
+                    for c in tree.childrenof(c_ := t):
+                        rc = int(f.rank[c]) if f.assigned[c] else 0
+                        E = f.X[t][o:o + rc, :]
+                        o += rc
+                        parts.append(B[c] @ E if c in B else np.zeros((tree.nvalues(c), f.rank[t])))
+                    B[t] = np.vstack(parts)
+        return B
+
+    UB = expand(ft, tt, 0)
+    VB = expand(fs, st, 1)
+    A = np.zeros((tt.npoints, st.npoints), np.complex128)
+    off, blocks = H
+    tr, sr = ft.rank, fs.rank
+    for t in range(tt.nnodes):
+        for i in range(p.tf.nodetofarnodesptr[t], p.tf.nodetofarnodesptr[t + 1]):
+            s = p.tf.farnodes[i]
+            S = blocks[off[i]:off[i + 1]].reshape((tr[t], sr[s]), order="F")
+            A[np.ix_(tt.values(t), st.values(s))] += UB[t] @ S @ VB[s].T
+        for i in range(p.partition.nearptr[t], p.partition.nearptr[t + 1]):
+            s = p.partition.nearnodes[i]
+            for a in tt.values(t):
+                for b in st.values(s):
+                    A[a, b] += O.entry(p.k, p.sign, p.testpts[a], p.trialpts[b])
+    return A
+
+
+@pytest.fixture(scope="module")
+def grid():
+    p = grid_problem(1e-4)
+    ft, fs = oracle_assembly(p)
+    H, cp = oracle_operator(p, ft, fs)
+    return p, ft, fs, H, cp
+
+
+def test_partition_covers_every_entry_once(grid):
+    """test/admissibility/test_farinteractions.jl:71-80."""
+    p = grid[0]
+    tt, st = p.tree.testtree, p.tree.trialtree
+    cover = np.zeros((tt.npoints, st.npoints), np.int64)
+    for t in range(tt.nnodes):
+        for i in range(p.tf.nodetofarnodesptr[t], p.tf.nodetofarnodesptr[t + 1]):
+            cover[np.ix_(tt.values(t), st.values(p.tf.farnodes[i]))] += 1
+        for i in range(p.partition.nearptr[t], p.partition.nearptr[t + 1]):
+            cover[np.ix_(tt.values(t), st.values(p.partition.nearnodes[i]))] += 1
+    assert np.all(cover == 1)
+
+
+def test_matvec_is_the_compressed_operator(grid):
+    """The 5-stage product equals the dense assembly of U S V + near (all three ops)."""
+    p, ft, fs, H, cp = grid
+    A = dense_h2(p, ft, fs, cp)
+    rng = np.random.default_rng(2)
+    x = geometry.complex_gaussian(rng, (p.trialpts.shape[0], 2))
+    u = geometry.complex_gaussian(rng, (p.testpts.shape[0], 2))
+    assert relerr(H.matvec(x), A @ x) < 1e-13
+    assert relerr(H.matvec(u, op=1), A.T @ u) < 1e-13
+    assert relerr(H.matvec(u, op=2), A.conj().T @ u) < 1e-13
+    y0 = geometry.complex_gaussian(rng, (p.testpts.shape[0], 2))
+    al, be = 0.3 + 0.2j, -1.1
+    assert relerr(H.matvec(x, alpha=al, beta=be, y=y0), al * (A @ x) + be * y0) < 1e-13
+    rows = np.array([0, 5, 77, 1023])
+    yr = H.matvec(x, rows=rows)
+    assert relerr(yr[rows], (A @ x)[rows]) < 1e-13
+
+
+def test_matvec_far_plus_near(grid):
+    p, ft, fs, H, cp = grid
+    rng = np.random.default_rng(4)
+    x = geometry.complex_gaussian(rng, (p.trialpts.shape[0],))
+    far = H.matvec(x, near=False)
+    full = H.matvec(x)
+    rows = np.arange(p.testpts.shape[0])
+    dense = O.dense_rows(p.k, p.sign, p.testpts, p.trialpts, rows, x)
+    assert relerr(full, dense) < 5 * p.tol
+    assert relerr(far, full) > 0
+
+
+@pytest.mark.parametrize("tol", [1e-2, 1e-4, 1e-6])
+def test_matvec_vs_direct_summation(tol):
+    """norm(nca * x - y) / norm(y) < 5 tol (test/nca/test_petrovgalerkinnca.jl:102)."""
+    p = sphere_problem(4096, k=1.0, tol=tol)
+    ft, fs = oracle_assembly(p)
+    H, _ = oracle_operator(p, ft, fs)
+    rng = np.random.default_rng(2)
+    x = geometry.complex_gaussian(rng, (4096,))
+    rows = rng.choice(4096, 300, replace=False)
+    ref = O.dense_rows(p.k, p.sign, p.testpts, p.trialpts, rows, x)[:, 0]
+    assert relerr(H.matvec(x)[rows, 0], ref) < 5 * tol
+    ref_t = O.dense_rows(p.k, p.sign, p.trialpts, p.testpts, rows, x)[:, 0]  # symmetric kernel
+    assert relerr(H.matvec(x, op=1)[rows, 0], ref_t) < 5 * tol
