@@ -186,6 +186,13 @@ int32_t nca_h2_matvec(nca_h2* h2, int32_t op, int64_t nrhs, const double* x, int
 int32_t nca_h2_matvec_device(nca_h2* h2, int32_t op, int64_t nrhs, const void* x, int64_t ldx, void* y,
                              int64_t ldy, const double* alpha, const double* beta);
 
+/* Measurement aid, no counterpart in the reference: one product as
+ * nca_h2_matvec_device (nrhs <= 16) with HIP events recorded on the context
+ * stream between the stages; stage_ms[0..5] = gather + clear, leaf projection,
+ * upward transfer, coupling, downward transfer, leaf expansion + near field. */
+int32_t nca_h2_matvec_profile(nca_h2* h2, int32_t op, int64_t nrhs, const void* x, int64_t ldx, void* y,
+                              int64_t ldy, const double* alpha, const double* beta, double* stage_ms);
+
 typedef struct {
     int64_t coupling_blocks, coupling_entries;
     int64_t basis_entries_test, basis_entries_trial;

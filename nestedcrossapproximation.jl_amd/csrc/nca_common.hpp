@@ -76,9 +76,14 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    // Set-up copies: complete before returning, so the host source may be a
+    // temporary and later null-stream reads see the data.
     void upload(const T* h, size_t count, hipStream_t s) {
         if (count > n) alloc(count);
-        if (count) NCA_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+        if (count) {
+            NCA_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+            NCA_HIP(hipStreamSynchronize(s));
+        }
     }
     void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
     size_t bytes() const { return n * sizeof(T); }

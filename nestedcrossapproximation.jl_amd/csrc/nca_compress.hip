@@ -462,6 +462,13 @@ int32_t nca_stack_destroy(nca_stack* st) {
 
 namespace nca {
 
+// Device -> host on the context stream, complete on return.
+static void d2h(nca_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    NCA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    NCA_HIP(hipStreamSynchronize(ctx->stream));
+}
+
 // Device-side results of one ACA launch set.
 struct AcaResult {
     DevBuf<int32_t> ranks;
@@ -557,9 +564,9 @@ int32_t nca_aca_fullpiv(nca_ctx* ctx, const nca_stack* st, double tol, int32_t s
         AcaResult res;
         launch_aca(ctx, st, tol, sampled, res);
         if (st->nb == 0) return;
-        NCA_HIP(hipMemcpy(ranks, res.ranks.p, sizeof(int32_t) * st->nb, hipMemcpyDeviceToHost));
-        NCA_HIP(hipMemcpy(rowpivots, res.rowpiv.p, sizeof(int64_t) * st->nb * st->maxrank, hipMemcpyDeviceToHost));
-        NCA_HIP(hipMemcpy(colpivots, res.colpiv.p, sizeof(int64_t) * st->nb * st->maxrank, hipMemcpyDeviceToHost));
+        d2h(ctx, ranks, res.ranks.p, sizeof(int32_t) * st->nb);
+        d2h(ctx, rowpivots, res.rowpiv.p, sizeof(int64_t) * st->nb * st->maxrank);
+        d2h(ctx, colpivots, res.colpiv.p, sizeof(int64_t) * st->nb * st->maxrank);
     });
 }
 
@@ -592,8 +599,8 @@ int32_t nca_skeleton_solve(nca_ctx* ctx, const nca_stack* st, const int32_t* ran
         launch_skeleton(ctx, st, ranks, d_ranks, d_rp, d_cp, xoff, d_X, d_indep);
         if (nb == 0) return;
         std::vector<int64_t> indep(nb);
-        NCA_HIP(hipMemcpy(indep.data(), d_indep.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost));
-        if (xoff[nb] > 0) NCA_HIP(hipMemcpy(X, d_X.p, sizeof(double2) * xoff[nb], hipMemcpyDeviceToHost));
+        d2h(ctx, indep.data(), d_indep.p, sizeof(int64_t) * nb);
+        if (xoff[nb] > 0) d2h(ctx, X, d_X.p, sizeof(double2) * xoff[nb]);
         std::copy(indep.begin(), indep.end(), independent);
         singular_message(st, ranks, indep);
     });
@@ -611,7 +618,7 @@ int32_t nca_compress_level(nca_ctx* ctx, const nca_stack* st, double tol, int32_
         launch_aca(ctx, st, tol, sampled, res);
         xoffsets[0] = 0;
         if (nb == 0) return;
-        NCA_HIP(hipMemcpy(ranks, res.ranks.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
+        d2h(ctx, ranks, res.ranks.p, sizeof(int32_t) * nb);
         std::vector<int64_t> xoff(nb + 1, 0);
         for (int64_t b = 0; b < nb; ++b)
             xoff[b + 1] = xoff[b] + (st->rowoffsets[b + 1] - st->rowoffsets[b]) * (int64_t)ranks[b];
@@ -621,11 +628,11 @@ int32_t nca_compress_level(nca_ctx* ctx, const nca_stack* st, double tol, int32_
         DevBuf<double2> d_X;
         DevBuf<int64_t> d_indep;
         launch_skeleton(ctx, st, ranks, res.ranks, res.rowpiv, res.colpiv, xoff, d_X, d_indep);
-        NCA_HIP(hipMemcpy(rowpivots, res.rowpiv.p, sizeof(int64_t) * nb * st->maxrank, hipMemcpyDeviceToHost));
-        NCA_HIP(hipMemcpy(colpivots, res.colpiv.p, sizeof(int64_t) * nb * st->maxrank, hipMemcpyDeviceToHost));
+        d2h(ctx, rowpivots, res.rowpiv.p, sizeof(int64_t) * nb * st->maxrank);
+        d2h(ctx, colpivots, res.colpiv.p, sizeof(int64_t) * nb * st->maxrank);
         std::vector<int64_t> indep(nb);
-        NCA_HIP(hipMemcpy(indep.data(), d_indep.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost));
-        if (xoff[nb] > 0) NCA_HIP(hipMemcpy(X, d_X.p, sizeof(double2) * xoff[nb], hipMemcpyDeviceToHost));
+        d2h(ctx, indep.data(), d_indep.p, sizeof(int64_t) * nb);
+        if (xoff[nb] > 0) d2h(ctx, X, d_X.p, sizeof(double2) * xoff[nb]);
         std::copy(xoff.begin(), xoff.end(), xoffsets);
         std::copy(indep.begin(), indep.end(), independent);
         singular_message(st, ranks, indep);
@@ -719,7 +726,7 @@ int32_t nca_coupling_download(nca_ctx* ctx, const nca_coupling* cp, int64_t* off
         set_device(ctx);
         if (offsets) std::copy(cp->offsets.begin(), cp->offsets.end(), offsets);
         if (blocks && cp->nentries > 0)
-            NCA_HIP(hipMemcpy(blocks, cp->blocks.p, sizeof(double2) * cp->nentries, hipMemcpyDeviceToHost));
+            d2h(ctx, blocks, cp->blocks.p, sizeof(double2) * cp->nentries);
     });
 }
 

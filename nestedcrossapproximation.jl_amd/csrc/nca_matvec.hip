@@ -42,7 +42,7 @@ struct SideDev {
     DevBuf<int64_t> nearptr;
     DevBuf<int32_t> nearnodes;
     // host copies used during creation
-    std::vector<int32_t> h_lo, h_hi, h_cptr, h_perm;
+    std::vector<int32_t> h_lo, h_hi, h_cptr, h_perm, h_bnodes, h_leaves;
     int64_t basis_entries = 0, transfer_entries = 0;
 };
 
@@ -308,9 +308,18 @@ __global__ void __launch_bounds__(kWave)
 inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // ---------------------------------------------------------------------------
+// Optional stage timing: events recorded on the stream between the stages.
+struct StageEvents {
+    hipEvent_t ev[7];
+    bool on = false;
+    void mark(int i, hipStream_t st) {
+        if (on) NCA_HIP(hipEventRecord(ev[i], st));
+    }
+};
+
 template <int NR, int OP>
 void run_chunk(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, int64_t R, int64_t q0,
-               double2 alpha, double2 beta) {
+               double2 alpha, double2 beta, StageEvents* evs) {
     constexpr bool FWD = OP == NCA_OP_N;
     constexpr bool CJ = OP == NCA_OP_C;
     hipStream_t st = h->ctx->stream;
@@ -319,9 +328,11 @@ void run_chunk(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy
     double2* xs = h->ws_xs.p;
     double2* xh = h->ws_xh.p;
     double2* yh = h->ws_yh.p;
+    if (evs) evs->mark(1, st);
     if (src.nbasis > 0)
         hipLaunchKernelGGL((k_project<NR, !FWD, CJ>), dim3((unsigned)src.nbasis), dim3(kWave), 0, st, src.bnodes.p,
                            src.boff.p, src.bblocks.p, src.lo.p, src.hi.p, src.cptr.p, xs, R, q0, xh);
+    if (evs) evs->mark(2, st);
     // upward: deepest level first; source transfers (trial E forward, test E transposed)
     for (int64_t l = src.nlev - 1; l >= 0; --l) {
         const int64_t np = src.levptr[l + 1] - src.levptr[l];
@@ -329,6 +340,7 @@ void run_chunk(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy
         hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ>), dim3((unsigned)np), dim3(kWave), 0, st, src.parents.p,
                            src.chdptr.p, src.chds.p, src.toff.p, src.tblocks.p, src.cptr.p, src.levptr[l], R, q0, xh);
     }
+    if (evs) evs->mark(3, st);
     const double2* S = h->coupling->blocks.p;
     if (FWD) {
         if (h->coupling->nentries > 0)
@@ -339,6 +351,7 @@ void run_chunk(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy
             hipLaunchKernelGGL((k_couple_rev<NR, CJ>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->r_ptr.p,
                                h->r_test.p, h->r_off.p, src.cptr.p, dst.cptr.p, xh, R, q0, yh);
     }
+    if (evs) evs->mark(4, st);
     // downward: root level first; destination transfers (test E forward, trial E transposed)
     for (int64_t l = 0; l < dst.nlev; ++l) {
         const int64_t np = dst.levptr[l + 1] - dst.levptr[l];
@@ -346,21 +359,24 @@ void run_chunk(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy
         hipLaunchKernelGGL((k_transfer<NR, true, !FWD, CJ>), dim3((unsigned)np), dim3(kWave), 0, st, dst.parents.p,
                            dst.chdptr.p, dst.chds.p, dst.toff.p, dst.tblocks.p, dst.cptr.p, dst.levptr[l], R, q0, yh);
     }
+    if (evs) evs->mark(5, st);
     if (dst.nleaves > 0)
         hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st, dst.leaves.p,
                            dst.leafbasis.p, dst.boff.p, dst.bblocks.p, dst.lo.p, dst.hi.p, dst.perm.p, dst.cptr.p,
                            dst.pts.p, dst.nearptr.p, dst.nearnodes.p, src.lo.p, src.hi.p, src.pts.p, yh, xs, R, q0,
                            h->k, h->sign, 1, alpha, beta, y, ldy);
+    if (evs) evs->mark(6, st);
     NCA_HIP(hipGetLastError());
 }
 
 template <int OP>
 void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, int64_t R, double2 alpha,
-            double2 beta) {
+            double2 beta, StageEvents* evs) {
     constexpr bool FWD = OP == NCA_OP_N;
     hipStream_t st = h->ctx->stream;
     SideDev& src = FWD ? h->trial : h->test;
     SideDev& dst = FWD ? h->test : h->trial;
+    if (evs) evs->mark(0, st);
     if (src.npts * R > 0)
         hipLaunchKernelGGL(k_gather, dim3(blocks_for(src.npts * R, 256)), dim3(256), 0, st, x, ldx, src.perm.p,
                            src.npts, R, h->ws_xs.p);
@@ -371,11 +387,11 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     int64_t q0 = 0;
     while (q0 < R) {
         const int64_t left = R - q0;
-        if (left >= 16) { run_chunk<16, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta); q0 += 16; }
-        else if (left >= 8) { run_chunk<8, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta); q0 += 8; }
-        else if (left >= 4) { run_chunk<4, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta); q0 += 4; }
-        else if (left >= 2) { run_chunk<2, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta); q0 += 2; }
-        else { run_chunk<1, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta); q0 += 1; }
+        if (left >= 16) { run_chunk<16, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta, evs); q0 += 16; }
+        else if (left >= 8) { run_chunk<8, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta, evs); q0 += 8; }
+        else if (left >= 4) { run_chunk<4, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta, evs); q0 += 4; }
+        else if (left >= 2) { run_chunk<2, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta, evs); q0 += 2; }
+        else { run_chunk<1, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta, evs); q0 += 1; }
     }
 }
 
@@ -389,7 +405,7 @@ void ensure_workspace(nca_h2* h, int64_t R) {
 }
 
 void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t ldx, double2* y, int64_t ldy,
-                   const double* alpha, const double* beta) {
+                   const double* alpha, const double* beta, StageEvents* evs = nullptr) {
     NCA_REQUIRE(op >= NCA_OP_N && op <= NCA_OP_C, NCA_ERR_ARGUMENT, "nca_h2_matvec: op must be 0 (N), 1 (T) or 2 (C)");
     NCA_REQUIRE(R >= 0, NCA_ERR_ARGUMENT, "nca_h2_matvec: negative nrhs");
     const int64_t nx = op == NCA_OP_N ? h->N : h->M, ny = op == NCA_OP_N ? h->M : h->N;
@@ -398,9 +414,9 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
     ensure_workspace(h, R);
     const double2 al = make_double2(alpha ? alpha[0] : 1.0, alpha ? alpha[1] : 0.0);
     const double2 be = make_double2(beta ? beta[0] : 0.0, beta ? beta[1] : 0.0);
-    if (op == NCA_OP_N) run_op<NCA_OP_N>(h, x, ldx, y, ldy, R, al, be);
-    else if (op == NCA_OP_T) run_op<NCA_OP_T>(h, x, ldx, y, ldy, R, al, be);
-    else run_op<NCA_OP_C>(h, x, ldx, y, ldy, R, al, be);
+    if (op == NCA_OP_N) run_op<NCA_OP_N>(h, x, ldx, y, ldy, R, al, be, evs);
+    else if (op == NCA_OP_T) run_op<NCA_OP_T>(h, x, ldx, y, ldy, R, al, be, evs);
+    else run_op<NCA_OP_C>(h, x, ldx, y, ldy, R, al, be, evs);
 }
 
 // ---------------------------------------------------------------------------
@@ -458,6 +474,7 @@ void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, const int
     }
     for (int64_t i = 0; i <= nbasis; ++i) bo[i] = nbasis ? boff[i] - boff[0] : 0;
     sd.basis_entries = nbasis ? bo[nbasis] : 0;
+    sd.h_bnodes = bn;
     sd.bnodes.upload(bn, st);
     sd.boff.upload(bo, st);
     if (nbasis) sd.bblocks.upload(reinterpret_cast<const double2*>(bblocks) + boff[0], (size_t)bo[nbasis], st);
@@ -539,6 +556,7 @@ void build_leaves(SideDev& sd, hipStream_t st) {
         NCA_REQUIRE(run == 1, NCA_ERR_ARGUMENT, "nca_h2_create: the leaves do not partition the points");
     }
     sd.nleaves = (int64_t)leaves.size();
+    sd.h_leaves = leaves;
     sd.leaves.upload(leaves, st);
 }
 
@@ -576,11 +594,9 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
         // basis slot of every leaf
         for (SideDev* sd : {&h->test, &h->trial}) {
             std::vector<int32_t> slot(sd->nn, -1), lb;
-            std::vector<int32_t> bn(sd->nbasis);
-            if (sd->nbasis) NCA_HIP(hipMemcpy(bn.data(), sd->bnodes.p, sizeof(int32_t) * sd->nbasis, hipMemcpyDeviceToHost));
+            const std::vector<int32_t>& bn = sd->h_bnodes;
             for (int64_t i = 0; i < sd->nbasis; ++i) slot[bn[i]] = (int32_t)i;
-            std::vector<int32_t> lv(sd->nleaves);
-            if (sd->nleaves) NCA_HIP(hipMemcpy(lv.data(), sd->leaves.p, sizeof(int32_t) * sd->nleaves, hipMemcpyDeviceToHost));
+            const std::vector<int32_t>& lv = sd->h_leaves;
             std::vector<char> isleaf(sd->nn, 0);
             for (int32_t l : lv) {
                 lb.push_back(slot[l]);
@@ -713,6 +729,33 @@ int32_t nca_h2_matvec_device(nca_h2* h2, int32_t op, int64_t nrhs, const void* x
         NCA_REQUIRE(h2 && (nrhs == 0 || (x && y)), NCA_ERR_ARGUMENT, "nca_h2_matvec_device: invalid argument");
         set_device(h2->ctx);
         matvec_device(h2, op, nrhs, static_cast<const double2*>(x), ldx, static_cast<double2*>(y), ldy, alpha, beta);
+    });
+}
+
+int32_t nca_h2_matvec_profile(nca_h2* h2, int32_t op, int64_t nrhs, const void* x, int64_t ldx, void* y,
+                              int64_t ldy, const double* alpha, const double* beta, double* stage_ms) {
+    return guarded([&] {
+        NCA_REQUIRE(h2 && stage_ms && (nrhs == 0 || (x && y)), NCA_ERR_ARGUMENT,
+                    "nca_h2_matvec_profile: invalid argument");
+        NCA_REQUIRE(nrhs <= 16, NCA_ERR_ARGUMENT, "nca_h2_matvec_profile: at most 16 right-hand sides");
+        set_device(h2->ctx);
+        StageEvents evs;
+        for (auto& e : evs.ev) NCA_HIP(hipEventCreate(&e));
+        evs.on = true;
+        try {
+            matvec_device(h2, op, nrhs, static_cast<const double2*>(x), ldx, static_cast<double2*>(y), ldy, alpha,
+                          beta, &evs);
+            NCA_HIP(hipEventSynchronize(evs.ev[6]));
+            for (int i = 0; i < 6; ++i) {
+                float ms = 0.f;
+                NCA_HIP(hipEventElapsedTime(&ms, evs.ev[i], evs.ev[i + 1]));
+                stage_ms[i] = ms;
+            }
+        } catch (...) {
+            for (auto& e : evs.ev) (void)hipEventDestroy(e);
+            throw;
+        }
+        for (auto& e : evs.ev) (void)hipEventDestroy(e);
     });
 }
 

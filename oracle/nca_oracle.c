@@ -510,17 +510,18 @@ void orc_h2_matvec(const orc_h2* h, int32_t op, int64_t nrhs, const double* xin,
             int64_t s = nodes[i] - 1;
             int64_t r = src_cptr[s + 1] - src_cptr[s];
             int64_t ns = src_hi[s] - src_lo[s] + 1;
-            for (int64_t q = 0; q < nrhs; ++q)
-                for (int64_t a = 0; a < r; ++a) {
-                    cplx sum = {0.0, 0.0};
-                    for (int64_t j = 0; j < ns; ++j) {
-                        /* forward: V (r x ns) as is; transposed: U (ns x r) transposed */
-                        cplx b = ld_c(blocks, off[i] + (fwd ? a + j * r : j + a * ns), cj);
-                        cplx xv = X[(src_perm[src_lo[s] - 1 + j] - 1) + q * ldx];
-                        sum = cadd(sum, cmul(b, xv));
-                    }
-                    xh[(src_cptr[s] - 1 + a) + q * Ks] = sum;
+            cplx* sum = (cplx*)calloc((size_t)nrhs, sizeof(cplx));
+            for (int64_t a = 0; a < r; ++a) {
+                for (int64_t q = 0; q < nrhs; ++q) sum[q].re = sum[q].im = 0.0;
+                for (int64_t j = 0; j < ns; ++j) {
+                    /* forward: V (r x ns) as is; transposed: U (ns x r) transposed */
+                    cplx b = ld_c(blocks, off[i] + (fwd ? a + j * r : j + a * ns), cj);
+                    int64_t col = src_perm[src_lo[s] - 1 + j] - 1;
+                    for (int64_t q = 0; q < nrhs; ++q) sum[q] = cadd(sum[q], cmul(b, X[col + q * ldx]));
                 }
+                for (int64_t q = 0; q < nrhs; ++q) xh[(src_cptr[s] - 1 + a) + q * Ks] = sum[q];
+            }
+            free(sum);
         }
     }
     /* Stage 2: upward transfer, deepest level first (aggregate.jl:44-67). */
@@ -540,16 +541,15 @@ void orc_h2_matvec(const orc_h2* h, int32_t op, int64_t nrhs, const double* xin,
                 for (int64_t e = chdptr[idx] - 1; e < chdptr[idx + 1] - 1; ++e) {
                     int64_t c = chds[e] - 1;
                     int64_t rc = src_cptr[c + 1] - src_cptr[c];
-                    for (int64_t q = 0; q < nrhs; ++q)
-                        for (int64_t a = 0; a < rp; ++a) {
-                            cplx sum = xh[(src_cptr[p] - 1 + a) + q * Ks];
-                            for (int64_t j = 0; j < rc; ++j) {
-                                /* forward: trial E (rp x rc); transposed: test E (rc x rp) transposed */
-                                cplx b = ld_c(blocks, off[e] + (fwd ? a + j * rp : j + a * rc), cj);
-                                sum = cadd(sum, cmul(b, xh[(src_cptr[c] - 1 + j) + q * Ks]));
-                            }
-                            xh[(src_cptr[p] - 1 + a) + q * Ks] = sum;
+                    for (int64_t a = 0; a < rp; ++a) {
+                        cplx* out = xh + (src_cptr[p] - 1 + a);
+                        for (int64_t j = 0; j < rc; ++j) {
+                            /* forward: trial E (rp x rc); transposed: test E (rc x rp) transposed */
+                            cplx b = ld_c(blocks, off[e] + (fwd ? a + j * rp : j + a * rc), cj);
+                            const cplx* in = xh + (src_cptr[c] - 1 + j);
+                            for (int64_t q = 0; q < nrhs; ++q) out[q * Ks] = cadd(out[q * Ks], cmul(b, in[q * Ks]));
                         }
+                    }
                 }
             }
         }
@@ -562,14 +562,14 @@ void orc_h2_matvec(const orc_h2* h, int32_t op, int64_t nrhs, const double* xin,
             for (int64_t i = h->cp_ptr[t] - 1; i < h->cp_ptr[t + 1] - 1; ++i) {
                 int64_t s = h->cp_nodes[i] - 1;
                 int64_t rs = h->trial_cptr[s + 1] - h->trial_cptr[s];
-                for (int64_t q = 0; q < nrhs; ++q)
-                    for (int64_t a = 0; a < rt; ++a) {
-                        cplx sum = yh[(h->test_cptr[t] - 1 + a) + q * Kd];
-                        for (int64_t j = 0; j < rs; ++j)
-                            sum = cadd(sum, cmul(ld_c(h->cp_blocks, h->cp_off[i] + a + j * rt, cj),
-                                                 xh[(h->trial_cptr[s] - 1 + j) + q * Ks]));
-                        yh[(h->test_cptr[t] - 1 + a) + q * Kd] = sum;
+                for (int64_t a = 0; a < rt; ++a) {
+                    cplx* out = yh + (h->test_cptr[t] - 1 + a);
+                    for (int64_t j = 0; j < rs; ++j) {
+                        cplx b = ld_c(h->cp_blocks, h->cp_off[i] + a + j * rt, cj);
+                        const cplx* in = xh + (h->trial_cptr[s] - 1 + j);
+                        for (int64_t q = 0; q < nrhs; ++q) out[q * Kd] = cadd(out[q * Kd], cmul(b, in[q * Ks]));
                     }
+                }
             }
         }
     } else {
@@ -578,14 +578,14 @@ void orc_h2_matvec(const orc_h2* h, int32_t op, int64_t nrhs, const double* xin,
             for (int64_t i = h->cp_ptr[t] - 1; i < h->cp_ptr[t + 1] - 1; ++i) {
                 int64_t s = h->cp_nodes[i] - 1;
                 int64_t rs = h->trial_cptr[s + 1] - h->trial_cptr[s];
-                for (int64_t q = 0; q < nrhs; ++q)
-                    for (int64_t a = 0; a < rs; ++a) {
-                        cplx sum = yh[(h->trial_cptr[s] - 1 + a) + q * Kd];
-                        for (int64_t j = 0; j < rt; ++j)
-                            sum = cadd(sum, cmul(ld_c(h->cp_blocks, h->cp_off[i] + j + a * rt, cj),
-                                                 xh[(h->test_cptr[t] - 1 + j) + q * Ks]));
-                        yh[(h->trial_cptr[s] - 1 + a) + q * Kd] = sum;
+                for (int64_t a = 0; a < rs; ++a) {
+                    cplx* out = yh + (h->trial_cptr[s] - 1 + a);
+                    for (int64_t j = 0; j < rt; ++j) {
+                        cplx b = ld_c(h->cp_blocks, h->cp_off[i] + j + a * rt, cj);
+                        const cplx* in = xh + (h->test_cptr[t] - 1 + j);
+                        for (int64_t q = 0; q < nrhs; ++q) out[q * Kd] = cadd(out[q * Kd], cmul(b, in[q * Ks]));
                     }
+                }
             }
         }
     }
@@ -606,16 +606,15 @@ void orc_h2_matvec(const orc_h2* h, int32_t op, int64_t nrhs, const double* xin,
                 for (int64_t e = chdptr[idx] - 1; e < chdptr[idx + 1] - 1; ++e) {
                     int64_t c = chds[e] - 1;
                     int64_t rc = dst_cptr[c + 1] - dst_cptr[c];
-                    for (int64_t q = 0; q < nrhs; ++q)
-                        for (int64_t a = 0; a < rc; ++a) {
-                            cplx sum = yh[(dst_cptr[c] - 1 + a) + q * Kd];
-                            for (int64_t j = 0; j < rp; ++j) {
-                                /* forward: test E (rc x rp); transposed: trial E (rp x rc) transposed */
-                                cplx b = ld_c(blocks, off[e] + (fwd ? a + j * rc : j + a * rp), cj);
-                                sum = cadd(sum, cmul(b, yh[(dst_cptr[p] - 1 + j) + q * Kd]));
-                            }
-                            yh[(dst_cptr[c] - 1 + a) + q * Kd] = sum;
+                    for (int64_t a = 0; a < rc; ++a) {
+                        cplx* out = yh + (dst_cptr[c] - 1 + a);
+                        for (int64_t j = 0; j < rp; ++j) {
+                            /* forward: test E (rc x rp); transposed: trial E (rp x rc) transposed */
+                            cplx b = ld_c(blocks, off[e] + (fwd ? a + j * rc : j + a * rp), cj);
+                            const cplx* in = yh + (dst_cptr[p] - 1 + j);
+                            for (int64_t q = 0; q < nrhs; ++q) out[q * Kd] = cadd(out[q * Kd], cmul(b, in[q * Kd]));
                         }
+                    }
                 }
             }
         }
@@ -631,18 +630,18 @@ void orc_h2_matvec(const orc_h2* h, int32_t op, int64_t nrhs, const double* xin,
             int64_t t = nodes[i] - 1;
             int64_t r = dst_cptr[t + 1] - dst_cptr[t];
             int64_t nt = dst_hi[t] - dst_lo[t] + 1;
-            for (int64_t q = 0; q < nrhs; ++q)
-                for (int64_t a = 0; a < nt; ++a) {
-                    int64_t row = dst_perm[dst_lo[t] - 1 + a] - 1;
-                    if (want && !want[row]) continue;
-                    cplx sum = acc[row + q * ndst_pts];
-                    for (int64_t j = 0; j < r; ++j) {
-                        /* forward: U (nt x r) as is; transposed: V (r x nt) transposed */
-                        cplx b = ld_c(blocks, off[i] + (fwd ? a + j * nt : j + a * r), cj);
-                        sum = cadd(sum, cmul(b, yh[(dst_cptr[t] - 1 + j) + q * Kd]));
-                    }
-                    acc[row + q * ndst_pts] = sum;
+            for (int64_t a = 0; a < nt; ++a) {
+                int64_t row = dst_perm[dst_lo[t] - 1 + a] - 1;
+                if (want && !want[row]) continue;
+                cplx* out = acc + row;
+                for (int64_t j = 0; j < r; ++j) {
+                    /* forward: U (nt x r) as is; transposed: V (r x nt) transposed */
+                    cplx b = ld_c(blocks, off[i] + (fwd ? a + j * nt : j + a * r), cj);
+                    const cplx* in = yh + (dst_cptr[t] - 1 + j);
+                    for (int64_t q = 0; q < nrhs; ++q)
+                        out[q * ndst_pts] = cadd(out[q * ndst_pts], cmul(b, in[q * Kd]));
                 }
+            }
         }
     }
     /* Near field (petrovgalerkinnca.jl:474,531,587; blocks of abstractnca.jl:134-178). */
@@ -653,17 +652,15 @@ void orc_h2_matvec(const orc_h2* h, int32_t op, int64_t nrhs, const double* xin,
                 for (int64_t a = h->test_lo[t] - 1; a < h->test_hi[t] && h->nearptr[t + 1] > h->nearptr[t]; ++a) {
                     int64_t row = h->test_perm[a] - 1;
                     if (want && !want[row]) continue;
-                    for (int64_t q = 0; q < nrhs; ++q) {
-                        cplx sum = acc[row + q * ndst_pts];
-                        for (int64_t i = h->nearptr[t] - 1; i < h->nearptr[t + 1] - 1; ++i) {
-                            int64_t s = h->nearnodes[i] - 1;
-                            for (int64_t b = h->trial_lo[s] - 1; b < h->trial_hi[s]; ++b) {
-                                int64_t col = h->trial_perm[b] - 1;
-                                cplx g = entry(h->k, h->sign, h->testpts + 3 * row, h->trialpts + 3 * col);
-                                sum = cadd(sum, cmul(g, X[col + q * ldx]));
-                            }
+                    cplx* out = acc + row;
+                    for (int64_t i = h->nearptr[t] - 1; i < h->nearptr[t + 1] - 1; ++i) {
+                        int64_t s = h->nearnodes[i] - 1;
+                        for (int64_t b = h->trial_lo[s] - 1; b < h->trial_hi[s]; ++b) {
+                            int64_t col = h->trial_perm[b] - 1;
+                            cplx g = entry(h->k, h->sign, h->testpts + 3 * row, h->trialpts + 3 * col);
+                            for (int64_t q = 0; q < nrhs; ++q)
+                                out[q * ndst_pts] = cadd(out[q * ndst_pts], cmul(g, X[col + q * ldx]));
                         }
-                        acc[row + q * ndst_pts] = sum;
                     }
                 }
             }
@@ -684,18 +681,16 @@ void orc_h2_matvec(const orc_h2* h, int32_t op, int64_t nrhs, const double* xin,
                 for (int64_t b = h->trial_lo[s] - 1; b < h->trial_hi[s] && tptr[s + 1] > tptr[s]; ++b) {
                     int64_t row = h->trial_perm[b] - 1;
                     if (want && !want[row]) continue;
-                    for (int64_t q = 0; q < nrhs; ++q) {
-                        cplx sum = acc[row + q * ndst_pts];
-                        for (int64_t i = tptr[s]; i < tptr[s + 1]; ++i) {
-                            int64_t t = tidx[i];
-                            for (int64_t a = h->test_lo[t] - 1; a < h->test_hi[t]; ++a) {
-                                int64_t col = h->test_perm[a] - 1;
-                                cplx g = entry(h->k, h->sign, h->testpts + 3 * col, h->trialpts + 3 * row);
-                                if (cj) g = cconj(g);
-                                sum = cadd(sum, cmul(g, X[col + q * ldx]));
-                            }
+                    cplx* out = acc + row;
+                    for (int64_t i = tptr[s]; i < tptr[s + 1]; ++i) {
+                        int64_t t = tidx[i];
+                        for (int64_t a = h->test_lo[t] - 1; a < h->test_hi[t]; ++a) {
+                            int64_t col = h->test_perm[a] - 1;
+                            cplx g = entry(h->k, h->sign, h->testpts + 3 * col, h->trialpts + 3 * row);
+                            if (cj) g = cconj(g);
+                            for (int64_t q = 0; q < nrhs; ++q)
+                                out[q * ndst_pts] = cadd(out[q * ndst_pts], cmul(g, X[col + q * ldx]));
                         }
-                        acc[row + q * ndst_pts] = sum;
                     }
                 }
             }

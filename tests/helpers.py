@@ -76,3 +76,12 @@ def oracle_operator(p, ft, fs):
 def relerr(a, b):
     nb = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def bitwise_equal(a, b):
+    """Same shape and the same IEEE bit patterns in every entry."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    if a.shape != b.shape or a.dtype != b.dtype:
+        return False
+    return np.array_equal(a.view(np.uint64), b.view(np.uint64))

@@ -12,7 +12,7 @@ from nca_amd import compressors as C
 from nca_amd import geometry
 from nca_amd import nca as NC
 from oracle import oracle as O
-from helpers import grid_problem, sphere_problem, oracle_assembly, oracle_operator, relerr
+from helpers import grid_problem, sphere_problem, oracle_assembly, oracle_operator, relerr, bitwise_equal
 
 pytestmark = pytest.mark.gpu
 
@@ -64,7 +64,7 @@ def test_level_entries_bitwise(ctx, transposed, sign):
     st = C.assemble_level(A, rp, ri, cp, ci, 12, transposed)
     got = st.download()
     assert got.shape == ref.shape
-    assert np.array_equal(got.view(np.float64), ref.view(np.float64))
+    assert bitwise_equal(got, ref)
 
 
 def test_level_assemble_rejects_wide_block(ctx):
@@ -174,7 +174,7 @@ def test_skeleton_solve_closed_form_and_bitwise(ctx):
         np.testing.assert_allclose(Xi, ref, rtol=1e-11, atol=1e-11)
         Xo, sto = O.skeleton_solve(L, rows + 1)
         assert sto == -1
-        assert np.array_equal(Xi.view(np.float64), Xo.view(np.float64))
+        assert bitwise_equal(Xi, Xo)
 
 
 def test_skeleton_solve_singular(ctx):
@@ -205,7 +205,7 @@ def compare_sides(fg, fo, nn):
             continue
         assert fg.rank[t] == fo.rank[t], t
         assert np.array_equal(fg.pivots[t], fo.pivots[t]), t
-        assert np.array_equal(fg.X[t].view(np.float64), fo.X[t].view(np.float64)), t
+        assert bitwise_equal(fg.X[t], fo.X[t]), t
 
 
 @pytest.fixture(scope="module")
@@ -242,7 +242,7 @@ def test_coupling_bitwise(case, request):
     ooff, oblocks = O.coupling(p.k, p.sign, p.testpts, p.trialpts, p.tf.nodetofarnodesptr, p.tf.farnodes,
                                tp[0], tp[1], sp[0], sp[1])
     assert np.array_equal(off, ooff)
-    assert np.array_equal(blocks.view(np.float64), oblocks.view(np.float64))
+    assert bitwise_equal(blocks, oblocks)
 
 
 def build_operator(ctx, p, A, ft, fs):
