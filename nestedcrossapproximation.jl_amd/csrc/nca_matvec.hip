@@ -41,6 +41,12 @@ struct SideDev {
     // near lists by node of this side (as destination): ptr (nn+1), nodes of the other side
     DevBuf<int64_t> nearptr;
     DevBuf<int32_t> nearnodes;
+    // the same near lists as merged contiguous point ranges of the other side
+    DevBuf<int64_t> nrptr;
+    DevBuf<int32_t> nrlo, nrhi;
+    // 16-row tiles of the leaves (leaf index, first row) for the MFMA expansion kernel
+    int64_t ntiles = 0;
+    DevBuf<int32_t> tile_leaf, tile_row0;
     // host copies used during creation
     std::vector<int32_t> h_lo, h_hi, h_cptr, h_perm, h_bnodes, h_leaves;
     int64_t basis_entries = 0, transfer_entries = 0;
@@ -59,6 +65,9 @@ struct nca_h2 {
     // forward coupling: per test node entry base and column base, per column the source coefficient
     nca::DevBuf<int64_t> f_base, f_colbase;
     nca::DevBuf<int32_t> f_ncols, f_colcoef;
+    // 16-row tiles of the forward coupling (test node, first row)
+    int64_t f_ntiles = 0;
+    nca::DevBuf<int32_t> f_tile_node, f_tile_row0;
     // reverse coupling: per trial node list of (test node, block offset)
     nca::DevBuf<int64_t> r_ptr, r_off;
     nca::DevBuf<int32_t> r_test;
@@ -305,6 +314,180 @@ __global__ void __launch_bounds__(kWave)
     }
 }
 
+// ---------------------------------------------------------------------------
+// MFMA kernels (16 right-hand sides): a 16 x 4 operand tile (rows x sources or
+// columns) times a 4 x 16 tile (sources x right-hand sides) with
+// v_mfma_f64_16x16x4f64. Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]
+// and holds D[l / 16 + 4 v][l % 16], v = 0..3 (the gfx950 layout of this
+// instruction, checked on the device). A complex product takes four
+// real products accumulated separately: Re = P1 - P2, Im = P3 + P4.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+struct Cacc {
+    dbl4 p1, p2, p3, p4;
+};
+
+__device__ __forceinline__ void cmfma(double2 a, double2 b, Cacc& c) {
+    c.p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, c.p1, 0, 0, 0);
+    c.p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, c.p2, 0, 0, 0);
+    c.p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, c.p3, 0, 0, 0);
+    c.p4 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, c.p4, 0, 0, 0);
+}
+
+__device__ __forceinline__ void czero(Cacc& c) {
+    c.p1 = c.p2 = c.p3 = c.p4 = dbl4{0.0, 0.0, 0.0, 0.0};
+}
+
+constexpr double kInvFourPi = 0x1.45f306dc9c883p-4;
+
+// G(x, y) for the matvec: 1/r from the hardware reciprocal square root refined
+// by two Newton steps, r = r^2 / r, the same sin/cos as the assembly.
+__device__ __forceinline__ double2 helmholtz_mv(double k, int sign, double x0, double x1, double x2, double y0,
+                                                double y1, double y2) {
+    const double dx = x0 - y0, dy = x1 - y1, dz = x2 - y2;
+    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+    if (r2 == 0.0) return make_double2(0.0, 0.0);
+    double ri = __builtin_amdgcn_rsq(r2);
+    const double h = 0.5 * r2;
+    ri = ri * fma(-h * ri, ri, 1.5);
+    ri = ri * fma(-h * ri, ri, 1.5);
+    const double r = r2 * ri;
+    double s, c;
+    sincos_cw(k * r, s, c);
+    const double sc = ri * kInvFourPi;
+    return make_double2(c * sc, (sign < 0 ? -s : s) * sc);
+}
+
+// Stage 5 + near field for 16 right-hand sides: one wave per 16-row tile of a
+// destination leaf. The basis rows enter as the first "sources" (A = U_t or
+// V_t^T, B = yh_t); then the near-field sources, walked over merged contiguous
+// point ranges four at a time (A = G evaluated by the lane, B = xs).
+template <bool FWD, bool CJ>
+__global__ void __launch_bounds__(256)
+    k_leaf_out_mfma(const int32_t* __restrict__ tile_leaf, const int32_t* __restrict__ tile_row0, int64_t ntiles,
+                    const int32_t* __restrict__ leaves, const int32_t* __restrict__ leafbasis,
+                    const int64_t* __restrict__ boff, const double2* __restrict__ bblocks,
+                    const int32_t* __restrict__ dlo, const int32_t* __restrict__ dhi,
+                    const int32_t* __restrict__ dperm, const int32_t* __restrict__ dcptr,
+                    const double* __restrict__ dpts, const int64_t* __restrict__ nrptr,
+                    const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
+                    const double* __restrict__ spts, const double2* __restrict__ yh, const double2* __restrict__ xs,
+                    int64_t R, int64_t q0, double kw, int sign, double2 alpha, double2 beta,
+                    double2* __restrict__ y, int64_t ldy) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= ntiles) return;
+    const int li = tile_leaf[w], row0 = tile_row0[w];
+    const int t = leaves[li], bi = leafbasis[li];
+    const int l0 = dlo[t], nt = dhi[t] - l0;
+    const int nrows = min(16, nt - row0);
+    const int i = lane & 15, kk = lane >> 4, j = lane & 15;
+    const bool rowok = i < nrows;
+    Cacc acc;
+    czero(acc);
+    if (bi >= 0) {
+        const int rt = dcptr[t + 1] - dcptr[t];
+        const double2* B = bblocks + boff[bi];
+        for (int k0 = 0; k0 < rt; k0 += 4) {
+            const int k = k0 + kk;
+            double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+            if (k < rt) {
+                if (rowok)
+                    a = ldc<CJ>(B, FWD ? (int64_t)(row0 + i) + (int64_t)k * nt : (int64_t)k + (int64_t)(row0 + i) * rt);
+                b = yh[(int64_t)(dcptr[t] + k) * R + q0 + j];
+            }
+            cmfma(a, b, acc);
+        }
+    }
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+    if (rowok) {
+        const double* xp = dpts + 3 * (int64_t)(l0 + row0 + i);
+        x0 = xp[0];
+        x1 = xp[1];
+        x2 = xp[2];
+    }
+    int64_t e = nrptr[t];
+    const int64_t eend = nrptr[t + 1];
+    int pos = e < eend ? nrlo[e] : 0;
+    while (e < eend) {
+        int64_t le = e;
+        int lp = pos + kk;
+        bool valid = true;
+        while (lp >= nrhi[le]) {
+            const int over = lp - nrhi[le];
+            if (++le >= eend) { valid = false; break; }
+            lp = nrlo[le] + over;
+        }
+        double2 g = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+        if (valid) {
+            b = xs[(int64_t)lp * R + q0 + j];
+            if (rowok) {
+                const double* yp = spts + 3 * (int64_t)lp;
+                g = FWD ? helmholtz_mv(kw, sign, x0, x1, x2, yp[0], yp[1], yp[2])
+                        : helmholtz_mv(kw, sign, yp[0], yp[1], yp[2], x0, x1, x2);
+                if (CJ) g.y = -g.y;
+            }
+        }
+        cmfma(g, b, acc);
+        pos += 4;
+        while (e < eend && pos >= nrhi[e]) {
+            const int over = pos - nrhi[e];
+            if (++e < eend) pos = nrlo[e] + over;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int row = kk + 4 * v;
+        if (row < nrows) {
+            double2* out = y + (int64_t)dperm[l0 + row0 + row] + (q0 + j) * ldy;
+            double2 val = cmul(alpha, make_double2(acc.p1[v] - acc.p2[v], acc.p3[v] + acc.p4[v]));
+            if (beta.x != 0.0 || beta.y != 0.0) val = cadd(val, cmul(beta, *out));
+            *out = val;
+        }
+    }
+}
+
+// Stage 3 forward for 16 right-hand sides: one wave per 16-row tile of a test
+// cluster; A = the r_t x C coupling row (four columns per step), B = the
+// source coefficients of those columns.
+template <bool CJ>
+__global__ void __launch_bounds__(256)
+    k_couple_fwd_mfma(const double2* __restrict__ S, const int32_t* __restrict__ tile_node,
+                      const int32_t* __restrict__ tile_row0, int64_t ntiles, const int64_t* __restrict__ base,
+                      const int64_t* __restrict__ colbase, const int32_t* __restrict__ ncols,
+                      const int32_t* __restrict__ colcoef, const int32_t* __restrict__ cptr_dst,
+                      const double2* __restrict__ xh, int64_t R, int64_t q0, double2* __restrict__ yh) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= ntiles) return;
+    const int t = tile_node[w], row0 = tile_row0[w];
+    const int rt = cptr_dst[t + 1] - cptr_dst[t];
+    const int C = ncols[t];
+    const int nrows = min(16, rt - row0);
+    const int i = lane & 15, kk = lane >> 4, j = lane & 15;
+    const double2* St = S + base[t];
+    const int32_t* cc = colcoef + colbase[t];
+    Cacc acc;
+    czero(acc);
+    for (int c0 = 0; c0 < C; c0 += 4) {
+        const int c = c0 + kk;
+        double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+        if (c < C) {
+            if (i < nrows) a = ldc<CJ>(St, (int64_t)(row0 + i) + (int64_t)c * rt);
+            b = xh[(int64_t)cc[c] * R + q0 + j];
+        }
+        cmfma(a, b, acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int row = kk + 4 * v;
+        if (row < nrows) {
+            double2* out = yh + (int64_t)(cptr_dst[t] + row0 + row) * R + q0 + j;
+            *out = cadd(*out, make_double2(acc.p1[v] - acc.p2[v], acc.p3[v] + acc.p4[v]));
+        }
+    }
+}
+
 inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // ---------------------------------------------------------------------------
@@ -343,7 +526,11 @@ void run_chunk(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy
     if (evs) evs->mark(3, st);
     const double2* S = h->coupling->blocks.p;
     if (FWD) {
-        if (h->coupling->nentries > 0)
+        if (NR == 16 && h->f_ntiles > 0)
+            hipLaunchKernelGGL((k_couple_fwd_mfma<CJ>), dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S,
+                               h->f_tile_node.p, h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p,
+                               h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
+        else if (NR != 16 && h->coupling->nentries > 0)
             hipLaunchKernelGGL((k_couple_fwd<NR, CJ>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->f_base.p,
                                h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
     } else {
@@ -360,7 +547,12 @@ void run_chunk(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy
                            dst.chdptr.p, dst.chds.p, dst.toff.p, dst.tblocks.p, dst.cptr.p, dst.levptr[l], R, q0, yh);
     }
     if (evs) evs->mark(5, st);
-    if (dst.nleaves > 0)
+    if (NR == 16 && dst.ntiles > 0)
+        hipLaunchKernelGGL((k_leaf_out_mfma<FWD, CJ>), dim3(blocks_for(dst.ntiles, 4)), dim3(256), 0, st,
+                           dst.tile_leaf.p, dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p,
+                           dst.bblocks.p, dst.lo.p, dst.hi.p, dst.perm.p, dst.cptr.p, dst.pts.p, dst.nrptr.p,
+                           dst.nrlo.p, dst.nrhi.p, src.pts.p, yh, xs, R, q0, h->k, h->sign, alpha, beta, y, ldy);
+    else if (NR != 16 && dst.nleaves > 0)
         hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st, dst.leaves.p,
                            dst.leafbasis.p, dst.boff.p, dst.bblocks.p, dst.lo.p, dst.hi.p, dst.perm.p, dst.cptr.p,
                            dst.pts.p, dst.nearptr.p, dst.nearnodes.p, src.lo.p, src.hi.p, src.pts.p, yh, xs, R, q0,
@@ -558,6 +750,15 @@ void build_leaves(SideDev& sd, hipStream_t st) {
     sd.nleaves = (int64_t)leaves.size();
     sd.h_leaves = leaves;
     sd.leaves.upload(leaves, st);
+    std::vector<int32_t> tl, tr;
+    for (int64_t li = 0; li < sd.nleaves; ++li)
+        for (int r0 = 0; r0 < sd.h_hi[leaves[li]] - sd.h_lo[leaves[li]]; r0 += 16) {
+            tl.push_back((int32_t)li);
+            tr.push_back(r0);
+        }
+    sd.ntiles = (int64_t)tl.size();
+    sd.tile_leaf.upload(tl, st);
+    sd.tile_row0.upload(tr, st);
 }
 
 }  // namespace nca
@@ -639,6 +840,16 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
                     }
                 ncols[t] = (int32_t)C;
             }
+            std::vector<int32_t> tnode, trow;
+            for (int64_t t = 0; t < ntest; ++t)
+                if (ncols[t] > 0)
+                    for (int r0 = 0; r0 < coupling->testrank[t]; r0 += 16) {
+                        tnode.push_back((int32_t)t);
+                        trow.push_back(r0);
+                    }
+            h->f_ntiles = (int64_t)tnode.size();
+            h->f_tile_node.upload(tnode, st);
+            h->f_tile_row0.upload(trow, st);
             h->f_base.upload(base, st);
             h->f_colbase.upload(colbase, st);
             h->f_ncols.upload(ncols, st);
@@ -688,6 +899,29 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
             h->test.nearnodes.upload(nn32, st);
             h->trial.nearptr.upload(tp, st);
             h->trial.nearnodes.upload(tn, st);
+            // merged contiguous source ranges per destination node, both orientations
+            auto merge = [](SideDev& dsd, const SideDev& ssd, const std::vector<int64_t>& ptr,
+                            const std::vector<int32_t>& nodes, hipStream_t s) {
+                std::vector<int64_t> mptr(dsd.nn + 1, 0);
+                std::vector<int32_t> mlo, mhi;
+                std::vector<std::pair<int32_t, int32_t>> rg;
+                for (int64_t t = 0; t < dsd.nn; ++t) {
+                    rg.clear();
+                    for (int64_t i = ptr[t]; i < ptr[t + 1]; ++i)
+                        if (ssd.h_hi[nodes[i]] > ssd.h_lo[nodes[i]]) rg.emplace_back(ssd.h_lo[nodes[i]], ssd.h_hi[nodes[i]]);
+                    std::sort(rg.begin(), rg.end());
+                    for (auto& r : rg) {
+                        if ((int64_t)mlo.size() > mptr[t] && mhi.back() == r.first) mhi.back() = r.second;
+                        else { mlo.push_back(r.first); mhi.push_back(r.second); }
+                    }
+                    mptr[t + 1] = (int64_t)mlo.size();
+                }
+                dsd.nrptr.upload(mptr, s);
+                dsd.nrlo.upload(mlo, s);
+                dsd.nrhi.upload(mhi, s);
+            };
+            merge(h->test, h->trial, np, nn32, st);
+            merge(h->trial, h->test, tp, tn, st);
             h->stats.near_pairs = nnz;
             h->stats.near_entries = near_entries;
             h->stats.plan_bytes += nnz * 12 + (ntest + ntrial) * 8;
