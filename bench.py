@@ -40,6 +40,7 @@ class Hip:
         self.lib = ctypes.CDLL("libamdhip64.so")
         self.lib.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
         self.lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.lib.hipDeviceSynchronize.argtypes = []
         self.lib.hipFree.argtypes = [ctypes.c_void_p]
 
     def malloc(self, nbytes):
@@ -92,6 +93,8 @@ def main():
     ap.add_argument("--check-rows", type=int, default=256)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--profile-only", action="store_true", help="assembly + timed sweeps only")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="copy this many MiB device to device first (a kernel of known traffic for counter scaling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -129,6 +132,14 @@ def main():
     dy = hip.malloc(M * R * 16)
     hip.h2d(dx, X)
     hip.sync()
+    if args.copy_probe_mb > 0:
+        nb = args.copy_probe_mb << 20
+        a, b = hip.malloc(nb), hip.malloc(nb)
+        assert hip.lib.hipMemcpy(ctypes.c_void_p(b), ctypes.c_void_p(a), ctypes.c_size_t(nb), 3) == 0
+        hip.sync()
+        hip.free(a)
+        hip.free(b)
+        log(f"copy probe: {args.copy_probe_mb} MiB device to device")
 
     def sweep():
         core.matvec_device(0, R, dx, N, dy, M)

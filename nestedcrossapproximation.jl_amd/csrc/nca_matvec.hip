@@ -11,6 +11,7 @@
 // Coefficient vectors hold the R right-hand sides interleaved: xh[k * R + q].
 
 #include <algorithm>
+#include <cstdlib>
 #include <memory>
 
 #include "nca_common.hpp"
@@ -75,6 +76,12 @@ struct nca_h2 {
     int64_t ws_rhs = 0;
     nca::DevBuf<double2> ws_xs, ws_xh, ws_yh, ws_xin, ws_yio;
     nca_h2_stats stats{};
+    // Launch tuning (environment overrides NCA_XCD_REMAP, NCA_MFMA_K3). Measured
+    // on the 1M-point sphere, 16 RHS: XCD-contiguous block order doubles the
+    // near-field and coupling times (their sources are reused through the
+    // chip-wide last-level cache when the whole chip works on one region), the
+    // three-product complex MFMA is faster than four.
+    int tune_remap = 0, tune_k3 = 1;
 };
 
 namespace nca {
@@ -323,19 +330,35 @@ __global__ void __launch_bounds__(kWave)
 // real products accumulated separately: Re = P1 - P2, Im = P3 + P4.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+// Three real products per complex product (Karatsuba): P1 = Ar Br, P2 = Ai Bi,
+// P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 - P2.
+// K3: three real products (Karatsuba), P3 = (Ar + Ai)(Br + Bi), Im = P3 - P1 - P2;
+// else four, P3 = Ar Bi, P4 = Ai Br, Im = P3 + P4.
+template <bool K3>
 struct Cacc {
     dbl4 p1, p2, p3, p4;
+    __device__ __forceinline__ void zero() { p1 = p2 = p3 = p4 = dbl4{0.0, 0.0, 0.0, 0.0}; }
+    __device__ __forceinline__ void mma(double2 a, double2 b) {
+        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, p1, 0, 0, 0);
+        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, p2, 0, 0, 0);
+        if (K3) {
+            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x + a.y, b.x + b.y, p3, 0, 0, 0);
+        } else {
+            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, p3, 0, 0, 0);
+            p4 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, p4, 0, 0, 0);
+        }
+    }
+    __device__ __forceinline__ double2 get(int v) const {
+        return K3 ? make_double2(p1[v] - p2[v], p3[v] - p1[v] - p2[v]) : make_double2(p1[v] - p2[v], p3[v] + p4[v]);
+    }
 };
 
-__device__ __forceinline__ void cmfma(double2 a, double2 b, Cacc& c) {
-    c.p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, c.p1, 0, 0, 0);
-    c.p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, c.p2, 0, 0, 0);
-    c.p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, c.p3, 0, 0, 0);
-    c.p4 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, c.p4, 0, 0, 0);
-}
-
-__device__ __forceinline__ void czero(Cacc& c) {
-    c.p1 = c.p2 = c.p3 = c.p4 = dbl4{0.0, 0.0, 0.0, 0.0};
+// Workgroup id -> work index. remap != 0: each XCD (workgroups are dealt round
+// robin over the 8 XCDs) gets one contiguous run of work, so neighbouring tiles
+// share sources through that XCD's L2; nblk is then a multiple of 8.
+__device__ __forceinline__ int64_t xcd_block(int64_t nblk, int remap) {
+    const int64_t b = blockIdx.x;
+    return remap ? (b & 7) * (nblk >> 3) + (b >> 3) : b;
 }
 
 constexpr double kInvFourPi = 0x1.45f306dc9c883p-4;
@@ -352,8 +375,33 @@ __device__ __forceinline__ double2 helmholtz_mv(double k, int sign, double x0, d
     ri = ri * fma(-h * ri, ri, 1.5);
     ri = ri * fma(-h * ri, ri, 1.5);
     const double r = r2 * ri;
-    double s, c;
-    sincos_cw(k * r, s, c);
+    // sin/cos: reduction as sincos_cw, Taylor polynomials through y^15 / y^16
+    // (truncation below 7e-17 on |y| <= pi/4), branch-free quadrant select
+    const double x = k * r;
+    const double q = rint(x * kTwoOverPi);
+    double y = fma(-q, kPio2Hi, x);
+    y = fma(-q, kPio2Mid, y);
+    const double z = y * y;
+    double ps = 0x1.6124613a86d09p-33;
+    ps = fma(ps, z, -0x1.ae64567f544e4p-26);
+    ps = fma(ps, z, 0x1.71de3a556c734p-19);
+    ps = fma(ps, z, -0x1.a01a01a01a01ap-13);
+    ps = fma(ps, z, 0x1.1111111111111p-7);
+    ps = fma(ps, z, -0x1.5555555555555p-3);
+    const double sn = fma(y * z, ps, y);
+    double pc = 0x1.ae7f3e733b81fp-45;
+    pc = fma(pc, z, -0x1.93974a8c07c9dp-37);
+    pc = fma(pc, z, 0x1.1eed8eff8d898p-29);
+    pc = fma(pc, z, -0x1.27e4fb7789f5cp-22);
+    pc = fma(pc, z, 0x1.a01a01a01a01ap-16);
+    pc = fma(pc, z, -0x1.6c16c16c16c17p-10);
+    pc = fma(pc, z, 0x1.5555555555555p-5);
+    const double cs = fma(z * z, pc, fma(-0.5, z, 1.0));
+    const int n = (int)q;
+    double s = (n & 1) ? cs : sn;
+    double c = (n & 1) ? sn : cs;
+    if (n & 2) s = -s;
+    if ((n + 1) & 2) c = -c;
     const double sc = ri * kInvFourPi;
     return make_double2(c * sc, (sign < 0 ? -s : s) * sc);
 }
@@ -362,7 +410,7 @@ __device__ __forceinline__ double2 helmholtz_mv(double k, int sign, double x0, d
 // destination leaf. The basis rows enter as the first "sources" (A = U_t or
 // V_t^T, B = yh_t); then the near-field sources, walked over merged contiguous
 // point ranges four at a time (A = G evaluated by the lane, B = xs).
-template <bool FWD, bool CJ>
+template <bool FWD, bool CJ, bool K3>
 __global__ void __launch_bounds__(256)
     k_leaf_out_mfma(const int32_t* __restrict__ tile_leaf, const int32_t* __restrict__ tile_row0, int64_t ntiles,
                     const int32_t* __restrict__ leaves, const int32_t* __restrict__ leafbasis,
@@ -373,9 +421,9 @@ __global__ void __launch_bounds__(256)
                     const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
                     const double* __restrict__ spts, const double2* __restrict__ yh, const double2* __restrict__ xs,
                     int64_t R, int64_t q0, double kw, int sign, double2 alpha, double2 beta,
-                    double2* __restrict__ y, int64_t ldy) {
+                    double2* __restrict__ y, int64_t ldy, int remap) {
     const int lane = threadIdx.x & 63;
-    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t w = xcd_block(gridDim.x, remap) * 4 + (threadIdx.x >> 6);
     if (w >= ntiles) return;
     const int li = tile_leaf[w], row0 = tile_row0[w];
     const int t = leaves[li], bi = leafbasis[li];
@@ -383,8 +431,8 @@ __global__ void __launch_bounds__(256)
     const int nrows = min(16, nt - row0);
     const int i = lane & 15, kk = lane >> 4, j = lane & 15;
     const bool rowok = i < nrows;
-    Cacc acc;
-    czero(acc);
+    Cacc<K3> acc;
+    acc.zero();
     if (bi >= 0) {
         const int rt = dcptr[t + 1] - dcptr[t];
         const double2* B = bblocks + boff[bi];
@@ -396,7 +444,7 @@ __global__ void __launch_bounds__(256)
                     a = ldc<CJ>(B, FWD ? (int64_t)(row0 + i) + (int64_t)k * nt : (int64_t)k + (int64_t)(row0 + i) * rt);
                 b = yh[(int64_t)(dcptr[t] + k) * R + q0 + j];
             }
-            cmfma(a, b, acc);
+            acc.mma(a, b);
         }
     }
     double x0 = 0.0, x1 = 0.0, x2 = 0.0;
@@ -409,18 +457,25 @@ __global__ void __launch_bounds__(256)
     int64_t e = nrptr[t];
     const int64_t eend = nrptr[t + 1];
     int pos = e < eend ? nrlo[e] : 0;
+    int hi_e = e < eend ? nrhi[e] : 0;
+    const double2* xsl = xs + q0 + j;
     while (e < eend) {
-        int64_t le = e;
         int lp = pos + kk;
         bool valid = true;
-        while (lp >= nrhi[le]) {
-            const int over = lp - nrhi[le];
-            if (++le >= eend) { valid = false; break; }
-            lp = nrlo[le] + over;
+        if (pos + 4 > hi_e) {
+            // the four sources cross the end of the range: walk per lane
+            int64_t le = e;
+            int lhi = hi_e;
+            while (lp >= lhi) {
+                const int over = lp - lhi;
+                if (++le >= eend) { valid = false; break; }
+                lp = nrlo[le] + over;
+                lhi = nrhi[le];
+            }
         }
         double2 g = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
         if (valid) {
-            b = xs[(int64_t)lp * R + q0 + j];
+            b = xsl[(int64_t)lp * R];
             if (rowok) {
                 const double* yp = spts + 3 * (int64_t)lp;
                 g = FWD ? helmholtz_mv(kw, sign, x0, x1, x2, yp[0], yp[1], yp[2])
@@ -428,11 +483,13 @@ __global__ void __launch_bounds__(256)
                 if (CJ) g.y = -g.y;
             }
         }
-        cmfma(g, b, acc);
+        acc.mma(g, b);
         pos += 4;
-        while (e < eend && pos >= nrhi[e]) {
-            const int over = pos - nrhi[e];
-            if (++e < eend) pos = nrlo[e] + over;
+        while (pos >= hi_e) {
+            const int over = pos - hi_e;
+            if (++e >= eend) break;
+            pos = nrlo[e] + over;
+            hi_e = nrhi[e];
         }
     }
 #pragma unroll
@@ -440,7 +497,7 @@ __global__ void __launch_bounds__(256)
         const int row = kk + 4 * v;
         if (row < nrows) {
             double2* out = y + (int64_t)dperm[l0 + row0 + row] + (q0 + j) * ldy;
-            double2 val = cmul(alpha, make_double2(acc.p1[v] - acc.p2[v], acc.p3[v] + acc.p4[v]));
+            double2 val = cmul(alpha, acc.get(v));
             if (beta.x != 0.0 || beta.y != 0.0) val = cadd(val, cmul(beta, *out));
             *out = val;
         }
@@ -450,15 +507,16 @@ __global__ void __launch_bounds__(256)
 // Stage 3 forward for 16 right-hand sides: one wave per 16-row tile of a test
 // cluster; A = the r_t x C coupling row (four columns per step), B = the
 // source coefficients of those columns.
-template <bool CJ>
+template <bool CJ, bool K3>
 __global__ void __launch_bounds__(256)
     k_couple_fwd_mfma(const double2* __restrict__ S, const int32_t* __restrict__ tile_node,
                       const int32_t* __restrict__ tile_row0, int64_t ntiles, const int64_t* __restrict__ base,
                       const int64_t* __restrict__ colbase, const int32_t* __restrict__ ncols,
                       const int32_t* __restrict__ colcoef, const int32_t* __restrict__ cptr_dst,
-                      const double2* __restrict__ xh, int64_t R, int64_t q0, double2* __restrict__ yh) {
+                      const double2* __restrict__ xh, int64_t R, int64_t q0, double2* __restrict__ yh,
+                      int remap) {
     const int lane = threadIdx.x & 63;
-    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t w = xcd_block(gridDim.x, remap) * 4 + (threadIdx.x >> 6);
     if (w >= ntiles) return;
     const int t = tile_node[w], row0 = tile_row0[w];
     const int rt = cptr_dst[t + 1] - cptr_dst[t];
@@ -467,8 +525,8 @@ __global__ void __launch_bounds__(256)
     const int i = lane & 15, kk = lane >> 4, j = lane & 15;
     const double2* St = S + base[t];
     const int32_t* cc = colcoef + colbase[t];
-    Cacc acc;
-    czero(acc);
+    Cacc<K3> acc;
+    acc.zero();
     for (int c0 = 0; c0 < C; c0 += 4) {
         const int c = c0 + kk;
         double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
@@ -476,19 +534,21 @@ __global__ void __launch_bounds__(256)
             if (i < nrows) a = ldc<CJ>(St, (int64_t)(row0 + i) + (int64_t)c * rt);
             b = xh[(int64_t)cc[c] * R + q0 + j];
         }
-        cmfma(a, b, acc);
+        acc.mma(a, b);
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int row = kk + 4 * v;
         if (row < nrows) {
             double2* out = yh + (int64_t)(cptr_dst[t] + row0 + row) * R + q0 + j;
-            *out = cadd(*out, make_double2(acc.p1[v] - acc.p2[v], acc.p3[v] + acc.p4[v]));
+            *out = cadd(*out, acc.get(v));
         }
     }
 }
 
 inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+// Grid sizes of the XCD-remapped kernels are padded to a multiple of 8.
+inline unsigned grid8(unsigned g) { return (g + 7u) & ~7u; }
 
 // ---------------------------------------------------------------------------
 // Optional stage timing: events recorded on the stream between the stages.
@@ -526,10 +586,17 @@ void run_chunk(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy
     if (evs) evs->mark(3, st);
     const double2* S = h->coupling->blocks.p;
     if (FWD) {
-        if (NR == 16 && h->f_ntiles > 0)
-            hipLaunchKernelGGL((k_couple_fwd_mfma<CJ>), dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S,
-                               h->f_tile_node.p, h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p,
-                               h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
+        if (NR == 16 && h->f_ntiles > 0) {
+            const unsigned g = h->tune_remap ? grid8(blocks_for(h->f_ntiles, 4)) : blocks_for(h->f_ntiles, 4);
+            if (h->tune_k3)
+                hipLaunchKernelGGL((k_couple_fwd_mfma<CJ, true>), dim3(g), dim3(256), 0, st, S, h->f_tile_node.p,
+                                   h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
+                                   h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh, h->tune_remap);
+            else
+                hipLaunchKernelGGL((k_couple_fwd_mfma<CJ, false>), dim3(g), dim3(256), 0, st, S, h->f_tile_node.p,
+                                   h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
+                                   h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh, h->tune_remap);
+        }
         else if (NR != 16 && h->coupling->nentries > 0)
             hipLaunchKernelGGL((k_couple_fwd<NR, CJ>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->f_base.p,
                                h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
@@ -547,11 +614,14 @@ void run_chunk(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy
                            dst.chdptr.p, dst.chds.p, dst.toff.p, dst.tblocks.p, dst.cptr.p, dst.levptr[l], R, q0, yh);
     }
     if (evs) evs->mark(5, st);
-    if (NR == 16 && dst.ntiles > 0)
-        hipLaunchKernelGGL((k_leaf_out_mfma<FWD, CJ>), dim3(blocks_for(dst.ntiles, 4)), dim3(256), 0, st,
-                           dst.tile_leaf.p, dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p,
-                           dst.bblocks.p, dst.lo.p, dst.hi.p, dst.perm.p, dst.cptr.p, dst.pts.p, dst.nrptr.p,
-                           dst.nrlo.p, dst.nrhi.p, src.pts.p, yh, xs, R, q0, h->k, h->sign, alpha, beta, y, ldy);
+    if (NR == 16 && dst.ntiles > 0) {
+        const unsigned g = h->tune_remap ? grid8(blocks_for(dst.ntiles, 4)) : blocks_for(dst.ntiles, 4);
+        auto kern = h->tune_k3 ? k_leaf_out_mfma<FWD, CJ, true> : k_leaf_out_mfma<FWD, CJ, false>;
+        hipLaunchKernelGGL(kern, dim3(g), dim3(256), 0, st, dst.tile_leaf.p, dst.tile_row0.p, dst.ntiles,
+                           dst.leaves.p, dst.leafbasis.p, dst.boff.p, dst.bblocks.p, dst.lo.p, dst.hi.p, dst.perm.p,
+                           dst.cptr.p, dst.pts.p, dst.nrptr.p, dst.nrlo.p, dst.nrhi.p, src.pts.p, yh, xs, R, q0, h->k,
+                           h->sign, alpha, beta, y, ldy, h->tune_remap);
+    }
     else if (NR != 16 && dst.nleaves > 0)
         hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st, dst.leaves.p,
                            dst.leafbasis.p, dst.boff.p, dst.bblocks.p, dst.lo.p, dst.hi.p, dst.perm.p, dst.cptr.p,
@@ -784,6 +854,8 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
         h->M = d->M;
         h->N = d->N;
         h->coupling = coupling;
+        if (const char* v = getenv("NCA_XCD_REMAP")) h->tune_remap = atoi(v) != 0;
+        if (const char* v = getenv("NCA_MFMA_K3")) h->tune_k3 = atoi(v) != 0;
         build_side(h->test, st, d->M, d->nn_test, d->test_perm, d->test_lo, d->test_hi, d->test_cptr, test,
                    d->nbasis_test, d->basis_test_nodes, d->basis_test_off, d->basis_test, true, d->nlev_test,
                    d->tt_levptr, d->tt_levidcs, d->tt_chdptr, d->tt_chds, d->tt_off, d->tt_blocks);
