@@ -27,6 +27,7 @@ from nca_amd import geometry  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
 
 
 def log(*a):
@@ -144,51 +145,77 @@ def main():
     def sweep():
         core.matvec_device(0, R, dx, N, dy, M)
 
-    for _ in range(args.warmup):
-        sweep()
-    ctx.synchronize()
-    hip.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sweep()
-    ctx.synchronize()
-    hip.sync()
-    t_sweeps = time.perf_counter() - t0
-    ms_per_step = 1e3 * t_sweeps / args.steps
-    value = R * args.steps / t_sweeps
-    log(f"FP64: {ms_per_step:.3f} ms per sweep of {R} RHS -> {value:.1f} matvec/s")
-
-    # per-stage times from HIP events on the context stream
     names = ["gather", "leaf_projection", "upward", "coupling", "downward", "leaf_expansion_near"]
-    prof = np.zeros(6)
-    nprof = 5
-    for _ in range(nprof):
-        ms = np.zeros(6)
-        al = np.array([1.0, 0.0])
-        be = np.array([0.0, 0.0])
-        nca_amd.runtime.call(nca_amd.hip_lib().nca_h2_matvec_profile(
-            core.handle, 0, R, ctypes.c_void_p(dx), N, ctypes.c_void_p(dy), M, al.ctypes.data_as(ctypes.c_void_p),
-            be.ctypes.data_as(ctypes.c_void_p), ms.ctypes.data_as(ctypes.c_void_p)))
-        prof += ms
-    prof /= nprof
-    sbytes = stage_bytes(st, R, M, N)
-    stages = {n: {"ms": round(float(prof[i]), 4), "GB/s": round(sbytes[n] / (prof[i] * 1e-3) / 1e9, 1)}
-              for i, n in enumerate(names)}
-    log("stages:", stages)
+
+    def measure(label):
+        for _ in range(args.warmup):
+            sweep()
+        ctx.synchronize()
+        hip.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sweep()
+        ctx.synchronize()
+        hip.sync()
+        t_sweeps = time.perf_counter() - t0
+        ms = 1e3 * t_sweeps / args.steps
+        log(f"{label}: {ms:.3f} ms per sweep of {R} RHS -> {R * args.steps / t_sweeps:.1f} matvec/s")
+        # per-stage times from HIP events on the context stream
+        prof = np.zeros(6)
+        nprof = 5
+        for _ in range(nprof):
+            sm = np.zeros(6)
+            al = np.array([1.0, 0.0])
+            be = np.array([0.0, 0.0])
+            nca_amd.runtime.call(nca_amd.hip_lib().nca_h2_matvec_profile(
+                core.handle, 0, R, ctypes.c_void_p(dx), N, ctypes.c_void_p(dy), M,
+                al.ctypes.data_as(ctypes.c_void_p), be.ctypes.data_as(ctypes.c_void_p),
+                sm.ctypes.data_as(ctypes.c_void_p)))
+            prof += sm
+        prof /= nprof
+        Y = np.zeros((M, R), np.complex128, order="F")
+        hip.d2h(Y, dy)
+        return ms, R * args.steps / t_sweeps, prof, Y
+
+    results = {}
+    core.set_precision(nca_amd.NCA_PREC_FP64)
+    results["fp64"] = measure("FP64")
+    core.set_precision(nca_amd.NCA_PREC_MIXED)
+    results["mixed"] = measure("MIXED")
+    Y64, Ymix = results["fp64"][3], results["mixed"][3]
+    mixed_vs_fp64 = float(np.linalg.norm(Ymix - Y64) / np.linalg.norm(Y64))
+    log(f"MIXED vs FP64 GPU result: rel err {mixed_vs_fp64:.3e}")
+
+    per_prec = {}
+    for label, s_op in (("fp64", 16), ("mixed", 8)):
+        ms, val, prof, _ = results[label]
+        sbytes = stage_bytes(st, R, M, N, s_op)
+        b_alg = sum(sbytes.values())
+        per_prec[label] = {
+            "matvec_per_s": round(val, 2), "ms_per_sweep": round(ms, 4),
+            "sweep_alg_bytes": int(b_alg), "sweep_hbm_gbs": round(b_alg / (ms * 1e-3) / 1e9, 1),
+            "stages": {n: {"ms": round(float(prof[i]), 4), "GB/s": round(sbytes[n] / (prof[i] * 1e-3) / 1e9, 1)}
+                       for i, n in enumerate(names)},
+        }
+        log(label, "stages:", per_prec[label]["stages"])
+    head = "mixed"
+    ms_per_step, value, prof, _ = results[head]
+    sbytes = stage_bytes(st, R, M, N, 8)
     b_alg = sum(sbytes.values())
     sweep_gbs = b_alg / (ms_per_step * 1e-3) / 1e9
+    stages = per_prec[head]["stages"]
 
-    # dominant kernel: the coupling stream is the HBM-bound one; the near field
-    # is arithmetic (kernel evaluations)
+    # roofline of the dominant kernel of the headline (MIXED) sweep
     dom = names[int(np.argmax(prof))]
-    near_evals = st["near_entries"]
     if dom == "leaf_expansion_near":
-        flops = near_evals * (70.0 + 8.0 * R) + 8.0 * R * st["basis_entries_test"]
+        # complex FP32 MFMA: 8 flop per near pair and right-hand side, plus the basis rows
+        flops = 8.0 * R * (st["near_entries"] + st["basis_entries_test"])
         achieved = flops / (prof[5] * 1e-3) / 1e12
-        roofline = {"kernel": "k_leaf_out (leaf expansion + near field)", "bound": "mfma",
-                    "achieved": round(achieved, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                    "note": "FP64 VALU kernel evaluations + MACs; 70 flop per evaluation counted"}
+        roofline = {"kernel": "k_leaf_out_mfma<PrecF32> (leaf expansion + near field)", "bound": "mfma",
+                    "achieved": round(achieved, 2), "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+                    "note": "MAC flops only (8 per near pair per RHS); the FP32/FP64 VALU kernel evaluations "
+                            f"({st['near_entries']} per sweep) are not counted"}
     else:
         i = names.index(dom)
         achieved = sbytes[dom] / (prof[i] * 1e-3) / 1e9
@@ -206,12 +233,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 (complex fp64)",
+        "dtype": "c32 operators on MFMA, fp64 phase k*r and coefficient accumulation (MIXED); fp64 path in precisions.fp64",
         "data": "synthetic (Fibonacci sphere points, seeded complex Gaussian right-hand sides)",
         "config": {"workload": f"helmholtz_sphere_N{args.n}_k{args.k:g}_tol{args.tol:g}_rhs{R}",
                    "n_points": args.n, "k": args.k, "tol": args.tol, "minvalues": args.minvalues,
-                   "maxrank": args.maxrank, "nrhs": R, "precision": "fp64"},
+                   "maxrank": args.maxrank, "nrhs": R, "precision": "mixed"},
         "sweep_hbm_gbs": round(sweep_gbs, 1),
+        "precisions": per_prec,
+        "mixed_vs_fp64_relerr": mixed_vs_fp64,
+        "mixed_speedup_vs_fp64": round(per_prec["fp64"]["ms_per_sweep"] / per_prec["mixed"]["ms_per_sweep"], 3),
         "sweep_alg_bytes": int(b_alg),
         "stages": stages,
         "assembly_s": round(tm["assembly"], 3),
@@ -223,14 +253,13 @@ def main():
 
     if not args.profile_only and args.check_rows > 0:
         from oracle import oracle as O
-        Y = np.zeros((M, R), np.complex128, order="F")
-        hip.d2h(Y, dy)
         rows = np.sort(rng.choice(M, args.check_rows, replace=False))
         t0 = time.perf_counter()
         ref = O.dense_rows(args.k, 1, pts, pts, rows, X[:, :1])[:, 0]
-        err = float(np.linalg.norm(Y[rows, 0] - ref) / np.linalg.norm(ref))
+        err = {lab: float(np.linalg.norm(results[lab][3][rows, 0] - ref) / np.linalg.norm(ref))
+               for lab in ("fp64", "mixed")}
         result["error_vs_direct_sum"] = {"rows": int(args.check_rows), "rhs": 1, "relerr": err}
-        log(f"error vs dense direct summation on {args.check_rows} rows: {err:.3e} ({time.perf_counter()-t0:.1f}s)")
+        log(f"error vs dense direct summation on {args.check_rows} rows: {err} ({time.perf_counter()-t0:.1f}s)")
 
     if not args.profile_only and args.cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(core, pts, args, X, R)

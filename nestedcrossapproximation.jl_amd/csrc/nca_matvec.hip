@@ -36,6 +36,8 @@ struct SideDev {
     DevBuf<int32_t> parents, chds;
     DevBuf<int64_t> chdptr, toff;
     DevBuf<double2> tblocks;
+    // c32 copies of the bases and transfers for the MIXED path
+    DevBuf<float2> bblocks32, tblocks32;
     // all leaves of the tree (destinations of stage 5) and their basis slot (-1: none)
     int64_t nleaves = 0;
     DevBuf<int32_t> leaves, leafbasis;
@@ -75,6 +77,10 @@ struct nca_h2 {
     // workspace
     int64_t ws_rhs = 0;
     nca::DevBuf<double2> ws_xs, ws_xh, ws_yh, ws_xin, ws_yio;
+    nca::DevBuf<float2> ws_xs32;
+    // c32 copy of the coupling blocks for the MIXED path
+    nca::DevBuf<float2> S32;
+    bool mixed_ready = false;
     nca_h2_stats stats{};
     // Launch tuning (environment overrides NCA_XCD_REMAP, NCA_MFMA_K3). Measured
     // on the 1M-point sphere, 16 RHS: XCD-contiguous block order doubles the
@@ -87,48 +93,61 @@ struct nca_h2 {
 namespace nca {
 
 // ---------------------------------------------------------------------------
+// Storage types: operator blocks and the gathered right-hand sides are double2
+// (FP64) or float2 (MIXED); coefficient vectors and outputs are always double2.
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ void store_as(double2* p, double2 v) { *p = v; }
+__device__ __forceinline__ void store_as(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+
+template <bool CJ, class T2>
+__device__ __forceinline__ double2 ldc(const T2* p, int64_t i) {
+    double2 v = to_d2(p[i]);
+    if (CJ) v.y = -v.y;
+    return v;
+}
+
 __global__ void k_fill(double2* __restrict__ p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = make_double2(0.0, 0.0);
 }
 
+__global__ void k_to_c32(const double2* __restrict__ in, float2* __restrict__ out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
+
 // xs[k*R + q] = x[perm[k] + q*ldx]
+template <class XT>
 __global__ void k_gather(const double2* __restrict__ x, int64_t ldx, const int32_t* __restrict__ perm, int64_t npts,
-                         int64_t R, double2* __restrict__ xs) {
+                         int64_t R, XT* __restrict__ xs) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npts * R) return;
     const int64_t kk = i / R, q = i - kk * R;
-    xs[i] = x[(int64_t)perm[kk] + q * ldx];
-}
-
-template <bool CJ>
-__device__ __forceinline__ double2 ldc(const double2* p, int64_t i) {
-    double2 v = p[i];
-    if (CJ) v.y = -v.y;
-    return v;
+    store_as(xs + i, x[(int64_t)perm[kk] + q * ldx]);
 }
 
 // Stage 1: one wave per source leaf. TR: the block is the destination-side
 // basis used transposed (U_t, |t| x r), else V_s (r x |s|).
-template <int NR, bool TR, bool CJ>
+template <int NR, bool TR, bool CJ, class BT, class XT>
 __global__ void __launch_bounds__(kWave)
-    k_project(const int32_t* __restrict__ bnodes, const int64_t* __restrict__ boff, const double2* __restrict__ blocks,
+    k_project(const int32_t* __restrict__ bnodes, const int64_t* __restrict__ boff, const BT* __restrict__ blocks,
               const int32_t* __restrict__ lo, const int32_t* __restrict__ hi, const int32_t* __restrict__ cptr,
-              const double2* __restrict__ xs, int64_t R, int64_t q0, double2* __restrict__ xh) {
+              const XT* __restrict__ xs, int64_t R, int64_t q0, double2* __restrict__ xh) {
     const int i = blockIdx.x;
     const int s = bnodes[i];
     const int r = cptr[s + 1] - cptr[s];
     const int l0 = lo[s], ns = hi[s] - l0;
-    const double2* B = blocks + boff[i];
+    const BT* B = blocks + boff[i];
     for (int a = threadIdx.x; a < r; a += kWave) {
         double2 acc[NR];
 #pragma unroll
         for (int q = 0; q < NR; ++q) acc[q] = make_double2(0.0, 0.0);
         for (int j = 0; j < ns; ++j) {
             const double2 b = ldc<CJ>(B, TR ? (int64_t)j + (int64_t)a * ns : (int64_t)a + (int64_t)j * r);
-            const double2* xv = xs + (int64_t)(l0 + j) * R + q0;
+            const XT* xv = xs + (int64_t)(l0 + j) * R + q0;
 #pragma unroll
-            for (int q = 0; q < NR; ++q) acc[q] = cfma(b, xv[q], acc[q]);
+            for (int q = 0; q < NR; ++q) acc[q] = cfma(b, to_d2(xv[q]), acc[q]);
         }
         double2* out = xh + (int64_t)(cptr[s] + a) * R + q0;
 #pragma unroll
@@ -139,11 +158,11 @@ __global__ void __launch_bounds__(kWave)
 // Stage 2 (TR = false: E (r_p x r_c), forward trial transfers; TR = true: E
 // (r_c x r_p) used transposed) and stage 4 (DOWN: children accumulate from the
 // parent; TR = false: E (r_c x r_p), TR = true: E (r_p x r_c) transposed).
-// One wave per parent of the level.
-template <int NR, bool DOWN, bool TR, bool CJ>
+// One wave per parent of the level; a parent's edges run on the same lanes.
+template <int NR, bool DOWN, bool TR, bool CJ, class BT>
 __global__ void __launch_bounds__(kWave)
     k_transfer(const int32_t* __restrict__ parents, const int64_t* __restrict__ chdptr,
-               const int32_t* __restrict__ chds, const int64_t* __restrict__ toff, const double2* __restrict__ blocks,
+               const int32_t* __restrict__ chds, const int64_t* __restrict__ toff, const BT* __restrict__ blocks,
                const int32_t* __restrict__ cptr, int64_t pbase, int64_t R, int64_t q0, double2* __restrict__ h) {
     const int64_t pi = pbase + blockIdx.x;
     const int p = parents[pi];
@@ -151,7 +170,7 @@ __global__ void __launch_bounds__(kWave)
     for (int64_t e = chdptr[pi]; e < chdptr[pi + 1]; ++e) {
         const int c = chds[e];
         const int rc = cptr[c + 1] - cptr[c];
-        const double2* E = blocks + toff[e];
+        const BT* E = blocks + toff[e];
         const int nout = DOWN ? rc : rp, nin = DOWN ? rp : rc;
         const int outbase = DOWN ? cptr[c] : cptr[p], inbase = DOWN ? cptr[p] : cptr[c];
         for (int a = threadIdx.x; a < nout; a += kWave) {
@@ -168,17 +187,14 @@ __global__ void __launch_bounds__(kWave)
 #pragma unroll
             for (int q = 0; q < NR; ++q) out[q] = cadd(out[q], acc[q]);
         }
-        // the parent's coefficients are complete before the next edge only
-        // matters for the upward pass, where different edges add into the same
-        // parent from the same lanes: no cross-lane hazard.
     }
 }
 
 // Stage 3 forward: one wave per test cluster t. The coupling blocks of its row
 // form one r_t x C column-major matrix; lane = (row, column slot).
-template <int NR, bool CJ>
+template <int NR, bool CJ, class ST>
 __global__ void __launch_bounds__(kWave)
-    k_couple_fwd(const double2* __restrict__ S, const int64_t* __restrict__ base, const int64_t* __restrict__ colbase,
+    k_couple_fwd(const ST* __restrict__ S, const int64_t* __restrict__ base, const int64_t* __restrict__ colbase,
                  const int32_t* __restrict__ ncols, const int32_t* __restrict__ colcoef,
                  const int32_t* __restrict__ cptr_dst, const double2* __restrict__ xh, int64_t R, int64_t q0,
                  double2* __restrict__ yh) {
@@ -187,7 +203,7 @@ __global__ void __launch_bounds__(kWave)
     const int rt = cptr_dst[t + 1] - cptr_dst[t];
     const int C = ncols[t];
     if (rt == 0 || C == 0) return;
-    const double2* St = S + base[t];
+    const ST* St = S + base[t];
     const int32_t* cc = colcoef + colbase[t];
     const int lane = threadIdx.x;
     for (int r0 = 0; r0 < rt; r0 += kWave) {
@@ -221,9 +237,9 @@ __global__ void __launch_bounds__(kWave)
 }
 
 // Stage 3 reverse (T/C): one wave per trial cluster s: yh_s += sum_t S_ts^T xh_t.
-template <int NR, bool CJ>
+template <int NR, bool CJ, class ST>
 __global__ void __launch_bounds__(kWave)
-    k_couple_rev(const double2* __restrict__ S, const int64_t* __restrict__ rptr, const int32_t* __restrict__ rtest,
+    k_couple_rev(const ST* __restrict__ S, const int64_t* __restrict__ rptr, const int32_t* __restrict__ rtest,
                  const int64_t* __restrict__ roff, const int32_t* __restrict__ cptr_src,
                  const int32_t* __restrict__ cptr_dst, const double2* __restrict__ xh, int64_t R, int64_t q0,
                  double2* __restrict__ yh) {
@@ -237,7 +253,7 @@ __global__ void __launch_bounds__(kWave)
         for (int64_t e = rptr[s]; e < rptr[s + 1]; ++e) {
             const int t = rtest[e];
             const int rt = cptr_src[t + 1] - cptr_src[t];
-            const double2* Sb = S + roff[e] + (int64_t)a * rt;
+            const ST* Sb = S + roff[e] + (int64_t)a * rt;
             const double2* xv0 = xh + (int64_t)cptr_src[t] * R + q0;
             for (int j = 0; j < rt; ++j) {
                 const double2 sv = ldc<CJ>(Sb, j);
@@ -251,120 +267,12 @@ __global__ void __launch_bounds__(kWave)
     }
 }
 
-// Stage 5 + near field, one wave per destination leaf; lane = (row, slot).
-// FWD: destination = test side, G(x_dst, y_src); else G(x_src, y_dst).
-template <int NR, bool FWD, bool CJ>
-__global__ void __launch_bounds__(kWave)
-    k_leaf_out(const int32_t* __restrict__ leaves, const int32_t* __restrict__ leafbasis,
-               const int64_t* __restrict__ boff, const double2* __restrict__ bblocks, const int32_t* __restrict__ dlo,
-               const int32_t* __restrict__ dhi, const int32_t* __restrict__ dperm, const int32_t* __restrict__ dcptr,
-               const double* __restrict__ dpts, const int64_t* __restrict__ nearptr,
-               const int32_t* __restrict__ nearnodes, const int32_t* __restrict__ slo, const int32_t* __restrict__ shi,
-               const double* __restrict__ spts, const double2* __restrict__ yh, const double2* __restrict__ xs,
-               int64_t R, int64_t q0, double kw, int sign, int nearon, double2 alpha, double2 beta,
-               double2* __restrict__ y, int64_t ldy) {
-    __shared__ double2 red[kWave];
-    const int t = leaves[blockIdx.x];
-    const int bi = leafbasis[blockIdx.x];
-    const int l0 = dlo[t], nt = dhi[t] - l0;
-    const int rt = dcptr[t + 1] - dcptr[t];
-    const int lane = threadIdx.x;
-    for (int r0 = 0; r0 < nt; r0 += kWave) {
-        const int rows = min(nt - r0, kWave);
-        const int ncs = kWave / rows;
-        const int row = r0 + lane % rows, slot = lane / rows;
-        double2 acc[NR];
-#pragma unroll
-        for (int q = 0; q < NR; ++q) acc[q] = make_double2(0.0, 0.0);
-        if (slot < ncs) {
-            if (bi >= 0) {
-                const double2* B = bblocks + boff[bi];
-                for (int j = slot; j < rt; j += ncs) {
-                    // FWD: U (nt x r); else V (r x nt) transposed
-                    const double2 b = ldc<CJ>(B, FWD ? (int64_t)row + (int64_t)j * nt : (int64_t)j + (int64_t)row * rt);
-                    const double2* hv = yh + (int64_t)(dcptr[t] + j) * R + q0;
-#pragma unroll
-                    for (int q = 0; q < NR; ++q) acc[q] = cfma(b, hv[q], acc[q]);
-                }
-            }
-            if (nearon) {
-                const double x0 = dpts[3 * (l0 + row)], x1 = dpts[3 * (l0 + row) + 1], x2 = dpts[3 * (l0 + row) + 2];
-                for (int64_t e = nearptr[t]; e < nearptr[t + 1]; ++e) {
-                    const int s = nearnodes[e];
-                    for (int b = slo[s] + slot; b < shi[s]; b += ncs) {
-                        const double y0 = spts[3 * b], y1 = spts[3 * b + 1], y2 = spts[3 * b + 2];
-                        double2 g = FWD ? helmholtz(kw, sign, x0, x1, x2, y0, y1, y2)
-                                        : helmholtz(kw, sign, y0, y1, y2, x0, x1, x2);
-                        if (CJ) g.y = -g.y;
-                        const double2* xv = xs + (int64_t)b * R + q0;
-#pragma unroll
-                        for (int q = 0; q < NR; ++q) acc[q] = cfma(g, xv[q], acc[q]);
-                    }
-                }
-            }
-        }
-        const int64_t dst = dperm[l0 + row];
-#pragma unroll
-        for (int q = 0; q < NR; ++q) {
-            __syncthreads();
-            red[lane] = (slot < ncs) ? acc[q] : make_double2(0.0, 0.0);
-            __syncthreads();
-            if (slot == 0) {
-                double2 sum = red[lane];
-                for (int c = 1; c < ncs; ++c) sum = cadd(sum, red[lane + c * rows]);
-                double2* out = y + dst + (q0 + q) * ldy;
-                double2 v = cmul(alpha, sum);
-                if (beta.x != 0.0 || beta.y != 0.0) v = cadd(v, cmul(beta, *out));
-                *out = v;
-            }
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// MFMA kernels (16 right-hand sides): a 16 x 4 operand tile (rows x sources or
-// columns) times a 4 x 16 tile (sources x right-hand sides) with
-// v_mfma_f64_16x16x4f64. Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]
-// and holds D[l / 16 + 4 v][l % 16], v = 0..3 (the gfx950 layout of this
-// instruction, checked on the device). A complex product takes four
-// real products accumulated separately: Re = P1 - P2, Im = P3 + P4.
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-
-// Three real products per complex product (Karatsuba): P1 = Ar Br, P2 = Ai Bi,
-// P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 - P2.
-// K3: three real products (Karatsuba), P3 = (Ar + Ai)(Br + Bi), Im = P3 - P1 - P2;
-// else four, P3 = Ar Bi, P4 = Ai Br, Im = P3 + P4.
-template <bool K3>
-struct Cacc {
-    dbl4 p1, p2, p3, p4;
-    __device__ __forceinline__ void zero() { p1 = p2 = p3 = p4 = dbl4{0.0, 0.0, 0.0, 0.0}; }
-    __device__ __forceinline__ void mma(double2 a, double2 b) {
-        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, p1, 0, 0, 0);
-        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, p2, 0, 0, 0);
-        if (K3) {
-            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x + a.y, b.x + b.y, p3, 0, 0, 0);
-        } else {
-            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, p3, 0, 0, 0);
-            p4 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, p4, 0, 0, 0);
-        }
-    }
-    __device__ __forceinline__ double2 get(int v) const {
-        return K3 ? make_double2(p1[v] - p2[v], p3[v] - p1[v] - p2[v]) : make_double2(p1[v] - p2[v], p3[v] + p4[v]);
-    }
-};
-
-// Workgroup id -> work index. remap != 0: each XCD (workgroups are dealt round
-// robin over the 8 XCDs) gets one contiguous run of work, so neighbouring tiles
-// share sources through that XCD's L2; nblk is then a multiple of 8.
-__device__ __forceinline__ int64_t xcd_block(int64_t nblk, int remap) {
-    const int64_t b = blockIdx.x;
-    return remap ? (b & 7) * (nblk >> 3) + (b >> 3) : b;
-}
-
 constexpr double kInvFourPi = 0x1.45f306dc9c883p-4;
 
-// G(x, y) for the matvec: 1/r from the hardware reciprocal square root refined
-// by two Newton steps, r = r^2 / r, the same sin/cos as the assembly.
+// G(x, y) for the FP64 matvec: 1/r from the hardware reciprocal square root
+// refined by two Newton steps, r = r^2 / r; sin/cos reduced as sincos_cw, then
+// Taylor polynomials through y^15 / y^16 (truncation below 7e-17 on
+// |y| <= pi/4) and a branch-free quadrant select.
 __device__ __forceinline__ double2 helmholtz_mv(double k, int sign, double x0, double x1, double x2, double y0,
                                                 double y1, double y2) {
     const double dx = x0 - y0, dy = x1 - y1, dz = x2 - y2;
@@ -375,8 +283,6 @@ __device__ __forceinline__ double2 helmholtz_mv(double k, int sign, double x0, d
     ri = ri * fma(-h * ri, ri, 1.5);
     ri = ri * fma(-h * ri, ri, 1.5);
     const double r = r2 * ri;
-    // sin/cos: reduction as sincos_cw, Taylor polynomials through y^15 / y^16
-    // (truncation below 7e-17 on |y| <= pi/4), branch-free quadrant select
     const double x = k * r;
     const double q = rint(x * kTwoOverPi);
     double y = fma(-q, kPio2Hi, x);
@@ -406,22 +312,208 @@ __device__ __forceinline__ double2 helmholtz_mv(double k, int sign, double x0, d
     return make_double2(c * sc, (sign < 0 ? -s : s) * sc);
 }
 
+// G(x, y) for the MIXED matvec: coordinate differences in FP64, then FP32
+// distance and 1/r (hardware rsq + one Newton step); the phase k r and its
+// reduction by pi/2 in FP64; sin/cos of the reduced argument in FP32 (Taylor
+// through y^9 / y^10, truncation below 3e-9 on |y| <= pi/4).
+__device__ __forceinline__ float2 helmholtz_f32(double k, int sign, double x0, double x1, double x2, double y0,
+                                                double y1, double y2) {
+    const float dx = (float)(x0 - y0), dy = (float)(x1 - y1), dz = (float)(x2 - y2);
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    if (r2 == 0.0f) return make_float2(0.0f, 0.0f);
+    float ri = __builtin_amdgcn_rsqf(r2);
+    ri = ri * fmaf(-0.5f * r2 * ri, ri, 1.5f);
+    const float r = r2 * ri;
+    const double x = k * (double)r;
+    const double q = rint(x * kTwoOverPi);
+    const float y = (float)fma(-q, kPio2Mid, fma(-q, kPio2Hi, x));
+    const float z = y * y;
+    float ps = 0x1.71de3ap-19f;
+    ps = fmaf(ps, z, -0x1.a01a02p-13f);
+    ps = fmaf(ps, z, 0x1.111112p-7f);
+    ps = fmaf(ps, z, -0x1.555556p-3f);
+    const float sn = fmaf(y * z, ps, y);
+    float pc = -0x1.27e4fcp-22f;
+    pc = fmaf(pc, z, 0x1.a01a02p-16f);
+    pc = fmaf(pc, z, -0x1.6c16c2p-10f);
+    pc = fmaf(pc, z, 0x1.555556p-5f);
+    const float cs = fmaf(z * z, pc, fmaf(-0.5f, z, 1.0f));
+    const int n = (int)q;
+    float s = (n & 1) ? cs : sn;
+    float c = (n & 1) ? sn : cs;
+    if (n & 2) s = -s;
+    if ((n + 1) & 2) c = -c;
+    const float sc = ri * (float)kInvFourPi;
+    return make_float2(c * sc, (sign < 0 ? -s : s) * sc);
+}
+
+// Stage 5 + near field, VALU form for fewer than 16 right-hand sides: one wave
+// per destination leaf; lane = (row, slot). FWD: destination = test side.
+template <int NR, bool FWD, bool CJ, class BT, class XT>
+__global__ void __launch_bounds__(kWave)
+    k_leaf_out(const int32_t* __restrict__ leaves, const int32_t* __restrict__ leafbasis,
+               const int64_t* __restrict__ boff, const BT* __restrict__ bblocks, const int32_t* __restrict__ dlo,
+               const int32_t* __restrict__ dhi, const int32_t* __restrict__ dperm, const int32_t* __restrict__ dcptr,
+               const double* __restrict__ dpts, const int64_t* __restrict__ nearptr,
+               const int32_t* __restrict__ nearnodes, const int32_t* __restrict__ slo, const int32_t* __restrict__ shi,
+               const double* __restrict__ spts, const double2* __restrict__ yh, const XT* __restrict__ xs,
+               int64_t R, int64_t q0, double kw, int sign, double2 alpha, double2 beta, double2* __restrict__ y,
+               int64_t ldy) {
+    __shared__ double2 red[kWave];
+    const int t = leaves[blockIdx.x];
+    const int bi = leafbasis[blockIdx.x];
+    const int l0 = dlo[t], nt = dhi[t] - l0;
+    const int rt = dcptr[t + 1] - dcptr[t];
+    const int lane = threadIdx.x;
+    for (int r0 = 0; r0 < nt; r0 += kWave) {
+        const int rows = min(nt - r0, kWave);
+        const int ncs = kWave / rows;
+        const int row = r0 + lane % rows, slot = lane / rows;
+        double2 acc[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] = make_double2(0.0, 0.0);
+        if (slot < ncs) {
+            if (bi >= 0) {
+                const BT* B = bblocks + boff[bi];
+                for (int j = slot; j < rt; j += ncs) {
+                    // FWD: U (nt x r); else V (r x nt) transposed
+                    const double2 b = ldc<CJ>(B, FWD ? (int64_t)row + (int64_t)j * nt : (int64_t)j + (int64_t)row * rt);
+                    const double2* hv = yh + (int64_t)(dcptr[t] + j) * R + q0;
+#pragma unroll
+                    for (int q = 0; q < NR; ++q) acc[q] = cfma(b, hv[q], acc[q]);
+                }
+            }
+            const double x0 = dpts[3 * (l0 + row)], x1 = dpts[3 * (l0 + row) + 1], x2 = dpts[3 * (l0 + row) + 2];
+            for (int64_t e = nearptr[t]; e < nearptr[t + 1]; ++e) {
+                const int s = nearnodes[e];
+                for (int b = slo[s] + slot; b < shi[s]; b += ncs) {
+                    const double y0 = spts[3 * b], y1 = spts[3 * b + 1], y2 = spts[3 * b + 2];
+                    double2 g = FWD ? helmholtz(kw, sign, x0, x1, x2, y0, y1, y2)
+                                    : helmholtz(kw, sign, y0, y1, y2, x0, x1, x2);
+                    if (CJ) g.y = -g.y;
+                    const XT* xv = xs + (int64_t)b * R + q0;
+#pragma unroll
+                    for (int q = 0; q < NR; ++q) acc[q] = cfma(g, to_d2(xv[q]), acc[q]);
+                }
+            }
+        }
+        const int64_t dst = dperm[l0 + row];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            __syncthreads();
+            red[lane] = (slot < ncs) ? acc[q] : make_double2(0.0, 0.0);
+            __syncthreads();
+            if (slot == 0) {
+                double2 sum = red[lane];
+                for (int c = 1; c < ncs; ++c) sum = cadd(sum, red[lane + c * rows]);
+                double2* out = y + dst + (q0 + q) * ldy;
+                double2 v = cmul(alpha, sum);
+                if (beta.x != 0.0 || beta.y != 0.0) v = cadd(v, cmul(beta, *out));
+                *out = v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA kernels (16 right-hand sides): a 16 x 4 operand tile (rows x sources or
+// columns) times a 4 x 16 tile (sources x right-hand sides). Lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16]; the 16 x 16 result layout differs
+// per instruction on gfx950 (checked on the device):
+//   v_mfma_f64_16x16x4f64: lane l holds D[l / 16 + 4 v][l % 16], v = 0..3
+//   v_mfma_f32_16x16x4f32: lane l holds D[4 (l / 16) + v][l % 16]
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+struct PrecF64 {
+    using T = double;
+    using T2 = double2;
+    using V4 = dbl4;
+    __device__ static V4 mma(T a, T b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int kk, int v) { return kk + 4 * v; }
+    __device__ static T2 cvt(double2 v) { return v; }
+    __device__ static T2 cvt(float2 v) { return make_double2(v.x, v.y); }
+    __device__ static T2 eval(double k, int sign, double x0, double x1, double x2, double y0, double y1, double y2) {
+        return helmholtz_mv(k, sign, x0, x1, x2, y0, y1, y2);
+    }
+};
+
+struct PrecF32 {
+    using T = float;
+    using T2 = float2;
+    using V4 = flt4;
+    __device__ static V4 mma(T a, T b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int kk, int v) { return 4 * kk + v; }
+    __device__ static T2 cvt(double2 v) { return make_float2((float)v.x, (float)v.y); }
+    __device__ static T2 cvt(float2 v) { return v; }
+    __device__ static T2 eval(double k, int sign, double x0, double x1, double x2, double y0, double y1, double y2) {
+        return helmholtz_f32(k, sign, x0, x1, x2, y0, y1, y2);
+    }
+};
+
+template <class P>
+__device__ __forceinline__ typename P::T2 czero2() {
+    typename P::T2 v;
+    v.x = 0;
+    v.y = 0;
+    return v;
+}
+
+// Complex product accumulator. K3: three real products (Karatsuba),
+// P3 = (Ar + Ai)(Br + Bi), Im = P3 - P1 - P2; else four, Im = P3 + P4.
+template <class P, bool K3>
+struct Cacc {
+    typename P::V4 p1, p2, p3, p4;
+    __device__ __forceinline__ void zero() {
+        for (int v = 0; v < 4; ++v) p1[v] = p2[v] = p3[v] = p4[v] = 0;
+    }
+    __device__ __forceinline__ void mma(typename P::T2 a, typename P::T2 b) {
+        p1 = P::mma(a.x, b.x, p1);
+        p2 = P::mma(a.y, b.y, p2);
+        if (K3) {
+            p3 = P::mma(a.x + a.y, b.x + b.y, p3);
+        } else {
+            p3 = P::mma(a.x, b.y, p3);
+            p4 = P::mma(a.y, b.x, p4);
+        }
+    }
+    __device__ __forceinline__ double2 get(int v) const {
+        const double r1 = p1[v], r2 = p2[v], r3 = p3[v];
+        return K3 ? make_double2(r1 - r2, r3 - r1 - r2) : make_double2(r1 - r2, r3 + (double)p4[v]);
+    }
+};
+
+// Workgroup id -> work index. remap != 0: each XCD (workgroups are dealt round
+// robin over the 8 XCDs) gets one contiguous run of work; nblk is then a
+// multiple of 8.
+__device__ __forceinline__ int64_t xcd_block(int64_t nblk, int remap) {
+    const int64_t b = blockIdx.x;
+    return remap ? (b & 7) * (nblk >> 3) + (b >> 3) : b;
+}
+
+template <bool CJ, class P>
+__device__ __forceinline__ typename P::T2 conjif(typename P::T2 v) {
+    if (CJ) v.y = -v.y;
+    return v;
+}
+
 // Stage 5 + near field for 16 right-hand sides: one wave per 16-row tile of a
 // destination leaf. The basis rows enter as the first "sources" (A = U_t or
 // V_t^T, B = yh_t); then the near-field sources, walked over merged contiguous
 // point ranges four at a time (A = G evaluated by the lane, B = xs).
-template <bool FWD, bool CJ, bool K3>
+template <class P, bool FWD, bool CJ, bool K3>
 __global__ void __launch_bounds__(256)
     k_leaf_out_mfma(const int32_t* __restrict__ tile_leaf, const int32_t* __restrict__ tile_row0, int64_t ntiles,
                     const int32_t* __restrict__ leaves, const int32_t* __restrict__ leafbasis,
-                    const int64_t* __restrict__ boff, const double2* __restrict__ bblocks,
+                    const int64_t* __restrict__ boff, const typename P::T2* __restrict__ bblocks,
                     const int32_t* __restrict__ dlo, const int32_t* __restrict__ dhi,
                     const int32_t* __restrict__ dperm, const int32_t* __restrict__ dcptr,
                     const double* __restrict__ dpts, const int64_t* __restrict__ nrptr,
                     const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
-                    const double* __restrict__ spts, const double2* __restrict__ yh, const double2* __restrict__ xs,
-                    int64_t R, int64_t q0, double kw, int sign, double2 alpha, double2 beta,
-                    double2* __restrict__ y, int64_t ldy, int remap) {
+                    const double* __restrict__ spts, const double2* __restrict__ yh,
+                    const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, double kw, int sign,
+                    double2 alpha, double2 beta, double2* __restrict__ y, int64_t ldy, int remap) {
+    using T2 = typename P::T2;
     const int lane = threadIdx.x & 63;
     const int64_t w = xcd_block(gridDim.x, remap) * 4 + (threadIdx.x >> 6);
     if (w >= ntiles) return;
@@ -431,18 +523,18 @@ __global__ void __launch_bounds__(256)
     const int nrows = min(16, nt - row0);
     const int i = lane & 15, kk = lane >> 4, j = lane & 15;
     const bool rowok = i < nrows;
-    Cacc<K3> acc;
+    Cacc<P, K3> acc;
     acc.zero();
     if (bi >= 0) {
         const int rt = dcptr[t + 1] - dcptr[t];
-        const double2* B = bblocks + boff[bi];
+        const T2* B = bblocks + boff[bi];
         for (int k0 = 0; k0 < rt; k0 += 4) {
             const int k = k0 + kk;
-            double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+            T2 a = czero2<P>(), b = czero2<P>();
             if (k < rt) {
                 if (rowok)
-                    a = ldc<CJ>(B, FWD ? (int64_t)(row0 + i) + (int64_t)k * nt : (int64_t)k + (int64_t)(row0 + i) * rt);
-                b = yh[(int64_t)(dcptr[t] + k) * R + q0 + j];
+                    a = conjif<CJ, P>(B[FWD ? (int64_t)(row0 + i) + (int64_t)k * nt : (int64_t)k + (int64_t)(row0 + i) * rt]);
+                b = P::cvt(yh[(int64_t)(dcptr[t] + k) * R + q0 + j]);
             }
             acc.mma(a, b);
         }
@@ -458,7 +550,7 @@ __global__ void __launch_bounds__(256)
     const int64_t eend = nrptr[t + 1];
     int pos = e < eend ? nrlo[e] : 0;
     int hi_e = e < eend ? nrhi[e] : 0;
-    const double2* xsl = xs + q0 + j;
+    const T2* xsl = xs + q0 + j;
     while (e < eend) {
         int lp = pos + kk;
         bool valid = true;
@@ -473,14 +565,14 @@ __global__ void __launch_bounds__(256)
                 lhi = nrhi[le];
             }
         }
-        double2 g = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+        T2 g = czero2<P>(), b = czero2<P>();
         if (valid) {
             b = xsl[(int64_t)lp * R];
             if (rowok) {
                 const double* yp = spts + 3 * (int64_t)lp;
-                g = FWD ? helmholtz_mv(kw, sign, x0, x1, x2, yp[0], yp[1], yp[2])
-                        : helmholtz_mv(kw, sign, yp[0], yp[1], yp[2], x0, x1, x2);
-                if (CJ) g.y = -g.y;
+                g = FWD ? P::eval(kw, sign, x0, x1, x2, yp[0], yp[1], yp[2])
+                        : P::eval(kw, sign, yp[0], yp[1], yp[2], x0, x1, x2);
+                g = conjif<CJ, P>(g);
             }
         }
         acc.mma(g, b);
@@ -494,7 +586,7 @@ __global__ void __launch_bounds__(256)
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const int row = kk + 4 * v;
+        const int row = P::row(kk, v);
         if (row < nrows) {
             double2* out = y + (int64_t)dperm[l0 + row0 + row] + (q0 + j) * ldy;
             double2 val = cmul(alpha, acc.get(v));
@@ -507,14 +599,15 @@ __global__ void __launch_bounds__(256)
 // Stage 3 forward for 16 right-hand sides: one wave per 16-row tile of a test
 // cluster; A = the r_t x C coupling row (four columns per step), B = the
 // source coefficients of those columns.
-template <bool CJ, bool K3>
+template <class P, bool CJ, bool K3>
 __global__ void __launch_bounds__(256)
-    k_couple_fwd_mfma(const double2* __restrict__ S, const int32_t* __restrict__ tile_node,
+    k_couple_fwd_mfma(const typename P::T2* __restrict__ S, const int32_t* __restrict__ tile_node,
                       const int32_t* __restrict__ tile_row0, int64_t ntiles, const int64_t* __restrict__ base,
                       const int64_t* __restrict__ colbase, const int32_t* __restrict__ ncols,
                       const int32_t* __restrict__ colcoef, const int32_t* __restrict__ cptr_dst,
                       const double2* __restrict__ xh, int64_t R, int64_t q0, double2* __restrict__ yh,
                       int remap) {
+    using T2 = typename P::T2;
     const int lane = threadIdx.x & 63;
     const int64_t w = xcd_block(gridDim.x, remap) * 4 + (threadIdx.x >> 6);
     if (w >= ntiles) return;
@@ -523,22 +616,22 @@ __global__ void __launch_bounds__(256)
     const int C = ncols[t];
     const int nrows = min(16, rt - row0);
     const int i = lane & 15, kk = lane >> 4, j = lane & 15;
-    const double2* St = S + base[t];
+    const T2* St = S + base[t];
     const int32_t* cc = colcoef + colbase[t];
-    Cacc<K3> acc;
+    Cacc<P, K3> acc;
     acc.zero();
     for (int c0 = 0; c0 < C; c0 += 4) {
         const int c = c0 + kk;
-        double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+        T2 a = czero2<P>(), b = czero2<P>();
         if (c < C) {
-            if (i < nrows) a = ldc<CJ>(St, (int64_t)(row0 + i) + (int64_t)c * rt);
-            b = xh[(int64_t)cc[c] * R + q0 + j];
+            if (i < nrows) a = conjif<CJ, P>(St[(int64_t)(row0 + i) + (int64_t)c * rt]);
+            b = P::cvt(xh[(int64_t)cc[c] * R + q0 + j]);
         }
         acc.mma(a, b);
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const int row = kk + 4 * v;
+        const int row = P::row(kk, v);
         if (row < nrows) {
             double2* out = yh + (int64_t)(cptr_dst[t] + row0 + row) * R + q0 + j;
             *out = cadd(*out, acc.get(v));
@@ -560,49 +653,72 @@ struct StageEvents {
     }
 };
 
-template <int NR, int OP>
-void run_chunk(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, int64_t R, int64_t q0,
-               double2 alpha, double2 beta, StageEvents* evs) {
+// Operator blocks of one precision: the FP64 originals or their c32 copies.
+template <bool MIX>
+struct Ops;
+template <>
+struct Ops<false> {
+    using T2 = double2;
+    using P = PrecF64;
+    static const T2* basis(const SideDev& s) { return s.bblocks.p; }
+    static const T2* transfer(const SideDev& s) { return s.tblocks.p; }
+    static const T2* coupling(const nca_h2* h) { return h->coupling->blocks.p; }
+    static T2* xs(nca_h2* h) { return h->ws_xs.p; }
+};
+template <>
+struct Ops<true> {
+    using T2 = float2;
+    using P = PrecF32;
+    static const T2* basis(const SideDev& s) { return s.bblocks32.p; }
+    static const T2* transfer(const SideDev& s) { return s.tblocks32.p; }
+    static const T2* coupling(const nca_h2* h) { return h->S32.p; }
+    static T2* xs(nca_h2* h) { return h->ws_xs32.p; }
+};
+
+template <int NR, int OP, bool MIX>
+void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double2 alpha, double2 beta,
+               StageEvents* evs) {
     constexpr bool FWD = OP == NCA_OP_N;
     constexpr bool CJ = OP == NCA_OP_C;
+    using O = Ops<MIX>;
+    using T2 = typename O::T2;
+    using P = typename O::P;
+    constexpr bool K3 = !MIX;  // three products only where the FP64 accumulation absorbs the cancellation
     hipStream_t st = h->ctx->stream;
     SideDev& src = FWD ? h->trial : h->test;
     SideDev& dst = FWD ? h->test : h->trial;
-    double2* xs = h->ws_xs.p;
+    const T2* xs = O::xs(h);
     double2* xh = h->ws_xh.p;
     double2* yh = h->ws_yh.p;
     if (evs) evs->mark(1, st);
     if (src.nbasis > 0)
-        hipLaunchKernelGGL((k_project<NR, !FWD, CJ>), dim3((unsigned)src.nbasis), dim3(kWave), 0, st, src.bnodes.p,
-                           src.boff.p, src.bblocks.p, src.lo.p, src.hi.p, src.cptr.p, xs, R, q0, xh);
+        hipLaunchKernelGGL((k_project<NR, !FWD, CJ, T2, T2>), dim3((unsigned)src.nbasis), dim3(kWave), 0, st,
+                           src.bnodes.p, src.boff.p, O::basis(src), src.lo.p, src.hi.p, src.cptr.p, xs, R, q0, xh);
     if (evs) evs->mark(2, st);
     // upward: deepest level first; source transfers (trial E forward, test E transposed)
     for (int64_t l = src.nlev - 1; l >= 0; --l) {
         const int64_t np = src.levptr[l + 1] - src.levptr[l];
         if (np == 0) continue;
-        hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ>), dim3((unsigned)np), dim3(kWave), 0, st, src.parents.p,
-                           src.chdptr.p, src.chds.p, src.toff.p, src.tblocks.p, src.cptr.p, src.levptr[l], R, q0, xh);
+        hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
+                           src.parents.p, src.chdptr.p, src.chds.p, src.toff.p, O::transfer(src), src.cptr.p,
+                           src.levptr[l], R, q0, xh);
     }
     if (evs) evs->mark(3, st);
-    const double2* S = h->coupling->blocks.p;
+    const T2* S = O::coupling(h);
     if (FWD) {
         if (NR == 16 && h->f_ntiles > 0) {
             const unsigned g = h->tune_remap ? grid8(blocks_for(h->f_ntiles, 4)) : blocks_for(h->f_ntiles, 4);
-            if (h->tune_k3)
-                hipLaunchKernelGGL((k_couple_fwd_mfma<CJ, true>), dim3(g), dim3(256), 0, st, S, h->f_tile_node.p,
-                                   h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
-                                   h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh, h->tune_remap);
-            else
-                hipLaunchKernelGGL((k_couple_fwd_mfma<CJ, false>), dim3(g), dim3(256), 0, st, S, h->f_tile_node.p,
-                                   h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
-                                   h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh, h->tune_remap);
-        }
-        else if (NR != 16 && h->coupling->nentries > 0)
-            hipLaunchKernelGGL((k_couple_fwd<NR, CJ>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->f_base.p,
+            auto kern = (h->tune_k3 && K3) ? k_couple_fwd_mfma<P, CJ, K3> : k_couple_fwd_mfma<P, CJ, false>;
+            hipLaunchKernelGGL(kern, dim3(g), dim3(256), 0, st, S, h->f_tile_node.p, h->f_tile_row0.p, h->f_ntiles,
+                               h->f_base.p, h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh,
+                               h->tune_remap);
+        } else if (NR != 16 && h->coupling->nentries > 0) {
+            hipLaunchKernelGGL((k_couple_fwd<NR, CJ, T2>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->f_base.p,
                                h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
+        }
     } else {
         if (h->coupling->nentries > 0)
-            hipLaunchKernelGGL((k_couple_rev<NR, CJ>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->r_ptr.p,
+            hipLaunchKernelGGL((k_couple_rev<NR, CJ, T2>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->r_ptr.p,
                                h->r_test.p, h->r_off.p, src.cptr.p, dst.cptr.p, xh, R, q0, yh);
     }
     if (evs) evs->mark(4, st);
@@ -610,28 +726,29 @@ void run_chunk(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy
     for (int64_t l = 0; l < dst.nlev; ++l) {
         const int64_t np = dst.levptr[l + 1] - dst.levptr[l];
         if (np == 0) continue;
-        hipLaunchKernelGGL((k_transfer<NR, true, !FWD, CJ>), dim3((unsigned)np), dim3(kWave), 0, st, dst.parents.p,
-                           dst.chdptr.p, dst.chds.p, dst.toff.p, dst.tblocks.p, dst.cptr.p, dst.levptr[l], R, q0, yh);
+        hipLaunchKernelGGL((k_transfer<NR, true, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
+                           dst.parents.p, dst.chdptr.p, dst.chds.p, dst.toff.p, O::transfer(dst), dst.cptr.p,
+                           dst.levptr[l], R, q0, yh);
     }
     if (evs) evs->mark(5, st);
     if (NR == 16 && dst.ntiles > 0) {
         const unsigned g = h->tune_remap ? grid8(blocks_for(dst.ntiles, 4)) : blocks_for(dst.ntiles, 4);
-        auto kern = h->tune_k3 ? k_leaf_out_mfma<FWD, CJ, true> : k_leaf_out_mfma<FWD, CJ, false>;
+        auto kern = (h->tune_k3 && K3) ? k_leaf_out_mfma<P, FWD, CJ, K3> : k_leaf_out_mfma<P, FWD, CJ, false>;
         hipLaunchKernelGGL(kern, dim3(g), dim3(256), 0, st, dst.tile_leaf.p, dst.tile_row0.p, dst.ntiles,
-                           dst.leaves.p, dst.leafbasis.p, dst.boff.p, dst.bblocks.p, dst.lo.p, dst.hi.p, dst.perm.p,
+                           dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p,
                            dst.cptr.p, dst.pts.p, dst.nrptr.p, dst.nrlo.p, dst.nrhi.p, src.pts.p, yh, xs, R, q0, h->k,
                            h->sign, alpha, beta, y, ldy, h->tune_remap);
+    } else if (NR != 16 && dst.nleaves > 0) {
+        hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ, T2, T2>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st,
+                           dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p,
+                           dst.cptr.p, dst.pts.p, dst.nearptr.p, dst.nearnodes.p, src.lo.p, src.hi.p, src.pts.p, yh, xs,
+                           R, q0, h->k, h->sign, alpha, beta, y, ldy);
     }
-    else if (NR != 16 && dst.nleaves > 0)
-        hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st, dst.leaves.p,
-                           dst.leafbasis.p, dst.boff.p, dst.bblocks.p, dst.lo.p, dst.hi.p, dst.perm.p, dst.cptr.p,
-                           dst.pts.p, dst.nearptr.p, dst.nearnodes.p, src.lo.p, src.hi.p, src.pts.p, yh, xs, R, q0,
-                           h->k, h->sign, 1, alpha, beta, y, ldy);
     if (evs) evs->mark(6, st);
     NCA_HIP(hipGetLastError());
 }
 
-template <int OP>
+template <int OP, bool MIX>
 void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, int64_t R, double2 alpha,
             double2 beta, StageEvents* evs) {
     constexpr bool FWD = OP == NCA_OP_N;
@@ -640,8 +757,8 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     SideDev& dst = FWD ? h->test : h->trial;
     if (evs) evs->mark(0, st);
     if (src.npts * R > 0)
-        hipLaunchKernelGGL(k_gather, dim3(blocks_for(src.npts * R, 256)), dim3(256), 0, st, x, ldx, src.perm.p,
-                           src.npts, R, h->ws_xs.p);
+        hipLaunchKernelGGL((k_gather<typename Ops<MIX>::T2>), dim3(blocks_for(src.npts * R, 256)), dim3(256), 0, st,
+                           x, ldx, src.perm.p, src.npts, R, Ops<MIX>::xs(h));
     if (src.K * R > 0)
         hipLaunchKernelGGL(k_fill, dim3(blocks_for(src.K * R, 256)), dim3(256), 0, st, h->ws_xh.p, src.K * R);
     if (dst.K * R > 0)
@@ -649,21 +766,47 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     int64_t q0 = 0;
     while (q0 < R) {
         const int64_t left = R - q0;
-        if (left >= 16) { run_chunk<16, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta, evs); q0 += 16; }
-        else if (left >= 8) { run_chunk<8, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta, evs); q0 += 8; }
-        else if (left >= 4) { run_chunk<4, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta, evs); q0 += 4; }
-        else if (left >= 2) { run_chunk<2, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta, evs); q0 += 2; }
-        else { run_chunk<1, OP>(h, x, ldx, y, ldy, R, q0, alpha, beta, evs); q0 += 1; }
+        if (left >= 16) { run_chunk<16, OP, MIX>(h, R, q0, y, ldy, alpha, beta, evs); q0 += 16; }
+        else if (left >= 8) { run_chunk<8, OP, MIX>(h, R, q0, y, ldy, alpha, beta, evs); q0 += 8; }
+        else if (left >= 4) { run_chunk<4, OP, MIX>(h, R, q0, y, ldy, alpha, beta, evs); q0 += 4; }
+        else if (left >= 2) { run_chunk<2, OP, MIX>(h, R, q0, y, ldy, alpha, beta, evs); q0 += 2; }
+        else { run_chunk<1, OP, MIX>(h, R, q0, y, ldy, alpha, beta, evs); q0 += 1; }
     }
 }
 
+// c32 copies of the operator for the MIXED path (made once, on first use).
+void ensure_mixed(nca_h2* h) {
+    if (h->mixed_ready) return;
+    hipStream_t st = h->ctx->stream;
+    auto conv = [&](const DevBuf<double2>& in, DevBuf<float2>& out, int64_t n) {
+        out.alloc((size_t)std::max<int64_t>(n, 1));
+        if (n > 0)
+            hipLaunchKernelGGL(k_to_c32, dim3(blocks_for(n, 256)), dim3(256), 0, st, in.p, out.p, n);
+    };
+    conv(h->coupling->blocks, h->S32, h->coupling->nentries);
+    for (SideDev* sd : {&h->test, &h->trial}) {
+        conv(sd->bblocks, sd->bblocks32, sd->basis_entries);
+        conv(sd->tblocks, sd->tblocks32, sd->transfer_entries);
+    }
+    NCA_HIP(hipGetLastError());
+    NCA_HIP(hipStreamSynchronize(st));
+    h->mixed_ready = true;
+}
+
 void ensure_workspace(nca_h2* h, int64_t R) {
-    if (R <= h->ws_rhs) return;
     const int64_t np = std::max(h->M, h->N), K = std::max(h->test.K, h->trial.K);
-    h->ws_xs.alloc((size_t)std::max<int64_t>(np * R, 1));
-    h->ws_xh.alloc((size_t)std::max<int64_t>(K * R, 1));
-    h->ws_yh.alloc((size_t)std::max<int64_t>(K * R, 1));
-    h->ws_rhs = R;
+    if (R > h->ws_rhs) {
+        h->ws_xh.alloc((size_t)std::max<int64_t>(K * R, 1));
+        h->ws_yh.alloc((size_t)std::max<int64_t>(K * R, 1));
+        h->ws_xs.release();
+        h->ws_xs32.release();
+        h->ws_rhs = R;
+    }
+    if (h->precision == NCA_PREC_MIXED) {
+        if (!h->ws_xs32.p) h->ws_xs32.alloc((size_t)std::max<int64_t>(np * h->ws_rhs, 1));
+    } else {
+        if (!h->ws_xs.p) h->ws_xs.alloc((size_t)std::max<int64_t>(np * h->ws_rhs, 1));
+    }
 }
 
 void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t ldx, double2* y, int64_t ldy,
@@ -673,12 +816,20 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
     const int64_t nx = op == NCA_OP_N ? h->N : h->M, ny = op == NCA_OP_N ? h->M : h->N;
     NCA_REQUIRE(ldx >= nx && ldy >= ny, NCA_ERR_DIMENSION, "DimensionMismatch: leading dimension too small");
     if (R == 0 || ny == 0) return;
+    const bool mix = h->precision == NCA_PREC_MIXED;
+    if (mix) ensure_mixed(h);
     ensure_workspace(h, R);
     const double2 al = make_double2(alpha ? alpha[0] : 1.0, alpha ? alpha[1] : 0.0);
     const double2 be = make_double2(beta ? beta[0] : 0.0, beta ? beta[1] : 0.0);
-    if (op == NCA_OP_N) run_op<NCA_OP_N>(h, x, ldx, y, ldy, R, al, be, evs);
-    else if (op == NCA_OP_T) run_op<NCA_OP_T>(h, x, ldx, y, ldy, R, al, be, evs);
-    else run_op<NCA_OP_C>(h, x, ldx, y, ldy, R, al, be, evs);
+    if (mix) {
+        if (op == NCA_OP_N) run_op<NCA_OP_N, true>(h, x, ldx, y, ldy, R, al, be, evs);
+        else if (op == NCA_OP_T) run_op<NCA_OP_T, true>(h, x, ldx, y, ldy, R, al, be, evs);
+        else run_op<NCA_OP_C, true>(h, x, ldx, y, ldy, R, al, be, evs);
+    } else {
+        if (op == NCA_OP_N) run_op<NCA_OP_N, false>(h, x, ldx, y, ldy, R, al, be, evs);
+        else if (op == NCA_OP_T) run_op<NCA_OP_T, false>(h, x, ldx, y, ldy, R, al, be, evs);
+        else run_op<NCA_OP_C, false>(h, x, ldx, y, ldy, R, al, be, evs);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -309,3 +309,27 @@ def test_petrovgalerkinnca_end_to_end(ctx):
     y = W @ x
     ref = O.dense_rows(p.k, 1, p.testpts, p.trialpts, np.arange(1024), x)[:, 0]
     assert relerr(y, ref) < 5e-4
+
+
+@pytest.mark.parametrize("case", ["grid_case", "sphere_case"])
+def test_matvec_mixed_precision(ctx, case, request):
+    """MIXED (complex FP32 operators and near field, FP64 phase and coefficient
+    accumulation) within 1e-5 relative of the FP64 oracle; FP64 again after
+    switching back."""
+    say(f"mixed precision {case}")
+    p, A, ft, fs, oft, ofs = request.getfixturevalue(case)
+    W = build_operator(ctx, p, A, ft, fs)
+    H, _ = oracle_operator(p, oft, ofs)
+    rng = np.random.default_rng(12)
+    N, M = p.trialpts.shape[0], p.testpts.shape[0]
+    W.core_set = W.core.set_precision if hasattr(W.core, "set_precision") else None
+    NC.PetrovGalerkinNCACore.set_precision(W.core, nca_amd.NCA_PREC_MIXED)
+    for R in (16, 3):
+        x = geometry.complex_gaussian(rng, (N, R))
+        u = geometry.complex_gaussian(rng, (M, R))
+        assert relerr(W @ x, H.matvec(x)) <= 1e-5
+        assert relerr(NC.transpose(W) @ u, H.matvec(u, op=1)) <= 1e-5
+        assert relerr(NC.adjoint(W) @ u, H.matvec(u, op=2)) <= 1e-5
+    NC.PetrovGalerkinNCACore.set_precision(W.core, nca_amd.NCA_PREC_FP64)
+    x = geometry.complex_gaussian(rng, (N, 16))
+    assert relerr(W @ x, H.matvec(x)) <= 1e-12
