@@ -415,6 +415,73 @@ __global__ void __launch_bounds__(kWave)
     }
 }
 
+// 16 right-hand sides, lane = (4 operator rows) x (16 right-hand sides): every
+// lane does one complex FMA per operator entry, the right-hand-side loads are
+// 256-byte contiguous, the operator entry is shared by 16 lanes.
+template <bool TR, bool CJ, class BT, class XT>
+__global__ void __launch_bounds__(kWave)
+    k_project16(const int32_t* __restrict__ bnodes, const int64_t* __restrict__ boff, const BT* __restrict__ blocks,
+                const int32_t* __restrict__ lo, const int32_t* __restrict__ hi, const int32_t* __restrict__ cptr,
+                const XT* __restrict__ xs, int64_t R, int64_t q0, double2* __restrict__ xh) {
+    const int i = blockIdx.x;
+    const int s = bnodes[i];
+    const int r = cptr[s + 1] - cptr[s];
+    const int l0 = lo[s], ns = hi[s] - l0;
+    const BT* B = blocks + boff[i];
+    const int q = threadIdx.x & 15, sub = threadIdx.x >> 4;
+    for (int a = sub; a < r; a += 4) {
+        double2 acc = make_double2(0.0, 0.0);
+        for (int j = 0; j < ns; ++j) {
+            const double2 b = ldc<CJ>(B, TR ? (int64_t)j + (int64_t)a * ns : (int64_t)a + (int64_t)j * r);
+            acc = cfma(b, to_d2(xs[(int64_t)(l0 + j) * R + q0 + q]), acc);
+        }
+        double2* out = xh + (int64_t)(cptr[s] + a) * R + q0 + q;
+        *out = cadd(*out, acc);
+    }
+}
+
+template <bool DOWN, bool TR, bool CJ, class BT>
+__global__ void __launch_bounds__(kWave)
+    k_transfer16(const int32_t* __restrict__ parents, const int64_t* __restrict__ chdptr,
+                 const int32_t* __restrict__ chds, const int64_t* __restrict__ toff, const BT* __restrict__ blocks,
+                 const int32_t* __restrict__ cptr, int64_t pbase, int64_t R, int64_t q0, double2* __restrict__ h) {
+    const int64_t pi = pbase + blockIdx.x;
+    const int p = parents[pi];
+    const int rp = cptr[p + 1] - cptr[p];
+    const int q = threadIdx.x & 15, sub = threadIdx.x >> 4;
+    if (DOWN) {
+        for (int64_t e = chdptr[pi]; e < chdptr[pi + 1]; ++e) {
+            const int c = chds[e];
+            const int rc = cptr[c + 1] - cptr[c];
+            const BT* E = blocks + toff[e];
+            for (int a = sub; a < rc; a += 4) {
+                double2 acc = make_double2(0.0, 0.0);
+                for (int j = 0; j < rp; ++j) {
+                    const double2 b = ldc<CJ>(E, TR ? (int64_t)j + (int64_t)a * rp : (int64_t)a + (int64_t)j * rc);
+                    acc = cfma(b, h[(int64_t)(cptr[p] + j) * R + q0 + q], acc);
+                }
+                double2* out = h + (int64_t)(cptr[c] + a) * R + q0 + q;
+                *out = cadd(*out, acc);
+            }
+        }
+    } else {
+        for (int a = sub; a < rp; a += 4) {
+            double2 acc = make_double2(0.0, 0.0);
+            for (int64_t e = chdptr[pi]; e < chdptr[pi + 1]; ++e) {
+                const int c = chds[e];
+                const int rc = cptr[c + 1] - cptr[c];
+                const BT* E = blocks + toff[e];
+                for (int j = 0; j < rc; ++j) {
+                    const double2 b = ldc<CJ>(E, TR ? (int64_t)j + (int64_t)a * rc : (int64_t)a + (int64_t)j * rp);
+                    acc = cfma(b, h[(int64_t)(cptr[c] + j) * R + q0 + q], acc);
+                }
+            }
+            double2* out = h + (int64_t)(cptr[p] + a) * R + q0 + q;
+            *out = cadd(*out, acc);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // MFMA kernels (16 right-hand sides): a 16 x 4 operand tile (rows x sources or
 // columns) times a 4 x 16 tile (sources x right-hand sides). Lane l supplies
@@ -691,17 +758,27 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
     double2* xh = h->ws_xh.p;
     double2* yh = h->ws_yh.p;
     if (evs) evs->mark(1, st);
-    if (src.nbasis > 0)
-        hipLaunchKernelGGL((k_project<NR, !FWD, CJ, T2, T2>), dim3((unsigned)src.nbasis), dim3(kWave), 0, st,
-                           src.bnodes.p, src.boff.p, O::basis(src), src.lo.p, src.hi.p, src.cptr.p, xs, R, q0, xh);
+    if (src.nbasis > 0) {
+        if (NR == 16)
+            hipLaunchKernelGGL((k_project16<!FWD, CJ, T2, T2>), dim3((unsigned)src.nbasis), dim3(kWave), 0, st,
+                               src.bnodes.p, src.boff.p, O::basis(src), src.lo.p, src.hi.p, src.cptr.p, xs, R, q0, xh);
+        else
+            hipLaunchKernelGGL((k_project<NR, !FWD, CJ, T2, T2>), dim3((unsigned)src.nbasis), dim3(kWave), 0, st,
+                               src.bnodes.p, src.boff.p, O::basis(src), src.lo.p, src.hi.p, src.cptr.p, xs, R, q0, xh);
+    }
     if (evs) evs->mark(2, st);
     // upward: deepest level first; source transfers (trial E forward, test E transposed)
     for (int64_t l = src.nlev - 1; l >= 0; --l) {
         const int64_t np = src.levptr[l + 1] - src.levptr[l];
         if (np == 0) continue;
-        hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
-                           src.parents.p, src.chdptr.p, src.chds.p, src.toff.p, O::transfer(src), src.cptr.p,
-                           src.levptr[l], R, q0, xh);
+        if (NR == 16)
+            hipLaunchKernelGGL((k_transfer16<false, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
+                               src.parents.p, src.chdptr.p, src.chds.p, src.toff.p, O::transfer(src), src.cptr.p,
+                               src.levptr[l], R, q0, xh);
+        else
+            hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
+                               src.parents.p, src.chdptr.p, src.chds.p, src.toff.p, O::transfer(src), src.cptr.p,
+                               src.levptr[l], R, q0, xh);
     }
     if (evs) evs->mark(3, st);
     const T2* S = O::coupling(h);
@@ -726,9 +803,14 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
     for (int64_t l = 0; l < dst.nlev; ++l) {
         const int64_t np = dst.levptr[l + 1] - dst.levptr[l];
         if (np == 0) continue;
-        hipLaunchKernelGGL((k_transfer<NR, true, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
-                           dst.parents.p, dst.chdptr.p, dst.chds.p, dst.toff.p, O::transfer(dst), dst.cptr.p,
-                           dst.levptr[l], R, q0, yh);
+        if (NR == 16)
+            hipLaunchKernelGGL((k_transfer16<true, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
+                               dst.parents.p, dst.chdptr.p, dst.chds.p, dst.toff.p, O::transfer(dst), dst.cptr.p,
+                               dst.levptr[l], R, q0, yh);
+        else
+            hipLaunchKernelGGL((k_transfer<NR, true, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
+                               dst.parents.p, dst.chdptr.p, dst.chds.p, dst.toff.p, O::transfer(dst), dst.cptr.p,
+                               dst.levptr[l], R, q0, yh);
     }
     if (evs) evs->mark(5, st);
     if (NR == 16 && dst.ntiles > 0) {
