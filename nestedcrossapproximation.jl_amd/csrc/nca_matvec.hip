@@ -82,12 +82,11 @@ struct nca_h2 {
     nca::DevBuf<float2> S32;
     bool mixed_ready = false;
     nca_h2_stats stats{};
-    // Launch tuning (environment overrides NCA_XCD_REMAP, NCA_MFMA_K3). Measured
-    // on the 1M-point sphere, 16 RHS: XCD-contiguous block order doubles the
-    // near-field and coupling times (their sources are reused through the
-    // chip-wide last-level cache when the whole chip works on one region), the
-    // three-product complex MFMA is faster than four.
-    int tune_remap = 0, tune_k3 = 1;
+    // Complex products on MFMA with three real products (Karatsuba) instead of
+    // four, per precision (environment overrides NCA_MFMA_K3, NCA_MIXED_K3).
+    // Measured on the 1M-point sphere, 16 RHS: three are faster in both; in
+    // FP32 they raise the MIXED error against FP64 from 1.0e-6 to 1.5e-6.
+    int tune_k3 = 1, tune_k3_mixed = 1;
 };
 
 namespace nca {
@@ -269,21 +268,29 @@ __global__ void __launch_bounds__(kWave)
 
 constexpr double kInvFourPi = 0x1.45f306dc9c883p-4;
 
-// G(x, y) for the FP64 matvec: 1/r from the hardware reciprocal square root
-// refined by two Newton steps, r = r^2 / r; sin/cos reduced as sincos_cw, then
-// Taylor polynomials through y^15 / y^16 (truncation below 7e-17 on
-// |y| <= pi/4) and a branch-free quadrant select.
-__device__ __forceinline__ double2 helmholtz_mv(double k, int sign, double x0, double x1, double x2, double y0,
+// Near-field constants of one product: k, k / (2 pi) and the imaginary-part
+// scale +-1 / (4 pi) (the sign of the kernel's phase times -1 for the adjoint).
+struct NearK {
+    double k;
+    double k2pi;
+    double ims;
+};
+
+// G(x, y) for the FP64 near field, branch free (r = 0 gives 0): 1/r from the
+// hardware reciprocal square root refined by two Newton steps; sin/cos reduced
+// by pi/2 in three parts, Taylor polynomials through y^13 / y^14 (truncation
+// below 3e-14 on |y| <= pi/4), quadrant by selects.
+__device__ __forceinline__ double2 helmholtz_mv(const NearK& nk, double x0, double x1, double x2, double y0,
                                                 double y1, double y2) {
     const double dx = x0 - y0, dy = x1 - y1, dz = x2 - y2;
     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-    if (r2 == 0.0) return make_double2(0.0, 0.0);
-    double ri = __builtin_amdgcn_rsq(r2);
-    const double h = 0.5 * r2;
+    const bool nz = r2 > 0.0;
+    const double r2s = nz ? r2 : 1.0;
+    double ri = __builtin_amdgcn_rsq(r2s);
+    const double h = 0.5 * r2s;
     ri = ri * fma(-h * ri, ri, 1.5);
     ri = ri * fma(-h * ri, ri, 1.5);
-    const double r = r2 * ri;
-    const double x = k * r;
+    const double x = nk.k * (r2s * ri);
     const double q = rint(x * kTwoOverPi);
     double y = fma(-q, kPio2Hi, x);
     y = fma(-q, kPio2Mid, y);
@@ -295,8 +302,7 @@ __device__ __forceinline__ double2 helmholtz_mv(double k, int sign, double x0, d
     ps = fma(ps, z, 0x1.1111111111111p-7);
     ps = fma(ps, z, -0x1.5555555555555p-3);
     const double sn = fma(y * z, ps, y);
-    double pc = 0x1.ae7f3e733b81fp-45;
-    pc = fma(pc, z, -0x1.93974a8c07c9dp-37);
+    double pc = -0x1.93974a8c07c9dp-37;
     pc = fma(pc, z, 0x1.1eed8eff8d898p-29);
     pc = fma(pc, z, -0x1.27e4fb7789f5cp-22);
     pc = fma(pc, z, 0x1.a01a01a01a01ap-16);
@@ -308,43 +314,32 @@ __device__ __forceinline__ double2 helmholtz_mv(double k, int sign, double x0, d
     double c = (n & 1) ? sn : cs;
     if (n & 2) s = -s;
     if ((n + 1) & 2) c = -c;
-    const double sc = ri * kInvFourPi;
-    return make_double2(c * sc, (sign < 0 ? -s : s) * sc);
+    const double scr = nz ? ri * kInvFourPi : 0.0;
+    const double sci = nz ? ri * nk.ims : 0.0;
+    return make_double2(c * scr, s * sci);
 }
 
-// G(x, y) for the MIXED matvec: coordinate differences in FP64, then FP32
-// distance and 1/r (hardware rsq + one Newton step); the phase k r and its
-// reduction by pi/2 in FP64; sin/cos of the reduced argument in FP32 (Taylor
-// through y^9 / y^10, truncation below 3e-9 on |y| <= pi/4).
-__device__ __forceinline__ float2 helmholtz_f32(double k, int sign, double x0, double x1, double x2, double y0,
+// G(x, y) for the MIXED near field, branch free: coordinate differences,
+// distance and the phase k r / (2 pi) in FP64 (1/r from the FP32 hardware
+// reciprocal square root plus one FP64 Newton step, relative error ~1e-14),
+// the phase reduced to [-1/2, 1/2] revolutions in FP64; sin/cos of the reduced
+// phase and the 1/(4 pi r) scale in FP32 (hardware sin/cos of 2 pi f).
+// Distances below 1e-15 count as zero.
+__device__ __forceinline__ float2 helmholtz_f32(const NearK& nk, double x0, double x1, double x2, double y0,
                                                 double y1, double y2) {
-    const float dx = (float)(x0 - y0), dy = (float)(x1 - y1), dz = (float)(x2 - y2);
-    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    if (r2 == 0.0f) return make_float2(0.0f, 0.0f);
-    float ri = __builtin_amdgcn_rsqf(r2);
-    ri = ri * fmaf(-0.5f * r2 * ri, ri, 1.5f);
-    const float r = r2 * ri;
-    const double x = k * (double)r;
-    const double q = rint(x * kTwoOverPi);
-    const float y = (float)fma(-q, kPio2Mid, fma(-q, kPio2Hi, x));
-    const float z = y * y;
-    float ps = 0x1.71de3ap-19f;
-    ps = fmaf(ps, z, -0x1.a01a02p-13f);
-    ps = fmaf(ps, z, 0x1.111112p-7f);
-    ps = fmaf(ps, z, -0x1.555556p-3f);
-    const float sn = fmaf(y * z, ps, y);
-    float pc = -0x1.27e4fcp-22f;
-    pc = fmaf(pc, z, 0x1.a01a02p-16f);
-    pc = fmaf(pc, z, -0x1.6c16c2p-10f);
-    pc = fmaf(pc, z, 0x1.555556p-5f);
-    const float cs = fmaf(z * z, pc, fmaf(-0.5f, z, 1.0f));
-    const int n = (int)q;
-    float s = (n & 1) ? cs : sn;
-    float c = (n & 1) ? sn : cs;
-    if (n & 2) s = -s;
-    if ((n + 1) & 2) c = -c;
-    const float sc = ri * (float)kInvFourPi;
-    return make_float2(c * sc, (sign < 0 ? -s : s) * sc);
+    const double dx = x0 - y0, dy = x1 - y1, dz = x2 - y2;
+    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+    const bool nz = r2 > 1e-30;
+    const double r2s = nz ? r2 : 1.0;
+    const float rf = __builtin_amdgcn_rsqf((float)r2s);
+    double ri = (double)rf;
+    ri = ri * fma(-0.5 * r2s * ri, ri, 1.5);
+    const double t = (r2s * ri) * nk.k2pi;
+    const float f = (float)(t - rint(t));
+    const float c = __builtin_amdgcn_cosf(f), s = __builtin_amdgcn_sinf(f);
+    const float scr = nz ? rf * (float)kInvFourPi : 0.0f;
+    const float sci = nz ? rf * (float)nk.ims : 0.0f;
+    return make_float2(c * scr, s * sci);
 }
 
 // Stage 5 + near field, VALU form for fewer than 16 right-hand sides: one wave
@@ -500,8 +495,8 @@ struct PrecF64 {
     __device__ static int row(int kk, int v) { return kk + 4 * v; }
     __device__ static T2 cvt(double2 v) { return v; }
     __device__ static T2 cvt(float2 v) { return make_double2(v.x, v.y); }
-    __device__ static T2 eval(double k, int sign, double x0, double x1, double x2, double y0, double y1, double y2) {
-        return helmholtz_mv(k, sign, x0, x1, x2, y0, y1, y2);
+    __device__ static T2 eval(const NearK& nk, double x0, double x1, double x2, double y0, double y1, double y2) {
+        return helmholtz_mv(nk, x0, x1, x2, y0, y1, y2);
     }
 };
 
@@ -513,8 +508,8 @@ struct PrecF32 {
     __device__ static int row(int kk, int v) { return 4 * kk + v; }
     __device__ static T2 cvt(double2 v) { return make_float2((float)v.x, (float)v.y); }
     __device__ static T2 cvt(float2 v) { return v; }
-    __device__ static T2 eval(double k, int sign, double x0, double x1, double x2, double y0, double y1, double y2) {
-        return helmholtz_f32(k, sign, x0, x1, x2, y0, y1, y2);
+    __device__ static T2 eval(const NearK& nk, double x0, double x1, double x2, double y0, double y1, double y2) {
+        return helmholtz_f32(nk, x0, x1, x2, y0, y1, y2);
     }
 };
 
@@ -550,14 +545,6 @@ struct Cacc {
     }
 };
 
-// Workgroup id -> work index. remap != 0: each XCD (workgroups are dealt round
-// robin over the 8 XCDs) gets one contiguous run of work; nblk is then a
-// multiple of 8.
-__device__ __forceinline__ int64_t xcd_block(int64_t nblk, int remap) {
-    const int64_t b = blockIdx.x;
-    return remap ? (b & 7) * (nblk >> 3) + (b >> 3) : b;
-}
-
 template <bool CJ, class P>
 __device__ __forceinline__ typename P::T2 conjif(typename P::T2 v) {
     if (CJ) v.y = -v.y;
@@ -566,8 +553,11 @@ __device__ __forceinline__ typename P::T2 conjif(typename P::T2 v) {
 
 // Stage 5 + near field for 16 right-hand sides: one wave per 16-row tile of a
 // destination leaf. The basis rows enter as the first "sources" (A = U_t or
-// V_t^T, B = yh_t); then the near-field sources, walked over merged contiguous
-// point ranges four at a time (A = G evaluated by the lane, B = xs).
+// V_t^T, B = yh_t); then the near-field sources, four at a time over each
+// merged contiguous point range (A = G evaluated by the lane, B = xs). All
+// control flow is uniform over the wave: lanes past a range end read its last
+// source and contribute B = 0; rows past the tile end repeat its last row and
+// are not stored.
 template <class P, bool FWD, bool CJ, bool K3>
 __global__ void __launch_bounds__(256)
     k_leaf_out_mfma(const int32_t* __restrict__ tile_leaf, const int32_t* __restrict__ tile_row0, int64_t ntiles,
@@ -578,77 +568,47 @@ __global__ void __launch_bounds__(256)
                     const double* __restrict__ dpts, const int64_t* __restrict__ nrptr,
                     const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
                     const double* __restrict__ spts, const double2* __restrict__ yh,
-                    const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, double kw, int sign,
-                    double2 alpha, double2 beta, double2* __restrict__ y, int64_t ldy, int remap) {
+                    const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, NearK nk, double2 alpha,
+                    double2 beta, double2* __restrict__ y, int64_t ldy) {
     using T2 = typename P::T2;
     const int lane = threadIdx.x & 63;
-    const int64_t w = xcd_block(gridDim.x, remap) * 4 + (threadIdx.x >> 6);
+    const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (w >= ntiles) return;
     const int li = tile_leaf[w], row0 = tile_row0[w];
     const int t = leaves[li], bi = leafbasis[li];
     const int l0 = dlo[t], nt = dhi[t] - l0;
     const int nrows = min(16, nt - row0);
     const int i = lane & 15, kk = lane >> 4, j = lane & 15;
-    const bool rowok = i < nrows;
+    const int rowc = row0 + min(i, nrows - 1);
     Cacc<P, K3> acc;
     acc.zero();
     if (bi >= 0) {
         const int rt = dcptr[t + 1] - dcptr[t];
         const T2* B = bblocks + boff[bi];
+        const double2* yt = yh + (int64_t)dcptr[t] * R + q0 + j;
         for (int k0 = 0; k0 < rt; k0 += 4) {
             const int k = k0 + kk;
-            T2 a = czero2<P>(), b = czero2<P>();
-            if (k < rt) {
-                if (rowok)
-                    a = conjif<CJ, P>(B[FWD ? (int64_t)(row0 + i) + (int64_t)k * nt : (int64_t)k + (int64_t)(row0 + i) * rt]);
-                b = P::cvt(yh[(int64_t)(dcptr[t] + k) * R + q0 + j]);
-            }
+            const int kc = min(k, rt - 1);
+            const T2 a = conjif<CJ, P>(B[FWD ? (int64_t)rowc + (int64_t)kc * nt : (int64_t)kc + (int64_t)rowc * rt]);
+            T2 b = P::cvt(yt[(int64_t)kc * R]);
+            if (k >= rt) b = czero2<P>();
             acc.mma(a, b);
         }
     }
-    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
-    if (rowok) {
-        const double* xp = dpts + 3 * (int64_t)(l0 + row0 + i);
-        x0 = xp[0];
-        x1 = xp[1];
-        x2 = xp[2];
-    }
-    int64_t e = nrptr[t];
-    const int64_t eend = nrptr[t + 1];
-    int pos = e < eend ? nrlo[e] : 0;
-    int hi_e = e < eend ? nrhi[e] : 0;
+    const double* xp = dpts + 3 * (int64_t)(l0 + rowc);
+    const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
     const T2* xsl = xs + q0 + j;
-    while (e < eend) {
-        int lp = pos + kk;
-        bool valid = true;
-        if (pos + 4 > hi_e) {
-            // the four sources cross the end of the range: walk per lane
-            int64_t le = e;
-            int lhi = hi_e;
-            while (lp >= lhi) {
-                const int over = lp - lhi;
-                if (++le >= eend) { valid = false; break; }
-                lp = nrlo[le] + over;
-                lhi = nrhi[le];
-            }
-        }
-        T2 g = czero2<P>(), b = czero2<P>();
-        if (valid) {
-            b = xsl[(int64_t)lp * R];
-            if (rowok) {
-                const double* yp = spts + 3 * (int64_t)lp;
-                g = FWD ? P::eval(kw, sign, x0, x1, x2, yp[0], yp[1], yp[2])
-                        : P::eval(kw, sign, yp[0], yp[1], yp[2], x0, x1, x2);
-                g = conjif<CJ, P>(g);
-            }
-        }
-        acc.mma(g, b);
-        pos += 4;
-        while (pos >= hi_e) {
-            const int over = pos - hi_e;
-            if (++e >= eend) break;
-            pos = nrlo[e] + over;
-            hi_e = nrhi[e];
+    const int64_t e1 = nrptr[t + 1];
+    for (int64_t e = nrptr[t]; e < e1; ++e) {
+        const int lo = nrlo[e], hi = nrhi[e];
+        for (int p = lo; p < hi; p += 4) {
+            const int lp = p + kk;
+            const int lc = min(lp, hi - 1);
+            const double* yp = spts + 3 * (int64_t)lc;
+            T2 b = xsl[(int64_t)lc * R];
+            const T2 g = P::eval(nk, x0, x1, x2, yp[0], yp[1], yp[2]);
+            if (lp >= hi) b = czero2<P>();
+            acc.mma(g, b);
         }
     }
 #pragma unroll
@@ -665,35 +625,35 @@ __global__ void __launch_bounds__(256)
 
 // Stage 3 forward for 16 right-hand sides: one wave per 16-row tile of a test
 // cluster; A = the r_t x C coupling row (four columns per step), B = the
-// source coefficients of those columns.
+// source coefficients of those columns. Control flow is uniform over the wave.
 template <class P, bool CJ, bool K3>
 __global__ void __launch_bounds__(256)
     k_couple_fwd_mfma(const typename P::T2* __restrict__ S, const int32_t* __restrict__ tile_node,
                       const int32_t* __restrict__ tile_row0, int64_t ntiles, const int64_t* __restrict__ base,
                       const int64_t* __restrict__ colbase, const int32_t* __restrict__ ncols,
                       const int32_t* __restrict__ colcoef, const int32_t* __restrict__ cptr_dst,
-                      const double2* __restrict__ xh, int64_t R, int64_t q0, double2* __restrict__ yh,
-                      int remap) {
+                      const double2* __restrict__ xh, int64_t R, int64_t q0, double2* __restrict__ yh) {
     using T2 = typename P::T2;
     const int lane = threadIdx.x & 63;
-    const int64_t w = xcd_block(gridDim.x, remap) * 4 + (threadIdx.x >> 6);
+    const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (w >= ntiles) return;
     const int t = tile_node[w], row0 = tile_row0[w];
     const int rt = cptr_dst[t + 1] - cptr_dst[t];
     const int C = ncols[t];
     const int nrows = min(16, rt - row0);
     const int i = lane & 15, kk = lane >> 4, j = lane & 15;
-    const T2* St = S + base[t];
+    const int rowc = row0 + min(i, nrows - 1);
+    const T2* St = S + base[t] + rowc;
     const int32_t* cc = colcoef + colbase[t];
+    const double2* xq = xh + q0 + j;
     Cacc<P, K3> acc;
     acc.zero();
     for (int c0 = 0; c0 < C; c0 += 4) {
         const int c = c0 + kk;
-        T2 a = czero2<P>(), b = czero2<P>();
-        if (c < C) {
-            if (i < nrows) a = conjif<CJ, P>(St[(int64_t)(row0 + i) + (int64_t)c * rt]);
-            b = P::cvt(xh[(int64_t)cc[c] * R + q0 + j]);
-        }
+        const int cl = min(c, C - 1);
+        const T2 a = conjif<CJ, P>(St[(int64_t)cl * rt]);
+        T2 b = P::cvt(xq[(int64_t)cc[cl] * R]);
+        if (c >= C) b = czero2<P>();
         acc.mma(a, b);
     }
 #pragma unroll
@@ -707,8 +667,6 @@ __global__ void __launch_bounds__(256)
 }
 
 inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
-// Grid sizes of the XCD-remapped kernels are padded to a multiple of 8.
-inline unsigned grid8(unsigned g) { return (g + 7u) & ~7u; }
 
 // ---------------------------------------------------------------------------
 // Optional stage timing: events recorded on the stream between the stages.
@@ -750,7 +708,6 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
     using O = Ops<MIX>;
     using T2 = typename O::T2;
     using P = typename O::P;
-    constexpr bool K3 = !MIX;  // three products only where the FP64 accumulation absorbs the cancellation
     hipStream_t st = h->ctx->stream;
     SideDev& src = FWD ? h->trial : h->test;
     SideDev& dst = FWD ? h->test : h->trial;
@@ -784,11 +741,11 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
     const T2* S = O::coupling(h);
     if (FWD) {
         if (NR == 16 && h->f_ntiles > 0) {
-            const unsigned g = h->tune_remap ? grid8(blocks_for(h->f_ntiles, 4)) : blocks_for(h->f_ntiles, 4);
-            auto kern = (h->tune_k3 && K3) ? k_couple_fwd_mfma<P, CJ, K3> : k_couple_fwd_mfma<P, CJ, false>;
-            hipLaunchKernelGGL(kern, dim3(g), dim3(256), 0, st, S, h->f_tile_node.p, h->f_tile_row0.p, h->f_ntiles,
-                               h->f_base.p, h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh,
-                               h->tune_remap);
+            const bool k3 = MIX ? h->tune_k3_mixed : h->tune_k3;
+            auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true> : k_couple_fwd_mfma<P, CJ, false>;
+            hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
+                               h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
+                               h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
         } else if (NR != 16 && h->coupling->nentries > 0) {
             hipLaunchKernelGGL((k_couple_fwd<NR, CJ, T2>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->f_base.p,
                                h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
@@ -814,12 +771,15 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
     }
     if (evs) evs->mark(5, st);
     if (NR == 16 && dst.ntiles > 0) {
-        const unsigned g = h->tune_remap ? grid8(blocks_for(dst.ntiles, 4)) : blocks_for(dst.ntiles, 4);
-        auto kern = (h->tune_k3 && K3) ? k_leaf_out_mfma<P, FWD, CJ, K3> : k_leaf_out_mfma<P, FWD, CJ, false>;
-        hipLaunchKernelGGL(kern, dim3(g), dim3(256), 0, st, dst.tile_leaf.p, dst.tile_row0.p, dst.ntiles,
-                           dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p,
-                           dst.cptr.p, dst.pts.p, dst.nrptr.p, dst.nrlo.p, dst.nrhi.p, src.pts.p, yh, xs, R, q0, h->k,
-                           h->sign, alpha, beta, y, ldy, h->tune_remap);
+        // conj(G) for the adjoint flips the imaginary scale
+        const double ims = ((h->sign < 0) != CJ) ? -kInvFourPi : kInvFourPi;
+        const NearK nk{h->k, h->k * (0.5 / M_PI), ims};
+        const bool k3 = MIX ? h->tune_k3_mixed : h->tune_k3;
+        auto kern = k3 ? k_leaf_out_mfma<P, FWD, CJ, true> : k_leaf_out_mfma<P, FWD, CJ, false>;
+        hipLaunchKernelGGL(kern, dim3(blocks_for(dst.ntiles, 4)), dim3(256), 0, st, dst.tile_leaf.p,
+                           dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst),
+                           dst.lo.p, dst.hi.p, dst.perm.p, dst.cptr.p, dst.pts.p, dst.nrptr.p, dst.nrlo.p, dst.nrhi.p,
+                           src.pts.p, yh, xs, R, q0, nk, alpha, beta, y, ldy);
     } else if (NR != 16 && dst.nleaves > 0) {
         hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ, T2, T2>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st,
                            dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p,
@@ -1087,8 +1047,8 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
         h->M = d->M;
         h->N = d->N;
         h->coupling = coupling;
-        if (const char* v = getenv("NCA_XCD_REMAP")) h->tune_remap = atoi(v) != 0;
         if (const char* v = getenv("NCA_MFMA_K3")) h->tune_k3 = atoi(v) != 0;
+        if (const char* v = getenv("NCA_MIXED_K3")) h->tune_k3_mixed = atoi(v) != 0;
         build_side(h->test, st, d->M, d->nn_test, d->test_perm, d->test_lo, d->test_hi, d->test_cptr, test,
                    d->nbasis_test, d->basis_test_nodes, d->basis_test_off, d->basis_test, true, d->nlev_test,
                    d->tt_levptr, d->tt_levidcs, d->tt_chdptr, d->tt_chds, d->tt_off, d->tt_blocks);

@@ -271,18 +271,21 @@ def test_matvec_parity(ctx, case, request):
     H, _ = oracle_operator(p, oft, ofs)
     rng = np.random.default_rng(2)
     N, M = p.trialpts.shape[0], p.testpts.shape[0]
-    for R in (1, 3, 16):
+    for R in (1, 3, 16, 19):
         x = geometry.complex_gaussian(rng, (N, R))
         u = geometry.complex_gaussian(rng, (M, R))
         assert relerr(W @ x, H.matvec(x)) <= 1e-12
         assert relerr(NC.transpose(W) @ u, H.matvec(u, op=1)) <= 1e-12
         assert relerr(NC.adjoint(W) @ u, H.matvec(u, op=2)) <= 1e-12
-    x = geometry.complex_gaussian(rng, (N,))
-    y0 = geometry.complex_gaussian(rng, (M,))
     al, be = 0.3 + 0.2j, -1.1
-    y = y0.copy()
-    NC.mul(y, W, x, al, be)
-    assert relerr(y, H.matvec(x, alpha=al, beta=be, y=y0)[:, 0]) <= 1e-12
+    for shape in ((N,), (N, 16)):
+        x = geometry.complex_gaussian(rng, shape)
+        y0 = geometry.complex_gaussian(rng, (M,) + shape[1:])
+        y = y0.copy()
+        NC.mul(y, W, x, al, be)
+        ref = H.matvec(x, alpha=al, beta=be, y=y0)
+        assert relerr(y, ref.reshape(y.shape)) <= 1e-12
+    x = geometry.complex_gaussian(rng, (N,))
     # repeated products are bitwise stable
     assert np.array_equal(W @ x, W @ x)
     with pytest.raises(ValueError):
