@@ -51,7 +51,7 @@ struct SideDev {
     int64_t ntiles = 0;
     DevBuf<int32_t> tile_leaf, tile_row0;
     // host copies used during creation
-    std::vector<int32_t> h_lo, h_hi, h_cptr, h_perm, h_bnodes, h_leaves;
+    std::vector<int32_t> h_lo, h_hi, h_cptr, h_perm, h_bnodes, h_leaves, h_leafslot;
     int64_t basis_entries = 0, transfer_entries = 0;
 };
 
@@ -81,12 +81,16 @@ struct nca_h2 {
     // c32 copy of the coupling blocks for the MIXED path
     nca::DevBuf<float2> S32;
     bool mixed_ready = false;
+    // (cos, sin)(n pi / 128), n = 0..255, for the FP64 near field
+    nca::DevBuf<double2> sctab;
     nca_h2_stats stats{};
     // Complex products on MFMA with three real products (Karatsuba) instead of
     // four, per precision (environment overrides NCA_MFMA_K3, NCA_MIXED_K3).
     // Measured on the 1M-point sphere, 16 RHS: three are faster in both; in
     // FP32 they raise the MIXED error against FP64 from 1.0e-6 to 1.5e-6.
     int tune_k3 = 1, tune_k3_mixed = 1;
+    // XCD-contiguous work order (environment NCA_ORDER_NEAR, NCA_ORDER_COUPLE).
+    int tune_order_near = 0, tune_order_couple = 0;
 };
 
 namespace nca {
@@ -276,12 +280,19 @@ struct NearK {
     double ims;
 };
 
+// Table of (cos, sin)(n pi / 128), n = 0..255, for the FP64 near field.
+constexpr int kSinCosTab = 256;
+constexpr double kTabScale = 0x1.45f306dc9c883p+5;   // 128 / pi
+constexpr double kPi128Hi = 0x1.921fb54442d18p-6;    // pi / 128, leading part
+constexpr double kPi128Lo = 0x1.1a62633145c07p-60;   // pi / 128 - kPi128Hi
+
 // G(x, y) for the FP64 near field, branch free (r = 0 gives 0): 1/r from the
-// hardware reciprocal square root refined by two Newton steps; sin/cos reduced
-// by pi/2 in three parts, Taylor polynomials through y^13 / y^14 (truncation
-// below 3e-14 on |y| <= pi/4), quadrant by selects.
-__device__ __forceinline__ double2 helmholtz_mv(const NearK& nk, double x0, double x1, double x2, double y0,
-                                                double y1, double y2) {
+// hardware reciprocal square root refined by two Newton steps; the phase
+// x = k r split as x = n pi / 128 + y, |y| <= pi / 256 (two-part constant), then
+// sin/cos x = table(n) rotated by sin/cos y from short Taylor polynomials
+// (truncation below 5e-15).
+__device__ __forceinline__ double2 helmholtz_mv(const NearK& nk, const double2* tab, double x0, double x1,
+                                                double x2, double y0, double y1, double y2) {
     const double dx = x0 - y0, dy = x1 - y1, dz = x2 - y2;
     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
     const bool nz = r2 > 0.0;
@@ -291,29 +302,15 @@ __device__ __forceinline__ double2 helmholtz_mv(const NearK& nk, double x0, doub
     ri = ri * fma(-h * ri, ri, 1.5);
     ri = ri * fma(-h * ri, ri, 1.5);
     const double x = nk.k * (r2s * ri);
-    const double q = rint(x * kTwoOverPi);
-    double y = fma(-q, kPio2Hi, x);
-    y = fma(-q, kPio2Mid, y);
+    const double q = rint(x * kTabScale);
+    double y = fma(-q, kPi128Hi, x);
+    y = fma(-q, kPi128Lo, y);
+    const double2 cs = tab[(int)q & (kSinCosTab - 1)];
     const double z = y * y;
-    double ps = 0x1.6124613a86d09p-33;
-    ps = fma(ps, z, -0x1.ae64567f544e4p-26);
-    ps = fma(ps, z, 0x1.71de3a556c734p-19);
-    ps = fma(ps, z, -0x1.a01a01a01a01ap-13);
-    ps = fma(ps, z, 0x1.1111111111111p-7);
-    ps = fma(ps, z, -0x1.5555555555555p-3);
-    const double sn = fma(y * z, ps, y);
-    double pc = -0x1.93974a8c07c9dp-37;
-    pc = fma(pc, z, 0x1.1eed8eff8d898p-29);
-    pc = fma(pc, z, -0x1.27e4fb7789f5cp-22);
-    pc = fma(pc, z, 0x1.a01a01a01a01ap-16);
-    pc = fma(pc, z, -0x1.6c16c16c16c17p-10);
-    pc = fma(pc, z, 0x1.5555555555555p-5);
-    const double cs = fma(z * z, pc, fma(-0.5, z, 1.0));
-    const int n = (int)q;
-    double s = (n & 1) ? cs : sn;
-    double c = (n & 1) ? sn : cs;
-    if (n & 2) s = -s;
-    if ((n + 1) & 2) c = -c;
+    const double sy = fma(y * z, fma(z, 0x1.1111111111111p-7, -0x1.5555555555555p-3), y);
+    const double cy = fma(z, fma(z, 0x1.5555555555555p-5, -0.5), 1.0);
+    const double c = fma(cs.x, cy, -(cs.y * sy));
+    const double s = fma(cs.y, cy, cs.x * sy);
     const double scr = nz ? ri * kInvFourPi : 0.0;
     const double sci = nz ? ri * nk.ims : 0.0;
     return make_double2(c * scr, s * sci);
@@ -495,9 +492,12 @@ struct PrecF64 {
     __device__ static int row(int kk, int v) { return kk + 4 * v; }
     __device__ static T2 cvt(double2 v) { return v; }
     __device__ static T2 cvt(float2 v) { return make_double2(v.x, v.y); }
-    __device__ static T2 eval(const NearK& nk, double x0, double x1, double x2, double y0, double y1, double y2) {
-        return helmholtz_mv(nk, x0, x1, x2, y0, y1, y2);
+    static constexpr bool kTab = true;
+    __device__ static T2 eval(const NearK& nk, const double2* tab, double x0, double x1, double x2, double y0,
+                              double y1, double y2) {
+        return helmholtz_mv(nk, tab, x0, x1, x2, y0, y1, y2);
     }
+
 };
 
 struct PrecF32 {
@@ -508,7 +508,9 @@ struct PrecF32 {
     __device__ static int row(int kk, int v) { return 4 * kk + v; }
     __device__ static T2 cvt(double2 v) { return make_float2((float)v.x, (float)v.y); }
     __device__ static T2 cvt(float2 v) { return v; }
-    __device__ static T2 eval(const NearK& nk, double x0, double x1, double x2, double y0, double y1, double y2) {
+    static constexpr bool kTab = false;
+    __device__ static T2 eval(const NearK& nk, const double2*, double x0, double x1, double x2, double y0, double y1,
+                              double y2) {
         return helmholtz_f32(nk, x0, x1, x2, y0, y1, y2);
     }
 };
@@ -545,21 +547,100 @@ struct Cacc {
     }
 };
 
+// Workgroup -> work block, a bijection on [0, gridDim.x). Workgroups are dealt
+// round robin over the 8 XCDs (each with its own L2). order 0: identity;
+// order 1: each XCD gets one contiguous eighth; order G >= 2: rounds of 8 G
+// workgroups in which each XCD gets G consecutive work blocks (the last
+// partial round keeps the identity).
+__device__ __forceinline__ int work_block(int order) {
+    const int b = blockIdx.x;
+    if (order == 0) return b;
+    const int n = gridDim.x, x = b & 7, i = b >> 3;
+    if (order == 1) return x * (n >> 3) + min(x, n & 7) + i;
+    const int r8 = 8 * order;
+    if (b >= (n / r8) * r8) return b;
+    return (b / r8) * r8 + x * order + i % order;
+}
+
 template <bool CJ, class P>
 __device__ __forceinline__ typename P::T2 conjif(typename P::T2 v) {
     if (CJ) v.y = -v.y;
     return v;
 }
 
+// Lane i of each 16-lane row receives the value of lane (i + S) mod 16 of the
+// same row (DPP row rotate right by 16 - S).
+template <int S>
+__device__ __forceinline__ float rot16(float v) {
+    if (S == 0) return v;
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x120 + (16 - S), 0xf, 0xf, false));
+}
+template <int S>
+__device__ __forceinline__ double rot16(double v) {
+    if (S == 0) return v;
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)u, 0x120 + (16 - S), 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), 0x120 + (16 - S), 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int S, class T2>
+__device__ __forceinline__ T2 rot16v(T2 v) {
+    v.x = rot16<S>(v.x);
+    v.y = rot16<S>(v.y);
+    return v;
+}
+
+// Near-field walk of one tile with H valid-row slots (H = 16, 8 or 4). Lane
+// (i, kk) = (lane % 16, lane / 16) evaluates G for row i % H and source
+// p + 4 (i / H) + kk, so one step evaluates 64 entries over 4 * 16 / H sources;
+// MFMA m takes A = the values of group m, rotated down to lanes 0..H-1 of each
+// 16-lane row (the rows >= H of the product are not stored), and B = xs of
+// sources p + 4 m .. p + 4 m + 3 (all loads of a step are issued before its
+// evaluation). Control flow is uniform over the wave: lanes past a range end
+// read its last source and contribute B = 0. The xs offsets are 32-bit
+// (points x right-hand sides < 2^31).
+template <class P, int H, bool K3>
+__device__ __forceinline__ void near_walk(Cacc<P, K3>& acc, const NearK& nk, const double2* tab, int lane,
+                                          double x0, double x1, double x2, int64_t e0, int64_t e1,
+                                          const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
+                                          const double* __restrict__ spts, const typename P::T2* __restrict__ xsl,
+                                          uint32_t R) {
+    using T2 = typename P::T2;
+    constexpr int G = 16 / H;
+    const int kk = lane >> 4, grp = (lane & 15) / H;
+    for (int64_t e = e0; e < e1; ++e) {
+        const int lo = nrlo[e], hi = nrhi[e];
+        for (int p = lo; p < hi; p += 4 * G) {
+            const double* yp = spts + 3 * (int64_t)min(p + 4 * grp + kk, hi - 1);
+            T2 b[G];
+#pragma unroll
+            for (int m = 0; m < G; ++m) {
+                const int lb = p + 4 * m + kk;
+                b[m] = xsl[(uint32_t)min(lb, hi - 1) * R];
+                if (lb >= hi) b[m] = czero2<P>();
+            }
+            const T2 g = P::eval(nk, tab, x0, x1, x2, yp[0], yp[1], yp[2]);
+#pragma unroll
+            for (int m = 0; m < G; ++m) {
+                if (p + 4 * m >= hi) continue;
+                T2 a;
+                if (m == 0) a = g;
+                else if (m == 1) a = rot16v<H>(g);
+                else if (m == 2) a = rot16v<(2 * H) & 15>(g);
+                else a = rot16v<(3 * H) & 15>(g);
+                acc.mma(a, b[m]);
+            }
+        }
+    }
+}
+
 // Stage 5 + near field for 16 right-hand sides: one wave per 16-row tile of a
 // destination leaf. The basis rows enter as the first "sources" (A = U_t or
-// V_t^T, B = yh_t); then the near-field sources, four at a time over each
-// merged contiguous point range (A = G evaluated by the lane, B = xs). All
-// control flow is uniform over the wave: lanes past a range end read its last
-// source and contribute B = 0; rows past the tile end repeat its last row and
-// are not stored.
+// V_t^T, B = yh_t); then the near-field sources over each merged contiguous
+// point range, with 16, 8 or 4 row slots by the rows of the tile. Rows past
+// the tile end repeat its last row and are not stored.
 template <class P, bool FWD, bool CJ, bool K3>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 4)
     k_leaf_out_mfma(const int32_t* __restrict__ tile_leaf, const int32_t* __restrict__ tile_row0, int64_t ntiles,
                     const int32_t* __restrict__ leaves, const int32_t* __restrict__ leafbasis,
                     const int64_t* __restrict__ boff, const typename P::T2* __restrict__ bblocks,
@@ -568,21 +649,28 @@ __global__ void __launch_bounds__(256)
                     const double* __restrict__ dpts, const int64_t* __restrict__ nrptr,
                     const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
                     const double* __restrict__ spts, const double2* __restrict__ yh,
-                    const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, NearK nk, double2 alpha,
-                    double2 beta, double2* __restrict__ y, int64_t ldy) {
+                    const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, NearK nk,
+                    const double2* __restrict__ sctab, double2 alpha, double2 beta, double2* __restrict__ y,
+                    int64_t ldy, int order) {
     using T2 = typename P::T2;
+    __shared__ double2 tab[P::kTab ? kSinCosTab : 1];
+    if (P::kTab) {
+        static_assert(!P::kTab || kSinCosTab == 256, "one table entry per thread");
+        tab[threadIdx.x] = sctab[threadIdx.x];
+        __syncthreads();
+    }
     const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
     if (w >= ntiles) return;
     const int li = tile_leaf[w], row0 = tile_row0[w];
     const int t = leaves[li], bi = leafbasis[li];
     const int l0 = dlo[t], nt = dhi[t] - l0;
     const int nrows = min(16, nt - row0);
     const int i = lane & 15, kk = lane >> 4, j = lane & 15;
-    const int rowc = row0 + min(i, nrows - 1);
     Cacc<P, K3> acc;
     acc.zero();
     if (bi >= 0) {
+        const int rowc = row0 + min(i, nrows - 1);
         const int rt = dcptr[t + 1] - dcptr[t];
         const T2* B = bblocks + boff[bi];
         const double2* yt = yh + (int64_t)dcptr[t] * R + q0 + j;
@@ -595,22 +683,17 @@ __global__ void __launch_bounds__(256)
             acc.mma(a, b);
         }
     }
-    const double* xp = dpts + 3 * (int64_t)(l0 + rowc);
+    const int H = nrows > 8 ? 16 : (nrows > 4 ? 8 : 4);
+    const double* xp = dpts + 3 * (int64_t)(l0 + row0 + min(i & (H - 1), nrows - 1));
     const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
     const T2* xsl = xs + q0 + j;
-    const int64_t e1 = nrptr[t + 1];
-    for (int64_t e = nrptr[t]; e < e1; ++e) {
-        const int lo = nrlo[e], hi = nrhi[e];
-        for (int p = lo; p < hi; p += 4) {
-            const int lp = p + kk;
-            const int lc = min(lp, hi - 1);
-            const double* yp = spts + 3 * (int64_t)lc;
-            T2 b = xsl[(int64_t)lc * R];
-            const T2 g = P::eval(nk, x0, x1, x2, yp[0], yp[1], yp[2]);
-            if (lp >= hi) b = czero2<P>();
-            acc.mma(g, b);
-        }
-    }
+    const int64_t e0 = nrptr[t], e1 = nrptr[t + 1];
+    if (H == 16)
+        near_walk<P, 16, K3>(acc, nk, tab, lane, x0, x1, x2, e0, e1, nrlo, nrhi, spts, xsl, (uint32_t)R);
+    else if (H == 8)
+        near_walk<P, 8, K3>(acc, nk, tab, lane, x0, x1, x2, e0, e1, nrlo, nrhi, spts, xsl, (uint32_t)R);
+    else
+        near_walk<P, 4, K3>(acc, nk, tab, lane, x0, x1, x2, e0, e1, nrlo, nrhi, spts, xsl, (uint32_t)R);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int row = P::row(kk, v);
@@ -625,17 +708,20 @@ __global__ void __launch_bounds__(256)
 
 // Stage 3 forward for 16 right-hand sides: one wave per 16-row tile of a test
 // cluster; A = the r_t x C coupling row (four columns per step), B = the
-// source coefficients of those columns. Control flow is uniform over the wave.
+// source coefficients of those columns (their coefficient index is loaded one
+// step ahead). Control flow is uniform over the wave; offsets inside a block
+// and into xh are 32-bit (coefficients x right-hand sides < 2^31).
 template <class P, bool CJ, bool K3>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 4)
     k_couple_fwd_mfma(const typename P::T2* __restrict__ S, const int32_t* __restrict__ tile_node,
                       const int32_t* __restrict__ tile_row0, int64_t ntiles, const int64_t* __restrict__ base,
                       const int64_t* __restrict__ colbase, const int32_t* __restrict__ ncols,
                       const int32_t* __restrict__ colcoef, const int32_t* __restrict__ cptr_dst,
-                      const double2* __restrict__ xh, int64_t R, int64_t q0, double2* __restrict__ yh) {
+                      const double2* __restrict__ xh, int64_t R, int64_t q0, double2* __restrict__ yh,
+                      int order) {
     using T2 = typename P::T2;
     const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
     if (w >= ntiles) return;
     const int t = tile_node[w], row0 = tile_row0[w];
     const int rt = cptr_dst[t + 1] - cptr_dst[t];
@@ -646,13 +732,16 @@ __global__ void __launch_bounds__(256)
     const T2* St = S + base[t] + rowc;
     const int32_t* cc = colcoef + colbase[t];
     const double2* xq = xh + q0 + j;
+    const uint32_t ur = (uint32_t)rt, uR = (uint32_t)R;
     Cacc<P, K3> acc;
     acc.zero();
+    int cnext = cc[min(kk, C - 1)];
     for (int c0 = 0; c0 < C; c0 += 4) {
         const int c = c0 + kk;
-        const int cl = min(c, C - 1);
-        const T2 a = conjif<CJ, P>(St[(int64_t)cl * rt]);
-        T2 b = P::cvt(xq[(int64_t)cc[cl] * R]);
+        const int ccur = cnext;
+        cnext = cc[min(c + 4, C - 1)];
+        const T2 a = conjif<CJ, P>(St[(uint32_t)min(c, C - 1) * ur]);
+        T2 b = P::cvt(xq[(uint32_t)ccur * uR]);
         if (c >= C) b = czero2<P>();
         acc.mma(a, b);
     }
@@ -745,7 +834,7 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
             auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true> : k_couple_fwd_mfma<P, CJ, false>;
             hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
                                h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
-                               h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
+                               h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh, h->tune_order_couple);
         } else if (NR != 16 && h->coupling->nentries > 0) {
             hipLaunchKernelGGL((k_couple_fwd<NR, CJ, T2>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->f_base.p,
                                h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
@@ -779,7 +868,7 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
         hipLaunchKernelGGL(kern, dim3(blocks_for(dst.ntiles, 4)), dim3(256), 0, st, dst.tile_leaf.p,
                            dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst),
                            dst.lo.p, dst.hi.p, dst.perm.p, dst.cptr.p, dst.pts.p, dst.nrptr.p, dst.nrlo.p, dst.nrhi.p,
-                           src.pts.p, yh, xs, R, q0, nk, alpha, beta, y, ldy);
+                           src.pts.p, yh, xs, R, q0, nk, h->sctab.p, alpha, beta, y, ldy, h->tune_order_near);
     } else if (NR != 16 && dst.nleaves > 0) {
         hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ, T2, T2>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st,
                            dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p,
@@ -857,6 +946,8 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
     NCA_REQUIRE(R >= 0, NCA_ERR_ARGUMENT, "nca_h2_matvec: negative nrhs");
     const int64_t nx = op == NCA_OP_N ? h->N : h->M, ny = op == NCA_OP_N ? h->M : h->N;
     NCA_REQUIRE(ldx >= nx && ldy >= ny, NCA_ERR_DIMENSION, "DimensionMismatch: leading dimension too small");
+    NCA_REQUIRE(std::max({h->M, h->N, h->test.K, h->trial.K}) * R < ((int64_t)1 << 31), NCA_ERR_LIMIT,
+                "nca_h2_matvec: points (or coefficients) x nrhs must stay below 2^31 (split the right-hand sides)");
     if (R == 0 || ny == 0) return;
     const bool mix = h->precision == NCA_PREC_MIXED;
     if (mix) ensure_mixed(h);
@@ -1012,8 +1103,32 @@ void build_leaves(SideDev& sd, hipStream_t st) {
     }
     sd.nleaves = (int64_t)leaves.size();
     sd.h_leaves = leaves;
+}
+
+// Order of the leaves (and their 16-row tiles) in the near-field launch, from
+// the merged near ranges: 0 node id, 1 point order, 2 decreasing near-field
+// work (longest first). Uploads leaves, leafbasis and the tiles.
+void order_leaves(SideDev& sd, const std::vector<int32_t>& leafslot, const std::vector<int64_t>& mptr,
+                  const std::vector<int32_t>& mlo, const std::vector<int32_t>& mhi, int mode, hipStream_t st) {
+    std::vector<int64_t> idx(sd.nleaves);
+    for (int64_t i = 0; i < sd.nleaves; ++i) idx[i] = i;
+    const std::vector<int32_t>& lv = sd.h_leaves;
+    if (mode == 1)
+        std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return sd.h_lo[lv[a]] < sd.h_lo[lv[b]]; });
+    if (mode == 2) {
+        std::vector<int64_t> work(sd.nleaves, 0);
+        for (int64_t i = 0; i < sd.nleaves; ++i)
+            for (int64_t e = mptr[lv[i]]; e < mptr[lv[i] + 1]; ++e) work[i] += mhi[e] - mlo[e];
+        std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return work[a] > work[b]; });
+    }
+    std::vector<int32_t> leaves(sd.nleaves), lb(sd.nleaves), tl, tr;
+    for (int64_t i = 0; i < sd.nleaves; ++i) {
+        leaves[i] = lv[idx[i]];
+        lb[i] = leafslot[idx[i]];
+    }
+    sd.h_leaves = leaves;
     sd.leaves.upload(leaves, st);
-    std::vector<int32_t> tl, tr;
+    sd.leafbasis.upload(lb, st);
     for (int64_t li = 0; li < sd.nleaves; ++li)
         for (int r0 = 0; r0 < sd.h_hi[leaves[li]] - sd.h_lo[leaves[li]]; r0 += 16) {
             tl.push_back((int32_t)li);
@@ -1047,8 +1162,18 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
         h->M = d->M;
         h->N = d->N;
         h->coupling = coupling;
+        {
+            std::vector<double2> tab(kSinCosTab);
+            for (int n = 0; n < kSinCosTab; ++n) {
+                const long double a = (long double)n * 3.14159265358979323846264338327950288L / 128.0L;
+                tab[n] = make_double2((double)cosl(a), (double)sinl(a));
+            }
+            h->sctab.upload(tab, st);
+        }
         if (const char* v = getenv("NCA_MFMA_K3")) h->tune_k3 = atoi(v) != 0;
         if (const char* v = getenv("NCA_MIXED_K3")) h->tune_k3_mixed = atoi(v) != 0;
+        if (const char* v = getenv("NCA_ORDER_NEAR")) h->tune_order_near = atoi(v);
+        if (const char* v = getenv("NCA_ORDER_COUPLE")) h->tune_order_couple = atoi(v);
         build_side(h->test, st, d->M, d->nn_test, d->test_perm, d->test_lo, d->test_hi, d->test_cptr, test,
                    d->nbasis_test, d->basis_test_nodes, d->basis_test_off, d->basis_test, true, d->nlev_test,
                    d->tt_levptr, d->tt_levidcs, d->tt_chdptr, d->tt_chds, d->tt_off, d->tt_blocks);
@@ -1070,7 +1195,7 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
             }
             for (int64_t i = 0; i < sd->nbasis; ++i)
                 NCA_REQUIRE(isleaf[bn[i]], NCA_ERR_ARGUMENT, "nca_h2_create: a basis belongs to a node that is not a leaf");
-            sd->leafbasis.upload(lb, st);
+            sd->h_leafslot = lb;
         }
         // coupling plan must match the store
         const int64_t ntest = d->nn_test;
@@ -1105,8 +1230,24 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
                     }
                 ncols[t] = (int32_t)C;
             }
+            // launch order of the test clusters: 0 node id, 1 depth-first point
+            // order, 2 decreasing row work
+            std::vector<int64_t> torder(ntest);
+            for (int64_t t = 0; t < ntest; ++t) torder[t] = t;
+            int cmode = 0;
+            if (const char* v = getenv("NCA_COUPLE_ORDER")) cmode = atoi(v);
+            if (cmode == 1)
+                std::sort(torder.begin(), torder.end(), [&](int64_t a, int64_t b) {
+                    const SideDev& sd = h->test;
+                    if (sd.h_lo[a] != sd.h_lo[b]) return sd.h_lo[a] < sd.h_lo[b];
+                    if (sd.h_hi[a] != sd.h_hi[b]) return sd.h_hi[a] > sd.h_hi[b];
+                    return a < b;
+                });
+            if (cmode == 2)
+                std::stable_sort(torder.begin(), torder.end(),
+                                 [&](int64_t a, int64_t b) { return ncols[a] > ncols[b]; });
             std::vector<int32_t> tnode, trow;
-            for (int64_t t = 0; t < ntest; ++t)
+            for (const int64_t t : torder)
                 if (ncols[t] > 0)
                     for (int r0 = 0; r0 < coupling->testrank[t]; r0 += 16) {
                         tnode.push_back((int32_t)t);
@@ -1184,6 +1325,9 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
                 dsd.nrptr.upload(mptr, s);
                 dsd.nrlo.upload(mlo, s);
                 dsd.nrhi.upload(mhi, s);
+                int mode = 0;
+                if (const char* v = getenv("NCA_LEAF_ORDER")) mode = atoi(v);
+                order_leaves(dsd, dsd.h_leafslot, mptr, mlo, mhi, mode, s);
             };
             merge(h->test, h->trial, np, nn32, st);
             merge(h->trial, h->test, tp, tn, st);
