@@ -77,7 +77,7 @@ struct nca_h2 {
     // workspace
     int64_t ws_rhs = 0;
     nca::DevBuf<double2> ws_xs, ws_xh, ws_yh, ws_xin, ws_yio;
-    nca::DevBuf<float2> ws_xs32;
+    nca::DevBuf<float2> ws_xs32, ws_xh32;
     // c32 copy of the coupling blocks for the MIXED path
     nca::DevBuf<float2> S32;
     bool mixed_ready = false;
@@ -118,6 +118,16 @@ __global__ void k_fill(double2* __restrict__ p, int64_t n) {
 __global__ void k_to_c32(const double2* __restrict__ in, float2* __restrict__ out, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
+
+// out[k*16 + q] = c32(in[k*R + q0 + q]), q < 16: one 16-column chunk of a
+// coefficient vector, compacted.
+__global__ void k_chunk_to_c32(const double2* __restrict__ in, int64_t R, int64_t q0, int64_t K,
+                               float2* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K * 16) return;
+    const double2 v = in[(i >> 4) * R + q0 + (i & 15)];
+    out[i] = make_float2((float)v.x, (float)v.y);
 }
 
 // xs[k*R + q] = x[perm[k] + q*ldx]
@@ -717,8 +727,8 @@ __global__ void __launch_bounds__(256, 4)
                       const int32_t* __restrict__ tile_row0, int64_t ntiles, const int64_t* __restrict__ base,
                       const int64_t* __restrict__ colbase, const int32_t* __restrict__ ncols,
                       const int32_t* __restrict__ colcoef, const int32_t* __restrict__ cptr_dst,
-                      const double2* __restrict__ xh, int64_t R, int64_t q0, double2* __restrict__ yh,
-                      int order) {
+                      const typename P::T2* __restrict__ xq0, uint32_t xstride, int64_t R, int64_t q0,
+                      double2* __restrict__ yh, int order) {
     using T2 = typename P::T2;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
@@ -731,19 +741,29 @@ __global__ void __launch_bounds__(256, 4)
     const int rowc = row0 + min(i, nrows - 1);
     const T2* St = S + base[t] + rowc;
     const int32_t* cc = colcoef + colbase[t];
-    const double2* xq = xh + q0 + j;
-    const uint32_t ur = (uint32_t)rt, uR = (uint32_t)R;
+    const T2* xq = xq0 + j;
+    const uint32_t ur = (uint32_t)rt;
     Cacc<P, K3> acc;
     acc.zero();
-    int cnext = cc[min(kk, C - 1)];
-    for (int c0 = 0; c0 < C; c0 += 4) {
-        const int c = c0 + kk;
-        const int ccur = cnext;
-        cnext = cc[min(c + 4, C - 1)];
-        const T2 a = conjif<CJ, P>(St[(uint32_t)min(c, C - 1) * ur]);
-        T2 b = P::cvt(xq[(uint32_t)ccur * uR]);
-        if (c >= C) b = czero2<P>();
-        acc.mma(a, b);
+    // kU steps of four columns per iteration: all their loads are in flight
+    // together; the coefficient indices are loaded one iteration ahead
+    constexpr int kU = 4;
+    int cn[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) cn[u] = cc[min(4 * u + kk, C - 1)];
+    for (int c0 = 0; c0 < C; c0 += 4 * kU) {
+        T2 a[kU], b[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int c = c0 + 4 * u + kk;
+            a[u] = conjif<CJ, P>(St[(uint32_t)min(c, C - 1) * ur]);
+            b[u] = xq[(uint32_t)cn[u] * xstride];
+            if (c >= C) b[u] = czero2<P>();
+            cn[u] = cc[min(c + 4 * kU, C - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (c0 + 4 * u < C) acc.mma(a[u], b[u]);
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -832,9 +852,21 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
         if (NR == 16 && h->f_ntiles > 0) {
             const bool k3 = MIX ? h->tune_k3_mixed : h->tune_k3;
             auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true> : k_couple_fwd_mfma<P, CJ, false>;
+            // MIXED: the source coefficients of this chunk as a compact c32 copy
+            const T2* xq0;
+            uint32_t xstride;
+            if constexpr (MIX) {
+                hipLaunchKernelGGL(k_chunk_to_c32, dim3(blocks_for(src.K * 16, 256)), dim3(256), 0, st, xh, R, q0,
+                                   src.K, h->ws_xh32.p);
+                xq0 = h->ws_xh32.p;
+                xstride = 16;
+            } else {
+                xq0 = xh + q0;
+                xstride = (uint32_t)R;
+            }
             hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
                                h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
-                               h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh, h->tune_order_couple);
+                               h->f_colcoef.p, dst.cptr.p, xq0, xstride, R, q0, yh, h->tune_order_couple);
         } else if (NR != 16 && h->coupling->nentries > 0) {
             hipLaunchKernelGGL((k_couple_fwd<NR, CJ, T2>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->f_base.p,
                                h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
@@ -935,6 +967,7 @@ void ensure_workspace(nca_h2* h, int64_t R) {
     }
     if (h->precision == NCA_PREC_MIXED) {
         if (!h->ws_xs32.p) h->ws_xs32.alloc((size_t)std::max<int64_t>(np * h->ws_rhs, 1));
+        if (!h->ws_xh32.p) h->ws_xh32.alloc((size_t)std::max<int64_t>(K * 16, 1));
     } else {
         if (!h->ws_xs.p) h->ws_xs.alloc((size_t)std::max<int64_t>(np * h->ws_rhs, 1));
     }
