@@ -23,7 +23,7 @@ constexpr int kWave = 64;
 
 struct SideDev {
     int64_t npts = 0, nn = 0, K = 0;
-    DevBuf<int32_t> perm, lo, hi, cptr;
+    DevBuf<int32_t> perm, iperm, lo, hi, cptr;  // iperm: original index -> tree position
     DevBuf<double> pts;  // coordinates in tree order, 3 per position
     // leaf bases
     int64_t nbasis = 0;
@@ -130,14 +130,29 @@ __global__ void k_chunk_to_c32(const double2* __restrict__ in, int64_t R, int64_
     out[i] = make_float2((float)v.x, (float)v.y);
 }
 
-// xs[k*R + q] = x[perm[k] + q*ldx]
+// xs[iperm[i]*R + q] = x[i + q*ldx], moved through LDS in tiles of 64
+// consecutive input points x 16 columns: reads run along the points of one
+// column, writes along the 16 columns of one point.
 template <class XT>
-__global__ void k_gather(const double2* __restrict__ x, int64_t ldx, const int32_t* __restrict__ perm, int64_t npts,
-                         int64_t R, XT* __restrict__ xs) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npts * R) return;
-    const int64_t kk = i / R, q = i - kk * R;
-    store_as(xs + i, x[(int64_t)perm[kk] + q * ldx]);
+__global__ void __launch_bounds__(256)
+    k_gather_tiled(const double2* __restrict__ x, int64_t ldx, const int32_t* __restrict__ iperm, int64_t npts,
+                   int64_t R, XT* __restrict__ xs) {
+    __shared__ double2 tile[16][65];
+    const int t = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 16;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int ql = (t >> 6) + 4 * r;
+        const int64_t p = p0 + (t & 63), q = c0 + ql;
+        if (p < npts && q < R) tile[ql][t & 63] = x[p + q * ldx];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int pl = (t >> 4) + 16 * r;
+        const int64_t p = p0 + pl, q = c0 + (t & 15);
+        if (p < npts && q < R) store_as(xs + (int64_t)iperm[p] * R + q, tile[t & 15][pl]);
+    }
 }
 
 // Stage 1: one wave per source leaf. TR: the block is the destination-side
@@ -920,8 +935,9 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     SideDev& dst = FWD ? h->test : h->trial;
     if (evs) evs->mark(0, st);
     if (src.npts * R > 0)
-        hipLaunchKernelGGL((k_gather<typename Ops<MIX>::T2>), dim3(blocks_for(src.npts * R, 256)), dim3(256), 0, st,
-                           x, ldx, src.perm.p, src.npts, R, Ops<MIX>::xs(h));
+        hipLaunchKernelGGL((k_gather_tiled<typename Ops<MIX>::T2>),
+                           dim3(blocks_for(src.npts, 64), blocks_for(R, 16)), dim3(256), 0, st, x, ldx, src.iperm.p,
+                           src.npts, R, Ops<MIX>::xs(h));
     if (src.K * R > 0)
         hipLaunchKernelGGL(k_fill, dim3(blocks_for(src.K * R, 256)), dim3(256), 0, st, h->ws_xh.p, src.K * R);
     if (dst.K * R > 0)
@@ -1033,6 +1049,11 @@ void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, const int
     for (int64_t i = 0; i < npts; ++i)
         for (int d = 0; d < 3; ++d) pp[3 * i + d] = pts->host[3 * sd.h_perm[i] + d];
     sd.perm.upload(sd.h_perm, st);
+    {
+        std::vector<int32_t> ip(npts);
+        for (int64_t i = 0; i < npts; ++i) ip[sd.h_perm[i]] = (int32_t)i;
+        sd.iperm.upload(ip, st);
+    }
     sd.lo.upload(sd.h_lo, st);
     sd.hi.upload(sd.h_hi, st);
     sd.cptr.upload(sd.h_cptr, st);
