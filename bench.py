@@ -286,7 +286,7 @@ def cpu_baseline(core, pts, args, X, R):
     t0 = time.perf_counter()
     H.matvec(Xs, rows=np.zeros(0, np.int64))
     t_coef = time.perf_counter() - t0
-    nsample = max(1024, M // 256)
+    nsample = max(1024, M // 4)
     rows = np.sort(rng.choice(M, nsample, replace=False))
     t0 = time.perf_counter()
     H.matvec(Xs, rows=rows)

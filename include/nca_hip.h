@@ -34,6 +34,7 @@ typedef struct nca_points nca_points;
 typedef struct nca_stack nca_stack;
 typedef struct nca_coupling nca_coupling;
 typedef struct nca_h2 nca_h2;
+typedef struct nca_h2_graph nca_h2_graph;
 
 enum {
     NCA_OK = 0,
@@ -185,6 +186,18 @@ int32_t nca_h2_matvec(nca_h2* h2, int32_t op, int64_t nrhs, const double* x, int
 /* Same with device pointers, enqueued on the context stream (no host sync). */
 int32_t nca_h2_matvec_device(nca_h2* h2, int32_t op, int64_t nrhs, const void* x, int64_t ldx, void* y,
                              int64_t ldy, const double* alpha, const double* beta);
+
+/* hipGraph capture of one device sweep, no counterpart in the reference (its
+ * CUDA extension launches each kernel from Julia): nca_h2_graph_create records
+ * nca_h2_matvec_device(h2, op, nrhs, x, ldx, y, ldy, alpha, beta) once, bound to
+ * those device buffers and to the current precision; nca_h2_graph_launch
+ * replays it on the context stream (asynchronous, like the device matvec).
+ * Launching after nca_h2_set_precision changed the precision is an
+ * NCA_ERR_ARGUMENT; the graph must be destroyed before its operator. */
+int32_t nca_h2_graph_create(nca_h2* h2, int32_t op, int64_t nrhs, const void* x, int64_t ldx, void* y,
+                            int64_t ldy, const double* alpha, const double* beta, nca_h2_graph** out);
+int32_t nca_h2_graph_launch(nca_h2_graph* graph);
+int32_t nca_h2_graph_destroy(nca_h2_graph* graph);
 
 /* Measurement aid, no counterpart in the reference: one product as
  * nca_h2_matvec_device (nrhs <= 16) with HIP events recorded on the context

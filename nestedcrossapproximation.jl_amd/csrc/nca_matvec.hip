@@ -93,6 +93,14 @@ struct nca_h2 {
     int tune_order_near = 0, tune_order_couple = 0;
 };
 
+// One captured device sweep (see nca_h2_graph_create).
+struct nca_h2_graph {
+    nca_h2* h2 = nullptr;
+    int precision = NCA_PREC_FP64;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
 namespace nca {
 
 // ---------------------------------------------------------------------------
@@ -733,9 +741,9 @@ __global__ void __launch_bounds__(256, 4)
 
 // Stage 3 forward for 16 right-hand sides: one wave per 16-row tile of a test
 // cluster; A = the r_t x C coupling row (four columns per step), B = the
-// source coefficients of those columns (their coefficient index is loaded one
-// step ahead). Control flow is uniform over the wave; offsets inside a block
-// and into xh are 32-bit (coefficients x right-hand sides < 2^31).
+// source coefficients of those columns. Control flow is uniform over the wave;
+// offsets inside a block and into xh are 32-bit (coefficients x right-hand
+// sides < 2^31).
 template <class P, bool CJ, bool K3>
 __global__ void __launch_bounds__(256, 4)
     k_couple_fwd_mfma(const typename P::T2* __restrict__ S, const int32_t* __restrict__ tile_node,
@@ -760,25 +768,45 @@ __global__ void __launch_bounds__(256, 4)
     const uint32_t ur = (uint32_t)rt;
     Cacc<P, K3> acc;
     acc.zero();
-    // kU steps of four columns per iteration: all their loads are in flight
-    // together; the coefficient indices are loaded one iteration ahead
+    // Software pipeline over groups of kU steps (four columns each), two
+    // register sets in turn: the operands of group n + 1 and the coefficient
+    // indices of group n + 2 are in flight while the MFMAs of group n run.
+    // Columns past C repeat column C - 1; their B is zeroed at use.
     constexpr int kU = 4;
-    int cn[kU];
+    T2 a0[kU], b0[kU], a1[kU], b1[kU];
+    int ci0[kU], ci1[kU];
+    auto load_idx = [&](int g0, int (&ci)[kU]) {
 #pragma unroll
-    for (int u = 0; u < kU; ++u) cn[u] = cc[min(4 * u + kk, C - 1)];
-    for (int c0 = 0; c0 < C; c0 += 4 * kU) {
-        T2 a[kU], b[kU];
+        for (int u = 0; u < kU; ++u) ci[u] = cc[min(g0 + 4 * u + kk, C - 1)];
+    };
+    auto load_ops = [&](int g0, const int (&ci)[kU], T2 (&a)[kU], T2 (&b)[kU]) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            const int c = c0 + 4 * u + kk;
-            a[u] = conjif<CJ, P>(St[(uint32_t)min(c, C - 1) * ur]);
-            b[u] = xq[(uint32_t)cn[u] * xstride];
-            if (c >= C) b[u] = czero2<P>();
-            cn[u] = cc[min(c + 4 * kU, C - 1)];
+            a[u] = St[(uint32_t)min(g0 + 4 * u + kk, C - 1) * ur];
+            b[u] = xq[(uint32_t)ci[u] * xstride];
         }
+    };
+    auto mma = [&](int g0, const T2 (&a)[kU], const T2 (&b)[kU]) {
 #pragma unroll
-        for (int u = 0; u < kU; ++u)
-            if (c0 + 4 * u < C) acc.mma(a[u], b[u]);
+        for (int u = 0; u < kU; ++u) {
+            T2 bu = b[u];
+            if (g0 + 4 * u + kk >= C) bu = czero2<P>();
+            acc.mma(conjif<CJ, P>(a[u]), bu);
+        }
+    };
+    load_idx(0, ci0);
+    load_ops(0, ci0, a0, b0);
+    load_idx(4 * kU, ci1);
+    for (int c0 = 0; c0 < C; c0 += 8 * kU) {
+        load_ops(c0 + 4 * kU, ci1, a1, b1);
+        load_idx(c0 + 8 * kU, ci0);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(c0, a0, b0);
+        if (c0 + 4 * kU >= C) break;
+        load_ops(c0 + 8 * kU, ci0, a0, b0);
+        load_idx(c0 + 12 * kU, ci1);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(c0 + 4 * kU, a1, b1);
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -1426,6 +1454,55 @@ int32_t nca_h2_matvec_device(nca_h2* h2, int32_t op, int64_t nrhs, const void* x
         NCA_REQUIRE(h2 && (nrhs == 0 || (x && y)), NCA_ERR_ARGUMENT, "nca_h2_matvec_device: invalid argument");
         set_device(h2->ctx);
         matvec_device(h2, op, nrhs, static_cast<const double2*>(x), ldx, static_cast<double2*>(y), ldy, alpha, beta);
+    });
+}
+
+int32_t nca_h2_graph_create(nca_h2* h2, int32_t op, int64_t nrhs, const void* x, int64_t ldx, void* y,
+                            int64_t ldy, const double* alpha, const double* beta, nca_h2_graph** out) {
+    return guarded([&] {
+        NCA_REQUIRE(h2 && out && nrhs > 0 && x && y, NCA_ERR_ARGUMENT, "nca_h2_graph_create: invalid argument");
+        set_device(h2->ctx);
+        hipStream_t st = h2->ctx->stream;
+        // everything that allocates or converts runs before the capture
+        if (h2->precision == NCA_PREC_MIXED) ensure_mixed(h2);
+        ensure_workspace(h2, nrhs);
+        NCA_HIP(hipStreamSynchronize(st));
+        auto g = std::make_unique<nca_h2_graph>();
+        g->h2 = h2;
+        g->precision = h2->precision;
+        NCA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            matvec_device(h2, op, nrhs, static_cast<const double2*>(x), ldx, static_cast<double2*>(y), ldy, alpha,
+                          beta);
+        } catch (...) {
+            hipGraph_t broken = nullptr;
+            (void)hipStreamEndCapture(st, &broken);
+            if (broken) (void)hipGraphDestroy(broken);
+            throw;
+        }
+        NCA_HIP(hipStreamEndCapture(st, &g->graph));
+        NCA_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        *out = g.release();
+    });
+}
+
+int32_t nca_h2_graph_launch(nca_h2_graph* g) {
+    return guarded([&] {
+        NCA_REQUIRE(g && g->exec, NCA_ERR_ARGUMENT, "nca_h2_graph_launch: invalid argument");
+        NCA_REQUIRE(g->h2->precision == g->precision, NCA_ERR_ARGUMENT,
+                    "nca_h2_graph_launch: the operator's precision changed since the capture");
+        set_device(g->h2->ctx);
+        NCA_HIP(hipGraphLaunch(g->exec, g->h2->ctx->stream));
+    });
+}
+
+int32_t nca_h2_graph_destroy(nca_h2_graph* g) {
+    return guarded([&] {
+        if (!g) return;
+        set_device(g->h2->ctx);
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g;
     });
 }
 

@@ -8,9 +8,9 @@ from ._abi import (NCA_OP_N, NCA_OP_T, NCA_OP_C, NCA_PREC_FP64, NCA_PREC_MIXED, 
 from ._native import hip_lib, tree_lib, NativeLibraryMissing
 from .runtime import Context, default_context, HelmholtzKernel, HIPKernelMatrix
 from .trees import (TwoNTree, KMeansTree, BlockTree, interactions, fardata, farfield, farpivots, FarData,
-                    Partition)
+                    Partition, forest, forest_partition)
 from .compressors import (ACA, BatchedBottomUp, GeometricPivoting, assemble_level, testbases, trialbases,
                           assemble_couplingstore, concat_ranges)
-from .nca import PetrovGalerkinNCA, PetrovGalerkinNCACore, H2Workspace, mul, transpose, adjoint, \
-    ConcurrentWorkspaceError, h2desc
+from .nca import PetrovGalerkinNCA, PetrovGalerkinNCACore, H2Workspace, H2Graph, BatchedNCA, mul, transpose, \
+    adjoint, ConcurrentWorkspaceError, h2desc
 from . import geometry

@@ -74,6 +74,9 @@ SIGNATURES = {
     "nca_h2_matvec": (I32, [P, I32, I64, P, I64, P, I64, P, P]),
     "nca_h2_matvec_device": (I32, [P, I32, I64, P, I64, P, I64, P, P]),
     "nca_h2_matvec_profile": (I32, [P, I32, I64, P, I64, P, I64, P, P, P]),
+    "nca_h2_graph_create": (I32, [P, I32, I64, P, I64, P, I64, P, P, PP]),
+    "nca_h2_graph_launch": (I32, [P]),
+    "nca_h2_graph_destroy": (I32, [P]),
     "nca_h2_get_stats": (I32, [P, P]),
 }
 

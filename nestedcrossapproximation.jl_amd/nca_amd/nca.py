@@ -72,14 +72,14 @@ class PetrovGalerkinNCACore:
     """Read-only part: the device operator plus the host-side plans and factors."""
 
     def __init__(self, kernel, testpoints, trialpoints, tree, testcompressor, trialcompressor, eta=1.0,
-                 maxrank=50, precision=_abi.NCA_PREC_FP64, ctx=None, verbose=False):
+                 maxrank=50, precision=_abi.NCA_PREC_FP64, ctx=None, verbose=False, partition=None):
         self.ctx = ctx or default_context()
         self.kernel = kernel
         self.tree = tree
         self.maxrank = maxrank
         timings = {}
         t0 = time.perf_counter()
-        self.partition = _trees.interactions(tree, eta)
+        self.partition = _trees.interactions(tree, eta) if partition is None else partition
         self.testfardata, self.trialfardata = _trees.fardata(tree, self.partition)
         timings["partition"] = time.perf_counter() - t0
         self.farmatrix = HIPKernelMatrix(kernel, testpoints, trialpoints, self.ctx)
@@ -127,10 +127,40 @@ class PetrovGalerkinNCACore:
         call(hip_lib().nca_h2_matvec_device(self.handle, int(op), int(nrhs), ctypes.c_void_p(xptr), int(ldx),
                                             ctypes.c_void_p(yptr), int(ldy), _p(al), _p(be)))
 
+    def graph(self, op, nrhs, xptr, ldx, yptr, ldy, alpha=1.0, beta=0.0):
+        """Capture matvec_device(op, nrhs, ...) on these device buffers as a
+        hipGraph; ``launch()`` replays it on the context stream."""
+        return H2Graph(self, op, nrhs, xptr, ldx, yptr, ldy, alpha, beta)
+
     def __del__(self):
         try:
             if getattr(self, "handle", None):
                 hip_lib().nca_h2_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class H2Graph:
+    """One captured device sweep of a core (bound to its device buffers and to
+    the precision at capture time)."""
+
+    def __init__(self, core, op, nrhs, xptr, ldx, yptr, ldy, alpha=1.0, beta=0.0):
+        al = np.array([complex(alpha).real, complex(alpha).imag])
+        be = np.array([complex(beta).real, complex(beta).imag])
+        g = ctypes.c_void_p()
+        call(hip_lib().nca_h2_graph_create(core.handle, int(op), int(nrhs), ctypes.c_void_p(xptr), int(ldx),
+                                           ctypes.c_void_p(yptr), int(ldy), _p(al), _p(be), ctypes.byref(g)))
+        self.core = core  # the graph must not outlive its operator
+        self.handle = g
+
+    def launch(self):
+        call(hip_lib().nca_h2_graph_launch(self.handle))
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                hip_lib().nca_h2_graph_destroy(self.handle)
                 self.handle = None
         except Exception:
             pass
@@ -207,3 +237,43 @@ def PetrovGalerkinNCA(kernel, testpoints, trialpoints, tree, testcompressor, tri
     """Assemble the H^2 matrix and return it as an H2Workspace (petrovgalerkinnca.jl:327-331)."""
     return H2Workspace(PetrovGalerkinNCACore(kernel, testpoints, trialpoints, tree, testcompressor,
                                              trialcompressor, **kw))
+
+
+class BatchedNCA:
+    """Independent problems as one block-diagonal H^2 operator.
+
+    Problem p has its own test and trial points and cluster trees; the trees
+    become one forest (trees.forest) whose partition holds each problem's near
+    and far pairs only. One compressor launch per forest level then serves all
+    problems, and one matvec sweep (optionally one captured hipGraph) applies
+    all of them. Vectors are stacked problem by problem: rows
+    [testoffsets[p], testoffsets[p + 1]) of y and [trialoffsets[p],
+    trialoffsets[p + 1]) of x belong to problem p.
+    """
+
+    def __init__(self, kernel, testpoints, trialpoints, testtrees, trialtrees, testcompressor, trialcompressor,
+                 eta=1.0, **kw):
+        if not (len(testpoints) == len(trialpoints) == len(testtrees) == len(trialtrees)) or not testtrees:
+            raise ValueError("BatchedNCA: one test/trial point set and tree per problem")
+        self.nproblems = len(testtrees)
+        ttest, self.testmaps, self.testoffsets = _trees.forest(testtrees)
+        ttrial, self.trialmaps, self.trialoffsets = _trees.forest(trialtrees)
+        parts = [_trees.interactions(_trees.BlockTree(a, b), eta) for a, b in zip(testtrees, trialtrees)]
+        self.partitions = parts
+        part = _trees.forest_partition(self.testmaps, self.trialmaps, parts, ttest.nnodes)
+        self.tree = _trees.BlockTree(ttest, ttrial)
+        tp = np.concatenate([np.asarray(x, np.float64) for x in testpoints], axis=0)
+        sp = np.concatenate([np.asarray(x, np.float64) for x in trialpoints], axis=0)
+        self.core = PetrovGalerkinNCACore(kernel, tp, sp, self.tree, testcompressor, trialcompressor, eta=eta,
+                                          partition=part, **kw)
+        self.workspace = H2Workspace(self.core)
+
+    def split(self, y, side="test"):
+        off = self.testoffsets if side == "test" else self.trialoffsets
+        return [y[off[p]:off[p + 1]] for p in range(self.nproblems)]
+
+    def stack(self, xs):
+        return np.concatenate([np.asarray(x) for x in xs], axis=0)
+
+    def __matmul__(self, x):
+        return self.workspace @ x

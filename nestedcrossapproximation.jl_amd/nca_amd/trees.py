@@ -201,3 +201,82 @@ def farpivots(selftree, othertree, fd, maxrank):
     L.ncat_csr_export(h, _ptr(ptr), _ptr(idx))
     L.ncat_csr_free(h)
     return ptr, idx
+
+
+def forest(trees):
+    """One cluster tree over independent problems: a new root whose children are
+    the roots of ``trees`` (each over its own point set). Points, positions and
+    node ids of problem p are offset; ids stay level by level (level l + 1 of
+    the forest holds level l of every tree, problem by problem), so one
+    compressor launch per forest level serves all problems. Returns
+    (tree, nodemaps, pointoffsets): nodemaps[p][i] = forest id of node i of
+    tree p; problem p owns points (and positions) [pointoffsets[p],
+    pointoffsets[p + 1])."""
+    P = len(trees)
+    poff = np.zeros(P + 1, np.int64)
+    np.cumsum([t.npoints for t in trees], out=poff[1:])
+    maxlev = max(int(t.level.max()) for t in trees)
+    # forest ids: root, then for each tree level all trees' nodes of that level
+    nodemaps = [np.empty(t.nnodes, np.int64) for t in trees]
+    nid = 1
+    for lev in range(maxlev + 1):
+        for p, t in enumerate(trees):
+            nodes = np.nonzero(t.level == lev)[0]
+            nodemaps[p][nodes] = np.arange(nid, nid + len(nodes))
+            nid += len(nodes)
+    nn = nid
+    parent = np.full(nn, -1, np.int64)
+    level = np.zeros(nn, np.int64)
+    lo = np.zeros(nn, np.int64)
+    hi = np.zeros(nn, np.int64)
+    center = np.zeros((nn, 3), np.float64)
+    radius = np.zeros(nn, np.float64)
+    nchild = np.zeros(nn, np.int64)
+    lo[0], hi[0] = 0, poff[-1]
+    nchild[0] = P
+    pts = np.concatenate([t.points for t in trees], axis=0)
+    perm = np.concatenate([t.perm + poff[p] for p, t in enumerate(trees)])
+    center[0] = pts.mean(axis=0) if len(pts) else 0.0
+    radius[0] = float(np.max(np.linalg.norm(pts - center[0], axis=1))) if len(pts) else 0.0
+    for p, t in enumerate(trees):
+        m = nodemaps[p]
+        parent[m] = np.where(t.parent >= 0, m[np.maximum(t.parent, 0)], 0)
+        level[m] = t.level + 1
+        lo[m] = t.lo + poff[p]
+        hi[m] = t.hi + poff[p]
+        center[m] = t.center
+        radius[m] = t.radius
+        nchild[m] = np.diff(t.childptr)
+    childptr = np.zeros(nn + 1, np.int64)
+    np.cumsum(nchild, out=childptr[1:])
+    children = np.empty(max(nn - 1, 0), np.int64)
+    children[0:P] = [nodemaps[p][0] for p in range(P)]
+    for p, t in enumerate(trees):
+        m = nodemaps[p]
+        for i in range(t.nnodes):
+            c = t.children[t.childptr[i]:t.childptr[i + 1]]
+            if len(c):
+                children[childptr[m[i]]:childptr[m[i] + 1]] = m[c]
+    tree = ClusterTree(pts, parent, level, lo, hi, childptr, children, perm, center, radius, trees[0].kind)
+    return tree, nodemaps, poff
+
+
+def forest_partition(testmaps, trialmaps, partitions, ntest):
+    """The block-diagonal partition of a forest: problem p's near and far pairs
+    with forest ids (no pairs across problems, none at the forest root)."""
+    near = [[] for _ in range(ntest)]
+    far = [[] for _ in range(ntest)]
+    for tm, sm, part in zip(testmaps, trialmaps, partitions):
+        for i in range(len(tm)):
+            near[tm[i]] = sm[part.nearnodes[part.nearptr[i]:part.nearptr[i + 1]]].tolist()
+            far[tm[i]] = sm[part.farnodes[part.farptr[i]:part.farptr[i + 1]]].tolist()
+
+    def csr(lists):
+        ptr = np.zeros(ntest + 1, np.int64)
+        np.cumsum([len(l) for l in lists], out=ptr[1:])
+        idx = np.fromiter((v for l in lists for v in l), np.int64, count=int(ptr[-1]))
+        return ptr, idx
+
+    nearptr, nearnodes = csr(near)
+    farptr, farnodes = csr(far)
+    return Partition(nearptr, nearnodes, farptr, farnodes)

@@ -85,3 +85,39 @@ def bitwise_equal(a, b):
     if a.shape != b.shape or a.dtype != b.dtype:
         return False
     return np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+class DeviceBuffers:
+    """Device memory through the HIP runtime (test plumbing). hipMemcpy runs on
+    the null stream: synchronize the context stream before reading results."""
+
+    def __init__(self):
+        import ctypes
+        self.ct = ctypes
+        self.lib = ctypes.CDLL("libamdhip64.so")
+        self.lib.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.lib.hipFree.argtypes = [ctypes.c_void_p]
+        self.held = []
+
+    def upload(self, arr):
+        arr = np.asfortranarray(arr)
+        p = self.ct.c_void_p()
+        assert self.lib.hipMalloc(self.ct.byref(p), self.ct.c_size_t(max(arr.nbytes, 16))) == 0
+        self.held.append(p.value)
+        self.copy_in(p.value, arr)
+        return p.value
+
+    def copy_in(self, dst, arr):
+        arr = np.asfortranarray(arr)
+        assert self.lib.hipMemcpy(self.ct.c_void_p(dst), arr.ctypes.data_as(self.ct.c_void_p), arr.nbytes, 1) == 0
+
+    def download(self, src, shape):
+        out = np.zeros(shape, np.complex128, order="F")
+        assert self.lib.hipMemcpy(out.ctypes.data_as(self.ct.c_void_p), self.ct.c_void_p(src), out.nbytes, 2) == 0
+        return out
+
+    def free(self):
+        for p in self.held:
+            self.lib.hipFree(self.ct.c_void_p(p))
+        self.held = []

@@ -127,3 +127,33 @@ def test_batched_bottom_up_constructor():
         C.BatchedBottomUp(compressor=C.ACA(1e-3))
     b = C.BatchedBottomUp(C.GeometricPivoting(), C.ACA(1e-3))
     assert b.compressor.tol == 1e-3
+
+
+def test_forest_is_block_diagonal():
+    """A forest of independent trees: nested ranges, level-by-level ids, and
+    near + far blocks (with their ancestors) cover every point pair of each
+    problem exactly once and no pair across problems."""
+    from nca_amd import trees as T
+    sets = [geometry.fibonacci_sphere(700), geometry.ellipsoid(500, (1.5, 0.8, 1.1)),
+            geometry.ellipsoid(300, (0.6, 1.3, 0.9))]
+    trees = [nca_amd.TwoNTree(p, minvalues=32) for p in sets]
+    f, maps, off = T.forest(trees)
+    assert f.nnodes == 1 + sum(t.nnodes for t in trees)
+    assert np.all(np.diff(f.level) >= 0)
+    for n in range(f.nnodes):
+        ch = f.childrenof(n)
+        if len(ch):
+            assert f.lo[ch].min() == f.lo[n] and f.hi[ch].max() == f.hi[n]
+            assert np.all(f.parent[ch] == n)
+    parts = [T.interactions(T.BlockTree(t, t)) for t in trees]
+    P = T.forest_partition(maps, maps, parts, f.nnodes)
+    n = int(off[-1])
+    cover = np.zeros((n, n), np.int8)
+    for t in range(f.nnodes):
+        for ptr, nodes in ((P.nearptr, P.nearnodes), (P.farptr, P.farnodes)):
+            for s in nodes[ptr[t]:ptr[t + 1]]:
+                cover[np.ix_(f.perm[f.lo[t]:f.hi[t]], f.perm[f.lo[s]:f.hi[s]])] += 1
+    blk = np.zeros((n, n), np.int8)
+    for p in range(len(sets)):
+        blk[off[p]:off[p + 1], off[p]:off[p + 1]] = 1
+    assert np.array_equal(cover, blk)
