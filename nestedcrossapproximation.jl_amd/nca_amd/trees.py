@@ -263,20 +263,18 @@ def forest(trees):
 
 def forest_partition(testmaps, trialmaps, partitions, ntest):
     """The block-diagonal partition of a forest: problem p's near and far pairs
-    with forest ids (no pairs across problems, none at the forest root)."""
-    near = [[] for _ in range(ntest)]
-    far = [[] for _ in range(ntest)]
-    for tm, sm, part in zip(testmaps, trialmaps, partitions):
-        for i in range(len(tm)):
-            near[tm[i]] = sm[part.nearnodes[part.nearptr[i]:part.nearptr[i + 1]]].tolist()
-            far[tm[i]] = sm[part.farnodes[part.farptr[i]:part.farptr[i + 1]]].tolist()
+    with forest ids (no pairs across problems, none at the forest root); each
+    node keeps its problem's order of near and far nodes."""
 
-    def csr(lists):
+    def merged(ptrname, nodename):
+        rows = np.concatenate([np.repeat(tm, np.diff(getattr(pt, ptrname)))
+                               for tm, pt in zip(testmaps, partitions)])
+        cols = np.concatenate([sm[getattr(pt, nodename)] for sm, pt in zip(trialmaps, partitions)])
+        order = np.argsort(rows, kind="stable")
         ptr = np.zeros(ntest + 1, np.int64)
-        np.cumsum([len(l) for l in lists], out=ptr[1:])
-        idx = np.fromiter((v for l in lists for v in l), np.int64, count=int(ptr[-1]))
-        return ptr, idx
+        np.cumsum(np.bincount(rows, minlength=ntest), out=ptr[1:])
+        return ptr, cols[order].astype(np.int64)
 
-    nearptr, nearnodes = csr(near)
-    farptr, farnodes = csr(far)
+    nearptr, nearnodes = merged("nearptr", "nearnodes")
+    farptr, farnodes = merged("farptr", "farnodes")
     return Partition(nearptr, nearnodes, farptr, farnodes)
