@@ -103,6 +103,7 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="copy this many MiB device to device first (a kernel of known traffic for counter scaling)")
     ap.add_argument("--workload", choices=["sphere1m", "batch64"], default="sphere1m")
+    ap.add_argument("--autotune", type=int, default=1, help="run nca_h2_autotune per precision before timing")
     args = ap.parse_args()
     if args.workload == "batch64":
         if args.gpus != 1:
@@ -189,11 +190,15 @@ def main():
         hip.d2h(Y, dy)
         return ms, R * args.steps / t_sweeps, prof, Y
 
-    results = {}
-    core.set_precision(nca_amd.NCA_PREC_FP64)
-    results["fp64"] = measure("FP64")
-    core.set_precision(nca_amd.NCA_PREC_MIXED)
-    results["mixed"] = measure("MIXED")
+    results, tuning = {}, {}
+    for label, prec in (("fp64", nca_amd.NCA_PREC_FP64), ("mixed", nca_amd.NCA_PREC_MIXED)):
+        core.set_precision(prec)
+        if args.autotune:
+            t0 = time.perf_counter()
+            tuning[label] = core.autotune(reps=3)
+            tuning[label]["seconds"] = round(time.perf_counter() - t0, 3)
+            log(f"autotune {label}: {tuning[label]}")
+        results[label] = measure(label.upper())
     Y64, Ymix = results["fp64"][3], results["mixed"][3]
     mixed_vs_fp64 = float(np.linalg.norm(Ymix - Y64) / np.linalg.norm(Y64))
     log(f"MIXED vs FP64 GPU result: rel err {mixed_vs_fp64:.3e}")
@@ -260,6 +265,7 @@ def main():
         "host_prep_s": {"tree": round(t_tree, 3), "partition": round(tm["partition"], 3),
                         "phase1": round(tm["phase1"], 3), "h2_create": round(tm["h2_create"], 3)},
         "operator": {k_: int(v) for k_, v in st.items()},
+        "autotune": tuning,
         "roofline": roofline,
     }
 

@@ -199,6 +199,23 @@ int32_t nca_h2_graph_create(nca_h2* h2, int32_t op, int64_t nrhs, const void* x,
 int32_t nca_h2_graph_launch(nca_h2_graph* graph);
 int32_t nca_h2_graph_destroy(nca_h2_graph* graph);
 
+/* Autotuner, no counterpart in the reference: times the forward coupling and
+ * the leaf expansion + near field over 16 right-hand sides in the current
+ * precision for every launch choice (MFMA complex products with three or four
+ * real products; tile orders; workgroup-to-XCD orders), `reps` timed sweeps
+ * each (median), and keeps the fastest per stage. Choices are stored in the
+ * operator and used by every later product; results are unchanged within the
+ * precision's tolerance. out may be NULL. */
+typedef struct {
+    int32_t near_k3, leaf_order, near_xcd_order;
+    int32_t couple_k3, couple_tile_order, couple_xcd_order;
+    int32_t candidates;
+    int32_t reserved;
+    double near_ms, couple_ms;             /* stage times of the chosen configuration */
+    double near_ms_start, couple_ms_start; /* stage times of the configuration before tuning */
+} nca_h2_tuning;
+int32_t nca_h2_autotune(nca_h2* h2, int32_t reps, nca_h2_tuning* out);
+
 /* Measurement aid, no counterpart in the reference: one product as
  * nca_h2_matvec_device (nrhs <= 16) with HIP events recorded on the context
  * stream between the stages; stage_ms[0..5] = gather + clear, leaf projection,

@@ -51,7 +51,10 @@ struct SideDev {
     int64_t ntiles = 0;
     DevBuf<int32_t> tile_leaf, tile_row0;
     // host copies used during creation
-    std::vector<int32_t> h_lo, h_hi, h_cptr, h_perm, h_bnodes, h_leaves, h_leafslot;
+    std::vector<int32_t> h_lo, h_hi, h_cptr, h_perm, h_bnodes, h_leaves, h_leafslot, h_leaves_node_order;
+    // host copies of the merged near ranges (to reorder the leaves)
+    std::vector<int64_t> h_nrptr;
+    std::vector<int32_t> h_nrlo, h_nrhi;
     int64_t basis_entries = 0, transfer_entries = 0;
 };
 
@@ -84,13 +87,17 @@ struct nca_h2 {
     // (cos, sin)(n pi / 128), n = 0..255, for the FP64 near field
     nca::DevBuf<double2> sctab;
     nca_h2_stats stats{};
-    // Complex products on MFMA with three real products (Karatsuba) instead of
-    // four, per precision (environment overrides NCA_MFMA_K3, NCA_MIXED_K3).
-    // Measured on the 1M-point sphere, 16 RHS: three are faster in both; in
-    // FP32 they raise the MIXED error against FP64 from 1.0e-6 to 1.5e-6.
-    int tune_k3 = 1, tune_k3_mixed = 1;
-    // XCD-contiguous work order (environment NCA_ORDER_NEAR, NCA_ORDER_COUPLE).
-    int tune_order_near = 0, tune_order_couple = 0;
+    // Launch choices, set by nca_h2_autotune (defaults measured on the
+    // 1M-point sphere with 16 RHS; environment overrides at creation):
+    // near_k3 / couple_k3 [precision]: complex products on MFMA with three
+    //   real products (Karatsuba) instead of four (NCA_MFMA_K3, NCA_MIXED_K3);
+    //   in FP32 three raise the MIXED error against FP64 from 1.0e-6 to 1.5e-6;
+    // near_xcd / couple_xcd: work_block order (NCA_ORDER_NEAR, NCA_ORDER_COUPLE);
+    // leaf_order: order_leaves mode (NCA_LEAF_ORDER);
+    // couple_order: order_coupling_tiles mode (NCA_COUPLE_ORDER).
+    int near_k3[2] = {1, 1}, couple_k3[2] = {1, 1};
+    int near_xcd = 0, couple_xcd = 0, leaf_order = 0, couple_order = 0;
+    std::vector<int32_t> h_ncols;  // host copy: columns of every test node's coupling row
 };
 
 // One captured device sweep (see nca_h2_graph_create).
@@ -893,7 +900,7 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
     const T2* S = O::coupling(h);
     if (FWD) {
         if (NR == 16 && h->f_ntiles > 0) {
-            const bool k3 = MIX ? h->tune_k3_mixed : h->tune_k3;
+            const bool k3 = h->couple_k3[MIX ? 1 : 0] != 0;
             auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true> : k_couple_fwd_mfma<P, CJ, false>;
             // MIXED: the source coefficients of this chunk as a compact c32 copy
             const T2* xq0;
@@ -909,7 +916,7 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
             }
             hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
                                h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
-                               h->f_colcoef.p, dst.cptr.p, xq0, xstride, R, q0, yh, h->tune_order_couple);
+                               h->f_colcoef.p, dst.cptr.p, xq0, xstride, R, q0, yh, h->couple_xcd);
         } else if (NR != 16 && h->coupling->nentries > 0) {
             hipLaunchKernelGGL((k_couple_fwd<NR, CJ, T2>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->f_base.p,
                                h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
@@ -938,12 +945,12 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
         // conj(G) for the adjoint flips the imaginary scale
         const double ims = ((h->sign < 0) != CJ) ? -kInvFourPi : kInvFourPi;
         const NearK nk{h->k, h->k * (0.5 / M_PI), ims};
-        const bool k3 = MIX ? h->tune_k3_mixed : h->tune_k3;
+        const bool k3 = h->near_k3[MIX ? 1 : 0] != 0;
         auto kern = k3 ? k_leaf_out_mfma<P, FWD, CJ, true> : k_leaf_out_mfma<P, FWD, CJ, false>;
         hipLaunchKernelGGL(kern, dim3(blocks_for(dst.ntiles, 4)), dim3(256), 0, st, dst.tile_leaf.p,
                            dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst),
                            dst.lo.p, dst.hi.p, dst.perm.p, dst.cptr.p, dst.pts.p, dst.nrptr.p, dst.nrlo.p, dst.nrhi.p,
-                           src.pts.p, yh, xs, R, q0, nk, h->sctab.p, alpha, beta, y, ldy, h->tune_order_near);
+                           src.pts.p, yh, xs, R, q0, nk, h->sctab.p, alpha, beta, y, ldy, h->near_xcd);
     } else if (NR != 16 && dst.nleaves > 0) {
         hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ, T2, T2>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st,
                            dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p,
@@ -1185,16 +1192,20 @@ void build_leaves(SideDev& sd, hipStream_t st) {
     }
     sd.nleaves = (int64_t)leaves.size();
     sd.h_leaves = leaves;
+    sd.h_leaves_node_order = leaves;
 }
 
 // Order of the leaves (and their 16-row tiles) in the near-field launch, from
 // the merged near ranges: 0 node id, 1 point order, 2 decreasing near-field
 // work (longest first). Uploads leaves, leafbasis and the tiles.
-void order_leaves(SideDev& sd, const std::vector<int32_t>& leafslot, const std::vector<int64_t>& mptr,
-                  const std::vector<int32_t>& mlo, const std::vector<int32_t>& mhi, int mode, hipStream_t st) {
+void order_leaves(SideDev& sd, int mode, hipStream_t st) {
+    const std::vector<int32_t>& leafslot = sd.h_leafslot;
+    const std::vector<int64_t>& mptr = sd.h_nrptr;
+    const std::vector<int32_t>& mlo = sd.h_nrlo;
+    const std::vector<int32_t>& mhi = sd.h_nrhi;
     std::vector<int64_t> idx(sd.nleaves);
     for (int64_t i = 0; i < sd.nleaves; ++i) idx[i] = i;
-    const std::vector<int32_t>& lv = sd.h_leaves;
+    const std::vector<int32_t>& lv = sd.h_leaves_node_order;
     if (mode == 1)
         std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return sd.h_lo[lv[a]] < sd.h_lo[lv[b]]; });
     if (mode == 2) {
@@ -1219,6 +1230,63 @@ void order_leaves(SideDev& sd, const std::vector<int32_t>& leafslot, const std::
     sd.ntiles = (int64_t)tl.size();
     sd.tile_leaf.upload(tl, st);
     sd.tile_row0.upload(tr, st);
+}
+
+// Launch order of the forward-coupling tiles (16 rows of one test cluster):
+// 0 node id, 1 depth-first point order (nested ranges after their parent),
+// 2 decreasing row work (columns).
+void order_coupling_tiles(nca_h2* h, int mode, hipStream_t st) {
+    const int64_t ntest = h->test.nn;
+    const std::vector<int32_t>& ncols = h->h_ncols;
+    std::vector<int64_t> torder(ntest);
+    for (int64_t t = 0; t < ntest; ++t) torder[t] = t;
+    if (mode == 1)
+        std::sort(torder.begin(), torder.end(), [&](int64_t a, int64_t b) {
+            const SideDev& sd = h->test;
+            if (sd.h_lo[a] != sd.h_lo[b]) return sd.h_lo[a] < sd.h_lo[b];
+            if (sd.h_hi[a] != sd.h_hi[b]) return sd.h_hi[a] > sd.h_hi[b];
+            return a < b;
+        });
+    if (mode == 2)
+        std::stable_sort(torder.begin(), torder.end(), [&](int64_t a, int64_t b) { return ncols[a] > ncols[b]; });
+    std::vector<int32_t> tnode, trow;
+    for (const int64_t t : torder)
+        if (ncols[t] > 0)
+            for (int r0 = 0; r0 < h->coupling->testrank[t]; r0 += 16) {
+                tnode.push_back((int32_t)t);
+                trow.push_back(r0);
+            }
+    h->f_ntiles = (int64_t)tnode.size();
+    h->f_tile_node.upload(tnode, st);
+    h->f_tile_row0.upload(trow, st);
+}
+
+// Stage times (ms) of nreps profiled FWD sweeps over 16 RHS of zeros: the
+// median of the coupling and of the leaf expansion + near field stage.
+void time_stages(nca_h2* h, const double2* x, double2* y, int nreps, double& couple_ms, double& near_ms) {
+    std::vector<double> c, n;
+    StageEvents evs;
+    for (auto& e : evs.ev) NCA_HIP(hipEventCreate(&e));
+    evs.on = true;
+    try {
+        for (int r = 0; r < nreps; ++r) {
+            matvec_device(h, NCA_OP_N, 16, x, h->N, y, h->M, nullptr, nullptr, &evs);
+            NCA_HIP(hipEventSynchronize(evs.ev[6]));
+            float a = 0.f, b = 0.f;
+            NCA_HIP(hipEventElapsedTime(&a, evs.ev[3], evs.ev[4]));
+            NCA_HIP(hipEventElapsedTime(&b, evs.ev[5], evs.ev[6]));
+            c.push_back(a);
+            n.push_back(b);
+        }
+    } catch (...) {
+        for (auto& e : evs.ev) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (auto& e : evs.ev) (void)hipEventDestroy(e);
+    std::sort(c.begin(), c.end());
+    std::sort(n.begin(), n.end());
+    couple_ms = c[c.size() / 2];
+    near_ms = n[n.size() / 2];
 }
 
 }  // namespace nca
@@ -1252,10 +1320,12 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
             }
             h->sctab.upload(tab, st);
         }
-        if (const char* v = getenv("NCA_MFMA_K3")) h->tune_k3 = atoi(v) != 0;
-        if (const char* v = getenv("NCA_MIXED_K3")) h->tune_k3_mixed = atoi(v) != 0;
-        if (const char* v = getenv("NCA_ORDER_NEAR")) h->tune_order_near = atoi(v);
-        if (const char* v = getenv("NCA_ORDER_COUPLE")) h->tune_order_couple = atoi(v);
+        if (const char* v = getenv("NCA_MFMA_K3")) h->near_k3[0] = h->couple_k3[0] = atoi(v) != 0;
+        if (const char* v = getenv("NCA_MIXED_K3")) h->near_k3[1] = h->couple_k3[1] = atoi(v) != 0;
+        if (const char* v = getenv("NCA_ORDER_NEAR")) h->near_xcd = atoi(v);
+        if (const char* v = getenv("NCA_ORDER_COUPLE")) h->couple_xcd = atoi(v);
+        if (const char* v = getenv("NCA_LEAF_ORDER")) h->leaf_order = atoi(v);
+        if (const char* v = getenv("NCA_COUPLE_ORDER")) h->couple_order = atoi(v);
         build_side(h->test, st, d->M, d->nn_test, d->test_perm, d->test_lo, d->test_hi, d->test_cptr, test,
                    d->nbasis_test, d->basis_test_nodes, d->basis_test_off, d->basis_test, true, d->nlev_test,
                    d->tt_levptr, d->tt_levidcs, d->tt_chdptr, d->tt_chds, d->tt_off, d->tt_blocks);
@@ -1312,32 +1382,8 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
                     }
                 ncols[t] = (int32_t)C;
             }
-            // launch order of the test clusters: 0 node id, 1 depth-first point
-            // order, 2 decreasing row work
-            std::vector<int64_t> torder(ntest);
-            for (int64_t t = 0; t < ntest; ++t) torder[t] = t;
-            int cmode = 0;
-            if (const char* v = getenv("NCA_COUPLE_ORDER")) cmode = atoi(v);
-            if (cmode == 1)
-                std::sort(torder.begin(), torder.end(), [&](int64_t a, int64_t b) {
-                    const SideDev& sd = h->test;
-                    if (sd.h_lo[a] != sd.h_lo[b]) return sd.h_lo[a] < sd.h_lo[b];
-                    if (sd.h_hi[a] != sd.h_hi[b]) return sd.h_hi[a] > sd.h_hi[b];
-                    return a < b;
-                });
-            if (cmode == 2)
-                std::stable_sort(torder.begin(), torder.end(),
-                                 [&](int64_t a, int64_t b) { return ncols[a] > ncols[b]; });
-            std::vector<int32_t> tnode, trow;
-            for (const int64_t t : torder)
-                if (ncols[t] > 0)
-                    for (int r0 = 0; r0 < coupling->testrank[t]; r0 += 16) {
-                        tnode.push_back((int32_t)t);
-                        trow.push_back(r0);
-                    }
-            h->f_ntiles = (int64_t)tnode.size();
-            h->f_tile_node.upload(tnode, st);
-            h->f_tile_row0.upload(trow, st);
+            h->h_ncols = ncols;
+            order_coupling_tiles(h.get(), h->couple_order, st);
             h->f_base.upload(base, st);
             h->f_colbase.upload(colbase, st);
             h->f_ncols.upload(ncols, st);
@@ -1407,12 +1453,14 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
                 dsd.nrptr.upload(mptr, s);
                 dsd.nrlo.upload(mlo, s);
                 dsd.nrhi.upload(mhi, s);
-                int mode = 0;
-                if (const char* v = getenv("NCA_LEAF_ORDER")) mode = atoi(v);
-                order_leaves(dsd, dsd.h_leafslot, mptr, mlo, mhi, mode, s);
+                dsd.h_nrptr = std::move(mptr);
+                dsd.h_nrlo = std::move(mlo);
+                dsd.h_nrhi = std::move(mhi);
             };
             merge(h->test, h->trial, np, nn32, st);
             merge(h->trial, h->test, tp, tn, st);
+            order_leaves(h->test, h->leaf_order, st);
+            order_leaves(h->trial, h->leaf_order, st);
             h->stats.near_pairs = nnz;
             h->stats.near_entries = near_entries;
             h->stats.plan_bytes += nnz * 12 + (ntest + ntrial) * 8;
@@ -1503,6 +1551,89 @@ int32_t nca_h2_graph_destroy(nca_h2_graph* g) {
         if (g->exec) (void)hipGraphExecDestroy(g->exec);
         if (g->graph) (void)hipGraphDestroy(g->graph);
         delete g;
+    });
+}
+
+int32_t nca_h2_autotune(nca_h2* h2, int32_t reps, nca_h2_tuning* out) {
+    return guarded([&] {
+        NCA_REQUIRE(h2 && reps > 0, NCA_ERR_ARGUMENT, "nca_h2_autotune: invalid argument");
+        if (h2->M == 0 || h2->N == 0) {
+            if (out) *out = nca_h2_tuning{};
+            return;
+        }
+        set_device(h2->ctx);
+        hipStream_t st = h2->ctx->stream;
+        const int prec = h2->precision == NCA_PREC_MIXED ? 1 : 0;
+        DevBuf<double2> x((size_t)std::max<int64_t>(h2->N * 16, 1)), y((size_t)std::max<int64_t>(h2->M * 16, 1));
+        hipLaunchKernelGGL(k_fill, dim3(blocks_for((int64_t)x.n, 256)), dim3(256), 0, st, x.p, (int64_t)x.n);
+        hipLaunchKernelGGL(k_fill, dim3(blocks_for((int64_t)y.n, 256)), dim3(256), 0, st, y.p, (int64_t)y.n);
+        double c0 = 0, n0 = 0;
+        time_stages(h2, x.p, y.p, reps, c0, n0);  // warm-up and the starting configuration
+        time_stages(h2, x.p, y.p, reps, c0, n0);
+        nca_h2_tuning r{};
+        r.couple_ms_start = c0;
+        r.near_ms_start = n0;
+        int count = 0;
+        // near field: products x leaf order x workgroup order
+        int bk3 = h2->near_k3[prec], blo = h2->leaf_order, bx = h2->near_xcd;
+        double best = n0;
+        for (int k3 = 0; k3 < 2; ++k3)
+            for (int lo = 0; lo < 3; ++lo)
+                for (int xo = 0; xo < 2; ++xo) {
+                    h2->near_k3[prec] = k3;
+                    h2->near_xcd = xo;
+                    if (h2->leaf_order != lo) {
+                        order_leaves(h2->test, lo, st);
+                        order_leaves(h2->trial, lo, st);
+                        h2->leaf_order = lo;
+                    }
+                    double c, n;
+                    time_stages(h2, x.p, y.p, reps, c, n);
+                    ++count;
+                    if (n < best) { best = n; bk3 = k3; blo = lo; bx = xo; }
+                }
+        h2->near_k3[prec] = bk3;
+        h2->near_xcd = bx;
+        if (h2->leaf_order != blo) {
+            order_leaves(h2->test, blo, st);
+            order_leaves(h2->trial, blo, st);
+            h2->leaf_order = blo;
+        }
+        r.near_ms = best;
+        // coupling: products x tile order x workgroup order
+        int ck3 = h2->couple_k3[prec], co = h2->couple_order, cx = h2->couple_xcd;
+        double cbest = c0;
+        if (h2->f_ntiles > 0)
+            for (int k3 = 0; k3 < 2; ++k3)
+                for (int o = 0; o < 3; ++o)
+                    for (int xo = 0; xo < 2; ++xo) {
+                        h2->couple_k3[prec] = k3;
+                        h2->couple_xcd = xo;
+                        if (h2->couple_order != o) {
+                            order_coupling_tiles(h2, o, st);
+                            h2->couple_order = o;
+                        }
+                        double c, n;
+                        time_stages(h2, x.p, y.p, reps, c, n);
+                        ++count;
+                        if (c < cbest) { cbest = c; ck3 = k3; co = o; cx = xo; }
+                    }
+        h2->couple_k3[prec] = ck3;
+        h2->couple_xcd = cx;
+        if (h2->couple_order != co) {
+            order_coupling_tiles(h2, co, st);
+            h2->couple_order = co;
+        }
+        r.couple_ms = cbest;
+        r.near_k3 = bk3;
+        r.leaf_order = blo;
+        r.near_xcd_order = bx;
+        r.couple_k3 = ck3;
+        r.couple_tile_order = co;
+        r.couple_xcd_order = cx;
+        r.candidates = count;
+        NCA_HIP(hipStreamSynchronize(st));
+        if (out) *out = r;
     });
 }
 

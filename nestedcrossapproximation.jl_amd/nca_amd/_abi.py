@@ -47,6 +47,12 @@ class H2Stats(ctypes.Structure):
         "coeffs_test", "coeffs_trial", "plan_bytes")]
 
 
+class H2Tuning(ctypes.Structure):
+    _fields_ = [(n, I32) for n in ("near_k3", "leaf_order", "near_xcd_order", "couple_k3", "couple_tile_order",
+                                   "couple_xcd_order", "candidates", "reserved")] + \
+               [(n, ctypes.c_double) for n in ("near_ms", "couple_ms", "near_ms_start", "couple_ms_start")]
+
+
 SIGNATURES = {
     "nca_last_error": (ctypes.c_char_p, []),
     "nca_abi_version": (I32, []),
@@ -74,6 +80,7 @@ SIGNATURES = {
     "nca_h2_matvec": (I32, [P, I32, I64, P, I64, P, I64, P, P]),
     "nca_h2_matvec_device": (I32, [P, I32, I64, P, I64, P, I64, P, P]),
     "nca_h2_matvec_profile": (I32, [P, I32, I64, P, I64, P, I64, P, P, P]),
+    "nca_h2_autotune": (I32, [P, I32, P]),
     "nca_h2_graph_create": (I32, [P, I32, I64, P, I64, P, I64, P, P, PP]),
     "nca_h2_graph_launch": (I32, [P]),
     "nca_h2_graph_destroy": (I32, [P]),

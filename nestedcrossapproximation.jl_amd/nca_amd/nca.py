@@ -127,6 +127,13 @@ class PetrovGalerkinNCACore:
         call(hip_lib().nca_h2_matvec_device(self.handle, int(op), int(nrhs), ctypes.c_void_p(xptr), int(ldx),
                                             ctypes.c_void_p(yptr), int(ldy), _p(al), _p(be)))
 
+    def autotune(self, reps=3):
+        """Time the launch choices of the coupling and near-field kernels in the
+        current precision and keep the fastest (nca_h2_autotune)."""
+        t = _abi.H2Tuning()
+        call(hip_lib().nca_h2_autotune(self.handle, int(reps), ctypes.byref(t)))
+        return {n: getattr(t, n) for n, _ in _abi.H2Tuning._fields_ if n != "reserved"}
+
     def graph(self, op, nrhs, xptr, ldx, yptr, ldy, alpha=1.0, beta=0.0):
         """Capture matvec_device(op, nrhs, ...) on these device buffers as a
         hipGraph; ``launch()`` replays it on the context stream."""

@@ -1,4 +1,5 @@
-"""hipGraph-captured sweeps and block-diagonal batching of independent problems.
+"""hipGraph-captured sweeps, block-diagonal batching of independent problems and
+the autotuner.
 
 Graph replays are checked against the eager sweep on every replay, with eager
 work (uploads, other products) between the replays. The batched operator (one
@@ -133,3 +134,30 @@ def test_batched_forest_matches_separate_problems(ctx):
         del g
     finally:
         dev.free()
+
+
+@pytest.mark.parametrize("precision", [nca_amd.NCA_PREC_FP64, nca_amd.NCA_PREC_MIXED])
+def test_autotune_keeps_results(ctx, precision):
+    """The autotuner only changes launch choices: products stay within the
+    precision's tolerance of the oracle, the choice is a timed candidate and
+    the tuned stage times are not above the starting ones."""
+    say(f"autotune precision={precision}")
+    from helpers import sphere_problem
+    p = sphere_problem(4096)
+    core = _core(ctx, p.testpts, tol=p.tol, k=p.k, minvalues=64)
+    core.set_precision(precision)
+    t = core.autotune(reps=2)
+    assert t["candidates"] == 24
+    assert t["near_k3"] in (0, 1) and t["leaf_order"] in (0, 1, 2) and t["near_xcd_order"] in (0, 1)
+    assert t["couple_k3"] in (0, 1) and t["couple_tile_order"] in (0, 1, 2) and t["couple_xcd_order"] in (0, 1)
+    assert t["near_ms"] <= t["near_ms_start"] and t["couple_ms"] <= t["couple_ms_start"]
+    rng = np.random.default_rng(4)
+    x = geometry.complex_gaussian(rng, (p.testpts.shape[0], 16))
+    W = NC.H2Workspace(core)
+    W2 = _core(ctx, p.testpts, tol=p.tol, k=p.k, minvalues=64)
+    W2.set_precision(precision)
+    ref = NC.H2Workspace(W2) @ x
+    bound = 1e-12 if precision == nca_amd.NCA_PREC_FP64 else 1e-5
+    assert relerr(W @ x, ref) <= bound
+    u = geometry.complex_gaussian(rng, (p.testpts.shape[0], 16))
+    assert relerr(NC.adjoint(W) @ u, NC.adjoint(NC.H2Workspace(W2)) @ u) <= bound
