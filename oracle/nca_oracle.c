@@ -469,6 +469,147 @@ static inline cplx ld_c(const double* base, int64_t idx, int conj) {
     return v;
 }
 
+/* Stored near field (see nca_oracle.h). */
+struct orc_near {
+    int64_t nb;          /* test clusters with near interactions */
+    int64_t* node;       /* their 0-based ids */
+    int64_t* rowptr;     /* rows of block b: test positions [rowptr[b], rowptr[b+1]) of values(node) */
+    int64_t* colptr;     /* columns of block b: colidx[colptr[b] .. colptr[b+1]) */
+    int64_t* colidx;     /* 0-based trial point ids */
+    int64_t* boff;       /* block b starts at re + boff[b] (and im + boff[b]) */
+    double *re, *im;
+    int64_t entries;
+};
+
+orc_near* orc_near_assemble(const orc_h2* h) {
+    orc_near* s = (orc_near*)calloc(1, sizeof(orc_near));
+    int64_t nb = 0, ncol = 0, nent = 0;
+    for (int64_t t = 0; t < h->nn_test; ++t)
+        if (h->nearptr[t + 1] > h->nearptr[t] && h->test_hi[t] >= h->test_lo[t]) ++nb;
+    s->nb = nb;
+    s->node = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb + 1));
+    s->rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb + 1));
+    s->colptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb + 1));
+    s->boff = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb + 1));
+    nb = 0;
+    s->colptr[0] = 0;
+    s->boff[0] = 0;
+    for (int64_t t = 0; t < h->nn_test; ++t) {
+        if (!(h->nearptr[t + 1] > h->nearptr[t] && h->test_hi[t] >= h->test_lo[t])) continue;
+        int64_t m = h->test_hi[t] - h->test_lo[t] + 1, n = 0;
+        for (int64_t i = h->nearptr[t] - 1; i < h->nearptr[t + 1] - 1; ++i) {
+            int64_t q = h->nearnodes[i] - 1;
+            n += h->trial_hi[q] - h->trial_lo[q] + 1;
+        }
+        s->node[nb] = t;
+        s->colptr[nb + 1] = s->colptr[nb] + n;
+        s->boff[nb + 1] = s->boff[nb] + m * n;
+        ++nb;
+        ncol += n;
+        nent += m * n;
+    }
+    s->entries = nent;
+    s->colidx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ncol > 0 ? ncol : 1));
+    s->re = (double*)malloc(sizeof(double) * (size_t)(nent > 0 ? nent : 1));
+    s->im = (double*)malloc(sizeof(double) * (size_t)(nent > 0 ? nent : 1));
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t b = 0; b < s->nb; ++b) {
+        int64_t t = s->node[b];
+        int64_t m = h->test_hi[t] - h->test_lo[t] + 1;
+        int64_t c = s->colptr[b];
+        for (int64_t i = h->nearptr[t] - 1; i < h->nearptr[t + 1] - 1; ++i) {
+            int64_t q = h->nearnodes[i] - 1;
+            for (int64_t v = h->trial_lo[q] - 1; v < h->trial_hi[q]; ++v) s->colidx[c++] = h->trial_perm[v] - 1;
+        }
+        int64_t n = s->colptr[b + 1] - s->colptr[b];
+        double* re = s->re + s->boff[b];
+        double* im = s->im + s->boff[b];
+        for (int64_t j = 0; j < n; ++j) {
+            int64_t col = s->colidx[s->colptr[b] + j];
+            for (int64_t a = 0; a < m; ++a) {
+                int64_t row = h->test_perm[h->test_lo[t] - 1 + a] - 1;
+                cplx g = entry(h->k, h->sign, h->testpts + 3 * row, h->trialpts + 3 * col);
+                re[a + j * m] = g.re;
+                im[a + j * m] = g.im;
+            }
+        }
+    }
+    return s;
+}
+
+int64_t orc_near_entries(const orc_near* s) { return s ? s->entries : 0; }
+
+void orc_near_free(orc_near* s) {
+    if (!s) return;
+    free(s->node);
+    free(s->rowptr);
+    free(s->colptr);
+    free(s->colidx);
+    free(s->boff);
+    free(s->re);
+    free(s->im);
+    free(s);
+}
+
+/* acc[row + q ldacc] += (stored block row of t) x for every stored cluster:
+ * gather the block's x rows, multiply column by column (rows innermost), add
+ * into the output rows. */
+__attribute__((optimize("O3"))) static void near_apply(const orc_h2* h, const orc_near* s, int64_t nrhs,
+                                                       const cplx* X, int64_t ldx, cplx* acc, int64_t ldacc) {
+#pragma omp parallel
+    {
+        int64_t cap = 0;
+        double *yr = NULL, *yi = NULL, *xr = NULL, *xi = NULL;
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t b = 0; b < s->nb; ++b) {
+            int64_t t = s->node[b];
+            int64_t m = h->test_hi[t] - h->test_lo[t] + 1;
+            int64_t n = s->colptr[b + 1] - s->colptr[b];
+            if (m * nrhs > cap || n * nrhs > cap) {
+                cap = (m > n ? m : n) * nrhs;
+                yr = (double*)realloc(yr, sizeof(double) * (size_t)cap);
+                yi = (double*)realloc(yi, sizeof(double) * (size_t)cap);
+                xr = (double*)realloc(xr, sizeof(double) * (size_t)cap);
+                xi = (double*)realloc(xi, sizeof(double) * (size_t)cap);
+            }
+            const int64_t* cols = s->colidx + s->colptr[b];
+            for (int64_t j = 0; j < n; ++j)
+                for (int64_t q = 0; q < nrhs; ++q) {
+                    cplx v = X[cols[j] + q * ldx];
+                    xr[q + j * nrhs] = v.re;
+                    xi[q + j * nrhs] = v.im;
+                }
+            for (int64_t i = 0; i < m * nrhs; ++i) yr[i] = yi[i] = 0.0;
+            const double* re = s->re + s->boff[b];
+            const double* im = s->im + s->boff[b];
+            for (int64_t j = 0; j < n; ++j) {
+                const double* br = re + j * m;
+                const double* bi = im + j * m;
+                for (int64_t q = 0; q < nrhs; ++q) {
+                    const double vr = xr[q + j * nrhs], vi = xi[q + j * nrhs];
+                    double* outr = yr + q * m;
+                    double* outi = yi + q * m;
+                    for (int64_t a = 0; a < m; ++a) {
+                        outr[a] += br[a] * vr - bi[a] * vi;
+                        outi[a] += br[a] * vi + bi[a] * vr;
+                    }
+                }
+            }
+            for (int64_t a = 0; a < m; ++a) {
+                int64_t row = h->test_perm[h->test_lo[t] - 1 + a] - 1;
+                for (int64_t q = 0; q < nrhs; ++q) {
+                    acc[row + q * ldacc].re += yr[a + q * m];
+                    acc[row + q * ldacc].im += yi[a + q * m];
+                }
+            }
+        }
+        free(yr);
+        free(yi);
+        free(xr);
+        free(xi);
+    }
+}
+
 void orc_h2_matvec(const orc_h2* h, int32_t op, int64_t nrhs, const double* xin, int64_t ldx, double* yio,
                    int64_t ldy, const double* alpha, const double* beta, int32_t nearon, int64_t nrows,
                    const int64_t* rows) {
@@ -646,7 +787,9 @@ void orc_h2_matvec(const orc_h2* h, int32_t op, int64_t nrhs, const double* xin,
     }
     /* Near field (petrovgalerkinnca.jl:474,531,587; blocks of abstractnca.jl:134-178). */
     if (nearon) {
-        if (fwd) {
+        if (fwd && h->near_store && !want) {
+            near_apply(h, h->near_store, nrhs, X, ldx, acc, ndst_pts);
+        } else if (fwd) {
 #pragma omp parallel for schedule(dynamic, 4)
             for (int64_t t = 0; t < h->nn_test; ++t) {
                 for (int64_t a = h->test_lo[t] - 1; a < h->test_hi[t] && h->nearptr[t + 1] > h->nearptr[t]; ++a) {

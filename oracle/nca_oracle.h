@@ -83,7 +83,20 @@ typedef struct {
     const double* cp_blocks;
     /* near interactions (1-based CSR of trial nodes per test node) */
     const int64_t *nearptr, *nearnodes;
+    /* optional stored near field (orc_near_assemble); NULL: evaluated on the fly */
+    const struct orc_near* near_store;
 } orc_h2;
+
+/* Stored near field as the reference's CPU path keeps it
+ * (src/nca/abstractnca.jl:134-178, a BlockSparseMatrix): one dense block row per
+ * test cluster with near interactions, rows = values(t), columns = the
+ * concatenated values of its near trial clusters (real and imaginary parts
+ * split, column-major). With h->near_store set, the forward product over all
+ * rows multiplies these blocks instead of evaluating the kernel. */
+typedef struct orc_near orc_near;
+orc_near* orc_near_assemble(const orc_h2* h);
+int64_t orc_near_entries(const orc_near* s);
+void orc_near_free(orc_near* s);
 
 /* y = alpha * op(A) * x + beta * y for op 0 (N), 1 (T), 2 (C); x, y hold nrhs
  * columns with leading dimensions ldx, ldy. near: 1 evaluates the near field,

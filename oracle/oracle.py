@@ -40,6 +40,7 @@ class H2(ctypes.Structure):
         ("tt_blocks", P), ("st_blocks", P),
         ("cp_ptr", P), ("cp_nodes", P), ("cp_off", P), ("cp_blocks", P),
         ("nearptr", P), ("nearnodes", P),
+        ("near_store", P),
     ]
 
 
@@ -53,6 +54,11 @@ def lib():
             build()
         L = ctypes.CDLL(LIB)
         L.orc_sincos.argtypes = [D, P, P]
+        L.orc_near_assemble.restype = P
+        L.orc_near_assemble.argtypes = [P]
+        L.orc_near_entries.restype = I64
+        L.orc_near_entries.argtypes = [P]
+        L.orc_near_free.argtypes = [P]
         L.orc_entry.argtypes = [D, I32, P, P, P]
         L.orc_assemble_level.restype = I32
         L.orc_assemble_level.argtypes = [D, I32, P, P, I64, P, P, P, P, I64, I32, P]
@@ -198,7 +204,7 @@ class OracleH2:
             self._keep.append(a)
             setattr(h, name, _p(a))
         for name, _ in H2._fields_:
-            if name in ("M", "N", "nn_test", "nn_trial", "k", "sign", "testpts", "trialpts"):
+            if name in ("M", "N", "nn_test", "nn_trial", "k", "sign", "testpts", "trialpts", "near_store"):
                 continue
             if name in ("nbasis_test", "nbasis_trial", "nlev_test", "nlev_trial"):
                 setattr(h, name, int(desc[name]))
@@ -213,6 +219,28 @@ class OracleH2:
             setattr(h, name, _p(a))
         self.h = h
         self.M, self.N = desc["M"], desc["N"]
+        self._near = None
+
+    def store_near(self):
+        """Assemble and keep the near field as dense block rows (the reference's
+        CPU layout); later forward products over all rows use it. Returns the
+        number of stored entries."""
+        self.drop_near()
+        self._near = lib().orc_near_assemble(ctypes.byref(self.h))
+        self.h.near_store = self._near
+        return int(lib().orc_near_entries(self._near))
+
+    def drop_near(self):
+        if self._near:
+            self.h.near_store = None
+            lib().orc_near_free(self._near)
+            self._near = None
+
+    def __del__(self):
+        try:
+            self.drop_near()
+        except Exception:
+            pass
 
     def matvec(self, x, op=0, alpha=1.0, beta=0.0, y=None, near=True, rows=None):
         x = np.asfortranarray(np.asarray(x, dtype=np.complex128).reshape(x.shape[0], -1))

@@ -324,3 +324,28 @@ def test_matvec_vs_direct_summation(tol):
     assert relerr(H.matvec(x)[rows, 0], ref) < 5 * tol
     ref_t = O.dense_rows(p.k, p.sign, p.trialpts, p.testpts, rows, x)[:, 0]  # symmetric kernel
     assert relerr(H.matvec(x, op=1)[rows, 0], ref_t) < 5 * tol
+
+
+def test_stored_near_field_matches_on_the_fly():
+    """The stored near field (dense block rows, the reference's CPU layout)
+    gives the on-the-fly product: same entries, other summation order."""
+    p = sphere_problem(2048, tol=1e-4)
+    ft, fs = oracle_assembly(p)
+    H, _ = oracle_operator(p, ft, fs)
+    rng = np.random.default_rng(9)
+    x = geometry.complex_gaussian(rng, (p.trialpts.shape[0], 5))
+    y0 = H.matvec(x)
+    n = H.store_near()
+    nn = p.partition.nearptr
+    sz_t = p.tree.testtree.hi - p.tree.testtree.lo
+    sz_s = p.tree.trialtree.hi - p.tree.trialtree.lo
+    rows = np.repeat(np.arange(len(nn) - 1), np.diff(nn))
+    assert n == int(np.sum(sz_t[rows] * sz_s[p.partition.nearnodes]))
+    y1 = H.matvec(x)
+    assert relerr(y1, y0) <= 1e-13
+    # restricted rows and transposed products still evaluate on the fly
+    u = geometry.complex_gaussian(rng, (p.testpts.shape[0], 2))
+    H2 = oracle_operator(p, ft, fs)[0]
+    assert np.array_equal(H.matvec(u, op=1), H2.matvec(u, op=1))
+    H.drop_near()
+    assert np.array_equal(H.matvec(x), y0)
