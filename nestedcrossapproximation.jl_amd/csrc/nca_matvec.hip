@@ -100,6 +100,10 @@ struct nca_h2 {
     std::vector<int32_t> h_ncols;  // host copy: columns of every test node's coupling row
 };
 
+// Rows of padding after the gathered right-hand sides and the tree-ordered
+// points of each side (read, never used, by the last near-field step of a range).
+constexpr int64_t kNearPad = 16;
+
 // One captured device sweep (see nca_h2_graph_create).
 struct nca_h2_graph {
     nca_h2* h2 = nullptr;
@@ -630,47 +634,62 @@ __device__ __forceinline__ T2 rot16v(T2 v) {
     return v;
 }
 
-// Near-field walk of one tile with H valid-row slots (H = 16, 8 or 4). Lane
-// (i, kk) = (lane % 16, lane / 16) evaluates G for row i % H and source
-// p + 4 (i / H) + kk, so one step evaluates 64 entries over 4 * 16 / H sources;
-// MFMA m takes A = the values of group m, rotated down to lanes 0..H-1 of each
-// 16-lane row (the rows >= H of the product are not stored), and B = xs of
-// sources p + 4 m .. p + 4 m + 3 (all loads of a step are issued before its
-// evaluation). Control flow is uniform over the wave: lanes past a range end
-// read its last source and contribute B = 0. The xs offsets are 32-bit
-// (points x right-hand sides < 2^31).
+// One near-field step of a tile with H valid-row slots (H = 16, 8 or 4) at
+// source position p. Lane (i, kk) = (lane % 16, lane / 16) evaluates G for row
+// i % H and source p + 4 (i / H) + kk, so a step evaluates 64 entries over
+// 4 * 16 / H sources; MFMA m takes A = the values of group m, rotated down to
+// lanes 0..H-1 of each 16-lane row (the rows >= H of the product are not
+// stored), and B = xs of sources p + 4 m .. p + 4 m + 3. MASK (the last step of
+// a range): sources at or past hi read the rows after them (other points or
+// the zero padding) and contribute B = 0. xp/yp: xs and points at position p.
+template <class P, int H, bool K3, bool MASK>
+__device__ __forceinline__ void near_step(Cacc<P, K3>& acc, const NearK& nk, const double2* tab, int kk,
+                                          int ysrc, double x0, double x1, double x2, int p, int hi,
+                                          const double* __restrict__ yp, const typename P::T2* __restrict__ xp,
+                                          uint32_t R) {
+    using T2 = typename P::T2;
+    constexpr int G = 16 / H;
+    const double* ys = yp + 3 * ysrc;
+    T2 b[G];
+#pragma unroll
+    for (int m = 0; m < G; ++m) {
+        b[m] = xp[(uint32_t)(4 * m + kk) * R];
+        if (MASK && p + 4 * m + kk >= hi) b[m] = czero2<P>();
+    }
+    const T2 g = P::eval(nk, tab, x0, x1, x2, ys[0], ys[1], ys[2]);
+#pragma unroll
+    for (int m = 0; m < G; ++m) {
+        if (MASK && p + 4 * m >= hi) continue;
+        T2 a;
+        if (m == 0) a = g;
+        else if (m == 1) a = rot16v<H>(g);
+        else if (m == 2) a = rot16v<(2 * H) & 15>(g);
+        else a = rot16v<(3 * H) & 15>(g);
+        acc.mma(a, b[m]);
+    }
+}
+
+// Near-field walk of one tile over its merged source ranges: full steps
+// without masks, then one masked step for the rest of the range. Control flow
+// is uniform over the wave. The xs offsets are 32-bit (points x right-hand
+// sides < 2^31).
 template <class P, int H, bool K3>
 __device__ __forceinline__ void near_walk(Cacc<P, K3>& acc, const NearK& nk, const double2* tab, int lane,
                                           double x0, double x1, double x2, int64_t e0, int64_t e1,
                                           const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
                                           const double* __restrict__ spts, const typename P::T2* __restrict__ xsl,
                                           uint32_t R) {
-    using T2 = typename P::T2;
-    constexpr int G = 16 / H;
-    const int kk = lane >> 4, grp = (lane & 15) / H;
+    constexpr int S = 4 * (16 / H);
+    const int kk = lane >> 4, ysrc = 4 * ((lane & 15) / H) + kk;
     for (int64_t e = e0; e < e1; ++e) {
         const int lo = nrlo[e], hi = nrhi[e];
-        for (int p = lo; p < hi; p += 4 * G) {
-            const double* yp = spts + 3 * (int64_t)min(p + 4 * grp + kk, hi - 1);
-            T2 b[G];
-#pragma unroll
-            for (int m = 0; m < G; ++m) {
-                const int lb = p + 4 * m + kk;
-                b[m] = xsl[(uint32_t)min(lb, hi - 1) * R];
-                if (lb >= hi) b[m] = czero2<P>();
-            }
-            const T2 g = P::eval(nk, tab, x0, x1, x2, yp[0], yp[1], yp[2]);
-#pragma unroll
-            for (int m = 0; m < G; ++m) {
-                if (p + 4 * m >= hi) continue;
-                T2 a;
-                if (m == 0) a = g;
-                else if (m == 1) a = rot16v<H>(g);
-                else if (m == 2) a = rot16v<(2 * H) & 15>(g);
-                else a = rot16v<(3 * H) & 15>(g);
-                acc.mma(a, b[m]);
-            }
-        }
+        int p = lo;
+        for (; p + S <= hi; p += S)
+            near_step<P, H, K3, false>(acc, nk, tab, kk, ysrc, x0, x1, x2, p, hi, spts + 3 * (int64_t)p,
+                                       xsl + (uint32_t)p * R, R);
+        if (p < hi)
+            near_step<P, H, K3, true>(acc, nk, tab, kk, ysrc, x0, x1, x2, p, hi, spts + 3 * (int64_t)p,
+                                      xsl + (uint32_t)p * R, R);
     }
 }
 
@@ -1017,10 +1036,10 @@ void ensure_workspace(nca_h2* h, int64_t R) {
         h->ws_rhs = R;
     }
     if (h->precision == NCA_PREC_MIXED) {
-        if (!h->ws_xs32.p) h->ws_xs32.alloc((size_t)std::max<int64_t>(np * h->ws_rhs, 1));
+        if (!h->ws_xs32.p) h->ws_xs32.alloc((size_t)std::max<int64_t>((np + kNearPad) * h->ws_rhs, 1));
         if (!h->ws_xh32.p) h->ws_xh32.alloc((size_t)std::max<int64_t>(K * 16, 1));
     } else {
-        if (!h->ws_xs.p) h->ws_xs.alloc((size_t)std::max<int64_t>(np * h->ws_rhs, 1));
+        if (!h->ws_xs.p) h->ws_xs.alloc((size_t)std::max<int64_t>((np + kNearPad) * h->ws_rhs, 1));
     }
 }
 
@@ -1080,7 +1099,9 @@ void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, const int
     for (int64_t t = 0; t < nn; ++t)
         NCA_REQUIRE(sd.h_cptr[t + 1] >= sd.h_cptr[t], NCA_ERR_ARGUMENT, "nca_h2_create: decreasing coefficient plan");
     sd.K = sd.h_cptr[nn];
-    std::vector<double> pp(3 * npts);
+    // kNearPad zero points after the last one: near-field steps may read up to
+    // 15 positions past a range end (their contributions are zeroed)
+    std::vector<double> pp(3 * (npts + kNearPad), 0.0);
     for (int64_t i = 0; i < npts; ++i)
         for (int d = 0; d < 3; ++d) pp[3 * i + d] = pts->host[3 * sd.h_perm[i] + d];
     sd.perm.upload(sd.h_perm, st);
