@@ -316,12 +316,10 @@ __global__ void __launch_bounds__(kWave)
 
 constexpr double kInvFourPi = 0x1.45f306dc9c883p-4;
 
-// Near-field constants of one product: k, k / (2 pi) and the imaginary-part
-// scale +-1 / (4 pi) (the sign of the kernel's phase times -1 for the adjoint).
+// Near-field constants of one product: k and k / (2 pi).
 struct NearK {
     double k;
     double k2pi;
-    double ims;
 };
 
 // Table of (cos, sin)(n pi / 128), n = 0..255, for the FP64 near field.
@@ -330,17 +328,19 @@ constexpr double kTabScale = 0x1.45f306dc9c883p+5;   // 128 / pi
 constexpr double kPi128Hi = 0x1.921fb54442d18p-6;    // pi / 128, leading part
 constexpr double kPi128Lo = 0x1.1a62633145c07p-60;   // pi / 128 - kPi128Hi
 
-// G(x, y) for the FP64 near field, branch free (r = 0 gives 0): 1/r from the
-// hardware reciprocal square root refined by two Newton steps; the phase
-// x = k r split as x = n pi / 128 + y, |y| <= pi / 256 (two-part constant), then
-// sin/cos x = table(n) rotated by sin/cos y from short Taylor polynomials
-// (truncation below 5e-15).
+// 4 pi G(x, y) = (cos, sin)(k r) / r for the FP64 near field (the 1/(4 pi) is
+// applied to the accumulated sum; NEG: the conjugate, for a negative phase
+// sign or the adjoint), branch free, r = 0 gives 0: 1/r from the hardware
+// reciprocal square root refined by two Newton steps; the phase x = k r split
+// as x = n pi / 128 + y, |y| <= pi / 256 (two-part constant), then sin/cos x =
+// table(n) rotated by sin/cos y from short Taylor polynomials (truncation below
+// 5e-15).
+template <bool NEG>
 __device__ __forceinline__ double2 helmholtz_mv(const NearK& nk, const double2* tab, double x0, double x1,
                                                 double x2, double y0, double y1, double y2) {
     const double dx = x0 - y0, dy = x1 - y1, dz = x2 - y2;
     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-    const bool nz = r2 > 0.0;
-    const double r2s = nz ? r2 : 1.0;
+    const double r2s = fmax(r2, 1e-300);
     double ri = __builtin_amdgcn_rsq(r2s);
     const double h = 0.5 * r2s;
     ri = ri * fma(-h * ri, ri, 1.5);
@@ -355,32 +355,30 @@ __device__ __forceinline__ double2 helmholtz_mv(const NearK& nk, const double2* 
     const double cy = fma(z, fma(z, 0x1.5555555555555p-5, -0.5), 1.0);
     const double c = fma(cs.x, cy, -(cs.y * sy));
     const double s = fma(cs.y, cy, cs.x * sy);
-    const double scr = nz ? ri * kInvFourPi : 0.0;
-    const double sci = nz ? ri * nk.ims : 0.0;
-    return make_double2(c * scr, s * sci);
+    const double sc = r2 > 0.0 ? ri : 0.0;
+    return make_double2(c * sc, NEG ? -(s * sc) : s * sc);
 }
 
-// G(x, y) for the MIXED near field, branch free: coordinate differences,
-// distance and the phase k r / (2 pi) in FP64 (1/r from the FP32 hardware
-// reciprocal square root plus one FP64 Newton step, relative error ~1e-14),
-// the phase reduced to [-1/2, 1/2] revolutions in FP64; sin/cos of the reduced
-// phase and the 1/(4 pi r) scale in FP32 (hardware sin/cos of 2 pi f).
-// Distances below 1e-15 count as zero.
+// 4 pi G(x, y) for the MIXED near field (NEG as above), branch free:
+// coordinate differences, distance and the phase k r / (2 pi) in FP64 (1/r
+// from the FP32 hardware reciprocal square root plus one FP64 Newton step,
+// relative error ~1e-14), the phase reduced to [-1/2, 1/2] revolutions in
+// FP64; sin/cos of the reduced phase and the 1/r scale in FP32 (hardware
+// sin/cos of 2 pi f). Distances below 1e-15 count as zero.
+template <bool NEG>
 __device__ __forceinline__ float2 helmholtz_f32(const NearK& nk, double x0, double x1, double x2, double y0,
                                                 double y1, double y2) {
     const double dx = x0 - y0, dy = x1 - y1, dz = x2 - y2;
     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-    const bool nz = r2 > 1e-30;
-    const double r2s = nz ? r2 : 1.0;
+    const double r2s = fmax(r2, 1e-30);
     const float rf = __builtin_amdgcn_rsqf((float)r2s);
     double ri = (double)rf;
     ri = ri * fma(-0.5 * r2s * ri, ri, 1.5);
     const double t = (r2s * ri) * nk.k2pi;
     const float f = (float)(t - rint(t));
     const float c = __builtin_amdgcn_cosf(f), s = __builtin_amdgcn_sinf(f);
-    const float scr = nz ? rf * (float)kInvFourPi : 0.0f;
-    const float sci = nz ? rf * (float)nk.ims : 0.0f;
-    return make_float2(c * scr, s * sci);
+    const float sc = r2 > 1e-30 ? rf : 0.0f;
+    return make_float2(c * sc, NEG ? -(s * sc) : s * sc);
 }
 
 // Stage 5 + near field, VALU form for fewer than 16 right-hand sides: one wave
@@ -537,9 +535,10 @@ struct PrecF64 {
     __device__ static T2 cvt(double2 v) { return v; }
     __device__ static T2 cvt(float2 v) { return make_double2(v.x, v.y); }
     static constexpr bool kTab = true;
+    template <bool NEG>
     __device__ static T2 eval(const NearK& nk, const double2* tab, double x0, double x1, double x2, double y0,
                               double y1, double y2) {
-        return helmholtz_mv(nk, tab, x0, x1, x2, y0, y1, y2);
+        return helmholtz_mv<NEG>(nk, tab, x0, x1, x2, y0, y1, y2);
     }
 
 };
@@ -553,9 +552,10 @@ struct PrecF32 {
     __device__ static T2 cvt(double2 v) { return make_float2((float)v.x, (float)v.y); }
     __device__ static T2 cvt(float2 v) { return v; }
     static constexpr bool kTab = false;
+    template <bool NEG>
     __device__ static T2 eval(const NearK& nk, const double2*, double x0, double x1, double x2, double y0, double y1,
                               double y2) {
-        return helmholtz_f32(nk, x0, x1, x2, y0, y1, y2);
+        return helmholtz_f32<NEG>(nk, x0, x1, x2, y0, y1, y2);
     }
 };
 
@@ -583,6 +583,14 @@ struct Cacc {
         } else {
             p3 = P::mma(a.x, b.y, p3);
             p4 = P::mma(a.y, b.x, p4);
+        }
+    }
+    __device__ __forceinline__ void scale(typename P::T f) {
+        for (int v = 0; v < 4; ++v) {
+            p1[v] *= f;
+            p2[v] *= f;
+            p3[v] *= f;
+            if (!K3) p4[v] *= f;
         }
     }
     __device__ __forceinline__ double2 get(int v) const {
@@ -642,7 +650,7 @@ __device__ __forceinline__ T2 rot16v(T2 v) {
 // stored), and B = xs of sources p + 4 m .. p + 4 m + 3. MASK (the last step of
 // a range): sources at or past hi read the rows after them (other points or
 // the zero padding) and contribute B = 0. xp/yp: xs and points at position p.
-template <class P, int H, bool K3, bool MASK>
+template <class P, int H, bool K3, bool NEG, bool MASK>
 __device__ __forceinline__ void near_step(Cacc<P, K3>& acc, const NearK& nk, const double2* tab, int kk,
                                           int ysrc, double x0, double x1, double x2, int p, int hi,
                                           const double* __restrict__ yp, const typename P::T2* __restrict__ xp,
@@ -656,7 +664,7 @@ __device__ __forceinline__ void near_step(Cacc<P, K3>& acc, const NearK& nk, con
         b[m] = xp[(uint32_t)(4 * m + kk) * R];
         if (MASK && p + 4 * m + kk >= hi) b[m] = czero2<P>();
     }
-    const T2 g = P::eval(nk, tab, x0, x1, x2, ys[0], ys[1], ys[2]);
+    const T2 g = P::template eval<NEG>(nk, tab, x0, x1, x2, ys[0], ys[1], ys[2]);
 #pragma unroll
     for (int m = 0; m < G; ++m) {
         if (MASK && p + 4 * m >= hi) continue;
@@ -673,7 +681,7 @@ __device__ __forceinline__ void near_step(Cacc<P, K3>& acc, const NearK& nk, con
 // without masks, then one masked step for the rest of the range. Control flow
 // is uniform over the wave. The xs offsets are 32-bit (points x right-hand
 // sides < 2^31).
-template <class P, int H, bool K3>
+template <class P, int H, bool K3, bool NEG>
 __device__ __forceinline__ void near_walk(Cacc<P, K3>& acc, const NearK& nk, const double2* tab, int lane,
                                           double x0, double x1, double x2, int64_t e0, int64_t e1,
                                           const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
@@ -685,10 +693,10 @@ __device__ __forceinline__ void near_walk(Cacc<P, K3>& acc, const NearK& nk, con
         const int lo = nrlo[e], hi = nrhi[e];
         int p = lo;
         for (; p + S <= hi; p += S)
-            near_step<P, H, K3, false>(acc, nk, tab, kk, ysrc, x0, x1, x2, p, hi, spts + 3 * (int64_t)p,
+            near_step<P, H, K3, NEG, false>(acc, nk, tab, kk, ysrc, x0, x1, x2, p, hi, spts + 3 * (int64_t)p,
                                        xsl + (uint32_t)p * R, R);
         if (p < hi)
-            near_step<P, H, K3, true>(acc, nk, tab, kk, ysrc, x0, x1, x2, p, hi, spts + 3 * (int64_t)p,
+            near_step<P, H, K3, NEG, true>(acc, nk, tab, kk, ysrc, x0, x1, x2, p, hi, spts + 3 * (int64_t)p,
                                       xsl + (uint32_t)p * R, R);
     }
 }
@@ -698,7 +706,7 @@ __device__ __forceinline__ void near_walk(Cacc<P, K3>& acc, const NearK& nk, con
 // V_t^T, B = yh_t); then the near-field sources over each merged contiguous
 // point range, with 16, 8 or 4 row slots by the rows of the tile. Rows past
 // the tile end repeat its last row and are not stored.
-template <class P, bool FWD, bool CJ, bool K3>
+template <class P, bool FWD, bool CJ, bool K3, bool NEG>
 __global__ void __launch_bounds__(256, 4)
     k_leaf_out_mfma(const int32_t* __restrict__ tile_leaf, const int32_t* __restrict__ tile_row0, int64_t ntiles,
                     const int32_t* __restrict__ leaves, const int32_t* __restrict__ leafbasis,
@@ -728,6 +736,19 @@ __global__ void __launch_bounds__(256, 4)
     const int i = lane & 15, kk = lane >> 4, j = lane & 15;
     Cacc<P, K3> acc;
     acc.zero();
+    // near field first (4 pi G), scaled by 1 / (4 pi) before the basis rows
+    const int H = nrows > 8 ? 16 : (nrows > 4 ? 8 : 4);
+    const double* xp = dpts + 3 * (int64_t)(l0 + row0 + min(i & (H - 1), nrows - 1));
+    const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
+    const T2* xsl = xs + q0 + j;
+    const int64_t e0 = nrptr[t], e1 = nrptr[t + 1];
+    if (H == 16)
+        near_walk<P, 16, K3, NEG>(acc, nk, tab, lane, x0, x1, x2, e0, e1, nrlo, nrhi, spts, xsl, (uint32_t)R);
+    else if (H == 8)
+        near_walk<P, 8, K3, NEG>(acc, nk, tab, lane, x0, x1, x2, e0, e1, nrlo, nrhi, spts, xsl, (uint32_t)R);
+    else
+        near_walk<P, 4, K3, NEG>(acc, nk, tab, lane, x0, x1, x2, e0, e1, nrlo, nrhi, spts, xsl, (uint32_t)R);
+    acc.scale((typename P::T)kInvFourPi);
     if (bi >= 0) {
         const int rowc = row0 + min(i, nrows - 1);
         const int rt = dcptr[t + 1] - dcptr[t];
@@ -742,17 +763,6 @@ __global__ void __launch_bounds__(256, 4)
             acc.mma(a, b);
         }
     }
-    const int H = nrows > 8 ? 16 : (nrows > 4 ? 8 : 4);
-    const double* xp = dpts + 3 * (int64_t)(l0 + row0 + min(i & (H - 1), nrows - 1));
-    const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
-    const T2* xsl = xs + q0 + j;
-    const int64_t e0 = nrptr[t], e1 = nrptr[t + 1];
-    if (H == 16)
-        near_walk<P, 16, K3>(acc, nk, tab, lane, x0, x1, x2, e0, e1, nrlo, nrhi, spts, xsl, (uint32_t)R);
-    else if (H == 8)
-        near_walk<P, 8, K3>(acc, nk, tab, lane, x0, x1, x2, e0, e1, nrlo, nrhi, spts, xsl, (uint32_t)R);
-    else
-        near_walk<P, 4, K3>(acc, nk, tab, lane, x0, x1, x2, e0, e1, nrlo, nrhi, spts, xsl, (uint32_t)R);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int row = P::row(kk, v);
@@ -961,11 +971,12 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
     }
     if (evs) evs->mark(5, st);
     if (NR == 16 && dst.ntiles > 0) {
-        // conj(G) for the adjoint flips the imaginary scale
-        const double ims = ((h->sign < 0) != CJ) ? -kInvFourPi : kInvFourPi;
-        const NearK nk{h->k, h->k * (0.5 / M_PI), ims};
+        // conj(G) for a negative phase sign, again conjugated for the adjoint
+        const bool neg = (h->sign < 0) != CJ;
+        const NearK nk{h->k, h->k * (0.5 / M_PI)};
         const bool k3 = h->near_k3[MIX ? 1 : 0] != 0;
-        auto kern = k3 ? k_leaf_out_mfma<P, FWD, CJ, true> : k_leaf_out_mfma<P, FWD, CJ, false>;
+        auto kern = k3 ? (neg ? k_leaf_out_mfma<P, FWD, CJ, true, true> : k_leaf_out_mfma<P, FWD, CJ, true, false>)
+                       : (neg ? k_leaf_out_mfma<P, FWD, CJ, false, true> : k_leaf_out_mfma<P, FWD, CJ, false, false>);
         hipLaunchKernelGGL(kern, dim3(blocks_for(dst.ntiles, 4)), dim3(256), 0, st, dst.tile_leaf.p,
                            dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst),
                            dst.lo.p, dst.hi.p, dst.perm.p, dst.cptr.p, dst.pts.p, dst.nrptr.p, dst.nrlo.p, dst.nrhi.p,

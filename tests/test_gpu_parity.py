@@ -336,3 +336,26 @@ def test_matvec_mixed_precision(ctx, case, request):
     NC.PetrovGalerkinNCACore.set_precision(W.core, nca_amd.NCA_PREC_FP64)
     x = geometry.complex_gaussian(rng, (N, 16))
     assert relerr(W @ x, H.matvec(x)) <= 1e-12
+
+
+def test_matvec_negative_phase_sign(ctx):
+    """exp(-i k r)/(4 pi r): the device products (N, T, C; FP64 and MIXED,
+    16 and 3 right-hand sides) against the oracle's product on the operator's
+    own factors."""
+    say("matvec sign -1")
+    pts = geometry.fibonacci_sphere(3000)
+    tree = nca_amd.TwoNTree(pts, minvalues=32)
+    cmp = C.BatchedBottomUp(C.GeometricPivoting(), C.ACA(1e-4))
+    core = nca_amd.PetrovGalerkinNCACore(nca_amd.HelmholtzKernel(1.5, -1), pts, pts,
+                                         nca_amd.BlockTree(tree, tree), cmp, cmp, ctx=ctx)
+    off, blocks = core.couplingstore.download()
+    H = O.OracleH2(core.desc, 1.5, -1, pts, pts, off, blocks)
+    W = NC.H2Workspace(core)
+    rng = np.random.default_rng(31)
+    for prec, bound in ((nca_amd.NCA_PREC_FP64, 1e-12), (nca_amd.NCA_PREC_MIXED, 1e-5)):
+        core.set_precision(prec)
+        for R in (16, 3):
+            x = geometry.complex_gaussian(rng, (pts.shape[0], R))
+            assert relerr(W @ x, H.matvec(x)) <= bound
+            assert relerr(NC.transpose(W) @ x, H.matvec(x, op=1)) <= bound
+            assert relerr(NC.adjoint(W) @ x, H.matvec(x, op=2)) <= bound
