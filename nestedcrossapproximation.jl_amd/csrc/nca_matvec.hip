@@ -449,73 +449,6 @@ __global__ void __launch_bounds__(kWave)
     }
 }
 
-// 16 right-hand sides, lane = (4 operator rows) x (16 right-hand sides): every
-// lane does one complex FMA per operator entry, the right-hand-side loads are
-// 256-byte contiguous, the operator entry is shared by 16 lanes.
-template <bool TR, bool CJ, class BT, class XT>
-__global__ void __launch_bounds__(kWave)
-    k_project16(const int32_t* __restrict__ bnodes, const int64_t* __restrict__ boff, const BT* __restrict__ blocks,
-                const int32_t* __restrict__ lo, const int32_t* __restrict__ hi, const int32_t* __restrict__ cptr,
-                const XT* __restrict__ xs, int64_t R, int64_t q0, double2* __restrict__ xh) {
-    const int i = blockIdx.x;
-    const int s = bnodes[i];
-    const int r = cptr[s + 1] - cptr[s];
-    const int l0 = lo[s], ns = hi[s] - l0;
-    const BT* B = blocks + boff[i];
-    const int q = threadIdx.x & 15, sub = threadIdx.x >> 4;
-    for (int a = sub; a < r; a += 4) {
-        double2 acc = make_double2(0.0, 0.0);
-        for (int j = 0; j < ns; ++j) {
-            const double2 b = ldc<CJ>(B, TR ? (int64_t)j + (int64_t)a * ns : (int64_t)a + (int64_t)j * r);
-            acc = cfma(b, to_d2(xs[(int64_t)(l0 + j) * R + q0 + q]), acc);
-        }
-        double2* out = xh + (int64_t)(cptr[s] + a) * R + q0 + q;
-        *out = cadd(*out, acc);
-    }
-}
-
-template <bool DOWN, bool TR, bool CJ, class BT>
-__global__ void __launch_bounds__(kWave)
-    k_transfer16(const int32_t* __restrict__ parents, const int64_t* __restrict__ chdptr,
-                 const int32_t* __restrict__ chds, const int64_t* __restrict__ toff, const BT* __restrict__ blocks,
-                 const int32_t* __restrict__ cptr, int64_t pbase, int64_t R, int64_t q0, double2* __restrict__ h) {
-    const int64_t pi = pbase + blockIdx.x;
-    const int p = parents[pi];
-    const int rp = cptr[p + 1] - cptr[p];
-    const int q = threadIdx.x & 15, sub = threadIdx.x >> 4;
-    if (DOWN) {
-        for (int64_t e = chdptr[pi]; e < chdptr[pi + 1]; ++e) {
-            const int c = chds[e];
-            const int rc = cptr[c + 1] - cptr[c];
-            const BT* E = blocks + toff[e];
-            for (int a = sub; a < rc; a += 4) {
-                double2 acc = make_double2(0.0, 0.0);
-                for (int j = 0; j < rp; ++j) {
-                    const double2 b = ldc<CJ>(E, TR ? (int64_t)j + (int64_t)a * rp : (int64_t)a + (int64_t)j * rc);
-                    acc = cfma(b, h[(int64_t)(cptr[p] + j) * R + q0 + q], acc);
-                }
-                double2* out = h + (int64_t)(cptr[c] + a) * R + q0 + q;
-                *out = cadd(*out, acc);
-            }
-        }
-    } else {
-        for (int a = sub; a < rp; a += 4) {
-            double2 acc = make_double2(0.0, 0.0);
-            for (int64_t e = chdptr[pi]; e < chdptr[pi + 1]; ++e) {
-                const int c = chds[e];
-                const int rc = cptr[c + 1] - cptr[c];
-                const BT* E = blocks + toff[e];
-                for (int j = 0; j < rc; ++j) {
-                    const double2 b = ldc<CJ>(E, TR ? (int64_t)j + (int64_t)a * rc : (int64_t)a + (int64_t)j * rp);
-                    acc = cfma(b, h[(int64_t)(cptr[c] + j) * R + q0 + q], acc);
-                }
-            }
-            double2* out = h + (int64_t)(cptr[p] + a) * R + q0 + q;
-            *out = cadd(*out, acc);
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // MFMA kernels (16 right-hand sides): a 16 x 4 operand tile (rows x sources or
 // columns) times a 4 x 16 tile (sources x right-hand sides). Lane l supplies
@@ -854,6 +787,120 @@ __global__ void __launch_bounds__(256, 4)
     }
 }
 
+// Stages 1, 2 and 4 for 16 right-hand sides as small MFMA GEMMs, one wave per
+// node (4 per workgroup): out (m x 16) += A (m x K) B (K x 16) over 16-row
+// tiles of m and 4-wide steps of K. A rows past m repeat row m - 1 (not
+// stored); B rows past K are zero. Control flow is uniform over the wave.
+template <class P, bool K3, class AF, class BF>
+__device__ __forceinline__ void gemm16_tile(Cacc<P, K3>& acc, int lane, int a0, int m, int K, AF afun, BF bfun) {
+    using T2 = typename P::T2;
+    const int i = lane & 15, kk = lane >> 4;
+    const int ar = a0 + min(i, m - 1 - a0);
+    for (int k0 = 0; k0 < K; k0 += 4) {
+        const int k = k0 + kk, kc = min(k, K - 1);
+        const T2 a = afun(ar, kc);
+        T2 b = bfun(kc, i);
+        if (k >= K) b = czero2<P>();
+        acc.mma(a, b);
+    }
+}
+
+template <class P, bool K3>
+__device__ __forceinline__ void gemm16_store(const Cacc<P, K3>& acc, int lane, int a0, int m, double2* out,
+                                             int64_t R) {
+    const int kk = lane >> 4, j = lane & 15;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int row = a0 + P::row(kk, v);
+        if (row < m) {
+            double2* o = out + (int64_t)row * R + j;
+            *o = cadd(*o, acc.get(v));
+        }
+    }
+}
+
+// Stage 1: xh_s += V_s xs_s (TR: U_s^T). One wave per source basis.
+template <class P, bool TR, bool CJ, bool K3>
+__global__ void __launch_bounds__(256)
+    k_project_mfma(const int32_t* __restrict__ bnodes, const int64_t* __restrict__ boff,
+                   const typename P::T2* __restrict__ blocks, int64_t nbasis, const int32_t* __restrict__ lo,
+                   const int32_t* __restrict__ hi, const int32_t* __restrict__ cptr,
+                   const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, double2* __restrict__ xh) {
+    using T2 = typename P::T2;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    if (w >= nbasis) return;
+    const int s = bnodes[w];
+    const int r = cptr[s + 1] - cptr[s];
+    const int l0 = lo[s], ns = hi[s] - l0;
+    if (r == 0 || ns == 0) return;
+    const T2* B = blocks + boff[w];
+    const T2* xv = xs + (int64_t)l0 * R + q0;
+    for (int a0 = 0; a0 < r; a0 += 16) {
+        Cacc<P, K3> acc;
+        acc.zero();
+        gemm16_tile<P, K3>(
+            acc, lane, a0, r, ns,
+            [&](int a, int j) { return conjif<CJ, P>(B[TR ? (int64_t)j + (int64_t)a * ns : (int64_t)a + (int64_t)j * r]); },
+            [&](int j, int q) { return xv[(int64_t)j * R + q]; });
+        gemm16_store<P, K3>(acc, lane, a0, r, xh + (int64_t)cptr[s] * R + q0, R);
+    }
+}
+
+// Stage 2 (up): xh_p += sum_c E_c xh_c, E_c r_p x r_c (TR: r_c x r_p used
+// transposed); stage 4 (DOWN): yh_c += E_c yh_p, E_c r_c x r_p (TR: r_p x r_c
+// transposed). One wave per parent of the level.
+template <class P, bool DOWN, bool TR, bool CJ, bool K3>
+__global__ void __launch_bounds__(256)
+    k_transfer_mfma(const int32_t* __restrict__ parents, const int64_t* __restrict__ chdptr,
+                    const int32_t* __restrict__ chds, const int64_t* __restrict__ toff,
+                    const typename P::T2* __restrict__ blocks, const int32_t* __restrict__ cptr, int64_t pbase,
+                    int64_t np, int64_t R, int64_t q0, double2* __restrict__ h) {
+    using T2 = typename P::T2;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    if (w >= np) return;
+    const int64_t pi = pbase + w;
+    const int p = parents[pi];
+    const int rp = cptr[p + 1] - cptr[p];
+    if (rp == 0) return;
+    const int64_t e0 = chdptr[pi], e1 = chdptr[pi + 1];
+    if (DOWN) {
+        const double2* hp = h + (int64_t)cptr[p] * R + q0;
+        for (int64_t e = e0; e < e1; ++e) {
+            const int c = chds[e];
+            const int rc = cptr[c + 1] - cptr[c];
+            const T2* E = blocks + toff[e];
+            for (int a0 = 0; a0 < rc; a0 += 16) {
+                Cacc<P, K3> acc;
+                acc.zero();
+                gemm16_tile<P, K3>(
+                    acc, lane, a0, rc, rp,
+                    [&](int a, int b) { return conjif<CJ, P>(E[TR ? (int64_t)b + (int64_t)a * rp : (int64_t)a + (int64_t)b * rc]); },
+                    [&](int b, int q) { return P::cvt(hp[(int64_t)b * R + q]); });
+                gemm16_store<P, K3>(acc, lane, a0, rc, h + (int64_t)cptr[c] * R + q0, R);
+            }
+        }
+    } else {
+        for (int a0 = 0; a0 < rp; a0 += 16) {
+            Cacc<P, K3> acc;
+            acc.zero();
+            for (int64_t e = e0; e < e1; ++e) {
+                const int c = chds[e];
+                const int rc = cptr[c + 1] - cptr[c];
+                if (rc == 0) continue;
+                const T2* E = blocks + toff[e];
+                const double2* hc = h + (int64_t)cptr[c] * R + q0;
+                gemm16_tile<P, K3>(
+                    acc, lane, a0, rp, rc,
+                    [&](int a, int b) { return conjif<CJ, P>(E[TR ? (int64_t)b + (int64_t)a * rc : (int64_t)a + (int64_t)b * rp]); },
+                    [&](int b, int q) { return P::cvt(hc[(int64_t)b * R + q]); });
+            }
+            gemm16_store<P, K3>(acc, lane, a0, rp, h + (int64_t)cptr[p] * R + q0, R);
+        }
+    }
+}
+
 inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // ---------------------------------------------------------------------------
@@ -905,8 +952,9 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
     if (evs) evs->mark(1, st);
     if (src.nbasis > 0) {
         if (NR == 16)
-            hipLaunchKernelGGL((k_project16<!FWD, CJ, T2, T2>), dim3((unsigned)src.nbasis), dim3(kWave), 0, st,
-                               src.bnodes.p, src.boff.p, O::basis(src), src.lo.p, src.hi.p, src.cptr.p, xs, R, q0, xh);
+            hipLaunchKernelGGL((k_project_mfma<P, !FWD, CJ, true>), dim3(blocks_for(src.nbasis, 4)), dim3(256), 0, st,
+                               src.bnodes.p, src.boff.p, O::basis(src), src.nbasis, src.lo.p, src.hi.p, src.cptr.p, xs,
+                               R, q0, xh);
         else
             hipLaunchKernelGGL((k_project<NR, !FWD, CJ, T2, T2>), dim3((unsigned)src.nbasis), dim3(kWave), 0, st,
                                src.bnodes.p, src.boff.p, O::basis(src), src.lo.p, src.hi.p, src.cptr.p, xs, R, q0, xh);
@@ -917,9 +965,9 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
         const int64_t np = src.levptr[l + 1] - src.levptr[l];
         if (np == 0) continue;
         if (NR == 16)
-            hipLaunchKernelGGL((k_transfer16<false, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
+            hipLaunchKernelGGL((k_transfer_mfma<P, false, !FWD, CJ, true>), dim3(blocks_for(np, 4)), dim3(256), 0, st,
                                src.parents.p, src.chdptr.p, src.chds.p, src.toff.p, O::transfer(src), src.cptr.p,
-                               src.levptr[l], R, q0, xh);
+                               src.levptr[l], np, R, q0, xh);
         else
             hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
                                src.parents.p, src.chdptr.p, src.chds.p, src.toff.p, O::transfer(src), src.cptr.p,
@@ -961,9 +1009,9 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
         const int64_t np = dst.levptr[l + 1] - dst.levptr[l];
         if (np == 0) continue;
         if (NR == 16)
-            hipLaunchKernelGGL((k_transfer16<true, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
+            hipLaunchKernelGGL((k_transfer_mfma<P, true, !FWD, CJ, true>), dim3(blocks_for(np, 4)), dim3(256), 0, st,
                                dst.parents.p, dst.chdptr.p, dst.chds.p, dst.toff.p, O::transfer(dst), dst.cptr.p,
-                               dst.levptr[l], R, q0, yh);
+                               dst.levptr[l], np, R, q0, yh);
         else
             hipLaunchKernelGGL((k_transfer<NR, true, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
                                dst.parents.p, dst.chdptr.p, dst.chds.p, dst.toff.p, O::transfer(dst), dst.cptr.p,
