@@ -726,6 +726,7 @@ __global__ void __launch_bounds__(256, 4)
     const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
     if (w >= ntiles) return;
     const int t = tile_node[w], row0 = tile_row0[w];
+    if (t < 0) return;  // padding tile
     const int rt = cptr_dst[t + 1] - cptr_dst[t];
     const int C = ncols[t];
     const int nrows = min(16, rt - row0);
@@ -1314,7 +1315,7 @@ void order_leaves(SideDev& sd, int mode, hipStream_t st) {
 
 // Launch order of the forward-coupling tiles (16 rows of one test cluster):
 // 0 node id, 1 depth-first point order (nested ranges after their parent),
-// 2 decreasing row work (columns).
+// 2 decreasing row work (columns), 3 per level, one contiguous run per XCD.
 void order_coupling_tiles(nca_h2* h, int mode, hipStream_t st) {
     const int64_t ntest = h->test.nn;
     const std::vector<int32_t>& ncols = h->h_ncols;
@@ -1330,12 +1331,56 @@ void order_coupling_tiles(nca_h2* h, int mode, hipStream_t st) {
     if (mode == 2)
         std::stable_sort(torder.begin(), torder.end(), [&](int64_t a, int64_t b) { return ncols[a] > ncols[b]; });
     std::vector<int32_t> tnode, trow;
-    for (const int64_t t : torder)
-        if (ncols[t] > 0)
-            for (int r0 = 0; r0 < h->coupling->testrank[t]; r0 += 16) {
-                tnode.push_back((int32_t)t);
-                trow.push_back(r0);
-            }
+    if (mode != 3) {
+        for (const int64_t t : torder)
+            if (ncols[t] > 0)
+                for (int r0 = 0; r0 < h->coupling->testrank[t]; r0 += 16) {
+                    tnode.push_back((int32_t)t);
+                    trow.push_back(r0);
+                }
+    } else {
+        // 3: level by level (node id order inside a level), each level's tiles
+        // cut into 8 contiguous runs dealt so that workgroup b (4 tiles, placed
+        // on XCD b mod 8) takes its XCD's run; runs are padded with empty
+        // tiles (node -1)
+        const SideDev& sd = h->test;
+        std::vector<int64_t> dfs(ntest);
+        for (int64_t t = 0; t < ntest; ++t) dfs[t] = t;
+        std::sort(dfs.begin(), dfs.end(), [&](int64_t a, int64_t b) {
+            if (sd.h_lo[a] != sd.h_lo[b]) return sd.h_lo[a] < sd.h_lo[b];
+            if (sd.h_hi[a] != sd.h_hi[b]) return sd.h_hi[a] > sd.h_hi[b];
+            return a < b;
+        });
+        std::vector<int> depth(ntest, 0);
+        std::vector<int64_t> stack;
+        for (const int64_t t : dfs) {
+            while (!stack.empty() && !(sd.h_lo[t] >= sd.h_lo[stack.back()] && sd.h_hi[t] <= sd.h_hi[stack.back()]))
+                stack.pop_back();
+            depth[t] = (int)stack.size();
+            stack.push_back(t);
+        }
+        const int maxd = ntest ? *std::max_element(depth.begin(), depth.end()) : 0;
+        for (int d = 0; d <= maxd; ++d) {
+            std::vector<int32_t> ln, lr;
+            for (int64_t t = 0; t < ntest; ++t)
+                if (depth[t] == d && ncols[t] > 0)
+                    for (int r0 = 0; r0 < h->coupling->testrank[t]; r0 += 16) {
+                        ln.push_back((int32_t)t);
+                        lr.push_back(r0);
+                    }
+            const int64_t nblk = ((int64_t)ln.size() + 3) / 4, per = (nblk + 7) / 8;
+            for (int64_t i = 0; i < per; ++i)
+                for (int64_t x = 0; x < 8; ++x) {
+                    const int64_t blk = x * per + i;
+                    for (int u = 0; u < 4; ++u) {
+                        const int64_t k = blk * 4 + u;
+                        const bool ok = blk < nblk && k < (int64_t)ln.size() && i < per;
+                        tnode.push_back(ok ? ln[k] : -1);
+                        trow.push_back(ok ? lr[k] : 0);
+                    }
+                }
+        }
+    }
     h->f_ntiles = (int64_t)tnode.size();
     h->f_tile_node.upload(tnode, st);
     h->f_tile_row0.upload(trow, st);
@@ -1685,7 +1730,7 @@ int32_t nca_h2_autotune(nca_h2* h2, int32_t reps, nca_h2_tuning* out) {
         double cbest = c0;
         if (h2->f_ntiles > 0)
             for (int k3 = 0; k3 < 2; ++k3)
-                for (int o = 0; o < 3; ++o)
+                for (int o = 0; o < 4; ++o)
                     for (int xo = 0; xo < 2; ++xo) {
                         h2->couple_k3[prec] = k3;
                         h2->couple_xcd = xo;
