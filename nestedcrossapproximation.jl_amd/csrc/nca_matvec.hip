@@ -463,6 +463,7 @@ struct PrecF64 {
     using T = double;
     using T2 = double2;
     using V4 = dbl4;
+    static constexpr bool kF64 = true;
     __device__ static V4 mma(T a, T b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
     __device__ static int row(int kk, int v) { return kk + 4 * v; }
     __device__ static T2 cvt(double2 v) { return v; }
@@ -480,6 +481,7 @@ struct PrecF32 {
     using T = float;
     using T2 = float2;
     using V4 = flt4;
+    static constexpr bool kF64 = false;
     __device__ static V4 mma(T a, T b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
     __device__ static int row(int kk, int v) { return 4 * kk + v; }
     __device__ static T2 cvt(double2 v) { return make_float2((float)v.x, (float)v.y); }
@@ -529,6 +531,48 @@ struct Cacc {
     __device__ __forceinline__ double2 get(int v) const {
         const double r1 = p1[v], r2 = p2[v], r3 = p3[v];
         return K3 ? make_double2(r1 - r2, r3 - r1 - r2) : make_double2(r1 - r2, r3 + (double)p4[v]);
+    }
+};
+
+// Real embedding of a complex product for tiles of at most HE rows (HE = 8 or
+// 4): MFMA rows [0, HE) carry the real part and [HE, 2 HE) the imaginary part
+// of the same output rows; two real MFMAs per step, D += A1 Br + A2 Bi with
+// A1 = (Ar; Ai), A2 = (-Ai; Ar). The caller gives every lane the complex A
+// entry of row (lane % 16) % HE (lanes of MFMA rows >= 2 HE: any finite value).
+template <class P, int HE>
+struct Cemb {
+    typename P::V4 d;
+    __device__ __forceinline__ void zero() {
+        for (int v = 0; v < 4; ++v) d[v] = 0;
+    }
+    __device__ __forceinline__ void mma(typename P::T2 a, typename P::T2 b) {
+        const bool re = (threadIdx.x & 15) < HE;
+        const typename P::T a1 = re ? a.x : a.y;
+        const typename P::T a2 = re ? -a.y : a.x;
+        d = P::mma(a1, b.x, d);
+        d = P::mma(a2, b.y, d);
+    }
+    __device__ __forceinline__ void scale(typename P::T f) {
+        for (int v = 0; v < 4; ++v) d[v] *= f;
+    }
+    // emit(row, value) for the output rows (< HE) this lane holds; every lane
+    // of the wave must call it (cross-lane exchange in FP32).
+    template <class F>
+    __device__ __forceinline__ void store(F emit) const {
+        const int kk = (threadIdx.x & 63) >> 4;
+        if constexpr (P::kF64) {  // lane holds MFMA rows kk + 4 v
+            if (HE == 8) {
+                emit(kk, make_double2(d[0], d[2]));
+                emit(kk + 4, make_double2(d[1], d[3]));
+            } else {
+                emit(kk, make_double2(d[0], d[1]));
+            }
+        } else {  // lane holds MFMA rows 4 kk + v
+            float im[4];
+            for (int v = 0; v < 4; ++v) im[v] = __shfl_xor(d[v], HE == 8 ? 32 : 16);
+            if (kk < HE / 4)
+                for (int v = 0; v < 4; ++v) emit(4 * kk + v, make_double2(d[v], im[v]));
+        }
     }
 };
 
@@ -708,40 +752,16 @@ __global__ void __launch_bounds__(256, 4)
     }
 }
 
-// Stage 3 forward for 16 right-hand sides: one wave per 16-row tile of a test
-// cluster; A = the r_t x C coupling row (four columns per step), B = the
-// source coefficients of those columns. Control flow is uniform over the wave;
-// offsets inside a block and into xh are 32-bit (coefficients x right-hand
-// sides < 2^31).
-template <class P, bool CJ, bool K3>
-__global__ void __launch_bounds__(256, 4)
-    k_couple_fwd_mfma(const typename P::T2* __restrict__ S, const int32_t* __restrict__ tile_node,
-                      const int32_t* __restrict__ tile_row0, int64_t ntiles, const int64_t* __restrict__ base,
-                      const int64_t* __restrict__ colbase, const int32_t* __restrict__ ncols,
-                      const int32_t* __restrict__ colcoef, const int32_t* __restrict__ cptr_dst,
-                      const typename P::T2* __restrict__ xq0, uint32_t xstride, int64_t R, int64_t q0,
-                      double2* __restrict__ yh, int order) {
+// Stage 3 forward, one tile: acc += (coupling row of the tile) (source
+// coefficients), four columns per step. Software pipeline over groups of kU
+// steps, two register sets in turn: the operands of group n + 1 and the
+// coefficient indices of group n + 2 are in flight while the MFMAs of group n
+// run. Columns past C repeat column C - 1; their B is zeroed at use.
+template <class P, bool CJ, class ACC>
+__device__ __forceinline__ void couple_run(ACC& acc, const typename P::T2* __restrict__ St, uint32_t ur,
+                                           const int32_t* __restrict__ cc, int C,
+                                           const typename P::T2* __restrict__ xq, uint32_t xstride, int kk) {
     using T2 = typename P::T2;
-    const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
-    if (w >= ntiles) return;
-    const int t = tile_node[w], row0 = tile_row0[w];
-    if (t < 0) return;  // padding tile
-    const int rt = cptr_dst[t + 1] - cptr_dst[t];
-    const int C = ncols[t];
-    const int nrows = min(16, rt - row0);
-    const int i = lane & 15, kk = lane >> 4, j = lane & 15;
-    const int rowc = row0 + min(i, nrows - 1);
-    const T2* St = S + base[t] + rowc;
-    const int32_t* cc = colcoef + colbase[t];
-    const T2* xq = xq0 + j;
-    const uint32_t ur = (uint32_t)rt;
-    Cacc<P, K3> acc;
-    acc.zero();
-    // Software pipeline over groups of kU steps (four columns each), two
-    // register sets in turn: the operands of group n + 1 and the coefficient
-    // indices of group n + 2 are in flight while the MFMAs of group n run.
-    // Columns past C repeat column C - 1; their B is zeroed at use.
     constexpr int kU = 4;
     T2 a0[kU], b0[kU], a1[kU], b1[kU];
     int ci0[kU], ci1[kU];
@@ -778,13 +798,53 @@ __global__ void __launch_bounds__(256, 4)
         __builtin_amdgcn_sched_barrier(0);
         mma(c0 + 4 * kU, a1, b1);
     }
+}
+
+// Stage 3 forward for 16 right-hand sides: one wave per tile of at most 16
+// rows of a test cluster; A = the r_t x C coupling row, B = the source
+// coefficients of its columns. Tiles of more than 8 rows use the complex
+// (three- or four-product) accumulator, smaller ones the real embedding.
+// Control flow is uniform over the wave; offsets inside a block and into xh
+// are 32-bit (coefficients x right-hand sides < 2^31).
+template <class P, bool CJ, bool K3>
+__global__ void __launch_bounds__(256, 4)
+    k_couple_fwd_mfma(const typename P::T2* __restrict__ S, const int32_t* __restrict__ tile_node,
+                      const int32_t* __restrict__ tile_row0, int64_t ntiles, const int64_t* __restrict__ base,
+                      const int64_t* __restrict__ colbase, const int32_t* __restrict__ ncols,
+                      const int32_t* __restrict__ colcoef, const int32_t* __restrict__ cptr_dst,
+                      const typename P::T2* __restrict__ xq0, uint32_t xstride, int64_t R, int64_t q0,
+                      double2* __restrict__ yh, int order) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
+    if (w >= ntiles) return;
+    const int t = tile_node[w], row0 = tile_row0[w];
+    if (t < 0) return;  // padding tile
+    const int rt = cptr_dst[t + 1] - cptr_dst[t];
+    const int C = ncols[t];
+    const int nrows = min(16, rt - row0);
+    const int i = lane & 15, kk = lane >> 4, j = lane & 15;
+    const int32_t* cc = colcoef + colbase[t];
+    const auto* xq = xq0 + j;
+    double2* out = yh + (int64_t)(cptr_dst[t] + row0) * R + q0 + j;
+    auto emit = [&](int row, double2 v) {
+        if (row < nrows) out[(int64_t)row * R] = cadd(out[(int64_t)row * R], v);
+    };
+    if (nrows > 8) {
+        Cacc<P, K3> acc;
+        acc.zero();
+        couple_run<P, CJ>(acc, S + base[t] + row0 + min(i, nrows - 1), (uint32_t)rt, cc, C, xq, xstride, kk);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int row = P::row(kk, v);
-        if (row < nrows) {
-            double2* out = yh + (int64_t)(cptr_dst[t] + row0 + row) * R + q0 + j;
-            *out = cadd(*out, acc.get(v));
-        }
+        for (int v = 0; v < 4; ++v) emit(P::row(kk, v), acc.get(v));
+    } else if (nrows > 4) {
+        Cemb<P, 8> acc;
+        acc.zero();
+        couple_run<P, CJ>(acc, S + base[t] + row0 + min(i & 7, nrows - 1), (uint32_t)rt, cc, C, xq, xstride, kk);
+        acc.store(emit);
+    } else {
+        Cemb<P, 4> acc;
+        acc.zero();
+        couple_run<P, CJ>(acc, S + base[t] + row0 + min(i & 3, nrows - 1), (uint32_t)rt, cc, C, xq, xstride, kk);
+        acc.store(emit);
     }
 }
 
