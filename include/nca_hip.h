@@ -210,7 +210,7 @@ typedef struct {
     int32_t near_k3, leaf_order, near_xcd_order;
     int32_t couple_k3, couple_tile_order, couple_xcd_order;
     int32_t candidates;
-    int32_t reserved;
+    int32_t near_stagger;                  /* near-field walks start at a tile-dependent range */
     double near_ms, couple_ms;             /* stage times of the chosen configuration */
     double near_ms_start, couple_ms_start; /* stage times of the configuration before tuning */
 } nca_h2_tuning;

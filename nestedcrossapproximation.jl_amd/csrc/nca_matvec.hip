@@ -94,9 +94,10 @@ struct nca_h2 {
     //   in FP32 three raise the MIXED error against FP64 from 1.0e-6 to 1.5e-6;
     // near_xcd / couple_xcd: work_block order (NCA_ORDER_NEAR, NCA_ORDER_COUPLE);
     // leaf_order: order_leaves mode (NCA_LEAF_ORDER);
-    // couple_order: order_coupling_tiles mode (NCA_COUPLE_ORDER).
+    // couple_order: order_coupling_tiles mode (NCA_COUPLE_ORDER);
+    // near_stagger: near-field walks start at a tile-dependent range (NCA_NEAR_STAGGER).
     int near_k3[2] = {1, 1}, couple_k3[2] = {1, 1};
-    int near_xcd = 0, couple_xcd = 0, leaf_order = 0, couple_order = 0;
+    int near_xcd = 0, couple_xcd = 0, leaf_order = 0, couple_order = 0, near_stagger = 0;
     std::vector<int32_t> h_ncols;  // host copy: columns of every test node's coupling row
 };
 
@@ -660,13 +661,19 @@ __device__ __forceinline__ void near_step(Cacc<P, K3>& acc, const NearK& nk, con
 // sides < 2^31).
 template <class P, int H, bool K3, bool NEG>
 __device__ __forceinline__ void near_walk(Cacc<P, K3>& acc, const NearK& nk, const double2* tab, int lane,
-                                          double x0, double x1, double x2, int64_t e0, int64_t e1,
+                                          double x0, double x1, double x2, int64_t e0, int64_t e1, int rot,
                                           const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
                                           const double* __restrict__ spts, const typename P::T2* __restrict__ xsl,
                                           uint32_t R) {
     constexpr int S = 4 * (16 / H);
     const int kk = lane >> 4, ysrc = 4 * ((lane & 15) / H) + kk;
-    for (int64_t e = e0; e < e1; ++e) {
+    const int nr = (int)(e1 - e0);
+    for (int c = 0; c < nr; ++c) {
+        // rot: the walk starts at range rot mod nr (tiles that share sources
+        // then read them at different times)
+        int ec = c + (nr ? rot % nr : 0);
+        if (ec >= nr) ec -= nr;
+        const int64_t e = e0 + ec;
         const int lo = nrlo[e], hi = nrhi[e];
         int p = lo;
         for (; p + S <= hi; p += S)
@@ -695,7 +702,7 @@ __global__ void __launch_bounds__(256, 4)
                     const double* __restrict__ spts, const double2* __restrict__ yh,
                     const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, NearK nk,
                     const double2* __restrict__ sctab, double2 alpha, double2 beta, double2* __restrict__ y,
-                    int64_t ldy, int order) {
+                    int64_t ldy, int order, int stagger) {
     using T2 = typename P::T2;
     __shared__ double2 tab[P::kTab ? kSinCosTab : 1];
     if (P::kTab) {
@@ -719,12 +726,13 @@ __global__ void __launch_bounds__(256, 4)
     const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
     const T2* xsl = xs + q0 + j;
     const int64_t e0 = nrptr[t], e1 = nrptr[t + 1];
+    const int rot = stagger ? w * 7 : 0;
     if (H == 16)
-        near_walk<P, 16, K3, NEG>(acc, nk, tab, lane, x0, x1, x2, e0, e1, nrlo, nrhi, spts, xsl, (uint32_t)R);
+        near_walk<P, 16, K3, NEG>(acc, nk, tab, lane, x0, x1, x2, e0, e1, rot, nrlo, nrhi, spts, xsl, (uint32_t)R);
     else if (H == 8)
-        near_walk<P, 8, K3, NEG>(acc, nk, tab, lane, x0, x1, x2, e0, e1, nrlo, nrhi, spts, xsl, (uint32_t)R);
+        near_walk<P, 8, K3, NEG>(acc, nk, tab, lane, x0, x1, x2, e0, e1, rot, nrlo, nrhi, spts, xsl, (uint32_t)R);
     else
-        near_walk<P, 4, K3, NEG>(acc, nk, tab, lane, x0, x1, x2, e0, e1, nrlo, nrhi, spts, xsl, (uint32_t)R);
+        near_walk<P, 4, K3, NEG>(acc, nk, tab, lane, x0, x1, x2, e0, e1, rot, nrlo, nrhi, spts, xsl, (uint32_t)R);
     acc.scale((typename P::T)kInvFourPi);
     if (bi >= 0) {
         const int rowc = row0 + min(i, nrows - 1);
@@ -1089,7 +1097,8 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
         hipLaunchKernelGGL(kern, dim3(blocks_for(dst.ntiles, 4)), dim3(256), 0, st, dst.tile_leaf.p,
                            dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst),
                            dst.lo.p, dst.hi.p, dst.perm.p, dst.cptr.p, dst.pts.p, dst.nrptr.p, dst.nrlo.p, dst.nrhi.p,
-                           src.pts.p, yh, xs, R, q0, nk, h->sctab.p, alpha, beta, y, ldy, h->near_xcd);
+                           src.pts.p, yh, xs, R, q0, nk, h->sctab.p, alpha, beta, y, ldy, h->near_xcd,
+                           h->near_stagger);
     } else if (NR != 16 && dst.nleaves > 0) {
         hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ, T2, T2>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st,
                            dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p,
@@ -1511,6 +1520,7 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
         if (const char* v = getenv("NCA_ORDER_COUPLE")) h->couple_xcd = atoi(v);
         if (const char* v = getenv("NCA_LEAF_ORDER")) h->leaf_order = atoi(v);
         if (const char* v = getenv("NCA_COUPLE_ORDER")) h->couple_order = atoi(v);
+        if (const char* v = getenv("NCA_NEAR_STAGGER")) h->near_stagger = atoi(v) != 0;
         build_side(h->test, st, d->M, d->nn_test, d->test_perm, d->test_lo, d->test_hi, d->test_cptr, test,
                    d->nbasis_test, d->basis_test_nodes, d->basis_test_off, d->basis_test, true, d->nlev_test,
                    d->tt_levptr, d->tt_levidcs, d->tt_chdptr, d->tt_chds, d->tt_off, d->tt_blocks);
@@ -1760,13 +1770,14 @@ int32_t nca_h2_autotune(nca_h2* h2, int32_t reps, nca_h2_tuning* out) {
         r.near_ms_start = n0;
         int count = 0;
         // near field: products x leaf order x workgroup order
-        int bk3 = h2->near_k3[prec], blo = h2->leaf_order, bx = h2->near_xcd;
+        int bk3 = h2->near_k3[prec], blo = h2->leaf_order, bx = h2->near_xcd, bst = h2->near_stagger;
         double best = n0;
         for (int k3 = 0; k3 < 2; ++k3)
             for (int lo = 0; lo < 3; ++lo)
-                for (int xo = 0; xo < 2; ++xo) {
+                for (int xo = 0; xo < 4; ++xo) {
                     h2->near_k3[prec] = k3;
-                    h2->near_xcd = xo;
+                    h2->near_xcd = xo & 1;
+                    h2->near_stagger = xo >> 1;
                     if (h2->leaf_order != lo) {
                         order_leaves(h2->test, lo, st);
                         order_leaves(h2->trial, lo, st);
@@ -1775,10 +1786,11 @@ int32_t nca_h2_autotune(nca_h2* h2, int32_t reps, nca_h2_tuning* out) {
                     double c, n;
                     time_stages(h2, x.p, y.p, reps, c, n);
                     ++count;
-                    if (n < best) { best = n; bk3 = k3; blo = lo; bx = xo; }
+                    if (n < best) { best = n; bk3 = k3; blo = lo; bx = xo & 1; bst = xo >> 1; }
                 }
         h2->near_k3[prec] = bk3;
         h2->near_xcd = bx;
+        h2->near_stagger = bst;
         if (h2->leaf_order != blo) {
             order_leaves(h2->test, blo, st);
             order_leaves(h2->trial, blo, st);
@@ -1813,6 +1825,7 @@ int32_t nca_h2_autotune(nca_h2* h2, int32_t reps, nca_h2_tuning* out) {
         r.near_k3 = bk3;
         r.leaf_order = blo;
         r.near_xcd_order = bx;
+        r.near_stagger = bst;
         r.couple_k3 = ck3;
         r.couple_tile_order = co;
         r.couple_xcd_order = cx;

@@ -49,7 +49,7 @@ class H2Stats(ctypes.Structure):
 
 class H2Tuning(ctypes.Structure):
     _fields_ = [(n, I32) for n in ("near_k3", "leaf_order", "near_xcd_order", "couple_k3", "couple_tile_order",
-                                   "couple_xcd_order", "candidates", "reserved")] + \
+                                   "couple_xcd_order", "candidates", "near_stagger")] + \
                [(n, ctypes.c_double) for n in ("near_ms", "couple_ms", "near_ms_start", "couple_ms_start")]
 
 

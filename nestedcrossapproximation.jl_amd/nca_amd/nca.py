@@ -132,7 +132,7 @@ class PetrovGalerkinNCACore:
         current precision and keep the fastest (nca_h2_autotune)."""
         t = _abi.H2Tuning()
         call(hip_lib().nca_h2_autotune(self.handle, int(reps), ctypes.byref(t)))
-        return {n: getattr(t, n) for n, _ in _abi.H2Tuning._fields_ if n != "reserved"}
+        return {n: getattr(t, n) for n, _ in _abi.H2Tuning._fields_}
 
     def graph(self, op, nrhs, xptr, ldx, yptr, ldy, alpha=1.0, beta=0.0):
         """Capture matvec_device(op, nrhs, ...) on these device buffers as a
