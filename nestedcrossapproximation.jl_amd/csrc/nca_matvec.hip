@@ -11,6 +11,7 @@
 // Coefficient vectors hold the R right-hand sides interleaved: xh[k * R + q].
 
 #include <algorithm>
+#include <type_traits>
 #include <cstdlib>
 #include <memory>
 
@@ -628,8 +629,8 @@ __device__ __forceinline__ T2 rot16v(T2 v) {
 // stored), and B = xs of sources p + 4 m .. p + 4 m + 3. MASK (the last step of
 // a range): sources at or past hi read the rows after them (other points or
 // the zero padding) and contribute B = 0. xp/yp: xs and points at position p.
-template <class P, int H, bool K3, bool NEG, bool MASK>
-__device__ __forceinline__ void near_step(Cacc<P, K3>& acc, const NearK& nk, const double2* tab, int kk,
+template <class P, int H, class ACC, bool EMB, bool NEG, bool MASK>
+__device__ __forceinline__ void near_step(ACC& acc, const NearK& nk, const double2* tab, int kk,
                                           int ysrc, double x0, double x1, double x2, int p, int hi,
                                           const double* __restrict__ yp, const typename P::T2* __restrict__ xp,
                                           uint32_t R) {
@@ -643,14 +644,26 @@ __device__ __forceinline__ void near_step(Cacc<P, K3>& acc, const NearK& nk, con
         if (MASK && p + 4 * m + kk >= hi) b[m] = czero2<P>();
     }
     const T2 g = P::template eval<NEG>(nk, tab, x0, x1, x2, ys[0], ys[1], ys[2]);
+    const int i = threadIdx.x & 15;
 #pragma unroll
     for (int m = 0; m < G; ++m) {
         if (MASK && p + 4 * m >= hi) continue;
         T2 a;
-        if (m == 0) a = g;
-        else if (m == 1) a = rot16v<H>(g);
-        else if (m == 2) a = rot16v<(2 * H) & 15>(g);
-        else a = rot16v<(3 * H) & 15>(g);
+        if constexpr (!EMB) {
+            if (m == 0) a = g;
+            else if (m == 1) a = rot16v<H>(g);
+            else if (m == 2) a = rot16v<(2 * H) & 15>(g);
+            else a = rot16v<(3 * H) & 15>(g);
+        } else if constexpr (H == 8) {
+            // embedding: lanes i and i + 8 both need row i % 8 of group m
+            const T2 r8 = rot16v<8>(g);
+            a = ((i < 8) == (m == 0)) ? g : r8;
+        } else {
+            // embedding, 4 rows: lanes 0..3 need lane 4 m + i, lanes 4..7 lane 4 m + i - 4
+            const T2 ra = m == 0 ? g : (m == 1 ? rot16v<4>(g) : (m == 2 ? rot16v<8>(g) : rot16v<12>(g)));
+            const T2 rb = m == 0 ? rot16v<12>(g) : (m == 1 ? g : (m == 2 ? rot16v<4>(g) : rot16v<8>(g)));
+            a = i < 4 ? ra : rb;
+        }
         acc.mma(a, b[m]);
     }
 }
@@ -659,8 +672,8 @@ __device__ __forceinline__ void near_step(Cacc<P, K3>& acc, const NearK& nk, con
 // without masks, then one masked step for the rest of the range. Control flow
 // is uniform over the wave. The xs offsets are 32-bit (points x right-hand
 // sides < 2^31).
-template <class P, int H, bool K3, bool NEG>
-__device__ __forceinline__ void near_walk(Cacc<P, K3>& acc, const NearK& nk, const double2* tab, int lane,
+template <class P, int H, class ACC, bool EMB, bool NEG>
+__device__ __forceinline__ void near_walk(ACC& acc, const NearK& nk, const double2* tab, int lane,
                                           double x0, double x1, double x2, int64_t e0, int64_t e1, int rot,
                                           const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
                                           const double* __restrict__ spts, const typename P::T2* __restrict__ xsl,
@@ -677,10 +690,10 @@ __device__ __forceinline__ void near_walk(Cacc<P, K3>& acc, const NearK& nk, con
         const int lo = nrlo[e], hi = nrhi[e];
         int p = lo;
         for (; p + S <= hi; p += S)
-            near_step<P, H, K3, NEG, false>(acc, nk, tab, kk, ysrc, x0, x1, x2, p, hi, spts + 3 * (int64_t)p,
+            near_step<P, H, ACC, EMB, NEG, false>(acc, nk, tab, kk, ysrc, x0, x1, x2, p, hi, spts + 3 * (int64_t)p,
                                        xsl + (uint32_t)p * R, R);
         if (p < hi)
-            near_step<P, H, K3, NEG, true>(acc, nk, tab, kk, ysrc, x0, x1, x2, p, hi, spts + 3 * (int64_t)p,
+            near_step<P, H, ACC, EMB, NEG, true>(acc, nk, tab, kk, ysrc, x0, x1, x2, p, hi, spts + 3 * (int64_t)p,
                                       xsl + (uint32_t)p * R, R);
     }
 }
@@ -718,45 +731,55 @@ __global__ void __launch_bounds__(256, 4)
     const int l0 = dlo[t], nt = dhi[t] - l0;
     const int nrows = min(16, nt - row0);
     const int i = lane & 15, kk = lane >> 4, j = lane & 15;
-    Cacc<P, K3> acc;
-    acc.zero();
-    // near field first (4 pi G), scaled by 1 / (4 pi) before the basis rows
-    const int H = nrows > 8 ? 16 : (nrows > 4 ? 8 : 4);
-    const double* xp = dpts + 3 * (int64_t)(l0 + row0 + min(i & (H - 1), nrows - 1));
-    const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
-    const T2* xsl = xs + q0 + j;
     const int64_t e0 = nrptr[t], e1 = nrptr[t + 1];
     const int rot = stagger ? w * 7 : 0;
-    if (H == 16)
-        near_walk<P, 16, K3, NEG>(acc, nk, tab, lane, x0, x1, x2, e0, e1, rot, nrlo, nrhi, spts, xsl, (uint32_t)R);
-    else if (H == 8)
-        near_walk<P, 8, K3, NEG>(acc, nk, tab, lane, x0, x1, x2, e0, e1, rot, nrlo, nrhi, spts, xsl, (uint32_t)R);
-    else
-        near_walk<P, 4, K3, NEG>(acc, nk, tab, lane, x0, x1, x2, e0, e1, rot, nrlo, nrhi, spts, xsl, (uint32_t)R);
-    acc.scale((typename P::T)kInvFourPi);
-    if (bi >= 0) {
-        const int rowc = row0 + min(i, nrows - 1);
-        const int rt = dcptr[t + 1] - dcptr[t];
-        const T2* B = bblocks + boff[bi];
-        const double2* yt = yh + (int64_t)dcptr[t] * R + q0 + j;
-        for (int k0 = 0; k0 < rt; k0 += 4) {
-            const int k = k0 + kk;
-            const int kc = min(k, rt - 1);
-            const T2 a = conjif<CJ, P>(B[FWD ? (int64_t)rowc + (int64_t)kc * nt : (int64_t)kc + (int64_t)rowc * rt]);
-            T2 b = P::cvt(yt[(int64_t)kc * R]);
-            if (k >= rt) b = czero2<P>();
-            acc.mma(a, b);
+    const T2* xsl = xs + q0 + j;
+    const double2* yt = yh + (int64_t)dcptr[t] * R + q0 + j;
+    const int rt = dcptr[t + 1] - dcptr[t];
+    const T2* B = bi >= 0 ? bblocks + boff[bi] : nullptr;
+    // near field first (4 pi G), scaled by 1 / (4 pi), then the basis rows
+    auto tile = [&](auto& acc, auto hval, auto emb, int rsel) {
+        constexpr int H = decltype(hval)::value;
+        constexpr bool EMB = decltype(emb)::value;
+        acc.zero();
+        const double* xp = dpts + 3 * (int64_t)(l0 + row0 + min(rsel, nrows - 1));
+        near_walk<P, H, std::remove_reference_t<decltype(acc)>, EMB, NEG>(acc, nk, tab, lane, xp[0], xp[1], xp[2], e0,
+                                                                         e1, rot, nrlo, nrhi, spts, xsl, (uint32_t)R);
+        acc.scale((typename P::T)kInvFourPi);
+        if (B) {
+            const int rowc = row0 + min(rsel, nrows - 1);
+            for (int k0 = 0; k0 < rt; k0 += 4) {
+                const int k = k0 + kk;
+                const int kc = min(k, rt - 1);
+                const T2 a =
+                    conjif<CJ, P>(B[FWD ? (int64_t)rowc + (int64_t)kc * nt : (int64_t)kc + (int64_t)rowc * rt]);
+                T2 b = P::cvt(yt[(int64_t)kc * R]);
+                if (k >= rt) b = czero2<P>();
+                acc.mma(a, b);
+            }
         }
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int row = P::row(kk, v);
+    };
+    auto emit = [&](int row, double2 v) {
         if (row < nrows) {
             double2* out = y + (int64_t)dperm[l0 + row0 + row] + (q0 + j) * ldy;
-            double2 val = cmul(alpha, acc.get(v));
+            double2 val = cmul(alpha, v);
             if (beta.x != 0.0 || beta.y != 0.0) val = cadd(val, cmul(beta, *out));
             *out = val;
         }
+    };
+    if (nrows > 8) {
+        Cacc<P, K3> acc;
+        tile(acc, std::integral_constant<int, 16>(), std::false_type(), i);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) emit(P::row(kk, v), acc.get(v));
+    } else if (nrows > 4) {
+        Cemb<P, 8> acc;
+        tile(acc, std::integral_constant<int, 8>(), std::true_type(), i & 7);
+        acc.store(emit);
+    } else {
+        Cemb<P, 4> acc;
+        tile(acc, std::integral_constant<int, 4>(), std::true_type(), i & 3);
+        acc.store(emit);
     }
 }
 
