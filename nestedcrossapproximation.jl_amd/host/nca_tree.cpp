@@ -333,20 +333,21 @@ Partition interactions(const TreeView& a, const TreeView& b, double eta) {
 
 // Phase 1 stand-in for geometric (tree-mimicry) pivoting. For node t:
 //   F(t) = own far nodes, then the far nodes of every ancestor;
-//   n = min(maxrank, |t|) capped by the points F(t) holds (_clustersize);
+//   n = min(maxrank, ceil(oversample |t|)) capped by the points F(t) holds
+//   (the reference's _clustersize takes oversample = 1);
 //   candidates = an even subsample of every far node's points;
 //   first pivot = candidate nearest to the center of t, every next pivot
-//   maximizes d_min^2 / d_c^(2 kFarWeight), d_min = distance to the chosen
+//   maximizes d_min^2 / d_c^(2 weight), d_min = distance to the chosen
 //   pivots, d_c = distance to the center of t.
-// Ties go to the first candidate in far-field order. kFarWeight > 1 favours the
-// nearer far field, where the kernel varies fastest: on the 1M-point sphere
-// (k = 2, tol 1e-4) it lowers the H^2 error against direct summation from
-// 6.9e-4 (weight 1) to 2.1e-4 (weight 1.2).
-constexpr double kFarWeight = 1.2;
+// Ties go to the first candidate in far-field order. The defaults of the host
+// mirror (weight 1, oversample 2) were measured against dense direct summation:
+// 65,536-point sphere, k = 1, tol 1e-6: 2.5 tol (oversample 1: 18 tol);
+// 1M-point sphere, k = 2, tol 1e-4: 2.4 tol with 6.2e8 coupling entries
+// (oversample 1, weight 1.2: 2.0 tol, 6.9e8 entries).
 Csr farpivots(int64_t nn_self, const int64_t* parent, const int64_t* lo, const int64_t* hi,
               const double* center, const int64_t* other_lo, const int64_t* other_hi,
               const int64_t* other_perm, const double* other_pts, const int64_t* farptr,
-              const int64_t* farnodes, int64_t maxrank) {
+              const int64_t* farnodes, int64_t maxrank, double weight, double oversample) {
     std::vector<std::vector<int64_t>> result(nn_self);
 #pragma omp parallel
     {
@@ -358,7 +359,8 @@ Csr farpivots(int64_t nn_self, const int64_t* parent, const int64_t* lo, const i
             for (int64_t node = t; node >= 0; node = parent[node])
                 for (int64_t k = farptr[node]; k < farptr[node + 1]; ++k) F.push_back(farnodes[k]);
             if (F.empty()) continue;
-            const int64_t limit = std::min<int64_t>(maxrank, hi[t] - lo[t]);
+            const int64_t limit =
+                std::min<int64_t>(maxrank, (int64_t)std::ceil(oversample * (double)(hi[t] - lo[t])));
             int64_t n = 0, total = 0;
             for (int64_t f : F) {
                 const int64_t sz = other_hi[f] - other_lo[f];
@@ -387,7 +389,7 @@ Csr farpivots(int64_t nn_self, const int64_t* parent, const int64_t* lo, const i
                 d2c[q] = std::max(dist2(other_pts + 3 * pool[q], ct), 1e-300);
                 if (d2c[q] < d2c[first]) first = q;
             }
-            for (int64_t q = 0; q < np; ++q) d2c[q] = std::pow(d2c[q], kFarWeight);
+            for (int64_t q = 0; q < np; ++q) d2c[q] = std::pow(d2c[q], weight);
             std::vector<int64_t>& out = result[t];
             out.reserve(n);
             int64_t cur = first;
@@ -486,9 +488,10 @@ void ncat_partition_free(ncat_partition* h) { delete reinterpret_cast<Partition*
 ncat_csr* ncat_farpivots(int64_t nn_self, const int64_t* parent, const int64_t* lo, const int64_t* hi,
                          const double* center, const int64_t* other_lo, const int64_t* other_hi,
                          const int64_t* other_perm, const double* other_pts, const int64_t* farptr,
-                         const int64_t* farnodes, int64_t maxrank) {
+                         const int64_t* farnodes, int64_t maxrank, double weight, double oversample) {
     return reinterpret_cast<ncat_csr*>(new Csr(farpivots(nn_self, parent, lo, hi, center, other_lo, other_hi,
-                                                         other_perm, other_pts, farptr, farnodes, maxrank)));
+                                                         other_perm, other_pts, farptr, farnodes, maxrank, weight,
+                                                         oversample)));
 }
 
 int64_t ncat_csr_nnz(const ncat_csr* h) { return (int64_t)reinterpret_cast<const Csr*>(h)->idx.size(); }

@@ -67,7 +67,7 @@ def _declare_tree(L):
     L.ncat_partition_free.restype = None
     L.ncat_partition_free.argtypes = [P]
     L.ncat_farpivots.restype = P
-    L.ncat_farpivots.argtypes = [I64, P, P, P, P, P, P, P, P, P, P, I64]
+    L.ncat_farpivots.argtypes = [I64, P, P, P, P, P, P, P, P, P, P, I64, ctypes.c_double, ctypes.c_double]
     L.ncat_csr_nnz.restype = I64
     L.ncat_csr_nnz.argtypes = [P]
     L.ncat_csr_export.restype = None

@@ -46,10 +46,16 @@ class ACA:
 
 
 class GeometricPivoting:
-    """Phase-1 far-pivot selector (stand-in for TreeMimicryPivoting; see nca_tree.cpp)."""
+    """Phase-1 far-pivot selector (stand-in for TreeMimicryPivoting; see
+    nca_tree.cpp): up to min(maxrank, oversample |t|) far points per cluster,
+    spread by maximal separation relative to (distance to the cluster)^weight."""
+
+    def __init__(self, weight=1.0, oversample=2.0):
+        self.weight = float(weight)
+        self.oversample = float(oversample)
 
     def select(self, selftree, othertree, fd, maxrank):
-        return _trees.farpivots(selftree, othertree, fd, maxrank)
+        return _trees.farpivots(selftree, othertree, fd, maxrank, self.weight, self.oversample)
 
 
 class BatchedBottomUp:

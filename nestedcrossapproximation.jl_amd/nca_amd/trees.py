@@ -186,7 +186,7 @@ def farfield(tree, fd, node):
     return np.asarray(out, dtype=np.int64)
 
 
-def farpivots(selftree, othertree, fd, maxrank):
+def farpivots(selftree, othertree, fd, maxrank, weight=1.0, oversample=2.0):
     """Phase 1 (batchedbottomup.jl:257-266): geometric far pivots of every node
     of ``selftree`` among the points of ``othertree``. Returns a 0-based CSR
     (ptr, point ids)."""
@@ -194,7 +194,7 @@ def farpivots(selftree, othertree, fd, maxrank):
     pts = np.ascontiguousarray(othertree.points)
     h = L.ncat_farpivots(selftree.nnodes, _ptr(selftree.parent), _ptr(selftree.lo), _ptr(selftree.hi),
                          _ptr(selftree.center), _ptr(othertree.lo), _ptr(othertree.hi), _ptr(othertree.perm),
-                         _ptr(pts), _ptr(fd.nodetofarnodesptr), _ptr(fd.farnodes), int(maxrank))
+                         _ptr(pts), _ptr(fd.nodetofarnodesptr), _ptr(fd.farnodes), int(maxrank), float(weight), float(oversample))
     nnz = L.ncat_csr_nnz(h)
     ptr = np.empty(selftree.nnodes + 1, np.int64)
     idx = np.empty(nnz, np.int64)

@@ -359,3 +359,28 @@ def test_matvec_negative_phase_sign(ctx):
             assert relerr(W @ x, H.matvec(x)) <= bound
             assert relerr(NC.transpose(W) @ x, H.matvec(x, op=1)) <= bound
             assert relerr(NC.adjoint(W) @ x, H.matvec(x, op=2)) <= bound
+
+
+def test_config_sphere65k_fp64(ctx):
+    """The FP64 one-GPU configuration at its full size: N = 65,536 Fibonacci
+    sphere, k = 1, octree minvalues 64, tol 1e-6, one seeded complex Gaussian
+    right-hand side. Pivots, ranks, factors and coupling bit for bit against the
+    oracle, the product within 1e-12 of the oracle's, and within 5 tol of dense
+    direct summation on 200 sampled rows."""
+    say("config sphere 65536 fp64")
+    p = sphere_problem(65536, k=1.0, tol=1e-6)
+    A, ft, fs = gpu_assembly(ctx, p)
+    oft, ofs = oracle_assembly(p)
+    compare_sides(ft, oft, p.tree.testtree.nnodes)
+    compare_sides(fs, ofs, p.tree.trialtree.nnodes)
+    cs = C.assemble_couplingstore(A, ft.pivot_csr(), fs.pivot_csr(), p.tf)
+    off, blocks = cs.download()
+    H, (ooff, oblocks) = oracle_operator(p, oft, ofs)
+    assert np.array_equal(off, ooff) and bitwise_equal(blocks, oblocks)
+    W = build_operator(ctx, p, A, ft, fs)
+    x = geometry.complex_gaussian(np.random.default_rng(2), (p.trialpts.shape[0],))
+    y = W @ x
+    assert relerr(y, H.matvec(x)[:, 0]) <= 1e-12
+    rows = np.random.default_rng(3).choice(p.testpts.shape[0], 200, replace=False)
+    ref = O.dense_rows(p.k, p.sign, p.testpts, p.trialpts, rows, x)[:, 0]
+    assert relerr(y[rows], ref) < 5 * p.tol

@@ -85,19 +85,24 @@ def test_farfield_and_phase1_pivots():
     tf, _ = nca_amd.fardata(tree, part)
     ptr, idx = nca_amd.farpivots(tr, tr, tf, 50)
     assert np.array_equal(nca_amd.farpivots(tr, tr, tf, 50)[1], idx)  # deterministic
+    # oversample 1 is the reference's count rule _selectfarpivots/_clustersize
+    p1, i1 = nca_amd.farpivots(tr, tr, tf, 50, oversample=1.0)
     for t in range(tr.nnodes):
         F = nca_amd.farfield(tr, tf, t)
         own = tf.farnodesof(t)
         assert list(F[:len(own)]) == list(own)
         piv = idx[ptr[t]:ptr[t + 1]]
+        piv1 = i1[p1[t]:p1[t + 1]]
         if len(F) == 0:
-            assert len(piv) == 0
+            assert len(piv) == 0 and len(piv1) == 0
             continue
         farpts = np.concatenate([tr.values(f) for f in F])
         # _selectfarpivots: min(maxrank, |t|, far points), distinct, from the far field
-        assert len(piv) == min(50, tr.nvalues(t), len(farpts))
-        assert len(set(piv.tolist())) == len(piv)
-        assert set(piv.tolist()) <= set(farpts.tolist())
+        assert len(piv1) == min(50, tr.nvalues(t), len(farpts))
+        assert len(piv) == min(50, 2 * tr.nvalues(t), len(farpts))
+        for pv in (piv, piv1):
+            assert len(set(pv.tolist())) == len(pv)
+            assert set(pv.tolist()) <= set(farpts.tolist())
 
 
 def test_plan_from_pivots():
