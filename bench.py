@@ -224,20 +224,36 @@ def main():
 
     # roofline of the dominant kernel of the headline (MIXED) sweep
     dom = names[int(np.argmax(prof))]
+    traffic_db = {}
+    tpath = os.path.join(ROOT, "profiles", "h2_matvec_sphere1m_rhs16_hbm_traffic.json")
+    if os.path.exists(tpath) and args.n == 1048576 and R == 16:
+        traffic_db = json.load(open(tpath))["kernels"]
+
+    def traffic_of(prefix):
+        for name, t in traffic_db.items():
+            if name.startswith(prefix):
+                return int(t["read_bytes"] + t["write_bytes"])
+        return None
+
     if dom == "leaf_expansion_near":
         # complex FP32 MFMA: 8 flop per near pair and right-hand side, plus the basis rows
         flops = 8.0 * R * (st["near_entries"] + st["basis_entries_test"])
         achieved = flops / (prof[5] * 1e-3) / 1e12
         roofline = {"kernel": "k_leaf_out_mfma<PrecF32> (leaf expansion + near field)", "bound": "mfma",
                     "achieved": round(achieved, 2), "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4), "traffic": None,
-                    "note": "MAC flops only (8 per near pair per RHS); the FP32/FP64 VALU kernel evaluations "
-                            f"({st['near_entries']} per sweep) are not counted"}
+                    "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4),
+                    "traffic": traffic_of("k_leaf_out_mfma<PrecF32"),
+                    "note": "MAC flops only (8 per near pair per RHS); the kernel evaluations "
+                            f"({st['near_entries']} per sweep, VALU) are not counted. VALU and MFMA issue "
+                            "serially on a SIMD; the kernel runs at ~53 % VALU + ~37 % MFMA busy "
+                            "(profiles/h2_matvec_sphere1m_rhs16_sq_counters.md). traffic: HBM bytes per launch "
+                            "from FETCH_SIZE x 2 + WRITE_SIZE (profiles/h2_matvec_sphere1m_rhs16_hbm_traffic.md)"}
     else:
         i = names.index(dom)
         achieved = sbytes[dom] / (prof[i] * 1e-3) / 1e9
         roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic_of("k_couple_fwd_mfma<PrecF32") if dom == "coupling" else None}
 
     result = {
         "metric": "H2 matvec/s + HBM GB/s (N=1M, mixed vs FP64 at 1e-4); assembly time",

@@ -98,7 +98,7 @@ struct nca_h2 {
     // couple_order: order_coupling_tiles mode (NCA_COUPLE_ORDER);
     // near_stagger: near-field walks start at a tile-dependent range (NCA_NEAR_STAGGER).
     int near_k3[2] = {1, 1}, couple_k3[2] = {1, 1};
-    int near_xcd = 0, couple_xcd = 0, leaf_order = 0, couple_order = 0, near_stagger = 0;
+    int near_xcd = 0, couple_xcd = 0, leaf_order = 2, couple_order = 3, near_stagger = 0;
     std::vector<int32_t> h_ncols;  // host copy: columns of every test node's coupling row
 };
 
