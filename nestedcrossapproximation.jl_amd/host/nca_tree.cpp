@@ -494,6 +494,21 @@ ncat_csr* ncat_farpivots(int64_t nn_self, const int64_t* parent, const int64_t* 
                                                          oversample)));
 }
 
+// CSR transpose with the rows of every column in increasing order (a
+// counting sort; farinteractions.jl:66-92).
+ncat_csr* ncat_csr_transpose(int64_t nrows, const int64_t* ptr, const int64_t* idx, int64_t ncols) {
+    Csr* c = new Csr();
+    c->ptr.assign(ncols + 1, 0);
+    const int64_t nnz = ptr[nrows] - ptr[0];
+    for (int64_t k = 0; k < nnz; ++k) c->ptr[idx[ptr[0] + k] + 1]++;
+    for (int64_t j = 0; j < ncols; ++j) c->ptr[j + 1] += c->ptr[j];
+    c->idx.resize(nnz);
+    std::vector<int64_t> cur(c->ptr.begin(), c->ptr.end() - 1);
+    for (int64_t r = 0; r < nrows; ++r)
+        for (int64_t k = ptr[r]; k < ptr[r + 1]; ++k) c->idx[cur[idx[k]]++] = r;
+    return reinterpret_cast<ncat_csr*>(c);
+}
+
 int64_t ncat_csr_nnz(const ncat_csr* h) { return (int64_t)reinterpret_cast<const Csr*>(h)->idx.size(); }
 
 void ncat_csr_export(const ncat_csr* h, int64_t* ptr, int64_t* idx) {

@@ -66,6 +66,8 @@ def _declare_tree(L):
     L.ncat_partition_export.argtypes = [P, P, P, P, P]
     L.ncat_partition_free.restype = None
     L.ncat_partition_free.argtypes = [P]
+    L.ncat_csr_transpose.restype = P
+    L.ncat_csr_transpose.argtypes = [I64, P, P, I64]
     L.ncat_farpivots.restype = P
     L.ncat_farpivots.argtypes = [I64, P, P, P, P, P, P, P, P, P, P, I64, ctypes.c_double, ctypes.c_double]
     L.ncat_csr_nnz.restype = I64

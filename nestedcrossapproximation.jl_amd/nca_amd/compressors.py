@@ -195,32 +195,126 @@ class SideFactors:
         return ptr, idx.astype(np.int64)
 
 
-def _compresslevel(cfg, storage, subtree, activeclusters, childranges, tvalues_ptr, tvalues_idx, out):
+class LevelFactors:
+    """SideFactors of the batched driver in flat arrays: per node its rank, rows
+    m, pivots (a slice of one flat point-id array) and its block X (m x r,
+    column-major) at xoff[t] of the packed factors of all levels. The per-node
+    views pivots[t] and X[t] read like SideFactors."""
+
+    def __init__(self, nn):
+        self.nn = nn
+        self.assigned = np.zeros(nn, bool)
+        self.rank = np.zeros(nn, np.int64)
+        self.m = np.zeros(nn, np.int64)
+        self.pstart = np.zeros(nn, np.int64)
+        self.xoff = np.zeros(nn, np.int64)
+        self._piv = [np.zeros(0, np.int64)]
+        self._npiv = 0
+        self._X = [np.zeros(0, np.complex128)]
+        self._nx = 0
+        self.level_transfer_nodes = {}
+        self.pivots = _NodeView(self, "pivots")
+        self.X = _NodeView(self, "X")
+
+    @property
+    def pivflat(self):
+        if len(self._piv) > 1:
+            self._piv = [np.concatenate(self._piv)]
+        return self._piv[0]
+
+    @property
+    def Xflat(self):
+        if len(self._X) > 1:
+            self._X = [np.concatenate(self._X)]
+        return self._X[0]
+
+    def add_level(self, nodes, rows, ranks, pivots, xoff_level, X_level):
+        """Record the blocks of one level (nodes in block order)."""
+        k = len(nodes)
+        ranks = np.asarray(ranks, np.int64)
+        self.assigned[nodes] = True
+        self.rank[nodes] = ranks
+        self.m[nodes] = rows
+        self.pstart[nodes] = self._npiv + np.cumsum(ranks) - ranks
+        self._piv.append(np.asarray(pivots, np.int64))
+        self._npiv += len(pivots)
+        self.xoff[nodes] = self._nx + xoff_level[:k]
+        self._X.append(np.asarray(X_level, np.complex128))
+        self._nx += len(X_level)
+
+    def pivot_csr(self):
+        ptr = np.zeros(self.nn + 1, np.int64)
+        r = np.where(self.assigned, self.rank, 0)
+        np.cumsum(r, out=ptr[1:])
+        return ptr, self.pivflat[concat_ranges(self.pstart, r)]
+
+
+class _NodeView:
+    def __init__(self, f, what):
+        self.f, self.what = f, what
+
+    def __getitem__(self, t):
+        f = self.f
+        if not f.assigned[t]:
+            return None
+        r, m = int(f.rank[t]), int(f.m[t])
+        if self.what == "pivots":
+            return f.pivflat[f.pstart[t]:f.pstart[t] + r]
+        return f.Xflat[f.xoff[t]:f.xoff[t] + m * r].reshape((m, r), order="F")
+
+
+def _level_rows(tree, f, active, leaf):
+    """Rows of every active block: a leaf's points, else the concatenated
+    pivots of its children in ChildIterator order (batchedbottomup.jl:282-303).
+    Returns (rowptr, rowidx)."""
+    isleaf = leaf[active]
+    nch = np.diff(tree.childptr)[active]
+    nseg = np.where(isleaf, 1, nch)
+    segptr = np.zeros(len(active) + 1, np.int64)
+    np.cumsum(nseg, out=segptr[1:])
+    seg_start = np.zeros(segptr[-1], np.int64)
+    seg_len = np.zeros(segptr[-1], np.int64)
+    lf = active[isleaf]
+    seg_start[segptr[:-1][isleaf]] = tree.lo[lf]
+    seg_len[segptr[:-1][isleaf]] = tree.hi[lf] - tree.lo[lf]
+    inner = ~isleaf
+    if np.any(inner):
+        ch = tree.children[concat_ranges(tree.childptr[active[inner]], nch[inner])]
+        pos = concat_ranges(segptr[:-1][inner], nch[inner])
+        npts = tree.perm.shape[0]
+        seg_start[pos] = npts + f.pstart[ch]
+        seg_len[pos] = np.where(f.assigned[ch], f.rank[ch], 0)
+    src = np.concatenate([tree.perm.astype(np.int64), f.pivflat])
+    counts = np.add.reduceat(seg_len, segptr[:-1]) if len(seg_len) else np.zeros(len(active), np.int64)
+    counts = np.where(nseg > 0, counts, 0)
+    rowptr = np.zeros(len(active) + 1, np.int64)
+    np.cumsum(counts, out=rowptr[1:])
+    return rowptr, src[concat_ranges(seg_start, seg_len)]
+
+
+def _compresslevel(cfg, storage, active, rowptr, rowidx, out):
     """compresslevel_test!/trial! for a HIP LevelStack (batchedbottomup.jl:145-244,
     compresslevel.jl:242-316): batched ACA + skeleton solve, then the pivots
     tpivots[t] = tvalues[t][rowpivots] and the factor X of every block."""
     ranks, rp, cp, xoff, X = compress_level(storage, cfg.tol, sampled=True)
+    ranks = ranks.astype(np.int64)
     m = np.diff(storage.rowoffsets)
-    for i, t in enumerate(activeclusters):
-        r = int(ranks[i])
-        vals = tvalues_idx[tvalues_ptr[i]:tvalues_ptr[i + 1]]
-        out.assigned[t] = True
-        out.rank[t] = r
-        out.pivots[t] = vals[rp[:r, i] - 1].astype(np.int64)
-        out.m[t] = m[i]
-        out.X[t] = X[xoff[i]:xoff[i + 1]].reshape((m[i], r), order="F") if r else np.zeros((m[i], 0), np.complex128)
+    mr = rp.shape[0]
+    take = np.arange(mr)[None, :] < ranks[:, None]
+    pos = (rowptr[:-1, None] + rp.T - 1)[take]
+    out.add_level(active, m, ranks, rowidx[pos], xoff, X)
 
 
 def batched_side(compressor, farmatrix, tree, fd, side, maxrank=50, farpivots=None):
     """testbases (side 0) / trialbases (side 1) of BatchedBottomUp, returning
-    SideFactors. farpivots: optional Phase-1 result (0-based CSR)."""
+    LevelFactors. farpivots: optional Phase-1 result (0-based CSR)."""
     selftree = tree.testtree if side == 0 else tree.trialtree
     othertree = tree.trialtree if side == 0 else tree.testtree
     if farpivots is None:
         farpivots = compressor.pivotselector.select(selftree, othertree, fd, maxrank)
     fptr, fidx = farpivots
     nfar = np.diff(fptr)
-    out = SideFactors(selftree.nnodes)
+    out = LevelFactors(selftree.nnodes)
     out.farpivots = farpivots
     leaf = selftree.leafmask()
     for lev in sorted(selftree.levels().tolist(), reverse=True):
@@ -229,33 +323,18 @@ def batched_side(compressor, farmatrix, tree, fd, side, maxrank=50, farpivots=No
         if len(active) == 0:
             continue
         t0 = time.perf_counter()
-        # rows: leaf -> its points, inner -> the concatenated pivots of its children
-        starts, lengths = [], []
-        counts = np.zeros(len(active), np.int64)
-        rows_list = []
-        for i, t in enumerate(active):
-            if leaf[t]:
-                rows_list.append(selftree.perm[selftree.lo[t]:selftree.hi[t]])
-            else:
-                ch = selftree.childrenof(t)
-                rows_list.append(np.concatenate([out.pivots[c] if out.pivots[c] is not None else np.zeros(0, np.int64)
-                                                 for c in ch]) if len(ch) else np.zeros(0, np.int64))
-            counts[i] = len(rows_list[-1])
-        rowptr = np.zeros(len(active) + 1, np.int64)
-        np.cumsum(counts, out=rowptr[1:])
-        rowidx = np.concatenate(rows_list).astype(np.int64) if rowptr[-1] else np.zeros(0, np.int64)
+        rowptr, rowidx = _level_rows(selftree, out, active, leaf)
         colptr = np.zeros(len(active) + 1, np.int64)
         np.cumsum(nfar[active], out=colptr[1:])
-        colidx = concat_ranges(fptr[active], nfar[active])
-        colidx = fidx[colidx]
+        colidx = fidx[concat_ranges(fptr[active], nfar[active])]
         storage = assemble_level(farmatrix, rowptr, rowidx, colptr, colidx, maxrank, transposed=(side == 1))
         t1 = time.perf_counter()
-        _compresslevel(compressor.compressor, storage, selftree, active, None, rowptr, rowidx, out)
+        _compresslevel(compressor.compressor, storage, active, rowptr, rowidx, out)
         t2 = time.perf_counter()
         if _timing():
             print(f"[batchedbottomup {'test' if side == 0 else 'trial'}] level {lev + 1}: {len(active)} blocks | "
                   f"assemble={t1 - t0:.4f}s | compress(aca+basis/transfer)={t2 - t1:.4f}s", flush=True)
-        out.level_transfer_nodes[lev] = [int(t) for t in nodes if not leaf[t] and out.assigned[t]]
+        out.level_transfer_nodes[lev] = nodes[~leaf[nodes] & out.assigned[nodes]].tolist()
     return out
 
 
@@ -276,10 +355,60 @@ class TransferStore:
         self.blocks = blocks
 
 
+def _stores_flat(f, tree, side):
+    """stores_from_factors for LevelFactors, by index arithmetic on the packed
+    factors (same layout and order as the per-node path below)."""
+    leaf = tree.leafmask()
+    Xf = f.Xflat
+    # bases: leaves with a block, node order; test X (m x r), trial X^T (r x m)
+    bn = np.nonzero(f.assigned & leaf)[0]
+    m, r, xo = f.m[bn], f.rank[bn], f.xoff[bn]
+    sz = m * r
+    boff = np.zeros(len(bn) + 1, np.int64)
+    np.cumsum(sz, out=boff[1:])
+    q = np.arange(boff[-1], dtype=np.int64) - np.repeat(boff[:-1], sz)
+    if side == 0:
+        src = np.repeat(xo, sz) + q
+    else:
+        rr, mm = np.repeat(r, sz), np.repeat(m, sz)
+        src = np.repeat(xo, sz) + q // np.maximum(rr, 1) + (q % np.maximum(rr, 1)) * mm
+    basis = BasisStore(bn.astype(np.int64), boff, Xf[src])
+    # transfers: levels ascending, parents in level_transfer_nodes order, all children
+    parents = [np.asarray(f.level_transfer_nodes.get(lev, []), np.int64) for lev in sorted(tree.levels().tolist())]
+    levptr = np.zeros(len(parents) + 1, np.int64)
+    np.cumsum([len(p_) for p_ in parents], out=levptr[1:])
+    P = np.concatenate(parents) if len(parents) else np.zeros(0, np.int64)
+    nch = np.diff(tree.childptr)[P]
+    chdptr = np.zeros(len(P) + 1, np.int64)
+    np.cumsum(nch, out=chdptr[1:])
+    C = tree.children[concat_ranges(tree.childptr[P], nch)] if len(P) else np.zeros(0, np.int64)
+    Pe = np.repeat(P, nch)                       # parent of every edge
+    rc = np.where(f.assigned[C], f.rank[C], 0)
+    # row offset of each child inside its parent's block: exclusive sum of rc per parent
+    csum = np.cumsum(rc)
+    first = np.repeat(chdptr[:-1], nch)
+    oc = csum - rc - np.where(first > 0, csum[first - 1], 0) if len(C) else np.zeros(0, np.int64)
+    rp, mp, xp = f.rank[Pe], f.m[Pe], f.xoff[Pe]
+    esz = rc * rp
+    toff = np.zeros(len(C) + 1, np.int64)
+    np.cumsum(esz, out=toff[1:])
+    q = np.arange(toff[-1], dtype=np.int64) - np.repeat(toff[:-1], esz)
+    rcr, rpr = np.repeat(rc, esz), np.repeat(rp, esz)
+    if side == 0:   # E (rc x rp) column-major: q = a + b rc
+        a_, b_ = q % np.maximum(rcr, 1), q // np.maximum(rcr, 1)
+    else:           # E^T (rp x rc) column-major: q = b + a rp
+        b_, a_ = q % np.maximum(rpr, 1), q // np.maximum(rpr, 1)
+    src = np.repeat(xp, esz) + np.repeat(oc, esz) + a_ + b_ * np.repeat(mp, esz)
+    transfer = TransferStore(levptr, P, chdptr, C.astype(np.int64), toff, Xf[src])
+    return basis, transfer
+
+
 def stores_from_factors(f, tree, side):
     """_build_basis_storage / _build_transfer_storage (nestedbasis.jl:1-4,
     transfermatrices.jl:58-95) from the block factors X: test basis X (m x r),
     trial basis X^T; test transfer X[childrows, :], trial transfer its transpose."""
+    if isinstance(f, LevelFactors):
+        return _stores_flat(f, tree, side)
     leaf = tree.leafmask()
     bnodes = [t for t in range(f.nn) if f.assigned[t] and leaf[t]]
     bl, boff = [], [0]

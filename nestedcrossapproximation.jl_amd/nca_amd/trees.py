@@ -161,12 +161,15 @@ class FarData:
 
 def transpose_csr(ptr, idx, ncols):
     """CSR transpose with rows kept in increasing order (farinteractions.jl:66-92)."""
-    counts = np.bincount(idx, minlength=ncols).astype(np.int64)
-    tptr = np.zeros(ncols + 1, np.int64)
-    np.cumsum(counts, out=tptr[1:])
-    rows = np.repeat(np.arange(ptr.shape[0] - 1, dtype=np.int64), np.diff(ptr))
-    order = np.argsort(idx, kind="stable")
-    return tptr, rows[order]
+    L = tree_lib()
+    p = np.ascontiguousarray(ptr, np.int64)
+    i = np.ascontiguousarray(idx, np.int64)
+    h = L.ncat_csr_transpose(p.shape[0] - 1, _ptr(p), _ptr(i) if len(i) else None, int(ncols))
+    tptr = np.empty(ncols + 1, np.int64)
+    tidx = np.empty(L.ncat_csr_nnz(h), np.int64)
+    L.ncat_csr_export(h, _ptr(tptr), _ptr(tidx))
+    L.ncat_csr_free(h)
+    return tptr, tidx
 
 
 def fardata(tree, partition):

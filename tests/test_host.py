@@ -162,3 +162,43 @@ def test_forest_is_block_diagonal():
     for p in range(len(sets)):
         blk[off[p]:off[p + 1], off[p]:off[p + 1]] = 1
     assert np.array_equal(cover, blk)
+
+
+def test_flat_factors_match_per_node_path():
+    """The batched driver's flat factors (LevelFactors) give the same rows per
+    level, pivots and packed bases/transfers as the per-node construction."""
+    from helpers import sphere_problem, oracle_assembly
+    p = sphere_problem(4096, tol=1e-4)
+    ft, fs = oracle_assembly(p)
+    for f, tree, side in ((ft, p.tree.testtree, 0), (fs, p.tree.trialtree, 1)):
+        g = C.LevelFactors(f.nn)
+        leaf = tree.leafmask()
+        for lev in sorted(tree.levels().tolist(), reverse=True):
+            nodes = np.array([t for t in tree.levelnodes(lev) if f.assigned[t]], np.int64)
+            if len(nodes) == 0:
+                continue
+            # rows of the level from the flat factors of the deeper levels
+            rowptr, rowidx = C._level_rows(tree, g, nodes, leaf)
+            for i, t in enumerate(nodes):
+                want = tree.perm[tree.lo[t]:tree.hi[t]] if leaf[t] else np.concatenate(
+                    [f.pivots[c] for c in tree.childrenof(t) if f.assigned[c]] or [np.zeros(0, np.int64)])
+                assert np.array_equal(rowidx[rowptr[i]:rowptr[i + 1]], want)
+            ranks = f.rank[nodes]
+            X = [f.X[t].ravel(order="F") for t in nodes]
+            xoff = np.zeros(len(nodes) + 1, np.int64)
+            np.cumsum([len(x) for x in X], out=xoff[1:])
+            g.add_level(nodes, f.m[nodes], ranks, np.concatenate([f.pivots[t] for t in nodes]), xoff,
+                        np.concatenate(X))
+            g.level_transfer_nodes[lev] = [int(t) for t in tree.levelnodes(lev) if not leaf[t] and f.assigned[t]]
+        for t in range(f.nn):
+            if f.assigned[t]:
+                assert np.array_equal(g.pivots[t], f.pivots[t]) and np.array_equal(g.X[t], f.X[t])
+        for a, b in zip(g.pivot_csr(), f.pivot_csr()):
+            assert np.array_equal(a, b)
+        f.level_transfer_nodes = {k: v for k, v in g.level_transfer_nodes.items()}
+        b1, t1 = C.stores_from_factors(f, tree, side)
+        b2, t2 = C.stores_from_factors(g, tree, side)
+        assert np.array_equal(b1.nodes, b2.nodes) and np.array_equal(b1.offsets, b2.offsets)
+        assert np.array_equal(b1.blocks, b2.blocks)
+        for n in ("leveltoidcsptr", "levelidcs", "idxtochdsptr", "chdsidcs", "offsets", "blocks"):
+            assert np.array_equal(getattr(t1, n), getattr(t2, n)), n
