@@ -177,6 +177,49 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
 int32_t nca_h2_destroy(nca_h2* h2);
 int32_t nca_h2_set_precision(nca_h2* h2, int32_t precision);
 
+/* Directional (wideband) H^2: one basis per (cluster, direction). Everything
+ * PetrovGalerkinNCACore holds for fardata::DirectionalData
+ * (src/nca/petrovgalerkinnca.jl:43-101) with the directional plans of
+ * src/nca/abstractnca.jl:5-9,41-48,78-82, all 1-based: coefficients, transfers
+ * and coupling are addressed by direction index (diridx), trees, bases and the
+ * near field by node. The coefficients of a node's directions must be
+ * contiguous in diridx order, as plan_from_pivots gives them. */
+typedef struct {
+    int64_t M, N, nn_test, nn_trial, ne_test, ne_trial;
+    const int64_t *test_perm, *test_lo, *test_hi;
+    const int64_t *trial_perm, *trial_lo, *trial_hi;
+    /* DirectionalData.nodetodirsptr of both trees (nn+1) */
+    const int64_t *test_dirptr, *trial_dirptr;
+    /* CoefficientPlan.globalidxtocoeffptr over direction indices (ne+1) */
+    const int64_t *test_cptr, *trial_cptr;
+    /* BasisStore with DirBasisTraversalPlan (nodes, idxtodiridxptr, diridx):
+     * nbnodes nodes; block i (test |t| x r, trial r x |s|) of direction index
+     * diridx[i] at offsets off[i] (0-based complex, nblocks+1) */
+    int64_t nbnodes_test, nbnodes_trial;
+    const int64_t *bt_nodes, *bt_dirptr, *bt_diridx, *bt_off;
+    const int64_t *bs_nodes, *bs_dirptr, *bs_diridx, *bs_off;
+    const double *bt_blocks, *bs_blocks;
+    /* TransferStore with DirTransferTraversalPlan over direction indices
+     * (leveltoidcsptr, levelidcs, idxtochdsptr, chdsidcs) + block offsets; the
+     * group arrays of the plan are not needed: the downward pass sums per child */
+    int64_t nlev_test, nlev_trial;
+    const int64_t *tt_levptr, *tt_levidcs, *tt_chdptr, *tt_chds, *tt_off;
+    const int64_t *st_levptr, *st_levidcs, *st_chdptr, *st_chds, *st_off;
+    const double *tt_blocks, *st_blocks;
+    /* DirCouplingTraversalPlan (nodetodiridxptr over test nodes, tidcs, sidcs);
+     * the coupling store holds the blocks in this order, assembled by
+     * nca_coupling_assemble over direction indices */
+    const int64_t *cp_nodeptr, *cp_tidcs, *cp_sidcs;
+    /* near interactions: test node -> near trial nodes */
+    const int64_t *nearptr, *nearnodes;
+} nca_dh2_desc;
+
+/* Build the device-resident directional operator; the handle works with every
+ * nca_h2_* call below. The coupling store must outlive it. */
+int32_t nca_dh2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test,
+                       const nca_points* trial, const nca_dh2_desc* desc, const nca_coupling* coupling,
+                       int32_t precision, nca_h2** out);
+
 /* Y = alpha * op(A) * X + beta * Y with op N/T/C, nrhs columns. Replaces
  * LinearMaps._unsafe_mul!(y, A::Union{PetrovGalerkinNCA, TransposeMap,
  * AdjointMap}, x) (src/nca/petrovgalerkinnca.jl:415-425, stages :431-589) and

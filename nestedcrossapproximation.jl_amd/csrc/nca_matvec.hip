@@ -11,6 +11,10 @@
 // Coefficient vectors hold the R right-hand sides interleaved: xh[k * R + q].
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <thread>
+#include <cstdio>
 #include <type_traits>
 #include <cstdlib>
 #include <memory>
@@ -23,8 +27,13 @@ namespace nca {
 constexpr int kWave = 64;
 
 struct SideDev {
-    int64_t npts = 0, nn = 0, K = 0;
-    DevBuf<int32_t> perm, iperm, lo, hi, cptr;  // iperm: original index -> tree position
+    // nn cluster nodes; ne coefficient blocks: the nodes themselves, or the
+    // direction indices (cluster, direction) of a directional operator
+    int64_t npts = 0, nn = 0, ne = 0, K = 0;
+    DevBuf<int32_t> perm, iperm, lo, hi, cptr;  // iperm: original index -> tree position; cptr: ne+1
+    // per node the first coefficient of its blocks (nn+1); empty when ne == nn
+    DevBuf<int32_t> ncptr;
+    const int32_t* node_cptr() const { return ncptr.p ? ncptr.p : cptr.p; }
     DevBuf<double> pts;  // coordinates in tree order, 3 per position
     // leaf bases
     int64_t nbasis = 0;
@@ -37,11 +46,22 @@ struct SideDev {
     DevBuf<int32_t> parents, chds;
     DevBuf<int64_t> chdptr, toff;
     DevBuf<double2> tblocks;
+    // downward transfers by child when a child block has several parents
+    // (directions): per level [dlevptr[l], dlevptr[l+1]) of dchild, each with
+    // its parents dpar[dptr[i], dptr[i+1]) and their transfer blocks dtoff
+    bool down_by_child = false;
+    std::vector<int64_t> dlevptr;
+    DevBuf<int32_t> dchild, dpar;
+    DevBuf<int64_t> dptr, dtoff;
     // c32 copies of the bases and transfers for the MIXED path
     DevBuf<float2> bblocks32, tblocks32;
     // all leaves of the tree (destinations of stage 5) and their basis slot (-1: none)
     int64_t nleaves = 0;
     DevBuf<int32_t> leaves, leafbasis;
+    // basis slots of nodes that are not leaves (directional bases above
+    // low-frequency children), applied after the leaves
+    int64_t nibasis = 0;
+    DevBuf<int32_t> ibslot;
     // near lists by node of this side (as destination): ptr (nn+1), nodes of the other side
     DevBuf<int64_t> nearptr;
     DevBuf<int32_t> nearnodes;
@@ -53,6 +73,7 @@ struct SideDev {
     DevBuf<int32_t> tile_leaf, tile_row0;
     // host copies used during creation
     std::vector<int32_t> h_lo, h_hi, h_cptr, h_perm, h_bnodes, h_leaves, h_leafslot, h_leaves_node_order;
+    std::vector<int32_t> h_ncptr, h_enode;  // per node first coefficient (nn+1); node of every block
     // host copies of the merged near ranges (to reorder the leaves)
     std::vector<int64_t> h_nrptr;
     std::vector<int32_t> h_nrlo, h_nrhi;
@@ -993,6 +1014,35 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Stage 5 for a basis above the leaves: y[rows of t] += alpha U_t yh_t
+// (!FWD: V_t^T, CJ: conjugated) for nq right-hand sides, after the leaf
+// kernel; one workgroup per such basis, one thread per (row, rhs).
+template <bool FWD, bool CJ, class T2>
+__global__ void __launch_bounds__(256)
+    k_basis_out(const int32_t* __restrict__ slots, const int32_t* __restrict__ bnodes,
+                const int64_t* __restrict__ boff, const T2* __restrict__ blocks, const int32_t* __restrict__ lo,
+                const int32_t* __restrict__ hi, const int32_t* __restrict__ perm, const int32_t* __restrict__ cptr,
+                const double2* __restrict__ yh, int64_t R, int64_t q0, int64_t nq, double2 alpha,
+                double2* __restrict__ y, int64_t ldy) {
+    const int w = slots[blockIdx.x];
+    const int t = bnodes[w];
+    const int l0 = lo[t], nt = hi[t] - l0;
+    const int c0 = cptr[t], r = cptr[t + 1] - c0;
+    const T2* B = blocks + boff[w];
+    for (int64_t idx = threadIdx.x; idx < (int64_t)nt * nq; idx += blockDim.x) {
+        const int row = (int)(idx / nq), q = (int)(idx % nq);
+        double2 acc = make_double2(0.0, 0.0);
+        for (int k = 0; k < r; ++k) {
+            const T2 b = B[FWD ? (int64_t)row + (int64_t)k * nt : (int64_t)k + (int64_t)row * r];
+            double2 bv = make_double2((double)b.x, CJ ? -(double)b.y : (double)b.y);
+            const double2 v = yh[(int64_t)(c0 + k) * R + q0 + q];
+            acc = cadd(acc, cmul(bv, v));
+        }
+        double2* o = y + (int64_t)perm[l0 + row] + (q0 + q) * ldy;
+        *o = cadd(*o, cmul(alpha, acc));
+    }
+}
+
 inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // ---------------------------------------------------------------------------
@@ -1045,11 +1095,12 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
     if (src.nbasis > 0) {
         if (NR == 16)
             hipLaunchKernelGGL((k_project_mfma<P, !FWD, CJ, true>), dim3(blocks_for(src.nbasis, 4)), dim3(256), 0, st,
-                               src.bnodes.p, src.boff.p, O::basis(src), src.nbasis, src.lo.p, src.hi.p, src.cptr.p, xs,
-                               R, q0, xh);
+                               src.bnodes.p, src.boff.p, O::basis(src), src.nbasis, src.lo.p, src.hi.p,
+                               src.node_cptr(), xs, R, q0, xh);
         else
             hipLaunchKernelGGL((k_project<NR, !FWD, CJ, T2, T2>), dim3((unsigned)src.nbasis), dim3(kWave), 0, st,
-                               src.bnodes.p, src.boff.p, O::basis(src), src.lo.p, src.hi.p, src.cptr.p, xs, R, q0, xh);
+                               src.bnodes.p, src.boff.p, O::basis(src), src.lo.p, src.hi.p, src.node_cptr(), xs, R,
+                               q0, xh);
     }
     if (evs) evs->mark(2, st);
     // upward: deepest level first; source transfers (trial E forward, test E transposed)
@@ -1087,17 +1138,33 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
                                h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
                                h->f_colcoef.p, dst.cptr.p, xq0, xstride, R, q0, yh, h->couple_xcd);
         } else if (NR != 16 && h->coupling->nentries > 0) {
-            hipLaunchKernelGGL((k_couple_fwd<NR, CJ, T2>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->f_base.p,
+            hipLaunchKernelGGL((k_couple_fwd<NR, CJ, T2>), dim3((unsigned)dst.ne), dim3(kWave), 0, st, S, h->f_base.p,
                                h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
         }
     } else {
         if (h->coupling->nentries > 0)
-            hipLaunchKernelGGL((k_couple_rev<NR, CJ, T2>), dim3((unsigned)dst.nn), dim3(kWave), 0, st, S, h->r_ptr.p,
+            hipLaunchKernelGGL((k_couple_rev<NR, CJ, T2>), dim3((unsigned)dst.ne), dim3(kWave), 0, st, S, h->r_ptr.p,
                                h->r_test.p, h->r_off.p, src.cptr.p, dst.cptr.p, xh, R, q0, yh);
     }
     if (evs) evs->mark(4, st);
     // downward: root level first; destination transfers (test E forward, trial E transposed)
-    for (int64_t l = 0; l < dst.nlev; ++l) {
+    if (dst.down_by_child) {
+        // child blocks with several parents: one wave per child sums over its
+        // parents, the upward kernel's form (out = child, in = parents)
+        for (size_t l = 0; l + 1 < dst.dlevptr.size(); ++l) {
+            const int64_t nc = dst.dlevptr[l + 1] - dst.dlevptr[l];
+            if (nc == 0) continue;
+            if (NR == 16)
+                hipLaunchKernelGGL((k_transfer_mfma<P, false, !FWD, CJ, true>), dim3(blocks_for(nc, 4)), dim3(256), 0,
+                                   st, dst.dchild.p, dst.dptr.p, dst.dpar.p, dst.dtoff.p, O::transfer(dst), dst.cptr.p,
+                                   dst.dlevptr[l], nc, R, q0, yh);
+            else
+                hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ, T2>), dim3((unsigned)nc), dim3(kWave), 0, st,
+                                   dst.dchild.p, dst.dptr.p, dst.dpar.p, dst.dtoff.p, O::transfer(dst), dst.cptr.p,
+                                   dst.dlevptr[l], R, q0, yh);
+        }
+    }
+    for (int64_t l = 0; l < (dst.down_by_child ? 0 : dst.nlev); ++l) {
         const int64_t np = dst.levptr[l + 1] - dst.levptr[l];
         if (np == 0) continue;
         if (NR == 16)
@@ -1119,15 +1186,20 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
                        : (neg ? k_leaf_out_mfma<P, FWD, CJ, false, true> : k_leaf_out_mfma<P, FWD, CJ, false, false>);
         hipLaunchKernelGGL(kern, dim3(blocks_for(dst.ntiles, 4)), dim3(256), 0, st, dst.tile_leaf.p,
                            dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst),
-                           dst.lo.p, dst.hi.p, dst.perm.p, dst.cptr.p, dst.pts.p, dst.nrptr.p, dst.nrlo.p, dst.nrhi.p,
+                           dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(), dst.pts.p, dst.nrptr.p, dst.nrlo.p,
+                           dst.nrhi.p,
                            src.pts.p, yh, xs, R, q0, nk, h->sctab.p, alpha, beta, y, ldy, h->near_xcd,
                            h->near_stagger);
     } else if (NR != 16 && dst.nleaves > 0) {
         hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ, T2, T2>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st,
                            dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p,
-                           dst.cptr.p, dst.pts.p, dst.nearptr.p, dst.nearnodes.p, src.lo.p, src.hi.p, src.pts.p, yh, xs,
-                           R, q0, h->k, h->sign, alpha, beta, y, ldy);
+                           dst.node_cptr(), dst.pts.p, dst.nearptr.p, dst.nearnodes.p, src.lo.p, src.hi.p, src.pts.p,
+                           yh, xs, R, q0, h->k, h->sign, alpha, beta, y, ldy);
     }
+    if (dst.nibasis > 0)
+        hipLaunchKernelGGL((k_basis_out<FWD, CJ, T2>), dim3((unsigned)dst.nibasis), dim3(256), 0, st, dst.ibslot.p,
+                           dst.bnodes.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(),
+                           yh, R, q0, (int64_t)NR, alpha, y, ldy);
     if (evs) evs->mark(6, st);
     NCA_HIP(hipGetLastError());
 }
@@ -1222,13 +1294,18 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
 
 // ---------------------------------------------------------------------------
 // Creation: 1-based Julia-shaped plans -> device arrays.
-void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, const int64_t* perm, const int64_t* lo,
-                const int64_t* hi, const int64_t* cptr, const nca_points* pts, int64_t nbasis, const int64_t* bnodes,
-                const int64_t* boff, const double* bblocks, bool testside, int64_t nlev, const int64_t* levptr,
-                const int64_t* levidcs, const int64_t* chdptr, const int64_t* chds, const int64_t* toff,
-                const double* tblocks) {
+// ne coefficient blocks; dirptr (nn+1, 1-based) gives the blocks of every
+// node (null: one block per node). Bases belong to nodes and cover all
+// coefficients of their node; transfers connect blocks and, with multiparent,
+// a child block may have several parents.
+void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, int64_t ne, const int64_t* dirptr,
+                bool multiparent, const int64_t* perm, const int64_t* lo, const int64_t* hi, const int64_t* cptr,
+                const nca_points* pts, int64_t nbasis, const int64_t* bnodes, const int64_t* boff,
+                const double* bblocks, bool testside, int64_t nlev, const int64_t* levptr, const int64_t* levidcs,
+                const int64_t* chdptr, const int64_t* chds, const int64_t* toff, const double* tblocks) {
     sd.npts = npts;
     sd.nn = nn;
+    sd.ne = ne;
     NCA_REQUIRE(pts->n == npts, NCA_ERR_DIMENSION, "nca_h2_create: point count differs from the matrix size");
     sd.h_perm.resize(npts);
     std::vector<char> seen(npts, 0);
@@ -1240,17 +1317,33 @@ void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, const int
     }
     sd.h_lo.resize(nn);
     sd.h_hi.resize(nn);
-    sd.h_cptr.resize(nn + 1);
+    sd.h_cptr.resize(ne + 1);
     for (int64_t t = 0; t < nn; ++t) {
         sd.h_lo[t] = (int32_t)(lo[t] - 1);
         sd.h_hi[t] = (int32_t)hi[t];
         NCA_REQUIRE(sd.h_lo[t] >= 0 && sd.h_hi[t] >= sd.h_lo[t] && sd.h_hi[t] <= npts, NCA_ERR_ARGUMENT,
                     "nca_h2_create: node range out of bounds");
     }
-    for (int64_t t = 0; t <= nn; ++t) sd.h_cptr[t] = (int32_t)(cptr[t] - 1);
-    for (int64_t t = 0; t < nn; ++t)
-        NCA_REQUIRE(sd.h_cptr[t + 1] >= sd.h_cptr[t], NCA_ERR_ARGUMENT, "nca_h2_create: decreasing coefficient plan");
-    sd.K = sd.h_cptr[nn];
+    for (int64_t e = 0; e <= ne; ++e) sd.h_cptr[e] = (int32_t)(cptr[e] - 1);
+    NCA_REQUIRE(sd.h_cptr[0] == 0, NCA_ERR_ARGUMENT, "nca_h2_create: the coefficient plan must start at 1");
+    for (int64_t e = 0; e < ne; ++e)
+        NCA_REQUIRE(sd.h_cptr[e + 1] >= sd.h_cptr[e], NCA_ERR_ARGUMENT, "nca_h2_create: decreasing coefficient plan");
+    sd.K = sd.h_cptr[ne];
+    sd.h_enode.resize(ne);
+    sd.h_ncptr.resize(nn + 1);
+    if (dirptr) {
+        NCA_REQUIRE(dirptr[0] == 1 && dirptr[nn] - 1 == ne, NCA_ERR_ARGUMENT,
+                    "nca_h2_create: direction pointer does not cover the direction indices");
+        for (int64_t t = 0; t < nn; ++t) {
+            NCA_REQUIRE(dirptr[t + 1] >= dirptr[t], NCA_ERR_ARGUMENT, "nca_h2_create: decreasing direction pointer");
+            for (int64_t e = dirptr[t] - 1; e < dirptr[t + 1] - 1; ++e) sd.h_enode[e] = (int32_t)t;
+        }
+        for (int64_t t = 0; t <= nn; ++t) sd.h_ncptr[t] = sd.h_cptr[dirptr[t] - 1];
+    } else {
+        NCA_REQUIRE(ne == nn, NCA_ERR_ARGUMENT, "nca_h2_create: one coefficient block per node expected");
+        for (int64_t e = 0; e < ne; ++e) sd.h_enode[e] = (int32_t)e;
+        sd.h_ncptr = sd.h_cptr;
+    }
     // kNearPad zero points after the last one: near-field steps may read up to
     // 15 positions past a range end (their contributions are zeroed)
     std::vector<double> pp(3 * (npts + kNearPad), 0.0);
@@ -1265,6 +1358,7 @@ void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, const int
     sd.lo.upload(sd.h_lo, st);
     sd.hi.upload(sd.h_hi, st);
     sd.cptr.upload(sd.h_cptr, st);
+    if (dirptr) sd.ncptr.upload(sd.h_ncptr, st);
     sd.pts.upload(pp, st);
     // bases
     sd.nbasis = nbasis;
@@ -1274,7 +1368,7 @@ void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, const int
         const int64_t t = bnodes[i] - 1;
         NCA_REQUIRE(t >= 0 && t < nn, NCA_ERR_ARGUMENT, "nca_h2_create: basis node out of range");
         bn[i] = (int32_t)t;
-        const int64_t expect = (int64_t)(sd.h_hi[t] - sd.h_lo[t]) * (sd.h_cptr[t + 1] - sd.h_cptr[t]);
+        const int64_t expect = (int64_t)(sd.h_hi[t] - sd.h_lo[t]) * (sd.h_ncptr[t + 1] - sd.h_ncptr[t]);
         NCA_REQUIRE(boff[i + 1] - boff[i] == expect, NCA_ERR_DIMENSION,
                     std::string("DimensionMismatch: ") + (testside ? "test" : "trial") + " basis of node " +
                         std::to_string(t + 1) + " has " + std::to_string(boff[i + 1] - boff[i]) + " entries, " +
@@ -1296,15 +1390,15 @@ void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, const int
     const int64_t nedge = npar ? chdptr[npar] - chdptr[0] : 0;
     ch.resize(nedge);
     to.resize(nedge + 1);
-    std::vector<char> ischild(nn, 0);
+    std::vector<char> ischild(ne, 0);
     for (int64_t i = 0; i < npar; ++i) {
         const int64_t p = levidcs[i] - 1;
-        NCA_REQUIRE(p >= 0 && p < nn, NCA_ERR_ARGUMENT, "nca_h2_create: transfer parent out of range");
+        NCA_REQUIRE(p >= 0 && p < ne, NCA_ERR_ARGUMENT, "nca_h2_create: transfer parent out of range");
         par[i] = (int32_t)p;
         cptr0[i + 1] = chdptr[i + 1] - chdptr[0];
         for (int64_t e = chdptr[i] - chdptr[0]; e < chdptr[i + 1] - chdptr[0]; ++e) {
             const int64_t c = chds[e] - 1;
-            NCA_REQUIRE(c >= 0 && c < nn && !ischild[c], NCA_ERR_ARGUMENT,
+            NCA_REQUIRE(c >= 0 && c < ne && (multiparent || !ischild[c]), NCA_ERR_ARGUMENT,
                         "nca_h2_create: transfer child out of range or with two parents");
             ischild[c] = 1;
             ch[e] = (int32_t)c;
@@ -1322,6 +1416,43 @@ void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, const int
     sd.chds.upload(ch, st);
     sd.toff.upload(to, st);
     if (nedge && to[nedge]) sd.tblocks.upload(reinterpret_cast<const double2*>(tblocks) + toff[0], (size_t)to[nedge], st);
+    sd.down_by_child = multiparent;
+    if (multiparent) {
+        // per level of parents: the children (first appearance order), each
+        // with its incoming (parent, block) edges in plan order
+        std::vector<int32_t> dch, dpa;
+        std::vector<int64_t> dpt(1, 0), dto;
+        sd.dlevptr.assign(1, 0);
+        std::vector<int64_t> slot(ne, -1);
+        for (int64_t l = 0; l < nlev; ++l) {
+            std::vector<int32_t> kids;
+            std::vector<std::vector<std::pair<int32_t, int64_t>>> in;
+            for (int64_t i = sd.levptr[l]; i < sd.levptr[l + 1]; ++i)
+                for (int64_t e = cptr0[i]; e < cptr0[i + 1]; ++e) {
+                    const int32_t c = ch[e];
+                    if (slot[c] < 0) {
+                        slot[c] = (int64_t)kids.size();
+                        kids.push_back(c);
+                        in.emplace_back();
+                    }
+                    in[slot[c]].emplace_back(par[i], to[e]);
+                }
+            for (size_t q = 0; q < kids.size(); ++q) {
+                dch.push_back(kids[q]);
+                for (auto& pe : in[q]) {
+                    dpa.push_back(pe.first);
+                    dto.push_back(pe.second);
+                }
+                dpt.push_back((int64_t)dpa.size());
+                slot[kids[q]] = -1;
+            }
+            sd.dlevptr.push_back((int64_t)dch.size());
+        }
+        sd.dchild.upload(dch, st);
+        sd.dpar.upload(dpa, st);
+        sd.dptr.upload(dpt, st);
+        sd.dtoff.upload(dto, st);
+    }
 }
 
 // Leaves of the side = nodes whose range is not split by any other node: a
@@ -1405,19 +1536,61 @@ void order_leaves(SideDev& sd, int mode, hipStream_t st) {
     sd.tile_row0.upload(tr, st);
 }
 
+// Host loop over [0, n) on up to 16 threads (dynamic chunks); f must not throw.
+template <class F>
+void parallel_for(int64_t n, F&& f) {
+    const int64_t nt = std::min<int64_t>(std::max<unsigned>(1u, std::thread::hardware_concurrency()), 16);
+    if (n < 8192 || nt == 1) {
+        for (int64_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    constexpr int64_t kChunk = 512;
+    std::vector<std::thread> pool;
+    for (int64_t k = 0; k < nt; ++k)
+        pool.emplace_back([&] {
+            for (;;) {
+                const int64_t b = next.fetch_add(kChunk);
+                if (b >= n) break;
+                const int64_t e = std::min(n, b + kChunk);
+                for (int64_t i = b; i < e; ++i) f(i);
+            }
+        });
+    for (auto& t : pool) t.join();
+}
+
+// Host phase timing of nca_h2_create (environment NCA_CREATE_TIMING): each
+// mark prints the time since the previous one to stderr.
+struct PhaseTimer {
+    bool on;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    explicit PhaseTimer(bool o) : on(o) {}
+    void mark(const char* what) {
+        if (!on) return;
+        const auto t = std::chrono::steady_clock::now();
+        fprintf(stderr, "[nca_h2_create] %-16s %8.1f ms\n", what,
+                std::chrono::duration<double, std::milli>(t - t0).count());
+        t0 = t;
+    }
+};
+
 // Launch order of the forward-coupling tiles (16 rows of one test cluster):
 // 0 node id, 1 depth-first point order (nested ranges after their parent),
 // 2 decreasing row work (columns), 3 per level, one contiguous run per XCD.
 void order_coupling_tiles(nca_h2* h, int mode, hipStream_t st) {
-    const int64_t ntest = h->test.nn;
+    // rows are coefficient blocks (test nodes, or their directions); the tree
+    // orders use the node of each block
+    const int64_t ntest = h->test.ne;
     const std::vector<int32_t>& ncols = h->h_ncols;
+    const std::vector<int32_t>& enode = h->test.h_enode;
     std::vector<int64_t> torder(ntest);
     for (int64_t t = 0; t < ntest; ++t) torder[t] = t;
     if (mode == 1)
         std::sort(torder.begin(), torder.end(), [&](int64_t a, int64_t b) {
             const SideDev& sd = h->test;
-            if (sd.h_lo[a] != sd.h_lo[b]) return sd.h_lo[a] < sd.h_lo[b];
-            if (sd.h_hi[a] != sd.h_hi[b]) return sd.h_hi[a] > sd.h_hi[b];
+            const int32_t na = enode[a], nb = enode[b];
+            if (sd.h_lo[na] != sd.h_lo[nb]) return sd.h_lo[na] < sd.h_lo[nb];
+            if (sd.h_hi[na] != sd.h_hi[nb]) return sd.h_hi[na] > sd.h_hi[nb];
             return a < b;
         });
     if (mode == 2)
@@ -1436,21 +1609,24 @@ void order_coupling_tiles(nca_h2* h, int mode, hipStream_t st) {
         // on XCD b mod 8) takes its XCD's run; runs are padded with empty
         // tiles (node -1)
         const SideDev& sd = h->test;
-        std::vector<int64_t> dfs(ntest);
-        for (int64_t t = 0; t < ntest; ++t) dfs[t] = t;
+        const int64_t nn = sd.nn;
+        std::vector<int64_t> dfs(nn);
+        for (int64_t t = 0; t < nn; ++t) dfs[t] = t;
         std::sort(dfs.begin(), dfs.end(), [&](int64_t a, int64_t b) {
             if (sd.h_lo[a] != sd.h_lo[b]) return sd.h_lo[a] < sd.h_lo[b];
             if (sd.h_hi[a] != sd.h_hi[b]) return sd.h_hi[a] > sd.h_hi[b];
             return a < b;
         });
-        std::vector<int> depth(ntest, 0);
+        std::vector<int> ndepth(nn, 0);
         std::vector<int64_t> stack;
         for (const int64_t t : dfs) {
             while (!stack.empty() && !(sd.h_lo[t] >= sd.h_lo[stack.back()] && sd.h_hi[t] <= sd.h_hi[stack.back()]))
                 stack.pop_back();
-            depth[t] = (int)stack.size();
+            ndepth[t] = (int)stack.size();
             stack.push_back(t);
         }
+        std::vector<int> depth(ntest, 0);
+        for (int64_t t = 0; t < ntest; ++t) depth[t] = ndepth[enode[t]];
         const int maxd = ntest ? *std::max_element(depth.begin(), depth.end()) : 0;
         for (int d = 0; d <= maxd; ++d) {
             std::vector<int32_t> ln, lr;
@@ -1506,15 +1682,22 @@ void time_stages(nca_h2* h, const double2* x, double2* y, int nreps, double& cou
     near_ms = n[n.size() / 2];
 }
 
-}  // namespace nca
+// Coefficient blocks of a directional operator: per side the number of
+// direction indices and their pointer per node (1-based, nn+1). The plain
+// operator has one block per node (dirptr null).
+struct BlockLayout {
+    int64_t ne_test = 0, ne_trial = 0;
+    const int64_t* test_dirptr = nullptr;
+    const int64_t* trial_dirptr = nullptr;
+};
 
-using namespace nca;
-
-extern "C" {
-
-int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test, const nca_points* trial,
-                      const nca_h2_desc* d, const nca_coupling* coupling, int32_t precision, nca_h2** out) {
-    return guarded([&] {
+// nca_h2_create with d's coefficient plans, transfers and coupling plan over
+// the coefficient blocks of lay (trees, bases and near field over nodes).
+void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test, const nca_points* trial,
+                 const nca_h2_desc* d, const BlockLayout& lay, const nca_coupling* coupling, int32_t precision,
+                 nca_h2** out) {
+    {
+        const bool directional = lay.test_dirptr != nullptr;
         NCA_REQUIRE(ctx && kernel && test && trial && d && coupling && out, NCA_ERR_ARGUMENT,
                     "nca_h2_create: invalid argument");
         NCA_REQUIRE(precision == NCA_PREC_FP64 || precision == NCA_PREC_MIXED, NCA_ERR_ARGUMENT,
@@ -1522,6 +1705,7 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
         set_device(ctx);
         hipStream_t st = ctx->stream;
         auto h = std::make_unique<nca_h2>();
+        PhaseTimer timer(getenv("NCA_CREATE_TIMING") != nullptr);
         h->ctx = ctx;
         h->k = kernel->wavenumber;
         h->sign = kernel->sign;
@@ -1544,12 +1728,16 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
         if (const char* v = getenv("NCA_LEAF_ORDER")) h->leaf_order = atoi(v);
         if (const char* v = getenv("NCA_COUPLE_ORDER")) h->couple_order = atoi(v);
         if (const char* v = getenv("NCA_NEAR_STAGGER")) h->near_stagger = atoi(v) != 0;
-        build_side(h->test, st, d->M, d->nn_test, d->test_perm, d->test_lo, d->test_hi, d->test_cptr, test,
-                   d->nbasis_test, d->basis_test_nodes, d->basis_test_off, d->basis_test, true, d->nlev_test,
-                   d->tt_levptr, d->tt_levidcs, d->tt_chdptr, d->tt_chds, d->tt_off, d->tt_blocks);
-        build_side(h->trial, st, d->N, d->nn_trial, d->trial_perm, d->trial_lo, d->trial_hi, d->trial_cptr, trial,
-                   d->nbasis_trial, d->basis_trial_nodes, d->basis_trial_off, d->basis_trial, false, d->nlev_trial,
-                   d->st_levptr, d->st_levidcs, d->st_chdptr, d->st_chds, d->st_off, d->st_blocks);
+        timer.mark("sides");
+        build_side(h->test, st, d->M, d->nn_test, lay.ne_test, lay.test_dirptr, directional, d->test_perm,
+                   d->test_lo, d->test_hi, d->test_cptr, test, d->nbasis_test, d->basis_test_nodes,
+                   d->basis_test_off, d->basis_test, true, d->nlev_test, d->tt_levptr, d->tt_levidcs, d->tt_chdptr,
+                   d->tt_chds, d->tt_off, d->tt_blocks);
+        build_side(h->trial, st, d->N, d->nn_trial, lay.ne_trial, lay.trial_dirptr, directional, d->trial_perm,
+                   d->trial_lo, d->trial_hi, d->trial_cptr, trial, d->nbasis_trial, d->basis_trial_nodes,
+                   d->basis_trial_off, d->basis_trial, false, d->nlev_trial, d->st_levptr, d->st_levidcs,
+                   d->st_chdptr, d->st_chds, d->st_off, d->st_blocks);
+        timer.mark("build_side");
         build_leaves(h->test, st);
         build_leaves(h->trial, st);
         // basis slot of every leaf
@@ -1563,13 +1751,30 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
                 lb.push_back(slot[l]);
                 isleaf[l] = 1;
             }
-            for (int64_t i = 0; i < sd->nbasis; ++i)
-                NCA_REQUIRE(isleaf[bn[i]], NCA_ERR_ARGUMENT, "nca_h2_create: a basis belongs to a node that is not a leaf");
+            // directional bases may sit above the leaves; they are applied
+            // after the leaves (their nodes do not overlap: the nodes below
+            // one of them are low frequency and carry no such basis)
+            std::vector<int32_t> ib;
+            std::vector<char> cover(sd->npts + 1, 0);
+            for (int64_t i = 0; i < sd->nbasis; ++i) {
+                if (isleaf[bn[i]]) continue;
+                NCA_REQUIRE(directional, NCA_ERR_ARGUMENT,
+                            "nca_h2_create: a basis belongs to a node that is not a leaf");
+                for (int32_t q = sd->h_lo[bn[i]]; q < sd->h_hi[bn[i]]; ++q) {
+                    NCA_REQUIRE(!cover[q], NCA_ERR_ARGUMENT, "nca_h2_create: nested bases above the leaves");
+                    cover[q] = 1;
+                }
+                ib.push_back((int32_t)i);
+            }
+            sd->nibasis = (int64_t)ib.size();
+            sd->ibslot.upload(ib, st);
             sd->h_leafslot = lb;
         }
-        // coupling plan must match the store
-        const int64_t ntest = d->nn_test;
-        NCA_REQUIRE(coupling->ntest == ntest && coupling->ntrial == d->nn_trial, NCA_ERR_DIMENSION,
+        timer.mark("leaves");
+        // coupling plan must match the store (over coefficient blocks)
+        const int64_t ntest = lay.ne_test;
+        const int64_t nn_test = d->nn_test;
+        NCA_REQUIRE(coupling->ntest == ntest && coupling->ntrial == lay.ne_trial, NCA_ERR_DIMENSION,
                     "nca_h2_create: coupling store of another tree pair");
         for (int64_t t = 0; t <= ntest; ++t)
             NCA_REQUIRE(d->cp_ptr[t] - d->cp_ptr[0] == coupling->ptr[t], NCA_ERR_ARGUMENT,
@@ -1580,26 +1785,33 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
         for (int64_t t = 0; t < ntest; ++t)
             NCA_REQUIRE(coupling->testrank[t] == h->test.h_cptr[t + 1] - h->test.h_cptr[t], NCA_ERR_DIMENSION,
                         "nca_h2_create: coupling ranks differ from the coefficient plan");
-        for (int64_t s = 0; s < d->nn_trial; ++s)
+        for (int64_t s = 0; s < lay.ne_trial; ++s)
             NCA_REQUIRE(coupling->trialrank[s] == h->trial.h_cptr[s + 1] - h->trial.h_cptr[s], NCA_ERR_DIMENSION,
                         "nca_h2_create: coupling ranks differ from the coefficient plan");
+        timer.mark("coupling checks");
         // forward coupling: per test node base/colbase/ncols; per column its source coefficient
         {
-            std::vector<int64_t> base(ntest), colbase(ntest);
-            std::vector<int32_t> ncols(ntest), colcoef;
-            for (int64_t t = 0; t < ntest; ++t) {
+            std::vector<int64_t> base(ntest), colbase(ntest + 1, 0);
+            std::vector<int32_t> ncols(ntest);
+            parallel_for(ntest, [&](int64_t t) {
                 base[t] = coupling->offsets[coupling->ptr[t]];
-                colbase[t] = (int64_t)colcoef.size();
-                const int rt = coupling->testrank[t];
                 int64_t C = 0;
-                if (rt > 0)
-                    for (int64_t i = coupling->ptr[t]; i < coupling->ptr[t + 1]; ++i) {
-                        const int64_t s = coupling->nodes[i];
-                        for (int j = 0; j < coupling->trialrank[s]; ++j) colcoef.push_back(h->trial.h_cptr[s] + j);
-                        C += coupling->trialrank[s];
-                    }
+                if (coupling->testrank[t] > 0)
+                    for (int64_t i = coupling->ptr[t]; i < coupling->ptr[t + 1]; ++i)
+                        C += coupling->trialrank[coupling->nodes[i]];
                 ncols[t] = (int32_t)C;
-            }
+            });
+            for (int64_t t = 0; t < ntest; ++t) colbase[t + 1] = colbase[t] + ncols[t];
+            std::vector<int32_t> colcoef((size_t)colbase[ntest]);
+            parallel_for(ntest, [&](int64_t t) {
+                if (ncols[t] == 0) return;
+                int64_t c = colbase[t];
+                for (int64_t i = coupling->ptr[t]; i < coupling->ptr[t + 1]; ++i) {
+                    const int64_t s = coupling->nodes[i];
+                    for (int j = 0; j < coupling->trialrank[s]; ++j) colcoef[c++] = h->trial.h_cptr[s] + j;
+                }
+            });
+            colbase.pop_back();
             h->h_ncols = ncols;
             order_coupling_tiles(h.get(), h->couple_order, st);
             h->f_base.upload(base, st);
@@ -1608,9 +1820,10 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
             h->f_colcoef.upload(colcoef, st);
             h->stats.plan_bytes += (int64_t)colcoef.size() * 4 + ntest * 20;
         }
-        // reverse coupling: blocks by trial node
+        timer.mark("forward plan");
+        // reverse coupling: blocks by trial block
         {
-            const int64_t ntrial = d->nn_trial;
+            const int64_t ntrial = lay.ne_trial;
             std::vector<int64_t> rptr(ntrial + 1, 0), roff(coupling->nblocks);
             std::vector<int32_t> rtest(coupling->nblocks);
             for (int64_t i = 0; i < coupling->nblocks; ++i) rptr[coupling->nodes[i] + 1]++;
@@ -1627,22 +1840,29 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
             h->r_off.upload(roff, st);
             h->r_test.upload(rtest, st);
         }
-        // near lists, both orientations
+        timer.mark("reverse plan");
+        // near lists (over nodes), both orientations
         {
+            const int64_t ntest = nn_test;
             const int64_t ntrial = d->nn_trial;
             const int64_t nnz = d->nearptr[ntest] - d->nearptr[0];
             std::vector<int64_t> np(ntest + 1), tp(ntrial + 1, 0);
             std::vector<int32_t> nn32(nnz), tn(nnz);
             int64_t near_entries = 0;
             for (int64_t t = 0; t <= ntest; ++t) np[t] = d->nearptr[t] - d->nearptr[0];
-            for (int64_t t = 0; t < ntest; ++t)
-                for (int64_t i = np[t]; i < np[t + 1]; ++i) {
-                    const int64_t s = d->nearnodes[i] - 1;
-                    NCA_REQUIRE(s >= 0 && s < ntrial, NCA_ERR_ARGUMENT, "nca_h2_create: near node out of range");
-                    nn32[i] = (int32_t)s;
-                    tp[s + 1]++;
-                    near_entries += (int64_t)(h->test.h_hi[t] - h->test.h_lo[t]) * (h->trial.h_hi[s] - h->trial.h_lo[s]);
-                }
+            for (int64_t i = 0; i < nnz; ++i) {
+                const int64_t s = d->nearnodes[i] - 1;
+                NCA_REQUIRE(s >= 0 && s < ntrial, NCA_ERR_ARGUMENT, "nca_h2_create: near node out of range");
+                nn32[i] = (int32_t)s;
+                tp[s + 1]++;
+            }
+            std::vector<int64_t> rowent(ntest, 0);
+            parallel_for(ntest, [&](int64_t t) {
+                int64_t e = 0;
+                for (int64_t i = np[t]; i < np[t + 1]; ++i) e += h->trial.h_hi[nn32[i]] - h->trial.h_lo[nn32[i]];
+                rowent[t] = e * (h->test.h_hi[t] - h->test.h_lo[t]);
+            });
+            for (int64_t t = 0; t < ntest; ++t) near_entries += rowent[t];
             for (int64_t s = 0; s < ntrial; ++s) tp[s + 1] += tp[s];
             std::vector<int64_t> cur(tp.begin(), tp.end() - 1);
             for (int64_t t = 0; t < ntest; ++t)
@@ -1654,20 +1874,34 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
             // merged contiguous source ranges per destination node, both orientations
             auto merge = [](SideDev& dsd, const SideDev& ssd, const std::vector<int64_t>& ptr,
                             const std::vector<int32_t>& nodes, hipStream_t s) {
-                std::vector<int64_t> mptr(dsd.nn + 1, 0);
-                std::vector<int32_t> mlo, mhi;
-                std::vector<std::pair<int32_t, int32_t>> rg;
-                for (int64_t t = 0; t < dsd.nn; ++t) {
-                    rg.clear();
+                // per node: its source ranges sorted and merged where contiguous
+                const int64_t nn = dsd.nn;
+                std::vector<int64_t> cnt(nn, 0);
+                std::vector<std::vector<std::pair<int32_t, int32_t>>> per(nn);
+                parallel_for(nn, [&](int64_t t) {
+                    auto& rg = per[t];
                     for (int64_t i = ptr[t]; i < ptr[t + 1]; ++i)
                         if (ssd.h_hi[nodes[i]] > ssd.h_lo[nodes[i]]) rg.emplace_back(ssd.h_lo[nodes[i]], ssd.h_hi[nodes[i]]);
                     std::sort(rg.begin(), rg.end());
-                    for (auto& r : rg) {
-                        if ((int64_t)mlo.size() > mptr[t] && mhi.back() == r.first) mhi.back() = r.second;
-                        else { mlo.push_back(r.first); mhi.push_back(r.second); }
+                    size_t k = 0;
+                    for (size_t q = 0; q < rg.size(); ++q) {
+                        if (k > 0 && rg[k - 1].second == rg[q].first) rg[k - 1].second = rg[q].second;
+                        else rg[k++] = rg[q];
                     }
-                    mptr[t + 1] = (int64_t)mlo.size();
-                }
+                    rg.resize(k);
+                    cnt[t] = (int64_t)k;
+                });
+                std::vector<int64_t> mptr(nn + 1, 0);
+                for (int64_t t = 0; t < nn; ++t) mptr[t + 1] = mptr[t] + cnt[t];
+                std::vector<int32_t> mlo((size_t)mptr[nn]), mhi((size_t)mptr[nn]);
+                parallel_for(nn, [&](int64_t t) {
+                    int64_t o = mptr[t];
+                    for (auto& r : per[t]) {
+                        mlo[o] = r.first;
+                        mhi[o] = r.second;
+                        ++o;
+                    }
+                });
                 dsd.nrptr.upload(mptr, s);
                 dsd.nrlo.upload(mlo, s);
                 dsd.nrhi.upload(mhi, s);
@@ -1683,6 +1917,7 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
             h->stats.near_entries = near_entries;
             h->stats.plan_bytes += nnz * 12 + (ntest + ntrial) * 8;
         }
+        timer.mark("near plans");
         h->stats.coupling_blocks = coupling->nblocks;
         h->stats.coupling_entries = coupling->nentries;
         h->stats.basis_entries_test = h->test.basis_entries;
@@ -1692,7 +1927,165 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
         h->stats.coeffs_test = h->test.K;
         h->stats.coeffs_trial = h->trial.K;
         NCA_HIP(hipStreamSynchronize(st));
+        timer.mark("stats + sync");
         *out = h.release();
+    }
+}
+
+}  // namespace nca
+
+using namespace nca;
+
+extern "C" {
+
+int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test, const nca_points* trial,
+                      const nca_h2_desc* d, const nca_coupling* coupling, int32_t precision, nca_h2** out) {
+    return guarded([&] {
+        NCA_REQUIRE(d, NCA_ERR_ARGUMENT, "nca_h2_create: invalid argument");
+        BlockLayout lay;
+        lay.ne_test = d->nn_test;
+        lay.ne_trial = d->nn_trial;
+        create_core(ctx, kernel, test, trial, d, lay, coupling, precision, out);
+    });
+}
+
+// The directional plans lowered onto the block form of create_core: the
+// bases of a node's directions become one basis of the node over all its
+// coefficients (test [U_1 ... U_k], trial [V_1; ...; V_k]); the coupling plan
+// becomes a CSR over test direction indices.
+int32_t nca_dh2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test, const nca_points* trial,
+                       const nca_dh2_desc* d, const nca_coupling* coupling, int32_t precision, nca_h2** out) {
+    return guarded([&] {
+        NCA_REQUIRE(d && coupling, NCA_ERR_ARGUMENT, "nca_dh2_create: invalid argument");
+        NCA_REQUIRE(d->test_dirptr && d->trial_dirptr, NCA_ERR_ARGUMENT, "nca_dh2_create: missing direction pointers");
+        struct Lowered {
+            std::vector<int64_t> nodes, off;
+            std::vector<double2> blocks;
+        };
+        auto lower = [](bool testside, int64_t nn, const int64_t* lo, const int64_t* hi, const int64_t* dirptr,
+                        const int64_t* cptr, int64_t nb, const int64_t* bnodes, const int64_t* bdirptr,
+                        const int64_t* bdiridx, const int64_t* boff, const double* blocks, Lowered& L) {
+            const double2* B = reinterpret_cast<const double2*>(blocks);
+            L.off.assign(1, 0);
+            const char* side = testside ? "test" : "trial";
+            for (int64_t i = 0; i < nb; ++i) {
+                const int64_t t = bnodes[i] - 1;
+                NCA_REQUIRE(t >= 0 && t < nn, NCA_ERR_ARGUMENT, "nca_dh2_create: basis node out of range");
+                const int64_t m = hi[t] - lo[t] + 1;
+                const int64_t e0 = dirptr[t] - 1, e1 = dirptr[t + 1] - 1;
+                const int64_t c0 = cptr[e0] - 1, R = cptr[e1] - cptr[e0];
+                const int64_t j0 = bdirptr[i] - 1, j1 = bdirptr[i + 1] - 1;
+                // listed directions: increasing, of this node; the others hold no coefficients
+                int64_t prev = e0 - 1, listed = 0;
+                for (int64_t j = j0; j < j1; ++j) {
+                    const int64_t e = bdiridx[j] - 1;
+                    NCA_REQUIRE(e > prev && e < e1, NCA_ERR_ARGUMENT,
+                                std::string("nca_dh2_create: ") + side + " basis directions of node " +
+                                    std::to_string(t + 1) + " are not increasing direction indices of the node");
+                    for (int64_t q = prev + 1; q < e; ++q)
+                        NCA_REQUIRE(cptr[q + 1] == cptr[q], NCA_ERR_ARGUMENT,
+                                    "nca_dh2_create: a direction with coefficients has no basis");
+                    const int64_t r = cptr[e + 1] - cptr[e];
+                    NCA_REQUIRE(boff[j + 1] - boff[j] == m * r, NCA_ERR_DIMENSION,
+                                std::string("DimensionMismatch: ") + side + " basis of direction index " +
+                                    std::to_string(e + 1) + " has " + std::to_string(boff[j + 1] - boff[j]) +
+                                    " entries, " + std::to_string(m * r) + " expected");
+                    listed += r;
+                    prev = e;
+                }
+                for (int64_t q = prev + 1; q < e1; ++q)
+                    NCA_REQUIRE(cptr[q + 1] == cptr[q], NCA_ERR_ARGUMENT,
+                                "nca_dh2_create: a direction with coefficients has no basis");
+                NCA_REQUIRE(listed == R, NCA_ERR_ARGUMENT, "nca_dh2_create: basis ranks differ from the plan");
+                const size_t base = L.blocks.size();
+                L.blocks.resize(base + (size_t)(m * R));
+                int64_t col = 0;
+                for (int64_t j = j0; j < j1; ++j) {
+                    const int64_t e = bdiridx[j] - 1, r = cptr[e + 1] - cptr[e];
+                    const double2* Be = B + boff[j];
+                    const int64_t ro = cptr[e] - 1 - c0;  // first coefficient of e within the node
+                    if (testside) {
+                        // U_t(:, ro + k) = U_e(:, k)
+                        for (int64_t k = 0; k < r; ++k)
+                            for (int64_t a = 0; a < m; ++a) L.blocks[base + a + (ro + k) * m] = Be[a + k * m];
+                    } else {
+                        // V_t(ro + k, :) = V_e(k, :)
+                        for (int64_t a = 0; a < m; ++a)
+                            for (int64_t k = 0; k < r; ++k) L.blocks[base + (ro + k) + a * R] = Be[k + a * r];
+                    }
+                    col += r;
+                }
+                L.nodes.push_back(t + 1);
+                L.off.push_back(L.off.back() + m * R);
+            }
+            if (L.blocks.empty()) L.blocks.resize(1);
+        };
+        Lowered lt, ls;
+        lower(true, d->nn_test, d->test_lo, d->test_hi, d->test_dirptr, d->test_cptr, d->nbnodes_test, d->bt_nodes,
+              d->bt_dirptr, d->bt_diridx, d->bt_off, d->bt_blocks, lt);
+        lower(false, d->nn_trial, d->trial_lo, d->trial_hi, d->trial_dirptr, d->trial_cptr, d->nbnodes_trial,
+              d->bs_nodes, d->bs_dirptr, d->bs_diridx, d->bs_off, d->bs_blocks, ls);
+        // coupling plan: blocks of every test node, grouped by increasing test direction index
+        const int64_t nblk = d->cp_nodeptr[d->nn_test] - d->cp_nodeptr[0];
+        NCA_REQUIRE(nblk == coupling->nblocks, NCA_ERR_ARGUMENT, "nca_dh2_create: coupling plan differs from the store");
+        std::vector<int64_t> cp_ptr(d->ne_test + 1, 0);
+        int64_t last = -1;
+        for (int64_t t = 0; t < d->nn_test; ++t)
+            for (int64_t i = d->cp_nodeptr[t] - d->cp_nodeptr[0]; i < d->cp_nodeptr[t + 1] - d->cp_nodeptr[0]; ++i) {
+                const int64_t e = d->cp_tidcs[i] - 1;
+                NCA_REQUIRE(e >= d->test_dirptr[t] - 1 && e < d->test_dirptr[t + 1] - 1 && e >= last,
+                            NCA_ERR_ARGUMENT,
+                            "nca_dh2_create: coupling blocks not grouped by the test directions of their node");
+                last = e;
+                cp_ptr[e + 1]++;
+            }
+        for (int64_t e = 0; e < d->ne_test; ++e) cp_ptr[e + 1] += cp_ptr[e];
+        for (auto& v : cp_ptr) v += 1;
+        nca_h2_desc h{};
+        h.M = d->M;
+        h.N = d->N;
+        h.nn_test = d->nn_test;
+        h.nn_trial = d->nn_trial;
+        h.test_perm = d->test_perm;
+        h.test_lo = d->test_lo;
+        h.test_hi = d->test_hi;
+        h.trial_perm = d->trial_perm;
+        h.trial_lo = d->trial_lo;
+        h.trial_hi = d->trial_hi;
+        h.test_cptr = d->test_cptr;
+        h.trial_cptr = d->trial_cptr;
+        h.nbasis_test = (int64_t)lt.nodes.size();
+        h.nbasis_trial = (int64_t)ls.nodes.size();
+        h.basis_test_nodes = lt.nodes.data();
+        h.basis_test_off = lt.off.data();
+        h.basis_trial_nodes = ls.nodes.data();
+        h.basis_trial_off = ls.off.data();
+        h.basis_test = reinterpret_cast<const double*>(lt.blocks.data());
+        h.basis_trial = reinterpret_cast<const double*>(ls.blocks.data());
+        h.nlev_test = d->nlev_test;
+        h.nlev_trial = d->nlev_trial;
+        h.tt_levptr = d->tt_levptr;
+        h.tt_levidcs = d->tt_levidcs;
+        h.tt_chdptr = d->tt_chdptr;
+        h.tt_chds = d->tt_chds;
+        h.tt_off = d->tt_off;
+        h.st_levptr = d->st_levptr;
+        h.st_levidcs = d->st_levidcs;
+        h.st_chdptr = d->st_chdptr;
+        h.st_chds = d->st_chds;
+        h.st_off = d->st_off;
+        h.tt_blocks = d->tt_blocks;
+        h.st_blocks = d->st_blocks;
+        h.cp_ptr = cp_ptr.data();
+        h.cp_nodes = d->cp_sidcs + (d->cp_nodeptr[0] - 1);
+        h.nearptr = d->nearptr;
+        h.nearnodes = d->nearnodes;
+        BlockLayout lay;
+        lay.ne_test = d->ne_test;
+        lay.ne_trial = d->ne_trial;
+        lay.test_dirptr = d->test_dirptr;
+        lay.trial_dirptr = d->trial_dirptr;
+        create_core(ctx, kernel, test, trial, &h, lay, coupling, precision, out);
     });
 }
 

@@ -243,15 +243,28 @@ struct TreeView {
     bool leaf(int64_t i) const { return childptr[i + 1] == childptr[i]; }
 };
 
-// Standard admissibility with bounding balls: a pair is near iff
-// max(diam_t, diam_s) > eta * dist(t, s), dist = |c_t - c_s| - (r_t + r_s).
-inline bool isnear_pair(const TreeView& a, const TreeView& b, int64_t t, int64_t s, double eta) {
-    double d = std::sqrt(dist2(a.center + 3 * t, b.center + 3 * s)) - (a.radius[t] + b.radius[s]);
-    double diam = 2.0 * std::max(a.radius[t], b.radius[s]);
-    return diam > eta * d;
+// Admissibility of a pair of bounding balls (radius = half size * sqrt(3)
+// for boxes), dist = |c_t - c_s| - (r_t + r_s).
+//   standard (wideband = 0): near iff max(diam_t, diam_s) > eta * dist;
+//   wideband (IsNearWidebandFunctor, nearinteractions.jl:83-103): with
+//   sep = max(dist, 0), a pair whose smaller diameter d has k d <= gamma is
+//   low frequency, near iff 2 max(r) > eta_lf sep; otherwise near iff
+//   4 k max(r)^2 > eta_hf sep.
+struct Criterion {
+    int wideband = 0;
+    double eta = 1.0, k = 0.0, etalf = 1.0, etahf = 5.0, gamma = 1.0;
+};
+
+inline bool isnear_pair(const TreeView& a, const TreeView& b, int64_t t, int64_t s, const Criterion& c) {
+    const double ra = a.radius[t], rb = b.radius[s];
+    const double d = std::sqrt(dist2(a.center + 3 * t, b.center + 3 * s)) - (ra + rb);
+    if (!c.wideband) return 2.0 * std::max(ra, rb) > c.eta * d;
+    const double sep = std::max(d, 0.0);
+    if (c.k * std::min(2.0 * ra, 2.0 * rb) <= c.gamma) return 2.0 * std::max(ra, rb) > c.etalf * sep;
+    return 4.0 * c.k * std::max(ra * ra, rb * rb) > c.etahf * sep;
 }
 
-void traverse(const TreeView& a, const TreeView& b, int64_t t, int64_t s, double eta,
+void traverse(const TreeView& a, const TreeView& b, int64_t t, int64_t s, const Criterion& eta,
               std::vector<std::pair<int64_t, int64_t>>& nearp,
               std::vector<std::pair<int64_t, int64_t>>& farp) {
     if (!isnear_pair(a, b, t, s, eta)) {
@@ -286,7 +299,7 @@ void to_csr(int64_t nrows, std::vector<std::pair<int64_t, int64_t>>& pairs, std:
     for (int64_t i = 0; i < nrows; ++i) std::sort(idx.begin() + ptr[i], idx.begin() + ptr[i + 1]);
 }
 
-Partition interactions(const TreeView& a, const TreeView& b, double eta) {
+Partition interactions(const TreeView& a, const TreeView& b, const Criterion& eta) {
     // Expand the pair traversal breadth-first to a frontier, then finish every
     // frontier pair depth-first in parallel; the CSR is sorted afterwards, so
     // the result does not depend on the schedule.
@@ -344,23 +357,23 @@ Partition interactions(const TreeView& a, const TreeView& b, double eta) {
 // 65,536-point sphere, k = 1, tol 1e-6: 2.5 tol (oversample 1: 18 tol);
 // 1M-point sphere, k = 2, tol 1e-4: 2.4 tol with 6.2e8 coupling entries
 // (oversample 1, weight 1.2: 2.0 tol, 6.9e8 entries).
-Csr farpivots(int64_t nn_self, const int64_t* parent, const int64_t* lo, const int64_t* hi,
-              const double* center, const int64_t* other_lo, const int64_t* other_hi,
-              const int64_t* other_perm, const double* other_pts, const int64_t* farptr,
-              const int64_t* farnodes, int64_t maxrank, double weight, double oversample) {
-    std::vector<std::vector<int64_t>> result(nn_self);
+// farfield(t, F) fills F with the far nodes of item t; nvals(t) = |t|;
+// center(t) = its center (3 doubles).
+template <class FarField, class NVals, class Center>
+Csr select_farpivots(int64_t nitems, FarField farfield, NVals nvals, Center center, const int64_t* other_lo,
+                     const int64_t* other_hi, const int64_t* other_perm, const double* other_pts, int64_t maxrank,
+                     double weight, double oversample) {
+    std::vector<std::vector<int64_t>> result(nitems);
 #pragma omp parallel
     {
         std::vector<int64_t> F, pool;
         std::vector<double> d2c, mind;
 #pragma omp for schedule(dynamic, 64)
-        for (int64_t t = 0; t < nn_self; ++t) {
+        for (int64_t t = 0; t < nitems; ++t) {
             F.clear();
-            for (int64_t node = t; node >= 0; node = parent[node])
-                for (int64_t k = farptr[node]; k < farptr[node + 1]; ++k) F.push_back(farnodes[k]);
+            farfield(t, F);
             if (F.empty()) continue;
-            const int64_t limit =
-                std::min<int64_t>(maxrank, (int64_t)std::ceil(oversample * (double)(hi[t] - lo[t])));
+            const int64_t limit = std::min<int64_t>(maxrank, (int64_t)std::ceil(oversample * (double)nvals(t)));
             int64_t n = 0, total = 0;
             for (int64_t f : F) {
                 const int64_t sz = other_hi[f] - other_lo[f];
@@ -381,7 +394,7 @@ Csr farpivots(int64_t nn_self, const int64_t* parent, const int64_t* lo, const i
                 per *= 2;
             }
             const int64_t np = (int64_t)pool.size();
-            const double* ct = center + 3 * t;
+            const double* ct = center(t);
             d2c.resize(np);
             mind.assign(np, std::numeric_limits<double>::infinity());
             int64_t first = 0;
@@ -412,12 +425,27 @@ Csr farpivots(int64_t nn_self, const int64_t* parent, const int64_t* lo, const i
         }
     }
     Csr c;
-    c.ptr.assign(nn_self + 1, 0);
-    for (int64_t t = 0; t < nn_self; ++t) c.ptr[t + 1] = c.ptr[t] + (int64_t)result[t].size();
-    c.idx.resize(c.ptr[nn_self]);
-    for (int64_t t = 0; t < nn_self; ++t)
+    c.ptr.assign(nitems + 1, 0);
+    for (int64_t t = 0; t < nitems; ++t) c.ptr[t + 1] = c.ptr[t] + (int64_t)result[t].size();
+    c.idx.resize(c.ptr[nitems]);
+    for (int64_t t = 0; t < nitems; ++t)
         std::copy(result[t].begin(), result[t].end(), c.idx.begin() + c.ptr[t]);
     return c;
+}
+
+// Far field of a node: its own far nodes, then those of every ancestor.
+Csr farpivots(int64_t nn_self, const int64_t* parent, const int64_t* lo, const int64_t* hi,
+              const double* center, const int64_t* other_lo, const int64_t* other_hi,
+              const int64_t* other_perm, const double* other_pts, const int64_t* farptr,
+              const int64_t* farnodes, int64_t maxrank, double weight, double oversample) {
+    return select_farpivots(
+        nn_self,
+        [&](int64_t t, std::vector<int64_t>& F) {
+            for (int64_t node = t; node >= 0; node = parent[node])
+                for (int64_t k = farptr[node]; k < farptr[node + 1]; ++k) F.push_back(farnodes[k]);
+        },
+        [&](int64_t t) { return hi[t] - lo[t]; }, [&](int64_t t) { return center + 3 * t; }, other_lo, other_hi,
+        other_perm, other_pts, maxrank, weight, oversample);
 }
 
 }  // namespace
@@ -465,7 +493,26 @@ ncat_partition* ncat_interactions(int64_t nn_a, const int64_t* childptr_a, const
                                   const double* center_b, const double* radius_b, double eta) {
     TreeView a{nn_a, childptr_a, children_a, center_a, radius_a};
     TreeView b{nn_b, childptr_b, children_b, center_b, radius_b};
-    return reinterpret_cast<ncat_partition*>(new Partition(interactions(a, b, eta)));
+    Criterion c;
+    c.eta = eta;
+    return reinterpret_cast<ncat_partition*>(new Partition(interactions(a, b, c)));
+}
+
+// interactions with the wideband admissibility isnearwideband(k; etalf, etahf, gamma)
+ncat_partition* ncat_interactions_wideband(int64_t nn_a, const int64_t* childptr_a, const int64_t* children_a,
+                                           const double* center_a, const double* radius_a, int64_t nn_b,
+                                           const int64_t* childptr_b, const int64_t* children_b,
+                                           const double* center_b, const double* radius_b, double k, double etalf,
+                                           double etahf, double gamma) {
+    TreeView a{nn_a, childptr_a, children_a, center_a, radius_a};
+    TreeView b{nn_b, childptr_b, children_b, center_b, radius_b};
+    Criterion c;
+    c.wideband = 1;
+    c.k = k;
+    c.etalf = etalf;
+    c.etahf = etahf;
+    c.gamma = gamma;
+    return reinterpret_cast<ncat_partition*>(new Partition(interactions(a, b, c)));
 }
 
 void ncat_partition_sizes(const ncat_partition* h, int64_t* nnear, int64_t* nfar) {
@@ -492,6 +539,105 @@ ncat_csr* ncat_farpivots(int64_t nn_self, const int64_t* parent, const int64_t* 
     return reinterpret_cast<ncat_csr*>(new Csr(farpivots(nn_self, parent, lo, hi, center, other_lo, other_hi,
                                                          other_perm, other_pts, farptr, farnodes, maxrank, weight,
                                                          oversample)));
+}
+
+// Phase 1 over explicit far fields (one list of far nodes of the other tree
+// per item, 0-based CSR ffptr/ffnodes), item t of node itemnode[t]: the
+// directional form, one item per (cluster, direction) with its dirfarfield.
+ncat_csr* ncat_farpivots_lists(int64_t nitems, const int64_t* itemnode, const int64_t* lo, const int64_t* hi,
+                               const double* center, const int64_t* other_lo, const int64_t* other_hi,
+                               const int64_t* other_perm, const double* other_pts, const int64_t* ffptr,
+                               const int64_t* ffnodes, int64_t maxrank, double weight, double oversample) {
+    return reinterpret_cast<ncat_csr*>(new Csr(select_farpivots(
+        nitems,
+        [&](int64_t t, std::vector<int64_t>& F) { F.assign(ffnodes + ffptr[t], ffnodes + ffptr[t + 1]); },
+        [&](int64_t t) { return hi[itemnode[t]] - lo[itemnode[t]]; },
+        [&](int64_t t) { return center + 3 * itemnode[t]; }, other_lo, other_hi, other_perm, other_pts, maxrank,
+        weight, oversample)));
+}
+
+// dirfarfield of every direction index (abstractdirections.jl:205-243), as a
+// 0-based CSR of far nodes. Inputs (0-based): parent (-1 at the root),
+// dirptr (nn+1) direction indices of each node, dirs (direction labels, 0 =
+// low frequency), chdirptr (nn+1) / chdirmap: for each direction of the parent
+// the local direction of the node, diridxtofarptr (ne+1) and farnodes.
+// Label 0: the node's own far nodes of direction 0, then those of each
+// ancestor while it holds direction 0. Other labels: the own far nodes, then
+// level by level upwards the far nodes of the parent directions that map onto
+// the current set, in increasing local order.
+ncat_csr* ncat_dirfarfield(int64_t nn, const int64_t* parent, const int64_t* dirptr, const int64_t* dirs,
+                           const int64_t* chdirptr, const int64_t* chdirmap, const int64_t* diridxtofarptr,
+                           const int64_t* farnodes) {
+    const int64_t ne = dirptr[nn];
+    std::vector<int64_t> node_of(ne);
+    for (int64_t t = 0; t < nn; ++t)
+        for (int64_t e = dirptr[t]; e < dirptr[t + 1]; ++e) node_of[e] = t;
+    std::vector<std::vector<int64_t>> out(ne);
+#pragma omp parallel
+    {
+        std::vector<int64_t> S, next;
+        std::vector<char> mark;
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t e = 0; e < ne; ++e) {
+            std::vector<int64_t>& F = out[e];
+            auto own = [&](int64_t d) {
+                for (int64_t k = diridxtofarptr[d]; k < diridxtofarptr[d + 1]; ++k) F.push_back(farnodes[k]);
+            };
+            const int64_t t = node_of[e];
+            if (dirs[e] == 0) {
+                for (int64_t cur = t; cur >= 0; cur = parent[cur]) {
+                    int64_t z = -1;
+                    for (int64_t d = dirptr[cur]; d < dirptr[cur + 1]; ++d)
+                        if (dirs[d] == 0) { z = d; break; }
+                    if (z < 0) break;
+                    own(z);
+                }
+                continue;
+            }
+            own(e);
+            S.assign(1, e - dirptr[t]);
+            for (int64_t cur = t; parent[cur] >= 0 && !S.empty(); cur = parent[cur]) {
+                const int64_t nd = dirptr[cur + 1] - dirptr[cur];
+                mark.assign((size_t)nd, 0);
+                for (int64_t l : S) mark[l] = 1;
+                next.clear();
+                for (int64_t j = 0; j < chdirptr[cur + 1] - chdirptr[cur]; ++j) {
+                    const int64_t l = chdirmap[chdirptr[cur] + j];
+                    if (l >= 0 && l < nd && mark[l]) next.push_back(j);
+                }
+                const int64_t P = parent[cur];
+                for (int64_t j : next) own(dirptr[P] + j);
+                S.swap(next);
+            }
+        }
+    }
+    Csr* c = new Csr();
+    c->ptr.assign(ne + 1, 0);
+    for (int64_t e = 0; e < ne; ++e) c->ptr[e + 1] = c->ptr[e] + (int64_t)out[e].size();
+    c->idx.resize(c->ptr[ne]);
+    for (int64_t e = 0; e < ne; ++e) std::copy(out[e].begin(), out[e].end(), c->idx.begin() + c->ptr[e]);
+    return reinterpret_cast<ncat_csr*>(c);
+}
+
+// Packs nb complex blocks (interleaved doubles): block b is rows[b] x cols[b]
+// column-major at dst + 2 dst_off[b], read from src at src_off[b] with leading
+// dimension ld[b], as is (transpose = 0: dst(i, j) = src(i + j ld)) or
+// transposed (transpose = 1: dst(i, j) = src(j + i ld)).
+void ncat_gather_blocks(int64_t nb, const int64_t* src_off, const int64_t* ld, const int64_t* rows,
+                        const int64_t* cols, int32_t transpose, const double* src, const int64_t* dst_off,
+                        double* dst) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t b = 0; b < nb; ++b) {
+        const int64_t m = rows[b], n = cols[b], l = ld[b];
+        const double* s0 = src + 2 * src_off[b];
+        double* d0 = dst + 2 * dst_off[b];
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i < m; ++i) {
+                const int64_t k = transpose ? j + i * l : i + j * l;
+                d0[2 * (i + j * m)] = s0[2 * k];
+                d0[2 * (i + j * m) + 1] = s0[2 * k + 1];
+            }
+    }
 }
 
 // CSR transpose with the rows of every column in increasing order (a

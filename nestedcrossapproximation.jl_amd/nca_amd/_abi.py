@@ -40,6 +40,26 @@ class H2Desc(ctypes.Structure):
     ]
 
 
+class DH2Desc(ctypes.Structure):
+    _fields_ = [
+        ("M", I64), ("N", I64), ("nn_test", I64), ("nn_trial", I64), ("ne_test", I64), ("ne_trial", I64),
+        ("test_perm", P), ("test_lo", P), ("test_hi", P),
+        ("trial_perm", P), ("trial_lo", P), ("trial_hi", P),
+        ("test_dirptr", P), ("trial_dirptr", P),
+        ("test_cptr", P), ("trial_cptr", P),
+        ("nbnodes_test", I64), ("nbnodes_trial", I64),
+        ("bt_nodes", P), ("bt_dirptr", P), ("bt_diridx", P), ("bt_off", P),
+        ("bs_nodes", P), ("bs_dirptr", P), ("bs_diridx", P), ("bs_off", P),
+        ("bt_blocks", P), ("bs_blocks", P),
+        ("nlev_test", I64), ("nlev_trial", I64),
+        ("tt_levptr", P), ("tt_levidcs", P), ("tt_chdptr", P), ("tt_chds", P), ("tt_off", P),
+        ("st_levptr", P), ("st_levidcs", P), ("st_chdptr", P), ("st_chds", P), ("st_off", P),
+        ("tt_blocks", P), ("st_blocks", P),
+        ("cp_nodeptr", P), ("cp_tidcs", P), ("cp_sidcs", P),
+        ("nearptr", P), ("nearnodes", P),
+    ]
+
+
 class H2Stats(ctypes.Structure):
     _fields_ = [(n, I64) for n in (
         "coupling_blocks", "coupling_entries", "basis_entries_test", "basis_entries_trial",
@@ -75,6 +95,7 @@ SIGNATURES = {
     "nca_coupling_download": (I32, [P, P, P, P]),
     "nca_coupling_destroy": (I32, [P]),
     "nca_h2_create": (I32, [P, P, P, P, P, P, I32, PP]),
+    "nca_dh2_create": (I32, [P, P, P, P, P, P, I32, PP]),
     "nca_h2_destroy": (I32, [P]),
     "nca_h2_set_precision": (I32, [P, I32]),
     "nca_h2_matvec": (I32, [P, I32, I64, P, I64, P, I64, P, P]),

@@ -60,16 +60,24 @@ def _declare_tree(L):
     L.ncat_tree_free.argtypes = [P]
     L.ncat_interactions.restype = P
     L.ncat_interactions.argtypes = [I64, P, P, P, P, I64, P, P, P, P, D]
+    L.ncat_interactions_wideband.restype = P
+    L.ncat_interactions_wideband.argtypes = [I64, P, P, P, P, I64, P, P, P, P, D, D, D, D]
     L.ncat_partition_sizes.restype = None
     L.ncat_partition_sizes.argtypes = [P, P, P]
     L.ncat_partition_export.restype = None
     L.ncat_partition_export.argtypes = [P, P, P, P, P]
     L.ncat_partition_free.restype = None
     L.ncat_partition_free.argtypes = [P]
+    L.ncat_gather_blocks.restype = None
+    L.ncat_gather_blocks.argtypes = [I64, P, P, P, P, ctypes.c_int32, P, P, P]
     L.ncat_csr_transpose.restype = P
     L.ncat_csr_transpose.argtypes = [I64, P, P, I64]
     L.ncat_farpivots.restype = P
     L.ncat_farpivots.argtypes = [I64, P, P, P, P, P, P, P, P, P, P, I64, ctypes.c_double, ctypes.c_double]
+    L.ncat_farpivots_lists.restype = P
+    L.ncat_farpivots_lists.argtypes = [I64, P, P, P, P, P, P, P, P, P, P, I64, ctypes.c_double, ctypes.c_double]
+    L.ncat_dirfarfield.restype = P
+    L.ncat_dirfarfield.argtypes = [I64, P, P, P, P, P, P, P]
     L.ncat_csr_nnz.restype = I64
     L.ncat_csr_nnz.argtypes = [P]
     L.ncat_csr_export.restype = None

@@ -355,24 +355,34 @@ class TransferStore:
         self.blocks = blocks
 
 
+def _gather_blocks(src, src_off, ld, rows, cols, transpose):
+    """Pack complex blocks rows[b] x cols[b] (column-major) read from src at
+    src_off[b] with leading dimension ld[b], as is or transposed (host lib)."""
+    from ._native import tree_lib
+    sz = rows * cols
+    dst_off = np.zeros(len(sz) + 1, np.int64)
+    np.cumsum(sz, out=dst_off[1:])
+    out = np.empty(max(int(dst_off[-1]), 1), np.complex128)
+    arrs = [np.ascontiguousarray(a, np.int64) for a in (src_off, ld, rows, cols, dst_off)]
+    if len(sz):
+        tree_lib().ncat_gather_blocks(len(sz), _p(arrs[0]), _p(arrs[1]), _p(arrs[2]), _p(arrs[3]), int(transpose),
+                                      _p(src), _p(arrs[4]), _p(out))
+    return dst_off, out[:dst_off[-1]]
+
+
 def _stores_flat(f, tree, side):
-    """stores_from_factors for LevelFactors, by index arithmetic on the packed
-    factors (same layout and order as the per-node path below)."""
+    """stores_from_factors for LevelFactors: the same layout and order as the
+    per-node path below, packed by the host library from the flat factors."""
     leaf = tree.leafmask()
-    Xf = f.Xflat
+    Xf = np.ascontiguousarray(f.Xflat)
     # bases: leaves with a block, node order; test X (m x r), trial X^T (r x m)
     bn = np.nonzero(f.assigned & leaf)[0]
     m, r, xo = f.m[bn], f.rank[bn], f.xoff[bn]
-    sz = m * r
-    boff = np.zeros(len(bn) + 1, np.int64)
-    np.cumsum(sz, out=boff[1:])
-    q = np.arange(boff[-1], dtype=np.int64) - np.repeat(boff[:-1], sz)
     if side == 0:
-        src = np.repeat(xo, sz) + q
+        boff, bl = _gather_blocks(Xf, xo, m, m, r, False)
     else:
-        rr, mm = np.repeat(r, sz), np.repeat(m, sz)
-        src = np.repeat(xo, sz) + q // np.maximum(rr, 1) + (q % np.maximum(rr, 1)) * mm
-    basis = BasisStore(bn.astype(np.int64), boff, Xf[src])
+        boff, bl = _gather_blocks(Xf, xo, m, r, m, True)
+    basis = BasisStore(bn.astype(np.int64), boff, bl)
     # transfers: levels ascending, parents in level_transfer_nodes order, all children
     parents = [np.asarray(f.level_transfer_nodes.get(lev, []), np.int64) for lev in sorted(tree.levels().tolist())]
     levptr = np.zeros(len(parents) + 1, np.int64)
@@ -387,19 +397,13 @@ def _stores_flat(f, tree, side):
     # row offset of each child inside its parent's block: exclusive sum of rc per parent
     csum = np.cumsum(rc)
     first = np.repeat(chdptr[:-1], nch)
-    oc = csum - rc - np.where(first > 0, csum[first - 1], 0) if len(C) else np.zeros(0, np.int64)
+    oc = csum - rc - np.where(first > 0, csum[np.maximum(first - 1, 0)], 0) if len(C) else np.zeros(0, np.int64)
     rp, mp, xp = f.rank[Pe], f.m[Pe], f.xoff[Pe]
-    esz = rc * rp
-    toff = np.zeros(len(C) + 1, np.int64)
-    np.cumsum(esz, out=toff[1:])
-    q = np.arange(toff[-1], dtype=np.int64) - np.repeat(toff[:-1], esz)
-    rcr, rpr = np.repeat(rc, esz), np.repeat(rp, esz)
-    if side == 0:   # E (rc x rp) column-major: q = a + b rc
-        a_, b_ = q % np.maximum(rcr, 1), q // np.maximum(rcr, 1)
-    else:           # E^T (rp x rc) column-major: q = b + a rp
-        b_, a_ = q % np.maximum(rpr, 1), q // np.maximum(rpr, 1)
-    src = np.repeat(xp, esz) + np.repeat(oc, esz) + a_ + b_ * np.repeat(mp, esz)
-    transfer = TransferStore(levptr, P, chdptr, C.astype(np.int64), toff, Xf[src])
+    if side == 0:   # E = X_p[oc : oc + rc, :] (rc x rp)
+        toff, tl = _gather_blocks(Xf, xp + oc, mp, rc, rp, False)
+    else:           # E^T (rp x rc)
+        toff, tl = _gather_blocks(Xf, xp + oc, mp, rp, rc, True)
+    transfer = TransferStore(levptr, P, chdptr, C.astype(np.int64), toff, tl)
     return basis, transfer
 
 

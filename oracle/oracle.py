@@ -278,3 +278,113 @@ def set_threads(n):
 
 def num_threads():
     return int(lib().orc_num_threads())
+
+
+def _near_block(k, sign, tp, sp):
+    """Helmholtz block exp(i sign k r) / (4 pi r), 0 at r = 0 (the oracle's entry)."""
+    d = tp[:, None, :] - sp[None, :, :]
+    r2 = (d[:, :, 0] * d[:, :, 0] + d[:, :, 1] * d[:, :, 1]) + d[:, :, 2] * d[:, :, 2]
+    r = np.sqrt(r2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        g = (np.cos(k * r) + 1j * sign * np.sin(k * r)) / (4 * np.pi * r)
+    g[r2 == 0.0] = 0.0
+    return g
+
+
+class OracleDH2:
+    """The directional H^2 product restated stage by stage from the
+    reference's directional methods, in numpy, for small problems:
+    _project_to_coefficients! (src/nca/aggregate.jl:21-42),
+    _aggregate_coefficients! (aggregate.jl:44-67), _couple_forward! with a
+    DirCouplingTraversalPlan (src/nca/couple.jl:26-50),
+    _disaggregate_coefficients! by groups (src/nca/disaggregate.jl:27-60),
+    _project_to_output! (disaggregate.jl:82-103), then y += Near x
+    (src/nca/petrovgalerkinnca.jl:474). Takes the 1-based nca_dh2_desc arrays
+    (as a dict) and the coupling blocks (0-based complex offsets, blocks)."""
+
+    def __init__(self, desc, k, sign, testpts, trialpts, cp_off, cp_blocks):
+        self.d, self.k, self.sign = desc, float(k), int(sign)
+        self.tp = np.asarray(testpts, np.float64)
+        self.sp = np.asarray(trialpts, np.float64)
+        self.cp_off, self.cp_blocks = np.asarray(cp_off), np.asarray(cp_blocks)
+
+    def _values(self, side, node):
+        d = self.d
+        perm, lo, hi = (d["test_perm"], d["test_lo"], d["test_hi"]) if side == 0 else \
+            (d["trial_perm"], d["trial_lo"], d["trial_hi"])
+        return perm[lo[node] - 1:hi[node]] - 1
+
+    def matvec(self, X, near=True):
+        d = self.d
+        X = np.asarray(X, np.complex128)
+        vec = X.ndim == 1
+        X = X.reshape(X.shape[0], -1)
+        R = X.shape[1]
+        tc, sc = d["test_cptr"] - 1, d["trial_cptr"] - 1
+        xh = np.zeros((sc[d["ne_trial"]], R), np.complex128)
+        yh = np.zeros((tc[d["ne_test"]], R), np.complex128)
+        Y = np.zeros((d["M"], R), np.complex128)
+        # stage 1: every direction of a basis cluster projects the cluster's values
+        for i in range(d["nbnodes_trial"]):
+            node = d["bs_nodes"][i] - 1
+            xn = X[self._values(1, node)]
+            m = len(xn)
+            for l in range(d["bs_dirptr"][i] - 1, d["bs_dirptr"][i + 1] - 1):
+                e = d["bs_diridx"][l] - 1
+                r = sc[e + 1] - sc[e]
+                V = d["bs_blocks"][d["bs_off"][l]:d["bs_off"][l + 1]].reshape((r, m), order="F")
+                xh[sc[e]:sc[e + 1]] = V @ xn
+        # stage 2: levels from the deepest, parent direction += E child direction
+        lp, li, cp, ch, off = (d["st_levptr"] - 1, d["st_levidcs"] - 1, d["st_chdptr"] - 1, d["st_chds"] - 1,
+                               d["st_off"])
+        for lev in range(d["nlev_trial"] - 1, -1, -1):
+            for idx in range(lp[lev], lp[lev + 1]):
+                p = li[idx]
+                rp = sc[p + 1] - sc[p]
+                for c_ in range(cp[idx], cp[idx + 1]):
+                    c = ch[c_]
+                    rc = sc[c + 1] - sc[c]
+                    E = d["st_blocks"][off[c_]:off[c_ + 1]].reshape((rp, rc), order="F")
+                    xh[sc[p]:sc[p + 1]] += E @ xh[sc[c]:sc[c + 1]]
+        # stage 3: per test cluster its (test direction, trial direction) blocks
+        nptr, tid, sid = d["cp_nodeptr"] - 1, d["cp_tidcs"] - 1, d["cp_sidcs"] - 1
+        for t in range(d["nn_test"]):
+            for i in range(nptr[t], nptr[t + 1]):
+                a, b = tid[i], sid[i]
+                rt, rs = tc[a + 1] - tc[a], sc[b + 1] - sc[b]
+                S = self.cp_blocks[self.cp_off[i]:self.cp_off[i + 1]].reshape((rt, rs), order="F")
+                yh[tc[a]:tc[a + 1]] += S @ xh[sc[b]:sc[b + 1]]
+        # stage 4: levels from the root, child direction += E parent direction
+        lp, li, cp, ch, off = (d["tt_levptr"] - 1, d["tt_levidcs"] - 1, d["tt_chdptr"] - 1, d["tt_chds"] - 1,
+                               d["tt_off"])
+        for lev in range(d["nlev_test"]):
+            for idx in range(lp[lev], lp[lev + 1]):
+                p = li[idx]
+                rp = tc[p + 1] - tc[p]
+                for c_ in range(cp[idx], cp[idx + 1]):
+                    c = ch[c_]
+                    rc = tc[c + 1] - tc[c]
+                    E = d["tt_blocks"][off[c_]:off[c_ + 1]].reshape((rc, rp), order="F")
+                    yh[tc[c]:tc[c + 1]] += E @ yh[tc[p]:tc[p + 1]]
+        # stage 5: every direction of a basis cluster adds to the cluster's values
+        for i in range(d["nbnodes_test"]):
+            node = d["bt_nodes"][i] - 1
+            rows = self._values(0, node)
+            m = len(rows)
+            for l in range(d["bt_dirptr"][i] - 1, d["bt_dirptr"][i + 1] - 1):
+                e = d["bt_diridx"][l] - 1
+                r = tc[e + 1] - tc[e]
+                U = d["bt_blocks"][d["bt_off"][l]:d["bt_off"][l + 1]].reshape((m, r), order="F")
+                Y[rows] += U @ yh[tc[e]:tc[e + 1]]
+        if near:
+            np_, nn_ = d["nearptr"] - 1, d["nearnodes"] - 1
+            for t in range(d["nn_test"]):
+                if np_[t + 1] == np_[t]:
+                    continue
+                rows = self._values(0, t)
+                cols = np.concatenate([self._values(1, s) for s in nn_[np_[t]:np_[t + 1]]])
+                Y[rows] += _near_block(self.k, self.sign, self.tp[rows], self.sp[cols]) @ X[cols]
+        return Y[:, 0] if vec else Y
+
+    def dense(self, near=True):
+        return self.matvec(np.eye(self.d["N"], dtype=np.complex128), near=near)

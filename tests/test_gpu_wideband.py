@@ -1,0 +1,147 @@
+"""Directional (wideband) H^2 on the GPU, through the C-ABI (nca_dh2_create).
+
+Parity bar as for the plain operator: ranks and pivots of every (cluster,
+direction) block bit for bit against the oracle's level driver on the same
+blocks and Phase-1 pivots; the directional product within 1e-12 relative of
+the oracle's stage-by-stage restatement (oracle.OracleDH2) with the same
+factors, for op N, T and C; the error against dense direct summation below
+10 tol (test/nca/test_petrovgalerkinnca.jl:104-153, wideband with
+etalf = etahf = gamma = 1); MIXED within 1e-5 of FP64."""
+import math
+
+import numpy as np
+import pytest
+
+import nca_amd
+from nca_amd import _abi
+from nca_amd import compressors as C
+from nca_amd import directional as D
+from nca_amd import geometry
+from nca_amd import trees as T
+from nca_amd import wideband as W
+from oracle import oracle as O
+from helpers import oracle_factors, relerr
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # tree kind, points, wavelength: together they hold low- and high-frequency
+    ("kmeans", 3000, 1.5),  # clusters, bases above the leaves and child directions with several parents
+    ("twon", 3000, 1.5),
+    ("twon", 4000, 1.0),
+]
+
+
+def say(msg):
+    print(f"[gpu-wideband] {msg}", flush=True)
+
+
+_cores = {}
+
+
+def build(case, tol=1e-4):
+    key = (case, tol)
+    if key not in _cores:
+        kind, n, lam = case
+        pts = geometry.fibonacci_sphere(n)
+        tr = nca_amd.TwoNTree(pts, minvalues=16) if kind == "twon" else nca_amd.KMeansTree(pts, 2, 16)
+        k = 2 * math.pi / lam
+        isn = D.isnearwideband(k, etalf=1.0, etahf=1.0, gamma=1.0)
+        comp = W.BottomUp(C.ACA(tol))
+        core = W.DirectionalNCACore(nca_amd.HelmholtzKernel(k), pts, pts, T.BlockTree(tr, tr), comp, comp, isn,
+                                    maxrank=50)
+        _cores[key] = (core, pts, k)
+    return _cores[key]
+
+
+def oracle_of(core, pts, k):
+    off, blocks = core.couplingstore.download()
+    return O.OracleDH2(core.desc, k, 1, pts, pts, off, blocks)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}{c[1]}" for c in CASES])
+def test_directional_pivots_bitwise(case):
+    say(f"pivots {case}")
+    core, pts, k = build(case)
+    for side, blocks, f in ((0, core.testblocks, core.testfactors), (1, core.trialblocks, core.trialfactors)):
+        fp = f.farpivots
+        res = O.bases(side, k, 1, pts, pts, blocks, fp[0], fp[1], 50, 1e-4)
+        assert res[4] == 0
+        ref = oracle_factors(res, blocks, fp)
+        assert np.array_equal(ref.assigned, f.assigned)
+        assert np.array_equal(ref.rank, f.rank)
+        for a, b in zip(ref.pivot_csr(), f.pivot_csr()):
+            assert np.array_equal(a, b)
+        for e in np.nonzero(f.assigned)[0][::7]:
+            assert np.allclose(ref.X[e], f.X[e], rtol=1e-11, atol=1e-13)
+    ib = core.testblocks.isbasis & ~core.tree.testtree.leafmask()[core.testblocks.node] & core.testfactors.assigned
+    say(f"  direction indices {core.testblocks.nnodes}, bases above the leaves {int(ib.sum())}")
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}{c[1]}" for c in CASES])
+def test_directional_matvec_parity(case):
+    say(f"matvec {case}")
+    core, pts, k = build(case)
+    ref = oracle_of(core, pts, k)
+    n = len(pts)
+    rng = np.random.default_rng(11)
+    A = nca_amd.H2Workspace(core)
+    for R in (1, 16, 19):
+        X = rng.standard_normal((n, R)) + 1j * rng.standard_normal((n, R))
+        Y = A @ X
+        assert relerr(Y, ref.matvec(X)) < 1e-12, R
+    dense = ref.dense()
+    x = rng.standard_normal((n, 16)) + 1j * rng.standard_normal((n, 16))
+    assert relerr(NCmul(nca_amd.nca.transpose(A), x), dense.T @ x) < 1e-12
+    assert relerr(NCmul(nca_amd.nca.adjoint(A), x), dense.conj().T @ x) < 1e-12
+    y0 = rng.standard_normal((n, 16)) + 1j * rng.standard_normal((n, 16))
+    y = y0.copy()
+    nca_amd.nca.mul(y, A, x, 0.5 - 0.25j, 2.0)
+    assert relerr(y, (0.5 - 0.25j) * (dense @ x) + 2.0 * y0) < 1e-12
+
+
+def NCmul(A, x):
+    return A @ x
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}{c[1]}" for c in CASES])
+def test_directional_direct_sum_and_mixed(case):
+    say(f"direct sum + mixed {case}")
+    core, pts, k = build(case)
+    n = len(pts)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((n, 16)) + 1j * rng.standard_normal((n, 16))
+    A = nca_amd.H2Workspace(core)
+    y64 = A @ x
+    rows = np.sort(rng.choice(n, 400, replace=False))
+    yd = O.dense_rows(k, 1, pts, pts, rows, x)
+    err = relerr(y64[rows], yd)
+    say(f"  direct-sum error {err:.2e}")
+    assert err < 10 * 1e-4
+    core.set_precision(_abi.NCA_PREC_MIXED)
+    try:
+        ym = A @ x
+    finally:
+        core.set_precision(_abi.NCA_PREC_FP64)
+    assert relerr(ym, y64) < 1e-5
+
+
+def test_directional_graph_replays():
+    """A captured directional sweep replays bit for bit like the eager one."""
+    import torch
+    core, pts, k = build(CASES[0])
+    n = len(pts)
+    rng = np.random.default_rng(3)
+    x = torch.tensor(rng.standard_normal((16, n)) + 1j * rng.standard_normal((16, n)), dtype=torch.complex128,
+                     device="cuda")
+    y = torch.zeros((16, n), dtype=torch.complex128, device="cuda")
+    core.matvec_device(_abi.NCA_OP_N, 16, x.data_ptr(), n, y.data_ptr(), n)
+    torch.cuda.synchronize()
+    want = y.clone()
+    g = core.graph(_abi.NCA_OP_N, 16, x.data_ptr(), n, y.data_ptr(), n)
+    for _ in range(3):
+        y.zero_()
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(y, want)
+    del g
