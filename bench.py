@@ -91,9 +91,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1048576)
-    ap.add_argument("--nrhs", type=int, default=16)
-    ap.add_argument("--k", type=float, default=2.0)
+    ap.add_argument("--n", type=int, default=None, help="points (sphere1m 1048576, wideband262k 262144)")
+    ap.add_argument("--nrhs", type=int, default=None, help="right-hand sides (sphere1m 16, wideband262k 8)")
+    ap.add_argument("--k", type=float, default=None, help="wavenumber (sphere1m 2, wideband262k 16 pi)")
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--minvalues", type=int, default=64)
     ap.add_argument("--maxrank", type=int, default=50)
@@ -102,14 +102,21 @@ def main():
     ap.add_argument("--profile-only", action="store_true", help="assembly + timed sweeps only")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="copy this many MiB device to device first (a kernel of known traffic for counter scaling)")
-    ap.add_argument("--workload", choices=["sphere1m", "batch64"], default="sphere1m")
+    ap.add_argument("--workload", choices=["sphere1m", "batch64", "wideband262k"], default="sphere1m")
     ap.add_argument("--autotune", type=int, default=1, help="run nca_h2_autotune per precision before timing")
     args = ap.parse_args()
-    if args.workload == "batch64":
+    wide = args.workload == "wideband262k"
+    if args.n is None:
+        args.n = 262144 if wide else 1048576
+    if args.nrhs is None:
+        args.nrhs = 8 if wide else 16
+    if args.k is None:
+        args.k = 16 * np.pi if wide else 2.0
+    if args.workload in ("batch64", "wideband262k"):
         if args.gpus != 1:
             print("bench.py runs on one GPU only", file=sys.stderr)
             sys.exit(2)
-        return bench_batch(args)
+        return bench_batch(args) if args.workload == "batch64" else bench_wideband(args)
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); multi-GPU sharding is a design "
@@ -388,6 +395,98 @@ def bench_batch(args):
         "operator": {k_: int(v) for k_, v in B.core.stats().items()},
     }
     log(f"mixed vs fp64 {err_mixed:.3e}; direct-sum errors {errs}; total {time.perf_counter() - t_all:.1f}s")
+    print(json.dumps(result), flush=True)
+
+
+def bench_wideband(args):
+    """Directional (wideband) H^2: N points of the unit sphere, k = 16 pi
+    (16 wavelengths across), 2-means tree with minvalues 64, isnearwideband(k)
+    (etalf 1, etahf 5, gamma 1), BottomUp per (cluster, direction) at tol,
+    nrhs right-hand sides; FP64 and MIXED sweeps."""
+    from nca_amd import directional as D
+    from nca_amd import wideband as W
+    from oracle import oracle as O
+    t_all = time.perf_counter()
+    R, N = args.nrhs, args.n
+    pts = geometry.fibonacci_sphere(N)
+    t0 = time.perf_counter()
+    tr = nca_amd.KMeansTree(pts, 2, args.minvalues)
+    t_tree = time.perf_counter() - t0
+    isn = D.isnearwideband(args.k)
+    ctx = nca_amd.Context(0)
+    comp = W.BottomUp(C.ACA(args.tol))
+    core = W.DirectionalNCACore(nca_amd.HelmholtzKernel(args.k), pts, pts, nca_amd.BlockTree(tr, tr), comp, comp,
+                                isn, maxrank=args.maxrank, ctx=ctx)
+    tm = core.timings
+    st = core.stats()
+    ne = core.testdata.ndiridx
+    log(f"wideband N={N} k={args.k:.4f}: tree {tr.nnodes} nodes ({t_tree:.2f}s); {ne} direction indices; partition + "
+        f"directions {tm['partition']:.2f}s, phase1 {tm['phase1']:.2f}s, GPU assembly {tm['assembly']:.2f}s, "
+        f"h2 create {tm['h2_create']:.2f}s; stats {st}")
+    hip = Hip()
+    rng = np.random.default_rng(8)
+    X = np.asfortranarray(geometry.complex_gaussian(rng, (N, R)))
+    dx, dy = hip.malloc(X.nbytes), hip.malloc(X.nbytes)
+    hip.h2d(dx, X)
+    hip.sync()
+    names = ["gather", "leaf_projection", "upward", "coupling", "downward", "leaf_expansion_near"]
+    out, tuning = {}, {}
+    for label, prec in (("fp64", nca_amd.NCA_PREC_FP64), ("mixed", nca_amd.NCA_PREC_MIXED)):
+        core.set_precision(prec)
+        if args.autotune:
+            tuning[label] = core.autotune(reps=3)
+        for _ in range(args.warmup):
+            core.matvec_device(0, R, dx, N, dy, N)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            core.matvec_device(0, R, dx, N, dy, N)
+        ctx.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / args.steps
+        prof = np.zeros(6)
+        for _ in range(3):
+            sm = np.zeros(6)
+            al, be = np.array([1.0, 0.0]), np.array([0.0, 0.0])
+            nca_amd.runtime.call(nca_amd.hip_lib().nca_h2_matvec_profile(
+                core.handle, 0, R, ctypes.c_void_p(dx), N, ctypes.c_void_p(dy), N,
+                al.ctypes.data_as(ctypes.c_void_p), be.ctypes.data_as(ctypes.c_void_p),
+                sm.ctypes.data_as(ctypes.c_void_p)))
+            prof += sm / 3
+        Y = np.zeros((N, R), np.complex128, order="F")
+        hip.d2h(Y, dy)
+        sb = stage_bytes(st, R, N, N, 16 if label == "fp64" else 8)
+        out[label] = {"ms_per_sweep": round(ms, 4), "matvec_per_s": round(R / (ms * 1e-3), 2),
+                      "sweep_alg_bytes": int(sum(sb.values())),
+                      "sweep_hbm_gbs": round(sum(sb.values()) / (ms * 1e-3) / 1e9, 1),
+                      "stages_ms": {n: round(float(prof[i]), 4) for i, n in enumerate(names)}, "Y": Y}
+        log(f"{label}: {ms:.3f} ms per sweep of {R} RHS; stages {out[label]['stages_ms']}")
+    err_mixed = float(np.linalg.norm(out["mixed"]["Y"] - out["fp64"]["Y"]) / np.linalg.norm(out["fp64"]["Y"]))
+    rows = np.sort(rng.choice(N, args.check_rows, replace=False))
+    ref = O.dense_rows(args.k, 1, pts, pts, rows, X[:, :1])[:, 0]
+    err = {lab: float(np.linalg.norm(out[lab]["Y"][rows, 0] - ref) / np.linalg.norm(ref)) for lab in out}
+    hip.free(dx)
+    hip.free(dy)
+    ms = out["mixed"]["ms_per_sweep"]
+    result = {
+        "metric": "directional H2 matvec/s (N=262k sphere, k=16 pi, 8 RHS, mixed vs FP64 at 1e-4); assembly time",
+        "value": out["mixed"]["matvec_per_s"], "unit": "matvec/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c32 operators on MFMA, fp64 phase and coefficient accumulation (MIXED); fp64 in precisions",
+        "data": "synthetic (Fibonacci sphere points, seeded complex Gaussian right-hand sides)",
+        "config": {"workload": f"helmholtz_wideband_sphere_N{N}_k{args.k:.4g}_tol{args.tol:g}_rhs{R}",
+                   "n_points": N, "k": args.k, "tol": args.tol, "minvalues": args.minvalues,
+                   "maxrank": args.maxrank, "nrhs": R, "isnear": "isnearwideband(k; etalf=1, etahf=5, gamma=1)",
+                   "direction_indices": int(ne)},
+        "precisions": {lab: {key: v for key, v in d.items() if key != "Y"} for lab, d in out.items()},
+        "mixed_vs_fp64_relerr": err_mixed,
+        "error_vs_direct_sum": {"rows": int(args.check_rows), "rhs": 1, "relerr": err},
+        "assembly_s": round(tm["assembly"], 3),
+        "host_prep_s": {"tree": round(t_tree, 3), "partition_and_directions": round(tm["partition"], 3),
+                        "phase1": round(tm["phase1"], 3), "h2_create": round(tm["h2_create"], 3)},
+        "operator": {k_: int(v) for k_, v in st.items()},
+        "autotune": tuning,
+    }
+    log(f"mixed vs fp64 {err_mixed:.3e}; direct-sum errors {err}; total {time.perf_counter() - t_all:.1f}s")
     print(json.dumps(result), flush=True)
 
 

@@ -101,6 +101,8 @@ struct nca_h2 {
     nca::DevBuf<int32_t> r_test;
     // workspace
     int64_t ws_rhs = 0;
+    // pad a last chunk of 4..15 right-hand sides to 16 (MFMA kernels)
+    int pad_rhs = 1;
     nca::DevBuf<double2> ws_xs, ws_xh, ws_yh, ws_xin, ws_yio;
     nca::DevBuf<float2> ws_xs32, ws_xh32;
     // c32 copy of the coupling blocks for the MIXED path
@@ -178,7 +180,8 @@ __global__ void k_chunk_to_c32(const double2* __restrict__ in, int64_t R, int64_
 template <class XT>
 __global__ void __launch_bounds__(256)
     k_gather_tiled(const double2* __restrict__ x, int64_t ldx, const int32_t* __restrict__ iperm, int64_t npts,
-                   int64_t R, XT* __restrict__ xs) {
+                   int64_t R, int64_t Rw, XT* __restrict__ xs) {
+    // R live columns, Rw >= R columns written (the padding ones as zeros)
     __shared__ double2 tile[16][65];
     const int t = threadIdx.x;
     const int64_t p0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 16;
@@ -186,14 +189,14 @@ __global__ void __launch_bounds__(256)
     for (int r = 0; r < 4; ++r) {
         const int ql = (t >> 6) + 4 * r;
         const int64_t p = p0 + (t & 63), q = c0 + ql;
-        if (p < npts && q < R) tile[ql][t & 63] = x[p + q * ldx];
+        tile[ql][t & 63] = (p < npts && q < R) ? x[p + q * ldx] : make_double2(0.0, 0.0);
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int pl = (t >> 4) + 16 * r;
         const int64_t p = p0 + pl, q = c0 + (t & 15);
-        if (p < npts && q < R) store_as(xs + (int64_t)iperm[p] * R + q, tile[t & 15][pl]);
+        if (p < npts && q < Rw) store_as(xs + (int64_t)iperm[p] * Rw + q, tile[t & 15][pl]);
     }
 }
 
@@ -736,7 +739,7 @@ __global__ void __launch_bounds__(256, 4)
                     const double* __restrict__ spts, const double2* __restrict__ yh,
                     const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, NearK nk,
                     const double2* __restrict__ sctab, double2 alpha, double2 beta, double2* __restrict__ y,
-                    int64_t ldy, int order, int stagger) {
+                    int64_t ldy, int order, int stagger, int64_t rlive) {
     using T2 = typename P::T2;
     __shared__ double2 tab[P::kTab ? kSinCosTab : 1];
     if (P::kTab) {
@@ -781,7 +784,7 @@ __global__ void __launch_bounds__(256, 4)
         }
     };
     auto emit = [&](int row, double2 v) {
-        if (row < nrows) {
+        if (row < nrows && q0 + j < rlive) {
             double2* out = y + (int64_t)dperm[l0 + row0 + row] + (q0 + j) * ldy;
             double2 val = cmul(alpha, v);
             if (beta.x != 0.0 || beta.y != 0.0) val = cadd(val, cmul(beta, *out));
@@ -1077,9 +1080,11 @@ struct Ops<true> {
     static T2* xs(nca_h2* h) { return h->ws_xs32.p; }
 };
 
+// R: columns of the workspaces; rlive <= R: columns of x and y (the others
+// are zero padding of the last 16-column chunk)
 template <int NR, int OP, bool MIX>
-void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double2 alpha, double2 beta,
-               StageEvents* evs) {
+void run_chunk(nca_h2* h, int64_t R, int64_t rlive, int64_t q0, double2* y, int64_t ldy, double2 alpha,
+               double2 beta, StageEvents* evs) {
     constexpr bool FWD = OP == NCA_OP_N;
     constexpr bool CJ = OP == NCA_OP_C;
     using O = Ops<MIX>;
@@ -1189,7 +1194,7 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
                            dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(), dst.pts.p, dst.nrptr.p, dst.nrlo.p,
                            dst.nrhi.p,
                            src.pts.p, yh, xs, R, q0, nk, h->sctab.p, alpha, beta, y, ldy, h->near_xcd,
-                           h->near_stagger);
+                           h->near_stagger, rlive);
     } else if (NR != 16 && dst.nleaves > 0) {
         hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ, T2, T2>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st,
                            dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p,
@@ -1199,9 +1204,17 @@ void run_chunk(nca_h2* h, int64_t R, int64_t q0, double2* y, int64_t ldy, double
     if (dst.nibasis > 0)
         hipLaunchKernelGGL((k_basis_out<FWD, CJ, T2>), dim3((unsigned)dst.nibasis), dim3(256), 0, st, dst.ibslot.p,
                            dst.bnodes.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(),
-                           yh, R, q0, (int64_t)NR, alpha, y, ldy);
+                           yh, R, q0, std::min<int64_t>(NR, rlive - q0), alpha, y, ldy);
     if (evs) evs->mark(6, st);
     NCA_HIP(hipGetLastError());
+}
+
+// Workspace columns for R right-hand sides: a last chunk of 4 to 15 columns
+// is padded to 16 (zero columns) and runs the MFMA kernels, unless pad_rhs is
+// off; smaller remainders run the VALU kernels.
+inline int64_t work_rhs(const nca_h2* h, int64_t R) {
+    const int64_t rem = R % 16;
+    return (h->pad_rhs && rem >= 4) ? R - rem + 16 : R;
 }
 
 template <int OP, bool MIX>
@@ -1212,22 +1225,23 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     SideDev& src = FWD ? h->trial : h->test;
     SideDev& dst = FWD ? h->test : h->trial;
     if (evs) evs->mark(0, st);
+    const int64_t Rw = work_rhs(h, R);
     if (src.npts * R > 0)
         hipLaunchKernelGGL((k_gather_tiled<typename Ops<MIX>::T2>),
-                           dim3(blocks_for(src.npts, 64), blocks_for(R, 16)), dim3(256), 0, st, x, ldx, src.iperm.p,
-                           src.npts, R, Ops<MIX>::xs(h));
-    if (src.K * R > 0)
-        hipLaunchKernelGGL(k_fill, dim3(blocks_for(src.K * R, 256)), dim3(256), 0, st, h->ws_xh.p, src.K * R);
-    if (dst.K * R > 0)
-        hipLaunchKernelGGL(k_fill, dim3(blocks_for(dst.K * R, 256)), dim3(256), 0, st, h->ws_yh.p, dst.K * R);
+                           dim3(blocks_for(src.npts, 64), blocks_for(Rw, 16)), dim3(256), 0, st, x, ldx, src.iperm.p,
+                           src.npts, R, Rw, Ops<MIX>::xs(h));
+    if (src.K * Rw > 0)
+        hipLaunchKernelGGL(k_fill, dim3(blocks_for(src.K * Rw, 256)), dim3(256), 0, st, h->ws_xh.p, src.K * Rw);
+    if (dst.K * Rw > 0)
+        hipLaunchKernelGGL(k_fill, dim3(blocks_for(dst.K * Rw, 256)), dim3(256), 0, st, h->ws_yh.p, dst.K * Rw);
     int64_t q0 = 0;
-    while (q0 < R) {
-        const int64_t left = R - q0;
-        if (left >= 16) { run_chunk<16, OP, MIX>(h, R, q0, y, ldy, alpha, beta, evs); q0 += 16; }
-        else if (left >= 8) { run_chunk<8, OP, MIX>(h, R, q0, y, ldy, alpha, beta, evs); q0 += 8; }
-        else if (left >= 4) { run_chunk<4, OP, MIX>(h, R, q0, y, ldy, alpha, beta, evs); q0 += 4; }
-        else if (left >= 2) { run_chunk<2, OP, MIX>(h, R, q0, y, ldy, alpha, beta, evs); q0 += 2; }
-        else { run_chunk<1, OP, MIX>(h, R, q0, y, ldy, alpha, beta, evs); q0 += 1; }
+    while (q0 < Rw) {
+        const int64_t left = Rw - q0;
+        if (left >= 16) { run_chunk<16, OP, MIX>(h, Rw, R, q0, y, ldy, alpha, beta, evs); q0 += 16; }
+        else if (left >= 8) { run_chunk<8, OP, MIX>(h, Rw, R, q0, y, ldy, alpha, beta, evs); q0 += 8; }
+        else if (left >= 4) { run_chunk<4, OP, MIX>(h, Rw, R, q0, y, ldy, alpha, beta, evs); q0 += 4; }
+        else if (left >= 2) { run_chunk<2, OP, MIX>(h, Rw, R, q0, y, ldy, alpha, beta, evs); q0 += 2; }
+        else { run_chunk<1, OP, MIX>(h, Rw, R, q0, y, ldy, alpha, beta, evs); q0 += 1; }
     }
 }
 
@@ -1273,12 +1287,12 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
     NCA_REQUIRE(R >= 0, NCA_ERR_ARGUMENT, "nca_h2_matvec: negative nrhs");
     const int64_t nx = op == NCA_OP_N ? h->N : h->M, ny = op == NCA_OP_N ? h->M : h->N;
     NCA_REQUIRE(ldx >= nx && ldy >= ny, NCA_ERR_DIMENSION, "DimensionMismatch: leading dimension too small");
-    NCA_REQUIRE(std::max({h->M, h->N, h->test.K, h->trial.K}) * R < ((int64_t)1 << 31), NCA_ERR_LIMIT,
+    NCA_REQUIRE(std::max({h->M, h->N, h->test.K, h->trial.K}) * work_rhs(h, R) < ((int64_t)1 << 31), NCA_ERR_LIMIT,
                 "nca_h2_matvec: points (or coefficients) x nrhs must stay below 2^31 (split the right-hand sides)");
     if (R == 0 || ny == 0) return;
     const bool mix = h->precision == NCA_PREC_MIXED;
     if (mix) ensure_mixed(h);
-    ensure_workspace(h, R);
+    ensure_workspace(h, work_rhs(h, R));
     const double2 al = make_double2(alpha ? alpha[0] : 1.0, alpha ? alpha[1] : 0.0);
     const double2 be = make_double2(beta ? beta[0] : 0.0, beta ? beta[1] : 0.0);
     if (mix) {
@@ -1728,6 +1742,7 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
         if (const char* v = getenv("NCA_LEAF_ORDER")) h->leaf_order = atoi(v);
         if (const char* v = getenv("NCA_COUPLE_ORDER")) h->couple_order = atoi(v);
         if (const char* v = getenv("NCA_NEAR_STAGGER")) h->near_stagger = atoi(v) != 0;
+        if (const char* v = getenv("NCA_PAD_RHS")) h->pad_rhs = atoi(v) != 0;
         timer.mark("sides");
         build_side(h->test, st, d->M, d->nn_test, lay.ne_test, lay.test_dirptr, directional, d->test_perm,
                    d->test_lo, d->test_hi, d->test_cptr, test, d->nbasis_test, d->basis_test_nodes,
@@ -2124,7 +2139,7 @@ int32_t nca_h2_graph_create(nca_h2* h2, int32_t op, int64_t nrhs, const void* x,
         hipStream_t st = h2->ctx->stream;
         // everything that allocates or converts runs before the capture
         if (h2->precision == NCA_PREC_MIXED) ensure_mixed(h2);
-        ensure_workspace(h2, nrhs);
+        ensure_workspace(h2, work_rhs(h2, nrhs));
         NCA_HIP(hipStreamSynchronize(st));
         auto g = std::make_unique<nca_h2_graph>();
         g->h2 = h2;
