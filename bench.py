@@ -420,6 +420,9 @@ def bench_wideband(args):
     tm = core.timings
     st = core.stats()
     ne = core.testdata.ndiridx
+    rt, rs = np.diff(core.testpivots[0]), np.diff(core.trialpivots[0])
+    _, tid, sid = core.couplingplan
+    st["coupling_columns"] = int(np.sum(rs[sid][rt[tid] > 0]))
     log(f"wideband N={N} k={args.k:.4f}: tree {tr.nnodes} nodes ({t_tree:.2f}s); {ne} direction indices; partition + "
         f"directions {tm['partition']:.2f}s, phase1 {tm['phase1']:.2f}s, GPU assembly {tm['assembly']:.2f}s, "
         f"h2 create {tm['h2_create']:.2f}s; stats {st}")

@@ -101,8 +101,11 @@ struct nca_h2 {
     nca::DevBuf<int32_t> r_test;
     // workspace
     int64_t ws_rhs = 0;
-    // pad a last chunk of 4..15 right-hand sides to 16 (MFMA kernels)
-    int pad_rhs = 1;
+    // run a last chunk of 1..3 right-hand sides on the MFMA kernels too
+    int mfma_small = 0;
+    // MIXED coupling reads a c32 copy of the source coefficients (worth it
+    // when the coupling gathers every coefficient several times)
+    int couple_x32 = 1;
     nca::DevBuf<double2> ws_xs, ws_xh, ws_yh, ws_xin, ws_yio;
     nca::DevBuf<float2> ws_xs32, ws_xh32;
     // c32 copy of the coupling blocks for the MIXED path
@@ -166,11 +169,12 @@ __global__ void k_to_c32(const double2* __restrict__ in, float2* __restrict__ ou
 
 // out[k*16 + q] = c32(in[k*R + q0 + q]), q < 16: one 16-column chunk of a
 // coefficient vector, compacted.
-__global__ void k_chunk_to_c32(const double2* __restrict__ in, int64_t R, int64_t q0, int64_t K,
+__global__ void k_chunk_to_c32(const double2* __restrict__ in, int64_t R, int64_t q0, int64_t K, int nq,
                                float2* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= K * 16) return;
-    const double2 v = in[(i >> 4) * R + q0 + (i & 15)];
+    const int q = (int)(i & 15);
+    const double2 v = q < nq ? in[(i >> 4) * R + q0 + q] : make_double2(0.0, 0.0);
     out[i] = make_float2((float)v.x, (float)v.y);
 }
 
@@ -309,13 +313,14 @@ __global__ void __launch_bounds__(kWave)
     }
 }
 
-// Stage 3 reverse (T/C): one wave per trial cluster s: yh_s += sum_t S_ts^T xh_t.
+// Stage 3 reverse (T/C): one wave per trial cluster s: yh_s += sum_t S_ts^T xh_t,
+// for the nq <= NR live columns q0 .. q0 + nq - 1.
 template <int NR, bool CJ, class ST>
 __global__ void __launch_bounds__(kWave)
     k_couple_rev(const ST* __restrict__ S, const int64_t* __restrict__ rptr, const int32_t* __restrict__ rtest,
                  const int64_t* __restrict__ roff, const int32_t* __restrict__ cptr_src,
                  const int32_t* __restrict__ cptr_dst, const double2* __restrict__ xh, int64_t R, int64_t q0,
-                 double2* __restrict__ yh) {
+                 int nq, double2* __restrict__ yh) {
     const int s = blockIdx.x;
     const int rs = cptr_dst[s + 1] - cptr_dst[s];
     if (rs == 0) return;
@@ -331,12 +336,14 @@ __global__ void __launch_bounds__(kWave)
             for (int j = 0; j < rt; ++j) {
                 const double2 sv = ldc<CJ>(Sb, j);
 #pragma unroll
-                for (int q = 0; q < NR; ++q) acc[q] = cfma(sv, xv0[(int64_t)j * R + q], acc[q]);
+                for (int q = 0; q < NR; ++q)
+                    if (q < nq) acc[q] = cfma(sv, xv0[(int64_t)j * R + q], acc[q]);
             }
         }
         double2* out = yh + (int64_t)(cptr_dst[s] + a) * R + q0;
 #pragma unroll
-        for (int q = 0; q < NR; ++q) out[q] = cadd(out[q], acc[q]);
+        for (int q = 0; q < NR; ++q)
+            if (q < nq) out[q] = cadd(out[q], acc[q]);
     }
 }
 
@@ -739,7 +746,7 @@ __global__ void __launch_bounds__(256, 4)
                     const double* __restrict__ spts, const double2* __restrict__ yh,
                     const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, NearK nk,
                     const double2* __restrict__ sctab, double2 alpha, double2 beta, double2* __restrict__ y,
-                    int64_t ldy, int order, int stagger, int64_t rlive) {
+                    int64_t ldy, int order, int stagger, int nq) {
     using T2 = typename P::T2;
     __shared__ double2 tab[P::kTab ? kSinCosTab : 1];
     if (P::kTab) {
@@ -757,8 +764,9 @@ __global__ void __launch_bounds__(256, 4)
     const int i = lane & 15, kk = lane >> 4, j = lane & 15;
     const int64_t e0 = nrptr[t], e1 = nrptr[t + 1];
     const int rot = stagger ? w * 7 : 0;
-    const T2* xsl = xs + q0 + j;
-    const double2* yt = yh + (int64_t)dcptr[t] * R + q0 + j;
+    const int jl = min(j, nq - 1);  // columns past the chunk's live ones read its last (not stored)
+    const T2* xsl = xs + q0 + jl;
+    const double2* yt = yh + (int64_t)dcptr[t] * R + q0 + jl;
     const int rt = dcptr[t + 1] - dcptr[t];
     const T2* B = bi >= 0 ? bblocks + boff[bi] : nullptr;
     // near field first (4 pi G), scaled by 1 / (4 pi), then the basis rows
@@ -784,7 +792,7 @@ __global__ void __launch_bounds__(256, 4)
         }
     };
     auto emit = [&](int row, double2 v) {
-        if (row < nrows && q0 + j < rlive) {
+        if (row < nrows && j < nq) {
             double2* out = y + (int64_t)dperm[l0 + row0 + row] + (q0 + j) * ldy;
             double2 val = cmul(alpha, v);
             if (beta.x != 0.0 || beta.y != 0.0) val = cadd(val, cmul(beta, *out));
@@ -812,29 +820,30 @@ __global__ void __launch_bounds__(256, 4)
 // steps, two register sets in turn: the operands of group n + 1 and the
 // coefficient indices of group n + 2 are in flight while the MFMAs of group n
 // run. Columns past C repeat column C - 1; their B is zeroed at use.
-template <class P, bool CJ, class ACC>
+template <class P, bool CJ, class ACC, class XT>
 __device__ __forceinline__ void couple_run(ACC& acc, const typename P::T2* __restrict__ St, uint32_t ur,
-                                           const int32_t* __restrict__ cc, int C,
-                                           const typename P::T2* __restrict__ xq, uint32_t xstride, int kk) {
+                                           const int32_t* __restrict__ cc, int C, const XT* __restrict__ xq,
+                                           uint32_t xstride, int kk) {
     using T2 = typename P::T2;
     constexpr int kU = 4;
-    T2 a0[kU], b0[kU], a1[kU], b1[kU];
+    T2 a0[kU], a1[kU];
+    XT b0[kU], b1[kU];
     int ci0[kU], ci1[kU];
     auto load_idx = [&](int g0, int (&ci)[kU]) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) ci[u] = cc[min(g0 + 4 * u + kk, C - 1)];
     };
-    auto load_ops = [&](int g0, const int (&ci)[kU], T2 (&a)[kU], T2 (&b)[kU]) {
+    auto load_ops = [&](int g0, const int (&ci)[kU], T2 (&a)[kU], XT (&b)[kU]) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             a[u] = St[(uint32_t)min(g0 + 4 * u + kk, C - 1) * ur];
             b[u] = xq[(uint32_t)ci[u] * xstride];
         }
     };
-    auto mma = [&](int g0, const T2 (&a)[kU], const T2 (&b)[kU]) {
+    auto mma = [&](int g0, const T2 (&a)[kU], const XT (&b)[kU]) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            T2 bu = b[u];
+            T2 bu = P::cvt(b[u]);
             if (g0 + 4 * u + kk >= C) bu = czero2<P>();
             acc.mma(conjif<CJ, P>(a[u]), bu);
         }
@@ -860,14 +869,15 @@ __device__ __forceinline__ void couple_run(ACC& acc, const typename P::T2* __res
 // coefficients of its columns. Tiles of more than 8 rows use the complex
 // (three- or four-product) accumulator, smaller ones the real embedding.
 // Control flow is uniform over the wave; offsets inside a block and into xh
-// are 32-bit (coefficients x right-hand sides < 2^31).
-template <class P, bool CJ, bool K3>
+// are 32-bit (coefficients x right-hand sides < 2^31). XT: type of the source
+// coefficients (the FP64 vector, or its c32 copy in MIXED).
+template <class P, bool CJ, bool K3, class XT>
 __global__ void __launch_bounds__(256, 4)
     k_couple_fwd_mfma(const typename P::T2* __restrict__ S, const int32_t* __restrict__ tile_node,
                       const int32_t* __restrict__ tile_row0, int64_t ntiles, const int64_t* __restrict__ base,
                       const int64_t* __restrict__ colbase, const int32_t* __restrict__ ncols,
                       const int32_t* __restrict__ colcoef, const int32_t* __restrict__ cptr_dst,
-                      const typename P::T2* __restrict__ xq0, uint32_t xstride, int64_t R, int64_t q0,
+                      const XT* __restrict__ xq0, uint32_t xstride, int64_t R, int64_t q0, int nq,
                       double2* __restrict__ yh, int order) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
@@ -879,26 +889,33 @@ __global__ void __launch_bounds__(256, 4)
     const int nrows = min(16, rt - row0);
     const int i = lane & 15, kk = lane >> 4, j = lane & 15;
     const int32_t* cc = colcoef + colbase[t];
-    const auto* xq = xq0 + j;
+    const auto* xq = xq0 + min(j, nq - 1);
+    // every row of every test block with coefficients has one tile (rows
+    // without far columns store zeros), so yh needs no clearing
     double2* out = yh + (int64_t)(cptr_dst[t] + row0) * R + q0 + j;
     auto emit = [&](int row, double2 v) {
-        if (row < nrows) out[(int64_t)row * R] = cadd(out[(int64_t)row * R], v);
+        if (row < nrows && j < nq) out[(int64_t)row * R] = v;
     };
     if (nrows > 8) {
         Cacc<P, K3> acc;
         acc.zero();
-        couple_run<P, CJ>(acc, S + base[t] + row0 + min(i, nrows - 1), (uint32_t)rt, cc, C, xq, xstride, kk);
+        if (C > 0)
+            couple_run<P, CJ>(acc, S + base[t] + row0 + min(i, nrows - 1), (uint32_t)rt, cc, C, xq, xstride, kk);
 #pragma unroll
         for (int v = 0; v < 4; ++v) emit(P::row(kk, v), acc.get(v));
     } else if (nrows > 4) {
         Cemb<P, 8> acc;
         acc.zero();
-        couple_run<P, CJ>(acc, S + base[t] + row0 + min(i & 7, nrows - 1), (uint32_t)rt, cc, C, xq, xstride, kk);
+        if (C > 0)
+            couple_run<P, CJ>(acc, S + base[t] + row0 + min(i & 7, nrows - 1), (uint32_t)rt, cc, C, xq, xstride,
+                              kk);
         acc.store(emit);
     } else {
         Cemb<P, 4> acc;
         acc.zero();
-        couple_run<P, CJ>(acc, S + base[t] + row0 + min(i & 3, nrows - 1), (uint32_t)rt, cc, C, xq, xstride, kk);
+        if (C > 0)
+            couple_run<P, CJ>(acc, S + base[t] + row0 + min(i & 3, nrows - 1), (uint32_t)rt, cc, C, xq, xstride,
+                              kk);
         acc.store(emit);
     }
 }
@@ -921,27 +938,29 @@ __device__ __forceinline__ void gemm16_tile(Cacc<P, K3>& acc, int lane, int a0, 
     }
 }
 
+// out (+)= the tile: add into out, or store (the caller writes every element once)
 template <class P, bool K3>
 __device__ __forceinline__ void gemm16_store(const Cacc<P, K3>& acc, int lane, int a0, int m, double2* out,
-                                             int64_t R) {
+                                             int64_t R, int nq, bool add) {
     const int kk = lane >> 4, j = lane & 15;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int row = a0 + P::row(kk, v);
-        if (row < m) {
+        if (row < m && j < nq) {
             double2* o = out + (int64_t)row * R + j;
-            *o = cadd(*o, acc.get(v));
+            *o = add ? cadd(*o, acc.get(v)) : acc.get(v);
         }
     }
 }
 
-// Stage 1: xh_s += V_s xs_s (TR: U_s^T). One wave per source basis.
+// Stage 1: xh_s = V_s xs_s (TR: U_s^T). One wave per source basis; it writes
+// every coefficient of its node, so xh needs no clearing.
 template <class P, bool TR, bool CJ, bool K3>
 __global__ void __launch_bounds__(256)
     k_project_mfma(const int32_t* __restrict__ bnodes, const int64_t* __restrict__ boff,
                    const typename P::T2* __restrict__ blocks, int64_t nbasis, const int32_t* __restrict__ lo,
                    const int32_t* __restrict__ hi, const int32_t* __restrict__ cptr,
-                   const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, double2* __restrict__ xh) {
+                   const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, int nq, double2* __restrict__ xh) {
     using T2 = typename P::T2;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
@@ -958,20 +977,21 @@ __global__ void __launch_bounds__(256)
         gemm16_tile<P, K3>(
             acc, lane, a0, r, ns,
             [&](int a, int j) { return conjif<CJ, P>(B[TR ? (int64_t)j + (int64_t)a * ns : (int64_t)a + (int64_t)j * r]); },
-            [&](int j, int q) { return xv[(int64_t)j * R + q]; });
-        gemm16_store<P, K3>(acc, lane, a0, r, xh + (int64_t)cptr[s] * R + q0, R);
+            [&](int j, int q) { return xv[(int64_t)j * R + min(q, nq - 1)]; });
+        gemm16_store<P, K3>(acc, lane, a0, r, xh + (int64_t)cptr[s] * R + q0, R, nq, false);
     }
 }
 
-// Stage 2 (up): xh_p += sum_c E_c xh_c, E_c r_p x r_c (TR: r_c x r_p used
-// transposed); stage 4 (DOWN): yh_c += E_c yh_p, E_c r_c x r_p (TR: r_p x r_c
-// transposed). One wave per parent of the level.
+// Stage 2 (up): xh_p = sum_c E_c xh_c, E_c r_p x r_c (TR: r_c x r_p used
+// transposed; accumulate: += instead, for the child-centric downward pass);
+// stage 4 (DOWN): yh_c += E_c yh_p, E_c r_c x r_p (TR: r_p x r_c transposed).
+// One wave per parent of the level.
 template <class P, bool DOWN, bool TR, bool CJ, bool K3>
 __global__ void __launch_bounds__(256)
     k_transfer_mfma(const int32_t* __restrict__ parents, const int64_t* __restrict__ chdptr,
                     const int32_t* __restrict__ chds, const int64_t* __restrict__ toff,
                     const typename P::T2* __restrict__ blocks, const int32_t* __restrict__ cptr, int64_t pbase,
-                    int64_t np, int64_t R, int64_t q0, double2* __restrict__ h) {
+                    int64_t np, int64_t R, int64_t q0, int nq, int accumulate, double2* __restrict__ h) {
     using T2 = typename P::T2;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
@@ -993,8 +1013,8 @@ __global__ void __launch_bounds__(256)
                 gemm16_tile<P, K3>(
                     acc, lane, a0, rc, rp,
                     [&](int a, int b) { return conjif<CJ, P>(E[TR ? (int64_t)b + (int64_t)a * rp : (int64_t)a + (int64_t)b * rc]); },
-                    [&](int b, int q) { return P::cvt(hp[(int64_t)b * R + q]); });
-                gemm16_store<P, K3>(acc, lane, a0, rc, h + (int64_t)cptr[c] * R + q0, R);
+                    [&](int b, int q) { return P::cvt(hp[(int64_t)b * R + min(q, nq - 1)]); });
+                gemm16_store<P, K3>(acc, lane, a0, rc, h + (int64_t)cptr[c] * R + q0, R, nq, true);
             }
         }
     } else {
@@ -1010,9 +1030,9 @@ __global__ void __launch_bounds__(256)
                 gemm16_tile<P, K3>(
                     acc, lane, a0, rp, rc,
                     [&](int a, int b) { return conjif<CJ, P>(E[TR ? (int64_t)b + (int64_t)a * rc : (int64_t)a + (int64_t)b * rp]); },
-                    [&](int b, int q) { return P::cvt(hc[(int64_t)b * R + q]); });
+                    [&](int b, int q) { return P::cvt(hc[(int64_t)b * R + min(q, nq - 1)]); });
             }
-            gemm16_store<P, K3>(acc, lane, a0, rp, h + (int64_t)cptr[p] * R + q0, R);
+            gemm16_store<P, K3>(acc, lane, a0, rp, h + (int64_t)cptr[p] * R + q0, R, nq, accumulate != 0);
         }
     }
 }
@@ -1080,11 +1100,11 @@ struct Ops<true> {
     static T2* xs(nca_h2* h) { return h->ws_xs32.p; }
 };
 
-// R: columns of the workspaces; rlive <= R: columns of x and y (the others
-// are zero padding of the last 16-column chunk)
+// Columns q0 .. q0 + nq - 1 of R; NR = 16 runs the MFMA kernels for any
+// nq <= 16 (lanes of the columns past nq compute, and are not stored).
 template <int NR, int OP, bool MIX>
-void run_chunk(nca_h2* h, int64_t R, int64_t rlive, int64_t q0, double2* y, int64_t ldy, double2 alpha,
-               double2 beta, StageEvents* evs) {
+void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy, double2 alpha, double2 beta,
+               StageEvents* evs) {
     constexpr bool FWD = OP == NCA_OP_N;
     constexpr bool CJ = OP == NCA_OP_C;
     using O = Ops<MIX>;
@@ -1101,7 +1121,7 @@ void run_chunk(nca_h2* h, int64_t R, int64_t rlive, int64_t q0, double2* y, int6
         if (NR == 16)
             hipLaunchKernelGGL((k_project_mfma<P, !FWD, CJ, true>), dim3(blocks_for(src.nbasis, 4)), dim3(256), 0, st,
                                src.bnodes.p, src.boff.p, O::basis(src), src.nbasis, src.lo.p, src.hi.p,
-                               src.node_cptr(), xs, R, q0, xh);
+                               src.node_cptr(), xs, R, q0, nq, xh);
         else
             hipLaunchKernelGGL((k_project<NR, !FWD, CJ, T2, T2>), dim3((unsigned)src.nbasis), dim3(kWave), 0, st,
                                src.bnodes.p, src.boff.p, O::basis(src), src.lo.p, src.hi.p, src.node_cptr(), xs, R,
@@ -1115,7 +1135,7 @@ void run_chunk(nca_h2* h, int64_t R, int64_t rlive, int64_t q0, double2* y, int6
         if (NR == 16)
             hipLaunchKernelGGL((k_transfer_mfma<P, false, !FWD, CJ, true>), dim3(blocks_for(np, 4)), dim3(256), 0, st,
                                src.parents.p, src.chdptr.p, src.chds.p, src.toff.p, O::transfer(src), src.cptr.p,
-                               src.levptr[l], np, R, q0, xh);
+                               src.levptr[l], np, R, q0, nq, 0, xh);
         else
             hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
                                src.parents.p, src.chdptr.p, src.chds.p, src.toff.p, O::transfer(src), src.cptr.p,
@@ -1126,22 +1146,23 @@ void run_chunk(nca_h2* h, int64_t R, int64_t rlive, int64_t q0, double2* y, int6
     if (FWD) {
         if (NR == 16 && h->f_ntiles > 0) {
             const bool k3 = h->couple_k3[MIX ? 1 : 0] != 0;
-            auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true> : k_couple_fwd_mfma<P, CJ, false>;
             // MIXED: the source coefficients of this chunk as a compact c32 copy
-            const T2* xq0;
-            uint32_t xstride;
-            if constexpr (MIX) {
+            // when the coupling gathers them several times over (couple_x32)
+            if (MIX && h->couple_x32) {
+                auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true, T2> : k_couple_fwd_mfma<P, CJ, false, T2>;
                 hipLaunchKernelGGL(k_chunk_to_c32, dim3(blocks_for(src.K * 16, 256)), dim3(256), 0, st, xh, R, q0,
-                                   src.K, h->ws_xh32.p);
-                xq0 = h->ws_xh32.p;
-                xstride = 16;
+                                   src.K, nq, h->ws_xh32.p);
+                hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
+                                   h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
+                                   h->f_colcoef.p, dst.cptr.p, reinterpret_cast<const T2*>(h->ws_xh32.p), 16u, R,
+                                   q0, nq, yh, h->couple_xcd);
             } else {
-                xq0 = xh + q0;
-                xstride = (uint32_t)R;
+                auto kern =
+                    k3 ? k_couple_fwd_mfma<P, CJ, true, double2> : k_couple_fwd_mfma<P, CJ, false, double2>;
+                hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
+                                   h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
+                                   h->f_colcoef.p, dst.cptr.p, xh + q0, (uint32_t)R, R, q0, nq, yh, h->couple_xcd);
             }
-            hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
-                               h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
-                               h->f_colcoef.p, dst.cptr.p, xq0, xstride, R, q0, yh, h->couple_xcd);
         } else if (NR != 16 && h->coupling->nentries > 0) {
             hipLaunchKernelGGL((k_couple_fwd<NR, CJ, T2>), dim3((unsigned)dst.ne), dim3(kWave), 0, st, S, h->f_base.p,
                                h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
@@ -1149,7 +1170,7 @@ void run_chunk(nca_h2* h, int64_t R, int64_t rlive, int64_t q0, double2* y, int6
     } else {
         if (h->coupling->nentries > 0)
             hipLaunchKernelGGL((k_couple_rev<NR, CJ, T2>), dim3((unsigned)dst.ne), dim3(kWave), 0, st, S, h->r_ptr.p,
-                               h->r_test.p, h->r_off.p, src.cptr.p, dst.cptr.p, xh, R, q0, yh);
+                               h->r_test.p, h->r_off.p, src.cptr.p, dst.cptr.p, xh, R, q0, nq, yh);
     }
     if (evs) evs->mark(4, st);
     // downward: root level first; destination transfers (test E forward, trial E transposed)
@@ -1162,7 +1183,7 @@ void run_chunk(nca_h2* h, int64_t R, int64_t rlive, int64_t q0, double2* y, int6
             if (NR == 16)
                 hipLaunchKernelGGL((k_transfer_mfma<P, false, !FWD, CJ, true>), dim3(blocks_for(nc, 4)), dim3(256), 0,
                                    st, dst.dchild.p, dst.dptr.p, dst.dpar.p, dst.dtoff.p, O::transfer(dst), dst.cptr.p,
-                                   dst.dlevptr[l], nc, R, q0, yh);
+                                   dst.dlevptr[l], nc, R, q0, nq, 1, yh);
             else
                 hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ, T2>), dim3((unsigned)nc), dim3(kWave), 0, st,
                                    dst.dchild.p, dst.dptr.p, dst.dpar.p, dst.dtoff.p, O::transfer(dst), dst.cptr.p,
@@ -1175,7 +1196,7 @@ void run_chunk(nca_h2* h, int64_t R, int64_t rlive, int64_t q0, double2* y, int6
         if (NR == 16)
             hipLaunchKernelGGL((k_transfer_mfma<P, true, !FWD, CJ, true>), dim3(blocks_for(np, 4)), dim3(256), 0, st,
                                dst.parents.p, dst.chdptr.p, dst.chds.p, dst.toff.p, O::transfer(dst), dst.cptr.p,
-                               dst.levptr[l], np, R, q0, yh);
+                               dst.levptr[l], np, R, q0, nq, 1, yh);
         else
             hipLaunchKernelGGL((k_transfer<NR, true, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
                                dst.parents.p, dst.chdptr.p, dst.chds.p, dst.toff.p, O::transfer(dst), dst.cptr.p,
@@ -1194,7 +1215,7 @@ void run_chunk(nca_h2* h, int64_t R, int64_t rlive, int64_t q0, double2* y, int6
                            dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(), dst.pts.p, dst.nrptr.p, dst.nrlo.p,
                            dst.nrhi.p,
                            src.pts.p, yh, xs, R, q0, nk, h->sctab.p, alpha, beta, y, ldy, h->near_xcd,
-                           h->near_stagger, rlive);
+                           h->near_stagger, nq);
     } else if (NR != 16 && dst.nleaves > 0) {
         hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ, T2, T2>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st,
                            dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p,
@@ -1204,18 +1225,11 @@ void run_chunk(nca_h2* h, int64_t R, int64_t rlive, int64_t q0, double2* y, int6
     if (dst.nibasis > 0)
         hipLaunchKernelGGL((k_basis_out<FWD, CJ, T2>), dim3((unsigned)dst.nibasis), dim3(256), 0, st, dst.ibslot.p,
                            dst.bnodes.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(),
-                           yh, R, q0, std::min<int64_t>(NR, rlive - q0), alpha, y, ldy);
+                           yh, R, q0, (int64_t)nq, alpha, y, ldy);
     if (evs) evs->mark(6, st);
     NCA_HIP(hipGetLastError());
 }
 
-// Workspace columns for R right-hand sides: a last chunk of 4 to 15 columns
-// is padded to 16 (zero columns) and runs the MFMA kernels, unless pad_rhs is
-// off; smaller remainders run the VALU kernels.
-inline int64_t work_rhs(const nca_h2* h, int64_t R) {
-    const int64_t rem = R % 16;
-    return (h->pad_rhs && rem >= 4) ? R - rem + 16 : R;
-}
 
 template <int OP, bool MIX>
 void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, int64_t R, double2 alpha,
@@ -1225,23 +1239,29 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     SideDev& src = FWD ? h->trial : h->test;
     SideDev& dst = FWD ? h->test : h->trial;
     if (evs) evs->mark(0, st);
-    const int64_t Rw = work_rhs(h, R);
     if (src.npts * R > 0)
         hipLaunchKernelGGL((k_gather_tiled<typename Ops<MIX>::T2>),
-                           dim3(blocks_for(src.npts, 64), blocks_for(Rw, 16)), dim3(256), 0, st, x, ldx, src.iperm.p,
-                           src.npts, R, Rw, Ops<MIX>::xs(h));
-    if (src.K * Rw > 0)
-        hipLaunchKernelGGL(k_fill, dim3(blocks_for(src.K * Rw, 256)), dim3(256), 0, st, h->ws_xh.p, src.K * Rw);
-    if (dst.K * Rw > 0)
-        hipLaunchKernelGGL(k_fill, dim3(blocks_for(dst.K * Rw, 256)), dim3(256), 0, st, h->ws_yh.p, dst.K * Rw);
+                           dim3(blocks_for(src.npts, 64), blocks_for(R, 16)), dim3(256), 0, st, x, ldx, src.iperm.p,
+                           src.npts, R, R, Ops<MIX>::xs(h));
+    // The MFMA chunks write every coefficient of xh (projection, upward pass)
+    // and, forward, of yh (coupling) exactly once; the VALU chunks and the
+    // reverse coupling add, so they need cleared vectors.
+    const bool valu = !h->mfma_small && (R % 16) > 0 && (R % 16) < 4;
+    if (src.K * R > 0 && valu)
+        hipLaunchKernelGGL(k_fill, dim3(blocks_for(src.K * R, 256)), dim3(256), 0, st, h->ws_xh.p, src.K * R);
+    if (dst.K * R > 0 && (valu || !FWD))
+        hipLaunchKernelGGL(k_fill, dim3(blocks_for(dst.K * R, 256)), dim3(256), 0, st, h->ws_yh.p, dst.K * R);
+    // chunks of up to 16 columns on the MFMA kernels; a last chunk of 1 to 3
+    // columns on the VALU kernels unless mfma_small is set
     int64_t q0 = 0;
-    while (q0 < Rw) {
-        const int64_t left = Rw - q0;
-        if (left >= 16) { run_chunk<16, OP, MIX>(h, Rw, R, q0, y, ldy, alpha, beta, evs); q0 += 16; }
-        else if (left >= 8) { run_chunk<8, OP, MIX>(h, Rw, R, q0, y, ldy, alpha, beta, evs); q0 += 8; }
-        else if (left >= 4) { run_chunk<4, OP, MIX>(h, Rw, R, q0, y, ldy, alpha, beta, evs); q0 += 4; }
-        else if (left >= 2) { run_chunk<2, OP, MIX>(h, Rw, R, q0, y, ldy, alpha, beta, evs); q0 += 2; }
-        else { run_chunk<1, OP, MIX>(h, Rw, R, q0, y, ldy, alpha, beta, evs); q0 += 1; }
+    while (q0 < R) {
+        const int64_t left = R - q0;
+        if (left >= 4 || h->mfma_small) {
+            const int nq = (int)std::min<int64_t>(16, left);
+            run_chunk<16, OP, MIX>(h, R, nq, q0, y, ldy, alpha, beta, evs);
+            q0 += nq;
+        } else if (left >= 2) { run_chunk<2, OP, MIX>(h, R, 2, q0, y, ldy, alpha, beta, evs); q0 += 2; }
+        else { run_chunk<1, OP, MIX>(h, R, 1, q0, y, ldy, alpha, beta, evs); q0 += 1; }
     }
 }
 
@@ -1287,12 +1307,12 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
     NCA_REQUIRE(R >= 0, NCA_ERR_ARGUMENT, "nca_h2_matvec: negative nrhs");
     const int64_t nx = op == NCA_OP_N ? h->N : h->M, ny = op == NCA_OP_N ? h->M : h->N;
     NCA_REQUIRE(ldx >= nx && ldy >= ny, NCA_ERR_DIMENSION, "DimensionMismatch: leading dimension too small");
-    NCA_REQUIRE(std::max({h->M, h->N, h->test.K, h->trial.K}) * work_rhs(h, R) < ((int64_t)1 << 31), NCA_ERR_LIMIT,
+    NCA_REQUIRE(std::max({h->M, h->N, h->test.K, h->trial.K}) * R < ((int64_t)1 << 31), NCA_ERR_LIMIT,
                 "nca_h2_matvec: points (or coefficients) x nrhs must stay below 2^31 (split the right-hand sides)");
     if (R == 0 || ny == 0) return;
     const bool mix = h->precision == NCA_PREC_MIXED;
     if (mix) ensure_mixed(h);
-    ensure_workspace(h, work_rhs(h, R));
+    ensure_workspace(h, R);
     const double2 al = make_double2(alpha ? alpha[0] : 1.0, alpha ? alpha[1] : 0.0);
     const double2 be = make_double2(beta ? beta[0] : 0.0, beta ? beta[1] : 0.0);
     if (mix) {
@@ -1588,7 +1608,8 @@ struct PhaseTimer {
     }
 };
 
-// Launch order of the forward-coupling tiles (16 rows of one test cluster):
+// Launch order of the forward-coupling tiles (16 rows of one test cluster;
+// every cluster with coefficients has tiles, also one without far columns):
 // 0 node id, 1 depth-first point order (nested ranges after their parent),
 // 2 decreasing row work (columns), 3 per level, one contiguous run per XCD.
 void order_coupling_tiles(nca_h2* h, int mode, hipStream_t st) {
@@ -1612,7 +1633,7 @@ void order_coupling_tiles(nca_h2* h, int mode, hipStream_t st) {
     std::vector<int32_t> tnode, trow;
     if (mode != 3) {
         for (const int64_t t : torder)
-            if (ncols[t] > 0)
+            if (h->coupling->testrank[t] > 0)
                 for (int r0 = 0; r0 < h->coupling->testrank[t]; r0 += 16) {
                     tnode.push_back((int32_t)t);
                     trow.push_back(r0);
@@ -1645,7 +1666,7 @@ void order_coupling_tiles(nca_h2* h, int mode, hipStream_t st) {
         for (int d = 0; d <= maxd; ++d) {
             std::vector<int32_t> ln, lr;
             for (int64_t t = 0; t < ntest; ++t)
-                if (depth[t] == d && ncols[t] > 0)
+                if (depth[t] == d && h->coupling->testrank[t] > 0)
                     for (int r0 = 0; r0 < h->coupling->testrank[t]; r0 += 16) {
                         ln.push_back((int32_t)t);
                         lr.push_back(r0);
@@ -1742,7 +1763,8 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
         if (const char* v = getenv("NCA_LEAF_ORDER")) h->leaf_order = atoi(v);
         if (const char* v = getenv("NCA_COUPLE_ORDER")) h->couple_order = atoi(v);
         if (const char* v = getenv("NCA_NEAR_STAGGER")) h->near_stagger = atoi(v) != 0;
-        if (const char* v = getenv("NCA_PAD_RHS")) h->pad_rhs = atoi(v) != 0;
+        if (const char* v = getenv("NCA_MFMA_SMALL")) h->mfma_small = atoi(v) != 0;
+        const char* cx32 = getenv("NCA_COUPLE_X32");
         timer.mark("sides");
         build_side(h->test, st, d->M, d->nn_test, lay.ne_test, lay.test_dirptr, directional, d->test_perm,
                    d->test_lo, d->test_hi, d->test_cptr, test, d->nbasis_test, d->basis_test_nodes,
@@ -1834,6 +1856,9 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
             h->f_ncols.upload(ncols, st);
             h->f_colcoef.upload(colcoef, st);
             h->stats.plan_bytes += (int64_t)colcoef.size() * 4 + ntest * 20;
+            // the c32 copy costs a pass over all source coefficients: keep it
+            // when the columns read each coefficient more than twice on average
+            h->couple_x32 = cx32 ? atoi(cx32) != 0 : (int64_t)colcoef.size() > 2 * h->trial.K;
         }
         timer.mark("forward plan");
         // reverse coupling: blocks by trial block
@@ -2139,7 +2164,7 @@ int32_t nca_h2_graph_create(nca_h2* h2, int32_t op, int64_t nrhs, const void* x,
         hipStream_t st = h2->ctx->stream;
         // everything that allocates or converts runs before the capture
         if (h2->precision == NCA_PREC_MIXED) ensure_mixed(h2);
-        ensure_workspace(h2, work_rhs(h2, nrhs));
+        ensure_workspace(h2, nrhs);
         NCA_HIP(hipStreamSynchronize(st));
         auto g = std::make_unique<nca_h2_graph>();
         g->h2 = h2;

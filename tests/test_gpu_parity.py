@@ -271,7 +271,9 @@ def test_matvec_parity(ctx, case, request):
     H, _ = oracle_operator(p, oft, ofs)
     rng = np.random.default_rng(2)
     N, M = p.trialpts.shape[0], p.testpts.shape[0]
-    for R in (1, 3, 16, 19):
+    # 5, 8 and 24 (= 16 + 8) run a last chunk padded to 16 columns on the MFMA
+    # kernels; 1, 3 and 19 (= 16 + 3) a last chunk on the VALU kernels
+    for R in (1, 3, 5, 8, 16, 19, 24):
         x = geometry.complex_gaussian(rng, (N, R))
         u = geometry.complex_gaussian(rng, (M, R))
         assert relerr(W @ x, H.matvec(x)) <= 1e-12
@@ -327,7 +329,7 @@ def test_matvec_mixed_precision(ctx, case, request):
     N, M = p.trialpts.shape[0], p.testpts.shape[0]
     W.core_set = W.core.set_precision if hasattr(W.core, "set_precision") else None
     NC.PetrovGalerkinNCACore.set_precision(W.core, nca_amd.NCA_PREC_MIXED)
-    for R in (16, 3):
+    for R in (16, 3, 8):
         x = geometry.complex_gaussian(rng, (N, R))
         u = geometry.complex_gaussian(rng, (M, R))
         assert relerr(W @ x, H.matvec(x)) <= 1e-5
