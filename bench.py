@@ -12,6 +12,11 @@ compressor launch per tree level for all problems) and applied to 32 right-hand
 sides per problem in one hipGraph-captured sweep; value = products per second
 over all problems (64 x 32 x steps / time).
 
+--workload wideband262k: directional H^2 on N = 262,144 Fibonacci points of the
+unit sphere, k = 16 pi, 2-means tree with minvalues 64, isnearwideband(k),
+BottomUp per (cluster, direction) at tol 1e-4, 8 seeded complex Gaussian
+right-hand sides; value = products per second of the MIXED sweep.
+
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
