@@ -124,6 +124,26 @@ int32_t nca_compress_level(nca_ctx* ctx, const nca_stack* st, double tol, int32_
                            int64_t* rowpivots, int64_t* colpivots, int64_t xcapacity, int64_t* xoffsets,
                            double* X, int64_t* independent);
 
+/* ---- Phase 1 on the device ----------------------------------------------- */
+/* Far pivots of every item (a cluster, or a (cluster, direction) pair) among
+ * the points of its far field, computed on the device. Replaces the Phase-1
+ * selection _selectfarpivots (src/compressors/batchedbottomup.jl:54-72,
+ * farfield farinteractions.jl:38-60, count rule _clustersize pivots.jl:15-25;
+ * the reference's TreeMimicryPivoting lives in AdaptiveCrossApproximation.jl
+ * and is not vendored): n = min(maxrank, ceil(oversample |t|)) pivots, capped by
+ * the far field's points; candidates an even subsample of every far node, the
+ * first pivot nearest to the item's center, every next one maximizing
+ * d_min^2 / d_c^(2 weight). Equal, for weight 1, to the host selection
+ * ncat_farpivots. centers: 3 x nitems; nvals: |t| per item; ffptr/ffnodes:
+ * 1-based CSR of far nodes of the other tree (other_lo/hi/perm: its 1-based
+ * layout). Outputs pivptr (nitems+1, 0-based) and piv (0-based point ids);
+ * pivcap must hold the total (else NCA_ERR_LIMIT with the total in the message). */
+int32_t nca_farpivots(nca_ctx* ctx, const nca_points* other, int64_t nitems, const double* centers,
+                      const int64_t* nvals, const int64_t* ffptr, const int64_t* ffnodes, int64_t nn_other,
+                      const int64_t* other_lo, const int64_t* other_hi, const int64_t* other_perm,
+                      int64_t maxrank, double weight, double oversample, int64_t* pivptr, int64_t pivcap,
+                      int64_t* piv);
+
 /* ---- seam 3: coupling matrices ----------------------------------------- */
 /* Replaces assemble_couplingstore(farmatrix, testpivots, trialpivots,
  * testfardata, trialfardata) (src/couplingmatrices.jl:11-53): block i of the

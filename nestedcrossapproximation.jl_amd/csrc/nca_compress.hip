@@ -10,6 +10,7 @@
 // irrelevant for the pivots themselves.
 
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 
 #include "nca_common.hpp"
@@ -552,6 +553,185 @@ static void singular_message(const nca_stack* st, const int32_t* ranks, const st
                                               std::to_string(indep[b]) + " of them are linearly independent");
 }
 
+
+// ---------------------------------------------------------------------------
+// Phase 1 on the device (the far-pivot selection of host/nca_tree.cpp,
+// select_farpivots, operation for operation; bit-identical for weight 1).
+// One workgroup per item: candidates = an even subsample of each far node's
+// points (per point count doubled until the pool holds the item's n pivots),
+// the first pivot is the candidate nearest to the item's center, every next
+// one maximizes d_min^2 / d_c^(2 weight); ties go to the first candidate.
+constexpr int kP1Threads = 256;
+constexpr int kP1PerThread = 16;
+constexpr int kP1Pool = kP1Threads * kP1PerThread;
+constexpr int kP1MaxFar = 4096;
+
+__device__ __forceinline__ double p1_dist2(const double* a, double b0, double b1, double b2) {
+    const double dx = a[0] - b0, dy = a[1] - b1, dz = a[2] - b2;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// block-wide reductions over kP1Threads threads
+__device__ __forceinline__ int64_t p1_sum(int64_t v, int64_t* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int tid = threadIdx.x;
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+}
+
+// best (value, index) pair: larger value (or smaller if MIN), then smaller index
+template <bool MIN>
+__device__ __forceinline__ void p1_best(double& v, int& q, double* redv, int* redq) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(v, o);
+        const int oq = __shfl_xor(q, o);
+        const bool take = oq >= 0 && (q < 0 || (MIN ? ov < v : ov > v) || (ov == v && oq < q));
+        if (take) { v = ov; q = oq; }
+    }
+    const int tid = threadIdx.x;
+    __syncthreads();
+    if ((tid & 63) == 0) { redv[tid >> 6] = v; redq[tid >> 6] = q; }
+    __syncthreads();
+    v = redv[0];
+    q = redq[0];
+    for (int w = 1; w < 4; ++w) {
+        const double ov = redv[w];
+        const int oq = redq[w];
+        const bool take = oq >= 0 && (q < 0 || (MIN ? ov < v : ov > v) || (ov == v && oq < q));
+        if (take) { v = ov; q = oq; }
+    }
+}
+
+__global__ void __launch_bounds__(kP1Threads)
+    k_farpivots(const double* __restrict__ pts, const int32_t* __restrict__ olo, const int32_t* __restrict__ ohi,
+                const int32_t* __restrict__ operm, const double* __restrict__ centers,
+                const int64_t* __restrict__ ffptr, const int32_t* __restrict__ ffnodes,
+                const int64_t* __restrict__ pivptr, double weight, int64_t* __restrict__ piv) {
+    __shared__ int32_t scan[kP1MaxFar + 1];
+    __shared__ int64_t red[4];
+    __shared__ double redv[4];
+    __shared__ int redq[4];
+    __shared__ double pc[3];
+    __shared__ int32_t pcur;
+    const int64_t it = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int64_t n = pivptr[it + 1] - pivptr[it];
+    if (n == 0) return;
+    const int32_t* F = ffnodes + ffptr[it];
+    const int nf = (int)(ffptr[it + 1] - ffptr[it]);
+    auto sz = [&](int f) { return (int64_t)(ohi[F[f]] - olo[F[f]]); };
+    int64_t tot = 0;
+    for (int f = tid; f < nf; f += kP1Threads) tot += sz(f);
+    const int64_t total = p1_sum(tot, red);
+    int64_t per = max((int64_t)1, (max(8 * n, (int64_t)512) + nf - 1) / nf);
+    int64_t pool = 0;
+    for (;;) {
+        int64_t c = 0;
+        for (int f = tid; f < nf; f += kP1Threads) c += min(sz(f), per);
+        pool = p1_sum(c, red);
+        if (pool >= n || pool >= total) break;
+        per *= 2;
+    }
+    // exclusive scan of the per-node candidate counts (contiguous chunks per thread)
+    const int chunk = (nf + kP1Threads - 1) / kP1Threads;
+    const int fa = min(nf, tid * chunk), fb = min(nf, fa + chunk);
+    int64_t loc = 0;
+    for (int f = fa; f < fb; ++f) loc += min(sz(f), per);
+    __shared__ int64_t part[kP1Threads];
+    part[tid] = loc;
+    __syncthreads();
+    if (tid == 0) {
+        int64_t run = 0;
+        for (int t = 0; t < kP1Threads; ++t) {
+            const int64_t v = part[t];
+            part[t] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    {
+        int64_t run = part[tid];
+        for (int f = fa; f < fb; ++f) {
+            scan[f] = (int32_t)run;
+            run += min(sz(f), per);
+        }
+        if (fb == nf && fa < fb) scan[nf] = (int32_t)run;
+        if (nf == 0 && tid == 0) scan[0] = 0;
+    }
+    __syncthreads();
+    const double* ct = centers + 3 * it;
+    const double c0 = ct[0], c1 = ct[1], c2 = ct[2];
+    double px[kP1PerThread], py[kP1PerThread], pz[kP1PerThread], d2c[kP1PerThread], mind[kP1PerThread];
+    int32_t pid[kP1PerThread];
+    double bv = 0.0;
+    int bq = -1;
+#pragma unroll
+    for (int m = 0; m < kP1PerThread; ++m) {
+        const int q = tid + kP1Threads * m;
+        pid[m] = -1;
+        mind[m] = -1.0;
+        if (q < pool) {
+            // far node of candidate q: last f with scan[f] <= q
+            int lo = 0, hi = nf;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (scan[mid] <= q) lo = mid; else hi = mid;
+            }
+            const int64_t s = sz(lo), k = min(s, per), j = q - scan[lo];
+            const int32_t p = operm[olo[F[lo]] + (int32_t)((j * s) / k)];
+            pid[m] = p;
+            px[m] = pts[3 * (int64_t)p];
+            py[m] = pts[3 * (int64_t)p + 1];
+            pz[m] = pts[3 * (int64_t)p + 2];
+            const double pp[3] = {px[m], py[m], pz[m]};
+            d2c[m] = max(p1_dist2(pp, c0, c1, c2), 1e-300);
+            mind[m] = __longlong_as_double(0x7ff0000000000000LL);  // +inf
+            if (bq < 0 || d2c[m] < bv) { bv = d2c[m]; bq = q; }
+        }
+    }
+    p1_best<true>(bv, bq, redv, redq);
+    int cur = bq;
+    if (weight != 1.0) {
+#pragma unroll
+        for (int m = 0; m < kP1PerThread; ++m)
+            if (pid[m] >= 0) d2c[m] = pow(d2c[m], weight);
+    }
+    int64_t* out = piv + pivptr[it];
+    for (int64_t sel = 0; sel < n; ++sel) {
+        if (cur % kP1Threads == tid) {
+            const int m = cur / kP1Threads;
+#pragma unroll
+            for (int mm = 0; mm < kP1PerThread; ++mm)
+                if (mm == m) {
+                    mind[mm] = -1.0;
+                    pc[0] = px[mm];
+                    pc[1] = py[mm];
+                    pc[2] = pz[mm];
+                    pcur = pid[mm];
+                }
+        }
+        __syncthreads();
+        if (tid == 0) out[sel] = pcur;
+        const double a0 = pc[0], a1 = pc[1], a2 = pc[2];
+        double best = -1.0;
+        int arg = -1;
+#pragma unroll
+        for (int m = 0; m < kP1PerThread; ++m) {
+            if (pid[m] < 0 || mind[m] < 0.0) continue;
+            const double pp[3] = {px[m], py[m], pz[m]};
+            const double d2 = p1_dist2(pp, a0, a1, a2);
+            if (d2 < mind[m]) mind[m] = d2;
+            const double score = mind[m] / d2c[m];
+            if (score > best) { best = score; arg = tid + kP1Threads * m; }
+        }
+        p1_best<false>(best, arg, redv, redq);
+        if (arg < 0) break;
+        cur = arg;
+    }
+}
+
 }  // namespace nca
 
 extern "C" {
@@ -736,6 +916,91 @@ int32_t nca_coupling_destroy(nca_coupling* cp) {
             if (cp->ctx) set_device(cp->ctx);
             delete cp;
         }
+    });
+}
+
+
+// Phase 1 on the device; see k_farpivots. Counts follow the host rule:
+// n = min(maxrank, ceil(oversample |t|)) capped by the far field's points.
+int32_t nca_farpivots(nca_ctx* ctx, const nca_points* other, int64_t nitems, const double* centers,
+                      const int64_t* nvals, const int64_t* ffptr, const int64_t* ffnodes, int64_t nn_other,
+                      const int64_t* other_lo, const int64_t* other_hi, const int64_t* other_perm, int64_t maxrank,
+                      double weight, double oversample, int64_t* pivptr, int64_t pivcap, int64_t* piv) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && other && nitems >= 0 && ffptr && pivptr && (nitems == 0 || (centers && nvals)),
+                    NCA_ERR_ARGUMENT, "nca_farpivots: invalid argument");
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        const int64_t npts = other->n;
+        std::vector<int32_t> lo(nn_other), hi(nn_other), perm(npts);
+        for (int64_t i = 0; i < nn_other; ++i) {
+            lo[i] = (int32_t)(other_lo[i] - 1);
+            hi[i] = (int32_t)other_hi[i];
+            NCA_REQUIRE(lo[i] >= 0 && hi[i] >= lo[i] && hi[i] <= npts, NCA_ERR_ARGUMENT,
+                        "nca_farpivots: node range out of bounds");
+        }
+        for (int64_t i = 0; i < npts; ++i) {
+            perm[i] = (int32_t)(other_perm[i] - 1);
+            NCA_REQUIRE(perm[i] >= 0 && perm[i] < npts, NCA_ERR_ARGUMENT, "nca_farpivots: perm out of range");
+        }
+        const int64_t nff = ffptr[nitems] - ffptr[0];
+        std::vector<int64_t> fp(nitems + 1), cnt(nitems + 1, 0);
+        std::vector<int32_t> fn(nff);
+        pivptr[0] = 0;
+        for (int64_t t = 0; t < nitems; ++t) {
+            fp[t] = ffptr[t] - ffptr[0];
+            NCA_REQUIRE(ffptr[t + 1] - ffptr[t] <= kP1MaxFar, NCA_ERR_LIMIT,
+                        "nca_farpivots: more than 4096 far nodes in one far field");
+            int64_t total = 0;
+            for (int64_t k = ffptr[t]; k < ffptr[t + 1]; ++k) {
+                const int64_t f = ffnodes[k - ffptr[0]] - 1;
+                NCA_REQUIRE(f >= 0 && f < nn_other, NCA_ERR_ARGUMENT, "nca_farpivots: far node out of range");
+                fn[k - ffptr[0]] = (int32_t)f;
+                total += hi[f] - lo[f];
+            }
+            const int64_t limit =
+                std::min<int64_t>(maxrank, (int64_t)std::ceil(oversample * (double)nvals[t]));
+            const int64_t n = std::max<int64_t>(0, std::min(limit, total));
+            NCA_REQUIRE(n <= kP1Pool, NCA_ERR_LIMIT, "nca_farpivots: more than 4096 pivots");
+            pivptr[t + 1] = pivptr[t] + n;
+        }
+        fp[nitems] = nff;
+        NCA_REQUIRE(pivptr[nitems] <= pivcap, NCA_ERR_LIMIT,
+                    "nca_farpivots: pivot capacity too small (" + std::to_string(pivptr[nitems]) + " needed)");
+        // pool sizes are bounded on the device; check them on the host first
+        for (int64_t t = 0; t < nitems; ++t) {
+            const int64_t n = pivptr[t + 1] - pivptr[t];
+            if (n == 0) continue;
+            const int64_t nf = fp[t + 1] - fp[t];
+            int64_t total = 0;
+            for (int64_t k = fp[t]; k < fp[t + 1]; ++k) total += hi[fn[k]] - lo[fn[k]];
+            int64_t per = std::max<int64_t>(1, (std::max<int64_t>(8 * n, 512) + nf - 1) / nf), pool = 0;
+            for (;;) {
+                pool = 0;
+                for (int64_t k = fp[t]; k < fp[t + 1]; ++k) pool += std::min<int64_t>(hi[fn[k]] - lo[fn[k]], per);
+                if (pool >= n || pool >= total) break;
+                per *= 2;
+            }
+            NCA_REQUIRE(pool <= kP1Pool, NCA_ERR_LIMIT, "nca_farpivots: candidate pool above 4096 points");
+        }
+        DevBuf<int32_t> dlo, dhi, dperm, dfn;
+        DevBuf<int64_t> dfp, dptr, dpiv;
+        DevBuf<double> dctr;
+        dlo.upload(lo, st);
+        dhi.upload(hi, st);
+        dperm.upload(perm, st);
+        dfn.upload(fn, st);
+        dfp.upload(fp, st);
+        std::vector<int64_t> pp(pivptr, pivptr + nitems + 1);
+        dptr.upload(pp, st);
+        dctr.upload(centers, (size_t)(3 * nitems), st);
+        dpiv.alloc((size_t)std::max<int64_t>(pivptr[nitems], 1));
+        if (nitems > 0)
+            hipLaunchKernelGGL(k_farpivots, dim3((unsigned)nitems), dim3(kP1Threads), 0, st, other->xyz.p, dlo.p,
+                               dhi.p, dperm.p, dctr.p, dfp.p, dfn.p, dptr.p, weight, dpiv.p);
+        NCA_HIP(hipGetLastError());
+        if (pivptr[nitems] > 0) d2h(ctx, piv, dpiv.p, sizeof(int64_t) * pivptr[nitems]);
+        NCA_HIP(hipStreamSynchronize(st));
     });
 }
 

@@ -619,6 +619,26 @@ ncat_csr* ncat_dirfarfield(int64_t nn, const int64_t* parent, const int64_t* dir
     return reinterpret_cast<ncat_csr*>(c);
 }
 
+// farfield of every node as a 0-based CSR: its own far nodes, then those of
+// every ancestor (farinteractions.jl:38-60).
+ncat_csr* ncat_farfield_csr(int64_t nn, const int64_t* parent, const int64_t* farptr, const int64_t* farnodes) {
+    Csr* c = new Csr();
+    c->ptr.assign(nn + 1, 0);
+    for (int64_t t = 0; t < nn; ++t) {
+        int64_t k = 0;
+        for (int64_t a = t; a >= 0; a = parent[a]) k += farptr[a + 1] - farptr[a];
+        c->ptr[t + 1] = c->ptr[t] + k;
+    }
+    c->idx.resize(c->ptr[nn]);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t t = 0; t < nn; ++t) {
+        int64_t o = c->ptr[t];
+        for (int64_t a = t; a >= 0; a = parent[a])
+            for (int64_t k = farptr[a]; k < farptr[a + 1]; ++k) c->idx[o++] = farnodes[k];
+    }
+    return reinterpret_cast<ncat_csr*>(c);
+}
+
 // Packs nb complex blocks (interleaved doubles): block b is rows[b] x cols[b]
 // column-major at dst + 2 dst_off[b], read from src at src_off[b] with leading
 // dimension ld[b], as is (transpose = 0: dst(i, j) = src(i + j ld)) or

@@ -76,6 +76,8 @@ def _declare_tree(L):
     L.ncat_farpivots.argtypes = [I64, P, P, P, P, P, P, P, P, P, P, I64, ctypes.c_double, ctypes.c_double]
     L.ncat_farpivots_lists.restype = P
     L.ncat_farpivots_lists.argtypes = [I64, P, P, P, P, P, P, P, P, P, P, I64, ctypes.c_double, ctypes.c_double]
+    L.ncat_farfield_csr.restype = P
+    L.ncat_farfield_csr.argtypes = [I64, P, P, P]
     L.ncat_dirfarfield.restype = P
     L.ncat_dirfarfield.argtypes = [I64, P, P, P, P, P, P, P]
     L.ncat_csr_nnz.restype = I64

@@ -48,14 +48,42 @@ class ACA:
 class GeometricPivoting:
     """Phase-1 far-pivot selector (stand-in for TreeMimicryPivoting; see
     nca_tree.cpp): up to min(maxrank, oversample |t|) far points per cluster,
-    spread by maximal separation relative to (distance to the cluster)^weight."""
+    spread by maximal separation relative to (distance to the cluster)^weight.
+    device: select on the GPU (nca_farpivots) when the caller passes the other
+    side's device points, else on the host; both give the same pivots for
+    weight 1."""
 
-    def __init__(self, weight=1.0, oversample=2.0):
+    def __init__(self, weight=1.0, oversample=2.0, device=True):
         self.weight = float(weight)
         self.oversample = float(oversample)
+        self.device = bool(device)
 
-    def select(self, selftree, othertree, fd, maxrank):
+    def select(self, selftree, othertree, fd, maxrank, otherpoints=None):
+        if self.device and otherpoints is not None:
+            ffptr, ffnodes = _trees.farfield_csr(selftree, fd)
+            return device_farpivots(otherpoints, selftree.center, selftree.hi - selftree.lo, ffptr, ffnodes,
+                                    othertree, maxrank, self.weight, self.oversample)
         return _trees.farpivots(selftree, othertree, fd, maxrank, self.weight, self.oversample)
+
+
+def device_farpivots(otherpoints, centers, nvals, ffptr, ffnodes, othertree, maxrank, weight, oversample):
+    """Phase 1 on the device (nca_farpivots) for items with explicit far fields
+    (0-based CSR of nodes of othertree), centers (nitems x 3) and sizes nvals.
+    Returns a 0-based CSR of point ids."""
+    n = len(ffptr) - 1
+    c = np.ascontiguousarray(centers, np.float64)
+    nv = np.ascontiguousarray(nvals, np.int64)
+    fp1 = np.ascontiguousarray(np.asarray(ffptr) + 1, np.int64)
+    fn1 = np.ascontiguousarray(np.asarray(ffnodes) + 1, np.int64)
+    lay = othertree.layout_1based()
+    ptr = np.zeros(n + 1, np.int64)
+    cap = max(n * int(maxrank), 1)
+    piv = np.zeros(cap, np.int64)
+    call(hip_lib().nca_farpivots(otherpoints.ctx.handle, otherpoints.handle, n, _p(c), _p(nv), _p(fp1),
+                                 _p(fn1) if len(fn1) else None, othertree.nnodes, _p(lay["nodelo"]),
+                                 _p(lay["nodehi"]), _p(lay["perm"]), int(maxrank), float(weight), float(oversample),
+                                 _p(ptr), cap, _p(piv)))
+    return ptr, piv[:ptr[-1]].copy()
 
 
 class BatchedBottomUp:
@@ -311,7 +339,8 @@ def batched_side(compressor, farmatrix, tree, fd, side, maxrank=50, farpivots=No
     selftree = tree.testtree if side == 0 else tree.trialtree
     othertree = tree.trialtree if side == 0 else tree.testtree
     if farpivots is None:
-        farpivots = compressor.pivotselector.select(selftree, othertree, fd, maxrank)
+        farpivots = compressor.pivotselector.select(selftree, othertree, fd, maxrank,
+                                                    farmatrix.trial if side == 0 else farmatrix.test)
     fptr, fidx = farpivots
     nfar = np.diff(fptr)
     out = LevelFactors(selftree.nnodes)

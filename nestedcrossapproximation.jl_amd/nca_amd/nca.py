@@ -84,7 +84,8 @@ class PetrovGalerkinNCACore:
         timings["partition"] = time.perf_counter() - t0
         self.farmatrix = HIPKernelMatrix(kernel, testpoints, trialpoints, self.ctx)
         t0 = time.perf_counter()
-        fp_test = testcompressor.pivotselector.select(tree.testtree, tree.trialtree, self.testfardata, maxrank)
+        fp_test = testcompressor.pivotselector.select(tree.testtree, tree.trialtree, self.testfardata, maxrank,
+                                                      self.farmatrix.trial)
         ts, ss = testcompressor.pivotselector, trialcompressor.pivotselector
         tf, sf = self.testfardata, self.trialfardata
         if (tree.testtree is tree.trialtree and type(ts) is type(ss) and vars(ts) == vars(ss)
@@ -92,7 +93,8 @@ class PetrovGalerkinNCACore:
                 and np.array_equal(tf.farnodes, sf.farnodes)):
             fp_trial = fp_test  # one tree, symmetric partition: the same selection
         else:
-            fp_trial = trialcompressor.pivotselector.select(tree.trialtree, tree.testtree, self.trialfardata, maxrank)
+            fp_trial = trialcompressor.pivotselector.select(tree.trialtree, tree.testtree, self.trialfardata,
+                                                            maxrank, self.farmatrix.test)
         timings["phase1"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         self.testbasis, self.testtransfer, self.testfactors = _cmp.testbases(

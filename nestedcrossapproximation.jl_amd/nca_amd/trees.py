@@ -189,6 +189,20 @@ def farfield(tree, fd, node):
     return np.asarray(out, dtype=np.int64)
 
 
+def farfield_csr(tree, fd):
+    """farfield of every node (own far nodes, then the ancestors') as a 0-based CSR."""
+    L = tree_lib()
+    par = np.ascontiguousarray(tree.parent, np.int64)
+    fp = np.ascontiguousarray(fd.nodetofarnodesptr, np.int64)
+    fn = np.ascontiguousarray(fd.farnodes, np.int64)
+    h = L.ncat_farfield_csr(tree.nnodes, _ptr(par), _ptr(fp), _ptr(fn) if len(fn) else None)
+    ptr = np.empty(tree.nnodes + 1, np.int64)
+    idx = np.empty(L.ncat_csr_nnz(h), np.int64)
+    L.ncat_csr_export(h, _ptr(ptr), _ptr(idx) if len(idx) else None)
+    L.ncat_csr_free(h)
+    return ptr, idx
+
+
 def farpivots(selftree, othertree, fd, maxrank, weight=1.0, oversample=2.0):
     """Phase 1 (batchedbottomup.jl:257-266): geometric far pivots of every node
     of ``selftree`` among the points of ``othertree``. Returns a 0-based CSR

@@ -139,9 +139,15 @@ class DirectionBlocks:
         )
 
 
-def farpivots(blocks, othertree, ffptr, ffnodes, maxrank, selector):
-    """Phase 1 of every direction index over its dirfarfield (nca_tree.cpp
-    ncat_farpivots_lists). Returns a 0-based CSR of point ids."""
+def farpivots(blocks, othertree, ffptr, ffnodes, maxrank, selector, otherpoints=None):
+    """Phase 1 of every direction index over its dirfarfield: on the device
+    (nca_farpivots) when the selector allows it and the other side's device
+    points are given, else on the host (nca_tree.cpp ncat_farpivots_lists).
+    Returns a 0-based CSR of point ids."""
+    if getattr(selector, "device", False) and otherpoints is not None:
+        t = blocks.tree
+        return _cmp.device_farpivots(otherpoints, t.center[blocks.node], (t.hi - t.lo)[blocks.node], ffptr, ffnodes,
+                                     othertree, maxrank, selector.weight, selector.oversample)
     L = tree_lib()
     t = blocks.tree
     arrs = [np.ascontiguousarray(a, np.int64) for a in (blocks.node, t.lo, t.hi, othertree.lo, othertree.hi,
@@ -300,8 +306,10 @@ class DirectionalNCACore(_nca.PetrovGalerkinNCACore):
         sff = _dir.dirfarfield_all(tree.trialtree, self.trialdata)
         self.testblocks = DirectionBlocks(tree.testtree, self.testdata, tff[0])
         self.trialblocks = DirectionBlocks(tree.trialtree, self.trialdata, sff[0])
-        fp_test = farpivots(self.testblocks, tree.trialtree, tff[0], tff[1], maxrank, testcompressor.pivotselector)
-        fp_trial = farpivots(self.trialblocks, tree.testtree, sff[0], sff[1], maxrank, trialcompressor.pivotselector)
+        fp_test = farpivots(self.testblocks, tree.trialtree, tff[0], tff[1], maxrank, testcompressor.pivotselector,
+                            self.farmatrix.trial)
+        fp_trial = farpivots(self.trialblocks, tree.testtree, sff[0], sff[1], maxrank,
+                             trialcompressor.pivotselector, self.farmatrix.test)
         timings["phase1"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         bt = _trees.BlockTree(self.testblocks, self.trialblocks)
