@@ -16,6 +16,8 @@
 // All indices in this file are 0-based.
 
 #include <algorithm>
+#include <stdexcept>
+#include <memory>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -448,9 +450,299 @@ Csr farpivots(int64_t nn_self, const int64_t* parent, const int64_t* lo, const i
         other_perm, other_pts, maxrank, weight, oversample);
 }
 
+// ---------------------------------------------------------------------------
+// Directional far data (the host-side subdivision of the far interactions into
+// directions): twondirections.jl:259-308 for TwoN trees with a direction tree,
+// boundingballdirections.jl:44-152 for bounding-ball trees with per-cluster
+// Fibonacci direction sets. Direction labels: 0 = low frequency, positive =
+// a direction-tree node (1-based) or an index into the cluster's set (1-based).
+struct DirData {
+    std::vector<int64_t> dirptr, dirs, diridxtofarptr, chdirptr, chdirmap;
+};
+
+inline double dot3(const double* a, const double* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+// directions(Val(3), diam, k, gamma): Fibonacci points, n from the aperture
+std::vector<double> fib_directions(double diam, double k, double gamma) {
+    const double kd = std::max(k * diam, std::numeric_limits<double>::epsilon());
+    const double a = std::asin(std::min(1.0, gamma / kd));
+    const int64_t n = std::max<int64_t>(
+        6, (int64_t)std::ceil(6.0 * std::pow(4.0, std::log(std::acos(1.0 / std::sqrt(3.0)) / a) / std::log(2.0))));
+    std::vector<double> v(3 * n);
+    const double ga = M_PI * (3.0 - std::sqrt(5.0));
+    for (int64_t i = 0; i < n; ++i) {
+        const double z = 1.0 - 2.0 * ((double)i + 0.5) / (double)n;
+        const double r = std::sqrt(std::max(0.0, 1.0 - z * z));
+        v[3 * i] = r * std::cos((double)i * ga);
+        v[3 * i + 1] = r * std::sin((double)i * ga);
+        v[3 * i + 2] = z;
+    }
+    return v;
+}
+
+int64_t nearest(const std::vector<double>& ev, const double* vec) {
+    int64_t best = 0;
+    double bd = dot3(&ev[0], vec);
+    for (int64_t i = 1; i < (int64_t)ev.size() / 3; ++i) {
+        const double d = dot3(&ev[3 * i], vec);
+        if (d > bd) { bd = d; best = i; }
+    }
+    return best;
+}
+
+// Direction tree of 3D TwoN trees: 6 roots (+x -x +y -y +z -z), 4 nested patches per level.
+struct DirTree {
+    int maxlevel = 1, dirroot = 0;
+    std::vector<double> dir;  // 3 per node, index 0 unused
+    std::vector<int64_t> parent, firstchild;
+    DirTree(int maxlevel_, int dirroot_) : maxlevel(maxlevel_), dirroot(dirroot_) {
+        int64_t total = 6 * (((int64_t)1 << (2 * maxlevel)) - 1) / 3;
+        dir.assign(3 * (total + 1), 0.0);
+        parent.assign(total + 1, 0);
+        firstchild.assign(total + 1, 0);
+        std::vector<int> axis(total + 1, 0);
+        int64_t nxt = 1;
+        for (int ax = 0; ax < 3; ++ax)
+            for (double sg : {1.0, -1.0}) {
+                dir[3 * nxt + ax] = sg;
+                axis[nxt] = ax;
+                ++nxt;
+            }
+        int64_t start = 1, count = 6;
+        double scale = 0.5;
+        for (int lev = 0; lev + 1 < maxlevel; ++lev) {
+            for (int64_t node = start; node < start + count; ++node) {
+                firstchild[node] = nxt;
+                const int ax = axis[node];
+                int others[2], k = 0;
+                for (int d = 0; d < 3; ++d)
+                    if (d != ax) others[k++] = d;
+                for (int mask = 0; mask < 4; ++mask) {
+                    for (int d = 0; d < 3; ++d) dir[3 * nxt + d] = dir[3 * node + d];
+                    for (int b = 0; b < 2; ++b) dir[3 * nxt + others[b]] += scale * (((mask >> b) & 1) ? 1.0 : -1.0);
+                    axis[nxt] = ax;
+                    parent[nxt] = node;
+                    ++nxt;
+                }
+            }
+            start += count;
+            count *= 4;
+            scale /= 2.0;
+        }
+        for (int64_t i = 1; i <= total; ++i) {
+            const double nr = std::sqrt((dir[3 * i] * dir[3 * i] + dir[3 * i + 1] * dir[3 * i + 1]) +
+                                        dir[3 * i + 2] * dir[3 * i + 2]);
+            for (int d = 0; d < 3; ++d) dir[3 * i + d] /= nr;
+        }
+    }
+    int64_t direction(const double* v, int clusterlevel) const {
+        const int depth = std::min(dirroot - clusterlevel + 1, maxlevel);
+        int64_t cur = 0;
+        for (int lev = 0; lev < depth; ++lev) {
+            const int64_t c0 = lev == 0 ? 1 : firstchild[cur];
+            const int nc = lev == 0 ? 6 : 4;
+            int64_t best = c0;
+            double bd = -std::numeric_limits<double>::infinity();
+            for (int64_t c = c0; c < c0 + nc; ++c) {
+                const double d = dot3(&dir[3 * c], v);
+                if (d > bd) { bd = d; best = c; }
+            }
+            cur = best;
+        }
+        return cur;
+    }
+};
+
+DirData directional_fardata(int twon, int64_t nn, const int64_t* parent, const int64_t* level,
+                            const int64_t* childptr, const int64_t* children, const double* center,
+                            const double* radius, const int64_t* farptr, int64_t* farnodes, const double* fcenter,
+                            double k, double gamma, double etahf) {
+    // low-frequency classification: TwoN per level, bounding balls per node (generalized radius)
+    auto lf_level = [&](int64_t lev) {
+        const double h0 = radius[0] / std::sqrt(3.0);
+        return k * (2.0 * std::sqrt(3.0) * h0 / std::ldexp(1.0, (int)lev)) <= gamma;
+    };
+    std::vector<double> grad(nn, 0.0);
+    if (!twon && nn > 0) {
+        grad[0] = radius[0];
+        for (int64_t p = 0; p < nn; ++p) {
+            const int64_t nc = childptr[p + 1] - childptr[p];
+            if (!nc) continue;
+            double sum = 0.0;
+            for (int64_t c = childptr[p]; c < childptr[p + 1]; ++c) sum += radius[children[c]];
+            const double g = std::min(grad[p], sum / (double)nc);
+            for (int64_t c = childptr[p]; c < childptr[p + 1]; ++c) grad[children[c]] = g;
+        }
+    }
+    auto lf_node = [&](int64_t node) { return twon ? lf_level(level[node]) : k * (2.0 * grad[node]) <= gamma; };
+    std::unique_ptr<DirTree> dt;
+    if (twon) {
+        int64_t lflevel = 0;
+        while (!lf_level(lflevel)) ++lflevel;
+        const int64_t dirroot = lflevel - 1;
+        int64_t first = dirroot;
+        for (int64_t t = 0; t < nn; ++t)
+            if (farptr[t + 1] > farptr[t] && !lf_level(level[t]) && level[t] < first) first = level[t];
+        dt.reset(new DirTree((int)(dirroot - first + 1), (int)dirroot));
+    }
+    std::vector<std::vector<int64_t>> dirs(nn), dcount(nn), chmap(nn);
+    std::vector<std::vector<double>> dvec(nn);
+    std::vector<char> hasvec(nn, 0);
+    std::vector<int64_t> order(nn);
+    for (int64_t i = 0; i < nn; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return level[a] < level[b]; });
+    // level by level; the nodes of a level only read their parents' results
+    std::vector<int64_t> lstart(1, 0);
+    for (int64_t i = 1; i < nn; ++i)
+        if (level[order[i]] != level[order[i - 1]]) lstart.push_back(i);
+    lstart.push_back(nn);
+    for (size_t L = 0; L + 1 < lstart.size(); ++L)
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t oi = lstart[L]; oi < lstart[L + 1]; ++oi) {
+        const int64_t node = order[oi];
+        const int64_t pnode = parent[node];
+        const bool hasfars = farptr[node + 1] > farptr[node];
+        const bool hasparentdirs = pnode >= 0 && !dirs[pnode].empty();
+        if (lf_node(node)) {
+            const bool inherit = hasparentdirs && level[node] > 0 && lf_node(pnode);
+            if (hasfars || inherit) dirs[node] = {0};
+            if (inherit) chmap[node] = {0};
+            continue;
+        }
+        const bool hasparent = twon ? hasparentdirs : (pnode >= 0 && hasvec[pnode]);
+        if (!hasfars && !hasparent) continue;
+        // (a direction tree is only needed for TwoN trees; dt is shared read-only)
+        const int64_t a = farptr[node], b = farptr[node + 1];
+        std::vector<double> total;
+        if (!twon) total = fib_directions(2.0 * radius[node], k, gamma);
+        std::vector<int64_t> nd(b - a);
+        for (int64_t i = a; i < b; ++i) {
+            double v[3];
+            for (int d = 0; d < 3; ++d) v[d] = center[3 * node + d] - fcenter[3 * farnodes[i] + d];
+            nd[i - a] = twon ? dt->direction(v, (int)level[node]) : nearest(total, v) + 1;
+        }
+        std::vector<int64_t> perm(b - a);
+        for (int64_t i = 0; i < b - a; ++i) perm[i] = i;
+        std::stable_sort(perm.begin(), perm.end(), [&](int64_t x, int64_t y) { return nd[x] < nd[y]; });
+        std::vector<int64_t> fn(b - a), sd(b - a);
+        for (int64_t i = 0; i < b - a; ++i) {
+            fn[i] = farnodes[a + perm[i]];
+            sd[i] = nd[perm[i]];
+        }
+        std::copy(fn.begin(), fn.end(), farnodes + a);
+        std::vector<int64_t> uniq(sd);
+        uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+        const size_t noriginal = uniq.size();
+        std::vector<int64_t> pm(pnode >= 0 ? dirs[pnode].size() : 0, 0);
+        auto addmap = [&](size_t idx, int64_t c) {
+            auto it = std::find(uniq.begin(), uniq.end(), c);
+            if (it == uniq.end()) {
+                uniq.push_back(c);
+                pm[idx] = (int64_t)uniq.size();
+            } else {
+                pm[idx] = (int64_t)(it - uniq.begin()) + 1;
+            }
+        };
+        if (twon) {
+            if (hasparentdirs)
+                for (size_t i = 0; i < dirs[pnode].size(); ++i) addmap(i, dt->parent[dirs[pnode][i]]);
+        } else if (pnode >= 0 && hasvec[pnode] && !dvec[pnode].empty()) {
+            double ctopc[3];
+            for (int d = 0; d < 3; ++d) ctopc[d] = center[3 * node + d] - center[3 * pnode + d];
+            const double mult = k * (2.0 * radius[pnode]) / etahf + 2.0 * radius[pnode];
+            const size_t np = dvec[pnode].size() / 3;
+            std::vector<int64_t> cd(np);
+            for (size_t i = 0; i < np; ++i) {
+                double tgt[3];
+                for (int d = 0; d < 3; ++d) tgt[d] = ctopc[d] + mult * dvec[pnode][3 * i + d];
+                cd[i] = nearest(total, tgt) + 1;
+            }
+            for (size_t i = 0; i < np; ++i) addmap(i, cd[i]);
+        }
+        std::vector<int64_t> cnt(uniq.size(), 0);
+        for (size_t u = 0; u < noriginal; ++u)
+            cnt[u] = (int64_t)std::count(sd.begin(), sd.end(), uniq[u]);
+        if (!twon) {
+            dvec[node].resize(3 * uniq.size());
+            for (size_t u = 0; u < uniq.size(); ++u)
+                for (int d = 0; d < 3; ++d) dvec[node][3 * u + d] = total[3 * (uniq[u] - 1) + d];
+            hasvec[node] = 1;
+        }
+        std::vector<int64_t> m0;
+        for (int64_t v : pm) m0.push_back(v - 1);
+        chmap[node] = m0;
+        dirs[node] = uniq;
+        dcount[node] = cnt;
+    }
+    DirData D;
+    D.dirptr.assign(nn + 1, 0);
+    D.chdirptr.assign(nn + 1, 0);
+    for (int64_t t = 0; t < nn; ++t) {
+        if (!dirs[t].empty() && dcount[t].empty()) dcount[t] = {farptr[t + 1] - farptr[t]};
+        int64_t nz = 0;
+        for (int64_t v : chmap[t]) nz += v >= 0;
+        D.dirptr[t + 1] = D.dirptr[t] + (int64_t)dirs[t].size();
+        D.chdirptr[t + 1] = D.chdirptr[t] + (dirs[t].size() == 1 && dirs[t][0] == 0 ? (int64_t)chmap[t].size() : nz);
+    }
+    const int64_t ne = D.dirptr[nn];
+    D.dirs.resize(ne);
+    D.diridxtofarptr.assign(ne + 1, 0);
+    D.chdirmap.resize(D.chdirptr[nn]);
+    for (int64_t t = 0; t < nn; ++t) {
+        int64_t off = farptr[t];
+        for (size_t i = 0; i < dirs[t].size(); ++i) {
+            D.dirs[D.dirptr[t] + i] = dirs[t][i];
+            D.diridxtofarptr[D.dirptr[t] + i] = off;
+            off += dcount[t][i];
+        }
+        if (!dirs[t].empty() && off != farptr[t + 1])
+            throw std::runtime_error("directional far data: direction counts do not sum to the far nodes");
+        std::copy(chmap[t].begin(), chmap[t].begin() + (D.chdirptr[t + 1] - D.chdirptr[t]),
+                  D.chdirmap.begin() + D.chdirptr[t]);
+    }
+    D.diridxtofarptr[ne] = farptr[nn];
+    return D;
+}
+
 }  // namespace
 
 extern "C" {
+
+typedef struct ncat_dirdata ncat_dirdata;
+
+// directional_fardata over one tree; farnodes (0-based CSR of far nodes of
+// the other tree, fcenter its centers) is reordered by direction in place.
+// Returns null on inconsistent input.
+ncat_dirdata* ncat_directional_fardata(int32_t twon, int64_t nn, const int64_t* parent, const int64_t* level,
+                                       const int64_t* childptr, const int64_t* children, const double* center,
+                                       const double* radius, const int64_t* farptr, int64_t* farnodes,
+                                       const double* fcenter, double k, double gamma, double etahf) {
+    try {
+        return reinterpret_cast<ncat_dirdata*>(new DirData(directional_fardata(
+            twon, nn, parent, level, childptr, children, center, radius, farptr, farnodes, fcenter, k, gamma, etahf)));
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+void ncat_dirdata_sizes(const ncat_dirdata* h, int64_t* ne, int64_t* nmap) {
+    const DirData& d = *reinterpret_cast<const DirData*>(h);
+    *ne = (int64_t)d.dirs.size();
+    *nmap = (int64_t)d.chdirmap.size();
+}
+
+void ncat_dirdata_export(const ncat_dirdata* h, int64_t* dirptr, int64_t* dirs, int64_t* diridxtofarptr,
+                         int64_t* chdirptr, int64_t* chdirmap) {
+    const DirData& d = *reinterpret_cast<const DirData*>(h);
+    std::copy(d.dirptr.begin(), d.dirptr.end(), dirptr);
+    std::copy(d.dirs.begin(), d.dirs.end(), dirs);
+    std::copy(d.diridxtofarptr.begin(), d.diridxtofarptr.end(), diridxtofarptr);
+    std::copy(d.chdirptr.begin(), d.chdirptr.end(), chdirptr);
+    std::copy(d.chdirmap.begin(), d.chdirmap.end(), chdirmap);
+}
+
+void ncat_dirdata_free(ncat_dirdata* h) { delete reinterpret_cast<DirData*>(h); }
+
 
 typedef struct ncat_tree ncat_tree;
 typedef struct ncat_partition ncat_partition;

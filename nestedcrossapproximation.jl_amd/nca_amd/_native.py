@@ -76,6 +76,14 @@ def _declare_tree(L):
     L.ncat_farpivots.argtypes = [I64, P, P, P, P, P, P, P, P, P, P, I64, ctypes.c_double, ctypes.c_double]
     L.ncat_farpivots_lists.restype = P
     L.ncat_farpivots_lists.argtypes = [I64, P, P, P, P, P, P, P, P, P, P, I64, ctypes.c_double, ctypes.c_double]
+    L.ncat_directional_fardata.restype = P
+    L.ncat_directional_fardata.argtypes = [ctypes.c_int32, I64, P, P, P, P, P, P, P, P, P, D, D, D]
+    L.ncat_dirdata_sizes.restype = None
+    L.ncat_dirdata_sizes.argtypes = [P, P, P]
+    L.ncat_dirdata_export.restype = None
+    L.ncat_dirdata_export.argtypes = [P] * 6
+    L.ncat_dirdata_free.restype = None
+    L.ncat_dirdata_free.argtypes = [P]
     L.ncat_farfield_csr.restype = P
     L.ncat_farfield_csr.argtypes = [I64, P, P, P]
     L.ncat_dirfarfield.restype = P

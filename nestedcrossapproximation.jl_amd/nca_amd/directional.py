@@ -342,8 +342,42 @@ def _run_counts(sorted_labels, unique_labels):
 
 def directionalfardata(fardata, tree, fartree, isnear):
     """directionalfardata (twondirections.jl:259-308 for TwoN trees,
-    boundingballdirections.jl:44-152 for bounding-ball trees). Reorders the
-    far nodes of ``fardata`` by direction in place (a stable sort)."""
+    boundingballdirections.jl:44-152 for bounding-ball trees) in the host
+    library (nca_tree.cpp ncat_directional_fardata). Reorders the far nodes of
+    ``fardata`` by direction in place (a stable sort)."""
+    if not isinstance(isnear.islf, IsLowFrequency):
+        return directionalfardata_reference(fardata, tree, fartree, isnear)
+    L = tree_lib()
+    nn = tree.nnodes
+    i64 = lambda a: np.ascontiguousarray(a, np.int64)
+    par, lev, cp, ch = i64(tree.parent), i64(tree.level), i64(tree.childptr), i64(tree.children)
+    ctr, rad = np.ascontiguousarray(tree.center, np.float64), np.ascontiguousarray(tree.radius, np.float64)
+    fctr = np.ascontiguousarray(fartree.center, np.float64)
+    fp = i64(fardata.nodetofarnodesptr)
+    fn = i64(fardata.farnodes)
+    h = L.ncat_directional_fardata(int(tree.kind == "twon"), nn, _ptr(par), _ptr(lev), _ptr(cp),
+                                   _ptr(ch) if len(ch) else None, _ptr(ctr), _ptr(rad), _ptr(fp),
+                                   _ptr(fn) if len(fn) else None, _ptr(fctr), isnear.k, isnear.islf.gamma,
+                                   isnear.etahf)
+    if not h:
+        raise AssertionError("directional far data: direction counts do not sum to the far nodes")
+    ne, nmap = ctypes.c_int64(), ctypes.c_int64()
+    L.ncat_dirdata_sizes(h, ctypes.byref(ne), ctypes.byref(nmap))
+    dirptr = np.empty(nn + 1, np.int64)
+    dirs = np.empty(ne.value, np.int64)
+    dfp = np.empty(ne.value + 1, np.int64)
+    chptr = np.empty(nn + 1, np.int64)
+    chmap = np.empty(nmap.value, np.int64)
+    L.ncat_dirdata_export(h, _ptr(dirptr), _ptr(dirs) if ne.value else None, _ptr(dfp), _ptr(chptr),
+                          _ptr(chmap) if nmap.value else None)
+    L.ncat_dirdata_free(h)
+    fardata.farnodes[:] = fn
+    return DirectionalData(fardata, dirptr, dirs, dfp, chptr, chmap, isnear)
+
+
+def directionalfardata_reference(fardata, tree, fartree, isnear):
+    """The same subdivision written out in numpy (for any low-frequency
+    classifier; the library form above is checked against it)."""
     ptr = fardata.nodetofarnodesptr
     nodes = fardata.farnodes
     nn = len(ptr) - 1

@@ -366,3 +366,23 @@ def test_directional_admissibility(gamma, ownbound, inheritedbound):
                     worst_inh = max(worst_inh, out)
     assert worst_own <= ownbound
     assert worst_inh <= inheritedbound
+
+
+@pytest.mark.parametrize("kind,lam", [("twon", 1.5), ("twon", 0.2), ("kmeans", 1.5), ("kmeans", 0.02),
+                                      ("kmeans", 50.0)])
+def test_library_directional_data_equals_numpy(kind, lam):
+    """The host library's subdivision equals the numpy statement of the same
+    algorithm: direction labels, counts, maps and the far-node order."""
+    pts = np.random.default_rng(3).random((400, 3))
+    tr = nca_amd.TwoNTree(pts, minvalues=4) if kind == "twon" else nca_amd.KMeansTree(pts, 2, 4)
+    tree = T.BlockTree(tr, tr)
+    isn = D.isnearwideband(2 * math.pi / lam, gamma=1.0)
+    part = D.interactions(tree, isn)
+    tf, _ = T.fardata(tree, part)
+    a = T.FarData(tf.nodetofarnodesptr.copy(), tf.farnodes.copy())
+    b = T.FarData(tf.nodetofarnodesptr.copy(), tf.farnodes.copy())
+    da = D.directionalfardata(a, tr, tr, isn)
+    db = D.directionalfardata_reference(b, tr, tr, isn)
+    for name in ("nodetodirsptr", "dirs", "diridxtofarptr", "nodetochdirptr", "partochddirmap"):
+        assert np.array_equal(getattr(da, name), getattr(db, name)), name
+    assert np.array_equal(a.farnodes, b.farnodes)
