@@ -349,3 +349,45 @@ def test_stored_near_field_matches_on_the_fly():
     assert np.array_equal(H.matvec(u, op=1), H2.matvec(u, op=1))
     H.drop_near()
     assert np.array_equal(H.matvec(x), y0)
+
+
+@pytest.mark.parametrize("kind,n,lam", [("kmeans", 3000, 1.5), ("twon", 3000, 1.5)])
+def test_oracle_directional_vs_direct_summation(kind, n, lam):
+    """The oracle's directional product (OracleDH2: the reference's directional
+    stages restated in numpy) over the oracle's own level driver on the
+    (cluster, direction) blocks: within 10 tol of dense direct summation for
+    tol 1e-2 and 1e-4 (test/nca/test_petrovgalerkinnca.jl:104-153, wideband
+    with etalf = etahf = gamma = 1)."""
+    import math
+    import nca_amd
+    from nca_amd import directional as D, wideband as W, compressors as C, trees as T, geometry
+    from helpers import oracle_factors
+    pts = geometry.fibonacci_sphere(n)
+    tr = nca_amd.TwoNTree(pts, minvalues=16) if kind == "twon" else nca_amd.KMeansTree(pts, 2, 16)
+    tree = T.BlockTree(tr, tr)
+    k = 2 * math.pi / lam
+    isn = D.isnearwideband(k, etalf=1.0, etahf=1.0, gamma=1.0)
+    part = D.interactions(tree, isn)
+    td, sd = D.fardata(tree, isn, part)
+    tff, sff = D.dirfarfield_all(tr, td), D.dirfarfield_all(tr, sd)
+    tb, sb = W.DirectionBlocks(tr, td, tff[0]), W.DirectionBlocks(tr, sd, sff[0])
+    sel = C.GeometricPivoting(device=False)
+    fpt = W.farpivots(tb, tr, tff[0], tff[1], 50, sel)
+    fps = W.farpivots(sb, tr, sff[0], sff[1], 50, sel)
+    x = np.random.default_rng(4).standard_normal(n) + 1j * np.random.default_rng(5).standard_normal(n)
+    rows = np.arange(0, n, 7)
+    yd = O.dense_rows(k, 1, pts, pts, rows, x)[:, 0]
+    for tol in (1e-2, 1e-4):
+        tres = O.bases(0, k, 1, pts, pts, tb, fpt[0], fpt[1], 50, tol)
+        sres = O.bases(1, k, 1, pts, pts, sb, fps[0], fps[1], 50, tol)
+        assert tres[4] == 0 and sres[4] == 0
+        ft, fs = oracle_factors(tres, tb, fpt), oracle_factors(sres, sb, fps)
+        tB, tT = W.dir_stores(ft, tb, 0)
+        sB, sT = W.dir_stores(fs, sb, 1)
+        cplan = W.coupling_plan(td, sd)
+        tp_, sp_ = ft.pivot_csr(), fs.pivot_csr()
+        off, blocks = O.coupling(k, 1, pts, pts, td.diridxtofarptr, cplan[2], tp_[0], tp_[1], sp_[0], sp_[1])
+        A = O.OracleDH2(W.dh2desc(tree, part, td, sd, ft, fs, tB, tT, sB, sT, cplan), k, 1, pts, pts, off, blocks)
+        y = A.matvec(x)
+        err = np.linalg.norm(y[rows] - yd) / np.linalg.norm(yd)
+        assert err < 10 * tol, (tol, err)
