@@ -20,6 +20,13 @@ def coefficient_ptr(f):
     return ptr
 
 
+def _same_points(a, b):
+    if a is b:
+        return True
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and np.array_equal(a, b)
+
+
 def h2desc(tree, partition, testfardata, testf, trialf, testbasis, testtransfer, trialbasis, trialtransfer):
     """The nca_h2_desc arrays, 1-based as the Julia structs hold them."""
     tt, st = tree.testtree, tree.trialtree
@@ -72,7 +79,8 @@ class PetrovGalerkinNCACore:
     """Read-only part: the device operator plus the host-side plans and factors."""
 
     def __init__(self, kernel, testpoints, trialpoints, tree, testcompressor, trialcompressor, eta=1.0,
-                 maxrank=50, precision=_abi.NCA_PREC_FP64, ctx=None, verbose=False, partition=None):
+                 maxrank=50, precision=_abi.NCA_PREC_FP64, ctx=None, verbose=False, partition=None,
+                 reuse_symmetric=True):
         self.ctx = ctx or default_context()
         self.kernel = kernel
         self.tree = tree
@@ -99,8 +107,18 @@ class PetrovGalerkinNCACore:
         t0 = time.perf_counter()
         self.testbasis, self.testtransfer, self.testfactors = _cmp.testbases(
             testcompressor, self.farmatrix, tree, self.testfardata, maxrank, fp_test)
-        self.trialbasis, self.trialtransfer, self.trialfactors = _cmp.trialbases(
-            trialcompressor, self.farmatrix, tree, self.trialfardata, maxrank, fp_trial)
+        # One tree, one point set, one partition and the same compressor: the
+        # trial blocks transpose(A[cols, rows]) hold the test blocks' numbers
+        # (the kernel is symmetric), so the trial pass would repeat the test
+        # pass bit for bit; its factors are reused and packed transposed.
+        self.symmetric = bool(reuse_symmetric and fp_trial is fp_test and _same_points(testpoints, trialpoints)
+                              and vars(testcompressor.compressor) == vars(trialcompressor.compressor))
+        if self.symmetric:
+            self.trialfactors = self.testfactors
+            self.trialbasis, self.trialtransfer = _cmp.stores_from_factors(self.testfactors, tree.trialtree, 1)
+        else:
+            self.trialbasis, self.trialtransfer, self.trialfactors = _cmp.trialbases(
+                trialcompressor, self.farmatrix, tree, self.trialfardata, maxrank, fp_trial)
         self.testpivots = self.testfactors.pivot_csr()
         self.trialpivots = self.trialfactors.pivot_csr()
         self.couplingstore = _cmp.assemble_couplingstore(self.farmatrix, self.testpivots, self.trialpivots,

@@ -289,7 +289,8 @@ class DirectionalNCACore(_nca.PetrovGalerkinNCACore):
     statistics are those of the plain core."""
 
     def __init__(self, kernel, testpoints, trialpoints, tree, testcompressor, trialcompressor, isnear,
-                 maxrank=50, precision=_abi.NCA_PREC_FP64, ctx=None, verbose=False, partition=None):
+                 maxrank=50, precision=_abi.NCA_PREC_FP64, ctx=None, verbose=False, partition=None,
+                 reuse_symmetric=True):
         self.ctx = ctx or default_context()
         self.kernel = kernel
         self.tree = tree
@@ -302,19 +303,34 @@ class DirectionalNCACore(_nca.PetrovGalerkinNCACore):
         timings["partition"] = time.perf_counter() - t0
         self.farmatrix = HIPKernelMatrix(kernel, testpoints, trialpoints, self.ctx)
         t0 = time.perf_counter()
-        tff = _dir.dirfarfield_all(tree.testtree, self.testdata)
-        sff = _dir.dirfarfield_all(tree.trialtree, self.trialdata)
-        self.testblocks = DirectionBlocks(tree.testtree, self.testdata, tff[0])
-        self.trialblocks = DirectionBlocks(tree.trialtree, self.trialdata, sff[0])
-        fp_test = farpivots(self.testblocks, tree.trialtree, tff[0], tff[1], maxrank, testcompressor.pivotselector,
-                            self.farmatrix.trial)
-        fp_trial = farpivots(self.trialblocks, tree.testtree, sff[0], sff[1], maxrank,
-                             trialcompressor.pivotselector, self.farmatrix.test)
+        td, sd = self.testdata, self.trialdata
+        ts, ss = testcompressor.pivotselector, trialcompressor.pivotselector
+        # one tree, one point set, the same directional data and compressors:
+        # the trial side repeats the test side bit for bit (symmetric kernel)
+        self.symmetric = bool(
+            reuse_symmetric and tree.testtree is tree.trialtree and _nca._same_points(testpoints, trialpoints)
+            and type(ts) is type(ss) and vars(ts) == vars(ss)
+            and vars(testcompressor.compressor) == vars(trialcompressor.compressor)
+            and all(np.array_equal(getattr(td, n), getattr(sd, n)) for n in
+                    ("nodetodirsptr", "dirs", "diridxtofarptr", "nodetochdirptr", "partochddirmap"))
+            and np.array_equal(td.fardata.farnodes, sd.fardata.farnodes))
+        tff = _dir.dirfarfield_all(tree.testtree, td)
+        self.testblocks = DirectionBlocks(tree.testtree, td, tff[0])
+        fp_test = farpivots(self.testblocks, tree.trialtree, tff[0], tff[1], maxrank, ts, self.farmatrix.trial)
+        if self.symmetric:
+            self.trialblocks, fp_trial = self.testblocks, fp_test
+        else:
+            sff = _dir.dirfarfield_all(tree.trialtree, sd)
+            self.trialblocks = DirectionBlocks(tree.trialtree, sd, sff[0])
+            fp_trial = farpivots(self.trialblocks, tree.testtree, sff[0], sff[1], maxrank, ss, self.farmatrix.test)
         timings["phase1"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         bt = _trees.BlockTree(self.testblocks, self.trialblocks)
         self.testfactors = _cmp.batched_side(testcompressor, self.farmatrix, bt, None, 0, maxrank, fp_test)
-        self.trialfactors = _cmp.batched_side(trialcompressor, self.farmatrix, bt, None, 1, maxrank, fp_trial)
+        if self.symmetric:
+            self.trialfactors = self.testfactors
+        else:
+            self.trialfactors = _cmp.batched_side(trialcompressor, self.farmatrix, bt, None, 1, maxrank, fp_trial)
         self.testbasis, self.testtransfer = dir_stores(self.testfactors, self.testblocks, 0)
         self.trialbasis, self.trialtransfer = dir_stores(self.trialfactors, self.trialblocks, 1)
         self.testpivots = self.testfactors.pivot_csr()

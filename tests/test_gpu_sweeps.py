@@ -161,3 +161,24 @@ def test_autotune_keeps_results(ctx, precision):
     assert relerr(W @ x, ref) <= bound
     u = geometry.complex_gaussian(rng, (p.testpts.shape[0], 16))
     assert relerr(NC.adjoint(W) @ u, NC.adjoint(NC.H2Workspace(W2)) @ u) <= bound
+
+
+def test_symmetric_reuse_equals_trial_pass():
+    """One tree and one point set: the reused (transposed) test factors equal
+    the ones a separate trial pass computes, bit for bit, and so do the products."""
+    say("symmetric reuse")
+    pts = geometry.fibonacci_sphere(6000)
+    tr = nca_amd.TwoNTree(pts, minvalues=32)
+    cmp = C.BatchedBottomUp(C.GeometricPivoting(), C.ACA(1e-4))
+    cores = [nca_amd.PetrovGalerkinNCACore(nca_amd.HelmholtzKernel(1.3), pts, pts, nca_amd.BlockTree(tr, tr),
+                                           cmp, cmp, reuse_symmetric=r) for r in (True, False)]
+    assert cores[0].symmetric and not cores[1].symmetric
+    a, b = cores
+    for x, y in zip(a.trialfactors.pivot_csr(), b.trialfactors.pivot_csr()):
+        assert np.array_equal(x, y)
+    for name in ("nodes", "offsets", "blocks"):
+        assert np.array_equal(getattr(a.trialbasis, name), getattr(b.trialbasis, name))
+    for name in ("levelidcs", "chdsidcs", "offsets", "blocks"):
+        assert np.array_equal(getattr(a.trialtransfer, name), getattr(b.trialtransfer, name))
+    x = geometry.complex_gaussian(np.random.default_rng(1), (pts.shape[0], 16))
+    assert np.array_equal(NC.H2Workspace(a) @ x, NC.H2Workspace(b) @ x)
