@@ -10,6 +10,8 @@
 // irrelevant for the pivots themselves.
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cmath>
 #include <numeric>
 
@@ -732,6 +734,27 @@ __global__ void __launch_bounds__(kP1Threads)
     }
 }
 
+// Host loop over [0, n) on up to 16 threads (dynamic chunks); f must not throw.
+template <class F>
+static void host_parallel_for(int64_t n, F&& f) {
+    const int64_t nt = std::min<int64_t>(std::max<unsigned>(1u, std::thread::hardware_concurrency()), 16);
+    if (n < 4096 || nt == 1) {
+        for (int64_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    std::vector<std::thread> pool;
+    for (int64_t k = 0; k < nt; ++k)
+        pool.emplace_back([&] {
+            for (;;) {
+                const int64_t b = next.fetch_add(256);
+                if (b >= n) break;
+                for (int64_t i = b; i < std::min(n, b + 256); ++i) f(i);
+            }
+        });
+    for (auto& t : pool) t.join();
+}
+
 }  // namespace nca
 
 extern "C" {
@@ -944,36 +967,26 @@ int32_t nca_farpivots(nca_ctx* ctx, const nca_points* other, int64_t nitems, con
             NCA_REQUIRE(perm[i] >= 0 && perm[i] < npts, NCA_ERR_ARGUMENT, "nca_farpivots: perm out of range");
         }
         const int64_t nff = ffptr[nitems] - ffptr[0];
-        std::vector<int64_t> fp(nitems + 1), cnt(nitems + 1, 0);
+        std::vector<int64_t> fp(nitems + 1), cnt(nitems, 0);
         std::vector<int32_t> fn(nff);
-        pivptr[0] = 0;
-        for (int64_t t = 0; t < nitems; ++t) {
-            fp[t] = ffptr[t] - ffptr[0];
-            NCA_REQUIRE(ffptr[t + 1] - ffptr[t] <= kP1MaxFar, NCA_ERR_LIMIT,
-                        "nca_farpivots: more than 4096 far nodes in one far field");
+        // per item: far nodes (checked), pivot count, and the candidate pool
+        // size the device will build (bounded by kP1Pool); 0 ok, else an error code
+        std::vector<char> bad(nitems, 0);
+        for (int64_t t = 0; t <= nitems; ++t) fp[t] = ffptr[t] - ffptr[0];
+        host_parallel_for(nitems, [&](int64_t t) {
+            if (fp[t + 1] - fp[t] > kP1MaxFar) { bad[t] = 1; return; }
             int64_t total = 0;
-            for (int64_t k = ffptr[t]; k < ffptr[t + 1]; ++k) {
-                const int64_t f = ffnodes[k - ffptr[0]] - 1;
-                NCA_REQUIRE(f >= 0 && f < nn_other, NCA_ERR_ARGUMENT, "nca_farpivots: far node out of range");
-                fn[k - ffptr[0]] = (int32_t)f;
+            for (int64_t k = fp[t]; k < fp[t + 1]; ++k) {
+                const int64_t f = ffnodes[k] - 1;
+                if (f < 0 || f >= nn_other) { bad[t] = 2; return; }
+                fn[k] = (int32_t)f;
                 total += hi[f] - lo[f];
             }
-            const int64_t limit =
-                std::min<int64_t>(maxrank, (int64_t)std::ceil(oversample * (double)nvals[t]));
+            const int64_t limit = std::min<int64_t>(maxrank, (int64_t)std::ceil(oversample * (double)nvals[t]));
             const int64_t n = std::max<int64_t>(0, std::min(limit, total));
-            NCA_REQUIRE(n <= kP1Pool, NCA_ERR_LIMIT, "nca_farpivots: more than 4096 pivots");
-            pivptr[t + 1] = pivptr[t] + n;
-        }
-        fp[nitems] = nff;
-        NCA_REQUIRE(pivptr[nitems] <= pivcap, NCA_ERR_LIMIT,
-                    "nca_farpivots: pivot capacity too small (" + std::to_string(pivptr[nitems]) + " needed)");
-        // pool sizes are bounded on the device; check them on the host first
-        for (int64_t t = 0; t < nitems; ++t) {
-            const int64_t n = pivptr[t + 1] - pivptr[t];
-            if (n == 0) continue;
+            cnt[t] = n;
+            if (n == 0) return;
             const int64_t nf = fp[t + 1] - fp[t];
-            int64_t total = 0;
-            for (int64_t k = fp[t]; k < fp[t + 1]; ++k) total += hi[fn[k]] - lo[fn[k]];
             int64_t per = std::max<int64_t>(1, (std::max<int64_t>(8 * n, 512) + nf - 1) / nf), pool = 0;
             for (;;) {
                 pool = 0;
@@ -981,8 +994,17 @@ int32_t nca_farpivots(nca_ctx* ctx, const nca_points* other, int64_t nitems, con
                 if (pool >= n || pool >= total) break;
                 per *= 2;
             }
-            NCA_REQUIRE(pool <= kP1Pool, NCA_ERR_LIMIT, "nca_farpivots: candidate pool above 4096 points");
+            if (pool > kP1Pool) bad[t] = 3;
+        });
+        for (int64_t t = 0; t < nitems; ++t) {
+            NCA_REQUIRE(bad[t] != 1, NCA_ERR_LIMIT, "nca_farpivots: more than 4096 far nodes in one far field");
+            NCA_REQUIRE(bad[t] != 2, NCA_ERR_ARGUMENT, "nca_farpivots: far node out of range");
+            NCA_REQUIRE(bad[t] != 3, NCA_ERR_LIMIT, "nca_farpivots: candidate pool above 4096 points");
         }
+        pivptr[0] = 0;
+        for (int64_t t = 0; t < nitems; ++t) pivptr[t + 1] = pivptr[t] + cnt[t];
+        NCA_REQUIRE(pivptr[nitems] <= pivcap, NCA_ERR_LIMIT,
+                    "nca_farpivots: pivot capacity too small (" + std::to_string(pivptr[nitems]) + " needed)");
         DevBuf<int32_t> dlo, dhi, dperm, dfn;
         DevBuf<int64_t> dfp, dptr, dpiv;
         DevBuf<double> dctr;

@@ -2008,6 +2008,7 @@ int32_t nca_dh2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_poin
             const double2* B = reinterpret_cast<const double2*>(blocks);
             L.off.assign(1, 0);
             const char* side = testside ? "test" : "trial";
+            // checks and sizes in order, then the packing in parallel
             for (int64_t i = 0; i < nb; ++i) {
                 const int64_t t = bnodes[i] - 1;
                 NCA_REQUIRE(t >= 0 && t < nn, NCA_ERR_ARGUMENT, "nca_dh2_create: basis node out of range");
@@ -2037,10 +2038,17 @@ int32_t nca_dh2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_poin
                     NCA_REQUIRE(cptr[q + 1] == cptr[q], NCA_ERR_ARGUMENT,
                                 "nca_dh2_create: a direction with coefficients has no basis");
                 NCA_REQUIRE(listed == R, NCA_ERR_ARGUMENT, "nca_dh2_create: basis ranks differ from the plan");
-                const size_t base = L.blocks.size();
-                L.blocks.resize(base + (size_t)(m * R));
-                int64_t col = 0;
-                for (int64_t j = j0; j < j1; ++j) {
+                L.nodes.push_back(t + 1);
+                L.off.push_back(L.off.back() + m * R);
+            }
+            L.blocks.resize((size_t)std::max<int64_t>(L.off.back(), 1));
+            parallel_for(nb, [&](int64_t i) {
+                const int64_t t = bnodes[i] - 1;
+                const int64_t m = hi[t] - lo[t] + 1;
+                const int64_t e0 = dirptr[t] - 1, e1 = dirptr[t + 1] - 1;
+                const int64_t c0 = cptr[e0] - 1, R = cptr[e1] - cptr[e0];
+                const int64_t base = L.off[i];
+                for (int64_t j = bdirptr[i] - 1; j < bdirptr[i + 1] - 1; ++j) {
                     const int64_t e = bdiridx[j] - 1, r = cptr[e + 1] - cptr[e];
                     const double2* Be = B + boff[j];
                     const int64_t ro = cptr[e] - 1 - c0;  // first coefficient of e within the node
@@ -2053,12 +2061,8 @@ int32_t nca_dh2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_poin
                         for (int64_t a = 0; a < m; ++a)
                             for (int64_t k = 0; k < r; ++k) L.blocks[base + (ro + k) + a * R] = Be[k + a * r];
                     }
-                    col += r;
                 }
-                L.nodes.push_back(t + 1);
-                L.off.push_back(L.off.back() + m * R);
-            }
-            if (L.blocks.empty()) L.blocks.resize(1);
+            });
         };
         Lowered lt, ls;
         lower(true, d->nn_test, d->test_lo, d->test_hi, d->test_dirptr, d->test_cptr, d->nbnodes_test, d->bt_nodes,
