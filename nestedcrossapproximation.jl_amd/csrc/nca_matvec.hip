@@ -815,38 +815,64 @@ __global__ void __launch_bounds__(256, 4)
     }
 }
 
+// Raw buffer loads over one tile's coupling block and column map: reads past
+// the end of the descriptor's range return 0, so the pipeline below runs
+// without clamps or masks (a zero A column meets whatever coefficient the
+// zero index names).
+constexpr int kBufDword3 = 0x00020000;  // gfx9 buffer descriptor, raw dword access
+
+template <class T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ float2 buf_load<float2>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ double2 buf_load<double2>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ int buf_load<int>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+
 // Stage 3 forward, one tile: acc += (coupling row of the tile) (source
 // coefficients), four columns per step. Software pipeline over groups of kU
 // steps, two register sets in turn: the operands of group n + 1 and the
 // coefficient indices of group n + 2 are in flight while the MFMAs of group n
-// run. Columns past C repeat column C - 1; their B is zeroed at use.
+// run. Sb: the tile's r_t x C block (column-major, ur = r_t rows), rowsel:
+// the lane's row. Columns past C read A = 0 (and source index 0), so the
+// pipeline needs neither clamps nor masks.
 template <class P, bool CJ, class ACC, class XT>
-__device__ __forceinline__ void couple_run(ACC& acc, const typename P::T2* __restrict__ St, uint32_t ur,
-                                           const int32_t* __restrict__ cc, int C, const XT* __restrict__ xq,
-                                           uint32_t xstride, int kk) {
+__device__ __forceinline__ void couple_run(ACC& acc, const typename P::T2* __restrict__ Sb, uint32_t ur,
+                                           uint32_t rowsel, const int32_t* __restrict__ cc, int C,
+                                           const XT* __restrict__ xq, uint32_t xstride, int kk) {
     using T2 = typename P::T2;
     constexpr int kU = 4;
+    constexpr uint32_t kA = sizeof(T2);
+    ur = __builtin_amdgcn_readfirstlane(ur);
+    C = __builtin_amdgcn_readfirstlane(C);
+    const __amdgpu_buffer_rsrc_t ra =
+        __builtin_amdgcn_make_buffer_rsrc((void*)Sb, (short)0, (int)((uint32_t)C * ur * kA), kBufDword3);
+    const __amdgpu_buffer_rsrc_t rc = __builtin_amdgcn_make_buffer_rsrc((void*)cc, (short)0, C * 4, kBufDword3);
+    const uint32_t va = ((uint32_t)kk * ur + rowsel) * kA, vc = (uint32_t)kk * 4u;
     T2 a0[kU], a1[kU];
     XT b0[kU], b1[kU];
     int ci0[kU], ci1[kU];
     auto load_idx = [&](int g0, int (&ci)[kU]) {
 #pragma unroll
-        for (int u = 0; u < kU; ++u) ci[u] = cc[min(g0 + 4 * u + kk, C - 1)];
+        for (int u = 0; u < kU; ++u) ci[u] = buf_load<int>(rc, vc, (uint32_t)(g0 + 4 * u) * 4u);
     };
     auto load_ops = [&](int g0, const int (&ci)[kU], T2 (&a)[kU], XT (&b)[kU]) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            a[u] = St[(uint32_t)min(g0 + 4 * u + kk, C - 1) * ur];
+            a[u] = buf_load<T2>(ra, va, (uint32_t)(g0 + 4 * u) * ur * kA);
             b[u] = xq[(uint32_t)ci[u] * xstride];
         }
     };
-    auto mma = [&](int g0, const T2 (&a)[kU], const XT (&b)[kU]) {
+    auto mma = [&](const T2 (&a)[kU], const XT (&b)[kU]) {
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            T2 bu = P::cvt(b[u]);
-            if (g0 + 4 * u + kk >= C) bu = czero2<P>();
-            acc.mma(conjif<CJ, P>(a[u]), bu);
-        }
+        for (int u = 0; u < kU; ++u) acc.mma(conjif<CJ, P>(a[u]), P::cvt(b[u]));
     };
     load_idx(0, ci0);
     load_ops(0, ci0, a0, b0);
@@ -855,12 +881,12 @@ __device__ __forceinline__ void couple_run(ACC& acc, const typename P::T2* __res
         load_ops(c0 + 4 * kU, ci1, a1, b1);
         load_idx(c0 + 8 * kU, ci0);
         __builtin_amdgcn_sched_barrier(0);
-        mma(c0, a0, b0);
+        mma(a0, b0);
         if (c0 + 4 * kU >= C) break;
         load_ops(c0 + 8 * kU, ci0, a0, b0);
         load_idx(c0 + 12 * kU, ci1);
         __builtin_amdgcn_sched_barrier(0);
-        mma(c0 + 4 * kU, a1, b1);
+        mma(a1, b1);
     }
 }
 
@@ -900,22 +926,23 @@ __global__ void __launch_bounds__(256, 4)
         Cacc<P, K3> acc;
         acc.zero();
         if (C > 0)
-            couple_run<P, CJ>(acc, S + base[t] + row0 + min(i, nrows - 1), (uint32_t)rt, cc, C, xq, xstride, kk);
+            couple_run<P, CJ>(acc, S + base[t], (uint32_t)rt, (uint32_t)(row0 + min(i, nrows - 1)), cc, C, xq,
+                              xstride, kk);
 #pragma unroll
         for (int v = 0; v < 4; ++v) emit(P::row(kk, v), acc.get(v));
     } else if (nrows > 4) {
         Cemb<P, 8> acc;
         acc.zero();
         if (C > 0)
-            couple_run<P, CJ>(acc, S + base[t] + row0 + min(i & 7, nrows - 1), (uint32_t)rt, cc, C, xq, xstride,
-                              kk);
+            couple_run<P, CJ>(acc, S + base[t], (uint32_t)rt, (uint32_t)(row0 + min(i & 7, nrows - 1)), cc, C,
+                              xq, xstride, kk);
         acc.store(emit);
     } else {
         Cemb<P, 4> acc;
         acc.zero();
         if (C > 0)
-            couple_run<P, CJ>(acc, S + base[t] + row0 + min(i & 3, nrows - 1), (uint32_t)rt, cc, C, xq, xstride,
-                              kk);
+            couple_run<P, CJ>(acc, S + base[t], (uint32_t)rt, (uint32_t)(row0 + min(i & 3, nrows - 1)), cc, C,
+                              xq, xstride, kk);
         acc.store(emit);
     }
 }
