@@ -124,6 +124,29 @@ int32_t nca_compress_level(nca_ctx* ctx, const nca_stack* st, double tol, int32_
                            int64_t* rowpivots, int64_t* colpivots, int64_t xcapacity, int64_t* xoffsets,
                            double* X, int64_t* independent);
 
+/* ---- admissibility partition on the device ------------------------------- */
+typedef struct nca_partition nca_partition;
+/* Near and far cluster pairs of a test and a trial tree, the result of
+ * interactions(tree, isnear) (AdaptiveCrossApproximation.jl, not vendored; the
+ * near CSR feeds assemblenears src/nca/abstractnca.jl:134-178, the far CSR
+ * farinteractions src/admissibility/farinteractions.jl:62-92), computed by a
+ * level-synchronous dual traversal on the device. Trees: 1-based child CSR
+ * (childptr nn+1, children), centers 3 x nn, radii (half size * sqrt(3) for
+ * boxes). wideband = 0: near iff max(diam) > params[0] * dist; wideband = 1:
+ * isnearwideband with params = {-, k, eta_lf, eta_hf, gamma}
+ * (src/admissibility/nearinteractions.jl:83-103); dist = |c_t - c_s| - (r_t + r_s).
+ * A pair is split until it is far or both nodes are leaves. */
+int32_t nca_interactions(nca_ctx* ctx, int64_t nn_test, const int64_t* childptr_test,
+                         const int64_t* children_test, const double* center_test, const double* radius_test,
+                         int64_t nn_trial, const int64_t* childptr_trial, const int64_t* children_trial,
+                         const double* center_trial, const double* radius_trial, int32_t wideband,
+                         const double* params, int64_t* nnear, int64_t* nfar, nca_partition** out);
+/* 1-based CSR by test node, trial nodes of a row increasing: nearptr and
+ * farptr nn_test+1, nearnodes nnear, farnodes nfar. */
+int32_t nca_partition_export(const nca_partition* p, int64_t* nearptr, int64_t* nearnodes, int64_t* farptr,
+                             int64_t* farnodes);
+int32_t nca_partition_destroy(nca_partition* p);
+
 /* ---- Phase 1 on the device ----------------------------------------------- */
 /* Far pivots of every item (a cluster, or a (cluster, direction) pair) among
  * the points of its far field, computed on the device. Replaces the Phase-1

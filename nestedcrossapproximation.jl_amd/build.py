@@ -26,6 +26,7 @@ HIP_UNITS = [
     ("nca_api.hip", []),
     ("nca_compress.hip", ["-ffp-contract=off"]),
     ("nca_matvec.hip", []),
+    ("nca_partition.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -87,7 +87,9 @@ class PetrovGalerkinNCACore:
         self.maxrank = maxrank
         timings = {}
         t0 = time.perf_counter()
-        self.partition = _trees.interactions(tree, eta) if partition is None else partition
+        if partition is None:
+            partition = _trees.interactions_device(tree, self.ctx, eta)
+        self.partition = partition
         self.testfardata, self.trialfardata = _trees.fardata(tree, self.partition)
         timings["partition"] = time.perf_counter() - t0
         self.farmatrix = HIPKernelMatrix(kernel, testpoints, trialpoints, self.ctx)

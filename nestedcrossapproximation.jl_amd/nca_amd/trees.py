@@ -148,6 +148,38 @@ def interactions(tree, eta=1.0):
     return Partition(nearptr, nearnodes, farptr, farnodes)
 
 
+def interactions_device(tree, ctx, eta=1.0, wideband=None):
+    """The partition of ``interactions`` (or of the wideband criterion, with
+    ``wideband`` = an IsNearWideband) computed on the device (nca_interactions);
+    equal to the host traversal's."""
+    from ._native import hip_lib
+    from .runtime import call
+    a, b = tree.testtree, tree.trialtree
+    arr = []
+    for t in (a, b):
+        arr.append([np.ascontiguousarray(t.childptr + 1, np.int64),
+                    np.ascontiguousarray(t.children + 1, np.int64) if len(t.children) else np.zeros(1, np.int64),
+                    np.ascontiguousarray(t.center, np.float64), np.ascontiguousarray(t.radius, np.float64)])
+    if wideband is None:
+        params = np.array([float(eta), 0.0, 0.0, 0.0, 0.0])
+    else:
+        params = np.array([0.0, wideband.k, wideband.etalf, wideband.etahf, wideband.islf.gamma])
+    nnear, nfar = ctypes.c_int64(), ctypes.c_int64()
+    h = ctypes.c_void_p()
+    call(hip_lib().nca_interactions(ctx.handle, a.nnodes, *[_ptr(x) for x in arr[0]], b.nnodes,
+                                    *[_ptr(x) for x in arr[1]], int(wideband is not None), _ptr(params),
+                                    ctypes.byref(nnear), ctypes.byref(nfar), ctypes.byref(h)))
+    try:
+        nearptr = np.empty(a.nnodes + 1, np.int64)
+        farptr = np.empty(a.nnodes + 1, np.int64)
+        nearnodes = np.empty(max(nnear.value, 1), np.int64)
+        farnodes = np.empty(max(nfar.value, 1), np.int64)
+        call(hip_lib().nca_partition_export(h, _ptr(nearptr), _ptr(nearnodes), _ptr(farptr), _ptr(farnodes)))
+    finally:
+        hip_lib().nca_partition_destroy(h)
+    return Partition(nearptr - 1, nearnodes[:nnear.value] - 1, farptr - 1, farnodes[:nfar.value] - 1)
+
+
 class FarData:
     """Far nodes of every node of one tree, CSR (farinteractions.jl:16-19), 0-based."""
 

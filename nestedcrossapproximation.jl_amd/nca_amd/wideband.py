@@ -298,7 +298,9 @@ class DirectionalNCACore(_nca.PetrovGalerkinNCACore):
         self.isnear = isnear
         timings = {}
         t0 = time.perf_counter()
-        self.partition = _dir.interactions(tree, isnear) if partition is None else partition
+        if partition is None:
+            partition = _trees.interactions_device(tree, self.ctx, wideband=isnear)
+        self.partition = partition
         self.testdata, self.trialdata = _dir.fardata(tree, isnear, self.partition)
         timings["partition"] = time.perf_counter() - t0
         self.farmatrix = HIPKernelMatrix(kernel, testpoints, trialpoints, self.ctx)
