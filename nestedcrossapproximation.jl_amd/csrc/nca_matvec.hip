@@ -68,6 +68,7 @@ struct SideDev {
     // the same near lists as merged contiguous point ranges of the other side
     DevBuf<int64_t> nrptr;
     DevBuf<int32_t> nrlo, nrhi;
+    int64_t max_range = 0;  // points of the longest merged range
     // 16-row tiles of the leaves (leaf index, first row) for the MFMA expansion kernel
     int64_t ntiles = 0;
     DevBuf<int32_t> tile_leaf, tile_row0;
@@ -349,10 +350,11 @@ __global__ void __launch_bounds__(kWave)
 
 constexpr double kInvFourPi = 0x1.45f306dc9c883p-4;
 
-// Near-field constants of one product: k and k / (2 pi).
+// Near-field constants of one product: k, k / (2 pi) and k / (4 pi).
 struct NearK {
     double k;
     double k2pi;
+    double k4pi;
 };
 
 // Table of (cos, sin)(n pi / 128), n = 0..255, for the FP64 near field.
@@ -393,24 +395,26 @@ __device__ __forceinline__ double2 helmholtz_mv(const NearK& nk, const double2* 
 }
 
 // 4 pi G(x, y) for the MIXED near field (NEG as above), branch free:
-// coordinate differences, distance and the phase k r / (2 pi) in FP64 (1/r
-// from the FP32 hardware reciprocal square root plus one FP64 Newton step,
-// relative error ~1e-14), the phase reduced to [-1/2, 1/2] revolutions in
-// FP64; sin/cos of the reduced phase and the 1/r scale in FP32 (hardware
-// sin/cos of 2 pi f). Distances below 1e-15 count as zero.
+// coordinate differences, squared distance and the phase k r / (2 pi) in
+// FP64; sin/cos of the phase reduced to [-1/2, 1/2] revolutions and the 1/r
+// scale in FP32 (hardware sin/cos of 2 pi f, hardware reciprocal square
+// root). The squared distance carries 1e-30 (below half an ulp of any
+// r^2 >= 1e-12), so r = 0 needs no clamp; r^2 <= 2e-30 counts as r = 0 and
+// gives 0. The phase takes r0 = r^2 rsq(r^2) plus one Newton correction
+// (r^2 - r0^2) rsq(r^2) / 2, relative error ~1e-14.
 template <bool NEG>
 __device__ __forceinline__ float2 helmholtz_f32(const NearK& nk, double x0, double x1, double x2, double y0,
                                                 double y1, double y2) {
     const double dx = x0 - y0, dy = x1 - y1, dz = x2 - y2;
-    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-    const double r2s = fmax(r2, 1e-30);
-    const float rf = __builtin_amdgcn_rsqf((float)r2s);
-    double ri = (double)rf;
-    ri = ri * fma(-0.5 * r2s * ri, ri, 1.5);
-    const double t = (r2s * ri) * nk.k2pi;
+    const double r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, 1e-30)));
+    const float rf = __builtin_amdgcn_rsqf((float)r2);
+    const double ri = (double)rf;
+    const double r0 = r2 * ri;
+    const double e = fma(-r0, r0, r2);
+    const double t = fma(e, ri * nk.k4pi, r0 * nk.k2pi);
     const float f = (float)(t - rint(t));
     const float c = __builtin_amdgcn_cosf(f), s = __builtin_amdgcn_sinf(f);
-    const float sc = r2 > 1e-30 ? rf : 0.0f;
+    const float sc = r2 > 2e-30 ? rf : 0.0f;
     return make_float2(c * sc, NEG ? -(s * sc) : s * sc);
 }
 
@@ -652,33 +656,62 @@ __device__ __forceinline__ T2 rot16v(T2 v) {
     return v;
 }
 
+// Raw buffer loads: reads past the end of a descriptor's range return 0, so
+// the near-field walk and the coupling pipeline below run without clamps or
+// masks. The varying part of an address goes into the scalar offset, the
+// lane's part into the vector offset (both within the range check).
+constexpr int kBufDword3 = 0x00020000;  // gfx9 buffer descriptor, raw dword access
+
+template <class T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ float2 buf_load<float2>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ double2 buf_load<double2>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ int buf_load<int>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, kBufDword3);
+}
+
 // One near-field step of a tile with H valid-row slots (H = 16, 8 or 4) at
-// source position p. Lane (i, kk) = (lane % 16, lane / 16) evaluates G for row
-// i % H and source p + 4 (i / H) + kk, so a step evaluates 64 entries over
-// 4 * 16 / H sources; MFMA m takes A = the values of group m, rotated down to
-// lanes 0..H-1 of each 16-lane row (the rows >= H of the product are not
-// stored), and B = xs of sources p + 4 m .. p + 4 m + 3. MASK (the last step of
-// a range): sources at or past hi read the rows after them (other points or
-// the zero padding) and contribute B = 0. xp/yp: xs and points at position p.
-template <class P, int H, class ACC, bool EMB, bool NEG, bool MASK>
-__device__ __forceinline__ void near_step(ACC& acc, const NearK& nk, const double2* tab, int kk,
-                                          int ysrc, double x0, double x1, double x2, int p, int hi,
-                                          const double* __restrict__ yp, const typename P::T2* __restrict__ xp,
-                                          uint32_t R) {
+// source position p of a range. Lane (i, kk) = (lane % 16, lane / 16)
+// evaluates G for row i % H and source p + 4 (i / H) + kk, so a step evaluates
+// 64 entries over 4 * 16 / H sources; MFMA m takes A = the values of group m,
+// rotated down to lanes 0..H-1 of each 16-lane row (the rows >= H of the
+// product are not stored), and B = xs of sources p + 4 m .. p + 4 m + 3.
+// rp / rx: the range's points and right-hand sides; sp / sx: the step's byte
+// offsets into them; vp / vx[m]: the lane's. Sources past the range end read
+// points 0 and B = 0 (buffer range check). LAST (the last step of a range):
+// groups that start at or past the end skip their products.
+template <class P, int H, class ACC, bool EMB, bool NEG, bool LAST>
+__device__ __forceinline__ void near_step(ACC& acc, const NearK& nk, const double2* tab, double x0, double x1,
+                                          double x2, int p, int hi, __amdgpu_buffer_rsrc_t rp,
+                                          __amdgpu_buffer_rsrc_t rx, uint32_t sp, uint32_t sx, uint32_t vp,
+                                          const uint32_t (&vx)[16 / H]) {
     using T2 = typename P::T2;
     constexpr int G = 16 / H;
-    const double* ys = yp + 3 * ysrc;
     T2 b[G];
 #pragma unroll
-    for (int m = 0; m < G; ++m) {
-        b[m] = xp[(uint32_t)(4 * m + kk) * R];
-        if (MASK && p + 4 * m + kk >= hi) b[m] = czero2<P>();
-    }
-    const T2 g = P::template eval<NEG>(nk, tab, x0, x1, x2, ys[0], ys[1], ys[2]);
+    for (int m = 0; m < G; ++m) b[m] = buf_load<T2>(rx, vx[m], sx);
+    const double2 yxy = buf_load<double2>(rp, vp, sp);
+    const double yz = buf_load<double>(rp, vp + 16u, sp);
+    const T2 g = P::template eval<NEG>(nk, tab, x0, x1, x2, yxy.x, yxy.y, yz);
     const int i = threadIdx.x & 15;
 #pragma unroll
     for (int m = 0; m < G; ++m) {
-        if (MASK && p + 4 * m >= hi) continue;
+        if (LAST && p + 4 * m >= hi) continue;
         T2 a;
         if constexpr (!EMB) {
             if (m == 0) a = g;
@@ -699,18 +732,26 @@ __device__ __forceinline__ void near_step(ACC& acc, const NearK& nk, const doubl
     }
 }
 
-// Near-field walk of one tile over its merged source ranges: full steps
-// without masks, then one masked step for the rest of the range. Control flow
-// is uniform over the wave. The xs offsets are 32-bit (points x right-hand
-// sides < 2^31).
+// Near-field walk of one tile over its merged source ranges: one buffer
+// descriptor pair per range (points, and the right-hand sides from column q0
+// on), full steps, then one last step whose sources past the range end read
+// zeros. Control flow is uniform over the wave. Byte offsets inside a range
+// are 32-bit (range length x right-hand sides x element size < 2^32).
 template <class P, int H, class ACC, bool EMB, bool NEG>
 __device__ __forceinline__ void near_walk(ACC& acc, const NearK& nk, const double2* tab, int lane,
                                           double x0, double x1, double x2, int64_t e0, int64_t e1, int rot,
                                           const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
-                                          const double* __restrict__ spts, const typename P::T2* __restrict__ xsl,
-                                          uint32_t R) {
-    constexpr int S = 4 * (16 / H);
+                                          const double* __restrict__ spts, const typename P::T2* __restrict__ xsq,
+                                          uint32_t R, int jl) {
+    using T2 = typename P::T2;
+    constexpr int S = 4 * (16 / H), G = 16 / H;
+    constexpr uint32_t kT = sizeof(T2);
     const int kk = lane >> 4, ysrc = 4 * ((lane & 15) / H) + kk;
+    const uint32_t vp = (uint32_t)ysrc * 24u;
+    uint32_t vx[G];
+#pragma unroll
+    for (int m = 0; m < G; ++m) vx[m] = ((uint32_t)(4 * m + kk) * R + (uint32_t)jl) * kT;
+    const uint32_t rowb = R * kT;
     const int nr = (int)(e1 - e0);
     for (int c = 0; c < nr; ++c) {
         // rot: the walk starts at range rot mod nr (tiles that share sources
@@ -719,13 +760,14 @@ __device__ __forceinline__ void near_walk(ACC& acc, const NearK& nk, const doubl
         if (ec >= nr) ec -= nr;
         const int64_t e = e0 + ec;
         const int lo = nrlo[e], hi = nrhi[e];
+        const uint32_t n = (uint32_t)(hi - lo);
+        const __amdgpu_buffer_rsrc_t rp = buf_rsrc(spts + 3 * (int64_t)lo, n * 24u);
+        const __amdgpu_buffer_rsrc_t rx = buf_rsrc(xsq + (int64_t)lo * R, n * rowb);
         int p = lo;
-        for (; p + S <= hi; p += S)
-            near_step<P, H, ACC, EMB, NEG, false>(acc, nk, tab, kk, ysrc, x0, x1, x2, p, hi, spts + 3 * (int64_t)p,
-                                       xsl + (uint32_t)p * R, R);
-        if (p < hi)
-            near_step<P, H, ACC, EMB, NEG, true>(acc, nk, tab, kk, ysrc, x0, x1, x2, p, hi, spts + 3 * (int64_t)p,
-                                      xsl + (uint32_t)p * R, R);
+        uint32_t sp = 0, sx = 0;
+        for (; p + S <= hi; p += S, sp += S * 24u, sx += S * rowb)
+            near_step<P, H, ACC, EMB, NEG, false>(acc, nk, tab, x0, x1, x2, p, hi, rp, rx, sp, sx, vp, vx);
+        if (p < hi) near_step<P, H, ACC, EMB, NEG, true>(acc, nk, tab, x0, x1, x2, p, hi, rp, rx, sp, sx, vp, vx);
     }
 }
 
@@ -765,7 +807,6 @@ __global__ void __launch_bounds__(256, 4)
     const int64_t e0 = nrptr[t], e1 = nrptr[t + 1];
     const int rot = stagger ? w * 7 : 0;
     const int jl = min(j, nq - 1);  // columns past the chunk's live ones read its last (not stored)
-    const T2* xsl = xs + q0 + jl;
     const double2* yt = yh + (int64_t)dcptr[t] * R + q0 + jl;
     const int rt = dcptr[t + 1] - dcptr[t];
     const T2* B = bi >= 0 ? bblocks + boff[bi] : nullptr;
@@ -776,7 +817,8 @@ __global__ void __launch_bounds__(256, 4)
         acc.zero();
         const double* xp = dpts + 3 * (int64_t)(l0 + row0 + min(rsel, nrows - 1));
         near_walk<P, H, std::remove_reference_t<decltype(acc)>, EMB, NEG>(acc, nk, tab, lane, xp[0], xp[1], xp[2], e0,
-                                                                         e1, rot, nrlo, nrhi, spts, xsl, (uint32_t)R);
+                                                                         e1, rot, nrlo, nrhi, spts, xs + q0,
+                                                                         (uint32_t)R, jl);
         acc.scale((typename P::T)kInvFourPi);
         if (B) {
             const int rowc = row0 + min(rsel, nrows - 1);
@@ -813,27 +855,6 @@ __global__ void __launch_bounds__(256, 4)
         tile(acc, std::integral_constant<int, 4>(), std::true_type(), i & 3);
         acc.store(emit);
     }
-}
-
-// Raw buffer loads over one tile's coupling block and column map: reads past
-// the end of the descriptor's range return 0, so the pipeline below runs
-// without clamps or masks (a zero A column meets whatever coefficient the
-// zero index names).
-constexpr int kBufDword3 = 0x00020000;  // gfx9 buffer descriptor, raw dword access
-
-template <class T>
-__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
-template <>
-__device__ __forceinline__ float2 buf_load<float2>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
-template <>
-__device__ __forceinline__ double2 buf_load<double2>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
-}
-template <>
-__device__ __forceinline__ int buf_load<int>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
 }
 
 // Stage 3 forward, one tile: acc += (coupling row of the tile) (source
@@ -1233,7 +1254,7 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
     if (NR == 16 && dst.ntiles > 0) {
         // conj(G) for a negative phase sign, again conjugated for the adjoint
         const bool neg = (h->sign < 0) != CJ;
-        const NearK nk{h->k, h->k * (0.5 / M_PI)};
+        const NearK nk{h->k, h->k * (0.5 / M_PI), h->k * (0.25 / M_PI)};
         const bool k3 = h->near_k3[MIX ? 1 : 0] != 0;
         auto kern = k3 ? (neg ? k_leaf_out_mfma<P, FWD, CJ, true, true> : k_leaf_out_mfma<P, FWD, CJ, true, false>)
                        : (neg ? k_leaf_out_mfma<P, FWD, CJ, false, true> : k_leaf_out_mfma<P, FWD, CJ, false, false>);
@@ -1336,6 +1357,10 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
     NCA_REQUIRE(ldx >= nx && ldy >= ny, NCA_ERR_DIMENSION, "DimensionMismatch: leading dimension too small");
     NCA_REQUIRE(std::max({h->M, h->N, h->test.K, h->trial.K}) * R < ((int64_t)1 << 31), NCA_ERR_LIMIT,
                 "nca_h2_matvec: points (or coefficients) x nrhs must stay below 2^31 (split the right-hand sides)");
+    NCA_REQUIRE(std::max(h->test.max_range, h->trial.max_range) * R * (int64_t)sizeof(double2) < ((int64_t)1 << 32),
+                NCA_ERR_LIMIT,
+                "nca_h2_matvec: a near-field source range x nrhs x 16 bytes must stay below 2^32 (split the "
+                "right-hand sides)");
     if (R == 0 || ny == 0) return;
     const bool mix = h->precision == NCA_PREC_MIXED;
     if (mix) ensure_mixed(h);
@@ -1969,6 +1994,8 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
                         ++o;
                     }
                 });
+                dsd.max_range = 0;
+                for (size_t q = 0; q < mlo.size(); ++q) dsd.max_range = std::max<int64_t>(dsd.max_range, mhi[q] - mlo[q]);
                 dsd.nrptr.upload(mptr, s);
                 dsd.nrlo.upload(mlo, s);
                 dsd.nrhi.upload(mhi, s);
