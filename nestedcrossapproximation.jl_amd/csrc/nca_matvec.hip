@@ -685,33 +685,65 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uin
     return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, kBufDword3);
 }
 
-// One near-field step of a tile with H valid-row slots (H = 16, 8 or 4) at
-// source position p of a range. Lane (i, kk) = (lane % 16, lane / 16)
-// evaluates G for row i % H and source p + 4 (i / H) + kk, so a step evaluates
-// 64 entries over 4 * 16 / H sources; MFMA m takes A = the values of group m,
-// rotated down to lanes 0..H-1 of each 16-lane row (the rows >= H of the
-// product are not stored), and B = xs of sources p + 4 m .. p + 4 m + 3.
-// rp / rx: the range's points and right-hand sides; sp / sx: the step's byte
-// offsets into them; vp / vx[m]: the lane's. Sources past the range end read
-// points 0 and B = 0 (buffer range check). LAST (the last step of a range):
-// groups that start at or past the end skip their products.
-template <class P, int H, class ACC, bool EMB, bool NEG, bool LAST>
-__device__ __forceinline__ void near_step(ACC& acc, const NearK& nk, const double2* tab, double x0, double x1,
-                                          double x2, int p, int hi, __amdgpu_buffer_rsrc_t rp,
-                                          __amdgpu_buffer_rsrc_t rx, uint32_t sp, uint32_t sx, uint32_t vp,
-                                          const uint32_t (&vx)[16 / H]) {
+// Operands of one near-field step: B rows of the G groups, and the lane's
+// source point.
+template <class P, int G>
+struct NearOps {
+    typename P::T2 b[G];
+    double2 yxy;
+    double yz;
+    uint32_t pw;  // G = 1: dword i < 6 of the row's source point in lane i of each 16-lane row
+};
+
+// The double in dwords D, D + 1 of a 16-lane row (DPP row_newbcast).
+template <int D>
+__device__ __forceinline__ double row_double(uint32_t w) {
+    const int lo = __builtin_amdgcn_mov_dpp((int)w, 0x150 + D, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)w, 0x150 + D + 1, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Loads of one step: xs rows (vx[m]: the lane's offset of group m) and the
+// source points, at byte offsets sp / sx into the range's points (rp) and
+// right-hand sides (rx); reads past the range end return 0. G = 1 (every
+// lane of a 16-lane row needs the same source): one dword per lane, vp = the
+// lane's dword of its row's point, broadcast over the row when computing;
+// else the lane's own point at vp.
+template <class P, int G>
+__device__ __forceinline__ void near_load(NearOps<P, G>& o, __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rx,
+                                          uint32_t sp, uint32_t sx, uint32_t vp, const uint32_t (&vx)[G]) {
+#pragma unroll
+    for (int m = 0; m < G; ++m) o.b[m] = buf_load<typename P::T2>(rx, vx[m], sx);
+    if constexpr (G == 1) {
+        o.pw = (uint32_t)buf_load<int>(rp, vp, sp);
+    } else {
+        o.yxy = buf_load<double2>(rp, vp, sp);
+        o.yz = buf_load<double>(rp, vp + 16u, sp);
+    }
+}
+
+// One near-field step of a tile with H valid-row slots (H = 16, 8 or 4).
+// Lane (i, kk) = (lane % 16, lane / 16) evaluates G for row i % H and source
+// 4 (i / H) + kk of the step, so a step evaluates 64 entries over 4 * 16 / H
+// sources; MFMA m takes A = the values of group m, rotated down to lanes
+// 0..H-1 of each 16-lane row (the rows >= H of the product are not stored),
+// and B = xs of sources 4 m .. 4 m + 3 of the step. Sources past the range end
+// read points 0 and B = 0; groups that start there (left: sources of the range
+// from the step on) skip their products.
+template <class P, int H, class ACC, bool EMB, bool NEG>
+__device__ __forceinline__ void near_compute(ACC& acc, const NearK& nk, const double2* tab, double x0, double x1,
+                                             double x2, const NearOps<P, 16 / H>& o, int left) {
     using T2 = typename P::T2;
     constexpr int G = 16 / H;
-    T2 b[G];
-#pragma unroll
-    for (int m = 0; m < G; ++m) b[m] = buf_load<T2>(rx, vx[m], sx);
-    const double2 yxy = buf_load<double2>(rp, vp, sp);
-    const double yz = buf_load<double>(rp, vp + 16u, sp);
-    const T2 g = P::template eval<NEG>(nk, tab, x0, x1, x2, yxy.x, yxy.y, yz);
+    T2 g;
+    if constexpr (G == 1)
+        g = P::template eval<NEG>(nk, tab, x0, x1, x2, row_double<0>(o.pw), row_double<2>(o.pw), row_double<4>(o.pw));
+    else
+        g = P::template eval<NEG>(nk, tab, x0, x1, x2, o.yxy.x, o.yxy.y, o.yz);
     const int i = threadIdx.x & 15;
 #pragma unroll
     for (int m = 0; m < G; ++m) {
-        if (LAST && p + 4 * m >= hi) continue;
+        if (m > 0 && 4 * m >= left) continue;
         T2 a;
         if constexpr (!EMB) {
             if (m == 0) a = g;
@@ -728,15 +760,27 @@ __device__ __forceinline__ void near_step(ACC& acc, const NearK& nk, const doubl
             const T2 rb = m == 0 ? rot16v<12>(g) : (m == 1 ? g : (m == 2 ? rot16v<4>(g) : rot16v<8>(g)));
             a = i < 4 ? ra : rb;
         }
-        acc.mma(a, b[m]);
+        acc.mma(a, o.b[m]);
     }
 }
 
+// Steps per iteration of the MIXED near-field walk for tiles of more than 8
+// rows (their loads are all issued before the first of them computes).
+#ifndef NCA_NEAR_UNROLL
+#define NCA_NEAR_UNROLL 2
+#endif
+
 // Near-field walk of one tile over its merged source ranges: one buffer
 // descriptor pair per range (points, and the right-hand sides from column q0
-// on), full steps, then one last step whose sources past the range end read
-// zeros. Control flow is uniform over the wave. Byte offsets inside a range
-// are 32-bit (range length x right-hand sides x element size < 2^32).
+// on). MIXED tiles of more than 8 rows take NCA_NEAR_UNROLL steps per
+// iteration while the range has sources for all of them (the last may read
+// past its end: zeros), all their loads issued before the first computes;
+// the rest of a range, FP64 tiles and smaller ones take one step per
+// iteration (in FP64 a second register set would cost occupancy: 76 VGPRs,
+// 6 waves). Control flow is uniform over the wave; byte offsets inside
+// a range are 32-bit (range length x right-hand sides x element size < 2^32).
+// rot: the walk starts at range rot mod nr (tiles that share sources then read
+// them at different times).
 template <class P, int H, class ACC, bool EMB, bool NEG>
 __device__ __forceinline__ void near_walk(ACC& acc, const NearK& nk, const double2* tab, int lane,
                                           double x0, double x1, double x2, int64_t e0, int64_t e1, int rot,
@@ -745,17 +789,16 @@ __device__ __forceinline__ void near_walk(ACC& acc, const NearK& nk, const doubl
                                           uint32_t R, int jl) {
     using T2 = typename P::T2;
     constexpr int S = 4 * (16 / H), G = 16 / H;
+    constexpr int U = (H == 16 && !P::kF64) ? NCA_NEAR_UNROLL : 1;
     constexpr uint32_t kT = sizeof(T2);
     const int kk = lane >> 4, ysrc = 4 * ((lane & 15) / H) + kk;
-    const uint32_t vp = (uint32_t)ysrc * 24u;
+    const uint32_t vp = G == 1 ? (uint32_t)kk * 24u + 4u * (uint32_t)min(lane & 15, 5) : (uint32_t)ysrc * 24u;
     uint32_t vx[G];
 #pragma unroll
     for (int m = 0; m < G; ++m) vx[m] = ((uint32_t)(4 * m + kk) * R + (uint32_t)jl) * kT;
     const uint32_t rowb = R * kT;
     const int nr = (int)(e1 - e0);
     for (int c = 0; c < nr; ++c) {
-        // rot: the walk starts at range rot mod nr (tiles that share sources
-        // then read them at different times)
         int ec = c + (nr ? rot % nr : 0);
         if (ec >= nr) ec -= nr;
         const int64_t e = e0 + ec;
@@ -763,11 +806,24 @@ __device__ __forceinline__ void near_walk(ACC& acc, const NearK& nk, const doubl
         const uint32_t n = (uint32_t)(hi - lo);
         const __amdgpu_buffer_rsrc_t rp = buf_rsrc(spts + 3 * (int64_t)lo, n * 24u);
         const __amdgpu_buffer_rsrc_t rx = buf_rsrc(xsq + (int64_t)lo * R, n * rowb);
-        int p = lo;
+        int left = (int)n;
         uint32_t sp = 0, sx = 0;
-        for (; p + S <= hi; p += S, sp += S * 24u, sx += S * rowb)
-            near_step<P, H, ACC, EMB, NEG, false>(acc, nk, tab, x0, x1, x2, p, hi, rp, rx, sp, sx, vp, vx);
-        if (p < hi) near_step<P, H, ACC, EMB, NEG, true>(acc, nk, tab, x0, x1, x2, p, hi, rp, rx, sp, sx, vp, vx);
+        if constexpr (U > 1) {
+            for (; left > (U - 1) * S; left -= U * S, sp += U * S * 24u, sx += U * S * rowb) {
+                NearOps<P, G> o[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    near_load<P, G>(o[u], rp, rx, sp + u * S * 24u, sx + u * S * rowb, vp, vx);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    near_compute<P, H, ACC, EMB, NEG>(acc, nk, tab, x0, x1, x2, o[u], left - u * S);
+            }
+        }
+        for (; left > 0; left -= S, sp += S * 24u, sx += S * rowb) {
+            NearOps<P, G> o;
+            near_load<P, G>(o, rp, rx, sp, sx, vp, vx);
+            near_compute<P, H, ACC, EMB, NEG>(acc, nk, tab, x0, x1, x2, o, left);
+        }
     }
 }
 
