@@ -690,9 +690,8 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uin
 template <class P, int G>
 struct NearOps {
     typename P::T2 b[G];
-    double2 yxy;
-    double yz;
     uint32_t pw;  // G = 1: dword i < 6 of the row's source point in lane i of each 16-lane row
+    double pq;    // G > 1: coordinate j < 3 of the quad's source point in lane j of each quad
 };
 
 // The double in dwords D, D + 1 of a 16-lane row (DPP row_newbcast).
@@ -703,23 +702,28 @@ __device__ __forceinline__ double row_double(uint32_t w) {
     return __hiloint2double(hi, lo);
 }
 
+// Coordinate L of a quad's point held in lane L of the quad (DPP quad_perm).
+template <int L>
+__device__ __forceinline__ double quad_double(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)u, 0x55 * L, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), 0x55 * L, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
 // Loads of one step: xs rows (vx[m]: the lane's offset of group m) and the
 // source points, at byte offsets sp / sx into the range's points (rp) and
-// right-hand sides (rx); reads past the range end return 0. G = 1 (every
-// lane of a 16-lane row needs the same source): one dword per lane, vp = the
-// lane's dword of its row's point, broadcast over the row when computing;
-// else the lane's own point at vp.
+// right-hand sides (rx); reads past the range end return 0. Every lane of a
+// 16-lane row (G = 1) or of a quad (G > 1) needs the same source point, so
+// each lane loads one piece of it (vp: a dword of the row's point, or a
+// coordinate of the quad's) and the computing step broadcasts them.
 template <class P, int G>
 __device__ __forceinline__ void near_load(NearOps<P, G>& o, __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rx,
                                           uint32_t sp, uint32_t sx, uint32_t vp, const uint32_t (&vx)[G]) {
 #pragma unroll
     for (int m = 0; m < G; ++m) o.b[m] = buf_load<typename P::T2>(rx, vx[m], sx);
-    if constexpr (G == 1) {
-        o.pw = (uint32_t)buf_load<int>(rp, vp, sp);
-    } else {
-        o.yxy = buf_load<double2>(rp, vp, sp);
-        o.yz = buf_load<double>(rp, vp + 16u, sp);
-    }
+    if constexpr (G == 1) o.pw = (uint32_t)buf_load<int>(rp, vp, sp);
+    else o.pq = buf_load<double>(rp, vp, sp);
 }
 
 // One near-field step of a tile with H valid-row slots (H = 16, 8 or 4).
@@ -739,7 +743,8 @@ __device__ __forceinline__ void near_compute(ACC& acc, const NearK& nk, const do
     if constexpr (G == 1)
         g = P::template eval<NEG>(nk, tab, x0, x1, x2, row_double<0>(o.pw), row_double<2>(o.pw), row_double<4>(o.pw));
     else
-        g = P::template eval<NEG>(nk, tab, x0, x1, x2, o.yxy.x, o.yxy.y, o.yz);
+        g = P::template eval<NEG>(nk, tab, x0, x1, x2, quad_double<0>(o.pq), quad_double<1>(o.pq),
+                                  quad_double<2>(o.pq));
     const int i = threadIdx.x & 15;
 #pragma unroll
     for (int m = 0; m < G; ++m) {
@@ -792,7 +797,8 @@ __device__ __forceinline__ void near_walk(ACC& acc, const NearK& nk, const doubl
     constexpr int U = (H == 16 && !P::kF64) ? NCA_NEAR_UNROLL : 1;
     constexpr uint32_t kT = sizeof(T2);
     const int kk = lane >> 4, ysrc = 4 * ((lane & 15) / H) + kk;
-    const uint32_t vp = G == 1 ? (uint32_t)kk * 24u + 4u * (uint32_t)min(lane & 15, 5) : (uint32_t)ysrc * 24u;
+    const uint32_t vp = G == 1 ? (uint32_t)kk * 24u + 4u * (uint32_t)min(lane & 15, 5)
+                               : (uint32_t)ysrc * 24u + 8u * (uint32_t)min(lane & 3, 2);
     uint32_t vx[G];
 #pragma unroll
     for (int m = 0; m < G; ++m) vx[m] = ((uint32_t)(4 * m + kk) * R + (uint32_t)jl) * kT;
