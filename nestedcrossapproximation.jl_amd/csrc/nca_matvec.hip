@@ -396,7 +396,8 @@ __device__ __forceinline__ double2 helmholtz_mv(const NearK& nk, const double2* 
 
 // 4 pi G(x, y) for the MIXED near field (NEG as above), branch free:
 // coordinate differences, squared distance and the phase k r / (2 pi) in
-// FP64; sin/cos of the phase reduced to [-1/2, 1/2] revolutions and the 1/r
+// FP64; sin/cos of the phase reduced to [0, 1) revolutions (its fraction,
+// exact in FP64, then rounded to FP32: 6e-8 revolutions at most) and the 1/r
 // scale in FP32 (hardware sin/cos of 2 pi f, hardware reciprocal square
 // root). The squared distance carries 1e-30 (below half an ulp of any
 // r^2 >= 1e-12), so r = 0 needs no clamp; r^2 <= 2e-30 counts as r = 0 and
@@ -412,7 +413,7 @@ __device__ __forceinline__ float2 helmholtz_f32(const NearK& nk, double x0, doub
     const double r0 = r2 * ri;
     const double e = fma(-r0, r0, r2);
     const double t = fma(e, ri * nk.k4pi, r0 * nk.k2pi);
-    const float f = (float)(t - rint(t));
+    const float f = (float)__builtin_amdgcn_fract(t);
     const float c = __builtin_amdgcn_cosf(f), s = __builtin_amdgcn_sinf(f);
     const float sc = r2 > 2e-30 ? rf : 0.0f;
     return make_float2(c * sc, NEG ? -(s * sc) : s * sc);
