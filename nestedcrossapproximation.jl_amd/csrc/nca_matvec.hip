@@ -170,12 +170,13 @@ __global__ void k_to_c32(const double2* __restrict__ in, float2* __restrict__ ou
 
 // out[k*16 + q] = c32(in[k*R + q0 + q]), q < 16: one 16-column chunk of a
 // coefficient vector, compacted.
-__global__ void k_chunk_to_c32(const double2* __restrict__ in, int64_t R, int64_t q0, int64_t K, int nq,
+template <class IT>
+__global__ void k_chunk_to_c32(const IT* __restrict__ in, int64_t R, int64_t q0, int64_t K, int nq,
                                float2* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= K * 16) return;
     const int q = (int)(i & 15);
-    const double2 v = q < nq ? in[(i >> 4) * R + q0 + q] : make_double2(0.0, 0.0);
+    const double2 v = q < nq ? to_d2(in[(i >> 4) * R + q0 + q]) : make_double2(0.0, 0.0);
     out[i] = make_float2((float)v.x, (float)v.y);
 }
 
@@ -211,7 +212,7 @@ template <int NR, bool TR, bool CJ, class BT, class XT>
 __global__ void __launch_bounds__(kWave)
     k_project(const int32_t* __restrict__ bnodes, const int64_t* __restrict__ boff, const BT* __restrict__ blocks,
               const int32_t* __restrict__ lo, const int32_t* __restrict__ hi, const int32_t* __restrict__ cptr,
-              const XT* __restrict__ xs, int64_t R, int64_t q0, double2* __restrict__ xh) {
+              const XT* __restrict__ xs, int64_t R, int64_t q0, BT* __restrict__ xh) {
     const int i = blockIdx.x;
     const int s = bnodes[i];
     const int r = cptr[s + 1] - cptr[s];
@@ -227,9 +228,9 @@ __global__ void __launch_bounds__(kWave)
 #pragma unroll
             for (int q = 0; q < NR; ++q) acc[q] = cfma(b, to_d2(xv[q]), acc[q]);
         }
-        double2* out = xh + (int64_t)(cptr[s] + a) * R + q0;
+        BT* out = xh + (int64_t)(cptr[s] + a) * R + q0;
 #pragma unroll
-        for (int q = 0; q < NR; ++q) out[q] = cadd(out[q], acc[q]);
+        for (int q = 0; q < NR; ++q) store_as(out + q, cadd(to_d2(out[q]), acc[q]));
     }
 }
 
@@ -241,7 +242,7 @@ template <int NR, bool DOWN, bool TR, bool CJ, class BT>
 __global__ void __launch_bounds__(kWave)
     k_transfer(const int32_t* __restrict__ parents, const int64_t* __restrict__ chdptr,
                const int32_t* __restrict__ chds, const int64_t* __restrict__ toff, const BT* __restrict__ blocks,
-               const int32_t* __restrict__ cptr, int64_t pbase, int64_t R, int64_t q0, double2* __restrict__ h) {
+               const int32_t* __restrict__ cptr, int64_t pbase, int64_t R, int64_t q0, BT* __restrict__ h) {
     const int64_t pi = pbase + blockIdx.x;
     const int p = parents[pi];
     const int rp = cptr[p + 1] - cptr[p];
@@ -257,13 +258,13 @@ __global__ void __launch_bounds__(kWave)
             for (int q = 0; q < NR; ++q) acc[q] = make_double2(0.0, 0.0);
             for (int j = 0; j < nin; ++j) {
                 const double2 b = ldc<CJ>(E, TR ? (int64_t)j + (int64_t)a * nin : (int64_t)a + (int64_t)j * nout);
-                const double2* hv = h + (int64_t)(inbase + j) * R + q0;
+                const BT* hv = h + (int64_t)(inbase + j) * R + q0;
 #pragma unroll
-                for (int q = 0; q < NR; ++q) acc[q] = cfma(b, hv[q], acc[q]);
+                for (int q = 0; q < NR; ++q) acc[q] = cfma(b, to_d2(hv[q]), acc[q]);
             }
-            double2* out = h + (int64_t)(outbase + a) * R + q0;
+            BT* out = h + (int64_t)(outbase + a) * R + q0;
 #pragma unroll
-            for (int q = 0; q < NR; ++q) out[q] = cadd(out[q], acc[q]);
+            for (int q = 0; q < NR; ++q) store_as(out + q, cadd(to_d2(out[q]), acc[q]));
         }
     }
 }
@@ -274,8 +275,8 @@ template <int NR, bool CJ, class ST>
 __global__ void __launch_bounds__(kWave)
     k_couple_fwd(const ST* __restrict__ S, const int64_t* __restrict__ base, const int64_t* __restrict__ colbase,
                  const int32_t* __restrict__ ncols, const int32_t* __restrict__ colcoef,
-                 const int32_t* __restrict__ cptr_dst, const double2* __restrict__ xh, int64_t R, int64_t q0,
-                 double2* __restrict__ yh) {
+                 const int32_t* __restrict__ cptr_dst, const ST* __restrict__ xh, int64_t R, int64_t q0,
+                 ST* __restrict__ yh) {
     __shared__ double2 red[kWave];
     const int t = blockIdx.x;
     const int rt = cptr_dst[t + 1] - cptr_dst[t];
@@ -294,9 +295,9 @@ __global__ void __launch_bounds__(kWave)
         if (slot < ncs) {
             for (int col = slot; col < C; col += ncs) {
                 const double2 sv = ldc<CJ>(St, (int64_t)row + (int64_t)col * rt);
-                const double2* xv = xh + (int64_t)cc[col] * R + q0;
+                const ST* xv = xh + (int64_t)cc[col] * R + q0;
 #pragma unroll
-                for (int q = 0; q < NR; ++q) acc[q] = cfma(sv, xv[q], acc[q]);
+                for (int q = 0; q < NR; ++q) acc[q] = cfma(sv, to_d2(xv[q]), acc[q]);
             }
         }
 #pragma unroll
@@ -307,8 +308,8 @@ __global__ void __launch_bounds__(kWave)
             if (slot == 0) {
                 double2 sum = red[lane];
                 for (int c = 1; c < ncs; ++c) sum = cadd(sum, red[lane + c * rows]);
-                double2* out = yh + (int64_t)(cptr_dst[t] + row) * R + q0 + q;
-                *out = cadd(*out, sum);
+                ST* out = yh + (int64_t)(cptr_dst[t] + row) * R + q0 + q;
+                store_as(out, cadd(to_d2(*out), sum));
             }
         }
     }
@@ -320,8 +321,8 @@ template <int NR, bool CJ, class ST>
 __global__ void __launch_bounds__(kWave)
     k_couple_rev(const ST* __restrict__ S, const int64_t* __restrict__ rptr, const int32_t* __restrict__ rtest,
                  const int64_t* __restrict__ roff, const int32_t* __restrict__ cptr_src,
-                 const int32_t* __restrict__ cptr_dst, const double2* __restrict__ xh, int64_t R, int64_t q0,
-                 int nq, double2* __restrict__ yh) {
+                 const int32_t* __restrict__ cptr_dst, const ST* __restrict__ xh, int64_t R, int64_t q0,
+                 int nq, ST* __restrict__ yh) {
     const int s = blockIdx.x;
     const int rs = cptr_dst[s + 1] - cptr_dst[s];
     if (rs == 0) return;
@@ -333,18 +334,18 @@ __global__ void __launch_bounds__(kWave)
             const int t = rtest[e];
             const int rt = cptr_src[t + 1] - cptr_src[t];
             const ST* Sb = S + roff[e] + (int64_t)a * rt;
-            const double2* xv0 = xh + (int64_t)cptr_src[t] * R + q0;
+            const ST* xv0 = xh + (int64_t)cptr_src[t] * R + q0;
             for (int j = 0; j < rt; ++j) {
                 const double2 sv = ldc<CJ>(Sb, j);
 #pragma unroll
                 for (int q = 0; q < NR; ++q)
-                    if (q < nq) acc[q] = cfma(sv, xv0[(int64_t)j * R + q], acc[q]);
+                    if (q < nq) acc[q] = cfma(sv, to_d2(xv0[(int64_t)j * R + q]), acc[q]);
             }
         }
-        double2* out = yh + (int64_t)(cptr_dst[s] + a) * R + q0;
+        ST* out = yh + (int64_t)(cptr_dst[s] + a) * R + q0;
 #pragma unroll
         for (int q = 0; q < NR; ++q)
-            if (q < nq) out[q] = cadd(out[q], acc[q]);
+            if (q < nq) store_as(out + q, cadd(to_d2(out[q]), acc[q]));
     }
 }
 
@@ -428,7 +429,7 @@ __global__ void __launch_bounds__(kWave)
                const int32_t* __restrict__ dhi, const int32_t* __restrict__ dperm, const int32_t* __restrict__ dcptr,
                const double* __restrict__ dpts, const int64_t* __restrict__ nearptr,
                const int32_t* __restrict__ nearnodes, const int32_t* __restrict__ slo, const int32_t* __restrict__ shi,
-               const double* __restrict__ spts, const double2* __restrict__ yh, const XT* __restrict__ xs,
+               const double* __restrict__ spts, const BT* __restrict__ yh, const XT* __restrict__ xs,
                int64_t R, int64_t q0, double kw, int sign, double2 alpha, double2 beta, double2* __restrict__ y,
                int64_t ldy) {
     __shared__ double2 red[kWave];
@@ -450,9 +451,9 @@ __global__ void __launch_bounds__(kWave)
                 for (int j = slot; j < rt; j += ncs) {
                     // FWD: U (nt x r); else V (r x nt) transposed
                     const double2 b = ldc<CJ>(B, FWD ? (int64_t)row + (int64_t)j * nt : (int64_t)j + (int64_t)row * rt);
-                    const double2* hv = yh + (int64_t)(dcptr[t] + j) * R + q0;
+                    const BT* hv = yh + (int64_t)(dcptr[t] + j) * R + q0;
 #pragma unroll
-                    for (int q = 0; q < NR; ++q) acc[q] = cfma(b, hv[q], acc[q]);
+                    for (int q = 0; q < NR; ++q) acc[q] = cfma(b, to_d2(hv[q]), acc[q]);
                 }
             }
             const double x0 = dpts[3 * (l0 + row)], x1 = dpts[3 * (l0 + row) + 1], x2 = dpts[3 * (l0 + row) + 2];
@@ -848,7 +849,7 @@ __global__ void __launch_bounds__(256, 4)
                     const int32_t* __restrict__ dperm, const int32_t* __restrict__ dcptr,
                     const double* __restrict__ dpts, const int64_t* __restrict__ nrptr,
                     const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
-                    const double* __restrict__ spts, const double2* __restrict__ yh,
+                    const double* __restrict__ spts, const typename P::T2* __restrict__ yh,
                     const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, NearK nk,
                     const double2* __restrict__ sctab, double2 alpha, double2 beta, double2* __restrict__ y,
                     int64_t ldy, int order, int stagger, int nq) {
@@ -870,7 +871,7 @@ __global__ void __launch_bounds__(256, 4)
     const int64_t e0 = nrptr[t], e1 = nrptr[t + 1];
     const int rot = stagger ? w * 7 : 0;
     const int jl = min(j, nq - 1);  // columns past the chunk's live ones read its last (not stored)
-    const double2* yt = yh + (int64_t)dcptr[t] * R + q0 + jl;
+    const T2* yt = yh + (int64_t)dcptr[t] * R + q0 + jl;
     const int rt = dcptr[t + 1] - dcptr[t];
     const T2* B = bi >= 0 ? bblocks + boff[bi] : nullptr;
     // near field first (4 pi G), scaled by 1 / (4 pi), then the basis rows
@@ -988,7 +989,7 @@ __global__ void __launch_bounds__(256, 4)
                       const int64_t* __restrict__ colbase, const int32_t* __restrict__ ncols,
                       const int32_t* __restrict__ colcoef, const int32_t* __restrict__ cptr_dst,
                       const XT* __restrict__ xq0, uint32_t xstride, int64_t R, int64_t q0, int nq,
-                      double2* __restrict__ yh, int order) {
+                      typename P::T2* __restrict__ yh, int order) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
     if (w >= ntiles) return;
@@ -1002,9 +1003,9 @@ __global__ void __launch_bounds__(256, 4)
     const auto* xq = xq0 + min(j, nq - 1);
     // every row of every test block with coefficients has one tile (rows
     // without far columns store zeros), so yh needs no clearing
-    double2* out = yh + (int64_t)(cptr_dst[t] + row0) * R + q0 + j;
+    typename P::T2* out = yh + (int64_t)(cptr_dst[t] + row0) * R + q0 + j;
     auto emit = [&](int row, double2 v) {
-        if (row < nrows && j < nq) out[(int64_t)row * R] = v;
+        if (row < nrows && j < nq) store_as(out + (int64_t)row * R, v);
     };
     if (nrows > 8) {
         Cacc<P, K3> acc;
@@ -1050,16 +1051,16 @@ __device__ __forceinline__ void gemm16_tile(Cacc<P, K3>& acc, int lane, int a0, 
 }
 
 // out (+)= the tile: add into out, or store (the caller writes every element once)
-template <class P, bool K3>
-__device__ __forceinline__ void gemm16_store(const Cacc<P, K3>& acc, int lane, int a0, int m, double2* out,
+template <class P, bool K3, class CT>
+__device__ __forceinline__ void gemm16_store(const Cacc<P, K3>& acc, int lane, int a0, int m, CT* out,
                                              int64_t R, int nq, bool add) {
     const int kk = lane >> 4, j = lane & 15;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int row = a0 + P::row(kk, v);
         if (row < m && j < nq) {
-            double2* o = out + (int64_t)row * R + j;
-            *o = add ? cadd(*o, acc.get(v)) : acc.get(v);
+            CT* o = out + (int64_t)row * R + j;
+            store_as(o, add ? cadd(to_d2(*o), acc.get(v)) : acc.get(v));
         }
     }
 }
@@ -1071,7 +1072,8 @@ __global__ void __launch_bounds__(256)
     k_project_mfma(const int32_t* __restrict__ bnodes, const int64_t* __restrict__ boff,
                    const typename P::T2* __restrict__ blocks, int64_t nbasis, const int32_t* __restrict__ lo,
                    const int32_t* __restrict__ hi, const int32_t* __restrict__ cptr,
-                   const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, int nq, double2* __restrict__ xh) {
+                   const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, int nq,
+                   typename P::T2* __restrict__ xh) {
     using T2 = typename P::T2;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
@@ -1102,7 +1104,7 @@ __global__ void __launch_bounds__(256)
     k_transfer_mfma(const int32_t* __restrict__ parents, const int64_t* __restrict__ chdptr,
                     const int32_t* __restrict__ chds, const int64_t* __restrict__ toff,
                     const typename P::T2* __restrict__ blocks, const int32_t* __restrict__ cptr, int64_t pbase,
-                    int64_t np, int64_t R, int64_t q0, int nq, int accumulate, double2* __restrict__ h) {
+                    int64_t np, int64_t R, int64_t q0, int nq, int accumulate, typename P::T2* __restrict__ h) {
     using T2 = typename P::T2;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
@@ -1113,7 +1115,7 @@ __global__ void __launch_bounds__(256)
     if (rp == 0) return;
     const int64_t e0 = chdptr[pi], e1 = chdptr[pi + 1];
     if (DOWN) {
-        const double2* hp = h + (int64_t)cptr[p] * R + q0;
+        const T2* hp = h + (int64_t)cptr[p] * R + q0;
         for (int64_t e = e0; e < e1; ++e) {
             const int c = chds[e];
             const int rc = cptr[c + 1] - cptr[c];
@@ -1137,7 +1139,7 @@ __global__ void __launch_bounds__(256)
                 const int rc = cptr[c + 1] - cptr[c];
                 if (rc == 0) continue;
                 const T2* E = blocks + toff[e];
-                const double2* hc = h + (int64_t)cptr[c] * R + q0;
+                const T2* hc = h + (int64_t)cptr[c] * R + q0;
                 gemm16_tile<P, K3>(
                     acc, lane, a0, rp, rc,
                     [&](int a, int b) { return conjif<CJ, P>(E[TR ? (int64_t)b + (int64_t)a * rc : (int64_t)a + (int64_t)b * rp]); },
@@ -1156,7 +1158,7 @@ __global__ void __launch_bounds__(256)
     k_basis_out(const int32_t* __restrict__ slots, const int32_t* __restrict__ bnodes,
                 const int64_t* __restrict__ boff, const T2* __restrict__ blocks, const int32_t* __restrict__ lo,
                 const int32_t* __restrict__ hi, const int32_t* __restrict__ perm, const int32_t* __restrict__ cptr,
-                const double2* __restrict__ yh, int64_t R, int64_t q0, int64_t nq, double2 alpha,
+                const T2* __restrict__ yh, int64_t R, int64_t q0, int64_t nq, double2 alpha,
                 double2* __restrict__ y, int64_t ldy) {
     const int w = slots[blockIdx.x];
     const int t = bnodes[w];
@@ -1169,7 +1171,7 @@ __global__ void __launch_bounds__(256)
         for (int k = 0; k < r; ++k) {
             const T2 b = B[FWD ? (int64_t)row + (int64_t)k * nt : (int64_t)k + (int64_t)row * r];
             double2 bv = make_double2((double)b.x, CJ ? -(double)b.y : (double)b.y);
-            const double2 v = yh[(int64_t)(c0 + k) * R + q0 + q];
+            const double2 v = to_d2(yh[(int64_t)(c0 + k) * R + q0 + q]);
             acc = cadd(acc, cmul(bv, v));
         }
         double2* o = y + (int64_t)perm[l0 + row] + (q0 + q) * ldy;
@@ -1225,8 +1227,10 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
     SideDev& src = FWD ? h->trial : h->test;
     SideDev& dst = FWD ? h->test : h->trial;
     const T2* xs = O::xs(h);
-    double2* xh = h->ws_xh.p;
-    double2* yh = h->ws_yh.p;
+    // coefficient vectors in the operator's precision (the MIXED tiles are
+    // FP32 products; FP32 storage halves their traffic)
+    T2* xh = reinterpret_cast<T2*>(h->ws_xh.p);
+    T2* yh = reinterpret_cast<T2*>(h->ws_yh.p);
     if (evs) evs->mark(1, st);
     if (src.nbasis > 0) {
         if (NR == 16)
@@ -1257,19 +1261,19 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
     if (FWD) {
         if (NR == 16 && h->f_ntiles > 0) {
             const bool k3 = h->couple_k3[MIX ? 1 : 0] != 0;
-            // MIXED: the source coefficients of this chunk as a compact c32 copy
-            // when the coupling gathers them several times over (couple_x32)
-            if (MIX && h->couple_x32) {
+            // MIXED with more than 16 columns: the source coefficients of this
+            // chunk as a compact copy when the coupling gathers them several
+            // times over (couple_x32)
+            if (MIX && h->couple_x32 && R > 16) {
                 auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true, T2> : k_couple_fwd_mfma<P, CJ, false, T2>;
-                hipLaunchKernelGGL(k_chunk_to_c32, dim3(blocks_for(src.K * 16, 256)), dim3(256), 0, st, xh, R, q0,
-                                   src.K, nq, h->ws_xh32.p);
+                hipLaunchKernelGGL(k_chunk_to_c32<T2>, dim3(blocks_for(src.K * 16, 256)), dim3(256), 0, st, xh, R,
+                                   q0, src.K, nq, h->ws_xh32.p);
                 hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
                                    h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
                                    h->f_colcoef.p, dst.cptr.p, reinterpret_cast<const T2*>(h->ws_xh32.p), 16u, R,
                                    q0, nq, yh, h->couple_xcd);
             } else {
-                auto kern =
-                    k3 ? k_couple_fwd_mfma<P, CJ, true, double2> : k_couple_fwd_mfma<P, CJ, false, double2>;
+                auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true, T2> : k_couple_fwd_mfma<P, CJ, false, T2>;
                 hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
                                    h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
                                    h->f_colcoef.p, dst.cptr.p, xh + q0, (uint32_t)R, R, q0, nq, yh, h->couple_xcd);
@@ -1358,10 +1362,13 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     // and, forward, of yh (coupling) exactly once; the VALU chunks and the
     // reverse coupling add, so they need cleared vectors.
     const bool valu = !h->mfma_small && (R % 16) > 0 && (R % 16) < 4;
-    if (src.K * R > 0 && valu)
-        hipLaunchKernelGGL(k_fill, dim3(blocks_for(src.K * R, 256)), dim3(256), 0, st, h->ws_xh.p, src.K * R);
-    if (dst.K * R > 0 && (valu || !FWD))
-        hipLaunchKernelGGL(k_fill, dim3(blocks_for(dst.K * R, 256)), dim3(256), 0, st, h->ws_yh.p, dst.K * R);
+    // coefficient vectors hold K x R elements of the operator's type; cleared in 16-byte units
+    using CT = typename Ops<MIX>::T2;
+    const int64_t nx16 = (src.K * R * (int64_t)sizeof(CT) + 15) / 16, ny16 = (dst.K * R * (int64_t)sizeof(CT) + 15) / 16;
+    if (nx16 > 0 && valu)
+        hipLaunchKernelGGL(k_fill, dim3(blocks_for(nx16, 256)), dim3(256), 0, st, h->ws_xh.p, nx16);
+    if (ny16 > 0 && (valu || !FWD))
+        hipLaunchKernelGGL(k_fill, dim3(blocks_for(ny16, 256)), dim3(256), 0, st, h->ws_yh.p, ny16);
     // chunks of up to 16 columns on the MFMA kernels; a last chunk of 1 to 3
     // columns on the VALU kernels unless mfma_small is set
     int64_t q0 = 0;
