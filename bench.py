@@ -278,7 +278,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c32 operators and coefficients on MFMA, fp64 distance and phase k*r (MIXED); fp64 path in precisions.fp64",
+        "dtype": "c32 operators and source coefficients on MFMA, fp64 destination coefficients, distance and phase k*r (MIXED); fp64 path in precisions.fp64",
         "data": "synthetic (Fibonacci sphere points, seeded complex Gaussian right-hand sides)",
         "config": {"workload": f"helmholtz_sphere_N{args.n}_k{args.k:g}_tol{args.tol:g}_rhs{R}",
                    "n_points": args.n, "k": args.k, "tol": args.tol, "minvalues": args.minvalues,
@@ -387,7 +387,7 @@ def bench_batch(args):
         "metric": "batched H2 matvec/s (64 independent ellipsoids, 32 RHS each, hipGraph sweep)",
         "value": out["mixed"]["matvec_per_s"], "unit": "matvec/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c32 operators and coefficients on MFMA, fp64 distance and phase k*r (MIXED); fp64 in precisions",
+        "dtype": "c32 operators and source coefficients on MFMA, fp64 destination coefficients, distance and phase k*r (MIXED); fp64 in precisions",
         "data": "synthetic (seeded Fibonacci ellipsoids, complex Gaussian right-hand sides)",
         "config": {"workload": "helmholtz_batch64_ellipsoids_k1_tol1e-4_rhs32", "problems": P, "n_points": N,
                    "sizes_min_max": [int(sizes.min()), int(sizes.max())], "forest_levels": nlev},
@@ -479,7 +479,7 @@ def bench_wideband(args):
         "metric": "directional H2 matvec/s (N=262k sphere, k=16 pi, 8 RHS, mixed vs FP64 at 1e-4); assembly time",
         "value": out["mixed"]["matvec_per_s"], "unit": "matvec/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c32 operators and coefficients on MFMA, fp64 distance and phase k*r (MIXED); fp64 in precisions",
+        "dtype": "c32 operators and source coefficients on MFMA, fp64 destination coefficients, distance and phase k*r (MIXED); fp64 in precisions",
         "data": "synthetic (Fibonacci sphere points, seeded complex Gaussian right-hand sides)",
         "config": {"workload": f"helmholtz_wideband_sphere_N{N}_k{args.k:.4g}_tol{args.tol:g}_rhs{R}",
                    "n_points": N, "k": args.k, "tol": args.tol, "minvalues": args.minvalues,

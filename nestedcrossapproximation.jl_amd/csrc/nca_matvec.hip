@@ -238,11 +238,11 @@ __global__ void __launch_bounds__(kWave)
 // (r_c x r_p) used transposed) and stage 4 (DOWN: children accumulate from the
 // parent; TR = false: E (r_c x r_p), TR = true: E (r_p x r_c) transposed).
 // One wave per parent of the level; a parent's edges run on the same lanes.
-template <int NR, bool DOWN, bool TR, bool CJ, class BT>
+template <int NR, bool DOWN, bool TR, bool CJ, class BT, class CT>
 __global__ void __launch_bounds__(kWave)
     k_transfer(const int32_t* __restrict__ parents, const int64_t* __restrict__ chdptr,
                const int32_t* __restrict__ chds, const int64_t* __restrict__ toff, const BT* __restrict__ blocks,
-               const int32_t* __restrict__ cptr, int64_t pbase, int64_t R, int64_t q0, BT* __restrict__ h) {
+               const int32_t* __restrict__ cptr, int64_t pbase, int64_t R, int64_t q0, CT* __restrict__ h) {
     const int64_t pi = pbase + blockIdx.x;
     const int p = parents[pi];
     const int rp = cptr[p + 1] - cptr[p];
@@ -258,11 +258,11 @@ __global__ void __launch_bounds__(kWave)
             for (int q = 0; q < NR; ++q) acc[q] = make_double2(0.0, 0.0);
             for (int j = 0; j < nin; ++j) {
                 const double2 b = ldc<CJ>(E, TR ? (int64_t)j + (int64_t)a * nin : (int64_t)a + (int64_t)j * nout);
-                const BT* hv = h + (int64_t)(inbase + j) * R + q0;
+                const CT* hv = h + (int64_t)(inbase + j) * R + q0;
 #pragma unroll
                 for (int q = 0; q < NR; ++q) acc[q] = cfma(b, to_d2(hv[q]), acc[q]);
             }
-            BT* out = h + (int64_t)(outbase + a) * R + q0;
+            CT* out = h + (int64_t)(outbase + a) * R + q0;
 #pragma unroll
             for (int q = 0; q < NR; ++q) store_as(out + q, cadd(to_d2(out[q]), acc[q]));
         }
@@ -276,7 +276,7 @@ __global__ void __launch_bounds__(kWave)
     k_couple_fwd(const ST* __restrict__ S, const int64_t* __restrict__ base, const int64_t* __restrict__ colbase,
                  const int32_t* __restrict__ ncols, const int32_t* __restrict__ colcoef,
                  const int32_t* __restrict__ cptr_dst, const ST* __restrict__ xh, int64_t R, int64_t q0,
-                 ST* __restrict__ yh) {
+                 double2* __restrict__ yh) {
     __shared__ double2 red[kWave];
     const int t = blockIdx.x;
     const int rt = cptr_dst[t + 1] - cptr_dst[t];
@@ -308,8 +308,8 @@ __global__ void __launch_bounds__(kWave)
             if (slot == 0) {
                 double2 sum = red[lane];
                 for (int c = 1; c < ncs; ++c) sum = cadd(sum, red[lane + c * rows]);
-                ST* out = yh + (int64_t)(cptr_dst[t] + row) * R + q0 + q;
-                store_as(out, cadd(to_d2(*out), sum));
+                double2* out = yh + (int64_t)(cptr_dst[t] + row) * R + q0 + q;
+                *out = cadd(*out, sum);
             }
         }
     }
@@ -322,7 +322,7 @@ __global__ void __launch_bounds__(kWave)
     k_couple_rev(const ST* __restrict__ S, const int64_t* __restrict__ rptr, const int32_t* __restrict__ rtest,
                  const int64_t* __restrict__ roff, const int32_t* __restrict__ cptr_src,
                  const int32_t* __restrict__ cptr_dst, const ST* __restrict__ xh, int64_t R, int64_t q0,
-                 int nq, ST* __restrict__ yh) {
+                 int nq, double2* __restrict__ yh) {
     const int s = blockIdx.x;
     const int rs = cptr_dst[s + 1] - cptr_dst[s];
     if (rs == 0) return;
@@ -342,10 +342,10 @@ __global__ void __launch_bounds__(kWave)
                     if (q < nq) acc[q] = cfma(sv, to_d2(xv0[(int64_t)j * R + q]), acc[q]);
             }
         }
-        ST* out = yh + (int64_t)(cptr_dst[s] + a) * R + q0;
+        double2* out = yh + (int64_t)(cptr_dst[s] + a) * R + q0;
 #pragma unroll
         for (int q = 0; q < NR; ++q)
-            if (q < nq) store_as(out + q, cadd(to_d2(out[q]), acc[q]));
+            if (q < nq) out[q] = cadd(out[q], acc[q]);
     }
 }
 
@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(kWave)
                const int32_t* __restrict__ dhi, const int32_t* __restrict__ dperm, const int32_t* __restrict__ dcptr,
                const double* __restrict__ dpts, const int64_t* __restrict__ nearptr,
                const int32_t* __restrict__ nearnodes, const int32_t* __restrict__ slo, const int32_t* __restrict__ shi,
-               const double* __restrict__ spts, const BT* __restrict__ yh, const XT* __restrict__ xs,
+               const double* __restrict__ spts, const double2* __restrict__ yh, const XT* __restrict__ xs,
                int64_t R, int64_t q0, double kw, int sign, double2 alpha, double2 beta, double2* __restrict__ y,
                int64_t ldy) {
     __shared__ double2 red[kWave];
@@ -451,7 +451,7 @@ __global__ void __launch_bounds__(kWave)
                 for (int j = slot; j < rt; j += ncs) {
                     // FWD: U (nt x r); else V (r x nt) transposed
                     const double2 b = ldc<CJ>(B, FWD ? (int64_t)row + (int64_t)j * nt : (int64_t)j + (int64_t)row * rt);
-                    const BT* hv = yh + (int64_t)(dcptr[t] + j) * R + q0;
+                    const double2* hv = yh + (int64_t)(dcptr[t] + j) * R + q0;
 #pragma unroll
                     for (int q = 0; q < NR; ++q) acc[q] = cfma(b, to_d2(hv[q]), acc[q]);
                 }
@@ -849,7 +849,7 @@ __global__ void __launch_bounds__(256, 4)
                     const int32_t* __restrict__ dperm, const int32_t* __restrict__ dcptr,
                     const double* __restrict__ dpts, const int64_t* __restrict__ nrptr,
                     const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
-                    const double* __restrict__ spts, const typename P::T2* __restrict__ yh,
+                    const double* __restrict__ spts, const double2* __restrict__ yh,
                     const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, NearK nk,
                     const double2* __restrict__ sctab, double2 alpha, double2 beta, double2* __restrict__ y,
                     int64_t ldy, int order, int stagger, int nq) {
@@ -871,7 +871,7 @@ __global__ void __launch_bounds__(256, 4)
     const int64_t e0 = nrptr[t], e1 = nrptr[t + 1];
     const int rot = stagger ? w * 7 : 0;
     const int jl = min(j, nq - 1);  // columns past the chunk's live ones read its last (not stored)
-    const T2* yt = yh + (int64_t)dcptr[t] * R + q0 + jl;
+    const double2* yt = yh + (int64_t)dcptr[t] * R + q0 + jl;
     const int rt = dcptr[t + 1] - dcptr[t];
     const T2* B = bi >= 0 ? bblocks + boff[bi] : nullptr;
     // near field first (4 pi G), scaled by 1 / (4 pi), then the basis rows
@@ -989,7 +989,7 @@ __global__ void __launch_bounds__(256, 4)
                       const int64_t* __restrict__ colbase, const int32_t* __restrict__ ncols,
                       const int32_t* __restrict__ colcoef, const int32_t* __restrict__ cptr_dst,
                       const XT* __restrict__ xq0, uint32_t xstride, int64_t R, int64_t q0, int nq,
-                      typename P::T2* __restrict__ yh, int order) {
+                      double2* __restrict__ yh, int order) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
     if (w >= ntiles) return;
@@ -1003,7 +1003,7 @@ __global__ void __launch_bounds__(256, 4)
     const auto* xq = xq0 + min(j, nq - 1);
     // every row of every test block with coefficients has one tile (rows
     // without far columns store zeros), so yh needs no clearing
-    typename P::T2* out = yh + (int64_t)(cptr_dst[t] + row0) * R + q0 + j;
+    double2* out = yh + (int64_t)(cptr_dst[t] + row0) * R + q0 + j;
     auto emit = [&](int row, double2 v) {
         if (row < nrows && j < nq) store_as(out + (int64_t)row * R, v);
     };
@@ -1099,12 +1099,12 @@ __global__ void __launch_bounds__(256)
 // transposed; accumulate: += instead, for the child-centric downward pass);
 // stage 4 (DOWN): yh_c += E_c yh_p, E_c r_c x r_p (TR: r_p x r_c transposed).
 // One wave per parent of the level.
-template <class P, bool DOWN, bool TR, bool CJ, bool K3>
+template <class P, bool DOWN, bool TR, bool CJ, bool K3, class CT>
 __global__ void __launch_bounds__(256)
     k_transfer_mfma(const int32_t* __restrict__ parents, const int64_t* __restrict__ chdptr,
                     const int32_t* __restrict__ chds, const int64_t* __restrict__ toff,
                     const typename P::T2* __restrict__ blocks, const int32_t* __restrict__ cptr, int64_t pbase,
-                    int64_t np, int64_t R, int64_t q0, int nq, int accumulate, typename P::T2* __restrict__ h) {
+                    int64_t np, int64_t R, int64_t q0, int nq, int accumulate, CT* __restrict__ h) {
     using T2 = typename P::T2;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
@@ -1115,7 +1115,7 @@ __global__ void __launch_bounds__(256)
     if (rp == 0) return;
     const int64_t e0 = chdptr[pi], e1 = chdptr[pi + 1];
     if (DOWN) {
-        const T2* hp = h + (int64_t)cptr[p] * R + q0;
+        const CT* hp = h + (int64_t)cptr[p] * R + q0;
         for (int64_t e = e0; e < e1; ++e) {
             const int c = chds[e];
             const int rc = cptr[c + 1] - cptr[c];
@@ -1139,7 +1139,7 @@ __global__ void __launch_bounds__(256)
                 const int rc = cptr[c + 1] - cptr[c];
                 if (rc == 0) continue;
                 const T2* E = blocks + toff[e];
-                const T2* hc = h + (int64_t)cptr[c] * R + q0;
+                const CT* hc = h + (int64_t)cptr[c] * R + q0;
                 gemm16_tile<P, K3>(
                     acc, lane, a0, rp, rc,
                     [&](int a, int b) { return conjif<CJ, P>(E[TR ? (int64_t)b + (int64_t)a * rc : (int64_t)a + (int64_t)b * rp]); },
@@ -1158,7 +1158,7 @@ __global__ void __launch_bounds__(256)
     k_basis_out(const int32_t* __restrict__ slots, const int32_t* __restrict__ bnodes,
                 const int64_t* __restrict__ boff, const T2* __restrict__ blocks, const int32_t* __restrict__ lo,
                 const int32_t* __restrict__ hi, const int32_t* __restrict__ perm, const int32_t* __restrict__ cptr,
-                const T2* __restrict__ yh, int64_t R, int64_t q0, int64_t nq, double2 alpha,
+                const double2* __restrict__ yh, int64_t R, int64_t q0, int64_t nq, double2 alpha,
                 double2* __restrict__ y, int64_t ldy) {
     const int w = slots[blockIdx.x];
     const int t = bnodes[w];
@@ -1171,7 +1171,7 @@ __global__ void __launch_bounds__(256)
         for (int k = 0; k < r; ++k) {
             const T2 b = B[FWD ? (int64_t)row + (int64_t)k * nt : (int64_t)k + (int64_t)row * r];
             double2 bv = make_double2((double)b.x, CJ ? -(double)b.y : (double)b.y);
-            const double2 v = to_d2(yh[(int64_t)(c0 + k) * R + q0 + q]);
+            const double2 v = yh[(int64_t)(c0 + k) * R + q0 + q];
             acc = cadd(acc, cmul(bv, v));
         }
         double2* o = y + (int64_t)perm[l0 + row] + (q0 + q) * ldy;
@@ -1227,10 +1227,12 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
     SideDev& src = FWD ? h->trial : h->test;
     SideDev& dst = FWD ? h->test : h->trial;
     const T2* xs = O::xs(h);
-    // coefficient vectors in the operator's precision (the MIXED tiles are
-    // FP32 products; FP32 storage halves their traffic)
+    // source coefficients in the operator's precision (each is one tile's
+    // product; in MIXED an FP32 one, and c32 storage halves their traffic);
+    // destination coefficients in FP64, since they sum the coupling and the
+    // downward contributions of every level
     T2* xh = reinterpret_cast<T2*>(h->ws_xh.p);
-    T2* yh = reinterpret_cast<T2*>(h->ws_yh.p);
+    double2* yh = h->ws_yh.p;
     if (evs) evs->mark(1, st);
     if (src.nbasis > 0) {
         if (NR == 16)
@@ -1248,11 +1250,11 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
         const int64_t np = src.levptr[l + 1] - src.levptr[l];
         if (np == 0) continue;
         if (NR == 16)
-            hipLaunchKernelGGL((k_transfer_mfma<P, false, !FWD, CJ, true>), dim3(blocks_for(np, 4)), dim3(256), 0, st,
-                               src.parents.p, src.chdptr.p, src.chds.p, src.toff.p, O::transfer(src), src.cptr.p,
+            hipLaunchKernelGGL((k_transfer_mfma<P, false, !FWD, CJ, true, T2>), dim3(blocks_for(np, 4)), dim3(256), 0,
+                               st, src.parents.p, src.chdptr.p, src.chds.p, src.toff.p, O::transfer(src), src.cptr.p,
                                src.levptr[l], np, R, q0, nq, 0, xh);
         else
-            hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
+            hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ, T2, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
                                src.parents.p, src.chdptr.p, src.chds.p, src.toff.p, O::transfer(src), src.cptr.p,
                                src.levptr[l], R, q0, xh);
     }
@@ -1296,12 +1298,12 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
             const int64_t nc = dst.dlevptr[l + 1] - dst.dlevptr[l];
             if (nc == 0) continue;
             if (NR == 16)
-                hipLaunchKernelGGL((k_transfer_mfma<P, false, !FWD, CJ, true>), dim3(blocks_for(nc, 4)), dim3(256), 0,
-                                   st, dst.dchild.p, dst.dptr.p, dst.dpar.p, dst.dtoff.p, O::transfer(dst), dst.cptr.p,
-                                   dst.dlevptr[l], nc, R, q0, nq, 1, yh);
+                hipLaunchKernelGGL((k_transfer_mfma<P, false, !FWD, CJ, true, double2>), dim3(blocks_for(nc, 4)),
+                                   dim3(256), 0, st, dst.dchild.p, dst.dptr.p, dst.dpar.p, dst.dtoff.p,
+                                   O::transfer(dst), dst.cptr.p, dst.dlevptr[l], nc, R, q0, nq, 1, yh);
             else
-                hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ, T2>), dim3((unsigned)nc), dim3(kWave), 0, st,
-                                   dst.dchild.p, dst.dptr.p, dst.dpar.p, dst.dtoff.p, O::transfer(dst), dst.cptr.p,
+                hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ, T2, double2>), dim3((unsigned)nc), dim3(kWave), 0,
+                                   st, dst.dchild.p, dst.dptr.p, dst.dpar.p, dst.dtoff.p, O::transfer(dst), dst.cptr.p,
                                    dst.dlevptr[l], R, q0, yh);
         }
     }
@@ -1309,11 +1311,11 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
         const int64_t np = dst.levptr[l + 1] - dst.levptr[l];
         if (np == 0) continue;
         if (NR == 16)
-            hipLaunchKernelGGL((k_transfer_mfma<P, true, !FWD, CJ, true>), dim3(blocks_for(np, 4)), dim3(256), 0, st,
-                               dst.parents.p, dst.chdptr.p, dst.chds.p, dst.toff.p, O::transfer(dst), dst.cptr.p,
-                               dst.levptr[l], np, R, q0, nq, 1, yh);
+            hipLaunchKernelGGL((k_transfer_mfma<P, true, !FWD, CJ, true, double2>), dim3(blocks_for(np, 4)), dim3(256),
+                               0, st, dst.parents.p, dst.chdptr.p, dst.chds.p, dst.toff.p, O::transfer(dst),
+                               dst.cptr.p, dst.levptr[l], np, R, q0, nq, 1, yh);
         else
-            hipLaunchKernelGGL((k_transfer<NR, true, !FWD, CJ, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
+            hipLaunchKernelGGL((k_transfer<NR, true, !FWD, CJ, T2, double2>), dim3((unsigned)np), dim3(kWave), 0, st,
                                dst.parents.p, dst.chdptr.p, dst.chds.p, dst.toff.p, O::transfer(dst), dst.cptr.p,
                                dst.levptr[l], R, q0, yh);
     }
@@ -1362,9 +1364,10 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     // and, forward, of yh (coupling) exactly once; the VALU chunks and the
     // reverse coupling add, so they need cleared vectors.
     const bool valu = !h->mfma_small && (R % 16) > 0 && (R % 16) < 4;
-    // coefficient vectors hold K x R elements of the operator's type; cleared in 16-byte units
+    // source coefficients: K x R elements of the operator's type, destination
+    // coefficients FP64; cleared in 16-byte units
     using CT = typename Ops<MIX>::T2;
-    const int64_t nx16 = (src.K * R * (int64_t)sizeof(CT) + 15) / 16, ny16 = (dst.K * R * (int64_t)sizeof(CT) + 15) / 16;
+    const int64_t nx16 = (src.K * R * (int64_t)sizeof(CT) + 15) / 16, ny16 = dst.K * R;
     if (nx16 > 0 && valu)
         hipLaunchKernelGGL(k_fill, dim3(blocks_for(nx16, 256)), dim3(256), 0, st, h->ws_xh.p, nx16);
     if (ny16 > 0 && (valu || !FWD))

@@ -318,9 +318,10 @@ def test_petrovgalerkinnca_end_to_end(ctx):
 
 @pytest.mark.parametrize("case", ["grid_case", "sphere_case"])
 def test_matvec_mixed_precision(ctx, case, request):
-    """MIXED (complex FP32 operators, coefficient vectors and near field, FP64
-    distance and phase) within 1e-5 relative of the FP64 oracle for op N, T and
-    C at 16, 3 and 8 right-hand sides; FP64 again after switching back."""
+    """MIXED (complex FP32 operators, source coefficients and near field; FP64
+    destination coefficients, distance and phase) within 1e-5 relative of the
+    FP64 oracle for op N, T and C at 16, 3 and 8 right-hand sides; FP64 again
+    after switching back."""
     say(f"mixed precision {case}")
     p, A, ft, fs, oft, ofs = request.getfixturevalue(case)
     W = build_operator(ctx, p, A, ft, fs)
