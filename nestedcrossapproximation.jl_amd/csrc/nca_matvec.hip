@@ -53,6 +53,9 @@ struct SideDev {
     std::vector<int64_t> dlevptr;
     DevBuf<int32_t> dchild, dpar;
     DevBuf<int64_t> dptr, dtoff;
+    // per block: 1 if it has coefficients and no downward transfer writes it
+    // (roots, children of parents without coefficients)
+    DevBuf<uint8_t> orph;
     // c32 copies of the bases and transfers for the MIXED path
     DevBuf<float2> bblocks32, tblocks32;
     // all leaves of the tree (destinations of stage 5) and their basis slot (-1: none)
@@ -108,7 +111,7 @@ struct nca_h2 {
     // when the coupling gathers every coefficient several times)
     int couple_x32 = 1;
     nca::DevBuf<double2> ws_xs, ws_xh, ws_yh, ws_xin, ws_yio;
-    nca::DevBuf<float2> ws_xs32, ws_xh32;
+    nca::DevBuf<float2> ws_xs32, ws_xh32, ws_yc;
     // c32 copy of the coupling blocks for the MIXED path
     nca::DevBuf<float2> S32;
     bool mixed_ready = false;
@@ -982,14 +985,15 @@ __device__ __forceinline__ void couple_run(ACC& acc, const typename P::T2* __res
 // Control flow is uniform over the wave; offsets inside a block and into xh
 // are 32-bit (coefficients x right-hand sides < 2^31). XT: type of the source
 // coefficients (the FP64 vector, or its c32 copy in MIXED).
-template <class P, bool CJ, bool K3, class XT>
+template <class P, bool CJ, bool K3, class XT, class YT>
 __global__ void __launch_bounds__(256, 4)
     k_couple_fwd_mfma(const typename P::T2* __restrict__ S, const int32_t* __restrict__ tile_node,
                       const int32_t* __restrict__ tile_row0, int64_t ntiles, const int64_t* __restrict__ base,
                       const int64_t* __restrict__ colbase, const int32_t* __restrict__ ncols,
                       const int32_t* __restrict__ colcoef, const int32_t* __restrict__ cptr_dst,
                       const XT* __restrict__ xq0, uint32_t xstride, int64_t R, int64_t q0, int nq,
-                      double2* __restrict__ yh, int order) {
+                      YT* __restrict__ yh, double2* __restrict__ yfin, const uint8_t* __restrict__ fin,
+                      int order) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
     if (w >= ntiles) return;
@@ -1003,9 +1007,14 @@ __global__ void __launch_bounds__(256, 4)
     const auto* xq = xq0 + min(j, nq - 1);
     // every row of every test block with coefficients has one tile (rows
     // without far columns store zeros), so yh needs no clearing
-    double2* out = yh + (int64_t)(cptr_dst[t] + row0) * R + q0 + j;
+    // fin[t]: a block no downward transfer writes takes its final value here, in FP64
+    const int64_t o0 = (int64_t)(cptr_dst[t] + row0) * R + q0 + j;
+    const bool tofin = fin && fin[t];
     auto emit = [&](int row, double2 v) {
-        if (row < nrows && j < nq) store_as(out + (int64_t)row * R, v);
+        if (row < nrows && j < nq) {
+            if (tofin) yfin[o0 + (int64_t)row * R] = v;
+            else store_as(yh + o0 + (int64_t)row * R, v);
+        }
     };
     if (nrows > 8) {
         Cacc<P, K3> acc;
@@ -1065,6 +1074,22 @@ __device__ __forceinline__ void gemm16_store(const Cacc<P, K3>& acc, int lane, i
     }
 }
 
+// out = base + the tile (FP64 sum of an FP32 tile result and a c32 base of
+// the same layout), for the rows < m and columns < nq.
+template <class P, bool K3>
+__device__ __forceinline__ void gemm16_store_base(const Cacc<P, K3>& acc, int lane, int a0, int m, double2* out,
+                                                  const float2* base, int64_t R, int nq) {
+    const int kk = lane >> 4, j = lane & 15;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int row = a0 + P::row(kk, v);
+        if (row < m && j < nq) {
+            const int64_t o = (int64_t)row * R + j;
+            out[o] = cadd(to_d2(base[o]), acc.get(v));
+        }
+    }
+}
+
 // Stage 1: xh_s = V_s xs_s (TR: U_s^T). One wave per source basis; it writes
 // every coefficient of its node, so xh needs no clearing.
 template <class P, bool TR, bool CJ, bool K3>
@@ -1098,13 +1123,16 @@ __global__ void __launch_bounds__(256)
 // Stage 2 (up): xh_p = sum_c E_c xh_c, E_c r_p x r_c (TR: r_c x r_p used
 // transposed; accumulate: += instead, for the child-centric downward pass);
 // stage 4 (DOWN): yh_c += E_c yh_p, E_c r_c x r_p (TR: r_p x r_c transposed).
-// One wave per parent of the level.
+// One wave per parent of the level. base (MIXED forward downward pass): the
+// c32 coupling result of the written blocks; with it, out = base + the tile
+// (instead of +=), summed and stored in FP64.
 template <class P, bool DOWN, bool TR, bool CJ, bool K3, class CT>
 __global__ void __launch_bounds__(256)
     k_transfer_mfma(const int32_t* __restrict__ parents, const int64_t* __restrict__ chdptr,
                     const int32_t* __restrict__ chds, const int64_t* __restrict__ toff,
                     const typename P::T2* __restrict__ blocks, const int32_t* __restrict__ cptr, int64_t pbase,
-                    int64_t np, int64_t R, int64_t q0, int nq, int accumulate, CT* __restrict__ h) {
+                    int64_t np, int64_t R, int64_t q0, int nq, int accumulate, CT* __restrict__ h,
+                    const float2* __restrict__ base) {
     using T2 = typename P::T2;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
@@ -1127,6 +1155,13 @@ __global__ void __launch_bounds__(256)
                     acc, lane, a0, rc, rp,
                     [&](int a, int b) { return conjif<CJ, P>(E[TR ? (int64_t)b + (int64_t)a * rp : (int64_t)a + (int64_t)b * rc]); },
                     [&](int b, int q) { return P::cvt(hp[(int64_t)b * R + min(q, nq - 1)]); });
+                if constexpr (std::is_same_v<CT, double2>) {
+                    if (base) {
+                        gemm16_store_base<P, K3>(acc, lane, a0, rc, h + (int64_t)cptr[c] * R + q0,
+                                                 base + (int64_t)cptr[c] * R + q0, R, nq);
+                        continue;
+                    }
+                }
                 gemm16_store<P, K3>(acc, lane, a0, rc, h + (int64_t)cptr[c] * R + q0, R, nq, true);
             }
         }
@@ -1144,6 +1179,13 @@ __global__ void __launch_bounds__(256)
                     acc, lane, a0, rp, rc,
                     [&](int a, int b) { return conjif<CJ, P>(E[TR ? (int64_t)b + (int64_t)a * rc : (int64_t)a + (int64_t)b * rp]); },
                     [&](int b, int q) { return P::cvt(hc[(int64_t)b * R + min(q, nq - 1)]); });
+            }
+            if constexpr (std::is_same_v<CT, double2>) {
+                if (base) {
+                    gemm16_store_base<P, K3>(acc, lane, a0, rp, h + (int64_t)cptr[p] * R + q0,
+                                             base + (int64_t)cptr[p] * R + q0, R, nq);
+                    continue;
+                }
             }
             gemm16_store<P, K3>(acc, lane, a0, rp, h + (int64_t)cptr[p] * R + q0, R, nq, accumulate != 0);
         }
@@ -1252,7 +1294,7 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
         if (NR == 16)
             hipLaunchKernelGGL((k_transfer_mfma<P, false, !FWD, CJ, true, T2>), dim3(blocks_for(np, 4)), dim3(256), 0,
                                st, src.parents.p, src.chdptr.p, src.chds.p, src.toff.p, O::transfer(src), src.cptr.p,
-                               src.levptr[l], np, R, q0, nq, 0, xh);
+                               src.levptr[l], np, R, q0, nq, 0, xh, nullptr);
         else
             hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ, T2, T2>), dim3((unsigned)np), dim3(kWave), 0, st,
                                src.parents.p, src.chdptr.p, src.chds.p, src.toff.p, O::transfer(src), src.cptr.p,
@@ -1260,6 +1302,14 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
     }
     if (evs) evs->mark(3, st);
     const T2* S = O::coupling(h);
+    // MIXED forward products: the coupling result (one FP32 tile per block)
+    // goes to the c32 buffer yc; the downward pass sums it with the parents'
+    // transfers in FP64 and stores yh once per block; the blocks no downward
+    // transfer writes take the coupling result in yh directly
+    constexpr bool kCombine = MIX && FWD && NR == 16;
+    float2* yc = kCombine ? h->ws_yc.p : nullptr;
+    using YT = std::conditional_t<kCombine, float2, double2>;
+    YT* ycoup = kCombine ? reinterpret_cast<YT*>(yc) : reinterpret_cast<YT*>(yh);
     if (FWD) {
         if (NR == 16 && h->f_ntiles > 0) {
             const bool k3 = h->couple_k3[MIX ? 1 : 0] != 0;
@@ -1267,18 +1317,19 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
             // chunk as a compact copy when the coupling gathers them several
             // times over (couple_x32)
             if (MIX && h->couple_x32 && R > 16) {
-                auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true, T2> : k_couple_fwd_mfma<P, CJ, false, T2>;
+                auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true, T2, YT> : k_couple_fwd_mfma<P, CJ, false, T2, YT>;
                 hipLaunchKernelGGL(k_chunk_to_c32<T2>, dim3(blocks_for(src.K * 16, 256)), dim3(256), 0, st, xh, R,
                                    q0, src.K, nq, h->ws_xh32.p);
                 hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
                                    h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
                                    h->f_colcoef.p, dst.cptr.p, reinterpret_cast<const T2*>(h->ws_xh32.p), 16u, R,
-                                   q0, nq, yh, h->couple_xcd);
+                                   q0, nq, ycoup, yh, kCombine ? dst.orph.p : nullptr, h->couple_xcd);
             } else {
-                auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true, T2> : k_couple_fwd_mfma<P, CJ, false, T2>;
+                auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true, T2, YT> : k_couple_fwd_mfma<P, CJ, false, T2, YT>;
                 hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
                                    h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
-                                   h->f_colcoef.p, dst.cptr.p, xh + q0, (uint32_t)R, R, q0, nq, yh, h->couple_xcd);
+                                   h->f_colcoef.p, dst.cptr.p, xh + q0, (uint32_t)R, R, q0, nq, ycoup, yh,
+                                   kCombine ? dst.orph.p : nullptr, h->couple_xcd);
             }
         } else if (NR != 16 && h->coupling->nentries > 0) {
             hipLaunchKernelGGL((k_couple_fwd<NR, CJ, T2>), dim3((unsigned)dst.ne), dim3(kWave), 0, st, S, h->f_base.p,
@@ -1300,7 +1351,7 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
             if (NR == 16)
                 hipLaunchKernelGGL((k_transfer_mfma<P, false, !FWD, CJ, true, double2>), dim3(blocks_for(nc, 4)),
                                    dim3(256), 0, st, dst.dchild.p, dst.dptr.p, dst.dpar.p, dst.dtoff.p,
-                                   O::transfer(dst), dst.cptr.p, dst.dlevptr[l], nc, R, q0, nq, 1, yh);
+                                   O::transfer(dst), dst.cptr.p, dst.dlevptr[l], nc, R, q0, nq, 1, yh, yc);
             else
                 hipLaunchKernelGGL((k_transfer<NR, false, !FWD, CJ, T2, double2>), dim3((unsigned)nc), dim3(kWave), 0,
                                    st, dst.dchild.p, dst.dptr.p, dst.dpar.p, dst.dtoff.p, O::transfer(dst), dst.cptr.p,
@@ -1313,7 +1364,7 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
         if (NR == 16)
             hipLaunchKernelGGL((k_transfer_mfma<P, true, !FWD, CJ, true, double2>), dim3(blocks_for(np, 4)), dim3(256),
                                0, st, dst.parents.p, dst.chdptr.p, dst.chds.p, dst.toff.p, O::transfer(dst),
-                               dst.cptr.p, dst.levptr[l], np, R, q0, nq, 1, yh);
+                               dst.cptr.p, dst.levptr[l], np, R, q0, nq, 1, yh, yc);
         else
             hipLaunchKernelGGL((k_transfer<NR, true, !FWD, CJ, T2, double2>), dim3((unsigned)np), dim3(kWave), 0, st,
                                dst.parents.p, dst.chdptr.p, dst.chds.p, dst.toff.p, O::transfer(dst), dst.cptr.p,
@@ -1412,11 +1463,13 @@ void ensure_workspace(nca_h2* h, int64_t R) {
         h->ws_yh.alloc((size_t)std::max<int64_t>(K * R, 1));
         h->ws_xs.release();
         h->ws_xs32.release();
+        h->ws_yc.release();
         h->ws_rhs = R;
     }
     if (h->precision == NCA_PREC_MIXED) {
         if (!h->ws_xs32.p) h->ws_xs32.alloc((size_t)std::max<int64_t>((np + kNearPad) * h->ws_rhs, 1));
         if (!h->ws_xh32.p) h->ws_xh32.alloc((size_t)std::max<int64_t>(K * 16, 1));
+        if (!h->ws_yc.p) h->ws_yc.alloc((size_t)std::max<int64_t>(K * h->ws_rhs, 1));
     } else {
         if (!h->ws_xs.p) h->ws_xs.alloc((size_t)std::max<int64_t>((np + kNearPad) * h->ws_rhs, 1));
     }
@@ -1612,6 +1665,16 @@ void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, int64_t n
         sd.dptr.upload(dpt, st);
         sd.dtoff.upload(dto, st);
     }
+    // the blocks the downward pass writes: every child with coefficients (by
+    // child), or every child with coefficients of a parent with coefficients
+    auto rank = [&](int64_t e) { return sd.h_cptr[e + 1] - sd.h_cptr[e]; };
+    std::vector<char> written(ne, 0);
+    for (int64_t i = 0; i < npar; ++i)
+        for (int64_t e = cptr0[i]; e < cptr0[i + 1]; ++e)
+            if (rank(ch[e]) > 0 && (multiparent || rank(par[i]) > 0)) written[ch[e]] = 1;
+    std::vector<uint8_t> orph(std::max<int64_t>(ne, 1), 0);
+    for (int64_t e = 0; e < ne; ++e) orph[e] = rank(e) > 0 && !written[e];
+    sd.orph.upload(orph, st);
 }
 
 // Leaves of the side = nodes whose range is not split by any other node: a
