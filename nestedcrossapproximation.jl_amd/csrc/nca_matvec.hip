@@ -370,7 +370,7 @@ constexpr double kPi128Lo = 0x1.1a62633145c07p-60;   // pi / 128 - kPi128Hi
 // 4 pi G(x, y) = (cos, sin)(k r) / r for the FP64 near field (the 1/(4 pi) is
 // applied to the accumulated sum; NEG: the conjugate, for a negative phase
 // sign or the adjoint), branch free, r = 0 gives 0: 1/r from the hardware
-// reciprocal square root refined by two Newton steps; the phase x = k r split
+// reciprocal square root (~2^-23) refined by one Newton step (~2^-46); the phase x = k r split
 // as x = n pi / 128 + y, |y| <= pi / 256 (two-part constant), then sin/cos x =
 // table(n) rotated by sin/cos y from short Taylor polynomials (truncation below
 // 5e-15).
@@ -382,7 +382,6 @@ __device__ __forceinline__ double2 helmholtz_mv(const NearK& nk, const double2* 
     const double r2s = fmax(r2, 1e-300);
     double ri = __builtin_amdgcn_rsq(r2s);
     const double h = 0.5 * r2s;
-    ri = ri * fma(-h * ri, ri, 1.5);
     ri = ri * fma(-h * ri, ri, 1.5);
     const double x = nk.k * (r2s * ri);
     const double q = rint(x * kTabScale);
