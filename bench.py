@@ -257,8 +257,9 @@ def main():
                     "traffic": traffic_of("k_leaf_out_mfma<PrecF32"),
                     "note": "MAC flops only (8 per near pair per RHS); the kernel evaluations "
                             f"({st['near_entries']} per sweep, VALU) are not counted. VALU and MFMA issue "
-                            "serially on a SIMD; the kernel runs at ~53 % VALU + ~37 % MFMA busy "
-                            "(profiles/h2_matvec_sphere1m_rhs16_sq_counters.md). traffic: HBM bytes per launch "
+                            "serially on a SIMD; the kernel runs at ~57 % VALU + ~39 % MFMA busy, i.e. it is "
+                            "issue-bound (profiles/h2_matvec_sphere1m_rhs16_sq_counters.md, "
+                            "profiles/h2_matvec_sphere1m_rhs16_memory_pipeline.md). traffic: HBM bytes per launch "
                             "from FETCH_SIZE x 2 + WRITE_SIZE (profiles/h2_matvec_sphere1m_rhs16_hbm_traffic.md)"}
     else:
         i = names.index(dom)
