@@ -68,7 +68,8 @@ struct SideDev {
     // near lists by node of this side (as destination): ptr (nn+1), nodes of the other side
     DevBuf<int64_t> nearptr;
     DevBuf<int32_t> nearnodes;
-    // the same near lists as merged contiguous point ranges of the other side
+    // the same near lists as merged contiguous point ranges of the other side;
+    // bit 31 of nrhi: the range may hold a point at distance 0 from the node's
     DevBuf<int64_t> nrptr;
     DevBuf<int32_t> nrlo, nrhi;
     int64_t max_range = 0;  // points of the longest merged range
@@ -81,6 +82,8 @@ struct SideDev {
     // host copies of the merged near ranges (to reorder the leaves)
     std::vector<int64_t> h_nrptr;
     std::vector<int32_t> h_nrlo, h_nrhi;
+    // per node: tight bounding box of its points (x, y, z minima, then maxima)
+    std::vector<double> h_box;
     int64_t basis_entries = 0, transfer_entries = 0;
 };
 
@@ -369,17 +372,18 @@ constexpr double kPi128Lo = 0x1.1a62633145c07p-60;   // pi / 128 - kPi128Hi
 
 // 4 pi G(x, y) = (cos, sin)(k r) / r for the FP64 near field (the 1/(4 pi) is
 // applied to the accumulated sum; NEG: the conjugate, for a negative phase
-// sign or the adjoint), branch free, r = 0 gives 0: 1/r from the hardware
+// sign or the adjoint), branch free, r = 0 gives 0 (GUARD; without it the
+// source range is known to stay more than 1e-14 away): 1/r from the hardware
 // reciprocal square root (~2^-23) refined by one Newton step (~2^-46); the phase x = k r split
 // as x = n pi / 128 + y, |y| <= pi / 256 (two-part constant), then sin/cos x =
 // table(n) rotated by sin/cos y from short Taylor polynomials (truncation below
 // 5e-15).
-template <bool NEG>
+template <bool NEG, bool GUARD>
 __device__ __forceinline__ double2 helmholtz_mv(const NearK& nk, const double2* tab, double x0, double x1,
                                                 double x2, double y0, double y1, double y2) {
     const double dx = x0 - y0, dy = x1 - y1, dz = x2 - y2;
     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-    const double r2s = fmax(r2, 1e-300);
+    const double r2s = GUARD ? fmax(r2, 1e-300) : r2;
     double ri = __builtin_amdgcn_rsq(r2s);
     const double h = 0.5 * r2s;
     ri = ri * fma(-h * ri, ri, 1.5);
@@ -393,7 +397,7 @@ __device__ __forceinline__ double2 helmholtz_mv(const NearK& nk, const double2* 
     const double cy = fma(z, fma(z, 0x1.5555555555555p-5, -0.5), 1.0);
     const double c = fma(cs.x, cy, -(cs.y * sy));
     const double s = fma(cs.y, cy, cs.x * sy);
-    const double sc = r2 > 0.0 ? ri : 0.0;
+    const double sc = !GUARD || r2 > 0.0 ? ri : 0.0;
     return make_double2(c * sc, NEG ? -(s * sc) : s * sc);
 }
 
@@ -402,15 +406,16 @@ __device__ __forceinline__ double2 helmholtz_mv(const NearK& nk, const double2* 
 // FP64; sin/cos of the phase reduced to [0, 1) revolutions (its fraction,
 // exact in FP64, then rounded to FP32: 6e-8 revolutions at most) and the 1/r
 // scale in FP32 (hardware sin/cos of 2 pi f, hardware reciprocal square
-// root). The squared distance carries 1e-30 (below half an ulp of any
-// r^2 >= 1e-12), so r = 0 needs no clamp; r^2 <= 2e-30 counts as r = 0 and
-// gives 0. The phase takes r0 = r^2 rsq(r^2) plus one Newton correction
+// root). GUARD (a source range that may hold a point at distance 0): the
+// squared distance carries 1e-30 (below half an ulp of any r^2 >= 1e-12), so
+// r = 0 needs no clamp, and r^2 <= 2e-30 counts as r = 0 and gives 0; other
+// ranges are more than 1e-14 away and skip both. The phase takes r0 = r^2 rsq(r^2) plus one Newton correction
 // (r^2 - r0^2) rsq(r^2) / 2, relative error ~1e-14.
-template <bool NEG>
+template <bool NEG, bool GUARD>
 __device__ __forceinline__ float2 helmholtz_f32(const NearK& nk, double x0, double x1, double x2, double y0,
                                                 double y1, double y2) {
     const double dx = x0 - y0, dy = x1 - y1, dz = x2 - y2;
-    const double r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, 1e-30)));
+    const double r2 = fma(dz, dz, fma(dy, dy, GUARD ? fma(dx, dx, 1e-30) : dx * dx));
     const float rf = __builtin_amdgcn_rsqf((float)r2);
     const double ri = (double)rf;
     const double r0 = r2 * ri;
@@ -418,7 +423,7 @@ __device__ __forceinline__ float2 helmholtz_f32(const NearK& nk, double x0, doub
     const double t = fma(e, ri * nk.k4pi, r0 * nk.k2pi);
     const float f = (float)__builtin_amdgcn_fract(t);
     const float c = __builtin_amdgcn_cosf(f), s = __builtin_amdgcn_sinf(f);
-    const float sc = r2 > 2e-30 ? rf : 0.0f;
+    const float sc = !GUARD || r2 > 2e-30 ? rf : 0.0f;
     return make_float2(c * sc, NEG ? -(s * sc) : s * sc);
 }
 
@@ -510,10 +515,10 @@ struct PrecF64 {
     __device__ static T2 cvt(double2 v) { return v; }
     __device__ static T2 cvt(float2 v) { return make_double2(v.x, v.y); }
     static constexpr bool kTab = true;
-    template <bool NEG>
+    template <bool NEG, bool GUARD>
     __device__ static T2 eval(const NearK& nk, const double2* tab, double x0, double x1, double x2, double y0,
                               double y1, double y2) {
-        return helmholtz_mv<NEG>(nk, tab, x0, x1, x2, y0, y1, y2);
+        return helmholtz_mv<NEG, GUARD>(nk, tab, x0, x1, x2, y0, y1, y2);
     }
 
 };
@@ -528,10 +533,10 @@ struct PrecF32 {
     __device__ static T2 cvt(double2 v) { return make_float2((float)v.x, (float)v.y); }
     __device__ static T2 cvt(float2 v) { return v; }
     static constexpr bool kTab = false;
-    template <bool NEG>
+    template <bool NEG, bool GUARD>
     __device__ static T2 eval(const NearK& nk, const double2*, double x0, double x1, double x2, double y0, double y1,
                               double y2) {
-        return helmholtz_f32<NEG>(nk, x0, x1, x2, y0, y1, y2);
+        return helmholtz_f32<NEG, GUARD>(nk, x0, x1, x2, y0, y1, y2);
     }
 };
 
@@ -738,17 +743,18 @@ __device__ __forceinline__ void near_load(NearOps<P, G>& o, __amdgpu_buffer_rsrc
 // and B = xs of sources 4 m .. 4 m + 3 of the step. Sources past the range end
 // read points 0 and B = 0; groups that start there (left: sources of the range
 // from the step on) skip their products.
-template <class P, int H, class ACC, bool EMB, bool NEG>
+template <class P, int H, class ACC, bool EMB, bool NEG, bool GUARD>
 __device__ __forceinline__ void near_compute(ACC& acc, const NearK& nk, const double2* tab, double x0, double x1,
                                              double x2, const NearOps<P, 16 / H>& o, int left) {
     using T2 = typename P::T2;
     constexpr int G = 16 / H;
     T2 g;
     if constexpr (G == 1)
-        g = P::template eval<NEG>(nk, tab, x0, x1, x2, row_double<0>(o.pw), row_double<2>(o.pw), row_double<4>(o.pw));
+        g = P::template eval<NEG, GUARD>(nk, tab, x0, x1, x2, row_double<0>(o.pw), row_double<2>(o.pw),
+                                         row_double<4>(o.pw));
     else
-        g = P::template eval<NEG>(nk, tab, x0, x1, x2, quad_double<0>(o.pq), quad_double<1>(o.pq),
-                                  quad_double<2>(o.pq));
+        g = P::template eval<NEG, GUARD>(nk, tab, x0, x1, x2, quad_double<0>(o.pq), quad_double<1>(o.pq),
+                                         quad_double<2>(o.pq));
     const int i = threadIdx.x & 15;
 #pragma unroll
     for (int m = 0; m < G; ++m) {
@@ -812,28 +818,35 @@ __device__ __forceinline__ void near_walk(ACC& acc, const NearK& nk, const doubl
         int ec = c + (nr ? rot % nr : 0);
         if (ec >= nr) ec -= nr;
         const int64_t e = e0 + ec;
-        const int lo = nrlo[e], hi = nrhi[e];
+        const int lo = nrlo[e], hr = nrhi[e];
+        const int hi = hr & 0x7fffffff;  // bit 31: the range keeps the zero-distance guard
         const uint32_t n = (uint32_t)(hi - lo);
         const __amdgpu_buffer_rsrc_t rp = buf_rsrc(spts + 3 * (int64_t)lo, n * 24u);
         const __amdgpu_buffer_rsrc_t rx = buf_rsrc(xsq + (int64_t)lo * R, n * rowb);
-        int left = (int)n;
-        uint32_t sp = 0, sx = 0;
-        if constexpr (U > 1) {
-            for (; left > (U - 1) * S; left -= U * S, sp += U * S * 24u, sx += U * S * rowb) {
-                NearOps<P, G> o[U];
+        auto steps = [&](auto guard) {
+            constexpr bool GU = decltype(guard)::value;
+            int left = (int)n;
+            uint32_t sp = 0, sx = 0;
+            if constexpr (U > 1) {
+                for (; left > (U - 1) * S; left -= U * S, sp += U * S * 24u, sx += U * S * rowb) {
+                    NearOps<P, G> o[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                    near_load<P, G>(o[u], rp, rx, sp + u * S * 24u, sx + u * S * rowb, vp, vx);
+                    for (int u = 0; u < U; ++u)
+                        near_load<P, G>(o[u], rp, rx, sp + u * S * 24u, sx + u * S * rowb, vp, vx);
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                    near_compute<P, H, ACC, EMB, NEG>(acc, nk, tab, x0, x1, x2, o[u], left - u * S);
+                    for (int u = 0; u < U; ++u)
+                        near_compute<P, H, ACC, EMB, NEG, GU>(acc, nk, tab, x0, x1, x2, o[u], left - u * S);
+                }
             }
-        }
-        for (; left > 0; left -= S, sp += S * 24u, sx += S * rowb) {
-            NearOps<P, G> o;
-            near_load<P, G>(o, rp, rx, sp, sx, vp, vx);
-            near_compute<P, H, ACC, EMB, NEG>(acc, nk, tab, x0, x1, x2, o, left);
-        }
+            for (; left > 0; left -= S, sp += S * 24u, sx += S * rowb) {
+                NearOps<P, G> o;
+                near_load<P, G>(o, rp, rx, sp, sx, vp, vx);
+                near_compute<P, H, ACC, EMB, NEG, GU>(acc, nk, tab, x0, x1, x2, o, left);
+            }
+        };
+        // FP64: one guarded step loop (a second copy costs the kernel occupancy)
+        if (P::kF64 || hr < 0) steps(std::true_type());
+        else steps(std::false_type());
     }
 }
 
@@ -1509,6 +1522,29 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
 // node (null: one block per node). Bases belong to nodes and cover all
 // coefficients of their node; transfers connect blocks and, with multiparent,
 // a child block may have several parents.
+// Host loop over [0, n) on up to 16 threads (dynamic chunks); f must not throw.
+template <class F>
+void parallel_for(int64_t n, F&& f) {
+    const int64_t nt = std::min<int64_t>(std::max<unsigned>(1u, std::thread::hardware_concurrency()), 16);
+    if (n < 8192 || nt == 1) {
+        for (int64_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    constexpr int64_t kChunk = 512;
+    std::vector<std::thread> pool;
+    for (int64_t k = 0; k < nt; ++k)
+        pool.emplace_back([&] {
+            for (;;) {
+                const int64_t b = next.fetch_add(kChunk);
+                if (b >= n) break;
+                const int64_t e = std::min(n, b + kChunk);
+                for (int64_t i = b; i < e; ++i) f(i);
+            }
+        });
+    for (auto& t : pool) t.join();
+}
+
 void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, int64_t ne, const int64_t* dirptr,
                 bool multiparent, const int64_t* perm, const int64_t* lo, const int64_t* hi, const int64_t* cptr,
                 const nca_points* pts, int64_t nbasis, const int64_t* bnodes, const int64_t* boff,
@@ -1555,11 +1591,24 @@ void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, int64_t n
         for (int64_t e = 0; e < ne; ++e) sd.h_enode[e] = (int32_t)e;
         sd.h_ncptr = sd.h_cptr;
     }
-    // kNearPad zero points after the last one: near-field steps may read up to
-    // 15 positions past a range end (their contributions are zeroed)
+    // kNearPad zero points after the last one (slack: the near-field loads are
+    // range-checked and read zeros past a range end)
     std::vector<double> pp(3 * (npts + kNearPad), 0.0);
     for (int64_t i = 0; i < npts; ++i)
         for (int d = 0; d < 3; ++d) pp[3 * i + d] = pts->host[3 * sd.h_perm[i] + d];
+    sd.h_box.assign(6 * (size_t)nn, 0.0);
+    parallel_for(nn, [&](int64_t t) {
+        double* b = &sd.h_box[6 * t];
+        for (int d = 0; d < 3; ++d) {
+            b[d] = HUGE_VAL;
+            b[3 + d] = -HUGE_VAL;
+        }
+        for (int64_t i = sd.h_lo[t]; i < sd.h_hi[t]; ++i)
+            for (int d = 0; d < 3; ++d) {
+                b[d] = std::min(b[d], pp[3 * i + d]);
+                b[3 + d] = std::max(b[3 + d], pp[3 * i + d]);
+            }
+    });
     sd.perm.upload(sd.h_perm, st);
     {
         std::vector<int32_t> ip(npts);
@@ -1757,28 +1806,6 @@ void order_leaves(SideDev& sd, int mode, hipStream_t st) {
     sd.tile_row0.upload(tr, st);
 }
 
-// Host loop over [0, n) on up to 16 threads (dynamic chunks); f must not throw.
-template <class F>
-void parallel_for(int64_t n, F&& f) {
-    const int64_t nt = std::min<int64_t>(std::max<unsigned>(1u, std::thread::hardware_concurrency()), 16);
-    if (n < 8192 || nt == 1) {
-        for (int64_t i = 0; i < n; ++i) f(i);
-        return;
-    }
-    std::atomic<int64_t> next{0};
-    constexpr int64_t kChunk = 512;
-    std::vector<std::thread> pool;
-    for (int64_t k = 0; k < nt; ++k)
-        pool.emplace_back([&] {
-            for (;;) {
-                const int64_t b = next.fetch_add(kChunk);
-                if (b >= n) break;
-                const int64_t e = std::min(n, b + kChunk);
-                for (int64_t i = b; i < e; ++i) f(i);
-            }
-        });
-    for (auto& t : pool) t.join();
-}
 
 // Host phase timing of nca_h2_create (environment NCA_CREATE_TIMING): each
 // mark prints the time since the previous one to stderr.
@@ -2102,17 +2129,33 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
             auto merge = [](SideDev& dsd, const SideDev& ssd, const std::vector<int64_t>& ptr,
                             const std::vector<int32_t>& nodes, hipStream_t s) {
                 // per node: its source ranges sorted and merged where contiguous
+                // and alike: a range keeps the zero-distance guard unless the
+                // point boxes are more than 1e-14 apart (then every pair has
+                // r^2 > 1e-28, far above the guard's 2e-30)
                 const int64_t nn = dsd.nn;
                 std::vector<int64_t> cnt(nn, 0);
-                std::vector<std::vector<std::pair<int32_t, int32_t>>> per(nn);
+                struct Rg { int32_t lo, hi; bool guard; };
+                std::vector<std::vector<Rg>> per(nn);
+                auto box_d2 = [](const double* a, const double* b) {
+                    double d2 = 0.0;
+                    for (int d = 0; d < 3; ++d) {
+                        const double g = std::max({0.0, a[d] - b[3 + d], b[d] - a[3 + d]});
+                        d2 += g * g;
+                    }
+                    return d2;
+                };
                 parallel_for(nn, [&](int64_t t) {
                     auto& rg = per[t];
-                    for (int64_t i = ptr[t]; i < ptr[t + 1]; ++i)
-                        if (ssd.h_hi[nodes[i]] > ssd.h_lo[nodes[i]]) rg.emplace_back(ssd.h_lo[nodes[i]], ssd.h_hi[nodes[i]]);
-                    std::sort(rg.begin(), rg.end());
+                    for (int64_t i = ptr[t]; i < ptr[t + 1]; ++i) {
+                        const int32_t q = nodes[i];
+                        if (ssd.h_hi[q] > ssd.h_lo[q])
+                            rg.push_back({ssd.h_lo[q], ssd.h_hi[q],
+                                          !(box_d2(&dsd.h_box[6 * t], &ssd.h_box[6 * (size_t)q]) > 1e-28)});
+                    }
+                    std::sort(rg.begin(), rg.end(), [](const Rg& a, const Rg& b) { return a.lo < b.lo; });
                     size_t k = 0;
                     for (size_t q = 0; q < rg.size(); ++q) {
-                        if (k > 0 && rg[k - 1].second == rg[q].first) rg[k - 1].second = rg[q].second;
+                        if (k > 0 && rg[k - 1].hi == rg[q].lo && rg[k - 1].guard == rg[q].guard) rg[k - 1].hi = rg[q].hi;
                         else rg[k++] = rg[q];
                     }
                     rg.resize(k);
@@ -2120,12 +2163,13 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
                 });
                 std::vector<int64_t> mptr(nn + 1, 0);
                 for (int64_t t = 0; t < nn; ++t) mptr[t + 1] = mptr[t] + cnt[t];
-                std::vector<int32_t> mlo((size_t)mptr[nn]), mhi((size_t)mptr[nn]);
+                std::vector<int32_t> mlo((size_t)mptr[nn]), mhi((size_t)mptr[nn]), mhid((size_t)mptr[nn]);
                 parallel_for(nn, [&](int64_t t) {
                     int64_t o = mptr[t];
                     for (auto& r : per[t]) {
-                        mlo[o] = r.first;
-                        mhi[o] = r.second;
+                        mlo[o] = r.lo;
+                        mhi[o] = r.hi;
+                        mhid[o] = (int32_t)((uint32_t)r.hi | (r.guard ? 0x80000000u : 0u));
                         ++o;
                     }
                 });
@@ -2133,7 +2177,7 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
                 for (size_t q = 0; q < mlo.size(); ++q) dsd.max_range = std::max<int64_t>(dsd.max_range, mhi[q] - mlo[q]);
                 dsd.nrptr.upload(mptr, s);
                 dsd.nrlo.upload(mlo, s);
-                dsd.nrhi.upload(mhi, s);
+                dsd.nrhi.upload(mhid, s);
                 dsd.h_nrptr = std::move(mptr);
                 dsd.h_nrlo = std::move(mlo);
                 dsd.h_nrhi = std::move(mhi);

@@ -1,7 +1,8 @@
 """Degenerate and ragged inputs through the C-ABI: a single point, problems
 with no admissible pair (near field only), test and trial sets of different
-sizes far apart (far field only at the root), zero right-hand sides, and a
-wideband problem without any direction. Each product is checked against the
+sizes far apart (far field only at the root), zero right-hand sides, a
+wideband problem without any direction, and coincident points under different
+test and trial trees. Each product is checked against the
 oracle's (dense direct summation where the operator is exact)."""
 import math
 
@@ -97,3 +98,30 @@ def test_wideband_without_directions():
     assert core.testdata.ndiridx == 0
     x = geometry.complex_gaussian(np.random.default_rng(2), (200, 4))
     assert relerr(NC.H2Workspace(core) @ x, O.dense_rows(k, 1, pts, pts, np.arange(200), x)) <= 1e-12
+
+
+def test_coincident_points_in_different_trees():
+    """The same 2,000 points on both sides under different trees (2^n tree for
+    the test side, 2-means for the trial side): coincident points meet in leaf
+    pairs other than a leaf with itself, where the near field must still give
+    G = 0 at distance 0. FP64 within 1e-12 and MIXED within 1e-5 of the
+    oracle's product on the same factors, op N and C."""
+    say("coincident points, different trees")
+    pts = geometry.fibonacci_sphere(2000)
+    tt = nca_amd.TwoNTree(pts, minvalues=32)
+    st = nca_amd.KMeansTree(pts, 2, 24)
+    cmp = C.BatchedBottomUp(C.GeometricPivoting(), C.ACA(1e-4))
+    core = nca_amd.PetrovGalerkinNCACore(nca_amd.HelmholtzKernel(2.0), pts, pts, nca_amd.BlockTree(tt, st), cmp,
+                                         cmp)
+    assert core.stats()["near_pairs"] > 0
+    off, blocks = core.couplingstore.download()
+    H = O.OracleH2(core.desc, 2.0, 1, pts, pts, off, blocks)
+    A = NC.H2Workspace(core)
+    rng = np.random.default_rng(17)
+    for prec, bound in ((nca_amd.NCA_PREC_FP64, 1e-12), (nca_amd.NCA_PREC_MIXED, 1e-5)):
+        core.set_precision(prec)
+        for R in (16, 5):
+            x = geometry.complex_gaussian(rng, (2000, R))
+            assert relerr(A @ x, H.matvec(x)) <= bound, (prec, R)
+            assert relerr(NC.adjoint(A) @ x, H.matvec(x, op=2)) <= bound, (prec, R)
+    core.set_precision(nca_amd.NCA_PREC_FP64)
