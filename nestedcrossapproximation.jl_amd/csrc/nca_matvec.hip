@@ -110,9 +110,13 @@ struct nca_h2 {
     int64_t ws_rhs = 0;
     // run a last chunk of 1..3 right-hand sides on the MFMA kernels too
     int mfma_small = 0;
-    // MIXED coupling reads a c32 copy of the source coefficients (worth it
-    // when the coupling gathers every coefficient several times)
+    // MIXED coupling with more than 16 right-hand sides reads a compact copy
+    // of the chunk's source coefficients (worth it when the coupling gathers
+    // every coefficient several times)
     int couple_x32 = 1;
+    // ws_xh: source coefficients in the operator's element type (c32 in
+    // MIXED, in the same allocation); ws_yh: destination coefficients, FP64;
+    // ws_yc: the c32 coupling results of MIXED forward products
     nca::DevBuf<double2> ws_xs, ws_xh, ws_yh, ws_xin, ws_yio;
     nca::DevBuf<float2> ws_xs32, ws_xh32, ws_yc;
     // c32 copy of the coupling blocks for the MIXED path
@@ -1018,7 +1022,7 @@ __global__ void __launch_bounds__(256, 4)
     const int32_t* cc = colcoef + colbase[t];
     const auto* xq = xq0 + min(j, nq - 1);
     // every row of every test block with coefficients has one tile (rows
-    // without far columns store zeros), so yh needs no clearing
+    // without far columns store zeros), so the output needs no clearing
     // fin[t]: a block no downward transfer writes takes its final value here, in FP64
     const int64_t o0 = (int64_t)(cptr_dst[t] + row0) * R + q0 + j;
     const bool tofin = fin && fin[t];
@@ -1424,8 +1428,9 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
                            dim3(blocks_for(src.npts, 64), blocks_for(R, 16)), dim3(256), 0, st, x, ldx, src.iperm.p,
                            src.npts, R, R, Ops<MIX>::xs(h));
     // The MFMA chunks write every coefficient of xh (projection, upward pass)
-    // and, forward, of yh (coupling) exactly once; the VALU chunks and the
-    // reverse coupling add, so they need cleared vectors.
+    // and, forward, of yh (coupling; in MIXED the coupling and the downward
+    // pass) exactly once; the VALU chunks and the reverse coupling add, so they
+    // need cleared vectors.
     const bool valu = !h->mfma_small && (R % 16) > 0 && (R % 16) < 4;
     // source coefficients: K x R elements of the operator's type, destination
     // coefficients FP64; cleared in 16-byte units
