@@ -320,8 +320,8 @@ def test_petrovgalerkinnca_end_to_end(ctx):
 def test_matvec_mixed_precision(ctx, case, request):
     """MIXED (complex FP32 operators, source coefficients and near field; FP64
     destination coefficients, distance and phase) within 1e-5 relative of the
-    FP64 oracle for op N, T and C at 16, 3 and 8 right-hand sides; FP64 again
-    after switching back."""
+    FP64 oracle for op N, T and C at 16, 3, 8, 19 and 40 right-hand sides (the
+    last two in several column chunks); FP64 again after switching back."""
     say(f"mixed precision {case}")
     p, A, ft, fs, oft, ofs = request.getfixturevalue(case)
     W = build_operator(ctx, p, A, ft, fs)
@@ -330,7 +330,7 @@ def test_matvec_mixed_precision(ctx, case, request):
     N, M = p.trialpts.shape[0], p.testpts.shape[0]
     W.core_set = W.core.set_precision if hasattr(W.core, "set_precision") else None
     NC.PetrovGalerkinNCACore.set_precision(W.core, nca_amd.NCA_PREC_MIXED)
-    for R in (16, 3, 8):
+    for R in (16, 3, 8, 19, 40):
         x = geometry.complex_gaussian(rng, (N, R))
         u = geometry.complex_gaussian(rng, (M, R))
         assert relerr(W @ x, H.matvec(x)) <= 1e-5

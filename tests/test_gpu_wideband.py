@@ -6,7 +6,8 @@ blocks and Phase-1 pivots; the directional product within 1e-12 relative of
 the oracle's stage-by-stage restatement (oracle.OracleDH2) with the same
 factors, for op N, T and C; the error against dense direct summation below
 10 tol (test/nca/test_petrovgalerkinnca.jl:104-153, wideband with
-etalf = etahf = gamma = 1); MIXED within 1e-5 of FP64."""
+etalf = etahf = gamma = 1); MIXED within 1e-5 of FP64 at 16 and 19
+right-hand sides."""
 import math
 
 import numpy as np
@@ -116,12 +117,16 @@ def test_directional_direct_sum_and_mixed(case):
     err = relerr(y64[rows], yd)
     say(f"  direct-sum error {err:.2e}")
     assert err < 10 * 1e-4
+    x19 = rng.standard_normal((n, 19)) + 1j * rng.standard_normal((n, 19))
+    y19 = A @ x19
     core.set_precision(_abi.NCA_PREC_MIXED)
     try:
         ym = A @ x
+        ym19 = A @ x19  # two column chunks (16 + 3)
     finally:
         core.set_precision(_abi.NCA_PREC_FP64)
     assert relerr(ym, y64) < 1e-5
+    assert relerr(ym19, y19) < 1e-5
 
 
 def test_directional_graph_replays():
