@@ -166,6 +166,17 @@ int32_t nca_farpivots(nca_ctx* ctx, const nca_points* other, int64_t nitems, con
                       const int64_t* other_lo, const int64_t* other_hi, const int64_t* other_perm,
                       int64_t maxrank, double weight, double oversample, int64_t* pivptr, int64_t pivcap,
                       int64_t* piv);
+/* The same selection with the far fields built on the device, as farfield
+ * (farinteractions.jl:38-60) builds them: the far nodes of a node, then those
+ * of its parent and so on to the root. farptr/farnodes: 1-based CSR of the far
+ * nodes of every node of the self tree (FarData, farinteractions.jl:62-92);
+ * parent: 1-based parent of every node (0 at a root). Same outputs and limits
+ * (a candidate pool above 4096 points is reported after the launch). */
+int32_t nca_farpivots_tree(nca_ctx* ctx, const nca_points* other, int64_t nitems, const double* centers,
+                           const int64_t* nvals, const int64_t* parent, const int64_t* farptr,
+                           const int64_t* farnodes, int64_t nn_other, const int64_t* other_lo,
+                           const int64_t* other_hi, const int64_t* other_perm, int64_t maxrank, double weight,
+                           double oversample, int64_t* pivptr, int64_t pivcap, int64_t* piv);
 
 /* ---- seam 3: coupling matrices ----------------------------------------- */
 /* Replaces assemble_couplingstore(farmatrix, testpivots, trialpivots,

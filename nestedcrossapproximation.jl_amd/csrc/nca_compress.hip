@@ -606,12 +606,21 @@ __device__ __forceinline__ void p1_best(double& v, int& q, double* redv, int* re
     }
 }
 
+// TREE: the item's far field is built here, in the order of farfield
+// (farinteractions.jl:38-60): the far nodes of the item, then those of its
+// parent, and so on to the root (ffptr / ffnodes: the far lists by node,
+// parent: -1 at a root); else ffptr / ffnodes hold the far field of every
+// item. A candidate pool above kP1Pool sets *err (the host checked it up
+// front for explicit far fields).
+template <bool TREE>
 __global__ void __launch_bounds__(kP1Threads)
     k_farpivots(const double* __restrict__ pts, const int32_t* __restrict__ olo, const int32_t* __restrict__ ohi,
                 const int32_t* __restrict__ operm, const double* __restrict__ centers,
                 const int64_t* __restrict__ ffptr, const int32_t* __restrict__ ffnodes,
-                const int64_t* __restrict__ pivptr, double weight, int64_t* __restrict__ piv) {
+                const int32_t* __restrict__ parent, const int64_t* __restrict__ pivptr, double weight,
+                int64_t* __restrict__ piv, int32_t* __restrict__ err) {
     __shared__ int32_t scan[kP1MaxFar + 1];
+    __shared__ int32_t fs[TREE ? kP1MaxFar : 1];
     __shared__ int64_t red[4];
     __shared__ double redv[4];
     __shared__ int redq[4];
@@ -621,8 +630,22 @@ __global__ void __launch_bounds__(kP1Threads)
     const int tid = threadIdx.x;
     const int64_t n = pivptr[it + 1] - pivptr[it];
     if (n == 0) return;
-    const int32_t* F = ffnodes + ffptr[it];
-    const int nf = (int)(ffptr[it + 1] - ffptr[it]);
+    const int32_t* F;
+    int nf;
+    if constexpr (TREE) {
+        int m = 0;
+        for (int64_t a = it; a >= 0; a = parent[a]) {
+            const int64_t b = ffptr[a], e = ffptr[a + 1];
+            for (int64_t k = tid; k < e - b; k += kP1Threads) fs[m + k] = ffnodes[b + k];
+            m += (int)(e - b);
+        }
+        __syncthreads();
+        F = fs;
+        nf = m;
+    } else {
+        F = ffnodes + ffptr[it];
+        nf = (int)(ffptr[it + 1] - ffptr[it]);
+    }
     auto sz = [&](int f) { return (int64_t)(ohi[F[f]] - olo[F[f]]); };
     int64_t tot = 0;
     for (int f = tid; f < nf; f += kP1Threads) tot += sz(f);
@@ -635,6 +658,10 @@ __global__ void __launch_bounds__(kP1Threads)
         pool = p1_sum(c, red);
         if (pool >= n || pool >= total) break;
         per *= 2;
+    }
+    if (pool > kP1Pool) {
+        if (tid == 0) atomicOr(err, 1);
+        return;
     }
     // exclusive scan of the per-node candidate counts (contiguous chunks per thread)
     const int chunk = (nf + kP1Threads - 1) / kP1Threads;
@@ -1005,7 +1032,7 @@ int32_t nca_farpivots(nca_ctx* ctx, const nca_points* other, int64_t nitems, con
         for (int64_t t = 0; t < nitems; ++t) pivptr[t + 1] = pivptr[t] + cnt[t];
         NCA_REQUIRE(pivptr[nitems] <= pivcap, NCA_ERR_LIMIT,
                     "nca_farpivots: pivot capacity too small (" + std::to_string(pivptr[nitems]) + " needed)");
-        DevBuf<int32_t> dlo, dhi, dperm, dfn;
+        DevBuf<int32_t> dlo, dhi, dperm, dfn, derr(1);
         DevBuf<int64_t> dfp, dptr, dpiv;
         DevBuf<double> dctr;
         dlo.upload(lo, st);
@@ -1017,10 +1044,120 @@ int32_t nca_farpivots(nca_ctx* ctx, const nca_points* other, int64_t nitems, con
         dptr.upload(pp, st);
         dctr.upload(centers, (size_t)(3 * nitems), st);
         dpiv.alloc((size_t)std::max<int64_t>(pivptr[nitems], 1));
+        const int32_t zero = 0;
+        derr.upload(&zero, 1, st);
         if (nitems > 0)
-            hipLaunchKernelGGL(k_farpivots, dim3((unsigned)nitems), dim3(kP1Threads), 0, st, other->xyz.p, dlo.p,
-                               dhi.p, dperm.p, dctr.p, dfp.p, dfn.p, dptr.p, weight, dpiv.p);
+            hipLaunchKernelGGL(k_farpivots<false>, dim3((unsigned)nitems), dim3(kP1Threads), 0, st, other->xyz.p,
+                               dlo.p, dhi.p, dperm.p, dctr.p, dfp.p, dfn.p, nullptr, dptr.p, weight, dpiv.p, derr.p);
         NCA_HIP(hipGetLastError());
+        if (pivptr[nitems] > 0) d2h(ctx, piv, dpiv.p, sizeof(int64_t) * pivptr[nitems]);
+        NCA_HIP(hipStreamSynchronize(st));
+    });
+}
+
+// Phase 1 on the device with the far fields built there: the far lists by
+// node (farptr / farnodes, 1-based; the admissible pairs of the self tree)
+// and the parent of every node (1-based, 0 at a root). Counts as in
+// nca_farpivots, with the far-field sizes summed along the parent chain.
+int32_t nca_farpivots_tree(nca_ctx* ctx, const nca_points* other, int64_t nitems, const double* centers,
+                           const int64_t* nvals, const int64_t* parent, const int64_t* farptr,
+                           const int64_t* farnodes, int64_t nn_other, const int64_t* other_lo,
+                           const int64_t* other_hi, const int64_t* other_perm, int64_t maxrank, double weight,
+                           double oversample, int64_t* pivptr, int64_t pivcap, int64_t* piv) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && other && nitems >= 0 && farptr && pivptr && (nitems == 0 || (centers && nvals && parent)),
+                    NCA_ERR_ARGUMENT, "nca_farpivots_tree: invalid argument");
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        const int64_t npts = other->n;
+        std::vector<int32_t> lo(nn_other), hi(nn_other), perm(npts), par(nitems);
+        std::vector<char> bad(std::max<int64_t>({nn_other, npts, nitems, (int64_t)1}), 0);
+        host_parallel_for(nn_other, [&](int64_t i) {
+            lo[i] = (int32_t)(other_lo[i] - 1);
+            hi[i] = (int32_t)other_hi[i];
+            if (lo[i] < 0 || hi[i] < lo[i] || hi[i] > npts) bad[i] = 1;
+        });
+        for (int64_t i = 0; i < nn_other; ++i)
+            NCA_REQUIRE(!bad[i], NCA_ERR_ARGUMENT, "nca_farpivots_tree: node range out of bounds");
+        host_parallel_for(npts, [&](int64_t i) {
+            perm[i] = (int32_t)(other_perm[i] - 1);
+            bad[i] = perm[i] < 0 || perm[i] >= npts;
+        });
+        for (int64_t i = 0; i < npts; ++i) NCA_REQUIRE(!bad[i], NCA_ERR_ARGUMENT, "nca_farpivots_tree: perm out of range");
+        for (int64_t t = 0; t < nitems; ++t) {
+            par[t] = (int32_t)(parent[t] - 1);
+            NCA_REQUIRE(par[t] >= -1 && par[t] < nitems && par[t] != t, NCA_ERR_ARGUMENT,
+                        "nca_farpivots_tree: parent out of range");
+        }
+        const int64_t nfar = farptr[nitems] - farptr[0];
+        std::vector<int64_t> fp(nitems + 1);
+        std::vector<int32_t> fn(nfar);
+        for (int64_t t = 0; t <= nitems; ++t) fp[t] = farptr[t] - farptr[0];
+        // own far lists: checked, and their node and point counts
+        std::vector<int64_t> ownn(nitems), ownpts(nitems);
+        host_parallel_for(nitems, [&](int64_t t) {
+            int64_t p = 0;
+            for (int64_t k = fp[t]; k < fp[t + 1]; ++k) {
+                const int64_t f = farnodes[k] - 1;
+                if (f < 0 || f >= nn_other) { bad[t] = 1; return; }
+                fn[k] = (int32_t)f;
+                p += hi[f] - lo[f];
+            }
+            bad[t] = 0;
+            ownn[t] = fp[t + 1] - fp[t];
+            ownpts[t] = p;
+        });
+        for (int64_t t = 0; t < nitems; ++t) NCA_REQUIRE(!bad[t], NCA_ERR_ARGUMENT, "nca_farpivots_tree: far node out of range");
+        // far-field node and point counts along the parent chains (memoized;
+        // a chain longer than nitems is a cycle)
+        std::vector<int64_t> ffn(nitems, -1), ffp(nitems, 0);
+        std::vector<int64_t> chain;
+        for (int64_t t = 0; t < nitems; ++t) {
+            chain.clear();
+            int64_t a = t;
+            while (a >= 0 && ffn[a] < 0) {
+                chain.push_back(a);
+                NCA_REQUIRE((int64_t)chain.size() <= nitems, NCA_ERR_ARGUMENT, "nca_farpivots_tree: parent cycle");
+                a = par[a];
+            }
+            int64_t n0 = a >= 0 ? ffn[a] : 0, p0 = a >= 0 ? ffp[a] : 0;
+            for (size_t i = chain.size(); i-- > 0;) {
+                n0 += ownn[chain[i]];
+                p0 += ownpts[chain[i]];
+                ffn[chain[i]] = n0;
+                ffp[chain[i]] = p0;
+            }
+        }
+        pivptr[0] = 0;
+        for (int64_t t = 0; t < nitems; ++t) {
+            NCA_REQUIRE(ffn[t] <= kP1MaxFar, NCA_ERR_LIMIT, "nca_farpivots_tree: more than 4096 far nodes in one far field");
+            const int64_t limit = std::min<int64_t>(maxrank, (int64_t)std::ceil(oversample * (double)nvals[t]));
+            pivptr[t + 1] = pivptr[t] + std::max<int64_t>(0, std::min(limit, ffp[t]));
+        }
+        NCA_REQUIRE(pivptr[nitems] <= pivcap, NCA_ERR_LIMIT,
+                    "nca_farpivots_tree: pivot capacity too small (" + std::to_string(pivptr[nitems]) + " needed)");
+        DevBuf<int32_t> dlo, dhi, dperm, dfn, dpar, derr(1);
+        DevBuf<int64_t> dfp, dptr, dpiv;
+        DevBuf<double> dctr;
+        dlo.upload(lo, st);
+        dhi.upload(hi, st);
+        dperm.upload(perm, st);
+        dfn.upload(fn, st);
+        dfp.upload(fp, st);
+        dpar.upload(par, st);
+        std::vector<int64_t> pp(pivptr, pivptr + nitems + 1);
+        dptr.upload(pp, st);
+        dctr.upload(centers, (size_t)(3 * nitems), st);
+        dpiv.alloc((size_t)std::max<int64_t>(pivptr[nitems], 1));
+        const int32_t zero = 0;
+        derr.upload(&zero, 1, st);
+        if (nitems > 0)
+            hipLaunchKernelGGL(k_farpivots<true>, dim3((unsigned)nitems), dim3(kP1Threads), 0, st, other->xyz.p,
+                               dlo.p, dhi.p, dperm.p, dctr.p, dfp.p, dfn.p, dpar.p, dptr.p, weight, dpiv.p, derr.p);
+        NCA_HIP(hipGetLastError());
+        int32_t e = 0;
+        d2h(ctx, &e, derr.p, sizeof(int32_t));
+        NCA_REQUIRE(e == 0, NCA_ERR_LIMIT, "nca_farpivots_tree: candidate pool above 4096 points");
         if (pivptr[nitems] > 0) d2h(ctx, piv, dpiv.p, sizeof(int64_t) * pivptr[nitems]);
         NCA_HIP(hipStreamSynchronize(st));
     });

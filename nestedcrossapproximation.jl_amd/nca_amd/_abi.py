@@ -92,6 +92,7 @@ SIGNATURES = {
     "nca_compress_level": (I32, [P, P, D, I32, P, P, P, I64, P, P, P]),
     "nca_coupling_assemble": (I32, [P, P, P, P, I64, P, P, I64, P, P, P, P, PP]),
     "nca_farpivots": (I32, [P, P, I64, P, P, P, P, I64, P, P, P, I64, D, D, P, I64, P]),
+    "nca_farpivots_tree": (I32, [P, P, I64, P, P, P, P, P, I64, P, P, P, I64, D, D, P, I64, P]),
     "nca_interactions": (I32, [P, I64, P, P, P, P, I64, P, P, P, P, I32, P, P, P, PP]),
     "nca_partition_export": (I32, [P, P, P, P, P]),
     "nca_partition_destroy": (I32, [P]),

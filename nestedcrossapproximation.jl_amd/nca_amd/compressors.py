@@ -60,9 +60,8 @@ class GeometricPivoting:
 
     def select(self, selftree, othertree, fd, maxrank, otherpoints=None):
         if self.device and otherpoints is not None:
-            ffptr, ffnodes = _trees.farfield_csr(selftree, fd)
-            return device_farpivots(otherpoints, selftree.center, selftree.hi - selftree.lo, ffptr, ffnodes,
-                                    othertree, maxrank, self.weight, self.oversample)
+            return device_farpivots_tree(otherpoints, selftree, fd, othertree, maxrank, self.weight,
+                                         self.oversample)
         return _trees.farpivots(selftree, othertree, fd, maxrank, self.weight, self.oversample)
 
 
@@ -83,6 +82,28 @@ def device_farpivots(otherpoints, centers, nvals, ffptr, ffnodes, othertree, max
                                  _p(fn1) if len(fn1) else None, othertree.nnodes, _p(lay["nodelo"]),
                                  _p(lay["nodehi"]), _p(lay["perm"]), int(maxrank), float(weight), float(oversample),
                                  _p(ptr), cap, _p(piv)))
+    return ptr, piv[:ptr[-1]].copy()
+
+
+def device_farpivots_tree(otherpoints, selftree, fd, othertree, maxrank, weight, oversample):
+    """Phase 1 on the device (nca_farpivots_tree) for every node of selftree,
+    its far field (own far nodes, then its ancestors') built on the device
+    from the FarData CSR and the parent array. Returns a 0-based CSR of point
+    ids, equal to device_farpivots on farfield_csr(selftree, fd)."""
+    n = selftree.nnodes
+    c = np.ascontiguousarray(selftree.center, np.float64)
+    nv = np.ascontiguousarray(selftree.hi - selftree.lo, np.int64)
+    par1 = np.ascontiguousarray(np.asarray(selftree.parent, np.int64) + 1)
+    fp1 = np.ascontiguousarray(np.asarray(fd.nodetofarnodesptr, np.int64) + 1)
+    fn1 = np.ascontiguousarray(np.asarray(fd.farnodes, np.int64) + 1)
+    lay = othertree.layout_1based()
+    ptr = np.zeros(n + 1, np.int64)
+    cap = max(n * int(maxrank), 1)
+    piv = np.zeros(cap, np.int64)
+    call(hip_lib().nca_farpivots_tree(otherpoints.ctx.handle, otherpoints.handle, n, _p(c), _p(nv), _p(par1),
+                                      _p(fp1), _p(fn1) if len(fn1) else None, othertree.nnodes,
+                                      _p(lay["nodelo"]), _p(lay["nodehi"]), _p(lay["perm"]), int(maxrank),
+                                      float(weight), float(oversample), _p(ptr), cap, _p(piv)))
     return ptr, piv[:ptr[-1]].copy()
 
 

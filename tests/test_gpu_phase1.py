@@ -1,7 +1,8 @@
-"""Phase 1 on the device (nca_farpivots) against the host selection
-(nca_tree.cpp): the same far pivots, bit for bit, for weight 1 — plain
-clusters (own + ancestors' far field) of one and of two trees, and
-(cluster, direction) items over their dirfarfield."""
+"""Phase 1 on the device (nca_farpivots, and nca_farpivots_tree, which builds
+the far fields itself) against the host selection (nca_tree.cpp): the same far
+pivots, bit for bit, for weight 1 — plain clusters (own + ancestors' far field)
+of one and of two trees, and (cluster, direction) items over their
+dirfarfield."""
 import math
 
 import numpy as np
@@ -45,6 +46,9 @@ def test_device_phase1_equals_host(case):
             dev = C.device_farpivots(pts, selftree.center, selftree.hi - selftree.lo, ff[0], ff[1], othertree,
                                      maxrank, 1.0, over)
             assert np.array_equal(host[0], dev[0]) and np.array_equal(host[1], dev[1])
+            # far fields built on the device from the FarData CSR and the parents
+            devt = C.device_farpivots_tree(pts, selftree, fd, othertree, maxrank, 1.0, over)
+            assert np.array_equal(host[0], devt[0]) and np.array_equal(host[1], devt[1])
     # the selector routes to the device when it gets the other side's points
     got = sel.select(tt, st, tf, 50, A.trial)
     want = T.farpivots(tt, st, tf, 50)
@@ -78,6 +82,8 @@ def test_device_phase1_weight():
     assert np.array_equal(host[0], dev[0])
     same = np.mean(host[1] == dev[1])
     say(f"  weight 1.3: {100 * same:.2f} % of the pivots identical")
+    devt = C.device_farpivots_tree(A.trial, tr, tf, tr, 50, 1.3, 2.0)
+    assert np.array_equal(dev[0], devt[0]) and np.array_equal(dev[1], devt[1])
     for t in range(0, tr.nnodes, 37):
         farpts = set(np.concatenate([tr.values(f) for f in ff[1][ff[0][t]:ff[0][t + 1]]] or [[]]).tolist())
         assert set(dev[1][dev[0][t]:dev[0][t + 1]].tolist()) <= farpts
