@@ -1550,6 +1550,49 @@ void parallel_for(int64_t n, F&& f) {
     for (auto& t : pool) t.join();
 }
 
+// Transpose of a CSR (nrows rows, entries col(i) < ncols) on up to 16
+// threads: tptr (ncols + 1) and, for every entry i of row r in column c, its
+// place slot(c, r) in column order, rows increasing within a column (as a
+// serial counting sort gives). emit(i, r, pos) is called once per entry.
+template <class Col, class Emit>
+void csr_transpose(int64_t nrows, const int64_t* ptr, int64_t ncols, Col col, std::vector<int64_t>& tptr, Emit emit) {
+    const int64_t nt = std::min<int64_t>(std::max<unsigned>(1u, std::thread::hardware_concurrency()), 16);
+    const int64_t nnz = ptr[nrows] - ptr[0];
+    const int64_t nth = nnz < (1 << 16) ? 1 : nt;
+    // contiguous row ranges of about nnz / nth entries
+    std::vector<int64_t> rb(nth + 1, nrows);
+    rb[0] = 0;
+    for (int64_t k = 1, r = 0; k < nth; ++k) {
+        const int64_t want = ptr[0] + nnz * k / nth;
+        while (r < nrows && ptr[r] < want) ++r;
+        rb[k] = r;
+    }
+    std::vector<std::vector<int64_t>> cnt(nth, std::vector<int64_t>(ncols, 0));
+    auto run = [&](auto&& body) {
+        std::vector<std::thread> pool;
+        for (int64_t k = 0; k < nth; ++k) pool.emplace_back([&, k] { body(k); });
+        for (auto& t : pool) t.join();
+    };
+    run([&](int64_t k) {
+        for (int64_t r = rb[k]; r < rb[k + 1]; ++r)
+            for (int64_t i = ptr[r]; i < ptr[r + 1]; ++i) cnt[k][col(i)]++;
+    });
+    tptr.assign(ncols + 1, 0);
+    for (int64_t c = 0; c < ncols; ++c) {
+        int64_t run_ = tptr[c];
+        for (int64_t k = 0; k < nth; ++k) {
+            const int64_t v = cnt[k][c];
+            cnt[k][c] = run_;
+            run_ += v;
+        }
+        tptr[c + 1] = run_;
+    }
+    run([&](int64_t k) {
+        for (int64_t r = rb[k]; r < rb[k + 1]; ++r)
+            for (int64_t i = ptr[r]; i < ptr[r + 1]; ++i) emit(i, r, cnt[k][col(i)]++);
+    });
+}
+
 void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, int64_t ne, const int64_t* dirptr,
                 bool multiparent, const int64_t* perm, const int64_t* lo, const int64_t* hi, const int64_t* cptr,
                 const nca_points* pts, int64_t nbasis, const int64_t* bnodes, const int64_t* boff,
@@ -2083,18 +2126,14 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
         // reverse coupling: blocks by trial block
         {
             const int64_t ntrial = lay.ne_trial;
-            std::vector<int64_t> rptr(ntrial + 1, 0), roff(coupling->nblocks);
+            std::vector<int64_t> rptr, roff(coupling->nblocks);
             std::vector<int32_t> rtest(coupling->nblocks);
-            for (int64_t i = 0; i < coupling->nblocks; ++i) rptr[coupling->nodes[i] + 1]++;
-            for (int64_t s = 0; s < ntrial; ++s) rptr[s + 1] += rptr[s];
-            std::vector<int64_t> cur(rptr.begin(), rptr.end() - 1);
-            for (int64_t t = 0; t < ntest; ++t)
-                for (int64_t i = coupling->ptr[t]; i < coupling->ptr[t + 1]; ++i) {
-                    const int64_t s = coupling->nodes[i];
-                    rtest[cur[s]] = (int32_t)t;
-                    roff[cur[s]] = coupling->offsets[i];
-                    cur[s]++;
-                }
+            csr_transpose(
+                ntest, coupling->ptr.data(), ntrial, [&](int64_t i) { return coupling->nodes[i]; }, rptr,
+                [&](int64_t i, int64_t t, int64_t pos) {
+                    rtest[pos] = (int32_t)t;
+                    roff[pos] = coupling->offsets[i];
+                });
             h->r_ptr.upload(rptr, st);
             h->r_off.upload(roff, st);
             h->r_test.upload(rtest, st);
@@ -2105,16 +2144,17 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
             const int64_t ntest = nn_test;
             const int64_t ntrial = d->nn_trial;
             const int64_t nnz = d->nearptr[ntest] - d->nearptr[0];
-            std::vector<int64_t> np(ntest + 1), tp(ntrial + 1, 0);
+            std::vector<int64_t> np(ntest + 1), tp;
             std::vector<int32_t> nn32(nnz), tn(nnz);
             int64_t near_entries = 0;
             for (int64_t t = 0; t <= ntest; ++t) np[t] = d->nearptr[t] - d->nearptr[0];
-            for (int64_t i = 0; i < nnz; ++i) {
+            std::atomic<bool> bad{false};
+            parallel_for(nnz, [&](int64_t i) {
                 const int64_t s = d->nearnodes[i] - 1;
-                NCA_REQUIRE(s >= 0 && s < ntrial, NCA_ERR_ARGUMENT, "nca_h2_create: near node out of range");
-                nn32[i] = (int32_t)s;
-                tp[s + 1]++;
-            }
+                if (s < 0 || s >= ntrial) bad = true;
+                else nn32[i] = (int32_t)s;
+            });
+            NCA_REQUIRE(!bad, NCA_ERR_ARGUMENT, "nca_h2_create: near node out of range");
             std::vector<int64_t> rowent(ntest, 0);
             parallel_for(ntest, [&](int64_t t) {
                 int64_t e = 0;
@@ -2122,10 +2162,9 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
                 rowent[t] = e * (h->test.h_hi[t] - h->test.h_lo[t]);
             });
             for (int64_t t = 0; t < ntest; ++t) near_entries += rowent[t];
-            for (int64_t s = 0; s < ntrial; ++s) tp[s + 1] += tp[s];
-            std::vector<int64_t> cur(tp.begin(), tp.end() - 1);
-            for (int64_t t = 0; t < ntest; ++t)
-                for (int64_t i = np[t]; i < np[t + 1]; ++i) tn[cur[nn32[i]]++] = (int32_t)t;
+            csr_transpose(
+                ntest, np.data(), ntrial, [&](int64_t i) { return nn32[i]; }, tp,
+                [&](int64_t, int64_t t, int64_t pos) { tn[pos] = (int32_t)t; });
             h->test.nearptr.upload(np, st);
             h->test.nearnodes.upload(nn32, st);
             h->trial.nearptr.upload(tp, st);
