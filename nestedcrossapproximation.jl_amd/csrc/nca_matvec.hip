@@ -698,11 +698,11 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uin
     return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, kBufDword3);
 }
 
-// Operands of one near-field step: B rows of the G groups, and the lane's
-// source point.
-template <class P, int G>
+// Operands of one near-field step: B rows of the G groups for each of the NC
+// 16-column chunks, and the lane's source point.
+template <class P, int G, int NC>
 struct NearOps {
-    typename P::T2 b[G];
+    typename P::T2 b[NC][G];
     uint32_t pw;  // G = 1: dword i < 6 of the row's source point in lane i of each 16-lane row
     double pq;    // G > 1: coordinate j < 3 of the quad's source point in lane j of each quad
 };
@@ -724,17 +724,21 @@ __device__ __forceinline__ double quad_double(double v) {
     return __hiloint2double(hi, lo);
 }
 
-// Loads of one step: xs rows (vx[m]: the lane's offset of group m) and the
+// Loads of one step: xs rows (vx[c][m]: the lane's offset of group m in
+// column chunk c) and the
 // source points, at byte offsets sp / sx into the range's points (rp) and
 // right-hand sides (rx); reads past the range end return 0. Every lane of a
 // 16-lane row (G = 1) or of a quad (G > 1) needs the same source point, so
 // each lane loads one piece of it (vp: a dword of the row's point, or a
 // coordinate of the quad's) and the computing step broadcasts them.
-template <class P, int G>
-__device__ __forceinline__ void near_load(NearOps<P, G>& o, __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rx,
-                                          uint32_t sp, uint32_t sx, uint32_t vp, const uint32_t (&vx)[G]) {
+template <class P, int G, int NC>
+__device__ __forceinline__ void near_load(NearOps<P, G, NC>& o, __amdgpu_buffer_rsrc_t rp,
+                                          __amdgpu_buffer_rsrc_t rx, uint32_t sp, uint32_t sx, uint32_t vp,
+                                          const uint32_t (&vx)[NC][G]) {
 #pragma unroll
-    for (int m = 0; m < G; ++m) o.b[m] = buf_load<typename P::T2>(rx, vx[m], sx);
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int m = 0; m < G; ++m) o.b[c][m] = buf_load<typename P::T2>(rx, vx[c][m], sx);
     if constexpr (G == 1) o.pw = (uint32_t)buf_load<int>(rp, vp, sp);
     else o.pq = buf_load<double>(rp, vp, sp);
 }
@@ -746,10 +750,11 @@ __device__ __forceinline__ void near_load(NearOps<P, G>& o, __amdgpu_buffer_rsrc
 // 0..H-1 of each 16-lane row (the rows >= H of the product are not stored),
 // and B = xs of sources 4 m .. 4 m + 3 of the step. Sources past the range end
 // read points 0 and B = 0; groups that start there (left: sources of the range
-// from the step on) skip their products.
-template <class P, int H, class ACC, bool EMB, bool NEG, bool GUARD>
-__device__ __forceinline__ void near_compute(ACC& acc, const NearK& nk, const double2* tab, double x0, double x1,
-                                             double x2, const NearOps<P, 16 / H>& o, int left) {
+// from the step on) skip their products. The evaluated entries serve the NC
+// column chunks (one accumulator each).
+template <class P, int H, class ACC, bool EMB, bool NEG, bool GUARD, int NC>
+__device__ __forceinline__ void near_compute(ACC (&acc)[NC], const NearK& nk, const double2* tab, double x0,
+                                             double x1, double x2, const NearOps<P, 16 / H, NC>& o, int left) {
     using T2 = typename P::T2;
     constexpr int G = 16 / H;
     T2 g;
@@ -779,7 +784,8 @@ __device__ __forceinline__ void near_compute(ACC& acc, const NearK& nk, const do
             const T2 rb = m == 0 ? rot16v<12>(g) : (m == 1 ? g : (m == 2 ? rot16v<4>(g) : rot16v<8>(g)));
             a = i < 4 ? ra : rb;
         }
-        acc.mma(a, o.b[m]);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c].mma(a, o.b[c][m]);
     }
 }
 
@@ -800,22 +806,24 @@ __device__ __forceinline__ void near_compute(ACC& acc, const NearK& nk, const do
 // a range are 32-bit (range length x right-hand sides x element size < 2^32).
 // rot: the walk starts at range rot mod nr (tiles that share sources then read
 // them at different times).
-template <class P, int H, class ACC, bool EMB, bool NEG>
-__device__ __forceinline__ void near_walk(ACC& acc, const NearK& nk, const double2* tab, int lane,
+template <class P, int H, class ACC, bool EMB, bool NEG, int NC>
+__device__ __forceinline__ void near_walk(ACC (&acc)[NC], const NearK& nk, const double2* tab, int lane,
                                           double x0, double x1, double x2, int64_t e0, int64_t e1, int rot,
                                           const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
                                           const double* __restrict__ spts, const typename P::T2* __restrict__ xsq,
-                                          uint32_t R, int jl) {
+                                          uint32_t R, const int (&jl)[NC]) {
     using T2 = typename P::T2;
     constexpr int S = 4 * (16 / H), G = 16 / H;
-    constexpr int U = (H == 16 && !P::kF64) ? NCA_NEAR_UNROLL : 1;
+    constexpr int U = (H == 16 && !P::kF64 && NC == 1) ? NCA_NEAR_UNROLL : 1;
     constexpr uint32_t kT = sizeof(T2);
     const int kk = lane >> 4, ysrc = 4 * ((lane & 15) / H) + kk;
     const uint32_t vp = G == 1 ? (uint32_t)kk * 24u + 4u * (uint32_t)min(lane & 15, 5)
                                : (uint32_t)ysrc * 24u + 8u * (uint32_t)min(lane & 3, 2);
-    uint32_t vx[G];
+    uint32_t vx[NC][G];
 #pragma unroll
-    for (int m = 0; m < G; ++m) vx[m] = ((uint32_t)(4 * m + kk) * R + (uint32_t)jl) * kT;
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int m = 0; m < G; ++m) vx[c][m] = ((uint32_t)(4 * m + kk) * R + (uint32_t)jl[c]) * kT;
     const uint32_t rowb = R * kT;
     const int nr = (int)(e1 - e0);
     for (int c = 0; c < nr; ++c) {
@@ -833,19 +841,19 @@ __device__ __forceinline__ void near_walk(ACC& acc, const NearK& nk, const doubl
             uint32_t sp = 0, sx = 0;
             if constexpr (U > 1) {
                 for (; left > (U - 1) * S; left -= U * S, sp += U * S * 24u, sx += U * S * rowb) {
-                    NearOps<P, G> o[U];
+                    NearOps<P, G, NC> o[U];
 #pragma unroll
                     for (int u = 0; u < U; ++u)
-                        near_load<P, G>(o[u], rp, rx, sp + u * S * 24u, sx + u * S * rowb, vp, vx);
+                        near_load<P, G, NC>(o[u], rp, rx, sp + u * S * 24u, sx + u * S * rowb, vp, vx);
 #pragma unroll
                     for (int u = 0; u < U; ++u)
-                        near_compute<P, H, ACC, EMB, NEG, GU>(acc, nk, tab, x0, x1, x2, o[u], left - u * S);
+                        near_compute<P, H, ACC, EMB, NEG, GU, NC>(acc, nk, tab, x0, x1, x2, o[u], left - u * S);
                 }
             }
             for (; left > 0; left -= S, sp += S * 24u, sx += S * rowb) {
-                NearOps<P, G> o;
-                near_load<P, G>(o, rp, rx, sp, sx, vp, vx);
-                near_compute<P, H, ACC, EMB, NEG, GU>(acc, nk, tab, x0, x1, x2, o, left);
+                NearOps<P, G, NC> o;
+                near_load<P, G, NC>(o, rp, rx, sp, sx, vp, vx);
+                near_compute<P, H, ACC, EMB, NEG, GU, NC>(acc, nk, tab, x0, x1, x2, o, left);
             }
         };
         // FP64: one guarded step loop (a second copy costs the kernel occupancy)
@@ -854,12 +862,13 @@ __device__ __forceinline__ void near_walk(ACC& acc, const NearK& nk, const doubl
     }
 }
 
-// Stage 5 + near field for 16 right-hand sides: one wave per 16-row tile of a
-// destination leaf. The basis rows enter as the first "sources" (A = U_t or
-// V_t^T, B = yh_t); then the near-field sources over each merged contiguous
-// point range, with 16, 8 or 4 row slots by the rows of the tile. Rows past
-// the tile end repeat its last row and are not stored.
-template <class P, bool FWD, bool CJ, bool K3, bool NEG>
+// Stage 5 + near field for 16 NC right-hand sides (NC = 1 or 2 chunks of 16
+// columns: each evaluated entry serves all of them): one wave per 16-row tile
+// of a destination leaf. The near-field sources over each merged contiguous
+// point range, with 16, 8 or 4 row slots by the rows of the tile, then the
+// basis rows (A = U_t or V_t^T, B = yh_t). Rows past the tile end repeat its
+// last row and are not stored.
+template <class P, bool FWD, bool CJ, bool K3, bool NEG, int NC>
 __global__ void __launch_bounds__(256, 4)
     k_leaf_out_mfma(const int32_t* __restrict__ tile_leaf, const int32_t* __restrict__ tile_row0, int64_t ntiles,
                     const int32_t* __restrict__ leaves, const int32_t* __restrict__ leafbasis,
@@ -889,54 +898,71 @@ __global__ void __launch_bounds__(256, 4)
     const int i = lane & 15, kk = lane >> 4, j = lane & 15;
     const int64_t e0 = nrptr[t], e1 = nrptr[t + 1];
     const int rot = stagger ? w * 7 : 0;
-    const int jl = min(j, nq - 1);  // columns past the chunk's live ones read its last (not stored)
-    const double2* yt = yh + (int64_t)dcptr[t] * R + q0 + jl;
+    // columns past the live ones read the last live one (not stored)
+    int jl[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) jl[c] = min(16 * c + j, nq - 1);
     const int rt = dcptr[t + 1] - dcptr[t];
     const T2* B = bi >= 0 ? bblocks + boff[bi] : nullptr;
     // near field first (4 pi G), scaled by 1 / (4 pi), then the basis rows
     auto tile = [&](auto& acc, auto hval, auto emb, int rsel) {
         constexpr int H = decltype(hval)::value;
         constexpr bool EMB = decltype(emb)::value;
-        acc.zero();
+        using ACC = std::remove_reference_t<decltype(acc[0])>;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c].zero();
         const double* xp = dpts + 3 * (int64_t)(l0 + row0 + min(rsel, nrows - 1));
-        near_walk<P, H, std::remove_reference_t<decltype(acc)>, EMB, NEG>(acc, nk, tab, lane, xp[0], xp[1], xp[2], e0,
-                                                                         e1, rot, nrlo, nrhi, spts, xs + q0,
-                                                                         (uint32_t)R, jl);
-        acc.scale((typename P::T)kInvFourPi);
+        near_walk<P, H, ACC, EMB, NEG, NC>(acc, nk, tab, lane, xp[0], xp[1], xp[2], e0, e1, rot, nrlo, nrhi, spts,
+                                           xs + q0, (uint32_t)R, jl);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c].scale((typename P::T)kInvFourPi);
         if (B) {
             const int rowc = row0 + min(rsel, nrows - 1);
+            const double2* yt = yh + (int64_t)dcptr[t] * R + q0;
             for (int k0 = 0; k0 < rt; k0 += 4) {
                 const int k = k0 + kk;
                 const int kc = min(k, rt - 1);
                 const T2 a =
                     conjif<CJ, P>(B[FWD ? (int64_t)rowc + (int64_t)kc * nt : (int64_t)kc + (int64_t)rowc * rt]);
-                T2 b = P::cvt(yt[(int64_t)kc * R]);
-                if (k >= rt) b = czero2<P>();
-                acc.mma(a, b);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    T2 b = P::cvt(yt[(int64_t)kc * R + jl[c]]);
+                    if (k >= rt) b = czero2<P>();
+                    acc[c].mma(a, b);
+                }
             }
         }
     };
-    auto emit = [&](int row, double2 v) {
-        if (row < nrows && j < nq) {
-            double2* out = y + (int64_t)dperm[l0 + row0 + row] + (q0 + j) * ldy;
-            double2 val = cmul(alpha, v);
-            if (beta.x != 0.0 || beta.y != 0.0) val = cadd(val, cmul(beta, *out));
-            *out = val;
-        }
+    auto emitter = [&](int c) {
+        return [&, c](int row, double2 v) {
+            const int col = 16 * c + j;
+            if (row < nrows && col < nq) {
+                double2* out = y + (int64_t)dperm[l0 + row0 + row] + (q0 + col) * ldy;
+                double2 val = cmul(alpha, v);
+                if (beta.x != 0.0 || beta.y != 0.0) val = cadd(val, cmul(beta, *out));
+                *out = val;
+            }
+        };
     };
     if (nrows > 8) {
-        Cacc<P, K3> acc;
+        Cacc<P, K3> acc[NC];
         tile(acc, std::integral_constant<int, 16>(), std::false_type(), i);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) emit(P::row(kk, v), acc.get(v));
+        for (int c = 0; c < NC; ++c) {
+            const auto emit = emitter(c);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) emit(P::row(kk, v), acc[c].get(v));
+        }
     } else if (nrows > 4) {
-        Cemb<P, 8> acc;
+        Cemb<P, 8> acc[NC];
         tile(acc, std::integral_constant<int, 8>(), std::true_type(), i & 7);
-        acc.store(emit);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c].store(emitter(c));
     } else {
-        Cemb<P, 4> acc;
+        Cemb<P, 4> acc[NC];
         tile(acc, std::integral_constant<int, 4>(), std::true_type(), i & 3);
-        acc.store(emit);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c].store(emitter(c));
     }
 }
 
@@ -1271,11 +1297,48 @@ struct Ops<true> {
     static T2* xs(nca_h2* h) { return h->ws_xs32.p; }
 };
 
+// Stage 5 and the near field on the MFMA kernel for columns q0 .. q0 + nq - 1
+// (nq <= 16 NC): one launch for NC chunks of 16 columns, each kernel
+// evaluation serving all of them; then the bases above the leaves.
+template <int NC, int OP, bool MIX>
+void run_leaf(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy, double2 alpha, double2 beta) {
+    constexpr bool FWD = OP == NCA_OP_N;
+    constexpr bool CJ = OP == NCA_OP_C;
+    using O = Ops<MIX>;
+    using T2 = typename O::T2;
+    using P = typename O::P;
+    hipStream_t st = h->ctx->stream;
+    SideDev& src = FWD ? h->trial : h->test;
+    SideDev& dst = FWD ? h->test : h->trial;
+    const double2* yh = h->ws_yh.p;
+    if (dst.ntiles > 0) {
+        // conj(G) for a negative phase sign, again conjugated for the adjoint
+        const bool neg = (h->sign < 0) != CJ;
+        const NearK nk{h->k, h->k * (0.5 / M_PI), h->k * (0.25 / M_PI)};
+        const bool k3 = h->near_k3[MIX ? 1 : 0] != 0;
+        auto kern = k3 ? (neg ? k_leaf_out_mfma<P, FWD, CJ, true, true, NC> : k_leaf_out_mfma<P, FWD, CJ, true, false, NC>)
+                       : (neg ? k_leaf_out_mfma<P, FWD, CJ, false, true, NC>
+                              : k_leaf_out_mfma<P, FWD, CJ, false, false, NC>);
+        hipLaunchKernelGGL(kern, dim3(blocks_for(dst.ntiles, 4)), dim3(256), 0, st, dst.tile_leaf.p,
+                           dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst),
+                           dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(), dst.pts.p, dst.nrptr.p, dst.nrlo.p,
+                           dst.nrhi.p, src.pts.p, yh, O::xs(h), R, q0, nk, h->sctab.p, alpha, beta, y, ldy,
+                           h->near_xcd, h->near_stagger, nq);
+    }
+    if (dst.nibasis > 0)
+        hipLaunchKernelGGL((k_basis_out<FWD, CJ, T2>), dim3((unsigned)dst.nibasis), dim3(256), 0, st, dst.ibslot.p,
+                           dst.bnodes.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(),
+                           yh, R, q0, (int64_t)nq, alpha, y, ldy);
+    NCA_HIP(hipGetLastError());
+}
+
 // Columns q0 .. q0 + nq - 1 of R; NR = 16 runs the MFMA kernels for any
 // nq <= 16 (lanes of the columns past nq compute, and are not stored).
+// leaf = false: stop after the downward pass (the caller runs stage 5 and the
+// near field for this chunk together with the next one).
 template <int NR, int OP, bool MIX>
 void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy, double2 alpha, double2 beta,
-               StageEvents* evs) {
+               StageEvents* evs, bool leaf = true) {
     constexpr bool FWD = OP == NCA_OP_N;
     constexpr bool CJ = OP == NCA_OP_C;
     using O = Ops<MIX>;
@@ -1387,33 +1450,22 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
                                dst.levptr[l], R, q0, yh);
     }
     if (evs) evs->mark(5, st);
-    if (NR == 16 && dst.ntiles > 0) {
-        // conj(G) for a negative phase sign, again conjugated for the adjoint
-        const bool neg = (h->sign < 0) != CJ;
-        const NearK nk{h->k, h->k * (0.5 / M_PI), h->k * (0.25 / M_PI)};
-        const bool k3 = h->near_k3[MIX ? 1 : 0] != 0;
-        auto kern = k3 ? (neg ? k_leaf_out_mfma<P, FWD, CJ, true, true> : k_leaf_out_mfma<P, FWD, CJ, true, false>)
-                       : (neg ? k_leaf_out_mfma<P, FWD, CJ, false, true> : k_leaf_out_mfma<P, FWD, CJ, false, false>);
-        hipLaunchKernelGGL(kern, dim3(blocks_for(dst.ntiles, 4)), dim3(256), 0, st, dst.tile_leaf.p,
-                           dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst),
-                           dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(), dst.pts.p, dst.nrptr.p, dst.nrlo.p,
-                           dst.nrhi.p,
-                           src.pts.p, yh, xs, R, q0, nk, h->sctab.p, alpha, beta, y, ldy, h->near_xcd,
-                           h->near_stagger, nq);
-    } else if (NR != 16 && dst.nleaves > 0) {
+    if (!leaf) return;
+    if (NR == 16) {
+        run_leaf<1, OP, MIX>(h, R, nq, q0, y, ldy, alpha, beta);
+    } else if (dst.nleaves > 0) {
         hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ, T2, T2>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st,
                            dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p,
                            dst.node_cptr(), dst.pts.p, dst.nearptr.p, dst.nearnodes.p, src.lo.p, src.hi.p, src.pts.p,
                            yh, xs, R, q0, h->k, h->sign, alpha, beta, y, ldy);
+        if (dst.nibasis > 0)
+            hipLaunchKernelGGL((k_basis_out<FWD, CJ, T2>), dim3((unsigned)dst.nibasis), dim3(256), 0, st,
+                               dst.ibslot.p, dst.bnodes.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p,
+                               dst.node_cptr(), yh, R, q0, (int64_t)nq, alpha, y, ldy);
     }
-    if (dst.nibasis > 0)
-        hipLaunchKernelGGL((k_basis_out<FWD, CJ, T2>), dim3((unsigned)dst.nibasis), dim3(256), 0, st, dst.ibslot.p,
-                           dst.bnodes.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(),
-                           yh, R, q0, (int64_t)nq, alpha, y, ldy);
     if (evs) evs->mark(6, st);
     NCA_HIP(hipGetLastError());
 }
-
 
 template <int OP, bool MIX>
 void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, int64_t R, double2 alpha,
@@ -1441,12 +1493,22 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     if (ny16 > 0 && (valu || !FWD))
         hipLaunchKernelGGL(k_fill, dim3(blocks_for(ny16, 256)), dim3(256), 0, st, h->ws_yh.p, ny16);
     // chunks of up to 16 columns on the MFMA kernels; a last chunk of 1 to 3
-    // columns on the VALU kernels unless mfma_small is set
+    // columns on the VALU kernels unless mfma_small is set. Two MFMA chunks in
+    // a row share one near-field launch (each kernel evaluation serves 32
+    // columns).
     int64_t q0 = 0;
     while (q0 < R) {
         const int64_t left = R - q0;
         if (left >= 4 || h->mfma_small) {
             const int nq = (int)std::min<int64_t>(16, left);
+            const int nq2 = (int)std::min<int64_t>(16, left - nq);
+            if (nq == 16 && nq2 > 0 && (nq2 >= 4 || h->mfma_small)) {
+                run_chunk<16, OP, MIX>(h, R, nq, q0, y, ldy, alpha, beta, evs, false);
+                run_chunk<16, OP, MIX>(h, R, nq2, q0 + 16, y, ldy, alpha, beta, evs, false);
+                run_leaf<2, OP, MIX>(h, R, nq + nq2, q0, y, ldy, alpha, beta);
+                q0 += nq + nq2;
+                continue;
+            }
             run_chunk<16, OP, MIX>(h, R, nq, q0, y, ldy, alpha, beta, evs);
             q0 += nq;
         } else if (left >= 2) { run_chunk<2, OP, MIX>(h, R, 2, q0, y, ldy, alpha, beta, evs); q0 += 2; }
