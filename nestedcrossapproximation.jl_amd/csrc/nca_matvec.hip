@@ -966,19 +966,19 @@ __global__ void __launch_bounds__(256, 4)
     }
 }
 
-// Stage 3 forward, one tile: acc += (coupling row of the tile) (source
-// coefficients), four columns per step. Software pipeline over groups of kU
-// steps, two register sets in turn: the operands of group n + 1 and the
-// coefficient indices of group n + 2 are in flight while the MFMAs of group n
-// run. Sb: the tile's r_t x C block (column-major, ur = r_t rows), rowsel:
+// Stage 3 forward, one tile: acc[c] += (coupling row of the tile) (source
+// coefficients of column chunk c), four columns per step. Software pipeline
+// over groups of kU steps, two register sets in turn: the operands of group
+// n + 1 and the coefficient indices of group n + 2 are in flight while the
+// MFMAs of group n run. Sb: the tile's r_t x C block (column-major, ur = r_t rows), rowsel:
 // the lane's row. Columns past C read A = 0 (and source index 0), so the
 // pipeline needs neither clamps nor masks.
-template <class P, bool CJ, class ACC, class XT>
-__device__ __forceinline__ void couple_run(ACC& acc, const typename P::T2* __restrict__ Sb, uint32_t ur,
+template <class P, bool CJ, int NC, class ACC, class XT>
+__device__ __forceinline__ void couple_run(ACC (&acc)[NC], const typename P::T2* __restrict__ Sb, uint32_t ur,
                                            uint32_t rowsel, const int32_t* __restrict__ cc, int C,
-                                           const XT* __restrict__ xq, uint32_t xstride, int kk) {
+                                           const XT* const (&xq)[NC], uint32_t xstride, int kk) {
     using T2 = typename P::T2;
-    constexpr int kU = 4;
+    constexpr int kU = NC == 1 ? 4 : 2;  // steps per group (two chunks: half the depth, for registers)
     constexpr uint32_t kA = sizeof(T2);
     ur = __builtin_amdgcn_readfirstlane(ur);
     C = __builtin_amdgcn_readfirstlane(C);
@@ -987,22 +987,25 @@ __device__ __forceinline__ void couple_run(ACC& acc, const typename P::T2* __res
     const __amdgpu_buffer_rsrc_t rc = __builtin_amdgcn_make_buffer_rsrc((void*)cc, (short)0, C * 4, kBufDword3);
     const uint32_t va = ((uint32_t)kk * ur + rowsel) * kA, vc = (uint32_t)kk * 4u;
     T2 a0[kU], a1[kU];
-    XT b0[kU], b1[kU];
+    XT b0[NC][kU], b1[NC][kU];
     int ci0[kU], ci1[kU];
     auto load_idx = [&](int g0, int (&ci)[kU]) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) ci[u] = buf_load<int>(rc, vc, (uint32_t)(g0 + 4 * u) * 4u);
     };
-    auto load_ops = [&](int g0, const int (&ci)[kU], T2 (&a)[kU], XT (&b)[kU]) {
+    auto load_ops = [&](int g0, const int (&ci)[kU], T2 (&a)[kU], XT (&b)[NC][kU]) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             a[u] = buf_load<T2>(ra, va, (uint32_t)(g0 + 4 * u) * ur * kA);
-            b[u] = xq[(uint32_t)ci[u] * xstride];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) b[c][u] = xq[c][(uint32_t)ci[u] * xstride];
         }
     };
-    auto mma = [&](const T2 (&a)[kU], const XT (&b)[kU]) {
+    auto mma = [&](const T2 (&a)[kU], const XT (&b)[NC][kU]) {
 #pragma unroll
-        for (int u = 0; u < kU; ++u) acc.mma(conjif<CJ, P>(a[u]), P::cvt(b[u]));
+        for (int u = 0; u < kU; ++u)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) acc[c].mma(conjif<CJ, P>(a[u]), P::cvt(b[c][u]));
     };
     load_idx(0, ci0);
     load_ops(0, ci0, a0, b0);
@@ -1020,14 +1023,15 @@ __device__ __forceinline__ void couple_run(ACC& acc, const typename P::T2* __res
     }
 }
 
-// Stage 3 forward for 16 right-hand sides: one wave per tile of at most 16
-// rows of a test cluster; A = the r_t x C coupling row, B = the source
+// Stage 3 forward for 16 NC right-hand sides (NC chunks of 16 columns: each
+// coupling entry loaded once serves all of them): one wave per tile of at
+// most 16 rows of a test cluster; A = the r_t x C coupling row, B = the source
 // coefficients of its columns. Tiles of more than 8 rows use the complex
 // (three- or four-product) accumulator, smaller ones the real embedding.
 // Control flow is uniform over the wave; offsets inside a block and into xh
 // are 32-bit (coefficients x right-hand sides < 2^31). XT: type of the source
-// coefficients (the FP64 vector, or its c32 copy in MIXED).
-template <class P, bool CJ, bool K3, class XT, class YT>
+// coefficients (the operator's, or a compact c32 copy in MIXED).
+template <class P, bool CJ, bool K3, class XT, class YT, int NC>
 __global__ void __launch_bounds__(256, 4)
     k_couple_fwd_mfma(const typename P::T2* __restrict__ S, const int32_t* __restrict__ tile_node,
                       const int32_t* __restrict__ tile_row0, int64_t ntiles, const int64_t* __restrict__ base,
@@ -1046,40 +1050,50 @@ __global__ void __launch_bounds__(256, 4)
     const int nrows = min(16, rt - row0);
     const int i = lane & 15, kk = lane >> 4, j = lane & 15;
     const int32_t* cc = colcoef + colbase[t];
-    const auto* xq = xq0 + min(j, nq - 1);
+    // columns past the live ones read the last live one (not stored)
+    const XT* xq[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) xq[c] = xq0 + min(16 * c + j, nq - 1);
     // every row of every test block with coefficients has one tile (rows
     // without far columns store zeros), so the output needs no clearing
     // fin[t]: a block no downward transfer writes takes its final value here, in FP64
     const int64_t o0 = (int64_t)(cptr_dst[t] + row0) * R + q0 + j;
     const bool tofin = fin && fin[t];
-    auto emit = [&](int row, double2 v) {
-        if (row < nrows && j < nq) {
-            if (tofin) yfin[o0 + (int64_t)row * R] = v;
-            else store_as(yh + o0 + (int64_t)row * R, v);
-        }
+    auto emitter = [&](int c) {
+        return [&, c](int row, double2 v) {
+            if (row < nrows && 16 * c + j < nq) {
+                const int64_t o = o0 + 16 * c + (int64_t)row * R;
+                if (tofin) yfin[o] = v;
+                else store_as(yh + o, v);
+            }
+        };
+    };
+    auto run = [&](auto& acc, int rowsel) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c].zero();
+        if (C > 0)
+            couple_run<P, CJ, NC>(acc, S + base[t], (uint32_t)rt, (uint32_t)(row0 + min(rowsel, nrows - 1)), cc, C,
+                                  xq, xstride, kk);
     };
     if (nrows > 8) {
-        Cacc<P, K3> acc;
-        acc.zero();
-        if (C > 0)
-            couple_run<P, CJ>(acc, S + base[t], (uint32_t)rt, (uint32_t)(row0 + min(i, nrows - 1)), cc, C, xq,
-                              xstride, kk);
+        Cacc<P, K3> acc[NC];
+        run(acc, i);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) emit(P::row(kk, v), acc.get(v));
+        for (int c = 0; c < NC; ++c) {
+            const auto emit = emitter(c);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) emit(P::row(kk, v), acc[c].get(v));
+        }
     } else if (nrows > 4) {
-        Cemb<P, 8> acc;
-        acc.zero();
-        if (C > 0)
-            couple_run<P, CJ>(acc, S + base[t], (uint32_t)rt, (uint32_t)(row0 + min(i & 7, nrows - 1)), cc, C,
-                              xq, xstride, kk);
-        acc.store(emit);
+        Cemb<P, 8> acc[NC];
+        run(acc, i & 7);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c].store(emitter(c));
     } else {
-        Cemb<P, 4> acc;
-        acc.zero();
-        if (C > 0)
-            couple_run<P, CJ>(acc, S + base[t], (uint32_t)rt, (uint32_t)(row0 + min(i & 3, nrows - 1)), cc, C,
-                              xq, xstride, kk);
-        acc.store(emit);
+        Cemb<P, 4> acc[NC];
+        run(acc, i & 3);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c].store(emitter(c));
     }
 }
 
@@ -1332,13 +1346,59 @@ void run_leaf(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy,
     NCA_HIP(hipGetLastError());
 }
 
+// Stage 3 forward on the MFMA kernel for columns q0 .. q0 + nq - 1 (nq <=
+// 16 NC): one launch for NC chunks of 16 columns, each coupling entry read
+// once for all of them.
+template <int NC, int OP, bool MIX>
+void run_couple(nca_h2* h, int64_t R, int nq, int64_t q0) {
+    static_assert(OP == NCA_OP_N, "forward coupling");
+    constexpr bool CJ = false;
+    using O = Ops<MIX>;
+    using T2 = typename O::T2;
+    using P = typename O::P;
+    hipStream_t st = h->ctx->stream;
+    SideDev& src = h->trial;
+    SideDev& dst = h->test;
+    if (h->f_ntiles == 0) return;
+    const T2* xh = reinterpret_cast<const T2*>(h->ws_xh.p);
+    double2* yh = h->ws_yh.p;
+    const T2* S = O::coupling(h);
+    // MIXED forward products: the coupling result (one FP32 tile per block)
+    // goes to the c32 buffer yc; the downward pass sums it with the parents'
+    // transfers in FP64 and stores yh once per block; the blocks no downward
+    // transfer writes take the coupling result in yh directly
+    using YT = std::conditional_t<MIX, float2, double2>;
+    YT* ycoup = MIX ? reinterpret_cast<YT*>(h->ws_yc.p) : reinterpret_cast<YT*>(yh);
+    const uint8_t* fin = MIX ? dst.orph.p : nullptr;
+    const bool k3 = h->couple_k3[MIX ? 1 : 0] != 0;
+    auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true, T2, YT, NC> : k_couple_fwd_mfma<P, CJ, false, T2, YT, NC>;
+    if (NC == 1 && MIX && h->couple_x32 && R > 16) {
+        // one 16-column chunk of more: its source coefficients as a compact
+        // copy when the coupling gathers them several times over (couple_x32)
+        hipLaunchKernelGGL(k_chunk_to_c32<T2>, dim3(blocks_for(src.K * 16, 256)), dim3(256), 0, st, xh, R, q0,
+                           src.K, nq, h->ws_xh32.p);
+        hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
+                           h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p,
+                           dst.cptr.p, reinterpret_cast<const T2*>(h->ws_xh32.p), 16u, R, q0, nq, ycoup, yh, fin,
+                           h->couple_xcd);
+    } else {
+        hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
+                           h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p,
+                           dst.cptr.p, xh + q0, (uint32_t)R, R, q0, nq, ycoup, yh, fin, h->couple_xcd);
+    }
+    NCA_HIP(hipGetLastError());
+}
+
+// Phases of one column chunk.
+enum : int { kPhaseUp = 1, kPhaseCouple = 2, kPhaseDown = 4, kPhaseLeaf = 8, kPhaseAll = 15 };
+
 // Columns q0 .. q0 + nq - 1 of R; NR = 16 runs the MFMA kernels for any
 // nq <= 16 (lanes of the columns past nq compute, and are not stored).
-// leaf = false: stop after the downward pass (the caller runs stage 5 and the
-// near field for this chunk together with the next one).
+// phases: the stages to run (the caller runs the others for this chunk
+// together with the next one).
 template <int NR, int OP, bool MIX>
 void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy, double2 alpha, double2 beta,
-               StageEvents* evs, bool leaf = true) {
+               StageEvents* evs, int phases = kPhaseAll) {
     constexpr bool FWD = OP == NCA_OP_N;
     constexpr bool CJ = OP == NCA_OP_C;
     using O = Ops<MIX>;
@@ -1354,6 +1414,9 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
     // downward contributions of every level
     T2* xh = reinterpret_cast<T2*>(h->ws_xh.p);
     double2* yh = h->ws_yh.p;
+    constexpr bool kCombine = MIX && FWD && NR == 16;
+    float2* yc = kCombine ? h->ws_yc.p : nullptr;
+    if (phases & kPhaseUp) {
     if (evs) evs->mark(1, st);
     if (src.nbasis > 0) {
         if (NR == 16)
@@ -1380,37 +1443,13 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
                                src.levptr[l], R, q0, xh);
     }
     if (evs) evs->mark(3, st);
+    }
+    if (phases & kPhaseCouple) {
     const T2* S = O::coupling(h);
-    // MIXED forward products: the coupling result (one FP32 tile per block)
-    // goes to the c32 buffer yc; the downward pass sums it with the parents'
-    // transfers in FP64 and stores yh once per block; the blocks no downward
-    // transfer writes take the coupling result in yh directly
-    constexpr bool kCombine = MIX && FWD && NR == 16;
-    float2* yc = kCombine ? h->ws_yc.p : nullptr;
-    using YT = std::conditional_t<kCombine, float2, double2>;
-    YT* ycoup = kCombine ? reinterpret_cast<YT*>(yc) : reinterpret_cast<YT*>(yh);
-    if (FWD) {
-        if (NR == 16 && h->f_ntiles > 0) {
-            const bool k3 = h->couple_k3[MIX ? 1 : 0] != 0;
-            // MIXED with more than 16 columns: the source coefficients of this
-            // chunk as a compact copy when the coupling gathers them several
-            // times over (couple_x32)
-            if (MIX && h->couple_x32 && R > 16) {
-                auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true, T2, YT> : k_couple_fwd_mfma<P, CJ, false, T2, YT>;
-                hipLaunchKernelGGL(k_chunk_to_c32<T2>, dim3(blocks_for(src.K * 16, 256)), dim3(256), 0, st, xh, R,
-                                   q0, src.K, nq, h->ws_xh32.p);
-                hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
-                                   h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
-                                   h->f_colcoef.p, dst.cptr.p, reinterpret_cast<const T2*>(h->ws_xh32.p), 16u, R,
-                                   q0, nq, ycoup, yh, kCombine ? dst.orph.p : nullptr, h->couple_xcd);
-            } else {
-                auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true, T2, YT> : k_couple_fwd_mfma<P, CJ, false, T2, YT>;
-                hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
-                                   h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p,
-                                   h->f_colcoef.p, dst.cptr.p, xh + q0, (uint32_t)R, R, q0, nq, ycoup, yh,
-                                   kCombine ? dst.orph.p : nullptr, h->couple_xcd);
-            }
-        } else if (NR != 16 && h->coupling->nentries > 0) {
+    if constexpr (FWD) {
+        if constexpr (NR == 16) {
+            run_couple<1, OP, MIX>(h, R, nq, q0);
+        } else if (h->coupling->nentries > 0) {
             hipLaunchKernelGGL((k_couple_fwd<NR, CJ, T2>), dim3((unsigned)dst.ne), dim3(kWave), 0, st, S, h->f_base.p,
                                h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
         }
@@ -1420,6 +1459,8 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
                                h->r_test.p, h->r_off.p, src.cptr.p, dst.cptr.p, xh, R, q0, nq, yh);
     }
     if (evs) evs->mark(4, st);
+    }
+    if (phases & kPhaseDown) {
     // downward: root level first; destination transfers (test E forward, trial E transposed)
     if (dst.down_by_child) {
         // child blocks with several parents: one wave per child sums over its
@@ -1450,7 +1491,8 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
                                dst.levptr[l], R, q0, yh);
     }
     if (evs) evs->mark(5, st);
-    if (!leaf) return;
+    }
+    if (!(phases & kPhaseLeaf)) return;
     if (NR == 16) {
         run_leaf<1, OP, MIX>(h, R, nq, q0, y, ldy, alpha, beta);
     } else if (dst.nleaves > 0) {
@@ -1503,8 +1545,19 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
             const int nq = (int)std::min<int64_t>(16, left);
             const int nq2 = (int)std::min<int64_t>(16, left - nq);
             if (nq == 16 && nq2 > 0 && (nq2 >= 4 || h->mfma_small)) {
-                run_chunk<16, OP, MIX>(h, R, nq, q0, y, ldy, alpha, beta, evs, false);
-                run_chunk<16, OP, MIX>(h, R, nq2, q0 + 16, y, ldy, alpha, beta, evs, false);
+                // two chunks: stages 1-2 of each, one forward MIXED coupling
+                // launch (each coupling entry read once) or one per chunk,
+                // stage 4 of each, one near-field launch
+                run_chunk<16, OP, MIX>(h, R, nq, q0, y, ldy, alpha, beta, evs, kPhaseUp);
+                run_chunk<16, OP, MIX>(h, R, nq2, q0 + 16, y, ldy, alpha, beta, evs, kPhaseUp);
+                if constexpr (OP == NCA_OP_N && MIX) {  // FP64: two chunks in one tile would spill
+                    run_couple<2, OP, MIX>(h, R, nq + nq2, q0);
+                } else {
+                    run_chunk<16, OP, MIX>(h, R, nq, q0, y, ldy, alpha, beta, evs, kPhaseCouple);
+                    run_chunk<16, OP, MIX>(h, R, nq2, q0 + 16, y, ldy, alpha, beta, evs, kPhaseCouple);
+                }
+                run_chunk<16, OP, MIX>(h, R, nq, q0, y, ldy, alpha, beta, evs, kPhaseDown);
+                run_chunk<16, OP, MIX>(h, R, nq2, q0 + 16, y, ldy, alpha, beta, evs, kPhaseDown);
                 run_leaf<2, OP, MIX>(h, R, nq + nq2, q0, y, ldy, alpha, beta);
                 q0 += nq + nq2;
                 continue;
