@@ -1,0 +1,108 @@
+"""The precision workload at its full size on the GPU path: N = 1,048,576
+Fibonacci sphere points (test = trial), k = 2, octree minvalues 64, ACA tol
+1e-4, maxrank 50, 16 seeded complex Gaussian right-hand sides.
+
+At this size the oracle's own product is not run; the checks are the
+size-independent ones: the bottom-up pivots, ranks and factors bit for bit
+against the oracle's (from the same tree, partition and Phase-1 far pivots,
+built on the host), the MIXED product within 1e-5 of the FP64 one, both within
+5 tol of dense direct summation on sampled rows, and the identities every
+product must keep: <y, A x> = <A^H y, x> and y^T (A x) = (A^T y)^T x, linearity
+across right-hand sides, and alpha/beta of mul!."""
+import numpy as np
+import pytest
+
+import nca_amd
+from nca_amd import compressors as C
+from nca_amd import geometry
+from nca_amd import nca as NC
+from oracle import oracle as O
+from helpers import Problem, oracle_factors, relerr, bitwise_equal
+
+pytestmark = pytest.mark.gpu
+
+N, K, TOL, MAXRANK, NRHS = 1048576, 2.0, 1e-4, 50, 16
+
+
+def say(msg):
+    print(f"[gpu-fullsize] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def full():
+    say(f"assemble N={N} k={K} tol={TOL}")
+    pts = geometry.fibonacci_sphere(N)
+    tr = nca_amd.TwoNTree(pts, minvalues=64)
+    tree = nca_amd.BlockTree(tr, tr)
+    cmp = C.BatchedBottomUp(C.GeometricPivoting(), C.ACA(TOL))
+    core = NC.PetrovGalerkinNCACore(nca_amd.HelmholtzKernel(K), pts, pts, tree, cmp, cmp, maxrank=MAXRANK,
+                                    ctx=nca_amd.default_context())
+    X = np.asfortranarray(geometry.complex_gaussian(np.random.default_rng(16), (N, NRHS)))
+    return pts, tree, core, X
+
+
+def product(core, precision, X, op=nca_amd.NCA_OP_N, alpha=1.0, beta=0.0, Y0=None):
+    core.set_precision(precision)
+    W = NC.H2Workspace(core, op)
+    Y = np.zeros((N, X.shape[1]), np.complex128) if Y0 is None else Y0.copy()
+    return NC.mul(Y, W, X, alpha, beta)
+
+
+def test_fullsize_pivots_equal_oracle(full):
+    """Ranks, pivots and skeleton factors of every test cluster bit for bit
+    (the trial side is the test side reused: the problem is symmetric)."""
+    pts, tree, core, _ = full
+    say("oracle bases (test side)")
+    p = Problem(pts, pts, tree, K, 1, TOL, MAXRANK)
+    res = O.bases(0, K, 1, pts, pts, tree.testtree, p.fpt[0], p.fpt[1], MAXRANK, TOL)
+    assert res[4] == 0
+    fo = oracle_factors(res, tree.testtree, p.fpt)
+    fg = core.testfactors
+    nn = tree.testtree.nnodes
+    assert np.array_equal(fg.assigned[:nn], fo.assigned[:nn])
+    for t in np.nonzero(fo.assigned[:nn])[0]:
+        assert fg.rank[t] == fo.rank[t], t
+        assert np.array_equal(fg.pivots[t], fo.pivots[t]), t
+        assert bitwise_equal(fg.X[t], fo.X[t]), t
+    tp, sp = core.testpivots, core.trialpivots
+    assert np.array_equal(tp[0], sp[0]) and np.array_equal(tp[1], sp[1])
+
+
+def test_fullsize_mixed_and_direct_summation(full):
+    """MIXED within 1e-5 of FP64 over all 16 right-hand sides; both within 5 tol
+    of dense direct summation on 256 sampled rows (4 right-hand sides)."""
+    pts, _, core, X = full
+    say("FP64 and MIXED products, 16 RHS")
+    Y64 = product(core, nca_amd.NCA_PREC_FP64, X)
+    Ymix = product(core, nca_amd.NCA_PREC_MIXED, X)
+    err = relerr(Ymix, Y64)
+    say(f"MIXED vs FP64 {err:.3e}")
+    assert err <= 1e-5
+    rows = np.sort(np.random.default_rng(7).choice(N, 256, replace=False))
+    ref = O.dense_rows(K, 1, pts, pts, rows, np.asfortranarray(X[:, :4]))
+    e64, emix = relerr(Y64[rows, :4], ref), relerr(Ymix[rows, :4], ref)
+    say(f"vs direct summation: FP64 {e64:.3e}, MIXED {emix:.3e}")
+    assert e64 < 5 * TOL and emix < 5 * TOL
+
+
+@pytest.mark.parametrize("precision,bound", [(nca_amd.NCA_PREC_FP64, 1e-12), (nca_amd.NCA_PREC_MIXED, 1e-5)])
+def test_fullsize_adjoint_transpose_linearity(full, precision, bound):
+    """<y, A x> = <A^H y, x>, y^T A x = (A^T y)^T x, A (x1 + 2i x2) = A x1 + 2i A x2
+    and mul!(y, A, x, alpha, beta) = alpha A x + beta y, at full size."""
+    _, _, core, X = full
+    say(f"identities, precision {precision}")
+    x, y = X[:, :2], X[:, 2:4]
+    Ax = product(core, precision, x)
+    AHy = product(core, precision, y, nca_amd.NCA_OP_C)
+    ATy = product(core, precision, y, nca_amd.NCA_OP_T)
+    for q in range(2):
+        lhs, rhs = np.vdot(y[:, q], Ax[:, q]), np.vdot(AHy[:, q], x[:, q])
+        assert abs(lhs - rhs) <= bound * np.linalg.norm(y[:, q]) * np.linalg.norm(Ax[:, q])
+        lhs, rhs = y[:, q] @ Ax[:, q], ATy[:, q] @ x[:, q]
+        assert abs(lhs - rhs) <= bound * np.linalg.norm(y[:, q]) * np.linalg.norm(Ax[:, q])
+    comb = product(core, precision, np.asfortranarray(x[:, :1] + 2j * x[:, 1:2]))[:, 0]
+    assert relerr(comb, Ax[:, 0] + 2j * Ax[:, 1]) <= bound
+    alpha, beta = 0.5 - 1.5j, 2.0 + 0.25j
+    Y0 = X[:, 4:6].copy()
+    got = product(core, precision, x, alpha=alpha, beta=beta, Y0=Y0)
+    assert relerr(got, alpha * Ax + beta * Y0) <= bound
