@@ -40,6 +40,9 @@ struct SideDev {
     DevBuf<int32_t> bnodes;
     DevBuf<int64_t> boff;
     DevBuf<double2> bblocks;
+    // leaf projection tiles: (basis index, first row) pairs, 16 rows each
+    int64_t nptile = 0;
+    DevBuf<int32_t> ptile;
     // transfers, grouped by level: parents [levptr[l], levptr[l+1]) of parents array
     int64_t nlev = 0;
     std::vector<int64_t> levptr;
@@ -1146,34 +1149,34 @@ __device__ __forceinline__ void gemm16_store_base(const Cacc<P, K3>& acc, int la
     }
 }
 
-// Stage 1: xh_s = V_s xs_s (TR: U_s^T). One wave per source basis; it writes
-// every coefficient of its node, so xh needs no clearing.
+// Stage 1: xh_s = V_s xs_s (TR: U_s^T). One wave per 16-row tile of a source
+// basis (ptile: basis index, first row), the tiles of a basis side by side in
+// one workgroup where they fit, so its source rows are read from the same L1;
+// the tiles cover every coefficient of the node, so xh needs no clearing.
 template <class P, bool TR, bool CJ, bool K3>
 __global__ void __launch_bounds__(256)
-    k_project_mfma(const int32_t* __restrict__ bnodes, const int64_t* __restrict__ boff,
-                   const typename P::T2* __restrict__ blocks, int64_t nbasis, const int32_t* __restrict__ lo,
-                   const int32_t* __restrict__ hi, const int32_t* __restrict__ cptr,
+    k_project_mfma(const int32_t* __restrict__ ptile, int64_t ntiles, const int32_t* __restrict__ bnodes,
+                   const int64_t* __restrict__ boff, const typename P::T2* __restrict__ blocks,
+                   const int32_t* __restrict__ lo, const int32_t* __restrict__ hi, const int32_t* __restrict__ cptr,
                    const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, int nq,
                    typename P::T2* __restrict__ xh) {
     using T2 = typename P::T2;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-    if (w >= nbasis) return;
-    const int s = bnodes[w];
+    if (w >= ntiles) return;
+    const int b = ptile[2 * w], a0 = ptile[2 * w + 1];
+    const int s = bnodes[b];
     const int r = cptr[s + 1] - cptr[s];
     const int l0 = lo[s], ns = hi[s] - l0;
-    if (r == 0 || ns == 0) return;
-    const T2* B = blocks + boff[w];
+    const T2* B = blocks + boff[b];
     const T2* xv = xs + (int64_t)l0 * R + q0;
-    for (int a0 = 0; a0 < r; a0 += 16) {
-        Cacc<P, K3> acc;
-        acc.zero();
-        gemm16_tile<P, K3>(
-            acc, lane, a0, r, ns,
-            [&](int a, int j) { return conjif<CJ, P>(B[TR ? (int64_t)j + (int64_t)a * ns : (int64_t)a + (int64_t)j * r]); },
-            [&](int j, int q) { return xv[(int64_t)j * R + min(q, nq - 1)]; });
-        gemm16_store<P, K3>(acc, lane, a0, r, xh + (int64_t)cptr[s] * R + q0, R, nq, false);
-    }
+    Cacc<P, K3> acc;
+    acc.zero();
+    gemm16_tile<P, K3>(
+        acc, lane, a0, r, ns,
+        [&](int a, int j) { return conjif<CJ, P>(B[TR ? (int64_t)j + (int64_t)a * ns : (int64_t)a + (int64_t)j * r]); },
+        [&](int j, int q) { return xv[(int64_t)j * R + min(q, nq - 1)]; });
+    gemm16_store<P, K3>(acc, lane, a0, r, xh + (int64_t)cptr[s] * R + q0, R, nq, false);
 }
 
 // Stage 2 (up): xh_p = sum_c E_c xh_c, E_c r_p x r_c (TR: r_c x r_p used
@@ -1419,11 +1422,12 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
     if (phases & kPhaseUp) {
     if (evs) evs->mark(1, st);
     if (src.nbasis > 0) {
-        if (NR == 16)
-            hipLaunchKernelGGL((k_project_mfma<P, !FWD, CJ, true>), dim3(blocks_for(src.nbasis, 4)), dim3(256), 0, st,
-                               src.bnodes.p, src.boff.p, O::basis(src), src.nbasis, src.lo.p, src.hi.p,
-                               src.node_cptr(), xs, R, q0, nq, xh);
-        else
+        if (NR == 16) {
+            if (src.nptile > 0)
+                hipLaunchKernelGGL((k_project_mfma<P, !FWD, CJ, true>), dim3(blocks_for(src.nptile, 4)), dim3(256), 0,
+                                   st, src.ptile.p, src.nptile, src.bnodes.p, src.boff.p, O::basis(src), src.lo.p,
+                                   src.hi.p, src.node_cptr(), xs, R, q0, nq, xh);
+        } else
             hipLaunchKernelGGL((k_project<NR, !FWD, CJ, T2, T2>), dim3((unsigned)src.nbasis), dim3(kWave), 0, st,
                                src.bnodes.p, src.boff.p, O::basis(src), src.lo.p, src.hi.p, src.node_cptr(), xs, R,
                                q0, xh);
@@ -1802,6 +1806,20 @@ void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, int64_t n
     sd.h_bnodes = bn;
     sd.bnodes.upload(bn, st);
     sd.boff.upload(bo, st);
+    {
+        std::vector<int32_t> pt;
+        for (int64_t i = 0; i < nbasis; ++i) {
+            const int32_t t = bn[i];
+            const int64_t r = sd.h_ncptr[t + 1] - sd.h_ncptr[t];
+            if (sd.h_hi[t] == sd.h_lo[t]) continue;
+            for (int64_t a0 = 0; a0 < r; a0 += 16) {
+                pt.push_back((int32_t)i);
+                pt.push_back((int32_t)a0);
+            }
+        }
+        sd.nptile = (int64_t)pt.size() / 2;
+        sd.ptile.upload(pt, st);
+    }
     if (nbasis) sd.bblocks.upload(reinterpret_cast<const double2*>(bblocks) + boff[0], (size_t)bo[nbasis], st);
     // transfers
     sd.nlev = nlev;
