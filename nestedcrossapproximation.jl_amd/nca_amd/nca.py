@@ -293,7 +293,12 @@ class BatchedNCA:
             raise ValueError("BatchedNCA: one test/trial point set and tree per problem")
         self.nproblems = len(testtrees)
         ttest, self.testmaps, self.testoffsets = _trees.forest(testtrees)
-        ttrial, self.trialmaps, self.trialoffsets = _trees.forest(trialtrees)
+        if all(a is b for a, b in zip(testtrees, trialtrees)):
+            # one tree per problem on both sides: one forest, so a symmetric
+            # batch (same points, same trees) assembles one side and reuses it
+            ttrial, self.trialmaps, self.trialoffsets = ttest, self.testmaps, self.testoffsets
+        else:
+            ttrial, self.trialmaps, self.trialoffsets = _trees.forest(trialtrees)
         parts = [_trees.interactions(_trees.BlockTree(a, b), eta) for a, b in zip(testtrees, trialtrees)]
         self.partitions = parts
         part = _trees.forest_partition(self.testmaps, self.trialmaps, parts, ttest.nnodes)
