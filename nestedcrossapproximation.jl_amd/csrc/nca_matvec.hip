@@ -981,7 +981,8 @@ __device__ __forceinline__ void couple_run(ACC (&acc)[NC], const typename P::T2*
                                            uint32_t rowsel, const int32_t* __restrict__ cc, int C,
                                            const XT* const (&xq)[NC], uint32_t xstride, int kk) {
     using T2 = typename P::T2;
-    constexpr int kU = NC == 1 ? 4 : 2;  // steps per group (two chunks: half the depth, for registers)
+    // steps per group: two chunks take half the depth (a quarter in FP64), for registers
+    constexpr int kU = NC == 1 ? 4 : (P::kF64 ? 1 : 2);
     constexpr uint32_t kA = sizeof(T2);
     ur = __builtin_amdgcn_readfirstlane(ur);
     C = __builtin_amdgcn_readfirstlane(C);
@@ -1554,7 +1555,7 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
                 // stage 4 of each, one near-field launch
                 run_chunk<16, OP, MIX>(h, R, nq, q0, y, ldy, alpha, beta, evs, kPhaseUp);
                 run_chunk<16, OP, MIX>(h, R, nq2, q0 + 16, y, ldy, alpha, beta, evs, kPhaseUp);
-                if constexpr (OP == NCA_OP_N && MIX) {  // FP64: two chunks in one tile would spill
+                if constexpr (OP == NCA_OP_N) {
                     run_couple<2, OP, MIX>(h, R, nq + nq2, q0);
                 } else {
                     run_chunk<16, OP, MIX>(h, R, nq, q0, y, ldy, alpha, beta, evs, kPhaseCouple);
