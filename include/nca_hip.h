@@ -156,7 +156,7 @@ int32_t nca_partition_destroy(nca_partition* p);
  * and is not vendored): n = min(maxrank, ceil(oversample |t|)) pivots, capped by
  * the far field's points; candidates an even subsample of every far node, the
  * first pivot nearest to the item's center, every next one maximizing
- * d_min^2 / d_c^(2 weight). Equal, for weight 1, to the host selection
+ * d_min^2 / d_c^(2 weight). Equal, for weights 1 and 1.5, to the host selection
  * ncat_farpivots. centers: 3 x nitems; nvals: |t| per item; ffptr/ffnodes:
  * 1-based CSR of far nodes of the other tree (other_lo/hi/perm: its 1-based
  * layout). Outputs pivptr (nitems+1, 0-based) and piv (0-based point ids);

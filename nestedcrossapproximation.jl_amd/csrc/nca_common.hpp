@@ -71,6 +71,10 @@ struct DevBuf {
         n = count;
         if (count) NCA_HIP(hipMalloc(&p, count * sizeof(T)));
     }
+    // Grow only: keeps the allocation when it already holds count elements.
+    void reserve(size_t count) {
+        if (count > n || !p) alloc(count);
+    }
     void release() {
         if (p) (void)hipFree(p);
         p = nullptr;

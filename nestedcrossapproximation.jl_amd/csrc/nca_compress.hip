@@ -558,7 +558,8 @@ static void singular_message(const nca_stack* st, const int32_t* ranks, const st
 
 // ---------------------------------------------------------------------------
 // Phase 1 on the device (the far-pivot selection of host/nca_tree.cpp,
-// select_farpivots, operation for operation; bit-identical for weight 1).
+// select_farpivots, operation for operation; bit-identical for weights 1 and
+// 1.5, whose power d^w is d and d sqrt(d) on both sides, correctly rounded).
 // One workgroup per item: candidates = an even subsample of each far node's
 // points (per point count doubled until the pool holds the item's n pivots),
 // the first pivot is the candidate nearest to the item's center, every next
@@ -722,7 +723,11 @@ __global__ void __launch_bounds__(kP1Threads)
     }
     p1_best<true>(bv, bq, redv, redq);
     int cur = bq;
-    if (weight != 1.0) {
+    if (weight == 1.5) {
+#pragma unroll
+        for (int m = 0; m < kP1PerThread; ++m)
+            if (pid[m] >= 0) d2c[m] = d2c[m] * sqrt(d2c[m]);
+    } else if (weight != 1.0) {
 #pragma unroll
         for (int m = 0; m < kP1PerThread; ++m)
             if (pid[m] >= 0) d2c[m] = pow(d2c[m], weight);

@@ -18,6 +18,7 @@
 #include <type_traits>
 #include <cstdlib>
 #include <memory>
+#include <mutex>
 
 #include "nca_common.hpp"
 #include "nca_math.hpp"
@@ -122,6 +123,17 @@ struct nca_h2 {
     // ws_yc: the c32 coupling results of MIXED forward products
     nca::DevBuf<double2> ws_xs, ws_xh, ws_yh, ws_xin, ws_yio;
     nca::DevBuf<float2> ws_xs32, ws_xh32, ws_yc;
+    // The device workspace the stage kernels of the product in flight use:
+    // the operator's own buffers above, or carved from the static scratch
+    // array of nca_h2_matvec_device_noalloc.
+    struct WsView {
+        double2* xs = nullptr;    // gathered right-hand sides, FP64
+        float2* xs32 = nullptr;   // gathered right-hand sides, c32 (MIXED)
+        double2* xh = nullptr;    // source coefficients (c32 in MIXED, same bytes)
+        double2* yh = nullptr;    // destination coefficients, FP64
+        float2* yc = nullptr;     // c32 coupling results (MIXED forward)
+        float2* xh32 = nullptr;   // compact c32 copy of a 16-column chunk's source coefficients
+    } ws;
     // c32 copy of the coupling blocks for the MIXED path
     nca::DevBuf<float2> S32;
     bool mixed_ready = false;
@@ -140,6 +152,14 @@ struct nca_h2 {
     int near_k3[2] = {1, 1}, couple_k3[2] = {1, 1};
     int near_xcd = 0, couple_xcd = 0, leaf_order = 2, couple_order = 3, near_stagger = 0;
     std::vector<int32_t> h_ncols;  // host copy: columns of every test node's coupling row
+    // Every entry point that touches the workspace, the launch plans or the
+    // precision holds this lock, so products of several host threads over one
+    // operator run one after the other (each in full) on the context stream.
+    mutable std::mutex mu;
+    // Bumped whenever a device buffer a captured sweep points to is freed or
+    // reallocated, or a launch plan changes; a graph captured at another
+    // generation refuses to launch.
+    uint64_t generation = 0;
 };
 
 // Rows of padding after the gathered right-hand sides and the tree-ordered
@@ -150,6 +170,7 @@ constexpr int64_t kNearPad = 16;
 struct nca_h2_graph {
     nca_h2* h2 = nullptr;
     int precision = NCA_PREC_FP64;
+    uint64_t generation = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -1303,7 +1324,7 @@ struct Ops<false> {
     static const T2* basis(const SideDev& s) { return s.bblocks.p; }
     static const T2* transfer(const SideDev& s) { return s.tblocks.p; }
     static const T2* coupling(const nca_h2* h) { return h->coupling->blocks.p; }
-    static T2* xs(nca_h2* h) { return h->ws_xs.p; }
+    static T2* xs(nca_h2* h) { return h->ws.xs; }
 };
 template <>
 struct Ops<true> {
@@ -1312,7 +1333,7 @@ struct Ops<true> {
     static const T2* basis(const SideDev& s) { return s.bblocks32.p; }
     static const T2* transfer(const SideDev& s) { return s.tblocks32.p; }
     static const T2* coupling(const nca_h2* h) { return h->S32.p; }
-    static T2* xs(nca_h2* h) { return h->ws_xs32.p; }
+    static T2* xs(nca_h2* h) { return h->ws.xs32; }
 };
 
 // Stage 5 and the near field on the MFMA kernel for columns q0 .. q0 + nq - 1
@@ -1328,7 +1349,7 @@ void run_leaf(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy,
     hipStream_t st = h->ctx->stream;
     SideDev& src = FWD ? h->trial : h->test;
     SideDev& dst = FWD ? h->test : h->trial;
-    const double2* yh = h->ws_yh.p;
+    const double2* yh = h->ws.yh;
     if (dst.ntiles > 0) {
         // conj(G) for a negative phase sign, again conjugated for the adjoint
         const bool neg = (h->sign < 0) != CJ;
@@ -1364,15 +1385,15 @@ void run_couple(nca_h2* h, int64_t R, int nq, int64_t q0) {
     SideDev& src = h->trial;
     SideDev& dst = h->test;
     if (h->f_ntiles == 0) return;
-    const T2* xh = reinterpret_cast<const T2*>(h->ws_xh.p);
-    double2* yh = h->ws_yh.p;
+    const T2* xh = reinterpret_cast<const T2*>(h->ws.xh);
+    double2* yh = h->ws.yh;
     const T2* S = O::coupling(h);
     // MIXED forward products: the coupling result (one FP32 tile per block)
     // goes to the c32 buffer yc; the downward pass sums it with the parents'
     // transfers in FP64 and stores yh once per block; the blocks no downward
     // transfer writes take the coupling result in yh directly
     using YT = std::conditional_t<MIX, float2, double2>;
-    YT* ycoup = MIX ? reinterpret_cast<YT*>(h->ws_yc.p) : reinterpret_cast<YT*>(yh);
+    YT* ycoup = MIX ? reinterpret_cast<YT*>(h->ws.yc) : reinterpret_cast<YT*>(yh);
     const uint8_t* fin = MIX ? dst.orph.p : nullptr;
     const bool k3 = h->couple_k3[MIX ? 1 : 0] != 0;
     auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true, T2, YT, NC> : k_couple_fwd_mfma<P, CJ, false, T2, YT, NC>;
@@ -1380,10 +1401,10 @@ void run_couple(nca_h2* h, int64_t R, int nq, int64_t q0) {
         // one 16-column chunk of more: its source coefficients as a compact
         // copy when the coupling gathers them several times over (couple_x32)
         hipLaunchKernelGGL(k_chunk_to_c32<T2>, dim3(blocks_for(src.K * 16, 256)), dim3(256), 0, st, xh, R, q0,
-                           src.K, nq, h->ws_xh32.p);
+                           src.K, nq, h->ws.xh32);
         hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
                            h->f_tile_row0.p, h->f_ntiles, h->f_base.p, h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p,
-                           dst.cptr.p, reinterpret_cast<const T2*>(h->ws_xh32.p), 16u, R, q0, nq, ycoup, yh, fin,
+                           dst.cptr.p, reinterpret_cast<const T2*>(h->ws.xh32), 16u, R, q0, nq, ycoup, yh, fin,
                            h->couple_xcd);
     } else {
         hipLaunchKernelGGL(kern, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, S, h->f_tile_node.p,
@@ -1416,10 +1437,10 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
     // product; in MIXED an FP32 one, and c32 storage halves their traffic);
     // destination coefficients in FP64, since they sum the coupling and the
     // downward contributions of every level
-    T2* xh = reinterpret_cast<T2*>(h->ws_xh.p);
-    double2* yh = h->ws_yh.p;
+    T2* xh = reinterpret_cast<T2*>(h->ws.xh);
+    double2* yh = h->ws.yh;
     constexpr bool kCombine = MIX && FWD && NR == 16;
-    float2* yc = kCombine ? h->ws_yc.p : nullptr;
+    float2* yc = kCombine ? h->ws.yc : nullptr;
     if (phases & kPhaseUp) {
     if (evs) evs->mark(1, st);
     if (src.nbasis > 0) {
@@ -1536,9 +1557,9 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     using CT = typename Ops<MIX>::T2;
     const int64_t nx16 = (src.K * R * (int64_t)sizeof(CT) + 15) / 16, ny16 = dst.K * R;
     if (nx16 > 0 && valu)
-        hipLaunchKernelGGL(k_fill, dim3(blocks_for(nx16, 256)), dim3(256), 0, st, h->ws_xh.p, nx16);
+        hipLaunchKernelGGL(k_fill, dim3(blocks_for(nx16, 256)), dim3(256), 0, st, h->ws.xh, nx16);
     if (ny16 > 0 && (valu || !FWD))
-        hipLaunchKernelGGL(k_fill, dim3(blocks_for(ny16, 256)), dim3(256), 0, st, h->ws_yh.p, ny16);
+        hipLaunchKernelGGL(k_fill, dim3(blocks_for(ny16, 256)), dim3(256), 0, st, h->ws.yh, ny16);
     // chunks of up to 16 columns on the MFMA kernels; a last chunk of 1 to 3
     // columns on the VALU kernels unless mfma_small is set. Two MFMA chunks in
     // a row share one near-field launch (each kernel evaluation serves 32
@@ -1577,6 +1598,7 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
 // c32 copies of the operator for the MIXED path (made once, on first use).
 void ensure_mixed(nca_h2* h) {
     if (h->mixed_ready) return;
+    ++h->generation;
     hipStream_t st = h->ctx->stream;
     auto conv = [&](const DevBuf<double2>& in, DevBuf<float2>& out, int64_t n) {
         out.alloc((size_t)std::max<int64_t>(n, 1));
@@ -1596,6 +1618,7 @@ void ensure_mixed(nca_h2* h) {
 void ensure_workspace(nca_h2* h, int64_t R) {
     const int64_t np = std::max(h->M, h->N), K = std::max(h->test.K, h->trial.K);
     if (R > h->ws_rhs) {
+        ++h->generation;
         h->ws_xh.alloc((size_t)std::max<int64_t>(K * R, 1));
         h->ws_yh.alloc((size_t)std::max<int64_t>(K * R, 1));
         h->ws_xs.release();
@@ -1612,8 +1635,8 @@ void ensure_workspace(nca_h2* h, int64_t R) {
     }
 }
 
-void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t ldx, double2* y, int64_t ldy,
-                   const double* alpha, const double* beta, StageEvents* evs = nullptr) {
+// Argument checks of a device product; false when there is nothing to do.
+bool check_product(const nca_h2* h, int32_t op, int64_t R, int64_t ldx, int64_t ldy) {
     NCA_REQUIRE(op >= NCA_OP_N && op <= NCA_OP_C, NCA_ERR_ARGUMENT, "nca_h2_matvec: op must be 0 (N), 1 (T) or 2 (C)");
     NCA_REQUIRE(R >= 0, NCA_ERR_ARGUMENT, "nca_h2_matvec: negative nrhs");
     const int64_t nx = op == NCA_OP_N ? h->N : h->M, ny = op == NCA_OP_N ? h->M : h->N;
@@ -1624,10 +1647,13 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
                 NCA_ERR_LIMIT,
                 "nca_h2_matvec: a near-field source range x nrhs x 16 bytes must stay below 2^32 (split the "
                 "right-hand sides)");
-    if (R == 0 || ny == 0) return;
+    return R > 0 && ny > 0;
+}
+
+// The sweep itself on the workspace h->ws (set by the caller).
+void run_sweep(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t ldx, double2* y, int64_t ldy,
+               const double* alpha, const double* beta, StageEvents* evs) {
     const bool mix = h->precision == NCA_PREC_MIXED;
-    if (mix) ensure_mixed(h);
-    ensure_workspace(h, R);
     const double2 al = make_double2(alpha ? alpha[0] : 1.0, alpha ? alpha[1] : 0.0);
     const double2 be = make_double2(beta ? beta[0] : 0.0, beta ? beta[1] : 0.0);
     if (mix) {
@@ -1639,6 +1665,15 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
         else if (op == NCA_OP_T) run_op<NCA_OP_T, false>(h, x, ldx, y, ldy, R, al, be, evs);
         else run_op<NCA_OP_C, false>(h, x, ldx, y, ldy, R, al, be, evs);
     }
+}
+
+void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t ldx, double2* y, int64_t ldy,
+                   const double* alpha, const double* beta, StageEvents* evs = nullptr) {
+    if (!check_product(h, op, R, ldx, ldy)) return;
+    if (h->precision == NCA_PREC_MIXED) ensure_mixed(h);
+    ensure_workspace(h, R);
+    h->ws = nca_h2::WsView{h->ws_xs.p, h->ws_xs32.p, h->ws_xh.p, h->ws_yh.p, h->ws_yc.p, h->ws_xh32.p};
+    run_sweep(h, op, R, x, ldx, y, ldy, alpha, beta, evs);
 }
 
 // ---------------------------------------------------------------------------
@@ -2558,6 +2593,7 @@ int32_t nca_h2_set_precision(nca_h2* h2, int32_t precision) {
         NCA_REQUIRE(h2, NCA_ERR_ARGUMENT, "nca_h2_set_precision: null operator");
         NCA_REQUIRE(precision == NCA_PREC_FP64 || precision == NCA_PREC_MIXED, NCA_ERR_ARGUMENT,
                     "nca_h2_set_precision: unknown precision");
+        std::lock_guard<std::mutex> lock(h2->mu);
         h2->precision = precision;
     });
 }
@@ -2566,6 +2602,7 @@ int32_t nca_h2_matvec_device(nca_h2* h2, int32_t op, int64_t nrhs, const void* x
                              int64_t ldy, const double* alpha, const double* beta) {
     return guarded([&] {
         NCA_REQUIRE(h2 && (nrhs == 0 || (x && y)), NCA_ERR_ARGUMENT, "nca_h2_matvec_device: invalid argument");
+        std::lock_guard<std::mutex> lock(h2->mu);
         set_device(h2->ctx);
         matvec_device(h2, op, nrhs, static_cast<const double2*>(x), ldx, static_cast<double2*>(y), ldy, alpha, beta);
     });
@@ -2575,6 +2612,7 @@ int32_t nca_h2_graph_create(nca_h2* h2, int32_t op, int64_t nrhs, const void* x,
                             int64_t ldy, const double* alpha, const double* beta, nca_h2_graph** out) {
     return guarded([&] {
         NCA_REQUIRE(h2 && out && nrhs > 0 && x && y, NCA_ERR_ARGUMENT, "nca_h2_graph_create: invalid argument");
+        std::lock_guard<std::mutex> lock(h2->mu);
         set_device(h2->ctx);
         hipStream_t st = h2->ctx->stream;
         // everything that allocates or converts runs before the capture
@@ -2584,6 +2622,7 @@ int32_t nca_h2_graph_create(nca_h2* h2, int32_t op, int64_t nrhs, const void* x,
         auto g = std::make_unique<nca_h2_graph>();
         g->h2 = h2;
         g->precision = h2->precision;
+        g->generation = h2->generation;
         NCA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         try {
             matvec_device(h2, op, nrhs, static_cast<const double2*>(x), ldx, static_cast<double2*>(y), ldy, alpha,
@@ -2603,8 +2642,12 @@ int32_t nca_h2_graph_create(nca_h2* h2, int32_t op, int64_t nrhs, const void* x,
 int32_t nca_h2_graph_launch(nca_h2_graph* g) {
     return guarded([&] {
         NCA_REQUIRE(g && g->exec, NCA_ERR_ARGUMENT, "nca_h2_graph_launch: invalid argument");
+        std::lock_guard<std::mutex> lock(g->h2->mu);
         NCA_REQUIRE(g->h2->precision == g->precision, NCA_ERR_ARGUMENT,
                     "nca_h2_graph_launch: the operator's precision changed since the capture");
+        NCA_REQUIRE(g->h2->generation == g->generation, NCA_ERR_ARGUMENT,
+                    "nca_h2_graph_launch: the operator's workspace or launch plans were reallocated since the "
+                    "capture (a product with more right-hand sides, or nca_h2_autotune); capture the sweep again");
         set_device(g->h2->ctx);
         NCA_HIP(hipGraphLaunch(g->exec, g->h2->ctx->stream));
     });
@@ -2623,12 +2666,15 @@ int32_t nca_h2_graph_destroy(nca_h2_graph* g) {
 int32_t nca_h2_autotune(nca_h2* h2, int32_t reps, nca_h2_tuning* out) {
     return guarded([&] {
         NCA_REQUIRE(h2 && reps > 0, NCA_ERR_ARGUMENT, "nca_h2_autotune: invalid argument");
+        std::lock_guard<std::mutex> lock(h2->mu);
         if (h2->M == 0 || h2->N == 0) {
             if (out) *out = nca_h2_tuning{};
             return;
         }
         set_device(h2->ctx);
         hipStream_t st = h2->ctx->stream;
+        // the tuner reorders the launch plans and may reallocate them
+        ++h2->generation;
         const int prec = h2->precision == NCA_PREC_MIXED ? 1 : 0;
         DevBuf<double2> x((size_t)std::max<int64_t>(h2->N * 16, 1)), y((size_t)std::max<int64_t>(h2->M * 16, 1));
         hipLaunchKernelGGL(k_fill, dim3(blocks_for((int64_t)x.n, 256)), dim3(256), 0, st, x.p, (int64_t)x.n);
@@ -2712,6 +2758,7 @@ int32_t nca_h2_matvec_profile(nca_h2* h2, int32_t op, int64_t nrhs, const void* 
         NCA_REQUIRE(h2 && stage_ms && (nrhs == 0 || (x && y)), NCA_ERR_ARGUMENT,
                     "nca_h2_matvec_profile: invalid argument");
         NCA_REQUIRE(nrhs <= 16, NCA_ERR_ARGUMENT, "nca_h2_matvec_profile: at most 16 right-hand sides");
+        std::lock_guard<std::mutex> lock(h2->mu);
         set_device(h2->ctx);
         StageEvents evs;
         for (auto& e : evs.ev) NCA_HIP(hipEventCreate(&e));
@@ -2738,13 +2785,14 @@ int32_t nca_h2_matvec(nca_h2* h2, int32_t op, int64_t nrhs, const double* x, int
     return guarded([&] {
         NCA_REQUIRE(h2 && (nrhs == 0 || (x && y)), NCA_ERR_ARGUMENT, "nca_h2_matvec: invalid argument");
         NCA_REQUIRE(op >= NCA_OP_N && op <= NCA_OP_C, NCA_ERR_ARGUMENT, "nca_h2_matvec: op must be 0 (N), 1 (T) or 2 (C)");
+        std::lock_guard<std::mutex> lock(h2->mu);
         set_device(h2->ctx);
         const int64_t nx = op == NCA_OP_N ? h2->N : h2->M, ny = op == NCA_OP_N ? h2->M : h2->N;
         NCA_REQUIRE(ldx >= nx && ldy >= ny, NCA_ERR_DIMENSION, "DimensionMismatch: leading dimension too small");
         if (nrhs == 0 || ny == 0) return;
         hipStream_t st = h2->ctx->stream;
-        h2->ws_xin.alloc((size_t)std::max<int64_t>(nx * nrhs, 1));
-        h2->ws_yio.alloc((size_t)(ny * nrhs));
+        h2->ws_xin.reserve((size_t)std::max<int64_t>(nx * nrhs, 1));
+        h2->ws_yio.reserve((size_t)(ny * nrhs));
         if (nx > 0)
             NCA_HIP(hipMemcpy2DAsync(h2->ws_xin.p, nx * sizeof(double2), x, ldx * sizeof(double2), nx * sizeof(double2),
                                      nrhs, hipMemcpyHostToDevice, st));
@@ -2762,6 +2810,7 @@ int32_t nca_h2_matvec(nca_h2* h2, int32_t op, int64_t nrhs, const double* x, int
 int32_t nca_h2_get_stats(const nca_h2* h2, nca_h2_stats* out) {
     return guarded([&] {
         NCA_REQUIRE(h2 && out, NCA_ERR_ARGUMENT, "nca_h2_get_stats: invalid argument");
+        std::lock_guard<std::mutex> lock(h2->mu);
         *out = h2->stats;
     });
 }

@@ -349,16 +349,18 @@ Partition interactions(const TreeView& a, const TreeView& b, const Criterion& et
 // Phase 1 stand-in for geometric (tree-mimicry) pivoting. For node t:
 //   F(t) = own far nodes, then the far nodes of every ancestor;
 //   n = min(maxrank, ceil(oversample |t|)) capped by the points F(t) holds
-//   (the reference's _clustersize takes oversample = 1);
+//   (oversample = 1 is the reference's count rule, _clustersize);
 //   candidates = an even subsample of every far node's points;
 //   first pivot = candidate nearest to the center of t, every next pivot
 //   maximizes d_min^2 / d_c^(2 weight), d_min = distance to the chosen
 //   pivots, d_c = distance to the center of t.
 // Ties go to the first candidate in far-field order. The defaults of the host
-// mirror (weight 1, oversample 2) were measured against dense direct summation:
-// 65,536-point sphere, k = 1, tol 1e-6: 2.5 tol (oversample 1: 18 tol);
-// 1M-point sphere, k = 2, tol 1e-4: 2.4 tol with 6.2e8 coupling entries
-// (oversample 1, weight 1.2: 2.0 tol, 6.9e8 entries).
+// mirror (weight 1.5, oversample 1) were measured against dense direct
+// summation on 400 rows (oracle operator): 65,536-point sphere, k = 1, tol
+// 1e-6: 1.8 tol (weight 1: 21 tol, weight 1.2: 10 tol; weight 1 with
+// oversample 2: 2.6 tol); 1M-point sphere, k = 2, tol 1e-4: 1.9 tol with
+// 7.8e8 coupling entries (weight 1.2: 2.1 tol, 6.9e8; weight 1 with
+// oversample 2: 2.5 tol, 6.2e8; weight 1: 6.9 tol).
 // farfield(t, F) fills F with the far nodes of item t; nvals(t) = |t|;
 // center(t) = its center (3 doubles).
 template <class FarField, class NVals, class Center>
@@ -404,7 +406,10 @@ Csr select_farpivots(int64_t nitems, FarField farfield, NVals nvals, Center cent
                 d2c[q] = std::max(dist2(other_pts + 3 * pool[q], ct), 1e-300);
                 if (d2c[q] < d2c[first]) first = q;
             }
-            for (int64_t q = 0; q < np; ++q) d2c[q] = std::pow(d2c[q], weight);
+            if (weight == 1.5)  // exactly as the device computes it
+                for (int64_t q = 0; q < np; ++q) d2c[q] = d2c[q] * std::sqrt(d2c[q]);
+            else if (weight != 1.0)
+                for (int64_t q = 0; q < np; ++q) d2c[q] = std::pow(d2c[q], weight);
             std::vector<int64_t>& out = result[t];
             out.reserve(n);
             int64_t cur = first;

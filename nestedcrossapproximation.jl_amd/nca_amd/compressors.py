@@ -49,11 +49,14 @@ class GeometricPivoting:
     """Phase-1 far-pivot selector (stand-in for TreeMimicryPivoting; see
     nca_tree.cpp): up to min(maxrank, oversample |t|) far points per cluster,
     spread by maximal separation relative to (distance to the cluster)^weight.
+    The default oversample = 1 is the reference's count rule
+    min(maxrank, |t|) (batchedbottomup.jl:57, _clustersize pivots.jl:15-25);
+    oversample > 1 draws more far pivots (an option, not the reference's rule).
     device: select on the GPU (nca_farpivots) when the caller passes the other
     side's device points, else on the host; both give the same pivots for
-    weight 1."""
+    weights 1 and 1.5."""
 
-    def __init__(self, weight=1.0, oversample=2.0, device=True):
+    def __init__(self, weight=1.5, oversample=1.0, device=True):
         self.weight = float(weight)
         self.oversample = float(oversample)
         self.device = bool(device)

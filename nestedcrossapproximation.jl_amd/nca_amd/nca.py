@@ -204,7 +204,9 @@ class H2Graph:
 
 class H2Workspace:
     """The H^2 matrix as a linear map over a shared core; one product at a time
-    (petrovgalerkinnca.jl:253-300)."""
+    per workspace (petrovgalerkinnca.jl:253-300). Workspaces of several threads
+    may share one core: the library holds the core's lock for the whole of each
+    product, so their products run one after the other on the device."""
 
     def __init__(self, core, op=_abi.NCA_OP_N):
         self.core = core

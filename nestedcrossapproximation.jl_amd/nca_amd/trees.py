@@ -235,7 +235,7 @@ def farfield_csr(tree, fd):
     return ptr, idx
 
 
-def farpivots(selftree, othertree, fd, maxrank, weight=1.0, oversample=2.0):
+def farpivots(selftree, othertree, fd, maxrank, weight=1.5, oversample=1.0):
     """Phase 1 (batchedbottomup.jl:257-266): geometric far pivots of every node
     of ``selftree`` among the points of ``othertree``. Returns a 0-based CSR
     (ptr, point ids)."""

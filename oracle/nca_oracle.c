@@ -862,14 +862,22 @@ void orc_dense_rows(double k, int32_t sign, const double* testpts, const double*
                     double* yout) {
     const cplx* X = (const cplx*)xin;
     cplx* Y = (cplx*)yout;
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int64_t i = 0; i < nrows; ++i) {
-        const double* x = testpts + 3 * rows[i];
-        for (int64_t q = 0; q < nrhs; ++q) {
-            cplx sum = {0.0, 0.0};
-            for (int64_t j = 0; j < N; ++j) sum = cadd(sum, cmul(entry(k, sign, x, trialpts + 3 * j), X[j + q * ldx]));
-            Y[i + q * nrows] = sum;
+#pragma omp parallel
+    {
+        /* each entry is evaluated once for all right-hand sides; every sum
+           still runs over j in increasing order */
+        cplx* sum = (cplx*)malloc(sizeof(cplx) * (size_t)(nrhs > 0 ? nrhs : 1));
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t i = 0; i < nrows; ++i) {
+            const double* x = testpts + 3 * rows[i];
+            for (int64_t q = 0; q < nrhs; ++q) sum[q].re = sum[q].im = 0.0;
+            for (int64_t j = 0; j < N; ++j) {
+                const cplx g = entry(k, sign, x, trialpts + 3 * j);
+                for (int64_t q = 0; q < nrhs; ++q) sum[q] = cadd(sum[q], cmul(g, X[j + q * ldx]));
+            }
+            for (int64_t q = 0; q < nrhs; ++q) Y[i + q * nrows] = sum[q];
         }
+        free(sum);
     }
 }
 

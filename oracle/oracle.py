@@ -190,6 +190,24 @@ def coupling(k, sign, testpts, trialpts, farptr, farnodes, testpivptr, testpiv, 
     return offs, blocks[:offs[-1]]
 
 
+def nodes_holding(lo, hi, perm, rows):
+    """Mask of the nodes whose value range [lo, hi) (tree positions, 0-based)
+    holds a point of ``rows`` (original indices); perm[position] = index."""
+    iperm = np.empty(len(perm), np.int64)
+    iperm[np.asarray(perm, np.int64)] = np.arange(len(perm))
+    pos = np.sort(iperm[np.asarray(rows, np.int64)])
+    return np.searchsorted(pos, hi) > np.searchsorted(pos, lo)
+
+
+def restrict_rows(ptr, idx, keep):
+    """The CSR (ptr, idx) with the rows where ``keep`` is false emptied."""
+    cnt = np.where(keep, np.diff(ptr), 0)
+    nptr = np.zeros(len(ptr), np.int64)
+    np.cumsum(cnt, out=nptr[1:])
+    sel = np.repeat(keep, np.diff(ptr))
+    return nptr, np.asarray(idx)[sel]
+
+
 class OracleH2:
     """Holds the 1-based arrays of an H^2 description (see nca_amd.h2desc) for orc_h2_matvec."""
 
@@ -314,12 +332,23 @@ class OracleDH2:
             (d["trial_perm"], d["trial_lo"], d["trial_hi"])
         return perm[lo[node] - 1:hi[node]] - 1
 
-    def matvec(self, X, near=True):
+    def matvec(self, X, near=True, rows=None):
+        """Y = A X; with ``rows`` (original test indices) only those rows of Y
+        are formed (the others are left zero): the coupling, the leaf
+        expansion and the near field then run only for the test clusters
+        that hold one of the rows, which is exact for those rows."""
         d = self.d
         X = np.asarray(X, np.complex128)
         vec = X.ndim == 1
         X = X.reshape(X.shape[0], -1)
         R = X.shape[1]
+        if rows is None:
+            need = np.ones(d["nn_test"], bool)
+            want = None
+        else:
+            need = nodes_holding(d["test_lo"] - 1, d["test_hi"], d["test_perm"] - 1, rows)
+            want = np.zeros(d["M"], bool)
+            want[np.asarray(rows, np.int64)] = True
         tc, sc = d["test_cptr"] - 1, d["trial_cptr"] - 1
         xh = np.zeros((sc[d["ne_trial"]], R), np.complex128)
         yh = np.zeros((tc[d["ne_test"]], R), np.complex128)
@@ -348,7 +377,7 @@ class OracleDH2:
                     xh[sc[p]:sc[p + 1]] += E @ xh[sc[c]:sc[c + 1]]
         # stage 3: per test cluster its (test direction, trial direction) blocks
         nptr, tid, sid = d["cp_nodeptr"] - 1, d["cp_tidcs"] - 1, d["cp_sidcs"] - 1
-        for t in range(d["nn_test"]):
+        for t in np.nonzero(need)[0]:
             for i in range(nptr[t], nptr[t + 1]):
                 a, b = tid[i], sid[i]
                 rt, rs = tc[a + 1] - tc[a], sc[b + 1] - sc[b]
@@ -369,21 +398,27 @@ class OracleDH2:
         # stage 5: every direction of a basis cluster adds to the cluster's values
         for i in range(d["nbnodes_test"]):
             node = d["bt_nodes"][i] - 1
-            rows = self._values(0, node)
-            m = len(rows)
+            if not need[node]:
+                continue
+            rows_ = self._values(0, node)
+            m = len(rows_)
             for l in range(d["bt_dirptr"][i] - 1, d["bt_dirptr"][i + 1] - 1):
                 e = d["bt_diridx"][l] - 1
                 r = tc[e + 1] - tc[e]
                 U = d["bt_blocks"][d["bt_off"][l]:d["bt_off"][l + 1]].reshape((m, r), order="F")
-                Y[rows] += U @ yh[tc[e]:tc[e + 1]]
+                Y[rows_] += U @ yh[tc[e]:tc[e + 1]]
         if near:
             np_, nn_ = d["nearptr"] - 1, d["nearnodes"] - 1
-            for t in range(d["nn_test"]):
+            for t in np.nonzero(need)[0]:
                 if np_[t + 1] == np_[t]:
                     continue
-                rows = self._values(0, t)
+                rows_ = self._values(0, t)
+                if want is not None:
+                    rows_ = rows_[want[rows_]]
                 cols = np.concatenate([self._values(1, s) for s in nn_[np_[t]:np_[t + 1]]])
-                Y[rows] += _near_block(self.k, self.sign, self.tp[rows], self.sp[cols]) @ X[cols]
+                Y[rows_] += _near_block(self.k, self.sign, self.tp[rows_], self.sp[cols]) @ X[cols]
+        if want is not None:
+            Y[~want] = 0.0
         return Y[:, 0] if vec else Y
 
     def dense(self, near=True):

@@ -6,6 +6,7 @@ import nca_amd
 from nca_amd import compressors as C
 from nca_amd import geometry
 from nca_amd import nca as NC
+from nca_amd import trees as T
 from oracle import oracle as O
 
 
@@ -71,6 +72,23 @@ def oracle_operator(p, ft, fs):
     off, blocks = O.coupling(p.k, p.sign, p.testpts, p.trialpts, p.tf.nodetofarnodesptr, p.tf.farnodes,
                              tp[0], tp[1], sp[0], sp[1])
     return O.OracleH2(d, p.k, p.sign, p.testpts, p.trialpts, off, blocks), (off, blocks)
+
+
+def oracle_operator_rows(p, ft, fs, rows):
+    """The oracle's operator for the rows ``rows`` only: coupling blocks are
+    assembled (and applied) for the test clusters that hold one of the rows,
+    which are all the clusters whose coupling reaches those rows through the
+    downward pass. matvec(..., rows=rows) is then exact on those rows."""
+    tt = p.tree.testtree
+    need = O.nodes_holding(tt.lo, tt.hi, tt.perm, rows)
+    fptr, fnodes = O.restrict_rows(p.tf.nodetofarnodesptr, p.tf.farnodes, need)
+    tb, tT = C.stores_from_factors(ft, tt, 0)
+    sb, sT = C.stores_from_factors(fs, p.tree.trialtree, 1)
+    far = T.FarData(fptr, fnodes)
+    d = NC.h2desc(p.tree, p.partition, far, ft, fs, tb, tT, sb, sT)
+    tp, sp = ft.pivot_csr(), fs.pivot_csr()
+    off, blocks = O.coupling(p.k, p.sign, p.testpts, p.trialpts, fptr, fnodes, tp[0], tp[1], sp[0], sp[1])
+    return O.OracleH2(d, p.k, p.sign, p.testpts, p.trialpts, off, blocks)
 
 
 def relerr(a, b):

@@ -2,13 +2,14 @@
 Fibonacci sphere points (test = trial), k = 2, octree minvalues 64, ACA tol
 1e-4, maxrank 50, 16 seeded complex Gaussian right-hand sides.
 
-At this size the oracle's own product is not run; the checks are the
-size-independent ones: the bottom-up pivots, ranks and factors bit for bit
-against the oracle's (from the same tree, partition and Phase-1 far pivots,
-built on the host), the MIXED product within 1e-5 of the FP64 one, both within
-5 tol of dense direct summation on sampled rows, and the identities every
-product must keep: <y, A x> = <A^H y, x> and y^T (A x) = (A^T y)^T x, linearity
-across right-hand sides, and alpha/beta of mul!."""
+Checks: the bottom-up pivots, ranks and factors bit for bit against the
+oracle's (from the same tree, partition and Phase-1 far pivots, built on the
+host); the FP64 product within 1e-12 and the MIXED product within 1e-5 of the
+oracle's compressed operator (its own factors and coupling blocks) on 1,000
+seeded rows over all 16 right-hand sides; both within 5 tol of dense direct
+summation on 1,000 rows; and the identities every product must keep:
+<y, A x> = <A^H y, x> and y^T (A x) = (A^T y)^T x, linearity across right-hand
+sides, and alpha/beta of mul!."""
 import numpy as np
 import pytest
 
@@ -17,7 +18,7 @@ from nca_amd import compressors as C
 from nca_amd import geometry
 from nca_amd import nca as NC
 from oracle import oracle as O
-from helpers import Problem, oracle_factors, relerr, bitwise_equal
+from helpers import Problem, oracle_factors, oracle_operator_rows, relerr, bitwise_equal
 
 pytestmark = pytest.mark.gpu
 
@@ -48,15 +49,26 @@ def product(core, precision, X, op=nca_amd.NCA_OP_N, alpha=1.0, beta=0.0, Y0=Non
     return NC.mul(Y, W, X, alpha, beta)
 
 
-def test_fullsize_pivots_equal_oracle(full):
-    """Ranks, pivots and skeleton factors of every test cluster bit for bit
-    (the trial side is the test side reused: the problem is symmetric)."""
-    pts, tree, core, _ = full
-    say("oracle bases (test side)")
+@pytest.fixture(scope="module")
+def oracle_side(full):
+    """The oracle's level driver on both sides, from the same inputs."""
+    pts, tree, _, _ = full
+    say("oracle bases (test and trial side)")
     p = Problem(pts, pts, tree, K, 1, TOL, MAXRANK)
     res = O.bases(0, K, 1, pts, pts, tree.testtree, p.fpt[0], p.fpt[1], MAXRANK, TOL)
     assert res[4] == 0
-    fo = oracle_factors(res, tree.testtree, p.fpt)
+    ft = oracle_factors(res, tree.testtree, p.fpt)
+    res = O.bases(1, K, 1, pts, pts, tree.trialtree, p.fps[0], p.fps[1], MAXRANK, TOL)
+    assert res[4] == 0
+    fs = oracle_factors(res, tree.trialtree, p.fps)
+    return p, ft, fs
+
+
+def test_fullsize_pivots_equal_oracle(full, oracle_side):
+    """Ranks, pivots and skeleton factors of every test cluster bit for bit
+    (the trial side is the test side reused: the problem is symmetric)."""
+    pts, tree, core, _ = full
+    p, fo, fs = oracle_side
     fg = core.testfactors
     nn = tree.testtree.nnodes
     assert np.array_equal(fg.assigned[:nn], fo.assigned[:nn])
@@ -66,11 +78,32 @@ def test_fullsize_pivots_equal_oracle(full):
         assert bitwise_equal(fg.X[t], fo.X[t]), t
     tp, sp = core.testpivots, core.trialpivots
     assert np.array_equal(tp[0], sp[0]) and np.array_equal(tp[1], sp[1])
+    # the oracle's own trial pass gives the same pivots as the reused test side
+    for a, b in zip(fs.pivot_csr(), sp):
+        assert np.array_equal(a, b)
+
+
+def test_fullsize_product_equals_oracle_on_rows(full, oracle_side):
+    """FP64 within 1e-12 and MIXED within 1e-5 of the oracle's compressed
+    operator (its own factors, its own coupling blocks, the near field
+    evaluated in FP64) on 1,000 seeded rows, all 16 right-hand sides."""
+    pts, _, core, X = full
+    p, ft, fs = oracle_side
+    rows = np.sort(np.random.default_rng(1000).choice(N, 1000, replace=False))
+    say("oracle product on 1,000 rows (coupling of the clusters holding them)")
+    H = oracle_operator_rows(p, ft, fs, rows)
+    ref = H.matvec(X, rows=rows)[rows]
+    Y64 = product(core, nca_amd.NCA_PREC_FP64, X)[rows]
+    Ymix = product(core, nca_amd.NCA_PREC_MIXED, X)[rows]
+    e64, emix = relerr(Y64, ref), relerr(Ymix, ref)
+    say(f"vs oracle on 1,000 rows x 16 RHS: FP64 {e64:.3e}, MIXED {emix:.3e}")
+    assert e64 <= 1e-12
+    assert emix <= 1e-5
 
 
 def test_fullsize_mixed_and_direct_summation(full):
     """MIXED within 1e-5 of FP64 over all 16 right-hand sides; both within 5 tol
-    of dense direct summation on 256 sampled rows (4 right-hand sides)."""
+    of dense direct summation on 1,000 sampled rows (4 right-hand sides)."""
     pts, _, core, X = full
     say("FP64 and MIXED products, 16 RHS")
     Y64 = product(core, nca_amd.NCA_PREC_FP64, X)
@@ -78,7 +111,7 @@ def test_fullsize_mixed_and_direct_summation(full):
     err = relerr(Ymix, Y64)
     say(f"MIXED vs FP64 {err:.3e}")
     assert err <= 1e-5
-    rows = np.sort(np.random.default_rng(7).choice(N, 256, replace=False))
+    rows = np.sort(np.random.default_rng(7).choice(N, 1000, replace=False))
     ref = O.dense_rows(K, 1, pts, pts, rows, np.asfortranarray(X[:, :4]))
     e64, emix = relerr(Y64[rows, :4], ref), relerr(Ymix[rows, :4], ref)
     say(f"vs direct summation: FP64 {e64:.3e}, MIXED {emix:.3e}")

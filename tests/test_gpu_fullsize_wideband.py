@@ -4,10 +4,13 @@ wavelengths across), 2-means tree with minvalues 64, isnearwideband(k) with
 etalf 1, etahf 5, gamma 1, BottomUp per (cluster, direction) block at tol
 1e-4, maxrank 50, 8 seeded complex Gaussian right-hand sides.
 
-The checks are the size-independent ones: ranks and pivots of all 775k test
-(cluster, direction) blocks bit for bit against the oracle's level driver on
-the same blocks and Phase-1 pivots; MIXED within 1e-5 of FP64; both within 10
-tol of dense direct summation on sampled rows (the reference's wideband test
+Checks: ranks, pivots and skeleton factors of all 775k test (cluster,
+direction) blocks bit for bit against the oracle's level driver on the same
+blocks and Phase-1 pivots; the coupling blocks of the clusters that hold 1,000
+seeded rows bit for bit against the oracle's; on those rows the FP64 product
+within 1e-12 and MIXED within 1e-5 of the oracle's directional product
+(OracleDH2) over all 8 right-hand sides; MIXED within 1e-5 of FP64; both within
+10 tol of dense direct summation on sampled rows (the reference's wideband test
 bound, test/nca/test_petrovgalerkinnca.jl:104-153); and <y, A x> = <A^H y, x>,
 y^T (A x) = (A^T y)^T x and linearity across right-hand sides."""
 import math
@@ -22,7 +25,7 @@ from nca_amd import geometry
 from nca_amd import nca as NC
 from nca_amd import wideband as W
 from oracle import oracle as O
-from helpers import oracle_factors, relerr
+from helpers import bitwise_equal, oracle_factors, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -63,6 +66,46 @@ def test_fullsize_wideband_pivots_equal_oracle(full):
     assert np.array_equal(ref.rank, f.rank)
     for a, b in zip(ref.pivot_csr(), f.pivot_csr()):
         assert np.array_equal(a, b)
+    for e in np.nonzero(f.assigned)[0]:
+        assert bitwise_equal(ref.X[e], f.X[e]), e
+
+
+def test_fullsize_wideband_product_equals_oracle_on_rows(full):
+    """On 1,000 seeded rows: the coupling blocks of every test cluster that
+    holds one of them bit for bit against the oracle's, then the FP64 product
+    within 1e-12 and MIXED within 1e-5 of the oracle's directional product
+    (its coupling blocks; factors bit for bit equal, checked above)."""
+    pts, core, X = full
+    rows = np.sort(np.random.default_rng(1000).choice(N, 1000, replace=False))
+    tt = core.tree.testtree
+    need = O.nodes_holding(tt.lo, tt.hi, tt.perm, rows)
+    td = core.testdata
+    fptr, snodes = td.diridxtofarptr, core.couplingplan[2]
+    keep = need[core.testblocks.node]
+    rptr, rnodes = O.restrict_rows(fptr, snodes, keep)
+    tp, sp = core.testpivots, core.trialpivots
+    say(f"oracle coupling for {int(keep.sum())} test direction indices")
+    roff, rblocks = O.coupling(K, 1, pts, pts, rptr, rnodes, tp[0], tp[1], sp[0], sp[1])
+    # back to the store's numbering: blocks of the other clusters have size 0
+    kept = np.nonzero(np.repeat(keep, np.diff(fptr)))[0]
+    sizes = np.zeros(len(snodes), np.int64)
+    sizes[kept] = np.diff(roff)
+    off = np.zeros(len(snodes) + 1, np.int64)
+    np.cumsum(sizes, out=off[1:])
+    goff, gblocks = core.couplingstore.download()
+    ks = sizes[kept]
+    gidx = np.repeat(goff[kept] - np.cumsum(ks) + ks, ks) + np.arange(int(ks.sum()))
+    assert bitwise_equal(gblocks[gidx], rblocks)
+    del gblocks
+    A = O.OracleDH2(core.desc, K, 1, pts, pts, off, rblocks)
+    say("oracle directional product on 1,000 rows")
+    ref = A.matvec(X, rows=rows)[rows]
+    Y64 = product(core, nca_amd.NCA_PREC_FP64, X)[rows]
+    Ymix = product(core, nca_amd.NCA_PREC_MIXED, X)[rows]
+    e64, emix = relerr(Y64, ref), relerr(Ymix, ref)
+    say(f"vs oracle on 1,000 rows x 8 RHS: FP64 {e64:.3e}, MIXED {emix:.3e}")
+    assert e64 <= 1e-12
+    assert emix <= 1e-5
 
 
 def test_fullsize_wideband_mixed_and_direct_summation(full):

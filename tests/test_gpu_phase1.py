@@ -69,6 +69,28 @@ def test_device_phase1_directional():
     assert np.array_equal(host[0], dev[0]) and np.array_equal(host[1], dev[1])
 
 
+def test_device_phase1_default_weight_bitwise():
+    """The default selector (weight 1.5, the reference's count rule): the
+    device computes d^1.5 as d sqrt(d), as the host does, so the pivots agree
+    bit for bit, with the far fields built on the device or given."""
+    pts = geometry.fibonacci_sphere(6000)
+    tr = nca_amd.TwoNTree(pts, minvalues=32)
+    tree = nca_amd.BlockTree(tr, tr)
+    tf, _ = nca_amd.fardata(tree, nca_amd.interactions(tree))
+    A = nca_amd.HIPKernelMatrix(nca_amd.HelmholtzKernel(1.0), pts, pts)
+    sel = C.GeometricPivoting()
+    assert sel.weight == 1.5 and sel.oversample == 1.0
+    host = T.farpivots(tr, tr, tf, 50)
+    dev = sel.select(tr, tr, tf, 50, A.trial)
+    assert np.array_equal(host[0], dev[0]) and np.array_equal(host[1], dev[1])
+    ff = T.farfield_csr(tr, tf)
+    devc = C.device_farpivots(A.trial, tr.center, tr.hi - tr.lo, ff[0], ff[1], tr, 50, 1.5, 1.0)
+    assert np.array_equal(devc[0], host[0]) and np.array_equal(devc[1], host[1])
+    # the count rule: min(maxrank, |t|) capped by the far field's points
+    nfar = np.array([sum(tr.nvalues(f) for f in ff[1][ff[0][t]:ff[0][t + 1]]) for t in range(tr.nnodes)])
+    assert np.array_equal(np.diff(host[0]), np.minimum(np.minimum(50, tr.hi - tr.lo), nfar))
+
+
 def test_device_phase1_weight():
     """weight != 1 goes through pow on both sides: same counts, pivots from the far field."""
     pts = geometry.fibonacci_sphere(6000)

@@ -12,7 +12,7 @@ import pytest
 import nca_amd
 from nca_amd import compressors as C, geometry, nca as NC
 from oracle import oracle as O
-from helpers import DeviceBuffers, relerr
+from helpers import DeviceBuffers, bitwise_equal, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -182,3 +182,83 @@ def test_symmetric_reuse_equals_trial_pass():
         assert np.array_equal(getattr(a.trialtransfer, name), getattr(b.trialtransfer, name))
     x = geometry.complex_gaussian(np.random.default_rng(1), (pts.shape[0], 16))
     assert np.array_equal(NC.H2Workspace(a) @ x, NC.H2Workspace(b) @ x)
+
+
+def test_threads_share_one_core(ctx):
+    """Two host threads, each with its own H2Workspace over one core, with
+    different right-hand side counts (3 and 19, so the second grows the device
+    workspace while the first may be running): every product equals the same
+    product run serially, bit for bit."""
+    import threading
+    say("two threads over one core")
+    pts = geometry.fibonacci_sphere(4000)
+    core = _core(ctx, pts)
+    rng = np.random.default_rng(33)
+    xa = geometry.complex_gaussian(rng, (pts.shape[0], 3))
+    xb = geometry.complex_gaussian(rng, (pts.shape[0], 19))
+    # serial references from a fresh core (its workspace starts empty as well)
+    ref_core = _core(ctx, pts)
+    ra, rb = NC.H2Workspace(ref_core) @ xa, NC.adjoint(NC.H2Workspace(ref_core)) @ xb
+    wa, wb = NC.H2Workspace(core), NC.adjoint(NC.H2Workspace(core))
+    out, errors = {"a": [], "b": []}, []
+
+    def run(name, w, x, n):
+        try:
+            for _ in range(n):
+                out[name].append(w @ x)
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    ta = threading.Thread(target=run, args=("a", wa, xa, 6))
+    tb = threading.Thread(target=run, args=("b", wb, xb, 6))
+    ta.start()
+    tb.start()
+    ta.join()
+    tb.join()
+    assert not errors, errors
+    assert len(out["a"]) == len(out["b"]) == 6
+    for y in out["a"]:
+        assert bitwise_equal(y, ra)
+    for y in out["b"]:
+        assert bitwise_equal(y, rb)
+
+
+def test_graph_rejects_reallocated_workspace(ctx):
+    """A captured sweep points to the operator's workspace and launch plans. A
+    product with more right-hand sides (which reallocates the workspace) or the
+    autotuner (which rewrites the plans) makes every later launch of that graph
+    fail on the host, without launching it; a graph captured afterwards runs and
+    matches the eager product."""
+    say("graph generation check")
+    pts = geometry.fibonacci_sphere(2000)
+    core = _core(ctx, pts)
+    N = pts.shape[0]
+    rng = np.random.default_rng(5)
+    X2 = geometry.complex_gaussian(rng, (N, 2))
+    dev = DeviceBuffers()
+    try:
+        dx = dev.upload(X2)
+        dy = dev.upload(np.zeros((N, 2), np.complex128))
+        g = core.graph(0, 2, dx, N, dy, N)
+        g.launch()
+        ctx.synchronize()
+        # an eager product with 32 right-hand sides grows the workspace
+        NC.H2Workspace(core) @ geometry.complex_gaussian(rng, (N, 32))
+        with pytest.raises(nca_amd.ArgumentError):
+            g.launch()
+        del g
+        g = core.graph(0, 2, dx, N, dy, N)
+        g.launch()
+        ctx.synchronize()
+        assert relerr(dev.download(dy, (N, 2)), NC.H2Workspace(core) @ X2) <= 1e-14
+        core.autotune(reps=1)
+        with pytest.raises(nca_amd.ArgumentError):
+            g.launch()
+        del g
+        g = core.graph(0, 2, dx, N, dy, N)
+        g.launch()
+        ctx.synchronize()
+        assert relerr(dev.download(dy, (N, 2)), NC.H2Workspace(core) @ X2) <= 1e-12
+        del g
+    finally:
+        dev.free()

@@ -83,10 +83,11 @@ def test_farfield_and_phase1_pivots():
     tree = nca_amd.BlockTree(tr, tr)
     part = nca_amd.interactions(tree)
     tf, _ = nca_amd.fardata(tree, part)
-    ptr, idx = nca_amd.farpivots(tr, tr, tf, 50)
-    assert np.array_equal(nca_amd.farpivots(tr, tr, tf, 50)[1], idx)  # deterministic
-    # oversample 1 is the reference's count rule _selectfarpivots/_clustersize
-    p1, i1 = nca_amd.farpivots(tr, tr, tf, 50, oversample=1.0)
+    # the default (oversample 1) is the reference's count rule _selectfarpivots/_clustersize
+    p1, i1 = nca_amd.farpivots(tr, tr, tf, 50)
+    assert np.array_equal(nca_amd.farpivots(tr, tr, tf, 50)[1], i1)  # deterministic
+    # oversample 2: the opt-in that draws twice as many far pivots
+    ptr, idx = nca_amd.farpivots(tr, tr, tf, 50, oversample=2.0)
     for t in range(tr.nnodes):
         F = nca_amd.farfield(tr, tf, t)
         own = tf.farnodesof(t)

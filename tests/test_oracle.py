@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from helpers import grid_problem, sphere_problem, oracle_assembly, oracle_operator, relerr
+from helpers import grid_problem, sphere_problem, oracle_assembly, oracle_operator, oracle_operator_rows, relerr
 from nca_amd import geometry
 
 
@@ -297,6 +297,11 @@ def test_matvec_is_the_compressed_operator(grid):
     rows = np.array([0, 5, 77, 1023])
     yr = H.matvec(x, rows=rows)
     assert relerr(yr[rows], (A @ x)[rows]) < 1e-13
+    # coupling assembled only for the clusters that hold the rows: same rows
+    Hr = oracle_operator_rows(p, ft, fs, rows)
+    assert np.array_equal(Hr.matvec(x, rows=rows)[rows], yr[rows])
+    need = O.nodes_holding(p.tree.testtree.lo, p.tree.testtree.hi, p.tree.testtree.perm, rows)
+    assert 0 < need.sum() < p.tree.testtree.nnodes
 
 
 def test_matvec_far_plus_near(grid):
@@ -391,3 +396,7 @@ def test_oracle_directional_vs_direct_summation(kind, n, lam):
         y = A.matvec(x)
         err = np.linalg.norm(y[rows] - yd) / np.linalg.norm(yd)
         assert err < 10 * tol, (tol, err)
+        # the row-restricted product (only clusters holding the rows) is exact there
+        yr = A.matvec(x, rows=rows[::5])
+        assert relerr(yr[rows[::5]], y[rows[::5]]) < 1e-14
+        assert not np.any(np.delete(yr, rows[::5]))
