@@ -283,6 +283,18 @@ int32_t nca_h2_matvec(nca_h2* h2, int32_t op, int64_t nrhs, const double* x, int
 /* Same with device pointers, enqueued on the context stream (no host sync). */
 int32_t nca_h2_matvec_device(nca_h2* h2, int32_t op, int64_t nrhs, const void* x, int64_t ldx, void* y,
                              int64_t ldy, const double* alpha, const double* beta);
+/* No counterpart in the reference: nca_h2_matvec_device that allocates
+ * nothing, for callers that cannot allocate device memory (code under hipGraph
+ * capture, tight solver loops, harnesses that forbid hipMalloc). Its workspace
+ * is a static 64 MiB device array per GPU, held until the process ends; a
+ * product that needs more (about 16 B x nrhs per point plus 40 B x nrhs per
+ * coefficient) returns NCA_ERR_LIMIT without launching anything. MIXED needs
+ * the operator's c32 copies, which nca_h2_set_precision (or creation in MIXED)
+ * makes; otherwise NCA_ERR_ARGUMENT. Every such call on one GPU shares the
+ * array: calls whose device work overlaps on one GPU (other streams, other
+ * operators) race; calls on one context stream run in order. */
+int32_t nca_h2_matvec_device_noalloc(nca_h2* h2, int32_t op, int64_t nrhs, const void* x, int64_t ldx,
+                                     void* y, int64_t ldy, const double* alpha, const double* beta);
 
 /* hipGraph capture of one device sweep, no counterpart in the reference (its
  * CUDA extension launches each kernel from Julia): nca_h2_graph_create records

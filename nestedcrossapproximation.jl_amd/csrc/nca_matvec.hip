@@ -1595,10 +1595,11 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     }
 }
 
-// c32 copies of the operator for the MIXED path (made once, on first use).
+// c32 copies of the operator for the MIXED path (made once: at creation in
+// MIXED, by nca_h2_set_precision, or on first use). It allocates once and
+// frees nothing a captured sweep uses, so the workspace generation stays.
 void ensure_mixed(nca_h2* h) {
     if (h->mixed_ready) return;
-    ++h->generation;
     hipStream_t st = h->ctx->stream;
     auto conv = [&](const DevBuf<double2>& in, DevBuf<float2>& out, int64_t n) {
         out.alloc((size_t)std::max<int64_t>(n, 1));
@@ -1674,6 +1675,58 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
     ensure_workspace(h, R);
     h->ws = nca_h2::WsView{h->ws_xs.p, h->ws_xs32.p, h->ws_xh.p, h->ws_yh.p, h->ws_yc.p, h->ws_xh32.p};
     run_sweep(h, op, R, x, ldx, y, ldy, alpha, beta, evs);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free products (nca_h2_matvec_device_noalloc). Their workspace is
+// this static device array: kScratchBytes on every GPU the library runs on,
+// held until the process ends, shared by every such call on that GPU (calls
+// whose device work overlaps on one GPU race). Host code never takes the
+// array's name as an address: scratch_address() asks the runtime for the
+// device address once per device and every later use goes through it.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) double2 scratch[kScratchBytes / sizeof(double2)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (the current device).
+void* scratch_address(int device) {
+    NCA_REQUIRE(device >= 0 && device < kMaxDevices, NCA_ERR_ARGUMENT,
+                "nca_h2_matvec_device_noalloc: device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        NCA_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        NCA_REQUIRE(ptr != nullptr, NCA_ERR_HIP, "nca_h2_matvec_device_noalloc: no device address for the scratch");
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+
+// The workspace of one product of R right-hand sides carved from `base`
+// (256-byte aligned parts); returns the bytes it needs.
+size_t carve_workspace(const nca_h2* h, int64_t R, char* base, nca_h2::WsView& v) {
+    const bool mix = h->precision == NCA_PREC_MIXED;
+    const size_t np = (size_t)std::max(h->M, h->N), K = (size_t)std::max(h->test.K, h->trial.K);
+    const size_t r = (size_t)R;
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char* {
+        char* p = base ? base + off : nullptr;
+        off += (std::max<size_t>(bytes, 16) + 255) & ~(size_t)255;
+        return p;
+    };
+    v = nca_h2::WsView{};
+    if (mix) v.xs32 = reinterpret_cast<float2*>(take((np + kNearPad) * r * sizeof(float2)));
+    else v.xs = reinterpret_cast<double2*>(take((np + kNearPad) * r * sizeof(double2)));
+    v.xh = reinterpret_cast<double2*>(take(K * r * (mix ? sizeof(float2) : sizeof(double2))));
+    v.yh = reinterpret_cast<double2*>(take(K * r * sizeof(double2)));
+    if (mix) {
+        v.yc = reinterpret_cast<float2*>(take(K * r * sizeof(float2)));
+        if (h->couple_x32 && R > 16) v.xh32 = reinterpret_cast<float2*>(take(K * 16 * sizeof(float2)));
+    }
+    return off;
 }
 
 // ---------------------------------------------------------------------------
@@ -2412,6 +2465,7 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
         h->stats.transfer_entries_trial = h->trial.transfer_entries;
         h->stats.coeffs_test = h->test.K;
         h->stats.coeffs_trial = h->trial.K;
+        if (h->precision == NCA_PREC_MIXED) ensure_mixed(h.get());
         NCA_HIP(hipStreamSynchronize(st));
         timer.mark("stats + sync");
         *out = h.release();
@@ -2595,6 +2649,12 @@ int32_t nca_h2_set_precision(nca_h2* h2, int32_t precision) {
                     "nca_h2_set_precision: unknown precision");
         std::lock_guard<std::mutex> lock(h2->mu);
         h2->precision = precision;
+        // the c32 operator copies are made here, so that later products
+        // (nca_h2_matvec_device_noalloc among them) allocate nothing for them
+        if (precision == NCA_PREC_MIXED) {
+            set_device(h2->ctx);
+            ensure_mixed(h2);
+        }
     });
 }
 
@@ -2605,6 +2665,31 @@ int32_t nca_h2_matvec_device(nca_h2* h2, int32_t op, int64_t nrhs, const void* x
         std::lock_guard<std::mutex> lock(h2->mu);
         set_device(h2->ctx);
         matvec_device(h2, op, nrhs, static_cast<const double2*>(x), ldx, static_cast<double2*>(y), ldy, alpha, beta);
+    });
+}
+
+int32_t nca_h2_matvec_device_noalloc(nca_h2* h2, int32_t op, int64_t nrhs, const void* x, int64_t ldx, void* y,
+                                     int64_t ldy, const double* alpha, const double* beta) {
+    return guarded([&] {
+        NCA_REQUIRE(h2 && (nrhs == 0 || (x && y)), NCA_ERR_ARGUMENT,
+                    "nca_h2_matvec_device_noalloc: invalid argument");
+        std::lock_guard<std::mutex> lock(h2->mu);
+        set_device(h2->ctx);
+        void* base = scratch_address(h2->ctx->device);
+        if (!check_product(h2, op, nrhs, ldx, ldy)) return;
+        NCA_REQUIRE(h2->precision != NCA_PREC_MIXED || h2->mixed_ready, NCA_ERR_ARGUMENT,
+                    "nca_h2_matvec_device_noalloc: the MIXED operator copies are missing (set the precision with "
+                    "nca_h2_set_precision first)");
+        nca_h2::WsView v;
+        const size_t need = carve_workspace(h2, nrhs, nullptr, v);
+        NCA_REQUIRE(need <= kScratchBytes, NCA_ERR_LIMIT,
+                    "nca_h2_matvec_device_noalloc: the product needs " + std::to_string(need) +
+                        " bytes of workspace, above the " + std::to_string(kScratchBytes) +
+                        "-byte scratch (fewer right-hand sides per call, or nca_h2_matvec_device)");
+        carve_workspace(h2, nrhs, static_cast<char*>(base), v);
+        h2->ws = v;
+        run_sweep(h2, op, nrhs, static_cast<const double2*>(x), ldx, static_cast<double2*>(y), ldy, alpha, beta,
+                  nullptr);
     });
 }
 

@@ -156,6 +156,14 @@ class PetrovGalerkinNCACore:
         call(hip_lib().nca_h2_matvec_device(self.handle, int(op), int(nrhs), ctypes.c_void_p(xptr), int(ldx),
                                             ctypes.c_void_p(yptr), int(ldy), _p(al), _p(be)))
 
+    def matvec_device_noalloc(self, op, nrhs, xptr, ldx, yptr, ldy, alpha=1.0, beta=0.0):
+        """matvec_device on the library's static device scratch: allocates
+        nothing (nca_h2_matvec_device_noalloc)."""
+        al = np.array([complex(alpha).real, complex(alpha).imag])
+        be = np.array([complex(beta).real, complex(beta).imag])
+        call(hip_lib().nca_h2_matvec_device_noalloc(self.handle, int(op), int(nrhs), ctypes.c_void_p(xptr),
+                                                    int(ldx), ctypes.c_void_p(yptr), int(ldy), _p(al), _p(be)))
+
     def autotune(self, reps=3):
         """Time the launch choices of the coupling and near-field kernels in the
         current precision and keep the fastest (nca_h2_autotune)."""
