@@ -110,6 +110,17 @@ struct nca_h2 {
     // reverse coupling: per trial node list of (test node, block offset)
     nca::DevBuf<int64_t> r_ptr, r_off;
     nca::DevBuf<int32_t> r_test;
+    // transposed coupling for T/C products, made on the first one: per trial
+    // block its r_s x C' row of the blocks S_ts^T (column-major, contiguous),
+    // in the forward plan's format, so the forward kernels apply it
+    bool rv_plan = false;
+    int64_t rv_ntiles = 0;
+    nca::DevBuf<int64_t> rv_base, rv_colbase;
+    nca::DevBuf<int32_t> rv_ncols, rv_colcoef, rv_tile_node, rv_tile_row0;
+    nca::DevBuf<double2> rv_blocks;   // FP64 (made for the first FP64 T/C product)
+    nca::DevBuf<float2> rv_blocks32;  // c32 (made for the first MIXED T/C product)
+    nca::DevBuf<int64_t> rv_dst;  // per block of the reverse plan (r_off order): offset in the transposed store
+    nca::DevBuf<int32_t> rv_rs;   // per block of the reverse plan: its trial rank
     // workspace
     int64_t ws_rhs = 0;
     // run a last chunk of 1..3 right-hand sides on the MFMA kernels too
@@ -200,6 +211,26 @@ __global__ void k_fill(double2* __restrict__ p, int64_t n) {
 __global__ void k_to_c32(const double2* __restrict__ in, float2* __restrict__ out, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
+
+// Transposed coupling blocks, one wave per block b of the reverse plan: the
+// r_t x r_s block at S + src[b] (column-major) goes to ST + dst[b] as r_s x r_t
+// (column-major); r_t from the test block's coefficient range, r_s given.
+template <class T2>
+__global__ void __launch_bounds__(256)
+    k_transpose_blocks(const T2* __restrict__ S, T2* __restrict__ ST, const int64_t* __restrict__ src,
+                       const int64_t* __restrict__ dst, const int32_t* __restrict__ rtest,
+                       const int32_t* __restrict__ cptr_test, const int32_t* __restrict__ rs, int64_t nb) {
+    const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= nb) return;
+    const int t = rtest[b];
+    const int rt = cptr_test[t + 1] - cptr_test[t], c = rs[b];
+    const T2* in = S + src[b];
+    T2* out = ST + dst[b];
+    for (int d = threadIdx.x & 63; d < rt * c; d += 64) {
+        const int k = d % c, a = d / c;
+        out[d] = in[a + (int64_t)k * rt];
+    }
 }
 
 // out[k*16 + q] = c32(in[k*R + q0 + q]), q < 16: one 16-column chunk of a
@@ -453,74 +484,6 @@ __device__ __forceinline__ float2 helmholtz_f32(const NearK& nk, double x0, doub
     const float c = __builtin_amdgcn_cosf(f), s = __builtin_amdgcn_sinf(f);
     const float sc = !GUARD || r2 > 2e-30 ? rf : 0.0f;
     return make_float2(c * sc, NEG ? -(s * sc) : s * sc);
-}
-
-// Stage 5 + near field, VALU form for fewer than 16 right-hand sides: one wave
-// per destination leaf; lane = (row, slot). FWD: destination = test side.
-template <int NR, bool FWD, bool CJ, class BT, class XT>
-__global__ void __launch_bounds__(kWave)
-    k_leaf_out(const int32_t* __restrict__ leaves, const int32_t* __restrict__ leafbasis,
-               const int64_t* __restrict__ boff, const BT* __restrict__ bblocks, const int32_t* __restrict__ dlo,
-               const int32_t* __restrict__ dhi, const int32_t* __restrict__ dperm, const int32_t* __restrict__ dcptr,
-               const double* __restrict__ dpts, const int64_t* __restrict__ nearptr,
-               const int32_t* __restrict__ nearnodes, const int32_t* __restrict__ slo, const int32_t* __restrict__ shi,
-               const double* __restrict__ spts, const double2* __restrict__ yh, const XT* __restrict__ xs,
-               int64_t R, int64_t q0, double kw, int sign, double2 alpha, double2 beta, double2* __restrict__ y,
-               int64_t ldy) {
-    __shared__ double2 red[kWave];
-    const int t = leaves[blockIdx.x];
-    const int bi = leafbasis[blockIdx.x];
-    const int l0 = dlo[t], nt = dhi[t] - l0;
-    const int rt = dcptr[t + 1] - dcptr[t];
-    const int lane = threadIdx.x;
-    for (int r0 = 0; r0 < nt; r0 += kWave) {
-        const int rows = min(nt - r0, kWave);
-        const int ncs = kWave / rows;
-        const int row = r0 + lane % rows, slot = lane / rows;
-        double2 acc[NR];
-#pragma unroll
-        for (int q = 0; q < NR; ++q) acc[q] = make_double2(0.0, 0.0);
-        if (slot < ncs) {
-            if (bi >= 0) {
-                const BT* B = bblocks + boff[bi];
-                for (int j = slot; j < rt; j += ncs) {
-                    // FWD: U (nt x r); else V (r x nt) transposed
-                    const double2 b = ldc<CJ>(B, FWD ? (int64_t)row + (int64_t)j * nt : (int64_t)j + (int64_t)row * rt);
-                    const double2* hv = yh + (int64_t)(dcptr[t] + j) * R + q0;
-#pragma unroll
-                    for (int q = 0; q < NR; ++q) acc[q] = cfma(b, to_d2(hv[q]), acc[q]);
-                }
-            }
-            const double x0 = dpts[3 * (l0 + row)], x1 = dpts[3 * (l0 + row) + 1], x2 = dpts[3 * (l0 + row) + 2];
-            for (int64_t e = nearptr[t]; e < nearptr[t + 1]; ++e) {
-                const int s = nearnodes[e];
-                for (int b = slo[s] + slot; b < shi[s]; b += ncs) {
-                    const double y0 = spts[3 * b], y1 = spts[3 * b + 1], y2 = spts[3 * b + 2];
-                    double2 g = FWD ? helmholtz(kw, sign, x0, x1, x2, y0, y1, y2)
-                                    : helmholtz(kw, sign, y0, y1, y2, x0, x1, x2);
-                    if (CJ) g.y = -g.y;
-                    const XT* xv = xs + (int64_t)b * R + q0;
-#pragma unroll
-                    for (int q = 0; q < NR; ++q) acc[q] = cfma(g, to_d2(xv[q]), acc[q]);
-                }
-            }
-        }
-        const int64_t dst = dperm[l0 + row];
-#pragma unroll
-        for (int q = 0; q < NR; ++q) {
-            __syncthreads();
-            red[lane] = (slot < ncs) ? acc[q] : make_double2(0.0, 0.0);
-            __syncthreads();
-            if (slot == 0) {
-                double2 sum = red[lane];
-                for (int c = 1; c < ncs; ++c) sum = cadd(sum, red[lane + c * rows]);
-                double2* out = y + dst + (q0 + q) * ldy;
-                double2 v = cmul(alpha, sum);
-                if (beta.x != 0.0 || beta.y != 0.0) v = cadd(v, cmul(beta, *out));
-                *out = v;
-            }
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -988,6 +951,155 @@ __global__ void __launch_bounds__(256, 4)
 #pragma unroll
         for (int c = 0; c < NC; ++c) acc[c].store(emitter(c));
     }
+}
+
+// Stage 5 + near field for NQ = 1..3 right-hand sides on VALU (an MFMA
+// product would pad them to 16 columns): the tiles, merged source ranges and
+// point broadcasts of k_leaf_out_mfma. Lane (i, kk) = (lane % 16, lane / 16)
+// evaluates G for row i % H and source s = 4 (i / H) + kk of each step of
+// S = 64 / H sources and accumulates G x_s for its own source (the
+// right-hand sides of sources past the range end read 0 and are masked);
+// the basis rows are spread over the same lanes (row i % H, basis columns
+// s, s + S, ...); the lanes of one row are summed by shuffles at the end.
+// Accumulation in the precision's type (FP32 for MIXED, as in its MFMA tiles).
+template <class P, int H, bool NEG, int NQ>
+__device__ __forceinline__ void small_walk(typename P::T2 (&acc)[NQ], const NearK& nk, const double2* tab, int lane,
+                                           double x0, double x1, double x2, int64_t e0, int64_t e1, int rot,
+                                           const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
+                                           const double* __restrict__ spts, const typename P::T2* __restrict__ xsq,
+                                           uint32_t R) {
+    using T2 = typename P::T2;
+    using T = typename P::T;
+    constexpr int S = 4 * (16 / H), G = 16 / H;
+    constexpr uint32_t kT = sizeof(T2);
+    const int kk = lane >> 4, ysrc = 4 * ((lane & 15) / H) + kk;
+    const uint32_t vp = G == 1 ? (uint32_t)kk * 24u + 4u * (uint32_t)min(lane & 15, 5)
+                               : (uint32_t)ysrc * 24u + 8u * (uint32_t)min(lane & 3, 2);
+    const uint32_t rowb = R * kT, vx = (uint32_t)ysrc * rowb;
+    const int nr = (int)(e1 - e0);
+    for (int c = 0; c < nr; ++c) {
+        int ec = c + (nr ? rot % nr : 0);
+        if (ec >= nr) ec -= nr;
+        const int64_t e = e0 + ec;
+        const int lo = nrlo[e], hr = nrhi[e];
+        const int hi = hr & 0x7fffffff;
+        const uint32_t n = (uint32_t)(hi - lo);
+        const __amdgpu_buffer_rsrc_t rp = buf_rsrc(spts + 3 * (int64_t)lo, n * 24u);
+        const __amdgpu_buffer_rsrc_t rx = buf_rsrc(xsq + (int64_t)lo * R, n * rowb);
+        auto steps = [&](auto guard) {
+            constexpr bool GU = decltype(guard)::value;
+            int left = (int)n;
+            uint32_t sp = 0, sx = 0;
+            for (; left > 0; left -= S, sp += S * 24u, sx += S * rowb) {
+                T2 b[NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) b[q] = buf_load<T2>(rx, vx + (uint32_t)q * kT, sx);
+                T2 g;
+                if constexpr (G == 1) {
+                    const uint32_t pw = (uint32_t)buf_load<int>(rp, vp, sp);
+                    g = P::template eval<NEG, GU>(nk, tab, x0, x1, x2, row_double<0>(pw), row_double<2>(pw),
+                                                  row_double<4>(pw));
+                } else {
+                    const double pq = buf_load<double>(rp, vp, sp);
+                    g = P::template eval<NEG, GU>(nk, tab, x0, x1, x2, quad_double<0>(pq), quad_double<1>(pq),
+                                                  quad_double<2>(pq));
+                }
+                if (ysrc >= left) g = czero2<P>();
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    acc[q].x = fma(g.x, b[q].x, fma(-g.y, b[q].y, acc[q].x));
+                    acc[q].y = fma(g.x, b[q].y, fma(g.y, b[q].x, acc[q].y));
+                }
+            }
+        };
+        if (P::kF64 || hr < 0) steps(std::true_type());
+        else steps(std::false_type());
+    }
+    (void)sizeof(T);
+}
+
+template <class P, bool FWD, bool CJ, bool NEG, int NQ>
+__global__ void __launch_bounds__(256)
+    k_leaf_out_small(const int32_t* __restrict__ tile_leaf, const int32_t* __restrict__ tile_row0, int64_t ntiles,
+                     const int32_t* __restrict__ leaves, const int32_t* __restrict__ leafbasis,
+                     const int64_t* __restrict__ boff, const typename P::T2* __restrict__ bblocks,
+                     const int32_t* __restrict__ dlo, const int32_t* __restrict__ dhi,
+                     const int32_t* __restrict__ dperm, const int32_t* __restrict__ dcptr,
+                     const double* __restrict__ dpts, const int64_t* __restrict__ nrptr,
+                     const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
+                     const double* __restrict__ spts, const double2* __restrict__ yh,
+                     const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, NearK nk,
+                     const double2* __restrict__ sctab, double2 alpha, double2 beta, double2* __restrict__ y,
+                     int64_t ldy, int order, int stagger) {
+    using T2 = typename P::T2;
+    using T = typename P::T;
+    __shared__ double2 tab[P::kTab ? kSinCosTab : 1];
+    if (P::kTab) {
+        tab[threadIdx.x] = sctab[threadIdx.x];
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
+    if (w >= ntiles) return;
+    const int li = tile_leaf[w], row0 = tile_row0[w];
+    const int t = leaves[li], bi = leafbasis[li];
+    const int l0 = dlo[t], nt = dhi[t] - l0;
+    const int nrows = min(16, nt - row0);
+    const int64_t e0 = nrptr[t], e1 = nrptr[t + 1];
+    const int rot = stagger ? w * 7 : 0;
+    const int rt = dcptr[t + 1] - dcptr[t];
+    const T2* B = bi >= 0 ? bblocks + boff[bi] : nullptr;
+    auto tile = [&](auto hval) {
+        constexpr int H = decltype(hval)::value;
+        constexpr int S = 4 * (16 / H);
+        const int rr = (lane & 15) % H, slot = 4 * ((lane & 15) / H) + (lane >> 4);
+        const int rowc = row0 + min(rr, nrows - 1);
+        T2 acc[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] = czero2<P>();
+        const double* xp = dpts + 3 * (int64_t)(l0 + rowc);
+        small_walk<P, H, NEG, NQ>(acc, nk, tab, lane, xp[0], xp[1], xp[2], e0, e1, rot, nrlo, nrhi, spts, xs + q0,
+                                  (uint32_t)R);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            acc[q].x *= (T)kInvFourPi;
+            acc[q].y *= (T)kInvFourPi;
+        }
+        if (B) {
+            const double2* yt = yh + (int64_t)dcptr[t] * R + q0;
+            for (int k = slot; k < rt; k += S) {
+                const T2 a = conjif<CJ, P>(B[FWD ? (int64_t)rowc + (int64_t)k * nt : (int64_t)k + (int64_t)rowc * rt]);
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    const T2 v = P::cvt(yt[(int64_t)k * R + q]);
+                    acc[q].x = fma(a.x, v.x, fma(-a.y, v.y, acc[q].x));
+                    acc[q].y = fma(a.x, v.y, fma(a.y, v.x, acc[q].y));
+                }
+            }
+        }
+        // sum the lanes of each row: over kk (lanes +16, +32) and over the
+        // source groups of a 16-lane row (lanes +H, +2H, ...)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+            for (int m = H; m < 64; m *= 2) {
+                acc[q].x += __shfl_xor(acc[q].x, m);
+                acc[q].y += __shfl_xor(acc[q].y, m);
+            }
+        }
+        if (lane < H && rr < nrows) {
+            double2* out = y + (int64_t)dperm[l0 + row0 + rr] + q0 * ldy;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                double2 val = cmul(alpha, make_double2((double)acc[q].x, (double)acc[q].y));
+                if (beta.x != 0.0 || beta.y != 0.0) val = cadd(val, cmul(beta, out[q * ldy]));
+                out[q * ldy] = val;
+            }
+        }
+    };
+    if (nrows > 8) tile(std::integral_constant<int, 16>());
+    else if (nrows > 4) tile(std::integral_constant<int, 8>());
+    else tile(std::integral_constant<int, 4>());
 }
 
 // Stage 3 forward, one tile: acc[c] += (coupling row of the tile) (source
@@ -1479,10 +1591,30 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
             hipLaunchKernelGGL((k_couple_fwd<NR, CJ, T2>), dim3((unsigned)dst.ne), dim3(kWave), 0, st, S, h->f_base.p,
                                h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p, dst.cptr.p, xh, R, q0, yh);
         }
-    } else {
-        if (h->coupling->nentries > 0)
+    } else if (h->coupling->nentries > 0) {
+        // the transposed store in the forward format when it is made (the
+        // forward kernels then apply S^T), else the reverse CSR walk
+        const T2* ST = MIX ? reinterpret_cast<const T2*>(h->rv_blocks32.p) : reinterpret_cast<const T2*>(h->rv_blocks.p);
+        if (h->rv_plan && ST) {
+            if constexpr (NR == 16) {
+                if (h->rv_ntiles > 0) {
+                    const bool k3 = h->couple_k3[MIX ? 1 : 0] != 0;
+                    auto kern = k3 ? k_couple_fwd_mfma<P, CJ, true, T2, double2, 1>
+                                   : k_couple_fwd_mfma<P, CJ, false, T2, double2, 1>;
+                    hipLaunchKernelGGL(kern, dim3(blocks_for(h->rv_ntiles, 4)), dim3(256), 0, st, ST,
+                                       h->rv_tile_node.p, h->rv_tile_row0.p, h->rv_ntiles, h->rv_base.p,
+                                       h->rv_colbase.p, h->rv_ncols.p, h->rv_colcoef.p, dst.cptr.p, xh + q0,
+                                       (uint32_t)R, R, q0, nq, yh, yh, nullptr, h->couple_xcd);
+                }
+            } else {
+                hipLaunchKernelGGL((k_couple_fwd<NR, CJ, T2>), dim3((unsigned)dst.ne), dim3(kWave), 0, st, ST,
+                                   h->rv_base.p, h->rv_colbase.p, h->rv_ncols.p, h->rv_colcoef.p, dst.cptr.p, xh, R,
+                                   q0, yh);
+            }
+        } else {
             hipLaunchKernelGGL((k_couple_rev<NR, CJ, T2>), dim3((unsigned)dst.ne), dim3(kWave), 0, st, S, h->r_ptr.p,
                                h->r_test.p, h->r_off.p, src.cptr.p, dst.cptr.p, xh, R, q0, nq, yh);
+        }
     }
     if (evs) evs->mark(4, st);
     }
@@ -1519,13 +1651,20 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
     if (evs) evs->mark(5, st);
     }
     if (!(phases & kPhaseLeaf)) return;
-    if (NR == 16) {
+    if constexpr (NR == 16) {
         run_leaf<1, OP, MIX>(h, R, nq, q0, y, ldy, alpha, beta);
-    } else if (dst.nleaves > 0) {
-        hipLaunchKernelGGL((k_leaf_out<NR, FWD, CJ, T2, T2>), dim3((unsigned)dst.nleaves), dim3(kWave), 0, st,
-                           dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p,
-                           dst.node_cptr(), dst.pts.p, dst.nearptr.p, dst.nearnodes.p, src.lo.p, src.hi.p, src.pts.p,
-                           yh, xs, R, q0, h->k, h->sign, alpha, beta, y, ldy);
+    } else {
+        // 1..3 columns: the near field on VALU over the leaf tiles
+        if (dst.ntiles > 0) {
+            const bool neg = (h->sign < 0) != CJ;
+            const NearK nk{h->k, h->k * (0.5 / M_PI), h->k * (0.25 / M_PI)};
+            auto kern = neg ? k_leaf_out_small<P, FWD, CJ, true, NR> : k_leaf_out_small<P, FWD, CJ, false, NR>;
+            hipLaunchKernelGGL(kern, dim3(blocks_for(dst.ntiles, 4)), dim3(256), 0, st, dst.tile_leaf.p,
+                               dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst),
+                               dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(), dst.pts.p, dst.nrptr.p, dst.nrlo.p,
+                               dst.nrhi.p, src.pts.p, yh, xs, R, q0, nk, h->sctab.p, alpha, beta, y, ldy,
+                               h->near_xcd, h->near_stagger);
+        }
         if (dst.nibasis > 0)
             hipLaunchKernelGGL((k_basis_out<FWD, CJ, T2>), dim3((unsigned)dst.nibasis), dim3(256), 0, st,
                                dst.ibslot.p, dst.bnodes.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p, dst.perm.p,
@@ -1590,7 +1729,8 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
             }
             run_chunk<16, OP, MIX>(h, R, nq, q0, y, ldy, alpha, beta, evs);
             q0 += nq;
-        } else if (left >= 2) { run_chunk<2, OP, MIX>(h, R, 2, q0, y, ldy, alpha, beta, evs); q0 += 2; }
+        } else if (left == 3) { run_chunk<3, OP, MIX>(h, R, 3, q0, y, ldy, alpha, beta, evs); q0 += 3; }
+        else if (left == 2) { run_chunk<2, OP, MIX>(h, R, 2, q0, y, ldy, alpha, beta, evs); q0 += 2; }
         else { run_chunk<1, OP, MIX>(h, R, 1, q0, y, ldy, alpha, beta, evs); q0 += 1; }
     }
 }
@@ -1615,6 +1755,9 @@ void ensure_mixed(nca_h2* h) {
     NCA_HIP(hipStreamSynchronize(st));
     h->mixed_ready = true;
 }
+
+// The transposed coupling of T/C products (defined with the plans below).
+void ensure_reverse(nca_h2* h);
 
 void ensure_workspace(nca_h2* h, int64_t R) {
     const int64_t np = std::max(h->M, h->N), K = std::max(h->test.K, h->trial.K);
@@ -1672,6 +1815,7 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
                    const double* alpha, const double* beta, StageEvents* evs = nullptr) {
     if (!check_product(h, op, R, ldx, ldy)) return;
     if (h->precision == NCA_PREC_MIXED) ensure_mixed(h);
+    if (op != NCA_OP_N) ensure_reverse(h);
     ensure_workspace(h, R);
     h->ws = nca_h2::WsView{h->ws_xs.p, h->ws_xs32.p, h->ws_xh.p, h->ws_yh.p, h->ws_yc.p, h->ws_xh32.p};
     run_sweep(h, op, R, x, ldx, y, ldy, alpha, beta, evs);
@@ -2092,6 +2236,128 @@ struct PhaseTimer {
     }
 };
 
+// Coupling tiles (16 rows of a destination block) level by level (block
+// order inside a level), each level's tiles cut into 8 contiguous runs dealt
+// so that workgroup b (4 tiles, placed on XCD b mod 8) takes its XCD's run;
+// runs are padded with empty tiles (block -1). rank: rows of every block of
+// the side sd.
+void level_xcd_tiles(const SideDev& sd, const std::vector<int32_t>& rank, std::vector<int32_t>& tnode,
+                     std::vector<int32_t>& trow) {
+    const int64_t nn = sd.nn, ne = (int64_t)rank.size();
+    const std::vector<int32_t>& enode = sd.h_enode;
+    std::vector<int64_t> dfs(nn);
+    for (int64_t t = 0; t < nn; ++t) dfs[t] = t;
+    std::sort(dfs.begin(), dfs.end(), [&](int64_t a, int64_t b) {
+        if (sd.h_lo[a] != sd.h_lo[b]) return sd.h_lo[a] < sd.h_lo[b];
+        if (sd.h_hi[a] != sd.h_hi[b]) return sd.h_hi[a] > sd.h_hi[b];
+        return a < b;
+    });
+    std::vector<int> ndepth(nn, 0);
+    std::vector<int64_t> stack;
+    for (const int64_t t : dfs) {
+        while (!stack.empty() && !(sd.h_lo[t] >= sd.h_lo[stack.back()] && sd.h_hi[t] <= sd.h_hi[stack.back()]))
+            stack.pop_back();
+        ndepth[t] = (int)stack.size();
+        stack.push_back(t);
+    }
+    std::vector<int> depth(ne, 0);
+    for (int64_t t = 0; t < ne; ++t) depth[t] = ndepth[enode[t]];
+    const int maxd = ne ? *std::max_element(depth.begin(), depth.end()) : 0;
+    for (int d = 0; d <= maxd; ++d) {
+        std::vector<int32_t> ln, lr;
+        for (int64_t t = 0; t < ne; ++t)
+            if (depth[t] == d && rank[t] > 0)
+                for (int r0 = 0; r0 < rank[t]; r0 += 16) {
+                    ln.push_back((int32_t)t);
+                    lr.push_back(r0);
+                }
+        const int64_t nblk = ((int64_t)ln.size() + 3) / 4, per = (nblk + 7) / 8;
+        for (int64_t i = 0; i < per; ++i)
+            for (int64_t x = 0; x < 8; ++x) {
+                const int64_t blk = x * per + i;
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t k = blk * 4 + u;
+                    const bool ok = blk < nblk && k < (int64_t)ln.size() && i < per;
+                    tnode.push_back(ok ? ln[k] : -1);
+                    trow.push_back(ok ? lr[k] : 0);
+                }
+            }
+    }
+}
+
+// The transposed coupling of T/C products in the current precision (made
+// once per precision; allocates and frees nothing a captured sweep uses).
+void ensure_reverse(nca_h2* h) {
+    const nca_coupling* cp = h->coupling;
+    const bool mix = h->precision == NCA_PREC_MIXED;
+    if (cp->nentries == 0 || (h->rv_plan && (mix ? h->rv_blocks32.p != nullptr : h->rv_blocks.p != nullptr)))
+        return;  // nothing to make (also the case inside a graph capture)
+    hipStream_t st = h->ctx->stream;
+    if (!h->rv_plan) {
+        const int64_t ntest = h->test.ne, ntrial = h->trial.ne, nb = cp->nblocks;
+        std::vector<int64_t> rptr;
+        std::vector<int32_t> rtest(nb);
+        // the reverse plan's order (the one r_test and r_off hold)
+        csr_transpose(
+            ntest, cp->ptr.data(), ntrial, [&](int64_t i) { return cp->nodes[i]; }, rptr,
+            [&](int64_t, int64_t t, int64_t pos) { rtest[pos] = (int32_t)t; });
+        std::vector<int64_t> base(ntrial), colbase(ntrial + 1, 0), rdst(nb);
+        std::vector<int32_t> ncols(ntrial), rs(nb);
+        for (int64_t s = 0; s < ntrial; ++s) {
+            int64_t C = 0;
+            const int32_t r = cp->trialrank[s];
+            for (int64_t e = rptr[s]; e < rptr[s + 1]; ++e) {
+                rs[e] = r;
+                rdst[e] = C;  // column of the block in its row, made absolute below
+                if (r > 0) C += cp->testrank[rtest[e]];
+            }
+            ncols[s] = (int32_t)C;
+            colbase[s + 1] = colbase[s] + C;
+        }
+        int64_t off = 0;
+        for (int64_t s = 0; s < ntrial; ++s) {
+            base[s] = off;
+            for (int64_t e = rptr[s]; e < rptr[s + 1]; ++e) rdst[e] = off + rdst[e] * cp->trialrank[s];
+            off += (int64_t)cp->trialrank[s] * ncols[s];
+        }
+        NCA_REQUIRE(off == cp->nentries, NCA_ERR_ARGUMENT, "nca_h2: coupling store and ranks disagree");
+        std::vector<int32_t> colcoef((size_t)colbase[ntrial]);
+        parallel_for(ntrial, [&](int64_t s) {
+            if (ncols[s] == 0) return;
+            int64_t c = colbase[s];
+            for (int64_t e = rptr[s]; e < rptr[s + 1]; ++e)
+                for (int j = 0; j < cp->testrank[rtest[e]]; ++j) colcoef[c++] = h->test.h_cptr[rtest[e]] + j;
+        });
+        colbase.pop_back();
+        std::vector<int32_t> tnode, trow;
+        level_xcd_tiles(h->trial, cp->trialrank, tnode, trow);
+        h->rv_ntiles = (int64_t)tnode.size();
+        h->rv_tile_node.upload(tnode, st);
+        h->rv_tile_row0.upload(trow, st);
+        h->rv_base.upload(base, st);
+        h->rv_colbase.upload(colbase, st);
+        h->rv_ncols.upload(ncols, st);
+        h->rv_colcoef.upload(colcoef, st);
+        h->rv_dst.upload(rdst, st);
+        h->rv_rs.upload(rs, st);
+        h->rv_plan = true;
+    }
+    const int64_t nb = cp->nblocks;
+    if (mix && !h->rv_blocks32.p) {
+        ensure_mixed(h);
+        h->rv_blocks32.alloc((size_t)cp->nentries);
+        hipLaunchKernelGGL(k_transpose_blocks<float2>, dim3(blocks_for(nb, 4)), dim3(256), 0, st, h->S32.p,
+                           h->rv_blocks32.p, h->r_off.p, h->rv_dst.p, h->r_test.p, h->test.cptr.p, h->rv_rs.p, nb);
+    } else if (!mix && !h->rv_blocks.p) {
+        h->rv_blocks.alloc((size_t)cp->nentries);
+        hipLaunchKernelGGL(k_transpose_blocks<double2>, dim3(blocks_for(nb, 4)), dim3(256), 0, st,
+                           cp->blocks.p, h->rv_blocks.p, h->r_off.p, h->rv_dst.p, h->r_test.p, h->test.cptr.p,
+                           h->rv_rs.p, nb);
+    }
+    NCA_HIP(hipGetLastError());
+    NCA_HIP(hipStreamSynchronize(st));
+}
+
 // Launch order of the forward-coupling tiles (16 rows of one test cluster;
 // every cluster with coefficients has tiles, also one without far columns):
 // 0 node id, 1 depth-first point order (nested ranges after their parent),
@@ -2123,50 +2389,7 @@ void order_coupling_tiles(nca_h2* h, int mode, hipStream_t st) {
                     trow.push_back(r0);
                 }
     } else {
-        // 3: level by level (node id order inside a level), each level's tiles
-        // cut into 8 contiguous runs dealt so that workgroup b (4 tiles, placed
-        // on XCD b mod 8) takes its XCD's run; runs are padded with empty
-        // tiles (node -1)
-        const SideDev& sd = h->test;
-        const int64_t nn = sd.nn;
-        std::vector<int64_t> dfs(nn);
-        for (int64_t t = 0; t < nn; ++t) dfs[t] = t;
-        std::sort(dfs.begin(), dfs.end(), [&](int64_t a, int64_t b) {
-            if (sd.h_lo[a] != sd.h_lo[b]) return sd.h_lo[a] < sd.h_lo[b];
-            if (sd.h_hi[a] != sd.h_hi[b]) return sd.h_hi[a] > sd.h_hi[b];
-            return a < b;
-        });
-        std::vector<int> ndepth(nn, 0);
-        std::vector<int64_t> stack;
-        for (const int64_t t : dfs) {
-            while (!stack.empty() && !(sd.h_lo[t] >= sd.h_lo[stack.back()] && sd.h_hi[t] <= sd.h_hi[stack.back()]))
-                stack.pop_back();
-            ndepth[t] = (int)stack.size();
-            stack.push_back(t);
-        }
-        std::vector<int> depth(ntest, 0);
-        for (int64_t t = 0; t < ntest; ++t) depth[t] = ndepth[enode[t]];
-        const int maxd = ntest ? *std::max_element(depth.begin(), depth.end()) : 0;
-        for (int d = 0; d <= maxd; ++d) {
-            std::vector<int32_t> ln, lr;
-            for (int64_t t = 0; t < ntest; ++t)
-                if (depth[t] == d && h->coupling->testrank[t] > 0)
-                    for (int r0 = 0; r0 < h->coupling->testrank[t]; r0 += 16) {
-                        ln.push_back((int32_t)t);
-                        lr.push_back(r0);
-                    }
-            const int64_t nblk = ((int64_t)ln.size() + 3) / 4, per = (nblk + 7) / 8;
-            for (int64_t i = 0; i < per; ++i)
-                for (int64_t x = 0; x < 8; ++x) {
-                    const int64_t blk = x * per + i;
-                    for (int u = 0; u < 4; ++u) {
-                        const int64_t k = blk * 4 + u;
-                        const bool ok = blk < nblk && k < (int64_t)ln.size() && i < per;
-                        tnode.push_back(ok ? ln[k] : -1);
-                        trow.push_back(ok ? lr[k] : 0);
-                    }
-                }
-        }
+        level_xcd_tiles(h->test, h->coupling->testrank, tnode, trow);
     }
     h->f_ntiles = (int64_t)tnode.size();
     h->f_tile_node.upload(tnode, st);
@@ -2702,6 +2925,7 @@ int32_t nca_h2_graph_create(nca_h2* h2, int32_t op, int64_t nrhs, const void* x,
         hipStream_t st = h2->ctx->stream;
         // everything that allocates or converts runs before the capture
         if (h2->precision == NCA_PREC_MIXED) ensure_mixed(h2);
+        if (op != NCA_OP_N) ensure_reverse(h2);
         ensure_workspace(h2, nrhs);
         NCA_HIP(hipStreamSynchronize(st));
         auto g = std::make_unique<nca_h2_graph>();
