@@ -14,8 +14,8 @@ over all problems (64 x 32 x steps / time).
 
 --workload wideband262k: directional H^2 on N = 262,144 Fibonacci points of the
 unit sphere, k = 16 pi, 2-means tree with minvalues 64, isnearwideband(k),
-BottomUp per (cluster, direction) at tol 1e-4, 8 seeded complex Gaussian
-right-hand sides; value = products per second of the MIXED sweep.
+BottomUp with the IACA per (cluster, direction) at tol 1e-4, 8 seeded complex
+Gaussian right-hand sides; value = products per second of the MIXED sweep.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -505,7 +505,7 @@ def bench_batch(args):
 def bench_wideband(args):
     """Directional (wideband) H^2: N points of the unit sphere, k = 16 pi
     (16 wavelengths across), 2-means tree with minvalues 64, isnearwideband(k)
-    (etalf 1, etahf 5, gamma 1), BottomUp per (cluster, direction) at tol,
+    (etalf 1, etahf 5, gamma 1), BottomUp with the IACA per (cluster, direction) at tol,
     nrhs right-hand sides; FP64 and MIXED sweeps."""
     from nca_amd import directional as D
     from nca_amd import wideband as W
@@ -518,7 +518,7 @@ def bench_wideband(args):
     t_tree = time.perf_counter() - t0
     isn = D.isnearwideband(args.k)
     ctx = nca_amd.Context(0)
-    comp = W.BottomUp(C.ACA(args.tol))
+    comp = W.BottomUp(C.IACA(args.tol))
     core = W.DirectionalNCACore(nca_amd.HelmholtzKernel(args.k), pts, pts, nca_amd.BlockTree(tr, tr), comp, comp,
                                 isn, maxrank=args.maxrank, ctx=ctx)
     tm = core.timings

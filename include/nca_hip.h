@@ -124,6 +124,26 @@ int32_t nca_compress_level(nca_ctx* ctx, const nca_stack* st, double tol, int32_
                            int64_t* rowpivots, int64_t* colpivots, int64_t xcapacity, int64_t* xoffsets,
                            double* X, int64_t* independent);
 
+/* Batched IACA: the factorization BottomUp runs per (cluster, direction)
+ * block with IACA(MaximumValue(), TreeMimicryPivoting(...),
+ * FNormExtrapolator(tol)) (compute_test_pivots!/compute_trial_pivots!
+ * src/pivoting/pivots.jl:215-364 through the IACA _compute_raw_pivots! :348-364, called
+ * from src/compressors/bottomup.jl:214-424). Each block's columns are its far
+ * candidates in geometric order (the stand-in for tree mimicry; trial blocks
+ * assembled transposed); step r takes column r, the row pivot is the first
+ * maximum of its residual column, and the candidate stops the factorization
+ * (uncounted) when r ||R_r(:, r)||^2 <= tol^2 sum_{l<r} ||A(:, l)||^2, its
+ * residual is zero, or r reaches min(maxrank, m, n). Outputs as
+ * nca_aca_fullpiv; the colpivots are 1..rank. The IACA and FNormExtrapolator of
+ * AdaptiveCrossApproximation.jl are not vendored: this is the published method
+ * restated, not that package's arithmetic. */
+int32_t nca_iaca(nca_ctx* ctx, const nca_stack* st, double tol, int32_t* ranks, int64_t* rowpivots,
+                 int64_t* colpivots);
+/* nca_iaca + the skeleton solve, as nca_compress_level. */
+int32_t nca_compress_level_iaca(nca_ctx* ctx, const nca_stack* st, double tol, int32_t* ranks, int64_t* rowpivots,
+                                int64_t* colpivots, int64_t xcapacity, int64_t* xoffsets, double* X,
+                                int64_t* independent);
+
 /* ---- admissibility partition on the device ------------------------------- */
 typedef struct nca_partition nca_partition;
 /* Near and far cluster pairs of a test and a trial tree, the result of

@@ -195,6 +195,133 @@ __global__ void __launch_bounds__(kWG)
     if (t == 0) ranks[b] = r;
 }
 
+// Batched incomplete ACA (IACA) with geometric column pivoting, one workgroup
+// per block (the method of the incomplete adaptive cross approximation with
+// tree-mimicry pivoting: the column pivots come from the geometry, the row
+// pivots from the residual columns by maximum value). The block's columns are
+// the candidates in their geometric order (Phase 1), so step r takes column
+// j = r; its residual R_r(:, j) gives the row pivot i = first argmax
+// |R_r(a, j)|^2. Stop, without counting the candidate, when r reaches
+// min(maxrank, m, n), when the candidate's residual column is zero, or when
+//   r ||R_r(:, j)||^2 <= tol^2 sum_{l < r} ||A(:, l)||^2     (r > 0),
+// i.e. the candidate column is represented to tol relative to the mean
+// accepted column. The residual update is k_aca's: R -= R(:, j) (R(i, :) /
+// R(i, j)) with row i and column j set to zero; the column sums use k_aca's
+// fixed order (slot e mod 256 over the column-major entries, halving tree).
+template <bool kLds>
+__global__ void __launch_bounds__(kWG)
+    k_iaca(const double2* __restrict__ buf, double2* __restrict__ work, int64_t ld,
+           const int64_t* __restrict__ rowoff, const int32_t* __restrict__ ncols,
+           const int32_t* __restrict__ blocklist, int64_t maxrank, double tol, int32_t* __restrict__ ranks,
+           int64_t* __restrict__ rowpiv, int64_t* __restrict__ colpiv) {
+    extern __shared__ double2 dyn[];
+    __shared__ ArgRed red;
+    const int t = threadIdx.x;
+    const int b = blocklist[blockIdx.x];
+    const int64_t r0 = rowoff[b];
+    const int m = (int)(rowoff[b + 1] - r0);
+    const int n = ncols[b];
+    for (int64_t i = t; i < maxrank; i += kWG) {
+        rowpiv[i + b * maxrank] = 0;
+        colpiv[i + b * maxrank] = 0;
+    }
+    int kmax = (int)maxrank;
+    kmax = m < kmax ? m : kmax;
+    kmax = n < kmax ? n : kmax;
+    if (kmax <= 0) {
+        if (t == 0) ranks[b] = 0;
+        return;
+    }
+    double2* R;
+    int64_t ldr;
+    double2 *u, *v;
+    if (kLds) {
+        R = dyn;
+        ldr = m;
+        u = dyn + kLdsCap;
+        v = u + kLdsMaxM;
+    } else {
+        R = work + r0;
+        ldr = ld;
+        u = dyn;
+        v = dyn + kGlobMaxM;
+    }
+    const int mn = m * n;
+    const int qstep = kWG / m, rstep = kWG % m;
+    // the copy, and the sums of column 0 (its residual is the original)
+    double p = 0.0, bv = -1.0;
+    int bi = 0;
+    {
+        int a = t % m, c = t / m;
+        for (int e = t; e < mn; e += kWG) {
+            const double2 val = buf[r0 + a + (int64_t)c * ld];
+            R[a + c * ldr] = val;
+            if (c == 0) {
+                const double w = cabs2(val);
+                p += w;
+                if (w > bv) { bv = w; bi = a; }
+            }
+            a += rstep;
+            c += qstep;
+            if (a >= m) { a -= m; ++c; }
+        }
+    }
+    double cn2, bmax;
+    int bidx;
+    reduce_sum_argmax(red, p, bv, bi, cn2, bmax, bidx);
+    double orig2 = cn2;  // ||A(:, j)||^2 of the candidate
+    double acc = 0.0;    // sum of ||A(:, l)||^2 over the accepted columns
+    const double tol2 = tol * tol;
+    int r = 0;
+    while (r < kmax) {
+        if (r > 0 && cn2 * (double)r <= tol2 * acc) break;
+        if (!(bmax > 0.0)) break;
+        const int i = bidx, j = r;
+        const double2 piv = R[i + j * ldr];
+        const double d = cabs2(piv);
+        const double2 inv = make_double2(piv.x / d, -piv.y / d);
+        __syncthreads();
+        for (int a = t; a < m; a += kWG) u[a] = R[a + j * ldr];
+        for (int c = t; c < n; c += kWG) v[c] = cmul(R[i + c * ldr], inv);
+        if (t == 0) {
+            rowpiv[r + b * maxrank] = i + 1;
+            colpiv[r + b * maxrank] = j + 1;
+        }
+        acc += orig2;
+        ++r;
+        __syncthreads();
+        p = 0.0;
+        double q = 0.0;
+        bv = -1.0;
+        bi = 0;
+        int a = t % m, c = t / m;
+        for (int e = t; e < mn; e += kWG) {
+            double2 val;
+            if (a == i || c == j) {
+                val = make_double2(0.0, 0.0);
+            } else {
+                val = csub(R[a + c * ldr], cmul(u[a], v[c]));
+            }
+            R[a + c * ldr] = val;
+            if (c == r) {
+                const double w = cabs2(val);
+                p += w;
+                if (w > bv) { bv = w; bi = a; }
+                q += cabs2(buf[r0 + a + (int64_t)c * ld]);
+            }
+            a += rstep;
+            c += qstep;
+            if (a >= m) { a -= m; ++c; }
+        }
+        reduce_sum_argmax(red, p, bv, bi, cn2, bmax, bidx);
+        __syncthreads();
+        double unusedv;
+        int unusedi;
+        reduce_sum_argmax(red, q, -1.0, 0, orig2, unusedv, unusedi);
+    }
+    if (t == 0) ranks[b] = r;
+}
+
 // ---------------------------------------------------------------------------
 // Skeleton solve, one workgroup per block: LU with partial pivoting (pivot =
 // first max |re|+|im|, LAPACK getrf form) of P = A[rowpiv, colpiv] in LDS, then
@@ -478,7 +605,9 @@ struct AcaResult {
     DevBuf<int64_t> rowpiv, colpiv;
 };
 
-static void launch_aca(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, AcaResult& res) {
+// iaca: the geometric-column IACA (k_iaca) instead of the fully pivoted ACA.
+static void launch_aca(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, AcaResult& res,
+                       bool iaca = false) {
     const int64_t nb = st->nb;
     res.ranks.alloc(std::max<int64_t>(nb, 1));
     res.rowpiv.alloc(std::max<int64_t>(nb * st->maxrank, 1));
@@ -507,18 +636,28 @@ static void launch_aca(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sa
     if (!small.empty()) {
         d_small.upload(small, ctx->stream);
         const size_t lds = sizeof(double2) * (kLdsCap + kLdsMaxM + kLdsMaxN);
-        hipLaunchKernelGGL(k_aca<true>, dim3((unsigned)small.size()), dim3(kWG), lds, ctx->stream, st->buffer.p,
-                           nullptr, st->nrows, d_rowoff.p, d_ncols.p, d_small.p, st->maxrank, tol, (int)sampled,
-                           res.ranks.p, res.rowpiv.p, res.colpiv.p);
+        if (iaca)
+            hipLaunchKernelGGL(k_iaca<true>, dim3((unsigned)small.size()), dim3(kWG), lds, ctx->stream, st->buffer.p,
+                               nullptr, st->nrows, d_rowoff.p, d_ncols.p, d_small.p, st->maxrank, tol, res.ranks.p,
+                               res.rowpiv.p, res.colpiv.p);
+        else
+            hipLaunchKernelGGL(k_aca<true>, dim3((unsigned)small.size()), dim3(kWG), lds, ctx->stream, st->buffer.p,
+                               nullptr, st->nrows, d_rowoff.p, d_ncols.p, d_small.p, st->maxrank, tol, (int)sampled,
+                               res.ranks.p, res.rowpiv.p, res.colpiv.p);
         NCA_HIP(hipGetLastError());
     }
     if (!big.empty()) {
         d_big.upload(big, ctx->stream);
         work.alloc((size_t)(st->nrows * st->maxrank));
         const size_t lds = sizeof(double2) * (kGlobMaxM + kGlobMaxN);
-        hipLaunchKernelGGL(k_aca<false>, dim3((unsigned)big.size()), dim3(kWG), lds, ctx->stream, st->buffer.p,
-                           work.p, st->nrows, d_rowoff.p, d_ncols.p, d_big.p, st->maxrank, tol, (int)sampled,
-                           res.ranks.p, res.rowpiv.p, res.colpiv.p);
+        if (iaca)
+            hipLaunchKernelGGL(k_iaca<false>, dim3((unsigned)big.size()), dim3(kWG), lds, ctx->stream, st->buffer.p,
+                               work.p, st->nrows, d_rowoff.p, d_ncols.p, d_big.p, st->maxrank, tol, res.ranks.p,
+                               res.rowpiv.p, res.colpiv.p);
+        else
+            hipLaunchKernelGGL(k_aca<false>, dim3((unsigned)big.size()), dim3(kWG), lds, ctx->stream, st->buffer.p,
+                               work.p, st->nrows, d_rowoff.p, d_ncols.p, d_big.p, st->maxrank, tol, (int)sampled,
+                               res.ranks.p, res.rowpiv.p, res.colpiv.p);
         NCA_HIP(hipGetLastError());
     }
     NCA_HIP(hipStreamSynchronize(ctx->stream));
@@ -841,16 +980,18 @@ int32_t nca_skeleton_solve(nca_ctx* ctx, const nca_stack* st, const int32_t* ran
     });
 }
 
-int32_t nca_compress_level(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, int32_t* ranks,
-                           int64_t* rowpivots, int64_t* colpivots, int64_t xcapacity, int64_t* xoffsets,
-                           double* X, int64_t* independent) {
-    return guarded([&] {
+}  // extern "C"
+
+namespace nca {
+static void compress_level_impl(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, bool iaca,
+                                int32_t* ranks, int64_t* rowpivots, int64_t* colpivots, int64_t xcapacity,
+                                int64_t* xoffsets, double* X, int64_t* independent) {
         NCA_REQUIRE(ctx && st && ranks && rowpivots && colpivots && xoffsets && independent, NCA_ERR_ARGUMENT,
                     "nca_compress_level: invalid argument");
         set_device(ctx);
         const int64_t nb = st->nb;
         AcaResult res;
-        launch_aca(ctx, st, tol, sampled, res);
+        launch_aca(ctx, st, tol, sampled, res, iaca);
         xoffsets[0] = 0;
         if (nb == 0) return;
         d2h(ctx, ranks, res.ranks.p, sizeof(int32_t) * nb);
@@ -871,6 +1012,40 @@ int32_t nca_compress_level(nca_ctx* ctx, const nca_stack* st, double tol, int32_
         std::copy(xoff.begin(), xoff.end(), xoffsets);
         std::copy(indep.begin(), indep.end(), independent);
         singular_message(st, ranks, indep);
+}
+}  // namespace nca
+
+extern "C" {
+
+int32_t nca_compress_level(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, int32_t* ranks,
+                           int64_t* rowpivots, int64_t* colpivots, int64_t xcapacity, int64_t* xoffsets,
+                           double* X, int64_t* independent) {
+    return guarded([&] {
+        compress_level_impl(ctx, st, tol, sampled, false, ranks, rowpivots, colpivots, xcapacity, xoffsets, X,
+                            independent);
+    });
+}
+
+int32_t nca_compress_level_iaca(nca_ctx* ctx, const nca_stack* st, double tol, int32_t* ranks, int64_t* rowpivots,
+                                int64_t* colpivots, int64_t xcapacity, int64_t* xoffsets, double* X,
+                                int64_t* independent) {
+    return guarded([&] {
+        compress_level_impl(ctx, st, tol, 0, true, ranks, rowpivots, colpivots, xcapacity, xoffsets, X,
+                            independent);
+    });
+}
+
+int32_t nca_iaca(nca_ctx* ctx, const nca_stack* st, double tol, int32_t* ranks, int64_t* rowpivots,
+                 int64_t* colpivots) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && st && ranks && rowpivots && colpivots, NCA_ERR_ARGUMENT, "nca_iaca: invalid argument");
+        set_device(ctx);
+        AcaResult res;
+        launch_aca(ctx, st, tol, 0, res, true);
+        if (st->nb == 0) return;
+        d2h(ctx, ranks, res.ranks.p, sizeof(int32_t) * st->nb);
+        d2h(ctx, rowpivots, res.rowpiv.p, sizeof(int64_t) * st->nb * st->maxrank);
+        d2h(ctx, colpivots, res.colpiv.p, sizeof(int64_t) * st->nb * st->maxrank);
     });
 }
 

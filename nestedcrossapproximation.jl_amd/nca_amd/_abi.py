@@ -90,6 +90,8 @@ SIGNATURES = {
     "nca_aca_fullpiv": (I32, [P, P, D, I32, P, P, P]),
     "nca_skeleton_solve": (I32, [P, P, P, P, P, P, P, P]),
     "nca_compress_level": (I32, [P, P, D, I32, P, P, P, I64, P, P, P]),
+    "nca_iaca": (I32, [P, P, D, P, P, P]),
+    "nca_compress_level_iaca": (I32, [P, P, D, P, P, P, I64, P, P, P]),
     "nca_coupling_assemble": (I32, [P, P, P, P, I64, P, P, I64, P, P, P, P, PP]),
     "nca_farpivots": (I32, [P, P, I64, P, P, P, P, I64, P, P, P, I64, D, D, P, I64, P]),
     "nca_farpivots_tree": (I32, [P, P, I64, P, P, P, P, P, I64, P, P, P, I64, D, D, P, I64, P]),

@@ -45,6 +45,21 @@ class ACA:
         self.tol = float(tol)
 
 
+class IACA:
+    """IACA(MaximumValue(), TreeMimicryPivoting(...), FNormExtrapolator(tol))
+    as BottomUp takes it (bottomup.jl:214-424, pivots.jl:215-364): the column
+    pivots follow the geometric far-pivot order of the Phase-1 selector (the
+    stand-in for tree mimicry), the row pivots are the maxima of the residual
+    columns, and the stop compares the candidate's residual column with the
+    mean accepted column (see k_iaca). On the trial side the blocks are
+    assembled transposed, so the same rule picks the cluster's columns by
+    maximum value and the far rows geometrically, as IACA(trialpivoting,
+    MaximumValue(), ...) does."""
+
+    def __init__(self, tol=1e-4):
+        self.tol = float(tol)
+
+
 class GeometricPivoting:
     """Phase-1 far-pivot selector (stand-in for TreeMimicryPivoting; see
     nca_tree.cpp): up to min(maxrank, oversample |t|) far points per cluster,
@@ -209,8 +224,20 @@ def skeleton_solve(stack, ranks, rowpivots, colpivots):
     return xoff, X[:xoff[-1]], indep[:nb], status
 
 
-def compress_level(stack, tol, sampled=True):
-    """ACA + skeleton solve in one call (pivots stay on the device in between)."""
+def iaca(stack, tol):
+    """Batched IACA on a stack (columns = candidates in geometric order):
+    (ranks, rowpivots, colpivots), maxrank x nb, 1-based."""
+    nb, mr = stack.nb, stack.maxrank
+    ranks = np.zeros(max(nb, 1), np.int32)
+    rp = np.zeros((mr, max(nb, 1)), np.int64, order="F")
+    cp = np.zeros((mr, max(nb, 1)), np.int64, order="F")
+    call(hip_lib().nca_iaca(stack.farmatrix.ctx.handle, stack.handle, float(tol), _p(ranks), _p(rp), _p(cp)))
+    return ranks[:nb], rp[:, :nb], cp[:, :nb]
+
+
+def compress_level(stack, tol, sampled=True, method="aca"):
+    """ACA (or IACA, method="iaca") + skeleton solve in one call (pivots stay
+    on the device in between)."""
     nb, mr = stack.nb, stack.maxrank
     m = np.diff(stack.rowoffsets)
     cap = int(np.sum(m * np.minimum(np.minimum(m, stack.ncols), mr))) if nb else 0
@@ -220,8 +247,13 @@ def compress_level(stack, tol, sampled=True):
     xoff = np.zeros(nb + 1, np.int64)
     X = np.zeros(max(cap, 1), np.complex128)
     indep = np.zeros(max(nb, 1), np.int64)
-    call(hip_lib().nca_compress_level(stack.farmatrix.ctx.handle, stack.handle, float(tol), int(bool(sampled)),
-                                      _p(ranks), _p(rp), _p(cp), cap, _p(xoff), _p(X), _p(indep)))
+    if method == "iaca":
+        call(hip_lib().nca_compress_level_iaca(stack.farmatrix.ctx.handle, stack.handle, float(tol), _p(ranks),
+                                               _p(rp), _p(cp), cap, _p(xoff), _p(X), _p(indep)))
+    else:
+        call(hip_lib().nca_compress_level(stack.farmatrix.ctx.handle, stack.handle, float(tol),
+                                          int(bool(sampled)), _p(ranks), _p(rp), _p(cp), cap, _p(xoff), _p(X),
+                                          _p(indep)))
     return ranks[:nb], rp[:, :nb], cp[:, :nb], xoff, X[:xoff[-1]]
 
 
@@ -348,7 +380,8 @@ def _compresslevel(cfg, storage, active, rowptr, rowidx, out):
     """compresslevel_test!/trial! for a HIP LevelStack (batchedbottomup.jl:145-244,
     compresslevel.jl:242-316): batched ACA + skeleton solve, then the pivots
     tpivots[t] = tvalues[t][rowpivots] and the factor X of every block."""
-    ranks, rp, cp, xoff, X = compress_level(storage, cfg.tol, sampled=True)
+    ranks, rp, cp, xoff, X = compress_level(storage, cfg.tol, sampled=True,
+                                            method="iaca" if isinstance(cfg, IACA) else "aca")
     ranks = ranks.astype(np.int64)
     m = np.diff(storage.rowoffsets)
     mr = rp.shape[0]

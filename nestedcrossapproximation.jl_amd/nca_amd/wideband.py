@@ -32,8 +32,10 @@ from . import nca as _nca
 
 class BottomUp:
     """BottomUp(; factorization, representor) (bottomup.jl:26-47) for the
-    directional path: ``factorization`` is the ACA of every (cluster,
-    direction) block, ``pivotselector`` the Phase-1 far-pivot selection."""
+    directional path: ``factorization`` factorizes every (cluster, direction)
+    block, the fully pivoted ACA over its far pivots or the IACA (columns in
+    the geometric order of the far pivots, rows by maximum value);
+    ``pivotselector`` is the Phase-1 far-pivot selection."""
 
     def __init__(self, factorization=None, pivotselector=None, representor=None):
         if representor is not None:

@@ -236,6 +236,96 @@ int64_t orc_aca_fullpiv(double* block, int64_t m, int64_t n, int64_t ld, int64_t
     return r;
 }
 
+/* Incomplete ACA with geometric column pivoting (IACA; Tetzner and Adrian,
+ * "The Incomplete Adaptive Cross Approximation ...", README.md:193 ref. [4];
+ * the glue is pivots.jl:215-364 with IACA(MaximumValue(), TreeMimicryPivoting,
+ * FNormExtrapolator) as BottomUp uses it, bottomup.jl:214-424). The block's
+ * columns are the far-field candidates in their geometric order, so step r
+ * takes column r; its residual column gives the row pivot (first maximum of
+ * |R(a, r)|^2). Stop, not counting the candidate, at r = min(maxrank, m, n),
+ * at a zero residual column, or when
+ *   r ||R_r(:, r)||^2 <= tol^2 sum_{l<r} ||A(:, l)||^2          (r > 0).
+ * AdaptiveCrossApproximation.jl (its IACA, TreeMimicryPivoting and
+ * FNormExtrapolator) is not vendored, so this criterion is a restatement of
+ * the method, not of that package's code. Residual update and sums as in
+ * orc_aca_fullpiv (R -= u v, pivot row and column set to zero; column sums in
+ * the slots e mod 256 of the column-major entries, halving tree). Returns the
+ * rank; rowpiv/colpiv 1-based; block is overwritten. */
+int64_t orc_iaca(double* block, int64_t m, int64_t n, int64_t ld, int64_t maxrank, double tol, int64_t* rowpiv,
+                 int64_t* colpiv) {
+    cplx* B = (cplx*)block;
+    int64_t kmax = maxrank;
+    if (m < kmax) kmax = m;
+    if (n < kmax) kmax = n;
+    if (kmax <= 0) return 0;
+    cplx* A0 = (cplx*)malloc(sizeof(cplx) * (size_t)(m * n > 0 ? m * n : 1));
+    for (int64_t c = 0; c < n; ++c)
+        for (int64_t a = 0; a < m; ++a) A0[a + c * m] = B[a + c * ld];
+    double p[ORC_SLOTS];
+    memset(p, 0, sizeof(p));
+    double best = -1.0;
+    int64_t bi = 0;
+    for (int64_t a = 0; a < m; ++a) {
+        double w = cabs2(B[a]);
+        p[a & (ORC_SLOTS - 1)] += w;
+        if (w > best) {
+            best = w;
+            bi = a;
+        }
+    }
+    double cn2 = sum_slots(p), orig2 = cn2, acc = 0.0;
+    const double tol2 = tol * tol;
+    cplx* u = (cplx*)malloc(sizeof(cplx) * (size_t)m);
+    cplx* v = (cplx*)malloc(sizeof(cplx) * (size_t)n);
+    int64_t r = 0;
+    while (r < kmax) {
+        if (r > 0 && cn2 * (double)r <= tol2 * acc) break;
+        if (!(best > 0.0)) break;
+        int64_t i = bi, j = r;
+        cplx piv = B[i + j * ld];
+        double d = cabs2(piv);
+        cplx inv = {piv.re / d, -piv.im / d};
+        for (int64_t a = 0; a < m; ++a) u[a] = B[a + j * ld];
+        for (int64_t c = 0; c < n; ++c) v[c] = cmul(B[i + c * ld], inv);
+        rowpiv[r] = i + 1;
+        colpiv[r] = j + 1;
+        acc += orig2;
+        ++r;
+        double q[ORC_SLOTS];
+        memset(p, 0, sizeof(p));
+        memset(q, 0, sizeof(q));
+        best = -1.0;
+        bi = 0;
+        for (int64_t c = 0; c < n; ++c)
+            for (int64_t a = 0; a < m; ++a) {
+                int64_t e = a + c * m;
+                cplx val;
+                if (a == i || c == j) {
+                    val.re = 0.0;
+                    val.im = 0.0;
+                } else {
+                    val = csub(B[a + c * ld], cmul(u[a], v[c]));
+                }
+                B[a + c * ld] = val;
+                if (c == r) {
+                    double w = cabs2(val);
+                    p[e & (ORC_SLOTS - 1)] += w;
+                    if (w > best) {
+                        best = w;
+                        bi = a;
+                    }
+                    q[e & (ORC_SLOTS - 1)] += cabs2(A0[a + c * m]);
+                }
+            }
+        cn2 = sum_slots(p);
+        orig2 = sum_slots(q);
+    }
+    free(u);
+    free(v);
+    free(A0);
+    return r;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Skeleton solve X = L * P^{-1}, P = L[rowpiv, :] (r x r), as nestedtestbasis /
  * testtransfer do with rdiv! (nestedbasis.jl:43-49, transfermatrices.jl:183-187)
@@ -386,7 +476,8 @@ int64_t orc_bases(int32_t side, double k, int32_t sign, const double* testpts, c
             memcpy(work, blk, sizeof(cplx) * (size_t)(m * n));
             int64_t* rpv = (int64_t*)malloc(sizeof(int64_t) * (size_t)(maxrank > 0 ? maxrank : 1));
             int64_t* cpv = (int64_t*)malloc(sizeof(int64_t) * (size_t)(maxrank > 0 ? maxrank : 1));
-            int64_t r = orc_aca_fullpiv((double*)work, m, n, m, maxrank, tol, sampled, rpv, cpv);
+            int64_t r = sampled == 2 ? orc_iaca((double*)work, m, n, m, maxrank, tol, rpv, cpv)
+                                     : orc_aca_fullpiv((double*)work, m, n, m, maxrank, tol, sampled, rpv, cpv);
             cplx* Lm = (cplx*)malloc(sizeof(cplx) * (size_t)(m * r > 0 ? m * r : 1));
             for (int64_t c = 0; c < r; ++c)
                 for (int64_t a = 0; a < m; ++a) Lm[a + c * m] = blk[a + (cpv[c] - 1) * m];

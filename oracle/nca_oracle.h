@@ -35,6 +35,12 @@ int32_t orc_assemble_level(double k, int32_t sign, const double* testpts, const 
 int64_t orc_aca_fullpiv(double* block, int64_t m, int64_t n, int64_t ld, int64_t maxrank,
                         double tol, int32_t sampled, int64_t* rowpiv, int64_t* colpiv);
 
+/* Incomplete ACA with geometric column pivoting on one m x n block whose
+ * columns are the candidates in geometric order (see nca_oracle.c); the
+ * block is overwritten. Returns the rank. */
+int64_t orc_iaca(double* block, int64_t m, int64_t n, int64_t ld, int64_t maxrank, double tol, int64_t* rowpiv,
+                 int64_t* colpiv);
+
 /* Skeleton solve X = L / L[rowpiv, :] with LU and the singularity rule of
  * utils.jl:57-80. Returns -1 when regular, else the number of independent
  * pivots. */
@@ -45,8 +51,10 @@ int64_t orc_skeleton_solve(const double* L, int64_t m, int64_t r, const int64_t*
  * (1-based inclusive positions), level (1-based), childptr (nn+1, 1-based),
  * children (1-based). farpivptr/farpiv: Phase-1 pivots per node (1-based).
  * Outputs: pivptr (nn+1) / pivots (capacity nn*maxrank), xoffsets (nn+1, 0-based
- * complex offsets) / X (capacity xcap complex). Returns 0, or -(node) of the
- * first singular cross matrix, or 1 when X is too small. */
+ * complex offsets) / X (capacity xcap complex). sampled: 0 or 1, the fully
+ * pivoted ACA without or with the sampled stop; 2, the IACA (orc_iaca).
+ * Returns 0, or -(node) of the first singular cross matrix, or 1 when X is
+ * too small. */
 int64_t orc_bases(int32_t side, double k, int32_t sign, const double* testpts, const double* trialpts,
                   int64_t nn, const int64_t* perm, const int64_t* nodelo, const int64_t* nodehi,
                   const int64_t* level, const int64_t* childptr, const int64_t* children,

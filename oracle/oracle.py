@@ -64,6 +64,8 @@ def lib():
         L.orc_assemble_level.argtypes = [D, I32, P, P, I64, P, P, P, P, I64, I32, P]
         L.orc_aca_fullpiv.restype = I64
         L.orc_aca_fullpiv.argtypes = [P, I64, I64, I64, I64, D, I32, P, P]
+        L.orc_iaca.restype = I64
+        L.orc_iaca.argtypes = [P, I64, I64, I64, I64, D, P, P]
         L.orc_skeleton_solve.restype = I64
         L.orc_skeleton_solve.argtypes = [P, I64, I64, P, P]
         L.orc_bases.restype = I64
@@ -134,6 +136,17 @@ def aca_fullpiv(block, maxrank, tol, sampled=True):
     return int(r), rp[:r].copy(), cp[:r].copy()
 
 
+def iaca(block, maxrank, tol):
+    """IACA with geometric column order on a copy of ``block`` (its columns
+    are the candidates in order); returns (rank, rowpiv, colpiv) 1-based."""
+    A = np.asfortranarray(np.array(block, dtype=np.complex128, copy=True))
+    m, n = A.shape
+    rp = np.zeros(max(maxrank, 1), np.int64)
+    cp = np.zeros(max(maxrank, 1), np.int64)
+    r = lib().orc_iaca(_p(A), m, n, max(m, 1), int(maxrank), float(tol), _p(rp), _p(cp))
+    return int(r), rp[:r].copy(), cp[:r].copy()
+
+
 def skeleton_solve(L, rowpiv):
     """X = L / L[rowpiv, :]; returns (X, independent) with independent = -1 when regular."""
     Lm = np.asfortranarray(np.array(L, dtype=np.complex128))
@@ -144,10 +157,13 @@ def skeleton_solve(L, rowpiv):
     return X, int(st)
 
 
-def bases(side, k, sign, testpts, trialpts, tree, farpivptr, farpiv, maxrank, tol, sampled=True):
+def bases(side, k, sign, testpts, trialpts, tree, farpivptr, farpiv, maxrank, tol, sampled=True, iaca=False):
     """testbases/trialbases(::BatchedBottomUp). tree: ClusterTree of this side;
-    farpivptr/farpiv: 0-based CSR of 0-based point ids (Phase 1).
-    Returns (pivptr 0-based, pivots 0-based, xoffsets, X, status)."""
+    farpivptr/farpiv: 0-based CSR of 0-based point ids (Phase 1). iaca: the
+    incomplete ACA over the far pivots in their order instead of the fully
+    pivoted ACA. Returns (pivptr 0-based, pivots 0-based, xoffsets, X, status)."""
+    if iaca:
+        sampled = 2
     lay = tree.layout_1based()
     nn = tree.nnodes
     fptr = _i64(np.asarray(farpivptr) + 1)
@@ -163,7 +179,7 @@ def bases(side, k, sign, testpts, trialpts, tree, farpivptr, farpiv, maxrank, to
         X = np.zeros(max(cap, 1), np.complex128)
         st = lib().orc_bases(int(side), float(k), int(sign), _p(tp), _p(sp), nn, _p(lay["perm"]), _p(lay["nodelo"]),
                              _p(lay["nodehi"]), _p(lay["level"]), _p(lay["childptr"]), _p(lay["children"]),
-                             _p(fptr), _p(fpiv), int(maxrank), float(tol), int(bool(sampled)), _p(pivptr),
+                             _p(fptr), _p(fpiv), int(maxrank), float(tol), int(sampled), _p(pivptr),
                              _p(pivots), _p(xoff), _p(X), cap)
         if st != 1:
             break

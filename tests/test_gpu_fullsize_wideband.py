@@ -1,8 +1,8 @@
 """The directional wideband workload at its full size on the GPU path: N =
 262,144 Fibonacci points of the unit sphere (test = trial), k = 16 pi (16
 wavelengths across), 2-means tree with minvalues 64, isnearwideband(k) with
-etalf 1, etahf 5, gamma 1, BottomUp per (cluster, direction) block at tol
-1e-4, maxrank 50, 8 seeded complex Gaussian right-hand sides.
+etalf 1, etahf 5, gamma 1, BottomUp with the IACA per (cluster, direction)
+block at tol 1e-4, maxrank 50, 8 seeded complex Gaussian right-hand sides.
 
 Checks: ranks, pivots and skeleton factors of all 775k test (cluster,
 direction) blocks bit for bit against the oracle's level driver on the same
@@ -42,7 +42,7 @@ def full():
     pts = geometry.fibonacci_sphere(N)
     tr = nca_amd.KMeansTree(pts, 2, 64)
     core = W.DirectionalNCACore(nca_amd.HelmholtzKernel(K), pts, pts, nca_amd.BlockTree(tr, tr),
-                                W.BottomUp(C.ACA(TOL)), W.BottomUp(C.ACA(TOL)), D.isnearwideband(K),
+                                W.BottomUp(C.IACA(TOL)), W.BottomUp(C.IACA(TOL)), D.isnearwideband(K),
                                 maxrank=MAXRANK, ctx=nca_amd.default_context())
     X = np.asfortranarray(geometry.complex_gaussian(np.random.default_rng(8), (N, NRHS)))
     return pts, core, X
@@ -59,7 +59,7 @@ def test_fullsize_wideband_pivots_equal_oracle(full):
     blocks, f = core.testblocks, core.testfactors
     say(f"oracle bases over {blocks.nnodes} direction indices (test side)")
     fp = f.farpivots
-    res = O.bases(0, K, 1, pts, pts, blocks, fp[0], fp[1], MAXRANK, TOL)
+    res = O.bases(0, K, 1, pts, pts, blocks, fp[0], fp[1], MAXRANK, TOL, iaca=True)
     assert res[4] == 0
     ref = oracle_factors(res, blocks, fp)
     assert np.array_equal(ref.assigned, f.assigned)

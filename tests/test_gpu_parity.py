@@ -100,6 +100,33 @@ def test_aca_bitwise_pivots_on_kernel_level(ctx, tol):
     _aca_compare(ctx, buf, ro, np.array([len(c) for c in cols]), 50, tol)
 
 
+@pytest.mark.parametrize("tol", [1e-2, 1e-4, 1e-8])
+def test_iaca_bitwise_pivots_on_kernel_level(ctx, tol):
+    """nca_iaca (geometric column order) against the oracle's orc_iaca, bit for
+    bit, on kernel blocks of the LDS and of the global-memory path."""
+    say(f"iaca kernel level tol={tol}")
+    rng = np.random.default_rng(19)
+    tp = rng.standard_normal((2000, 3))
+    sp = rng.standard_normal((2000, 3)) + np.array([6.0, 0.0, 0.0])
+    rows, cols = random_level(rng, 2000, 2000, 120, 50, mmax=90)
+    rows += [rng.integers(1, 2001, 400), rng.integers(1, 2001, 300)]
+    cols += [rng.integers(1, 2001, 50), rng.integers(1, 2001, 33)]
+    buf, ro = O.assemble_level(1.5, 1, tp, sp, rows, cols, 50)
+    ncols = np.array([len(c) for c in cols])
+    A = farmatrix(ctx, 1.5, tp, sp)
+    st = C.stack_from_host(A, buf, ro, ncols, 50)
+    ranks, rp, cp = C.iaca(st, tol)
+    for b in range(len(ncols)):
+        r, orp, ocp = O.iaca(buf[ro[b]:ro[b + 1], :ncols[b]], 50, tol)
+        assert ranks[b] == r, (b, ranks[b], r)
+        assert np.array_equal(rp[:r, b], orp) and np.array_equal(cp[:r, b], ocp)
+        assert np.all(rp[r:, b] == 0)
+    # the zero blocks of test/gpu/test_compresslevel.jl:191-211
+    z = np.zeros((13, 30), np.complex128, order="F")
+    st0 = C.stack_from_host(A, z, [0, 0, 4, 13], [3, 3, 3], 30)
+    assert list(C.iaca(st0, tol)[0]) == [0, 0, 0]
+
+
 def test_aca_zero_blocks(ctx):
     """test/gpu/test_compresslevel.jl:191-211: rowoffsets [0, 0, 4, 13] of zeros."""
     buf = np.zeros((13, 30), np.complex128, order="F")

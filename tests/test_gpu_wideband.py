@@ -1,8 +1,9 @@
 """Directional (wideband) H^2 on the GPU, through the C-ABI (nca_dh2_create).
 
-Parity bar as for the plain operator: ranks and pivots of every (cluster,
-direction) block bit for bit against the oracle's level driver on the same
-blocks and Phase-1 pivots; the directional product within 1e-12 relative of
+Parity bar as for the plain operator: ranks, pivots and factors of every
+(cluster, direction) block bit for bit against the oracle's level driver on
+the same blocks and Phase-1 pivots, with the fully pivoted ACA and with the
+IACA (BottomUp's factorization); the directional product within 1e-12 relative of
 the oracle's stage-by-stage restatement (oracle.OracleDH2) with the same
 factors, for op N, T and C; the error against dense direct summation below
 10 tol (test/nca/test_petrovgalerkinnca.jl:104-153, wideband with
@@ -39,15 +40,15 @@ def say(msg):
 _cores = {}
 
 
-def build(case, tol=1e-4):
-    key = (case, tol)
+def build(case, tol=1e-4, iaca=False):
+    key = (case, tol, iaca)
     if key not in _cores:
         kind, n, lam = case
         pts = geometry.fibonacci_sphere(n)
         tr = nca_amd.TwoNTree(pts, minvalues=16) if kind == "twon" else nca_amd.KMeansTree(pts, 2, 16)
         k = 2 * math.pi / lam
         isn = D.isnearwideband(k, etalf=1.0, etahf=1.0, gamma=1.0)
-        comp = W.BottomUp(C.ACA(tol))
+        comp = W.BottomUp(C.IACA(tol) if iaca else C.ACA(tol))
         core = W.DirectionalNCACore(nca_amd.HelmholtzKernel(k), pts, pts, T.BlockTree(tr, tr), comp, comp, isn,
                                     maxrank=50)
         _cores[key] = (core, pts, k)
@@ -59,13 +60,14 @@ def oracle_of(core, pts, k):
     return O.OracleDH2(core.desc, k, 1, pts, pts, off, blocks)
 
 
+@pytest.mark.parametrize("iaca", [False, True], ids=["aca", "iaca"])
 @pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}{c[1]}" for c in CASES])
-def test_directional_pivots_bitwise(case):
-    say(f"pivots {case}")
-    core, pts, k = build(case)
+def test_directional_pivots_bitwise(case, iaca):
+    say(f"pivots {case} iaca={iaca}")
+    core, pts, k = build(case, iaca=iaca)
     for side, blocks, f in ((0, core.testblocks, core.testfactors), (1, core.trialblocks, core.trialfactors)):
         fp = f.farpivots
-        res = O.bases(side, k, 1, pts, pts, blocks, fp[0], fp[1], 50, 1e-4)
+        res = O.bases(side, k, 1, pts, pts, blocks, fp[0], fp[1], 50, 1e-4, iaca=iaca)
         assert res[4] == 0
         ref = oracle_factors(res, blocks, fp)
         assert np.array_equal(ref.assigned, f.assigned)
@@ -103,10 +105,11 @@ def test_directional_matvec_parity(case):
     assert relerr(y, (0.5 - 0.25j) * ref.matvec(x) + 2.0 * y0) < 1e-12
 
 
+@pytest.mark.parametrize("iaca", [False, True], ids=["aca", "iaca"])
 @pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}{c[1]}" for c in CASES])
-def test_directional_direct_sum_and_mixed(case):
-    say(f"direct sum + mixed {case}")
-    core, pts, k = build(case)
+def test_directional_direct_sum_and_mixed(case, iaca):
+    say(f"direct sum + mixed {case} iaca={iaca}")
+    core, pts, k = build(case, iaca=iaca)
     n = len(pts)
     rng = np.random.default_rng(5)
     x = rng.standard_normal((n, 16)) + 1j * rng.standard_normal((n, 16))

@@ -162,6 +162,90 @@ def test_aca_matches_brute_force_on_kernel_blocks():
             assert got[0] == ref[0] and list(got[1]) == ref[1] and list(got[2]) == ref[2]
 
 
+# ---------------------------------------------------------------- IACA
+def brute_force_iaca(A, maxrank, tol):
+    """The IACA with geometric column order written with numpy: step r takes
+    column r, the row pivot is the first maximum of its residual column, and
+    the candidate stops the factorization when r ||R(:, r)||^2 <= tol^2
+    sum_{l<r} ||A(:, l)||^2."""
+    R = np.array(A, dtype=np.complex128)
+    m, n = R.shape
+    rows = []
+    acc = 0.0
+    while len(rows) < min(maxrank, m, n):
+        r = len(rows)
+        col = R[:, r]
+        c2 = np.sum(np.abs(col) ** 2)
+        if r > 0 and c2 * r <= tol * tol * acc:
+            break
+        i = int(np.argmax(np.abs(col) ** 2))
+        if col[i] == 0:
+            break
+        R = R - np.outer(col, R[i, :] / col[i])
+        R[i, :] = 0
+        R[:, r] = 0
+        rows.append(i + 1)
+        acc += np.sum(np.abs(A[:, r]) ** 2)
+    return len(rows), rows, list(range(1, len(rows) + 1))
+
+
+def test_iaca_contracts():
+    """Rank at most min(maxrank, m, n) (_ranklimit, test/pivoting/test_pivots.jl:77-82),
+    unique row pivots, column pivots 1..rank in order, zero blocks rank 0."""
+    rng = np.random.default_rng(3)
+    assert O.iaca(np.zeros((4, 3)), 30, 1e-3)[0] == 0
+    assert O.iaca(np.zeros((0, 3)), 30, 1e-3)[0] == 0
+    for m, n, mr in ((30, 20, 50), (7, 40, 50), (40, 40, 5), (12, 9, 12)):
+        A = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+        r, rp, cp = O.iaca(A, mr, 1e-14)
+        assert r == min(m, n, mr)
+        assert len(set(rp.tolist())) == r and list(cp) == list(range(1, r + 1))
+
+
+def test_iaca_stop_rule_scaling():
+    """A known answer fixed by the stop rule: column 0 = e_0 (norm 1), column
+    1 = delta e_1 (its residual after the first cross). With tol 1e-2 the
+    second candidate stops the factorization when delta^2 <= tol^2 * 1."""
+    for delta, want in ((0.5e-2, 1), (2e-2, 2)):
+        A = np.zeros((3, 2), np.complex128)
+        A[0, 0] = 1.0
+        A[1, 1] = delta
+        assert O.iaca(A, 50, 1e-2)[0] == want
+    # the mean over accepted columns: two unit columns, a third of 0.015
+    A = np.zeros((4, 3), np.complex128)
+    A[0, 0], A[1, 1], A[2, 2] = 1.0, 1.0, 0.015
+    assert O.iaca(A, 50, 1e-2)[0] == 3        # 2 * 0.015^2 > 1e-4 * 2
+    A[2, 2] = 0.009
+    assert O.iaca(A, 50, 1e-2)[0] == 2        # 2 * 0.009^2 <= 1e-4 * 2
+
+
+def test_iaca_exact_rank_blocks():
+    """Exact-rank blocks: the rank and CUR error < 1e-9 once the first r
+    columns span the column space (generic blocks)."""
+    rng = np.random.default_rng(11)
+    cg = lambda *s: rng.standard_normal(s) + 1j * rng.standard_normal(s)
+    for m, n, r in ((120, 25, 8), (64, 30, 17), (40, 40, 12)):
+        A = cg(m, r) @ cg(r, n)
+        rank, rp, cp = O.iaca(A, 50, 1e-10)
+        assert rank == r
+        rpz, cpz = rp - 1, cp - 1
+        recon = A[:, cpz] @ np.linalg.solve(A[np.ix_(rpz, cpz)], A[rpz, :])
+        assert relerr(recon, A) < 1e-9
+
+
+def test_iaca_matches_brute_force_on_kernel_blocks():
+    rng = np.random.default_rng(6)
+    for trial in range(20):
+        m, n = rng.integers(2, 40), rng.integers(2, 30)
+        x = rng.random((m, 3))
+        y = rng.random((n, 3)) + np.array([2.0 + rng.random(), 0, 0])
+        A = np.array([[G(3.0, a, b) for b in y] for a in x])
+        for tol in (1e-2, 1e-4, 1e-8):
+            got = O.iaca(A, 50, tol)
+            ref = brute_force_iaca(A, 50, tol)
+            assert got[0] == ref[0] and list(got[1]) == ref[1] and list(got[2]) == ref[2], (trial, tol)
+
+
 # ---------------------------------------------------------------- skeleton solve
 def test_skeleton_solve_closed_form():
     """Z = transpose(L / L[rows, :]) at rtol 1e-11, ranks up to 80
@@ -360,9 +444,10 @@ def test_stored_near_field_matches_on_the_fly():
 def test_oracle_directional_vs_direct_summation(kind, n, lam):
     """The oracle's directional product (OracleDH2: the reference's directional
     stages restated in numpy) over the oracle's own level driver on the
-    (cluster, direction) blocks: within 10 tol of dense direct summation for
-    tol 1e-2 and 1e-4 (test/nca/test_petrovgalerkinnca.jl:104-153, wideband
-    with etalf = etahf = gamma = 1)."""
+    (cluster, direction) blocks, with the fully pivoted ACA and with the IACA:
+    within 10 tol of dense direct summation for tol 1e-2 and 1e-4
+    (test/nca/test_petrovgalerkinnca.jl:104-153, wideband with etalf = etahf =
+    gamma = 1)."""
     import math
     import nca_amd
     from nca_amd import directional as D, wideband as W, compressors as C, trees as T, geometry
@@ -382,9 +467,9 @@ def test_oracle_directional_vs_direct_summation(kind, n, lam):
     x = np.random.default_rng(4).standard_normal(n) + 1j * np.random.default_rng(5).standard_normal(n)
     rows = np.arange(0, n, 7)
     yd = O.dense_rows(k, 1, pts, pts, rows, x)[:, 0]
-    for tol in (1e-2, 1e-4):
-        tres = O.bases(0, k, 1, pts, pts, tb, fpt[0], fpt[1], 50, tol)
-        sres = O.bases(1, k, 1, pts, pts, sb, fps[0], fps[1], 50, tol)
+    for tol, iaca in ((1e-2, False), (1e-4, False), (1e-2, True), (1e-4, True)):
+        tres = O.bases(0, k, 1, pts, pts, tb, fpt[0], fpt[1], 50, tol, iaca=iaca)
+        sres = O.bases(1, k, 1, pts, pts, sb, fps[0], fps[1], 50, tol, iaca=iaca)
         assert tres[4] == 0 and sres[4] == 0
         ft, fs = oracle_factors(tres, tb, fpt), oracle_factors(sres, sb, fps)
         tB, tT = W.dir_stores(ft, tb, 0)
@@ -395,7 +480,7 @@ def test_oracle_directional_vs_direct_summation(kind, n, lam):
         A = O.OracleDH2(W.dh2desc(tree, part, td, sd, ft, fs, tB, tT, sB, sT, cplan), k, 1, pts, pts, off, blocks)
         y = A.matvec(x)
         err = np.linalg.norm(y[rows] - yd) / np.linalg.norm(yd)
-        assert err < 10 * tol, (tol, err)
+        assert err < 10 * tol, (tol, iaca, err)
         # the row-restricted product (only clusters holding the rows) is exact there
         yr = A.matvec(x, rows=rows[::5])
         assert relerr(yr[rows[::5]], y[rows[::5]]) < 1e-14
