@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--minvalues", type=int, default=64)
     ap.add_argument("--maxrank", type=int, default=50)
     ap.add_argument("--check-rows", type=int, default=1000)
+    ap.add_argument("--stored-near", type=int, default=1,
+                    help="also time forward sweeps with the near field stored on the device")
     ap.add_argument("--single-rhs-65k", type=int, default=1,
                     help="also time the 65,536-point FP64 configuration (k = 1, tol 1e-6) at one RHS")
     ap.add_argument("--cpu-baseline", type=int, default=1)
@@ -326,6 +328,33 @@ def main():
         "autotune": tuning,
         "roofline": roofline,
     }
+
+    if not args.profile_only and args.stored_near:
+        # the reference CPU path's layout on the device: near entries stored
+        # once (assemblenears), forward sweeps stream them from HBM
+        sn = {}
+        for label, prec in (("fp64", nca_amd.NCA_PREC_FP64), ("mixed", nca_amd.NCA_PREC_MIXED)):
+            core.set_precision(prec)
+            t0 = time.perf_counter()
+            core.store_near(True)
+            t_store = time.perf_counter() - t0
+            ms = timed(0, R, args.steps, args.warmup)
+            ms1 = timed(0, 1, 20, 2)
+            Ys = np.zeros((M, R), np.complex128, order="F")
+            timed(0, R, 1, 0)
+            hip.d2h(Ys, dy)
+            prof = stage_profile(R)
+            sn[label] = {"ms_per_sweep": round(ms, 4), "matvec_per_s": round(R / (ms * 1e-3), 2),
+                         "ms_per_product_1rhs": round(ms1, 4), "store_s": round(t_store, 3),
+                         "leaf_expansion_near_ms": round(float(prof[5]), 4),
+                         "near_store_bytes": int((8 if label == "mixed" else 16) * st["near_entries"]),
+                         "relerr_vs_on_the_fly": float(np.linalg.norm(Ys - results[label][3]) /
+                                                       np.linalg.norm(results[label][3]))}
+            core.store_near(False)
+            log(f"stored near field {label}: {ms:.3f} ms per {R}-RHS sweep, 1 RHS {ms1:.3f} ms, store "
+                f"{t_store:.2f}s, near stage {prof[5]:.3f} ms")
+        result["stored_near"] = sn
+        core.set_precision(nca_amd.NCA_PREC_MIXED)
 
     if not args.profile_only and args.check_rows > 0:
         from oracle import oracle as O

@@ -303,6 +303,15 @@ int32_t nca_h2_matvec(nca_h2* h2, int32_t op, int64_t nrhs, const double* x, int
 /* Same with device pointers, enqueued on the context stream (no host sync). */
 int32_t nca_h2_matvec_device(nca_h2* h2, int32_t op, int64_t nrhs, const void* x, int64_t ldx, void* y,
                              int64_t ldy, const double* alpha, const double* beta);
+/* Stored near field, the layout of the reference CPU path (assemblenears,
+ * src/nca/abstractnca.jl:134-178: one dense block row per test cluster over
+ * its near clusters' points, applied by BlockSparseMatrices at
+ * src/nca/petrovgalerkinnca.jl:474). on = 1: forward products read the near
+ * entries from HBM instead of evaluating them (entries made on the device
+ * once per precision, 16 B (FP64) or 8 B (MIXED) each; transpose/adjoint
+ * products still evaluate them); on = 0: drop them and evaluate on the fly
+ * (the default). Changing it invalidates captured graphs of stored sweeps. */
+int32_t nca_h2_store_near(nca_h2* h2, int32_t on);
 /* No counterpart in the reference: nca_h2_matvec_device that allocates
  * nothing, for callers that cannot allocate device memory (code under hipGraph
  * capture, tight solver loops, harnesses that forbid hipMalloc). Its workspace

@@ -121,6 +121,18 @@ struct nca_h2 {
     nca::DevBuf<float2> rv_blocks32;  // c32 (made for the first MIXED T/C product)
     nca::DevBuf<int64_t> rv_dst;  // per block of the reverse plan (r_off order): offset in the transposed store
     nca::DevBuf<int32_t> rv_rs;   // per block of the reverse plan: its trial rank
+    // stored near field (nca_h2_store_near): forward products read the near
+    // entries from HBM instead of evaluating them. One "row" per test leaf (its
+    // points in tree order) over the concatenated points of its merged near
+    // ranges, in the forward coupling format, so the forward coupling kernels
+    // apply it into a per-point buffer that the leaf kernels add.
+    bool near_stored = false, ns_plan = false;
+    int64_t ns_ntiles = 0, ns_entries = 0, ns_nleaf = 0;
+    nca::DevBuf<int64_t> ns_base, ns_colbase;
+    nca::DevBuf<int32_t> ns_ncols, ns_colcoef, ns_tile_node, ns_tile_row0, ns_cptr;
+    nca::DevBuf<double2> ns_blocks;   // FP64 entries (made for the first FP64 product)
+    nca::DevBuf<float2> ns_blocks32;  // c32 entries (made for the first MIXED product)
+    nca::DevBuf<double2> ws_ynear;    // near-field result per point (tree order) x R
     // workspace
     int64_t ws_rhs = 0;
     // run a last chunk of 1..3 right-hand sides on the MFMA kernels too
@@ -144,6 +156,7 @@ struct nca_h2 {
         double2* yh = nullptr;    // destination coefficients, FP64
         float2* yc = nullptr;     // c32 coupling results (MIXED forward)
         float2* xh32 = nullptr;   // compact c32 copy of a 16-column chunk's source coefficients
+        double2* ynear = nullptr; // stored near field applied, per point (tree order), FP64
     } ws;
     // c32 copy of the coupling blocks for the MIXED path
     nca::DevBuf<float2> S32;
@@ -243,6 +256,28 @@ __global__ void k_chunk_to_c32(const IT* __restrict__ in, int64_t R, int64_t q0,
     const int q = (int)(i & 15);
     const double2 v = q < nq ? to_d2(in[(i >> 4) * R + q0 + q]) : make_double2(0.0, 0.0);
     out[i] = make_float2((float)v.x, (float)v.y);
+}
+
+// Stored near field, one workgroup per test leaf i: its nt x C row of
+// G(x_row, y_col) (column-major at base[i]); rows are its points (tree
+// positions cptr[i] ..), columns the source positions colcoef[colbase[i] ..];
+// the entry of helmholtz() (zero at r = 0), rounded to T2.
+template <class T2>
+__global__ void __launch_bounds__(256)
+    k_near_store(const double* __restrict__ dpts, const double* __restrict__ spts, const int32_t* __restrict__ cptr,
+                 const int64_t* __restrict__ base, const int64_t* __restrict__ colbase,
+                 const int32_t* __restrict__ ncols, const int32_t* __restrict__ colcoef, double k, int sign,
+                 T2* __restrict__ S) {
+    const int64_t i = blockIdx.x;
+    const int r0 = cptr[i], nt = cptr[i + 1] - r0, C = ncols[i];
+    const int64_t b = base[i], cb = colbase[i];
+    const int64_t total = (int64_t)nt * C;
+    for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
+        const int col = (int)(e / nt), row = (int)(e - (int64_t)col * nt);
+        const double* x = dpts + 3 * (int64_t)(r0 + row);
+        const double* y = spts + 3 * (int64_t)colcoef[cb + col];
+        store_as(S + b + e, helmholtz(k, sign, x[0], x[1], x[2], y[0], y[1], y[2]));
+    }
 }
 
 // xs[iperm[i]*R + q] = x[i + q*ldx], moved through LDS in tiles of 64
@@ -867,7 +902,7 @@ __global__ void __launch_bounds__(256, 4)
                     const double* __restrict__ spts, const double2* __restrict__ yh,
                     const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, NearK nk,
                     const double2* __restrict__ sctab, double2 alpha, double2 beta, double2* __restrict__ y,
-                    int64_t ldy, int order, int stagger, int nq) {
+                    int64_t ldy, int order, int stagger, int nq, const double2* __restrict__ ynear) {
     using T2 = typename P::T2;
     __shared__ double2 tab[P::kTab ? kSinCosTab : 1];
     if (P::kTab) {
@@ -898,11 +933,14 @@ __global__ void __launch_bounds__(256, 4)
         using ACC = std::remove_reference_t<decltype(acc[0])>;
 #pragma unroll
         for (int c = 0; c < NC; ++c) acc[c].zero();
-        const double* xp = dpts + 3 * (int64_t)(l0 + row0 + min(rsel, nrows - 1));
-        near_walk<P, H, ACC, EMB, NEG, NC>(acc, nk, tab, lane, xp[0], xp[1], xp[2], e0, e1, rot, nrlo, nrhi, spts,
-                                           xs + q0, (uint32_t)R, jl);
+        // the near field: evaluated here, or stored and applied before (ynear)
+        if (!ynear) {
+            const double* xp = dpts + 3 * (int64_t)(l0 + row0 + min(rsel, nrows - 1));
+            near_walk<P, H, ACC, EMB, NEG, NC>(acc, nk, tab, lane, xp[0], xp[1], xp[2], e0, e1, rot, nrlo, nrhi,
+                                               spts, xs + q0, (uint32_t)R, jl);
 #pragma unroll
-        for (int c = 0; c < NC; ++c) acc[c].scale((typename P::T)kInvFourPi);
+            for (int c = 0; c < NC; ++c) acc[c].scale((typename P::T)kInvFourPi);
+        }
         if (B) {
             const int rowc = row0 + min(rsel, nrows - 1);
             const double2* yt = yh + (int64_t)dcptr[t] * R + q0;
@@ -925,6 +963,7 @@ __global__ void __launch_bounds__(256, 4)
             const int col = 16 * c + j;
             if (row < nrows && col < nq) {
                 double2* out = y + (int64_t)dperm[l0 + row0 + row] + (q0 + col) * ldy;
+                if (ynear) v = cadd(v, ynear[(int64_t)(l0 + row0 + row) * R + q0 + col]);
                 double2 val = cmul(alpha, v);
                 if (beta.x != 0.0 || beta.y != 0.0) val = cadd(val, cmul(beta, *out));
                 *out = val;
@@ -1030,7 +1069,7 @@ __global__ void __launch_bounds__(256)
                      const double* __restrict__ spts, const double2* __restrict__ yh,
                      const typename P::T2* __restrict__ xs, int64_t R, int64_t q0, NearK nk,
                      const double2* __restrict__ sctab, double2 alpha, double2 beta, double2* __restrict__ y,
-                     int64_t ldy, int order, int stagger) {
+                     int64_t ldy, int order, int stagger, const double2* __restrict__ ynear) {
     using T2 = typename P::T2;
     using T = typename P::T;
     __shared__ double2 tab[P::kTab ? kSinCosTab : 1];
@@ -1057,13 +1096,15 @@ __global__ void __launch_bounds__(256)
         T2 acc[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) acc[q] = czero2<P>();
-        const double* xp = dpts + 3 * (int64_t)(l0 + rowc);
-        small_walk<P, H, NEG, NQ>(acc, nk, tab, lane, xp[0], xp[1], xp[2], e0, e1, rot, nrlo, nrhi, spts, xs + q0,
-                                  (uint32_t)R);
+        if (!ynear) {
+            const double* xp = dpts + 3 * (int64_t)(l0 + rowc);
+            small_walk<P, H, NEG, NQ>(acc, nk, tab, lane, xp[0], xp[1], xp[2], e0, e1, rot, nrlo, nrhi, spts,
+                                      xs + q0, (uint32_t)R);
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            acc[q].x *= (T)kInvFourPi;
-            acc[q].y *= (T)kInvFourPi;
+            for (int q = 0; q < NQ; ++q) {
+                acc[q].x *= (T)kInvFourPi;
+                acc[q].y *= (T)kInvFourPi;
+            }
         }
         if (B) {
             const double2* yt = yh + (int64_t)dcptr[t] * R + q0;
@@ -1091,7 +1132,9 @@ __global__ void __launch_bounds__(256)
             double2* out = y + (int64_t)dperm[l0 + row0 + rr] + q0 * ldy;
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-                double2 val = cmul(alpha, make_double2((double)acc[q].x, (double)acc[q].y));
+                double2 v = make_double2((double)acc[q].x, (double)acc[q].y);
+                if (ynear) v = cadd(v, ynear[(int64_t)(l0 + row0 + rr) * R + q0 + q]);
+                double2 val = cmul(alpha, v);
                 if (beta.x != 0.0 || beta.y != 0.0) val = cadd(val, cmul(beta, out[q * ldy]));
                 out[q * ldy] = val;
             }
@@ -1448,9 +1491,21 @@ struct Ops<true> {
     static T2* xs(nca_h2* h) { return h->ws.xs32; }
 };
 
+// The stored near entries of the current precision when forward products
+// use them (made and with a workspace), else null.
+template <bool MIX>
+const typename Ops<MIX>::T2* stored_near(const nca_h2* h) {
+    if (!h->near_stored || !h->ws.ynear) return nullptr;
+    if constexpr (MIX) return h->ns_blocks32.p;
+    else return h->ns_blocks.p;
+}
+
 // Stage 5 and the near field on the MFMA kernel for columns q0 .. q0 + nq - 1
 // (nq <= 16 NC): one launch for NC chunks of 16 columns, each kernel
-// evaluation serving all of them; then the bases above the leaves.
+// evaluation serving all of them; then the bases above the leaves. With the
+// near field stored, the forward coupling kernel applies it first into
+// ws.ynear (each stored entry read once for the NC chunks) and the leaf
+// kernel adds that instead of evaluating.
 template <int NC, int OP, bool MIX>
 void run_leaf(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy, double2 alpha, double2 beta) {
     constexpr bool FWD = OP == NCA_OP_N;
@@ -1462,6 +1517,18 @@ void run_leaf(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy,
     SideDev& src = FWD ? h->trial : h->test;
     SideDev& dst = FWD ? h->test : h->trial;
     const double2* yh = h->ws.yh;
+    const T2* NS = FWD ? stored_near<MIX>(h) : nullptr;
+    const double2* ynear = nullptr;
+    if (NS && h->ns_ntiles > 0) {
+        const bool k3 = h->couple_k3[MIX ? 1 : 0] != 0;
+        auto kern = k3 ? k_couple_fwd_mfma<P, false, true, T2, double2, NC>
+                       : k_couple_fwd_mfma<P, false, false, T2, double2, NC>;
+        hipLaunchKernelGGL(kern, dim3(blocks_for(h->ns_ntiles, 4)), dim3(256), 0, st, NS, h->ns_tile_node.p,
+                           h->ns_tile_row0.p, h->ns_ntiles, h->ns_base.p, h->ns_colbase.p, h->ns_ncols.p,
+                           h->ns_colcoef.p, h->ns_cptr.p, O::xs(h) + q0, (uint32_t)R, R, q0, nq, h->ws.ynear,
+                           h->ws.ynear, nullptr, 0);
+        ynear = h->ws.ynear;
+    }
     if (dst.ntiles > 0) {
         // conj(G) for a negative phase sign, again conjugated for the adjoint
         const bool neg = (h->sign < 0) != CJ;
@@ -1474,7 +1541,7 @@ void run_leaf(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy,
                            dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst),
                            dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(), dst.pts.p, dst.nrptr.p, dst.nrlo.p,
                            dst.nrhi.p, src.pts.p, yh, O::xs(h), R, q0, nk, h->sctab.p, alpha, beta, y, ldy,
-                           h->near_xcd, h->near_stagger, nq);
+                           h->near_xcd, h->near_stagger, nq, ynear);
     }
     if (dst.nibasis > 0)
         hipLaunchKernelGGL((k_basis_out<FWD, CJ, T2>), dim3((unsigned)dst.nibasis), dim3(256), 0, st, dst.ibslot.p,
@@ -1654,7 +1721,18 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
     if constexpr (NR == 16) {
         run_leaf<1, OP, MIX>(h, R, nq, q0, y, ldy, alpha, beta);
     } else {
-        // 1..3 columns: the near field on VALU over the leaf tiles
+        // 1..3 columns: the near field on VALU over the leaf tiles (or the
+        // stored one, applied by the VALU forward coupling kernel)
+        const T2* NS = FWD ? stored_near<MIX>(h) : nullptr;
+        const double2* ynear = nullptr;
+        if (NS && h->ns_nleaf > 0) {
+            const int64_t n16 = dst.npts * R;
+            hipLaunchKernelGGL(k_fill, dim3(blocks_for(n16, 256)), dim3(256), 0, st, h->ws.ynear, n16);
+            hipLaunchKernelGGL((k_couple_fwd<NR, false, T2>), dim3((unsigned)h->ns_nleaf), dim3(kWave), 0, st, NS,
+                               h->ns_base.p, h->ns_colbase.p, h->ns_ncols.p, h->ns_colcoef.p, h->ns_cptr.p, xs, R,
+                               q0, h->ws.ynear);
+            ynear = h->ws.ynear;
+        }
         if (dst.ntiles > 0) {
             const bool neg = (h->sign < 0) != CJ;
             const NearK nk{h->k, h->k * (0.5 / M_PI), h->k * (0.25 / M_PI)};
@@ -1663,7 +1741,7 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
                                dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst),
                                dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(), dst.pts.p, dst.nrptr.p, dst.nrlo.p,
                                dst.nrhi.p, src.pts.p, yh, xs, R, q0, nk, h->sctab.p, alpha, beta, y, ldy,
-                               h->near_xcd, h->near_stagger);
+                               h->near_xcd, h->near_stagger, ynear);
         }
         if (dst.nibasis > 0)
             hipLaunchKernelGGL((k_basis_out<FWD, CJ, T2>), dim3((unsigned)dst.nibasis), dim3(256), 0, st,
@@ -1756,8 +1834,10 @@ void ensure_mixed(nca_h2* h) {
     h->mixed_ready = true;
 }
 
-// The transposed coupling of T/C products (defined with the plans below).
+// The transposed coupling of T/C products and the stored near field of
+// forward products in the current precision (defined with the plans below).
 void ensure_reverse(nca_h2* h);
+void ensure_near_store(nca_h2* h);
 
 void ensure_workspace(nca_h2* h, int64_t R) {
     const int64_t np = std::max(h->M, h->N), K = std::max(h->test.K, h->trial.K);
@@ -1776,6 +1856,10 @@ void ensure_workspace(nca_h2* h, int64_t R) {
         if (!h->ws_yc.p) h->ws_yc.alloc((size_t)std::max<int64_t>(K * h->ws_rhs, 1));
     } else {
         if (!h->ws_xs.p) h->ws_xs.alloc((size_t)std::max<int64_t>((np + kNearPad) * h->ws_rhs, 1));
+    }
+    if (h->near_stored && h->ws_ynear.n < (size_t)(np * h->ws_rhs)) {
+        ++h->generation;
+        h->ws_ynear.alloc((size_t)std::max<int64_t>(np * h->ws_rhs, 1));
     }
 }
 
@@ -1816,8 +1900,10 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
     if (!check_product(h, op, R, ldx, ldy)) return;
     if (h->precision == NCA_PREC_MIXED) ensure_mixed(h);
     if (op != NCA_OP_N) ensure_reverse(h);
+    if (op == NCA_OP_N && h->near_stored) ensure_near_store(h);
     ensure_workspace(h, R);
-    h->ws = nca_h2::WsView{h->ws_xs.p, h->ws_xs32.p, h->ws_xh.p, h->ws_yh.p, h->ws_yc.p, h->ws_xh32.p};
+    h->ws = nca_h2::WsView{h->ws_xs.p, h->ws_xs32.p, h->ws_xh.p, h->ws_yh.p, h->ws_yc.p, h->ws_xh32.p,
+                           h->near_stored ? h->ws_ynear.p : nullptr};
     run_sweep(h, op, R, x, ldx, y, ldy, alpha, beta, evs);
 }
 
@@ -1870,6 +1956,9 @@ size_t carve_workspace(const nca_h2* h, int64_t R, char* base, nca_h2::WsView& v
         v.yc = reinterpret_cast<float2*>(take(K * r * sizeof(float2)));
         if (h->couple_x32 && R > 16) v.xh32 = reinterpret_cast<float2*>(take(K * 16 * sizeof(float2)));
     }
+    // the stored near field is used when its entries of this precision exist
+    if (h->near_stored && h->ns_plan && (mix ? h->ns_blocks32.p != nullptr : h->ns_blocks.p != nullptr))
+        v.ynear = reinterpret_cast<double2*>(take(np * r * sizeof(double2)));
     return off;
 }
 
@@ -2355,6 +2444,88 @@ void ensure_reverse(nca_h2* h) {
                            h->rv_rs.p, nb);
     }
     NCA_HIP(hipGetLastError());
+    NCA_HIP(hipStreamSynchronize(st));
+}
+
+// The stored near field of forward products (the reference CPU path keeps
+// the near field as dense block rows, assemblenears
+// src/nca/abstractnca.jl:134-178): per test leaf, in point order, one row of
+// its points over the points of its merged near ranges, entries evaluated
+// once on the device in the current precision (made once per precision;
+// allocates and frees nothing a captured sweep uses).
+void ensure_near_store(nca_h2* h) {
+    const bool mix = h->precision == NCA_PREC_MIXED;
+    if (h->ns_plan && (mix ? h->ns_blocks32.p != nullptr : h->ns_blocks.p != nullptr)) return;
+    hipStream_t st = h->ctx->stream;
+    SideDev& dst = h->test;
+    const SideDev& src = h->trial;
+    if (!h->ns_plan) {
+        std::vector<int32_t> lv = dst.h_leaves_node_order;
+        std::sort(lv.begin(), lv.end(), [&](int32_t a, int32_t b) { return dst.h_lo[a] < dst.h_lo[b]; });
+        const int64_t nleaf = (int64_t)lv.size();
+        std::vector<int32_t> cptr(nleaf + 1, 0), ncols(nleaf);
+        std::vector<int64_t> base(nleaf), colbase(nleaf + 1, 0);
+        for (int64_t i = 0; i < nleaf; ++i) {
+            cptr[i] = dst.h_lo[lv[i]];
+            NCA_REQUIRE(i == 0 ? cptr[0] == 0 : cptr[i] == dst.h_hi[lv[i - 1]], NCA_ERR_ARGUMENT,
+                        "nca_h2_store_near: the test leaves do not tile the points");
+            int64_t C = 0;
+            for (int64_t e = dst.h_nrptr[lv[i]]; e < dst.h_nrptr[lv[i] + 1]; ++e)
+                C += (dst.h_nrhi[e] & 0x7fffffff) - dst.h_nrlo[e];
+            NCA_REQUIRE(C < ((int64_t)1 << 31), NCA_ERR_LIMIT, "nca_h2_store_near: a near row is too long");
+            ncols[i] = (int32_t)C;
+            colbase[i + 1] = colbase[i] + C;
+        }
+        cptr[nleaf] = nleaf ? dst.h_hi[lv[nleaf - 1]] : 0;
+        NCA_REQUIRE(cptr[nleaf] == dst.npts, NCA_ERR_ARGUMENT, "nca_h2_store_near: the test leaves do not tile the points");
+        int64_t off = 0;
+        for (int64_t i = 0; i < nleaf; ++i) {
+            base[i] = off;
+            off += (int64_t)(cptr[i + 1] - cptr[i]) * ncols[i];
+        }
+        std::vector<int32_t> colcoef((size_t)colbase[nleaf]);
+        parallel_for(nleaf, [&](int64_t i) {
+            int64_t c = colbase[i];
+            for (int64_t e = dst.h_nrptr[lv[i]]; e < dst.h_nrptr[lv[i] + 1]; ++e)
+                for (int32_t q = dst.h_nrlo[e]; q < (dst.h_nrhi[e] & 0x7fffffff); ++q) colcoef[c++] = q;
+        });
+        colbase.pop_back();
+        // 16-row tiles, the longest rows first (balance)
+        std::vector<int64_t> order(nleaf);
+        for (int64_t i = 0; i < nleaf; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return ncols[a] > ncols[b]; });
+        std::vector<int32_t> tnode, trow;
+        for (const int64_t i : order)
+            for (int r0 = 0; r0 < cptr[i + 1] - cptr[i]; r0 += 16) {
+                tnode.push_back((int32_t)i);
+                trow.push_back(r0);
+            }
+        h->ns_nleaf = nleaf;
+        h->ns_entries = off;
+        h->ns_ntiles = (int64_t)tnode.size();
+        h->ns_cptr.upload(cptr, st);
+        h->ns_base.upload(base, st);
+        h->ns_colbase.upload(colbase, st);
+        h->ns_ncols.upload(ncols, st);
+        h->ns_colcoef.upload(colcoef, st);
+        h->ns_tile_node.upload(tnode, st);
+        h->ns_tile_row0.upload(trow, st);
+        h->ns_plan = true;
+    }
+    if (h->ns_nleaf > 0) {
+        if (mix) {
+            h->ns_blocks32.alloc((size_t)std::max<int64_t>(h->ns_entries, 1));
+            hipLaunchKernelGGL(k_near_store<float2>, dim3((unsigned)h->ns_nleaf), dim3(256), 0, st, dst.pts.p,
+                               src.pts.p, h->ns_cptr.p, h->ns_base.p, h->ns_colbase.p, h->ns_ncols.p, h->ns_colcoef.p,
+                               h->k, h->sign, h->ns_blocks32.p);
+        } else {
+            h->ns_blocks.alloc((size_t)std::max<int64_t>(h->ns_entries, 1));
+            hipLaunchKernelGGL(k_near_store<double2>, dim3((unsigned)h->ns_nleaf), dim3(256), 0, st, dst.pts.p,
+                               src.pts.p, h->ns_cptr.p, h->ns_base.p, h->ns_colbase.p, h->ns_ncols.p, h->ns_colcoef.p,
+                               h->k, h->sign, h->ns_blocks.p);
+        }
+        NCA_HIP(hipGetLastError());
+    }
     NCA_HIP(hipStreamSynchronize(st));
 }
 
@@ -2891,6 +3062,27 @@ int32_t nca_h2_matvec_device(nca_h2* h2, int32_t op, int64_t nrhs, const void* x
     });
 }
 
+int32_t nca_h2_store_near(nca_h2* h2, int32_t on) {
+    return guarded([&] {
+        NCA_REQUIRE(h2, NCA_ERR_ARGUMENT, "nca_h2_store_near: null operator");
+        std::lock_guard<std::mutex> lock(h2->mu);
+        set_device(h2->ctx);
+        if (on) {
+            h2->near_stored = true;
+            ensure_near_store(h2);
+        } else if (h2->near_stored || h2->ns_plan) {
+            ++h2->generation;  // a captured sweep may read the entries or ws_ynear
+            NCA_HIP(hipStreamSynchronize(h2->ctx->stream));
+            h2->near_stored = h2->ns_plan = false;
+            h2->ns_blocks.release();
+            h2->ns_blocks32.release();
+            h2->ws_ynear.release();
+            h2->ns_colcoef.release();
+            h2->ns_nleaf = h2->ns_ntiles = h2->ns_entries = 0;
+        }
+    });
+}
+
 int32_t nca_h2_matvec_device_noalloc(nca_h2* h2, int32_t op, int64_t nrhs, const void* x, int64_t ldx, void* y,
                                      int64_t ldy, const double* alpha, const double* beta) {
     return guarded([&] {
@@ -2926,6 +3118,7 @@ int32_t nca_h2_graph_create(nca_h2* h2, int32_t op, int64_t nrhs, const void* x,
         // everything that allocates or converts runs before the capture
         if (h2->precision == NCA_PREC_MIXED) ensure_mixed(h2);
         if (op != NCA_OP_N) ensure_reverse(h2);
+        if (op == NCA_OP_N && h2->near_stored) ensure_near_store(h2);
         ensure_workspace(h2, nrhs);
         NCA_HIP(hipStreamSynchronize(st));
         auto g = std::make_unique<nca_h2_graph>();

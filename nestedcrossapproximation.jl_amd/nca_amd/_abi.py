@@ -108,6 +108,7 @@ SIGNATURES = {
     "nca_h2_matvec": (I32, [P, I32, I64, P, I64, P, I64, P, P]),
     "nca_h2_matvec_device": (I32, [P, I32, I64, P, I64, P, I64, P, P]),
     "nca_h2_matvec_device_noalloc": (I32, [P, I32, I64, P, I64, P, I64, P, P]),
+    "nca_h2_store_near": (I32, [P, I32]),
     "nca_h2_matvec_profile": (I32, [P, I32, I64, P, I64, P, I64, P, P, P]),
     "nca_h2_autotune": (I32, [P, I32, P]),
     "nca_h2_graph_create": (I32, [P, I32, I64, P, I64, P, I64, P, P, PP]),

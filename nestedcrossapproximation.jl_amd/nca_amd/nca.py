@@ -156,6 +156,12 @@ class PetrovGalerkinNCACore:
         call(hip_lib().nca_h2_matvec_device(self.handle, int(op), int(nrhs), ctypes.c_void_p(xptr), int(ldx),
                                             ctypes.c_void_p(yptr), int(ldy), _p(al), _p(be)))
 
+    def store_near(self, on=True):
+        """Forward products read the near field from a store made on the device
+        (the reference CPU path's assemblenears layout) instead of evaluating
+        it; store_near(False) drops it (nca_h2_store_near)."""
+        call(hip_lib().nca_h2_store_near(self.handle, int(bool(on))))
+
     def matvec_device_noalloc(self, op, nrhs, xptr, ldx, yptr, ldy, alpha=1.0, beta=0.0):
         """matvec_device on the library's static device scratch: allocates
         nothing (nca_h2_matvec_device_noalloc)."""
