@@ -203,3 +203,47 @@ def test_flat_factors_match_per_node_path():
         assert np.array_equal(b1.blocks, b2.blocks)
         for n in ("leveltoidcsptr", "levelidcs", "idxtochdsptr", "chdsidcs", "offsets", "blocks"):
             assert np.array_equal(getattr(t1, n), getattr(t2, n)), n
+
+
+def test_julia_extension_binds_declared_symbols():
+    """The Julia package extension (nestedcrossapproximation.jl_amd/julia/ext)
+    calls only entry points that include/nca_hip.h declares and the library
+    exports, and its descriptor structs list the C structs' fields in order."""
+    import glob
+    import re
+    root = os.path.join(ROOT, "nestedcrossapproximation.jl_amd", "julia", "ext", "NestedCrossApproximationHIPExt")
+    files = sorted(glob.glob(os.path.join(root, "*.jl")))
+    assert len(files) >= 6
+    header = open(os.path.join(ROOT, "include", "nca_hip.h")).read()
+    declared = set(re.findall(r"\b(nca_[a-z0-9_]+)\(", header))
+    used = set()
+    for f in files:
+        used |= set(re.findall(r"ccall\(\(:(nca_[a-z0-9_]+),", open(f).read()))
+    assert used and used <= declared, sorted(used - declared)
+    lib = nca_amd.hip_lib()
+    for name in used:
+        assert hasattr(lib, name), name
+    main = open(os.path.join(root, "NestedCrossApproximationHIPExt.jl")).read()
+    for f in files:
+        if not f.endswith("NestedCrossApproximationHIPExt.jl"):
+            assert f'include("{os.path.basename(f)}")' in main
+
+    def c_fields(name):
+        end = header.index("} " + name + ";")
+        body = header[header.rindex("typedef struct {", 0, end) + len("typedef struct {"):end]
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+        out = []
+        for decl in body.split(";"):
+            if decl.strip():  # "const int64_t *a, *b": the last identifier of every comma piece
+                out += [re.findall(r"[A-Za-z_]\w*", piece)[-1] for piece in decl.split(",")]
+        return out
+
+    def jl_fields(name, text):
+        body = re.search(r"struct " + name + r"\b[^\n]*\n(.*?)\nend", text, re.S).group(1)
+        body = re.sub(r"#.*", "", body)
+        return re.findall(r"(\w+)::", body)
+
+    op = open(os.path.join(root, "operator.jl")).read()
+    dr = open(os.path.join(root, "directional.jl")).read()
+    assert jl_fields("H2DescC", op) == c_fields("nca_h2_desc")
+    assert jl_fields("DH2DescC", dr) == c_fields("nca_dh2_desc")
