@@ -144,6 +144,42 @@ int32_t nca_compress_level_iaca(nca_ctx* ctx, const nca_stack* st, double tol, i
                                 int64_t* colpivots, int64_t xcapacity, int64_t* xoffsets, double* X,
                                 int64_t* independent);
 
+/* ---- cluster trees on the device ----------------------------------------- */
+typedef struct nca_tree nca_tree;
+/* Regular 2^3 tree over the points: the H2Trees.TwoNTree the reference reads
+ * through halfsize/center/level (src/admissibility/nearinteractions.jl:70-80,
+ * src/admissibility/directionalsubdivision/twondirections.jl:132-158; H2Trees
+ * is not vendored). The bounding cube of the points (center = midpoint of the
+ * bounding box, half size = largest half extent) is halved while a box holds
+ * more than minvalues points and its half size exceeds minhalfsize; empty
+ * octants are dropped; a box whose points all lie in one octant and whose half
+ * size is at most 1e-12 stays a leaf. Octant o = (x >= c) | (y >= c) << 1 |
+ * (z >= c) << 2, children in increasing o, node ids level by level in the
+ * order of the parents, every node's values a contiguous range of the
+ * permutation (stable inside a node). radius = half size * sqrt(3). Built by
+ * level-synchronous kernels on the device. nnodes: the number of nodes. */
+int32_t nca_tree_twon(nca_ctx* ctx, const nca_points* pts, int64_t minvalues, double minhalfsize,
+                      int64_t* nnodes, nca_tree** out);
+/* Deterministic k-means tree (the KMeansTree of src/NestedCrossApproximation.jl:50-52
+ * and test/testutils.jl:32-42, KMeansTreeBuilder(numberofclusters, minvalues);
+ * H2Trees and ParallelKMeans are not vendored): a node with more than minvalues
+ * points is split into at most nclusters (2..8) clusters, seeded by
+ * farthest-point traversal from the point farthest from its centroid, then
+ * Lloyd iterations (at most 100, ties to the lower cluster); empty clusters are
+ * dropped, fewer than two nonempty ones leave a leaf. center = centroid,
+ * radius = largest distance to it. Sums over a node's points are fixed-order
+ * (slot q mod 256 in order, then a halving tree). One workgroup per node of a
+ * level. NCA_ERR_LIMIT above 8 clusters. */
+int32_t nca_tree_kmeans(nca_ctx* ctx, const nca_points* pts, int64_t nclusters, int64_t minvalues,
+                        int64_t* nnodes, nca_tree** out);
+/* The tree in the 1-based layout of the Julia tree: parent (0 at the root),
+ * level (1 at the root), lo/hi (values = perm[lo:hi]), childptr (nnodes+1) and
+ * children (nnodes-1) as CSR, perm (npoints), center (3 x nnodes), radius. Any
+ * output may be NULL. */
+int32_t nca_tree_export(const nca_tree* tree, int64_t* parent, int64_t* level, int64_t* lo, int64_t* hi,
+                        int64_t* childptr, int64_t* children, int64_t* perm, double* center, double* radius);
+int32_t nca_tree_destroy(nca_tree* tree);
+
 /* ---- admissibility partition on the device ------------------------------- */
 typedef struct nca_partition nca_partition;
 /* Near and far cluster pairs of a test and a trial tree, the result of

@@ -27,6 +27,7 @@ HIP_UNITS = [
     ("nca_compress.hip", ["-ffp-contract=off"]),
     ("nca_matvec.hip", []),
     ("nca_partition.hip", ["-ffp-contract=off"]),
+    ("nca_tree.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -135,10 +135,25 @@ Tree build_twon(const double* pts, int64_t n, int64_t minvalues, double minhalfs
 // Deterministic k-means splitting: seeds by farthest-point traversal from the
 // point farthest from the centroid, Lloyd iterations with ties to the lower
 // cluster. Center = centroid, radius = largest distance to it (bounding sphere).
+// Sums over a node's points are fixed-order so that a parallel builder can
+// produce the same bits: slot s accumulates the terms q = s, s + 256, ... in
+// order, then the slots are added by a halving tree (slot i += slot i + w for
+// w = 128, 64, ..., 1).
+constexpr int kSumSlots = 256;
+
+double slot_tree(double* part) {
+    for (int w = kSumSlots / 2; w > 0; w >>= 1)
+        for (int i = 0; i < w; ++i) part[i] += part[i + w];
+    return part[0];
+}
+
 void centroid_radius(const double* pts, const int64_t* ids, int64_t m, double* c, double* r) {
-    double s[3] = {0, 0, 0};
-    for (int64_t k = 0; k < m; ++k)
-        for (int d = 0; d < 3; ++d) s[d] += pts[3 * ids[k] + d];
+    double s[3];
+    for (int d = 0; d < 3; ++d) {
+        double part[kSumSlots] = {0};
+        for (int64_t k = 0; k < m; ++k) part[k % kSumSlots] += pts[3 * ids[k] + d];
+        s[d] = slot_tree(part);
+    }
     for (int d = 0; d < 3; ++d) c[d] = m > 0 ? s[d] / (double)m : 0.0;
     double r2 = 0.0;
     for (int64_t k = 0; k < m; ++k) r2 = std::max(r2, dist2(pts + 3 * ids[k], c));
@@ -163,7 +178,7 @@ Tree build_kmeans(const double* pts, int64_t n, int64_t nclusters, int64_t minva
     std::vector<int> assign;
     for (int64_t node = 0; node < (int64_t)t.parent.size(); ++node) {
         const int64_t lo = t.lo[node], hi = t.hi[node], m = hi - lo;
-        if (m <= minvalues) continue;
+        if (m <= minvalues || m == 0) continue;
         const int64_t* ids = &t.perm[lo];
         std::vector<double> cent(3 * k);
         const double* c0 = &t.center[3 * node];
@@ -203,10 +218,13 @@ Tree build_kmeans(const double* pts, int64_t n, int64_t nclusters, int64_t minva
             if (!changed) break;
             std::vector<double> s(3 * kc, 0.0);
             std::vector<int64_t> cnt(kc, 0);
+            std::vector<double> part((size_t)kSumSlots * 3 * kc, 0.0);
             for (int64_t q = 0; q < m; ++q) {
                 cnt[assign[q]]++;
-                for (int d = 0; d < 3; ++d) s[3 * assign[q] + d] += pts[3 * ids[q] + d];
+                for (int d = 0; d < 3; ++d)
+                    part[(size_t)(3 * assign[q] + d) * kSumSlots + q % kSumSlots] += pts[3 * ids[q] + d];
             }
+            for (int j = 0; j < 3 * kc; ++j) s[j] = slot_tree(&part[(size_t)j * kSumSlots]);
             for (int j = 0; j < kc; ++j)
                 if (cnt[j] > 0)
                     for (int d = 0; d < 3; ++d) cent[3 * j + d] = s[3 * j + d] / (double)cnt[j];

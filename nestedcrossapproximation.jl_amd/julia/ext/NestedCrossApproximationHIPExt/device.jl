@@ -18,6 +18,46 @@ struct TreeC
     nvalues::Vector{Int64}
 end
 
+# Cluster trees built on the device: the regular 2^3 tree (the TwoNTree the
+# reference reads through halfsize/center/level, nearinteractions.jl:70-80) and
+# the deterministic k-means tree (KMeansTree(pts; builder=KMeansTreeBuilder(...)),
+# src/NestedCrossApproximation.jl:50-52). Both return a TreeC for the
+# partition and Phase-1 bindings below.
+function _treec(h::Ptr{Cvoid}, nn::Int64, npoints::Integer)
+    parent, level, lo, hi = (Vector{Int64}(undef, nn) for _ in 1:4)
+    childptr = Vector{Int64}(undef, nn + 1)
+    children = Vector{Int64}(undef, max(nn - 1, 1))
+    perm = Vector{Int64}(undef, max(npoints, 1))
+    center = Vector{Float64}(undef, 3nn)
+    radius = Vector{Float64}(undef, nn)
+    try
+        check(ccall((:nca_tree_export, libnca_hip), Int32,
+            (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+             Ptr{Int64}, Ptr{Float64}, Ptr{Float64}),
+            h, parent, level, lo, hi, childptr, children, perm, center, radius))
+    finally
+        ccall((:nca_tree_destroy, libnca_hip), Int32, (Ptr{Cvoid},), h)
+    end
+    return TreeC(nn, childptr, children[1:(nn - 1)], center, radius, lo, hi, perm[1:npoints], parent,
+                 hi .- lo .+ 1)
+end
+
+function hiptwontree(ctx::Context, pts::Points; minvalues::Integer=64, minhalfsize::Real=0.0)
+    nn, h = Ref{Int64}(0), Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:nca_tree_twon, libnca_hip), Int32,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Float64, Ref{Int64}, Ref{Ptr{Cvoid}}),
+        ctx.ptr, pts.ptr, minvalues, minhalfsize, nn, h))
+    return _treec(h[], nn[], pts.n)
+end
+
+function hipkmeanstree(ctx::Context, pts::Points; numberofclusters::Integer=2, minvalues::Integer=32)
+    nn, h = Ref{Int64}(0), Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:nca_tree_kmeans, libnca_hip), Int32,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Ref{Int64}, Ref{Ptr{Cvoid}}),
+        ctx.ptr, pts.ptr, numberofclusters, minvalues, nn, h))
+    return _treec(h[], nn[], pts.n)
+end
+
 # Admissible and near pairs: far iff max diameter <= eta * distance, or the
 # wideband criterion isnearwideband (src/admissibility/nearinteractions.jl:83-103).
 function hipinteractions(ctx::Context, a::TreeC, b::TreeC; eta=1.0, k=0.0, wideband::Bool=false,

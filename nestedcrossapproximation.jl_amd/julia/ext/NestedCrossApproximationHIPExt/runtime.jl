@@ -40,12 +40,13 @@ end
 
 mutable struct Points      # device copy of a point set
     ptr::Ptr{Cvoid}
+    n::Int64
     function Points(ctx::Context, pts::AbstractVector{SVector{3,Float64}})
         v = Vector{SVector{3,Float64}}(pts)
         r = Ref{Ptr{Cvoid}}()
         GC.@preserve v check(ccall((:nca_points_create, libnca_hip), Int32,
             (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Ptr{Cvoid}}), ctx.ptr, Ptr{Float64}(pointer(v)), length(v), r))
-        p = new(r[])
+        p = new(r[], length(v))
         finalizer(p -> ccall((:nca_points_destroy, libnca_hip), Int32, (Ptr{Cvoid},), p.ptr), p)
         return p
     end

@@ -98,6 +98,10 @@ SIGNATURES = {
     "nca_interactions": (I32, [P, I64, P, P, P, P, I64, P, P, P, P, I32, P, P, P, PP]),
     "nca_partition_export": (I32, [P, P, P, P, P]),
     "nca_partition_destroy": (I32, [P]),
+    "nca_tree_twon": (I32, [P, P, I64, D, P, PP]),
+    "nca_tree_kmeans": (I32, [P, P, I64, I64, P, PP]),
+    "nca_tree_export": (I32, [P, P, P, P, P, P, P, P, P, P]),
+    "nca_tree_destroy": (I32, [P]),
     "nca_coupling_size": (I32, [P, P, P]),
     "nca_coupling_download": (I32, [P, P, P, P]),
     "nca_coupling_destroy": (I32, [P]),

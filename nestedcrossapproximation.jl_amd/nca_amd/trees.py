@@ -99,16 +99,50 @@ def _export_tree(handle, points, kind):
     return ClusterTree(points, parent, level, lo, hi, childptr, children, perm, center, radius, kind)
 
 
-def TwoNTree(points, minvalues=64, minhalfsize=0.0):
-    """Regular 2^3 tree (octree) over ``points`` (n x 3)."""
+def _device_tree(ctx, pts, kind, build):
+    """A tree built on the device (nca_tree_twon / nca_tree_kmeans), exported
+    from the 1-based layout of the C-ABI."""
+    from ._native import hip_lib
+    from .runtime import Points, call
+    L = hip_lib()
+    dpts = Points(ctx, pts)
+    nn = ctypes.c_int64()
+    h = ctypes.c_void_p()
+    call(build(L, dpts.handle, ctypes.byref(nn), ctypes.byref(h)))
+    try:
+        nn, n = nn.value, pts.shape[0]
+        parent, level, lo, hi = (np.empty(nn, np.int64) for _ in range(4))
+        childptr = np.empty(nn + 1, np.int64)
+        children = np.empty(max(nn - 1, 1), np.int64)
+        perm = np.empty(max(n, 1), np.int64)
+        center = np.empty((nn, 3), np.float64)
+        radius = np.empty(nn, np.float64)
+        call(L.nca_tree_export(h, _ptr(parent), _ptr(level), _ptr(lo), _ptr(hi), _ptr(childptr),
+                               _ptr(children), _ptr(perm), _ptr(center), _ptr(radius)))
+    finally:
+        L.nca_tree_destroy(h)
+    return ClusterTree(pts, parent - 1, level - 1, lo - 1, hi, childptr - 1, children[:nn - 1] - 1,
+                       perm[:n] - 1, center, radius, kind)
+
+
+def TwoNTree(points, minvalues=64, minhalfsize=0.0, ctx=None):
+    """Regular 2^3 tree (octree) over ``points`` (n x 3); built on the device of
+    ``ctx`` when one is given (the same tree)."""
     pts = np.ascontiguousarray(points, dtype=np.float64)
+    if ctx is not None:
+        return _device_tree(ctx, pts, "twon", lambda L, p, nn, h: L.nca_tree_twon(
+            ctx.handle, p, int(minvalues), float(minhalfsize), nn, h))
     h = tree_lib().ncat_tree_twon(_ptr(pts), pts.shape[0], int(minvalues), float(minhalfsize))
     return _export_tree(h, pts, "twon")
 
 
-def KMeansTree(points, numberofclusters=2, minvalues=32):
-    """Deterministic k-means tree over ``points`` (n x 3)."""
+def KMeansTree(points, numberofclusters=2, minvalues=32, ctx=None):
+    """Deterministic k-means tree over ``points`` (n x 3); built on the device
+    of ``ctx`` when one is given (the same tree)."""
     pts = np.ascontiguousarray(points, dtype=np.float64)
+    if ctx is not None:
+        return _device_tree(ctx, pts, "kmeans", lambda L, p, nn, h: L.nca_tree_kmeans(
+            ctx.handle, p, int(numberofclusters), int(minvalues), nn, h))
     h = tree_lib().ncat_tree_kmeans(_ptr(pts), pts.shape[0], int(numberofclusters), int(minvalues))
     return _export_tree(h, pts, "kmeans")
 
