@@ -144,6 +144,28 @@ int32_t nca_compress_level_iaca(nca_ctx* ctx, const nca_stack* st, double tol, i
                                 int64_t* colpivots, int64_t xcapacity, int64_t* xoffsets, double* X,
                                 int64_t* independent);
 
+/* ---- factors kept on the device ------------------------------------------ */
+typedef struct nca_factors nca_factors;
+/* A device pool of skeleton factors X for block ids 1..nids (the nodes of a
+ * tree, or the direction indices of directional data). The reference's GPU
+ * path downloads Z after every level (compresslevel.jl:131) and writes the
+ * bases and transfers on the CPU (compresslevel.jl:242-316); with a pool the
+ * factors stay on the device until nca_h2_create_resident /
+ * nca_dh2_create_resident pack the stores from them. */
+int32_t nca_factors_create(nca_ctx* ctx, int64_t nids, nca_factors** out);
+int32_t nca_factors_destroy(nca_factors* f);
+/* nca_compress_level (method 0, fully pivoted ACA with the sampled stop) or
+ * nca_compress_level_iaca (method 1) with X kept in the pool: block b of the
+ * stack becomes the factor of ids[b] (1-based; an id takes one factor).
+ * ranks, pivots and independent as nca_compress_level. */
+int32_t nca_compress_level_resident(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, int32_t method,
+                                    nca_factors* f, const int64_t* ids, int32_t* ranks, int64_t* rowpivots,
+                                    int64_t* colpivots, int64_t* independent);
+/* Total complex entries, and all factors downloaded: the X of every
+ * nca_compress_level_resident call in call order (the X of nca_compress_level). */
+int32_t nca_factors_size(const nca_factors* f, int64_t* nentries);
+int32_t nca_factors_download(nca_ctx* ctx, const nca_factors* f, double* X);
+
 /* ---- cluster trees on the device ----------------------------------------- */
 typedef struct nca_tree nca_tree;
 /* Regular 2^3 tree over the points: the H2Trees.TwoNTree the reference reads
@@ -284,6 +306,17 @@ typedef struct {
 int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test,
                       const nca_points* trial, const nca_h2_desc* desc, const nca_coupling* coupling,
                       int32_t precision, nca_h2** out);
+/* nca_h2_create with the basis and transfer stores packed on the device from
+ * resident factors (indexed by node): basis of node t = X_t (test) or X_t^T
+ * (trial); the transfer block of child c of p = the rows of X_p that hold c's
+ * pivots (test) or their transpose (trial), the rows of X_p being the pivots
+ * of the children in the order the transfer plan lists them. The desc's block
+ * pointers (basis_*, tt_blocks, st_blocks) are not read; its offsets must fit
+ * the factors (NCA_ERR_DIMENSION). The pools may be destroyed afterwards. */
+int32_t nca_h2_create_resident(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test,
+                               const nca_points* trial, const nca_h2_desc* desc, const nca_coupling* coupling,
+                               int32_t precision, const nca_factors* testf, const nca_factors* trialf,
+                               nca_h2** out);
 int32_t nca_h2_destroy(nca_h2* h2);
 int32_t nca_h2_set_precision(nca_h2* h2, int32_t precision);
 
@@ -329,6 +362,13 @@ typedef struct {
 int32_t nca_dh2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test,
                        const nca_points* trial, const nca_dh2_desc* desc, const nca_coupling* coupling,
                        int32_t precision, nca_h2** out);
+/* nca_dh2_create with the stores packed on the device from resident factors
+ * indexed by direction index (as nca_h2_create_resident; the bases of a
+ * node's directions become the node's basis over all its coefficients). */
+int32_t nca_dh2_create_resident(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test,
+                                const nca_points* trial, const nca_dh2_desc* desc, const nca_coupling* coupling,
+                                int32_t precision, const nca_factors* testf, const nca_factors* trialf,
+                                nca_h2** out);
 
 /* Y = alpha * op(A) * X + beta * Y with op N/T/C, nrhs columns. Replaces
  * LinearMaps._unsafe_mul!(y, A::Union{PetrovGalerkinNCA, TransposeMap,

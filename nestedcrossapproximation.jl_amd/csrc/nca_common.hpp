@@ -124,6 +124,40 @@ struct nca_coupling {
     nca::DevBuf<double2> blocks;
 };
 
+// Skeleton factors X kept on the device, indexed by block id (a node, or a
+// direction index): one device chunk per compressed level, each holding that
+// level's factors in stack order.
+struct nca_factors {
+    nca_ctx* ctx = nullptr;
+    int64_t nids = 0;
+    std::vector<int32_t> chunk, m, rank;  // per id; chunk -1: no factor
+    std::vector<int64_t> off;             // per id: complex offset in its chunk
+    std::vector<nca::DevBuf<double2>> chunks;
+    std::vector<int64_t> chunk_size;      // complex entries per chunk
+};
+
 namespace nca {
 inline void set_device(const nca_ctx* ctx) { NCA_HIP(hipSetDevice(ctx->device)); }
+
+// One block copy on the device: element (a, k) of the rows x cols source
+// (column-major, leading dimension ld) goes to dst[dst + a * sa + k * sk].
+struct PackJob {
+    const double2* src;
+    int64_t dst;
+    int32_t ld, rows, cols, sa, sk, pad;
+};
+
+// Runs the jobs into dst on stream st (the jobs are uploaded first).
+void pack_blocks(const std::vector<PackJob>& jobs, double2* dst, hipStream_t st);
+
+// Device address, rows and rank of the factor of block id (0-based).
+struct FactorRef {
+    const double2* p;
+    int64_t m, r;
+};
+inline FactorRef factor_of(const nca_factors* f, int64_t id) {
+    NCA_REQUIRE(id >= 0 && id < f->nids && f->chunk[id] >= 0, NCA_ERR_ARGUMENT,
+                "resident factors: block " + std::to_string(id + 1) + " has no factor");
+    return FactorRef{f->chunks[f->chunk[id]].p + f->off[id], f->m[id], f->rank[id]};
+}
 }  // namespace nca

@@ -1013,9 +1013,147 @@ static void compress_level_impl(nca_ctx* ctx, const nca_stack* st, double tol, i
         std::copy(indep.begin(), indep.end(), independent);
         singular_message(st, ranks, indep);
 }
+
+// compress_level_impl keeping X on the device: the level's factors become
+// one chunk of f, block b of the stack the factor of ids[b] (1-based).
+static void compress_level_keep(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, bool iaca,
+                                nca_factors* f, const int64_t* ids, int32_t* ranks, int64_t* rowpivots,
+                                int64_t* colpivots, int64_t* independent) {
+    NCA_REQUIRE(ctx && st && f && ranks && rowpivots && colpivots && independent && (st->nb == 0 || ids),
+                NCA_ERR_ARGUMENT, "nca_compress_level_resident: invalid argument");
+    NCA_REQUIRE(f->ctx == ctx, NCA_ERR_ARGUMENT, "nca_compress_level_resident: factors of another context");
+    const int64_t nb = st->nb;
+    {
+        std::vector<char> seen;
+        for (int64_t b = 0; b < nb; ++b) {
+            const int64_t id = ids[b] - 1;
+            NCA_REQUIRE(id >= 0 && id < f->nids, NCA_ERR_ARGUMENT,
+                        "nca_compress_level_resident: block id " + std::to_string(ids[b]) + " out of range");
+            NCA_REQUIRE(f->chunk[id] < 0, NCA_ERR_ARGUMENT,
+                        "nca_compress_level_resident: block id " + std::to_string(ids[b]) + " has a factor already");
+            f->chunk[id] = -2;  // taken by this call (duplicates fail here)
+        }
+    }
+    auto undo = [&] {
+        for (int64_t b = 0; b < nb; ++b)
+            if (f->chunk[ids[b] - 1] == -2) f->chunk[ids[b] - 1] = -1;
+    };
+    try {
+        set_device(ctx);
+        AcaResult res;
+        launch_aca(ctx, st, tol, sampled, res, iaca);
+        if (nb == 0) return;
+        d2h(ctx, ranks, res.ranks.p, sizeof(int32_t) * nb);
+        std::vector<int64_t> xoff(nb + 1, 0);
+        for (int64_t b = 0; b < nb; ++b)
+            xoff[b + 1] = xoff[b] + (st->rowoffsets[b + 1] - st->rowoffsets[b]) * (int64_t)ranks[b];
+        DevBuf<double2> d_X;
+        DevBuf<int64_t> d_indep;
+        launch_skeleton(ctx, st, ranks, res.ranks, res.rowpiv, res.colpiv, xoff, d_X, d_indep);
+        d2h(ctx, rowpivots, res.rowpiv.p, sizeof(int64_t) * nb * st->maxrank);
+        d2h(ctx, colpivots, res.colpiv.p, sizeof(int64_t) * nb * st->maxrank);
+        std::vector<int64_t> indep(nb);
+        d2h(ctx, indep.data(), d_indep.p, sizeof(int64_t) * nb);
+        const int32_t c = (int32_t)f->chunks.size();
+        f->chunks.push_back(std::move(d_X));
+        f->chunk_size.push_back(xoff[nb]);
+        for (int64_t b = 0; b < nb; ++b) {
+            const int64_t id = ids[b] - 1;
+            f->chunk[id] = c;
+            f->off[id] = xoff[b];
+            f->m[id] = (int32_t)(st->rowoffsets[b + 1] - st->rowoffsets[b]);
+            f->rank[id] = ranks[b];
+        }
+        std::copy(indep.begin(), indep.end(), independent);
+        singular_message(st, ranks, indep);
+    } catch (...) {
+        undo();
+        throw;
+    }
+}
+
+// Block copies, one 64-lane wave per job.
+__global__ void __launch_bounds__(256) k_pack_blocks(const PackJob* __restrict__ jobs, int64_t njobs,
+                                                     double2* __restrict__ dst) {
+    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= njobs) return;
+    const PackJob J = jobs[j];
+    const int lane = threadIdx.x & 63;
+    const int64_t total = (int64_t)J.rows * J.cols;
+    for (int64_t q = lane; q < total; q += 64) {
+        const int64_t a = q % J.rows, k = q / J.rows;
+        dst[J.dst + a * J.sa + k * J.sk] = J.src[a + k * J.ld];
+    }
+}
+
+void pack_blocks(const std::vector<PackJob>& jobs, double2* dst, hipStream_t st) {
+    if (jobs.empty()) return;
+    DevBuf<PackJob> dj;
+    dj.upload(jobs, st);
+    const int64_t nj = (int64_t)jobs.size();
+    hipLaunchKernelGGL(k_pack_blocks, dim3((unsigned)((nj + 3) / 4)), dim3(256), 0, st, dj.p, nj, dst);
+    NCA_HIP(hipGetLastError());
+    NCA_HIP(hipStreamSynchronize(st));
+}
 }  // namespace nca
 
 extern "C" {
+
+int32_t nca_factors_create(nca_ctx* ctx, int64_t nids, nca_factors** out) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && out && nids >= 0, NCA_ERR_ARGUMENT, "nca_factors_create: invalid argument");
+        auto f = std::make_unique<nca_factors>();
+        f->ctx = ctx;
+        f->nids = nids;
+        f->chunk.assign(nids, -1);
+        f->m.assign(nids, 0);
+        f->rank.assign(nids, 0);
+        f->off.assign(nids, 0);
+        *out = f.release();
+    });
+}
+
+int32_t nca_factors_destroy(nca_factors* f) {
+    return guarded([&] {
+        if (!f) return;
+        if (f->ctx) set_device(f->ctx);
+        delete f;
+    });
+}
+
+int32_t nca_factors_size(const nca_factors* f, int64_t* nentries) {
+    return guarded([&] {
+        NCA_REQUIRE(f && nentries, NCA_ERR_ARGUMENT, "nca_factors_size: invalid argument");
+        int64_t n = 0;
+        for (int64_t c : f->chunk_size) n += c;
+        *nentries = n;
+    });
+}
+
+int32_t nca_factors_download(nca_ctx* ctx, const nca_factors* f, double* X) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && f && X, NCA_ERR_ARGUMENT, "nca_factors_download: invalid argument");
+        set_device(ctx);
+        double2* dst = reinterpret_cast<double2*>(X);
+        for (size_t c = 0; c < f->chunks.size(); ++c) {
+            if (f->chunk_size[c] > 0)
+                NCA_HIP(hipMemcpyAsync(dst, f->chunks[c].p, sizeof(double2) * f->chunk_size[c],
+                                       hipMemcpyDeviceToHost, ctx->stream));
+            dst += f->chunk_size[c];
+        }
+        NCA_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int32_t nca_compress_level_resident(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, int32_t method,
+                                    nca_factors* f, const int64_t* ids, int32_t* ranks, int64_t* rowpivots,
+                                    int64_t* colpivots, int64_t* independent) {
+    return guarded([&] {
+        NCA_REQUIRE(method == 0 || method == 1, NCA_ERR_ARGUMENT, "nca_compress_level_resident: unknown method");
+        compress_level_keep(ctx, st, tol, method == 1 ? 0 : sampled, method == 1, f, ids, ranks, rowpivots,
+                            colpivots, independent);
+    });
+}
 
 int32_t nca_compress_level(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, int32_t* ranks,
                            int64_t* rowpivots, int64_t* colpivots, int64_t xcapacity, int64_t* xoffsets,

@@ -2038,7 +2038,8 @@ void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, int64_t n
                 bool multiparent, const int64_t* perm, const int64_t* lo, const int64_t* hi, const int64_t* cptr,
                 const nca_points* pts, int64_t nbasis, const int64_t* bnodes, const int64_t* boff,
                 const double* bblocks, bool testside, int64_t nlev, const int64_t* levptr, const int64_t* levidcs,
-                const int64_t* chdptr, const int64_t* chds, const int64_t* toff, const double* tblocks) {
+                const int64_t* chdptr, const int64_t* chds, const int64_t* toff, const double* tblocks,
+                const std::vector<PackJob>* bjobs = nullptr, const std::vector<PackJob>* tjobs = nullptr) {
     sd.npts = npts;
     sd.nn = nn;
     sd.ne = ne;
@@ -2142,7 +2143,13 @@ void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, int64_t n
         sd.nptile = (int64_t)pt.size() / 2;
         sd.ptile.upload(pt, st);
     }
-    if (nbasis) sd.bblocks.upload(reinterpret_cast<const double2*>(bblocks) + boff[0], (size_t)bo[nbasis], st);
+    if (nbasis && bjobs) {
+        // packed on the device from resident factors
+        sd.bblocks.alloc((size_t)bo[nbasis]);
+        pack_blocks(*bjobs, sd.bblocks.p, st);
+    } else if (nbasis) {
+        sd.bblocks.upload(reinterpret_cast<const double2*>(bblocks) + boff[0], (size_t)bo[nbasis], st);
+    }
     // transfers
     sd.nlev = nlev;
     sd.levptr.assign(nlev + 1, 0);
@@ -2178,7 +2185,12 @@ void build_side(SideDev& sd, hipStream_t st, int64_t npts, int64_t nn, int64_t n
     sd.chdptr.upload(cptr0, st);
     sd.chds.upload(ch, st);
     sd.toff.upload(to, st);
-    if (nedge && to[nedge]) sd.tblocks.upload(reinterpret_cast<const double2*>(tblocks) + toff[0], (size_t)to[nedge], st);
+    if (nedge && to[nedge] && tjobs) {
+        sd.tblocks.alloc((size_t)to[nedge]);
+        pack_blocks(*tjobs, sd.tblocks.p, st);
+    } else if (nedge && to[nedge]) {
+        sd.tblocks.upload(reinterpret_cast<const double2*>(tblocks) + toff[0], (size_t)to[nedge], st);
+    }
     sd.down_by_child = multiparent;
     if (multiparent) {
         // per level of parents: the children (first appearance order), each
@@ -2598,11 +2610,73 @@ void time_stages(nca_h2* h, const double2* x, double2* y, int nreps, double& cou
 // Coefficient blocks of a directional operator: per side the number of
 // direction indices and their pointer per node (1-based, nn+1). The plain
 // operator has one block per node (dirptr null).
+// Block copies that pack the stores from resident factors (test/trial bases
+// and transfers).
+struct ResidentJobs {
+    std::vector<PackJob> tb, tt, sb, st;
+};
+
 struct BlockLayout {
     int64_t ne_test = 0, ne_trial = 0;
     const int64_t* test_dirptr = nullptr;
     const int64_t* trial_dirptr = nullptr;
+    const ResidentJobs* jobs = nullptr;  // stores packed from resident factors
 };
+
+// Basis store from resident factors: basis i of node (or direction index)
+// bnodes[i] is X (m x r, test) or X^T (r x m, trial), at boff[i].
+void basis_jobs(const nca_factors* f, bool testside, int64_t nbasis, const int64_t* bnodes, const int64_t* boff,
+                const int64_t* cptr, std::vector<PackJob>& out) {
+    out.clear();
+    out.reserve(nbasis);
+    for (int64_t i = 0; i < nbasis; ++i) {
+        const int64_t id = bnodes[i] - 1;
+        const FactorRef F = factor_of(f, id);
+        NCA_REQUIRE(F.r == cptr[id + 1] - cptr[id] && boff[i + 1] - boff[i] == F.m * F.r, NCA_ERR_DIMENSION,
+                    "resident factors: the factor of basis block " + std::to_string(id + 1) +
+                        " differs from the plan's rank or size");
+        const int64_t dst = boff[i] - boff[0];
+        if (testside) out.push_back({F.p, dst, (int32_t)F.m, (int32_t)F.m, (int32_t)F.r, 1, (int32_t)F.m, 0});
+        else out.push_back({F.p, dst, (int32_t)F.m, (int32_t)F.m, (int32_t)F.r, (int32_t)F.r, 1, 0});
+    }
+}
+
+// Transfer store from resident factors: the rows of a parent's factor are
+// its children's pivots, in the order the plan lists the children, so the
+// block of child c is X_p[oc : oc + r_c, :] (test, r_c x r_p) or its transpose
+// (trial), oc = the ranks of the children before c.
+void transfer_jobs(const nca_factors* f, bool testside, int64_t nlev, const int64_t* levptr, const int64_t* levidcs,
+                   const int64_t* chdptr, const int64_t* chds, const int64_t* toff, const int64_t* cptr,
+                   std::vector<PackJob>& out) {
+    out.clear();
+    const int64_t npar = nlev ? levptr[nlev] - levptr[0] : 0;
+    for (int64_t i = 0; i < npar; ++i) {
+        const int64_t p = levidcs[i] - 1;
+        const FactorRef P = factor_of(f, p);
+        NCA_REQUIRE(P.r == cptr[p + 1] - cptr[p], NCA_ERR_DIMENSION,
+                    "resident factors: the factor of transfer parent " + std::to_string(p + 1) +
+                        " differs from the plan's rank");
+        int64_t oc = 0;
+        for (int64_t e = chdptr[i] - chdptr[0]; e < chdptr[i + 1] - chdptr[0]; ++e) {
+            const int64_t c = chds[e] - 1;
+            const int64_t rc = cptr[c + 1] - cptr[c];
+            NCA_REQUIRE(oc + rc <= P.m && toff[e + 1] - toff[e] == rc * P.r, NCA_ERR_DIMENSION,
+                        "resident factors: transfer block of child " + std::to_string(c + 1) +
+                            " does not fit its parent's factor");
+            const int64_t dst = toff[e] - toff[0];
+            if (rc > 0) {
+                if (testside)
+                    out.push_back({P.p + oc, dst, (int32_t)P.m, (int32_t)rc, (int32_t)P.r, 1, (int32_t)rc, 0});
+                else
+                    out.push_back({P.p + oc, dst, (int32_t)P.m, (int32_t)rc, (int32_t)P.r, (int32_t)P.r, 1, 0});
+            }
+            oc += rc;
+        }
+        NCA_REQUIRE(oc == P.m, NCA_ERR_DIMENSION,
+                    "resident factors: the children listed for transfer parent " + std::to_string(p + 1) +
+                        " do not make up its factor's rows");
+    }
+}
 
 // nca_h2_create with d's coefficient plans, transfers and coupling plan over
 // the coefficient blocks of lay (trees, bases and near field over nodes).
@@ -2647,11 +2721,13 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
         build_side(h->test, st, d->M, d->nn_test, lay.ne_test, lay.test_dirptr, directional, d->test_perm,
                    d->test_lo, d->test_hi, d->test_cptr, test, d->nbasis_test, d->basis_test_nodes,
                    d->basis_test_off, d->basis_test, true, d->nlev_test, d->tt_levptr, d->tt_levidcs, d->tt_chdptr,
-                   d->tt_chds, d->tt_off, d->tt_blocks);
+                   d->tt_chds, d->tt_off, d->tt_blocks, lay.jobs ? &lay.jobs->tb : nullptr,
+                   lay.jobs ? &lay.jobs->tt : nullptr);
         build_side(h->trial, st, d->N, d->nn_trial, lay.ne_trial, lay.trial_dirptr, directional, d->trial_perm,
                    d->trial_lo, d->trial_hi, d->trial_cptr, trial, d->nbasis_trial, d->basis_trial_nodes,
                    d->basis_trial_off, d->basis_trial, false, d->nlev_trial, d->st_levptr, d->st_levidcs,
-                   d->st_chdptr, d->st_chds, d->st_off, d->st_blocks);
+                   d->st_chdptr, d->st_chds, d->st_off, d->st_blocks, lay.jobs ? &lay.jobs->sb : nullptr,
+                   lay.jobs ? &lay.jobs->st : nullptr);
         timer.mark("build_side");
         build_leaves(h->test, st);
         build_leaves(h->trial, st);
@@ -2883,13 +2959,45 @@ int32_t nca_h2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_point
     });
 }
 
+int32_t nca_h2_create_resident(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test,
+                               const nca_points* trial, const nca_h2_desc* d, const nca_coupling* coupling,
+                               int32_t precision, const nca_factors* testf, const nca_factors* trialf,
+                               nca_h2** out) {
+    return guarded([&] {
+        NCA_REQUIRE(d && testf && trialf, NCA_ERR_ARGUMENT, "nca_h2_create_resident: invalid argument");
+        NCA_REQUIRE(testf->nids == d->nn_test && trialf->nids == d->nn_trial, NCA_ERR_DIMENSION,
+                    "nca_h2_create_resident: factors of another tree");
+        NCA_REQUIRE(testf->ctx == ctx && trialf->ctx == ctx, NCA_ERR_ARGUMENT,
+                    "nca_h2_create_resident: factors of another context");
+        set_device(ctx);
+        ResidentJobs rj;
+        basis_jobs(testf, true, d->nbasis_test, d->basis_test_nodes, d->basis_test_off, d->test_cptr, rj.tb);
+        basis_jobs(trialf, false, d->nbasis_trial, d->basis_trial_nodes, d->basis_trial_off, d->trial_cptr, rj.sb);
+        transfer_jobs(testf, true, d->nlev_test, d->tt_levptr, d->tt_levidcs, d->tt_chdptr, d->tt_chds, d->tt_off,
+                      d->test_cptr, rj.tt);
+        transfer_jobs(trialf, false, d->nlev_trial, d->st_levptr, d->st_levidcs, d->st_chdptr, d->st_chds, d->st_off,
+                      d->trial_cptr, rj.st);
+        BlockLayout lay;
+        lay.ne_test = d->nn_test;
+        lay.ne_trial = d->nn_trial;
+        lay.jobs = &rj;
+        create_core(ctx, kernel, test, trial, d, lay, coupling, precision, out);
+    });
+}
+
 // The directional plans lowered onto the block form of create_core: the
 // bases of a node's directions become one basis of the node over all its
 // coefficients (test [U_1 ... U_k], trial [V_1; ...; V_k]); the coupling plan
 // becomes a CSR over test direction indices.
-int32_t nca_dh2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test, const nca_points* trial,
-                       const nca_dh2_desc* d, const nca_coupling* coupling, int32_t precision, nca_h2** out) {
-    return guarded([&] {
+}  // extern "C"
+
+namespace nca {
+// nca_dh2_create, with the stores packed on the device from resident factors
+// when testf / trialf are given.
+void dh2_create_impl(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test, const nca_points* trial,
+                     const nca_dh2_desc* d, const nca_coupling* coupling, int32_t precision,
+                     const nca_factors* testf, const nca_factors* trialf, nca_h2** out) {
+    {
         NCA_REQUIRE(d && coupling, NCA_ERR_ARGUMENT, "nca_dh2_create: invalid argument");
         NCA_REQUIRE(d->test_dirptr && d->trial_dirptr, NCA_ERR_ARGUMENT, "nca_dh2_create: missing direction pointers");
         struct Lowered {
@@ -2898,7 +3006,8 @@ int32_t nca_dh2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_poin
         };
         auto lower = [](bool testside, int64_t nn, const int64_t* lo, const int64_t* hi, const int64_t* dirptr,
                         const int64_t* cptr, int64_t nb, const int64_t* bnodes, const int64_t* bdirptr,
-                        const int64_t* bdiridx, const int64_t* boff, const double* blocks, Lowered& L) {
+                        const int64_t* bdiridx, const int64_t* boff, const double* blocks, Lowered& L,
+                        const nca_factors* f, std::vector<PackJob>* jobs) {
             const double2* B = reinterpret_cast<const double2*>(blocks);
             L.off.assign(1, 0);
             const char* side = testside ? "test" : "trial";
@@ -2935,6 +3044,32 @@ int32_t nca_dh2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_poin
                 L.nodes.push_back(t + 1);
                 L.off.push_back(L.off.back() + m * R);
             }
+            if (f) {
+                // the node blocks packed on the device from the direction factors
+                jobs->clear();
+                for (int64_t i = 0; i < nb; ++i) {
+                    const int64_t t = bnodes[i] - 1;
+                    const int64_t m = hi[t] - lo[t] + 1;
+                    const int64_t e0 = dirptr[t] - 1, e1 = dirptr[t + 1] - 1;
+                    const int64_t c0 = cptr[e0] - 1, R = cptr[e1] - cptr[e0];
+                    for (int64_t j = bdirptr[i] - 1; j < bdirptr[i + 1] - 1; ++j) {
+                        const int64_t e = bdiridx[j] - 1, r = cptr[e + 1] - cptr[e];
+                        const FactorRef F = factor_of(f, e);
+                        NCA_REQUIRE(F.m == m && F.r == r, NCA_ERR_DIMENSION,
+                                    "resident factors: the factor of direction index " + std::to_string(e + 1) +
+                                        " differs from the plan's size");
+                        if (r == 0) continue;
+                        const int64_t ro = cptr[e] - 1 - c0;
+                        if (testside)  // U_t(:, ro + k) = X_e(:, k)
+                            jobs->push_back({F.p, L.off[i] + ro * m, (int32_t)m, (int32_t)m, (int32_t)r, 1,
+                                             (int32_t)m, 0});
+                        else  // V_t(ro + k, :) = X_e(:, k)^T
+                            jobs->push_back({F.p, L.off[i] + ro, (int32_t)m, (int32_t)m, (int32_t)r, (int32_t)R, 1,
+                                             0});
+                    }
+                }
+                return;
+            }
             L.blocks.resize((size_t)std::max<int64_t>(L.off.back(), 1));
             parallel_for(nb, [&](int64_t i) {
                 const int64_t t = bnodes[i] - 1;
@@ -2959,10 +3094,25 @@ int32_t nca_dh2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_poin
             });
         };
         Lowered lt, ls;
+        ResidentJobs rj;
+        if (testf || trialf) {
+            NCA_REQUIRE(testf && trialf, NCA_ERR_ARGUMENT, "nca_dh2_create_resident: factors of both sides needed");
+            NCA_REQUIRE(testf->nids == d->ne_test && trialf->nids == d->ne_trial, NCA_ERR_DIMENSION,
+                        "nca_dh2_create_resident: factors of another directional tree");
+            NCA_REQUIRE(testf->ctx == ctx && trialf->ctx == ctx, NCA_ERR_ARGUMENT,
+                        "nca_dh2_create_resident: factors of another context");
+            set_device(ctx);
+        }
         lower(true, d->nn_test, d->test_lo, d->test_hi, d->test_dirptr, d->test_cptr, d->nbnodes_test, d->bt_nodes,
-              d->bt_dirptr, d->bt_diridx, d->bt_off, d->bt_blocks, lt);
+              d->bt_dirptr, d->bt_diridx, d->bt_off, d->bt_blocks, lt, testf, &rj.tb);
         lower(false, d->nn_trial, d->trial_lo, d->trial_hi, d->trial_dirptr, d->trial_cptr, d->nbnodes_trial,
-              d->bs_nodes, d->bs_dirptr, d->bs_diridx, d->bs_off, d->bs_blocks, ls);
+              d->bs_nodes, d->bs_dirptr, d->bs_diridx, d->bs_off, d->bs_blocks, ls, trialf, &rj.sb);
+        if (testf) {
+            transfer_jobs(testf, true, d->nlev_test, d->tt_levptr, d->tt_levidcs, d->tt_chdptr, d->tt_chds, d->tt_off,
+                          d->test_cptr, rj.tt);
+            transfer_jobs(trialf, false, d->nlev_trial, d->st_levptr, d->st_levidcs, d->st_chdptr, d->st_chds,
+                          d->st_off, d->trial_cptr, rj.st);
+        }
         // coupling plan: blocks of every test node, grouped by increasing test direction index
         const int64_t nblk = d->cp_nodeptr[d->nn_test] - d->cp_nodeptr[0];
         NCA_REQUIRE(nblk == coupling->nblocks, NCA_ERR_ARGUMENT, "nca_dh2_create: coupling plan differs from the store");
@@ -3023,7 +3173,26 @@ int32_t nca_dh2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_poin
         lay.ne_trial = d->ne_trial;
         lay.test_dirptr = d->test_dirptr;
         lay.trial_dirptr = d->trial_dirptr;
+        if (testf) lay.jobs = &rj;
         create_core(ctx, kernel, test, trial, &h, lay, coupling, precision, out);
+    }
+}
+}  // namespace nca
+
+extern "C" {
+
+int32_t nca_dh2_create(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test, const nca_points* trial,
+                       const nca_dh2_desc* d, const nca_coupling* coupling, int32_t precision, nca_h2** out) {
+    return guarded([&] { dh2_create_impl(ctx, kernel, test, trial, d, coupling, precision, nullptr, nullptr, out); });
+}
+
+int32_t nca_dh2_create_resident(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* test,
+                                const nca_points* trial, const nca_dh2_desc* d, const nca_coupling* coupling,
+                                int32_t precision, const nca_factors* testf, const nca_factors* trialf,
+                                nca_h2** out) {
+    return guarded([&] {
+        NCA_REQUIRE(testf && trialf, NCA_ERR_ARGUMENT, "nca_dh2_create_resident: invalid argument");
+        dh2_create_impl(ctx, kernel, test, trial, d, coupling, precision, testf, trialf, out);
     });
 }
 

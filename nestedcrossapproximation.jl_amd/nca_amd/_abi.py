@@ -98,6 +98,13 @@ SIGNATURES = {
     "nca_interactions": (I32, [P, I64, P, P, P, P, I64, P, P, P, P, I32, P, P, P, PP]),
     "nca_partition_export": (I32, [P, P, P, P, P]),
     "nca_partition_destroy": (I32, [P]),
+    "nca_factors_create": (I32, [P, I64, PP]),
+    "nca_factors_destroy": (I32, [P]),
+    "nca_compress_level_resident": (I32, [P, P, D, I32, I32, P, P, P, P, P, P]),
+    "nca_factors_size": (I32, [P, P]),
+    "nca_factors_download": (I32, [P, P, P]),
+    "nca_h2_create_resident": (I32, [P, P, P, P, P, P, I32, P, P, PP]),
+    "nca_dh2_create_resident": (I32, [P, P, P, P, P, P, I32, P, P, PP]),
     "nca_tree_twon": (I32, [P, P, I64, D, P, PP]),
     "nca_tree_kmeans": (I32, [P, P, I64, I64, P, PP]),
     "nca_tree_export": (I32, [P, P, P, P, P, P, P, P, P, P]),

@@ -257,6 +257,56 @@ def compress_level(stack, tol, sampled=True, method="aca"):
     return ranks[:nb], rp[:, :nb], cp[:, :nb], xoff, X[:xoff[-1]]
 
 
+class FactorPool:
+    """Skeleton factors kept on the device (nca_factors) for block ids
+    0..nids-1: each compress_level_resident call adds one level."""
+
+    def __init__(self, ctx, nids):
+        self.ctx = ctx
+        self.nids = int(nids)
+        h = ctypes.c_void_p()
+        call(hip_lib().nca_factors_create(ctx.handle, self.nids, ctypes.byref(h)))
+        self.handle = h
+
+    def size(self):
+        n = ctypes.c_int64()
+        call(hip_lib().nca_factors_size(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def download(self):
+        """All factors, the levels in call order (the X of compress_level)."""
+        X = np.zeros(max(self.size(), 1), np.complex128)
+        call(hip_lib().nca_factors_download(self.ctx.handle, self.handle, _p(X)))
+        return X[:self.size()]
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                hip_lib().nca_factors_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def compress_level_resident(stack, tol, pool, ids, sampled=True, method="aca"):
+    """compress_level with X kept in ``pool``: block b becomes the factor of
+    ids[b] (0-based). Returns (ranks, rowpivots, colpivots, xoff) with xoff the
+    offsets X would have in compress_level's output."""
+    nb, mr = stack.nb, stack.maxrank
+    ranks = np.zeros(max(nb, 1), np.int32)
+    rp = np.zeros((mr, max(nb, 1)), np.int64, order="F")
+    cp = np.zeros((mr, max(nb, 1)), np.int64, order="F")
+    indep = np.zeros(max(nb, 1), np.int64)
+    ids1 = np.ascontiguousarray(np.asarray(ids, np.int64) + 1) if nb else np.zeros(1, np.int64)
+    call(hip_lib().nca_compress_level_resident(stack.farmatrix.ctx.handle, stack.handle, float(tol),
+                                               int(bool(sampled)), 1 if method == "iaca" else 0, pool.handle,
+                                               _p(ids1), _p(ranks), _p(rp), _p(cp), _p(indep)))
+    ranks = ranks[:nb]
+    xoff = np.zeros(nb + 1, np.int64)
+    np.cumsum(np.diff(stack.rowoffsets) * ranks, out=xoff[1:])
+    return ranks, rp[:, :nb], cp[:, :nb], xoff
+
+
 # --------------------------------------------------------------------------
 class SideFactors:
     """Result of one side: pivots per node (0-based CSR of point ids; nodes
@@ -296,6 +346,7 @@ class LevelFactors:
         self._npiv = 0
         self._X = [np.zeros(0, np.complex128)]
         self._nx = 0
+        self.pool = None  # a FactorPool when the factors stay on the device
         self.level_transfer_nodes = {}
         self.pivots = _NodeView(self, "pivots")
         self.X = _NodeView(self, "X")
@@ -308,6 +359,8 @@ class LevelFactors:
 
     @property
     def Xflat(self):
+        if self.pool is not None and self._nx and not len(self._X[0]):
+            self._X = [self.pool.download()]  # resident factors: downloaded once, when read
         if len(self._X) > 1:
             self._X = [np.concatenate(self._X)]
         return self._X[0]
@@ -323,8 +376,9 @@ class LevelFactors:
         self._piv.append(np.asarray(pivots, np.int64))
         self._npiv += len(pivots)
         self.xoff[nodes] = self._nx + xoff_level[:k]
-        self._X.append(np.asarray(X_level, np.complex128))
-        self._nx += len(X_level)
+        if X_level is not None:
+            self._X.append(np.asarray(X_level, np.complex128))
+        self._nx += int(xoff_level[k])
 
     def pivot_csr(self):
         ptr = np.zeros(self.nn + 1, np.int64)
@@ -379,9 +433,15 @@ def _level_rows(tree, f, active, leaf):
 def _compresslevel(cfg, storage, active, rowptr, rowidx, out):
     """compresslevel_test!/trial! for a HIP LevelStack (batchedbottomup.jl:145-244,
     compresslevel.jl:242-316): batched ACA + skeleton solve, then the pivots
-    tpivots[t] = tvalues[t][rowpivots] and the factor X of every block."""
-    ranks, rp, cp, xoff, X = compress_level(storage, cfg.tol, sampled=True,
-                                            method="iaca" if isinstance(cfg, IACA) else "aca")
+    tpivots[t] = tvalues[t][rowpivots] and the factor X of every block (kept
+    in out.pool on the device when the side has one)."""
+    method = "iaca" if isinstance(cfg, IACA) else "aca"
+    if out.pool is not None:
+        ranks, rp, cp, xoff = compress_level_resident(storage, cfg.tol, out.pool, active, sampled=True,
+                                                      method=method)
+        X = None
+    else:
+        ranks, rp, cp, xoff, X = compress_level(storage, cfg.tol, sampled=True, method=method)
     ranks = ranks.astype(np.int64)
     m = np.diff(storage.rowoffsets)
     mr = rp.shape[0]
@@ -390,9 +450,11 @@ def _compresslevel(cfg, storage, active, rowptr, rowidx, out):
     out.add_level(active, m, ranks, rowidx[pos], xoff, X)
 
 
-def batched_side(compressor, farmatrix, tree, fd, side, maxrank=50, farpivots=None):
+def batched_side(compressor, farmatrix, tree, fd, side, maxrank=50, farpivots=None, resident=False):
     """testbases (side 0) / trialbases (side 1) of BatchedBottomUp, returning
-    LevelFactors. farpivots: optional Phase-1 result (0-based CSR)."""
+    LevelFactors. farpivots: optional Phase-1 result (0-based CSR). resident:
+    the factors stay on the device (LevelFactors.pool) and are downloaded only
+    when read."""
     selftree = tree.testtree if side == 0 else tree.trialtree
     othertree = tree.trialtree if side == 0 else tree.testtree
     if farpivots is None:
@@ -402,6 +464,8 @@ def batched_side(compressor, farmatrix, tree, fd, side, maxrank=50, farpivots=No
     nfar = np.diff(fptr)
     out = LevelFactors(selftree.nnodes)
     out.farpivots = farpivots
+    if resident:
+        out.pool = FactorPool(farmatrix.ctx, selftree.nnodes)
     leaf = selftree.leafmask()
     for lev in sorted(selftree.levels().tolist(), reverse=True):
         nodes = selftree.levelnodes(lev)
@@ -441,6 +505,12 @@ class TransferStore:
         self.blocks = blocks
 
 
+def _offsets(sizes):
+    off = np.zeros(len(sizes) + 1, np.int64)
+    np.cumsum(sizes, out=off[1:])
+    return off
+
+
 def _gather_blocks(src, src_off, ld, rows, cols, transpose):
     """Pack complex blocks rows[b] x cols[b] (column-major) read from src at
     src_off[b] with leading dimension ld[b], as is or transposed (host lib)."""
@@ -456,15 +526,19 @@ def _gather_blocks(src, src_off, ld, rows, cols, transpose):
     return dst_off, out[:dst_off[-1]]
 
 
-def _stores_flat(f, tree, side):
+def _stores_flat(f, tree, side, blocks=True):
     """stores_from_factors for LevelFactors: the same layout and order as the
-    per-node path below, packed by the host library from the flat factors."""
+    per-node path below, packed by the host library from the flat factors.
+    blocks=False: plans and offsets only (blocks None), for stores that the
+    device packs from resident factors."""
     leaf = tree.leafmask()
-    Xf = np.ascontiguousarray(f.Xflat)
+    Xf = np.ascontiguousarray(f.Xflat) if blocks else None
     # bases: leaves with a block, node order; test X (m x r), trial X^T (r x m)
     bn = np.nonzero(f.assigned & leaf)[0]
     m, r, xo = f.m[bn], f.rank[bn], f.xoff[bn]
-    if side == 0:
+    if not blocks:
+        boff, bl = _offsets(m * r), None
+    elif side == 0:
         boff, bl = _gather_blocks(Xf, xo, m, m, r, False)
     else:
         boff, bl = _gather_blocks(Xf, xo, m, r, m, True)
@@ -485,7 +559,9 @@ def _stores_flat(f, tree, side):
     first = np.repeat(chdptr[:-1], nch)
     oc = csum - rc - np.where(first > 0, csum[np.maximum(first - 1, 0)], 0) if len(C) else np.zeros(0, np.int64)
     rp, mp, xp = f.rank[Pe], f.m[Pe], f.xoff[Pe]
-    if side == 0:   # E = X_p[oc : oc + rc, :] (rc x rp)
+    if not blocks:
+        toff, tl = _offsets(rc * rp), None
+    elif side == 0:   # E = X_p[oc : oc + rc, :] (rc x rp)
         toff, tl = _gather_blocks(Xf, xp + oc, mp, rc, rp, False)
     else:           # E^T (rp x rc)
         toff, tl = _gather_blocks(Xf, xp + oc, mp, rp, rc, True)
@@ -493,12 +569,13 @@ def _stores_flat(f, tree, side):
     return basis, transfer
 
 
-def stores_from_factors(f, tree, side):
+def stores_from_factors(f, tree, side, blocks=True):
     """_build_basis_storage / _build_transfer_storage (nestedbasis.jl:1-4,
     transfermatrices.jl:58-95) from the block factors X: test basis X (m x r),
-    trial basis X^T; test transfer X[childrows, :], trial transfer its transpose."""
+    trial basis X^T; test transfer X[childrows, :], trial transfer its transpose.
+    blocks=False (LevelFactors only): plans and offsets, blocks None."""
     if isinstance(f, LevelFactors):
-        return _stores_flat(f, tree, side)
+        return _stores_flat(f, tree, side, blocks)
     leaf = tree.leafmask()
     bnodes = [t for t in range(f.nn) if f.assigned[t] and leaf[t]]
     bl, boff = [], [0]
@@ -532,15 +609,15 @@ def stores_from_factors(f, tree, side):
     return basis, transfer
 
 
-def testbases(compressor, farmatrix, tree, fardata, maxrank=50, farpivots=None):
-    f = batched_side(compressor, farmatrix, tree, fardata, 0, maxrank, farpivots)
-    b, t = stores_from_factors(f, tree.testtree, 0)
+def testbases(compressor, farmatrix, tree, fardata, maxrank=50, farpivots=None, resident=False):
+    f = batched_side(compressor, farmatrix, tree, fardata, 0, maxrank, farpivots, resident)
+    b, t = stores_from_factors(f, tree.testtree, 0, blocks=not resident)
     return b, t, f
 
 
-def trialbases(compressor, farmatrix, tree, fardata, maxrank=50, farpivots=None):
-    f = batched_side(compressor, farmatrix, tree, fardata, 1, maxrank, farpivots)
-    b, t = stores_from_factors(f, tree.trialtree, 1)
+def trialbases(compressor, farmatrix, tree, fardata, maxrank=50, farpivots=None, resident=False):
+    f = batched_side(compressor, farmatrix, tree, fardata, 1, maxrank, farpivots, resident)
+    b, t = stores_from_factors(f, tree.trialtree, 1, blocks=not resident)
     return b, t, f
 
 

@@ -31,7 +31,7 @@ def h2desc(tree, partition, testfardata, testf, trialf, testbasis, testtransfer,
     """The nca_h2_desc arrays, 1-based as the Julia structs hold them."""
     tt, st = tree.testtree, tree.trialtree
     i64 = lambda a: np.ascontiguousarray(a, np.int64)
-    c128 = lambda a: np.ascontiguousarray(a, np.complex128) if len(a) else np.zeros(1, np.complex128)
+    c128 = lambda a: np.ascontiguousarray(a, np.complex128) if a is not None and len(a) else np.zeros(1, np.complex128)
     d = dict(
         M=tt.npoints, N=st.npoints, nn_test=tt.nnodes, nn_trial=st.nnodes,
         test_perm=i64(tt.perm + 1), test_lo=i64(tt.lo + 1), test_hi=i64(tt.hi),
@@ -80,11 +80,14 @@ class PetrovGalerkinNCACore:
 
     def __init__(self, kernel, testpoints, trialpoints, tree, testcompressor, trialcompressor, eta=1.0,
                  maxrank=50, precision=_abi.NCA_PREC_FP64, ctx=None, verbose=False, partition=None,
-                 reuse_symmetric=True):
+                 reuse_symmetric=True, resident=True):
         self.ctx = ctx or default_context()
         self.kernel = kernel
         self.tree = tree
         self.maxrank = maxrank
+        # resident: the factors stay on the device and the stores are packed
+        # there (nca_h2_create_resident); the host blocks are made on request
+        self.resident = bool(resident)
         timings = {}
         t0 = time.perf_counter()
         if partition is None:
@@ -107,8 +110,9 @@ class PetrovGalerkinNCACore:
                                                             maxrank, self.farmatrix.test)
         timings["phase1"] = time.perf_counter() - t0
         t0 = time.perf_counter()
+        res = self.resident
         self.testbasis, self.testtransfer, self.testfactors = _cmp.testbases(
-            testcompressor, self.farmatrix, tree, self.testfardata, maxrank, fp_test)
+            testcompressor, self.farmatrix, tree, self.testfardata, maxrank, fp_test, res)
         # One tree, one point set, one partition and the same compressor: the
         # trial blocks transpose(A[cols, rows]) hold the test blocks' numbers
         # (the kernel is symmetric), so the trial pass would repeat the test
@@ -117,30 +121,63 @@ class PetrovGalerkinNCACore:
                               and vars(testcompressor.compressor) == vars(trialcompressor.compressor))
         if self.symmetric:
             self.trialfactors = self.testfactors
-            self.trialbasis, self.trialtransfer = _cmp.stores_from_factors(self.testfactors, tree.trialtree, 1)
+            self.trialbasis, self.trialtransfer = _cmp.stores_from_factors(self.testfactors, tree.trialtree, 1,
+                                                                           blocks=not res)
         else:
             self.trialbasis, self.trialtransfer, self.trialfactors = _cmp.trialbases(
-                trialcompressor, self.farmatrix, tree, self.trialfardata, maxrank, fp_trial)
+                trialcompressor, self.farmatrix, tree, self.trialfardata, maxrank, fp_trial, res)
         self.testpivots = self.testfactors.pivot_csr()
         self.trialpivots = self.trialfactors.pivot_csr()
         self.couplingstore = _cmp.assemble_couplingstore(self.farmatrix, self.testpivots, self.trialpivots,
                                                          self.testfardata)
         timings["assembly"] = time.perf_counter() - t0
-        self.desc = h2desc(tree, self.partition, self.testfardata, self.testfactors, self.trialfactors,
-                           self.testbasis, self.testtransfer, self.trialbasis, self.trialtransfer)
+        self._desc = h2desc(tree, self.partition, self.testfardata, self.testfactors, self.trialfactors,
+                            self.testbasis, self.testtransfer, self.trialbasis, self.trialtransfer)
+        self._desc_blocks = not res
         t0 = time.perf_counter()
         h = ctypes.c_void_p()
         kern = kernel.abi()
-        abidesc = _abi_desc(self.desc)
-        call(hip_lib().nca_h2_create(self.ctx.handle, ctypes.byref(kern), self.farmatrix.test.handle,
-                                     self.farmatrix.trial.handle, ctypes.byref(abidesc), self.couplingstore.handle,
-                                     int(precision), ctypes.byref(h)))
+        abidesc = _abi_desc(self._desc)
+        if res:
+            call(hip_lib().nca_h2_create_resident(
+                self.ctx.handle, ctypes.byref(kern), self.farmatrix.test.handle, self.farmatrix.trial.handle,
+                ctypes.byref(abidesc), self.couplingstore.handle, int(precision), self.testfactors.pool.handle,
+                self.trialfactors.pool.handle, ctypes.byref(h)))
+        else:
+            call(hip_lib().nca_h2_create(self.ctx.handle, ctypes.byref(kern), self.farmatrix.test.handle,
+                                         self.farmatrix.trial.handle, ctypes.byref(abidesc),
+                                         self.couplingstore.handle, int(precision), ctypes.byref(h)))
         self.handle = h
         timings["h2_create"] = time.perf_counter() - t0
         self.timings = timings
         self.dim = (tree.testtree.npoints, tree.trialtree.npoints)
         if verbose:
             print("assembly timings:", {k: round(v, 4) for k, v in timings.items()}, flush=True)
+
+    def host_stores(self):
+        """The basis and transfer stores with their blocks on the host (packed
+        from the downloaded factors when the factors are resident)."""
+        if not getattr(self, "_desc_blocks", True):
+            t, s = self.tree.testtree, self.tree.trialtree
+            self.testbasis, self.testtransfer = _cmp.stores_from_factors(self.testfactors, t, 0)
+            self.trialbasis, self.trialtransfer = _cmp.stores_from_factors(self.trialfactors, s, 1)
+            self._desc = h2desc(self.tree, self.partition, self.testfardata, self.testfactors, self.trialfactors,
+                                self.testbasis, self.testtransfer, self.trialbasis, self.trialtransfer)
+            self._desc_blocks = True
+        return self.testbasis, self.testtransfer, self.trialbasis, self.trialtransfer
+
+    @property
+    def desc(self):
+        """The nca_h2_desc arrays with the stores' blocks (made on request for
+        resident factors)."""
+        if not getattr(self, "_desc_blocks", True):
+            self.host_stores()
+        return self._desc
+
+    @desc.setter
+    def desc(self, d):
+        self._desc = d
+        self._desc_blocks = True
 
     def stats(self):
         s = _abi.H2Stats()

@@ -191,10 +191,11 @@ class DirTransferStore:
         self.offsets, self.blocks = offsets, blocks
 
 
-def dir_stores(f, blocks, side):
+def dir_stores(f, blocks, side, host_blocks=True):
     """_build_dirbasis_storage / _build_dirtransfer_storage (nestedbasis.jl,
-    transfermatrices.jl:40-99) from the factors of the direction indices."""
-    b, t = _cmp.stores_from_factors(f, blocks, side)
+    transfermatrices.jl:40-99) from the factors of the direction indices
+    (host_blocks=False: plans and offsets only, for resident factors)."""
+    b, t = _cmp.stores_from_factors(f, blocks, side, blocks=host_blocks)
     node = blocks.node
     bn = node[b.nodes]
     nodes, counts = np.unique(bn, return_counts=True) if len(bn) else (np.zeros(0, np.int64), np.zeros(0, np.int64))
@@ -246,7 +247,7 @@ def dh2desc(tree, partition, testdata, trialdata, testf, trialf, tb, tt, sb, st,
     """The nca_dh2_desc arrays, 1-based."""
     ta, sa = tree.testtree, tree.trialtree
     i64 = lambda a: np.ascontiguousarray(a, np.int64)
-    c128 = lambda a: np.ascontiguousarray(a, np.complex128) if len(a) else np.zeros(1, np.complex128)
+    c128 = lambda a: np.ascontiguousarray(a, np.complex128) if a is not None and len(a) else np.zeros(1, np.complex128)
     nodeptr, tidcs, sidcs = cplan
     d = dict(
         M=ta.npoints, N=sa.npoints, nn_test=ta.nnodes, nn_trial=sa.nnodes,
@@ -292,12 +293,13 @@ class DirectionalNCACore(_nca.PetrovGalerkinNCACore):
 
     def __init__(self, kernel, testpoints, trialpoints, tree, testcompressor, trialcompressor, isnear,
                  maxrank=50, precision=_abi.NCA_PREC_FP64, ctx=None, verbose=False, partition=None,
-                 reuse_symmetric=True):
+                 reuse_symmetric=True, resident=True):
         self.ctx = ctx or default_context()
         self.kernel = kernel
         self.tree = tree
         self.maxrank = maxrank
         self.isnear = isnear
+        self.resident = res = bool(resident)
         timings = {}
         t0 = time.perf_counter()
         if partition is None:
@@ -330,13 +332,14 @@ class DirectionalNCACore(_nca.PetrovGalerkinNCACore):
         timings["phase1"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         bt = _trees.BlockTree(self.testblocks, self.trialblocks)
-        self.testfactors = _cmp.batched_side(testcompressor, self.farmatrix, bt, None, 0, maxrank, fp_test)
+        self.testfactors = _cmp.batched_side(testcompressor, self.farmatrix, bt, None, 0, maxrank, fp_test, res)
         if self.symmetric:
             self.trialfactors = self.testfactors
         else:
-            self.trialfactors = _cmp.batched_side(trialcompressor, self.farmatrix, bt, None, 1, maxrank, fp_trial)
-        self.testbasis, self.testtransfer = dir_stores(self.testfactors, self.testblocks, 0)
-        self.trialbasis, self.trialtransfer = dir_stores(self.trialfactors, self.trialblocks, 1)
+            self.trialfactors = _cmp.batched_side(trialcompressor, self.farmatrix, bt, None, 1, maxrank, fp_trial,
+                                                  res)
+        self.testbasis, self.testtransfer = dir_stores(self.testfactors, self.testblocks, 0, not res)
+        self.trialbasis, self.trialtransfer = dir_stores(self.trialfactors, self.trialblocks, 1, not res)
         self.testpivots = self.testfactors.pivot_csr()
         self.trialpivots = self.trialfactors.pivot_csr()
         self.couplingplan = coupling_plan(self.testdata, self.trialdata)
@@ -344,19 +347,38 @@ class DirectionalNCACore(_nca.PetrovGalerkinNCACore):
             self.farmatrix, self.testpivots, self.trialpivots,
             _trees.FarData(self.testdata.diridxtofarptr, self.couplingplan[2]))
         timings["assembly"] = time.perf_counter() - t0
-        self.desc = dh2desc(tree, self.partition, self.testdata, self.trialdata, self.testfactors,
-                            self.trialfactors, self.testbasis, self.testtransfer, self.trialbasis,
-                            self.trialtransfer, self.couplingplan)
+        self._desc = dh2desc(tree, self.partition, self.testdata, self.trialdata, self.testfactors,
+                             self.trialfactors, self.testbasis, self.testtransfer, self.trialbasis,
+                             self.trialtransfer, self.couplingplan)
+        self._desc_blocks = not res
         t0 = time.perf_counter()
         h = ctypes.c_void_p()
         kern = kernel.abi()
-        abidesc = _abi_dh2desc(self.desc)
-        call(hip_lib().nca_dh2_create(self.ctx.handle, ctypes.byref(kern), self.farmatrix.test.handle,
-                                      self.farmatrix.trial.handle, ctypes.byref(abidesc), self.couplingstore.handle,
-                                      int(precision), ctypes.byref(h)))
+        abidesc = _abi_dh2desc(self._desc)
+        if res:
+            call(hip_lib().nca_dh2_create_resident(
+                self.ctx.handle, ctypes.byref(kern), self.farmatrix.test.handle, self.farmatrix.trial.handle,
+                ctypes.byref(abidesc), self.couplingstore.handle, int(precision), self.testfactors.pool.handle,
+                self.trialfactors.pool.handle, ctypes.byref(h)))
+        else:
+            call(hip_lib().nca_dh2_create(self.ctx.handle, ctypes.byref(kern), self.farmatrix.test.handle,
+                                          self.farmatrix.trial.handle, ctypes.byref(abidesc),
+                                          self.couplingstore.handle, int(precision), ctypes.byref(h)))
         self.handle = h
         timings["h2_create"] = time.perf_counter() - t0
         self.timings = timings
         self.dim = (tree.testtree.npoints, tree.trialtree.npoints)
         if verbose:
             print("directional assembly timings:", {k: round(v, 4) for k, v in timings.items()}, flush=True)
+
+    def host_stores(self):
+        """The directional stores with their blocks on the host (packed from the
+        downloaded factors when the factors are resident)."""
+        if not getattr(self, "_desc_blocks", True):
+            self.testbasis, self.testtransfer = dir_stores(self.testfactors, self.testblocks, 0)
+            self.trialbasis, self.trialtransfer = dir_stores(self.trialfactors, self.trialblocks, 1)
+            self._desc = dh2desc(self.tree, self.partition, self.testdata, self.trialdata, self.testfactors,
+                                 self.trialfactors, self.testbasis, self.testtransfer, self.trialbasis,
+                                 self.trialtransfer, self.couplingplan)
+            self._desc_blocks = True
+        return self.testbasis, self.testtransfer, self.trialbasis, self.trialtransfer

@@ -174,6 +174,8 @@ def test_symmetric_reuse_equals_trial_pass():
                                            cmp, cmp, reuse_symmetric=r) for r in (True, False)]
     assert cores[0].symmetric and not cores[1].symmetric
     a, b = cores
+    a.host_stores()
+    b.host_stores()
     for x, y in zip(a.trialfactors.pivot_csr(), b.trialfactors.pivot_csr()):
         assert np.array_equal(x, y)
     for name in ("nodes", "offsets", "blocks"):
