@@ -161,8 +161,38 @@ int32_t nca_factors_destroy(nca_factors* f);
 int32_t nca_compress_level_resident(nca_ctx* ctx, const nca_stack* st, double tol, int32_t sampled, int32_t method,
                                     nca_factors* f, const int64_t* ids, int32_t* ranks, int64_t* rowpivots,
                                     int64_t* colpivots, int64_t* independent);
+/* The whole batched bottom-up pass of one side (the level loop of
+ * testbases/trialbases, src/compressors/batchedbottomup.jl:273-332, with
+ * assemble_level + compresslevel_test!/trial! per level,
+ * batchedbottomup.jl:80-244) on the device: one launch per level, one
+ * workgroup per block evaluates the block's entries, runs the ACA (method 0,
+ * sampled stop) or IACA (method 1), solves the skeleton and keeps X in f; the
+ * rows of an inner block are its children's pivots in child order, built on
+ * the device from the level below, so nothing returns to the host between
+ * levels. Same pivots, ranks and factors, bit for bit, as
+ * nca_level_assemble + nca_compress_level[_iaca] level by level.
+ * Tree over block ids 1..nids (nodes, or direction indices): childptr (nids+1)
+ * / children CSR, lo / hi (values perm[lo:hi] of a leaf, 1-based inclusive),
+ * perm (1-based ids of self's points); levptr (nlevels+1) / levids: the blocks
+ * of each level, deepest level first; fpptr (nids+1) / fpidx: the far pivots
+ * (columns, 1-based ids of other's points) of every block. transposed as
+ * nca_level_assemble (the trial side). Outputs per block id: ranks, mrows
+ * (rows of its block), independent (-1 regular; else the singularity rule's
+ * count and the call returns NCA_ERR_SINGULAR); pivots: the row pivots (1-based
+ * point ids) of the blocks in levids order, rank of them each (pivcap >= their
+ * total; sum over blocks of min(maxrank, columns) always suffices). Limits (NCA_ERR_LIMIT):
+ * maxrank <= 96, block rows <= 2048 (bounded up front from the children's
+ * possible ranks). */
+int32_t nca_bottomup_side(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* self,
+                          const nca_points* other, int32_t transposed, int64_t nids, const int64_t* childptr,
+                          const int64_t* children, const int64_t* lo, const int64_t* hi, const int64_t* perm,
+                          int64_t nlevels, const int64_t* levptr, const int64_t* levids, const int64_t* fpptr,
+                          const int64_t* fpidx, int64_t maxrank, double tol, int32_t sampled, int32_t method,
+                          nca_factors* f, int32_t* ranks, int64_t* mrows, int64_t pivcap, int64_t* pivots,
+                          int64_t* independent);
 /* Total complex entries, and all factors downloaded: the X of every
- * nca_compress_level_resident call in call order (the X of nca_compress_level). */
+ * nca_compress_level_resident or nca_bottomup_side call in call order (each
+ * call's blocks in stack order, or level by level: the X of nca_compress_level). */
 int32_t nca_factors_size(const nca_factors* f, int64_t* nentries);
 int32_t nca_factors_download(nca_ctx* ctx, const nca_factors* f, double* X);
 

@@ -133,7 +133,10 @@ struct nca_factors {
     std::vector<int32_t> chunk, m, rank;  // per id; chunk -1: no factor
     std::vector<int64_t> off;             // per id: complex offset in its chunk
     std::vector<nca::DevBuf<double2>> chunks;
-    std::vector<int64_t> chunk_size;      // complex entries per chunk
+    std::vector<int64_t> chunk_size;      // complex entries of the factors in a chunk
+    // chunks that are not packed (the factors at bounded offsets): their ids
+    // in download order; empty for packed chunks
+    std::vector<std::vector<int32_t>> chunk_ids;
 };
 
 namespace nca {

@@ -88,12 +88,172 @@ __device__ __forceinline__ void reduce_sum_argmax(ArgRed& red, double p, double 
     imax = red.idx[0];
 }
 
-// Batched fully pivoted ACA with the sampled stop; one workgroup per block.
+// Fully pivoted ACA with the sampled stop on one block, one workgroup; the
+// block's entries are src(a, c) (read once, into the residual R).
 //   r crosses accepted, residual R_r, s = block columns;
-//   stop when r == min(maxrank, m, n) or
+//   stop when r == kmax = min(maxrank, m, n) or
 //        ||R_r||_F^2 * (sampled ? s / (s - r) : 1) <= tol^2 ||A||_F^2;
 //   pivot = first argmax |R_r|^2 (column-major), R -= R(:,j) (R(i,:) / R(i,j)),
-//   row i and column j set to zero.
+//   row i and column j set to zero. piv(k, i, j) records cross k (thread 0);
+//   returns the rank.
+template <class Src, class Piv>
+__device__ __forceinline__ int aca_block(Src src, double2* R, int64_t ldr, double2* u, double2* v, int m, int n,
+                                         int kmax, double tol, int sampled, ArgRed& red, Piv piv) {
+    const int t = threadIdx.x;
+    const int mn = m * n;
+    const int qstep = kWG / m, rstep = kWG % m;
+    double p = 0.0, bv = -1.0;
+    int bi = 0;
+    {
+        int a = t % m, c = t / m;
+        for (int e = t; e < mn; e += kWG) {
+            const double2 val = src(a, c);
+            R[a + c * ldr] = val;
+            const double w = cabs2(val);
+            p += w;
+            if (w > bv) { bv = w; bi = e; }
+            a += rstep;
+            c += qstep;
+            if (a >= m) { a -= m; ++c; }
+        }
+    }
+    double nrm2, bmax;
+    int bidx;
+    reduce_sum_argmax(red, p, bv, bi, nrm2, bmax, bidx);
+    const double nrmA2 = nrm2;
+    const double tol2 = tol * tol;
+    int r = 0;
+    while (r < kmax) {
+        const double scale = sampled ? (double)n / (double)(n - r) : 1.0;
+        if (nrm2 * scale <= tol2 * nrmA2) break;
+        if (!(bmax > 0.0)) break;
+        const int i = bidx % m, j = bidx / m;
+        const double2 pv = R[i + j * ldr];
+        const double d = cabs2(pv);
+        const double2 inv = make_double2(pv.x / d, -pv.y / d);
+        for (int a = t; a < m; a += kWG) u[a] = R[a + j * ldr];
+        for (int c = t; c < n; c += kWG) v[c] = cmul(R[i + c * ldr], inv);
+        if (t == 0) piv(r, i, j);
+        ++r;
+        __syncthreads();
+        p = 0.0;
+        bv = -1.0;
+        bi = 0;
+        int a = t % m, c = t / m;
+        for (int e = t; e < mn; e += kWG) {
+            double2 val;
+            if (a == i || c == j) {
+                val = make_double2(0.0, 0.0);
+            } else {
+                val = csub(R[a + c * ldr], cmul(u[a], v[c]));
+            }
+            R[a + c * ldr] = val;
+            const double w = cabs2(val);
+            p += w;
+            if (w > bv) { bv = w; bi = e; }
+            a += rstep;
+            c += qstep;
+            if (a >= m) { a -= m; ++c; }
+        }
+        reduce_sum_argmax(red, p, bv, bi, nrm2, bmax, bidx);
+    }
+    return r;
+}
+
+// Incomplete ACA (IACA) with geometric column pivoting on one block, one
+// workgroup (the method of the incomplete adaptive cross approximation with
+// tree-mimicry pivoting: the column pivots come from the geometry, the row
+// pivots from the residual columns by maximum value). The block's columns are
+// the candidates in their geometric order (Phase 1), so step r takes column
+// j = r; its residual R_r(:, j) gives the row pivot i = first argmax
+// |R_r(a, j)|^2. Stop, without counting the candidate, when r reaches
+// kmax = min(maxrank, m, n), when the candidate's residual column is zero, or when
+//   r ||R_r(:, j)||^2 <= tol^2 sum_{l < r} ||A(:, l)||^2     (r > 0),
+// i.e. the candidate column is represented to tol relative to the mean
+// accepted column. The residual update is the ACA's: R -= R(:, j) (R(i, :) /
+// R(i, j)) with row i and column j set to zero; the column sums use the ACA's
+// fixed order (slot e mod 256 over the column-major entries, halving tree).
+// src(a, c): the block's entries (read for the residual and for the original
+// column norms).
+template <class Src, class Piv>
+__device__ __forceinline__ int iaca_block(Src src, double2* R, int64_t ldr, double2* u, double2* v, int m, int n,
+                                          int kmax, double tol, ArgRed& red, Piv piv) {
+    const int t = threadIdx.x;
+    const int mn = m * n;
+    const int qstep = kWG / m, rstep = kWG % m;
+    // the copy, and the sums of column 0 (its residual is the original)
+    double p = 0.0, bv = -1.0;
+    int bi = 0;
+    {
+        int a = t % m, c = t / m;
+        for (int e = t; e < mn; e += kWG) {
+            const double2 val = src(a, c);
+            R[a + c * ldr] = val;
+            if (c == 0) {
+                const double w = cabs2(val);
+                p += w;
+                if (w > bv) { bv = w; bi = a; }
+            }
+            a += rstep;
+            c += qstep;
+            if (a >= m) { a -= m; ++c; }
+        }
+    }
+    double cn2, bmax;
+    int bidx;
+    reduce_sum_argmax(red, p, bv, bi, cn2, bmax, bidx);
+    double orig2 = cn2;  // ||A(:, j)||^2 of the candidate
+    double acc = 0.0;    // sum of ||A(:, l)||^2 over the accepted columns
+    const double tol2 = tol * tol;
+    int r = 0;
+    while (r < kmax) {
+        if (r > 0 && cn2 * (double)r <= tol2 * acc) break;
+        if (!(bmax > 0.0)) break;
+        const int i = bidx, j = r;
+        const double2 pv = R[i + j * ldr];
+        const double d = cabs2(pv);
+        const double2 inv = make_double2(pv.x / d, -pv.y / d);
+        __syncthreads();
+        for (int a = t; a < m; a += kWG) u[a] = R[a + j * ldr];
+        for (int c = t; c < n; c += kWG) v[c] = cmul(R[i + c * ldr], inv);
+        if (t == 0) piv(r, i, j);
+        acc += orig2;
+        ++r;
+        __syncthreads();
+        p = 0.0;
+        double q = 0.0;
+        bv = -1.0;
+        bi = 0;
+        int a = t % m, c = t / m;
+        for (int e = t; e < mn; e += kWG) {
+            double2 val;
+            if (a == i || c == j) {
+                val = make_double2(0.0, 0.0);
+            } else {
+                val = csub(R[a + c * ldr], cmul(u[a], v[c]));
+            }
+            R[a + c * ldr] = val;
+            if (c == r) {
+                const double w = cabs2(val);
+                p += w;
+                if (w > bv) { bv = w; bi = a; }
+                q += cabs2(src(a, c));
+            }
+            a += rstep;
+            c += qstep;
+            if (a >= m) { a -= m; ++c; }
+        }
+        reduce_sum_argmax(red, p, bv, bi, cn2, bmax, bidx);
+        __syncthreads();
+        double unusedv;
+        int unusedi;
+        reduce_sum_argmax(red, q, -1.0, 0, orig2, unusedv, unusedi);
+    }
+    return r;
+}
+
+// Batched fully pivoted ACA (aca_block), one workgroup per block of the stack;
+// the residual in LDS (kLds) or in the global working copy work.
 template <bool kLds>
 __global__ void __launch_bounds__(kWG)
     k_aca(const double2* __restrict__ buf, double2* __restrict__ work, int64_t ld,
@@ -118,96 +278,20 @@ __global__ void __launch_bounds__(kWG)
         if (t == 0) ranks[b] = 0;
         return;
     }
-    double2* R;
-    int64_t ldr;
-    double2 *u, *v;
-    if (kLds) {
-        R = dyn;
-        ldr = m;
-        u = dyn + kLdsCap;
-        v = u + kLdsMaxM;
-    } else {
-        R = work + r0;
-        ldr = ld;
-        u = dyn;
-        v = dyn + kGlobMaxM;
-    }
-    const int mn = m * n;
-    const int qstep = kWG / m, rstep = kWG % m;
-    double p = 0.0, bv = -1.0;
-    int bi = 0;
-    {
-        int a = t % m, c = t / m;
-        for (int e = t; e < mn; e += kWG) {
-            const double2 val = buf[r0 + a + (int64_t)c * ld];
-            R[a + c * ldr] = val;
-            const double w = cabs2(val);
-            p += w;
-            if (w > bv) { bv = w; bi = e; }
-            a += rstep;
-            c += qstep;
-            if (a >= m) { a -= m; ++c; }
-        }
-    }
-    double nrm2, bmax;
-    int bidx;
-    reduce_sum_argmax(red, p, bv, bi, nrm2, bmax, bidx);
-    const double nrmA2 = nrm2;
-    const double tol2 = tol * tol;
-    int r = 0;
-    while (r < kmax) {
-        const double scale = sampled ? (double)n / (double)(n - r) : 1.0;
-        if (nrm2 * scale <= tol2 * nrmA2) break;
-        if (!(bmax > 0.0)) break;
-        const int i = bidx % m, j = bidx / m;
-        const double2 piv = R[i + j * ldr];
-        const double d = cabs2(piv);
-        const double2 inv = make_double2(piv.x / d, -piv.y / d);
-        for (int a = t; a < m; a += kWG) u[a] = R[a + j * ldr];
-        for (int c = t; c < n; c += kWG) v[c] = cmul(R[i + c * ldr], inv);
-        if (t == 0) {
-            rowpiv[r + b * maxrank] = i + 1;
-            colpiv[r + b * maxrank] = j + 1;
-        }
-        ++r;
-        __syncthreads();
-        p = 0.0;
-        bv = -1.0;
-        bi = 0;
-        int a = t % m, c = t / m;
-        for (int e = t; e < mn; e += kWG) {
-            double2 val;
-            if (a == i || c == j) {
-                val = make_double2(0.0, 0.0);
-            } else {
-                val = csub(R[a + c * ldr], cmul(u[a], v[c]));
-            }
-            R[a + c * ldr] = val;
-            const double w = cabs2(val);
-            p += w;
-            if (w > bv) { bv = w; bi = e; }
-            a += rstep;
-            c += qstep;
-            if (a >= m) { a -= m; ++c; }
-        }
-        reduce_sum_argmax(red, p, bv, bi, nrm2, bmax, bidx);
-    }
+    double2* R = kLds ? dyn : work + r0;
+    const int64_t ldr = kLds ? m : ld;
+    double2* u = kLds ? dyn + kLdsCap : dyn;
+    double2* v = kLds ? u + kLdsMaxM : dyn + kGlobMaxM;
+    auto src = [&](int a, int c) { return buf[r0 + a + (int64_t)c * ld]; };
+    auto piv = [&](int k, int i, int j) {
+        rowpiv[k + b * maxrank] = i + 1;
+        colpiv[k + b * maxrank] = j + 1;
+    };
+    const int r = aca_block(src, R, ldr, u, v, m, n, kmax, tol, sampled, red, piv);
     if (t == 0) ranks[b] = r;
 }
 
-// Batched incomplete ACA (IACA) with geometric column pivoting, one workgroup
-// per block (the method of the incomplete adaptive cross approximation with
-// tree-mimicry pivoting: the column pivots come from the geometry, the row
-// pivots from the residual columns by maximum value). The block's columns are
-// the candidates in their geometric order (Phase 1), so step r takes column
-// j = r; its residual R_r(:, j) gives the row pivot i = first argmax
-// |R_r(a, j)|^2. Stop, without counting the candidate, when r reaches
-// min(maxrank, m, n), when the candidate's residual column is zero, or when
-//   r ||R_r(:, j)||^2 <= tol^2 sum_{l < r} ||A(:, l)||^2     (r > 0),
-// i.e. the candidate column is represented to tol relative to the mean
-// accepted column. The residual update is k_aca's: R -= R(:, j) (R(i, :) /
-// R(i, j)) with row i and column j set to zero; the column sums use k_aca's
-// fixed order (slot e mod 256 over the column-major entries, halving tree).
+// Batched IACA (iaca_block), one workgroup per block of the stack.
 template <bool kLds>
 __global__ void __launch_bounds__(kWG)
     k_iaca(const double2* __restrict__ buf, double2* __restrict__ work, int64_t ld,
@@ -232,124 +316,36 @@ __global__ void __launch_bounds__(kWG)
         if (t == 0) ranks[b] = 0;
         return;
     }
-    double2* R;
-    int64_t ldr;
-    double2 *u, *v;
-    if (kLds) {
-        R = dyn;
-        ldr = m;
-        u = dyn + kLdsCap;
-        v = u + kLdsMaxM;
-    } else {
-        R = work + r0;
-        ldr = ld;
-        u = dyn;
-        v = dyn + kGlobMaxM;
-    }
-    const int mn = m * n;
-    const int qstep = kWG / m, rstep = kWG % m;
-    // the copy, and the sums of column 0 (its residual is the original)
-    double p = 0.0, bv = -1.0;
-    int bi = 0;
-    {
-        int a = t % m, c = t / m;
-        for (int e = t; e < mn; e += kWG) {
-            const double2 val = buf[r0 + a + (int64_t)c * ld];
-            R[a + c * ldr] = val;
-            if (c == 0) {
-                const double w = cabs2(val);
-                p += w;
-                if (w > bv) { bv = w; bi = a; }
-            }
-            a += rstep;
-            c += qstep;
-            if (a >= m) { a -= m; ++c; }
-        }
-    }
-    double cn2, bmax;
-    int bidx;
-    reduce_sum_argmax(red, p, bv, bi, cn2, bmax, bidx);
-    double orig2 = cn2;  // ||A(:, j)||^2 of the candidate
-    double acc = 0.0;    // sum of ||A(:, l)||^2 over the accepted columns
-    const double tol2 = tol * tol;
-    int r = 0;
-    while (r < kmax) {
-        if (r > 0 && cn2 * (double)r <= tol2 * acc) break;
-        if (!(bmax > 0.0)) break;
-        const int i = bidx, j = r;
-        const double2 piv = R[i + j * ldr];
-        const double d = cabs2(piv);
-        const double2 inv = make_double2(piv.x / d, -piv.y / d);
-        __syncthreads();
-        for (int a = t; a < m; a += kWG) u[a] = R[a + j * ldr];
-        for (int c = t; c < n; c += kWG) v[c] = cmul(R[i + c * ldr], inv);
-        if (t == 0) {
-            rowpiv[r + b * maxrank] = i + 1;
-            colpiv[r + b * maxrank] = j + 1;
-        }
-        acc += orig2;
-        ++r;
-        __syncthreads();
-        p = 0.0;
-        double q = 0.0;
-        bv = -1.0;
-        bi = 0;
-        int a = t % m, c = t / m;
-        for (int e = t; e < mn; e += kWG) {
-            double2 val;
-            if (a == i || c == j) {
-                val = make_double2(0.0, 0.0);
-            } else {
-                val = csub(R[a + c * ldr], cmul(u[a], v[c]));
-            }
-            R[a + c * ldr] = val;
-            if (c == r) {
-                const double w = cabs2(val);
-                p += w;
-                if (w > bv) { bv = w; bi = a; }
-                q += cabs2(buf[r0 + a + (int64_t)c * ld]);
-            }
-            a += rstep;
-            c += qstep;
-            if (a >= m) { a -= m; ++c; }
-        }
-        reduce_sum_argmax(red, p, bv, bi, cn2, bmax, bidx);
-        __syncthreads();
-        double unusedv;
-        int unusedi;
-        reduce_sum_argmax(red, q, -1.0, 0, orig2, unusedv, unusedi);
-    }
+    double2* R = kLds ? dyn : work + r0;
+    const int64_t ldr = kLds ? m : ld;
+    double2* u = kLds ? dyn + kLdsCap : dyn;
+    double2* v = kLds ? u + kLdsMaxM : dyn + kGlobMaxM;
+    auto src = [&](int a, int c) { return buf[r0 + a + (int64_t)c * ld]; };
+    auto piv = [&](int k, int i, int j) {
+        rowpiv[k + b * maxrank] = i + 1;
+        colpiv[k + b * maxrank] = j + 1;
+    };
+    const int r = iaca_block(src, R, ldr, u, v, m, n, kmax, tol, red, piv);
     if (t == 0) ranks[b] = r;
 }
 
 // ---------------------------------------------------------------------------
-// Skeleton solve, one workgroup per block: LU with partial pivoting (pivot =
-// first max |re|+|im|, LAPACK getrf form) of P = A[rowpiv, colpiv] in LDS, then
-// per row a of L = A[:, colpiv] (one thread per row): w U = l, y L = w,
-// interchanges undone last to first; X = L P^{-1} written m x r column-major.
-__global__ void __launch_bounds__(kWG)
-    k_skeleton(const double2* __restrict__ buf, int64_t ld, const int64_t* __restrict__ rowoff,
-               const int32_t* __restrict__ ranks, const int64_t* __restrict__ rowpiv,
-               const int64_t* __restrict__ colpiv, int64_t maxrank, const int64_t* __restrict__ xoff,
-               double2* __restrict__ X, int64_t* __restrict__ indep) {
-    extern __shared__ double2 A[];
-    __shared__ int ipiv[kSolveMaxRank];
-    __shared__ int cpiv[kSolveMaxRank];
-    __shared__ ArgRed red;
+// Skeleton solve on one block, one workgroup: LU with partial pivoting (pivot
+// = first max |re|+|im|, LAPACK getrf form) of P = A[rowpiv, colpiv] in LDS
+// (A: r x r), then per row a of L = A[:, colpiv] (one thread per row): w U = l,
+// y L = w, interchanges undone last to first; X = L P^{-1} written m x r
+// column-major at Xb. src(a, c): block entries; rp(k) / cp(k): 0-based pivots.
+// Returns (thread 0) the independent-pivot count of the singularity rule, -1
+// when regular.
+template <class Src, class RP, class CP>
+__device__ __forceinline__ int64_t skeleton_block(Src src, RP rp, CP cp, int m, int r, double2* A, int* ipiv,
+                                                  int* cpiv, ArgRed& red, double2* __restrict__ Xb) {
     const int t = threadIdx.x;
-    const int b = blockIdx.x;
-    const int r = ranks[b];
-    if (r == 0) {
-        if (t == 0) indep[b] = -1;
-        return;
-    }
-    const int64_t r0 = rowoff[b];
-    const int m = (int)(rowoff[b + 1] - r0);
-    for (int c = t; c < r; c += kWG) cpiv[c] = (int)colpiv[c + b * maxrank] - 1;
+    for (int c = t; c < r; c += kWG) cpiv[c] = cp(c);
     __syncthreads();
     for (int e = t; e < r * r; e += kWG) {
         const int kk = e % r, c = e / r;
-        A[e] = buf[r0 + (rowpiv[kk + b * maxrank] - 1) + (int64_t)cpiv[c] * ld];
+        A[e] = src(rp(kk), cpiv[c]);
     }
     __syncthreads();
     int info = 0;
@@ -384,6 +380,7 @@ __global__ void __launch_bounds__(kWG)
         }
         __syncthreads();
     }
+    int64_t result = -1;
     if (t == 0) {
         double d2max = 0.0, d2min = INFINITY;
         for (int i = 0; i < r; ++i) {
@@ -393,19 +390,16 @@ __global__ void __launch_bounds__(kWG)
         }
         const double eps = 2.220446049250313e-16;
         const double tol2 = (10.0 * eps) * (10.0 * eps) * d2max;
-        int64_t result = -1;
         if (!(info == 0 && d2min > tol2)) {
             int64_t rank = 0;
             for (int i = 0; i < r; ++i) rank += cabs2(A[i + i * r]) > tol2;
             result = rank;
         }
-        indep[b] = result;
     }
-    double2* Xb = X + xoff[b];
     for (int a = t; a < m; a += kWG) {
         double2* xr = Xb + a;
         for (int c = 0; c < r; ++c) {
-            double2 acc = buf[r0 + a + (int64_t)cpiv[c] * ld];
+            double2 acc = src(a, cpiv[c]);
             for (int kk = 0; kk < c; ++kk) acc = csub(acc, cmul(xr[(int64_t)kk * m], A[kk + c * r]));
             xr[(int64_t)c * m] = cdiv(acc, A[c + c * r]);
         }
@@ -423,6 +417,135 @@ __global__ void __launch_bounds__(kWG)
             }
         }
     }
+    return result;
+}
+
+// Batched skeleton solve (skeleton_block), one workgroup per block of the stack.
+__global__ void __launch_bounds__(kWG)
+    k_skeleton(const double2* __restrict__ buf, int64_t ld, const int64_t* __restrict__ rowoff,
+               const int32_t* __restrict__ ranks, const int64_t* __restrict__ rowpiv,
+               const int64_t* __restrict__ colpiv, int64_t maxrank, const int64_t* __restrict__ xoff,
+               double2* __restrict__ X, int64_t* __restrict__ indep) {
+    extern __shared__ double2 A[];
+    __shared__ int ipiv[kSolveMaxRank];
+    __shared__ int cpiv[kSolveMaxRank];
+    __shared__ ArgRed red;
+    const int t = threadIdx.x;
+    const int b = blockIdx.x;
+    const int r = ranks[b];
+    if (r == 0) {
+        if (t == 0) indep[b] = -1;
+        return;
+    }
+    const int64_t r0 = rowoff[b];
+    const int m = (int)(rowoff[b + 1] - r0);
+    auto src = [&](int a, int c) { return buf[r0 + a + (int64_t)c * ld]; };
+    auto rp = [&](int k) { return (int)rowpiv[k + b * maxrank] - 1; };
+    auto cp = [&](int k) { return (int)colpiv[k + b * maxrank] - 1; };
+    const int64_t res = skeleton_block(src, rp, cp, m, r, A, ipiv, cpiv, red, X + xoff[b]);
+    if (t == 0) indep[b] = res;
+}
+
+// ---------------------------------------------------------------------------
+// One level of the bottom-up compressor in one launch, one workgroup per
+// block: the block's rows (a leaf's points, else its children's pivots, read
+// from the pivots the previous launch left on the device), its entries
+// evaluated into the residual (LDS, or the block's global working copy when
+// it is larger), the ACA or IACA, the skeleton solve with the pivot rows and
+// columns evaluated again, and the factor X written to the level's chunk. The
+// arithmetic is that of k_assemble_level + k_aca / k_iaca + k_skeleton, so the
+// pivots, ranks and factors are the same bit for bit.
+struct LevelTree {
+    const int32_t* childptr;  // nids + 1, 0-based
+    const int32_t* children;
+    const int32_t* lo;        // values of a leaf: perm[lo, hi)
+    const int32_t* hi;
+    const int32_t* perm;
+};
+
+struct LevelOut {
+    int32_t* piv;     // nids x maxrank: pivot point ids (0-based)
+    int32_t* rank;    // nids
+    int32_t* mrows;   // nids
+    int64_t* indep;   // nids
+    int32_t* err;     // [0]: a block above the kernel limits, [1]: its id
+};
+
+template <bool IACA>
+__global__ void __launch_bounds__(kWG)
+    k_level_fused(const double* __restrict__ rowpts, const double* __restrict__ colpts, double k, int sign,
+                  int transposed, const int32_t* __restrict__ ids, const int32_t* __restrict__ ncols,
+                  const int64_t* __restrict__ colptr, const int32_t* __restrict__ colidx,
+                  const int64_t* __restrict__ xoff, const int64_t* __restrict__ workoff, LevelTree tr,
+                  int64_t maxrank, double tol, int sampled, double2* __restrict__ work, double2* __restrict__ X,
+                  LevelOut out) {
+    extern __shared__ double2 dyn[];
+    __shared__ ArgRed red;
+    __shared__ int rows[kGlobMaxM];
+    __shared__ int rpiv[kGlobMaxN], cpivl[kGlobMaxN];
+    __shared__ int ipiv[kSolveMaxRank], cpiv[kSolveMaxRank];
+    const int t = threadIdx.x;
+    const int b = blockIdx.x;
+    const int id = ids[b];
+    const int n = ncols[b];
+    const int32_t* cols = colidx + colptr[b];
+    // rows: the leaf's points or the children's pivots, in child order
+    // (the host bounded them by kGlobMaxM)
+    const int c0 = tr.childptr[id], c1 = tr.childptr[id + 1];
+    int m = 0;
+    if (c0 == c1) {
+        const int l0 = tr.lo[id];
+        m = tr.hi[id] - l0;
+        for (int a = t; a < min(m, kGlobMaxM); a += kWG) rows[a] = tr.perm[l0 + a];
+    } else {
+        for (int q = c0; q < c1; ++q) {
+            const int ch = tr.children[q];
+            const int rc = out.rank[ch];
+            for (int a = t; a < rc && m + a < kGlobMaxM; a += kWG) rows[m + a] = out.piv[(int64_t)ch * maxrank + a];
+            m += rc;
+        }
+    }
+    __syncthreads();
+    int kmax = (int)maxrank;
+    kmax = m < kmax ? m : kmax;
+    kmax = n < kmax ? n : kmax;
+    if (t == 0) {
+        out.mrows[id] = m;
+        out.rank[id] = 0;
+        out.indep[id] = -1;
+    }
+    if (kmax <= 0) return;
+    const bool lds = m * n <= kLdsCap && m <= kLdsMaxM && n <= kLdsMaxN;
+    if (!lds && (m > kGlobMaxM || n > kGlobMaxN || workoff[b] < 0)) {
+        if (t == 0) {
+            out.err[0] = 1;
+            out.err[1] = id;
+        }
+        return;
+    }
+    auto src = [&](int a, int c) {
+        const double* xr = rowpts + 3 * (int64_t)rows[a];
+        const double* xc = colpts + 3 * (int64_t)cols[c];
+        return transposed ? helmholtz(k, sign, xc[0], xc[1], xc[2], xr[0], xr[1], xr[2])
+                          : helmholtz(k, sign, xr[0], xr[1], xr[2], xc[0], xc[1], xc[2]);
+    };
+    auto piv = [&](int kk, int i, int j) {
+        rpiv[kk] = i;
+        cpivl[kk] = j;
+    };
+    double2* R = lds ? dyn : work + workoff[b];
+    double2* u = lds ? dyn + kLdsCap : dyn;
+    double2* v = lds ? u + kLdsMaxM : dyn + kGlobMaxM;
+    const int r = IACA ? iaca_block(src, R, m, u, v, m, n, kmax, tol, red, piv)
+                       : aca_block(src, R, m, u, v, m, n, kmax, tol, sampled, red, piv);
+    __syncthreads();
+    for (int q = t; q < r; q += kWG) out.piv[(int64_t)id * maxrank + q] = rows[rpiv[q]];
+    if (t == 0) out.rank[id] = r;
+    if (r == 0) return;
+    auto rp = [&](int q) { return rpiv[q]; };
+    auto cp = [&](int q) { return cpivl[q]; };
+    const int64_t res = skeleton_block(src, rp, cp, m, r, dyn, ipiv, cpiv, red, X + xoff[b]);
+    if (t == 0) out.indep[id] = res;
 }
 
 // ---------------------------------------------------------------------------
@@ -1057,6 +1180,7 @@ static void compress_level_keep(nca_ctx* ctx, const nca_stack* st, double tol, i
         const int32_t c = (int32_t)f->chunks.size();
         f->chunks.push_back(std::move(d_X));
         f->chunk_size.push_back(xoff[nb]);
+        f->chunk_ids.emplace_back();
         for (int64_t b = 0; b < nb; ++b) {
             const int64_t id = ids[b] - 1;
             f->chunk[id] = c;
@@ -1136,12 +1260,209 @@ int32_t nca_factors_download(nca_ctx* ctx, const nca_factors* f, double* X) {
         set_device(ctx);
         double2* dst = reinterpret_cast<double2*>(X);
         for (size_t c = 0; c < f->chunks.size(); ++c) {
-            if (f->chunk_size[c] > 0)
-                NCA_HIP(hipMemcpyAsync(dst, f->chunks[c].p, sizeof(double2) * f->chunk_size[c],
-                                       hipMemcpyDeviceToHost, ctx->stream));
+            if (f->chunk_size[c] > 0) {
+                if (f->chunk_ids[c].empty()) {
+                    NCA_HIP(hipMemcpyAsync(dst, f->chunks[c].p, sizeof(double2) * f->chunk_size[c],
+                                           hipMemcpyDeviceToHost, ctx->stream));
+                } else {
+                    // packed first, in the order of the chunk's ids
+                    std::vector<PackJob> jobs;
+                    int64_t o = 0;
+                    for (int32_t id : f->chunk_ids[c]) {
+                        const FactorRef F = factor_of(f, id);
+                        jobs.push_back({F.p, o, (int32_t)F.m, (int32_t)F.m, (int32_t)F.r, 1, (int32_t)F.m, 0});
+                        o += F.m * F.r;
+                    }
+                    DevBuf<double2> tmp((size_t)f->chunk_size[c]);
+                    pack_blocks(jobs, tmp.p, ctx->stream);
+                    NCA_HIP(hipMemcpyAsync(dst, tmp.p, sizeof(double2) * f->chunk_size[c], hipMemcpyDeviceToHost,
+                                           ctx->stream));
+                    NCA_HIP(hipStreamSynchronize(ctx->stream));
+                }
+            }
             dst += f->chunk_size[c];
         }
         NCA_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int32_t nca_bottomup_side(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* self,
+                          const nca_points* other, int32_t transposed, int64_t nids, const int64_t* childptr,
+                          const int64_t* children, const int64_t* lo, const int64_t* hi, const int64_t* perm,
+                          int64_t nlevels, const int64_t* levptr, const int64_t* levids, const int64_t* fpptr,
+                          const int64_t* fpidx, int64_t maxrank, double tol, int32_t sampled, int32_t method,
+                          nca_factors* f, int32_t* ranks, int64_t* mrows, int64_t pivcap, int64_t* pivots,
+                          int64_t* independent) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && kernel && self && other && f && childptr && lo && hi && levptr && fpptr && ranks &&
+                        mrows && pivots && independent && nids >= 0 && nlevels >= 0,
+                    NCA_ERR_ARGUMENT, "nca_bottomup_side: invalid argument");
+        NCA_REQUIRE(method == 0 || method == 1, NCA_ERR_ARGUMENT, "nca_bottomup_side: unknown method");
+        NCA_REQUIRE(f->ctx == ctx && f->nids == nids, NCA_ERR_ARGUMENT, "nca_bottomup_side: factors of another tree");
+        // the r x r cross matrix shares the dynamic LDS: maxrank <= kSolveMaxRank keeps
+        // it and the kernel's static LDS within the 160 KiB of a CU
+        NCA_REQUIRE(maxrank >= 0 && maxrank <= kSolveMaxRank, NCA_ERR_LIMIT,
+                    "nca_bottomup_side: maxrank above " + std::to_string(kSolveMaxRank));
+        NCA_REQUIRE(nids < (1LL << 31) && self->n < (1LL << 31) && other->n < (1LL << 31), NCA_ERR_LIMIT,
+                    "nca_bottomup_side: more than 2^31 blocks or points");
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        // tree (0-based, 32-bit)
+        std::vector<int32_t> cp(nids + 1), lo0(nids), hi0(nids);
+        for (int64_t i = 0; i <= nids; ++i) cp[i] = (int32_t)(childptr[i] - childptr[0]);
+        const int64_t nch = cp[nids];
+        std::vector<int32_t> ch(std::max<int64_t>(nch, 1)), pm(std::max<int64_t>(self->n, 1));
+        for (int64_t q = 0; q < nch; ++q) {
+            NCA_REQUIRE(children[q] >= 1 && children[q] <= nids, NCA_ERR_ARGUMENT, "nca_bottomup_side: child out of range");
+            ch[q] = (int32_t)(children[q] - 1);
+        }
+        for (int64_t i = 0; i < nids; ++i) {
+            lo0[i] = (int32_t)(lo[i] - 1);
+            hi0[i] = (int32_t)hi[i];
+            NCA_REQUIRE(lo0[i] >= 0 && hi0[i] >= lo0[i] && hi0[i] <= self->n, NCA_ERR_ARGUMENT,
+                        "nca_bottomup_side: value range out of bounds");
+        }
+        for (int64_t q = 0; q < self->n; ++q) {
+            NCA_REQUIRE(perm[q] >= 1 && perm[q] <= self->n, NCA_ERR_ARGUMENT, "nca_bottomup_side: perm out of range");
+            pm[q] = (int32_t)(perm[q] - 1);
+        }
+        // blocks of every level, their bounds (rows: a leaf's values, else the
+        // children's rank bounds min(maxrank, rows, columns)), factor and work offsets
+        const int64_t nblk = nlevels ? levptr[nlevels] - levptr[0] : 0;
+        std::vector<int32_t> bid(std::max<int64_t>(nblk, 1)), bn(std::max<int64_t>(nblk, 1));
+        std::vector<int64_t> bcol(std::max<int64_t>(nblk, 1)), bx(std::max<int64_t>(nblk, 1)),
+            bw(std::max<int64_t>(nblk, 1));
+        std::vector<int64_t> rb(nids, 0);
+        std::vector<char> seen(nids, 0);
+        std::vector<int64_t> lvwork(nlevels, 0);
+        int64_t xtot = 0, worktot = 0;
+        for (int64_t L = 0; L < nlevels; ++L) {
+            int64_t wl = 0;
+            for (int64_t q = levptr[L] - levptr[0]; q < levptr[L + 1] - levptr[0]; ++q) {
+                const int64_t id = levids[q] - 1;
+                NCA_REQUIRE(id >= 0 && id < nids && !seen[id] && f->chunk[id] < 0, NCA_ERR_ARGUMENT,
+                            "nca_bottomup_side: block id out of range, repeated or with a factor already");
+                seen[id] = 1;
+                int64_t mcap = 0;
+                if (cp[id] == cp[id + 1]) mcap = hi0[id] - lo0[id];
+                else
+                    for (int32_t k = cp[id]; k < cp[id + 1]; ++k) mcap += rb[ch[k]];
+                const int64_t n = fpptr[id + 1] - fpptr[id];
+                NCA_REQUIRE(n >= 0 && n <= maxrank, NCA_ERR_ARGUMENT,
+                            "a block has more columns than maxrank=" + std::to_string(maxrank));
+                NCA_REQUIRE(n == 0 || mcap <= kGlobMaxM, NCA_ERR_LIMIT,
+                            "nca_bottomup_side: block " + std::to_string(id + 1) + " may have " +
+                                std::to_string(mcap) + " rows, above " + std::to_string(kGlobMaxM));
+                rb[id] = std::min<int64_t>(std::min<int64_t>(maxrank, mcap), n);
+                bid[q] = (int32_t)id;
+                bn[q] = (int32_t)n;
+                bcol[q] = fpptr[id] - fpptr[0];
+                bx[q] = xtot;
+                xtot += mcap * rb[id];
+                const bool lds = mcap * n <= kLdsCap && mcap <= kLdsMaxM && n <= kLdsMaxN;
+                bw[q] = lds ? -1 : wl;
+                if (!lds) wl += mcap * n;
+            }
+            worktot = std::max(worktot, wl);
+        }
+        const int64_t ncolidx = fpptr[nids] - fpptr[0];
+        std::vector<int32_t> ci(std::max<int64_t>(ncolidx, 1));
+        for (int64_t q = 0; q < ncolidx; ++q) {
+            NCA_REQUIRE(fpidx[q] >= 1 && fpidx[q] <= other->n, NCA_ERR_ARGUMENT,
+                        "nca_bottomup_side: far pivot out of range");
+            ci[q] = (int32_t)(fpidx[q] - 1);
+        }
+        DevBuf<int32_t> d_cp, d_ch, d_lo, d_hi, d_pm, d_ci, d_bid, d_bn, d_piv, d_rank, d_m, d_err;
+        DevBuf<int64_t> d_bcol, d_bx, d_bw, d_indep;
+        d_cp.upload(cp, st);
+        d_ch.upload(ch, st);
+        d_lo.upload(lo0, st);
+        d_hi.upload(hi0, st);
+        d_pm.upload(pm, st);
+        d_ci.upload(ci, st);
+        d_bid.upload(bid, st);
+        d_bn.upload(bn, st);
+        d_bcol.upload(bcol, st);
+        d_bx.upload(bx, st);
+        d_bw.upload(bw, st);
+        d_piv.alloc((size_t)std::max<int64_t>(nids * maxrank, 1));
+        d_rank.alloc((size_t)std::max<int64_t>(nids, 1));
+        d_m.alloc((size_t)std::max<int64_t>(nids, 1));
+        d_indep.alloc((size_t)std::max<int64_t>(nids, 1));
+        d_err.alloc(2);
+        NCA_HIP(hipMemsetAsync(d_rank.p, 0, sizeof(int32_t) * std::max<int64_t>(nids, 1), st));
+        NCA_HIP(hipMemsetAsync(d_m.p, 0, sizeof(int32_t) * std::max<int64_t>(nids, 1), st));
+        NCA_HIP(hipMemsetAsync(d_piv.p, 0, sizeof(int32_t) * std::max<int64_t>(nids * maxrank, 1), st));
+        NCA_HIP(hipMemsetAsync(d_indep.p, 0xff, sizeof(int64_t) * std::max<int64_t>(nids, 1), st));
+        NCA_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int32_t) * 2, st));
+        DevBuf<double2> d_X((size_t)std::max<int64_t>(xtot, 1)), d_work((size_t)std::max<int64_t>(worktot, 1));
+        const size_t lds = sizeof(double2) * std::max<size_t>(
+                                                  std::max<size_t>(kLdsCap + kLdsMaxM + kLdsMaxN, kGlobMaxM + kGlobMaxN),
+                                                  (size_t)(maxrank * maxrank));
+        auto kern = method == 1 ? k_level_fused<true> : k_level_fused<false>;
+        NCA_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const nca_points* rowp = self;
+        const nca_points* colp = other;
+        LevelTree tr{d_cp.p, d_ch.p, d_lo.p, d_hi.p, d_pm.p};
+        LevelOut out{d_piv.p, d_rank.p, d_m.p, d_indep.p, d_err.p};
+        for (int64_t L = 0; L < nlevels; ++L) {
+            const int64_t q0 = levptr[L] - levptr[0], nb = levptr[L + 1] - levptr[L];
+            if (nb == 0) continue;
+            hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(kWG), lds, st, rowp->xyz.p, colp->xyz.p,
+                               kernel->wavenumber, (int)kernel->sign, (int)(transposed != 0), d_bid.p + q0,
+                               d_bn.p + q0, d_bcol.p + q0, d_ci.p, d_bx.p + q0, d_bw.p + q0, tr, maxrank, tol,
+                               (int)(sampled != 0), d_work.p, d_X.p, out);
+            NCA_HIP(hipGetLastError());
+        }
+        std::vector<int32_t> hr(std::max<int64_t>(nids, 1)), hm(std::max<int64_t>(nids, 1)),
+            hp(std::max<int64_t>(nids * maxrank, 1)), herr(2);
+        std::vector<int64_t> hind(std::max<int64_t>(nids, 1));
+        d2h(ctx, hr.data(), d_rank.p, sizeof(int32_t) * hr.size());
+        d2h(ctx, hm.data(), d_m.p, sizeof(int32_t) * hm.size());
+        d2h(ctx, hp.data(), d_piv.p, sizeof(int32_t) * hp.size());
+        d2h(ctx, hind.data(), d_indep.p, sizeof(int64_t) * hind.size());
+        d2h(ctx, herr.data(), d_err.p, sizeof(int32_t) * 2);
+        NCA_REQUIRE(herr[0] == 0, NCA_ERR_LIMIT,
+                    "nca_bottomup_side: block " + std::to_string(herr[1] + 1) + " exceeds " +
+                        std::to_string(kGlobMaxM) + " x " + std::to_string(kGlobMaxN));
+        int64_t npiv = 0;
+        for (int64_t q = 0; q < nblk; ++q) npiv += hr[bid[q]];
+        NCA_REQUIRE(npiv <= pivcap, NCA_ERR_DIMENSION,
+                    "nca_bottomup_side: the pivots need " + std::to_string(npiv) + " entries, pivcap is " +
+                        std::to_string(pivcap));
+        // the factors: one chunk at the bounded offsets, downloaded in level order
+        const int32_t c = (int32_t)f->chunks.size();
+        std::vector<int32_t> order;
+        int64_t packed = 0;
+        for (int64_t q = 0; q < nblk; ++q) {
+            const int32_t id = bid[q];
+            f->chunk[id] = c;
+            f->off[id] = bx[q];
+            f->m[id] = hm[id];
+            f->rank[id] = hr[id];
+            packed += (int64_t)hm[id] * hr[id];
+            order.push_back(id);
+        }
+        f->chunks.push_back(std::move(d_X));
+        f->chunk_size.push_back(packed);
+        f->chunk_ids.push_back(std::move(order));
+        for (int64_t i = 0; i < nids; ++i) {
+            ranks[i] = seen[i] ? hr[i] : 0;
+            mrows[i] = seen[i] ? hm[i] : 0;
+            independent[i] = seen[i] ? hind[i] : -1;
+        }
+        int64_t o = 0;
+        for (int64_t q = 0; q < nblk; ++q) {
+            const int64_t id = bid[q];
+            for (int32_t k = 0; k < hr[id]; ++k) pivots[o++] = (int64_t)hp[id * maxrank + k] + 1;
+        }
+        for (int64_t q = 0; q < nblk; ++q) {
+            const int32_t id = bid[q];
+            if (hind[id] >= 0)
+                throw Error(NCA_ERR_SINGULAR, "SingularCrossMatrixError: block " + std::to_string(id + 1) + ": " +
+                                                  std::to_string(hr[id]) + " pivots were selected but only " +
+                                                  std::to_string(hind[id]) + " of them are linearly independent");
+        }
     });
 }
 

@@ -450,11 +450,61 @@ def _compresslevel(cfg, storage, active, rowptr, rowidx, out):
     out.add_level(active, m, ranks, rowidx[pos], xoff, X)
 
 
-def batched_side(compressor, farmatrix, tree, fd, side, maxrank=50, farpivots=None, resident=False):
+def _bottomup_fused(compressor, farmatrix, selftree, side, maxrank, farpivots, out):
+    """All levels of batched_side in one call (nca_bottomup_side): one launch
+    per level assembles, compresses and solves every block, the rows of a
+    level built on the device from the pivots of the level below, the factors
+    kept in out.pool. Fills out as the level loop would."""
+    fptr, fidx = farpivots
+    nfar = np.diff(fptr)
+    leaf = selftree.leafmask()
+    levs = sorted(selftree.levels().tolist(), reverse=True)
+    actives = []
+    for lev in levs:
+        nodes = selftree.levelnodes(lev)
+        actives.append((lev, nodes, nodes[nfar[nodes] > 0]))
+    levptr = np.zeros(len(actives) + 1, np.int64)
+    np.cumsum([len(a) for _, _, a in actives], out=levptr[1:])
+    levids = np.concatenate([a for _, _, a in actives] + [np.zeros(0, np.int64)]).astype(np.int64)
+    nn = selftree.nnodes
+    selfpts, otherpts = (farmatrix.test, farmatrix.trial) if side == 0 else (farmatrix.trial, farmatrix.test)
+    i64 = lambda a: np.ascontiguousarray(a, np.int64) if len(a) else np.zeros(1, np.int64)
+    arr = dict(cp=i64(selftree.childptr + 1), ch=i64(selftree.children + 1), lo=i64(selftree.lo + 1),
+               hi=i64(selftree.hi), perm=i64(selftree.perm + 1), levptr=i64(levptr + 1), levids=i64(levids + 1),
+               fptr=i64(np.asarray(fptr) + 1), fidx=i64(np.asarray(fidx) + 1))
+    ranks = np.zeros(max(nn, 1), np.int32)
+    mrows = np.zeros(max(nn, 1), np.int64)
+    pivcap = int(np.minimum(nfar[levids], maxrank).sum()) if len(levids) else 0
+    piv = np.empty(max(pivcap, 1), np.int64)
+    indep = np.zeros(max(nn, 1), np.int64)
+    cfg = compressor.compressor
+    kern = farmatrix.kernel.abi()
+    call(hip_lib().nca_bottomup_side(
+        farmatrix.ctx.handle, ctypes.byref(kern), selfpts.handle, otherpts.handle, int(side == 1), nn,
+        _p(arr["cp"]), _p(arr["ch"]), _p(arr["lo"]), _p(arr["hi"]), _p(arr["perm"]), len(actives),
+        _p(arr["levptr"]), _p(arr["levids"]), _p(arr["fptr"]), _p(arr["fidx"]), int(maxrank), float(cfg.tol), 1,
+        1 if isinstance(cfg, IACA) else 0, out.pool.handle, _p(ranks), _p(mrows), pivcap, _p(piv), _p(indep)))
+    p0 = 0
+    for lev, nodes, active in actives:
+        if len(active):
+            r = ranks[active].astype(np.int64)
+            m = mrows[active]
+            xoff = np.zeros(len(active) + 1, np.int64)
+            np.cumsum(m * r, out=xoff[1:])
+            p1 = p0 + int(r.sum())
+            out.add_level(active, m, r, piv[p0:p1] - 1, xoff, None)
+            p0 = p1
+            out.level_transfer_nodes[lev] = nodes[~leaf[nodes] & out.assigned[nodes]].tolist()
+    return out
+
+
+def batched_side(compressor, farmatrix, tree, fd, side, maxrank=50, farpivots=None, resident=False, fused=None):
     """testbases (side 0) / trialbases (side 1) of BatchedBottomUp, returning
     LevelFactors. farpivots: optional Phase-1 result (0-based CSR). resident:
     the factors stay on the device (LevelFactors.pool) and are downloaded only
-    when read."""
+    when read. fused (default: resident): all levels in one call, one launch
+    per level (nca_bottomup_side); else one assembly and one compression call
+    per level."""
     selftree = tree.testtree if side == 0 else tree.trialtree
     othertree = tree.trialtree if side == 0 else tree.testtree
     if farpivots is None:
@@ -466,6 +516,17 @@ def batched_side(compressor, farmatrix, tree, fd, side, maxrank=50, farpivots=No
     out.farpivots = farpivots
     if resident:
         out.pool = FactorPool(farmatrix.ctx, selftree.nnodes)
+    if fused is None:
+        fused = resident
+    if fused:
+        if not resident:
+            raise ValueError("batched_side: the fused driver keeps the factors on the device (resident=True)")
+        t0 = time.perf_counter()
+        _bottomup_fused(compressor, farmatrix, selftree, side, maxrank, farpivots, out)
+        if _timing():
+            print(f"[batchedbottomup {'test' if side == 0 else 'trial'}] fused levels: "
+                  f"{time.perf_counter() - t0:.4f}s", flush=True)
+        return out
     leaf = selftree.leafmask()
     for lev in sorted(selftree.levels().tolist(), reverse=True):
         nodes = selftree.levelnodes(lev)
