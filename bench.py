@@ -135,7 +135,11 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="copy this many MiB device to device first (a kernel of known traffic for counter scaling)")
     ap.add_argument("--workload", choices=["sphere1m", "batch64", "wideband262k"], default="sphere1m")
-    ap.add_argument("--autotune", type=int, default=1, help="run nca_h2_autotune per precision before timing")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="launch choices per precision: 1 from the tuning table (measured when the operator is not "
+                         "in it), 2 measured (nca_h2_autotune), 0 the defaults")
+    ap.add_argument("--tuning-table", default=None, help="tuning table (default: the package's MI355X table)")
+    ap.add_argument("--tuning-save", default=None, help="write the tuning table, with this run's choices, here")
     args = ap.parse_args()
     wide = args.workload == "wideband262k"
     if args.n is None:
@@ -222,9 +226,7 @@ def main():
     for label, prec in (("fp64", nca_amd.NCA_PREC_FP64), ("mixed", nca_amd.NCA_PREC_MIXED)):
         core.set_precision(prec)
         if args.autotune:
-            t0 = time.perf_counter()
-            tuning[label] = core.autotune(reps=3)
-            tuning[label]["seconds"] = round(time.perf_counter() - t0, 3)
+            tuning[label] = tune_core(core, prec, args, f"sphere N={args.n} k={args.k} tol={args.tol}")
             log(f"autotune {label}: {tuning[label]}")
         ms = timed(0, R, args.steps, args.warmup)
         log(f"{label.upper()}: {ms:.3f} ms per sweep of {R} RHS -> {R / (ms * 1e-3):.1f} matvec/s")
@@ -473,8 +475,12 @@ def bench_batch(args):
     dx, dy = hip.malloc(X.nbytes), hip.malloc(X.nbytes)
     hip.h2d(dx, X)
     out = {}
+    tuning = {}
     for label, prec in (("fp64", nca_amd.NCA_PREC_FP64), ("mixed", nca_amd.NCA_PREC_MIXED)):
         B.core.set_precision(prec)
+        if args.autotune:
+            tuning[label] = tune_core(B.core, prec, args, f"batch of {P} ellipsoids, {N} points")
+            log(f"autotune {label}: {tuning[label]}")
         # eager sweeps
         for _ in range(args.warmup):
             B.core.matvec_device(0, R, dx, N, dy, N)
@@ -526,9 +532,32 @@ def bench_batch(args):
         "host_prep_s": {"trees": round(t_tree, 3), "partition": round(tm["partition"], 3),
                         "phase1": round(tm["phase1"], 3), "h2_create": round(tm["h2_create"], 3)},
         "operator": {k_: int(v) for k_, v in B.core.stats().items()},
+        "autotune": tuning,
     }
     log(f"mixed vs fp64 {err_mixed:.3e}; direct-sum errors {errs}; total {time.perf_counter() - t_all:.1f}s")
     print(json.dumps(result), flush=True)
+
+
+_tables = {}
+
+
+def tune_core(core, prec, args, workload):
+    """Launch choices for the core's current precision: from the tuning table
+    (--autotune 1; measured and added when the operator is not in it) or
+    measured (--autotune 2, added to the table); --tuning-save writes it."""
+    from nca_amd import tuning as TU
+    path = args.tuning_table or TU.DEFAULT_TABLE
+    table = _tables.setdefault(path, TU.TuningTable(path))
+    t0 = time.perf_counter()
+    if args.autotune == 1:
+        r = TU.tune(core, prec, table, workload=workload)
+    else:
+        r = TU.tune(core, prec, None)
+        table.put(r["key"], r, workload)
+    r["seconds"] = round(time.perf_counter() - t0, 3)
+    if args.tuning_save:
+        table.save(args.tuning_save)
+    return r
 
 
 def bench_wideband(args):
@@ -572,7 +601,8 @@ def bench_wideband(args):
     for label, prec in (("fp64", nca_amd.NCA_PREC_FP64), ("mixed", nca_amd.NCA_PREC_MIXED)):
         core.set_precision(prec)
         if args.autotune:
-            tuning[label] = core.autotune(reps=3)
+            tuning[label] = tune_core(core, prec, args, f"wideband N={N} k={args.k:.4f} tol={args.tol}")
+            log(f"autotune {label}: {tuning[label]}")
         for _ in range(args.warmup):
             core.matvec_device(0, R, dx, N, dy, N)
         ctx.synchronize()

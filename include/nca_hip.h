@@ -65,6 +65,9 @@ int32_t nca_ctx_destroy(nca_ctx* ctx);
 int32_t nca_ctx_synchronize(nca_ctx* ctx);
 /* The hipStream_t all work of the context runs on (for device-pointer callers). */
 int32_t nca_ctx_stream(nca_ctx* ctx, void** stream);
+/* The device's architecture name (gcnArchName, e.g. "gfx950:sramecc+:xnack-"),
+ * NUL-terminated in buf[len]; the key of tuning tables. */
+int32_t nca_ctx_device_name(nca_ctx* ctx, char* buf, int64_t len);
 
 /* Upload n points (3 x n, column-major). */
 int32_t nca_points_create(nca_ctx* ctx, const double* xyz, int64_t n, nca_points** out);
@@ -459,6 +462,11 @@ typedef struct {
     double near_ms_start, couple_ms_start; /* stage times of the configuration before tuning */
 } nca_h2_tuning;
 int32_t nca_h2_autotune(nca_h2* h2, int32_t reps, nca_h2_tuning* out);
+/* Applies launch choices that nca_h2_autotune returned earlier (a persisted
+ * tuning table) to the operator's current precision, without timing; the
+ * timing fields are not read. Captured graphs must be captured again
+ * (NCA_ERR_ARGUMENT at launch). No counterpart in the reference. */
+int32_t nca_h2_tuning_apply(nca_h2* h2, const nca_h2_tuning* t);
 
 /* Measurement aid, no counterpart in the reference: one product as
  * nca_h2_matvec_device (nrhs <= 16) with HIP events recorded on the context

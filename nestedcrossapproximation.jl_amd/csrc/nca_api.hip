@@ -1,4 +1,5 @@
 // Runtime entry points of libnca_hip.so: errors, contexts, point sets.
+#include <algorithm>
 #include <memory>
 
 #include "nca_common.hpp"
@@ -57,6 +58,18 @@ int32_t nca_ctx_stream(nca_ctx* ctx, void** stream) {
     return guarded([&] {
         NCA_REQUIRE(ctx && stream, NCA_ERR_ARGUMENT, "nca_ctx_stream: invalid argument");
         *stream = (void*)ctx->stream;
+    });
+}
+
+int32_t nca_ctx_device_name(nca_ctx* ctx, char* buf, int64_t len) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && buf && len > 0, NCA_ERR_ARGUMENT, "nca_ctx_device_name: invalid argument");
+        hipDeviceProp_t prop;
+        NCA_HIP(hipGetDeviceProperties(&prop, ctx->device));
+        std::string name = prop.gcnArchName;
+        const size_t n = std::min<size_t>(name.size(), (size_t)len - 1);
+        std::memcpy(buf, name.data(), n);
+        buf[n] = 0;
     });
 }
 

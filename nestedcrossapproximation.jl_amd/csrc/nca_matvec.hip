@@ -3423,6 +3423,38 @@ int32_t nca_h2_autotune(nca_h2* h2, int32_t reps, nca_h2_tuning* out) {
     });
 }
 
+int32_t nca_h2_tuning_apply(nca_h2* h2, const nca_h2_tuning* t) {
+    return guarded([&] {
+        NCA_REQUIRE(h2 && t, NCA_ERR_ARGUMENT, "nca_h2_tuning_apply: invalid argument");
+        NCA_REQUIRE(t->near_k3 >= 0 && t->near_k3 <= 1 && t->couple_k3 >= 0 && t->couple_k3 <= 1 &&
+                        t->leaf_order >= 0 && t->leaf_order <= 2 && t->near_xcd_order >= 0 &&
+                        t->near_xcd_order <= 1 && t->near_stagger >= 0 && t->near_stagger <= 1 &&
+                        t->couple_tile_order >= 0 && t->couple_tile_order <= 3 && t->couple_xcd_order >= 0 &&
+                        t->couple_xcd_order <= 1,
+                    NCA_ERR_ARGUMENT, "nca_h2_tuning_apply: a launch choice out of range");
+        std::lock_guard<std::mutex> lock(h2->mu);
+        set_device(h2->ctx);
+        hipStream_t st = h2->ctx->stream;
+        ++h2->generation;  // the launch plans may be reordered
+        const int prec = h2->precision == NCA_PREC_MIXED ? 1 : 0;
+        h2->near_k3[prec] = t->near_k3;
+        h2->near_xcd = t->near_xcd_order;
+        h2->near_stagger = t->near_stagger;
+        if (h2->leaf_order != t->leaf_order) {
+            order_leaves(h2->test, t->leaf_order, st);
+            order_leaves(h2->trial, t->leaf_order, st);
+            h2->leaf_order = t->leaf_order;
+        }
+        h2->couple_k3[prec] = t->couple_k3;
+        h2->couple_xcd = t->couple_xcd_order;
+        if (h2->f_ntiles > 0 && h2->couple_order != t->couple_tile_order) {
+            order_coupling_tiles(h2, t->couple_tile_order, st);
+            h2->couple_order = t->couple_tile_order;
+        }
+        NCA_HIP(hipStreamSynchronize(st));
+    });
+}
+
 int32_t nca_h2_matvec_profile(nca_h2* h2, int32_t op, int64_t nrhs, const void* x, int64_t ldx, void* y,
                               int64_t ldy, const double* alpha, const double* beta, double* stage_ms) {
     return guarded([&] {

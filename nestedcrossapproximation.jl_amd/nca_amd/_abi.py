@@ -80,6 +80,7 @@ SIGNATURES = {
     "nca_ctx_destroy": (I32, [P]),
     "nca_ctx_synchronize": (I32, [P]),
     "nca_ctx_stream": (I32, [P, PP]),
+    "nca_ctx_device_name": (I32, [P, P, I64]),
     "nca_points_create": (I32, [P, P, I64, PP]),
     "nca_points_destroy": (I32, [P]),
     "nca_level_assemble": (I32, [P, P, P, P, I64, P, P, P, P, I64, I32, PP]),
@@ -124,6 +125,7 @@ SIGNATURES = {
     "nca_h2_store_near": (I32, [P, I32]),
     "nca_h2_matvec_profile": (I32, [P, I32, I64, P, I64, P, I64, P, P, P]),
     "nca_h2_autotune": (I32, [P, I32, P]),
+    "nca_h2_tuning_apply": (I32, [P, P]),
     "nca_h2_graph_create": (I32, [P, I32, I64, P, I64, P, I64, P, P, PP]),
     "nca_h2_graph_launch": (I32, [P]),
     "nca_h2_graph_destroy": (I32, [P]),

@@ -163,6 +163,33 @@ def test_autotune_keeps_results(ctx, precision):
     assert relerr(NC.adjoint(W) @ u, NC.adjoint(NC.H2Workspace(W2)) @ u) <= bound
 
 
+@pytest.mark.parametrize("precision", [nca_amd.NCA_PREC_FP64, nca_amd.NCA_PREC_MIXED])
+def test_tuning_table_round_trip(ctx, precision, tmp_path):
+    """A tuning table: the first operator is measured and stored, a second one
+    of the same shape takes the stored choices without timing (the same
+    choices, so the same products bit for bit), the table survives a write and
+    a read, and choices out of range are refused."""
+    from nca_amd import tuning as TU
+    say(f"tuning table precision={precision}")
+    pts = geometry.fibonacci_sphere(5000)
+    a, b = _core(ctx, pts), _core(ctx, pts)
+    for c in (a, b):
+        c.set_precision(precision)
+    path = str(tmp_path / "table.json")
+    table = TU.TuningTable(path)
+    ra = TU.tune(a, precision, table, reps=1, workload="sphere 5000")
+    assert ra["source"] == "measured" and ra["key"] in table.entries
+    table.save()
+    rb = TU.tune(b, precision, TU.TuningTable(path))
+    assert rb["source"] == "table" and rb["key"] == ra["key"]
+    assert all(rb[n] == ra[n] for n in TU.CHOICES)
+    x = geometry.complex_gaussian(np.random.default_rng(6), (pts.shape[0], 16))
+    assert np.array_equal(NC.H2Workspace(a) @ x, NC.H2Workspace(b) @ x)
+    bad = dict(ra, couple_tile_order=7)
+    with pytest.raises(nca_amd.ArgumentError, match="out of range"):
+        TU.apply(b, bad)
+
+
 def test_symmetric_reuse_equals_trial_pass():
     """One tree and one point set: the reused (transposed) test factors equal
     the ones a separate trial pass computes, bit for bit, and so do the products."""

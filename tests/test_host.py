@@ -247,3 +247,26 @@ def test_julia_extension_binds_declared_symbols():
     dr = open(os.path.join(root, "directional.jl")).read()
     assert jl_fields("H2DescC", op) == c_fields("nca_h2_desc")
     assert jl_fields("DH2DescC", dr) == c_fields("nca_dh2_desc")
+
+
+def test_tuning_table_file(tmp_path):
+    """The tuning table keeps the launch choices (not the timing fields) per
+    key and reads back what it wrote; the shipped table parses and holds only
+    valid choices."""
+    from nca_amd import tuning as TU
+    t = TU.TuningTable(str(tmp_path / "t.json"))
+    choice = dict(near_k3=1, leaf_order=2, near_xcd_order=0, near_stagger=1, couple_k3=0, couple_tile_order=3,
+                  couple_xcd_order=1, near_ms=1.23456, couple_ms=0.5, candidates=40, near_ms_start=2.0)
+    t.put("gfx950|mixed|M1|N1", choice, workload="w")
+    t.save()
+    u = TU.TuningTable(t.path)
+    e = u.get("gfx950|mixed|M1|N1")
+    assert {n: e[n] for n in TU.CHOICES} == {n: choice[n] for n in TU.CHOICES}
+    assert e["workload"] == "w" and "candidates" not in e
+    shipped = TU.TuningTable()
+    ranges = dict(near_k3=2, leaf_order=3, near_xcd_order=2, near_stagger=2, couple_k3=2, couple_tile_order=4,
+                  couple_xcd_order=2)
+    for key, e in shipped.entries.items():
+        assert key.startswith("gfx950|")
+        for n, hi in ranges.items():
+            assert 0 <= e[n] < hi, (key, n)
