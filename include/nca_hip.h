@@ -193,6 +193,34 @@ int32_t nca_bottomup_side(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_p
                           const int64_t* fpidx, int64_t maxrank, double tol, int32_t sampled, int32_t method,
                           nca_factors* f, int32_t* ranks, int64_t* mrows, int64_t pivcap, int64_t* pivots,
                           int64_t* independent);
+/* testbases / trialbases(::TopDown) (src/compressors/topdown.jl:51-206) with
+ * the factorization ACA() on the device, the factors kept in f. Levels from
+ * the root down (levptr/levids: the nodes of every level, root level first);
+ * per level one launch, one workgroup per cluster: the partially pivoted ACA
+ * (compute_test_pivots!/compute_trial_pivots!, src/pivoting/pivots.jl:215-300;
+ * AdaptiveCrossApproximation.jl is not vendored: the published method
+ * restated, row pivot 0 first, pivots by maximum value, the Frobenius-norm
+ * stop |u|^2 |v|^2 <= tol^2 ||S||_F^2 with the cross kept) of the block of the
+ * cluster's points against its far field, evaluated entrywise: the points of
+ * its far nodes (farptr/farnodes, 1-based CSR into the other tree, whose
+ * layout is olo/ohi/operm) and then the column pivots of its parent. Then the
+ * factor of every cluster with pivots, X = A[rows, sigma] A[tau, sigma]^{-1},
+ * rows = a leaf's points (nestedtestbasis, nestedbasis.jl:43-57) or the
+ * children's pivots in child order (the stacked transfers of testtransfer,
+ * transfermatrices.jl:126-130), so the stores come from f as for BottomUp.
+ * transposed as nca_level_assemble (the trial side). Outputs: ranks (-1: no far
+ * field), mrows, independent as nca_bottomup_side; pivots / cpivots: the row
+ * (cluster) and column (far) pivots, 1-based point ids, of the clusters with
+ * pivots in levids order (pivcap >= their total). Limits: maxrank <= 96, a
+ * factor block of at most 2048 rows. The scratch of a launch is bounded by
+ * 4 GiB (clusters of a level are split into launches). */
+int32_t nca_topdown_side(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* self, const nca_points* other,
+                         int32_t transposed, int64_t nn, const int64_t* parent, const int64_t* childptr,
+                         const int64_t* children, const int64_t* lo, const int64_t* hi, const int64_t* perm,
+                         int64_t nn_other, const int64_t* olo, const int64_t* ohi, const int64_t* operm,
+                         const int64_t* farptr, const int64_t* farnodes, int64_t nlevels, const int64_t* levptr,
+                         const int64_t* levids, int64_t maxrank, double tol, nca_factors* f, int32_t* ranks,
+                         int64_t* mrows, int64_t pivcap, int64_t* pivots, int64_t* cpivots, int64_t* independent);
 /* Total complex entries, and all factors downloaded: the X of every
  * nca_compress_level_resident or nca_bottomup_side call in call order (each
  * call's blocks in stack order, or level by level: the X of nca_compress_level). */

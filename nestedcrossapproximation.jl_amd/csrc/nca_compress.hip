@@ -549,6 +549,291 @@ __global__ void __launch_bounds__(kWG)
 }
 
 // ---------------------------------------------------------------------------
+// TopDown (topdown.jl:51-206): one level of clusters, one workgroup per
+// cluster, the partially pivoted ACA of the block G(rows, cols) evaluated
+// entrywise (never assembled): rows = the cluster's points, cols = the points
+// of its far nodes (in CSR order) and then the column pivots of its parent.
+//   row pivot i_1 = row 0; step r: residual row R(i_r, :) into V(:, r); column
+//   pivot j_r = first maximum of |R(i_r, c)|^2 over unused columns (a zero
+//   maximum stops); V(:, r) /= R(i_r, j_r); U(:, r) = residual column j_r;
+//   S2 += |u|^2 |v|^2 + 2 Re sum_{l<r} (u_l^H u_r)(v_r v_l^H); the cross
+//   counts; stop when |u|^2 |v|^2 <= tol^2 S2 or r = min(maxrank, m, n); next
+//   row pivot = first maximum of |U(a, r)|^2 over unused rows.
+// Sums over rows (columns) are the fixed slot order (index mod 256 in
+// increasing index, halving tree), sums over l in increasing l; a thread owns
+// the rows a = t mod 256 of U and the columns c = t mod 256 of V. Scratch
+// per cluster: U (m x kmax), V (ncap x kmax), the columns' point ids, the used
+// flags. Outputs: rank (-1: no far field), the row and column pivots (point ids).
+struct PacaScratch {
+    double2* U;
+    double2* V;
+    int32_t* cols;
+    uint8_t* used;
+    const int64_t* uoff;  // per block of the launch
+    const int64_t* voff;
+    const int64_t* coff;  // columns and column flags
+    const int64_t* roff;  // row flags
+};
+
+struct PacaTree {
+    const int32_t* lo;       // self: values perm[lo, hi)
+    const int32_t* hi;
+    const int32_t* perm;
+    const int32_t* parent;   // -1 at a root
+    const int64_t* farptr;   // far nodes of the other tree
+    const int32_t* farnodes;
+    const int32_t* olo;      // other tree
+    const int32_t* ohi;
+    const int32_t* operm;
+};
+
+__device__ __forceinline__ void reduce4(double* r0, double* r1, double* r2, double* r3, double& a, double& b,
+                                        double& c, double& d) {
+    const int t = threadIdx.x;
+    r0[t] = a;
+    r1[t] = b;
+    r2[t] = c;
+    r3[t] = d;
+    __syncthreads();
+    for (int s = kWG / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            r0[t] = r0[t] + r0[t + s];
+            r1[t] = r1[t] + r1[t + s];
+            r2[t] = r2[t] + r2[t + s];
+            r3[t] = r3[t] + r3[t + s];
+        }
+        __syncthreads();
+    }
+    a = r0[0];
+    b = r1[0];
+    c = r2[0];
+    d = r3[0];
+    __syncthreads();
+}
+
+// best (value, index, payload) over the workgroup: larger value, ties to the smaller index
+__device__ __forceinline__ void reduce_best(double* rv, int* ri, double2* rp, double& v, int& i, double2& p) {
+    const int t = threadIdx.x;
+    rv[t] = v;
+    ri[t] = i;
+    rp[t] = p;
+    __syncthreads();
+    for (int s = kWG / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            const double v2 = rv[t + s];
+            const int i2 = ri[t + s];
+            if (i2 >= 0 && (ri[t] < 0 || v2 > rv[t] || (v2 == rv[t] && i2 < ri[t]))) {
+                rv[t] = v2;
+                ri[t] = i2;
+                rp[t] = rp[t + s];
+            }
+        }
+        __syncthreads();
+    }
+    v = rv[0];
+    i = ri[0];
+    p = rp[0];
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kWG)
+    k_paca_level(const double* __restrict__ rowpts, const double* __restrict__ colpts, double k, int sign,
+                 int transposed, const int32_t* __restrict__ ids, PacaTree tr, PacaScratch sc, int64_t maxrank,
+                 double tol, int32_t* __restrict__ rank, int32_t* __restrict__ rpiv, int32_t* __restrict__ cpiv) {
+    __shared__ double r0[kWG], r1[kWG], r2[kWG], r3[kWG];
+    __shared__ int ri[kWG];
+    __shared__ double2 rp[kWG];
+    __shared__ double2 ui[kSolveMaxRank], vj[kSolveMaxRank];
+    const int t = threadIdx.x;
+    const int b = blockIdx.x;
+    const int id = ids[b];
+    const int l0 = tr.lo[id];
+    const int m = tr.hi[id] - l0;
+    const int32_t* rows = tr.perm + l0;
+    int32_t* cols = sc.cols + sc.coff[b];
+    uint8_t* usedc = sc.used + sc.coff[b];
+    uint8_t* usedr = sc.used + sc.roff[b];
+    // columns: the far nodes' points, then the parent's column pivots
+    int n = 0;
+    for (int64_t q = tr.farptr[id]; q < tr.farptr[id + 1]; ++q) {
+        const int s = tr.farnodes[q];
+        const int o0 = tr.olo[s], ns = tr.ohi[s] - o0;
+        for (int a = t; a < ns; a += kWG) cols[n + a] = tr.operm[o0 + a];
+        n += ns;
+    }
+    const int pa = tr.parent[id];
+    const int rpa = pa >= 0 ? max(rank[pa], 0) : 0;
+    for (int a = t; a < rpa; a += kWG) cols[n + a] = cpiv[(int64_t)pa * maxrank + a];
+    n += rpa;
+    if (n == 0) {
+        if (t == 0) rank[id] = -1;
+        return;
+    }
+    for (int a = t; a < n; a += kWG) usedc[a] = 0;
+    for (int a = t; a < m; a += kWG) usedr[a] = 0;
+    int kmax = (int)maxrank;
+    kmax = m < kmax ? m : kmax;
+    kmax = n < kmax ? n : kmax;
+    double2* U = sc.U + sc.uoff[b];
+    double2* V = sc.V + sc.voff[b];
+    __syncthreads();
+    auto G = [&](int a, int c) {
+        const double* xr = rowpts + 3 * (int64_t)rows[a];
+        const double* xc = colpts + 3 * (int64_t)cols[c];
+        return transposed ? helmholtz(k, sign, xc[0], xc[1], xc[2], xr[0], xr[1], xr[2])
+                          : helmholtz(k, sign, xr[0], xr[1], xr[2], xc[0], xc[1], xc[2]);
+    };
+    const double tol2 = tol * tol;
+    double S2 = 0.0;
+    int i = 0, r = 0;
+    while (r < kmax) {
+        // U(i, l), l < r, from the thread that owns row i
+        if (t == (i & (kWG - 1)))
+            for (int l = 0; l < r; ++l) ui[l] = U[i + (int64_t)l * m];
+        __syncthreads();
+        double bv = -1.0;
+        int bj = -1;
+        double2 bp = make_double2(0.0, 0.0);
+        for (int c = t; c < n; c += kWG) {
+            double2 acc = G(i, c);
+            for (int l = 0; l < r; ++l) acc = csub(acc, cmul(ui[l], V[c + (int64_t)l * n]));
+            V[c + (int64_t)r * n] = acc;
+            if (!usedc[c]) {
+                const double w = cabs2(acc);
+                if (w > bv) { bv = w; bj = c; bp = acc; }
+            }
+        }
+        reduce_best(r0, ri, rp, bv, bj, bp);
+        if (!(bv > 0.0)) break;
+        const int j = bj;
+        const double d = cabs2(bp);
+        const double2 inv = make_double2(bp.x / d, -bp.y / d);
+        for (int c = t; c < n; c += kWG) V[c + (int64_t)r * n] = cmul(V[c + (int64_t)r * n], inv);
+        // V(j, l), l < r, from the thread that owns column j
+        if (t == (j & (kWG - 1)))
+            for (int l = 0; l < r; ++l) vj[l] = V[j + (int64_t)l * n];
+        __syncthreads();
+        for (int a = t; a < m; a += kWG) {
+            double2 acc = G(a, j);
+            for (int l = 0; l < r; ++l) acc = csub(acc, cmul(vj[l], U[a + (int64_t)l * m]));
+            U[a + (int64_t)r * m] = acc;
+        }
+        double nu2 = 0.0, nv2 = 0.0, z0 = 0.0, z1 = 0.0;
+        for (int a = t; a < m; a += kWG) nu2 += cabs2(U[a + (int64_t)r * m]);
+        for (int c = t; c < n; c += kWG) nv2 += cabs2(V[c + (int64_t)r * n]);
+        reduce4(r0, r1, r2, r3, nu2, nv2, z0, z1);
+        double cross = 0.0;
+        for (int l = 0; l < r; ++l) {
+            double ure = 0.0, uim = 0.0, vre = 0.0, vim = 0.0;
+            for (int a = t; a < m; a += kWG) {
+                const double2 q = cmul(make_double2(U[a + (int64_t)l * m].x, -U[a + (int64_t)l * m].y),
+                                       U[a + (int64_t)r * m]);
+                ure += q.x;
+                uim += q.y;
+            }
+            for (int c = t; c < n; c += kWG) {
+                const double2 vl = V[c + (int64_t)l * n];
+                const double2 q = cmul(V[c + (int64_t)r * n], make_double2(vl.x, -vl.y));
+                vre += q.x;
+                vim += q.y;
+            }
+            reduce4(r0, r1, r2, r3, ure, uim, vre, vim);
+            cross = cross + (ure * vre - uim * vim);
+        }
+        S2 = (S2 + nu2 * nv2) + 2.0 * cross;
+        if (t == 0) rpiv[(int64_t)id * maxrank + r] = rows[i];
+        if (t == (i & (kWG - 1))) usedr[i] = 1;
+        if (t == (j & (kWG - 1))) {
+            cpiv[(int64_t)id * maxrank + r] = cols[j];
+            usedc[j] = 1;
+        }
+        ++r;
+        if (nu2 * nv2 <= tol2 * S2) break;
+        __syncthreads();
+        double bu = -1.0;
+        int ba = -1;
+        double2 dummy = make_double2(0.0, 0.0);
+        for (int a = t; a < m; a += kWG)
+            if (!usedr[a]) {
+                const double w = cabs2(U[a + (int64_t)(r - 1) * m]);
+                if (w > bu) { bu = w; ba = a; }
+            }
+        reduce_best(r0, ri, rp, bu, ba, dummy);
+        if (ba < 0) break;
+        i = ba;
+    }
+    if (t == 0) rank[id] = r;
+}
+
+// The factors of TopDown, one workgroup per cluster with pivots:
+// X = A[rows, sigma] A[tau, sigma]^{-1}, rows = the points of a leaf or the
+// children's pivots in child order, solved as skeleton_block on the rows
+// followed by tau (the pivots of the cross matrix in the appended rows).
+__global__ void __launch_bounds__(kWG)
+    k_topdown_factors(const double* __restrict__ rowpts, const double* __restrict__ colpts, double k, int sign,
+                      int transposed, const int32_t* __restrict__ ids, const int64_t* __restrict__ xoff,
+                      const int32_t* __restrict__ childptr, const int32_t* __restrict__ children,
+                      const int32_t* __restrict__ lo, const int32_t* __restrict__ hi, const int32_t* __restrict__ perm,
+                      const int32_t* __restrict__ rank, const int32_t* __restrict__ rpiv,
+                      const int32_t* __restrict__ cpiv, int64_t maxrank, int32_t* __restrict__ mrows,
+                      int64_t* __restrict__ indep, double2* __restrict__ X, int32_t* __restrict__ err) {
+    extern __shared__ double2 dyn[];
+    __shared__ ArgRed red;
+    __shared__ int rows[kGlobMaxM + kSolveMaxRank];
+    __shared__ int sig[kSolveMaxRank];
+    __shared__ int ipiv[kSolveMaxRank], cpv[kSolveMaxRank];
+    const int t = threadIdx.x;
+    const int id = ids[blockIdx.x];
+    const int r = rank[id];
+    if (r < 0) {  // no far field: no factor
+        if (t == 0) {
+            mrows[id] = 0;
+            indep[id] = -1;
+        }
+        return;
+    }
+    int m = 0;
+    const int c0 = childptr[id], c1 = childptr[id + 1];
+    if (c0 == c1) {
+        m = hi[id] - lo[id];
+        for (int a = t; a < min(m, kGlobMaxM); a += kWG) rows[a] = perm[lo[id] + a];
+    } else {
+        for (int q = c0; q < c1; ++q) {
+            const int ch = children[q];
+            const int rc = max(rank[ch], 0);
+            for (int a = t; a < rc && m + a < kGlobMaxM; a += kWG) rows[m + a] = rpiv[(int64_t)ch * maxrank + a];
+            m += rc;
+        }
+    }
+    if (t == 0) {
+        mrows[id] = m;
+        indep[id] = -1;
+    }
+    if (m > kGlobMaxM) {
+        if (t == 0) {
+            err[0] = 1;
+            err[1] = id;
+        }
+        return;
+    }
+    for (int a = t; a < r; a += kWG) {
+        rows[m + a] = rpiv[(int64_t)id * maxrank + a];
+        sig[a] = cpiv[(int64_t)id * maxrank + a];
+    }
+    __syncthreads();
+    auto src = [&](int a, int c) {
+        const double* xr = rowpts + 3 * (int64_t)rows[a];
+        const double* xc = colpts + 3 * (int64_t)sig[c];
+        return transposed ? helmholtz(k, sign, xc[0], xc[1], xc[2], xr[0], xr[1], xr[2])
+                          : helmholtz(k, sign, xr[0], xr[1], xr[2], xc[0], xc[1], xc[2]);
+    };
+    auto rp = [&](int q) { return m + q; };
+    auto cp = [&](int q) { return q; };
+    const int64_t res = skeleton_block(src, rp, cp, m, r, dyn, ipiv, cpv, red, X + xoff[blockIdx.x]);
+    if (t == 0) indep[id] = res;
+}
+
+// ---------------------------------------------------------------------------
 // Coupling blocks of one test cluster t: the blocks of its CSR row are
 // contiguous and share r_t rows, so together they form one r_t x C column-major
 // matrix; one thread per entry, the block of a column found by a forward scan.
@@ -1463,6 +1748,224 @@ int32_t nca_bottomup_side(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_p
                                                   std::to_string(hr[id]) + " pivots were selected but only " +
                                                   std::to_string(hind[id]) + " of them are linearly independent");
         }
+    });
+}
+
+int32_t nca_topdown_side(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* self, const nca_points* other,
+                         int32_t transposed, int64_t nn, const int64_t* parent, const int64_t* childptr,
+                         const int64_t* children, const int64_t* lo, const int64_t* hi, const int64_t* perm,
+                         int64_t nn_other, const int64_t* olo, const int64_t* ohi, const int64_t* operm,
+                         const int64_t* farptr, const int64_t* farnodes, int64_t nlevels, const int64_t* levptr,
+                         const int64_t* levids, int64_t maxrank, double tol, nca_factors* f, int32_t* ranks,
+                         int64_t* mrows, int64_t pivcap, int64_t* pivots, int64_t* cpivots, int64_t* independent) {
+    return guarded([&] {
+        NCA_REQUIRE(ctx && kernel && self && other && f && parent && childptr && lo && hi && perm && olo && ohi &&
+                        operm && farptr && levptr && ranks && mrows && pivots && cpivots && independent && nn >= 0 &&
+                        nn_other >= 0 && nlevels >= 0,
+                    NCA_ERR_ARGUMENT, "nca_topdown_side: invalid argument");
+        NCA_REQUIRE(f->ctx == ctx && f->nids == nn, NCA_ERR_ARGUMENT, "nca_topdown_side: factors of another tree");
+        NCA_REQUIRE(maxrank >= 0 && maxrank <= kSolveMaxRank, NCA_ERR_LIMIT,
+                    "nca_topdown_side: maxrank above " + std::to_string(kSolveMaxRank));
+        NCA_REQUIRE(nn < (1LL << 31) && self->n < (1LL << 31) && other->n < (1LL << 31), NCA_ERR_LIMIT,
+                    "nca_topdown_side: more than 2^31 nodes or points");
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        // 0-based 32-bit copies, each value checked against [low, lim]
+        auto to32 = [](const int64_t* v, int64_t n, int64_t shift, int64_t lim, const char* what, int64_t low = 0) {
+            std::vector<int32_t> o((size_t)std::max<int64_t>(n, 1));
+            for (int64_t i = 0; i < n; ++i) {
+                const int64_t x = v[i] - shift;
+                NCA_REQUIRE(x >= low && x <= lim, NCA_ERR_ARGUMENT, std::string("nca_topdown_side: ") + what +
+                                                                          " out of range");
+                o[i] = (int32_t)x;
+            }
+            return o;
+        };
+        std::vector<int32_t> cp = to32(childptr, nn + 1, childptr[0], INT32_MAX, "childptr");
+        std::vector<int32_t> ch = to32(children, cp[nn], 1, nn - 1, "child");
+        std::vector<int32_t> lo0 = to32(lo, nn, 1, self->n, "value range"), hi0 = to32(hi, nn, 0, self->n, "value range");
+        std::vector<int32_t> pm = to32(perm, self->n, 1, self->n - 1, "perm");
+        std::vector<int32_t> pa = to32(parent, nn, 1, nn - 1, "parent", -1);
+        std::vector<int32_t> olo0 = to32(olo, nn_other, 1, other->n, "other value range"),
+                             ohi0 = to32(ohi, nn_other, 0, other->n, "other value range");
+        std::vector<int32_t> opm = to32(operm, other->n, 1, other->n - 1, "other perm");
+        const int64_t nfar = farptr[nn] - farptr[0];
+        std::vector<int32_t> fn = to32(farnodes, nfar, 1, nn_other - 1, "far node");
+        std::vector<int64_t> fp(nn + 1);
+        for (int64_t i = 0; i <= nn; ++i) fp[i] = farptr[i] - farptr[0];
+        // batches of one level: blocks whose far field may be nonempty, their
+        // scratch (U, V, columns, flags) bounded up front
+        struct Batch { int64_t first, count; };
+        std::vector<int32_t> bid;
+        std::vector<int64_t> buoff, bvoff, bcoff, broff;
+        std::vector<Batch> batches;
+        std::vector<int64_t> ncapv(nn, 0);
+        const int64_t budget = (int64_t)4 << 30;
+        int64_t umax = 0, vmax = 0, cmax = 0, fmax = 0;
+        for (int64_t L = 0; L < nlevels; ++L) {
+            int64_t u = 0, v = 0, c = 0, fl = 0;
+            int64_t start = (int64_t)bid.size();
+            std::vector<int64_t> rows_of;
+            auto close = [&] {
+                if ((int64_t)bid.size() > start) {
+                    batches.push_back({start, (int64_t)bid.size() - start});
+                    // row flags after every column flag of the batch
+                    for (int64_t q = start; q < (int64_t)bid.size(); ++q) broff[q] += c;
+                    umax = std::max(umax, u);
+                    vmax = std::max(vmax, v);
+                    cmax = std::max(cmax, c);
+                    fmax = std::max(fmax, c + fl);
+                }
+                start = (int64_t)bid.size();
+                u = v = c = fl = 0;
+            };
+            for (int64_t q = levptr[L] - levptr[0]; q < levptr[L + 1] - levptr[0]; ++q) {
+                const int64_t id = levids[q] - 1;
+                NCA_REQUIRE(id >= 0 && id < nn, NCA_ERR_ARGUMENT, "nca_topdown_side: level node out of range");
+                int64_t ncap = pa[id] >= 0 ? maxrank : 0;
+                for (int64_t e = fp[id]; e < fp[id + 1]; ++e) ncap += ohi0[fn[e]] - olo0[fn[e]];
+                ncapv[id] = ncap;
+                if (ncap == 0) continue;
+                const int64_t m = hi0[id] - lo0[id];
+                const int64_t kc = std::min<int64_t>(std::min<int64_t>(maxrank, m), ncap);
+                const int64_t bytes = (m + ncap) * kc * 16 + ncap * 5 + m;
+                if (u + v > 0 && (u + v) * 16 + (c * 5 + fl) + bytes > budget) close();
+                bid.push_back((int32_t)id);
+                buoff.push_back(u);
+                bvoff.push_back(v);
+                bcoff.push_back(c);
+                broff.push_back(fl);
+                u += m * kc;
+                v += ncap * kc;
+                c += ncap;
+                fl += m;
+            }
+            close();
+        }
+        DevBuf<int32_t> d_lo, d_hi, d_pm, d_pa, d_cp, d_ch, d_olo, d_ohi, d_opm, d_fn, d_bid, d_rank, d_rp, d_cpv,
+            d_m, d_err;
+        DevBuf<int64_t> d_fp, d_uo, d_vo, d_co, d_ro, d_indep;
+        d_lo.upload(lo0, st);
+        d_hi.upload(hi0, st);
+        d_pm.upload(pm, st);
+        d_pa.upload(pa, st);
+        d_cp.upload(cp, st);
+        d_ch.upload(ch, st);
+        d_olo.upload(olo0, st);
+        d_ohi.upload(ohi0, st);
+        d_opm.upload(opm, st);
+        d_fn.upload(fn, st);
+        d_fp.upload(fp, st);
+        if (!bid.empty()) {
+            d_bid.upload(bid, st);
+            d_uo.upload(buoff, st);
+            d_vo.upload(bvoff, st);
+            d_co.upload(bcoff, st);
+            d_ro.upload(broff, st);
+        }
+        const size_t nrk = (size_t)std::max<int64_t>(nn * maxrank, 1);
+        d_rank.alloc((size_t)std::max<int64_t>(nn, 1));
+        d_rp.alloc(nrk);
+        d_cpv.alloc(nrk);
+        NCA_HIP(hipMemsetAsync(d_rank.p, 0xff, sizeof(int32_t) * d_rank.n, st));
+        NCA_HIP(hipMemsetAsync(d_rp.p, 0, sizeof(int32_t) * nrk, st));
+        NCA_HIP(hipMemsetAsync(d_cpv.p, 0, sizeof(int32_t) * nrk, st));
+        DevBuf<double2> d_U((size_t)std::max<int64_t>(umax, 1)), d_V((size_t)std::max<int64_t>(vmax, 1));
+        DevBuf<int32_t> d_cols((size_t)std::max<int64_t>(cmax, 1));
+        DevBuf<uint8_t> d_used((size_t)std::max<int64_t>(fmax, 1));
+        PacaTree tr{d_lo.p, d_hi.p, d_pm.p, d_pa.p, d_fp.p, d_fn.p, d_olo.p, d_ohi.p, d_opm.p};
+        for (const Batch& B : batches) {
+            PacaScratch sc{d_U.p, d_V.p, d_cols.p, d_used.p, d_uo.p + B.first, d_vo.p + B.first, d_co.p + B.first,
+                           d_ro.p + B.first};
+            hipLaunchKernelGGL(k_paca_level, dim3((unsigned)B.count), dim3(kWG), 0, st, self->xyz.p, other->xyz.p,
+                               kernel->wavenumber, (int)kernel->sign, (int)(transposed != 0), d_bid.p + B.first, tr,
+                               sc, maxrank, tol, d_rank.p, d_rp.p, d_cpv.p);
+            NCA_HIP(hipGetLastError());
+        }
+        // factors of the blocks that may hold pivots, in level order
+        std::vector<int64_t> rb(nn, 0), fx(bid.size());
+        for (int64_t q = 0; q < (int64_t)bid.size(); ++q) {
+            const int32_t id = bid[q];
+            rb[id] = std::min<int64_t>(maxrank, hi0[id] - lo0[id]);
+        }
+        int64_t xtot = 0;
+        for (int64_t q = 0; q < (int64_t)bid.size(); ++q) {
+            const int32_t id = bid[q];
+            int64_t mcap = 0;
+            if (cp[id] == cp[id + 1]) mcap = hi0[id] - lo0[id];
+            else
+                for (int32_t e = cp[id]; e < cp[id + 1]; ++e) mcap += rb[ch[e]];
+            fx[q] = xtot;
+            xtot += mcap * rb[id];
+        }
+        DevBuf<int64_t> d_fx;
+        if (!fx.empty()) d_fx.upload(fx, st);
+        d_m.alloc((size_t)std::max<int64_t>(nn, 1));
+        d_indep.alloc((size_t)std::max<int64_t>(nn, 1));
+        d_err.alloc(2);
+        NCA_HIP(hipMemsetAsync(d_m.p, 0, sizeof(int32_t) * d_m.n, st));
+        NCA_HIP(hipMemsetAsync(d_indep.p, 0xff, sizeof(int64_t) * d_indep.n, st));
+        NCA_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int32_t) * 2, st));
+        DevBuf<double2> d_X((size_t)std::max<int64_t>(xtot, 1));
+        if (!bid.empty()) {
+            const size_t lds = sizeof(double2) * (size_t)std::max<int64_t>(maxrank * maxrank, 1);
+            NCA_HIP(hipFuncSetAttribute((const void*)k_topdown_factors, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds));
+            hipLaunchKernelGGL(k_topdown_factors, dim3((unsigned)bid.size()), dim3(kWG), lds, st, self->xyz.p,
+                               other->xyz.p, kernel->wavenumber, (int)kernel->sign, (int)(transposed != 0), d_bid.p,
+                               d_fx.p, d_cp.p, d_ch.p, d_lo.p, d_hi.p, d_pm.p, d_rank.p, d_rp.p, d_cpv.p, maxrank,
+                               d_m.p, d_indep.p, d_X.p, d_err.p);
+            NCA_HIP(hipGetLastError());
+        }
+        std::vector<int32_t> hr(d_rank.n), hm(d_m.n), hrp(nrk), hcp(nrk), herr(2);
+        std::vector<int64_t> hind(d_indep.n);
+        d2h(ctx, hr.data(), d_rank.p, sizeof(int32_t) * hr.size());
+        d2h(ctx, hm.data(), d_m.p, sizeof(int32_t) * hm.size());
+        d2h(ctx, hrp.data(), d_rp.p, sizeof(int32_t) * nrk);
+        d2h(ctx, hcp.data(), d_cpv.p, sizeof(int32_t) * nrk);
+        d2h(ctx, hind.data(), d_indep.p, sizeof(int64_t) * hind.size());
+        d2h(ctx, herr.data(), d_err.p, sizeof(int32_t) * 2);
+        NCA_REQUIRE(herr[0] == 0, NCA_ERR_LIMIT,
+                    "nca_topdown_side: block " + std::to_string(herr[1] + 1) + " has more than " +
+                        std::to_string(kGlobMaxM) + " rows");
+        int64_t npiv = 0;
+        for (int32_t id : bid) npiv += std::max(hr[id], 0);
+        NCA_REQUIRE(npiv <= pivcap, NCA_ERR_DIMENSION,
+                    "nca_topdown_side: the pivots need " + std::to_string(npiv) + " entries, pivcap is " +
+                        std::to_string(pivcap));
+        for (int32_t id : bid)
+            NCA_REQUIRE(f->chunk[id] < 0, NCA_ERR_ARGUMENT, "nca_topdown_side: a block has a factor already");
+        const int32_t c = (int32_t)f->chunks.size();
+        std::vector<int32_t> order;
+        int64_t packed = 0, o = 0;
+        for (int64_t q = 0; q < (int64_t)bid.size(); ++q) {
+            const int32_t id = bid[q];
+            if (hr[id] < 0) continue;
+            f->chunk[id] = c;
+            f->off[id] = fx[q];
+            f->m[id] = hm[id];
+            f->rank[id] = hr[id];
+            packed += (int64_t)hm[id] * hr[id];
+            order.push_back(id);
+            for (int32_t k2 = 0; k2 < hr[id]; ++k2) {
+                pivots[o] = (int64_t)hrp[(int64_t)id * maxrank + k2] + 1;
+                cpivots[o] = (int64_t)hcp[(int64_t)id * maxrank + k2] + 1;
+                ++o;
+            }
+        }
+        f->chunks.push_back(std::move(d_X));
+        f->chunk_size.push_back(packed);
+        f->chunk_ids.push_back(std::move(order));
+        for (int64_t i = 0; i < nn; ++i) {
+            ranks[i] = hr[i];
+            mrows[i] = hr[i] >= 0 ? hm[i] : 0;
+            independent[i] = hr[i] >= 0 ? hind[i] : -1;
+        }
+        for (int32_t id : bid)
+            if (hr[id] >= 0 && hind[id] >= 0)
+                throw Error(NCA_ERR_SINGULAR, "SingularCrossMatrixError: block " + std::to_string(id + 1) + ": " +
+                                                  std::to_string(hr[id]) + " pivots were selected but only " +
+                                                  std::to_string(hind[id]) + " of them are linearly independent");
     });
 }
 

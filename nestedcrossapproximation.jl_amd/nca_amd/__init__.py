@@ -9,7 +9,7 @@ from ._native import hip_lib, tree_lib, NativeLibraryMissing
 from .runtime import Context, default_context, HelmholtzKernel, HIPKernelMatrix
 from .trees import (TwoNTree, KMeansTree, BlockTree, interactions, fardata, farfield, farpivots, FarData,
                     Partition, forest, forest_partition)
-from .compressors import (ACA, IACA, BatchedBottomUp, GeometricPivoting, assemble_level, testbases, trialbases,
+from .compressors import (ACA, IACA, TopDown, BatchedBottomUp, GeometricPivoting, assemble_level, testbases, trialbases,
                           assemble_couplingstore, concat_ranges)
 from .nca import PetrovGalerkinNCA, PetrovGalerkinNCACore, H2Workspace, H2Graph, BatchedNCA, mul, transpose, \
     adjoint, ConcurrentWorkspaceError, h2desc

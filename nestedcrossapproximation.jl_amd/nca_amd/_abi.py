@@ -106,6 +106,8 @@ SIGNATURES = {
     "nca_factors_download": (I32, [P, P, P]),
     "nca_bottomup_side": (I32, [P, P, P, P, I32, I64, P, P, P, P, P, I64, P, P, P, P, I64, D, I32, I32, P, P, P, I64,
                                 P, P]),
+    "nca_topdown_side": (I32, [P, P, P, P, I32, I64, P, P, P, P, P, P, I64, P, P, P, P, P, I64, P, P, I64, D, P, P,
+                               P, I64, P, P, P]),
     "nca_h2_create_resident": (I32, [P, P, P, P, P, P, I32, P, P, PP]),
     "nca_dh2_create_resident": (I32, [P, P, P, P, P, P, I32, P, P, PP]),
     "nca_tree_twon": (I32, [P, P, I64, D, P, PP]),

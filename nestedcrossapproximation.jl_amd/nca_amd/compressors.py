@@ -45,6 +45,23 @@ class ACA:
         self.tol = float(tol)
 
 
+class TopDown:
+    """TopDown(factorization=ACA(), representor=None) (topdown.jl:26-47): the
+    nested bases from the root to the leaves, each cluster factorized against
+    the points of its far nodes and the column pivots of its parent with the
+    partially pivoted ACA (nca_topdown_side). Only the ACA factorization and no
+    representor are taken on the device."""
+
+    def __init__(self, factorization=None, representor=None):
+        factorization = ACA() if factorization is None else factorization
+        if not isinstance(factorization, ACA):
+            raise ValueError("TopDown on the device takes ACA() (the reference's default factorization)")
+        if representor is not None:
+            raise ValueError("TopDown on the device takes no representor")
+        self.factorization = factorization
+        self.representor = representor
+
+
 class IACA:
     """IACA(MaximumValue(), TreeMimicryPivoting(...), FNormExtrapolator(tol))
     as BottomUp takes it (bottomup.jl:214-424, pivots.jl:215-364): the column
@@ -495,6 +512,67 @@ def _bottomup_fused(compressor, farmatrix, selftree, side, maxrank, farpivots, o
             out.add_level(active, m, r, piv[p0:p1] - 1, xoff, None)
             p0 = p1
             out.level_transfer_nodes[lev] = nodes[~leaf[nodes] & out.assigned[nodes]].tolist()
+    return out
+
+
+def topdown_side(compressor, farmatrix, tree, fd, side, maxrank=50):
+    """testbases (side 0) / trialbases (side 1) of TopDown: LevelFactors with
+    the factors on the device (pool), the row pivots as pivots and the column
+    (far) pivots in colpivots (0-based CSR). fd: the far data of this side."""
+    selftree = tree.testtree if side == 0 else tree.trialtree
+    othertree = tree.trialtree if side == 0 else tree.testtree
+    nn = selftree.nnodes
+    leaf = selftree.leafmask()
+    levs = sorted(selftree.levels().tolist())
+    levnodes = [selftree.levelnodes(lev) for lev in levs]
+    levptr = np.zeros(len(levs) + 1, np.int64)
+    np.cumsum([len(x) for x in levnodes], out=levptr[1:])
+    levids = np.concatenate(levnodes + [np.zeros(0, np.int64)]).astype(np.int64)
+    selfpts, otherpts = (farmatrix.test, farmatrix.trial) if side == 0 else (farmatrix.trial, farmatrix.test)
+    i64 = lambda a: np.ascontiguousarray(a, np.int64) if len(a) else np.zeros(1, np.int64)
+    arr = dict(pa=i64(selftree.parent + 1), cp=i64(selftree.childptr + 1), ch=i64(selftree.children + 1),
+               lo=i64(selftree.lo + 1), hi=i64(selftree.hi), perm=i64(selftree.perm + 1),
+               olo=i64(othertree.lo + 1), ohi=i64(othertree.hi), operm=i64(othertree.perm + 1),
+               fptr=i64(fd.nodetofarnodesptr + 1), fnod=i64(fd.farnodes + 1), levptr=i64(levptr + 1),
+               levids=i64(levids + 1))
+    out = LevelFactors(nn)
+    out.pool = FactorPool(farmatrix.ctx, nn)
+    ranks = np.zeros(max(nn, 1), np.int32)
+    mrows = np.zeros(max(nn, 1), np.int64)
+    pivcap = int(np.minimum(selftree.hi - selftree.lo, maxrank).sum())
+    piv = np.empty(max(pivcap, 1), np.int64)
+    cpiv = np.empty(max(pivcap, 1), np.int64)
+    indep = np.zeros(max(nn, 1), np.int64)
+    kern = farmatrix.kernel.abi()
+    call(hip_lib().nca_topdown_side(
+        farmatrix.ctx.handle, ctypes.byref(kern), selfpts.handle, otherpts.handle, int(side == 1), nn,
+        _p(arr["pa"]), _p(arr["cp"]), _p(arr["ch"]), _p(arr["lo"]), _p(arr["hi"]), _p(arr["perm"]),
+        othertree.nnodes, _p(arr["olo"]), _p(arr["ohi"]), _p(arr["operm"]), _p(arr["fptr"]), _p(arr["fnod"]),
+        len(levs), _p(arr["levptr"]), _p(arr["levids"]), int(maxrank), float(compressor.factorization.tol),
+        out.pool.handle, _p(ranks), _p(mrows), pivcap, _p(piv), _p(cpiv), _p(indep)))
+    p0 = 0
+    cranks = np.zeros(nn, np.int64)
+    cparts = []
+    for lev, nodes in zip(levs, levnodes):
+        active = nodes[ranks[nodes] >= 0]
+        if len(active):
+            r = ranks[active].astype(np.int64)
+            m = mrows[active]
+            xoff = np.zeros(len(active) + 1, np.int64)
+            np.cumsum(m * r, out=xoff[1:])
+            p1 = p0 + int(r.sum())
+            out.add_level(active, m, r, piv[p0:p1] - 1, xoff, None)
+            cparts.append((active, r, cpiv[p0:p1] - 1))
+            cranks[active] = r
+            p0 = p1
+            out.level_transfer_nodes[lev] = nodes[~leaf[nodes] & out.assigned[nodes]].tolist()
+    # column pivots by node (0-based CSR)
+    cptr = np.zeros(nn + 1, np.int64)
+    np.cumsum(cranks, out=cptr[1:])
+    cidx = np.zeros(int(cptr[-1]), np.int64)
+    for active, r, c in cparts:
+        cidx[concat_ranges(cptr[active], r)] = c
+    out.colpivots = (cptr, cidx)
     return out
 
 

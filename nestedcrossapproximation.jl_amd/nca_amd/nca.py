@@ -97,35 +97,55 @@ class PetrovGalerkinNCACore:
         timings["partition"] = time.perf_counter() - t0
         self.farmatrix = HIPKernelMatrix(kernel, testpoints, trialpoints, self.ctx)
         t0 = time.perf_counter()
-        fp_test = testcompressor.pivotselector.select(tree.testtree, tree.trialtree, self.testfardata, maxrank,
-                                                      self.farmatrix.trial)
-        ts, ss = testcompressor.pivotselector, trialcompressor.pivotselector
         tf, sf = self.testfardata, self.trialfardata
-        if (tree.testtree is tree.trialtree and type(ts) is type(ss) and vars(ts) == vars(ss)
-                and np.array_equal(tf.nodetofarnodesptr, sf.nodetofarnodesptr)
-                and np.array_equal(tf.farnodes, sf.farnodes)):
-            fp_trial = fp_test  # one tree, symmetric partition: the same selection
-        else:
-            fp_trial = trialcompressor.pivotselector.select(tree.trialtree, tree.testtree, self.trialfardata,
-                                                            maxrank, self.farmatrix.test)
+        one_side = (tree.testtree is tree.trialtree and np.array_equal(tf.nodetofarnodesptr, sf.nodetofarnodesptr)
+                    and np.array_equal(tf.farnodes, sf.farnodes))
+        td_test, td_trial = isinstance(testcompressor, _cmp.TopDown), isinstance(trialcompressor, _cmp.TopDown)
+        # Phase 1 (far pivots) for the BatchedBottomUp sides
+        fp_test = fp_trial = None
+        if not td_test:
+            fp_test = testcompressor.pivotselector.select(tree.testtree, tree.trialtree, tf, maxrank,
+                                                          self.farmatrix.trial)
+        if not td_trial:
+            ts, ss = trialcompressor.pivotselector, getattr(testcompressor, "pivotselector", None)
+            if fp_test is not None and one_side and type(ts) is type(ss) and vars(ts) == vars(ss):
+                fp_trial = fp_test  # one tree, symmetric partition: the same selection
+            else:
+                fp_trial = ts.select(tree.trialtree, tree.testtree, sf, maxrank, self.farmatrix.test)
         timings["phase1"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         res = self.resident
-        self.testbasis, self.testtransfer, self.testfactors = _cmp.testbases(
-            testcompressor, self.farmatrix, tree, self.testfardata, maxrank, fp_test, res)
+        if td_test:
+            self.testfactors = _cmp.topdown_side(testcompressor, self.farmatrix, tree, tf, 0, maxrank)
+            self.testbasis, self.testtransfer = _cmp.stores_from_factors(self.testfactors, tree.testtree, 0,
+                                                                         blocks=not res)
+        else:
+            self.testbasis, self.testtransfer, self.testfactors = _cmp.testbases(
+                testcompressor, self.farmatrix, tree, tf, maxrank, fp_test, res)
         # One tree, one point set, one partition and the same compressor: the
         # trial blocks transpose(A[cols, rows]) hold the test blocks' numbers
         # (the kernel is symmetric), so the trial pass would repeat the test
         # pass bit for bit; its factors are reused and packed transposed.
-        self.symmetric = bool(reuse_symmetric and fp_trial is fp_test and _same_points(testpoints, trialpoints)
-                              and vars(testcompressor.compressor) == vars(trialcompressor.compressor))
+        if td_test and td_trial:
+            same_cfg = vars(testcompressor.factorization) == vars(trialcompressor.factorization)
+            same_inputs = one_side
+        elif not td_test and not td_trial:
+            same_cfg = vars(testcompressor.compressor) == vars(trialcompressor.compressor)
+            same_inputs = fp_trial is fp_test
+        else:
+            same_cfg = same_inputs = False
+        self.symmetric = bool(reuse_symmetric and same_inputs and same_cfg and _same_points(testpoints, trialpoints))
         if self.symmetric:
             self.trialfactors = self.testfactors
             self.trialbasis, self.trialtransfer = _cmp.stores_from_factors(self.testfactors, tree.trialtree, 1,
                                                                            blocks=not res)
+        elif td_trial:
+            self.trialfactors = _cmp.topdown_side(trialcompressor, self.farmatrix, tree, sf, 1, maxrank)
+            self.trialbasis, self.trialtransfer = _cmp.stores_from_factors(self.trialfactors, tree.trialtree, 1,
+                                                                           blocks=not res)
         else:
             self.trialbasis, self.trialtransfer, self.trialfactors = _cmp.trialbases(
-                trialcompressor, self.farmatrix, tree, self.trialfardata, maxrank, fp_trial, res)
+                trialcompressor, self.farmatrix, tree, sf, maxrank, fp_trial, res)
         self.testpivots = self.testfactors.pivot_csr()
         self.trialpivots = self.trialfactors.pivot_csr()
         self.couplingstore = _cmp.assemble_couplingstore(self.farmatrix, self.testpivots, self.trialpivots,

@@ -525,6 +525,261 @@ int64_t orc_bases(int32_t side, double k, int32_t sign, const double* testpts, c
 }
 
 /* ------------------------------------------------------------------------- */
+/* Partially pivoted ACA (the ACA() factorization of TopDown,
+ * src/compressors/topdown.jl:51-127 through compute_test_pivots!
+ * src/pivoting/pivots.jl:215-257; AdaptiveCrossApproximation.jl is not
+ * vendored, so this is the published method restated: Bebendorf's adaptive
+ * cross approximation with pivots by maximum value and the Frobenius-norm
+ * stop). Entries are evaluated, never assembled: entry (a, c) of the block is
+ * G(row a, col c), or G(col c, row a) with transposed != 0.
+ *   row pivot i_1 = row 0; step r: residual row R(i_r, :) = A(i_r, :) -
+ *   sum_l U(i_r, l) V(:, l); column pivot j_r = first maximum of |R(i_r, c)|^2
+ *   over unused columns (a zero maximum stops); V(:, r) = R(i_r, :) / R(i_r, j_r);
+ *   U(:, r) = A(:, j_r) - sum_l V(j_r, l) U(:, l); S2 += |u|^2 |v|^2 +
+ *   2 Re sum_{l<r} (u_l^H u_r)(v_r v_l^H) (the squared Frobenius norm of the
+ *   approximant); the cross counts, and the factorization stops when
+ *   |u|^2 |v|^2 <= tol^2 S2 or r = min(maxrank, m, n); else the next row pivot is
+ *   the first maximum of |U(a, r)|^2 over unused rows.
+ * Sums over rows or columns use the fixed slot order (index mod 256, halving
+ * tree); sums over l run in increasing l. rowpiv/colpiv: 1-based positions.
+ * U (m x kmax) and V (n x kmax) are written when not NULL. */
+int64_t orc_paca(double k, int32_t sign, int32_t transposed, const double* rowpts, const double* colpts,
+                 int64_t m, const int64_t* rows, int64_t n, const int64_t* cols, int64_t maxrank, double tol,
+                 int64_t* rowpiv, int64_t* colpiv, double* Uout, double* Vout) {
+    int64_t kmax = maxrank;
+    if (m < kmax) kmax = m;
+    if (n < kmax) kmax = n;
+    if (kmax <= 0) return 0;
+    cplx* U = (cplx*)malloc(sizeof(cplx) * (size_t)(m * kmax));
+    cplx* V = (cplx*)malloc(sizeof(cplx) * (size_t)(n * kmax));
+    char* usedr = (char*)calloc((size_t)m, 1);
+    char* usedc = (char*)calloc((size_t)n, 1);
+    double slots[ORC_SLOTS], slots2[ORC_SLOTS];
+#define ORC_G(a, c)                                                                          \
+    (transposed ? entry(k, sign, colpts + 3 * (cols[c] - 1), rowpts + 3 * (rows[a] - 1))     \
+                : entry(k, sign, rowpts + 3 * (rows[a] - 1), colpts + 3 * (cols[c] - 1)))
+    const double tol2 = tol * tol;
+    double S2 = 0.0;
+    int64_t i = 0, r = 0;
+    while (r < kmax) {
+        for (int64_t c = 0; c < n; ++c) {
+            cplx acc = ORC_G(i, c);
+            for (int64_t l = 0; l < r; ++l) acc = csub(acc, cmul(U[i + l * m], V[c + l * n]));
+            V[c + r * n] = acc;
+        }
+        double bv = -1.0;
+        int64_t j = -1;
+        for (int64_t c = 0; c < n; ++c)
+            if (!usedc[c]) {
+                double w = cabs2(V[c + r * n]);
+                if (w > bv) {
+                    bv = w;
+                    j = c;
+                }
+            }
+        if (!(bv > 0.0)) break;
+        cplx pv = V[j + r * n];
+        double d = cabs2(pv);
+        cplx inv = {pv.re / d, -pv.im / d};
+        for (int64_t c = 0; c < n; ++c) V[c + r * n] = cmul(V[c + r * n], inv);
+        for (int64_t a = 0; a < m; ++a) {
+            cplx acc = ORC_G(a, j);
+            for (int64_t l = 0; l < r; ++l) acc = csub(acc, cmul(V[j + l * n], U[a + l * m]));
+            U[a + r * m] = acc;
+        }
+        memset(slots, 0, sizeof(slots));
+        for (int64_t a = 0; a < m; ++a) slots[a % ORC_SLOTS] += cabs2(U[a + r * m]);
+        double nu2 = sum_slots(slots);
+        memset(slots, 0, sizeof(slots));
+        for (int64_t c = 0; c < n; ++c) slots[c % ORC_SLOTS] += cabs2(V[c + r * n]);
+        double nv2 = sum_slots(slots);
+        double cross = 0.0;
+        for (int64_t l = 0; l < r; ++l) {
+            memset(slots, 0, sizeof(slots));
+            memset(slots2, 0, sizeof(slots2));
+            for (int64_t a = 0; a < m; ++a) {
+                cplx q = cmul(cconj(U[a + l * m]), U[a + r * m]);
+                slots[a % ORC_SLOTS] += q.re;
+                slots2[a % ORC_SLOTS] += q.im;
+            }
+            cplx du = {sum_slots(slots), sum_slots(slots2)};
+            memset(slots, 0, sizeof(slots));
+            memset(slots2, 0, sizeof(slots2));
+            for (int64_t c = 0; c < n; ++c) {
+                cplx q = cmul(V[c + r * n], cconj(V[c + l * n]));
+                slots[c % ORC_SLOTS] += q.re;
+                slots2[c % ORC_SLOTS] += q.im;
+            }
+            cplx dv = {sum_slots(slots), sum_slots(slots2)};
+            cross = cross + (du.re * dv.re - du.im * dv.im);
+        }
+        S2 = (S2 + nu2 * nv2) + 2.0 * cross;
+        rowpiv[r] = i + 1;
+        colpiv[r] = j + 1;
+        usedr[i] = 1;
+        usedc[j] = 1;
+        ++r;
+        if (nu2 * nv2 <= tol2 * S2) break;
+        bv = -1.0;
+        i = -1;
+        for (int64_t a = 0; a < m; ++a)
+            if (!usedr[a]) {
+                double w = cabs2(U[a + (r - 1) * m]);
+                if (w > bv) {
+                    bv = w;
+                    i = a;
+                }
+            }
+        if (i < 0) break;
+    }
+#undef ORC_G
+    if (Uout) memcpy(Uout, U, sizeof(cplx) * (size_t)(m * r));
+    if (Vout)
+        for (int64_t l = 0; l < r; ++l) memcpy((cplx*)Vout + l * n, V + l * n, sizeof(cplx) * (size_t)n);
+    free(U);
+    free(V);
+    free(usedr);
+    free(usedc);
+    return r;
+}
+
+/* testbases/trialbases(::TopDown) (topdown.jl:51-206): levels from the root
+ * down; a cluster's columns are the points of its far nodes (CSR order) and
+ * then the column pivots of its parent; rows are its own points; the
+ * partially pivoted ACA picks the row pivots (the cluster's pivots) and the
+ * column pivots (handed to the children). Then the factor of every cluster
+ * with pivots, X = A[rows, sigma] A[tau, sigma]^{-1} with rows = the points of a
+ * leaf (nestedtestbasis, nestedbasis.jl:43-57) or the children's pivots in
+ * child order (the stacked transfers A[tau_c, sigma] A[tau, sigma]^{-1},
+ * testtransfer transfermatrices.jl:126-130), solved as orc_skeleton_solve on
+ * [A[rows, sigma]; A[tau, sigma]] with the pivots in the appended rows.
+ * side 0 = test (rows: test points), 1 = trial (rows: trial points, entries
+ * transposed). Self tree layout as orc_bases; other tree: operm/olo/ohi;
+ * farptr/farnodes: 1-based CSR of the far nodes (of the other tree) of every
+ * node. Outputs: pivptr/pivots (row pivots, 1-based point ids; capacity
+ * nn*maxrank), cpivptr/cpivots (column pivots), xoffsets/X as orc_bases.
+ * Returns 0, -(node) of the first singular cross matrix, or 1 when X is too small. */
+int64_t orc_topdown(int32_t side, double k, int32_t sign, const double* testpts, const double* trialpts,
+                    int64_t nn, const int64_t* perm, const int64_t* nodelo, const int64_t* nodehi,
+                    const int64_t* level, const int64_t* parent, const int64_t* childptr, const int64_t* children,
+                    const int64_t* operm, const int64_t* olo, const int64_t* ohi, const int64_t* farptr,
+                    const int64_t* farnodes, int64_t maxrank, double tol, int64_t* pivptr, int64_t* pivots,
+                    int64_t* cpivptr, int64_t* cpivots, int64_t* xoffsets, double* Xout, int64_t xcap) {
+    const double* selfpts = side == 0 ? testpts : trialpts;
+    const double* otherpts = side == 0 ? trialpts : testpts;
+    int64_t* rank = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nn > 0 ? nn : 1));
+    int64_t* rp = (int64_t*)calloc((size_t)(nn * maxrank > 0 ? nn * maxrank : 1), sizeof(int64_t));
+    int64_t* cp = (int64_t*)calloc((size_t)(nn * maxrank > 0 ? nn * maxrank : 1), sizeof(int64_t));
+    int64_t maxlev = 0, minlev = INT64_MAX;
+    for (int64_t t = 0; t < nn; ++t) {
+        rank[t] = -1;
+        if (level[t] > maxlev) maxlev = level[t];
+        if (level[t] < minlev) minlev = level[t];
+    }
+    for (int64_t lev = minlev; lev <= maxlev; ++lev) {
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int64_t t = 0; t < nn; ++t) {
+            if (level[t] != lev) continue;
+            const int64_t pa = parent[t] - 1;
+            const int64_t rpa = pa >= 0 && rank[pa] > 0 ? rank[pa] : 0;
+            int64_t n = rpa;
+            for (int64_t q = farptr[t] - 1; q < farptr[t + 1] - 1; ++q) {
+                const int64_t s = farnodes[q] - 1;
+                n += ohi[s] - olo[s] + 1;
+            }
+            if (n == 0) continue;
+            int64_t* cols = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+            int64_t o = 0;
+            for (int64_t q = farptr[t] - 1; q < farptr[t + 1] - 1; ++q) {
+                const int64_t s = farnodes[q] - 1;
+                for (int64_t a = olo[s] - 1; a < ohi[s]; ++a) cols[o++] = operm[a];
+            }
+            for (int64_t a = 0; a < rpa; ++a) cols[o++] = cp[pa * maxrank + a];
+            const int64_t m = nodehi[t] - nodelo[t] + 1;
+            const int64_t* rows = perm + nodelo[t] - 1;
+            int64_t rpv[1024], cpv[1024];
+            int64_t r = orc_paca(k, sign, side, selfpts, otherpts, m, rows, n, cols, maxrank > 1024 ? 1024 : maxrank,
+                                 tol, rpv, cpv, NULL, NULL);
+            for (int64_t a = 0; a < r; ++a) {
+                rp[t * maxrank + a] = rows[rpv[a] - 1];
+                cp[t * maxrank + a] = cols[cpv[a] - 1];
+            }
+            rank[t] = r;
+            free(cols);
+        }
+    }
+    /* factors */
+    int64_t ret = 0;
+    xoffsets[0] = 0;
+    pivptr[0] = 1;
+    cpivptr[0] = 1;
+    for (int64_t t = 0; t < nn; ++t) {
+        const int64_t r = rank[t] > 0 ? rank[t] : 0;
+        int64_t m = 0;
+        if (rank[t] >= 0) {
+            if (childptr[t + 1] == childptr[t]) m = nodehi[t] - nodelo[t] + 1;
+            else
+                for (int64_t c = childptr[t] - 1; c < childptr[t + 1] - 1; ++c) {
+                    const int64_t ch = children[c] - 1;
+                    if (rank[ch] > 0) m += rank[ch];
+                }
+        }
+        xoffsets[t + 1] = xoffsets[t] + m * r;
+        pivptr[t + 1] = pivptr[t] + r;
+        cpivptr[t + 1] = cpivptr[t] + r;
+        for (int64_t a = 0; a < r; ++a) {
+            pivots[pivptr[t] - 1 + a] = rp[t * maxrank + a];
+            cpivots[cpivptr[t] - 1 + a] = cp[t * maxrank + a];
+        }
+    }
+    if (xoffsets[nn] > xcap) ret = 1;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t t = 0; t < nn; ++t) {
+        const int64_t r = rank[t] > 0 ? rank[t] : 0;
+        if (r == 0 || ret == 1) continue;
+        const int64_t m = (xoffsets[t + 1] - xoffsets[t]) / r;
+        int64_t* rows = (int64_t*)malloc(sizeof(int64_t) * (size_t)(m + r));
+        if (childptr[t + 1] == childptr[t]) {
+            for (int64_t a = 0; a < m; ++a) rows[a] = perm[nodelo[t] - 1 + a];
+        } else {
+            int64_t o = 0;
+            for (int64_t c = childptr[t] - 1; c < childptr[t + 1] - 1; ++c) {
+                const int64_t ch = children[c] - 1;
+                for (int64_t a = 0; a < (rank[ch] > 0 ? rank[ch] : 0); ++a) rows[o++] = rp[ch * maxrank + a];
+            }
+        }
+        for (int64_t a = 0; a < r; ++a) rows[m + a] = rp[t * maxrank + a];
+        const int64_t me = m + r;
+        cplx* L = (cplx*)malloc(sizeof(cplx) * (size_t)(me * r));
+        for (int64_t c = 0; c < r; ++c)
+            for (int64_t a = 0; a < me; ++a) {
+                const double* x = selfpts + 3 * (rows[a] - 1);
+                const double* y = otherpts + 3 * (cp[t * maxrank + c] - 1);
+                L[a + c * me] = side == 0 ? entry(k, sign, x, y) : entry(k, sign, y, x);
+            }
+        int64_t* piv = (int64_t*)malloc(sizeof(int64_t) * (size_t)r);
+        for (int64_t a = 0; a < r; ++a) piv[a] = m + a + 1;
+        cplx* Xe = (cplx*)malloc(sizeof(cplx) * (size_t)(me * r));
+        const int64_t st = orc_skeleton_solve((const double*)L, me, r, piv, (double*)Xe);
+        if (st >= 0) {
+#pragma omp critical
+            { if (ret == 0 || -ret > t + 1) ret = -(t + 1); }
+        }
+        cplx* X = (cplx*)Xout + xoffsets[t];
+        for (int64_t c = 0; c < r; ++c)
+            for (int64_t a = 0; a < m; ++a) X[a + c * m] = Xe[a + c * me];
+        free(rows);
+        free(L);
+        free(piv);
+        free(Xe);
+    }
+    free(rank);
+    free(rp);
+    free(cp);
+    return ret;
+}
+
+/* ------------------------------------------------------------------------- */
 /* assemble_couplingstore (couplingmatrices.jl:11-53): block i of the test CSR
  * row of t is A[testpivots[t], trialpivots[farnodes[i]]]; empty sides give
  * empty blocks. */

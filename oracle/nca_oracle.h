@@ -62,6 +62,24 @@ int64_t orc_bases(int32_t side, double k, int32_t sign, const double* testpts, c
                   int32_t sampled, int64_t* pivptr, int64_t* pivots, int64_t* xoffsets, double* X,
                   int64_t xcap);
 
+/* Partially pivoted ACA on an m x n block evaluated entrywise: rows / cols are
+ * 1-based point ids into rowpts / colpts; entry G(row, col), or G(col, row)
+ * with transposed != 0 (pivots.jl:215-300 with ACA(); the published method
+ * restated). Returns the rank; rowpiv/colpiv 1-based positions (capacity
+ * min(maxrank, m, n)); U (m x rank) and V (n x rank) when not NULL. */
+int64_t orc_paca(double k, int32_t sign, int32_t transposed, const double* rowpts, const double* colpts,
+                 int64_t m, const int64_t* rows, int64_t n, const int64_t* cols, int64_t maxrank, double tol,
+                 int64_t* rowpiv, int64_t* colpiv, double* U, double* V);
+
+/* testbases/trialbases(::TopDown) for one side (topdown.jl:51-206): row and
+ * column pivots per node and the factors X in the layout of orc_bases. */
+int64_t orc_topdown(int32_t side, double k, int32_t sign, const double* testpts, const double* trialpts,
+                    int64_t nn, const int64_t* perm, const int64_t* nodelo, const int64_t* nodehi,
+                    const int64_t* level, const int64_t* parent, const int64_t* childptr, const int64_t* children,
+                    const int64_t* operm, const int64_t* olo, const int64_t* ohi, const int64_t* farptr,
+                    const int64_t* farnodes, int64_t maxrank, double tol, int64_t* pivptr, int64_t* pivots,
+                    int64_t* cpivptr, int64_t* cpivots, int64_t* xoffsets, double* X, int64_t xcap);
+
 /* assemble_couplingstore (couplingmatrices.jl:11-53). */
 void orc_coupling(double k, int32_t sign, const double* testpts, const double* trialpts, int64_t ntest,
                   const int64_t* farptr, const int64_t* farnodes, const int64_t* testpivptr,

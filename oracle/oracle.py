@@ -71,6 +71,11 @@ def lib():
         L.orc_bases.restype = I64
         L.orc_bases.argtypes = [I32, D, I32, P, P, I64, P, P, P, P, P, P, P, P, I64, D, I32, P, P, P, P, I64]
         L.orc_coupling.argtypes = [D, I32, P, P, I64, P, P, P, P, P, P, P, P]
+        L.orc_paca.restype = I64
+        L.orc_paca.argtypes = [D, I32, I32, P, P, I64, P, I64, P, I64, D, P, P, P, P]
+        L.orc_topdown.restype = I64
+        L.orc_topdown.argtypes = [I32, D, I32, P, P, I64, P, P, P, P, P, P, P, P, P, P, P, P, I64, D, P, P, P, P, P,
+                                  P, I64]
         L.orc_h2_matvec.argtypes = [P, I32, I64, P, I64, P, I64, P, P, I32, I64, P]
         L.orc_dense_rows.argtypes = [D, I32, P, P, I64, I64, P, I64, P, I64, P]
         L.orc_num_threads.restype = I32
@@ -186,6 +191,56 @@ def bases(side, k, sign, testpts, trialpts, tree, farpivptr, farpiv, maxrank, to
         cap = int(xoff[-1])
     pivptr0 = pivptr - 1
     return pivptr0, pivots[:pivptr0[-1]] - 1, xoff, X[:xoff[-1]].copy(), int(st)
+
+
+def paca(k, sign, transposed, rowpts, colpts, rows, cols, maxrank, tol):
+    """Partially pivoted ACA of the block G(rows, cols) (0-based point ids;
+    transposed: G(cols, rows)); returns (rank, rowpiv, colpiv) 1-based
+    positions and the factors U (m x rank), V (n x rank)."""
+    rp_ = np.ascontiguousarray(rowpts, dtype=np.float64)
+    cp_ = np.ascontiguousarray(colpts, dtype=np.float64)
+    r1, c1 = _i64(np.asarray(rows) + 1), _i64(np.asarray(cols) + 1)
+    m, n = len(rows), len(cols)
+    km = max(min(maxrank, m, n), 1)
+    rpv, cpv = np.zeros(km, np.int64), np.zeros(km, np.int64)
+    U = np.zeros((max(m, 1), km), np.complex128, order="F")
+    V = np.zeros((max(n, 1), km), np.complex128, order="F")
+    r = lib().orc_paca(float(k), int(sign), int(bool(transposed)), _p(rp_), _p(cp_), m, _p(r1), n, _p(c1),
+                       int(maxrank), float(tol), _p(rpv), _p(cpv), _p(U), _p(V))
+    U = np.asfortranarray(U.ravel(order="F")[:m * r].reshape((m, r), order="F"))
+    V = np.asfortranarray(V.ravel(order="F")[:n * r].reshape((n, r), order="F"))
+    return int(r), rpv[:r].copy(), cpv[:r].copy(), U, V
+
+
+def topdown(side, k, sign, testpts, trialpts, tree, othertree, farptr, farnodes, maxrank, tol):
+    """testbases/trialbases(::TopDown) of one side. tree / othertree:
+    ClusterTrees of this and of the other side; farptr / farnodes: 0-based CSR
+    of the far nodes of every node. Returns (pivptr, pivots, cpivptr, cpivots,
+    xoffsets, X, status), pivots 0-based point ids."""
+    lay, olay = tree.layout_1based(), othertree.layout_1based()
+    nn = tree.nnodes
+    parent = _i64(np.asarray(tree.parent) + 1)
+    fptr, fnod = _i64(np.asarray(farptr) + 1), _i64(np.asarray(farnodes) + 1) if len(farnodes) else np.zeros(1, np.int64)
+    pivptr, cpivptr = np.zeros(nn + 1, np.int64), np.zeros(nn + 1, np.int64)
+    pivots = np.zeros(max(nn * maxrank, 1), np.int64)
+    cpivots = np.zeros(max(nn * maxrank, 1), np.int64)
+    xoff = np.zeros(nn + 1, np.int64)
+    tp = np.ascontiguousarray(testpts, dtype=np.float64)
+    sp = np.ascontiguousarray(trialpts, dtype=np.float64)
+    sizes = tree.hi - tree.lo
+    cap = int(np.sum(sizes * maxrank))
+    for _ in range(2):
+        X = np.zeros(max(cap, 1), np.complex128)
+        st = lib().orc_topdown(int(side), float(k), int(sign), _p(tp), _p(sp), nn, _p(lay["perm"]),
+                               _p(lay["nodelo"]), _p(lay["nodehi"]), _p(lay["level"]), _p(parent),
+                               _p(lay["childptr"]), _p(lay["children"]), _p(olay["perm"]), _p(olay["nodelo"]),
+                               _p(olay["nodehi"]), _p(fptr), _p(fnod), int(maxrank), float(tol), _p(pivptr),
+                               _p(pivots), _p(cpivptr), _p(cpivots), _p(xoff), _p(X), cap)
+        if st != 1:
+            break
+        cap = int(xoff[-1])
+    pp, cpp = pivptr - 1, cpivptr - 1
+    return pp, pivots[:pp[-1]] - 1, cpp, cpivots[:cpp[-1]] - 1, xoff, X[:xoff[-1]].copy(), int(st)
 
 
 def coupling(k, sign, testpts, trialpts, farptr, farnodes, testpivptr, testpiv, trialpivptr, trialpiv):

@@ -55,6 +55,37 @@ def oracle_factors(res, tree, fp):
     return f
 
 
+def oracle_topdown(p, side, ranks_from=None):
+    """oracle.topdown for one side of problem p; returns SideFactors with the
+    rank of every node from ``ranks_from`` (assigned flags of the device
+    result) or from the oracle's pivots (nodes with pivots)."""
+    tree = p.tree.testtree if side == 0 else p.tree.trialtree
+    other = p.tree.trialtree if side == 0 else p.tree.testtree
+    fd = p.tf if side == 0 else p.sf
+    res = O.topdown(side, p.k, p.sign, p.testpts, p.trialpts, tree, other, fd.nodetofarnodesptr, fd.farnodes,
+                    p.maxrank, p.tol)
+    assert res[6] == 0
+    pivptr, piv, cpivptr, cpiv, xoff, X, st = res
+    f = C.SideFactors(tree.nnodes)
+    leaf = tree.leafmask()
+    assigned = np.diff(pivptr) > 0 if ranks_from is None else ranks_from
+    for lev in sorted(tree.levels().tolist(), reverse=True):
+        for t in tree.levelnodes(lev):
+            if not assigned[t]:
+                continue
+            f.assigned[t] = True
+            r = pivptr[t + 1] - pivptr[t]
+            f.rank[t] = r
+            f.pivots[t] = piv[pivptr[t]:pivptr[t + 1]]
+            m = tree.nvalues(t) if leaf[t] else int(sum(f.rank[c] for c in tree.childrenof(t) if f.assigned[c]))
+            f.m[t] = m
+            f.X[t] = X[xoff[t]:xoff[t + 1]].reshape((m, r), order="F")
+    for lev in tree.levels():
+        f.level_transfer_nodes[lev] = [int(t) for t in tree.levelnodes(lev) if not leaf[t] and f.assigned[t]]
+    f.colpivots = (cpivptr, cpiv)
+    return f
+
+
 def oracle_assembly(p):
     tres = O.bases(0, p.k, p.sign, p.testpts, p.trialpts, p.tree.testtree, p.fpt[0], p.fpt[1], p.maxrank, p.tol)
     sres = O.bases(1, p.k, p.sign, p.testpts, p.trialpts, p.tree.trialtree, p.fps[0], p.fps[1], p.maxrank, p.tol)

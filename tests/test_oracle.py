@@ -5,7 +5,8 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from helpers import grid_problem, sphere_problem, oracle_assembly, oracle_operator, oracle_operator_rows, relerr
+from helpers import (grid_problem, sphere_problem, oracle_assembly, oracle_operator, oracle_operator_rows,
+                     oracle_topdown, relerr, Problem)
 from nca_amd import geometry
 
 
@@ -485,3 +486,137 @@ def test_oracle_directional_vs_direct_summation(kind, n, lam):
         yr = A.matvec(x, rows=rows[::5])
         assert relerr(yr[rows[::5]], y[rows[::5]]) < 1e-14
         assert not np.any(np.delete(yr, rows[::5]))
+
+
+# ---------------------------------------------------------------- TopDown: partially pivoted ACA
+def brute_force_paca(A, maxrank, tol):
+    """The partially pivoted ACA written with numpy from its description: row
+    pivot 0 first, column pivot = first maximum of the residual row over unused
+    columns, next row pivot = first maximum of the new column over unused rows,
+    stop (keeping the cross) when |u| |v| <= tol ||U V^T||_F, exact norms."""
+    m, n = A.shape
+    U = np.zeros((m, 0), np.complex128)
+    V = np.zeros((n, 0), np.complex128)
+    usedr, usedc = np.zeros(m, bool), np.zeros(n, bool)
+    rows, cols = [], []
+    i = 0
+    while len(rows) < min(maxrank, m, n):
+        row = A[i, :] - U[i, :] @ V.T
+        w = np.where(usedc, -1.0, np.abs(row) ** 2)
+        j = int(np.argmax(w))
+        if not w[j] > 0:
+            break
+        v = row / row[j]
+        u = A[:, j] - U @ V[j, :]
+        U = np.column_stack([U, u])
+        V = np.column_stack([V, v])
+        rows.append(i + 1)
+        cols.append(j + 1)
+        usedr[i] = usedc[j] = True
+        if np.sum(np.abs(u) ** 2) * np.sum(np.abs(v) ** 2) <= tol * tol * np.sum(np.abs(U @ V.T) ** 2):
+            break
+        wr = np.where(usedr, -1.0, np.abs(u) ** 2)
+        if not np.any(~usedr):
+            break
+        i = int(np.argmax(wr))
+    return len(rows), rows, cols, U, V
+
+
+def kernel_block(k, x, y, sign=1):
+    return np.array([[O.entry(k, sign, a, b) for b in y] for a in x])
+
+
+def test_paca_matches_brute_force_on_kernel_blocks():
+    """Ranks and pivots of the oracle's partially pivoted ACA equal a numpy
+    restatement's on separated random clusters (both block orientations), and
+    U V^T reproduces the block to a small multiple of tol."""
+    rng = np.random.default_rng(12)
+    for trial in range(12):
+        m, n = int(rng.integers(3, 60)), int(rng.integers(3, 300))
+        x = rng.random((m, 3))
+        y = rng.random((n, 3)) * 1.5 + np.array([2.5 + rng.random(), 0.3, 0])
+        for tol in (1e-2, 1e-4, 1e-8):
+            for transposed in (0, 1):
+                A = kernel_block(2.0, x, y) if not transposed else kernel_block(2.0, y, x).T
+                r, rp, cp, U, V = O.paca(2.0, 1, transposed, x, y, np.arange(m), np.arange(n), 40, tol)
+                rr, rrp, rcp, RU, RV = brute_force_paca(A, 40, tol)
+                assert r == rr and list(rp) == rrp and list(cp) == rcp, (trial, tol, transposed)
+                assert np.allclose(U, RU, rtol=1e-10, atol=1e-14 * np.abs(A).max())
+                assert np.linalg.norm(A - U @ V.T) <= 20 * tol * np.linalg.norm(A) or r == min(40, m, n)
+
+
+def test_paca_exact_rank_one_and_zero_blocks():
+    """Columns of one repeated point: the block has rank 1 and the ACA stops
+    after one cross with U V^T = A; coincident points give a zero block and
+    rank 0."""
+    x = np.random.default_rng(3).random((17, 3))
+    y = np.repeat(np.array([[3.0, 0.5, 0.2]]), 9, axis=0)
+    A = kernel_block(1.5, x, y)
+    r, rp, cp, U, V = O.paca(1.5, 1, 0, x, y, np.arange(17), np.arange(9), 20, 1e-12)
+    assert r == 1 and rp[0] == 1
+    assert np.allclose(U @ V.T, A, rtol=0, atol=1e-15 * np.abs(A).max())
+    z = np.zeros((5, 3))
+    r, *_ = O.paca(1.5, 1, 0, z, z, np.arange(5), np.arange(5), 5, 1e-4)
+    assert r == 0
+
+
+def separated_squares(tol, sign=-1):
+    """The reference's "single far interaction" geometry
+    (test/nca/test_petrovgalerkinnca.jl:155-229): two 7 x 7 grids of the unit
+    square, the second shifted by (3, 0, 0); k = 2 pi / (100 h), h = 0.16,
+    exp(-i k r); 2-means trees with minvalues 60."""
+    tp = geometry.grid_square(7)
+    sp = tp + np.array([3.0, 0.0, 0.0])
+    tree = nca_amd_tree(tp, sp, 60)
+    return Problem(tp, sp, tree, 2 * np.pi / (100 * 0.16), sign, tol, 50)
+
+
+def nca_amd_tree(tp, sp, minvalues):
+    import nca_amd
+    return nca_amd.BlockTree(nca_amd.KMeansTree(tp, 2, minvalues), nca_amd.KMeansTree(sp, 2, minvalues))
+
+
+@pytest.mark.parametrize("tol", [1e-2, 1e-4, 1e-6])
+def test_topdown_oracle_single_far_interaction(tol):
+    """TopDown with ACA on the reference's single-far-interaction case: the
+    compressed operator within 5 tol of the dense matrix (the reference's
+    bound, test_petrovgalerkinnca.jl:229)."""
+    p = separated_squares(tol)
+    ft, fs = oracle_topdown(p, 0), oracle_topdown(p, 1)
+    _, H = oracle_operator(p, ft, fs)
+    A = dense_h2(p, ft, fs, H)
+    D = kernel_block(p.k, p.testpts, p.trialpts, p.sign)
+    x = geometry.complex_gaussian(np.random.default_rng(1), (p.trialpts.shape[0], 1))[:, 0]
+    assert relerr(A @ x, D @ x) < 5 * tol
+
+
+def test_topdown_oracle_readme_case():
+    """The README's example (README.md:95-137: two 32 x 32 grids, k = 2 pi / 10,
+    2-means trees with minvalues 32, TopDown with ACA(tol=1e-4)): a leaf's
+    factor interpolates at its pivots (X[tau, :] = I), the stacked transfers
+    of a parent are A[tau_c, sigma] A[tau, sigma]^{-1}, and the operator is
+    within 20 tol of direct summation (the README states no bound)."""
+    p = grid_problem(1e-4)
+    ft, fs = oracle_topdown(p, 0), oracle_topdown(p, 1)
+    tt = p.tree.testtree
+    checked = 0
+    for t in range(tt.nnodes):
+        if not ft.assigned[t] or ft.rank[t] == 0:
+            continue
+        piv = ft.pivots[t]
+        sigma = ft.colpivots[1][ft.colpivots[0][t]:ft.colpivots[0][t + 1]]
+        P = kernel_block(p.k, p.testpts[piv], p.trialpts[sigma])
+        if tt.isleaf(t):
+            pos = [int(np.nonzero(tt.values(t) == q)[0][0]) for q in piv]
+            assert np.allclose(ft.X[t][pos, :], np.eye(len(piv)), atol=1e-9)
+        else:
+            rows = np.concatenate([ft.pivots[c] for c in tt.childrenof(t) if ft.assigned[c]])
+            L = kernel_block(p.k, p.testpts[rows], p.trialpts[sigma])
+            assert np.allclose(ft.X[t] @ P, L, rtol=1e-8, atol=1e-10 * np.abs(L).max())
+        checked += 1
+    assert checked > 10
+    _, H = oracle_operator(p, ft, fs)
+    A = dense_h2(p, ft, fs, H)
+    D = kernel_block(p.k, p.testpts, p.trialpts)
+    x = geometry.complex_gaussian(np.random.default_rng(2), (p.trialpts.shape[0], 1))[:, 0]
+    assert relerr(A @ x, D @ x) < 20 * p.tol
