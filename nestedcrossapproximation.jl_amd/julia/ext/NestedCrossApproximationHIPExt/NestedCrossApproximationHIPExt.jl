@@ -23,6 +23,7 @@ include("compresslevel.jl")  # seams 1 and 2: level assembly, ACA / IACA + skele
 include("couplingstore.jl")  # seam 3: coupling blocks on the device
 include("operator.jl")       # seam 4: the device operator and mul!
 include("directional.jl")    # directional (wideband) operators
-include("device.jl")         # partition and Phase 1 on the device
+include("device.jl")         # trees, partition and Phase 1 on the device
+include("resident.jl")       # factors kept on the device: fused BottomUp side, TopDown, resident operator
 
 end # module

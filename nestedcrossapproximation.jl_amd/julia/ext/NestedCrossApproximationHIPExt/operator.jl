@@ -43,7 +43,8 @@ function _finalize(op::HIPOperator)
     return op
 end
 
-function HIPOperator(A::HIPKernelMatrix, core, partition; precision::Integer=NCA_PREC_FP64)
+# The nca_h2_desc of a core, and the host arrays it points into.
+function h2desc(core, partition)
     tp, tl, th = layout(H2Trees.testtree(core.tree))
     sp, sl, sh = layout(H2Trees.trialtree(core.tree))
     tb, tbo = packed(core.nestedtestbases.blocks)
@@ -67,6 +68,11 @@ function HIPOperator(A::HIPKernelMatrix, core, partition; precision::Integer=NCA
         pointer(splan.chdsidcs), pointer(sto),
         pointer(ttb), pointer(stb), pointer(cplan.nodetofarnodesptr), pointer(cplan.farnodes),
         pointer(nptr), pointer(nnodes))
+    return d, keep
+end
+
+function HIPOperator(A::HIPKernelMatrix, core, partition; precision::Integer=NCA_PREC_FP64)
+    d, keep = h2desc(core, partition)
     r = Ref{Ptr{Cvoid}}()
     k = Ref(A.kernel)
     GC.@preserve keep check(ccall((:nca_h2_create, libnca_hip), Int32,
