@@ -393,6 +393,9 @@ __global__ void __launch_bounds__(kWave)
 #pragma unroll
         for (int q = 0; q < NR; ++q) acc[q] = make_double2(0.0, 0.0);
         if (slot < ncs) {
+            // four columns in flight: their index, coefficient and operand loads
+            // are independent of each other
+#pragma unroll 4
             for (int col = slot; col < C; col += ncs) {
                 const double2 sv = ldc<CJ>(St, (int64_t)row + (int64_t)col * rt);
                 const ST* xv = xh + (int64_t)cc[col] * R + q0;
