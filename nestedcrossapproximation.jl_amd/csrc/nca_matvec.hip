@@ -97,6 +97,8 @@ struct nca_h2 {
     nca_ctx* ctx = nullptr;
     double k = 0.0;
     int sign = 1;
+    // the largest phase |k| r / (2 pi) over the near pairs (from the point boxes)
+    double near_turns = 0.0;
     int precision = NCA_PREC_FP64;
     int64_t M = 0, N = 0;
     nca::SideDev test, trial;
@@ -454,12 +456,21 @@ __global__ void __launch_bounds__(kWave)
 
 constexpr double kInvFourPi = 0x1.45f306dc9c883p-4;
 
-// Near-field constants of one product: k, k / (2 pi) and k / (4 pi).
+// Near-field constants of one product: k, k / (2 pi) and k / (4 pi); phase32:
+// every near pair's phase k r stays within kPhase32Turns revolutions (from
+// the point boxes), so MIXED forms r and the phase in FP32 (k2pif = k / (2 pi)).
 struct NearK {
     double k;
     double k2pi;
     double k4pi;
+    float k2pif;
+    int phase32;
 };
+constexpr double kPhase32Turns = 1.0;
+// turns: the largest phase k r / (2 pi) over the operator's near pairs
+inline NearK near_consts(double k, double turns) {
+    return NearK{k, k * (0.5 / M_PI), k * (0.25 / M_PI), (float)(k * (0.5 / M_PI)), turns <= kPhase32Turns ? 1 : 0};
+}
 
 // Table of (cos, sin)(n pi / 128), n = 0..255, for the FP64 near field.
 constexpr int kSinCosTab = 256;
@@ -507,18 +518,27 @@ __device__ __forceinline__ double2 helmholtz_mv(const NearK& nk, const double2* 
 // squared distance carries 1e-30 (below half an ulp of any r^2 >= 1e-12), so
 // r = 0 needs no clamp, and r^2 <= 2e-30 counts as r = 0 and gives 0; other
 // ranges are more than 1e-14 away and skip both. The phase takes r0 = r^2 rsq(r^2) plus one Newton correction
-// (r^2 - r0^2) rsq(r^2) / 2, relative error ~1e-14.
-template <bool NEG, bool GUARD>
+// (r^2 - r0^2) rsq(r^2) / 2, relative error ~1e-14. PH32 (every near pair
+// within kPhase32Turns revolutions): r = r^2 rsq(r^2) and the phase k r / (2 pi)
+// in FP32 instead, relative error < 4e-7, i.e. < 2.5e-6 radians at one
+// revolution (five VALU operations fewer per entry).
+template <bool NEG, bool GUARD, bool PH32>
 __device__ __forceinline__ float2 helmholtz_f32(const NearK& nk, double x0, double x1, double x2, double y0,
                                                 double y1, double y2) {
     const double dx = x0 - y0, dy = x1 - y1, dz = x2 - y2;
     const double r2 = fma(dz, dz, fma(dy, dy, GUARD ? fma(dx, dx, 1e-30) : dx * dx));
-    const float rf = __builtin_amdgcn_rsqf((float)r2);
-    const double ri = (double)rf;
-    const double r0 = r2 * ri;
-    const double e = fma(-r0, r0, r2);
-    const double t = fma(e, ri * nk.k4pi, r0 * nk.k2pi);
-    const float f = (float)__builtin_amdgcn_fract(t);
+    const float r2f = (float)r2;
+    const float rf = __builtin_amdgcn_rsqf(r2f);
+    float f;
+    if constexpr (PH32) {
+        f = __builtin_amdgcn_fractf((r2f * rf) * nk.k2pif);
+    } else {
+        const double ri = (double)rf;
+        const double r0 = r2 * ri;
+        const double e = fma(-r0, r0, r2);
+        const double t = fma(e, ri * nk.k4pi, r0 * nk.k2pi);
+        f = (float)__builtin_amdgcn_fract(t);
+    }
     const float c = __builtin_amdgcn_cosf(f), s = __builtin_amdgcn_sinf(f);
     const float sc = !GUARD || r2 > 2e-30 ? rf : 0.0f;
     return make_float2(c * sc, NEG ? -(s * sc) : s * sc);
@@ -544,7 +564,7 @@ struct PrecF64 {
     __device__ static T2 cvt(double2 v) { return v; }
     __device__ static T2 cvt(float2 v) { return make_double2(v.x, v.y); }
     static constexpr bool kTab = true;
-    template <bool NEG, bool GUARD>
+    template <bool NEG, bool GUARD, bool PH32>
     __device__ static T2 eval(const NearK& nk, const double2* tab, double x0, double x1, double x2, double y0,
                               double y1, double y2) {
         return helmholtz_mv<NEG, GUARD>(nk, tab, x0, x1, x2, y0, y1, y2);
@@ -562,10 +582,10 @@ struct PrecF32 {
     __device__ static T2 cvt(double2 v) { return make_float2((float)v.x, (float)v.y); }
     __device__ static T2 cvt(float2 v) { return v; }
     static constexpr bool kTab = false;
-    template <bool NEG, bool GUARD>
+    template <bool NEG, bool GUARD, bool PH32>
     __device__ static T2 eval(const NearK& nk, const double2*, double x0, double x1, double x2, double y0, double y1,
                               double y2) {
-        return helmholtz_f32<NEG, GUARD>(nk, x0, x1, x2, y0, y1, y2);
+        return helmholtz_f32<NEG, GUARD, PH32>(nk, x0, x1, x2, y0, y1, y2);
     }
 };
 
@@ -728,16 +748,17 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uin
 template <class P, int G, int NC>
 struct NearOps {
     typename P::T2 b[NC][G];
-    uint32_t pw;  // G = 1: dword i < 6 of the row's source point in lane i of each 16-lane row
-    double pq;    // G > 1: coordinate j < 3 of the quad's source point in lane j of each quad
+    // coordinate j < 3 of the source point in lane j of each 16-lane row
+    // (G = 1) or of each quad (G > 1)
+    double pq;
 };
 
-// The double in dwords D, D + 1 of a 16-lane row (DPP row_newbcast).
-template <int D>
-__device__ __forceinline__ double row_double(uint32_t w) {
-    const int lo = __builtin_amdgcn_mov_dpp((int)w, 0x150 + D, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_mov_dpp((int)w, 0x150 + D + 1, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
+// Coordinate L of a 16-lane row's point held in lane L of the row: one
+// 64-bit DPP move (row_newbcast).
+template <int L>
+__device__ __forceinline__ double row_double(double v) {
+    const long long u = __double_as_longlong(v);
+    return __longlong_as_double(__builtin_amdgcn_mov_dpp(u, 0x150 + L, 0xf, 0xf, false));
 }
 
 // Coordinate L of a quad's point held in lane L of the quad (DPP quad_perm).
@@ -754,8 +775,8 @@ __device__ __forceinline__ double quad_double(double v) {
 // source points, at byte offsets sp / sx into the range's points (rp) and
 // right-hand sides (rx); reads past the range end return 0. Every lane of a
 // 16-lane row (G = 1) or of a quad (G > 1) needs the same source point, so
-// each lane loads one piece of it (vp: a dword of the row's point, or a
-// coordinate of the quad's) and the computing step broadcasts them.
+// each lane loads one coordinate of it (vp) and the computing step
+// broadcasts them.
 template <class P, int G, int NC>
 __device__ __forceinline__ void near_load(NearOps<P, G, NC>& o, __amdgpu_buffer_rsrc_t rp,
                                           __amdgpu_buffer_rsrc_t rx, uint32_t sp, uint32_t sx, uint32_t vp,
@@ -764,8 +785,16 @@ __device__ __forceinline__ void near_load(NearOps<P, G, NC>& o, __amdgpu_buffer_
     for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int m = 0; m < G; ++m) o.b[c][m] = buf_load<typename P::T2>(rx, vx[c][m], sx);
-    if constexpr (G == 1) o.pw = (uint32_t)buf_load<int>(rp, vp, sp);
-    else o.pq = buf_load<double>(rp, vp, sp);
+    o.pq = buf_load<double>(rp, vp, sp);
+}
+
+// The lane's byte offset of its source-point coordinate in a step: coordinate
+// min(j, 2) of source kk (16-row tiles, j = lane % 16) or of the quad's source
+// ysrc (smaller tiles, j = lane % 4).
+template <int G>
+__device__ __forceinline__ uint32_t near_point_offset(int lane, int kk, int ysrc) {
+    return G == 1 ? (uint32_t)kk * 24u + 8u * (uint32_t)min(lane & 15, 2)
+                  : (uint32_t)ysrc * 24u + 8u * (uint32_t)min(lane & 3, 2);
 }
 
 // One near-field step of a tile with H valid-row slots (H = 16, 8 or 4).
@@ -777,18 +806,18 @@ __device__ __forceinline__ void near_load(NearOps<P, G, NC>& o, __amdgpu_buffer_
 // read points 0 and B = 0; groups that start there (left: sources of the range
 // from the step on) skip their products. The evaluated entries serve the NC
 // column chunks (one accumulator each).
-template <class P, int H, class ACC, bool EMB, bool NEG, bool GUARD, int NC>
+template <class P, int H, class ACC, bool EMB, bool NEG, bool GUARD, bool PH32, int NC>
 __device__ __forceinline__ void near_compute(ACC (&acc)[NC], const NearK& nk, const double2* tab, double x0,
                                              double x1, double x2, const NearOps<P, 16 / H, NC>& o, int left) {
     using T2 = typename P::T2;
     constexpr int G = 16 / H;
     T2 g;
     if constexpr (G == 1)
-        g = P::template eval<NEG, GUARD>(nk, tab, x0, x1, x2, row_double<0>(o.pw), row_double<2>(o.pw),
-                                         row_double<4>(o.pw));
+        g = P::template eval<NEG, GUARD, PH32>(nk, tab, x0, x1, x2, row_double<0>(o.pq), row_double<1>(o.pq),
+                                               row_double<2>(o.pq));
     else
-        g = P::template eval<NEG, GUARD>(nk, tab, x0, x1, x2, quad_double<0>(o.pq), quad_double<1>(o.pq),
-                                         quad_double<2>(o.pq));
+        g = P::template eval<NEG, GUARD, PH32>(nk, tab, x0, x1, x2, quad_double<0>(o.pq), quad_double<1>(o.pq),
+                                               quad_double<2>(o.pq));
     const int i = threadIdx.x & 15;
 #pragma unroll
     for (int m = 0; m < G; ++m) {
@@ -811,6 +840,22 @@ __device__ __forceinline__ void near_compute(ACC (&acc)[NC], const NearK& nk, co
         }
 #pragma unroll
         for (int c = 0; c < NC; ++c) acc[c].mma(a, o.b[c][m]);
+    }
+}
+
+// Calls steps(guard, ph32) with compile-time flags: FP64 always guarded, one
+// phase form; MIXED guarded only for ranges that need it, phase in FP32 when
+// the operator's near pairs allow it (both uniform over the wave).
+template <class P, class F>
+__device__ __forceinline__ void near_dispatch(F& steps, bool guard, bool ph32) {
+    if constexpr (P::kF64) {
+        steps(std::true_type(), std::false_type());
+    } else if (ph32) {
+        if (guard) steps(std::true_type(), std::true_type());
+        else steps(std::false_type(), std::true_type());
+    } else {
+        if (guard) steps(std::true_type(), std::false_type());
+        else steps(std::false_type(), std::false_type());
     }
 }
 
@@ -842,8 +887,7 @@ __device__ __forceinline__ void near_walk(ACC (&acc)[NC], const NearK& nk, const
     constexpr int U = (H == 16 && !P::kF64 && NC == 1) ? NCA_NEAR_UNROLL : 1;
     constexpr uint32_t kT = sizeof(T2);
     const int kk = lane >> 4, ysrc = 4 * ((lane & 15) / H) + kk;
-    const uint32_t vp = G == 1 ? (uint32_t)kk * 24u + 4u * (uint32_t)min(lane & 15, 5)
-                               : (uint32_t)ysrc * 24u + 8u * (uint32_t)min(lane & 3, 2);
+    const uint32_t vp = near_point_offset<G>(lane, kk, ysrc);
     uint32_t vx[NC][G];
 #pragma unroll
     for (int c = 0; c < NC; ++c)
@@ -860,8 +904,8 @@ __device__ __forceinline__ void near_walk(ACC (&acc)[NC], const NearK& nk, const
         const uint32_t n = (uint32_t)(hi - lo);
         const __amdgpu_buffer_rsrc_t rp = buf_rsrc(spts + 3 * (int64_t)lo, n * 24u);
         const __amdgpu_buffer_rsrc_t rx = buf_rsrc(xsq + (int64_t)lo * R, n * rowb);
-        auto steps = [&](auto guard) {
-            constexpr bool GU = decltype(guard)::value;
+        auto steps = [&](auto guard, auto ph32) {
+            constexpr bool GU = decltype(guard)::value, PH = decltype(ph32)::value;
             int left = (int)n;
             uint32_t sp = 0, sx = 0;
             if constexpr (U > 1) {
@@ -872,18 +916,18 @@ __device__ __forceinline__ void near_walk(ACC (&acc)[NC], const NearK& nk, const
                         near_load<P, G, NC>(o[u], rp, rx, sp + u * S * 24u, sx + u * S * rowb, vp, vx);
 #pragma unroll
                     for (int u = 0; u < U; ++u)
-                        near_compute<P, H, ACC, EMB, NEG, GU, NC>(acc, nk, tab, x0, x1, x2, o[u], left - u * S);
+                        near_compute<P, H, ACC, EMB, NEG, GU, PH, NC>(acc, nk, tab, x0, x1, x2, o[u],
+                                                                      left - u * S);
                 }
             }
             for (; left > 0; left -= S, sp += S * 24u, sx += S * rowb) {
                 NearOps<P, G, NC> o;
                 near_load<P, G, NC>(o, rp, rx, sp, sx, vp, vx);
-                near_compute<P, H, ACC, EMB, NEG, GU, NC>(acc, nk, tab, x0, x1, x2, o, left);
+                near_compute<P, H, ACC, EMB, NEG, GU, PH, NC>(acc, nk, tab, x0, x1, x2, o, left);
             }
         };
         // FP64: one guarded step loop (a second copy costs the kernel occupancy)
-        if (P::kF64 || hr < 0) steps(std::true_type());
-        else steps(std::false_type());
+        near_dispatch<P>(steps, hr < 0, nk.phase32 != 0);
     }
 }
 
@@ -1015,8 +1059,7 @@ __device__ __forceinline__ void small_walk(typename P::T2 (&acc)[NQ], const Near
     constexpr int S = 4 * (16 / H), G = 16 / H;
     constexpr uint32_t kT = sizeof(T2);
     const int kk = lane >> 4, ysrc = 4 * ((lane & 15) / H) + kk;
-    const uint32_t vp = G == 1 ? (uint32_t)kk * 24u + 4u * (uint32_t)min(lane & 15, 5)
-                               : (uint32_t)ysrc * 24u + 8u * (uint32_t)min(lane & 3, 2);
+    const uint32_t vp = near_point_offset<G>(lane, kk, ysrc);
     const uint32_t rowb = R * kT, vx = (uint32_t)ysrc * rowb;
     const int nr = (int)(e1 - e0);
     for (int c = 0; c < nr; ++c) {
@@ -1028,8 +1071,8 @@ __device__ __forceinline__ void small_walk(typename P::T2 (&acc)[NQ], const Near
         const uint32_t n = (uint32_t)(hi - lo);
         const __amdgpu_buffer_rsrc_t rp = buf_rsrc(spts + 3 * (int64_t)lo, n * 24u);
         const __amdgpu_buffer_rsrc_t rx = buf_rsrc(xsq + (int64_t)lo * R, n * rowb);
-        auto steps = [&](auto guard) {
-            constexpr bool GU = decltype(guard)::value;
+        auto steps = [&](auto guard, auto ph32) {
+            constexpr bool GU = decltype(guard)::value, PH = decltype(ph32)::value;
             int left = (int)n;
             uint32_t sp = 0, sx = 0;
             for (; left > 0; left -= S, sp += S * 24u, sx += S * rowb) {
@@ -1037,15 +1080,13 @@ __device__ __forceinline__ void small_walk(typename P::T2 (&acc)[NQ], const Near
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) b[q] = buf_load<T2>(rx, vx + (uint32_t)q * kT, sx);
                 T2 g;
-                if constexpr (G == 1) {
-                    const uint32_t pw = (uint32_t)buf_load<int>(rp, vp, sp);
-                    g = P::template eval<NEG, GU>(nk, tab, x0, x1, x2, row_double<0>(pw), row_double<2>(pw),
-                                                  row_double<4>(pw));
-                } else {
-                    const double pq = buf_load<double>(rp, vp, sp);
-                    g = P::template eval<NEG, GU>(nk, tab, x0, x1, x2, quad_double<0>(pq), quad_double<1>(pq),
-                                                  quad_double<2>(pq));
-                }
+                const double pq = buf_load<double>(rp, vp, sp);
+                if constexpr (G == 1)
+                    g = P::template eval<NEG, GU, PH>(nk, tab, x0, x1, x2, row_double<0>(pq), row_double<1>(pq),
+                                                      row_double<2>(pq));
+                else
+                    g = P::template eval<NEG, GU, PH>(nk, tab, x0, x1, x2, quad_double<0>(pq), quad_double<1>(pq),
+                                                      quad_double<2>(pq));
                 if (ysrc >= left) g = czero2<P>();
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) {
@@ -1054,8 +1095,7 @@ __device__ __forceinline__ void small_walk(typename P::T2 (&acc)[NQ], const Near
                 }
             }
         };
-        if (P::kF64 || hr < 0) steps(std::true_type());
-        else steps(std::false_type());
+        near_dispatch<P>(steps, hr < 0, nk.phase32 != 0);
     }
     (void)sizeof(T);
 }
@@ -1535,7 +1575,7 @@ void run_leaf(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy,
     if (dst.ntiles > 0) {
         // conj(G) for a negative phase sign, again conjugated for the adjoint
         const bool neg = (h->sign < 0) != CJ;
-        const NearK nk{h->k, h->k * (0.5 / M_PI), h->k * (0.25 / M_PI)};
+        const NearK nk = near_consts(h->k, h->near_turns);
         const bool k3 = h->near_k3[MIX ? 1 : 0] != 0;
         auto kern = k3 ? (neg ? k_leaf_out_mfma<P, FWD, CJ, true, true, NC> : k_leaf_out_mfma<P, FWD, CJ, true, false, NC>)
                        : (neg ? k_leaf_out_mfma<P, FWD, CJ, false, true, NC>
@@ -1738,7 +1778,7 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
         }
         if (dst.ntiles > 0) {
             const bool neg = (h->sign < 0) != CJ;
-            const NearK nk{h->k, h->k * (0.5 / M_PI), h->k * (0.25 / M_PI)};
+            const NearK nk = near_consts(h->k, h->near_turns);
             auto kern = neg ? k_leaf_out_small<P, FWD, CJ, true, NR> : k_leaf_out_small<P, FWD, CJ, false, NR>;
             hipLaunchKernelGGL(kern, dim3(blocks_for(dst.ntiles, 4)), dim3(256), 0, st, dst.tile_leaf.p,
                                dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst),
@@ -2857,6 +2897,28 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
                 rowent[t] = e * (h->test.h_hi[t] - h->test.h_lo[t]);
             });
             for (int64_t t = 0; t < ntest; ++t) near_entries += rowent[t];
+            // the largest distance of a near pair: farthest corners of the point boxes
+            std::vector<double> dmax2(ntest, 0.0);
+            parallel_for(ntest, [&](int64_t t) {
+                if (h->test.h_hi[t] <= h->test.h_lo[t]) return;
+                const double* a = &h->test.h_box[6 * t];
+                double m = 0.0;
+                for (int64_t i = np[t]; i < np[t + 1]; ++i) {
+                    const int32_t q = nn32[i];
+                    if (h->trial.h_hi[q] <= h->trial.h_lo[q]) continue;
+                    const double* b = &h->trial.h_box[6 * (size_t)q];
+                    double d2 = 0.0;
+                    for (int d = 0; d < 3; ++d) {
+                        const double g = std::max(a[3 + d] - b[d], b[3 + d] - a[d]);
+                        d2 += g * g;
+                    }
+                    m = std::max(m, d2);
+                }
+                dmax2[t] = m;
+            });
+            double dm2 = 0.0;
+            for (int64_t t = 0; t < ntest; ++t) dm2 = std::max(dm2, dmax2[t]);
+            h->near_turns = std::fabs(h->k) * 0.5 / M_PI * std::sqrt(dm2);
             csr_transpose(
                 ntest, np.data(), ntrial, [&](int64_t i) { return nn32[i]; }, tp,
                 [&](int64_t, int64_t t, int64_t pos) { tn[pos] = (int32_t)t; });

@@ -391,6 +391,46 @@ def test_matvec_negative_phase_sign(ctx):
             assert relerr(NC.adjoint(W) @ x, H.matvec(x, op=2)) <= bound
 
 
+def near_turns(k, pts, tree, part):
+    """The largest phase k r / (2 pi) a near pair can reach: the farthest
+    corners of the two point boxes (the bound the operator uses to form the
+    MIXED phase in FP32 up to one revolution)."""
+    box = [(pts[tree.values(t)].min(0), pts[tree.values(t)].max(0)) if len(tree.values(t)) else None
+           for t in range(tree.nnodes)]
+    d2 = 0.0
+    for t in range(tree.nnodes):
+        for s in part.nearnodes[part.nearptr[t]:part.nearptr[t + 1]]:
+            if box[t] is None or box[s] is None:
+                continue
+            g = np.maximum(box[t][1] - box[s][0], box[s][1] - box[t][0])
+            d2 = max(d2, float(g @ g))
+    return abs(k) / (2 * np.pi) * np.sqrt(d2)
+
+
+@pytest.mark.parametrize("k, short", [(1.5, True), (40.0, False)])
+def test_matvec_mixed_phase_forms(ctx, k, short):
+    """MIXED forms the near-field distance and phase in FP32 when every near
+    pair stays within one revolution, else in FP64: both forms within 1e-5 of
+    the FP64 oracle (N, T, C at 16, 3 and 1 right-hand sides)."""
+    pts = geometry.fibonacci_sphere(3000)
+    tree = nca_amd.TwoNTree(pts, minvalues=32)
+    bt = nca_amd.BlockTree(tree, tree)
+    assert (near_turns(k, pts, tree, nca_amd.interactions(bt, 1.0)) <= 1.0) == short
+    say(f"mixed phase forms k={k} (FP32 phase: {short})")
+    cmp = C.BatchedBottomUp(C.GeometricPivoting(), C.ACA(1e-4))
+    core = nca_amd.PetrovGalerkinNCACore(nca_amd.HelmholtzKernel(k), pts, pts, bt, cmp, cmp, ctx=ctx)
+    off, blocks = core.couplingstore.download()
+    H = O.OracleH2(core.desc, k, 1, pts, pts, off, blocks)
+    W = NC.H2Workspace(core)
+    core.set_precision(nca_amd.NCA_PREC_MIXED)
+    rng = np.random.default_rng(32)
+    for R in (16, 3, 1):
+        x = geometry.complex_gaussian(rng, (pts.shape[0], R))
+        assert relerr(W @ x, H.matvec(x)) <= 1e-5
+        assert relerr(NC.transpose(W) @ x, H.matvec(x, op=1)) <= 1e-5
+        assert relerr(NC.adjoint(W) @ x, H.matvec(x, op=2)) <= 1e-5
+
+
 def test_config_sphere65k_fp64(ctx):
     """The FP64 one-GPU configuration at its full size: N = 65,536 Fibonacci
     sphere, k = 1, octree minvalues 64, tol 1e-6, one seeded complex Gaussian
