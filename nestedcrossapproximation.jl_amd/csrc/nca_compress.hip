@@ -1166,7 +1166,7 @@ __global__ void __launch_bounds__(kP1Threads)
                 const int32_t* __restrict__ operm, const double* __restrict__ centers,
                 const int64_t* __restrict__ ffptr, const int32_t* __restrict__ ffnodes,
                 const int32_t* __restrict__ parent, const int64_t* __restrict__ pivptr, double weight,
-                int64_t* __restrict__ piv, int32_t* __restrict__ err) {
+                int64_t* __restrict__ piv, int32_t* __restrict__ err, const int32_t* __restrict__ items) {
     __shared__ int32_t scan[kP1MaxFar + 1];
     __shared__ int32_t fs[TREE ? kP1MaxFar : 1];
     __shared__ int64_t red[4];
@@ -1174,7 +1174,7 @@ __global__ void __launch_bounds__(kP1Threads)
     __shared__ int redq[4];
     __shared__ double pc[3];
     __shared__ int32_t pcur;
-    const int64_t it = blockIdx.x;
+    const int64_t it = items ? (int64_t)items[blockIdx.x] : (int64_t)blockIdx.x;
     const int tid = threadIdx.x;
     const int64_t n = pivptr[it + 1] - pivptr[it];
     if (n == 0) return;
@@ -1308,6 +1308,114 @@ __global__ void __launch_bounds__(kP1Threads)
             if (score > best) { best = score; arg = tid + kP1Threads * m; }
         }
         p1_best<false>(best, arg, redv, redq);
+        if (arg < 0) break;
+        cur = arg;
+    }
+}
+
+// The same selection with one wave per item, for explicit far fields whose
+// candidate pool fits kP1WavePer candidates per lane (candidate q in lane
+// q % 64, slot q / 64): no workgroup barriers, the chosen candidate's point
+// read by scalar loads. The host gives per item (items[w]) the per-node
+// count per and the pool size, and fscan, the exclusive scan of
+// min(points, per) over each item's far nodes (aligned with ffnodes). Same
+// candidates, same order of comparisons, so the same pivots as k_farpivots.
+constexpr int kP1WavePer = 16;
+constexpr int kP1WavePool = 64 * kP1WavePer;
+
+// best (value, index) over the wave: larger value (or smaller if MIN), then smaller index
+template <bool MIN>
+__device__ __forceinline__ int p1_wave_best(double v, int q) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(v, o);
+        const int oq = __shfl_xor(q, o);
+        const bool take = oq >= 0 && (q < 0 || (MIN ? ov < v : ov > v) || (ov == v && oq < q));
+        if (take) { v = ov; q = oq; }
+    }
+    return __builtin_amdgcn_readfirstlane(q);
+}
+
+__global__ void __launch_bounds__(256)
+    k_farpivots_wave(const double* __restrict__ pts, const int32_t* __restrict__ olo, const int32_t* __restrict__ ohi,
+                     const int32_t* __restrict__ operm, const double* __restrict__ centers,
+                     const int64_t* __restrict__ ffptr, const int32_t* __restrict__ ffnodes,
+                     const int32_t* __restrict__ fscan, const int32_t* __restrict__ items, int64_t nwork,
+                     const int32_t* __restrict__ iper, const int32_t* __restrict__ ipool,
+                     const int64_t* __restrict__ pivptr, double weight, int64_t* __restrict__ piv) {
+    __shared__ int32_t cand[4][kP1WavePool];
+    const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int64_t w = (int64_t)blockIdx.x * 4 + wv;
+    if (w >= nwork) return;
+    const int64_t it = items[w];
+    const int64_t n = pivptr[it + 1] - pivptr[it];
+    const int32_t* F = ffnodes + ffptr[it];
+    const int32_t* scan = fscan + ffptr[it];
+    const int nf = (int)(ffptr[it + 1] - ffptr[it]);
+    const int64_t per = iper[w];
+    const int pool = ipool[w];
+    const double* ct = centers + 3 * it;
+    const double c0 = ct[0], c1 = ct[1], c2 = ct[2];
+    double px[kP1WavePer], py[kP1WavePer], pz[kP1WavePer], d2c[kP1WavePer], mind[kP1WavePer];
+    double bv = 0.0;
+    int bq = -1;
+#pragma unroll
+    for (int m = 0; m < kP1WavePer; ++m) {
+        const int q = lane + 64 * m;
+        mind[m] = -1.0;
+        if (q < pool) {
+            // far node of candidate q: last f with scan[f] <= q
+            int lo = 0, hi = nf;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (scan[mid] <= q) lo = mid; else hi = mid;
+            }
+            const int32_t f = F[lo];
+            const int64_t sz = (int64_t)(ohi[f] - olo[f]), k = min(sz, per), j = q - scan[lo];
+            const int32_t p = operm[olo[f] + (int32_t)((j * sz) / k)];
+            cand[wv][q] = p;
+            px[m] = pts[3 * (int64_t)p];
+            py[m] = pts[3 * (int64_t)p + 1];
+            pz[m] = pts[3 * (int64_t)p + 2];
+            const double pp[3] = {px[m], py[m], pz[m]};
+            d2c[m] = max(p1_dist2(pp, c0, c1, c2), 1e-300);
+            mind[m] = __longlong_as_double(0x7ff0000000000000LL);  // +inf
+            if (bq < 0 || d2c[m] < bv) { bv = d2c[m]; bq = q; }
+        }
+    }
+    int cur = p1_wave_best<true>(bv, bq);
+    if (weight == 1.5) {
+#pragma unroll
+        for (int m = 0; m < kP1WavePer; ++m)
+            if (mind[m] >= 0.0) d2c[m] = d2c[m] * sqrt(d2c[m]);
+    } else if (weight != 1.0) {
+#pragma unroll
+        for (int m = 0; m < kP1WavePer; ++m)
+            if (mind[m] >= 0.0) d2c[m] = pow(d2c[m], weight);
+    }
+    __builtin_amdgcn_wave_barrier();
+    int64_t* out = piv + pivptr[it];
+    for (int64_t sel = 0; sel < n; ++sel) {
+        const int cl = cur & 63, cs = cur >> 6;
+        if (lane == cl) {
+#pragma unroll
+            for (int m = 0; m < kP1WavePer; ++m)
+                if (m == cs) mind[m] = -1.0;
+        }
+        const int32_t pc = __builtin_amdgcn_readfirstlane(cand[wv][cur]);
+        if (lane == 0) out[sel] = pc;
+        const double a0 = pts[3 * (int64_t)pc], a1 = pts[3 * (int64_t)pc + 1], a2 = pts[3 * (int64_t)pc + 2];
+        double best = -1.0;
+        int arg = -1;
+#pragma unroll
+        for (int m = 0; m < kP1WavePer; ++m) {
+            if (mind[m] < 0.0) continue;
+            const double pp[3] = {px[m], py[m], pz[m]};
+            const double d2 = p1_dist2(pp, a0, a1, a2);
+            if (d2 < mind[m]) mind[m] = d2;
+            const double score = mind[m] / d2c[m];
+            if (score > best) { best = score; arg = lane + 64 * m; }
+        }
+        arg = p1_wave_best<false>(best, arg);
         if (arg < 0) break;
         cur = arg;
     }
@@ -2137,7 +2245,7 @@ int32_t nca_farpivots(nca_ctx* ctx, const nca_points* other, int64_t nitems, con
         }
         const int64_t nff = ffptr[nitems] - ffptr[0];
         std::vector<int64_t> fp(nitems + 1), cnt(nitems, 0);
-        std::vector<int32_t> fn(nff);
+        std::vector<int32_t> fn(nff), fscan(nff), iper(nitems, 0), ipool(nitems, 0);
         // per item: far nodes (checked), pivot count, and the candidate pool
         // size the device will build (bounded by kP1Pool); 0 ok, else an error code
         std::vector<char> bad(nitems, 0);
@@ -2163,7 +2271,15 @@ int32_t nca_farpivots(nca_ctx* ctx, const nca_points* other, int64_t nitems, con
                 if (pool >= n || pool >= total) break;
                 per *= 2;
             }
-            if (pool > kP1Pool) bad[t] = 3;
+            if (pool > kP1Pool) { bad[t] = 3; return; }
+            // the one-wave kernel's inputs: per, pool and the scan of the per-node counts
+            iper[t] = (int32_t)per;
+            ipool[t] = (int32_t)pool;
+            int64_t run = 0;
+            for (int64_t k = fp[t]; k < fp[t + 1]; ++k) {
+                fscan[k] = (int32_t)run;
+                run += std::min<int64_t>(hi[fn[k]] - lo[fn[k]], per);
+            }
         });
         for (int64_t t = 0; t < nitems; ++t) {
             NCA_REQUIRE(bad[t] != 1, NCA_ERR_LIMIT, "nca_farpivots: more than 4096 far nodes in one far field");
@@ -2188,9 +2304,37 @@ int32_t nca_farpivots(nca_ctx* ctx, const nca_points* other, int64_t nitems, con
         dpiv.alloc((size_t)std::max<int64_t>(pivptr[nitems], 1));
         const int32_t zero = 0;
         derr.upload(&zero, 1, st);
-        if (nitems > 0)
-            hipLaunchKernelGGL(k_farpivots<false>, dim3((unsigned)nitems), dim3(kP1Threads), 0, st, other->xyz.p,
-                               dlo.p, dhi.p, dperm.p, dctr.p, dfp.p, dfn.p, nullptr, dptr.p, weight, dpiv.p, derr.p);
+        // items with pivots: one wave each when the pool fits kP1WavePool
+        // candidates, else one workgroup each
+        std::vector<int32_t> wave_items, wave_per, wave_pool, block_items;
+        for (int64_t t = 0; t < nitems; ++t) {
+            if (cnt[t] == 0) continue;
+            if (ipool[t] <= kP1WavePool) {
+                wave_items.push_back((int32_t)t);
+                wave_per.push_back(iper[t]);
+                wave_pool.push_back(ipool[t]);
+            } else {
+                block_items.push_back((int32_t)t);
+            }
+        }
+        DevBuf<int32_t> dwi, dwper, dwpool, dfs, dbi;
+        if (!wave_items.empty()) {
+            dwi.upload(wave_items, st);
+            dwper.upload(wave_per, st);
+            dwpool.upload(wave_pool, st);
+            dfs.upload(fscan, st);
+            const int64_t nw = (int64_t)wave_items.size();
+            hipLaunchKernelGGL(k_farpivots_wave, dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, st, other->xyz.p, dlo.p,
+                               dhi.p, dperm.p, dctr.p, dfp.p, dfn.p, dfs.p, dwi.p, nw, dwper.p, dwpool.p, dptr.p,
+                               weight, dpiv.p);
+            NCA_HIP(hipGetLastError());
+        }
+        if (!block_items.empty()) {
+            dbi.upload(block_items, st);
+            hipLaunchKernelGGL(k_farpivots<false>, dim3((unsigned)block_items.size()), dim3(kP1Threads), 0, st,
+                               other->xyz.p, dlo.p, dhi.p, dperm.p, dctr.p, dfp.p, dfn.p, nullptr, dptr.p, weight,
+                               dpiv.p, derr.p, dbi.p);
+        }
         NCA_HIP(hipGetLastError());
         if (pivptr[nitems] > 0) d2h(ctx, piv, dpiv.p, sizeof(int64_t) * pivptr[nitems]);
         NCA_HIP(hipStreamSynchronize(st));
@@ -2295,7 +2439,8 @@ int32_t nca_farpivots_tree(nca_ctx* ctx, const nca_points* other, int64_t nitems
         derr.upload(&zero, 1, st);
         if (nitems > 0)
             hipLaunchKernelGGL(k_farpivots<true>, dim3((unsigned)nitems), dim3(kP1Threads), 0, st, other->xyz.p,
-                               dlo.p, dhi.p, dperm.p, dctr.p, dfp.p, dfn.p, dpar.p, dptr.p, weight, dpiv.p, derr.p);
+                               dlo.p, dhi.p, dperm.p, dctr.p, dfp.p, dfn.p, dpar.p, dptr.p, weight, dpiv.p, derr.p,
+                               nullptr);
         NCA_HIP(hipGetLastError());
         int32_t e = 0;
         d2h(ctx, &e, derr.p, sizeof(int32_t));

@@ -109,3 +109,31 @@ def test_device_phase1_weight():
     for t in range(0, tr.nnodes, 37):
         farpts = set(np.concatenate([tr.values(f) for f in ff[1][ff[0][t]:ff[0][t + 1]]] or [[]]).tolist())
         assert set(dev[1][dev[0][t]:dev[0][t + 1]].tolist()) <= farpts
+
+
+def test_device_phase1_wave_and_block_items():
+    """Explicit far fields of very different sizes in one call: pools of at
+    most 1,024 candidates take the one-wave kernel, larger ones (more than
+    1,024 far nodes, one candidate each) the workgroup kernel; every item's
+    pivots equal the host selection bit for bit (weights 1.5 and 1)."""
+    from types import SimpleNamespace
+    pts = geometry.fibonacci_sphere(60000)
+    tr = nca_amd.TwoNTree(pts, minvalues=16)
+    leaves = np.array([t for t in range(tr.nnodes) if tr.isleaf(t)])
+    assert len(leaves) > 4100
+    rng = np.random.default_rng(7)
+    big = np.array([t for t in range(1, tr.nnodes) if 60 <= len(tr.values(t)) <= 400])
+    items = rng.choice(big, 8, replace=False)
+    sizes = [3, 100, 900, 1024, 1100, 2000, 4096, 40]
+    lists = [np.sort(rng.choice(leaves, nf, replace=False)) for nf in sizes]
+    ffptr = np.zeros(len(items) + 1, np.int64)
+    ffptr[1:] = np.cumsum([len(v) for v in lists])
+    ffnodes = np.concatenate(lists).astype(np.int64)
+    blocks = SimpleNamespace(tree=tr, node=np.asarray(items, np.int64), nnodes=len(items))
+    A = nca_amd.HIPKernelMatrix(nca_amd.HelmholtzKernel(1.0), pts, pts)
+    say("wave and block items")
+    for weight in (1.5, 1.0):
+        for maxrank in (50, 12):
+            host = W.farpivots(blocks, tr, ffptr, ffnodes, maxrank, C.GeometricPivoting(weight, device=False))
+            dev = W.farpivots(blocks, tr, ffptr, ffnodes, maxrank, C.GeometricPivoting(weight), A.trial)
+            assert np.array_equal(host[0], dev[0]) and np.array_equal(host[1], dev[1]), (weight, maxrank)
