@@ -74,7 +74,7 @@ class Hip:
         assert self.lib.hipDeviceSynchronize() == 0
 
 
-def stage_bytes(st, R, M, N, mixed, beta=False):
+def stage_bytes(st, R, M, N, mixed, beta=False, h16=False):
     """Algorithmic HBM bytes of each stage of one forward sweep over R
     right-hand sides, counting what the kernels of the 16-column chunks move:
     every operator entry once (s_op = 16 B FP64, 8 B MIXED), the gathered
@@ -83,15 +83,21 @@ def stage_bytes(st, R, M, N, mixed, beta=False):
     coupling writes its c32 result, which the downward pass reads), no vector
     clears (the MFMA stages write every coefficient once), and y read only
     when beta is nonzero. Leaf and internal coefficient counts come from the
-    ranks (``coeffs_*_leaves``)."""
+    ranks (``coeffs_*_leaves``). h16 (MIXED near field on the f16 MFMA): the
+    gather also reads its c32 copy once and writes the 8-byte split copy the
+    near field reads instead, and the per-workgroup column maxima (4 B per 64
+    points and column, written and read)."""
     s_op = 8 if mixed else 16
     s_x = 8 if mixed else 16
     Ks, Kd = st["coeffs_trial"], st["coeffs_test"]
     Ks_leaf, Kd_leaf = st["coeffs_trial_leaves"], st["coeffs_test_leaves"]
     Ks_int, Kd_int = Ks - Ks_leaf, Kd - Kd_leaf
     down_vec = (8 + 16) * R * Kd if mixed else (16 + 16) * R * Kd
+    gather = 16 * R * N + s_x * R * N + 4 * N
+    if h16:
+        gather += (8 + 8) * R * N + 2 * 4 * R * ((N + 63) // 64)
     return {
-        "gather": 16 * R * N + s_x * R * N + 4 * N,
+        "gather": gather,
         "leaf_projection": s_op * st["basis_entries_trial"] + s_x * R * N + s_x * R * Ks_leaf,
         "upward": s_op * st["transfer_entries_trial"] + s_x * R * (Ks + Ks_int),
         "coupling": s_op * st["coupling_entries"] + 4 * st["coupling_columns"] + s_x * R * Ks
@@ -100,6 +106,14 @@ def stage_bytes(st, R, M, N, mixed, beta=False):
         "leaf_expansion_near": s_op * st["basis_entries_test"] + 16 * R * Kd_leaf + s_x * R * N + 24 * (M + N)
         + 12 * st["near_pairs"] + (2 if beta else 1) * 16 * R * M,
     }
+
+
+def near_f16(core):
+    """True when the core's products in its current precision evaluate the
+    near field on the f16 MFMA (nca_h2_near_form form 2)."""
+    f = ctypes.c_int32()
+    nca_amd.runtime.call(nca_amd.hip_lib().nca_h2_near_form(core.handle, ctypes.byref(f), None, None))
+    return f.value == 2
 
 
 def operator_counts(core, tree, st):
@@ -235,7 +249,8 @@ def main():
         hip.d2h(Y, dy)
         results[label] = (ms, R / (ms * 1e-3), prof, Y)
         mixed = label == "mixed"
-        sbytes = stage_bytes(st, R, M, N, mixed)
+        h16 = near_f16(core)
+        sbytes = stage_bytes(st, R, M, N, mixed, h16=h16)
         b_alg = sum(sbytes.values())
         # transpose and adjoint sweeps (same operator, the other side's stages)
         ms_t = timed(1, R, max(args.steps // 2, 1), 1)
@@ -262,7 +277,7 @@ def main():
 
     head = "mixed"
     ms_per_step, value, prof, _ = results[head]
-    sbytes = stage_bytes(st, R, M, N, True)
+    sbytes = stage_bytes(st, R, M, N, True, h16=near_f16(core))
     b_alg = sum(sbytes.values())
     sweep_gbs = b_alg / (ms_per_step * 1e-3) / 1e9
     stages = per_prec[head]["stages"]
@@ -281,18 +296,23 @@ def main():
         return None
 
     if dom == "leaf_expansion_near":
-        # complex FP32 MFMA: 8 flop per near pair and right-hand side, plus the basis rows
+        # complex products at FP32 accuracy: 8 flop per near pair and right-hand
+        # side, plus the basis rows, against the FP32 MFMA peak
         flops = 8.0 * R * (st["near_entries"] + st["basis_entries_test"])
         achieved = flops / (prof[5] * 1e-3) / 1e12
-        roofline = {"kernel": "k_leaf_out_mfma<PrecF32> (leaf expansion + near field)", "bound": "mfma",
+        if near_f16(core):
+            kname, kpre = ("k_leaf_out_h16 (leaf expansion + near field; FP32 kernel values and right-hand sides "
+                           "split into f16 parts, three f16 MFMA products per real product)"), "k_leaf_out_h16"
+        else:
+            kname, kpre = "k_leaf_out_mfma<PrecF32> (leaf expansion + near field)", "k_leaf_out_mfma<PrecF32"
+        roofline = {"kernel": kname, "bound": "mfma",
                     "achieved": round(achieved, 2), "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4),
-                    "traffic": traffic_of("k_leaf_out_mfma<PrecF32"),
-                    "note": "MAC flops only (8 per near pair per RHS); the kernel evaluations "
-                            f"({st['near_entries']} per sweep, VALU) are not counted. VALU and MFMA issue "
-                            "serially on a SIMD, so the kernel is issue-bound (profiles/"
-                            "h2_matvec_sphere1m_rhs16_sq_counters.md, profiles/"
-                            "h2_matvec_sphere1m_rhs16_memory_pipeline.md). traffic: HBM bytes per launch "
+                    "traffic": traffic_of(kpre),
+                    "note": "MAC flops at FP32 accuracy only (8 per near pair per RHS), against the FP32 MFMA "
+                            f"peak; the kernel evaluations ({st['near_entries']} per sweep, VALU) are not "
+                            "counted. VALU and MFMA issue serially on a SIMD, so the kernel is issue-bound "
+                            "(profiles/h2_matvec_sphere1m_rhs16_sq_counters.md). traffic: HBM bytes per launch "
                             "from FETCH_SIZE x 2 + WRITE_SIZE (profiles/h2_matvec_sphere1m_rhs16_hbm_traffic.md)"}
     else:
         i = names.index(dom)
@@ -622,7 +642,7 @@ def bench_wideband(args):
             prof += sm / 3
         Y = np.zeros((N, R), np.complex128, order="F")
         hip.d2h(Y, dy)
-        sb = stage_bytes(st, R, N, N, label == "mixed")
+        sb = stage_bytes(st, R, N, N, label == "mixed", h16=near_f16(core))
         out[label] = {"ms_per_sweep": round(ms, 4), "matvec_per_s": round(R / (ms * 1e-3), 2),
                       "sweep_alg_bytes": int(sum(sb.values())),
                       "sweep_hbm_gbs": round(sum(sb.values()) / (ms * 1e-3) / 1e9, 1),

@@ -513,6 +513,17 @@ typedef struct {
 } nca_h2_stats;
 int32_t nca_h2_get_stats(const nca_h2* h2, nca_h2_stats* out);
 
+/* No counterpart in the reference: how the operator's forward products
+ * evaluate the near field in the current precision, for measurement and
+ * tests. form = 0: FP64 kernel values on the FP64 MFMA; 1: MIXED, FP32 values
+ * on the f32 MFMA (or the stored near field); 2: MIXED, FP32 values split
+ * into f16 parts on the f16 MFMA (the MIXED default when the near pairs'
+ * distances span at most 2^17; error per product term below 2^-21 of it).
+ * rmin, rmax (may be NULL): the smallest near-pair distance above the
+ * zero-distance guard and the largest, measured once the MIXED copies exist
+ * (before: 0 and the bound from the point boxes). */
+int32_t nca_h2_near_form(nca_h2* h2, int32_t* form, double* rmin, double* rmax);
+
 #ifdef __cplusplus
 }
 #endif

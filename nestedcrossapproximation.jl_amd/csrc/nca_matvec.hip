@@ -36,6 +36,7 @@ struct SideDev {
     DevBuf<int32_t> ncptr;
     const int32_t* node_cptr() const { return ncptr.p ? ncptr.p : cptr.p; }
     DevBuf<double> pts;  // coordinates in tree order, 3 per position
+    DevBuf<double> pts_h16;  // the same times t (the MIXED f16 near field; made with the c32 copies)
     // leaf bases
     int64_t nbasis = 0;
     DevBuf<int32_t> bnodes;
@@ -99,6 +100,14 @@ struct nca_h2 {
     int sign = 1;
     // the largest phase |k| r / (2 pi) over the near pairs (from the point boxes)
     double near_turns = 0.0;
+    // the largest distance of a near pair from the point boxes (bounds the
+    // phase) and, with the c32 copies, the measured smallest (above the
+    // zero-distance guard) and largest
+    double near_rmax = 0.0, near_rmin = 0.0, near_rmax_pairs = 0.0;
+    // MIXED near field on f16 MFMA: 1 when the near pairs allow it (distances
+    // within 2^17 of each other), with the point scale h16_t = 2^-e
+    int h16 = 0;
+    double h16_t = 1.0;
     int precision = NCA_PREC_FP64;
     int64_t M = 0, N = 0;
     nca::SideDev test, trial;
@@ -148,6 +157,10 @@ struct nca_h2 {
     // ws_yc: the c32 coupling results of MIXED forward products
     nca::DevBuf<double2> ws_xs, ws_xh, ws_yh, ws_xin, ws_yio;
     nca::DevBuf<float2> ws_xs32, ws_xh32, ws_yc;
+    // the f16 near field's split right-hand sides, the gather's per-workgroup
+    // column maxima and the column scales (s_q, 1 / s_q)
+    nca::DevBuf<uint2> ws_xs16;
+    nca::DevBuf<float> ws_xpart, ws_xsc;
     // The device workspace the stage kernels of the product in flight use:
     // the operator's own buffers above, or carved from the static scratch
     // array of nca_h2_matvec_device_noalloc.
@@ -159,6 +172,9 @@ struct nca_h2 {
         float2* yc = nullptr;     // c32 coupling results (MIXED forward)
         float2* xh32 = nullptr;   // compact c32 copy of a 16-column chunk's source coefficients
         double2* ynear = nullptr; // stored near field applied, per point (tree order), FP64
+        uint2* xs16 = nullptr;    // split right-hand sides of the f16 near field (MIXED)
+        float* xpart = nullptr;   // per gather workgroup and column: largest |re|, |im|
+        float* xsc = nullptr;     // per column: s_q, 1 / s_q
     } ws;
     // c32 copy of the coupling blocks for the MIXED path
     nca::DevBuf<float2> S32;
@@ -171,11 +187,13 @@ struct nca_h2 {
     // near_k3 / couple_k3 [precision]: complex products on MFMA with three
     //   real products (Karatsuba) instead of four (NCA_MFMA_K3, NCA_MIXED_K3);
     //   in FP32 three raise the MIXED error against FP64 from 1.0e-6 to 1.5e-6;
+    //   near_k3 = 2 (MIXED, the default): the near field on f16 MFMA with
+    //   split operands when the operator allows it (h16), else as 1;
     // near_xcd / couple_xcd: work_block order (NCA_ORDER_NEAR, NCA_ORDER_COUPLE);
     // leaf_order: order_leaves mode (NCA_LEAF_ORDER);
     // couple_order: order_coupling_tiles mode (NCA_COUPLE_ORDER);
     // near_stagger: near-field walks start at a tile-dependent range (NCA_NEAR_STAGGER).
-    int near_k3[2] = {1, 1}, couple_k3[2] = {1, 1};
+    int near_k3[2] = {1, 2}, couple_k3[2] = {1, 1};
     int near_xcd = 0, couple_xcd = 0, leaf_order = 2, couple_order = 3, near_stagger = 0;
     std::vector<int32_t> h_ncols;  // host copy: columns of every test node's coupling row
     // Every entry point that touches the workspace, the launch plans or the
@@ -284,11 +302,13 @@ __global__ void __launch_bounds__(256)
 
 // xs[iperm[i]*R + q] = x[i + q*ldx], moved through LDS in tiles of 64
 // consecutive input points x 16 columns: reads run along the points of one
-// column, writes along the 16 columns of one point.
+// column, writes along the 16 columns of one point. xpart (optional): per
+// workgroup row blockIdx.x and column q < R the largest |re|, |im| of the
+// tile's values rounded to FP32, at xpart[blockIdx.x R + q].
 template <class XT>
 __global__ void __launch_bounds__(256)
     k_gather_tiled(const double2* __restrict__ x, int64_t ldx, const int32_t* __restrict__ iperm, int64_t npts,
-                   int64_t R, int64_t Rw, XT* __restrict__ xs) {
+                   int64_t R, int64_t Rw, XT* __restrict__ xs, float* __restrict__ xpart) {
     // R live columns, Rw >= R columns written (the padding ones as zeros)
     __shared__ double2 tile[16][65];
     const int t = threadIdx.x;
@@ -300,6 +320,19 @@ __global__ void __launch_bounds__(256)
         tile[ql][t & 63] = (p < npts && q < R) ? x[p + q * ldx] : make_double2(0.0, 0.0);
     }
     __syncthreads();
+    if (xpart) {
+        // column t / 16 of the tile, points t % 16 + 16 r, then over the 16 lanes
+        const int ql = t >> 4;
+        float m = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const double2 v = tile[ql][(t & 15) + 16 * r];
+            m = fmaxf(m, fmaxf(fabsf((float)v.x), fabsf((float)v.y)));
+        }
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 16));
+        if ((t & 15) == 0 && c0 + ql < R) xpart[(int64_t)blockIdx.x * R + c0 + ql] = m;
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int pl = (t >> 4) + 16 * r;
@@ -465,11 +498,24 @@ struct NearK {
     double k4pi;
     float k2pif;
     int phase32;
+    // f16 path (points scaled by t): the zero-distance guard's 1e-30 and
+    // 2e-30 times t^2, and t / (4 pi)
+    double gadd, gmin;
+    float tsc;
 };
 constexpr double kPhase32Turns = 1.0;
 // turns: the largest phase k r / (2 pi) over the operator's near pairs
 inline NearK near_consts(double k, double turns) {
-    return NearK{k, k * (0.5 / M_PI), k * (0.25 / M_PI), (float)(k * (0.5 / M_PI)), turns <= kPhase32Turns ? 1 : 0};
+    return NearK{k,   k * (0.5 / M_PI), k * (0.25 / M_PI), (float)(k * (0.5 / M_PI)), turns <= kPhase32Turns ? 1 : 0,
+                 1e-30, 2e-30,           (float)(0.25 / M_PI)};
+}
+// The same for points scaled by t (a power of two): k / t, guards times t^2.
+inline NearK near_consts_scaled(double k, double turns, double t) {
+    NearK nk = near_consts(k / t, turns);
+    nk.gadd = 1e-30 * t * t;
+    nk.gmin = 2e-30 * t * t;
+    nk.tsc = (float)(t * (0.25 / M_PI));
+    return nk;
 }
 
 // Table of (cos, sin)(n pi / 128), n = 0..255, for the FP64 near field.
@@ -737,6 +783,10 @@ __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uin
 template <>
 __device__ __forceinline__ int buf_load<int>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     return (int)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+template <>
+__device__ __forceinline__ uint2 buf_load<uint2>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
@@ -1037,6 +1087,509 @@ __global__ void __launch_bounds__(256, 4)
 #pragma unroll
         for (int c = 0; c < NC; ++c) acc[c].store(emitter(c));
     }
+}
+
+// ---------------------------------------------------------------------------
+// MIXED near field on f16 MFMA (v_mfma_f32_16x16x32_f16: eight times the K
+// depth of the f32 16x16x4 product at about 0.6 of its issue time). Every
+// FP32 operand a is split into f16 parts ah = f16(a), al = f16(a - ah)
+// (|a - ah - al| <= 2^-22 |a| while both parts stay normal), and every real
+// product is formed as ah bh + ah bl + al bh with FP32 accumulation (the
+// dropped al bl is below 2^-22 |a b|). The parts stay in f16's range because
+// the kernel is evaluated on copies of the points scaled by t = 2^-e, chosen
+// so that 1/r' = 1/(t r) lies in (2^14, 2^15] at the closest near pair (the
+// operator takes this path only when its near pairs span at most 2^17 in
+// distance, so 1/r' >= 2^-3 at the farthest), and the gathered right-hand
+// sides are scaled per column by s_q = 2^m (largest |re|, |im| in [2^14,
+// 2^15)); the tile's sums are scaled back by t / (4 pi s_q).
+//
+// Gathered right-hand sides for this path (xs16): per tree position and
+// column one 8-byte entry (re hi | im hi << 16, re lo | im lo << 16), the
+// layout of the c32 copy. A tile's near field is walked per merged source
+// range in blocks of sub-steps whose kernel values and right-hand sides fill
+// the 32 K-slots of one f16 product: 16-row tiles take 8 sub-steps of 4
+// sources (lane (i, kk) evaluates row i x source 4 m + kk, K-slot 8 kk + m),
+// 8-row tiles 4 sub-steps of 8 sources and 4-row tiles 2 sub-steps of 16
+// (row i % H, source S m + 4 (i / H) + kk; the other sources of the lane's
+// row come from the lanes that evaluated them, by DPP), in the real
+// embedding of the f32 kernel (MFMA rows [0, H) the real parts, [H, 2 H) the
+// imaginary parts). The last block of a range is padded with zero slots;
+// sources past a range end read zero right-hand sides (buffer range check).
+
+typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ flt4 mma_h16(u32x4 a, u32x4 b, flt4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h16x8, a), __builtin_bit_cast(h16x8, b), c, 0,
+                                                  0, 0);
+}
+// c + a b for split operands: ah bh + ah bl + al bh
+__device__ __forceinline__ flt4 mma_h16x3(u32x4 ah, u32x4 al, u32x4 bh, u32x4 bl, flt4 c) {
+    c = mma_h16(ah, bh, c);
+    c = mma_h16(ah, bl, c);
+    return mma_h16(al, bh, c);
+}
+// The f16 product rounds its result with few guard bits below the exponent
+// of the larger of the accumulator and the products (measured: up to 13 ulp
+// of a large accumulator, against 1.9 for the f32 16x16x4 product) and,
+// from a zero accumulator, with a bias toward zero (0.6 ulp on average): a
+// block's products therefore go to a zero accumulator, and the block's sum
+// is added to the running sum by VALU adds (round to nearest).
+__device__ __forceinline__ void add_block(flt4& acc, flt4 blk) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] += blk[v];
+}
+// f16 parts of two FP32 values: hi = (f16(a), f16(b)), lo = (f16(a - hi.x),
+// f16(b - hi.y)), in three instructions: v_cvt_pk_f16_f32 for hi, then
+// v_fma_mix (a x 1 - hi, exact, rounded once to f16) into each half of lo.
+// Written out so that hi is the rounding of the given FP32 value itself (the
+// compiler may otherwise round a product feeding the conversion to f16
+// directly for one use and through FP32 for another; the two can differ by
+// one f16 ulp, and lo would then not complete hi).
+__device__ __forceinline__ void split_h16(float a, float b, uint32_t& hi, uint32_t& lo) {
+    asm("v_cvt_pk_f16_f32 %0, %1, %2" : "=v"(hi) : "v"(a), "v"(b));
+    asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2 op_sel_hi:[0,0,1]" : "=v"(lo) : "v"(a), "v"(hi));
+    asm("v_fma_mixhi_f16 %0, %1, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(lo) : "v"(b), "v"(hi));
+}
+// (low half of u, low half of v) and (high half of u, high half of v)
+__device__ __forceinline__ uint32_t pack_lo(uint32_t u, uint32_t v) { return __builtin_amdgcn_perm(v, u, 0x05040100u); }
+__device__ __forceinline__ uint32_t pack_hi(uint32_t u, uint32_t v) { return __builtin_amdgcn_perm(v, u, 0x07060302u); }
+// DPP row rotation of a 32-bit value (lane i of a 16-lane row gets lane (i + S) mod 16)
+template <int S>
+__device__ __forceinline__ uint32_t rot16u(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x120 + (16 - S), 0xf, 0xf, false);
+}
+
+// 4 pi G on the scaled points (NEG, GUARD and PH32 as for helmholtz_f32, the
+// guard thresholds from nk); PH32: the phase is below one revolution plus
+// rounding, inside the hardware sin/cos domain without reduction.
+template <bool NEG, bool GUARD, bool PH32>
+__device__ __forceinline__ float2 helmholtz_h16(const NearK& nk, double x0, double x1, double x2, double y0,
+                                                double y1, double y2) {
+    const double dx = x0 - y0, dy = x1 - y1, dz = x2 - y2;
+    const double r2 = fma(dz, dz, fma(dy, dy, GUARD ? fma(dx, dx, nk.gadd) : dx * dx));
+    const float r2f = (float)r2;
+    const float rf = __builtin_amdgcn_rsqf(r2f);
+    float f;
+    if constexpr (PH32) {
+        f = (r2f * rf) * nk.k2pif;
+    } else {
+        const double ri = (double)rf;
+        const double r0 = r2 * ri;
+        const double e = fma(-r0, r0, r2);
+        const double t = fma(e, ri * nk.k4pi, r0 * nk.k2pi);
+        f = (float)__builtin_amdgcn_fract(t);
+    }
+    const float c = __builtin_amdgcn_cosf(f), s = __builtin_amdgcn_sinf(f);
+    const float sc = !GUARD || r2 > nk.gmin ? rf : 0.0f;
+    return make_float2(c * sc, NEG ? -(s * sc) : s * sc);
+}
+
+// One block of a 16-row tile's walk: nsub (1..8) sub-steps of 4 sources at
+// byte offsets sp (points) and sx (right-hand sides) of the range; P, Q, I
+// accumulate Gr xr, Gi xi and Gr xi + Gi xr.
+template <int NC, bool NEG, bool GU, bool PH, bool FULL>
+__device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (&I)[NC], const NearK& nk, double x0,
+                                            double x1, double x2, __amdgpu_buffer_rsrc_t rp,
+                                            __amdgpu_buffer_rsrc_t rx, uint32_t sp, uint32_t sx, uint32_t vp,
+                                            const uint32_t (&vx)[NC][1], uint32_t rowb, int nsub) {
+    double pq[8];
+    uint2 xb[NC][8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        if (FULL || m < nsub) {
+            pq[m] = buf_load<double>(rp, vp, sp + (uint32_t)m * 96u);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) xb[c][m] = buf_load<uint2>(rx, vx[c][0], sx + (uint32_t)m * 4u * rowb);
+        } else {
+            pq[m] = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) xb[c][m] = make_uint2(0u, 0u);
+        }
+    }
+    u32x4 arh, aih, arl, ail;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        float2 g[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int m = 2 * v + e;
+            if (FULL || m < nsub)
+                g[e] = helmholtz_h16<NEG, GU, PH>(nk, x0, x1, x2, row_double<0>(pq[m]), row_double<1>(pq[m]),
+                                                  row_double<2>(pq[m]));
+            else
+                g[e] = make_float2(0.0f, 0.0f);
+        }
+        uint32_t h, l;
+        split_h16(g[0].x, g[1].x, h, l);
+        arh[v] = h;
+        arl[v] = l;
+        split_h16(g[0].y, g[1].y, h, l);
+        aih[v] = h;
+        ail[v] = l;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        u32x4 brh, bih, brl, bil;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            brh[v] = pack_lo(xb[c][2 * v].x, xb[c][2 * v + 1].x);
+            bih[v] = pack_hi(xb[c][2 * v].x, xb[c][2 * v + 1].x);
+            brl[v] = pack_lo(xb[c][2 * v].y, xb[c][2 * v + 1].y);
+            bil[v] = pack_hi(xb[c][2 * v].y, xb[c][2 * v + 1].y);
+        }
+        const flt4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+        add_block(P[c], mma_h16x3(arh, arl, brh, brl, z));
+        add_block(Q[c], mma_h16x3(aih, ail, bih, bil, z));
+        add_block(I[c], mma_h16x3(aih, ail, brh, brl, mma_h16x3(arh, arl, bih, bil, z)));
+    }
+}
+
+// One block of an 8-row (H = 8) or 4-row (H = 4) tile's walk in the real
+// embedding: nsub (1 .. 32 / S) sub-steps of S = 64 / H sources. Lane (i, kk)
+// evaluates row i % H x source S m + 4 (i / H) + kk and receives the other
+// sources of its row's sub-step from lanes i +- H (rotations); sel1 / sel2
+// pick, by v_perm, the parts of two slots for A1 (real rows: Gr, imaginary
+// rows: Gi) and A2 (real rows: Gi, imaginary rows: Gr) in slot order (imrow:
+// the lane's MFMA row holds an imaginary part). D1 += A1 xr, D2 += A2 xi; the
+// real rows are D1 - D2, the imaginary rows D1 + D2.
+template <int H, int NC, bool NEG, bool GU, bool PH, bool FULL>
+__device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], const NearK& nk, double x0, double x1,
+                                              double x2, __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rx,
+                                              uint32_t sp, uint32_t sx, uint32_t vp,
+                                              const uint32_t (&vx)[NC][16 / H], uint32_t rowb, int nsub, bool imrow,
+                                              uint32_t sel1, uint32_t sel2) {
+    constexpr int G = 16 / H, S = 64 / H, NS = 32 / S;
+    double pq[NS];
+    uint2 xb[NC][NS][G];
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+        if (FULL || m < nsub) {
+            pq[m] = buf_load<double>(rp, vp, sp + (uint32_t)(m * S * 24));
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int q = 0; q < G; ++q)
+                    xb[c][m][q] = buf_load<uint2>(rx, vx[c][q], sx + (uint32_t)(m * S) * rowb);
+        } else {
+            pq[m] = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int q = 0; q < G; ++q) xb[c][m][q] = make_uint2(0u, 0u);
+        }
+    }
+    u32x4 a1h, a2h, a1l, a2l;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+        float2 g = make_float2(0.0f, 0.0f);
+        if (FULL || m < nsub)
+            g = helmholtz_h16<NEG, GU, PH>(nk, x0, x1, x2, quad_double<0>(pq[m]), quad_double<1>(pq[m]),
+                                           quad_double<2>(pq[m]));
+        uint32_t oh, ol;
+        split_h16(g.x, g.y, oh, ol);
+        if constexpr (H == 8) {
+            // slots 2 m (lanes i < 8 own) and 2 m + 1 (lanes i >= 8 own); the order is in the selectors
+            const uint32_t ph = rot16u<8>(oh), pl = rot16u<8>(ol);
+            a1h[m] = __builtin_amdgcn_perm(ph, oh, sel1);
+            a2h[m] = __builtin_amdgcn_perm(ph, oh, sel2);
+            a1l[m] = __builtin_amdgcn_perm(pl, ol, sel1);
+            a2l[m] = __builtin_amdgcn_perm(pl, ol, sel2);
+        } else {
+            // slots 4 m + q from the lane 4 q + i % 4 of the row: rotation r_d gives
+            // group (i / 4 + d) mod 4; real rows (i / 4 = 0) pair (r0, r1), (r2, r3),
+            // imaginary rows (i / 4 = 1) (r3, r0), (r1, r2)
+            const uint32_t r1h = rot16u<4>(oh), r2h = rot16u<8>(oh), r3h = rot16u<12>(oh);
+            const uint32_t r1l = rot16u<4>(ol), r2l = rot16u<8>(ol), r3l = rot16u<12>(ol);
+            const uint32_t xh = imrow ? r3h : r1h, yhh = imrow ? r1h : r3h;
+            const uint32_t xl = imrow ? r3l : r1l, yl = imrow ? r1l : r3l;
+            a1h[2 * m] = __builtin_amdgcn_perm(xh, oh, sel1);
+            a2h[2 * m] = __builtin_amdgcn_perm(xh, oh, sel2);
+            a1l[2 * m] = __builtin_amdgcn_perm(xl, ol, sel1);
+            a2l[2 * m] = __builtin_amdgcn_perm(xl, ol, sel2);
+            a1h[2 * m + 1] = __builtin_amdgcn_perm(yhh, r2h, sel1);
+            a2h[2 * m + 1] = __builtin_amdgcn_perm(yhh, r2h, sel2);
+            a1l[2 * m + 1] = __builtin_amdgcn_perm(yl, r2l, sel1);
+            a2l[2 * m + 1] = __builtin_amdgcn_perm(yl, r2l, sel2);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        u32x4 brh, bih, brl, bil;
+#pragma unroll
+        for (int m = 0; m < NS; ++m)
+#pragma unroll
+            for (int q = 0; q < G; q += 2) {
+                const int v = m * (G / 2) + q / 2;
+                brh[v] = pack_lo(xb[c][m][q].x, xb[c][m][q + 1].x);
+                bih[v] = pack_hi(xb[c][m][q].x, xb[c][m][q + 1].x);
+                brl[v] = pack_lo(xb[c][m][q].y, xb[c][m][q + 1].y);
+                bil[v] = pack_hi(xb[c][m][q].y, xb[c][m][q + 1].y);
+            }
+        const flt4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+        add_block(D1[c], mma_h16x3(a1h, a1l, brh, brl, z));
+        add_block(D2[c], mma_h16x3(a2h, a2l, bih, bil, z));
+    }
+}
+
+// Stage 5 + near field of MIXED products on the f16 path for 16 NC
+// right-hand sides (the tiles, tile order and output of k_leaf_out_mfma;
+// dpts / spts: the scaled point copies; xs16: the split right-hand sides of
+// all R columns; xsc: per column s_q and 1 / s_q). The near sums are scaled
+// back (t / (4 pi s_q)), then the basis rows are added on the f32 MFMA.
+template <bool FWD, bool CJ, bool NEG, int NC>
+__global__ void __launch_bounds__(256, 4)
+    k_leaf_out_h16(const int32_t* __restrict__ tile_leaf, const int32_t* __restrict__ tile_row0, int64_t ntiles,
+                   const int32_t* __restrict__ leaves, const int32_t* __restrict__ leafbasis,
+                   const int64_t* __restrict__ boff, const float2* __restrict__ bblocks,
+                   const int32_t* __restrict__ dlo, const int32_t* __restrict__ dhi,
+                   const int32_t* __restrict__ dperm, const int32_t* __restrict__ dcptr,
+                   const double* __restrict__ dpts, const int64_t* __restrict__ nrptr,
+                   const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
+                   const double* __restrict__ spts, const double2* __restrict__ yh, const uint2* __restrict__ xs16,
+                   const float* __restrict__ xsc, int64_t R, int64_t q0, NearK nk, double2 alpha, double2 beta,
+                   double2* __restrict__ y, int64_t ldy, int order, int stagger, int nq) {
+    using P = PrecF32;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
+    if (w >= ntiles) return;
+    const int li = tile_leaf[w], row0 = tile_row0[w];
+    const int t = leaves[li], bi = leafbasis[li];
+    const int l0 = dlo[t], nt = dhi[t] - l0;
+    const int nrows = min(16, nt - row0);
+    const int i = lane & 15, kk = lane >> 4, j = lane & 15;
+    const int64_t e0 = nrptr[t], e1 = nrptr[t + 1];
+    const int nr = (int)(e1 - e0);
+    const int rot = stagger && nr ? (w * 7) % nr : 0;
+    int jl[NC];
+    float osc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        jl[c] = min(16 * c + j, nq - 1);
+        osc[c] = nk.tsc * xsc[2 * (q0 + jl[c]) + 1];
+    }
+    const int rt = dcptr[t + 1] - dcptr[t];
+    const float2* B = bi >= 0 ? bblocks + boff[bi] : nullptr;
+    const uint2* xsq = xs16 + q0;
+    const uint32_t rowb = (uint32_t)R * 8u;
+    // basis rows: acc += A B with A = U_t (or V_t^T) rows, B = yh_t
+    auto basis = [&](auto& acc, int rsel) {
+        if (!B) return;
+        const int rowc = row0 + min(rsel, nrows - 1);
+        const double2* yt = yh + (int64_t)dcptr[t] * R + q0;
+        for (int k0 = 0; k0 < rt; k0 += 4) {
+            const int k = k0 + kk;
+            const int kc = min(k, rt - 1);
+            const float2 a =
+                conjif<CJ, P>(B[FWD ? (int64_t)rowc + (int64_t)kc * nt : (int64_t)kc + (int64_t)rowc * rt]);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                float2 b = P::cvt(yt[(int64_t)kc * R + jl[c]]);
+                if (k >= rt) b = czero2<P>();
+                acc[c].mma(a, b);
+            }
+        }
+    };
+    auto emitter = [&](int c) {
+        return [&, c](int row, double2 v) {
+            const int col = 16 * c + j;
+            if (row < nrows && col < nq) {
+                double2* out = y + (int64_t)dperm[l0 + row0 + row] + (q0 + col) * ldy;
+                double2 val = cmul(alpha, v);
+                if (beta.x != 0.0 || beta.y != 0.0) val = cadd(val, cmul(beta, *out));
+                *out = val;
+            }
+        };
+    };
+    // the walk over the merged source ranges, blocks of B16 sub-steps of S sources
+    auto walk = [&](auto hval, auto ph, auto& A1, auto& A2, auto& A3, int rsel, bool imrow, uint32_t sel1,
+                    uint32_t sel2) {
+        constexpr int H = decltype(hval)::value;
+        constexpr bool PH = decltype(ph)::value;
+        constexpr int G = 16 / H, S = 64 / H, NB = 32 / S;
+        const double* xp = dpts + 3 * (int64_t)(l0 + row0 + min(rsel, nrows - 1));
+        const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
+        const int ysrc = 4 * ((lane & 15) / H) + kk;
+        const uint32_t vp = near_point_offset<G>(lane, kk, ysrc);
+        uint32_t vx[NC][G];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int q = 0; q < G; ++q) vx[c][q] = ((uint32_t)(4 * q + kk) * (uint32_t)R + (uint32_t)jl[c]) * 8u;
+        for (int cc = 0; cc < nr; ++cc) {
+            int ec = cc + rot;
+            if (ec >= nr) ec -= nr;
+            const int64_t e = e0 + ec;
+            const int lo = nrlo[e], hr = nrhi[e];
+            const int hi = hr & 0x7fffffff;
+            const uint32_t n = (uint32_t)(hi - lo);
+            const __amdgpu_buffer_rsrc_t rp = buf_rsrc(spts + 3 * (int64_t)lo, n * 24u);
+            const __amdgpu_buffer_rsrc_t rx = buf_rsrc(xsq + (int64_t)lo * R, n * rowb);
+            const int nsub = (int)((n + S - 1) / S);
+            auto blocks = [&](auto guard) {
+                constexpr bool GU = decltype(guard)::value;
+                int left = nsub;
+                uint32_t sp = 0, sx = 0;
+                for (; left >= NB; left -= NB, sp += NB * S * 24u, sx += (uint32_t)(NB * S) * rowb) {
+                    if constexpr (H == 16)
+                        h16_block16<NC, NEG, GU, PH, true>(A1, A2, A3, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx, rowb,
+                                                           NB);
+                    else
+                        h16_block_emb<H, NC, NEG, GU, PH, true>(A1, A2, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx, rowb,
+                                                                NB, imrow, sel1, sel2);
+                }
+                if (left > 0) {
+                    if constexpr (H == 16)
+                        h16_block16<NC, NEG, GU, PH, false>(A1, A2, A3, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx, rowb,
+                                                            left);
+                    else
+                        h16_block_emb<H, NC, NEG, GU, PH, false>(A1, A2, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx,
+                                                                 rowb, left, imrow, sel1, sel2);
+                }
+            };
+            if (hr < 0) blocks(std::true_type());
+            else blocks(std::false_type());
+        }
+    };
+    auto run = [&](auto ph) {
+        if (nrows > 8) {
+            flt4 Pa[NC], Qa[NC], Ia[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                for (int v = 0; v < 4; ++v) Pa[c][v] = Qa[c][v] = Ia[c][v] = 0.0f;
+            walk(std::integral_constant<int, 16>(), ph, Pa, Qa, Ia, i, false, 0u, 0u);
+            Cacc<P, false> acc[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                acc[c].p1 = Pa[c] * osc[c];
+                acc[c].p2 = Qa[c] * osc[c];
+                acc[c].p3 = Ia[c] * osc[c];
+                for (int v = 0; v < 4; ++v) acc[c].p4[v] = 0.0f;
+            }
+            basis(acc, i);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const auto emit = emitter(c);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) emit(P::row(kk, v), acc[c].get(v));
+            }
+        } else {
+            auto emb = [&](auto hval) {
+                constexpr int H = decltype(hval)::value;
+                // v_perm(S0, S1): S1 the lower slot's source on real rows, the
+                // higher one on imaginary rows (there the lane's own slot is the
+                // higher); real rows take re halves for A1, imaginary rows im halves
+                const bool imrow = (i / H) == 1;
+                const uint32_t sel1 = imrow ? 0x03020706u : 0x05040100u;
+                const uint32_t sel2 = imrow ? 0x01000504u : 0x07060302u;
+                flt4 D1[NC], D2[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                    for (int v = 0; v < 4; ++v) D1[c][v] = D2[c][v] = 0.0f;
+                walk(hval, ph, D1, D2, D1, i % H, imrow, sel1, sel2);
+                // this lane holds MFMA rows 4 kk + v: real parts when kk < H / 4
+                const float sg = kk < H / 4 ? -1.0f : 1.0f;
+                Cemb<P, H> acc[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                    for (int v = 0; v < 4; ++v) acc[c].d[v] = (D1[c][v] + sg * D2[c][v]) * osc[c];
+                basis(acc, i % H);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) acc[c].store(emitter(c));
+            };
+            if (nrows > 4) emb(std::integral_constant<int, 8>());
+            else emb(std::integral_constant<int, 4>());
+        }
+    };
+    if (nk.phase32) run(std::true_type());
+    else run(std::false_type());
+}
+
+// Smallest squared distance above the zero-distance guard (2e-30) and the
+// largest one over the near pairs of each destination tile (one wave per
+// tile, lanes over the sources of its merged ranges, the tile's rows in a
+// loop): out[2 w] (+inf without a pair), out[2 w + 1] (0 without one).
+__global__ void __launch_bounds__(256)
+    k_near_rmin(const int32_t* __restrict__ tile_leaf, const int32_t* __restrict__ tile_row0, int64_t ntiles,
+                const int32_t* __restrict__ leaves, const int32_t* __restrict__ dlo, const int32_t* __restrict__ dhi,
+                const double* __restrict__ dpts, const int64_t* __restrict__ nrptr,
+                const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi, const double* __restrict__ spts,
+                double* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= ntiles) return;
+    const int t = leaves[tile_leaf[w]], row0 = tile_row0[w];
+    const int l0 = dlo[t], nrows = min(16, dhi[t] - l0 - row0);
+    double m = HUGE_VAL, mx = 0.0;
+    for (int64_t e = nrptr[t]; e < nrptr[t + 1]; ++e) {
+        const int lo = nrlo[e], hi = nrhi[e] & 0x7fffffff;
+        for (int s = lo + lane; s < hi; s += 64) {
+            const double y0 = spts[3 * (int64_t)s], y1 = spts[3 * (int64_t)s + 1], y2 = spts[3 * (int64_t)s + 2];
+            for (int r = 0; r < nrows; ++r) {
+                const double* xp = dpts + 3 * (int64_t)(l0 + row0 + r);
+                const double dx = xp[0] - y0, dy = xp[1] - y1, dz = xp[2] - y2;
+                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                if (r2 > 2e-30) m = fmin(m, r2);
+                mx = fmax(mx, r2);
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        m = fmin(m, __shfl_xor(m, o));
+        mx = fmax(mx, __shfl_xor(mx, o));
+    }
+    if (lane == 0) {
+        out[2 * w] = m;
+        out[2 * w + 1] = mx;
+    }
+}
+
+// out = in * t (points, a power of two: exact)
+__global__ void k_scale_pts(const double* __restrict__ in, double* __restrict__ out, int64_t n, double t) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i] * t;
+}
+
+// Per column q of the gathered right-hand sides: the largest |re|, |im| over
+// the per-workgroup maxima of the gather (nparts per column), then s_q = 2^m
+// with that maximum times s_q in [2^14, 2^15) (1 for a zero or non-finite
+// maximum): xsc[2 q] = s_q, xsc[2 q + 1] = 1 / s_q. One workgroup per column.
+__global__ void __launch_bounds__(256)
+    k_colscale(const float* __restrict__ part, int64_t nparts, int64_t R, float* __restrict__ xsc) {
+    __shared__ float red[256];
+    const int64_t q = blockIdx.x;
+    float m = 0.0f;
+    for (int64_t b = threadIdx.x; b < nparts; b += 256) m = fmaxf(m, part[b * R + q]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float mx = red[0];
+        int ex = 0;
+        if (mx > 0.0f && mx <= 3.4e38f) {
+            (void)frexpf(mx, &ex);  // mx = f 2^ex, f in [0.5, 1)
+            ex = 15 - ex;
+            ex = ex > 100 ? 100 : (ex < -100 ? -100 : ex);
+        }
+        xsc[2 * q] = ldexpf(1.0f, ex);
+        xsc[2 * q + 1] = ldexpf(1.0f, -ex);
+    }
+}
+
+// xs16[p R + q] = the f16 parts of s_q xs32[p R + q] (p < npts, q < R).
+__global__ void k_xs16(const float2* __restrict__ xs32, const float* __restrict__ xsc, int64_t npts, int64_t R,
+                       uint2* __restrict__ xs16) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts * R) return;  // npts R < 2^31 (checked per product)
+    const float s = xsc[2 * ((uint32_t)i % (uint32_t)R)];
+    const float2 v = xs32[i];
+    uint32_t h, l;
+    split_h16(v.x * s, v.y * s, h, l);
+    xs16[i] = make_uint2(h, l);
 }
 
 // Stage 5 + near field for NQ = 1..3 right-hand sides on VALU (an MFMA
@@ -1543,6 +2096,15 @@ const typename Ops<MIX>::T2* stored_near(const nca_h2* h) {
     else return h->ns_blocks.p;
 }
 
+// The near field of this MIXED product's MFMA chunks runs on the f16 path:
+// the operator allows it, the launch choice asks for it, the workspace holds
+// its buffers, and it is evaluated (not stored).
+template <bool MIX>
+bool near_h16(const nca_h2* h, bool fwd) {
+    return MIX && h->h16 == 1 && h->near_k3[1] == 2 && h->ws.xs16 && h->ws.xpart && h->ws.xsc &&
+           !(fwd && stored_near<MIX>(h));
+}
+
 // Stage 5 and the near field on the MFMA kernel for columns q0 .. q0 + nq - 1
 // (nq <= 16 NC): one launch for NC chunks of 16 columns, each kernel
 // evaluation serving all of them; then the bases above the leaves. With the
@@ -1571,6 +2133,24 @@ void run_leaf(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy,
                            h->ns_colcoef.p, h->ns_cptr.p, O::xs(h) + q0, (uint32_t)R, R, q0, nq, h->ws.ynear,
                            h->ws.ynear, nullptr, 0);
         ynear = h->ws.ynear;
+    }
+    if constexpr (MIX) {
+        if (dst.ntiles > 0 && !NS && near_h16<MIX>(h, FWD)) {
+            const bool neg = (h->sign < 0) != CJ;
+            const NearK nk = near_consts_scaled(h->k, h->near_turns, h->h16_t);
+            auto kern = neg ? k_leaf_out_h16<FWD, CJ, true, NC> : k_leaf_out_h16<FWD, CJ, false, NC>;
+            hipLaunchKernelGGL(kern, dim3(blocks_for(dst.ntiles, 4)), dim3(256), 0, st, dst.tile_leaf.p,
+                               dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst),
+                               dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(), dst.pts_h16.p, dst.nrptr.p,
+                               dst.nrlo.p, dst.nrhi.p, src.pts_h16.p, yh, h->ws.xs16, h->ws.xsc, R, q0, nk, alpha,
+                               beta, y, ldy, h->near_xcd, h->near_stagger, nq);
+            if (dst.nibasis > 0)
+                hipLaunchKernelGGL((k_basis_out<FWD, CJ, T2>), dim3((unsigned)dst.nibasis), dim3(256), 0, st,
+                                   dst.ibslot.p, dst.bnodes.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p,
+                                   dst.perm.p, dst.node_cptr(), yh, R, q0, (int64_t)nq, alpha, y, ldy);
+            NCA_HIP(hipGetLastError());
+            return;
+        }
     }
     if (dst.ntiles > 0) {
         // conj(G) for a negative phase sign, again conjugated for the adjoint
@@ -1803,10 +2383,21 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     SideDev& src = FWD ? h->trial : h->test;
     SideDev& dst = FWD ? h->test : h->trial;
     if (evs) evs->mark(0, st);
+    // the f16 near field of the MFMA chunks (if any) needs the split copy of
+    // the gathered right-hand sides and their column scales
+    const bool h16 = near_h16<MIX>(h, FWD) && (R >= 4 || h->mfma_small) && src.npts * R > 0;
     if (src.npts * R > 0)
         hipLaunchKernelGGL((k_gather_tiled<typename Ops<MIX>::T2>),
                            dim3(blocks_for(src.npts, 64), blocks_for(R, 16)), dim3(256), 0, st, x, ldx, src.iperm.p,
-                           src.npts, R, R, Ops<MIX>::xs(h));
+                           src.npts, R, R, Ops<MIX>::xs(h), h16 ? h->ws.xpart : nullptr);
+    if constexpr (MIX) {
+        if (h16) {
+            hipLaunchKernelGGL(k_colscale, dim3((unsigned)R), dim3(256), 0, st, h->ws.xpart,
+                               (int64_t)blocks_for(src.npts, 64), R, h->ws.xsc);
+            hipLaunchKernelGGL(k_xs16, dim3(blocks_for(src.npts * R, 256)), dim3(256), 0, st, h->ws.xs32,
+                               h->ws.xsc, src.npts, R, h->ws.xs16);
+        }
+    }
     // The MFMA chunks write every coefficient of xh (projection, upward pass)
     // and, forward, of yh (coupling; in MIXED the coupling and the downward
     // pass) exactly once; the VALU chunks and the reverse coupling add, so they
@@ -1872,6 +2463,42 @@ void ensure_mixed(nca_h2* h) {
         conv(sd->bblocks, sd->bblocks32, sd->basis_entries);
         conv(sd->tblocks, sd->tblocks32, sd->transfer_entries);
     }
+    // f16 near field: the smallest near distance above the zero-distance
+    // guard (the forward tiles cover every near pair); the path is taken when
+    // the near distances span at most 2^17, with t = 2^-14-e for r_min = f 2^e
+    // (f in [0.5, 1)), so that 1 / (t r) lies in [2^-3, 2^15]
+    h->h16 = 0;
+    h->near_rmin = h->near_rmax_pairs = 0.0;
+    if (h->test.ntiles > 0) {
+        DevBuf<double> mins((size_t)(2 * h->test.ntiles));
+        hipLaunchKernelGGL(k_near_rmin, dim3(blocks_for(h->test.ntiles, 4)), dim3(256), 0, st, h->test.tile_leaf.p,
+                           h->test.tile_row0.p, h->test.ntiles, h->test.leaves.p, h->test.lo.p, h->test.hi.p,
+                           h->test.pts.p, h->test.nrptr.p, h->test.nrlo.p, h->test.nrhi.p, h->trial.pts.p, mins.p);
+        NCA_HIP(hipGetLastError());
+        std::vector<double> hm((size_t)(2 * h->test.ntiles));
+        NCA_HIP(hipMemcpyAsync(hm.data(), mins.p, hm.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        NCA_HIP(hipStreamSynchronize(st));
+        double r2 = HUGE_VAL, r2x = 0.0;
+        for (size_t q = 0; q < hm.size(); q += 2) {
+            r2 = std::min(r2, hm[q]);
+            r2x = std::max(r2x, hm[q + 1]);
+        }
+        const double rmin = std::sqrt(r2), rmax = std::sqrt(r2x);
+        h->near_rmin = std::isfinite(rmin) ? rmin : 0.0;
+        h->near_rmax_pairs = rmax;
+        if (std::isfinite(rmin) && rmin > 0.0 && rmax <= 131072.0 * rmin) {
+            int e = 0;
+            (void)std::frexp(rmin, &e);
+            h->h16_t = std::ldexp(1.0, -14 - e);
+            for (SideDev* sd : {&h->test, &h->trial}) {
+                const int64_t n = 3 * (sd->npts + kNearPad);
+                sd->pts_h16.alloc((size_t)n);
+                hipLaunchKernelGGL(k_scale_pts, dim3(blocks_for(n, 256)), dim3(256), 0, st, sd->pts.p,
+                                   sd->pts_h16.p, n, h->h16_t);
+            }
+            h->h16 = 1;
+        }
+    }
     NCA_HIP(hipGetLastError());
     NCA_HIP(hipStreamSynchronize(st));
     h->mixed_ready = true;
@@ -1891,12 +2518,20 @@ void ensure_workspace(nca_h2* h, int64_t R) {
         h->ws_xs.release();
         h->ws_xs32.release();
         h->ws_yc.release();
+        h->ws_xs16.release();
+        h->ws_xpart.release();
+        h->ws_xsc.release();
         h->ws_rhs = R;
     }
     if (h->precision == NCA_PREC_MIXED) {
         if (!h->ws_xs32.p) h->ws_xs32.alloc((size_t)std::max<int64_t>((np + kNearPad) * h->ws_rhs, 1));
         if (!h->ws_xh32.p) h->ws_xh32.alloc((size_t)std::max<int64_t>(K * 16, 1));
         if (!h->ws_yc.p) h->ws_yc.alloc((size_t)std::max<int64_t>(K * h->ws_rhs, 1));
+        if (h->h16 == 1) {
+            if (!h->ws_xs16.p) h->ws_xs16.alloc((size_t)std::max<int64_t>(np * h->ws_rhs, 1));
+            if (!h->ws_xpart.p) h->ws_xpart.alloc((size_t)std::max<int64_t>(blocks_for(np, 64) * h->ws_rhs, 1));
+            if (!h->ws_xsc.p) h->ws_xsc.alloc((size_t)std::max<int64_t>(2 * h->ws_rhs, 1));
+        }
     } else {
         if (!h->ws_xs.p) h->ws_xs.alloc((size_t)std::max<int64_t>((np + kNearPad) * h->ws_rhs, 1));
     }
@@ -1945,8 +2580,9 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
     if (op != NCA_OP_N) ensure_reverse(h);
     if (op == NCA_OP_N && h->near_stored) ensure_near_store(h);
     ensure_workspace(h, R);
-    h->ws = nca_h2::WsView{h->ws_xs.p, h->ws_xs32.p, h->ws_xh.p, h->ws_yh.p, h->ws_yc.p, h->ws_xh32.p,
-                           h->near_stored ? h->ws_ynear.p : nullptr};
+    h->ws = nca_h2::WsView{h->ws_xs.p,  h->ws_xs32.p, h->ws_xh.p,    h->ws_yh.p,
+                           h->ws_yc.p,  h->ws_xh32.p, h->near_stored ? h->ws_ynear.p : nullptr,
+                           h->ws_xs16.p, h->ws_xpart.p, h->ws_xsc.p};
     run_sweep(h, op, R, x, ldx, y, ldy, alpha, beta, evs);
 }
 
@@ -1998,6 +2634,11 @@ size_t carve_workspace(const nca_h2* h, int64_t R, char* base, nca_h2::WsView& v
     if (mix) {
         v.yc = reinterpret_cast<float2*>(take(K * r * sizeof(float2)));
         if (h->couple_x32 && R > 16) v.xh32 = reinterpret_cast<float2*>(take(K * 16 * sizeof(float2)));
+        if (h->h16 == 1 && h->near_k3[1] == 2) {
+            v.xs16 = reinterpret_cast<uint2*>(take(np * r * sizeof(uint2)));
+            v.xpart = reinterpret_cast<float*>(take((size_t)blocks_for((int64_t)np, 64) * r * sizeof(float)));
+            v.xsc = reinterpret_cast<float*>(take(2 * r * sizeof(float)));
+        }
     }
     // the stored near field is used when its entries of this precision exist
     if (h->near_stored && h->ns_plan && (mix ? h->ns_blocks32.p != nullptr : h->ns_blocks.p != nullptr))
@@ -2752,7 +3393,10 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
             h->sctab.upload(tab, st);
         }
         if (const char* v = getenv("NCA_MFMA_K3")) h->near_k3[0] = h->couple_k3[0] = atoi(v) != 0;
-        if (const char* v = getenv("NCA_MIXED_K3")) h->near_k3[1] = h->couple_k3[1] = atoi(v) != 0;
+        if (const char* v = getenv("NCA_MIXED_K3")) {
+            h->near_k3[1] = std::min(std::max(atoi(v), 0), 2);
+            h->couple_k3[1] = atoi(v) != 0;
+        }
         if (const char* v = getenv("NCA_ORDER_NEAR")) h->near_xcd = atoi(v);
         if (const char* v = getenv("NCA_ORDER_COUPLE")) h->couple_xcd = atoi(v);
         if (const char* v = getenv("NCA_LEAF_ORDER")) h->leaf_order = atoi(v);
@@ -2919,6 +3563,7 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
             double dm2 = 0.0;
             for (int64_t t = 0; t < ntest; ++t) dm2 = std::max(dm2, dmax2[t]);
             h->near_turns = std::fabs(h->k) * 0.5 / M_PI * std::sqrt(dm2);
+            h->near_rmax = std::sqrt(dm2);
             csr_transpose(
                 ntest, np.data(), ntrial, [&](int64_t i) { return nn32[i]; }, tp,
                 [&](int64_t, int64_t t, int64_t pos) { tn[pos] = (int32_t)t; });
@@ -3425,7 +4070,9 @@ int32_t nca_h2_autotune(nca_h2* h2, int32_t reps, nca_h2_tuning* out) {
         // near field: products x leaf order x workgroup order
         int bk3 = h2->near_k3[prec], blo = h2->leaf_order, bx = h2->near_xcd, bst = h2->near_stagger;
         double best = n0;
-        for (int k3 = 0; k3 < 2; ++k3)
+        // MIXED: also the f16 near field (near_k3 = 2) when the operator allows it
+        const int nk3 = prec == 1 && h2->h16 == 1 ? 3 : 2;
+        for (int k3 = 0; k3 < nk3; ++k3)
             for (int lo = 0; lo < 3; ++lo)
                 for (int xo = 0; xo < 4; ++xo) {
                     h2->near_k3[prec] = k3;
@@ -3491,7 +4138,8 @@ int32_t nca_h2_autotune(nca_h2* h2, int32_t reps, nca_h2_tuning* out) {
 int32_t nca_h2_tuning_apply(nca_h2* h2, const nca_h2_tuning* t) {
     return guarded([&] {
         NCA_REQUIRE(h2 && t, NCA_ERR_ARGUMENT, "nca_h2_tuning_apply: invalid argument");
-        NCA_REQUIRE(t->near_k3 >= 0 && t->near_k3 <= 1 && t->couple_k3 >= 0 && t->couple_k3 <= 1 &&
+        NCA_REQUIRE(t->near_k3 >= 0 && t->near_k3 <= (h2->precision == NCA_PREC_MIXED ? 2 : 1) &&
+                        t->couple_k3 >= 0 && t->couple_k3 <= 1 &&
                         t->leaf_order >= 0 && t->leaf_order <= 2 && t->near_xcd_order >= 0 &&
                         t->near_xcd_order <= 1 && t->near_stagger >= 0 && t->near_stagger <= 1 &&
                         t->couple_tile_order >= 0 && t->couple_tile_order <= 3 && t->couple_xcd_order >= 0 &&
@@ -3580,6 +4228,22 @@ int32_t nca_h2_get_stats(const nca_h2* h2, nca_h2_stats* out) {
         NCA_REQUIRE(h2 && out, NCA_ERR_ARGUMENT, "nca_h2_get_stats: invalid argument");
         std::lock_guard<std::mutex> lock(h2->mu);
         *out = h2->stats;
+    });
+}
+
+int32_t nca_h2_near_form(nca_h2* h2, int32_t* form, double* rmin, double* rmax) {
+    return guarded([&] {
+        NCA_REQUIRE(h2 && form, NCA_ERR_ARGUMENT, "nca_h2_near_form: invalid argument");
+        std::lock_guard<std::mutex> lock(h2->mu);
+        set_device(h2->ctx);
+        int32_t f = 0;
+        if (h2->precision == NCA_PREC_MIXED) {
+            ensure_mixed(h2);
+            f = h2->near_stored ? 1 : (h2->near_k3[1] == 2 && h2->h16 == 1 ? 2 : 1);
+        }
+        *form = f;
+        if (rmin) *rmin = h2->near_rmin;
+        if (rmax) *rmax = h2->near_rmax_pairs > 0.0 ? h2->near_rmax_pairs : h2->near_rmax;
     });
 }
 

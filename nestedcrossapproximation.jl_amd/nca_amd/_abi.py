@@ -132,6 +132,7 @@ SIGNATURES = {
     "nca_h2_graph_launch": (I32, [P]),
     "nca_h2_graph_destroy": (I32, [P]),
     "nca_h2_get_stats": (I32, [P, P]),
+    "nca_h2_near_form": (I32, [P, P, P, P]),
 }
 
 

@@ -204,6 +204,15 @@ class PetrovGalerkinNCACore:
         call(hip_lib().nca_h2_get_stats(self.handle, ctypes.byref(s)))
         return {n: getattr(s, n) for n, _ in _abi.H2Stats._fields_}
 
+    def near_form(self):
+        """(form, rmin, rmax) of nca_h2_near_form: 0 FP64, 1 MIXED on the f32
+        MFMA (or stored), 2 MIXED on the f16 MFMA with split operands."""
+        f = ctypes.c_int32()
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        call(hip_lib().nca_h2_near_form(self.handle, ctypes.byref(f), ctypes.byref(a), ctypes.byref(b)))
+        return f.value, a.value, b.value
+
     def set_precision(self, precision):
         call(hip_lib().nca_h2_set_precision(self.handle, int(precision)))
 

@@ -19,6 +19,16 @@ def test_mixed_worked_example():
     assert b["leaf_expansion_near"] == 8 * 120 + 16 * 2 * 8 + 8 * 2 * 20 + 24 * 40 + 12 * 7 + 16 * 2 * 20 == 3220
 
 
+def test_mixed_f16_near_field():
+    # the f16 near field adds to the gather a read of the c32 copy, the write of
+    # the split copy (8 B each per point and column) and the column maxima
+    # (4 B per 64 points and column, written and read); the near field reads
+    # the split copy instead of the c32 one (the same bytes)
+    b, h = bench.stage_bytes(ST, 2, 20, 20, True), bench.stage_bytes(ST, 2, 20, 20, True, h16=True)
+    assert h["gather"] - b["gather"] == 16 * 2 * 20 + 2 * 4 * 2 * 1
+    assert all(h[k] == b[k] for k in b if k != "gather")
+
+
 def test_fp64_worked_example():
     b = bench.stage_bytes(ST, 2, 20, 20, False)
     assert b["gather"] == 16 * 2 * 20 * 2 + 4 * 20

@@ -1,0 +1,123 @@
+"""The MIXED near field on f16 MFMA with split operands (every FP32 kernel
+value and right-hand side as two f16 parts; the MIXED default), through the
+C-ABI: it is the form MIXED products take when the near pairs allow it
+(nca_h2_near_form), it stays within the MIXED bar of the oracle's compressed
+FP64 operator for N, T and C on 16-, 8- and 4-row tiles and one or two
+16-column chunks, no less accurate than the f32 MFMA form; it is exact under
+power-of-two scaling of each right-hand side (the per-column scales), gives
+exact zeros for zero columns, and near pairs spanning more than 2^17 in
+distance fall back to the f32 form."""
+import numpy as np
+import pytest
+
+import nca_amd
+from nca_amd import compressors as C
+from nca_amd import geometry
+from nca_amd import nca as NC
+from nca_amd import tuning as TU
+from oracle import oracle as O
+from helpers import relerr
+
+pytestmark = pytest.mark.gpu
+
+# the launch choices of the defaults, with the near-field form varied
+BASE = dict(leaf_order=2, near_xcd_order=0, near_stagger=0, couple_k3=1, couple_tile_order=3, couple_xcd_order=0)
+
+
+def say(msg):
+    print(f"[gpu-near-f16] {msg}", flush=True)
+
+
+def core_of(pts, k, minvalues=32, tol=1e-4):
+    tree = nca_amd.TwoNTree(pts, minvalues=minvalues)
+    cmp = C.BatchedBottomUp(C.GeometricPivoting(), C.ACA(tol))
+    core = nca_amd.PetrovGalerkinNCACore(nca_amd.HelmholtzKernel(k), pts, pts, nca_amd.BlockTree(tree, tree), cmp,
+                                         cmp, ctx=nca_amd.default_context())
+    off, blocks = core.couplingstore.download()
+    return core, O.OracleH2(core.desc, k, 1, pts, pts, off, blocks)
+
+
+def tile_shapes(core_pts, minvalues):
+    tree = nca_amd.TwoNTree(core_pts, minvalues=minvalues)
+    sizes = np.array([tree.nvalues(t) for t in np.where(tree.leafmask())[0]])
+    rem = sizes % 16
+    return {"16": bool(np.any(sizes >= 16)), "8": bool(np.any((rem > 4) & (rem <= 8))),
+            "4": bool(np.any((rem > 0) & (rem <= 4)))}
+
+
+def test_f16_form_default_and_within_bar():
+    """8,192 sphere points (k = 2, octree leaves of 1..32 points: 16-, 8- and
+    4-row tiles): MIXED takes the f16 form, N/T/C products at 16, 8 and 40
+    right-hand sides are within 1e-5 of the FP64 oracle, not above twice the
+    f32 form's error, and differ from it (another arithmetic)."""
+    pts = geometry.fibonacci_sphere(8192)
+    assert all(tile_shapes(pts, 32).values())
+    core, H = core_of(pts, 2.0)
+    W = NC.H2Workspace(core)
+    core.set_precision(nca_amd.NCA_PREC_MIXED)
+    form, rmin, rmax = core.near_form()
+    say(f"form {form}, near distances {rmin:.3e} .. {rmax:.3e}")
+    assert form == 2 and 0 < rmin < rmax <= 2 ** 17 * rmin
+    rng = np.random.default_rng(41)
+    for R in (16, 8, 40):
+        x = geometry.complex_gaussian(rng, (pts.shape[0], R))
+        refs = [H.matvec(x, op=op) for op in (0, 1, 2)]
+        res = {}
+        for k3 in (2, 1):
+            TU.apply(core, dict(BASE, near_k3=k3))
+            assert core.near_form()[0] == (2 if k3 == 2 else 1)
+            res[k3] = [W @ x, NC.transpose(W) @ x, NC.adjoint(W) @ x]
+        for op in range(3):
+            e16, e32 = relerr(res[2][op], refs[op]), relerr(res[1][op], refs[op])
+            say(f"R={R} op={op}: f16 {e16:.2e} f32 {e32:.2e}")
+            assert e16 <= 1e-5 and e32 <= 1e-5
+            assert e16 <= 2 * e32
+            assert not np.array_equal(res[2][op], res[1][op])
+
+
+def test_f16_power_of_two_scaling_and_zero_columns():
+    """Each right-hand side scaled by its own power of two (2^-60 .. 2^60):
+    every output column is the unscaled one times the same power, bit for
+    bit (the per-column scales absorb the factor); a zero column gives exact
+    zeros and leaves the others unchanged."""
+    pts = geometry.fibonacci_sphere(4096)
+    core, _ = core_of(pts, 1.0)
+    W = NC.H2Workspace(core)
+    core.set_precision(nca_amd.NCA_PREC_MIXED)
+    assert core.near_form()[0] == 2
+    rng = np.random.default_rng(42)
+    x = geometry.complex_gaussian(rng, (pts.shape[0], 16))
+    y0 = W @ x
+    e = np.array([-60, -33, -7, 0, 1, 5, 17, 60, -1, 2, -2, 9, -20, 30, 44, -50], dtype=float)
+    ys = W @ (x * 2.0 ** e)
+    for q in range(16):
+        assert np.array_equal(ys[:, q], y0[:, q] * 2.0 ** e[q]), q
+    xz = x.copy()
+    xz[:, 5] = 0
+    yz = W @ xz
+    assert np.all(yz[:, 5] == 0)
+    others = [q for q in range(16) if q != 5]
+    assert np.array_equal(yz[:, others], y0[:, others])
+
+
+@pytest.mark.parametrize("gap, form", [(1e-4, 2), (1e-7, 1)])
+def test_f16_distance_span(gap, form):
+    """A 4,096-point sphere plus one point at distance `gap` from the first
+    (its near pairs reach 1.5 apart, through a sparse coarse leaf): near
+    pairs spanning up to 2^17 in distance keep the f16 form (1e-4: the point
+    scale moves), beyond it (1e-7) MIXED falls back to the f32 form; both
+    within 1e-5 of the FP64 oracle."""
+    base = geometry.fibonacci_sphere(4096)
+    d = np.cross(base[0], [0.0, 0.0, 1.0])
+    d /= np.linalg.norm(d)
+    pts = np.vstack([base, base[0] + gap * d])
+    core, H = core_of(pts, 1.0)
+    W = NC.H2Workspace(core)
+    core.set_precision(nca_amd.NCA_PREC_MIXED)
+    f, rmin, rmax = core.near_form()
+    say(f"gap {gap}: form {f}, near distances {rmin:.3e} .. {rmax:.3e}")
+    assert f == form and abs(rmin - gap) <= 1e-3 * gap
+    rng = np.random.default_rng(43)
+    x = geometry.complex_gaussian(rng, (pts.shape[0], 16))
+    assert relerr(W @ x, H.matvec(x)) <= 1e-5
+    assert relerr(NC.adjoint(W) @ x, H.matvec(x, op=2)) <= 1e-5
