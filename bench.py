@@ -74,7 +74,7 @@ class Hip:
         assert self.lib.hipDeviceSynchronize() == 0
 
 
-def stage_bytes(st, R, M, N, mixed, beta=False, h16=False):
+def stage_bytes(st, R, M, N, mixed, beta=False, h16=False, otf=False):
     """Algorithmic HBM bytes of each stage of one forward sweep over R
     right-hand sides, counting what the kernels of the 16-column chunks move:
     every operator entry once (s_op = 16 B FP64, 8 B MIXED), the gathered
@@ -86,7 +86,11 @@ def stage_bytes(st, R, M, N, mixed, beta=False, h16=False):
     ranks (``coeffs_*_leaves``). h16 (MIXED near field on the f16 MFMA): the
     gather also reads its c32 copy once and writes the 8-byte split copy the
     near field reads instead, and the per-workgroup column maxima (4 B per 64
-    points and column, written and read)."""
+    points and column, written and read). otf (MIXED coupling evaluated on the
+    fly): instead of the coupling entries, the source coefficients read, split
+    into f16 parts (written) and read once, the pivot coordinates of both
+    sides (24 B per coefficient) and the merged far ranges (at most 8 B per
+    coupling block)."""
     s_op = 8 if mixed else 16
     s_x = 8 if mixed else 16
     Ks, Kd = st["coeffs_trial"], st["coeffs_test"]
@@ -100,12 +104,21 @@ def stage_bytes(st, R, M, N, mixed, beta=False, h16=False):
         "gather": gather,
         "leaf_projection": s_op * st["basis_entries_trial"] + s_x * R * N + s_x * R * Ks_leaf,
         "upward": s_op * st["transfer_entries_trial"] + s_x * R * (Ks + Ks_int),
-        "coupling": s_op * st["coupling_entries"] + 4 * st["coupling_columns"] + s_x * R * Ks
+        "coupling": (s_op * st["coupling_entries"] + 4 * st["coupling_columns"] + s_x * R * Ks
+                     if not otf else 3 * 8 * R * Ks + 24 * (Ks + Kd) + 8 * st["coupling_blocks"])
         + (8 if mixed else 16) * R * Kd,
         "downward": s_op * st["transfer_entries_test"] + 16 * R * Kd_int + down_vec,
         "leaf_expansion_near": s_op * st["basis_entries_test"] + 16 * R * Kd_leaf + s_x * R * N + 24 * (M + N)
         + 12 * st["near_pairs"] + (2 if beta else 1) * 16 * R * M,
     }
+
+
+def couple_otf(core):
+    """True when the core's products in its current precision evaluate the
+    coupling on the fly (nca_h2_couple_form form 2)."""
+    f = ctypes.c_int32()
+    nca_amd.runtime.call(nca_amd.hip_lib().nca_h2_couple_form(core.handle, ctypes.byref(f), None, None))
+    return f.value == 2
 
 
 def near_f16(core):
@@ -249,15 +262,15 @@ def main():
         hip.d2h(Y, dy)
         results[label] = (ms, R / (ms * 1e-3), prof, Y)
         mixed = label == "mixed"
-        h16 = near_f16(core)
-        sbytes = stage_bytes(st, R, M, N, mixed, h16=h16)
+        h16, otf = near_f16(core), couple_otf(core)
+        sbytes = stage_bytes(st, R, M, N, mixed, h16=h16, otf=otf)
         b_alg = sum(sbytes.values())
         # transpose and adjoint sweeps (same operator, the other side's stages)
         ms_t = timed(1, R, max(args.steps // 2, 1), 1)
         ms_c = timed(2, R, max(args.steps // 2, 1), 1)
         # one right-hand side: iterative solvers call mul! one vector at a time
         ms_1 = timed(0, 1, 50, 5)
-        b1 = sum(stage_bytes(st, 1, M, N, mixed).values())
+        b1 = sum(stage_bytes(st, 1, M, N, mixed, otf=otf).values())
         per_prec[label] = {
             "matvec_per_s": round(R / (ms * 1e-3), 2), "ms_per_sweep": round(ms, 4),
             "sweep_alg_bytes": int(b_alg), "sweep_hbm_gbs": round(b_alg / (ms * 1e-3) / 1e9, 1),
@@ -277,7 +290,7 @@ def main():
 
     head = "mixed"
     ms_per_step, value, prof, _ = results[head]
-    sbytes = stage_bytes(st, R, M, N, True, h16=near_f16(core))
+    sbytes = stage_bytes(st, R, M, N, True, h16=near_f16(core), otf=couple_otf(core))
     b_alg = sum(sbytes.values())
     sweep_gbs = b_alg / (ms_per_step * 1e-3) / 1e9
     stages = per_prec[head]["stages"]
@@ -642,7 +655,7 @@ def bench_wideband(args):
             prof += sm / 3
         Y = np.zeros((N, R), np.complex128, order="F")
         hip.d2h(Y, dy)
-        sb = stage_bytes(st, R, N, N, label == "mixed", h16=near_f16(core))
+        sb = stage_bytes(st, R, N, N, label == "mixed", h16=near_f16(core), otf=couple_otf(core))
         out[label] = {"ms_per_sweep": round(ms, 4), "matvec_per_s": round(R / (ms * 1e-3), 2),
                       "sweep_alg_bytes": int(sum(sb.values())),
                       "sweep_hbm_gbs": round(sum(sb.values()) / (ms * 1e-3) / 1e9, 1),

@@ -121,6 +121,10 @@ struct nca_coupling {
     std::vector<int64_t> ptr, nodes;  // 0-based CSR (test node -> trial nodes)
     std::vector<int64_t> offsets;     // nblocks+1, 0-based complex offsets
     std::vector<int32_t> testrank, trialrank;
+    // the pivots (0-based point indices) of every node of both sides, 0-based
+    // CSR: block (t, s) = A[tpiv[tptr[t] ..], spiv[sptr[s] ..]]
+    std::vector<int64_t> tptr, sptr;
+    std::vector<int32_t> tpiv, spiv;
     nca::DevBuf<double2> blocks;
 };
 

@@ -2171,6 +2171,10 @@ int32_t nca_coupling_assemble(nca_ctx* ctx, const nca_helmholtz* kernel, const n
                         "coupling: trial pivot out of range");
             spiv[i] = (int32_t)(trialpiv[i] - 1);
         }
+        cp->tptr = tptr;
+        cp->sptr = sptr;
+        cp->tpiv = tpiv;
+        cp->spiv = spiv;
         cp->blocks.alloc((size_t)std::max<int64_t>(cp->nentries, 1));
         if (cp->nentries > 0) {
             DevBuf<int64_t> d_ptr, d_off, d_tptr, d_sptr;

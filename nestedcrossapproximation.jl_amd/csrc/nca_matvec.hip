@@ -108,6 +108,21 @@ struct nca_h2 {
     // within 2^17 of each other), with the point scale h16_t = 2^-e
     int h16 = 0;
     double h16_t = 1.0;
+    // MIXED coupling evaluated on the fly on the f16 MFMA (couple_k3 = 2): 1
+    // when available (operators of one coefficient block per node whose far
+    // pivot pairs span at most 2^17 in distance); the pivot scale, and the
+    // measured smallest and largest far pivot distances
+    int coup_h16 = 0;
+    double coup_t = 1.0, coup_rmin = 0.0, coup_rmax = 0.0;
+    // scaled pivot coordinates of both sides in coefficient order (3 per coefficient)
+    nca::DevBuf<double> piv_test, piv_trial;
+    // the far blocks' coefficients as merged ranges (bit 31 of the end: the
+    // zero-distance guard): per test block (trial coefficients) and per trial
+    // block (test coefficients), and the 16-row tiles of the trial blocks
+    nca::DevBuf<int64_t> cf_ptr, cr_ptr;
+    nca::DevBuf<int32_t> cf_lo, cf_hi, cr_lo, cr_hi;
+    int64_t cr_ntiles = 0;
+    nca::DevBuf<int32_t> cr_tile_node, cr_tile_row0;
     int precision = NCA_PREC_FP64;
     int64_t M = 0, N = 0;
     nca::SideDev test, trial;
@@ -161,6 +176,10 @@ struct nca_h2 {
     // column maxima and the column scales (s_q, 1 / s_q)
     nca::DevBuf<uint2> ws_xs16;
     nca::DevBuf<float> ws_xpart, ws_xsc;
+    // the on-the-fly coupling's split source coefficients (K x R), the column
+    // maxima of a chunk and its scales (up to 32 columns)
+    nca::DevBuf<uint2> ws_xh16;
+    nca::DevBuf<float> ws_xhpart, ws_xhsc;
     // The device workspace the stage kernels of the product in flight use:
     // the operator's own buffers above, or carved from the static scratch
     // array of nca_h2_matvec_device_noalloc.
@@ -175,6 +194,9 @@ struct nca_h2 {
         uint2* xs16 = nullptr;    // split right-hand sides of the f16 near field (MIXED)
         float* xpart = nullptr;   // per gather workgroup and column: largest |re|, |im|
         float* xsc = nullptr;     // per column: s_q, 1 / s_q
+        uint2* xh16 = nullptr;    // split source coefficients of the on-the-fly coupling
+        float* xhpart = nullptr;  // their per-workgroup column maxima (a chunk of up to 32 columns)
+        float* xhsc = nullptr;    // the chunk's column scales
     } ws;
     // c32 copy of the coupling blocks for the MIXED path
     nca::DevBuf<float2> S32;
@@ -189,11 +211,13 @@ struct nca_h2 {
     //   in FP32 three raise the MIXED error against FP64 from 1.0e-6 to 1.5e-6;
     //   near_k3 = 2 (MIXED, the default): the near field on f16 MFMA with
     //   split operands when the operator allows it (h16), else as 1;
+    //   couple_k3 = 2 (MIXED, the default): the coupling blocks evaluated on
+    //   the fly on the f16 path when the operator allows it (coup_h16), else as 1;
     // near_xcd / couple_xcd: work_block order (NCA_ORDER_NEAR, NCA_ORDER_COUPLE);
     // leaf_order: order_leaves mode (NCA_LEAF_ORDER);
     // couple_order: order_coupling_tiles mode (NCA_COUPLE_ORDER);
     // near_stagger: near-field walks start at a tile-dependent range (NCA_NEAR_STAGGER).
-    int near_k3[2] = {1, 2}, couple_k3[2] = {1, 1};
+    int near_k3[2] = {1, 2}, couple_k3[2] = {1, 2};
     int near_xcd = 0, couple_xcd = 0, leaf_order = 2, couple_order = 3, near_stagger = 0;
     std::vector<int32_t> h_ncols;  // host copy: columns of every test node's coupling row
     // Every entry point that touches the workspace, the launch plans or the
@@ -869,6 +893,12 @@ __device__ __forceinline__ void near_compute(ACC (&acc)[NC], const NearK& nk, co
         g = P::template eval<NEG, GUARD, PH32>(nk, tab, x0, x1, x2, quad_double<0>(o.pq), quad_double<1>(o.pq),
                                                quad_double<2>(o.pq));
     const int i = threadIdx.x & 15;
+    // the range's last step: sources past its end read the point (0, 0, 0),
+    // at which an unguarded kernel value may be infinite, with zero
+    // right-hand sides; their values are zeroed (a guarded evaluation gives 0
+    // there)
+    if constexpr (!GUARD)
+        if (left < 4 * G && 4 * (i / H) + (int)((threadIdx.x & 63) >> 4) >= left) g = czero2<P>();
 #pragma unroll
     for (int m = 0; m < G; ++m) {
         if (m > 0 && 4 * m >= left) continue;
@@ -1193,7 +1223,7 @@ template <int NC, bool NEG, bool GU, bool PH, bool FULL>
 __device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (&I)[NC], const NearK& nk, double x0,
                                             double x1, double x2, __amdgpu_buffer_rsrc_t rp,
                                             __amdgpu_buffer_rsrc_t rx, uint32_t sp, uint32_t sx, uint32_t vp,
-                                            const uint32_t (&vx)[NC][1], uint32_t rowb, int nsub) {
+                                            const uint32_t (&vx)[NC][1], uint32_t rowb, int nsub, int nval) {
     double pq[8];
     uint2 xb[NC][8];
 #pragma unroll
@@ -1208,6 +1238,7 @@ __device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (
             for (int c = 0; c < NC; ++c) xb[c][m] = make_uint2(0u, 0u);
         }
     }
+    const int kk = (threadIdx.x & 63) >> 4;
     u32x4 arh, aih, arl, ail;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -1215,11 +1246,16 @@ __device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
             const int m = 2 * v + e;
-            if (FULL || m < nsub)
+            if (FULL || m < nsub) {
                 g[e] = helmholtz_h16<NEG, GU, PH>(nk, x0, x1, x2, row_double<0>(pq[m]), row_double<1>(pq[m]),
                                                   row_double<2>(pq[m]));
-            else
+                // the range's last sub-step: sources past its end read the point
+                // (0, 0, 0), at which the kernel may be infinite, with zero
+                // right-hand sides; their values are zeroed
+                if (!FULL && 4 * m + 4 > nval && 4 * m + kk >= nval) g[e] = make_float2(0.0f, 0.0f);
+            } else {
                 g[e] = make_float2(0.0f, 0.0f);
+            }
         }
         uint32_t h, l;
         split_h16(g[0].x, g[1].x, h, l);
@@ -1258,8 +1294,8 @@ template <int H, int NC, bool NEG, bool GU, bool PH, bool FULL>
 __device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], const NearK& nk, double x0, double x1,
                                               double x2, __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rx,
                                               uint32_t sp, uint32_t sx, uint32_t vp,
-                                              const uint32_t (&vx)[NC][16 / H], uint32_t rowb, int nsub, bool imrow,
-                                              uint32_t sel1, uint32_t sel2) {
+                                              const uint32_t (&vx)[NC][16 / H], uint32_t rowb, int nsub, int nval,
+                                              bool imrow, uint32_t sel1, uint32_t sel2) {
     constexpr int G = 16 / H, S = 64 / H, NS = 32 / S;
     double pq[NS];
     uint2 xb[NC][NS][G];
@@ -1280,13 +1316,18 @@ __device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], co
                 for (int q = 0; q < G; ++q) xb[c][m][q] = make_uint2(0u, 0u);
         }
     }
+    // this lane's source in a sub-step, after the sub-step's first
+    const int own = 4 * ((int)(threadIdx.x & 15) / H) + (int)((threadIdx.x & 63) >> 4);
     u32x4 a1h, a2h, a1l, a2l;
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
         float2 g = make_float2(0.0f, 0.0f);
-        if (FULL || m < nsub)
+        if (FULL || m < nsub) {
             g = helmholtz_h16<NEG, GU, PH>(nk, x0, x1, x2, quad_double<0>(pq[m]), quad_double<1>(pq[m]),
                                            quad_double<2>(pq[m]));
+            // sources past the range end (the point (0, 0, 0)): zero values
+            if (!FULL && S * m + S > nval && S * m + own >= nval) g = make_float2(0.0f, 0.0f);
+        }
         uint32_t oh, ol;
         split_h16(g.x, g.y, oh, ol);
         if constexpr (H == 8) {
@@ -1333,6 +1374,147 @@ __device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], co
     }
 }
 
+// The f16 walk of one tile: H row slots (16, 8 or 4), this lane's row at
+// xp (3 doubles), merged source ranges e0 .. e0 + nr - 1 (from e0 + rot on,
+// cyclically; bit 31 of the end: the range keeps the zero-distance guard),
+// source points spts (3 doubles each), split right-hand sides xsq (R per
+// source, 8 bytes each, from the first column on), columns jl. Accumulates
+// Gr xr, Gi xi and Gr xi + Gi xr into A1, A2, A3 (H = 16), or A1 xr and
+// A2 xi into A1, A2 (the real embedding, H = 8 or 4).
+template <int H, int NC, bool NEG, bool PH>
+__device__ __forceinline__ void h16_walk(flt4 (&A1)[NC], flt4 (&A2)[NC], flt4 (&A3)[NC], const NearK& nk,
+                                         const double* xp, int lane, int64_t e0, int nr, int rot,
+                                         const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
+                                         const double* __restrict__ spts, const uint2* __restrict__ xsq, int64_t R,
+                                         const int (&jl)[NC], bool imrow, uint32_t sel1, uint32_t sel2) {
+    constexpr int G = 16 / H, S = 64 / H, NB = 32 / S;
+    const int kk = lane >> 4;
+    const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
+    const int ysrc = 4 * ((lane & 15) / H) + kk;
+    const uint32_t vp = near_point_offset<G>(lane, kk, ysrc);
+    const uint32_t rowb = (uint32_t)R * 8u;
+    uint32_t vx[NC][G];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int q = 0; q < G; ++q) vx[c][q] = ((uint32_t)(4 * q + kk) * (uint32_t)R + (uint32_t)jl[c]) * 8u;
+    for (int cc = 0; cc < nr; ++cc) {
+        int ec = cc + rot;
+        if (ec >= nr) ec -= nr;
+        const int64_t e = e0 + ec;
+        const int lo = nrlo[e], hr = nrhi[e];
+        const int hi = hr & 0x7fffffff;
+        const uint32_t n = (uint32_t)(hi - lo);
+        const __amdgpu_buffer_rsrc_t rp = buf_rsrc(spts + 3 * (int64_t)lo, n * 24u);
+        const __amdgpu_buffer_rsrc_t rx = buf_rsrc(xsq + (int64_t)lo * R, n * rowb);
+        // full blocks of 32 sources, then the tail (rem sources: fewer
+        // sub-steps, and the sources past the range end masked)
+        const int nfull = (int)(n / 32u), rem = (int)(n % 32u);
+        auto blocks = [&](auto guard) {
+            constexpr bool GU = decltype(guard)::value;
+            uint32_t sp = 0, sx = 0;
+            for (int b = 0; b < nfull; ++b, sp += 768u, sx += 32u * rowb) {
+                if constexpr (H == 16)
+                    h16_block16<NC, NEG, GU, PH, true>(A1, A2, A3, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx, rowb, NB,
+                                                       32);
+                else
+                    h16_block_emb<H, NC, NEG, GU, PH, true>(A1, A2, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx, rowb, NB,
+                                                            32, imrow, sel1, sel2);
+            }
+            if (rem > 0) {
+                const int nsub = (rem + S - 1) / S;
+                if constexpr (H == 16)
+                    h16_block16<NC, NEG, GU, PH, false>(A1, A2, A3, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx, rowb,
+                                                        nsub, rem);
+                else
+                    h16_block_emb<H, NC, NEG, GU, PH, false>(A1, A2, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx, rowb,
+                                                             nsub, rem, imrow, sel1, sel2);
+            }
+        };
+        if (hr < 0) blocks(std::true_type());
+        else blocks(std::false_type());
+    }
+}
+
+// One tile on the f16 path with nrows valid rows: the 16-row form above 8
+// rows, else the real embedding with 8 or 4 row slots; rowptr(rsel) gives
+// the coordinates of row slot rsel (rows past the tile repeat its last). The
+// sums, scaled by osc (per column chunk), go to fin(acc, rsel) as a
+// Cacc<PrecF32, false> or Cemb<PrecF32, H> accumulator per chunk.
+template <int NC, bool NEG, class RowF, class FinF>
+__device__ __forceinline__ void h16_tile(const NearK& nk, int nrows, int lane, RowF rowptr, int64_t e0, int nr,
+                                         int rot, const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
+                                         const double* __restrict__ spts, const uint2* __restrict__ xsq, int64_t R,
+                                         const int (&jl)[NC], const float (&osc)[NC], FinF fin) {
+    using P = PrecF32;
+    const int i = lane & 15, kk = lane >> 4;
+    auto run = [&](auto ph) {
+        constexpr bool PH = decltype(ph)::value;
+        if (nrows > 8) {
+            flt4 Pa[NC], Qa[NC], Ia[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                for (int v = 0; v < 4; ++v) Pa[c][v] = Qa[c][v] = Ia[c][v] = 0.0f;
+            h16_walk<16, NC, NEG, PH>(Pa, Qa, Ia, nk, rowptr(i), lane, e0, nr, rot, nrlo, nrhi, spts, xsq, R, jl,
+                                      false, 0u, 0u);
+            Cacc<P, false> acc[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                acc[c].p1 = Pa[c] * osc[c];
+                acc[c].p2 = Qa[c] * osc[c];
+                acc[c].p3 = Ia[c] * osc[c];
+                for (int v = 0; v < 4; ++v) acc[c].p4[v] = 0.0f;
+            }
+            fin(acc, i);
+        } else {
+            auto emb = [&](auto hval) {
+                constexpr int H = decltype(hval)::value;
+                // v_perm(S0, S1): S1 the lower slot's source on real rows, the
+                // higher one on imaginary rows (there the lane's own slot is the
+                // higher); real rows take re halves for A1, imaginary rows im halves
+                const bool imrow = (i / H) == 1;
+                const uint32_t sel1 = imrow ? 0x03020706u : 0x05040100u;
+                const uint32_t sel2 = imrow ? 0x01000504u : 0x07060302u;
+                flt4 D1[NC], D2[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                    for (int v = 0; v < 4; ++v) D1[c][v] = D2[c][v] = 0.0f;
+                h16_walk<H, NC, NEG, PH>(D1, D2, D1, nk, rowptr(i % H), lane, e0, nr, rot, nrlo, nrhi, spts, xsq, R,
+                                         jl, imrow, sel1, sel2);
+                // this lane holds MFMA rows 4 kk + v: real parts when kk < H / 4
+                const float sg = kk < H / 4 ? -1.0f : 1.0f;
+                Cemb<P, H> acc[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                    for (int v = 0; v < 4; ++v) acc[c].d[v] = (D1[c][v] + sg * D2[c][v]) * osc[c];
+                fin(acc, i % H);
+            };
+            if (nrows > 4) emb(std::integral_constant<int, 8>());
+            else emb(std::integral_constant<int, 4>());
+        }
+    };
+    if (nk.phase32) run(std::true_type());
+    else run(std::false_type());
+}
+
+// Hands every output entry of a tile's accumulators (16-row form or real
+// embedding) to emitter(c)(row, value).
+template <int NC, class EF>
+__device__ __forceinline__ void emit_tile(Cacc<PrecF32, false> (&acc)[NC], EF emitter) {
+    const int kk = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const auto emit = emitter(c);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) emit(PrecF32::row(kk, v), acc[c].get(v));
+    }
+}
+template <int HE, int NC, class EF>
+__device__ __forceinline__ void emit_tile(Cemb<PrecF32, HE> (&acc)[NC], EF emitter) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c].store(emitter(c));
+}
+
 // Stage 5 + near field of MIXED products on the f16 path for 16 NC
 // right-hand sides (the tiles, tile order and output of k_leaf_out_mfma;
 // dpts / spts: the scaled point copies; xs16: the split right-hand sides of
@@ -1358,9 +1540,9 @@ __global__ void __launch_bounds__(256, 4)
     const int t = leaves[li], bi = leafbasis[li];
     const int l0 = dlo[t], nt = dhi[t] - l0;
     const int nrows = min(16, nt - row0);
-    const int i = lane & 15, kk = lane >> 4, j = lane & 15;
-    const int64_t e0 = nrptr[t], e1 = nrptr[t + 1];
-    const int nr = (int)(e1 - e0);
+    const int kk = lane >> 4, j = lane & 15;
+    const int64_t e0 = nrptr[t];
+    const int nr = (int)(nrptr[t + 1] - e0);
     const int rot = stagger && nr ? (w * 7) % nr : 0;
     int jl[NC];
     float osc[NC];
@@ -1371,26 +1553,6 @@ __global__ void __launch_bounds__(256, 4)
     }
     const int rt = dcptr[t + 1] - dcptr[t];
     const float2* B = bi >= 0 ? bblocks + boff[bi] : nullptr;
-    const uint2* xsq = xs16 + q0;
-    const uint32_t rowb = (uint32_t)R * 8u;
-    // basis rows: acc += A B with A = U_t (or V_t^T) rows, B = yh_t
-    auto basis = [&](auto& acc, int rsel) {
-        if (!B) return;
-        const int rowc = row0 + min(rsel, nrows - 1);
-        const double2* yt = yh + (int64_t)dcptr[t] * R + q0;
-        for (int k0 = 0; k0 < rt; k0 += 4) {
-            const int k = k0 + kk;
-            const int kc = min(k, rt - 1);
-            const float2 a =
-                conjif<CJ, P>(B[FWD ? (int64_t)rowc + (int64_t)kc * nt : (int64_t)kc + (int64_t)rowc * rt]);
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                float2 b = P::cvt(yt[(int64_t)kc * R + jl[c]]);
-                if (k >= rt) b = czero2<P>();
-                acc[c].mma(a, b);
-            }
-        }
-    };
     auto emitter = [&](int c) {
         return [&, c](int row, double2 v) {
             const int col = 16 * c + j;
@@ -1402,108 +1564,145 @@ __global__ void __launch_bounds__(256, 4)
             }
         };
     };
-    // the walk over the merged source ranges, blocks of B16 sub-steps of S sources
-    auto walk = [&](auto hval, auto ph, auto& A1, auto& A2, auto& A3, int rsel, bool imrow, uint32_t sel1,
-                    uint32_t sel2) {
-        constexpr int H = decltype(hval)::value;
-        constexpr bool PH = decltype(ph)::value;
-        constexpr int G = 16 / H, S = 64 / H, NB = 32 / S;
-        const double* xp = dpts + 3 * (int64_t)(l0 + row0 + min(rsel, nrows - 1));
-        const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
-        const int ysrc = 4 * ((lane & 15) / H) + kk;
-        const uint32_t vp = near_point_offset<G>(lane, kk, ysrc);
-        uint32_t vx[NC][G];
+    // then the basis rows: acc += A B with A = U_t (or V_t^T) rows, B = yh_t
+    auto fin = [&](auto& acc, int rsel) {
+        if (B) {
+            const int rowc = row0 + min(rsel, nrows - 1);
+            const double2* yt = yh + (int64_t)dcptr[t] * R + q0;
+            for (int k0 = 0; k0 < rt; k0 += 4) {
+                const int k = k0 + kk;
+                const int kc = min(k, rt - 1);
+                const float2 a =
+                    conjif<CJ, P>(B[FWD ? (int64_t)rowc + (int64_t)kc * nt : (int64_t)kc + (int64_t)rowc * rt]);
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
-#pragma unroll
-            for (int q = 0; q < G; ++q) vx[c][q] = ((uint32_t)(4 * q + kk) * (uint32_t)R + (uint32_t)jl[c]) * 8u;
-        for (int cc = 0; cc < nr; ++cc) {
-            int ec = cc + rot;
-            if (ec >= nr) ec -= nr;
-            const int64_t e = e0 + ec;
-            const int lo = nrlo[e], hr = nrhi[e];
-            const int hi = hr & 0x7fffffff;
-            const uint32_t n = (uint32_t)(hi - lo);
-            const __amdgpu_buffer_rsrc_t rp = buf_rsrc(spts + 3 * (int64_t)lo, n * 24u);
-            const __amdgpu_buffer_rsrc_t rx = buf_rsrc(xsq + (int64_t)lo * R, n * rowb);
-            const int nsub = (int)((n + S - 1) / S);
-            auto blocks = [&](auto guard) {
-                constexpr bool GU = decltype(guard)::value;
-                int left = nsub;
-                uint32_t sp = 0, sx = 0;
-                for (; left >= NB; left -= NB, sp += NB * S * 24u, sx += (uint32_t)(NB * S) * rowb) {
-                    if constexpr (H == 16)
-                        h16_block16<NC, NEG, GU, PH, true>(A1, A2, A3, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx, rowb,
-                                                           NB);
-                    else
-                        h16_block_emb<H, NC, NEG, GU, PH, true>(A1, A2, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx, rowb,
-                                                                NB, imrow, sel1, sel2);
+                for (int c = 0; c < NC; ++c) {
+                    float2 b = P::cvt(yt[(int64_t)kc * R + jl[c]]);
+                    if (k >= rt) b = czero2<P>();
+                    acc[c].mma(a, b);
                 }
-                if (left > 0) {
-                    if constexpr (H == 16)
-                        h16_block16<NC, NEG, GU, PH, false>(A1, A2, A3, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx, rowb,
-                                                            left);
-                    else
-                        h16_block_emb<H, NC, NEG, GU, PH, false>(A1, A2, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx,
-                                                                 rowb, left, imrow, sel1, sel2);
+            }
+        }
+        emit_tile(acc, emitter);
+    };
+    auto rowptr = [&](int rsel) { return dpts + 3 * (int64_t)(l0 + row0 + min(rsel, nrows - 1)); };
+    h16_tile<NC, NEG>(nk, nrows, lane, rowptr, e0, nr, rot, nrlo, nrhi, spts, xs16 + q0, R, jl, osc, fin);
+}
+
+// Stage 3 of MIXED products on the f16 path, the coupling blocks evaluated
+// on the fly (block (t, s) = G(test pivots of t, trial pivots of s), the
+// store's entries): one wave per tile of at most 16 rows of a destination
+// block, its rows the scaled coordinates of the block's pivots (dpiv, in
+// coefficient order), its sources the pivots of its far blocks as merged
+// coefficient ranges (crptr / crlo / crhi, bit 31 of the end: the guard) of
+// spiv, B = the split source coefficients xs16 (R per coefficient) with the
+// chunk's column scales xsc (relative to q0). The tiles and the output are
+// those of k_couple_fwd_mfma (yh, or yfin in FP64 for the blocks fin marks).
+template <bool NEG, int NC, class YT>
+__global__ void __launch_bounds__(256, 4)
+    k_couple_h16(const int32_t* __restrict__ tile_node, const int32_t* __restrict__ tile_row0, int64_t ntiles,
+                 const int32_t* __restrict__ cptr_dst, const double* __restrict__ dpiv,
+                 const int64_t* __restrict__ crptr, const int32_t* __restrict__ crlo,
+                 const int32_t* __restrict__ crhi, const double* __restrict__ spiv, const uint2* __restrict__ xs16,
+                 const float* __restrict__ xsc, int64_t R, int64_t q0, int nq, NearK nk, YT* __restrict__ yh,
+                 double2* __restrict__ yfin, const uint8_t* __restrict__ fin, int order) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
+    if (w >= ntiles) return;
+    const int t = tile_node[w], row0 = tile_row0[w];
+    if (t < 0) return;  // padding tile
+    const int r0 = cptr_dst[t], rt = cptr_dst[t + 1] - r0;
+    const int nrows = min(16, rt - row0);
+    const int j = lane & 15;
+    const int64_t e0 = crptr[t];
+    const int nr = (int)(crptr[t + 1] - e0);
+    int jl[NC];
+    float osc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        jl[c] = min(16 * c + j, nq - 1);
+        osc[c] = nk.tsc * xsc[2 * jl[c] + 1];
+    }
+    const int64_t o0 = (int64_t)(r0 + row0) * R + q0 + j;
+    const bool tofin = fin && fin[t];
+    auto emitter = [&](int c) {
+        return [&, c](int row, double2 v) {
+            if (row < nrows && 16 * c + j < nq) {
+                const int64_t o = o0 + 16 * c + (int64_t)row * R;
+                if (tofin) yfin[o] = v;
+                else store_as(yh + o, v);
+            }
+        };
+    };
+    auto fn = [&](auto& acc, int) { emit_tile(acc, emitter); };
+    auto rowptr = [&](int rsel) { return dpiv + 3 * (int64_t)(r0 + row0 + min(rsel, nrows - 1)); };
+    h16_tile<NC, NEG>(nk, nrows, lane, rowptr, e0, nr, 0, crlo, crhi, spiv, xs16 + q0, R, jl, osc, fn);
+}
+
+// Smallest squared distance above the zero-distance guard (2e-30) and the
+// largest one over the pairs of each coupling tile (rows: the tile's
+// pivots, dpiv + 3 (cptr[t] + row), sources: spiv over its merged ranges):
+// out[2 w], out[2 w + 1] (+inf and 0 without a pair, or for a padding tile).
+__global__ void __launch_bounds__(256)
+    k_pair_rminmax(const int32_t* __restrict__ tile_node, const int32_t* __restrict__ tile_row0, int64_t ntiles,
+                   const int32_t* __restrict__ cptr_dst, const double* __restrict__ dpiv,
+                   const int64_t* __restrict__ crptr, const int32_t* __restrict__ crlo,
+                   const int32_t* __restrict__ crhi, const double* __restrict__ spiv, double* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= ntiles) return;
+    const int t = tile_node[w];
+    double m = HUGE_VAL, mx = 0.0;
+    if (t >= 0) {
+        const int r0 = cptr_dst[t] + tile_row0[w], nrows = min(16, cptr_dst[t + 1] - r0);
+        for (int64_t e = crptr[t]; e < crptr[t + 1]; ++e) {
+            const int lo = crlo[e], hi = crhi[e] & 0x7fffffff;
+            for (int s = lo + lane; s < hi; s += 64) {
+                const double y0 = spiv[3 * (int64_t)s], y1 = spiv[3 * (int64_t)s + 1], y2 = spiv[3 * (int64_t)s + 2];
+                for (int r = 0; r < nrows; ++r) {
+                    const double* xp = dpiv + 3 * (int64_t)(r0 + r);
+                    const double dx = xp[0] - y0, dy = xp[1] - y1, dz = xp[2] - y2;
+                    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                    if (r2 > 2e-30) m = fmin(m, r2);
+                    mx = fmax(mx, r2);
                 }
-            };
-            if (hr < 0) blocks(std::true_type());
-            else blocks(std::false_type());
-        }
-    };
-    auto run = [&](auto ph) {
-        if (nrows > 8) {
-            flt4 Pa[NC], Qa[NC], Ia[NC];
-#pragma unroll
-            for (int c = 0; c < NC; ++c)
-                for (int v = 0; v < 4; ++v) Pa[c][v] = Qa[c][v] = Ia[c][v] = 0.0f;
-            walk(std::integral_constant<int, 16>(), ph, Pa, Qa, Ia, i, false, 0u, 0u);
-            Cacc<P, false> acc[NC];
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                acc[c].p1 = Pa[c] * osc[c];
-                acc[c].p2 = Qa[c] * osc[c];
-                acc[c].p3 = Ia[c] * osc[c];
-                for (int v = 0; v < 4; ++v) acc[c].p4[v] = 0.0f;
             }
-            basis(acc, i);
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                const auto emit = emitter(c);
-#pragma unroll
-                for (int v = 0; v < 4; ++v) emit(P::row(kk, v), acc[c].get(v));
-            }
-        } else {
-            auto emb = [&](auto hval) {
-                constexpr int H = decltype(hval)::value;
-                // v_perm(S0, S1): S1 the lower slot's source on real rows, the
-                // higher one on imaginary rows (there the lane's own slot is the
-                // higher); real rows take re halves for A1, imaginary rows im halves
-                const bool imrow = (i / H) == 1;
-                const uint32_t sel1 = imrow ? 0x03020706u : 0x05040100u;
-                const uint32_t sel2 = imrow ? 0x01000504u : 0x07060302u;
-                flt4 D1[NC], D2[NC];
-#pragma unroll
-                for (int c = 0; c < NC; ++c)
-                    for (int v = 0; v < 4; ++v) D1[c][v] = D2[c][v] = 0.0f;
-                walk(hval, ph, D1, D2, D1, i % H, imrow, sel1, sel2);
-                // this lane holds MFMA rows 4 kk + v: real parts when kk < H / 4
-                const float sg = kk < H / 4 ? -1.0f : 1.0f;
-                Cemb<P, H> acc[NC];
-#pragma unroll
-                for (int c = 0; c < NC; ++c)
-                    for (int v = 0; v < 4; ++v) acc[c].d[v] = (D1[c][v] + sg * D2[c][v]) * osc[c];
-                basis(acc, i % H);
-#pragma unroll
-                for (int c = 0; c < NC; ++c) acc[c].store(emitter(c));
-            };
-            if (nrows > 4) emb(std::integral_constant<int, 8>());
-            else emb(std::integral_constant<int, 4>());
         }
-    };
-    if (nk.phase32) run(std::true_type());
-    else run(std::false_type());
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        m = fmin(m, __shfl_xor(m, o));
+        mx = fmax(mx, __shfl_xor(mx, o));
+    }
+    if (lane == 0) {
+        out[2 * w] = m;
+        out[2 * w + 1] = mx;
+    }
+}
+
+// Per-workgroup column maxima of |re|, |im| of a c32 [n][R] array over
+// columns q0 .. q0 + nc - 1 (64 rows x 16 columns per workgroup; lanes along
+// the columns): part[blockIdx.x nc + c].
+__global__ void __launch_bounds__(256)
+    k_colmax_c32(const float2* __restrict__ in, int64_t n, int64_t R, int64_t q0, int nc, float* __restrict__ part) {
+    __shared__ float red[16][17];
+    const int t = threadIdx.x, cl = t & 15, rg = t >> 4;
+    const int c = (int)blockIdx.y * 16 + cl;
+    float m = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int64_t row = (int64_t)blockIdx.x * 64 + rg + 16 * r;
+        if (row < n && c < nc) {
+            const float2 v = in[row * R + q0 + c];
+            m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+        }
+    }
+    red[rg][cl] = m;
+    __syncthreads();
+    if (t < 16) {
+        float a = red[0][t];
+        for (int q = 1; q < 16; ++q) a = fmaxf(a, red[q][t]);
+        const int cc = (int)blockIdx.y * 16 + t;
+        if (cc < nc) part[(int64_t)blockIdx.x * nc + cc] = a;
+    }
 }
 
 // Smallest squared distance above the zero-distance guard (2e-30) and the
@@ -1545,6 +1744,14 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// out[3 k + d] = pts[3 pos[k] + d] (k < K), zeros for k in [K, K + pad).
+__global__ void k_gather_pivots(const double* __restrict__ pts, const int32_t* __restrict__ pos, int64_t K,
+                                double* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K + kNearPad) return;
+    for (int d = 0; d < 3; ++d) out[3 * k + d] = k < K ? pts[3 * (int64_t)pos[k] + d] : 0.0;
+}
+
 // out = in * t (points, a power of two: exact)
 __global__ void k_scale_pts(const double* __restrict__ in, double* __restrict__ out, int64_t n, double t) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1580,16 +1787,19 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// xs16[p R + q] = the f16 parts of s_q xs32[p R + q] (p < npts, q < R).
-__global__ void k_xs16(const float2* __restrict__ xs32, const float* __restrict__ xsc, int64_t npts, int64_t R,
-                       uint2* __restrict__ xs16) {
+// out[p R + q0 + c] = the f16 parts of s_c in[p R + q0 + c] (p < n, c < nc;
+// s_c = xsc[2 c]).
+__global__ void k_split16(const float2* __restrict__ in, const float* __restrict__ xsc, int64_t n, int64_t R,
+                          int64_t q0, int nc, uint2* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npts * R) return;  // npts R < 2^31 (checked per product)
-    const float s = xsc[2 * ((uint32_t)i % (uint32_t)R)];
-    const float2 v = xs32[i];
+    if (i >= n * nc) return;  // n R < 2^31 (checked per product)
+    const uint32_t p = (uint32_t)i / (uint32_t)nc, c = (uint32_t)i % (uint32_t)nc;
+    const int64_t o = (int64_t)p * R + q0 + c;
+    const float s = xsc[2 * c];
+    const float2 v = in[o];
     uint32_t h, l;
     split_h16(v.x * s, v.y * s, h, l);
-    xs16[i] = make_uint2(h, l);
+    out[o] = make_uint2(h, l);
 }
 
 // Stage 5 + near field for NQ = 1..3 right-hand sides on VALU (an MFMA
@@ -2096,6 +2306,18 @@ const typename Ops<MIX>::T2* stored_near(const nca_h2* h) {
     else return h->ns_blocks.p;
 }
 
+// The operator's MIXED products evaluate the coupling on the fly (no stored
+// c32 or transposed copy needed).
+inline bool couple_otf(const nca_h2* h) {
+    return h->precision == NCA_PREC_MIXED && h->coup_h16 == 1 && h->couple_k3[1] == 2;
+}
+
+// This MIXED product's coupling is evaluated on the fly on the f16 path.
+template <bool MIX>
+bool couple_h16(const nca_h2* h) {
+    return MIX && h->coup_h16 == 1 && h->couple_k3[1] == 2 && h->ws.xh16 && h->ws.xhpart && h->ws.xhsc;
+}
+
 // The near field of this MIXED product's MFMA chunks runs on the f16 path:
 // the operator allows it, the launch choice asks for it, the workspace holds
 // its buffers, and it is evaluated (not stored).
@@ -2103,6 +2325,53 @@ template <bool MIX>
 bool near_h16(const nca_h2* h, bool fwd) {
     return MIX && h->h16 == 1 && h->near_k3[1] == 2 && h->ws.xs16 && h->ws.xpart && h->ws.xsc &&
            !(fwd && stored_near<MIX>(h));
+}
+
+// Stage 3 of a MIXED product evaluated on the fly for columns q0 .. q0 + nq -
+// 1 (nq <= 32): the source coefficients of those columns split into f16
+// parts with their column scales, then one k_couple_h16 launch; forward the
+// c32 result goes to yc (yfin for the blocks fin marks) or, with yc null,
+// to yh; transpose and adjoint write yh.
+template <int NC, int OP>
+void run_couple_h16(nca_h2* h, int64_t R, int nq, int64_t q0, float2* yc, const uint8_t* fin) {
+    constexpr bool FWD = OP == NCA_OP_N;
+    constexpr bool CJ = OP == NCA_OP_C;
+    hipStream_t st = h->ctx->stream;
+    const SideDev& src = FWD ? h->trial : h->test;
+    const SideDev& dst = FWD ? h->test : h->trial;
+    const float2* xh = reinterpret_cast<const float2*>(h->ws.xh);
+    if (src.K > 0) {
+        hipLaunchKernelGGL(k_colmax_c32, dim3(blocks_for(src.K, 64), blocks_for(nq, 16)), dim3(256), 0, st, xh,
+                           src.K, R, q0, nq, h->ws.xhpart);
+        hipLaunchKernelGGL(k_colscale, dim3((unsigned)nq), dim3(256), 0, st, h->ws.xhpart,
+                           (int64_t)blocks_for(src.K, 64), (int64_t)nq, h->ws.xhsc);
+        hipLaunchKernelGGL(k_split16, dim3(blocks_for(src.K * nq, 256)), dim3(256), 0, st, xh, h->ws.xhsc, src.K, R,
+                           q0, nq, h->ws.xh16);
+    }
+    const int64_t ntiles = FWD ? h->f_ntiles : h->cr_ntiles;
+    if (ntiles > 0) {
+        const bool neg = (h->sign < 0) != CJ;
+        const NearK nk = near_consts_scaled(h->k, std::fabs(h->k) * 0.5 / M_PI * h->coup_rmax, h->coup_t);
+        const int32_t* tn = FWD ? h->f_tile_node.p : h->cr_tile_node.p;
+        const int32_t* tr = FWD ? h->f_tile_row0.p : h->cr_tile_row0.p;
+        const double* dpiv = FWD ? h->piv_test.p : h->piv_trial.p;
+        const double* spiv = FWD ? h->piv_trial.p : h->piv_test.p;
+        const int64_t* crp = FWD ? h->cf_ptr.p : h->cr_ptr.p;
+        const int32_t* crl = FWD ? h->cf_lo.p : h->cr_lo.p;
+        const int32_t* crh = FWD ? h->cf_hi.p : h->cr_hi.p;
+        if (yc) {
+            auto kern = neg ? k_couple_h16<true, NC, float2> : k_couple_h16<false, NC, float2>;
+            hipLaunchKernelGGL(kern, dim3(blocks_for(ntiles, 4)), dim3(256), 0, st, tn, tr, ntiles, dst.cptr.p, dpiv,
+                               crp, crl, crh, spiv, h->ws.xh16, h->ws.xhsc, R, q0, nq, nk, yc, h->ws.yh, fin,
+                               h->couple_xcd);
+        } else {
+            auto kern = neg ? k_couple_h16<true, NC, double2> : k_couple_h16<false, NC, double2>;
+            hipLaunchKernelGGL(kern, dim3(blocks_for(ntiles, 4)), dim3(256), 0, st, tn, tr, ntiles, dst.cptr.p, dpiv,
+                               crp, crl, crh, spiv, h->ws.xh16, h->ws.xhsc, R, q0, nq, nk, h->ws.yh, h->ws.yh,
+                               nullptr, h->couple_xcd);
+        }
+    }
+    NCA_HIP(hipGetLastError());
 }
 
 // Stage 5 and the near field on the MFMA kernel for columns q0 .. q0 + nq - 1
@@ -2187,6 +2456,12 @@ void run_couple(nca_h2* h, int64_t R, int nq, int64_t q0) {
     SideDev& src = h->trial;
     SideDev& dst = h->test;
     if (h->f_ntiles == 0) return;
+    if constexpr (MIX) {
+        if (couple_h16<MIX>(h)) {
+            run_couple_h16<NC, OP>(h, R, nq, q0, h->ws.yc, dst.orph.p);
+            return;
+        }
+    }
     const T2* xh = reinterpret_cast<const T2*>(h->ws.xh);
     double2* yh = h->ws.yh;
     const T2* S = O::coupling(h);
@@ -2274,7 +2549,11 @@ void run_chunk(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy
     }
     if (phases & kPhaseCouple) {
     const T2* S = O::coupling(h);
-    if constexpr (FWD) {
+    if (h->coupling->nentries > 0 && couple_h16<MIX>(h) && (!FWD || NR < 16)) {
+        // on the fly: transpose / adjoint, and forward chunks of 1 to 3
+        // columns (into the cleared yh, which their VALU downward pass adds to)
+        run_couple_h16<1, OP>(h, R, nq, q0, nullptr, nullptr);
+    } else if constexpr (FWD) {
         if constexpr (NR == 16) {
             run_couple<1, OP, MIX>(h, R, nq, q0);
         } else if (h->coupling->nentries > 0) {
@@ -2394,8 +2673,8 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
         if (h16) {
             hipLaunchKernelGGL(k_colscale, dim3((unsigned)R), dim3(256), 0, st, h->ws.xpart,
                                (int64_t)blocks_for(src.npts, 64), R, h->ws.xsc);
-            hipLaunchKernelGGL(k_xs16, dim3(blocks_for(src.npts * R, 256)), dim3(256), 0, st, h->ws.xs32,
-                               h->ws.xsc, src.npts, R, h->ws.xs16);
+            hipLaunchKernelGGL(k_split16, dim3(blocks_for(src.npts * R, 256)), dim3(256), 0, st, h->ws.xs32,
+                               h->ws.xsc, src.npts, R, (int64_t)0, (int)R, h->ws.xs16);
         }
     }
     // The MFMA chunks write every coefficient of xh (projection, upward pass)
@@ -2447,6 +2726,22 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     }
 }
 
+// The set-up of the on-the-fly coupling (defined with the plans below).
+void setup_couple_h16(nca_h2* h);
+
+// The c32 copy of the coupling blocks (MIXED products that stream the stored
+// coupling: couple_k3 0 or 1, or the on-the-fly form unavailable). Made
+// outside any capture: by the MIXED set-up, or when such a choice is applied.
+void ensure_s32(nca_h2* h) {
+    if (h->S32.p) return;
+    hipStream_t st = h->ctx->stream;
+    const int64_t n = h->coupling->nentries;
+    h->S32.alloc((size_t)std::max<int64_t>(n, 1));
+    if (n > 0) hipLaunchKernelGGL(k_to_c32, dim3(blocks_for(n, 256)), dim3(256), 0, st, h->coupling->blocks.p, h->S32.p, n);
+    NCA_HIP(hipGetLastError());
+    NCA_HIP(hipStreamSynchronize(st));
+}
+
 // c32 copies of the operator for the MIXED path (made once: at creation in
 // MIXED, by nca_h2_set_precision, or on first use). It allocates once and
 // frees nothing a captured sweep uses, so the workspace generation stays.
@@ -2458,7 +2753,6 @@ void ensure_mixed(nca_h2* h) {
         if (n > 0)
             hipLaunchKernelGGL(k_to_c32, dim3(blocks_for(n, 256)), dim3(256), 0, st, in.p, out.p, n);
     };
-    conv(h->coupling->blocks, h->S32, h->coupling->nentries);
     for (SideDev* sd : {&h->test, &h->trial}) {
         conv(sd->bblocks, sd->bblocks32, sd->basis_entries);
         conv(sd->tblocks, sd->tblocks32, sd->transfer_entries);
@@ -2501,6 +2795,8 @@ void ensure_mixed(nca_h2* h) {
     }
     NCA_HIP(hipGetLastError());
     NCA_HIP(hipStreamSynchronize(st));
+    setup_couple_h16(h);
+    if (h->coup_h16 != 1 || h->couple_k3[1] != 2) ensure_s32(h);
     h->mixed_ready = true;
 }
 
@@ -2521,6 +2817,7 @@ void ensure_workspace(nca_h2* h, int64_t R) {
         h->ws_xs16.release();
         h->ws_xpart.release();
         h->ws_xsc.release();
+        h->ws_xh16.release();
         h->ws_rhs = R;
     }
     if (h->precision == NCA_PREC_MIXED) {
@@ -2531,6 +2828,11 @@ void ensure_workspace(nca_h2* h, int64_t R) {
             if (!h->ws_xs16.p) h->ws_xs16.alloc((size_t)std::max<int64_t>(np * h->ws_rhs, 1));
             if (!h->ws_xpart.p) h->ws_xpart.alloc((size_t)std::max<int64_t>(blocks_for(np, 64) * h->ws_rhs, 1));
             if (!h->ws_xsc.p) h->ws_xsc.alloc((size_t)std::max<int64_t>(2 * h->ws_rhs, 1));
+        }
+        if (h->coup_h16 == 1) {
+            if (!h->ws_xh16.p) h->ws_xh16.alloc((size_t)std::max<int64_t>(K * h->ws_rhs, 1));
+            if (!h->ws_xhpart.p) h->ws_xhpart.alloc((size_t)std::max<int64_t>(blocks_for(K, 64) * 32, 1));
+            if (!h->ws_xhsc.p) h->ws_xhsc.alloc(64);
         }
     } else {
         if (!h->ws_xs.p) h->ws_xs.alloc((size_t)std::max<int64_t>((np + kNearPad) * h->ws_rhs, 1));
@@ -2577,12 +2879,12 @@ void matvec_device(nca_h2* h, int32_t op, int64_t R, const double2* x, int64_t l
                    const double* alpha, const double* beta, StageEvents* evs = nullptr) {
     if (!check_product(h, op, R, ldx, ldy)) return;
     if (h->precision == NCA_PREC_MIXED) ensure_mixed(h);
-    if (op != NCA_OP_N) ensure_reverse(h);
+    if (op != NCA_OP_N && !couple_otf(h)) ensure_reverse(h);
     if (op == NCA_OP_N && h->near_stored) ensure_near_store(h);
     ensure_workspace(h, R);
-    h->ws = nca_h2::WsView{h->ws_xs.p,  h->ws_xs32.p, h->ws_xh.p,    h->ws_yh.p,
-                           h->ws_yc.p,  h->ws_xh32.p, h->near_stored ? h->ws_ynear.p : nullptr,
-                           h->ws_xs16.p, h->ws_xpart.p, h->ws_xsc.p};
+    h->ws = nca_h2::WsView{h->ws_xs.p,   h->ws_xs32.p,  h->ws_xh.p,   h->ws_yh.p,
+                           h->ws_yc.p,   h->ws_xh32.p,  h->near_stored ? h->ws_ynear.p : nullptr,
+                           h->ws_xs16.p, h->ws_xpart.p, h->ws_xsc.p, h->ws_xh16.p, h->ws_xhpart.p, h->ws_xhsc.p};
     run_sweep(h, op, R, x, ldx, y, ldy, alpha, beta, evs);
 }
 
@@ -2638,6 +2940,11 @@ size_t carve_workspace(const nca_h2* h, int64_t R, char* base, nca_h2::WsView& v
             v.xs16 = reinterpret_cast<uint2*>(take(np * r * sizeof(uint2)));
             v.xpart = reinterpret_cast<float*>(take((size_t)blocks_for((int64_t)np, 64) * r * sizeof(float)));
             v.xsc = reinterpret_cast<float*>(take(2 * r * sizeof(float)));
+        }
+        if (h->coup_h16 == 1 && h->couple_k3[1] == 2) {
+            v.xh16 = reinterpret_cast<uint2*>(take(K * r * sizeof(uint2)));
+            v.xhpart = reinterpret_cast<float*>(take((size_t)blocks_for((int64_t)K, 64) * 32 * sizeof(float)));
+            v.xhsc = reinterpret_cast<float*>(take(64 * sizeof(float)));
         }
     }
     // the stored near field is used when its entries of this precision exist
@@ -3130,6 +3437,7 @@ void ensure_reverse(nca_h2* h) {
     const int64_t nb = cp->nblocks;
     if (mix && !h->rv_blocks32.p) {
         ensure_mixed(h);
+        ensure_s32(h);
         h->rv_blocks32.alloc((size_t)cp->nentries);
         hipLaunchKernelGGL(k_transpose_blocks<float2>, dim3(blocks_for(nb, 4)), dim3(256), 0, st, h->S32.p,
                            h->rv_blocks32.p, h->r_off.p, h->rv_dst.p, h->r_test.p, h->test.cptr.p, h->rv_rs.p, nb);
@@ -3141,6 +3449,136 @@ void ensure_reverse(nca_h2* h) {
     }
     NCA_HIP(hipGetLastError());
     NCA_HIP(hipStreamSynchronize(st));
+}
+
+// On-the-fly MIXED coupling: for operators of one coefficient block per node,
+// the pivot coordinates of both sides in coefficient order (gathered on the
+// device from the tree-ordered points), the far blocks as merged coefficient
+// ranges per test block and per trial block (a range keeps the zero-distance
+// guard when the two clusters' point boxes come within 1e-14: radius-0
+// clusters can be admissible with themselves), the trial blocks' 16-row
+// tiles, and the smallest and largest far pivot distances (measured on the
+// device); available when those span at most 2^17, with the pivot coordinates
+// scaled by t = 2^-14-e for r_min = f 2^e, f in [0.5, 1).
+void setup_couple_h16(nca_h2* h) {
+    h->coup_h16 = 0;
+    const nca_coupling* cp = h->coupling;
+    if (cp->nentries == 0 || h->test.ne != h->test.nn || h->trial.ne != h->trial.nn || h->f_ntiles == 0 ||
+        (int64_t)cp->tptr.size() != h->test.ne + 1 || (int64_t)cp->sptr.size() != h->trial.ne + 1)
+        return;
+    hipStream_t st = h->ctx->stream;
+    // pivot coordinates (unscaled first): coefficient cptr[t] + i is pivot i of block t
+    auto pivots = [&](const SideDev& sd, const std::vector<int64_t>& pptr, const std::vector<int32_t>& piv,
+                      DevBuf<double>& out) {
+        std::vector<int32_t> inv(sd.npts);
+        for (int64_t i = 0; i < sd.npts; ++i) inv[sd.h_perm[i]] = (int32_t)i;
+        std::vector<int32_t> pos((size_t)sd.K);
+        for (int64_t t = 0; t < sd.ne; ++t) {
+            const int64_t r = sd.h_cptr[t + 1] - sd.h_cptr[t];
+            NCA_REQUIRE(r == pptr[t + 1] - pptr[t], NCA_ERR_ARGUMENT, "nca_h2: coupling pivots and ranks disagree");
+            for (int64_t i = 0; i < r; ++i) pos[sd.h_cptr[t] + i] = inv[piv[pptr[t] + i]];
+        }
+        DevBuf<int32_t> dpos;
+        dpos.upload(pos, st);
+        out.alloc((size_t)(3 * (sd.K + kNearPad)));
+        hipLaunchKernelGGL(k_gather_pivots, dim3(blocks_for(sd.K + kNearPad, 256)), dim3(256), 0, st, sd.pts.p,
+                           dpos.p, sd.K, out.p);
+        NCA_HIP(hipGetLastError());
+        NCA_HIP(hipStreamSynchronize(st));
+    };
+    pivots(h->test, cp->tptr, cp->tpiv, h->piv_test);
+    pivots(h->trial, cp->sptr, cp->spiv, h->piv_trial);
+    auto box_d2 = [](const double* a, const double* b) {
+        double d2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const double g = std::max({0.0, a[d] - b[3 + d], b[d] - a[3 + d]});
+            d2 += g * g;
+        }
+        return d2;
+    };
+    // merged coefficient ranges of the far blocks of each destination block
+    auto ranges = [&](const SideDev& dsd, const SideDev& ssd, const std::vector<int64_t>& ptr,
+                      const std::vector<int32_t>& nodes, const std::vector<int32_t>& srank, DevBuf<int64_t>& dptr,
+                      DevBuf<int32_t>& dlo, DevBuf<int32_t>& dhi) {
+        const int64_t nn = dsd.ne;
+        struct Rg { int32_t lo, hi; bool guard; };
+        std::vector<std::vector<Rg>> per(nn);
+        parallel_for(nn, [&](int64_t t) {
+            auto& rg = per[t];
+            for (int64_t i = ptr[t]; i < ptr[t + 1]; ++i) {
+                const int32_t q = nodes[i];
+                if (srank[q] > 0)
+                    rg.push_back({ssd.h_cptr[q], ssd.h_cptr[q + 1],
+                                  !(box_d2(&dsd.h_box[6 * t], &ssd.h_box[6 * (size_t)q]) > 1e-28)});
+            }
+            std::sort(rg.begin(), rg.end(), [](const Rg& a, const Rg& b) { return a.lo < b.lo; });
+            size_t k = 0;
+            for (size_t q = 0; q < rg.size(); ++q) {
+                if (k > 0 && rg[k - 1].hi == rg[q].lo && rg[k - 1].guard == rg[q].guard) rg[k - 1].hi = rg[q].hi;
+                else rg[k++] = rg[q];
+            }
+            rg.resize(k);
+        });
+        std::vector<int64_t> mp(nn + 1, 0);
+        for (int64_t t = 0; t < nn; ++t) mp[t + 1] = mp[t] + (int64_t)per[t].size();
+        std::vector<int32_t> lo((size_t)mp[nn]), hi((size_t)mp[nn]);
+        parallel_for(nn, [&](int64_t t) {
+            int64_t o = mp[t];
+            for (auto& r : per[t]) {
+                lo[o] = r.lo;
+                hi[o] = (int32_t)((uint32_t)r.hi | (r.guard ? 0x80000000u : 0u));
+                ++o;
+            }
+        });
+        dptr.upload(mp, st);
+        dlo.upload(lo, st);
+        dhi.upload(hi, st);
+    };
+    std::vector<int32_t> fnodes(cp->nodes.begin(), cp->nodes.end());
+    ranges(h->test, h->trial, cp->ptr, fnodes, cp->trialrank, h->cf_ptr, h->cf_lo, h->cf_hi);
+    std::vector<int64_t> rptr;
+    std::vector<int32_t> rnodes((size_t)cp->nblocks);
+    csr_transpose(
+        h->test.ne, cp->ptr.data(), h->trial.ne, [&](int64_t i) { return cp->nodes[i]; }, rptr,
+        [&](int64_t, int64_t t, int64_t pos) { rnodes[pos] = (int32_t)t; });
+    ranges(h->trial, h->test, rptr, rnodes, cp->testrank, h->cr_ptr, h->cr_lo, h->cr_hi);
+    std::vector<int32_t> tnode, trow;
+    level_xcd_tiles(h->trial, cp->trialrank, tnode, trow);
+    h->cr_ntiles = (int64_t)tnode.size();
+    h->cr_tile_node.upload(tnode, st);
+    h->cr_tile_row0.upload(trow, st);
+    // far pivot distances over the forward tiles (every far pair once)
+    DevBuf<double> mm((size_t)(2 * h->f_ntiles));
+    hipLaunchKernelGGL(k_pair_rminmax, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, h->f_tile_node.p,
+                       h->f_tile_row0.p, h->f_ntiles, h->test.cptr.p, h->piv_test.p, h->cf_ptr.p, h->cf_lo.p,
+                       h->cf_hi.p, h->piv_trial.p, mm.p);
+    NCA_HIP(hipGetLastError());
+    std::vector<double> hm((size_t)(2 * h->f_ntiles));
+    NCA_HIP(hipMemcpyAsync(hm.data(), mm.p, hm.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    NCA_HIP(hipStreamSynchronize(st));
+    double r2 = HUGE_VAL, r2x = 0.0;
+    for (size_t q = 0; q < hm.size(); q += 2) {
+        r2 = std::min(r2, hm[q]);
+        r2x = std::max(r2x, hm[q + 1]);
+    }
+    const double rmin = std::sqrt(r2), rmax = std::sqrt(r2x);
+    h->coup_rmin = std::isfinite(rmin) ? rmin : 0.0;
+    h->coup_rmax = rmax;
+    if (!(std::isfinite(rmin) && rmin > 0.0 && rmax <= 131072.0 * rmin)) {
+        h->piv_test.release();
+        h->piv_trial.release();
+        return;
+    }
+    int e = 0;
+    (void)std::frexp(rmin, &e);
+    h->coup_t = std::ldexp(1.0, -14 - e);
+    for (DevBuf<double>* b : {&h->piv_test, &h->piv_trial}) {
+        const int64_t n = (int64_t)b->n;
+        hipLaunchKernelGGL(k_scale_pts, dim3(blocks_for(n, 256)), dim3(256), 0, st, b->p, b->p, n, h->coup_t);
+    }
+    NCA_HIP(hipGetLastError());
+    NCA_HIP(hipStreamSynchronize(st));
+    h->coup_h16 = 1;
 }
 
 // The stored near field of forward products (the reference CPU path keeps
@@ -3393,10 +3831,7 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
             h->sctab.upload(tab, st);
         }
         if (const char* v = getenv("NCA_MFMA_K3")) h->near_k3[0] = h->couple_k3[0] = atoi(v) != 0;
-        if (const char* v = getenv("NCA_MIXED_K3")) {
-            h->near_k3[1] = std::min(std::max(atoi(v), 0), 2);
-            h->couple_k3[1] = atoi(v) != 0;
-        }
+        if (const char* v = getenv("NCA_MIXED_K3")) h->near_k3[1] = h->couple_k3[1] = std::min(std::max(atoi(v), 0), 2);
         if (const char* v = getenv("NCA_ORDER_NEAR")) h->near_xcd = atoi(v);
         if (const char* v = getenv("NCA_ORDER_COUPLE")) h->couple_xcd = atoi(v);
         if (const char* v = getenv("NCA_LEAF_ORDER")) h->leaf_order = atoi(v);
@@ -3996,7 +4431,7 @@ int32_t nca_h2_graph_create(nca_h2* h2, int32_t op, int64_t nrhs, const void* x,
         hipStream_t st = h2->ctx->stream;
         // everything that allocates or converts runs before the capture
         if (h2->precision == NCA_PREC_MIXED) ensure_mixed(h2);
-        if (op != NCA_OP_N) ensure_reverse(h2);
+        if (op != NCA_OP_N && !couple_otf(h2)) ensure_reverse(h2);
         if (op == NCA_OP_N && h2->near_stored) ensure_near_store(h2);
         ensure_workspace(h2, nrhs);
         NCA_HIP(hipStreamSynchronize(st));
@@ -4100,8 +4535,12 @@ int32_t nca_h2_autotune(nca_h2* h2, int32_t reps, nca_h2_tuning* out) {
         // coupling: products x tile order x workgroup order
         int ck3 = h2->couple_k3[prec], co = h2->couple_order, cx = h2->couple_xcd;
         double cbest = c0;
+        // MIXED: also the coupling evaluated on the fly (couple_k3 = 2); the
+        // stored forms need the c32 copy
+        const int ck = prec == 1 && h2->coup_h16 == 1 ? 3 : 2;
+        if (prec == 1) ensure_s32(h2);
         if (h2->f_ntiles > 0)
-            for (int k3 = 0; k3 < 2; ++k3)
+            for (int k3 = 0; k3 < ck; ++k3)
                 for (int o = 0; o < 4; ++o)
                     for (int xo = 0; xo < 2; ++xo) {
                         h2->couple_k3[prec] = k3;
@@ -4139,7 +4578,7 @@ int32_t nca_h2_tuning_apply(nca_h2* h2, const nca_h2_tuning* t) {
     return guarded([&] {
         NCA_REQUIRE(h2 && t, NCA_ERR_ARGUMENT, "nca_h2_tuning_apply: invalid argument");
         NCA_REQUIRE(t->near_k3 >= 0 && t->near_k3 <= (h2->precision == NCA_PREC_MIXED ? 2 : 1) &&
-                        t->couple_k3 >= 0 && t->couple_k3 <= 1 &&
+                        t->couple_k3 >= 0 && t->couple_k3 <= (h2->precision == NCA_PREC_MIXED ? 2 : 1) &&
                         t->leaf_order >= 0 && t->leaf_order <= 2 && t->near_xcd_order >= 0 &&
                         t->near_xcd_order <= 1 && t->near_stagger >= 0 && t->near_stagger <= 1 &&
                         t->couple_tile_order >= 0 && t->couple_tile_order <= 3 && t->couple_xcd_order >= 0 &&
@@ -4159,6 +4598,8 @@ int32_t nca_h2_tuning_apply(nca_h2* h2, const nca_h2_tuning* t) {
             h2->leaf_order = t->leaf_order;
         }
         h2->couple_k3[prec] = t->couple_k3;
+        // a stored MIXED coupling form needs the c32 copy
+        if (prec == 1 && h2->mixed_ready && !couple_otf(h2)) ensure_s32(h2);
         h2->couple_xcd = t->couple_xcd_order;
         if (h2->f_ntiles > 0 && h2->couple_order != t->couple_tile_order) {
             order_coupling_tiles(h2, t->couple_tile_order, st);
@@ -4228,6 +4669,22 @@ int32_t nca_h2_get_stats(const nca_h2* h2, nca_h2_stats* out) {
         NCA_REQUIRE(h2 && out, NCA_ERR_ARGUMENT, "nca_h2_get_stats: invalid argument");
         std::lock_guard<std::mutex> lock(h2->mu);
         *out = h2->stats;
+    });
+}
+
+int32_t nca_h2_couple_form(nca_h2* h2, int32_t* form, double* rmin, double* rmax) {
+    return guarded([&] {
+        NCA_REQUIRE(h2 && form, NCA_ERR_ARGUMENT, "nca_h2_couple_form: invalid argument");
+        std::lock_guard<std::mutex> lock(h2->mu);
+        set_device(h2->ctx);
+        int32_t f = 0;
+        if (h2->precision == NCA_PREC_MIXED) {
+            ensure_mixed(h2);
+            f = couple_otf(h2) ? 2 : 1;
+        }
+        *form = f;
+        if (rmin) *rmin = h2->coup_rmin;
+        if (rmax) *rmax = h2->coup_rmax;
     });
 }
 

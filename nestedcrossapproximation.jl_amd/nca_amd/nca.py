@@ -213,6 +213,15 @@ class PetrovGalerkinNCACore:
         call(hip_lib().nca_h2_near_form(self.handle, ctypes.byref(f), ctypes.byref(a), ctypes.byref(b)))
         return f.value, a.value, b.value
 
+    def couple_form(self):
+        """(form, rmin, rmax) of nca_h2_couple_form: 0 FP64 stored, 1 MIXED
+        stored (c32 copy), 2 MIXED evaluated on the fly on the f16 MFMA."""
+        f = ctypes.c_int32()
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        call(hip_lib().nca_h2_couple_form(self.handle, ctypes.byref(f), ctypes.byref(a), ctypes.byref(b)))
+        return f.value, a.value, b.value
+
     def set_precision(self, precision):
         call(hip_lib().nca_h2_set_precision(self.handle, int(precision)))
 

@@ -4,7 +4,7 @@ import bench
 
 ST = dict(coeffs_trial=10, coeffs_test=12, coeffs_trial_leaves=6, coeffs_test_leaves=8,
           basis_entries_trial=100, basis_entries_test=120, transfer_entries_trial=30, transfer_entries_test=40,
-          coupling_entries=500, coupling_columns=50, near_pairs=7)
+          coupling_entries=500, coupling_columns=50, coupling_blocks=9, near_pairs=7)
 
 
 def test_mixed_worked_example():
@@ -27,6 +27,16 @@ def test_mixed_f16_near_field():
     b, h = bench.stage_bytes(ST, 2, 20, 20, True), bench.stage_bytes(ST, 2, 20, 20, True, h16=True)
     assert h["gather"] - b["gather"] == 16 * 2 * 20 + 2 * 4 * 2 * 1
     assert all(h[k] == b[k] for k in b if k != "gather")
+
+
+def test_mixed_coupling_on_the_fly():
+    # the coupling evaluated on the fly reads no coupling entries or column
+    # maps: the source coefficients three times (read, split copy written and
+    # read), the pivot coordinates of both sides and the ranges; it writes
+    # the same c32 result
+    b, o = bench.stage_bytes(ST, 2, 20, 20, True), bench.stage_bytes(ST, 2, 20, 20, True, otf=True)
+    assert o["coupling"] == 3 * 8 * 2 * 10 + 24 * (10 + 12) + 8 * 9 + 8 * 2 * 12
+    assert all(o[k] == b[k] for k in b if k != "coupling")
 
 
 def test_fp64_worked_example():

@@ -1,12 +1,16 @@
-"""The MIXED near field on f16 MFMA with split operands (every FP32 kernel
-value and right-hand side as two f16 parts; the MIXED default), through the
-C-ABI: it is the form MIXED products take when the near pairs allow it
-(nca_h2_near_form), it stays within the MIXED bar of the oracle's compressed
-FP64 operator for N, T and C on 16-, 8- and 4-row tiles and one or two
-16-column chunks, no less accurate than the f32 MFMA form; it is exact under
-power-of-two scaling of each right-hand side (the per-column scales), gives
-exact zeros for zero columns, and near pairs spanning more than 2^17 in
-distance fall back to the f32 form."""
+"""The MIXED forms on f16 MFMA with split operands (every FP32 kernel value
+and right-hand side or source coefficient as two f16 parts; the MIXED
+defaults), through the C-ABI. The near field: the form MIXED products take
+when the near pairs allow it (nca_h2_near_form), within the MIXED bar of the
+oracle's compressed FP64 operator for N, T and C on 16-, 8- and 4-row tiles
+and one or two 16-column chunks, no less accurate than the f32 MFMA form;
+exact under power-of-two scaling of each right-hand side (the per-column
+scales), exact zeros for zero columns, and near pairs spanning more than 2^17
+in distance fall back to the f32 form. The coupling evaluated on the fly at
+the pivots (nca_h2_couple_form): within the bar for N, T and C at 16, 3 and
+40 right-hand sides, no less accurate than the stored c32 blocks. A point at
+the origin (where the sources past a range end read their coordinates)
+leaves every form finite."""
 import numpy as np
 import pytest
 
@@ -21,7 +25,7 @@ from helpers import relerr
 pytestmark = pytest.mark.gpu
 
 # the launch choices of the defaults, with the near-field form varied
-BASE = dict(leaf_order=2, near_xcd_order=0, near_stagger=0, couple_k3=1, couple_tile_order=3, couple_xcd_order=0)
+BASE = dict(leaf_order=2, near_xcd_order=0, near_stagger=0, couple_k3=2, couple_tile_order=3, couple_xcd_order=0)
 
 
 def say(msg):
@@ -121,3 +125,54 @@ def test_f16_distance_span(gap, form):
     x = geometry.complex_gaussian(rng, (pts.shape[0], 16))
     assert relerr(W @ x, H.matvec(x)) <= 1e-5
     assert relerr(NC.adjoint(W) @ x, H.matvec(x, op=2)) <= 1e-5
+
+
+def test_couple_on_the_fly_within_bar():
+    """8,192 sphere points, k = 2: MIXED evaluates the coupling on the fly
+    (form 2, far pivot distances measured); N, T and C at 16, 3 and 40
+    right-hand sides within 1e-5 of the FP64 oracle, not above twice the
+    stored c32 form's error, and different from it."""
+    pts = geometry.fibonacci_sphere(8192)
+    core, H = core_of(pts, 2.0)
+    W = NC.H2Workspace(core)
+    core.set_precision(nca_amd.NCA_PREC_MIXED)
+    form, rmin, rmax = core.couple_form()
+    say(f"coupling form {form}, far pivot distances {rmin:.3e} .. {rmax:.3e}")
+    assert form == 2 and 0 < rmin < rmax <= 2 ** 17 * rmin
+    rng = np.random.default_rng(44)
+    for R in (16, 3, 40):
+        x = geometry.complex_gaussian(rng, (pts.shape[0], R))
+        refs = [H.matvec(x, op=op) for op in (0, 1, 2)]
+        res = {}
+        for ck in (2, 1):
+            TU.apply(core, dict(BASE, near_k3=2, couple_k3=ck))
+            assert core.couple_form()[0] == ck
+            res[ck] = [W @ x, NC.transpose(W) @ x, NC.adjoint(W) @ x]
+        for op in range(3):
+            e2, e1 = relerr(res[2][op], refs[op]), relerr(res[1][op], refs[op])
+            say(f"R={R} op={op}: on the fly {e2:.2e} stored {e1:.2e}")
+            assert e2 <= 1e-5 and e1 <= 1e-5
+            assert e2 <= 2 * e1
+            assert not np.array_equal(res[2][op], res[1][op])
+
+
+def test_point_at_origin_every_form():
+    """The 32 x 32 grid of [0, 1]^2 (a point at the origin, the coordinates
+    read for sources past a range end) with itself, k = 2 pi / 10: every MIXED
+    form (near field f16, f32 four- or three-product; coupling on the fly or
+    stored) gives finite products within 1e-5 of the FP64 oracle, N and C."""
+    pts = geometry.grid_square(32)
+    assert np.any(np.all(pts == 0, axis=1))
+    core, H = core_of(pts, 2 * np.pi / 10)
+    W = NC.H2Workspace(core)
+    core.set_precision(nca_amd.NCA_PREC_MIXED)
+    rng = np.random.default_rng(45)
+    x = geometry.complex_gaussian(rng, (pts.shape[0], 16))
+    ref, refc = H.matvec(x), H.matvec(x, op=2)
+    for nk in (2, 1, 0):
+        for ck in (2, 1):
+            TU.apply(core, dict(BASE, near_k3=nk, couple_k3=ck))
+            y, yc = W @ x, NC.adjoint(W) @ x
+            say(f"near_k3={nk} couple_k3={ck}: {relerr(y, ref):.2e} {relerr(yc, refc):.2e}")
+            assert np.all(np.isfinite(y)) and np.all(np.isfinite(yc))
+            assert relerr(y, ref) <= 1e-5 and relerr(yc, refc) <= 1e-5

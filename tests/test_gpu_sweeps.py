@@ -147,12 +147,14 @@ def test_autotune_keeps_results(ctx, precision):
     core = _core(ctx, p.testpts, tol=p.tol, k=p.k, minvalues=64)
     core.set_precision(precision)
     t = core.autotune(reps=2)
-    # MIXED also times the f16 near field (near_k3 = 2): 3 x 3 x 4 near + 16 coupling candidates
+    # MIXED also times the f16 near field (near_k3 = 2) and the coupling
+    # evaluated on the fly (couple_k3 = 2): 3 x 3 x 4 near + 3 x 4 x 2 coupling candidates
     mixed = precision == nca_amd.NCA_PREC_MIXED
-    assert t["candidates"] == (52 if mixed else 40) and t["near_stagger"] in (0, 1)
+    assert t["candidates"] == (60 if mixed else 40) and t["near_stagger"] in (0, 1)
     assert t["near_k3"] in ((0, 1, 2) if mixed else (0, 1)) and t["leaf_order"] in (0, 1, 2)
     assert t["near_xcd_order"] in (0, 1)
-    assert t["couple_k3"] in (0, 1) and t["couple_tile_order"] in (0, 1, 2, 3) and t["couple_xcd_order"] in (0, 1)
+    assert t["couple_k3"] in ((0, 1, 2) if mixed else (0, 1)) and t["couple_tile_order"] in (0, 1, 2, 3)
+    assert t["couple_xcd_order"] in (0, 1)
     assert t["near_ms"] <= t["near_ms_start"] and t["couple_ms"] <= t["couple_ms_start"]
     rng = np.random.default_rng(4)
     x = geometry.complex_gaussian(rng, (p.testpts.shape[0], 16))

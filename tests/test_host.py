@@ -268,6 +268,7 @@ def test_tuning_table_file(tmp_path):
                   couple_xcd_order=2)
     for key, e in shipped.entries.items():
         assert key.startswith("gfx950|")
-        # near_k3 = 2 (the f16 near field) is a MIXED choice only
-        for n, hi in dict(ranges, near_k3=3 if "|mixed|" in key else 2).items():
+        # near_k3 / couple_k3 = 2 (the f16 forms) are MIXED choices only
+        k3 = 3 if "|mixed|" in key else 2
+        for n, hi in dict(ranges, near_k3=k3, couple_k3=k3).items():
             assert 0 <= e[n] < hi, (key, n)
