@@ -1160,16 +1160,13 @@ __device__ __forceinline__ flt4 mma_h16x3(u32x4 ah, u32x4 al, u32x4 bh, u32x4 bl
     c = mma_h16(ah, bl, c);
     return mma_h16(al, bh, c);
 }
-// The f16 product rounds its result with few guard bits below the exponent
-// of the larger of the accumulator and the products (measured: up to 13 ulp
-// of a large accumulator, against 1.9 for the f32 16x16x4 product) and,
-// from a zero accumulator, with a bias toward zero (0.6 ulp on average): a
-// block's products therefore go to a zero accumulator, and the block's sum
-// is added to the running sum by VALU adds (round to nearest).
-__device__ __forceinline__ void add_block(flt4& acc, flt4 blk) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) acc[v] += blk[v];
-}
+// The products accumulate in the running FP32 sums directly. The f16
+// product rounds its result with few guard bits below the exponent of the
+// larger of the accumulator and the products (measured: up to 13 ulp of a
+// large accumulator, against 1.9 for the f32 16x16x4 product); summing each
+// block from zero and adding it on VALU halves the MIXED error of the 1M
+// sphere (6.8e-7 -> 3.3e-7 against FP64) at 2.5 % of the sweep, which the
+// 1e-5 MIXED bar does not need.
 // f16 parts of two FP32 values: hi = (f16(a), f16(b)), lo = (f16(a - hi.x),
 // f16(b - hi.y)), in three instructions: v_cvt_pk_f16_f32 for hi, then
 // v_fma_mix (a x 1 - hi, exact, rounded once to f16) into each half of lo.
@@ -1275,10 +1272,9 @@ __device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (
             brl[v] = pack_lo(xb[c][2 * v].y, xb[c][2 * v + 1].y);
             bil[v] = pack_hi(xb[c][2 * v].y, xb[c][2 * v + 1].y);
         }
-        const flt4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-        add_block(P[c], mma_h16x3(arh, arl, brh, brl, z));
-        add_block(Q[c], mma_h16x3(aih, ail, bih, bil, z));
-        add_block(I[c], mma_h16x3(aih, ail, brh, brl, mma_h16x3(arh, arl, bih, bil, z)));
+        P[c] = mma_h16x3(arh, arl, brh, brl, P[c]);
+        Q[c] = mma_h16x3(aih, ail, bih, bil, Q[c]);
+        I[c] = mma_h16x3(aih, ail, brh, brl, mma_h16x3(arh, arl, bih, bil, I[c]));
     }
 }
 
@@ -1368,9 +1364,8 @@ __device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], co
                 brl[v] = pack_lo(xb[c][m][q].y, xb[c][m][q + 1].y);
                 bil[v] = pack_hi(xb[c][m][q].y, xb[c][m][q + 1].y);
             }
-        const flt4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-        add_block(D1[c], mma_h16x3(a1h, a1l, brh, brl, z));
-        add_block(D2[c], mma_h16x3(a2h, a2l, bih, bil, z));
+        D1[c] = mma_h16x3(a1h, a1l, brh, brl, D1[c]);
+        D2[c] = mma_h16x3(a2h, a2l, bih, bil, D2[c]);
     }
 }
 
@@ -1521,7 +1516,7 @@ __device__ __forceinline__ void emit_tile(Cemb<PrecF32, HE> (&acc)[NC], EF emitt
 // all R columns; xsc: per column s_q and 1 / s_q). The near sums are scaled
 // back (t / (4 pi s_q)), then the basis rows are added on the f32 MFMA.
 template <bool FWD, bool CJ, bool NEG, int NC>
-__global__ void __launch_bounds__(256, 4)
+__global__ void __launch_bounds__(256, NC == 1 ? 5 : 4)
     k_leaf_out_h16(const int32_t* __restrict__ tile_leaf, const int32_t* __restrict__ tile_row0, int64_t ntiles,
                    const int32_t* __restrict__ leaves, const int32_t* __restrict__ leafbasis,
                    const int64_t* __restrict__ boff, const float2* __restrict__ bblocks,
