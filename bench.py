@@ -302,11 +302,9 @@ def main():
     if os.path.exists(tpath) and args.n == 1048576 and R == 16:
         traffic_db = json.load(open(tpath))["kernels"]
 
-    def traffic_of(prefix):
-        for name, t in traffic_db.items():
-            if name.startswith(prefix):
-                return int(t["read_bytes"] + t["write_bytes"])
-        return None
+    def traffic_of(name):
+        t = traffic_db.get(name)
+        return int(t["read_bytes"] + t["write_bytes"]) if t else None
 
     if dom == "leaf_expansion_near":
         # complex products at FP32 accuracy: 8 flop per near pair and right-hand
@@ -315,9 +313,10 @@ def main():
         achieved = flops / (prof[5] * 1e-3) / 1e12
         if near_f16(core):
             kname, kpre = ("k_leaf_out_h16 (leaf expansion + near field; FP32 kernel values and right-hand sides "
-                           "split into f16 parts, three f16 MFMA products per real product)"), "k_leaf_out_h16"
+                           "split into f16 parts, three f16 MFMA products per real product)"), "k_leaf_out_h16<true, false, false, 1>"
         else:
-            kname, kpre = "k_leaf_out_mfma<PrecF32> (leaf expansion + near field)", "k_leaf_out_mfma<PrecF32"
+            kname, kpre = ("k_leaf_out_mfma<PrecF32> (leaf expansion + near field)",
+                           "k_leaf_out_mfma<PrecF32, true, false, true, false, 1>")
         roofline = {"kernel": kname, "bound": "mfma",
                     "achieved": round(achieved, 2), "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4),

@@ -527,8 +527,8 @@ int32_t nca_h2_near_form(nca_h2* h2, int32_t* form, double* rmin, double* rmax);
  * the stored blocks' c32 copy on the f32 MFMA; 2 MIXED, the blocks evaluated
  * on the fly at the pivots (G(test pivots of t, trial pivots of s), the
  * store's entries) on the f16 MFMA with split operands, no c32 copy kept
- * (the MIXED default for operators of one coefficient block per node whose
- * far pivot distances span at most 2^17). rmin, rmax: those distances,
+ * (the MIXED default when the far pivot distances span at most 2^17; plain
+ * and directional operators). rmin, rmax: those distances,
  * measured once the MIXED set-up ran (else 0). */
 int32_t nca_h2_couple_form(nca_h2* h2, int32_t* form, double* rmin, double* rmax);
 
