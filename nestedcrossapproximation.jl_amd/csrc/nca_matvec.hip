@@ -1144,6 +1144,11 @@ __global__ void __launch_bounds__(256, 4)
 // embedding of the f32 kernel (MFMA rows [0, H) the real parts, [H, 2 H) the
 // imaginary parts). The last block of a range is padded with zero slots;
 // sources past a range end read zero right-hand sides (buffer range check).
+// Each block's 32 source points go to a 1 KiB LDS buffer of the wave with
+// one direct-to-LDS buffer load (16 bytes a lane), and every lane reads the
+// coordinates of its sources from there (two LDS reads a sub-step, broadcast
+// within the lanes that share a source): one texture instruction a block
+// instead of one a sub-step, and no cross-lane moves on the VALU.
 
 typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
@@ -1212,28 +1217,47 @@ __device__ __forceinline__ float2 helmholtz_h16(const NearK& nk, double x0, doub
     return make_float2(c * sc, NEG ? -(s * sc) : s * sc);
 }
 
-// One block of a 16-row tile's walk: nsub (1..8) sub-steps of 4 sources at
-// byte offsets sp (points) and sx (right-hand sides) of the range; P, Q, I
-// accumulate Gr xr, Gi xi and Gr xi + Gi xr.
+typedef __attribute__((address_space(3))) double lds_double;
+// Copies the 1,024 bytes of a range's points from byte offset sp on (32
+// points and the start of the next block; bytes past the range end read 0)
+// into the wave's LDS buffer: lane l's 16 bytes to bytes 16 l (direct-to-LDS
+// buffer load). The wave's earlier reads of the buffer complete first.
+__device__ __forceinline__ void h16_stage_points(lds_double* lds, __amdgpu_buffer_rsrc_t rp, uint32_t sp, int lane) {
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rp, (__attribute__((address_space(3))) void*)lds, 16,
+                                             (uint32_t)lane * 16u + sp, 0, 0, 0);
+    asm volatile("" ::: "memory");
+}
+// Waits until at most N vector memory loads are in flight (loads complete in
+// issue order): with the N loads issued after a staging copy, the copy is in
+// LDS. The compiler does not order LDS reads after direct-to-LDS loads.
+template <int N>
+__device__ __forceinline__ void h16_wait_vm() {
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// One block of a 16-row tile's walk: nsub (1..8) sub-steps of 4 sources,
+// their points staged in LDS (lp: this lane's first source there, the
+// staging load the last vector load issued), their right-hand sides at byte
+// offset sx of the range; P, Q, I accumulate Gr xr, Gi xi and Gr xi + Gi xr.
 template <int NC, bool NEG, bool GU, bool PH, bool FULL>
 __device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (&I)[NC], const NearK& nk, double x0,
-                                            double x1, double x2, __amdgpu_buffer_rsrc_t rp,
-                                            __amdgpu_buffer_rsrc_t rx, uint32_t sp, uint32_t sx, uint32_t vp,
+                                            double x1, double x2, const lds_double* lp,
+                                            __amdgpu_buffer_rsrc_t rx, uint32_t sx,
                                             const uint32_t (&vx)[NC][1], uint32_t rowb, int nsub, int nval) {
-    double pq[8];
     uint2 xb[NC][8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         if (FULL || m < nsub) {
-            pq[m] = buf_load<double>(rp, vp, sp + (uint32_t)m * 96u);
 #pragma unroll
             for (int c = 0; c < NC; ++c) xb[c][m] = buf_load<uint2>(rx, vx[c][0], sx + (uint32_t)m * 4u * rowb);
         } else {
-            pq[m] = 0.0;
 #pragma unroll
             for (int c = 0; c < NC; ++c) xb[c][m] = make_uint2(0u, 0u);
         }
     }
+    if constexpr (FULL) h16_wait_vm<8 * NC>();
+    else h16_wait_vm<0>();
     const int kk = (threadIdx.x & 63) >> 4;
     u32x4 arh, aih, arl, ail;
 #pragma unroll
@@ -1243,8 +1267,7 @@ __device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (
         for (int e = 0; e < 2; ++e) {
             const int m = 2 * v + e;
             if (FULL || m < nsub) {
-                g[e] = helmholtz_h16<NEG, GU, PH>(nk, x0, x1, x2, row_double<0>(pq[m]), row_double<1>(pq[m]),
-                                                  row_double<2>(pq[m]));
+                g[e] = helmholtz_h16<NEG, GU, PH>(nk, x0, x1, x2, lp[12 * m], lp[12 * m + 1], lp[12 * m + 2]);
                 // the range's last sub-step: sources past its end read the point
                 // (0, 0, 0), at which the kernel may be infinite, with zero
                 // right-hand sides; their values are zeroed
@@ -1278,7 +1301,8 @@ __device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (
 }
 
 // One block of an 8-row (H = 8) or 4-row (H = 4) tile's walk in the real
-// embedding: nsub (1 .. 32 / S) sub-steps of S = 64 / H sources. Lane (i, kk)
+// embedding: nsub (1 .. 32 / S) sub-steps of S = 64 / H sources (points in
+// LDS as for h16_block16). Lane (i, kk)
 // evaluates row i % H x source S m + 4 (i / H) + kk and receives the other
 // sources of its row's sub-step from lanes i +- H (rotations); sel1 / sel2
 // pick, by v_perm, the parts of two slots for A1 (real rows: Gr, imaginary
@@ -1287,30 +1311,29 @@ __device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (
 // real rows are D1 - D2, the imaginary rows D1 + D2.
 template <int H, int NC, bool NEG, bool GU, bool PH, bool FULL>
 __device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], const NearK& nk, double x0, double x1,
-                                              double x2, __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rx,
-                                              uint32_t sp, uint32_t sx, uint32_t vp,
+                                              double x2, const lds_double* lp, __amdgpu_buffer_rsrc_t rx,
+                                              uint32_t sx,
                                               const uint32_t (&vx)[NC][16 / H], uint32_t rowb, int nsub, int nval,
                                               bool imrow, uint32_t sel1, uint32_t sel2) {
     constexpr int G = 16 / H, S = 64 / H, NS = 32 / S;
-    double pq[NS];
     uint2 xb[NC][NS][G];
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
         if (FULL || m < nsub) {
-            pq[m] = buf_load<double>(rp, vp, sp + (uint32_t)(m * S * 24));
 #pragma unroll
             for (int c = 0; c < NC; ++c)
 #pragma unroll
                 for (int q = 0; q < G; ++q)
                     xb[c][m][q] = buf_load<uint2>(rx, vx[c][q], sx + (uint32_t)(m * S) * rowb);
         } else {
-            pq[m] = 0.0;
 #pragma unroll
             for (int c = 0; c < NC; ++c)
 #pragma unroll
                 for (int q = 0; q < G; ++q) xb[c][m][q] = make_uint2(0u, 0u);
         }
     }
+    if constexpr (FULL) h16_wait_vm<8 * NC>();
+    else h16_wait_vm<0>();
     // this lane's source in a sub-step, after the sub-step's first
     const int own = 4 * ((int)(threadIdx.x & 15) / H) + (int)((threadIdx.x & 63) >> 4);
     u32x4 a1h, a2h, a1l, a2l;
@@ -1318,8 +1341,7 @@ __device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], co
     for (int m = 0; m < NS; ++m) {
         float2 g = make_float2(0.0f, 0.0f);
         if (FULL || m < nsub) {
-            g = helmholtz_h16<NEG, GU, PH>(nk, x0, x1, x2, quad_double<0>(pq[m]), quad_double<1>(pq[m]),
-                                           quad_double<2>(pq[m]));
+            g = helmholtz_h16<NEG, GU, PH>(nk, x0, x1, x2, lp[3 * S * m], lp[3 * S * m + 1], lp[3 * S * m + 2]);
             // sources past the range end (the point (0, 0, 0)): zero values
             if (!FULL && S * m + S > nval && S * m + own >= nval) g = make_float2(0.0f, 0.0f);
         }
@@ -1371,8 +1393,9 @@ __device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], co
 // The f16 walk of one tile: H row slots (16, 8 or 4), this lane's row at
 // xp (3 doubles), merged source ranges e0 .. e0 + nr - 1 (from e0 + rot on,
 // cyclically; bit 31 of the end: the range keeps the zero-distance guard),
-// source points spts (3 doubles each), split right-hand sides xsq (R per
-// source, 8 bytes each, from the first column on), columns jl. Accumulates
+// source points spts (3 doubles each, staged per block in the wave's 1 KiB
+// LDS buffer lds), split right-hand sides xsq (R per source, 8 bytes each,
+// from the first column on), columns jl. Accumulates
 // Gr xr, Gi xi and Gr xi + Gi xr into A1, A2, A3 (H = 16), or A1 xr and
 // A2 xi into A1, A2 (the real embedding, H = 8 or 4).
 template <int H, int NC, bool NEG, bool PH>
@@ -1380,12 +1403,13 @@ __device__ __forceinline__ void h16_walk(flt4 (&A1)[NC], flt4 (&A2)[NC], flt4 (&
                                          const double* xp, int lane, int64_t e0, int nr, int rot,
                                          const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
                                          const double* __restrict__ spts, const uint2* __restrict__ xsq, int64_t R,
-                                         const int (&jl)[NC], bool imrow, uint32_t sel1, uint32_t sel2) {
+                                         const int (&jl)[NC], bool imrow, uint32_t sel1, uint32_t sel2,
+                                         lds_double* lds) {
     constexpr int G = 16 / H, S = 64 / H, NB = 32 / S;
     const int kk = lane >> 4;
     const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
     const int ysrc = 4 * ((lane & 15) / H) + kk;
-    const uint32_t vp = near_point_offset<G>(lane, kk, ysrc);
+    const lds_double* lp = lds + 3 * ysrc;
     const uint32_t rowb = (uint32_t)R * 8u;
     uint32_t vx[NC][G];
 #pragma unroll
@@ -1408,21 +1432,22 @@ __device__ __forceinline__ void h16_walk(flt4 (&A1)[NC], flt4 (&A2)[NC], flt4 (&
             constexpr bool GU = decltype(guard)::value;
             uint32_t sp = 0, sx = 0;
             for (int b = 0; b < nfull; ++b, sp += 768u, sx += 32u * rowb) {
+                h16_stage_points(lds, rp, sp, lane);
                 if constexpr (H == 16)
-                    h16_block16<NC, NEG, GU, PH, true>(A1, A2, A3, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx, rowb, NB,
-                                                       32);
+                    h16_block16<NC, NEG, GU, PH, true>(A1, A2, A3, nk, x0, x1, x2, lp, rx, sx, vx, rowb, NB, 32);
                 else
-                    h16_block_emb<H, NC, NEG, GU, PH, true>(A1, A2, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx, rowb, NB,
-                                                            32, imrow, sel1, sel2);
+                    h16_block_emb<H, NC, NEG, GU, PH, true>(A1, A2, nk, x0, x1, x2, lp, rx, sx, vx, rowb, NB, 32,
+                                                            imrow, sel1, sel2);
             }
             if (rem > 0) {
                 const int nsub = (rem + S - 1) / S;
+                h16_stage_points(lds, rp, sp, lane);
                 if constexpr (H == 16)
-                    h16_block16<NC, NEG, GU, PH, false>(A1, A2, A3, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx, rowb,
-                                                        nsub, rem);
+                    h16_block16<NC, NEG, GU, PH, false>(A1, A2, A3, nk, x0, x1, x2, lp, rx, sx, vx, rowb, nsub,
+                                                        rem);
                 else
-                    h16_block_emb<H, NC, NEG, GU, PH, false>(A1, A2, nk, x0, x1, x2, rp, rx, sp, sx, vp, vx, rowb,
-                                                             nsub, rem, imrow, sel1, sel2);
+                    h16_block_emb<H, NC, NEG, GU, PH, false>(A1, A2, nk, x0, x1, x2, lp, rx, sx, vx, rowb, nsub,
+                                                             rem, imrow, sel1, sel2);
             }
         };
         if (hr < 0) blocks(std::true_type());
@@ -1434,12 +1459,13 @@ __device__ __forceinline__ void h16_walk(flt4 (&A1)[NC], flt4 (&A2)[NC], flt4 (&
 // rows, else the real embedding with 8 or 4 row slots; rowptr(rsel) gives
 // the coordinates of row slot rsel (rows past the tile repeat its last). The
 // sums, scaled by osc (per column chunk), go to fin(acc, rsel) as a
-// Cacc<PrecF32, false> or Cemb<PrecF32, H> accumulator per chunk.
+// Cacc<PrecF32, false> or Cemb<PrecF32, H> accumulator per chunk; lds: the
+// wave's 1 KiB LDS buffer for the source points.
 template <int NC, bool NEG, class RowF, class FinF>
 __device__ __forceinline__ void h16_tile(const NearK& nk, int nrows, int lane, RowF rowptr, int64_t e0, int nr,
                                          int rot, const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
                                          const double* __restrict__ spts, const uint2* __restrict__ xsq, int64_t R,
-                                         const int (&jl)[NC], const float (&osc)[NC], FinF fin) {
+                                         const int (&jl)[NC], const float (&osc)[NC], FinF fin, lds_double* lds) {
     using P = PrecF32;
     const int i = lane & 15, kk = lane >> 4;
     auto run = [&](auto ph) {
@@ -1450,7 +1476,7 @@ __device__ __forceinline__ void h16_tile(const NearK& nk, int nrows, int lane, R
             for (int c = 0; c < NC; ++c)
                 for (int v = 0; v < 4; ++v) Pa[c][v] = Qa[c][v] = Ia[c][v] = 0.0f;
             h16_walk<16, NC, NEG, PH>(Pa, Qa, Ia, nk, rowptr(i), lane, e0, nr, rot, nrlo, nrhi, spts, xsq, R, jl,
-                                      false, 0u, 0u);
+                                      false, 0u, 0u, lds);
             Cacc<P, false> acc[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
@@ -1474,7 +1500,7 @@ __device__ __forceinline__ void h16_tile(const NearK& nk, int nrows, int lane, R
                 for (int c = 0; c < NC; ++c)
                     for (int v = 0; v < 4; ++v) D1[c][v] = D2[c][v] = 0.0f;
                 h16_walk<H, NC, NEG, PH>(D1, D2, D1, nk, rowptr(i % H), lane, e0, nr, rot, nrlo, nrhi, spts, xsq, R,
-                                         jl, imrow, sel1, sel2);
+                                         jl, imrow, sel1, sel2, lds);
                 // this lane holds MFMA rows 4 kk + v: real parts when kk < H / 4
                 const float sg = kk < H / 4 ? -1.0f : 1.0f;
                 Cemb<P, H> acc[NC];
@@ -1579,7 +1605,10 @@ __global__ void __launch_bounds__(256, NC == 1 ? 5 : 4)
         emit_tile(acc, emitter);
     };
     auto rowptr = [&](int rsel) { return dpts + 3 * (int64_t)(l0 + row0 + min(rsel, nrows - 1)); };
-    h16_tile<NC, NEG>(nk, nrows, lane, rowptr, e0, nr, rot, nrlo, nrhi, spts, xs16 + q0, R, jl, osc, fin);
+    __shared__ double h16pts[4][128];
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    h16_tile<NC, NEG>(nk, nrows, lane, rowptr, e0, nr, rot, nrlo, nrhi, spts, xs16 + q0, R, jl, osc, fin,
+                      (lds_double*)h16pts[wv]);
 }
 
 // Stage 3 of MIXED products on the f16 path, the coupling blocks evaluated
@@ -1629,7 +1658,10 @@ __global__ void __launch_bounds__(256, 4)
     };
     auto fn = [&](auto& acc, int) { emit_tile(acc, emitter); };
     auto rowptr = [&](int rsel) { return dpiv + 3 * (int64_t)(r0 + row0 + min(rsel, nrows - 1)); };
-    h16_tile<NC, NEG>(nk, nrows, lane, rowptr, e0, nr, 0, crlo, crhi, spiv, xs16 + q0, R, jl, osc, fn);
+    __shared__ double h16pts[4][128];
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    h16_tile<NC, NEG>(nk, nrows, lane, rowptr, e0, nr, 0, crlo, crhi, spiv, xs16 + q0, R, jl, osc, fn,
+                      (lds_double*)h16pts[wv]);
 }
 
 // Smallest squared distance above the zero-distance guard (2e-30) and the
