@@ -282,6 +282,12 @@ def main():
                            "alg_bytes": int(b1), "hbm_gbs": round(b1 / (ms_1 * 1e-3) / 1e9, 1),
                            "hbm_frac": round(b1 / (ms_1 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         }
+        if mixed:
+            nr, cr, fn, fc = core.rel32(fractions=True)
+            per_prec[label]["rel32_tiles"] = {"near": round(fn, 4) if nr else 0.0,
+                                              "coupling": round(fc, 4) if cr else 0.0}
+            log(f"MIXED distances in FP32 from tile-relative coordinates on {fn:.4f} of the near-field "
+                f"tiles and {fc:.4f} of the coupling tiles")
         log(label, "stages:", per_prec[label]["stages"])
         log(f"{label}: T {ms_t:.3f} ms, C {ms_c:.3f} ms per {R}-RHS sweep; 1 RHS {ms_1:.3f} ms")
     Y64, Ymix = results["fp64"][3], results["mixed"][3]

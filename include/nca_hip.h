@@ -532,6 +532,22 @@ int32_t nca_h2_near_form(nca_h2* h2, int32_t* form, double* rmin, double* rmax);
  * measured once the MIXED set-up ran (else 0). */
 int32_t nca_h2_couple_form(nca_h2* h2, int32_t* form, double* rmin, double* rmax);
 
+/* No counterpart in the reference: the distance form of the f16 near field
+ * and coupling (forms 2 above) when every phase stays below one revolution:
+ * FP32 differences of coordinates taken relative to each tile's first row,
+ * instead of FP64 differences, for the tiles of the leaves (near field) and
+ * blocks (coupling) whose every tile meets the condition measured at the
+ * MIXED set-up: no pair strictly between 0 and the zero-distance guard, and
+ * the tile's rows within 32 times its smallest pair distance of its first
+ * row (a coordinate difference then errs by at most 33 * 2^-23 of the pair's
+ * distance). mode = 1 uses it there (the default), 0 keeps FP64
+ * differences everywhere, -1 only queries. near_on, couple_on (may be NULL):
+ * whether it is in use for any tile; near_frac, couple_frac (may be NULL):
+ * the fraction of tiles (both directions) that meet the condition (MIXED
+ * only, else 0). Changing the mode invalidates captured graphs. */
+int32_t nca_h2_rel32(nca_h2* h2, int32_t mode, int32_t* near_on, int32_t* couple_on, double* near_frac,
+                     double* couple_frac);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,6 +81,10 @@ struct SideDev {
     // 16-row tiles of the leaves (leaf index, first row) for the MFMA expansion kernel
     int64_t ntiles = 0;
     DevBuf<int32_t> tile_leaf, tile_row0;
+    // MIXED f16 forms: per node, 1 when every near-field tile of the leaf
+    // (near_rel) or every coupling tile of the block (coup_rel) with this
+    // side as destination meets the rel32 condition
+    DevBuf<uint8_t> near_rel, coup_rel;
     // host copies used during creation
     std::vector<int32_t> h_lo, h_hi, h_cptr, h_perm, h_bnodes, h_leaves, h_leafslot, h_leaves_node_order;
     std::vector<int32_t> h_ncptr, h_enode;  // per node first coefficient (nn+1); node of every block
@@ -113,6 +117,13 @@ struct nca_h2 {
     // pivot scale, and the measured smallest and largest far pivot distances
     int coup_h16 = 0;
     double coup_t = 1.0, coup_rmin = 0.0, coup_rmax = 0.0;
+    // f16 near field / coupling with FP32 distances from tile-relative
+    // coordinates (NearK::rel32): allowed by the measured tile statistics,
+    // and used unless rel32_off (nca_h2_rel32)
+    int rel32_off = 0;
+    // the fraction of near-field / coupling tiles of both directions that
+    // meet the rel32 condition
+    double near_rel_frac = 0.0, coup_rel_frac = 0.0;
     // scaled pivot coordinates of both sides in coefficient order (3 per coefficient)
     nca::DevBuf<double> piv_test, piv_trial;
     // the far blocks' coefficients as merged ranges (bit 31 of the end: the
@@ -525,12 +536,21 @@ struct NearK {
     // 2e-30 times t^2, and t / (4 pi)
     double gadd, gmin;
     float tsc;
+    // f16 path with phase32: rel32 = distances in FP32 from coordinates taken
+    // relative to each tile's first row, for the tiles whose node is flagged
+    // (rows within kRel32Span times the tile's smallest pair distance of its
+    // first row, no pair strictly between 0 and the guard); the guard's FP32
+    // form then is r^2 + gaddf > gminf, both far below every scaled nonzero
+    // distance
+    int rel32;
+    float gaddf, gminf;
 };
 constexpr double kPhase32Turns = 1.0;
 // turns: the largest phase k r / (2 pi) over the operator's near pairs
 inline NearK near_consts(double k, double turns) {
-    return NearK{k,   k * (0.5 / M_PI), k * (0.25 / M_PI), (float)(k * (0.5 / M_PI)), turns <= kPhase32Turns ? 1 : 0,
-                 1e-30, 2e-30,           (float)(0.25 / M_PI)};
+    return NearK{k,     k * (0.5 / M_PI), k * (0.25 / M_PI), (float)(k * (0.5 / M_PI)), turns <= kPhase32Turns ? 1 : 0,
+                 1e-30, 2e-30,            (float)(0.25 / M_PI),       0,
+                 0x1p-126f, 0x1p-100f};
 }
 // The same for points scaled by t (a power of two): k / t, guards times t^2.
 inline NearK near_consts_scaled(double k, double turns, double t) {
@@ -1217,7 +1237,41 @@ __device__ __forceinline__ float2 helmholtz_h16(const NearK& nk, double x0, doub
     return make_float2(c * sc, NEG ? -(s * sc) : s * sc);
 }
 
+// The same from FP32 coordinates relative to the tile's first row (rel32):
+// the difference of two such coordinates errs by a few units of the row's
+// distance from the first row plus the pair's distance, which the rel32
+// condition keeps within a small multiple of the pair's distance.
+template <bool NEG, bool GUARD>
+__device__ __forceinline__ float2 helmholtz_h16r(const NearK& nk, float x0, float x1, float x2, float y0, float y1,
+                                                 float y2) {
+    const float dx = x0 - y0, dy = x1 - y1, dz = x2 - y2;
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, GUARD ? fmaf(dx, dx, nk.gaddf) : dx * dx));
+    const float rf = __builtin_amdgcn_rsqf(r2);
+    const float f = (r2 * rf) * nk.k2pif;
+    const float c = __builtin_amdgcn_cosf(f), s = __builtin_amdgcn_sinf(f);
+    const float sc = !GUARD || r2 > nk.gminf ? rf : 0.0f;
+    return make_float2(c * sc, NEG ? -(s * sc) : s * sc);
+}
+
 typedef __attribute__((address_space(3))) double lds_double;
+typedef __attribute__((address_space(3))) float lds_float;
+// LDS element and row coordinate types of the f16 walk: FP64 points, or FP32
+// coordinates relative to the tile's first row (RL)
+template <bool RL>
+struct H16Pt {
+    typedef lds_double L;
+    typedef double X;
+};
+template <>
+struct H16Pt<true> {
+    typedef lds_float L;
+    typedef float X;
+};
+template <bool NEG, bool GU, bool PH, bool RL, class X, class L>
+__device__ __forceinline__ float2 h16_eval(const NearK& nk, X x0, X x1, X x2, const L* p) {
+    if constexpr (RL) return helmholtz_h16r<NEG, GU>(nk, x0, x1, x2, p[0], p[1], p[2]);
+    else return helmholtz_h16<NEG, GU, PH>(nk, x0, x1, x2, p[0], p[1], p[2]);
+}
 // Copies the 1,024 bytes of a range's points from byte offset sp on (32
 // points and the start of the next block; bytes past the range end read 0)
 // into the wave's LDS buffer: lane l's 16 bytes to bytes 16 l (direct-to-LDS
@@ -1240,11 +1294,13 @@ __device__ __forceinline__ void h16_wait_vm() {
 // their points staged in LDS (lp: this lane's first source there, the
 // staging load the last vector load issued), their right-hand sides at byte
 // offset sx of the range; P, Q, I accumulate Gr xr, Gi xi and Gr xi + Gi xr.
-template <int NC, bool NEG, bool GU, bool PH, bool FULL>
-__device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (&I)[NC], const NearK& nk, double x0,
-                                            double x1, double x2, const lds_double* lp,
+template <int NC, bool NEG, bool GU, bool PH, bool FULL, bool RL, class LandF>
+__device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (&I)[NC], const NearK& nk,
+                                            typename H16Pt<RL>::X x0, typename H16Pt<RL>::X x1,
+                                            typename H16Pt<RL>::X x2, const typename H16Pt<RL>::L* lp,
                                             __amdgpu_buffer_rsrc_t rx, uint32_t sx,
-                                            const uint32_t (&vx)[NC][1], uint32_t rowb, int nsub, int nval) {
+                                            const uint32_t (&vx)[NC][1], uint32_t rowb, int nsub, int nval,
+                                            LandF land) {
     uint2 xb[NC][8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
@@ -1256,8 +1312,7 @@ __device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (
             for (int c = 0; c < NC; ++c) xb[c][m] = make_uint2(0u, 0u);
         }
     }
-    if constexpr (FULL) h16_wait_vm<8 * NC>();
-    else h16_wait_vm<0>();
+    land(std::bool_constant<FULL>());
     const int kk = (threadIdx.x & 63) >> 4;
     u32x4 arh, aih, arl, ail;
 #pragma unroll
@@ -1267,7 +1322,7 @@ __device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (
         for (int e = 0; e < 2; ++e) {
             const int m = 2 * v + e;
             if (FULL || m < nsub) {
-                g[e] = helmholtz_h16<NEG, GU, PH>(nk, x0, x1, x2, lp[12 * m], lp[12 * m + 1], lp[12 * m + 2]);
+                g[e] = h16_eval<NEG, GU, PH, RL>(nk, x0, x1, x2, lp + 12 * m);
                 // the range's last sub-step: sources past its end read the point
                 // (0, 0, 0), at which the kernel may be infinite, with zero
                 // right-hand sides; their values are zeroed
@@ -1309,12 +1364,13 @@ __device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (
 // rows: Gi) and A2 (real rows: Gi, imaginary rows: Gr) in slot order (imrow:
 // the lane's MFMA row holds an imaginary part). D1 += A1 xr, D2 += A2 xi; the
 // real rows are D1 - D2, the imaginary rows D1 + D2.
-template <int H, int NC, bool NEG, bool GU, bool PH, bool FULL>
-__device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], const NearK& nk, double x0, double x1,
-                                              double x2, const lds_double* lp, __amdgpu_buffer_rsrc_t rx,
-                                              uint32_t sx,
+template <int H, int NC, bool NEG, bool GU, bool PH, bool FULL, bool RL, class LandF>
+__device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], const NearK& nk,
+                                              typename H16Pt<RL>::X x0, typename H16Pt<RL>::X x1,
+                                              typename H16Pt<RL>::X x2, const typename H16Pt<RL>::L* lp,
+                                              __amdgpu_buffer_rsrc_t rx, uint32_t sx,
                                               const uint32_t (&vx)[NC][16 / H], uint32_t rowb, int nsub, int nval,
-                                              bool imrow, uint32_t sel1, uint32_t sel2) {
+                                              bool imrow, uint32_t sel1, uint32_t sel2, LandF land) {
     constexpr int G = 16 / H, S = 64 / H, NS = 32 / S;
     uint2 xb[NC][NS][G];
 #pragma unroll
@@ -1332,8 +1388,7 @@ __device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], co
                 for (int q = 0; q < G; ++q) xb[c][m][q] = make_uint2(0u, 0u);
         }
     }
-    if constexpr (FULL) h16_wait_vm<8 * NC>();
-    else h16_wait_vm<0>();
+    land(std::bool_constant<FULL>());
     // this lane's source in a sub-step, after the sub-step's first
     const int own = 4 * ((int)(threadIdx.x & 15) / H) + (int)((threadIdx.x & 63) >> 4);
     u32x4 a1h, a2h, a1l, a2l;
@@ -1341,7 +1396,7 @@ __device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], co
     for (int m = 0; m < NS; ++m) {
         float2 g = make_float2(0.0f, 0.0f);
         if (FULL || m < nsub) {
-            g = helmholtz_h16<NEG, GU, PH>(nk, x0, x1, x2, lp[3 * S * m], lp[3 * S * m + 1], lp[3 * S * m + 2]);
+            g = h16_eval<NEG, GU, PH, RL>(nk, x0, x1, x2, lp + 3 * S * m);
             // sources past the range end (the point (0, 0, 0)): zero values
             if (!FULL && S * m + S > nval && S * m + own >= nval) g = make_float2(0.0f, 0.0f);
         }
@@ -1397,19 +1452,36 @@ __device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], co
 // LDS buffer lds), split right-hand sides xsq (R per source, 8 bytes each,
 // from the first column on), columns jl. Accumulates
 // Gr xr, Gi xi and Gr xi + Gi xr into A1, A2, A3 (H = 16), or A1 xr and
-// A2 xi into A1, A2 (the real embedding, H = 8 or 4).
-template <int H, int NC, bool NEG, bool PH>
+// A2 xi into A1, A2 (the real embedding, H = 8 or 4). RL: the points are
+// staged as FP32 coordinates relative to cp (the tile's first row; lane l
+// converts doubles 2 l and 2 l + 1 of a block), else copied as they are by
+// one direct-to-LDS load a block.
+template <int H, int NC, bool NEG, bool PH, bool RL>
 __device__ __forceinline__ void h16_walk(flt4 (&A1)[NC], flt4 (&A2)[NC], flt4 (&A3)[NC], const NearK& nk,
-                                         const double* xp, int lane, int64_t e0, int nr, int rot,
+                                         const double* xp, const double* cp, int lane, int64_t e0, int nr, int rot,
                                          const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
                                          const double* __restrict__ spts, const uint2* __restrict__ xsq, int64_t R,
                                          const int (&jl)[NC], bool imrow, uint32_t sel1, uint32_t sel2,
                                          lds_double* lds) {
     constexpr int G = 16 / H, S = 64 / H, NB = 32 / S;
+    using X = typename H16Pt<RL>::X;
     const int kk = lane >> 4;
-    const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
+    X x0, x1, x2;
+    double ca = 0.0, cb = 0.0;
+    if constexpr (RL) {
+        const double c[3] = {cp[0], cp[1], cp[2]};
+        x0 = (float)(xp[0] - c[0]);
+        x1 = (float)(xp[1] - c[1]);
+        x2 = (float)(xp[2] - c[2]);
+        ca = c[(2 * lane) % 3];
+        cb = c[(2 * lane + 1) % 3];
+    } else {
+        x0 = xp[0];
+        x1 = xp[1];
+        x2 = xp[2];
+    }
     const int ysrc = 4 * ((lane & 15) / H) + kk;
-    const lds_double* lp = lds + 3 * ysrc;
+    const typename H16Pt<RL>::L* lp = (const typename H16Pt<RL>::L*)lds + 3 * ysrc;
     const uint32_t rowb = (uint32_t)R * 8u;
     uint32_t vx[NC][G];
 #pragma unroll
@@ -1431,24 +1503,36 @@ __device__ __forceinline__ void h16_walk(flt4 (&A1)[NC], flt4 (&A2)[NC], flt4 (&
         auto blocks = [&](auto guard) {
             constexpr bool GU = decltype(guard)::value;
             uint32_t sp = 0, sx = 0;
-            for (int b = 0; b < nfull; ++b, sp += 768u, sx += 32u * rowb) {
-                h16_stage_points(lds, rp, sp, lane);
+            // one block: stage its points, then hand the block function the
+            // step that makes them readable once its right-hand side loads
+            // are issued
+            auto block = [&](auto full, int nsub, int nval) {
+                constexpr bool FU = decltype(full)::value;
+                double2 pv;
+                if constexpr (RL) pv = buf_load<double2>(rp, (uint32_t)lane * 16u, sp);
+                else h16_stage_points(lds, rp, sp, lane);
+                auto land = [&](auto) {
+                    if constexpr (RL) {
+                        // earlier LDS reads of this wave precede the write in
+                        // LDS order; the compiler keeps the reads below after it
+                        asm volatile("" ::: "memory");
+                        ((lds_float*)lds)[2 * lane] = (float)(pv.x - ca);
+                        ((lds_float*)lds)[2 * lane + 1] = (float)(pv.y - cb);
+                        asm volatile("" ::: "memory");
+                    } else {
+                        if constexpr (FU) h16_wait_vm<8 * NC>();
+                        else h16_wait_vm<0>();
+                    }
+                };
                 if constexpr (H == 16)
-                    h16_block16<NC, NEG, GU, PH, true>(A1, A2, A3, nk, x0, x1, x2, lp, rx, sx, vx, rowb, NB, 32);
+                    h16_block16<NC, NEG, GU, PH, FU, RL>(A1, A2, A3, nk, x0, x1, x2, lp, rx, sx, vx, rowb, nsub,
+                                                        nval, land);
                 else
-                    h16_block_emb<H, NC, NEG, GU, PH, true>(A1, A2, nk, x0, x1, x2, lp, rx, sx, vx, rowb, NB, 32,
-                                                            imrow, sel1, sel2);
-            }
-            if (rem > 0) {
-                const int nsub = (rem + S - 1) / S;
-                h16_stage_points(lds, rp, sp, lane);
-                if constexpr (H == 16)
-                    h16_block16<NC, NEG, GU, PH, false>(A1, A2, A3, nk, x0, x1, x2, lp, rx, sx, vx, rowb, nsub,
-                                                        rem);
-                else
-                    h16_block_emb<H, NC, NEG, GU, PH, false>(A1, A2, nk, x0, x1, x2, lp, rx, sx, vx, rowb, nsub,
-                                                             rem, imrow, sel1, sel2);
-            }
+                    h16_block_emb<H, NC, NEG, GU, PH, FU, RL>(A1, A2, nk, x0, x1, x2, lp, rx, sx, vx, rowb, nsub,
+                                                             nval, imrow, sel1, sel2, land);
+            };
+            for (int b = 0; b < nfull; ++b, sp += 768u, sx += 32u * rowb) block(std::true_type(), NB, 32);
+            if (rem > 0) block(std::false_type(), (rem + S - 1) / S, rem);
         };
         if (hr < 0) blocks(std::true_type());
         else blocks(std::false_type());
@@ -1457,26 +1541,28 @@ __device__ __forceinline__ void h16_walk(flt4 (&A1)[NC], flt4 (&A2)[NC], flt4 (&
 
 // One tile on the f16 path with nrows valid rows: the 16-row form above 8
 // rows, else the real embedding with 8 or 4 row slots; rowptr(rsel) gives
-// the coordinates of row slot rsel (rows past the tile repeat its last). The
+// the coordinates of row slot rsel (rows past the tile repeat its last); rel:
+// the tile takes the relative FP32 distance form (with phase32). The
 // sums, scaled by osc (per column chunk), go to fin(acc, rsel) as a
 // Cacc<PrecF32, false> or Cemb<PrecF32, H> accumulator per chunk; lds: the
 // wave's 1 KiB LDS buffer for the source points.
 template <int NC, bool NEG, class RowF, class FinF>
-__device__ __forceinline__ void h16_tile(const NearK& nk, int nrows, int lane, RowF rowptr, int64_t e0, int nr,
-                                         int rot, const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
+__device__ __forceinline__ void h16_tile(const NearK& nk, bool rel, int nrows, int lane, RowF rowptr, int64_t e0,
+                                         int nr, int rot, const int32_t* __restrict__ nrlo,
+                                         const int32_t* __restrict__ nrhi,
                                          const double* __restrict__ spts, const uint2* __restrict__ xsq, int64_t R,
                                          const int (&jl)[NC], const float (&osc)[NC], FinF fin, lds_double* lds) {
     using P = PrecF32;
     const int i = lane & 15, kk = lane >> 4;
-    auto run = [&](auto ph) {
-        constexpr bool PH = decltype(ph)::value;
+    auto run = [&](auto ph, auto rl) {
+        constexpr bool PH = decltype(ph)::value, RL = decltype(rl)::value;
         if (nrows > 8) {
             flt4 Pa[NC], Qa[NC], Ia[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c)
                 for (int v = 0; v < 4; ++v) Pa[c][v] = Qa[c][v] = Ia[c][v] = 0.0f;
-            h16_walk<16, NC, NEG, PH>(Pa, Qa, Ia, nk, rowptr(i), lane, e0, nr, rot, nrlo, nrhi, spts, xsq, R, jl,
-                                      false, 0u, 0u, lds);
+            h16_walk<16, NC, NEG, PH, RL>(Pa, Qa, Ia, nk, rowptr(i), rowptr(0), lane, e0, nr, rot, nrlo, nrhi, spts,
+                                          xsq, R, jl, false, 0u, 0u, lds);
             Cacc<P, false> acc[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
@@ -1499,8 +1585,8 @@ __device__ __forceinline__ void h16_tile(const NearK& nk, int nrows, int lane, R
 #pragma unroll
                 for (int c = 0; c < NC; ++c)
                     for (int v = 0; v < 4; ++v) D1[c][v] = D2[c][v] = 0.0f;
-                h16_walk<H, NC, NEG, PH>(D1, D2, D1, nk, rowptr(i % H), lane, e0, nr, rot, nrlo, nrhi, spts, xsq, R,
-                                         jl, imrow, sel1, sel2, lds);
+                h16_walk<H, NC, NEG, PH, RL>(D1, D2, D1, nk, rowptr(i % H), rowptr(0), lane, e0, nr, rot, nrlo, nrhi,
+                                             spts, xsq, R, jl, imrow, sel1, sel2, lds);
                 // this lane holds MFMA rows 4 kk + v: real parts when kk < H / 4
                 const float sg = kk < H / 4 ? -1.0f : 1.0f;
                 Cemb<P, H> acc[NC];
@@ -1513,8 +1599,9 @@ __device__ __forceinline__ void h16_tile(const NearK& nk, int nrows, int lane, R
             else emb(std::integral_constant<int, 4>());
         }
     };
-    if (nk.phase32) run(std::true_type());
-    else run(std::false_type());
+    if (nk.phase32 && rel) run(std::true_type(), std::true_type());
+    else if (nk.phase32) run(std::true_type(), std::false_type());
+    else run(std::false_type(), std::false_type());
 }
 
 // Hands every output entry of a tile's accumulators (16-row form or real
@@ -1550,8 +1637,9 @@ __global__ void __launch_bounds__(256, NC == 1 ? 5 : 4)
                    const double* __restrict__ dpts, const int64_t* __restrict__ nrptr,
                    const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
                    const double* __restrict__ spts, const double2* __restrict__ yh, const uint2* __restrict__ xs16,
-                   const float* __restrict__ xsc, int64_t R, int64_t q0, NearK nk, double2 alpha, double2 beta,
-                   double2* __restrict__ y, int64_t ldy, int order, int stagger, int nq) {
+                   const float* __restrict__ xsc, int64_t R, int64_t q0, NearK nk, const uint8_t* __restrict__ nrel,
+                   double2 alpha, double2 beta, double2* __restrict__ y, int64_t ldy, int order, int stagger,
+                   int nq) {
     using P = PrecF32;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
@@ -1607,7 +1695,10 @@ __global__ void __launch_bounds__(256, NC == 1 ? 5 : 4)
     auto rowptr = [&](int rsel) { return dpts + 3 * (int64_t)(l0 + row0 + min(rsel, nrows - 1)); };
     __shared__ double h16pts[4][128];
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    h16_tile<NC, NEG>(nk, nrows, lane, rowptr, e0, nr, rot, nrlo, nrhi, spts, xs16 + q0, R, jl, osc, fin,
+    // (one 16-column chunk only: with two, the relative form's registers
+    // spill and it runs slower)
+    const bool rel = NC == 1 && nk.rel32 && nrel[t];
+    h16_tile<NC, NEG>(nk, rel, nrows, lane, rowptr, e0, nr, rot, nrlo, nrhi, spts, xs16 + q0, R, jl, osc, fin,
                       (lds_double*)h16pts[wv]);
 }
 
@@ -1626,8 +1717,9 @@ __global__ void __launch_bounds__(256, 4)
                  const int32_t* __restrict__ cptr_dst, const double* __restrict__ dpiv,
                  const int64_t* __restrict__ crptr, const int32_t* __restrict__ crlo,
                  const int32_t* __restrict__ crhi, const double* __restrict__ spiv, const uint2* __restrict__ xs16,
-                 const float* __restrict__ xsc, int64_t R, int64_t q0, int nq, NearK nk, YT* __restrict__ yh,
-                 double2* __restrict__ yfin, const uint8_t* __restrict__ fin, int order) {
+                 const float* __restrict__ xsc, int64_t R, int64_t q0, int nq, NearK nk,
+                 const uint8_t* __restrict__ crel, YT* __restrict__ yh, double2* __restrict__ yfin,
+                 const uint8_t* __restrict__ fin, int order) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(work_block(order) * 4 + (int)(threadIdx.x >> 6));
     if (w >= ntiles) return;
@@ -1660,14 +1752,58 @@ __global__ void __launch_bounds__(256, 4)
     auto rowptr = [&](int rsel) { return dpiv + 3 * (int64_t)(r0 + row0 + min(rsel, nrows - 1)); };
     __shared__ double h16pts[4][128];
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    h16_tile<NC, NEG>(nk, nrows, lane, rowptr, e0, nr, 0, crlo, crhi, spiv, xs16 + q0, R, jl, osc, fn,
+    const bool rel = NC == 1 && nk.rel32 && crel[t];
+    h16_tile<NC, NEG>(nk, rel, nrows, lane, rowptr, e0, nr, 0, crlo, crhi, spiv, xs16 + q0, R, jl, osc, fn,
                       (lds_double*)h16pts[wv]);
 }
 
-// Smallest squared distance above the zero-distance guard (2e-30) and the
-// largest one over the pairs of each coupling tile (rows: the tile's
-// pivots, dpiv + 3 (cptr[t] + row), sources: spiv over its merged ranges):
-// out[2 w], out[2 w + 1] (+inf and 0 without a pair, or for a padding tile).
+// Squared-distance statistics of one tile (one wave; lanes over the sources
+// of its merged ranges e0 .. e1 - 1 of spts, the rows xrow + 3 r, r < nrows,
+// in a loop) to out[0 .. 3]: the smallest pair above the zero-distance guard
+// (2e-30; +inf without one), the largest pair (0 without one), the smallest
+// nonzero pair (+inf without one) and the largest squared distance of a row
+// from the first row.
+__device__ __forceinline__ void tile_pair_stats(const double* __restrict__ xrow, int nrows, int64_t e0, int64_t e1,
+                                                const int32_t* __restrict__ rlo, const int32_t* __restrict__ rhi,
+                                                const double* __restrict__ spts, int lane, double* __restrict__ out) {
+    double m = HUGE_VAL, mx = 0.0, mp = HUGE_VAL;
+    for (int64_t e = e0; e < e1; ++e) {
+        const int lo = rlo[e], hi = rhi[e] & 0x7fffffff;
+        for (int s = lo + lane; s < hi; s += 64) {
+            const double y0 = spts[3 * (int64_t)s], y1 = spts[3 * (int64_t)s + 1], y2 = spts[3 * (int64_t)s + 2];
+            for (int r = 0; r < nrows; ++r) {
+                const double* xp = xrow + 3 * (int64_t)r;
+                const double dx = xp[0] - y0, dy = xp[1] - y1, dz = xp[2] - y2;
+                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                if (r2 > 2e-30) m = fmin(m, r2);
+                if (r2 > 0.0) mp = fmin(mp, r2);
+                mx = fmax(mx, r2);
+            }
+        }
+    }
+    double ext = 0.0;
+    for (int r = lane; r < nrows; r += 64) {
+        const double dx = xrow[3 * (int64_t)r] - xrow[0], dy = xrow[3 * (int64_t)r + 1] - xrow[1],
+                     dz = xrow[3 * (int64_t)r + 2] - xrow[2];
+        ext = fmax(ext, fma(dz, dz, fma(dy, dy, dx * dx)));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        m = fmin(m, __shfl_xor(m, o));
+        mx = fmax(mx, __shfl_xor(mx, o));
+        mp = fmin(mp, __shfl_xor(mp, o));
+        ext = fmax(ext, __shfl_xor(ext, o));
+    }
+    if (lane == 0) {
+        out[0] = m;
+        out[1] = mx;
+        out[2] = mp;
+        out[3] = ext;
+    }
+}
+
+// tile_pair_stats of each coupling tile (rows: the tile's pivots, dpiv +
+// 3 (cptr[t] + row), sources: spiv over its merged ranges; one wave per
+// tile): out[4 w .. 4 w + 3] (+inf, 0, +inf, 0 for a padding tile).
 __global__ void __launch_bounds__(256)
     k_pair_rminmax(const int32_t* __restrict__ tile_node, const int32_t* __restrict__ tile_row0, int64_t ntiles,
                    const int32_t* __restrict__ cptr_dst, const double* __restrict__ dpiv,
@@ -1677,31 +1813,17 @@ __global__ void __launch_bounds__(256)
     const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= ntiles) return;
     const int t = tile_node[w];
-    double m = HUGE_VAL, mx = 0.0;
-    if (t >= 0) {
-        const int r0 = cptr_dst[t] + tile_row0[w], nrows = min(16, cptr_dst[t + 1] - r0);
-        for (int64_t e = crptr[t]; e < crptr[t + 1]; ++e) {
-            const int lo = crlo[e], hi = crhi[e] & 0x7fffffff;
-            for (int s = lo + lane; s < hi; s += 64) {
-                const double y0 = spiv[3 * (int64_t)s], y1 = spiv[3 * (int64_t)s + 1], y2 = spiv[3 * (int64_t)s + 2];
-                for (int r = 0; r < nrows; ++r) {
-                    const double* xp = dpiv + 3 * (int64_t)(r0 + r);
-                    const double dx = xp[0] - y0, dy = xp[1] - y1, dz = xp[2] - y2;
-                    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                    if (r2 > 2e-30) m = fmin(m, r2);
-                    mx = fmax(mx, r2);
-                }
-            }
+    if (t < 0) {
+        if (lane == 0) {
+            out[4 * w] = HUGE_VAL;
+            out[4 * w + 1] = 0.0;
+            out[4 * w + 2] = HUGE_VAL;
+            out[4 * w + 3] = 0.0;
         }
+        return;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        m = fmin(m, __shfl_xor(m, o));
-        mx = fmax(mx, __shfl_xor(mx, o));
-    }
-    if (lane == 0) {
-        out[2 * w] = m;
-        out[2 * w + 1] = mx;
-    }
+    const int r0 = cptr_dst[t] + tile_row0[w], nrows = min(16, cptr_dst[t + 1] - r0);
+    tile_pair_stats(dpiv + 3 * (int64_t)r0, nrows, crptr[t], crptr[t + 1], crlo, crhi, spiv, lane, out + 4 * w);
 }
 
 // Per-workgroup column maxima of |re|, |im| of a c32 [n][R] array over
@@ -1731,10 +1853,8 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// Smallest squared distance above the zero-distance guard (2e-30) and the
-// largest one over the near pairs of each destination tile (one wave per
-// tile, lanes over the sources of its merged ranges, the tile's rows in a
-// loop): out[2 w] (+inf without a pair), out[2 w + 1] (0 without one).
+// tile_pair_stats of each near-field destination tile (one wave per tile):
+// out[4 w .. 4 w + 3].
 __global__ void __launch_bounds__(256)
     k_near_rmin(const int32_t* __restrict__ tile_leaf, const int32_t* __restrict__ tile_row0, int64_t ntiles,
                 const int32_t* __restrict__ leaves, const int32_t* __restrict__ dlo, const int32_t* __restrict__ dhi,
@@ -1746,28 +1866,8 @@ __global__ void __launch_bounds__(256)
     if (w >= ntiles) return;
     const int t = leaves[tile_leaf[w]], row0 = tile_row0[w];
     const int l0 = dlo[t], nrows = min(16, dhi[t] - l0 - row0);
-    double m = HUGE_VAL, mx = 0.0;
-    for (int64_t e = nrptr[t]; e < nrptr[t + 1]; ++e) {
-        const int lo = nrlo[e], hi = nrhi[e] & 0x7fffffff;
-        for (int s = lo + lane; s < hi; s += 64) {
-            const double y0 = spts[3 * (int64_t)s], y1 = spts[3 * (int64_t)s + 1], y2 = spts[3 * (int64_t)s + 2];
-            for (int r = 0; r < nrows; ++r) {
-                const double* xp = dpts + 3 * (int64_t)(l0 + row0 + r);
-                const double dx = xp[0] - y0, dy = xp[1] - y1, dz = xp[2] - y2;
-                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                if (r2 > 2e-30) m = fmin(m, r2);
-                mx = fmax(mx, r2);
-            }
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        m = fmin(m, __shfl_xor(m, o));
-        mx = fmax(mx, __shfl_xor(mx, o));
-    }
-    if (lane == 0) {
-        out[2 * w] = m;
-        out[2 * w + 1] = mx;
-    }
+    tile_pair_stats(dpts + 3 * (int64_t)(l0 + row0), nrows, nrptr[t], nrptr[t + 1], nrlo, nrhi, spts, lane,
+                    out + 4 * w);
 }
 
 // out[3 k + d] = pts[3 pos[k] + d] (k < K), zeros for k in [K, K + pad).
@@ -2377,7 +2477,8 @@ void run_couple_h16(nca_h2* h, int64_t R, int nq, int64_t q0, float2* yc, const 
     const int64_t ntiles = FWD ? h->f_ntiles : h->cr_ntiles;
     if (ntiles > 0) {
         const bool neg = (h->sign < 0) != CJ;
-        const NearK nk = near_consts_scaled(h->k, std::fabs(h->k) * 0.5 / M_PI * h->coup_rmax, h->coup_t);
+        NearK nk = near_consts_scaled(h->k, std::fabs(h->k) * 0.5 / M_PI * h->coup_rmax, h->coup_t);
+        nk.rel32 = !h->rel32_off && dst.coup_rel.p != nullptr;
         const int32_t* tn = FWD ? h->f_tile_node.p : h->cr_tile_node.p;
         const int32_t* tr = FWD ? h->f_tile_row0.p : h->cr_tile_row0.p;
         const double* dpiv = FWD ? h->piv_test.p : h->piv_trial.p;
@@ -2388,13 +2489,13 @@ void run_couple_h16(nca_h2* h, int64_t R, int nq, int64_t q0, float2* yc, const 
         if (yc) {
             auto kern = neg ? k_couple_h16<true, NC, float2> : k_couple_h16<false, NC, float2>;
             hipLaunchKernelGGL(kern, dim3(blocks_for(ntiles, 4)), dim3(256), 0, st, tn, tr, ntiles, dst.cptr.p, dpiv,
-                               crp, crl, crh, spiv, h->ws.xh16, h->ws.xhsc, R, q0, nq, nk, yc, h->ws.yh, fin,
-                               h->couple_xcd);
+                               crp, crl, crh, spiv, h->ws.xh16, h->ws.xhsc, R, q0, nq, nk, dst.coup_rel.p, yc,
+                               h->ws.yh, fin, h->couple_xcd);
         } else {
             auto kern = neg ? k_couple_h16<true, NC, double2> : k_couple_h16<false, NC, double2>;
             hipLaunchKernelGGL(kern, dim3(blocks_for(ntiles, 4)), dim3(256), 0, st, tn, tr, ntiles, dst.cptr.p, dpiv,
-                               crp, crl, crh, spiv, h->ws.xh16, h->ws.xhsc, R, q0, nq, nk, h->ws.yh, h->ws.yh,
-                               nullptr, h->couple_xcd);
+                               crp, crl, crh, spiv, h->ws.xh16, h->ws.xhsc, R, q0, nq, nk, dst.coup_rel.p,
+                               h->ws.yh, h->ws.yh, nullptr, h->couple_xcd);
         }
     }
     NCA_HIP(hipGetLastError());
@@ -2432,13 +2533,14 @@ void run_leaf(nca_h2* h, int64_t R, int nq, int64_t q0, double2* y, int64_t ldy,
     if constexpr (MIX) {
         if (dst.ntiles > 0 && !NS && near_h16<MIX>(h, FWD)) {
             const bool neg = (h->sign < 0) != CJ;
-            const NearK nk = near_consts_scaled(h->k, h->near_turns, h->h16_t);
+            NearK nk = near_consts_scaled(h->k, h->near_turns, h->h16_t);
+            nk.rel32 = !h->rel32_off && dst.near_rel.p != nullptr;
             auto kern = neg ? k_leaf_out_h16<FWD, CJ, true, NC> : k_leaf_out_h16<FWD, CJ, false, NC>;
             hipLaunchKernelGGL(kern, dim3(blocks_for(dst.ntiles, 4)), dim3(256), 0, st, dst.tile_leaf.p,
                                dst.tile_row0.p, dst.ntiles, dst.leaves.p, dst.leafbasis.p, dst.boff.p, O::basis(dst),
                                dst.lo.p, dst.hi.p, dst.perm.p, dst.node_cptr(), dst.pts_h16.p, dst.nrptr.p,
-                               dst.nrlo.p, dst.nrhi.p, src.pts_h16.p, yh, h->ws.xs16, h->ws.xsc, R, q0, nk, alpha,
-                               beta, y, ldy, h->near_xcd, h->near_stagger, nq);
+                               dst.nrlo.p, dst.nrhi.p, src.pts_h16.p, yh, h->ws.xs16, h->ws.xsc, R, q0, nk,
+                               dst.near_rel.p, alpha, beta, y, ldy, h->near_xcd, h->near_stagger, nq);
             if (dst.nibasis > 0)
                 hipLaunchKernelGGL((k_basis_out<FWD, CJ, T2>), dim3((unsigned)dst.nibasis), dim3(256), 0, st,
                                    dst.ibslot.p, dst.bnodes.p, dst.boff.p, O::basis(dst), dst.lo.p, dst.hi.p,
@@ -2768,6 +2870,41 @@ void ensure_s32(nca_h2* h) {
     NCA_HIP(hipStreamSynchronize(st));
 }
 
+// The rel32 condition over tile statistics (tile_pair_stats, 4 per tile):
+// no pair strictly between 0 and the zero-distance guard, and the largest
+// distance of a row from the tile's first row c within kRel32Span of the
+// tile's smallest pair distance. For a pair (x, y) at distance r, |x - c| <=
+// kRel32Span r and |y - c| <= |x - c| + r; rounding x - c and y - c to FP32
+// and subtracting errs by at most 2^-24 (2 |x - c| + 2 r), i.e. by at most
+// 2^-23 (1 + kRel32Span) of r.
+// rel32_flags: per node of nn, 1 when every tile of the node (tnode[w])
+// meets the condition; returns the fraction of tiles that do.
+constexpr double kRel32Span = 32.0;
+double rel32_flags(const std::vector<double>& hm, const std::vector<int32_t>& tnode, int64_t nn,
+                   std::vector<uint8_t>& flag) {
+    flag.assign((size_t)std::max<int64_t>(nn, 1), 1);
+    int64_t ok = 0, nt = 0;
+    for (size_t w = 0; w < tnode.size(); ++w) {
+        const int32_t t = tnode[w];
+        if (t < 0) continue;  // padding tile
+        const double* q = hm.data() + 4 * w;
+        const bool good = q[2] > 2e-30 && (!std::isfinite(q[0]) || q[3] <= kRel32Span * kRel32Span * q[0]);
+        ++nt;
+        if (good) ++ok;
+        else flag[(size_t)t] = 0;
+    }
+    return nt ? (double)ok / (double)nt : 0.0;
+}
+// Node of each near-field tile of a side in the current leaf order (the
+// tiles run leaf by leaf, 16 rows each).
+std::vector<int32_t> near_tile_nodes(const SideDev& sd) {
+    std::vector<int32_t> out;
+    out.reserve((size_t)sd.ntiles);
+    for (int32_t t : sd.h_leaves)
+        for (int r0 = 0; r0 < sd.h_hi[t] - sd.h_lo[t]; r0 += 16) out.push_back(t);
+    return out;
+}
+
 // c32 copies of the operator for the MIXED path (made once: at creation in
 // MIXED, by nca_h2_set_precision, or on first use). It allocates once and
 // frees nothing a captured sweep uses, so the workspace generation stays.
@@ -2790,22 +2927,42 @@ void ensure_mixed(nca_h2* h) {
     h->h16 = 0;
     h->near_rmin = h->near_rmax_pairs = 0.0;
     if (h->test.ntiles > 0) {
-        DevBuf<double> mins((size_t)(2 * h->test.ntiles));
+        DevBuf<double> mins((size_t)(4 * h->test.ntiles));
         hipLaunchKernelGGL(k_near_rmin, dim3(blocks_for(h->test.ntiles, 4)), dim3(256), 0, st, h->test.tile_leaf.p,
                            h->test.tile_row0.p, h->test.ntiles, h->test.leaves.p, h->test.lo.p, h->test.hi.p,
                            h->test.pts.p, h->test.nrptr.p, h->test.nrlo.p, h->test.nrhi.p, h->trial.pts.p, mins.p);
         NCA_HIP(hipGetLastError());
-        std::vector<double> hm((size_t)(2 * h->test.ntiles));
+        std::vector<double> hm((size_t)(4 * h->test.ntiles));
         NCA_HIP(hipMemcpyAsync(hm.data(), mins.p, hm.size() * sizeof(double), hipMemcpyDeviceToHost, st));
         NCA_HIP(hipStreamSynchronize(st));
         double r2 = HUGE_VAL, r2x = 0.0;
-        for (size_t q = 0; q < hm.size(); q += 2) {
+        for (size_t q = 0; q < hm.size(); q += 4) {
             r2 = std::min(r2, hm[q]);
             r2x = std::max(r2x, hm[q + 1]);
         }
         const double rmin = std::sqrt(r2), rmax = std::sqrt(r2x);
         h->near_rmin = std::isfinite(rmin) ? rmin : 0.0;
         h->near_rmax_pairs = rmax;
+        std::vector<uint8_t> flag;
+        const double ff = rel32_flags(hm, near_tile_nodes(h->test), (int64_t)h->test.h_lo.size(), flag);
+        h->test.near_rel.upload(flag, st);
+        h->near_rel_frac = ff;
+        // the reverse tiles (transpose and adjoint products) as well
+        if (h->trial.ntiles > 0) {
+            DevBuf<double> rv((size_t)(4 * h->trial.ntiles));
+            hipLaunchKernelGGL(k_near_rmin, dim3(blocks_for(h->trial.ntiles, 4)), dim3(256), 0, st,
+                               h->trial.tile_leaf.p, h->trial.tile_row0.p, h->trial.ntiles, h->trial.leaves.p,
+                               h->trial.lo.p, h->trial.hi.p, h->trial.pts.p, h->trial.nrptr.p, h->trial.nrlo.p,
+                               h->trial.nrhi.p, h->test.pts.p, rv.p);
+            NCA_HIP(hipGetLastError());
+            std::vector<double> hv((size_t)(4 * h->trial.ntiles));
+            NCA_HIP(hipMemcpyAsync(hv.data(), rv.p, hv.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            NCA_HIP(hipStreamSynchronize(st));
+            const double fr = rel32_flags(hv, near_tile_nodes(h->trial), (int64_t)h->trial.h_lo.size(), flag);
+            h->trial.near_rel.upload(flag, st);
+            h->near_rel_frac = (ff * (double)h->test.ntiles + fr * (double)h->trial.ntiles) /
+                               (double)(h->test.ntiles + h->trial.ntiles);
+        }
         if (std::isfinite(rmin) && rmin > 0.0 && rmax <= 131072.0 * rmin) {
             int e = 0;
             (void)std::frexp(rmin, &e);
@@ -3576,17 +3733,41 @@ void setup_couple_h16(nca_h2* h) {
     h->cr_ntiles = (int64_t)tnode.size();
     h->cr_tile_node.upload(tnode, st);
     h->cr_tile_row0.upload(trow, st);
-    // far pivot distances over the forward tiles (every far pair once)
-    DevBuf<double> mm((size_t)(2 * h->f_ntiles));
-    hipLaunchKernelGGL(k_pair_rminmax, dim3(blocks_for(h->f_ntiles, 4)), dim3(256), 0, st, h->f_tile_node.p,
-                       h->f_tile_row0.p, h->f_ntiles, h->test.cptr.p, h->piv_test.p, h->cf_ptr.p, h->cf_lo.p,
-                       h->cf_hi.p, h->piv_trial.p, mm.p);
-    NCA_HIP(hipGetLastError());
-    std::vector<double> hm((size_t)(2 * h->f_ntiles));
-    NCA_HIP(hipMemcpyAsync(hm.data(), mm.p, hm.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-    NCA_HIP(hipStreamSynchronize(st));
+    // far pivot distances over the forward tiles (every far pair once), and
+    // the reverse tiles' statistics for the rel32 condition
+    auto stats = [&](int64_t nt, const int32_t* tn, const int32_t* tr, const int32_t* cpd, const double* dp,
+                     const int64_t* rp, const int32_t* rl, const int32_t* rh, const double* sp) {
+        std::vector<double> out((size_t)(4 * nt));
+        if (nt == 0) return out;
+        DevBuf<double> mm((size_t)(4 * nt));
+        hipLaunchKernelGGL(k_pair_rminmax, dim3(blocks_for(nt, 4)), dim3(256), 0, st, tn, tr, nt, cpd, dp, rp, rl,
+                           rh, sp, mm.p);
+        NCA_HIP(hipGetLastError());
+        NCA_HIP(hipMemcpyAsync(out.data(), mm.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        NCA_HIP(hipStreamSynchronize(st));
+        return out;
+    };
+    const std::vector<double> hm = stats(h->f_ntiles, h->f_tile_node.p, h->f_tile_row0.p, h->test.cptr.p,
+                                         h->piv_test.p, h->cf_ptr.p, h->cf_lo.p, h->cf_hi.p, h->piv_trial.p);
+    const std::vector<double> hr = stats(h->cr_ntiles, h->cr_tile_node.p, h->cr_tile_row0.p, h->trial.cptr.p,
+                                         h->piv_trial.p, h->cr_ptr.p, h->cr_lo.p, h->cr_hi.p, h->piv_test.p);
+    {
+        std::vector<int32_t> fnode((size_t)h->f_ntiles);
+        if (h->f_ntiles > 0) {
+            NCA_HIP(hipMemcpyAsync(fnode.data(), h->f_tile_node.p, fnode.size() * sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, st));
+            NCA_HIP(hipStreamSynchronize(st));
+        }
+        std::vector<uint8_t> flag;
+        const double ff = rel32_flags(hm, fnode, h->test.ne, flag);
+        h->test.coup_rel.upload(flag, st);
+        const double fr = rel32_flags(hr, tnode, h->trial.ne, flag);
+        h->trial.coup_rel.upload(flag, st);
+        h->coup_rel_frac = (ff * (double)h->f_ntiles + fr * (double)h->cr_ntiles) /
+                           (double)std::max<int64_t>(h->f_ntiles + h->cr_ntiles, 1);
+    }
     double r2 = HUGE_VAL, r2x = 0.0;
-    for (size_t q = 0; q < hm.size(); q += 2) {
+    for (size_t q = 0; q < hm.size(); q += 4) {
         r2 = std::min(r2, hm[q]);
         r2x = std::max(r2x, hm[q + 1]);
     }
@@ -4719,6 +4900,31 @@ int32_t nca_h2_couple_form(nca_h2* h2, int32_t* form, double* rmin, double* rmax
         *form = f;
         if (rmin) *rmin = h2->coup_rmin;
         if (rmax) *rmax = h2->coup_rmax;
+    });
+}
+
+int32_t nca_h2_rel32(nca_h2* h2, int32_t mode, int32_t* near_on, int32_t* couple_on, double* near_frac,
+                     double* couple_frac) {
+    return guarded([&] {
+        NCA_REQUIRE(h2 && mode >= -1 && mode <= 1, NCA_ERR_ARGUMENT, "nca_h2_rel32: invalid argument");
+        std::lock_guard<std::mutex> lock(h2->mu);
+        set_device(h2->ctx);
+        if (mode >= 0 && h2->rel32_off != (mode == 0)) {
+            ++h2->generation;  // captured sweeps hold the previous form
+            h2->rel32_off = mode == 0 ? 1 : 0;
+        }
+        const bool mix = h2->precision == NCA_PREC_MIXED;
+        if (mix) ensure_mixed(h2);
+        const bool on = mix && !h2->rel32_off;
+        // in use: the f16 forms run (forms 2 of nca_h2_near_form / _couple_form)
+        // with every phase below one revolution
+        const bool nf = on && !h2->near_stored && h2->near_k3[1] == 2 && h2->h16 == 1 &&
+                        h2->near_turns <= kPhase32Turns;
+        const bool cf = on && couple_otf(h2) && std::fabs(h2->k) * 0.5 / M_PI * h2->coup_rmax <= kPhase32Turns;
+        if (near_on) *near_on = nf && h2->near_rel_frac > 0.0 ? 1 : 0;
+        if (couple_on) *couple_on = cf && h2->coup_rel_frac > 0.0 ? 1 : 0;
+        if (near_frac) *near_frac = mix ? h2->near_rel_frac : 0.0;
+        if (couple_frac) *couple_frac = mix ? h2->coup_rel_frac : 0.0;
     });
 }
 

@@ -134,6 +134,7 @@ SIGNATURES = {
     "nca_h2_get_stats": (I32, [P, P]),
     "nca_h2_near_form": (I32, [P, P, P, P]),
     "nca_h2_couple_form": (I32, [P, P, P, P]),
+    "nca_h2_rel32": (I32, [P, I32, P, P, P, P]),
 }
 
 

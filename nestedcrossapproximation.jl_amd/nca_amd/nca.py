@@ -222,6 +222,22 @@ class PetrovGalerkinNCACore:
         call(hip_lib().nca_h2_couple_form(self.handle, ctypes.byref(f), ctypes.byref(a), ctypes.byref(b)))
         return f.value, a.value, b.value
 
+    def rel32(self, mode=-1, fractions=False):
+        """(near_on, couple_on) of nca_h2_rel32: whether the f16 near field and
+        coupling take FP32 distances from tile-relative coordinates on the
+        tiles that allow it; mode 1 allows it (default), 0 keeps FP64
+        differences, -1 only queries. fractions: also the fraction of tiles
+        that meet the condition (near, coupling)."""
+        a = ctypes.c_int32()
+        b = ctypes.c_int32()
+        fa = ctypes.c_double()
+        fb = ctypes.c_double()
+        call(hip_lib().nca_h2_rel32(self.handle, int(mode), ctypes.byref(a), ctypes.byref(b), ctypes.byref(fa),
+                                    ctypes.byref(fb)))
+        if fractions:
+            return bool(a.value), bool(b.value), fa.value, fb.value
+        return bool(a.value), bool(b.value)
+
     def set_precision(self, precision):
         call(hip_lib().nca_h2_set_precision(self.handle, int(precision)))
 

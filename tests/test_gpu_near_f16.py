@@ -8,7 +8,10 @@ exact under power-of-two scaling of each right-hand side (the per-column
 scales), exact zeros for zero columns, and near pairs spanning more than 2^17
 in distance fall back to the f32 form. The coupling evaluated on the fly at
 the pivots (nca_h2_couple_form): within the bar for N, T and C at 16, 3 and
-40 right-hand sides, no less accurate than the stored c32 blocks. A point at
+40 right-hand sides, no less accurate than the stored c32 blocks. Distances
+in FP32 from tile-relative coordinates where the tile statistics allow them
+(nca_h2_rel32), within the bar, and FP64 differences kept on the tiles whose
+rows lie more than 32 times their smallest pair distance apart. A point at
 the origin (where the sources past a range end read their coordinates)
 leaves every form finite."""
 import numpy as np
@@ -121,6 +124,11 @@ def test_f16_distance_span(gap, form):
     f, rmin, rmax = core.near_form()
     say(f"gap {gap}: form {f}, near distances {rmin:.3e} .. {rmax:.3e}")
     assert f == form and abs(rmin - gap) <= 1e-3 * gap
+    # the close pair's tiles hold rows more than 32 times its distance apart:
+    # they keep FP64 differences, the others take the relative form
+    nr, _, fn, _ = core.rel32(fractions=True)
+    say(f"gap {gap}: tiles on the relative form {fn:.4f}")
+    assert nr == (form == 2) and 0.5 < fn < 1
     rng = np.random.default_rng(43)
     x = geometry.complex_gaussian(rng, (pts.shape[0], 16))
     assert relerr(W @ x, H.matvec(x)) <= 1e-5
@@ -154,6 +162,40 @@ def test_couple_on_the_fly_within_bar():
             assert e2 <= 1e-5 and e1 <= 1e-5
             assert e2 <= 2 * e1
             assert not np.array_equal(res[2][op], res[1][op])
+
+
+def test_rel32_distances_within_bar():
+    """8,192 sphere points, k = 2: the f16 near field and coupling take their
+    distances in FP32 from tile-relative coordinates (nca_h2_rel32, allowed
+    by the tile statistics); N, T and C at 16 and 3 right-hand sides within
+    1e-5 of the FP64 oracle with it and without it (FP64 differences), the
+    relative form not above 1.5 times the other's error, and different from
+    it; switching back restores the first result bit for bit."""
+    pts = geometry.fibonacci_sphere(8192)
+    core, H = core_of(pts, 2.0)
+    W = NC.H2Workspace(core)
+    core.set_precision(nca_amd.NCA_PREC_MIXED)
+    TU.apply(core, dict(BASE, near_k3=2, couple_k3=2))
+    assert core.near_form()[0] == 2 and core.couple_form()[0] == 2
+    nr, cr, fn, fc = core.rel32(fractions=True)
+    say(f"tiles on the relative form: near {fn:.4f}, coupling {fc:.4f}")
+    assert (nr, cr) == (True, True) and fn >= 0.9 and fc >= 0.9
+    rng = np.random.default_rng(45)
+    for R in (16, 3):
+        x = geometry.complex_gaussian(rng, (pts.shape[0], R))
+        refs = [H.matvec(x, op=op) for op in (0, 1, 2)]
+        res = {}
+        for mode in (1, 0):
+            assert core.rel32(mode) == (bool(mode), bool(mode))
+            res[mode] = [W @ x, NC.transpose(W) @ x, NC.adjoint(W) @ x]
+        core.rel32(1)
+        assert np.array_equal(W @ x, res[1][0])
+        for op in range(3):
+            er, ed = relerr(res[1][op], refs[op]), relerr(res[0][op], refs[op])
+            say(f"R={R} op={op}: relative FP32 {er:.2e} FP64 differences {ed:.2e}")
+            assert er <= 1e-5 and ed <= 1e-5
+            assert er <= 1.5 * ed
+            assert not np.array_equal(res[1][op], res[0][op])
 
 
 def test_point_at_origin_every_form():
