@@ -1628,7 +1628,7 @@ __device__ __forceinline__ void emit_tile(Cemb<PrecF32, HE> (&acc)[NC], EF emitt
 // all R columns; xsc: per column s_q and 1 / s_q). The near sums are scaled
 // back (t / (4 pi s_q)), then the basis rows are added on the f32 MFMA.
 template <bool FWD, bool CJ, bool NEG, int NC>
-__global__ void __launch_bounds__(256, NC == 1 ? 5 : 4)
+__global__ void __launch_bounds__(256, NC == 1 ? 5 : 3)
     k_leaf_out_h16(const int32_t* __restrict__ tile_leaf, const int32_t* __restrict__ tile_row0, int64_t ntiles,
                    const int32_t* __restrict__ leaves, const int32_t* __restrict__ leafbasis,
                    const int64_t* __restrict__ boff, const float2* __restrict__ bblocks,
@@ -1695,9 +1695,7 @@ __global__ void __launch_bounds__(256, NC == 1 ? 5 : 4)
     auto rowptr = [&](int rsel) { return dpts + 3 * (int64_t)(l0 + row0 + min(rsel, nrows - 1)); };
     __shared__ double h16pts[4][128];
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    // (one 16-column chunk only: with two, the relative form's registers
-    // spill and it runs slower)
-    const bool rel = NC == 1 && nk.rel32 && nrel[t];
+    const bool rel = nk.rel32 && nrel[t];
     h16_tile<NC, NEG>(nk, rel, nrows, lane, rowptr, e0, nr, rot, nrlo, nrhi, spts, xs16 + q0, R, jl, osc, fin,
                       (lds_double*)h16pts[wv]);
 }
@@ -1712,7 +1710,7 @@ __global__ void __launch_bounds__(256, NC == 1 ? 5 : 4)
 // chunk's column scales xsc (relative to q0). The tiles and the output are
 // those of k_couple_fwd_mfma (yh, or yfin in FP64 for the blocks fin marks).
 template <bool NEG, int NC, class YT>
-__global__ void __launch_bounds__(256, 4)
+__global__ void __launch_bounds__(256, NC == 1 ? 4 : 3)
     k_couple_h16(const int32_t* __restrict__ tile_node, const int32_t* __restrict__ tile_row0, int64_t ntiles,
                  const int32_t* __restrict__ cptr_dst, const double* __restrict__ dpiv,
                  const int64_t* __restrict__ crptr, const int32_t* __restrict__ crlo,
@@ -1752,7 +1750,7 @@ __global__ void __launch_bounds__(256, 4)
     auto rowptr = [&](int rsel) { return dpiv + 3 * (int64_t)(r0 + row0 + min(rsel, nrows - 1)); };
     __shared__ double h16pts[4][128];
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const bool rel = NC == 1 && nk.rel32 && crel[t];
+    const bool rel = nk.rel32 && crel[t];
     h16_tile<NC, NEG>(nk, rel, nrows, lane, rowptr, e0, nr, 0, crlo, crhi, spiv, xs16 + q0, R, jl, osc, fn,
                       (lds_double*)h16pts[wv]);
 }
