@@ -84,13 +84,15 @@ def stage_bytes(st, R, M, N, mixed, beta=False, h16=False, otf=False):
     clears (the MFMA stages write every coefficient once), and y read only
     when beta is nonzero. Leaf and internal coefficient counts come from the
     ranks (``coeffs_*_leaves``). h16 (MIXED near field on the f16 MFMA): the
-    gather also reads its c32 copy once and writes the 8-byte split copy the
-    near field reads instead, and the per-workgroup column maxima (4 B per 64
-    points and column, written and read). otf (MIXED coupling evaluated on the
-    fly): instead of the coupling entries, the source coefficients read, split
-    into f16 parts (written) and read once, the pivot coordinates of both
-    sides (24 B per coefficient) and the merged far ranges (at most 8 B per
-    coupling block)."""
+    gather also reads its c32 copy once and writes the split copy the near
+    field reads instead (16 B per point and column: the f16 parts of the point
+    and of the point four places on, side by side), and the per-workgroup
+    column maxima (4 B per 64 points and column, written and read). otf
+    (MIXED coupling evaluated on the fly): instead of the coupling entries,
+    the source coefficients read (8 B), their split copy written and read
+    once (16 B each), the pivot coordinates of both sides (24 B per
+    coefficient) and the merged far ranges (at most 8 B per coupling
+    block)."""
     s_op = 8 if mixed else 16
     s_x = 8 if mixed else 16
     Ks, Kd = st["coeffs_trial"], st["coeffs_test"]
@@ -99,16 +101,17 @@ def stage_bytes(st, R, M, N, mixed, beta=False, h16=False, otf=False):
     down_vec = (8 + 16) * R * Kd if mixed else (16 + 16) * R * Kd
     gather = 16 * R * N + s_x * R * N + 4 * N
     if h16:
-        gather += (8 + 8) * R * N + 2 * 4 * R * ((N + 63) // 64)
+        gather += (8 + 16) * R * N + 2 * 4 * R * ((N + 63) // 64)
+    s_near = 16 if h16 else s_x
     return {
         "gather": gather,
         "leaf_projection": s_op * st["basis_entries_trial"] + s_x * R * N + s_x * R * Ks_leaf,
         "upward": s_op * st["transfer_entries_trial"] + s_x * R * (Ks + Ks_int),
         "coupling": (s_op * st["coupling_entries"] + 4 * st["coupling_columns"] + s_x * R * Ks
-                     if not otf else 3 * 8 * R * Ks + 24 * (Ks + Kd) + 8 * st["coupling_blocks"])
+                     if not otf else (8 + 16 + 16) * R * Ks + 24 * (Ks + Kd) + 8 * st["coupling_blocks"])
         + (8 if mixed else 16) * R * Kd,
         "downward": s_op * st["transfer_entries_test"] + 16 * R * Kd_int + down_vec,
-        "leaf_expansion_near": s_op * st["basis_entries_test"] + 16 * R * Kd_leaf + s_x * R * N + 24 * (M + N)
+        "leaf_expansion_near": s_op * st["basis_entries_test"] + 16 * R * Kd_leaf + s_near * R * N + 24 * (M + N)
         + 12 * st["near_pairs"] + (2 if beta else 1) * 16 * R * M,
     }
 

@@ -184,11 +184,11 @@ struct nca_h2 {
     nca::DevBuf<float2> ws_xs32, ws_xh32, ws_yc;
     // the f16 near field's split right-hand sides, the gather's per-workgroup
     // column maxima and the column scales (s_q, 1 / s_q)
-    nca::DevBuf<uint2> ws_xs16;
+    nca::DevBuf<uint4> ws_xs16;
     nca::DevBuf<float> ws_xpart, ws_xsc;
     // the on-the-fly coupling's split source coefficients (K x R), the column
     // maxima of a chunk and its scales (up to 32 columns)
-    nca::DevBuf<uint2> ws_xh16;
+    nca::DevBuf<uint4> ws_xh16;
     nca::DevBuf<float> ws_xhpart, ws_xhsc;
     // The device workspace the stage kernels of the product in flight use:
     // the operator's own buffers above, or carved from the static scratch
@@ -201,10 +201,10 @@ struct nca_h2 {
         float2* yc = nullptr;     // c32 coupling results (MIXED forward)
         float2* xh32 = nullptr;   // compact c32 copy of a 16-column chunk's source coefficients
         double2* ynear = nullptr; // stored near field applied, per point (tree order), FP64
-        uint2* xs16 = nullptr;    // split right-hand sides of the f16 near field (MIXED)
+        uint4* xs16 = nullptr;    // split right-hand sides of the f16 near field (MIXED)
         float* xpart = nullptr;   // per gather workgroup and column: largest |re|, |im|
         float* xsc = nullptr;     // per column: s_q, 1 / s_q
-        uint2* xh16 = nullptr;    // split source coefficients of the on-the-fly coupling
+        uint4* xh16 = nullptr;    // split source coefficients of the on-the-fly coupling
         float* xhpart = nullptr;  // their per-workgroup column maxima (a chunk of up to 32 columns)
         float* xhsc = nullptr;    // the chunk's column scales
     } ws;
@@ -828,6 +828,10 @@ __device__ __forceinline__ int buf_load<int>(__amdgpu_buffer_rsrc_t r, uint32_t 
     return (int)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
 }
 template <>
+__device__ __forceinline__ uint4 buf_load<uint4>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+template <>
 __device__ __forceinline__ uint2 buf_load<uint2>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
@@ -1152,9 +1156,12 @@ __global__ void __launch_bounds__(256, 4)
 // sides are scaled per column by s_q = 2^m (largest |re|, |im| in [2^14,
 // 2^15)); the tile's sums are scaled back by t / (4 pi s_q).
 //
-// Gathered right-hand sides for this path (xs16): per tree position and
-// column one 8-byte entry (re hi | im hi << 16, re lo | im lo << 16), the
-// layout of the c32 copy. A tile's near field is walked per merged source
+// Gathered right-hand sides for this path (xs16): per tree position p and
+// column one 16-byte entry holding the f16 parts of p and p + 4 side by side
+// (re hi, im hi, re lo, im lo, each as a pair), in the order of the c32
+// copy: every tile form below puts sources s and s + 4 of a block into one
+// K-slot pair, so one 16-byte load gives a lane the four B registers of a
+// slot pair without permutes. A tile's near field is walked per merged source
 // range in blocks of sub-steps whose kernel values and right-hand sides fill
 // the 32 K-slots of one f16 product: 16-row tiles take 8 sub-steps of 4
 // sources (lane (i, kk) evaluates row i x source 4 m + kk, K-slot 8 kk + m),
@@ -1290,6 +1297,37 @@ __device__ __forceinline__ void h16_wait_vm() {
     asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
 }
 
+// B operands of one block of the f16 walk, from the paired split copy
+// (per source s and column: the f16 parts of sources s and s + 4, see
+// k_split16): slot pair v (v < 4) of each column chunk is the entry of
+// source 8 v + kk of the block (vx[c]: the lane's column and kk, sx: the
+// block's offset, rowb: bytes per source); entries whose first source lies
+// at or past nsrc (the block's evaluated sources) are zero.
+template <int NC, bool FULL>
+__device__ __forceinline__ void h16_load_b(uint4 (&xb)[NC][4], __amdgpu_buffer_rsrc_t rx, uint32_t sx,
+                                           const uint32_t (&vx)[NC], uint32_t rowb, int nsrc) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        if (FULL || 8 * v < nsrc) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) xb[c][v] = buf_load<uint4>(rx, vx[c], sx + (uint32_t)(8 * v) * rowb);
+        } else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) xb[c][v] = make_uint4(0u, 0u, 0u, 0u);
+        }
+    }
+}
+// Re and im, high and low f16 parts of the four slot pairs.
+__device__ __forceinline__ void h16_b_parts(const uint4 (&xb)[4], u32x4& brh, u32x4& bih, u32x4& brl, u32x4& bil) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        brh[v] = xb[v].x;
+        bih[v] = xb[v].y;
+        brl[v] = xb[v].z;
+        bil[v] = xb[v].w;
+    }
+}
+
 // One block of a 16-row tile's walk: nsub (1..8) sub-steps of 4 sources,
 // their points staged in LDS (lp: this lane's first source there, the
 // staging load the last vector load issued), their right-hand sides at byte
@@ -1299,19 +1337,10 @@ __device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (
                                             typename H16Pt<RL>::X x0, typename H16Pt<RL>::X x1,
                                             typename H16Pt<RL>::X x2, const typename H16Pt<RL>::L* lp,
                                             __amdgpu_buffer_rsrc_t rx, uint32_t sx,
-                                            const uint32_t (&vx)[NC][1], uint32_t rowb, int nsub, int nval,
+                                            const uint32_t (&vx)[NC], uint32_t rowb, int nsub, int nval,
                                             LandF land) {
-    uint2 xb[NC][8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        if (FULL || m < nsub) {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) xb[c][m] = buf_load<uint2>(rx, vx[c][0], sx + (uint32_t)m * 4u * rowb);
-        } else {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) xb[c][m] = make_uint2(0u, 0u);
-        }
-    }
+    uint4 xb[NC][4];
+    h16_load_b<NC, FULL>(xb, rx, sx, vx, rowb, 4 * nsub);
     land(std::bool_constant<FULL>());
     const int kk = (threadIdx.x & 63) >> 4;
     u32x4 arh, aih, arl, ail;
@@ -1342,13 +1371,7 @@ __device__ __forceinline__ void h16_block16(flt4 (&P)[NC], flt4 (&Q)[NC], flt4 (
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
         u32x4 brh, bih, brl, bil;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            brh[v] = pack_lo(xb[c][2 * v].x, xb[c][2 * v + 1].x);
-            bih[v] = pack_hi(xb[c][2 * v].x, xb[c][2 * v + 1].x);
-            brl[v] = pack_lo(xb[c][2 * v].y, xb[c][2 * v + 1].y);
-            bil[v] = pack_hi(xb[c][2 * v].y, xb[c][2 * v + 1].y);
-        }
+        h16_b_parts(xb[c], brh, bih, brl, bil);
         P[c] = mma_h16x3(arh, arl, brh, brl, P[c]);
         Q[c] = mma_h16x3(aih, ail, bih, bil, Q[c]);
         I[c] = mma_h16x3(aih, ail, brh, brl, mma_h16x3(arh, arl, bih, bil, I[c]));
@@ -1369,25 +1392,11 @@ __device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], co
                                               typename H16Pt<RL>::X x0, typename H16Pt<RL>::X x1,
                                               typename H16Pt<RL>::X x2, const typename H16Pt<RL>::L* lp,
                                               __amdgpu_buffer_rsrc_t rx, uint32_t sx,
-                                              const uint32_t (&vx)[NC][16 / H], uint32_t rowb, int nsub, int nval,
+                                              const uint32_t (&vx)[NC], uint32_t rowb, int nsub, int nval,
                                               bool imrow, uint32_t sel1, uint32_t sel2, LandF land) {
-    constexpr int G = 16 / H, S = 64 / H, NS = 32 / S;
-    uint2 xb[NC][NS][G];
-#pragma unroll
-    for (int m = 0; m < NS; ++m) {
-        if (FULL || m < nsub) {
-#pragma unroll
-            for (int c = 0; c < NC; ++c)
-#pragma unroll
-                for (int q = 0; q < G; ++q)
-                    xb[c][m][q] = buf_load<uint2>(rx, vx[c][q], sx + (uint32_t)(m * S) * rowb);
-        } else {
-#pragma unroll
-            for (int c = 0; c < NC; ++c)
-#pragma unroll
-                for (int q = 0; q < G; ++q) xb[c][m][q] = make_uint2(0u, 0u);
-        }
-    }
+    constexpr int S = 64 / H, NS = 32 / S;
+    uint4 xb[NC][4];
+    h16_load_b<NC, FULL>(xb, rx, sx, vx, rowb, S * nsub);
     land(std::bool_constant<FULL>());
     // this lane's source in a sub-step, after the sub-step's first
     const int own = 4 * ((int)(threadIdx.x & 15) / H) + (int)((threadIdx.x & 63) >> 4);
@@ -1430,16 +1439,7 @@ __device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], co
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
         u32x4 brh, bih, brl, bil;
-#pragma unroll
-        for (int m = 0; m < NS; ++m)
-#pragma unroll
-            for (int q = 0; q < G; q += 2) {
-                const int v = m * (G / 2) + q / 2;
-                brh[v] = pack_lo(xb[c][m][q].x, xb[c][m][q + 1].x);
-                bih[v] = pack_hi(xb[c][m][q].x, xb[c][m][q + 1].x);
-                brl[v] = pack_lo(xb[c][m][q].y, xb[c][m][q + 1].y);
-                bil[v] = pack_hi(xb[c][m][q].y, xb[c][m][q + 1].y);
-            }
+        h16_b_parts(xb[c], brh, bih, brl, bil);
         D1[c] = mma_h16x3(a1h, a1l, brh, brl, D1[c]);
         D2[c] = mma_h16x3(a2h, a2l, bih, bil, D2[c]);
     }
@@ -1449,8 +1449,9 @@ __device__ __forceinline__ void h16_block_emb(flt4 (&D1)[NC], flt4 (&D2)[NC], co
 // xp (3 doubles), merged source ranges e0 .. e0 + nr - 1 (from e0 + rot on,
 // cyclically; bit 31 of the end: the range keeps the zero-distance guard),
 // source points spts (3 doubles each, staged per block in the wave's 1 KiB
-// LDS buffer lds), split right-hand sides xsq (R per source, 8 bytes each,
-// from the first column on), columns jl. Accumulates
+// LDS buffer lds), split right-hand sides xsq (R per source, 16 bytes each:
+// the paired parts of k_split16, from the first column on), columns jl.
+// Accumulates
 // Gr xr, Gi xi and Gr xi + Gi xr into A1, A2, A3 (H = 16), or A1 xr and
 // A2 xi into A1, A2 (the real embedding, H = 8 or 4). RL: the points are
 // staged as FP32 coordinates relative to cp (the tile's first row; lane l
@@ -1460,10 +1461,10 @@ template <int H, int NC, bool NEG, bool PH, bool RL>
 __device__ __forceinline__ void h16_walk(flt4 (&A1)[NC], flt4 (&A2)[NC], flt4 (&A3)[NC], const NearK& nk,
                                          const double* xp, const double* cp, int lane, int64_t e0, int nr, int rot,
                                          const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
-                                         const double* __restrict__ spts, const uint2* __restrict__ xsq, int64_t R,
+                                         const double* __restrict__ spts, const uint4* __restrict__ xsq, int64_t R,
                                          const int (&jl)[NC], bool imrow, uint32_t sel1, uint32_t sel2,
                                          lds_double* lds) {
-    constexpr int G = 16 / H, S = 64 / H, NB = 32 / S;
+    constexpr int S = 64 / H, NB = 32 / S;
     using X = typename H16Pt<RL>::X;
     const int kk = lane >> 4;
     X x0, x1, x2;
@@ -1482,12 +1483,10 @@ __device__ __forceinline__ void h16_walk(flt4 (&A1)[NC], flt4 (&A2)[NC], flt4 (&
     }
     const int ysrc = 4 * ((lane & 15) / H) + kk;
     const typename H16Pt<RL>::L* lp = (const typename H16Pt<RL>::L*)lds + 3 * ysrc;
-    const uint32_t rowb = (uint32_t)R * 8u;
-    uint32_t vx[NC][G];
+    const uint32_t rowb = (uint32_t)R * 16u;
+    uint32_t vx[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int q = 0; q < G; ++q) vx[c][q] = ((uint32_t)(4 * q + kk) * (uint32_t)R + (uint32_t)jl[c]) * 8u;
+    for (int c = 0; c < NC; ++c) vx[c] = ((uint32_t)kk * (uint32_t)R + (uint32_t)jl[c]) * 16u;
     for (int cc = 0; cc < nr; ++cc) {
         int ec = cc + rot;
         if (ec >= nr) ec -= nr;
@@ -1520,7 +1519,7 @@ __device__ __forceinline__ void h16_walk(flt4 (&A1)[NC], flt4 (&A2)[NC], flt4 (&
                         ((lds_float*)lds)[2 * lane + 1] = (float)(pv.y - cb);
                         asm volatile("" ::: "memory");
                     } else {
-                        if constexpr (FU) h16_wait_vm<8 * NC>();
+                        if constexpr (FU) h16_wait_vm<4 * NC>();
                         else h16_wait_vm<0>();
                     }
                 };
@@ -1550,7 +1549,7 @@ template <int NC, bool NEG, class RowF, class FinF>
 __device__ __forceinline__ void h16_tile(const NearK& nk, bool rel, int nrows, int lane, RowF rowptr, int64_t e0,
                                          int nr, int rot, const int32_t* __restrict__ nrlo,
                                          const int32_t* __restrict__ nrhi,
-                                         const double* __restrict__ spts, const uint2* __restrict__ xsq, int64_t R,
+                                         const double* __restrict__ spts, const uint4* __restrict__ xsq, int64_t R,
                                          const int (&jl)[NC], const float (&osc)[NC], FinF fin, lds_double* lds) {
     using P = PrecF32;
     const int i = lane & 15, kk = lane >> 4;
@@ -1636,7 +1635,7 @@ __global__ void __launch_bounds__(256, NC == 1 ? 5 : 3)
                    const int32_t* __restrict__ dperm, const int32_t* __restrict__ dcptr,
                    const double* __restrict__ dpts, const int64_t* __restrict__ nrptr,
                    const int32_t* __restrict__ nrlo, const int32_t* __restrict__ nrhi,
-                   const double* __restrict__ spts, const double2* __restrict__ yh, const uint2* __restrict__ xs16,
+                   const double* __restrict__ spts, const double2* __restrict__ yh, const uint4* __restrict__ xs16,
                    const float* __restrict__ xsc, int64_t R, int64_t q0, NearK nk, const uint8_t* __restrict__ nrel,
                    double2 alpha, double2 beta, double2* __restrict__ y, int64_t ldy, int order, int stagger,
                    int nq) {
@@ -1714,7 +1713,7 @@ __global__ void __launch_bounds__(256, NC == 1 ? 4 : 3)
     k_couple_h16(const int32_t* __restrict__ tile_node, const int32_t* __restrict__ tile_row0, int64_t ntiles,
                  const int32_t* __restrict__ cptr_dst, const double* __restrict__ dpiv,
                  const int64_t* __restrict__ crptr, const int32_t* __restrict__ crlo,
-                 const int32_t* __restrict__ crhi, const double* __restrict__ spiv, const uint2* __restrict__ xs16,
+                 const int32_t* __restrict__ crhi, const double* __restrict__ spiv, const uint4* __restrict__ xs16,
                  const float* __restrict__ xsc, int64_t R, int64_t q0, int nq, NearK nk,
                  const uint8_t* __restrict__ crel, YT* __restrict__ yh, double2* __restrict__ yfin,
                  const uint8_t* __restrict__ fin, int order) {
@@ -1911,19 +1910,25 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// out[p R + q0 + c] = the f16 parts of s_c in[p R + q0 + c] (p < n, c < nc;
-// s_c = xsc[2 c]).
+// out[p R + q0 + c] = the f16 parts of s_c in[p R + q0 + c] and of s_c
+// in[(p + 4) R + q0 + c] (zero for p + 4 >= n) as the f16 walk's B slot
+// pairs: (re high of p, of p + 4), (im high ...), (re low ...), (im low ...)
+// (p < n, c < nc; s_c = xsc[2 c]).
 __global__ void k_split16(const float2* __restrict__ in, const float* __restrict__ xsc, int64_t n, int64_t R,
-                          int64_t q0, int nc, uint2* __restrict__ out) {
+                          int64_t q0, int nc, uint4* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * nc) return;  // n R < 2^31 (checked per product)
     const uint32_t p = (uint32_t)i / (uint32_t)nc, c = (uint32_t)i % (uint32_t)nc;
     const int64_t o = (int64_t)p * R + q0 + c;
     const float s = xsc[2 * c];
     const float2 v = in[o];
-    uint32_t h, l;
-    split_h16(v.x * s, v.y * s, h, l);
-    out[o] = make_uint2(h, l);
+    uint32_t h0, l0, h1 = 0u, l1 = 0u;
+    split_h16(v.x * s, v.y * s, h0, l0);
+    if ((int64_t)p + 4 < n) {
+        const float2 w = in[o + 4 * R];
+        split_h16(w.x * s, w.y * s, h1, l1);
+    }
+    out[o] = make_uint4(pack_lo(h0, h1), pack_hi(h0, h1), pack_lo(l0, l1), pack_hi(l0, l1));
 }
 
 // Stage 5 + near field for NQ = 1..3 right-hand sides on VALU (an MFMA
@@ -3118,12 +3123,12 @@ size_t carve_workspace(const nca_h2* h, int64_t R, char* base, nca_h2::WsView& v
         v.yc = reinterpret_cast<float2*>(take(K * r * sizeof(float2)));
         if (h->couple_x32 && R > 16) v.xh32 = reinterpret_cast<float2*>(take(K * 16 * sizeof(float2)));
         if (h->h16 == 1 && h->near_k3[1] == 2) {
-            v.xs16 = reinterpret_cast<uint2*>(take(np * r * sizeof(uint2)));
+            v.xs16 = reinterpret_cast<uint4*>(take(np * r * sizeof(uint4)));
             v.xpart = reinterpret_cast<float*>(take((size_t)blocks_for((int64_t)np, 64) * r * sizeof(float)));
             v.xsc = reinterpret_cast<float*>(take(2 * r * sizeof(float)));
         }
         if (h->coup_h16 == 1 && h->couple_k3[1] == 2) {
-            v.xh16 = reinterpret_cast<uint2*>(take(K * r * sizeof(uint2)));
+            v.xh16 = reinterpret_cast<uint4*>(take(K * r * sizeof(uint4)));
             v.xhpart = reinterpret_cast<float*>(take((size_t)blocks_for((int64_t)K, 64) * 32 * sizeof(float)));
             v.xhsc = reinterpret_cast<float*>(take(64 * sizeof(float)));
         }

@@ -20,22 +20,23 @@ def test_mixed_worked_example():
 
 
 def test_mixed_f16_near_field():
-    # the f16 near field adds to the gather a read of the c32 copy, the write of
-    # the split copy (8 B each per point and column) and the column maxima
-    # (4 B per 64 points and column, written and read); the near field reads
-    # the split copy instead of the c32 one (the same bytes)
+    # the f16 near field adds to the gather a read of the c32 copy (8 B per
+    # point and column), the write of the paired split copy (16 B) and the
+    # column maxima (4 B per 64 points and column, written and read); the near
+    # field reads the split copy (16 B) instead of the c32 one (8 B)
     b, h = bench.stage_bytes(ST, 2, 20, 20, True), bench.stage_bytes(ST, 2, 20, 20, True, h16=True)
-    assert h["gather"] - b["gather"] == 16 * 2 * 20 + 2 * 4 * 2 * 1
-    assert all(h[k] == b[k] for k in b if k != "gather")
+    assert h["gather"] - b["gather"] == (8 + 16) * 2 * 20 + 2 * 4 * 2 * 1
+    assert h["leaf_expansion_near"] - b["leaf_expansion_near"] == (16 - 8) * 2 * 20
+    assert all(h[k] == b[k] for k in b if k not in ("gather", "leaf_expansion_near"))
 
 
 def test_mixed_coupling_on_the_fly():
     # the coupling evaluated on the fly reads no coupling entries or column
-    # maps: the source coefficients three times (read, split copy written and
-    # read), the pivot coordinates of both sides and the ranges; it writes
-    # the same c32 result
+    # maps: the source coefficients read (8 B), their paired split copy
+    # written and read (16 B each), the pivot coordinates of both sides and
+    # the ranges; it writes the same c32 result
     b, o = bench.stage_bytes(ST, 2, 20, 20, True), bench.stage_bytes(ST, 2, 20, 20, True, otf=True)
-    assert o["coupling"] == 3 * 8 * 2 * 10 + 24 * (10 + 12) + 8 * 9 + 8 * 2 * 12
+    assert o["coupling"] == (8 + 16 + 16) * 2 * 10 + 24 * (10 + 12) + 8 * 9 + 8 * 2 * 12
     assert all(o[k] == b[k] for k in b if k != "coupling")
 
 
