@@ -1867,6 +1867,38 @@ __global__ void __launch_bounds__(256)
                     out + 4 * w);
 }
 
+// The forward coupling's column map: for every test node t with columns,
+// out[colbase[t] ..] = the source coefficients of its blocks in block order
+// (block i: trial node s = nodes[i], coefficients scptr[s] .. scptr[s] +
+// srank[s] - 1). One wave per test node; a wave scan places 64 blocks at a
+// time.
+__global__ void __launch_bounds__(256)
+    k_colcoef(int64_t ntest, const int64_t* __restrict__ ptr, const int32_t* __restrict__ nodes,
+              const int32_t* __restrict__ srank, const int32_t* __restrict__ scptr,
+              const int64_t* __restrict__ colbase, const int32_t* __restrict__ ncols, int32_t* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= ntest || ncols[t] == 0) return;
+    int64_t c = colbase[t];
+    const int64_t i1 = ptr[t + 1];
+    for (int64_t i0 = ptr[t]; i0 < i1; i0 += 64) {
+        const int64_t i = i0 + lane;
+        int r = 0, q = 0;
+        if (i < i1) {
+            const int sn = nodes[i];
+            r = srank[sn];
+            q = scptr[sn];
+        }
+        int incl = r;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        for (int j = 0; j < r; ++j) out[c + (incl - r) + j] = q + j;
+        c += __shfl(incl, 63);
+    }
+}
+
 // out[3 k + d] = pts[3 pos[k] + d] (k < K), zeros for k in [K, K + pad).
 __global__ void k_gather_pivots(const double* __restrict__ pts, const int32_t* __restrict__ pos, int64_t K,
                                 double* __restrict__ out) {
@@ -3689,9 +3721,10 @@ void setup_couple_h16(nca_h2* h) {
                       DevBuf<int32_t>& dlo, DevBuf<int32_t>& dhi) {
         const int64_t nn = dsd.ne;
         struct Rg { int32_t lo, hi; bool guard; };
-        std::vector<std::vector<Rg>> per(nn);
-        parallel_for(nn, [&](int64_t t) {
-            auto& rg = per[t];
+        // block t's merged ranges in a per-thread scratch vector (run twice: to
+        // count, then to write, instead of one heap vector per block)
+        auto ranges_of = [&](int64_t t, std::vector<Rg>& rg) {
+            rg.clear();
             for (int64_t i = ptr[t]; i < ptr[t + 1]; ++i) {
                 const int32_t q = nodes[i];
                 // blocks are nodes, or direction indices of their node h_enode
@@ -3707,13 +3740,21 @@ void setup_couple_h16(nca_h2* h) {
                 else rg[k++] = rg[q];
             }
             rg.resize(k);
+        };
+        std::vector<int64_t> cnt(nn, 0);
+        parallel_for(nn, [&](int64_t t) {
+            thread_local std::vector<Rg> rg;
+            ranges_of(t, rg);
+            cnt[t] = (int64_t)rg.size();
         });
         std::vector<int64_t> mp(nn + 1, 0);
-        for (int64_t t = 0; t < nn; ++t) mp[t + 1] = mp[t] + (int64_t)per[t].size();
+        for (int64_t t = 0; t < nn; ++t) mp[t + 1] = mp[t] + cnt[t];
         std::vector<int32_t> lo((size_t)mp[nn]), hi((size_t)mp[nn]);
         parallel_for(nn, [&](int64_t t) {
+            thread_local std::vector<Rg> rg;
+            ranges_of(t, rg);
             int64_t o = mp[t];
-            for (auto& r : per[t]) {
+            for (auto& r : rg) {
                 lo[o] = r.lo;
                 hi[o] = (int32_t)((uint32_t)r.hi | (r.guard ? 0x80000000u : 0u));
                 ++o;
@@ -4133,26 +4174,33 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
                 ncols[t] = (int32_t)C;
             });
             for (int64_t t = 0; t < ntest; ++t) colbase[t + 1] = colbase[t] + ncols[t];
-            std::vector<int32_t> colcoef((size_t)colbase[ntest]);
-            parallel_for(ntest, [&](int64_t t) {
-                if (ncols[t] == 0) return;
-                int64_t c = colbase[t];
-                for (int64_t i = coupling->ptr[t]; i < coupling->ptr[t + 1]; ++i) {
-                    const int64_t s = coupling->nodes[i];
-                    for (int j = 0; j < coupling->trialrank[s]; ++j) colcoef[c++] = h->trial.h_cptr[s] + j;
-                }
-            });
+            const int64_t ncolumns = colbase[ntest];
             colbase.pop_back();
             h->h_ncols = ncols;
             order_coupling_tiles(h.get(), h->couple_order, st);
             h->f_base.upload(base, st);
             h->f_colbase.upload(colbase, st);
             h->f_ncols.upload(ncols, st);
-            h->f_colcoef.upload(colcoef, st);
-            h->stats.plan_bytes += (int64_t)colcoef.size() * 4 + ntest * 20;
+            // the column map itself is built on the device from the block lists
+            h->f_colcoef.alloc((size_t)std::max<int64_t>(ncolumns, 1));
+            if (ncolumns > 0) {
+                std::vector<int32_t> nodes32((size_t)coupling->nblocks);
+                parallel_for(coupling->nblocks, [&](int64_t i) { nodes32[i] = (int32_t)coupling->nodes[i]; });
+                DevBuf<int64_t> dptr;
+                DevBuf<int32_t> dnodes, drank, dcptr;
+                dptr.upload(coupling->ptr, st);
+                dnodes.upload(nodes32, st);
+                drank.upload(coupling->trialrank, st);
+                dcptr.upload(h->trial.h_cptr, st);
+                hipLaunchKernelGGL(k_colcoef, dim3(blocks_for(ntest, 4)), dim3(256), 0, st, ntest, dptr.p, dnodes.p,
+                                   drank.p, dcptr.p, h->f_colbase.p, h->f_ncols.p, h->f_colcoef.p);
+                NCA_HIP(hipGetLastError());
+                NCA_HIP(hipStreamSynchronize(st));
+            }
+            h->stats.plan_bytes += ncolumns * 4 + ntest * 20;
             // the c32 copy costs a pass over all source coefficients: keep it
             // when the columns read each coefficient more than twice on average
-            h->couple_x32 = cx32 ? atoi(cx32) != 0 : (int64_t)colcoef.size() > 2 * h->trial.K;
+            h->couple_x32 = cx32 ? atoi(cx32) != 0 : ncolumns > 2 * h->trial.K;
         }
         timer.mark("forward plan");
         // reverse coupling: blocks by trial block
@@ -4234,7 +4282,6 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
                 const int64_t nn = dsd.nn;
                 std::vector<int64_t> cnt(nn, 0);
                 struct Rg { int32_t lo, hi; bool guard; };
-                std::vector<std::vector<Rg>> per(nn);
                 auto box_d2 = [](const double* a, const double* b) {
                     double d2 = 0.0;
                     for (int d = 0; d < 3; ++d) {
@@ -4243,8 +4290,10 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
                     }
                     return d2;
                 };
-                parallel_for(nn, [&](int64_t t) {
-                    auto& rg = per[t];
+                // node t's merged ranges in a per-thread scratch vector (run twice:
+                // to count, then to write, instead of one heap vector per node)
+                auto ranges_of = [&](int64_t t, std::vector<Rg>& rg) {
+                    rg.clear();
                     for (int64_t i = ptr[t]; i < ptr[t + 1]; ++i) {
                         const int32_t q = nodes[i];
                         if (ssd.h_hi[q] > ssd.h_lo[q])
@@ -4258,14 +4307,20 @@ void create_core(nca_ctx* ctx, const nca_helmholtz* kernel, const nca_points* te
                         else rg[k++] = rg[q];
                     }
                     rg.resize(k);
-                    cnt[t] = (int64_t)k;
+                };
+                parallel_for(nn, [&](int64_t t) {
+                    thread_local std::vector<Rg> rg;
+                    ranges_of(t, rg);
+                    cnt[t] = (int64_t)rg.size();
                 });
                 std::vector<int64_t> mptr(nn + 1, 0);
                 for (int64_t t = 0; t < nn; ++t) mptr[t + 1] = mptr[t] + cnt[t];
                 std::vector<int32_t> mlo((size_t)mptr[nn]), mhi((size_t)mptr[nn]), mhid((size_t)mptr[nn]);
                 parallel_for(nn, [&](int64_t t) {
+                    thread_local std::vector<Rg> rg;
+                    ranges_of(t, rg);
                     int64_t o = mptr[t];
-                    for (auto& r : per[t]) {
+                    for (auto& r : rg) {
                         mlo[o] = r.lo;
                         mhi[o] = r.hi;
                         mhid[o] = (int32_t)((uint32_t)r.hi | (r.guard ? 0x80000000u : 0u));
