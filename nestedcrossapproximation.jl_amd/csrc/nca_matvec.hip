@@ -1709,7 +1709,7 @@ __global__ void __launch_bounds__(256, NC == 1 ? 5 : 3)
 // chunk's column scales xsc (relative to q0). The tiles and the output are
 // those of k_couple_fwd_mfma (yh, or yfin in FP64 for the blocks fin marks).
 template <bool NEG, int NC, class YT>
-__global__ void __launch_bounds__(256, NC == 1 ? 4 : 3)
+__global__ void __launch_bounds__(256, NC == 1 ? 5 : 3)
     k_couple_h16(const int32_t* __restrict__ tile_node, const int32_t* __restrict__ tile_row0, int64_t ntiles,
                  const int32_t* __restrict__ cptr_dst, const double* __restrict__ dpiv,
                  const int64_t* __restrict__ crptr, const int32_t* __restrict__ crlo,
