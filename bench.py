@@ -273,7 +273,8 @@ def main():
         ms_c = timed(2, R, max(args.steps // 2, 1), 1)
         # one right-hand side: iterative solvers call mul! one vector at a time
         ms_1 = timed(0, 1, 50, 5)
-        b1 = sum(stage_bytes(st, 1, M, N, mixed, otf=otf).values())
+        # one column runs on the MFMA chunk kernels in MIXED (the f16 near field)
+        b1 = sum(stage_bytes(st, 1, M, N, mixed, h16=h16, otf=otf).values())
         per_prec[label] = {
             "matvec_per_s": round(R / (ms * 1e-3), 2), "ms_per_sweep": round(ms, 4),
             "sweep_alg_bytes": int(b_alg), "sweep_hbm_gbs": round(b_alg / (ms * 1e-3) / 1e9, 1),
@@ -477,7 +478,8 @@ def bench_sphere65k(ctx, hip):
             core.matvec_device(0, 1, dx, n, dy, n)
         ctx.synchronize()
         ms = 1e3 * (time.perf_counter() - t0) / steps
-        b = sum(stage_bytes(st, 1, n, n, label == "mixed").values())
+        mix1 = label == "mixed"
+        b = sum(stage_bytes(st, 1, n, n, mix1, h16=mix1 and near_f16(core), otf=mix1 and couple_otf(core)).values())
         out[label] = {"ms_per_product": round(ms, 4), "matvec_per_s": round(1e3 / ms, 1),
                       "alg_bytes": int(b), "hbm_gbs": round(b / (ms * 1e-3) / 1e9, 1),
                       "hbm_frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}

@@ -171,8 +171,11 @@ struct nca_h2 {
     nca::DevBuf<double2> ws_ynear;    // near-field result per point (tree order) x R
     // workspace
     int64_t ws_rhs = 0;
-    // run a last chunk of 1..3 right-hand sides on the MFMA kernels too
-    int mfma_small = 0;
+    // run a last chunk of 1..3 right-hand sides on the MFMA kernels too: -1
+    // by precision (MIXED yes: its f16 near field costs the same at one column
+    // and beats the VALU kernel; FP64 no: its MFMA near field costs twice the
+    // VALU kernel's time at one column), 0 / 1 set by NCA_MFMA_SMALL
+    int mfma_small = -1;
     // MIXED coupling with more than 16 right-hand sides reads a compact copy
     // of the chunk's source coefficients (worth it when the coupling gathers
     // every coefficient several times)
@@ -2827,7 +2830,8 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     if (evs) evs->mark(0, st);
     // the f16 near field of the MFMA chunks (if any) needs the split copy of
     // the gathered right-hand sides and their column scales
-    const bool h16 = near_h16<MIX>(h, FWD) && (R >= 4 || h->mfma_small) && src.npts * R > 0;
+    const bool msmall = h->mfma_small >= 0 ? h->mfma_small != 0 : MIX;
+    const bool h16 = near_h16<MIX>(h, FWD) && (R >= 4 || msmall) && src.npts * R > 0;
     if (src.npts * R > 0)
         hipLaunchKernelGGL((k_gather_tiled<typename Ops<MIX>::T2>),
                            dim3(blocks_for(src.npts, 64), blocks_for(R, 16)), dim3(256), 0, st, x, ldx, src.iperm.p,
@@ -2844,7 +2848,7 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     // and, forward, of yh (coupling; in MIXED the coupling and the downward
     // pass) exactly once; the VALU chunks and the reverse coupling add, so they
     // need cleared vectors.
-    const bool valu = !h->mfma_small && (R % 16) > 0 && (R % 16) < 4;
+    const bool valu = !msmall && (R % 16) > 0 && (R % 16) < 4;
     // source coefficients: K x R elements of the operator's type, destination
     // coefficients FP64; cleared in 16-byte units
     using CT = typename Ops<MIX>::T2;
@@ -2854,16 +2858,16 @@ void run_op(nca_h2* h, const double2* x, int64_t ldx, double2* y, int64_t ldy, i
     if (ny16 > 0 && (valu || !FWD))
         hipLaunchKernelGGL(k_fill, dim3(blocks_for(ny16, 256)), dim3(256), 0, st, h->ws.yh, ny16);
     // chunks of up to 16 columns on the MFMA kernels; a last chunk of 1 to 3
-    // columns on the VALU kernels unless mfma_small is set. Two MFMA chunks in
+    // columns on the VALU kernels unless msmall. Two MFMA chunks in
     // a row share one near-field launch (each kernel evaluation serves 32
     // columns).
     int64_t q0 = 0;
     while (q0 < R) {
         const int64_t left = R - q0;
-        if (left >= 4 || h->mfma_small) {
+        if (left >= 4 || msmall) {
             const int nq = (int)std::min<int64_t>(16, left);
             const int nq2 = (int)std::min<int64_t>(16, left - nq);
-            if (nq == 16 && nq2 > 0 && (nq2 >= 4 || h->mfma_small)) {
+            if (nq == 16 && nq2 > 0 && (nq2 >= 4 || msmall)) {
                 // two chunks: stages 1-2 of each, one forward MIXED coupling
                 // launch (each coupling entry read once) or one per chunk,
                 // stage 4 of each, one near-field launch
